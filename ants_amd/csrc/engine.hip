@@ -1,0 +1,1080 @@
+// Host engine and C-ABI of the SyN + CC registration library.
+//
+// The engine replays SyNImageRegistrationMethod as RegistrationHelper
+// configures it (Examples/itkantsRegistrationHelper.hxx:1983-2220): per
+// level, smooth both full-resolution images, bring the four middle fields
+// to the level grid, then iterate: warp, CC value + both derivatives,
+// update-field smoothing + max norm, scaled composition onto the forward
+// middle fields, total-field smoothing, four fixed-point inversions, energy
+// and convergence.  All of it runs as HIP kernels on one stream; the host
+// reads back one small statistics block per iteration (energy and inner
+// step counts) to drive the convergence monitor and the observer.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "../../include/ants_syn.h"
+#include "kernels.hpp"
+
+using namespace ants;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                              \
+  do {                                                                                             \
+    hipError_t e_ = (expr);                                                                        \
+    if (e_ != hipSuccess) return fail(ANTS_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+#define ANTS_TRY(expr)         \
+  do {                         \
+    int rc_ = (expr);          \
+    if (rc_ != ANTS_OK) return rc_; \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Gaussian operator coefficients (ITK GaussianOperator::GenerateCoefficients):
+// c_i = e^-t I_i(t) grown until 1 + 2 sum c_i >= 1 - maxError, normalised.
+// Modified Bessel functions by the method of Numerical Recipes in C sec. 6.6
+// (polynomial fits of Abramowitz & Stegun 9.8.1-9.8.4, Miller recurrence).
+double bi0(double x) {
+  double ax = std::fabs(x);
+  if (ax < 3.75) {
+    double y = (x / 3.75) * (x / 3.75);
+    return 1.0 + y * (3.5156229 + y * (3.0899424 + y * (1.2067492 + y * (0.2659732 + y * (0.360768e-1 + y * 0.45813e-2)))));
+  }
+  double y = 3.75 / ax;
+  return (std::exp(ax) / std::sqrt(ax)) *
+         (0.39894228 + y * (0.1328592e-1 + y * (0.225319e-2 + y * (-0.157565e-2 + y * (0.916281e-2 + y * (-0.2057706e-1 + y * (0.2635537e-1 + y * (-0.1647633e-1 + y * 0.392377e-2))))))));
+}
+double bi1(double x) {
+  double ax = std::fabs(x), r;
+  if (ax < 3.75) {
+    double y = (x / 3.75) * (x / 3.75);
+    r = ax * (0.5 + y * (0.87890594 + y * (0.51498869 + y * (0.15084934 + y * (0.2658733e-1 + y * (0.301532e-2 + y * 0.32411e-3))))));
+  } else {
+    double y = 3.75 / ax;
+    r = 0.2282967e-1 + y * (-0.2895312e-1 + y * (0.1787654e-1 - y * 0.420059e-2));
+    r = 0.39894228 + y * (-0.3988024e-1 + y * (-0.362018e-2 + y * (0.163801e-2 + y * (-0.1031555e-1 + y * r))));
+    r *= std::exp(ax) / std::sqrt(ax);
+  }
+  return x < 0.0 ? -r : r;
+}
+double bin(int n, double x) {
+  if (n == 0) return bi0(x);
+  if (n == 1) return bi1(x);
+  if (x == 0.0) return 0.0;
+  const double tox = 2.0 / std::fabs(x);
+  double bip = 0.0, ans = 0.0, b = 1.0;
+  for (int j = 2 * (n + static_cast<int>(std::sqrt(40.0 * n))); j > 0; --j) {
+    const double bim = bip + j * tox * b;
+    bip = b;
+    b = bim;
+    if (std::fabs(b) > 1.0e10) {
+      ans *= 1.0e-10;
+      b *= 1.0e-10;
+      bip *= 1.0e-10;
+    }
+    if (j == n) ans = bip;
+  }
+  ans *= bi0(x) / b;
+  return (x < 0.0 && (n & 1)) ? -ans : ans;
+}
+
+int gaussian_taps(double variance, double max_error, int64_t max_width, Taps* t) {
+  std::vector<double> c;
+  const double et = std::exp(-variance), cap = 1.0 - max_error;
+  double sum = 0.0;
+  c.push_back(et * bi0(variance));
+  sum += c[0];
+  c.push_back(et * bi1(variance));
+  sum += 2.0 * c[1];
+  for (int i = 2; sum < cap; ++i) {
+    c.push_back(et * bin(i, variance));
+    sum += 2.0 * c[i];
+    if (c[i] < sum * std::numeric_limits<double>::epsilon()) break;
+    if (static_cast<int64_t>(c.size()) > max_width) break;
+  }
+  if (static_cast<int>(c.size()) > kMaxTaps)
+    return fail(ANTS_ERR_UNSUPPORTED, "Gaussian kernel radius %zu exceeds %d", c.size() - 1, kMaxTaps - 1);
+  std::memset(t, 0, sizeof(*t));
+  t->r = static_cast<int>(c.size()) - 1;
+  for (size_t i = 0; i < c.size(); ++i) t->c[i] = static_cast<float>(c[i] / sum);
+  return ANTS_OK;
+}
+
+// ---------------------------------------------------------------------------
+struct HostGrid {
+  int D;
+  int64_t n[3];
+  double sp[3], org[3];
+  int64_t count() const { return n[0] * n[1] * n[2]; }
+};
+
+bool identity_direction(const ants_image_desc* d) {
+  bool all_zero = true;
+  for (int i = 0; i < 9; ++i)
+    if (d->direction[i] != 0.0) all_zero = false;
+  if (all_zero) return true;
+  for (int r = 0; r < d->dim; ++r)
+    for (int c = 0; c < d->dim; ++c)
+      if (d->direction[r * d->dim + c] != (r == c ? 1.0 : 0.0)) return false;
+  return true;
+}
+
+int to_host_grid(const ants_image_desc* d, HostGrid* g) {
+  if (!d) return fail(ANTS_ERR_INVALID, "null image descriptor");
+  if (d->dim != 2 && d->dim != 3) return fail(ANTS_ERR_INVALID, "dim must be 2 or 3 (got %d)", d->dim);
+  if (!identity_direction(d))
+    return fail(ANTS_ERR_UNSUPPORTED, "non-identity image direction is not supported in this release");
+  g->D = d->dim;
+  for (int k = 0; k < 3; ++k) {
+    g->n[k] = k < d->dim ? d->size[k] : 1;
+    g->sp[k] = k < d->dim ? d->spacing[k] : 1.0;
+    g->org[k] = k < d->dim ? d->origin[k] : 0.0;
+    if (g->n[k] < 1) return fail(ANTS_ERR_INVALID, "image size must be positive");
+    if (!(g->sp[k] > 0.0)) return fail(ANTS_ERR_INVALID, "image spacing must be positive");
+  }
+  if (g->count() > (static_cast<int64_t>(1) << 31) - 1) return fail(ANTS_ERR_UNSUPPORTED, "image too large");
+  return ANTS_OK;
+}
+
+GridDev to_dev(const HostGrid& h) {
+  GridDev g{};
+  g.D = h.D;
+  for (int k = 0; k < 3; ++k) {
+    g.n[k] = static_cast<int>(h.n[k]);
+    g.sp[k] = static_cast<float>(h.sp[k]);
+    g.org[k] = static_cast<float>(h.org[k]);
+    g.dsp[k] = h.sp[k];
+    g.dorg[k] = h.org[k];
+  }
+  return g;
+}
+
+void to_desc(const HostGrid& h, ants_image_desc* d) {
+  std::memset(d, 0, sizeof(*d));
+  d->dim = h.D;
+  for (int k = 0; k < 3; ++k) {
+    d->size[k] = h.n[k];
+    d->spacing[k] = h.sp[k];
+    d->origin[k] = h.org[k];
+  }
+  for (int r = 0; r < h.D; ++r) d->direction[r * h.D + r] = 1.0;
+}
+
+// RegistrationHelper::CalculateShrinkFactorsPerDimension + ShrinkImageFilter
+// output information (Examples/itkantsRegistrationHelper.hxx:16-39, 593-641).
+HostGrid level_grid(const HostGrid& g, int factor) {
+  int mi = 0;
+  for (int d = 1; d < g.D; ++d)
+    if (g.sp[mi] > g.sp[d]) mi = d;
+  int f[3] = {1, 1, 1};
+  f[mi] = factor;
+  const double target = g.sp[mi] * factor;
+  for (int d = 0; d < g.D; ++d) {
+    if (d == mi) continue;
+    double best = std::fabs(g.sp[d] * factor - target);
+    int bf = factor;
+    for (int k = factor - 1; k > 0; --k) {
+      const double diff = std::fabs(g.sp[d] * k - target);
+      if (diff <= best) {  // the smaller factor wins ties
+        best = diff;
+        bf = k;
+      }
+    }
+    f[d] = bf;
+  }
+  HostGrid o = g;
+  for (int d = 0; d < g.D; ++d) {
+    o.sp[d] = g.sp[d] * f[d];
+    o.n[d] = std::max<int64_t>(1, static_cast<int64_t>(std::floor(static_cast<double>(g.n[d]) / f[d])));
+    const double in_centre = g.org[d] + (g.n[d] - 1) / 2.0 * g.sp[d];
+    const double out_centre = g.org[d] + (o.n[d] - 1) / 2.0 * o.sp[d];
+    o.org[d] = g.org[d] + (in_centre - out_centre);
+  }
+  return o;
+}
+
+int validate_params(const ants_syn_params* p) {
+  if (!p) return fail(ANTS_ERR_INVALID, "null parameters");
+  if (p->dim != 2 && p->dim != 3) return fail(ANTS_ERR_INVALID, "dim must be 2 or 3");
+  if (p->n_levels < 1 || p->n_levels > ANTS_MAX_LEVELS)
+    return fail(ANTS_ERR_INVALID, "number of levels must be in 1..%d", ANTS_MAX_LEVELS);
+  for (int l = 0; l < p->n_levels; ++l) {
+    if (p->shrink[l] < 1) return fail(ANTS_ERR_INVALID, "shrink factors must be >= 1");
+    if (p->iterations[l] < 0) return fail(ANTS_ERR_INVALID, "iterations must be >= 0");
+    if (!(p->sigma[l] >= 0.0)) return fail(ANTS_ERR_INVALID, "smoothing sigmas must be >= 0");
+  }
+  if (p->cc_radius < 1 || p->cc_radius > 16) return fail(ANTS_ERR_INVALID, "CC radius must be in 1..16");
+  if (!(p->learning_rate > 0.0)) return fail(ANTS_ERR_INVALID, "SyN gradient step must be > 0");
+  if (p->convergence_window < 1) return fail(ANTS_ERR_INVALID, "convergence window must be >= 1");
+  if (p->precision != ANTS_PRECISION_FP32 && p->precision != ANTS_PRECISION_BF16_PYRAMID)
+    return fail(ANTS_ERR_INVALID, "unknown precision mode");
+  if (p->precision == ANTS_PRECISION_BF16_PYRAMID)
+    return fail(ANTS_ERR_UNSUPPORTED, "BF16 pyramid precision is not available in this build");
+  return ANTS_OK;
+}
+
+// WindowConvergenceMonitoringFunction (see oracle): window of the last W
+// energies, running |E| total over the level, order-1 two-control-point
+// B-spline by multilevel B-spline approximation, value = -slope at t = 1.
+struct ConvergenceMonitor {
+  int window;
+  std::vector<double> e;
+  double total = 0.0;
+  explicit ConvergenceMonitor(int w) : window(w) {}
+  void add(double v) {
+    e.push_back(v);
+    if (static_cast<int>(e.size()) > window) e.erase(e.begin());
+    total += std::fabs(v);
+  }
+  double value() const {
+    if (static_cast<int>(e.size()) < window || window < 2) return std::numeric_limits<double>::max();
+    double num0 = 0, num1 = 0, den0 = 0, den1 = 0;
+    for (int n = 0; n < window; ++n) {
+      const double t = static_cast<double>(n) / (window - 1);
+      const double v = e[n] / total;
+      const double w0 = 1.0 - t, w1 = t, w2 = w0 * w0 + w1 * w1;
+      num0 += w0 * w0 * (w0 * v / w2);
+      num1 += w1 * w1 * (w1 * v / w2);
+      den0 += w0 * w0;
+      den1 += w1 * w1;
+    }
+    const double p0 = den0 > 0 ? num0 / den0 : 0.0, p1 = den1 > 0 ? num1 / den1 : 0.0;
+    return -(p1 - p0);
+  }
+};
+
+// Device buffer owned by the engine.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t b) {
+    if (b <= bytes) return ANTS_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    hipError_t e = hipMalloc(&p, b);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return fail(ANTS_ERR_OOM, "hipMalloc(%zu) failed: %s", b, hipGetErrorString(e));
+    }
+    bytes = b;
+    return ANTS_OK;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  float* f() const { return static_cast<float*>(p); }
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+};
+
+// HIP-event timing of kernel classes on one stream.  Events are recorded
+// around each launch of an enabled class and read after the iteration's
+// synchronisation; inversion launches are attributed afterwards from the
+// per-step `computed` flags (inactive launches carry no bytes and no time).
+struct Profiler {
+  unsigned int mask = 0;
+  hipStream_t s = nullptr;
+  ants_syn_kernel_stats acc{};
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  struct Rec {
+    int cls;
+    hipEvent_t a, b;
+    double bytes;
+    int batch, step;  // inversion launches
+  };
+  std::vector<Rec> recs;
+  hipEvent_t a_ = nullptr;
+
+  bool on(int cls) const { return (mask >> cls) & 1u; }
+  hipEvent_t next() {
+    if (used == pool.size()) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return nullptr;
+      pool.push_back(e);
+    }
+    return pool[used++];
+  }
+  void begin(int cls) {
+    a_ = nullptr;
+    if (!on(cls)) return;
+    a_ = next();
+    if (a_) (void)hipEventRecord(a_, s);
+  }
+  void end(int cls, double bytes, int batch = -1, int step = -1) {
+    if (!on(cls) || !a_) return;
+    hipEvent_t b = next();
+    if (!b) return;
+    (void)hipEventRecord(b, s);
+    recs.push_back(Rec{cls, a_, b, bytes, batch, step});
+    a_ = nullptr;
+  }
+  // stats: the iteration's statistics block (host copy); N voxels, D dims
+  void collect(const IterStats* st, int64_t N, int D) {
+    for (const Rec& r : recs) {
+      double bytes = r.bytes;
+      if (r.cls == ANTS_KC_INV) {
+        bytes = 0.0;
+        const int qs[2][2] = {{0, 2}, {1, 3}};
+        for (int k = 0; k < 2; ++k) {
+          const RedSlot* S = st->inv[qs[r.batch][k]];
+          const bool upd = r.step >= 1 && S[r.step - 1].computed;
+          const bool nrm = S[r.step].computed != 0;
+          if (upd)
+            bytes += 3.0 * D * 4.0 * N;
+          else if (nrm)
+            bytes += 2.0 * D * 4.0 * N;
+        }
+        if (bytes == 0.0) continue;
+      }
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, r.a, r.b) != hipSuccess) continue;
+      acc.ms[r.cls] += ms;
+      acc.launches[r.cls] += 1;
+      acc.bytes[r.cls] += bytes;
+    }
+    recs.clear();
+    used = 0;
+  }
+  ~Profiler() {
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+  }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+struct ants_syn_handle {
+  ants_syn_params P{};
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  DevBuf F, M, Fs, Ms, tmp, Fw, Mw;
+  DevBuf fld[8];
+  DevBuf stats, partials_cc, partials_inv, minmax;
+  IterStats* host_stats = nullptr;  // pinned
+  ants_syn_stats run_stats{};
+  HostGrid last_grid{};
+  bool have_state = false;
+  float* state[4] = {nullptr, nullptr, nullptr, nullptr};  // f2m fwd, f2m inv, m2m fwd, m2m inv
+  Profiler prof;
+
+  ~ants_syn_handle() {
+    DevBuf* all[] = {&F, &M, &Fs, &Ms, &tmp, &Fw, &Mw, &stats, &partials_cc, &partials_inv, &minmax};
+    for (DevBuf* b : all) b->release();
+    for (auto& b : fld) b.release();
+    if (host_stats) (void)hipHostFree(host_stats);
+    if (own_stream && stream) (void)hipStreamDestroy(stream);
+  }
+
+  int preprocess(const float* in, float* out, int64_t n);
+  int smooth_image(const HostGrid& g, const float* in, float* out, double sigma);
+  int smooth_fields(const HostGrid& g, int np, float* const* in, float* const* work, float* const* out,
+                    double variance, unsigned int* maxbits);
+  int run(const ants_image_desc* fd, const float* d_fixed, const ants_image_desc* md, const float* d_moving,
+          float* d_out_fwd, float* d_out_inv, ants_syn_iteration_cb cb, void* user);
+};
+
+namespace {
+
+// Device-side PreprocessImage: min/max reduction, host computes the window
+// exactly as the CPU path does, then one elementwise pass.
+int preprocess_on(hipStream_t s, DevBuf& mm, const float* in, float* out, int64_t n, uint32_t* host_pair) {
+  ANTS_TRY(mm.ensure(2 * sizeof(uint32_t)));
+  uint32_t* d = static_cast<uint32_t*>(mm.p);
+  launch_fill_u32(d, 0xffffffffu, 1, s);
+  launch_fill_u32(d + 1, 0u, 1, s);
+  launch_minmax(in, n, d, s);
+  HIP_TRY(hipMemcpyAsync(host_pair, d, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  auto decode = [](uint32_t o) {
+    uint32_t u = (o & 0x80000000u) ? (o & 0x7fffffffu) : ~o;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+  };
+  const double mn = decode(host_pair[0]), mx = decode(host_pair[1]);
+  const float lo = static_cast<float>(mn);
+  const float hi = static_cast<float>(mx + (mx - mn) / 2560.0);
+  const double scale = hi > lo ? 1.0 / (static_cast<double>(hi) - static_cast<double>(lo)) : 0.0;
+  const double shift = -static_cast<double>(lo) * scale;
+  launch_window(in, out, n, lo, hi, scale, shift, s);
+  HIP_TRY(hipGetLastError());
+  return ANTS_OK;
+}
+
+int smooth_image_on(hipStream_t s, const HostGrid& g, const float* in, float* out, float* tmp, double sigma,
+                    bool in_mm) {
+  const GridDev gd = to_dev(g);
+  const int64_t n = g.count();
+  if (!(sigma > 0.0)) {
+    if (out != in) HIP_TRY(hipMemcpyAsync(out, in, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+    return ANTS_OK;
+  }
+  // ping-pong so that the last pass lands in `out`
+  const float* src = in;
+  for (int d = 0; d < g.D; ++d) {
+    double var = sigma * sigma;
+    if (in_mm) var /= g.sp[d] * g.sp[d];
+    Taps t;
+    ANTS_TRY(gaussian_taps(var, 0.01, 32, &t));
+    float* dst = ((g.D - 1 - d) % 2 == 0) ? out : tmp;
+    launch_gauss_axis(gd, 1, src, dst, t, d, s);
+    src = dst;
+  }
+  HIP_TRY(hipGetLastError());
+  return ANTS_OK;
+}
+
+// SyN GaussianSmoothDisplacementField for np fields; maxbits (optional)
+// receives each result's max |u/spacing|.  work/out may alias in.
+int smooth_fields_on(hipStream_t s, const HostGrid& g, int np, float* const* in, float* const* work,
+                     float* const* out, double variance, unsigned int* maxbits, Profiler* prof = nullptr) {
+  const GridDev gd = to_dev(g);
+  const int D = g.D;
+  const double pass_bytes = 2.0 * D * 4.0 * static_cast<double>(g.count());
+  if (!(variance > 0.0)) {
+    // plain copy (no boundary handling); the max norm still comes from a pass
+    Taps t{};
+    t.r = 0;
+    t.c[0] = 1.f;
+    if (prof) prof->begin(ANTS_KC_SMOOTH);
+    launch_gauss_field_final(gd, np, in, in, out, t, 0, 1.f, 0.f, false, maxbits, s);
+    if (prof) prof->end(ANTS_KC_SMOOTH, np * pass_bytes);
+    HIP_TRY(hipGetLastError());
+    return ANTS_OK;
+  }
+  // axes 0..D-2 ping-pong between work and in; the last axis lands in out
+  const float* cur[4];
+  for (int k = 0; k < np; ++k) cur[k] = in[k];
+  for (int d = 0; d < D - 1; ++d) {
+    Taps t;
+    ANTS_TRY(gaussian_taps(variance, 0.001, g.n[d], &t));
+    for (int k = 0; k < np; ++k) {
+      float* dst = (d % 2 == 0) ? work[k] : in[k];
+      if (prof) prof->begin(ANTS_KC_SMOOTH);
+      launch_gauss_axis(gd, D, cur[k], dst, t, d, s);
+      if (prof) prof->end(ANTS_KC_SMOOTH, pass_bytes);
+      cur[k] = dst;
+    }
+  }
+  Taps t;
+  ANTS_TRY(gaussian_taps(variance, 0.001, g.n[D - 1], &t));
+  const float w1 = variance < 0.5 ? static_cast<float>(1.0 - variance / 0.5) : 1.f;
+  const float w2 = 1.f - w1;
+  const float* orig[4];
+  for (int k = 0; k < np; ++k) orig[k] = in[k];
+  if (w2 != 0.f && D % 2 == 1) {
+    // the ping-pong overwrote `in`; blending needs the original (variance < 0.5, rare): not supported in-place
+    return fail(ANTS_ERR_UNSUPPORTED, "update/total variance below 0.5 is not supported in this build");
+  }
+  if (prof) prof->begin(ANTS_KC_SMOOTH);
+  launch_gauss_field_final(gd, np, cur, orig, out, t, D - 1, w1, w2, true, maxbits, s);
+  if (prof) prof->end(ANTS_KC_SMOOTH, np * pass_bytes);
+  HIP_TRY(hipGetLastError());
+  return ANTS_OK;
+}
+
+}  // namespace
+
+int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const ants_image_desc* md,
+                         const float* d_moving, float* d_out_fwd, float* d_out_inv, ants_syn_iteration_cb cb,
+                         void* user) {
+  HostGrid gF, gM;
+  ANTS_TRY(to_host_grid(fd, &gF));
+  ANTS_TRY(to_host_grid(md, &gM));
+  if (gF.D != P.dim || gM.D != P.dim) return fail(ANTS_ERR_INVALID, "image dimension does not match parameters");
+  if (!d_fixed || !d_moving) return fail(ANTS_ERR_INVALID, "null image buffer");
+  const int D = P.dim;
+  const int64_t NF = gF.count(), NM = gM.count();
+  std::vector<HostGrid> lg(P.n_levels);
+  int64_t maxN = 0;
+  for (int l = 0; l < P.n_levels; ++l) {
+    lg[l] = level_grid(gF, P.shrink[l]);
+    maxN = std::max(maxN, lg[l].count());
+  }
+  const size_t fbytes = static_cast<size_t>(maxN) * D * sizeof(float);
+  ANTS_TRY(F.ensure(NF * sizeof(float)));
+  ANTS_TRY(M.ensure(NM * sizeof(float)));
+  ANTS_TRY(Fs.ensure(NF * sizeof(float)));
+  ANTS_TRY(Ms.ensure(NM * sizeof(float)));
+  ANTS_TRY(tmp.ensure(std::max(NF, NM) * sizeof(float)));
+  ANTS_TRY(Fw.ensure(maxN * sizeof(float)));
+  ANTS_TRY(Mw.ensure(maxN * sizeof(float)));
+  for (auto& b : fld) ANTS_TRY(b.ensure(fbytes));
+  ANTS_TRY(stats.ensure(sizeof(IterStats)));
+  int64_t ccp = 0, invp = 0;
+  for (int l = 0; l < P.n_levels; ++l) {
+    ccp = std::max(ccp, cc_partials_needed(to_dev(lg[l]), P.cc_radius));
+    invp = std::max(invp, inv_partials_needed(to_dev(lg[l])));
+  }
+  ANTS_TRY(partials_cc.ensure(ccp * sizeof(double)));
+  ANTS_TRY(partials_inv.ensure(2 * invp * sizeof(double)));
+  if (!host_stats) HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&host_stats), sizeof(IterStats), hipHostMallocDefault));
+
+  hipStream_t s = stream;
+  IterStats* dstats = static_cast<IterStats*>(stats.p);
+  std::memset(&run_stats, 0, sizeof(run_stats));
+  std::memset(&prof.acc, 0, sizeof(prof.acc));
+  prof.s = s;
+  prof.recs.clear();
+  prof.used = 0;
+  have_state = false;
+
+  // preprocessing (PreprocessImage) or plain copies
+  uint32_t* pair = reinterpret_cast<uint32_t*>(host_stats);
+  if (P.preprocess) {
+    ANTS_TRY(preprocess_on(s, minmax, d_fixed, F.f(), NF, pair));
+    ANTS_TRY(preprocess_on(s, minmax, d_moving, M.f(), NM, pair));
+  } else {
+    HIP_TRY(hipMemcpyAsync(F.p, d_fixed, NF * sizeof(float), hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemcpyAsync(M.p, d_moving, NM * sizeof(float), hipMemcpyDeviceToDevice, s));
+  }
+
+  // field slots: state (f2m_f, f2m_i, m2m_f, m2m_i) and work (4)
+  float* st[4] = {fld[0].f(), fld[1].f(), fld[2].f(), fld[3].f()};
+  float* wk[4] = {fld[4].f(), fld[5].f(), fld[6].f(), fld[7].f()};
+  const GridDev gFd = to_dev(gF), gMd = to_dev(gM);
+  const float eps = std::numeric_limits<float>::epsilon();
+  HostGrid prev{};
+  for (int l = 0; l < P.n_levels; ++l) {
+    const HostGrid& g = lg[l];
+    const GridDev gd = to_dev(g);
+    const int64_t N = g.count();
+    if (l == 0) {
+      for (int k = 0; k < 4; ++k) HIP_TRY(hipMemsetAsync(st[k], 0, N * D * sizeof(float), s));
+    } else {
+      const float* in[4] = {st[0], st[1], st[2], st[3]};
+      launch_resample_fields(to_dev(prev), gd, 4, in, wk, s);
+      HIP_TRY(hipGetLastError());
+      for (int k = 0; k < 4; ++k) std::swap(st[k], wk[k]);
+    }
+    prev = g;
+    run_stats.levels_run = l + 1;
+    run_stats.level_voxels[l] = N;
+    if (P.iterations[l] <= 0) continue;
+    ANTS_TRY(smooth_image_on(s, gF, F.f(), Fs.f(), tmp.f(), P.sigma[l], P.sigma_in_mm != 0));
+    ANTS_TRY(smooth_image_on(s, gM, M.f(), Ms.f(), tmp.f(), P.sigma[l], P.sigma_in_mm != 0));
+
+    ConvergenceMonitor mon(P.convergence_window);
+    bool converged = false;
+    int iter = 0;
+    // algorithmic bytes per voxel-iteration: image gathers + field passes
+    int s_max = 1;
+    for (int d = 0; d < D; ++d) s_max = std::max(s_max, static_cast<int>(std::lround(g.sp[d] / gF.sp[d])));
+    double gsz = 1.0;
+    for (int d = 0; d < D; ++d) gsz *= std::min(2, s_max);
+    const double e_i = 4.0, e_f = 4.0;
+    while (iter++ < P.iterations[l] && !converged) {
+      launch_fill_u32(reinterpret_cast<uint32_t*>(dstats), 0u, sizeof(IterStats) / 4, s);
+      // 1. warp both smoothed images through the inverse middle fields
+      const double Nd = static_cast<double>(N);
+      prof.begin(ANTS_KC_WARP);
+      launch_warp_pair(gd, gFd, Fs.f(), st[1], Fw.f(), gMd, Ms.f(), st[3], Mw.f(), s);
+      prof.end(ANTS_KC_WARP, Nd * (2.0 * D * e_f + 2.0 * gsz * e_i + 2.0 * e_i));
+      // 2. CC value and both derivatives (g_f2m -> wk[0], g_m2m -> wk[1])
+      prof.begin(ANTS_KC_CC);
+      launch_cc(gd, Fw.f(), Mw.f(), P.cc_radius, eps, wk[0], wk[1], static_cast<double*>(partials_cc.p),
+                &dstats->energy, s);
+      prof.end(ANTS_KC_CC, Nd * (2.0 * e_i + 2.0 * D * e_f));
+      HIP_TRY(hipGetLastError());
+      // 3. update-field smoothing + max norms (result in wk[0], wk[1]; scratch wk[2], wk[3])
+      {
+        float* in[2] = {wk[0], wk[1]};
+        float* work[2] = {wk[2], wk[3]};
+        float* out[2] = {D % 2 == 1 ? wk[2] : wk[0], D % 2 == 1 ? wk[3] : wk[1]};
+        ANTS_TRY(smooth_fields_on(s, g, 2, in, work, out, P.update_variance, dstats->maxnorm, &prof));
+        if (out[0] != wk[0]) {
+          std::swap(wk[0], wk[2]);
+          std::swap(wk[1], wk[3]);
+        }
+      }
+      // 4. scaled composition onto the forward middle fields: t -> wk[2], wk[3]
+      {
+        const float* disp[2] = {wk[0], wk[1]};
+        const float* warp[2] = {st[0], st[2]};
+        float* out[2] = {wk[2], wk[3]};
+        prof.begin(ANTS_KC_COMPOSE);
+        launch_compose(gd, 2, disp, warp, out, dstats->maxnorm, static_cast<float>(P.learning_rate), s);
+        prof.end(ANTS_KC_COMPOSE, 2.0 * Nd * 3.0 * D * e_f);
+        HIP_TRY(hipGetLastError());
+      }
+      // 5. total-field smoothing (a copy when the variance is 0)
+      if (P.total_variance > 0.0) {
+        float* in[2] = {wk[2], wk[3]};
+        float* work[2] = {wk[0], wk[1]};
+        float* out[2] = {D % 2 == 1 ? wk[0] : wk[2], D % 2 == 1 ? wk[1] : wk[3]};
+        ANTS_TRY(smooth_fields_on(s, g, 2, in, work, out, P.total_variance, dstats->maxnorm + 2, &prof));
+        if (out[0] != wk[2]) {
+          std::swap(wk[0], wk[2]);
+          std::swap(wk[1], wk[3]);
+        }
+      }
+      // 6. inversions: inverse middle fields from the previous inverses (in place),
+      //    then forward fields as inverses of the new inverses, started from t (in place)
+      {
+        const float* field[2] = {wk[2], wk[3]};
+        float* inv[2] = {st[1], st[3]};
+        RedSlot* slots[2] = {dstats->inv[0], dstats->inv[2]};
+        for (int j = 0; j <= kInvMaxIter; ++j) {
+          prof.begin(ANTS_KC_INV);
+          launch_inv_step(gd, 2, field, inv, slots, static_cast<double*>(partials_inv.p), j, s);
+          prof.end(ANTS_KC_INV, 0.0, 0, j);
+        }
+      }
+      {
+        const float* field[2] = {st[1], st[3]};
+        float* inv[2] = {wk[2], wk[3]};
+        RedSlot* slots[2] = {dstats->inv[1], dstats->inv[3]};
+        for (int j = 0; j <= kInvMaxIter; ++j) {
+          prof.begin(ANTS_KC_INV);
+          launch_inv_step(gd, 2, field, inv, slots, static_cast<double*>(partials_inv.p), j, s);
+          prof.end(ANTS_KC_INV, 0.0, 1, j);
+        }
+      }
+      HIP_TRY(hipGetLastError());
+      std::swap(st[0], wk[2]);
+      std::swap(st[2], wk[3]);
+      // 7. energy, convergence, observer
+      HIP_TRY(hipMemcpyAsync(host_stats, dstats, sizeof(IterStats), hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipStreamSynchronize(s));
+      prof.collect(host_stats, N, D);
+      const double value = -host_stats->energy.sum / static_cast<double>(N);
+      const double E = 0.5 * (value + value);
+      int kin[4];
+      for (int q = 0; q < 4; ++q) {
+        int c = 0;
+        for (int j = 0; j < kInvSlots; ++j) c += host_stats->inv[q][j].computed ? 1 : 0;
+        kin[q] = c;
+      }
+      mon.add(E);
+      const double conv = mon.value();
+      if (!P.force_iterations && conv < P.convergence_threshold) converged = true;
+      run_stats.iterations[l] = iter;
+      const int ksum = kin[0] + kin[1] + kin[2] + kin[3];
+      run_stats.inv_steps[l] += ksum;
+      run_stats.voxel_iterations += static_cast<double>(N);
+      run_stats.algorithmic_bytes +=
+          static_cast<double>(N) * ((2.0 * gsz * e_i + 14.0 * D * e_f) + (3.0 * ksum + 4.0) * D * e_f);
+      if (cb && cb(user, l, iter, E, conv, kin) != 0) return fail(ANTS_ERR_CANCELLED, "cancelled by the observer");
+    }
+  }
+  // final composition: fwd = M2M.inv o F2M.fwd ; inv = F2M.inv o M2M.fwd
+  const HostGrid& gl = prev;
+  const GridDev gld = to_dev(gl);
+  {
+    const float* disp[2] = {st[3], st[1]};
+    const float* warp[2] = {st[0], st[2]};
+    float* out[2] = {d_out_fwd ? d_out_fwd : wk[0], d_out_inv ? d_out_inv : wk[1]};
+    launch_compose(gld, 2, disp, warp, out, nullptr, 1.f, s);
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipStreamSynchronize(s));
+  for (int k = 0; k < 4; ++k) state[k] = st[k];
+  last_grid = gl;
+  have_state = true;
+  return ANTS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// C-ABI
+extern "C" {
+
+const char* ants_last_error(void) { return g_last_error.c_str(); }
+int ants_version(void) { return 1; }
+
+int ants_device_malloc(void** d_ptr, size_t bytes) {
+  if (!d_ptr) return fail(ANTS_ERR_INVALID, "null output pointer");
+  hipError_t e = hipMalloc(d_ptr, bytes);
+  if (e != hipSuccess) return fail(ANTS_ERR_OOM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+  return ANTS_OK;
+}
+int ants_device_free(void* d_ptr) {
+  HIP_TRY(hipFree(d_ptr));
+  return ANTS_OK;
+}
+int ants_memcpy_h2d(void* d_dst, const void* src, size_t bytes, void* stream) {
+  HIP_TRY(hipMemcpyAsync(d_dst, src, bytes, hipMemcpyHostToDevice, static_cast<hipStream_t>(stream)));
+  HIP_TRY(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+  return ANTS_OK;
+}
+int ants_memcpy_d2h(void* dst, const void* d_src, size_t bytes, void* stream) {
+  HIP_TRY(hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, static_cast<hipStream_t>(stream)));
+  HIP_TRY(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+  return ANTS_OK;
+}
+int ants_memset_device(void* d_dst, int value, size_t bytes, void* stream) {
+  HIP_TRY(hipMemsetAsync(d_dst, value, bytes, static_cast<hipStream_t>(stream)));
+  return ANTS_OK;
+}
+int ants_stream_create(void** stream) {
+  hipStream_t s;
+  HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = s;
+  return ANTS_OK;
+}
+int ants_stream_destroy(void* stream) {
+  HIP_TRY(hipStreamDestroy(static_cast<hipStream_t>(stream)));
+  return ANTS_OK;
+}
+int ants_stream_synchronize(void* stream) {
+  HIP_TRY(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+  return ANTS_OK;
+}
+int ants_device_synchronize(void) {
+  HIP_TRY(hipDeviceSynchronize());
+  return ANTS_OK;
+}
+int ants_event_create(void** event) {
+  hipEvent_t e;
+  HIP_TRY(hipEventCreate(&e));
+  *event = e;
+  return ANTS_OK;
+}
+int ants_event_destroy(void* event) {
+  HIP_TRY(hipEventDestroy(static_cast<hipEvent_t>(event)));
+  return ANTS_OK;
+}
+int ants_event_record(void* event, void* stream) {
+  HIP_TRY(hipEventRecord(static_cast<hipEvent_t>(event), static_cast<hipStream_t>(stream)));
+  return ANTS_OK;
+}
+int ants_event_elapsed_ms(float* ms, void* start, void* stop) {
+  HIP_TRY(hipEventSynchronize(static_cast<hipEvent_t>(stop)));
+  HIP_TRY(hipEventElapsedTime(ms, static_cast<hipEvent_t>(start), static_cast<hipEvent_t>(stop)));
+  return ANTS_OK;
+}
+
+int ants_syn_level_geometry(const ants_syn_params* p, const ants_image_desc* fixed, int level, ants_image_desc* out) {
+  ANTS_TRY(validate_params(p));
+  if (level < 0 || level >= p->n_levels) return fail(ANTS_ERR_INVALID, "level out of range");
+  HostGrid g;
+  ANTS_TRY(to_host_grid(fixed, &g));
+  if (!out) return fail(ANTS_ERR_INVALID, "null output descriptor");
+  to_desc(level_grid(g, p->shrink[level]), out);
+  return ANTS_OK;
+}
+
+int ants_syn_create(const ants_syn_params* p, ants_syn_handle** out) {
+  if (!out) return fail(ANTS_ERR_INVALID, "null handle pointer");
+  *out = nullptr;
+  ANTS_TRY(validate_params(p));
+  auto* h = new (std::nothrow) ants_syn_handle();
+  if (!h) return fail(ANTS_ERR_OOM, "out of host memory");
+  h->P = *p;
+  hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete h;
+    return fail(ANTS_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+  }
+  h->own_stream = true;
+  *out = h;
+  return ANTS_OK;
+}
+
+void ants_syn_destroy(ants_syn_handle* h) { delete h; }
+
+int ants_syn_set_stream(ants_syn_handle* h, void* stream) {
+  if (!h) return fail(ANTS_ERR_INVALID, "null handle");
+  if (h->own_stream && h->stream) HIP_TRY(hipStreamDestroy(h->stream));
+  h->stream = static_cast<hipStream_t>(stream);
+  h->own_stream = false;
+  return ANTS_OK;
+}
+
+int ants_syn_register_device(ants_syn_handle* h, const ants_image_desc* fixed, const float* d_fixed,
+                             const ants_image_desc* moving, const float* d_moving, float* d_out_fwd,
+                             float* d_out_inv, ants_syn_iteration_cb cb, void* user) {
+  if (!h) return fail(ANTS_ERR_INVALID, "null handle");
+  return h->run(fixed, d_fixed, moving, d_moving, d_out_fwd, d_out_inv, cb, user);
+}
+
+int ants_syn_register(ants_syn_handle* h, const ants_image_desc* fixed, const float* host_fixed,
+                      const ants_image_desc* moving, const float* host_moving, float* host_out_fwd,
+                      float* host_out_inv, ants_syn_iteration_cb cb, void* user) {
+  if (!h) return fail(ANTS_ERR_INVALID, "null handle");
+  HostGrid gF, gM;
+  ANTS_TRY(to_host_grid(fixed, &gF));
+  ANTS_TRY(to_host_grid(moving, &gM));
+  if (!host_fixed || !host_moving) return fail(ANTS_ERR_INVALID, "null image buffer");
+  const HostGrid gl = level_grid(gF, h->P.shrink[h->P.n_levels - 1]);
+  const size_t fb = static_cast<size_t>(gl.count()) * gF.D * sizeof(float);
+  DevBuf df, dm, dfwd, dinv;
+  ANTS_TRY(df.ensure(gF.count() * sizeof(float)));
+  ANTS_TRY(dm.ensure(gM.count() * sizeof(float)));
+  ANTS_TRY(dfwd.ensure(fb));
+  ANTS_TRY(dinv.ensure(fb));
+  HIP_TRY(hipMemcpyAsync(df.p, host_fixed, gF.count() * sizeof(float), hipMemcpyHostToDevice, h->stream));
+  HIP_TRY(hipMemcpyAsync(dm.p, host_moving, gM.count() * sizeof(float), hipMemcpyHostToDevice, h->stream));
+  int rc = h->run(fixed, df.f(), moving, dm.f(), dfwd.f(), dinv.f(), cb, user);
+  if (rc == ANTS_OK) {
+    if (host_out_fwd) HIP_TRY(hipMemcpyAsync(host_out_fwd, dfwd.p, fb, hipMemcpyDeviceToHost, h->stream));
+    if (host_out_inv) HIP_TRY(hipMemcpyAsync(host_out_inv, dinv.p, fb, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+  }
+  df.release();
+  dm.release();
+  dfwd.release();
+  dinv.release();
+  return rc;
+}
+
+int ants_syn_get_state_device(ants_syn_handle* h, float* d_f2m_fwd, float* d_f2m_inv, float* d_m2m_fwd,
+                              float* d_m2m_inv) {
+  if (!h) return fail(ANTS_ERR_INVALID, "null handle");
+  if (!h->have_state) return fail(ANTS_ERR_INVALID, "no registration has completed on this handle");
+  float* dst[4] = {d_f2m_fwd, d_f2m_inv, d_m2m_fwd, d_m2m_inv};
+  const size_t b = static_cast<size_t>(h->last_grid.count()) * h->last_grid.D * sizeof(float);
+  for (int k = 0; k < 4; ++k)
+    if (dst[k]) HIP_TRY(hipMemcpyAsync(dst[k], h->state[k], b, hipMemcpyDeviceToDevice, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  return ANTS_OK;
+}
+
+int ants_syn_get_stats(const ants_syn_handle* h, ants_syn_stats* out) {
+  if (!h || !out) return fail(ANTS_ERR_INVALID, "null argument");
+  *out = h->run_stats;
+  return ANTS_OK;
+}
+
+int ants_syn_set_profiling(ants_syn_handle* h, unsigned int class_mask) {
+  if (!h) return fail(ANTS_ERR_INVALID, "null handle");
+  h->prof.mask = class_mask;
+  return ANTS_OK;
+}
+
+int ants_syn_get_kernel_stats(const ants_syn_handle* h, ants_syn_kernel_stats* out) {
+  if (!h || !out) return fail(ANTS_ERR_INVALID, "null argument");
+  *out = h->prof.acc;
+  return ANTS_OK;
+}
+
+// ---- building blocks ----
+namespace {
+struct OpScratch {
+  DevBuf stats, partials;
+  ~OpScratch() {
+    stats.release();
+    partials.release();
+  }
+};
+}  // namespace
+
+int ants_op_smooth_image(const ants_image_desc* gdsc, const float* d_in, float* d_out, double sigma, int sigma_in_mm,
+                         void* stream) {
+  HostGrid g;
+  ANTS_TRY(to_host_grid(gdsc, &g));
+  if (!(sigma >= 0.0)) return fail(ANTS_ERR_INVALID, "sigma must be >= 0");
+  DevBuf tmp;
+  ANTS_TRY(tmp.ensure(g.count() * sizeof(float)));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int rc = smooth_image_on(s, g, d_in, d_out, tmp.f(), sigma, sigma_in_mm != 0);
+  if (rc == ANTS_OK) HIP_TRY(hipStreamSynchronize(s));
+  tmp.release();
+  return rc;
+}
+
+int ants_op_smooth_field(const ants_image_desc* gdsc, const float* d_in, float* d_out, double variance,
+                         void* stream) {
+  HostGrid g;
+  ANTS_TRY(to_host_grid(gdsc, &g));
+  const size_t b = static_cast<size_t>(g.count()) * g.D * sizeof(float);
+  DevBuf a, w, m;
+  ANTS_TRY(a.ensure(b));
+  ANTS_TRY(w.ensure(b));
+  ANTS_TRY(m.ensure(16));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  HIP_TRY(hipMemcpyAsync(a.p, d_in, b, hipMemcpyDeviceToDevice, s));
+  float* in[1] = {a.f()};
+  float* work[1] = {w.f()};
+  float* out[1] = {d_out};
+  HIP_TRY(hipMemsetAsync(m.p, 0, 16, s));
+  int rc = smooth_fields_on(s, g, 1, in, work, out, variance, static_cast<unsigned int*>(m.p));
+  if (rc == ANTS_OK) HIP_TRY(hipStreamSynchronize(s));
+  return rc;
+}
+
+int ants_op_warp_image(const ants_image_desc* gi, const float* d_img, const ants_image_desc* go, const float* d_field,
+                       float* d_out, void* stream) {
+  HostGrid a, b;
+  ANTS_TRY(to_host_grid(gi, &a));
+  ANTS_TRY(to_host_grid(go, &b));
+  if (a.D != b.D) return fail(ANTS_ERR_INVALID, "dimension mismatch");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  launch_warp_image(to_dev(a), d_img, to_dev(b), d_field, d_out, s);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(s));
+  return ANTS_OK;
+}
+
+int ants_op_cc(const ants_image_desc* gdsc, const float* d_fixed, const float* d_moving, int radius, float* d_g_f2m,
+               float* d_g_m2m, double* host_value, void* stream) {
+  HostGrid g;
+  ANTS_TRY(to_host_grid(gdsc, &g));
+  if (radius < 1 || radius > 16) return fail(ANTS_ERR_INVALID, "radius must be in 1..16");
+  const GridDev gd = to_dev(g);
+  OpScratch sc;
+  ANTS_TRY(sc.stats.ensure(sizeof(RedSlot)));
+  ANTS_TRY(sc.partials.ensure(cc_partials_needed(gd, radius) * sizeof(double)));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  HIP_TRY(hipMemsetAsync(sc.stats.p, 0, sizeof(RedSlot), s));
+  launch_cc(gd, d_fixed, d_moving, radius, std::numeric_limits<float>::epsilon(), d_g_f2m, d_g_m2m,
+            static_cast<double*>(sc.partials.p), static_cast<RedSlot*>(sc.stats.p), s);
+  HIP_TRY(hipGetLastError());
+  RedSlot r;
+  HIP_TRY(hipMemcpyAsync(&r, sc.stats.p, sizeof(r), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (host_value) *host_value = -r.sum / static_cast<double>(g.count());
+  return ANTS_OK;
+}
+
+int ants_op_compose(const ants_image_desc* gdsc, const float* d_disp, const float* d_warp, float* d_out,
+                    void* stream) {
+  HostGrid g;
+  ANTS_TRY(to_host_grid(gdsc, &g));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const float* disp[1] = {d_disp};
+  const float* warp[1] = {d_warp};
+  float* out[1] = {d_out};
+  launch_compose(to_dev(g), 1, disp, warp, out, nullptr, 1.f, s);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(s));
+  return ANTS_OK;
+}
+
+int ants_op_scale_update(const ants_image_desc* gdsc, float* d_u, double rate, double* host_max, void* stream) {
+  HostGrid g;
+  ANTS_TRY(to_host_grid(gdsc, &g));
+  const size_t b = static_cast<size_t>(g.count()) * g.D * sizeof(float);
+  DevBuf a, zero, m;
+  ANTS_TRY(a.ensure(b));
+  ANTS_TRY(zero.ensure(b));
+  ANTS_TRY(m.ensure(16));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  HIP_TRY(hipMemsetAsync(m.p, 0, 16, s));
+  HIP_TRY(hipMemsetAsync(zero.p, 0, b, s));
+  Taps t{};
+  t.r = 0;
+  t.c[0] = 1.f;
+  const GridDev gd = to_dev(g);
+  // copy + max norm, then u = 0 + (rate/max) * copy(x + 0): the same scaling the engine fuses into composition
+  const float* in[1] = {d_u};
+  float* out[1] = {a.f()};
+  launch_gauss_field_final(gd, 1, in, in, out, t, 0, 1.f, 0.f, false, static_cast<unsigned int*>(m.p), s);
+  const float* disp[1] = {a.f()};
+  const float* warp[1] = {zero.f()};
+  float* res[1] = {d_u};
+  launch_compose(gd, 1, disp, warp, res, static_cast<unsigned int*>(m.p), static_cast<float>(rate), s);
+  HIP_TRY(hipGetLastError());
+  unsigned int bits = 0;
+  HIP_TRY(hipMemcpyAsync(&bits, m.p, 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  float mx;
+  std::memcpy(&mx, &bits, 4);
+  if (host_max) *host_max = mx;
+  return ANTS_OK;
+}
+
+int ants_op_invert(const ants_image_desc* gdsc, const float* d_field, const float* d_init, float* d_inv,
+                   int* host_iters, double* host_max, double* host_mean, void* stream) {
+  HostGrid g;
+  ANTS_TRY(to_host_grid(gdsc, &g));
+  const GridDev gd = to_dev(g);
+  const size_t b = static_cast<size_t>(g.count()) * g.D * sizeof(float);
+  OpScratch sc;
+  ANTS_TRY(sc.stats.ensure(sizeof(RedSlot) * kInvSlots));
+  ANTS_TRY(sc.partials.ensure(inv_partials_needed(gd) * sizeof(double)));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  HIP_TRY(hipMemsetAsync(sc.stats.p, 0, sizeof(RedSlot) * kInvSlots, s));
+  if (d_init)
+    HIP_TRY(hipMemcpyAsync(d_inv, d_init, b, hipMemcpyDeviceToDevice, s));
+  else
+    HIP_TRY(hipMemsetAsync(d_inv, 0, b, s));
+  const float* field[1] = {d_field};
+  float* inv[1] = {d_inv};
+  RedSlot* slots[1] = {static_cast<RedSlot*>(sc.stats.p)};
+  for (int j = 0; j <= kInvMaxIter; ++j)
+    launch_inv_step(gd, 1, field, inv, slots, static_cast<double*>(sc.partials.p), j, s);
+  HIP_TRY(hipGetLastError());
+  RedSlot r[kInvSlots];
+  HIP_TRY(hipMemcpyAsync(r, sc.stats.p, sizeof(r), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  int k = 0;
+  for (int j = 0; j < kInvSlots; ++j)
+    if (r[j].computed) k = j + 1;
+  if (host_iters) *host_iters = k;
+  if (k > 0) {
+    float mx;
+    std::memcpy(&mx, &r[k - 1].maxbits, 4);
+    if (host_max) *host_max = mx;
+    if (host_mean) *host_mean = r[k - 1].sum / static_cast<double>(g.count());
+  }
+  return ANTS_OK;
+}
+
+int ants_op_resample_field(const ants_image_desc* gi, const float* d_in, const ants_image_desc* go, float* d_out,
+                           void* stream) {
+  HostGrid a, b;
+  ANTS_TRY(to_host_grid(gi, &a));
+  ANTS_TRY(to_host_grid(go, &b));
+  if (a.D != b.D) return fail(ANTS_ERR_INVALID, "dimension mismatch");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const float* in[1] = {d_in};
+  float* out[1] = {d_out};
+  launch_resample_fields(to_dev(a), to_dev(b), 1, in, out, s);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(s));
+  return ANTS_OK;
+}
+
+int ants_op_preprocess(int64_t n, const float* d_in, float* d_out, void* stream) {
+  if (n < 1) return fail(ANTS_ERR_INVALID, "empty image");
+  DevBuf mm;
+  uint32_t pair[2];
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int rc = preprocess_on(s, mm, d_in, d_out, n, pair);
+  if (rc == ANTS_OK) HIP_TRY(hipStreamSynchronize(s));
+  mm.release();
+  return rc;
+}
+
+int ants_apply_transform_labels_nn(const ants_image_desc* go, const float* d_field, const ants_image_desc* gl,
+                                   const int32_t* d_labels, int32_t* d_out, void* stream) {
+  HostGrid a, b;
+  ANTS_TRY(to_host_grid(go, &a));
+  ANTS_TRY(to_host_grid(gl, &b));
+  if (a.D != b.D) return fail(ANTS_ERR_INVALID, "dimension mismatch");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  launch_labels_nn(to_dev(a), d_field, to_dev(b), d_labels, d_out, s);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(s));
+  return ANTS_OK;
+}
+
+}  // extern "C"

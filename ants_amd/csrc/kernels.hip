@@ -1,0 +1,894 @@
+// SyN + CC kernels for MI355X (gfx950, wave64).
+//
+// Every kernel here is a stencil, gather or elementwise pass over fields of
+// a few bytes per voxel, so the path is HBM-bound and uses no MFMA.  Global
+// reductions are deterministic: each block writes one partial, the last
+// block to finish sums the partials in block order.
+#include "kernels.hpp"
+
+#include <hip/hip_runtime.h>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+
+namespace ants {
+namespace {
+
+constexpr int kBlock = 256;
+
+struct PtrPack4 {
+  const float* p[4];
+};
+struct MutPack4 {
+  float* p[4];
+};
+struct SlotPack4 {
+  RedSlot* p[4];
+};
+
+__device__ __forceinline__ int64_t lin(const GridDev& g, int x, int y, int z) {
+  return static_cast<int64_t>(x) + static_cast<int64_t>(g.n[0]) * (y + static_cast<int64_t>(g.n[1]) * z);
+}
+
+__device__ __forceinline__ void unlin(const GridDev& g, int64_t i, int& x, int& y, int& z) {
+  x = static_cast<int>(i % g.n[0]);
+  int64_t r = i / g.n[0];
+  y = static_cast<int>(r % g.n[1]);
+  z = static_cast<int>(r / g.n[1]);
+}
+
+__device__ __forceinline__ bool on_boundary(const GridDev& g, int x, int y, int z) {
+  bool b = x == 0 || x == g.n[0] - 1 || y == 0 || y == g.n[1] - 1;
+  if (g.D == 3) b = b || z == 0 || z == g.n[2] - 1;
+  return b;
+}
+
+// Continuous index in grid `gi` of the point (level-grid voxel of `go`) + u.
+template <int D>
+__device__ __forceinline__ void cidx_of(const GridDev& go, int x, int y, int z, const float* u, const GridDev& gi,
+                                        float c[3]) {
+  const int ix[3] = {x, y, z};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (d < D) {
+      float p = go.org[d] + static_cast<float>(ix[d]) * go.sp[d];
+      if (u) p += u[d];
+      c[d] = (p - gi.org[d]) / gi.sp[d];
+    } else {
+      c[d] = 0.f;
+    }
+  }
+}
+
+// ITK ImageFunction::IsInsideBuffer on a continuous index: -0.5 <= c < n - 0.5.
+template <int D>
+__device__ __forceinline__ bool inside(const GridDev& g, const float c[3]) {
+#pragma unroll
+  for (int d = 0; d < D; ++d)
+    if (!(c[d] >= -0.5f && c[d] < static_cast<float>(g.n[d]) - 0.5f)) return false;
+  return true;
+}
+
+// Linear interpolation of NC interleaved components at an inside point:
+// neighbours clamped to the buffer (ITK Linear/VectorLinear interpolators).
+template <int D, int NC>
+__device__ __forceinline__ void interp(const GridDev& g, const float* __restrict__ img, const float c[3], float* v) {
+  int lo[3], hi[3];
+  float w[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (d < D) {
+      float f = floorf(c[d]);
+      int b = static_cast<int>(f);
+      float t = c[d] - f;
+      if (b < 0) {
+        b = 0;
+        t = 0.f;
+      }
+      lo[d] = b;
+      hi[d] = b + 1 < g.n[d] ? b + 1 : b;
+      w[d] = t;
+    } else {
+      lo[d] = hi[d] = 0;
+      w[d] = 0.f;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NC; ++k) v[k] = 0.f;
+  const int64_t sy = g.n[0], sz = static_cast<int64_t>(g.n[0]) * g.n[1];
+  if (D == 2) {
+    const int64_t i00 = lo[0] + sy * lo[1], i10 = hi[0] + sy * lo[1];
+    const int64_t i01 = lo[0] + sy * hi[1], i11 = hi[0] + sy * hi[1];
+    const float wx = w[0], wy = w[1];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+      const float a = img[i00 * NC + k] + (img[i10 * NC + k] - img[i00 * NC + k]) * wx;
+      const float b = img[i01 * NC + k] + (img[i11 * NC + k] - img[i01 * NC + k]) * wx;
+      v[k] = a + (b - a) * wy;
+    }
+  } else {
+    const int64_t z0 = sz * lo[2], z1 = sz * hi[2], y0 = sy * lo[1], y1 = sy * hi[1];
+    const int64_t i000 = lo[0] + y0 + z0, i100 = hi[0] + y0 + z0, i010 = lo[0] + y1 + z0, i110 = hi[0] + y1 + z0;
+    const int64_t i001 = lo[0] + y0 + z1, i101 = hi[0] + y0 + z1, i011 = lo[0] + y1 + z1, i111 = hi[0] + y1 + z1;
+    const float wx = w[0], wy = w[1], wz = w[2];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+      const float a00 = img[i000 * NC + k] + (img[i100 * NC + k] - img[i000 * NC + k]) * wx;
+      const float a10 = img[i010 * NC + k] + (img[i110 * NC + k] - img[i010 * NC + k]) * wx;
+      const float a01 = img[i001 * NC + k] + (img[i101 * NC + k] - img[i001 * NC + k]) * wx;
+      const float a11 = img[i011 * NC + k] + (img[i111 * NC + k] - img[i011 * NC + k]) * wx;
+      const float b0 = a00 + (a10 - a00) * wy;
+      const float b1 = a01 + (a11 - a01) * wy;
+      v[k] = b0 + (b1 - b0) * wz;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Deterministic block + grid reduction.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// Block-level: returns the block total in thread 0 (fixed order).
+template <int NT>
+__device__ __forceinline__ void block_sum_max(double& s, float& m) {
+  __shared__ double ss[NT / 64];
+  __shared__ float sm[NT / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  s = wave_sum(s);
+  m = wave_max(m);
+  if (lane == 0) {
+    ss[wid] = s;
+    sm[wid] = m;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    float mm = 0.f;
+    for (int w = 0; w < NT / 64; ++w) {
+      t += ss[w];
+      mm = fmaxf(mm, sm[w]);
+    }
+    s = t;
+    m = mm;
+  }
+  __syncthreads();
+}
+
+// Every block of the (sub)grid calls this once with its per-thread values.
+template <int NT>
+__device__ void grid_reduce(double s, float m, double* partials, RedSlot* slot, int nblocks, int bid) {
+  __shared__ int is_last;
+  block_sum_max<NT>(s, m);
+  if (threadIdx.x == 0) {
+    partials[bid] = s;
+    atomicMax(&slot->maxbits, __float_as_uint(m));
+    __threadfence();
+    const unsigned int t = atomicAdd(&slot->ticket, 1u);
+    is_last = (t == static_cast<unsigned int>(nblocks - 1));
+  }
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nblocks; i += NT)
+    acc += __hip_atomic_load(&partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  float dummy = 0.f;
+  block_sum_max<NT>(acc, dummy);
+  if (threadIdx.x == 0) {
+    slot->sum = acc;
+    slot->computed = 1;
+    slot->ticket = 0u;
+  }
+}
+
+// ---------------------------------------------------------------------------
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, int64_t n) {
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+
+__device__ __forceinline__ uint32_t float_order(float f) {
+  uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__global__ void __launch_bounds__(kBlock) k_minmax(const float* __restrict__ in, int64_t n, uint32_t* out2) {
+  uint32_t mn = 0xffffffffu, mx = 0u;
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const uint32_t o = float_order(in[i]);
+    mn = min(mn, o);
+    mx = max(mx, o);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    mn = min(mn, static_cast<uint32_t>(__shfl_xor(static_cast<int>(mn), off, 64)));
+    mx = max(mx, static_cast<uint32_t>(__shfl_xor(static_cast<int>(mx), off, 64)));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMin(&out2[0], mn);
+    atomicMax(&out2[1], mx);
+  }
+}
+
+// IntensityWindowingImageFilter: clamp to [lo,hi], x*scale + shift in double.
+__global__ void k_window(const float* __restrict__ in, float* __restrict__ out, int64_t n, float lo, float hi,
+                         double scale, double shift) {
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const float x = in[i];
+    float v;
+    if (x < lo)
+      v = 0.f;
+    else if (x > hi)
+      v = 1.f;
+    else
+      v = static_cast<float>(static_cast<double>(x) * scale + shift);
+    out[i] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Separable Gaussian along one axis, edge replicated (ZeroFluxNeumann).
+template <int NC>
+__global__ void __launch_bounds__(kBlock) k_gauss_axis(GridDev g, const float* __restrict__ in, float* __restrict__ out,
+                                                       Taps t, int axis) {
+  const int64_t N = g.count();
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i >= N) return;
+  int x, y, z;
+  unlin(g, i, x, y, z);
+  const int pos = axis == 0 ? x : (axis == 1 ? y : z);
+  const int len = g.n[axis];
+  const int64_t stride = axis == 0 ? 1 : (axis == 1 ? g.n[0] : static_cast<int64_t>(g.n[0]) * g.n[1]);
+  const int64_t base = i - static_cast<int64_t>(pos) * stride;
+  float acc[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) acc[c] = 0.f;
+  for (int k = -t.r; k <= t.r; ++k) {
+    int p = pos + k;
+    p = p < 0 ? 0 : (p >= len ? len - 1 : p);
+    const float w = t.c[k < 0 ? -k : k];
+    const float* src = in + (base + p * stride) * NC;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) acc[c] += w * src[c];
+  }
+#pragma unroll
+  for (int c = 0; c < NC; ++c) out[i * NC + c] = acc[c];
+}
+
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_gauss_field_final(GridDev g, PtrPack4 in, PtrPack4 orig, MutPack4 out,
+                                                              Taps t, int axis, float w1, float w2,
+                                                              int boundary, unsigned int* maxbits) {
+  const int k = blockIdx.y;
+  const float* __restrict__ src = in.p[k];
+  const float* __restrict__ org = orig.p[k];
+  float* __restrict__ dst = out.p[k];
+  const int64_t N = g.count();
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  float nrm = 0.f;
+  if (i < N) {
+    int x, y, z;
+    unlin(g, i, x, y, z);
+    const int pos = axis == 0 ? x : (axis == 1 ? y : z);
+    const int len = g.n[axis];
+    const int64_t stride = axis == 0 ? 1 : (axis == 1 ? g.n[0] : static_cast<int64_t>(g.n[0]) * g.n[1]);
+    const int64_t base = i - static_cast<int64_t>(pos) * stride;
+    float acc[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) acc[c] = 0.f;
+    for (int kk = -t.r; kk <= t.r; ++kk) {
+      int p = pos + kk;
+      p = p < 0 ? 0 : (p >= len ? len - 1 : p);
+      const float w = t.c[kk < 0 ? -kk : kk];
+      const float* s = src + (base + p * stride) * D;
+#pragma unroll
+      for (int c = 0; c < D; ++c) acc[c] += w * s[c];
+    }
+    const bool bnd = boundary && on_boundary(g, x, y, z);
+    float s2 = 0.f;
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      const float v = bnd ? 0.f : (w2 != 0.f ? acc[c] * w1 + org[i * D + c] * w2 : acc[c] * w1);
+      dst[i * D + c] = v;
+      const float q = v / g.sp[c];
+      s2 += q * q;
+    }
+    nrm = sqrtf(s2);
+  }
+  nrm = wave_max(nrm);
+  if ((threadIdx.x & 63) == 0) atomicMax(&maxbits[k], __float_as_uint(nrm));
+}
+
+// ---------------------------------------------------------------------------
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_warp_pair(GridDev gl, GridDev gF, const float* __restrict__ Fs,
+                                                      const float* __restrict__ uF, float* __restrict__ Fw,
+                                                      GridDev gM, const float* __restrict__ Ms,
+                                                      const float* __restrict__ uM, float* __restrict__ Mw) {
+  const int64_t N = gl.count();
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i >= N) return;
+  int x, y, z;
+  unlin(gl, i, x, y, z);
+  float u[3], c[3], v;
+#pragma unroll
+  for (int d = 0; d < D; ++d) u[d] = uF[i * D + d];
+  cidx_of<D>(gl, x, y, z, u, gF, c);
+  v = 0.f;
+  if (inside<D>(gF, c)) interp<D, 1>(gF, Fs, c, &v);
+  Fw[i] = v;
+#pragma unroll
+  for (int d = 0; d < D; ++d) u[d] = uM[i * D + d];
+  cidx_of<D>(gl, x, y, z, u, gM, c);
+  v = 0.f;
+  if (inside<D>(gM, c)) interp<D, 1>(gM, Ms, c, &v);
+  Mw[i] = v;
+}
+
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_warp_image(GridDev gi, const float* __restrict__ img, GridDev go,
+                                                       const float* __restrict__ u, float* __restrict__ out) {
+  const int64_t N = go.count();
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i >= N) return;
+  int x, y, z;
+  unlin(go, i, x, y, z);
+  float uu[3] = {0.f, 0.f, 0.f}, c[3], v = 0.f;
+  if (u) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) uu[d] = u[i * D + d];
+  }
+  cidx_of<D>(go, x, y, z, uu, gi, c);
+  if (inside<D>(gi, c)) interp<D, 1>(gi, img, c, &v);
+  out[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// CentralDifferenceImageFunction component on grid values (half-voxel linear
+// samples): 0 at the last index, a half difference at index 0.
+__device__ __forceinline__ float cdiff(float vm, float v0, float vp, int pos, int len, float h) {
+  if (pos >= len - 1) return 0.f;
+  const float up = v0 + (vp - v0) * 0.5f;
+  const float dn = pos == 0 ? v0 : vm + (v0 - vm) * 0.5f;
+  return (up - dn) / h;
+}
+
+// Per-voxel CC value and both SyN derivatives from window sums.
+struct CCOut {
+  float cc;
+  float a1, a2;  // g_f2m = a1 * grad m ; g_m2m = a2 * grad f
+};
+__device__ __forceinline__ CCOut cc_voxel(float cnt, float sf, float sm, float sff, float smm, float sfm, float f,
+                                          float m, float eps) {
+  const float mf = sf / cnt, mm = sm / cnt;
+  const float sFF = sff - mf * sf - mf * sf + cnt * mf * mf;
+  const float sMM = smm - mm * sm - mm * sm + cnt * mm * mm;
+  const float sFM = sfm - mm * sf - mf * sm + cnt * mm * mf;
+  const float fA = f - mf, mA = m - mm;
+  CCOut o;
+  o.cc = 1.f;
+  o.a1 = 0.f;
+  o.a2 = 0.f;
+  if (sFF > eps && sMM > eps) {
+    const float p = sFF * sMM;
+    o.cc = sFM * sFM / p;
+    const float a = 2.f * sFM / p;
+    o.a1 = a * (fA - sFM / sMM * mA);
+    o.a2 = a * (mA - sFM / sFF * fA);
+  }
+  return o;
+}
+
+// Direct CC (any D): each thread sums its (2r+1)^D window from global memory.
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_cc_direct(GridDev g, const float* __restrict__ F,
+                                                      const float* __restrict__ M, int R, float eps,
+                                                      float* __restrict__ g1, float* __restrict__ g2,
+                                                      double* partials, RedSlot* slot) {
+  const int64_t N = g.count();
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  double cc_sum = 0.0;
+  if (i < N) {
+    int x, y, z;
+    unlin(g, i, x, y, z);
+    const int zlo = D == 3 ? max(z - R, 0) : 0, zhi = D == 3 ? min(z + R, g.n[2] - 1) : 0;
+    const int ylo = max(y - R, 0), yhi = min(y + R, g.n[1] - 1);
+    const int xlo = max(x - R, 0), xhi = min(x + R, g.n[0] - 1);
+    float sf = 0.f, sm = 0.f, sff = 0.f, smm = 0.f, sfm = 0.f;
+    for (int zz = zlo; zz <= zhi; ++zz)
+      for (int yy = ylo; yy <= yhi; ++yy)
+        for (int xx = xlo; xx <= xhi; ++xx) {
+          const int64_t j = lin(g, xx, yy, zz);
+          const float f = F[j], m = M[j];
+          sf += f;
+          sm += m;
+          sff += f * f;
+          smm += m * m;
+          sfm += f * m;
+        }
+    const float cnt = static_cast<float>((zhi - zlo + 1) * (yhi - ylo + 1) * (xhi - xlo + 1));
+    const float f = F[i], m = M[i];
+    const CCOut o = cc_voxel(cnt, sf, sm, sff, smm, sfm, f, m, eps);
+    cc_sum = o.cc;
+    const int pos[3] = {x, y, z};
+    const int64_t st[3] = {1, g.n[0], static_cast<int64_t>(g.n[0]) * g.n[1]};
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      float gm = 0.f, gf = 0.f;
+      if (o.a1 != 0.f || o.a2 != 0.f) {
+        const bool lo = pos[d] > 0, hi = pos[d] < g.n[d] - 1;
+        const float mm_ = lo ? M[i - st[d]] : m, mp = hi ? M[i + st[d]] : m;
+        const float fm_ = lo ? F[i - st[d]] : f, fp = hi ? F[i + st[d]] : f;
+        gm = cdiff(mm_, m, mp, pos[d], g.n[d], g.sp[d]);
+        gf = cdiff(fm_, f, fp, pos[d], g.n[d], g.sp[d]);
+      }
+      g1[i * D + d] = o.a1 * gm;
+      g2[i * D + d] = o.a2 * gf;
+    }
+  }
+  grid_reduce<kBlock>(cc_sum, 0.f, partials, slot, gridDim.x, blockIdx.x);
+}
+
+// 3-D CC with LDS-staged planes and a register ring along z.
+// Block = TX x TY output columns over a z-chunk.  Each step loads plane
+// zz (with an R halo in x and y) as the 5 quantities f, m, f^2, m^2, f*m,
+// forms the (2R+1)^2 in-plane box sums separably (x rows, then y), and
+// pushes them into a (2R+1)-deep register ring; the ring total is the
+// window sum for the voxel R planes behind.  Outside-volume samples are 0
+// and the count is analytic, which equals ITK's in-bounds-only sums.
+template <int R, int TX, int TY>
+__global__ void __launch_bounds__(TX* TY) k_cc3d(GridDev g, const float* __restrict__ F, const float* __restrict__ M,
+                                                 float eps, int zchunk, float* __restrict__ g1,
+                                                 float* __restrict__ g2, double* partials, RedSlot* slot) {
+  constexpr int W = 2 * R + 1;
+  constexpr int LX = TX + 2 * R, LY = TY + 2 * R;
+  constexpr int NT = TX * TY;
+  __shared__ float q[5][LY][LX];
+  __shared__ float sx[5][LY][TX];
+  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+  const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+  const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, z0 = blockIdx.z * zchunk;
+  const int z1 = min(z0 + zchunk, nz);
+  const int x = x0 + tx, y = y0 + ty;
+  const bool active = x < nx && y < ny;
+  const float cx = static_cast<float>(min(x + R, nx - 1) - max(x - R, 0) + 1);
+  const float cy = static_cast<float>(min(y + R, ny - 1) - max(y - R, 0) + 1);
+
+  float ring[5][W];
+#pragma unroll
+  for (int a = 0; a < 5; ++a)
+#pragma unroll
+    for (int b = 0; b < W; ++b) ring[a][b] = 0.f;
+  float rf[R + 2], rm[R + 2];                                // raw centre ring (newest last)
+  float rgf[R + 1][2], rgm[R + 1][2];                        // in-plane gradients ring (newest last)
+#pragma unroll
+  for (int b = 0; b < R + 2; ++b) rf[b] = rm[b] = 0.f;
+#pragma unroll
+  for (int b = 0; b < R + 1; ++b) rgf[b][0] = rgf[b][1] = rgm[b][0] = rgm[b][1] = 0.f;
+  double cc_sum = 0.0;
+  const int64_t plane = static_cast<int64_t>(nx) * ny;
+
+  for (int zz = z0 - R; zz < z1 + R; ++zz) {
+    const bool zin = zz >= 0 && zz < nz;
+    __syncthreads();
+    for (int k = threadIdx.x; k < LX * LY; k += NT) {
+      const int ly = k / LX, lx = k - ly * LX;
+      const int gx = x0 - R + lx, gy = y0 - R + ly;
+      float f = 0.f, m = 0.f;
+      if (zin && gx >= 0 && gx < nx && gy >= 0 && gy < ny) {
+        const int64_t j = static_cast<int64_t>(zz) * plane + static_cast<int64_t>(gy) * nx + gx;
+        f = F[j];
+        m = M[j];
+      }
+      q[0][ly][lx] = f;
+      q[1][ly][lx] = m;
+      q[2][ly][lx] = f * f;
+      q[3][ly][lx] = m * m;
+      q[4][ly][lx] = f * m;
+    }
+    __syncthreads();
+    for (int ly = ty; ly < LY; ly += TY) {
+#pragma unroll
+      for (int a = 0; a < 5; ++a) {
+        float s = 0.f;
+#pragma unroll
+        for (int b = 0; b < W; ++b) s += q[a][ly][tx + b];
+        sx[a][ly][tx] = s;
+      }
+    }
+    // centre values and in-plane gradients of plane zz at (x, y)
+    const float fc = q[0][ty + R][tx + R], mc = q[1][ty + R][tx + R];
+    float gfx = 0.f, gfy = 0.f, gmx = 0.f, gmy = 0.f;
+    if (active && zin) {
+      gfx = cdiff(q[0][ty + R][tx + R - 1], fc, q[0][ty + R][tx + R + 1], x, nx, g.sp[0]);
+      gmx = cdiff(q[1][ty + R][tx + R - 1], mc, q[1][ty + R][tx + R + 1], x, nx, g.sp[0]);
+      gfy = cdiff(q[0][ty + R - 1][tx + R], fc, q[0][ty + R + 1][tx + R], y, ny, g.sp[1]);
+      gmy = cdiff(q[1][ty + R - 1][tx + R], mc, q[1][ty + R + 1][tx + R], y, ny, g.sp[1]);
+    }
+    __syncthreads();
+    float p[5];
+#pragma unroll
+    for (int a = 0; a < 5; ++a) {
+      float s = 0.f;
+#pragma unroll
+      for (int b = 0; b < W; ++b) s += sx[a][ty + b][tx];
+      p[a] = s;
+    }
+#pragma unroll
+    for (int a = 0; a < 5; ++a) {
+#pragma unroll
+      for (int b = 0; b < W - 1; ++b) ring[a][b] = ring[a][b + 1];
+      ring[a][W - 1] = p[a];
+    }
+#pragma unroll
+    for (int b = 0; b < R + 1; ++b) {
+      rf[b] = rf[b + 1];
+      rm[b] = rm[b + 1];
+    }
+    rf[R + 1] = fc;
+    rm[R + 1] = mc;
+#pragma unroll
+    for (int b = 0; b < R; ++b) {
+      rgf[b][0] = rgf[b + 1][0];
+      rgf[b][1] = rgf[b + 1][1];
+      rgm[b][0] = rgm[b + 1][0];
+      rgm[b][1] = rgm[b + 1][1];
+    }
+    rgf[R][0] = gfx;
+    rgf[R][1] = gfy;
+    rgm[R][0] = gmx;
+    rgm[R][1] = gmy;
+
+    const int zc = zz - R;
+    if (active && zc >= z0 && zc < z1) {
+      float s[5];
+#pragma unroll
+      for (int a = 0; a < 5; ++a) {
+        float t = 0.f;
+#pragma unroll
+        for (int b = 0; b < W; ++b) t += ring[a][b];
+        s[a] = t;
+      }
+      const float cz = static_cast<float>(min(zc + R, nz - 1) - max(zc - R, 0) + 1);
+      // raw centre column: rf[R+1] = zz = zc+R ... rf[1] = zc, rf[0] = zc-1, rf[2] = zc+1
+      const float f = rf[1], m = rm[1];
+      const CCOut o = cc_voxel(cx * cy * cz, s[0], s[1], s[2], s[3], s[4], f, m, eps);
+      cc_sum += o.cc;
+      float gmz = 0.f, gfz = 0.f;
+      if (o.a1 != 0.f || o.a2 != 0.f) {
+        gmz = cdiff(zc > 0 ? rm[0] : m, m, rm[2], zc, nz, g.sp[2]);
+        gfz = cdiff(zc > 0 ? rf[0] : f, f, rf[2], zc, nz, g.sp[2]);
+      }
+      const int64_t i = static_cast<int64_t>(zc) * plane + static_cast<int64_t>(y) * nx + x;
+      g1[i * 3 + 0] = o.a1 * rgm[0][0];
+      g1[i * 3 + 1] = o.a1 * rgm[0][1];
+      g1[i * 3 + 2] = o.a1 * gmz;
+      g2[i * 3 + 0] = o.a2 * rgf[0][0];
+      g2[i * 3 + 1] = o.a2 * rgf[0][1];
+      g2[i * 3 + 2] = o.a2 * gfz;
+    }
+  }
+  const int nb = gridDim.x * gridDim.y * gridDim.z;
+  const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  grid_reduce<NT>(cc_sum, 0.f, partials, slot, nb, bid);
+}
+
+// ---------------------------------------------------------------------------
+// out = warp + scale * disp(x + warp(x)), 0 outside the buffer of disp.
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_compose(GridDev g, PtrPack4 disp, PtrPack4 warp, MutPack4 out,
+                                                    const unsigned int* maxbits, float rate) {
+  const int k = blockIdx.y;
+  const float* __restrict__ d = disp.p[k];
+  const float* __restrict__ w = warp.p[k];
+  float* __restrict__ o = out.p[k];
+  const int64_t N = g.count();
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i >= N) return;
+  float scale = 1.f;
+  if (maxbits) {
+    const float mx = __uint_as_float(maxbits[k]);
+    scale = mx > 0.f ? rate / mx : rate;
+  }
+  int x, y, z;
+  unlin(g, i, x, y, z);
+  float wv[3] = {0.f, 0.f, 0.f}, c[3], v[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int dd = 0; dd < D; ++dd) wv[dd] = w[i * D + dd];
+  cidx_of<D>(g, x, y, z, wv, g, c);
+  if (inside<D>(g, c)) interp<D, D>(g, d, c, v);
+#pragma unroll
+  for (int dd = 0; dd < D; ++dd) o[i * D + dd] = wv[dd] + scale * v[dd];
+}
+
+// ---------------------------------------------------------------------------
+// InvertDisplacementFieldImageFilter, one fused step.
+// Step j = 0 computes the norms n_0 of e_0 = inv_0 + field(x + inv_0).
+// Step j >= 1 runs only if n_{j-1} was computed: it recomputes e_{j-1}
+// (bitwise the same as in step j-1), applies update j-1
+//   inv_j = inv_{j-1} - eps * e_{j-1} * min(1, eps*max_{j-1}/|e_{j-1}|)
+// (eps 0.75 for the first update, 0.5 after; boundary voxels zeroed), and,
+// if j < 20 and max_{j-1} > 0.1 and mean_{j-1} > 0.001, the norms n_j of the
+// new inverse.  In-place on inv: each voxel reads and writes only itself.
+template <int D>
+__device__ __forceinline__ float inv_residual(const GridDev& g, const float* __restrict__ field, int x, int y, int z,
+                                              const float iv[3], float e[3]) {
+  float c[3], v[3] = {0.f, 0.f, 0.f};
+  cidx_of<D>(g, x, y, z, iv, g, c);
+  if (inside<D>(g, c)) interp<D, D>(g, field, c, v);
+  float s2 = 0.f;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    e[d] = iv[d] + v[d];
+    const float q = e[d] / g.sp[d];
+    s2 += q * q;
+  }
+  return sqrtf(s2);
+}
+
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_inv_step(GridDev g, PtrPack4 field, MutPack4 inv, SlotPack4 slots,
+                                                     double* partials, int64_t part_stride, int j) {
+  const int k = blockIdx.y;
+  RedSlot* st = slots.p[k];
+  bool do_update = false, do_norm = false;
+  float maxprev = 0.f, eps = 0.f;
+  if (j == 0) {
+    do_norm = true;
+  } else {
+    const RedSlot& prev = st[j - 1];
+    if (prev.computed) {
+      do_update = true;
+      maxprev = __uint_as_float(prev.maxbits);
+      const double mean = prev.sum / static_cast<double>(g.count());
+      eps = (j - 1 == 0) ? 0.75f : 0.5f;
+      do_norm = j < kInvMaxIter && maxprev > 0.1f && mean > 0.001;
+    }
+  }
+  if (!do_update && !do_norm) return;
+  const float* __restrict__ fld = field.p[k];
+  float* __restrict__ iv = inv.p[k];
+  const int64_t N = g.count();
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  double nsum = 0.0;
+  float nmax = 0.f;
+  if (i < N) {
+    int x, y, z;
+    unlin(g, i, x, y, z);
+    float cur[3] = {0.f, 0.f, 0.f}, e[3];
+#pragma unroll
+    for (int d = 0; d < D; ++d) cur[d] = iv[i * D + d];
+    if (do_update) {
+      const float nrm = inv_residual<D>(g, fld, x, y, z, cur, e);
+      const float lim = eps * maxprev;
+      float sc = 1.f;
+      if (nrm > lim) sc = lim / nrm;
+      const bool bnd = on_boundary(g, x, y, z);
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        cur[d] = bnd ? 0.f : cur[d] + (-e[d] * sc) * eps;
+        iv[i * D + d] = cur[d];
+      }
+    }
+    if (do_norm) {
+      const float nrm = inv_residual<D>(g, fld, x, y, z, cur, e);
+      nsum = nrm;
+      nmax = nrm;
+    }
+  }
+  if (do_norm)
+    grid_reduce<kBlock>(nsum, nmax, partials + k * part_stride, &st[j], gridDim.x, blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_resample_fields(GridDev gi, GridDev go, PtrPack4 in, MutPack4 out) {
+  const int k = blockIdx.y;
+  const int64_t N = go.count();
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i >= N) return;
+  int x, y, z;
+  unlin(go, i, x, y, z);
+  float c[3], v[3] = {0.f, 0.f, 0.f};
+  cidx_of<D>(go, x, y, z, nullptr, gi, c);
+  if (inside<D>(gi, c)) interp<D, D>(gi, in.p[k], c, v);
+#pragma unroll
+  for (int d = 0; d < D; ++d) out.p[k][i * D + d] = v[d];
+}
+
+// antsApplyTransforms NearestNeighbor: coordinates in double so that the
+// rounding decision (floor(c + 0.5), half up) matches a double CPU path bit for bit.
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_labels_nn(GridDev go, const float* __restrict__ u, GridDev gl,
+                                                      const int32_t* __restrict__ lab, int32_t* __restrict__ out) {
+  const int64_t N = go.count();
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i >= N) return;
+  int x, y, z;
+  unlin(go, i, x, y, z);
+  const int ix[3] = {x, y, z};
+  int idx[3] = {0, 0, 0};
+  bool in = true;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    double p = go.dorg[d] + static_cast<double>(ix[d]) * go.dsp[d];
+    if (u) p += static_cast<double>(u[i * D + d]);
+    const double c = (p - gl.dorg[d]) / gl.dsp[d];
+    if (!(c >= -0.5 && c < static_cast<double>(gl.n[d]) - 0.5)) in = false;
+    idx[d] = static_cast<int>(floor(c + 0.5));
+  }
+  int32_t v = 0;
+  if (in) v = lab[idx[0] + static_cast<int64_t>(gl.n[0]) * (idx[1] + static_cast<int64_t>(gl.n[1]) * idx[2])];
+  out[i] = v;
+}
+
+inline unsigned int blocks_for(int64_t n, int bs = kBlock) { return static_cast<unsigned int>((n + bs - 1) / bs); }
+inline unsigned int grid_stride_blocks(int64_t n) {
+  int64_t b = (n + kBlock - 1) / kBlock;
+  return static_cast<unsigned int>(b < 4096 ? (b < 1 ? 1 : b) : 4096);
+}
+
+template <class A>
+PtrPack4 pack(const A* const* p, int np) {
+  PtrPack4 r{};
+  for (int k = 0; k < np; ++k) r.p[k] = p[k];
+  return r;
+}
+MutPack4 mpack(float* const* p, int np) {
+  MutPack4 r{};
+  for (int k = 0; k < np; ++k) r.p[k] = p[k];
+  return r;
+}
+
+// CC tile shapes (3-D): TX x TY columns, z-chunk depth.
+constexpr int kCCTX = 64, kCCTY = 4, kCCZ = 32;
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+void launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_u32, dim3(grid_stride_blocks(n)), dim3(kBlock), 0, s, p, v, n);
+}
+
+void launch_minmax(const float* in, int64_t n, uint32_t* out2, hipStream_t s) {
+  hipLaunchKernelGGL(k_minmax, dim3(grid_stride_blocks(n)), dim3(kBlock), 0, s, in, n, out2);
+}
+
+void launch_window(const float* in, float* out, int64_t n, float lo, float hi, double scale, double shift,
+                   hipStream_t s) {
+  hipLaunchKernelGGL(k_window, dim3(grid_stride_blocks(n)), dim3(kBlock), 0, s, in, out, n, lo, hi, scale, shift);
+}
+
+void launch_gauss_axis(const GridDev& g, int ncomp, const float* in, float* out, const Taps& t, int axis,
+                       hipStream_t s) {
+  const dim3 grid(blocks_for(g.count()));
+  if (ncomp == 1)
+    hipLaunchKernelGGL(k_gauss_axis<1>, grid, dim3(kBlock), 0, s, g, in, out, t, axis);
+  else if (ncomp == 2)
+    hipLaunchKernelGGL(k_gauss_axis<2>, grid, dim3(kBlock), 0, s, g, in, out, t, axis);
+  else
+    hipLaunchKernelGGL(k_gauss_axis<3>, grid, dim3(kBlock), 0, s, g, in, out, t, axis);
+}
+
+void launch_gauss_field_final(const GridDev& g, int np, const float* const* in, const float* const* orig,
+                              float* const* out, const Taps& t, int axis, float w1, float w2, bool boundary,
+                              unsigned int* maxbits, hipStream_t s) {
+  const dim3 grid(blocks_for(g.count()), np);
+  if (g.D == 2)
+    hipLaunchKernelGGL(k_gauss_field_final<2>, grid, dim3(kBlock), 0, s, g, pack(in, np), pack(orig, np),
+                       mpack(out, np), t, axis, w1, w2, static_cast<int>(boundary), maxbits);
+  else
+    hipLaunchKernelGGL(k_gauss_field_final<3>, grid, dim3(kBlock), 0, s, g, pack(in, np), pack(orig, np),
+                       mpack(out, np), t, axis, w1, w2, static_cast<int>(boundary), maxbits);
+}
+
+void launch_warp_pair(const GridDev& gl, const GridDev& gF, const float* Fs, const float* uF, float* Fw,
+                      const GridDev& gM, const float* Ms, const float* uM, float* Mw, hipStream_t s) {
+  const dim3 grid(blocks_for(gl.count()));
+  if (gl.D == 2)
+    hipLaunchKernelGGL(k_warp_pair<2>, grid, dim3(kBlock), 0, s, gl, gF, Fs, uF, Fw, gM, Ms, uM, Mw);
+  else
+    hipLaunchKernelGGL(k_warp_pair<3>, grid, dim3(kBlock), 0, s, gl, gF, Fs, uF, Fw, gM, Ms, uM, Mw);
+}
+
+void launch_warp_image(const GridDev& gi, const float* img, const GridDev& go, const float* u, float* out,
+                       hipStream_t s) {
+  const dim3 grid(blocks_for(go.count()));
+  if (go.D == 2)
+    hipLaunchKernelGGL(k_warp_image<2>, grid, dim3(kBlock), 0, s, gi, img, go, u, out);
+  else
+    hipLaunchKernelGGL(k_warp_image<3>, grid, dim3(kBlock), 0, s, gi, img, go, u, out);
+}
+
+static bool cc_tiled(const GridDev& g, int radius) { return g.D == 3 && (radius == 2 || radius == 4); }
+
+int64_t cc_partials_needed(const GridDev& g, int radius) {
+  if (cc_tiled(g, radius)) {
+    return static_cast<int64_t>((g.n[0] + kCCTX - 1) / kCCTX) * ((g.n[1] + kCCTY - 1) / kCCTY) *
+           ((g.n[2] + kCCZ - 1) / kCCZ);
+  }
+  return blocks_for(g.count());
+}
+
+void launch_cc(const GridDev& g, const float* F, const float* M, int radius, float eps, float* gF2M, float* gM2M,
+               double* partials, RedSlot* slot, hipStream_t s) {
+  if (!cc_tiled(g, radius)) {
+    launch_cc_direct(g, F, M, radius, eps, gF2M, gM2M, partials, slot, s);
+    return;
+  }
+  const dim3 grid((g.n[0] + kCCTX - 1) / kCCTX, (g.n[1] + kCCTY - 1) / kCCTY, (g.n[2] + kCCZ - 1) / kCCZ);
+  if (radius == 4)
+    hipLaunchKernelGGL((k_cc3d<4, kCCTX, kCCTY>), grid, dim3(kCCTX * kCCTY), 0, s, g, F, M, eps, kCCZ, gF2M, gM2M,
+                       partials, slot);
+  else
+    hipLaunchKernelGGL((k_cc3d<2, kCCTX, kCCTY>), grid, dim3(kCCTX * kCCTY), 0, s, g, F, M, eps, kCCZ, gF2M, gM2M,
+                       partials, slot);
+}
+
+void launch_cc_direct(const GridDev& g, const float* F, const float* M, int radius, float eps, float* gF2M,
+                      float* gM2M, double* partials, RedSlot* slot, hipStream_t s) {
+  const dim3 grid(blocks_for(g.count()));
+  if (g.D == 2)
+    hipLaunchKernelGGL(k_cc_direct<2>, grid, dim3(kBlock), 0, s, g, F, M, radius, eps, gF2M, gM2M, partials, slot);
+  else
+    hipLaunchKernelGGL(k_cc_direct<3>, grid, dim3(kBlock), 0, s, g, F, M, radius, eps, gF2M, gM2M, partials, slot);
+}
+
+void launch_compose(const GridDev& g, int np, const float* const* disp, const float* const* warp, float* const* out,
+                    const unsigned int* maxbits, float rate, hipStream_t s) {
+  const dim3 grid(blocks_for(g.count()), np);
+  if (g.D == 2)
+    hipLaunchKernelGGL(k_compose<2>, grid, dim3(kBlock), 0, s, g, pack(disp, np), pack(warp, np), mpack(out, np),
+                       maxbits, rate);
+  else
+    hipLaunchKernelGGL(k_compose<3>, grid, dim3(kBlock), 0, s, g, pack(disp, np), pack(warp, np), mpack(out, np),
+                       maxbits, rate);
+}
+
+int64_t inv_partials_needed(const GridDev& g) { return blocks_for(g.count()); }
+
+void launch_inv_step(const GridDev& g, int np, const float* const* field, float* const* inv, RedSlot* const* slots,
+                     double* partials, int j, hipStream_t s) {
+  const dim3 grid(blocks_for(g.count()), np);
+  SlotPack4 sp{};
+  for (int k = 0; k < np; ++k) sp.p[k] = slots[k];
+  const int64_t stride = inv_partials_needed(g);
+  if (g.D == 2)
+    hipLaunchKernelGGL(k_inv_step<2>, grid, dim3(kBlock), 0, s, g, pack(field, np), mpack(inv, np), sp, partials,
+                       stride, j);
+  else
+    hipLaunchKernelGGL(k_inv_step<3>, grid, dim3(kBlock), 0, s, g, pack(field, np), mpack(inv, np), sp, partials,
+                       stride, j);
+}
+
+void launch_resample_fields(const GridDev& gi, const GridDev& go, int np, const float* const* in, float* const* out,
+                            hipStream_t s) {
+  const dim3 grid(blocks_for(go.count()), np);
+  if (go.D == 2)
+    hipLaunchKernelGGL(k_resample_fields<2>, grid, dim3(kBlock), 0, s, gi, go, pack(in, np), mpack(out, np));
+  else
+    hipLaunchKernelGGL(k_resample_fields<3>, grid, dim3(kBlock), 0, s, gi, go, pack(in, np), mpack(out, np));
+}
+
+void launch_labels_nn(const GridDev& go, const float* u, const GridDev& gl, const int32_t* labels, int32_t* out,
+                      hipStream_t s) {
+  const dim3 grid(blocks_for(go.count()));
+  if (go.D == 2)
+    hipLaunchKernelGGL(k_labels_nn<2>, grid, dim3(kBlock), 0, s, go, u, gl, labels, out);
+  else
+    hipLaunchKernelGGL(k_labels_nn<3>, grid, dim3(kBlock), 0, s, go, u, gl, labels, out);
+}
+
+}  // namespace ants

@@ -1,0 +1,95 @@
+// Host-side launch interface of the SyN kernels (gfx950).  All launches are
+// asynchronous on the given stream; pointers are device pointers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace ants {
+
+// Level/image grid as the kernels see it: identity direction, x fastest.
+struct GridDev {
+  int D;
+  int n[3];
+  float sp[3];
+  float org[3];
+  double dsp[3];   // exact geometry for double-precision coordinate paths
+  double dorg[3];
+  __host__ __device__ int64_t count() const { return static_cast<int64_t>(n[0]) * n[1] * n[2]; }
+};
+
+constexpr int kMaxTaps = 64;  // half-kernel length limit (radius <= 63)
+struct Taps {
+  float c[kMaxTaps];  // c[0..r]
+  int r;
+};
+
+// Deterministic device reduction slot: per-block partials are summed in block
+// order by the last block to finish (ticket), max is an atomic on the bits of
+// a non-negative float.
+struct RedSlot {
+  double sum;
+  unsigned int maxbits;
+  unsigned int ticket;
+  int computed;
+  int pad;
+};
+
+constexpr int kInvMaxIter = 20;            // InvertDisplacementFieldImageFilter MaximumNumberOfIterations (SyN)
+constexpr int kInvSlots = kInvMaxIter + 1;
+
+// Per-iteration device statistics block (zeroed by k_fill at iteration start).
+struct IterStats {
+  RedSlot energy;                // CC: sum of localCC over voxels
+  unsigned int maxnorm[4];       // update-field max |u/spacing| (bits), per problem
+  RedSlot inv[4][kInvSlots];     // inversion norms per problem per inner step
+};
+
+void launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
+void launch_minmax(const float* in, int64_t n, uint32_t* out2, hipStream_t s);  // out2: ordered bits (min, max)
+void launch_window(const float* in, float* out, int64_t n, float lo, float hi, double scale, double shift,
+                   hipStream_t s);
+
+// Separable 1-D Gaussian correlation along `axis` with edge replication, ncomp interleaved components.
+void launch_gauss_axis(const GridDev& g, int ncomp, const float* in, float* out, const Taps& t, int axis,
+                       hipStream_t s);
+// Same as the last pass of SyN field smoothing for `np` fields at once:
+// along `axis`, then boundary voxels zeroed, blend w1*smooth + w2*orig,
+// and the max norm |out/spacing| of each field into maxbits[k].
+// boundary: zero the boundary voxels (SyN smoothing); false for the plain
+// copy that variance <= 0 means (then only the max norm is taken).
+void launch_gauss_field_final(const GridDev& g, int np, const float* const* in, const float* const* orig,
+                              float* const* out, const Taps& t, int axis, float w1, float w2, bool boundary,
+                              unsigned int* maxbits, hipStream_t s);
+
+// Fw = Fs(x + uF(x)), Mw = Ms(x + uM(x)) on level grid gl (Fs on gF, Ms on gM).
+void launch_warp_pair(const GridDev& gl, const GridDev& gF, const float* Fs, const float* uF, float* Fw,
+                      const GridDev& gM, const float* Ms, const float* uM, float* Mw, hipStream_t s);
+// Single image warp (u may be null).
+void launch_warp_image(const GridDev& gi, const float* img, const GridDev& go, const float* u, float* out,
+                       hipStream_t s);
+
+// CC value + both derivatives.  `partials` must hold cc_partials_needed(g) doubles.
+int64_t cc_partials_needed(const GridDev& g, int radius);
+void launch_cc(const GridDev& g, const float* F, const float* M, int radius, float eps, float* gF2M, float* gM2M,
+               double* partials, RedSlot* slot, hipStream_t s);
+void launch_cc_direct(const GridDev& g, const float* F, const float* M, int radius, float eps, float* gF2M,
+                      float* gM2M, double* partials, RedSlot* slot, hipStream_t s);
+
+// out_k = warp_k + scale_k * disp_k(x + warp_k(x)), scale_k = rate / max_k when maxbits given (max>0), else 1.
+void launch_compose(const GridDev& g, int np, const float* const* disp, const float* const* warp,
+                    float* const* out, const unsigned int* maxbits, float rate, hipStream_t s);
+
+// One fused fixed-point inversion step j (0..20) for np problems (see kernels.hip).
+int64_t inv_partials_needed(const GridDev& g);
+void launch_inv_step(const GridDev& g, int np, const float* const* field, float* const* inv, RedSlot* const* slots,
+                     double* partials, int j, hipStream_t s);
+
+// Resample np fields from grid gi onto grid go (linear, 0 outside).
+void launch_resample_fields(const GridDev& gi, const GridDev& go, int np, const float* const* in,
+                            float* const* out, hipStream_t s);
+
+void launch_labels_nn(const GridDev& go, const float* u, const GridDev& gl, const int32_t* labels, int32_t* out,
+                      hipStream_t s);
+
+}  // namespace ants

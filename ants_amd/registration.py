@@ -1,0 +1,290 @@
+"""Host-side mirror of the antsRegistration SyN stage over the gfx950 library.
+
+``SyNRegistration`` is configured the way antsRegistration configures
+RegistrationHelper for `-m CC[...] -t SyN[...] -c [...] -f ... -s ...`
+(Examples/antsRegistrationTemplateHeader.h:542-815, 985-1086) and runs
+through the C-ABI (``ants_syn_register`` / ``ants_syn_register_device``).
+Results: forward and inverse displacement fields (numpy, (z,y,x,D) /
+(y,x,D), mm) and the per-iteration observer trace
+(Examples/antsDisplacementAndVelocityFieldRegistrationCommandIterationUpdate.h:117-195).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import re
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _lib as L
+
+
+# ---------------------------------------------------------------------------
+# antsCommandLineParser option-function grammar: name[p0,p1,...]
+# (Utilities/antsCommandLineParser.cxx:87-198) and MxNxO vectors (ConvertVector).
+def parse_option_function(text: str) -> Tuple[str, List[str]]:
+    text = text.strip()
+    m = re.fullmatch(r"([^\[\]]*)\[(.*)\]", text)
+    if not m:
+        return text, []
+    name, body = m.group(1), m.group(2)
+    return name, [p.strip() for p in body.split(",")] if body.strip() else []
+
+
+def parse_vector(text: str, typ=int) -> List:
+    return [typ(v) for v in text.strip().split("x") if v != ""]
+
+
+@dataclass
+class SyNTransform:
+    """`-t SyN[gradientStep, updateFieldVariance=3, totalFieldVariance=0]`
+    (Examples/antsRegistrationTemplateHeader.h:801-815)."""
+    gradient_step: float
+    update_variance: float = 3.0
+    total_variance: float = 0.0
+
+    @classmethod
+    def parse(cls, text: str) -> "SyNTransform":
+        name, p = parse_option_function(text)
+        if name.lower() != "syn":
+            raise ValueError(f"only the SyN transform is provided here, got {name!r}")
+        if not p:
+            raise ValueError("SyN needs a gradient step: SyN[step,...]")
+        return cls(float(p[0]), float(p[1]) if len(p) > 1 else 3.0, float(p[2]) if len(p) > 2 else 0.0)
+
+
+@dataclass
+class CCMetric:
+    """`-m CC[fixed, moving, weight=1, radius=4]`
+    (Examples/antsRegistrationTemplateHeader.h:1005-1086)."""
+    fixed: str = ""
+    moving: str = ""
+    weight: float = 1.0
+    radius: int = 4
+
+    @classmethod
+    def parse(cls, text: str) -> "CCMetric":
+        name, p = parse_option_function(text)
+        if name.upper() != "CC":
+            raise ValueError(f"only the CC metric is provided here, got {name!r}")
+        if len(p) < 2:
+            raise ValueError("CC needs fixed and moving images: CC[fixed,moving,...]")
+        return cls(p[0], p[1], float(p[2]) if len(p) > 2 else 1.0, int(p[3]) if len(p) > 3 else 4)
+
+
+@dataclass
+class Convergence:
+    """`-c [100x70x20, 1e-6, 10]` or `-c 100x70x20` (defaults 1e-6 and 10,
+    Examples/antsRegistrationTemplateHeader.h:546-611)."""
+    iterations: List[int]
+    threshold: float = 1e-6
+    window: int = 10
+
+    @classmethod
+    def parse(cls, text: str) -> "Convergence":
+        name, p = parse_option_function(text)
+        if not p:
+            return cls(parse_vector(name))
+        return cls(parse_vector(p[0]), float(p[1]) if len(p) > 1 else 1e-6, int(p[2]) if len(p) > 2 else 10)
+
+
+def parse_sigmas(text: str, n_levels: int) -> Tuple[List[float], bool]:
+    """`-s 2x1x0vox|mm` (no suffix = voxels; one value is repeated over the
+    levels), Examples/antsRegistrationTemplateHeader.h:627-651."""
+    t = text.strip()
+    in_mm = False
+    if "mm" in t:
+        t = t.replace("mm", "", 1)
+        in_mm = True
+    elif "vox" in t:
+        t = t.replace("vox", "", 1)
+    s = parse_vector(t, float)
+    if len(s) == 1:
+        s = s * n_levels
+    return s, in_mm
+
+
+@dataclass
+class SyNStage:
+    transform: SyNTransform
+    metric: CCMetric
+    convergence: Convergence
+    shrink: List[int]
+    sigmas: List[float]
+    sigmas_in_mm: bool = False
+
+    @classmethod
+    def from_args(cls, transform: str, metric: str, convergence: str, shrink: str, smoothing: str) -> "SyNStage":
+        t = SyNTransform.parse(transform)
+        m = CCMetric.parse(metric)
+        c = Convergence.parse(convergence)
+        f = parse_vector(shrink)
+        s, mm = parse_sigmas(smoothing, len(c.iterations))
+        if not (len(f) == len(s) == len(c.iterations)):
+            # RegistrationHelper refuses mismatched schedules (itkantsRegistrationHelper.hxx:1257-1283)
+            raise ValueError("the number of shrink factors, smoothing sigmas and iteration counts must match")
+        if m.weight != 1.0:
+            raise ValueError("a single CC metric with weight 1 is supported")
+        return cls(t, m, c, f, s, mm)
+
+
+@dataclass
+class IterationRecord:
+    level: int
+    iteration: int
+    energy: float
+    convergence: float
+    inv_iters: Tuple[int, int, int, int]
+
+
+@dataclass
+class SyNResult:
+    fwd: np.ndarray
+    inv: np.ndarray
+    trace: List[IterationRecord]
+    stats: dict
+
+    @property
+    def energy(self) -> np.ndarray:
+        return np.array([r.energy for r in self.trace])
+
+    @property
+    def levels(self) -> np.ndarray:
+        return np.array([r.level for r in self.trace])
+
+    def diagnostic_lines(self) -> List[str]:
+        """The observer's `1DIAGNOSTIC, iter, E, conv, ...` lines."""
+        return [f"1DIAGNOSTIC, {r.iteration:5d}, {r.energy:.12e}, {r.convergence:.12e}" for r in self.trace]
+
+
+class SyNRegistration:
+    """One SyN + CC stage (RegistrationHelper::AddSyNTransform + AddMetric +
+    schedule setters), run on the GPU through the C-ABI."""
+
+    def __init__(self, stage: SyNStage, dim: int, precision: str = "fp32", preprocess: bool = True,
+                 force_iterations: bool = False):
+        if dim not in (2, 3):
+            raise ValueError("dim must be 2 or 3")
+        self.stage = stage
+        self.dim = dim
+        p = L.SynParamsC()
+        p.dim = dim
+        n = len(stage.shrink)
+        if n > L.MAX_LEVELS:
+            raise ValueError(f"at most {L.MAX_LEVELS} levels")
+        p.n_levels = n
+        for i in range(n):
+            p.shrink[i] = int(stage.shrink[i])
+            p.sigma[i] = float(stage.sigmas[i])
+            p.iterations[i] = int(stage.convergence.iterations[i])
+        p.sigma_in_mm = int(stage.sigmas_in_mm)
+        p.convergence_threshold = stage.convergence.threshold
+        p.convergence_window = stage.convergence.window
+        p.learning_rate = stage.transform.gradient_step
+        p.update_variance = stage.transform.update_variance
+        p.total_variance = stage.transform.total_variance
+        p.cc_radius = stage.metric.radius
+        p.precision = {"fp32": 0, "bf16": 1}[precision]
+        p.preprocess = int(preprocess)
+        p.force_iterations = int(force_iterations)
+        self.params = p
+        h = C.c_void_p()
+        L.check(L.lib().ants_syn_create(C.byref(p), C.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            L.lib().ants_syn_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def level_geometry(self, size, spacing=None, origin=None, level: int = -1) -> L.ImageDesc:
+        lvl = level if level >= 0 else self.params.n_levels - 1
+        out = L.ImageDesc()
+        L.check(L.lib().ants_syn_level_geometry(C.byref(self.params), C.byref(L.desc(size, spacing, origin)),
+                                                lvl, C.byref(out)))
+        return out
+
+    def set_profiling(self, mask: int):
+        L.check(L.lib().ants_syn_set_profiling(self.handle, mask))
+
+    def kernel_stats(self) -> dict:
+        k = L.KernelStatsC()
+        L.check(L.lib().ants_syn_get_kernel_stats(self.handle, C.byref(k)))
+        return {n: {"ms": k.ms[i], "launches": int(k.launches[i]), "bytes": k.bytes[i]}
+                for i, n in enumerate(L.KC_NAMES)}
+
+    def stats(self) -> dict:
+        s = L.SynStatsC()
+        L.check(L.lib().ants_syn_get_stats(self.handle, C.byref(s)))
+        n = s.levels_run
+        return {"levels_run": n, "iterations": list(s.iterations[:n]), "level_voxels": list(s.level_voxels[:n]),
+                "inv_steps": list(s.inv_steps[:n]), "voxel_iterations": s.voxel_iterations,
+                "algorithmic_bytes": s.algorithmic_bytes}
+
+    def _callback(self, trace: list):
+        def cb(user, level, it, e, conv, kin):
+            trace.append(IterationRecord(level, it, e, conv, tuple(kin[i] for i in range(4))))
+            return 0
+        return L.ITER_CB(cb)
+
+    def register(self, fixed: np.ndarray, moving: np.ndarray, fixed_spacing=None, fixed_origin=None,
+                 moving_spacing=None, moving_origin=None) -> SyNResult:
+        """Host arrays in (numpy order, x fastest), host fields out."""
+        fixed = np.ascontiguousarray(fixed, dtype=np.float32)
+        moving = np.ascontiguousarray(moving, dtype=np.float32)
+        if fixed.ndim != self.dim or moving.ndim != self.dim:
+            raise ValueError("image dimension does not match the stage")
+        fsize = tuple(reversed(fixed.shape))
+        msize = tuple(reversed(moving.shape))
+        gf = L.desc(fsize, fixed_spacing, fixed_origin)
+        gm = L.desc(msize, moving_spacing, moving_origin)
+        out = self.level_geometry(fsize, fixed_spacing, fixed_origin)
+        oshape = tuple(int(out.size[k]) for k in reversed(range(self.dim))) + (self.dim,)
+        fwd = np.empty(oshape, np.float32)
+        inv = np.empty(oshape, np.float32)
+        trace: List[IterationRecord] = []
+        cb = self._callback(trace)
+        L.check(L.lib().ants_syn_register(self.handle, C.byref(gf), fixed.ctypes.data_as(C.c_void_p), C.byref(gm),
+                                          moving.ctypes.data_as(C.c_void_p), fwd.ctypes.data_as(C.c_void_p),
+                                          inv.ctypes.data_as(C.c_void_p), cb, None))
+        return SyNResult(fwd, inv, trace, self.stats())
+
+    def register_device(self, gf: L.ImageDesc, d_fixed: L.DeviceArray, gm: L.ImageDesc, d_moving: L.DeviceArray,
+                        d_fwd: Optional[L.DeviceArray], d_inv: Optional[L.DeviceArray], trace: Optional[list] = None):
+        """Device buffers in and out (inputs already resident in HBM)."""
+        tr = trace if trace is not None else []
+        cb = self._callback(tr)
+        L.check(L.lib().ants_syn_register_device(self.handle, C.byref(gf), d_fixed.ptr, C.byref(gm), d_moving.ptr,
+                                                 d_fwd.ptr if d_fwd else None, d_inv.ptr if d_inv else None, cb,
+                                                 None))
+        return tr
+
+    def state(self, out_shape) -> List[np.ndarray]:
+        """F2M fwd, F2M inv, M2M fwd, M2M inv after the last run (host copies),
+        the registration state --save-state keeps."""
+        bufs = [L.DeviceArray(out_shape) for _ in range(4)]
+        L.check(L.lib().ants_syn_get_state_device(self.handle, *[b.ptr for b in bufs]))
+        return [b.to_host() for b in bufs]
+
+
+def registration_from_cli(args: Sequence[str]) -> Tuple[SyNRegistration, SyNStage]:
+    """Build a SyNRegistration from antsRegistration-style arguments
+    (`-d 3 -m CC[f,m,1,4] -t SyN[0.1,3,0] -c [...] -f 4x2x1 -s 2x1x0vox`)."""
+    opts = {}
+    it = iter(args)
+    for a in it:
+        if a in ("-d", "--dimensionality", "-m", "--metric", "-t", "--transform", "-c", "--convergence", "-f",
+                 "--shrink-factors", "-s", "--smoothing-sigmas", "--float"):
+            opts[a.lstrip("-")[0] if len(a) == 2 else a] = next(it)
+    dim = int(opts.get("d", opts.get("--dimensionality", 3)))
+    stage = SyNStage.from_args(opts.get("t", opts.get("--transform")), opts.get("m", opts.get("--metric")),
+                               opts.get("c", opts.get("--convergence")), opts.get("f", opts.get("--shrink-factors")),
+                               opts.get("s", opts.get("--smoothing-sigmas")))
+    return SyNRegistration(stage, dim), stage

@@ -1,0 +1,158 @@
+"""Benchmark: 3-D SyN with the CC metric (r=4) on one 192x224x192 pair.
+
+One step = one complete SyN stage (SyN[0.1,3,0], CC r=4, -c [100x70x20,1e-6,10],
+-f 4x2x1, -s 2x1x0vox) on a synthetic ellipsoid-phantom pair whose images
+are already resident in HBM.  The metric is voxel-iterations per second
+(sum over levels of level voxels x iterations run, per wall second).
+
+Prints one JSON line.  Also reports the HBM roofline of the dominant kernel
+class (HIP events on the engine's stream over the timed steps) and the CPU
+restatement timed on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "SyN CC voxel-iters/s @192x224x192 vs CPU; pairs/s + p50/p99 latency @QPS"
+SCHEDULE = dict(transform="SyN[0.1,3,0]", metric="CC[fixed,moving,1,4]", convergence="[100x70x20,1e-6,10]",
+                shrink="4x2x1", smoothing="2x1x0vox")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(pair, seconds_hint: float = 20.0):
+    """The oracle's FP32 build of the same algorithm, all host threads, on a
+    truncated schedule of the same inputs (levels 4x2x1, iterations 10x5x2)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as o
+    iters = [10, 5, 2]
+    P = o.SynParams(shrink=[4, 2, 1], sigma=[2, 1, 0], iterations=iters, learning_rate=0.1, update_variance=3.0,
+                    total_variance=0.0, radius=4, force_iterations=True)
+    g = o.Grid(list(pair.size), list(pair.spacing), list(pair.origin))
+    threads = o.max_threads()
+    t0 = time.perf_counter()
+    r = o.syn_register(P, g, pair.fixed, g, pair.moving, precision="f32")
+    dt = time.perf_counter() - t0
+    vox = [o.shrink_grid(g, s).count for s in P.shrink]
+    work = float(sum(v * it for v, it in zip(vox, iters)))
+    return {"value": work / dt, "unit": "voxel-iters/s", "cores": threads, "kind": "port",
+            "sample": f"192x224x192 pair, schedule 10x5x2 (of 100x70x20), FP32 CPU restatement, "
+                      f"{threads} OpenMP threads, {work:.3g} voxel-iterations in {dt:.1f} s",
+            "seconds": dt}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--seed", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for {args.gpus}); "
+              "pairs would shard across GPUs with no communication", file=sys.stderr)
+        sys.exit(2)
+
+    from ants_amd import _lib as L
+    from ants_amd import phantoms as ph
+    from ants_amd.registration import SyNRegistration, SyNStage
+
+    t0 = time.perf_counter()
+    pair = ph.phantom_pair_3d(args.seed)
+    log(f"generated 192x224x192 phantom pair in {time.perf_counter() - t0:.1f} s")
+    stage = SyNStage.from_args(**SCHEDULE)
+    reg = SyNRegistration(stage, 3)
+    gf = L.desc(pair.size, pair.spacing, pair.origin)
+    d_fixed = L.DeviceArray.from_host(pair.fixed)
+    d_moving = L.DeviceArray.from_host(pair.moving)
+    out_shape = pair.fixed.shape + (3,)
+    d_fwd = L.DeviceArray(out_shape)
+    d_inv = L.DeviceArray(out_shape)
+
+    # profiled run (all kernel classes) to name the dominant class
+    reg.set_profiling(0x1F)
+    reg.register_device(gf, d_fixed, gf, d_moving, d_fwd, d_inv)
+    ks = reg.kernel_stats()
+    dominant = max(ks, key=lambda k: ks[k]["ms"])
+    log("kernel classes (ms, launches, GB/s):",
+        {k: (round(v["ms"], 2), v["launches"], round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1)) for k, v in ks.items()})
+    reg.set_profiling(1 << L.KC_NAMES.index(dominant))
+
+    for _ in range(max(args.warmup - 1, 0)):
+        reg.register_device(gf, d_fixed, gf, d_moving, d_fwd, d_inv)
+    L.check(L.lib().ants_device_synchronize())
+
+    times, work, traces, kacc = [], 0.0, [], {"ms": 0.0, "launches": 0, "bytes": 0.0}
+    for k in range(args.steps):
+        L.check(L.lib().ants_device_synchronize())
+        t = time.perf_counter()
+        tr = reg.register_device(gf, d_fixed, gf, d_moving, d_fwd, d_inv)
+        L.check(L.lib().ants_device_synchronize())
+        dt = time.perf_counter() - t
+        st = reg.stats()
+        kd = reg.kernel_stats()[dominant]
+        for key in kacc:
+            kacc[key] += kd[key]
+        times.append(dt)
+        work += st["voxel_iterations"]
+        traces.append(tr)
+        log(f"step {k}: {dt * 1e3:.1f} ms, iterations {st['iterations']}, inner inversion steps {st['inv_steps']}")
+    total = sum(times)
+    value = work / total
+    st = reg.stats()
+    avg_ms = kacc["ms"] / max(kacc["launches"], 1)
+    bytes_per_launch = kacc["bytes"] / max(kacc["launches"], 1)
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    whole_gbs = st["algorithmic_bytes"] / (total / args.steps) / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        log("timing the CPU restatement on a bounded sample ...")
+        cpu = cpu_baseline(pair)
+        log(f"cpu baseline: {cpu['value']:.4g} voxel-iters/s in {cpu['seconds']:.1f} s")
+        cpu.pop("seconds")
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "voxel-iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": total / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "3-D SyN CC r=4 on one 192x224x192 ellipsoid-phantom pair (1 mm), SyN[0.1,3,0], "
+                               "-c [100x70x20,1e-6,10] -f 4x2x1 -s 2x1x0vox, images resident in HBM",
+                   "global_batch": 1, "iterations_run": st["iterations"], "inv_steps": st["inv_steps"],
+                   "parallelism": "single GPU"},
+        "pairs_per_s": args.steps / total,
+        "whole_job_algorithmic_gbs": whole_gbs,
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": bytes_per_launch},
+        "cpu_baseline": cpu,
+        "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
+        "kernel_breakdown_ms": {k: round(v["ms"], 3) for k, v in ks.items()},
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

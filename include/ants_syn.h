@@ -1,0 +1,219 @@
+/*
+ * ants_syn.h -- C-ABI of the MI355X (gfx950) SyN + CC registration library.
+ *
+ * Drop-in for the SyN stage that antsRegistration drives through
+ * RegistrationHelper (Examples/itkantsRegistrationHelper.hxx:1983-2220):
+ * `--transform SyN[step,updateVariance,totalVariance]`
+ * `--metric CC[fixed,moving,1,radius]` with `-c/-f/-s` multi-resolution
+ * schedules.  Images are dense x-fastest float buffers with ITK's geometry
+ * (size, spacing, origin; identity direction in this release).  Displacement
+ * fields are ITK `Image<Vector<float,D>,D>` buffers: D interleaved float
+ * components per voxel (x fastest), physical units (mm).
+ *
+ * Conventions: every function returns ANTS_OK (0) or an error code; no C++
+ * exception crosses the ABI.  ants_last_error() returns the calling thread's
+ * last message.  Pointers named d_* are device pointers (HBM); host_* are
+ * host pointers.  `stream` is a hipStream_t (NULL = the default stream).
+ * A handle is single-threaded; distinct handles may be used concurrently.
+ */
+#ifndef ANTS_SYN_H
+#define ANTS_SYN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ANTS_OK 0
+#define ANTS_ERR_INVALID 1      /* bad argument or parameter combination */
+#define ANTS_ERR_HIP 2          /* HIP runtime failure */
+#define ANTS_ERR_OOM 3          /* device allocation or admission budget */
+#define ANTS_ERR_UNSUPPORTED 4  /* valid in ANTs, not supported here */
+#define ANTS_ERR_CANCELLED 5    /* iteration callback asked to stop */
+
+#define ANTS_MAX_LEVELS 8
+
+/* itk::Image geometry (ImageBase: size, spacing, origin, direction). */
+typedef struct ants_image_desc {
+  int dim;              /* 2 or 3 */
+  int64_t size[3];      /* size[2] ignored (treated as 1) when dim == 2 */
+  double spacing[3];
+  double origin[3];
+  double direction[9];  /* row-major dim x dim; all-zero means identity */
+} ants_image_desc;
+
+#define ANTS_PRECISION_FP32 0       /* FP32 images, fields and sums */
+#define ANTS_PRECISION_BF16_PYRAMID 1 /* BF16 smoothed images, FP32 sums/fields */
+
+/* The SyN stage parameters antsRegistration parses
+ * (Examples/antsRegistrationTemplateHeader.h:542-815, 985-1086) and
+ * RegistrationHelper hands to SyNImageRegistrationMethod
+ * (Examples/itkantsRegistrationHelper.hxx:2141-2164). */
+typedef struct ants_syn_params {
+  int dim;                              /* 2 or 3 */
+  int n_levels;                         /* -f/-s/-c entries */
+  int shrink[ANTS_MAX_LEVELS];          /* -f 4x2x1 (isotropic factor; per-axis rule of the helper) */
+  double sigma[ANTS_MAX_LEVELS];        /* -s 2x1x0 */
+  int sigma_in_mm;                      /* -s ...mm (1) or ...vox (0) */
+  int iterations[ANTS_MAX_LEVELS];      /* -c [100x70x20,...] */
+  double convergence_threshold;         /* -c [...,1e-6,...] */
+  int convergence_window;               /* -c [...,...,10] */
+  double learning_rate;                 /* SyN[step,...] */
+  double update_variance;               /* SyN[.,3,.] (voxel^2) */
+  double total_variance;                /* SyN[.,.,0] (voxel^2) */
+  int cc_radius;                        /* CC[f,m,1,radius] */
+  int precision;                        /* ANTS_PRECISION_* */
+  int preprocess;                       /* window both images to [0,1] as PreprocessImage does (1 = ANTs default) */
+  int force_iterations;                 /* 1: never stop early (parity harness), 0: convergence monitor */
+} ants_syn_params;
+
+/* Per-iteration observer (antsDisplacementAndVelocityFieldRegistrationCommandIterationUpdate
+ * IterationEvent, Examples/antsDisplacementAndVelocityFieldRegistrationCommandIterationUpdate.h:117-195).
+ * inv_iters[4]: inner steps of the four field inversions (F2M inverse,
+ * F2M forward, M2M inverse, M2M forward).  Return non-zero to cancel. */
+typedef int (*ants_syn_iteration_cb)(void* user, int level, int iteration, double energy, double convergence,
+                                     const int* inv_iters);
+
+typedef struct ants_syn_handle ants_syn_handle;
+
+/* ---- library / device helpers (no counterpart in ANTs: device plumbing) ---- */
+const char* ants_last_error(void);
+int ants_version(void);
+int ants_device_malloc(void** d_ptr, size_t bytes);
+int ants_device_free(void* d_ptr);
+int ants_memcpy_h2d(void* d_dst, const void* host_src, size_t bytes, void* stream);
+int ants_memcpy_d2h(void* host_dst, const void* d_src, size_t bytes, void* stream);
+int ants_memset_device(void* d_dst, int value, size_t bytes, void* stream);
+int ants_stream_create(void** stream);
+int ants_stream_destroy(void* stream);
+int ants_stream_synchronize(void* stream);
+int ants_device_synchronize(void);
+int ants_event_create(void** event);
+int ants_event_destroy(void* event);
+int ants_event_record(void* event, void* stream);
+int ants_event_elapsed_ms(float* ms, void* start, void* stop);
+
+/* ---- the SyN stage ---- */
+
+/* Level grid of level `level` (ShrinkImageFilter output information with
+ * RegistrationHelper::CalculateShrinkFactorsPerDimension,
+ * Examples/itkantsRegistrationHelper.hxx:16-39, 593-641). */
+int ants_syn_level_geometry(const ants_syn_params* p, const ants_image_desc* fixed, int level, ants_image_desc* out);
+
+/* Replaces SyNImageRegistrationMethod::New() + its setters
+ * (Examples/itkantsRegistrationHelper.hxx:2001-2164).  The handle owns its
+ * stream and device workspace. */
+int ants_syn_create(const ants_syn_params* p, ants_syn_handle** out);
+void ants_syn_destroy(ants_syn_handle* h);
+/* Use a caller stream instead of the handle's own. */
+int ants_syn_set_stream(ants_syn_handle* h, void* stream);
+
+/* Replaces SyNImageRegistrationMethod::Update() + GetModifiableTransform()
+ * (Examples/itkantsRegistrationHelper.hxx:2198-2218), host buffers in and
+ * out.  out_fwd / out_inv: D x (fixed voxels) floats each (the output
+ * DisplacementFieldTransform's displacement and inverse displacement
+ * fields on the last level's grid).  Either may be NULL. */
+int ants_syn_register(ants_syn_handle* h, const ants_image_desc* fixed, const float* host_fixed,
+                      const ants_image_desc* moving, const float* host_moving, float* host_out_fwd,
+                      float* host_out_inv, ants_syn_iteration_cb cb, void* user);
+
+/* Same, with the images already resident in HBM and device outputs. */
+int ants_syn_register_device(ants_syn_handle* h, const ants_image_desc* fixed, const float* d_fixed,
+                             const ants_image_desc* moving, const float* d_moving, float* d_out_fwd,
+                             float* d_out_inv, ants_syn_iteration_cb cb, void* user);
+
+/* Registration state after a run: FixedToMiddle (forward, inverse) and
+ * MovingToMiddle (forward, inverse) fields, as --save-state keeps them
+ * (Examples/itkantsRegistrationHelper.hxx:2206-2215).  Device buffers of
+ * D x voxels floats each; any may be NULL. */
+int ants_syn_get_state_device(ants_syn_handle* h, float* d_f2m_fwd, float* d_f2m_inv, float* d_m2m_fwd,
+                              float* d_m2m_inv);
+
+/* Work done by the last run: iterations per level, inner inversion steps,
+ * voxel-iterations and the algorithmic HBM byte count (per voxel-iteration
+ * B_vi = 2*g*e_i + 14*D*e_f + sum_j (3*K_j + 1)*D*e_f, summed). */
+typedef struct ants_syn_stats {
+  int levels_run;
+  int iterations[ANTS_MAX_LEVELS];
+  int64_t level_voxels[ANTS_MAX_LEVELS];
+  int64_t inv_steps[ANTS_MAX_LEVELS];   /* sum over iterations of the 4 inner-step counts */
+  double voxel_iterations;              /* sum_l N_l * iterations_l */
+  double algorithmic_bytes;             /* sum over iterations of N_l * B_vi */
+} ants_syn_stats;
+int ants_syn_get_stats(const ants_syn_handle* h, ants_syn_stats* out);
+
+/* Kernel classes timed with HIP events on the handle's stream when enabled
+ * by ants_syn_set_profiling (bit k enables class k). */
+#define ANTS_KC_WARP 0     /* image warps through the inverse middle fields */
+#define ANTS_KC_CC 1       /* CC value + both derivatives */
+#define ANTS_KC_SMOOTH 2   /* update/total field Gaussian passes */
+#define ANTS_KC_COMPOSE 3  /* scaled composition onto the forward fields */
+#define ANTS_KC_INV 4      /* fused fixed-point inversion steps (active launches only) */
+#define ANTS_KC_COUNT 5
+typedef struct ants_syn_kernel_stats {
+  double ms[ANTS_KC_COUNT];        /* summed event time of timed launches */
+  int64_t launches[ANTS_KC_COUNT]; /* timed launches */
+  double bytes[ANTS_KC_COUNT];     /* algorithmic HBM bytes of those launches */
+} ants_syn_kernel_stats;
+int ants_syn_set_profiling(ants_syn_handle* h, unsigned int class_mask);
+int ants_syn_get_kernel_stats(const ants_syn_handle* h, ants_syn_kernel_stats* out);
+
+/* ---- building blocks (device buffers) ----
+ * Each replaces the ITK filter that SyN calls at the cited ANTs site. */
+
+/* DiscreteGaussianImageFilter (variance sigma^2, MaximumError 0.01, width 32),
+ * ImageRegistrationMethodv4 per-level smoothing (itkantsRegistrationHelper.hxx:2149-2150). */
+int ants_op_smooth_image(const ants_image_desc* g, const float* d_in, float* d_out, double sigma, int sigma_in_mm,
+                         void* stream);
+
+/* SyN GaussianSmoothDisplacementField(field, variance): per-axis
+ * GaussianOperator(MaxError 0.001), boundary zeroed, blend below 0.5
+ * (variance set at itkantsRegistrationHelper.hxx:2161-2162). */
+int ants_op_smooth_field(const ants_image_desc* g, const float* d_in, float* d_out, double variance, void* stream);
+
+/* ResampleImageFilter + LinearInterpolateImageFunction through x -> x + u(x)
+ * (u on the output grid, NULL = identity), default value 0
+ * (itkantsRegistrationHelper.hxx:719-763). */
+int ants_op_warp_image(const ants_image_desc* g_img, const float* d_img, const ants_image_desc* g_out,
+                       const float* d_field, float* d_out, void* stream);
+
+/* ANTSNeighborhoodCorrelationImageToImageMetricv4::GetValueAndDerivative on
+ * two images of grid g, both SyN roles at once (metric set up at
+ * itkantsRegistrationHelper.hxx:931-948).  d_g_f2m / d_g_m2m: D x voxels.
+ * host_value receives -mean(localCC). */
+int ants_op_cc(const ants_image_desc* g, const float* d_fixed, const float* d_moving, int radius, float* d_g_f2m,
+               float* d_g_m2m, double* host_value, void* stream);
+
+/* ComposeDisplacementFieldsImageFilter: out = warp + disp(x + warp), 0 outside
+ * (itkantsRegistrationHelper.hxx:3698-3711). */
+int ants_op_compose(const ants_image_desc* g, const float* d_disp, const float* d_warp, float* d_out, void* stream);
+
+/* SyN ScaleUpdateField: u *= rate / max|u/spacing|; host_max receives the max. */
+int ants_op_scale_update(const ants_image_desc* g, float* d_u, double rate, double* host_max, void* stream);
+
+/* InvertDisplacementFieldImageFilter (20 steps, mean 0.001, max 0.1,
+ * boundary enforced) as SyN configures it; d_init may be NULL (zero start). */
+int ants_op_invert(const ants_image_desc* g, const float* d_field, const float* d_init, float* d_inv,
+                   int* host_iters, double* host_max, double* host_mean, void* stream);
+
+/* DisplacementFieldTransformParametersAdaptor: linear resample of a field onto
+ * a new grid (itkantsRegistrationHelper.hxx:2033-2055). */
+int ants_op_resample_field(const ants_image_desc* g_in, const float* d_in, const ants_image_desc* g_out,
+                           float* d_out, void* stream);
+
+/* PreprocessImage (itkantsRegistrationHelper.hxx:74-136) with the CLI
+ * defaults: window [min, max + range/2560] -> [0,1]. */
+int ants_op_preprocess(int64_t n, const float* d_in, float* d_out, void* stream);
+
+/* antsApplyTransforms -n NearestNeighbor of an integer label map through
+ * x -> x + u(x) (Examples/antsApplyTransforms.cxx:593-630): output grid g_out,
+ * labels on g_lab, 0 outside. */
+int ants_apply_transform_labels_nn(const ants_image_desc* g_out, const float* d_field, const ants_image_desc* g_lab,
+                                   const int32_t* d_labels, int32_t* d_out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ANTS_SYN_H */

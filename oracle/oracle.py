@@ -1,0 +1,337 @@
+"""ctypes binding of the CPU oracle (test infrastructure only).
+
+Only tests/, the smoke check and bench.py's cpu_baseline leg import this
+module.  It loads ``oracle/libsyn_oracle.so`` built from ``syn_oracle.cpp``
+(``make -C oracle``).  The FP64 entry points are the parity oracle; the FP32
+driver is the CPU baseline.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsyn_oracle.so")
+
+
+class OgGrid(C.Structure):
+    _fields_ = [("dim", C.c_int), ("n", C.c_int64 * 3), ("spacing", C.c_double * 3), ("origin", C.c_double * 3)]
+
+
+class OgParams(C.Structure):
+    _fields_ = [
+        ("n_levels", C.c_int),
+        ("shrink", C.c_int * 8),
+        ("sigma", C.c_double * 8),
+        ("sigma_in_mm", C.c_int),
+        ("iterations", C.c_int * 8),
+        ("learning_rate", C.c_double),
+        ("update_variance", C.c_double),
+        ("total_variance", C.c_double),
+        ("radius", C.c_int),
+        ("conv_threshold", C.c_double),
+        ("conv_window", C.c_int),
+        ("preprocess", C.c_int),
+        ("force_iterations", C.c_int),
+        ("threads", C.c_int),
+    ]
+
+
+class OgTrace(C.Structure):
+    _fields_ = [
+        ("max_records", C.c_int),
+        ("n_records", C.c_int),
+        ("level", C.POINTER(C.c_int)),
+        ("iteration", C.POINTER(C.c_int)),
+        ("energy", C.POINTER(C.c_double)),
+        ("convergence", C.POINTER(C.c_double)),
+        ("inv_iters", C.POINTER(C.c_int)),
+    ]
+
+
+FLT_EPSILON = float(np.finfo(np.float32).eps)
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run `make -C oracle`)")
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER
+        d = C.c_double
+        pd = P(C.c_double)
+        pg = P(OgGrid)
+        L.og_gaussian_kernel.argtypes = [d, d, C.c_int64, pd, C.c_int]
+        L.og_gaussian_kernel.restype = C.c_int
+        L.og_bessel_in.argtypes = [C.c_int, d]
+        L.og_bessel_in.restype = d
+        L.og_shrink_grid.argtypes = [pg, C.c_int, pg]
+        L.og_smooth_image_f64.argtypes = [pg, pd, pd, d, C.c_int]
+        L.og_smooth_field_f64.argtypes = [pg, pd, pd, d]
+        L.og_warp_image_f64.argtypes = [pg, pd, pg, pd, pd]
+        L.og_compose_f64.argtypes = [pg, pd, pd, pd]
+        L.og_resample_field_f64.argtypes = [pg, pd, pg, pd]
+        L.og_cc_f64.argtypes = [pg, pd, pd, C.c_int, d, pd, pd]
+        L.og_cc_f64.restype = d
+        L.og_scale_update_f64.argtypes = [pg, pd, d]
+        L.og_scale_update_f64.restype = d
+        L.og_invert_f64.argtypes = [pg, pd, pd, pd, pd, pd]
+        L.og_invert_f64.restype = C.c_int
+        L.og_preprocess_f64.argtypes = [C.c_int64, pd, pd]
+        L.og_conv_value.argtypes = [pd, C.c_int, C.c_int]
+        L.og_conv_value.restype = d
+        L.og_warp_labels_nn.argtypes = [pg, P(C.c_float), pg, P(C.c_int32), P(C.c_int32)]
+        L.og_syn_register_f64.argtypes = [P(OgParams), pg, pd, pg, pd, pd, pd, P(OgTrace), pd, d]
+        L.og_syn_register_f64.restype = C.c_int
+        L.og_syn_register_f32.argtypes = [P(OgParams), pg, P(C.c_float), pg, P(C.c_float), P(C.c_float),
+                                          P(C.c_float), P(OgTrace), P(C.c_float), d]
+        L.og_syn_register_f32.restype = C.c_int
+        L.og_max_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+@dataclass
+class Grid:
+    """Image grid: size (x fastest), spacing, origin; identity direction."""
+    size: Sequence[int]
+    spacing: Sequence[float] = None
+    origin: Sequence[float] = None
+
+    def __post_init__(self):
+        D = len(self.size)
+        if self.spacing is None:
+            self.spacing = [1.0] * D
+        if self.origin is None:
+            self.origin = [0.0] * D
+
+    @property
+    def dim(self):
+        return len(self.size)
+
+    @property
+    def count(self):
+        return int(np.prod(self.size))
+
+    @property
+    def shape(self):
+        """numpy shape (z, y, x) / (y, x) of an x-fastest array."""
+        return tuple(int(s) for s in reversed(self.size))
+
+    def c(self) -> OgGrid:
+        g = OgGrid()
+        g.dim = self.dim
+        for d in range(3):
+            g.n[d] = int(self.size[d]) if d < self.dim else 1
+            g.spacing[d] = float(self.spacing[d]) if d < self.dim else 1.0
+            g.origin[d] = float(self.origin[d]) if d < self.dim else 0.0
+        return g
+
+
+def _pd(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def gaussian_kernel(variance: float, max_error: float, max_width: int = 32) -> np.ndarray:
+    out = np.zeros(128)
+    r = lib().og_gaussian_kernel(variance, max_error, max_width, _pd(out), 128)
+    return out[: r + 1].copy()
+
+
+def bessel_in(n: int, x: float) -> float:
+    return lib().og_bessel_in(n, x)
+
+
+def shrink_grid(g: Grid, factor: int) -> Grid:
+    o = OgGrid()
+    lib().og_shrink_grid(C.byref(g.c()), factor, C.byref(o))
+    D = g.dim
+    return Grid([int(o.n[d]) for d in range(D)], [o.spacing[d] for d in range(D)], [o.origin[d] for d in range(D)])
+
+
+def smooth_image(g: Grid, img, sigma: float, in_mm: bool = False):
+    a = _f64(img)
+    out = np.empty_like(a)
+    lib().og_smooth_image_f64(C.byref(g.c()), _pd(a), _pd(out), sigma, int(in_mm))
+    return out
+
+
+def smooth_field(g: Grid, fld, variance: float):
+    a = _f64(fld)
+    out = np.empty_like(a)
+    lib().og_smooth_field_f64(C.byref(g.c()), _pd(a), _pd(out), variance)
+    return out
+
+
+def warp_image(gi: Grid, img, go: Grid, u=None):
+    a = _f64(img)
+    out = np.empty(go.shape, dtype=np.float64)
+    keep = _f64(u) if u is not None else None
+    up = _pd(keep) if keep is not None else None
+    lib().og_warp_image_f64(C.byref(gi.c()), _pd(a), C.byref(go.c()), up, _pd(out))
+    return out
+
+
+def compose(g: Grid, disp, warp):
+    d = _f64(disp)
+    w = _f64(warp)
+    out = np.empty_like(w)
+    lib().og_compose_f64(C.byref(g.c()), _pd(d), _pd(w), _pd(out))
+    return out
+
+
+def resample_field(gi: Grid, fld, go: Grid):
+    a = _f64(fld)
+    out = np.empty(go.shape + (go.dim,), dtype=np.float64)
+    lib().og_resample_field_f64(C.byref(gi.c()), _pd(a), C.byref(go.c()), _pd(out))
+    return out
+
+
+def cc(g: Grid, F, M, radius: int, eps: float = FLT_EPSILON):
+    """Returns (value, grad_f2m, grad_m2m)."""
+    f = _f64(F)
+    m = _f64(M)
+    g1 = np.empty(g.shape + (g.dim,))
+    g2 = np.empty(g.shape + (g.dim,))
+    v = lib().og_cc_f64(C.byref(g.c()), _pd(f), _pd(m), radius, eps, _pd(g1), _pd(g2))
+    return v, g1, g2
+
+
+def scale_update(g: Grid, u, lr: float):
+    a = _f64(u).copy()
+    mx = lib().og_scale_update_f64(C.byref(g.c()), _pd(a), lr)
+    return a, mx
+
+
+def invert(g: Grid, fld, init=None):
+    """Returns (inverse, inner_iterations, last_max, last_mean)."""
+    f = _f64(fld)
+    i0 = _f64(init) if init is not None else None
+    out = np.empty_like(f)
+    mx = C.c_double()
+    mn = C.c_double()
+    k = lib().og_invert_f64(C.byref(g.c()), _pd(f), _pd(i0) if i0 is not None else None, _pd(out), C.byref(mx), C.byref(mn))
+    return out, k, mx.value, mn.value
+
+
+def preprocess(img):
+    a = _f64(img)
+    out = np.empty_like(a)
+    lib().og_preprocess_f64(a.size, _pd(a), _pd(out))
+    return out
+
+
+def conv_value(energies, window: int) -> float:
+    e = _f64(energies)
+    return lib().og_conv_value(_pd(e), len(e), window)
+
+
+def warp_labels_nn(gout: Grid, u, glab: Grid, labels):
+    uu = np.ascontiguousarray(u, dtype=np.float32) if u is not None else None
+    lab = np.ascontiguousarray(labels, dtype=np.int32)
+    out = np.empty(gout.shape, dtype=np.int32)
+    lib().og_warp_labels_nn(C.byref(gout.c()), uu.ctypes.data_as(C.POINTER(C.c_float)) if uu is not None else None,
+                            C.byref(glab.c()), lab.ctypes.data_as(C.POINTER(C.c_int32)),
+                            out.ctypes.data_as(C.POINTER(C.c_int32)))
+    return out
+
+
+@dataclass
+class SynParams:
+    """SyN[learning_rate, update_variance, total_variance] with CC[., ., 1, radius]."""
+    shrink: List[int]
+    sigma: List[float]
+    iterations: List[int]
+    learning_rate: float = 0.1
+    update_variance: float = 3.0
+    total_variance: float = 0.0
+    radius: int = 4
+    conv_threshold: float = 1e-6
+    conv_window: int = 10
+    sigma_in_mm: bool = False
+    preprocess: bool = True
+    force_iterations: bool = False
+    threads: int = 0
+
+    def c(self) -> OgParams:
+        p = OgParams()
+        p.n_levels = len(self.shrink)
+        for i, (s, sg, it) in enumerate(zip(self.shrink, self.sigma, self.iterations)):
+            p.shrink[i] = int(s)
+            p.sigma[i] = float(sg)
+            p.iterations[i] = int(it)
+        p.sigma_in_mm = int(self.sigma_in_mm)
+        p.learning_rate = self.learning_rate
+        p.update_variance = self.update_variance
+        p.total_variance = self.total_variance
+        p.radius = self.radius
+        p.conv_threshold = self.conv_threshold
+        p.conv_window = self.conv_window
+        p.preprocess = int(self.preprocess)
+        p.force_iterations = int(self.force_iterations)
+        p.threads = self.threads
+        return p
+
+
+@dataclass
+class SynResult:
+    fwd: np.ndarray
+    inv: np.ndarray
+    level: np.ndarray
+    iteration: np.ndarray
+    energy: np.ndarray
+    convergence: np.ndarray
+    inv_iters: np.ndarray
+    state: Optional[np.ndarray] = None
+
+
+def syn_register(params: SynParams, gF: Grid, fixed, gM: Grid, moving, precision: str = "f64",
+                 eps: float = FLT_EPSILON, want_state: bool = False, max_records: int = 4096) -> SynResult:
+    L = lib()
+    dt = np.float64 if precision == "f64" else np.float32
+    ct = C.c_double if precision == "f64" else C.c_float
+    f = np.ascontiguousarray(fixed, dtype=dt)
+    m = np.ascontiguousarray(moving, dtype=dt)
+    last = shrink_grid(gF, params.shrink[-1])
+    fwd = np.empty(last.shape + (gF.dim,), dtype=dt)
+    inv = np.empty_like(fwd)
+    state = np.empty((4,) + fwd.shape, dtype=dt) if want_state else None
+    lv = np.zeros(max_records, np.int32)
+    it = np.zeros(max_records, np.int32)
+    en = np.zeros(max_records, np.float64)
+    cv = np.zeros(max_records, np.float64)
+    ki = np.zeros(max_records * 4, np.int32)
+    tr = OgTrace()
+    tr.max_records = max_records
+    tr.n_records = 0
+    tr.level = lv.ctypes.data_as(C.POINTER(C.c_int))
+    tr.iteration = it.ctypes.data_as(C.POINTER(C.c_int))
+    tr.energy = en.ctypes.data_as(C.POINTER(C.c_double))
+    tr.convergence = cv.ctypes.data_as(C.POINTER(C.c_double))
+    tr.inv_iters = ki.ctypes.data_as(C.POINTER(C.c_int))
+    p = params.c()
+    fn = L.og_syn_register_f64 if precision == "f64" else L.og_syn_register_f32
+    P = C.POINTER(ct)
+    rc = fn(C.byref(p), C.byref(gF.c()), f.ctypes.data_as(P), C.byref(gM.c()), m.ctypes.data_as(P),
+            fwd.ctypes.data_as(P), inv.ctypes.data_as(P), C.byref(tr),
+            state.ctypes.data_as(P) if state is not None else None, eps)
+    if rc != 0:
+        raise RuntimeError(f"oracle syn_register failed: {rc}")
+    n = tr.n_records
+    return SynResult(fwd, inv, lv[:n].copy(), it[:n].copy(), en[:n].copy(), cv[:n].copy(),
+                     ki[: 4 * n].reshape(n, 4).copy(), state)
+
+
+def max_threads() -> int:
+    return lib().og_max_threads()

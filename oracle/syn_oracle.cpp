@@ -1,0 +1,893 @@
+// TEST INFRASTRUCTURE ONLY.
+//
+// CPU restatement of the SyN stage with the ANTS neighbourhood cross-correlation
+// metric, as antsRegistration drives it through RegistrationHelper
+// (Examples/itkantsRegistrationHelper.hxx:1983-2220).  It is the parity oracle
+// for the HIP path and the timed CPU baseline ("port"): only tests/, the smoke
+// check and bench.py's cpu_baseline leg load it.  The shipped library never
+// links or calls it.
+//
+// The arithmetic of the path lives in ITK (pinned v6 18d89fd6 / v5.4.6, absent
+// from the reference tree), so each function below restates the ITK filter the
+// reference calls and cites the ANTs call site plus the in-tree restatements
+// of the same algorithm.  Every routine is templated on the value type:
+// double is the FP64 oracle; float models the reference's `--float 1` runs
+// and is the CPU baseline.  Coordinates and continuous indices are double.
+//
+// Layout: images are dense x-fastest arrays; displacement fields are D-vector
+// AoS (x fastest), in physical units (mm), on an identity-direction grid.
+//
+// Parity status per function is stated in the comment above it.
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+extern "C" {
+typedef struct {
+  int dim;             // 2 or 3
+  int64_t n[3];        // size, x fastest; n[2] == 1 when dim == 2
+  double spacing[3];
+  double origin[3];
+} og_grid;
+
+typedef struct {
+  int n_levels;
+  int shrink[8];
+  double sigma[8];
+  int sigma_in_mm;
+  int iterations[8];
+  double learning_rate;
+  double update_variance;
+  double total_variance;
+  int radius;
+  double conv_threshold;
+  int conv_window;
+  int preprocess;        // window both images to [0,1] as PreprocessImage does
+  int force_iterations;  // run every iteration (no convergence stop)
+  int threads;           // 0 = OpenMP default
+} og_params;
+
+typedef struct {
+  int max_records;
+  int n_records;
+  int* level;
+  int* iteration;
+  double* energy;
+  double* convergence;
+  int* inv_iters;        // 4 per record: F2M inv, F2M fwd, M2M inv, M2M fwd
+} og_trace;
+}
+
+namespace {
+
+struct Grid {
+  int D;
+  int64_t n[3];
+  double sp[3], org[3];
+  int64_t count() const { return n[0] * n[1] * n[2]; }
+};
+
+Grid to_grid(const og_grid* g) {
+  Grid r;
+  r.D = g->dim;
+  for (int d = 0; d < 3; ++d) {
+    r.n[d] = d < g->dim ? g->n[d] : 1;
+    r.sp[d] = d < g->dim ? g->spacing[d] : 1.0;
+    r.org[d] = d < g->dim ? g->origin[d] : 0.0;
+  }
+  return r;
+}
+
+void set_threads(int t) {
+#ifdef _OPENMP
+  if (t > 0) omp_set_num_threads(t);
+#else
+  (void)t;
+#endif
+}
+
+// ---------------------------------------------------------------------------
+// Modified Bessel functions, Numerical Recipes in C (2nd ed.) sec. 6.6, with
+// the polynomial fits of Abramowitz & Stegun 9.8.1-9.8.4.  ITK's
+// GaussianOperator builds its coefficients from this method.
+// Pinned by tests against scipy.special.ive (relative 1e-6).
+double bessel_i0(double x) {
+  double ax = std::fabs(x);
+  if (ax < 3.75) {
+    double y = x / 3.75;
+    y *= y;
+    return 1.0 + y * (3.5156229 + y * (3.0899424 + y * (1.2067492 + y * (0.2659732 + y * (0.360768e-1 + y * 0.45813e-2)))));
+  }
+  double y = 3.75 / ax;
+  return (std::exp(ax) / std::sqrt(ax)) *
+         (0.39894228 + y * (0.1328592e-1 + y * (0.225319e-2 + y * (-0.157565e-2 + y * (0.916281e-2 + y * (-0.2057706e-1 + y * (0.2635537e-1 + y * (-0.1647633e-1 + y * 0.392377e-2))))))));
+}
+
+double bessel_i1(double x) {
+  double ax = std::fabs(x), ans;
+  if (ax < 3.75) {
+    double y = x / 3.75;
+    y *= y;
+    ans = ax * (0.5 + y * (0.87890594 + y * (0.51498869 + y * (0.15084934 + y * (0.2658733e-1 + y * (0.301532e-2 + y * 0.32411e-3))))));
+  } else {
+    double y = 3.75 / ax;
+    ans = 0.2282967e-1 + y * (-0.2895312e-1 + y * (0.1787654e-1 - y * 0.420059e-2));
+    ans = 0.39894228 + y * (-0.3988024e-1 + y * (-0.362018e-2 + y * (0.163801e-2 + y * (-0.1031555e-1 + y * ans))));
+    ans *= std::exp(ax) / std::sqrt(ax);
+  }
+  return x < 0.0 ? -ans : ans;
+}
+
+// Miller's downward recurrence (NR bessi), normalised by I0.
+double bessel_in(int n, double x) {
+  if (n == 0) return bessel_i0(x);
+  if (n == 1) return bessel_i1(x);
+  if (x == 0.0) return 0.0;
+  const double acc = 40.0, bigno = 1.0e10, bigni = 1.0e-10;
+  double tox = 2.0 / std::fabs(x), bip = 0.0, ans = 0.0, bi = 1.0;
+  for (int j = 2 * (n + static_cast<int>(std::sqrt(acc * n))); j > 0; --j) {
+    double bim = bip + j * tox * bi;
+    bip = bi;
+    bi = bim;
+    if (std::fabs(bi) > bigno) {
+      ans *= bigni;
+      bi *= bigni;
+      bip *= bigni;
+    }
+    if (j == n) ans = bip;
+  }
+  ans *= bessel_i0(x) / bi;
+  return (x < 0.0 && (n & 1)) ? -ans : ans;
+}
+
+// ITK GaussianOperator::GenerateCoefficients: c_i = e^-t I_i(t), grown while
+// 1 + 2*sum(c_1..c_i) < 1 - maxError, stopped early when a coefficient is
+// negligible or the half-width passes maxWidth; normalised to sum 1 and
+// mirrored.  Returns the half kernel c_0..c_R.  Variance t is in voxels
+// (SyN passes it unscaled; DiscreteGaussian divides by spacing^2 for mm).
+// Pinned: radius/values against scipy.special.ive (SURVEY table t=3,4,1,0.5).
+std::vector<double> gaussian_half_kernel(double variance, double max_error, int64_t max_width) {
+  std::vector<double> c;
+  const double et = std::exp(-variance);
+  const double cap = 1.0 - max_error;
+  double sum = 0.0;
+  c.push_back(et * bessel_i0(variance));
+  sum += c[0];
+  c.push_back(et * bessel_i1(variance));
+  sum += 2.0 * c[1];
+  for (int i = 2; sum < cap; ++i) {
+    c.push_back(et * bessel_in(i, variance));
+    sum += 2.0 * c[i];
+    if (c[i] < sum * std::numeric_limits<double>::epsilon()) break;
+    if (static_cast<int64_t>(c.size()) > max_width) break;
+  }
+  for (double& v : c) v /= sum;
+  return c;
+}
+
+// ---------------------------------------------------------------------------
+// Separable 1-D correlation along axis `ax` with edge replication (ITK's
+// ZeroFluxNeumannBoundaryCondition), `ncomp` interleaved components.
+template <class R>
+void convolve_axis(const Grid& g, int ncomp, const R* in, R* out, const std::vector<double>& half, int ax) {
+  const int64_t rad = static_cast<int64_t>(half.size()) - 1;
+  const int64_t nx = g.n[0], ny = g.n[1], nz = g.n[2];
+  const int64_t stride = ax == 0 ? 1 : (ax == 1 ? nx : nx * ny);
+  const int64_t len = g.n[ax];
+  std::vector<R> k(2 * rad + 1);
+  for (int64_t i = -rad; i <= rad; ++i) k[i + rad] = static_cast<R>(half[std::abs(i)]);
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int64_t z = 0; z < nz; ++z)
+    for (int64_t y = 0; y < ny; ++y)
+      for (int64_t x = 0; x < nx; ++x) {
+        const int64_t idx = x + nx * (y + ny * z);
+        const int64_t pos = ax == 0 ? x : (ax == 1 ? y : z);
+        const int64_t base = idx - pos * stride;
+        for (int c = 0; c < ncomp; ++c) {
+          R acc = 0;
+          for (int64_t t = -rad; t <= rad; ++t) {
+            int64_t p = pos + t;
+            p = p < 0 ? 0 : (p >= len ? len - 1 : p);
+            acc += k[t + rad] * in[(base + p * stride) * ncomp + c];
+          }
+          out[idx * ncomp + c] = acc;
+        }
+      }
+}
+
+// DiscreteGaussianImageFilter as ImageRegistrationMethodv4 configures it per
+// level (variance sigma^2, MaximumError 0.01, MaximumKernelWidth 32).  sigma
+// is in voxels unless in_mm (then variance / spacing^2 per axis).
+// Helper call site: Examples/itkantsRegistrationHelper.hxx:1275-1289, 2141-2150.
+// Pinned: kernel values (above), impulse response and scipy correlate1d.
+template <class R>
+void smooth_image(const Grid& g, const R* in, R* out, double sigma, bool in_mm) {
+  const int64_t N = g.count();
+  std::vector<R> a(in, in + N), b(N);
+  if (sigma > 0.0) {
+    for (int d = 0; d < g.D; ++d) {
+      double var = sigma * sigma;
+      if (in_mm) var /= g.sp[d] * g.sp[d];
+      std::vector<double> half = gaussian_half_kernel(var, 0.01, 32);
+      convolve_axis<R>(g, 1, a.data(), b.data(), half, d);
+      std::swap(a, b);
+    }
+  }
+  std::memcpy(out, a.data(), sizeof(R) * N);
+}
+
+// SyN GaussianSmoothDisplacementField: per-axis GaussianOperator(variance,
+// MaxError 0.001, MaxWidth size[d]), then boundary voxels zeroed and the
+// interior blended with the input when variance < 0.5; variance <= 0 is a copy.
+// Restatements in tree: Utilities/itkSimulatedExponentialDisplacementFieldSource.hxx:140-238,
+// Utilities/itkDiReCTImageFilter.hxx:892-971.
+// Pinned: kernel values, impulse response, boundary-zero/blend closed forms.
+template <class R>
+void smooth_field(const Grid& g, const R* in, R* out, double variance) {
+  const int D = g.D;
+  const int64_t N = g.count();
+  if (variance <= 0.0) {
+    std::memcpy(out, in, sizeof(R) * N * D);
+    return;
+  }
+  std::vector<R> a(in, in + N * D), b(N * D);
+  for (int d = 0; d < D; ++d) {
+    std::vector<double> half = gaussian_half_kernel(variance, 0.001, g.n[d]);
+    convolve_axis<R>(g, D, a.data(), b.data(), half, d);
+    std::swap(a, b);
+  }
+  const double w1 = variance < 0.5 ? 1.0 - variance / 0.5 : 1.0;
+  const double w2 = 1.0 - w1;
+#pragma omp parallel for schedule(static)
+  for (int64_t z = 0; z < g.n[2]; ++z)
+    for (int64_t y = 0; y < g.n[1]; ++y)
+      for (int64_t x = 0; x < g.n[0]; ++x) {
+        const int64_t i = x + g.n[0] * (y + g.n[1] * z);
+        bool bnd = x == 0 || x == g.n[0] - 1 || y == 0 || y == g.n[1] - 1;
+        if (D == 3) bnd = bnd || z == 0 || z == g.n[2] - 1;
+        for (int c = 0; c < D; ++c)
+          out[i * D + c] = bnd ? R(0) : static_cast<R>(a[i * D + c] * w1 + in[i * D + c] * w2);
+      }
+}
+
+// ---------------------------------------------------------------------------
+// Buffer test and linear interpolation with ITK semantics (ImageFunction::
+// IsInsideBuffer: -0.5 <= cidx < n-0.5 per axis; Linear/VectorLinear
+// interpolate functions clamp neighbours to the buffer, which inside the
+// buffer equals edge-replicated trilinear interpolation).
+// Pinned: scipy.ndimage.map_coordinates(order=1, mode='nearest') on inside points.
+inline bool inside(const Grid& g, const double c[3]) {
+  for (int d = 0; d < g.D; ++d)
+    if (!(c[d] >= -0.5 && c[d] < static_cast<double>(g.n[d]) - 0.5)) return false;
+  return true;
+}
+
+inline void to_cidx(const Grid& g, const double p[3], double c[3]) {
+  for (int d = 0; d < 3; ++d) c[d] = d < g.D ? (p[d] - g.org[d]) / g.sp[d] : 0.0;
+}
+
+template <class R>
+void interp(const Grid& g, int ncomp, const R* img, const double c[3], double* outv) {
+  int64_t lo[3], hi[3];
+  double w[3];
+  for (int d = 0; d < 3; ++d) {
+    if (d >= g.D) {
+      lo[d] = hi[d] = 0;
+      w[d] = 0.0;
+      continue;
+    }
+    double f = std::floor(c[d]);
+    int64_t b = static_cast<int64_t>(f);
+    double t = c[d] - f;
+    if (b < 0) {  // c in [-0.5, 0): clamped to the first voxel
+      b = 0;
+      t = 0.0;
+    }
+    lo[d] = b;
+    hi[d] = b + 1 < g.n[d] ? b + 1 : b;
+    w[d] = t;
+  }
+  for (int k = 0; k < ncomp; ++k) outv[k] = 0.0;
+  for (int corner = 0; corner < 8; ++corner) {
+    double wt = 1.0;
+    int64_t ix[3];
+    for (int d = 0; d < 3; ++d) {
+      bool up = (corner >> d) & 1;
+      if (d >= g.D && up) {
+        wt = 0.0;
+        break;
+      }
+      wt *= up ? w[d] : 1.0 - w[d];
+      ix[d] = up ? hi[d] : lo[d];
+    }
+    if (wt == 0.0) continue;
+    const int64_t idx = ix[0] + g.n[0] * (ix[1] + g.n[1] * ix[2]);
+    for (int k = 0; k < ncomp; ++k) outv[k] += wt * static_cast<double>(img[idx * ncomp + k]);
+  }
+}
+
+inline void grid_point(const Grid& g, int64_t x, int64_t y, int64_t z, double p[3]) {
+  p[0] = g.org[0] + x * g.sp[0];
+  p[1] = g.org[1] + y * g.sp[1];
+  p[2] = g.D == 3 ? g.org[2] + z * g.sp[2] : 0.0;
+}
+
+// ResampleImageFilter(image on gi, transform x -> x + u(x), output grid go,
+// linear interpolator, default 0).  u is sampled at the output voxel itself
+// (the fields live on the output grid), as in SyN ComputeMetricGradientField
+// with DownsampleImagesForMetricDerivatives on.  u == nullptr: identity.
+// Helper call sites: Examples/itkantsRegistrationHelper.hxx:719-763; in-tree
+// mirror of the SyN resampling: Examples/antsDisplacementAndVelocityFieldRegistrationCommandIterationUpdate.h:342-381.
+template <class R>
+void warp_image(const Grid& gi, const R* img, const Grid& go, const R* u, R* out) {
+  const int D = go.D;
+#pragma omp parallel for schedule(static)
+  for (int64_t z = 0; z < go.n[2]; ++z)
+    for (int64_t y = 0; y < go.n[1]; ++y)
+      for (int64_t x = 0; x < go.n[0]; ++x) {
+        const int64_t i = x + go.n[0] * (y + go.n[1] * z);
+        double p[3], c[3];
+        grid_point(go, x, y, z, p);
+        if (u)
+          for (int d = 0; d < D; ++d) p[d] += static_cast<double>(u[i * D + d]);
+        to_cidx(gi, p, c);
+        double v = 0.0;
+        if (inside(gi, c)) interp<R>(gi, 1, img, c, &v);
+        out[i] = static_cast<R>(v);
+      }
+}
+
+// ComposeDisplacementFieldsImageFilter(displacement d, warping w):
+// out(x) = w(x) + d(x + w(x)), d linearly interpolated, 0 outside its buffer.
+// Output on w's grid.  Restatement: Utilities/itkComposeDiffeomorphismsImageFilter.hxx:76-104.
+// Pinned: constant fields compose additively; affine fields exactly.
+template <class R>
+void compose(const Grid& g, const R* disp, const R* warp, R* out) {
+  const int D = g.D;
+#pragma omp parallel for schedule(static)
+  for (int64_t z = 0; z < g.n[2]; ++z)
+    for (int64_t y = 0; y < g.n[1]; ++y)
+      for (int64_t x = 0; x < g.n[0]; ++x) {
+        const int64_t i = x + g.n[0] * (y + g.n[1] * z);
+        double p[3], c[3], v[3] = {0, 0, 0};
+        grid_point(g, x, y, z, p);
+        for (int d = 0; d < D; ++d) p[d] += static_cast<double>(warp[i * D + d]);
+        to_cidx(g, p, c);
+        if (inside(g, c)) interp<R>(g, D, disp, c, v);
+        for (int d = 0; d < D; ++d) out[i * D + d] = static_cast<R>(static_cast<double>(warp[i * D + d]) + v[d]);
+      }
+}
+
+// DisplacementFieldTransformParametersAdaptor: resample a field onto a new
+// grid with the identity transform, vector linear interpolation, 0 outside.
+// SyN level set-up, Examples/itkantsRegistrationHelper.hxx:2033-2055.
+template <class R>
+void resample_field(const Grid& gi, const R* in, const Grid& go, R* out) {
+  const int D = go.D;
+#pragma omp parallel for schedule(static)
+  for (int64_t z = 0; z < go.n[2]; ++z)
+    for (int64_t y = 0; y < go.n[1]; ++y)
+      for (int64_t x = 0; x < go.n[0]; ++x) {
+        const int64_t i = x + go.n[0] * (y + go.n[1] * z);
+        double p[3], c[3], v[3] = {0, 0, 0};
+        grid_point(go, x, y, z, p);
+        to_cidx(gi, p, c);
+        if (inside(gi, c)) interp<R>(gi, D, in, c, v);
+        for (int d = 0; d < D; ++d) out[i * D + d] = static_cast<R>(v[d]);
+      }
+}
+
+// ---------------------------------------------------------------------------
+// ANTSNeighborhoodCorrelationImageToImageMetricv4::GetValueAndDerivative with
+// identity transforms on images already resampled to the level grid.
+// Box sums over the in-image part of the (2r+1)^D window of f, m, f^2, m^2,
+// f*m and the count; sFF, sMM, sFM and the derivative
+//   2 sFM/(sFF sMM) (f~ - sFM/sMM m~) grad m
+// as in the in-tree restatement ImageRegistration/itkCrossCorrelationRegistrationFunction.cxx:178-306, 441-503.
+// A window with sFF or sMM not above eps keeps localCC = 1 and a zero
+// derivative.  Value = -sum(localCC)/N.  grad m is the
+// CentralDifferenceImageFunction rule (half-voxel linear samples; a component
+// is 0 where p+h/2 leaves the buffer, a half difference at index 0).
+// Both SyN calls are evaluated from one set of box sums: call 1 (fixed=F,
+// moving=M) gives the fixed-to-middle update `g_f2m`, call 2 (fixed=M,
+// moving=F) the moving-to-middle update `g_m2m`.
+// Pinned: brute-force window loops, CC=1 for M=aF+b, degenerate windows, and
+// a finite-difference check of d localCC / d m_centre.
+template <class R>
+inline R grad_component(const R* img, int64_t i, int64_t pos, int64_t len, int64_t stride, double h) {
+  if (pos == len - 1) return R(0);
+  const double up = 0.5 * (static_cast<double>(img[i]) + static_cast<double>(img[i + stride]));
+  const double dn = pos == 0 ? static_cast<double>(img[i]) : 0.5 * (static_cast<double>(img[i - stride]) + static_cast<double>(img[i]));
+  return static_cast<R>((up - dn) / h);
+}
+
+template <class R>
+double cc_metric(const Grid& g, const R* F, const R* M, int radius, double eps, R* g_f2m, R* g_m2m) {
+  const int D = g.D;
+  const int64_t N = g.count();
+  const int64_t nx = g.n[0], ny = g.n[1], nz = g.n[2];
+  // 6 quantities: 1, f, m, ff, mm, fm
+  std::vector<R> q(N * 6), t(N * 6);
+  for (int64_t i = 0; i < N; ++i) {
+    const R f = F[i], m = M[i];
+    R* o = &q[i * 6];
+    o[0] = 1;
+    o[1] = f;
+    o[2] = m;
+    o[3] = f * f;
+    o[4] = m * m;
+    o[5] = f * m;
+  }
+  for (int ax = 0; ax < D; ++ax) {
+    const int64_t stride = ax == 0 ? 1 : (ax == 1 ? nx : nx * ny);
+    const int64_t len = g.n[ax];
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int64_t z = 0; z < nz; ++z)
+      for (int64_t y = 0; y < ny; ++y)
+        for (int64_t x = 0; x < nx; ++x) {
+          const int64_t idx = x + nx * (y + ny * z);
+          const int64_t pos = ax == 0 ? x : (ax == 1 ? y : z);
+          const int64_t lo = std::max<int64_t>(0, pos - radius), hi = std::min<int64_t>(len - 1, pos + radius);
+          R acc[6] = {0, 0, 0, 0, 0, 0};
+          for (int64_t p = lo; p <= hi; ++p) {
+            const R* s = &q[(idx + (p - pos) * stride) * 6];
+            for (int k = 0; k < 6; ++k) acc[k] += s[k];
+          }
+          for (int k = 0; k < 6; ++k) t[idx * 6 + k] = acc[k];
+        }
+    std::swap(q, t);
+  }
+  // per-slab partial sums, summed in slab order: deterministic for any thread count
+  std::vector<double> part(nz, 0.0);
+#pragma omp parallel for schedule(static)
+  for (int64_t z = 0; z < nz; ++z) {
+    double local = 0.0;
+    for (int64_t y = 0; y < ny; ++y)
+      for (int64_t x = 0; x < nx; ++x) {
+        const int64_t i = x + nx * (y + ny * z);
+        const R* s = &q[i * 6];
+        const R cnt = s[0], sf = s[1], sm = s[2], sff = s[3], smm = s[4], sfm = s[5];
+        const R mf = sf / cnt, mm = sm / cnt;
+        const R sFF = sff - mf * sf - mf * sf + cnt * mf * mf;
+        const R sMM = smm - mm * sm - mm * sm + cnt * mm * mm;
+        const R sFM = sfm - mm * sf - mf * sm + cnt * mm * mf;
+        const R fA = F[i] - mf, mA = M[i] - mm;
+        R cc = 1;
+        R gm[3] = {0, 0, 0}, gf[3] = {0, 0, 0};
+        if (sFF > eps && sMM > eps) {
+          cc = sFM * sFM / (sFF * sMM);
+          const int64_t pos[3] = {x, y, z};
+          for (int d = 0; d < D; ++d) {
+            const int64_t stride = d == 0 ? 1 : (d == 1 ? nx : nx * ny);
+            gm[d] = grad_component<R>(M, i, pos[d], g.n[d], stride, g.sp[d]);
+            gf[d] = grad_component<R>(F, i, pos[d], g.n[d], stride, g.sp[d]);
+          }
+          const R a = R(2) * sFM / (sFF * sMM);
+          const R c1 = a * (fA - sFM / sMM * mA);
+          const R c2 = a * (mA - sFM / sFF * fA);
+          for (int d = 0; d < D; ++d) {
+            gm[d] *= c1;
+            gf[d] *= c2;
+          }
+        }
+        for (int d = 0; d < D; ++d) {
+          if (g_f2m) g_f2m[i * D + d] = gm[d];
+          if (g_m2m) g_m2m[i * D + d] = gf[d];
+        }
+        local += static_cast<double>(cc);
+      }
+    part[z] = local;
+  }
+  double total = 0.0;
+  for (int64_t z = 0; z < nz; ++z) total += part[z];
+  return -total / static_cast<double>(N);
+}
+
+// SyN ScaleUpdateField: u *= learningRate / max_i |u_i / spacing| (when > 0).
+template <class R>
+double scale_update(const Grid& g, R* u, double lr) {
+  const int D = g.D;
+  const int64_t N = g.count();
+  double mx = 0.0;
+#pragma omp parallel for reduction(max : mx) schedule(static)
+  for (int64_t i = 0; i < N; ++i) {
+    double s = 0.0;
+    for (int d = 0; d < D; ++d) {
+      double v = static_cast<double>(u[i * D + d]) / g.sp[d];
+      s += v * v;
+    }
+    s = std::sqrt(s);
+    if (s > mx) mx = s;
+  }
+  double scale = lr;
+  if (mx > 0.0) scale /= mx;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < N * D; ++i) u[i] = static_cast<R>(static_cast<double>(u[i]) * scale);
+  return mx;
+}
+
+// InvertDisplacementFieldImageFilter as SyN configures it (MaximumNumberOfIterations 20,
+// MeanErrorToleranceThreshold 0.001, MaxErrorToleranceThreshold 0.1,
+// EnforceBoundaryCondition on): inv starts at the initial estimate; each step
+// e = compose(field, inv), norms |e/spacing| (mean over all voxels, max),
+// eps = 0.75 on the first step then 0.5, e clamped to eps*max, inv += -eps*e,
+// boundary voxels of inv zeroed.  The loop runs while step < 20 and the last
+// max > 0.1 and mean > 0.001.  Restatement with the same constants:
+// ImageRegistration/itkANTSImageRegistrationOptimizer.h:1845-1988.
+// Pinned: a constant shift inverts exactly in the interior; the residual at
+// exit respects the tolerances; constants cross-checked with the restatement.
+template <class R>
+int invert_field(const Grid& g, const R* field, const R* init, R* inv, double* out_max, double* out_mean) {
+  const int D = g.D;
+  const int64_t N = g.count();
+  const int64_t nx = g.n[0], ny = g.n[1], nz = g.n[2];
+  if (init)
+    std::memcpy(inv, init, sizeof(R) * N * D);
+  else
+    std::fill(inv, inv + N * D, R(0));
+  std::vector<R> e(N * D), norm(N);
+  double maxn = std::numeric_limits<double>::max(), meann = std::numeric_limits<double>::max();
+  int it = 0;
+  while (it++ < 20 && maxn > 0.1 && meann > 0.001) {
+    compose<R>(g, field, inv, e.data());
+    std::vector<double> psum(nz, 0.0), pmax(nz, 0.0);
+#pragma omp parallel for schedule(static)
+    for (int64_t z = 0; z < nz; ++z) {
+      double s = 0.0, m = 0.0;
+      for (int64_t j = 0; j < nx * ny; ++j) {
+        const int64_t i = z * nx * ny + j;
+        double q = 0.0;
+        for (int d = 0; d < D; ++d) {
+          double v = static_cast<double>(e[i * D + d]) / g.sp[d];
+          q += v * v;
+        }
+        q = std::sqrt(q);
+        norm[i] = static_cast<R>(q);
+        s += q;
+        if (q > m) m = q;
+        for (int d = 0; d < D; ++d) e[i * D + d] = -e[i * D + d];
+      }
+      psum[z] = s;
+      pmax[z] = m;
+    }
+    meann = 0.0;
+    maxn = 0.0;
+    for (int64_t z = 0; z < nz; ++z) {
+      meann += psum[z];
+      maxn = std::max(maxn, pmax[z]);
+    }
+    meann /= static_cast<double>(N);
+    const double eps = it == 1 ? 0.75 : 0.5;
+    const double lim = eps * maxn;
+#pragma omp parallel for schedule(static)
+    for (int64_t z = 0; z < nz; ++z)
+      for (int64_t y = 0; y < ny; ++y)
+        for (int64_t x = 0; x < nx; ++x) {
+          const int64_t i = x + nx * (y + ny * z);
+          double sc = 1.0;
+          const double q = static_cast<double>(norm[i]);
+          if (q > lim) sc = lim / q;
+          bool bnd = x == 0 || x == nx - 1 || y == 0 || y == ny - 1;
+          if (D == 3) bnd = bnd || z == 0 || z == nz - 1;
+          for (int d = 0; d < D; ++d)
+            inv[i * D + d] = bnd ? R(0) : static_cast<R>(static_cast<double>(inv[i * D + d]) + static_cast<double>(e[i * D + d]) * sc * eps);
+        }
+  }
+  if (out_max) *out_max = maxn;
+  if (out_mean) *out_mean = meann;
+  return it - 1;
+}
+
+// WindowConvergenceMonitoringFunction (window W): energies pushed into a
+// window of the last W values, a running total of |E| over the whole level;
+// fewer than W values -> max(double); otherwise an order-1 B-spline with 2
+// control points, one level, is fitted by the multilevel B-spline
+// approximation (BSplineScatteredDataPointSetToImageFilter) to
+// (t_n = n/(W-1), E_n/total) and the convergence value is -dphi/dt at t=1.
+// Described at Examples/antsRegistration.cxx:397-404.
+// Parity unpinned beyond invariants: a constant window gives 0, a falling
+// energy a positive value (the exact normalisation lives in ITK only).
+struct ConvMonitor {
+  int window;
+  std::vector<double> vals;
+  double total = 0.0;
+  explicit ConvMonitor(int w) : window(w) {}
+  void add(double e) {
+    vals.push_back(e);
+    if (static_cast<int>(vals.size()) > window) vals.erase(vals.begin());
+    total += std::fabs(e);
+  }
+  double value() const {
+    if (static_cast<int>(vals.size()) < window || window < 2) return std::numeric_limits<double>::max();
+    double num[2] = {0, 0}, den[2] = {0, 0};
+    for (int n = 0; n < window; ++n) {
+      const double t = static_cast<double>(n) / static_cast<double>(window - 1);
+      const double v = vals[n] / total;
+      const double w[2] = {1.0 - t, t};
+      const double w2 = w[0] * w[0] + w[1] * w[1];
+      for (int k = 0; k < 2; ++k) {
+        num[k] += w[k] * w[k] * (w[k] * v / w2);
+        den[k] += w[k] * w[k];
+      }
+    }
+    const double p0 = den[0] > 0 ? num[0] / den[0] : 0.0;
+    const double p1 = den[1] > 0 ? num[1] / den[1] : 0.0;
+    return -(p1 - p0);
+  }
+};
+
+// RegistrationHelper::CalculateShrinkFactorsPerDimension (Examples/itkantsRegistrationHelper.hxx:593-641)
+// followed by ShrinkImageFilter output information (hxx:16-39): size floor(n/s)
+// (at least 1), spacing*s, origin moved so the physical centres coincide.
+Grid shrink_grid(const Grid& g, int factor) {
+  int minIndex = 0;
+  double minSp = g.sp[0];
+  for (int d = 1; d < g.D; ++d)
+    if (minSp > g.sp[d]) {
+      minSp = g.sp[d];
+      minIndex = d;
+    }
+  int f[3] = {1, 1, 1};
+  f[minIndex] = factor;
+  const double target = g.sp[minIndex] * factor;
+  for (int d = 0; d < g.D; ++d) {
+    if (d == minIndex) continue;
+    double best = std::fabs(g.sp[d] * factor - target);
+    int bf = factor;
+    for (int k = factor - 1; k > 0; --k) {
+      double diff = std::fabs(g.sp[d] * k - target);
+      if (diff <= best) {
+        best = diff;
+        bf = k;
+      }
+    }
+    f[d] = bf;
+  }
+  Grid o = g;
+  for (int d = 0; d < g.D; ++d) {
+    o.sp[d] = g.sp[d] * f[d];
+    o.n[d] = std::max<int64_t>(1, static_cast<int64_t>(std::floor(static_cast<double>(g.n[d]) / f[d])));
+    const double in_c = (g.n[d] - 1) / 2.0, out_c = (o.n[d] - 1) / 2.0;
+    o.org[d] = g.org[d] + (g.org[d] + in_c * g.sp[d]) - (g.org[d] + out_c * o.sp[d]);
+  }
+  return o;
+}
+
+// PreprocessImage (Examples/itkantsRegistrationHelper.hxx:74-136) with the
+// CLI defaults (no winsorizing, no histogram matching): lo = min,
+// hi = max + (max-min)/2560 (256-bin histogram, marginal scale 10), both
+// stored as float; IntensityWindowing to [0,1] with clamping.
+template <class R>
+void preprocess(int64_t N, const R* in, R* out) {
+  double mn = std::numeric_limits<double>::max(), mx = -std::numeric_limits<double>::max();
+  for (int64_t i = 0; i < N; ++i) {
+    mn = std::min(mn, static_cast<double>(in[i]));
+    mx = std::max(mx, static_cast<double>(in[i]));
+  }
+  const float lo = static_cast<float>(mn);
+  const float hi = static_cast<float>(mx + (mx - mn) / 2560.0);
+  const double scale = (hi > lo) ? 1.0 / (static_cast<double>(hi) - static_cast<double>(lo)) : 0.0;
+  const double shift = -static_cast<double>(lo) * scale;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < N; ++i) {
+    const double x = static_cast<double>(in[i]);
+    double v;
+    if (x < lo)
+      v = 0.0;
+    else if (x > hi)
+      v = 1.0;
+    else
+      v = x * scale + shift;
+    out[i] = static_cast<R>(v);
+  }
+}
+
+// The SyN stage: SyNImageRegistrationMethod level loop (InitializeRegistrationAtEachLevel,
+// StartOptimization, final composition) as configured by
+// Examples/itkantsRegistrationHelper.hxx:1983-2220.  Per iteration (both
+// updates from the same state): warp F and M through the inverse middle
+// fields, CC value and both derivatives, smooth at the update variance,
+// scale to the learning rate, compose onto the forward middle fields,
+// smooth at the total variance, invert (inverse from the previous inverse,
+// then forward as the inverse of the new inverse), E = mean of both values,
+// convergence monitor.  Final: fwd = M2M.inv o F2M.fwd, inv = F2M.inv o M2M.fwd
+// (same as Examples/itkantsRegistrationHelper.hxx:3698-3711).
+template <class R>
+int syn_register(const og_params* P, const Grid& gF, const R* fixed_in, const Grid& gM, const R* moving_in, R* out_fwd,
+                 R* out_inv, og_trace* trace, R* state4, double eps) {
+  set_threads(P->threads);
+  const int D = gF.D;
+  const int64_t NF = gF.count(), NM = gM.count();
+  std::vector<R> F(fixed_in, fixed_in + NF), M(moving_in, moving_in + NM);
+  if (P->preprocess) {
+    preprocess<R>(NF, fixed_in, F.data());
+    preprocess<R>(NM, moving_in, M.data());
+  }
+  std::vector<R> Fs(NF), Ms(NM);
+  std::vector<R> f2m_f, f2m_i, m2m_f, m2m_i;
+  Grid gPrev = gF;
+  if (trace) trace->n_records = 0;
+  for (int l = 0; l < P->n_levels; ++l) {
+    const Grid gl = shrink_grid(gF, P->shrink[l]);
+    const int64_t N = gl.count();
+    if (l == 0) {
+      f2m_f.assign(N * D, R(0));
+      f2m_i.assign(N * D, R(0));
+      m2m_f.assign(N * D, R(0));
+      m2m_i.assign(N * D, R(0));
+    } else {
+      std::vector<R> tmp(N * D);
+      resample_field<R>(gPrev, f2m_f.data(), gl, tmp.data());
+      f2m_f = tmp;
+      resample_field<R>(gPrev, f2m_i.data(), gl, tmp.data());
+      f2m_i = tmp;
+      resample_field<R>(gPrev, m2m_f.data(), gl, tmp.data());
+      m2m_f = tmp;
+      resample_field<R>(gPrev, m2m_i.data(), gl, tmp.data());
+      m2m_i = tmp;
+    }
+    gPrev = gl;
+    if (P->iterations[l] <= 0) continue;
+    smooth_image<R>(gF, F.data(), Fs.data(), P->sigma[l], P->sigma_in_mm != 0);
+    smooth_image<R>(gM, M.data(), Ms.data(), P->sigma[l], P->sigma_in_mm != 0);
+    std::vector<R> Fw(N), Mw(N), g1(N * D), g2(N * D), u1(N * D), u2(N * D), t1(N * D), t2(N * D), s1(N * D), s2(N * D);
+    ConvMonitor mon(P->conv_window);
+    bool converged = false;
+    int iter = 0;
+    while (iter++ < P->iterations[l] && !converged) {
+      warp_image<R>(gF, Fs.data(), gl, f2m_i.data(), Fw.data());
+      warp_image<R>(gM, Ms.data(), gl, m2m_i.data(), Mw.data());
+      const double value = cc_metric<R>(gl, Fw.data(), Mw.data(), P->radius, eps, g1.data(), g2.data());
+      smooth_field<R>(gl, g1.data(), u1.data(), P->update_variance);
+      smooth_field<R>(gl, g2.data(), u2.data(), P->update_variance);
+      scale_update<R>(gl, u1.data(), P->learning_rate);
+      scale_update<R>(gl, u2.data(), P->learning_rate);
+      compose<R>(gl, u1.data(), f2m_f.data(), t1.data());
+      compose<R>(gl, u2.data(), m2m_f.data(), t2.data());
+      smooth_field<R>(gl, t1.data(), s1.data(), P->total_variance);
+      smooth_field<R>(gl, t2.data(), s2.data(), P->total_variance);
+      int k[4];
+      std::vector<R> ninv(N * D), nfwd(N * D);
+      k[0] = invert_field<R>(gl, s1.data(), f2m_i.data(), ninv.data(), nullptr, nullptr);
+      k[1] = invert_field<R>(gl, ninv.data(), s1.data(), nfwd.data(), nullptr, nullptr);
+      f2m_i.swap(ninv);
+      f2m_f.swap(nfwd);
+      k[2] = invert_field<R>(gl, s2.data(), m2m_i.data(), ninv.data(), nullptr, nullptr);
+      k[3] = invert_field<R>(gl, ninv.data(), s2.data(), nfwd.data(), nullptr, nullptr);
+      m2m_i.swap(ninv);
+      m2m_f.swap(nfwd);
+      const double E = 0.5 * (value + value);
+      mon.add(E);
+      const double conv = mon.value();
+      if (!P->force_iterations && conv < P->conv_threshold) converged = true;
+      if (trace && trace->n_records < trace->max_records) {
+        const int r = trace->n_records++;
+        trace->level[r] = l;
+        trace->iteration[r] = iter;
+        trace->energy[r] = E;
+        trace->convergence[r] = conv;
+        for (int j = 0; j < 4; ++j) trace->inv_iters[r * 4 + j] = k[j];
+      }
+    }
+  }
+  // final composition on the last level grid (the full-resolution grid when the last shrink is 1)
+  const int64_t N = gPrev.count();
+  if (out_fwd) compose<R>(gPrev, m2m_i.data(), f2m_f.data(), out_fwd);
+  if (out_inv) compose<R>(gPrev, f2m_i.data(), m2m_f.data(), out_inv);
+  if (state4) {
+    std::memcpy(state4 + 0 * N * D, f2m_f.data(), sizeof(R) * N * D);
+    std::memcpy(state4 + 1 * N * D, f2m_i.data(), sizeof(R) * N * D);
+    std::memcpy(state4 + 2 * N * D, m2m_f.data(), sizeof(R) * N * D);
+    std::memcpy(state4 + 3 * N * D, m2m_i.data(), sizeof(R) * N * D);
+  }
+  return 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C entry points (ctypes).  `_f64` is the FP64 oracle, `_f32` the float model.
+extern "C" {
+
+int og_gaussian_kernel(double variance, double max_error, int64_t max_width, double* out, int max_out) {
+  std::vector<double> h = gaussian_half_kernel(variance, max_error, max_width);
+  const int n = static_cast<int>(h.size());
+  for (int i = 0; i < n && i < max_out; ++i) out[i] = h[i];
+  return n - 1;  // radius
+}
+
+double og_bessel_in(int n, double x) { return bessel_in(n, x); }
+
+void og_shrink_grid(const og_grid* g, int factor, og_grid* out) {
+  Grid r = shrink_grid(to_grid(g), factor);
+  out->dim = r.D;
+  for (int d = 0; d < 3; ++d) {
+    out->n[d] = r.n[d];
+    out->spacing[d] = r.sp[d];
+    out->origin[d] = r.org[d];
+  }
+}
+
+void og_smooth_image_f64(const og_grid* g, const double* in, double* out, double sigma, int in_mm) {
+  smooth_image<double>(to_grid(g), in, out, sigma, in_mm != 0);
+}
+void og_smooth_field_f64(const og_grid* g, const double* in, double* out, double variance) {
+  smooth_field<double>(to_grid(g), in, out, variance);
+}
+void og_warp_image_f64(const og_grid* gi, const double* img, const og_grid* go, const double* u, double* out) {
+  warp_image<double>(to_grid(gi), img, to_grid(go), u, out);
+}
+void og_compose_f64(const og_grid* g, const double* disp, const double* warp, double* out) {
+  compose<double>(to_grid(g), disp, warp, out);
+}
+void og_resample_field_f64(const og_grid* gi, const double* in, const og_grid* go, double* out) {
+  resample_field<double>(to_grid(gi), in, to_grid(go), out);
+}
+double og_cc_f64(const og_grid* g, const double* F, const double* M, int radius, double eps, double* g_f2m, double* g_m2m) {
+  return cc_metric<double>(to_grid(g), F, M, radius, eps, g_f2m, g_m2m);
+}
+double og_scale_update_f64(const og_grid* g, double* u, double lr) { return scale_update<double>(to_grid(g), u, lr); }
+int og_invert_f64(const og_grid* g, const double* field, const double* init, double* inv, double* out_max, double* out_mean) {
+  return invert_field<double>(to_grid(g), field, init, inv, out_max, out_mean);
+}
+void og_preprocess_f64(int64_t n, const double* in, double* out) { preprocess<double>(n, in, out); }
+
+double og_conv_value(const double* energies, int n, int window) {
+  ConvMonitor m(window);
+  for (int i = 0; i < n; ++i) m.add(energies[i]);
+  return m.value();
+}
+
+// Nearest-neighbour warp of an integer label map (antsApplyTransforms -n
+// NearestNeighbor, Examples/antsApplyTransforms.cxx:593-630): point x + u(x)
+// in double, continuous index in the label grid, outside -> 0, else the label
+// at floor(c + 0.5) per axis (ITK rounds half-integers up).
+void og_warp_labels_nn(const og_grid* gout, const float* u, const og_grid* glab, const int32_t* labels, int32_t* out) {
+  const Grid go = to_grid(gout), gl = to_grid(glab);
+  const int D = go.D;
+#pragma omp parallel for schedule(static)
+  for (int64_t z = 0; z < go.n[2]; ++z)
+    for (int64_t y = 0; y < go.n[1]; ++y)
+      for (int64_t x = 0; x < go.n[0]; ++x) {
+        const int64_t i = x + go.n[0] * (y + go.n[1] * z);
+        double p[3], c[3];
+        grid_point(go, x, y, z, p);
+        if (u)
+          for (int d = 0; d < D; ++d) p[d] += static_cast<double>(u[i * D + d]);
+        to_cidx(gl, p, c);
+        int32_t v = 0;
+        if (inside(gl, c)) {
+          int64_t ix[3] = {0, 0, 0};
+          for (int d = 0; d < D; ++d) ix[d] = static_cast<int64_t>(std::floor(c[d] + 0.5));
+          v = labels[ix[0] + gl.n[0] * (ix[1] + gl.n[1] * ix[2])];
+        }
+        out[i] = v;
+      }
+}
+
+int og_syn_register_f64(const og_params* P, const og_grid* gf, const double* fixed, const og_grid* gm, const double* moving,
+                        double* out_fwd, double* out_inv, og_trace* trace, double* state4, double eps) {
+  return syn_register<double>(P, to_grid(gf), fixed, to_grid(gm), moving, out_fwd, out_inv, trace, state4, eps);
+}
+
+int og_syn_register_f32(const og_params* P, const og_grid* gf, const float* fixed, const og_grid* gm, const float* moving,
+                        float* out_fwd, float* out_inv, og_trace* trace, float* state4, double eps) {
+  return syn_register<float>(P, to_grid(gf), fixed, to_grid(gm), moving, out_fwd, out_inv, trace, state4, eps);
+}
+
+int og_max_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+}

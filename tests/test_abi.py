@@ -1,0 +1,67 @@
+"""The C-ABI library builds, loads and exports every declared symbol; the
+host-side parsers mirror antsRegistration's option grammar.  CPU only: no
+compute call is made here."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle as o
+from ants_amd import _lib as L
+from ants_amd import registration as R
+
+
+def test_library_exports_every_header_symbol():
+    names = L.header_functions()
+    assert "ants_syn_register" in names and "ants_op_cc" in names and len(names) > 30
+    lib = L.lib()
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert missing == []
+
+
+def test_level_geometry_matches_oracle():
+    p = L.SynParamsC()
+    p.dim = 3
+    p.n_levels = 3
+    for i, (f, s, it) in enumerate([(4, 2.0, 100), (2, 1.0, 70), (1, 0.0, 20)]):
+        p.shrink[i], p.sigma[i], p.iterations[i] = f, s, it
+    p.learning_rate, p.update_variance, p.cc_radius, p.convergence_window = 0.1, 3.0, 4, 10
+    for size, sp, org in [((192, 224, 192), (1, 1, 1), (0, 0, 0)), ((97, 64, 33), (1.0, 0.8, 2.5), (3, -7, 11))]:
+        for lvl in range(3):
+            out = L.ImageDesc()
+            L.check(L.lib().ants_syn_level_geometry(C.byref(p), C.byref(L.desc(size, sp, org)), lvl, C.byref(out)))
+            ref = o.shrink_grid(o.Grid(list(size), list(sp), list(org)), p.shrink[lvl])
+            assert [out.size[k] for k in range(3)] == ref.size
+            np.testing.assert_allclose([out.spacing[k] for k in range(3)], ref.spacing)
+            np.testing.assert_allclose([out.origin[k] for k in range(3)], ref.origin)
+
+
+def test_invalid_parameters_are_reported():
+    p = L.SynParamsC()
+    p.dim = 4
+    h = C.c_void_p()
+    rc = L.lib().ants_syn_create(C.byref(p), C.byref(h))
+    assert rc == 1 and b"dim" in L.lib().ants_last_error()
+    p.dim, p.n_levels = 3, 0
+    assert L.lib().ants_syn_create(C.byref(p), C.byref(h)) == 1
+    p.n_levels, p.shrink[0], p.iterations[0], p.cc_radius, p.learning_rate, p.convergence_window = 1, 1, 1, 4, 0.1, 10
+    out = L.ImageDesc()
+    bad = L.desc((8, 8, 8))
+    bad.direction[1] = 0.5
+    assert L.lib().ants_syn_level_geometry(C.byref(p), C.byref(bad), 0, C.byref(out)) == 4
+
+
+def test_cli_grammar():
+    st = R.SyNStage.from_args("SyN[0.1,3,0]", "CC[fixed.nii.gz,moving.nii.gz,1,4]", "[100x70x20,1e-6,10]",
+                              "4x2x1", "2x1x0vox")
+    assert st.transform == R.SyNTransform(0.1, 3.0, 0.0)
+    assert st.metric.radius == 4 and st.convergence.iterations == [100, 70, 20]
+    assert st.shrink == [4, 2, 1] and st.sigmas == [2.0, 1.0, 0.0] and not st.sigmas_in_mm
+    st = R.SyNStage.from_args("SyN[0.25]", "CC[a,b,1]", "40x20x0", "4x2x1", "1mm")
+    assert st.transform.update_variance == 3.0 and st.transform.total_variance == 0.0
+    assert st.metric.radius == 4 and st.convergence.threshold == 1e-6 and st.convergence.window == 10
+    assert st.sigmas == [1.0, 1.0, 1.0] and st.sigmas_in_mm
+    with pytest.raises(ValueError):
+        R.SyNStage.from_args("SyN[0.1]", "CC[a,b,1,4]", "100x70", "4x2x1", "2x1x0")
+    with pytest.raises(ValueError):
+        R.SyNTransform.parse("Affine[0.1]")

@@ -1,0 +1,121 @@
+"""Whole SyN stage on the GPU against the FP64 oracle.
+
+Bars (stated in BASELINE.json's north star): per-iteration CC energy within
+1e-4 relative, forward and inverse fields within 0.05 voxel max-abs, and
+nearest-neighbour label warps bit-exact given the same field.  The full
+192x224x192 workload is checked through size-independent properties
+(inverse consistency, energy descent, determinism)."""
+import numpy as np
+import pytest
+
+import oracle as o
+from ants_amd import _lib as L
+from ants_amd import phantoms as ph
+from ants_amd.registration import SyNRegistration, SyNStage
+
+pytestmark = pytest.mark.gpu
+
+ENERGY_RTOL = 1e-4
+FIELD_ATOL_VOX = 0.05
+
+
+def run_pair(stage, pair, force=False):
+    reg = SyNRegistration(stage, len(pair.size), force_iterations=force)
+    res = reg.register(pair.fixed, pair.moving, pair.spacing, pair.origin, pair.spacing, pair.origin)
+    reg.close()
+    return res
+
+
+def run_oracle(stage, pair, force=False):
+    P = o.SynParams(shrink=stage.shrink, sigma=stage.sigmas, iterations=stage.convergence.iterations,
+                    learning_rate=stage.transform.gradient_step, update_variance=stage.transform.update_variance,
+                    total_variance=stage.transform.total_variance, radius=stage.metric.radius,
+                    conv_threshold=stage.convergence.threshold, conv_window=stage.convergence.window,
+                    sigma_in_mm=stage.sigmas_in_mm, force_iterations=force)
+    g = o.Grid(list(pair.size), list(pair.spacing), list(pair.origin))
+    return o.syn_register(P, g, pair.fixed, g, pair.moving)
+
+
+def compare(res, ref, spacing):
+    lv = res.levels
+    assert list(np.bincount(lv, minlength=3)) == list(np.bincount(ref.level, minlength=3)), \
+        "iteration counts per level differ (convergence decision)"
+    e, er = res.energy, ref.energy
+    rel = np.abs(e - er) / np.abs(er)
+    assert rel.max() < ENERGY_RTOL, f"energy rel err {rel.max():.3e} at record {rel.argmax()}"
+    sp = np.asarray(spacing)
+    dfwd = np.abs(res.fwd - ref.fwd) / sp
+    dinv = np.abs(res.inv - ref.inv) / sp
+    assert dfwd.max() < FIELD_ATOL_VOX, f"fwd field deviation {dfwd.max():.4f} voxel"
+    assert dinv.max() < FIELD_ATOL_VOX, f"inv field deviation {dinv.max():.4f} voxel"
+    return rel.max(), dfwd.max(), dinv.max()
+
+
+def test_syn_2d_blobs_full_schedule():
+    pair = ph.blob_pair_2d(0)
+    stage = SyNStage.from_args("SyN[0.25,3,0]", "CC[f,m,1,4]", "[100x70x50,1e-6,10]", "4x2x1", "2x1x0vox")
+    res = run_pair(stage, pair)
+    ref = run_oracle(stage, pair)
+    print("2d parity (energy rel, fwd vox, inv vox):", compare(res, ref, pair.spacing))
+    kin = np.array([r.inv_iters for r in res.trace])
+    np.testing.assert_array_equal(kin, ref.inv_iters)
+
+
+def test_syn_3d_phantom_small():
+    pair = ph.phantom_pair_3d(3, size=(48, 56, 48), max_disp_mm=4.0, svf_sigma_mm=6.0)
+    stage = SyNStage.from_args("SyN[0.1,3,0]", "CC[f,m,1,4]", "[20x10x5,1e-6,10]", "4x2x1", "2x1x0vox")
+    res = run_pair(stage, pair, force=True)
+    ref = run_oracle(stage, pair, force=True)
+    print("3d parity (energy rel, fwd vox, inv vox):", compare(res, ref, pair.spacing))
+
+
+def test_syn_3d_radius2_quick_schedule():
+    pair = ph.phantom_pair_3d(5, size=(40, 40, 40), max_disp_mm=3.0, svf_sigma_mm=6.0)
+    stage = SyNStage.from_args("SyN[0.1,3,0]", "CC[f,m,1,2]", "[40x20x0,1e-6,10]", "4x2x1", "2x1x0vox")
+    res = run_pair(stage, pair)
+    ref = run_oracle(stage, pair)
+    print("3d r2 quick parity:", compare(res, ref, pair.spacing))
+
+
+def test_label_warp_bit_exact_given_same_field():
+    pair = ph.phantom_pair_3d(3, size=(48, 56, 48), max_disp_mm=4.0, svf_sigma_mm=6.0)
+    stage = SyNStage.from_args("SyN[0.1,3,0]", "CC[f,m,1,4]", "[10x5x3,1e-6,10]", "4x2x1", "2x1x0vox")
+    res = run_pair(stage, pair)
+    import gpu_helpers as G
+    g = G.desc_of(pair.labels_moving.shape)
+    out = G.labels_nn(res.fwd, pair.labels_moving, g, g, pair.labels_moving.shape)
+    og = o.Grid(list(pair.size))
+    ref = o.warp_labels_nn(og, res.fwd, og, pair.labels_moving)
+    np.testing.assert_array_equal(out, ref)
+    # the registration brings the moving labels closer to the fixed labels
+    assert (out == pair.labels_fixed).mean() > (pair.labels_moving == pair.labels_fixed).mean()
+
+
+def test_deterministic_rerun():
+    pair = ph.blob_pair_2d(2, n=128)
+    stage = SyNStage.from_args("SyN[0.25,3,0]", "CC[f,m,1,4]", "[20x10x5,1e-6,10]", "4x2x1", "2x1x0vox")
+    a = run_pair(stage, pair)
+    b = run_pair(stage, pair)
+    np.testing.assert_array_equal(a.fwd, b.fwd)
+    np.testing.assert_array_equal(a.inv, b.inv)
+    np.testing.assert_array_equal(a.energy, b.energy)
+
+
+def test_full_size_properties():
+    """192x224x192 config: energy falls at every level, forward and inverse
+    fields are inverse-consistent in the interior, displacement is bounded."""
+    pair = ph.phantom_pair_3d(11)
+    stage = SyNStage.from_args("SyN[0.1,3,0]", "CC[f,m,1,4]", "[100x70x20,1e-6,10]", "4x2x1", "2x1x0vox")
+    res = run_pair(stage, pair)
+    lv = res.levels
+    for l in range(3):
+        e = res.energy[lv == l]
+        assert len(e) > 0 and e[-1] < e[0]
+    # inverse consistency: fwd(x) + inv(x + fwd(x)) ~ 0 away from the boundary
+    og = o.Grid(list(pair.size))
+    import gpu_helpers as G
+    comp = G.compose(res.inv, res.fwd)
+    core = comp[16:-16, 16:-16, 16:-16]
+    resid = np.sqrt((core ** 2).sum(-1))
+    assert np.percentile(resid, 99) < 0.5 and resid.mean() < 0.1
+    assert np.abs(res.fwd).max() < 20.0
