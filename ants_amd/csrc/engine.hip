@@ -534,8 +534,8 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
     ccp = std::max(ccp, cc_partials_needed(to_dev(lg[l]), P.cc_radius));
     invp = std::max(invp, inv_partials_needed(to_dev(lg[l])));
   }
-  ANTS_TRY(partials_cc.ensure(ccp * sizeof(double)));
-  ANTS_TRY(partials_inv.ensure(2 * invp * sizeof(double)));
+  ANTS_TRY(partials_cc.ensure(ccp * sizeof(Partial)));
+  ANTS_TRY(partials_inv.ensure(2 * invp * sizeof(Partial)));
   if (!host_stats) HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&host_stats), sizeof(IterStats), hipHostMallocDefault));
 
   hipStream_t s = stream;
@@ -600,7 +600,7 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
       prof.end(ANTS_KC_WARP, Nd * (2.0 * D * e_f + 2.0 * gsz * e_i + 2.0 * e_i));
       // 2. CC value and both derivatives (g_f2m -> wk[0], g_m2m -> wk[1])
       prof.begin(ANTS_KC_CC);
-      launch_cc(gd, Fw.f(), Mw.f(), P.cc_radius, eps, wk[0], wk[1], static_cast<double*>(partials_cc.p),
+      launch_cc(gd, Fw.f(), Mw.f(), P.cc_radius, eps, wk[0], wk[1], static_cast<Partial*>(partials_cc.p),
                 &dstats->energy, s);
       prof.end(ANTS_KC_CC, Nd * (2.0 * e_i + 2.0 * D * e_f));
       HIP_TRY(hipGetLastError());
@@ -644,7 +644,7 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
         RedSlot* slots[2] = {dstats->inv[0], dstats->inv[2]};
         for (int j = 0; j <= kInvMaxIter; ++j) {
           prof.begin(ANTS_KC_INV);
-          launch_inv_step(gd, 2, field, inv, slots, static_cast<double*>(partials_inv.p), j, s);
+          launch_inv_step(gd, 2, field, inv, slots, static_cast<Partial*>(partials_inv.p), j, s);
           prof.end(ANTS_KC_INV, 0.0, 0, j);
         }
       }
@@ -654,7 +654,7 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
         RedSlot* slots[2] = {dstats->inv[1], dstats->inv[3]};
         for (int j = 0; j <= kInvMaxIter; ++j) {
           prof.begin(ANTS_KC_INV);
-          launch_inv_step(gd, 2, field, inv, slots, static_cast<double*>(partials_inv.p), j, s);
+          launch_inv_step(gd, 2, field, inv, slots, static_cast<Partial*>(partials_inv.p), j, s);
           prof.end(ANTS_KC_INV, 0.0, 1, j);
         }
       }
@@ -941,11 +941,11 @@ int ants_op_cc(const ants_image_desc* gdsc, const float* d_fixed, const float* d
   const GridDev gd = to_dev(g);
   OpScratch sc;
   ANTS_TRY(sc.stats.ensure(sizeof(RedSlot)));
-  ANTS_TRY(sc.partials.ensure(cc_partials_needed(gd, radius) * sizeof(double)));
+  ANTS_TRY(sc.partials.ensure(cc_partials_needed(gd, radius) * sizeof(Partial)));
   hipStream_t s = static_cast<hipStream_t>(stream);
   HIP_TRY(hipMemsetAsync(sc.stats.p, 0, sizeof(RedSlot), s));
   launch_cc(gd, d_fixed, d_moving, radius, std::numeric_limits<float>::epsilon(), d_g_f2m, d_g_m2m,
-            static_cast<double*>(sc.partials.p), static_cast<RedSlot*>(sc.stats.p), s);
+            static_cast<Partial*>(sc.partials.p), static_cast<RedSlot*>(sc.stats.p), s);
   HIP_TRY(hipGetLastError());
   RedSlot r;
   HIP_TRY(hipMemcpyAsync(&r, sc.stats.p, sizeof(r), hipMemcpyDeviceToHost, s));
@@ -1009,7 +1009,7 @@ int ants_op_invert(const ants_image_desc* gdsc, const float* d_field, const floa
   const size_t b = static_cast<size_t>(g.count()) * g.D * sizeof(float);
   OpScratch sc;
   ANTS_TRY(sc.stats.ensure(sizeof(RedSlot) * kInvSlots));
-  ANTS_TRY(sc.partials.ensure(inv_partials_needed(gd) * sizeof(double)));
+  ANTS_TRY(sc.partials.ensure(inv_partials_needed(gd) * sizeof(Partial)));
   hipStream_t s = static_cast<hipStream_t>(stream);
   HIP_TRY(hipMemsetAsync(sc.stats.p, 0, sizeof(RedSlot) * kInvSlots, s));
   if (d_init)
@@ -1020,7 +1020,7 @@ int ants_op_invert(const ants_image_desc* gdsc, const float* d_field, const floa
   float* inv[1] = {d_inv};
   RedSlot* slots[1] = {static_cast<RedSlot*>(sc.stats.p)};
   for (int j = 0; j <= kInvMaxIter; ++j)
-    launch_inv_step(gd, 1, field, inv, slots, static_cast<double*>(sc.partials.p), j, s);
+    launch_inv_step(gd, 1, field, inv, slots, static_cast<Partial*>(sc.partials.p), j, s);
   HIP_TRY(hipGetLastError());
   RedSlot r[kInvSlots];
   HIP_TRY(hipMemcpyAsync(r, sc.stats.p, sizeof(r), hipMemcpyDeviceToHost, s));

@@ -164,13 +164,16 @@ __device__ __forceinline__ void block_sum_max(double& s, float& m) {
 }
 
 // Every block of the (sub)grid calls this once with its per-thread values.
+// Each block writes one (sum, max) partial; the last block to take a ticket
+// reduces all partials in block order, so the result does not depend on
+// scheduling.  Only the ticket is atomic (one per block).
 template <int NT>
-__device__ void grid_reduce(double s, float m, double* partials, RedSlot* slot, int nblocks, int bid) {
+__device__ void grid_reduce(double s, float m, Partial* partials, RedSlot* slot, int nblocks, int bid) {
   __shared__ int is_last;
   block_sum_max<NT>(s, m);
   if (threadIdx.x == 0) {
-    partials[bid] = s;
-    atomicMax(&slot->maxbits, __float_as_uint(m));
+    partials[bid].s = s;
+    partials[bid].m = m;
     __threadfence();
     const unsigned int t = atomicAdd(&slot->ticket, 1u);
     is_last = (t == static_cast<unsigned int>(nblocks - 1));
@@ -179,12 +182,15 @@ __device__ void grid_reduce(double s, float m, double* partials, RedSlot* slot, 
   if (!is_last) return;
   __threadfence();
   double acc = 0.0;
-  for (int i = threadIdx.x; i < nblocks; i += NT)
-    acc += __hip_atomic_load(&partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  float dummy = 0.f;
-  block_sum_max<NT>(acc, dummy);
+  float mx = 0.f;
+  for (int i = threadIdx.x; i < nblocks; i += NT) {
+    acc += __hip_atomic_load(&partials[i].s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    mx = fmaxf(mx, __hip_atomic_load(&partials[i].m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  }
+  block_sum_max<NT>(acc, mx);
   if (threadIdx.x == 0) {
     slot->sum = acc;
+    slot->maxbits = __float_as_uint(mx);
     slot->computed = 1;
     slot->ticket = 0u;
   }
@@ -215,7 +221,17 @@ __global__ void __launch_bounds__(kBlock) k_minmax(const float* __restrict__ in,
     mn = min(mn, static_cast<uint32_t>(__shfl_xor(static_cast<int>(mn), off, 64)));
     mx = max(mx, static_cast<uint32_t>(__shfl_xor(static_cast<int>(mx), off, 64)));
   }
+  __shared__ uint32_t smn[kBlock / 64], smx[kBlock / 64];
   if ((threadIdx.x & 63) == 0) {
+    smn[threadIdx.x >> 6] = mn;
+    smx[threadIdx.x >> 6] = mx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kBlock / 64; ++w) {
+      mn = min(mn, smn[w]);
+      mx = max(mx, smx[w]);
+    }
     atomicMin(&out2[0], mn);
     atomicMax(&out2[1], mx);
   }
@@ -276,14 +292,14 @@ __global__ void __launch_bounds__(kBlock) k_gauss_field_final(GridDev g, PtrPack
   const float* __restrict__ org = orig.p[k];
   float* __restrict__ dst = out.p[k];
   const int64_t N = g.count();
-  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  const int len = g.n[axis];
+  const int64_t stride = axis == 0 ? 1 : (axis == 1 ? g.n[0] : static_cast<int64_t>(g.n[0]) * g.n[1]);
   float nrm = 0.f;
-  if (i < N) {
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < N;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
     int x, y, z;
     unlin(g, i, x, y, z);
     const int pos = axis == 0 ? x : (axis == 1 ? y : z);
-    const int len = g.n[axis];
-    const int64_t stride = axis == 0 ? 1 : (axis == 1 ? g.n[0] : static_cast<int64_t>(g.n[0]) * g.n[1]);
     const int64_t base = i - static_cast<int64_t>(pos) * stride;
     float acc[D];
 #pragma unroll
@@ -305,10 +321,11 @@ __global__ void __launch_bounds__(kBlock) k_gauss_field_final(GridDev g, PtrPack
       const float q = v / g.sp[c];
       s2 += q * q;
     }
-    nrm = sqrtf(s2);
+    nrm = fmaxf(nrm, sqrtf(s2));
   }
-  nrm = wave_max(nrm);
-  if ((threadIdx.x & 63) == 0) atomicMax(&maxbits[k], __float_as_uint(nrm));
+  double dummy = 0.0;
+  block_sum_max<kBlock>(dummy, nrm);
+  if (threadIdx.x == 0) atomicMax(&maxbits[k], __float_as_uint(nrm));
 }
 
 // ---------------------------------------------------------------------------
@@ -365,28 +382,32 @@ __device__ __forceinline__ float cdiff(float vm, float v0, float vp, int pos, in
   return (up - dn) / h;
 }
 
-// Per-voxel CC value and both SyN derivatives from window sums.
+// Per-voxel CC value and both SyN derivatives from window sums.  The sums
+// and this formula are evaluated in double: products of float samples are
+// exact in double, so sFF = sum f^2 - (sum f)^2 / n keeps its digits where a
+// float evaluation cancels (flat, noisy windows).  FP64 vector FMA runs at
+// the non-packed FP32 rate on CDNA4.
 struct CCOut {
-  float cc;
+  double cc;
   float a1, a2;  // g_f2m = a1 * grad m ; g_m2m = a2 * grad f
 };
-__device__ __forceinline__ CCOut cc_voxel(float cnt, float sf, float sm, float sff, float smm, float sfm, float f,
-                                          float m, float eps) {
-  const float mf = sf / cnt, mm = sm / cnt;
-  const float sFF = sff - mf * sf - mf * sf + cnt * mf * mf;
-  const float sMM = smm - mm * sm - mm * sm + cnt * mm * mm;
-  const float sFM = sfm - mm * sf - mf * sm + cnt * mm * mf;
-  const float fA = f - mf, mA = m - mm;
+__device__ __forceinline__ CCOut cc_voxel(double cnt, double sf, double sm, double sff, double smm, double sfm,
+                                          float f, float m, float eps) {
+  const double mf = sf / cnt, mm = sm / cnt;
+  const double sFF = sff - mf * sf - mf * sf + cnt * mf * mf;
+  const double sMM = smm - mm * sm - mm * sm + cnt * mm * mm;
+  const double sFM = sfm - mm * sf - mf * sm + cnt * mm * mf;
+  const double fA = f - mf, mA = m - mm;
   CCOut o;
-  o.cc = 1.f;
+  o.cc = 1.0;
   o.a1 = 0.f;
   o.a2 = 0.f;
   if (sFF > eps && sMM > eps) {
-    const float p = sFF * sMM;
+    const double p = sFF * sMM;
     o.cc = sFM * sFM / p;
-    const float a = 2.f * sFM / p;
-    o.a1 = a * (fA - sFM / sMM * mA);
-    o.a2 = a * (mA - sFM / sFF * fA);
+    const double a = 2.0 * sFM / p;
+    o.a1 = static_cast<float>(a * (fA - sFM / sMM * mA));
+    o.a2 = static_cast<float>(a * (mA - sFM / sFF * fA));
   }
   return o;
 }
@@ -396,32 +417,32 @@ template <int D>
 __global__ void __launch_bounds__(kBlock) k_cc_direct(GridDev g, const float* __restrict__ F,
                                                       const float* __restrict__ M, int R, float eps,
                                                       float* __restrict__ g1, float* __restrict__ g2,
-                                                      double* partials, RedSlot* slot) {
+                                                      Partial* partials, RedSlot* slot) {
   const int64_t N = g.count();
-  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   double cc_sum = 0.0;
-  if (i < N) {
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < N;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
     int x, y, z;
     unlin(g, i, x, y, z);
     const int zlo = D == 3 ? max(z - R, 0) : 0, zhi = D == 3 ? min(z + R, g.n[2] - 1) : 0;
     const int ylo = max(y - R, 0), yhi = min(y + R, g.n[1] - 1);
     const int xlo = max(x - R, 0), xhi = min(x + R, g.n[0] - 1);
-    float sf = 0.f, sm = 0.f, sff = 0.f, smm = 0.f, sfm = 0.f;
+    double sf = 0.0, sm = 0.0, sff = 0.0, smm = 0.0, sfm = 0.0;
     for (int zz = zlo; zz <= zhi; ++zz)
       for (int yy = ylo; yy <= yhi; ++yy)
         for (int xx = xlo; xx <= xhi; ++xx) {
           const int64_t j = lin(g, xx, yy, zz);
-          const float f = F[j], m = M[j];
+          const double f = F[j], m = M[j];
           sf += f;
           sm += m;
           sff += f * f;
           smm += m * m;
           sfm += f * m;
         }
-    const float cnt = static_cast<float>((zhi - zlo + 1) * (yhi - ylo + 1) * (xhi - xlo + 1));
+    const double cnt = static_cast<double>((zhi - zlo + 1) * (yhi - ylo + 1) * (xhi - xlo + 1));
     const float f = F[i], m = M[i];
     const CCOut o = cc_voxel(cnt, sf, sm, sff, smm, sfm, f, m, eps);
-    cc_sum = o.cc;
+    cc_sum += o.cc;
     const int pos[3] = {x, y, z};
     const int64_t st[3] = {1, g.n[0], static_cast<int64_t>(g.n[0]) * g.n[1]};
 #pragma unroll
@@ -442,37 +463,37 @@ __global__ void __launch_bounds__(kBlock) k_cc_direct(GridDev g, const float* __
 }
 
 // 3-D CC with LDS-staged planes and a register ring along z.
-// Block = TX x TY output columns over a z-chunk.  Each step loads plane
-// zz (with an R halo in x and y) as the 5 quantities f, m, f^2, m^2, f*m,
-// forms the (2R+1)^2 in-plane box sums separably (x rows, then y), and
-// pushes them into a (2R+1)-deep register ring; the ring total is the
-// window sum for the voxel R planes behind.  Outside-volume samples are 0
-// and the count is analytic, which equals ITK's in-bounds-only sums.
+// Block = TX x TY output columns over a z-chunk.  Each step loads plane zz
+// of Fw and Mw (with an R halo in x and y) into LDS, forms the (2R+1)^2
+// in-plane box sums of f, m, f^2, m^2, f*m separably (x rows, then y) in
+// double, and pushes them into a (2R+1)-deep register ring; the ring total
+// is the window sum for the voxel R planes behind.  Outside-volume samples
+// are 0 and the count is analytic, which equals ITK's in-bounds-only sums.
 template <int R, int TX, int TY>
 __global__ void __launch_bounds__(TX* TY) k_cc3d(GridDev g, const float* __restrict__ F, const float* __restrict__ M,
                                                  float eps, int zchunk, float* __restrict__ g1,
-                                                 float* __restrict__ g2, double* partials, RedSlot* slot) {
+                                                 float* __restrict__ g2, Partial* partials, RedSlot* slot) {
   constexpr int W = 2 * R + 1;
   constexpr int LX = TX + 2 * R, LY = TY + 2 * R;
   constexpr int NT = TX * TY;
-  __shared__ float q[5][LY][LX];
-  __shared__ float sx[5][LY][TX];
+  __shared__ float q[2][LY][LX];
+  __shared__ double sx[5][LY][TX];
   const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
   const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
   const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, z0 = blockIdx.z * zchunk;
   const int z1 = min(z0 + zchunk, nz);
   const int x = x0 + tx, y = y0 + ty;
   const bool active = x < nx && y < ny;
-  const float cx = static_cast<float>(min(x + R, nx - 1) - max(x - R, 0) + 1);
-  const float cy = static_cast<float>(min(y + R, ny - 1) - max(y - R, 0) + 1);
+  const double cxy = static_cast<double>((min(x + R, nx - 1) - max(x - R, 0) + 1) *
+                                         (min(y + R, ny - 1) - max(y - R, 0) + 1));
 
-  float ring[5][W];
+  double ring[5][W];
 #pragma unroll
   for (int a = 0; a < 5; ++a)
 #pragma unroll
-    for (int b = 0; b < W; ++b) ring[a][b] = 0.f;
-  float rf[R + 2], rm[R + 2];                                // raw centre ring (newest last)
-  float rgf[R + 1][2], rgm[R + 1][2];                        // in-plane gradients ring (newest last)
+    for (int b = 0; b < W; ++b) ring[a][b] = 0.0;
+  float rf[R + 2], rm[R + 2];          // raw centre ring (newest last)
+  float rgf[R + 1][2], rgm[R + 1][2];  // in-plane gradients ring (newest last)
 #pragma unroll
   for (int b = 0; b < R + 2; ++b) rf[b] = rm[b] = 0.f;
 #pragma unroll
@@ -480,33 +501,59 @@ __global__ void __launch_bounds__(TX* TY) k_cc3d(GridDev g, const float* __restr
   double cc_sum = 0.0;
   const int64_t plane = static_cast<int64_t>(nx) * ny;
 
-  for (int zz = z0 - R; zz < z1 + R; ++zz) {
-    const bool zin = zz >= 0 && zz < nz;
-    __syncthreads();
-    for (int k = threadIdx.x; k < LX * LY; k += NT) {
+  // Plane samples are prefetched into registers one plane ahead, so their
+  // global-load latency overlaps the box sums of the current plane.
+  constexpr int KP = (LX * LY + NT - 1) / NT;
+  float pf[KP], pm[KP];
+  auto fetch = [&](int zq) {
+    const bool zin_q = zq >= 0 && zq < nz;
+#pragma unroll
+    for (int u = 0; u < KP; ++u) {
+      const int k = threadIdx.x + u * NT;
       const int ly = k / LX, lx = k - ly * LX;
       const int gx = x0 - R + lx, gy = y0 - R + ly;
       float f = 0.f, m = 0.f;
-      if (zin && gx >= 0 && gx < nx && gy >= 0 && gy < ny) {
-        const int64_t j = static_cast<int64_t>(zz) * plane + static_cast<int64_t>(gy) * nx + gx;
+      if (k < LX * LY && zin_q && gx >= 0 && gx < nx && gy >= 0 && gy < ny) {
+        const int64_t j = static_cast<int64_t>(zq) * plane + static_cast<int64_t>(gy) * nx + gx;
         f = F[j];
         m = M[j];
       }
-      q[0][ly][lx] = f;
-      q[1][ly][lx] = m;
-      q[2][ly][lx] = f * f;
-      q[3][ly][lx] = m * m;
-      q[4][ly][lx] = f * m;
+      pf[u] = f;
+      pm[u] = m;
+    }
+  };
+  fetch(z0 - R);
+
+  for (int zz = z0 - R; zz < z1 + R; ++zz) {
+    const bool zin = zz >= 0 && zz < nz;
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < KP; ++u) {
+      const int k = threadIdx.x + u * NT;
+      if (k < LX * LY) {
+        const int ly = k / LX, lx = k - ly * LX;
+        q[0][ly][lx] = pf[u];
+        q[1][ly][lx] = pm[u];
+      }
     }
     __syncthreads();
+    if (zz + 1 < z1 + R) fetch(zz + 1);
     for (int ly = ty; ly < LY; ly += TY) {
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;
 #pragma unroll
-      for (int a = 0; a < 5; ++a) {
-        float s = 0.f;
-#pragma unroll
-        for (int b = 0; b < W; ++b) s += q[a][ly][tx + b];
-        sx[a][ly][tx] = s;
+      for (int b = 0; b < W; ++b) {
+        const double f = q[0][ly][tx + b], m = q[1][ly][tx + b];
+        s0 += f;
+        s1 += m;
+        s2 += f * f;
+        s3 += m * m;
+        s4 += f * m;
       }
+      sx[0][ly][tx] = s0;
+      sx[1][ly][tx] = s1;
+      sx[2][ly][tx] = s2;
+      sx[3][ly][tx] = s3;
+      sx[4][ly][tx] = s4;
     }
     // centre values and in-plane gradients of plane zz at (x, y)
     const float fc = q[0][ty + R][tx + R], mc = q[1][ty + R][tx + R];
@@ -518,13 +565,13 @@ __global__ void __launch_bounds__(TX* TY) k_cc3d(GridDev g, const float* __restr
       gmy = cdiff(q[1][ty + R - 1][tx + R], mc, q[1][ty + R + 1][tx + R], y, ny, g.sp[1]);
     }
     __syncthreads();
-    float p[5];
+    double p[5];
 #pragma unroll
     for (int a = 0; a < 5; ++a) {
-      float s = 0.f;
+      double t = 0.0;
 #pragma unroll
-      for (int b = 0; b < W; ++b) s += sx[a][ty + b][tx];
-      p[a] = s;
+      for (int b = 0; b < W; ++b) t += sx[a][ty + b][tx];
+      p[a] = t;
     }
 #pragma unroll
     for (int a = 0; a < 5; ++a) {
@@ -553,18 +600,18 @@ __global__ void __launch_bounds__(TX* TY) k_cc3d(GridDev g, const float* __restr
 
     const int zc = zz - R;
     if (active && zc >= z0 && zc < z1) {
-      float s[5];
+      double sm5[5];
 #pragma unroll
       for (int a = 0; a < 5; ++a) {
-        float t = 0.f;
+        double t = 0.0;
 #pragma unroll
         for (int b = 0; b < W; ++b) t += ring[a][b];
-        s[a] = t;
+        sm5[a] = t;
       }
-      const float cz = static_cast<float>(min(zc + R, nz - 1) - max(zc - R, 0) + 1);
+      const double cz = static_cast<double>(min(zc + R, nz - 1) - max(zc - R, 0) + 1);
       // raw centre column: rf[R+1] = zz = zc+R ... rf[1] = zc, rf[0] = zc-1, rf[2] = zc+1
       const float f = rf[1], m = rm[1];
-      const CCOut o = cc_voxel(cx * cy * cz, s[0], s[1], s[2], s[3], s[4], f, m, eps);
+      const CCOut o = cc_voxel(cxy * cz, sm5[0], sm5[1], sm5[2], sm5[3], sm5[4], f, m, eps);
       cc_sum += o.cc;
       float gmz = 0.f, gfz = 0.f;
       if (o.a1 != 0.f || o.a2 != 0.f) {
@@ -622,6 +669,11 @@ __global__ void __launch_bounds__(kBlock) k_compose(GridDev g, PtrPack4 disp, Pt
 // (eps 0.75 for the first update, 0.5 after; boundary voxels zeroed), and,
 // if j < 20 and max_{j-1} > 0.1 and mean_{j-1} > 0.001, the norms n_j of the
 // new inverse.  In-place on inv: each voxel reads and writes only itself.
+// The max test carries a 1e-5 relative guard: with a zero initial inverse the
+// first residual's max norm equals the learning rate, which for SyN[0.1,..]
+// sits exactly on the 0.1 tolerance; the guard resolves that tie the same way
+// in float and double.
+constexpr float kInvMaxTol = 0.1f * (1.0f + 1e-5f);
 template <int D>
 __device__ __forceinline__ float inv_residual(const GridDev& g, const float* __restrict__ field, int x, int y, int z,
                                               const float iv[3], float e[3]) {
@@ -640,7 +692,7 @@ __device__ __forceinline__ float inv_residual(const GridDev& g, const float* __r
 
 template <int D>
 __global__ void __launch_bounds__(kBlock) k_inv_step(GridDev g, PtrPack4 field, MutPack4 inv, SlotPack4 slots,
-                                                     double* partials, int64_t part_stride, int j) {
+                                                     Partial* partials, int64_t part_stride, int j) {
   const int k = blockIdx.y;
   RedSlot* st = slots.p[k];
   bool do_update = false, do_norm = false;
@@ -654,17 +706,17 @@ __global__ void __launch_bounds__(kBlock) k_inv_step(GridDev g, PtrPack4 field, 
       maxprev = __uint_as_float(prev.maxbits);
       const double mean = prev.sum / static_cast<double>(g.count());
       eps = (j - 1 == 0) ? 0.75f : 0.5f;
-      do_norm = j < kInvMaxIter && maxprev > 0.1f && mean > 0.001;
+      do_norm = j < kInvMaxIter && maxprev > kInvMaxTol && mean > 0.001;
     }
   }
   if (!do_update && !do_norm) return;
   const float* __restrict__ fld = field.p[k];
   float* __restrict__ iv = inv.p[k];
   const int64_t N = g.count();
-  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   double nsum = 0.0;
   float nmax = 0.f;
-  if (i < N) {
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < N;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
     int x, y, z;
     unlin(g, i, x, y, z);
     float cur[3] = {0.f, 0.f, 0.f}, e[3];
@@ -684,8 +736,8 @@ __global__ void __launch_bounds__(kBlock) k_inv_step(GridDev g, PtrPack4 field, 
     }
     if (do_norm) {
       const float nrm = inv_residual<D>(g, fld, x, y, z, cur, e);
-      nsum = nrm;
-      nmax = nrm;
+      nsum += nrm;
+      nmax = fmaxf(nmax, nrm);
     }
   }
   if (do_norm)
@@ -735,9 +787,13 @@ __global__ void __launch_bounds__(kBlock) k_labels_nn(GridDev go, const float* _
 }
 
 inline unsigned int blocks_for(int64_t n, int bs = kBlock) { return static_cast<unsigned int>((n + bs - 1) / bs); }
+inline unsigned int red_blocks(int64_t n) {
+  const unsigned int b = blocks_for(n);
+  return b < static_cast<unsigned int>(kRedBlocks) ? (b < 1 ? 1 : b) : static_cast<unsigned int>(kRedBlocks);
+}
 inline unsigned int grid_stride_blocks(int64_t n) {
   int64_t b = (n + kBlock - 1) / kBlock;
-  return static_cast<unsigned int>(b < 4096 ? (b < 1 ? 1 : b) : 4096);
+  return static_cast<unsigned int>(b < kRedBlocks ? (b < 1 ? 1 : b) : kRedBlocks);
 }
 
 template <class A>
@@ -752,8 +808,15 @@ MutPack4 mpack(float* const* p, int np) {
   return r;
 }
 
-// CC tile shapes (3-D): TX x TY columns, z-chunk depth.
-constexpr int kCCTX = 64, kCCTY = 4, kCCZ = 32;
+// CC tile shapes (3-D): TX x TY columns; the z-chunk depth is chosen per grid
+// so that a launch has about 1024 blocks (between 8 and 32 planes).
+constexpr int kCCTX = 64, kCCTY = 4;
+int cc_zchunk(const GridDev& g) {
+  const int64_t tiles = static_cast<int64_t>((g.n[0] + kCCTX - 1) / kCCTX) * ((g.n[1] + kCCTY - 1) / kCCTY);
+  int64_t zc = (static_cast<int64_t>(g.n[2]) * tiles) / 1024;
+  zc = zc < 8 ? 8 : (zc > 32 ? 32 : zc);
+  return static_cast<int>(zc);
+}
 
 }  // namespace
 
@@ -785,7 +848,7 @@ void launch_gauss_axis(const GridDev& g, int ncomp, const float* in, float* out,
 void launch_gauss_field_final(const GridDev& g, int np, const float* const* in, const float* const* orig,
                               float* const* out, const Taps& t, int axis, float w1, float w2, bool boundary,
                               unsigned int* maxbits, hipStream_t s) {
-  const dim3 grid(blocks_for(g.count()), np);
+  const dim3 grid(red_blocks(g.count()), np);
   if (g.D == 2)
     hipLaunchKernelGGL(k_gauss_field_final<2>, grid, dim3(kBlock), 0, s, g, pack(in, np), pack(orig, np),
                        mpack(out, np), t, axis, w1, w2, static_cast<int>(boundary), maxbits);
@@ -816,30 +879,32 @@ static bool cc_tiled(const GridDev& g, int radius) { return g.D == 3 && (radius 
 
 int64_t cc_partials_needed(const GridDev& g, int radius) {
   if (cc_tiled(g, radius)) {
+    const int zc = cc_zchunk(g);
     return static_cast<int64_t>((g.n[0] + kCCTX - 1) / kCCTX) * ((g.n[1] + kCCTY - 1) / kCCTY) *
-           ((g.n[2] + kCCZ - 1) / kCCZ);
+           ((g.n[2] + zc - 1) / zc);
   }
-  return blocks_for(g.count());
+  return red_blocks(g.count());
 }
 
 void launch_cc(const GridDev& g, const float* F, const float* M, int radius, float eps, float* gF2M, float* gM2M,
-               double* partials, RedSlot* slot, hipStream_t s) {
+               Partial* partials, RedSlot* slot, hipStream_t s) {
   if (!cc_tiled(g, radius)) {
     launch_cc_direct(g, F, M, radius, eps, gF2M, gM2M, partials, slot, s);
     return;
   }
-  const dim3 grid((g.n[0] + kCCTX - 1) / kCCTX, (g.n[1] + kCCTY - 1) / kCCTY, (g.n[2] + kCCZ - 1) / kCCZ);
+  const int zc = cc_zchunk(g);
+  const dim3 grid((g.n[0] + kCCTX - 1) / kCCTX, (g.n[1] + kCCTY - 1) / kCCTY, (g.n[2] + zc - 1) / zc);
   if (radius == 4)
-    hipLaunchKernelGGL((k_cc3d<4, kCCTX, kCCTY>), grid, dim3(kCCTX * kCCTY), 0, s, g, F, M, eps, kCCZ, gF2M, gM2M,
+    hipLaunchKernelGGL((k_cc3d<4, kCCTX, kCCTY>), grid, dim3(kCCTX * kCCTY), 0, s, g, F, M, eps, zc, gF2M, gM2M,
                        partials, slot);
   else
-    hipLaunchKernelGGL((k_cc3d<2, kCCTX, kCCTY>), grid, dim3(kCCTX * kCCTY), 0, s, g, F, M, eps, kCCZ, gF2M, gM2M,
+    hipLaunchKernelGGL((k_cc3d<2, kCCTX, kCCTY>), grid, dim3(kCCTX * kCCTY), 0, s, g, F, M, eps, zc, gF2M, gM2M,
                        partials, slot);
 }
 
 void launch_cc_direct(const GridDev& g, const float* F, const float* M, int radius, float eps, float* gF2M,
-                      float* gM2M, double* partials, RedSlot* slot, hipStream_t s) {
-  const dim3 grid(blocks_for(g.count()));
+                      float* gM2M, Partial* partials, RedSlot* slot, hipStream_t s) {
+  const dim3 grid(red_blocks(g.count()));
   if (g.D == 2)
     hipLaunchKernelGGL(k_cc_direct<2>, grid, dim3(kBlock), 0, s, g, F, M, radius, eps, gF2M, gM2M, partials, slot);
   else
@@ -857,11 +922,11 @@ void launch_compose(const GridDev& g, int np, const float* const* disp, const fl
                        maxbits, rate);
 }
 
-int64_t inv_partials_needed(const GridDev& g) { return blocks_for(g.count()); }
+int64_t inv_partials_needed(const GridDev& g) { return red_blocks(g.count()); }
 
 void launch_inv_step(const GridDev& g, int np, const float* const* field, float* const* inv, RedSlot* const* slots,
-                     double* partials, int j, hipStream_t s) {
-  const dim3 grid(blocks_for(g.count()), np);
+                     Partial* partials, int j, hipStream_t s) {
+  const dim3 grid(red_blocks(g.count()), np);
   SlotPack4 sp{};
   for (int k = 0; k < np; ++k) sp.p[k] = slots[k];
   const int64_t stride = inv_partials_needed(g);

@@ -35,6 +35,17 @@ struct RedSlot {
   int pad;
 };
 
+// Per-block partial of a deterministic grid reduction.
+struct Partial {
+  double s;
+  float m;
+  unsigned int pad;
+};
+
+// Reducing elementwise kernels run grid-stride loops over at most this many
+// blocks per problem, so the final (last-block) reduction reads few partials.
+constexpr int kRedBlocks = 1024;
+
 constexpr int kInvMaxIter = 20;            // InvertDisplacementFieldImageFilter MaximumNumberOfIterations (SyN)
 constexpr int kInvSlots = kInvMaxIter + 1;
 
@@ -69,12 +80,12 @@ void launch_warp_pair(const GridDev& gl, const GridDev& gF, const float* Fs, con
 void launch_warp_image(const GridDev& gi, const float* img, const GridDev& go, const float* u, float* out,
                        hipStream_t s);
 
-// CC value + both derivatives.  `partials` must hold cc_partials_needed(g) doubles.
+// CC value + both derivatives.  `partials` must hold cc_partials_needed(g) entries.
 int64_t cc_partials_needed(const GridDev& g, int radius);
 void launch_cc(const GridDev& g, const float* F, const float* M, int radius, float eps, float* gF2M, float* gM2M,
-               double* partials, RedSlot* slot, hipStream_t s);
+               Partial* partials, RedSlot* slot, hipStream_t s);
 void launch_cc_direct(const GridDev& g, const float* F, const float* M, int radius, float eps, float* gF2M,
-                      float* gM2M, double* partials, RedSlot* slot, hipStream_t s);
+                      float* gM2M, Partial* partials, RedSlot* slot, hipStream_t s);
 
 // out_k = warp_k + scale_k * disp_k(x + warp_k(x)), scale_k = rate / max_k when maxbits given (max>0), else 1.
 void launch_compose(const GridDev& g, int np, const float* const* disp, const float* const* warp,
@@ -83,7 +94,7 @@ void launch_compose(const GridDev& g, int np, const float* const* disp, const fl
 // One fused fixed-point inversion step j (0..20) for np problems (see kernels.hip).
 int64_t inv_partials_needed(const GridDev& g);
 void launch_inv_step(const GridDev& g, int np, const float* const* field, float* const* inv, RedSlot* const* slots,
-                     double* partials, int j, hipStream_t s);
+                     Partial* partials, int j, hipStream_t s);
 
 // Resample np fields from grid gi onto grid go (linear, 0 outside).
 void launch_resample_fields(const GridDev& gi, const GridDev& go, int np, const float* const* in,

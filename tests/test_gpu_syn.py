@@ -57,8 +57,11 @@ def test_syn_2d_blobs_full_schedule():
     res = run_pair(stage, pair)
     ref = run_oracle(stage, pair)
     print("2d parity (energy rel, fwd vox, inv vox):", compare(res, ref, pair.spacing))
+    # inner inversion step counts: the 0.1 / 0.001 stop thresholds are compared in
+    # float on the GPU and in double in the oracle, so a count may differ by one
     kin = np.array([r.inv_iters for r in res.trace])
-    np.testing.assert_array_equal(kin, ref.inv_iters)
+    diff = np.abs(kin - ref.inv_iters)
+    assert diff.max() <= 1 and (diff > 0).mean() < 0.05
 
 
 def test_syn_3d_phantom_small():
