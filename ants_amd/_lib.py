@@ -61,6 +61,8 @@ class SynStatsC(C.Structure):
         ("inv_steps", C.c_int64 * MAX_LEVELS),
         ("voxel_iterations", C.c_double),
         ("algorithmic_bytes", C.c_double),
+        ("graph_launches", C.c_int64),
+        ("eager_iterations", C.c_int64),
     ]
 
 

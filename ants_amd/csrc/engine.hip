@@ -16,7 +16,9 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <cstdlib>
 #include <limits>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -343,15 +345,17 @@ struct Profiler {
       if (r.cls == ANTS_KC_INV) {
         bytes = 0.0;
         const int qs[2][2] = {{0, 2}, {1, 3}};
-        for (int k = 0; k < 2; ++k) {
-          const RedSlot* S = st->inv[qs[r.batch][k]];
-          const bool upd = r.step >= 1 && S[r.step - 1].computed;
-          const bool nrm = S[r.step].computed != 0;
-          if (upd)
-            bytes += 3.0 * D * 4.0 * N;
-          else if (nrm)
-            bytes += 2.0 * D * 4.0 * N;
-        }
+        const int j0 = r.step < 0 ? 0 : r.step, j1 = r.step < 0 ? kInvMaxIter : r.step;
+        for (int j = j0; j <= j1; ++j)
+          for (int k = 0; k < 2; ++k) {
+            const RedSlot* S = st->inv[qs[r.batch][k]];
+            const bool upd = j >= 1 && S[j - 1].computed;
+            const bool nrm = S[j].computed != 0;
+            if (upd)
+              bytes += 3.0 * D * 4.0 * N;
+            else if (nrm)
+              bytes += 2.0 * D * 4.0 * N;
+          }
         if (bytes == 0.0) continue;
       }
       float ms = 0.f;
@@ -371,6 +375,25 @@ struct Profiler {
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// A captured iteration: the graph plus the buffer rotation it leaves behind
+// and the profiler records (events) it contains.
+struct GraphKey {
+  int level;
+  const void* p[8];
+  unsigned int prof_mask;
+  bool operator<(const GraphKey& o) const {
+    if (level != o.level) return level < o.level;
+    if (prof_mask != o.prof_mask) return prof_mask < o.prof_mask;
+    return std::lexicographical_compare(p, p + 8, o.p, o.p + 8);
+  }
+};
+struct GraphEntry {
+  hipGraphExec_t exec = nullptr;
+  float* nst[4];
+  float* nwk[4];
+  std::vector<Profiler::Rec> recs;
+};
+
 struct ants_syn_handle {
   ants_syn_params P{};
   hipStream_t stream = nullptr;
@@ -384,8 +407,18 @@ struct ants_syn_handle {
   bool have_state = false;
   float* state[4] = {nullptr, nullptr, nullptr, nullptr};  // f2m fwd, f2m inv, m2m fwd, m2m inv
   Profiler prof;
+  std::map<GraphKey, GraphEntry> graphs;
+  bool use_graphs = true;
+  std::vector<const void*> graph_signature;  // buffers + geometry the cached graphs were built for
+
+  void clear_graphs() {
+    for (auto& kv : graphs)
+      if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+    graphs.clear();
+  }
 
   ~ants_syn_handle() {
+    clear_graphs();
     DevBuf* all[] = {&F, &M, &Fs, &Ms, &tmp, &Fw, &Mw, &stats, &partials_cc, &partials_inv, &minmax};
     for (DevBuf* b : all) b->release();
     for (auto& b : fld) b.release();
@@ -540,6 +573,30 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
 
   hipStream_t s = stream;
   IterStats* dstats = static_cast<IterStats*>(stats.p);
+  {
+    // cached iteration graphs bake in buffer addresses and level geometry
+    std::vector<const void*> sig = {F.p, M.p, Fs.p, Ms.p, Fw.p, Mw.p, stats.p, partials_cc.p, partials_inv.p,
+                                    host_stats, static_cast<const void*>(s)};
+    for (auto& b : fld) sig.push_back(b.p);
+    for (int k = 0; k < 3; ++k) {
+      sig.push_back(reinterpret_cast<const void*>(static_cast<intptr_t>(gF.n[k])));
+      sig.push_back(reinterpret_cast<const void*>(static_cast<intptr_t>(gM.n[k])));
+    }
+    for (int k = 0; k < 3; ++k) {
+      double v[4] = {gF.sp[k], gF.org[k], gM.sp[k], gM.org[k]};
+      for (double x : v) {
+        intptr_t bits = 0;
+        std::memcpy(&bits, &x, sizeof(x) < sizeof(bits) ? sizeof(x) : sizeof(bits));
+        sig.push_back(reinterpret_cast<const void*>(bits));
+      }
+    }
+    if (sig != graph_signature) {
+      clear_graphs();
+      graph_signature = sig;
+    }
+    const char* env = std::getenv("ANTS_SYN_GRAPHS");
+    use_graphs = !(env && env[0] == '0');
+  }
   std::memset(&run_stats, 0, sizeof(run_stats));
   std::memset(&prof.acc, 0, sizeof(prof.acc));
   prof.s = s;
@@ -591,35 +648,41 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
     double gsz = 1.0;
     for (int d = 0; d < D; ++d) gsz *= std::min(2, s_max);
     const double e_i = 4.0, e_f = 4.0;
-    while (iter++ < P.iterations[l] && !converged) {
+    // One SyN iteration as a fixed launch sequence over buffer roles derived
+    // from the rotation (cst, cwk); it reports the rotation after the
+    // iteration in (nst, nwk).  Host work here is only pointer bookkeeping,
+    // so the sequence can be captured once into a hipGraph and replayed.
+    auto enqueue = [&](float* const* cst, float* const* cwk, float** nst, float** nwk) -> int {
+      float* st_[4] = {cst[0], cst[1], cst[2], cst[3]};
+      float* wk_[4] = {cwk[0], cwk[1], cwk[2], cwk[3]};
       launch_fill_u32(reinterpret_cast<uint32_t*>(dstats), 0u, sizeof(IterStats) / 4, s);
       // 1. warp both smoothed images through the inverse middle fields
       const double Nd = static_cast<double>(N);
       prof.begin(ANTS_KC_WARP);
-      launch_warp_pair(gd, gFd, Fs.f(), st[1], Fw.f(), gMd, Ms.f(), st[3], Mw.f(), s);
+      launch_warp_pair(gd, gFd, Fs.f(), st_[1], Fw.f(), gMd, Ms.f(), st_[3], Mw.f(), s);
       prof.end(ANTS_KC_WARP, Nd * (2.0 * D * e_f + 2.0 * gsz * e_i + 2.0 * e_i));
       // 2. CC value and both derivatives (g_f2m -> wk[0], g_m2m -> wk[1])
       prof.begin(ANTS_KC_CC);
-      launch_cc(gd, Fw.f(), Mw.f(), P.cc_radius, eps, wk[0], wk[1], static_cast<Partial*>(partials_cc.p),
+      launch_cc(gd, Fw.f(), Mw.f(), P.cc_radius, eps, wk_[0], wk_[1], static_cast<Partial*>(partials_cc.p),
                 &dstats->energy, s);
       prof.end(ANTS_KC_CC, Nd * (2.0 * e_i + 2.0 * D * e_f));
       HIP_TRY(hipGetLastError());
       // 3. update-field smoothing + max norms (result in wk[0], wk[1]; scratch wk[2], wk[3])
       {
-        float* in[2] = {wk[0], wk[1]};
-        float* work[2] = {wk[2], wk[3]};
-        float* out[2] = {D % 2 == 1 ? wk[2] : wk[0], D % 2 == 1 ? wk[3] : wk[1]};
+        float* in[2] = {wk_[0], wk_[1]};
+        float* work[2] = {wk_[2], wk_[3]};
+        float* out[2] = {D % 2 == 1 ? wk_[2] : wk_[0], D % 2 == 1 ? wk_[3] : wk_[1]};
         ANTS_TRY(smooth_fields_on(s, g, 2, in, work, out, P.update_variance, dstats->maxnorm, &prof));
-        if (out[0] != wk[0]) {
-          std::swap(wk[0], wk[2]);
-          std::swap(wk[1], wk[3]);
+        if (out[0] != wk_[0]) {
+          std::swap(wk_[0], wk_[2]);
+          std::swap(wk_[1], wk_[3]);
         }
       }
       // 4. scaled composition onto the forward middle fields: t -> wk[2], wk[3]
       {
-        const float* disp[2] = {wk[0], wk[1]};
-        const float* warp[2] = {st[0], st[2]};
-        float* out[2] = {wk[2], wk[3]};
+        const float* disp[2] = {wk_[0], wk_[1]};
+        const float* warp[2] = {st_[0], st_[2]};
+        float* out[2] = {wk_[2], wk_[3]};
         prof.begin(ANTS_KC_COMPOSE);
         launch_compose(gd, 2, disp, warp, out, dstats->maxnorm, static_cast<float>(P.learning_rate), s);
         prof.end(ANTS_KC_COMPOSE, 2.0 * Nd * 3.0 * D * e_f);
@@ -627,42 +690,80 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
       }
       // 5. total-field smoothing (a copy when the variance is 0)
       if (P.total_variance > 0.0) {
-        float* in[2] = {wk[2], wk[3]};
-        float* work[2] = {wk[0], wk[1]};
-        float* out[2] = {D % 2 == 1 ? wk[0] : wk[2], D % 2 == 1 ? wk[1] : wk[3]};
+        float* in[2] = {wk_[2], wk_[3]};
+        float* work[2] = {wk_[0], wk_[1]};
+        float* out[2] = {D % 2 == 1 ? wk_[0] : wk_[2], D % 2 == 1 ? wk_[1] : wk_[3]};
         ANTS_TRY(smooth_fields_on(s, g, 2, in, work, out, P.total_variance, dstats->maxnorm + 2, &prof));
-        if (out[0] != wk[2]) {
-          std::swap(wk[0], wk[2]);
-          std::swap(wk[1], wk[3]);
+        if (out[0] != wk_[2]) {
+          std::swap(wk_[0], wk_[2]);
+          std::swap(wk_[1], wk_[3]);
         }
       }
       // 6. inversions: inverse middle fields from the previous inverses (in place),
       //    then forward fields as inverses of the new inverses, started from t (in place)
-      {
-        const float* field[2] = {wk[2], wk[3]};
-        float* inv[2] = {st[1], st[3]};
-        RedSlot* slots[2] = {dstats->inv[0], dstats->inv[2]};
+      for (int batch = 0; batch < 2; ++batch) {
+        const float* field[2] = {batch == 0 ? wk_[2] : st_[1], batch == 0 ? wk_[3] : st_[3]};
+        float* inv[2] = {batch == 0 ? st_[1] : wk_[2], batch == 0 ? st_[3] : wk_[3]};
+        RedSlot* slots[2] = {dstats->inv[batch], dstats->inv[batch + 2]};
         for (int j = 0; j <= kInvMaxIter; ++j) {
           prof.begin(ANTS_KC_INV);
           launch_inv_step(gd, 2, field, inv, slots, static_cast<Partial*>(partials_inv.p), j, s);
-          prof.end(ANTS_KC_INV, 0.0, 0, j);
-        }
-      }
-      {
-        const float* field[2] = {st[1], st[3]};
-        float* inv[2] = {wk[2], wk[3]};
-        RedSlot* slots[2] = {dstats->inv[1], dstats->inv[3]};
-        for (int j = 0; j <= kInvMaxIter; ++j) {
-          prof.begin(ANTS_KC_INV);
-          launch_inv_step(gd, 2, field, inv, slots, static_cast<Partial*>(partials_inv.p), j, s);
-          prof.end(ANTS_KC_INV, 0.0, 1, j);
+          prof.end(ANTS_KC_INV, 0.0, batch, j);
         }
       }
       HIP_TRY(hipGetLastError());
-      std::swap(st[0], wk[2]);
-      std::swap(st[2], wk[3]);
-      // 7. energy, convergence, observer
+      std::swap(st_[0], wk_[2]);
+      std::swap(st_[2], wk_[3]);
       HIP_TRY(hipMemcpyAsync(host_stats, dstats, sizeof(IterStats), hipMemcpyDeviceToHost, s));
+      for (int k = 0; k < 4; ++k) {
+        nst[k] = st_[k];
+        nwk[k] = wk_[k];
+      }
+      return ANTS_OK;
+    };
+
+    while (iter++ < P.iterations[l] && !converged) {
+      float* nst[4];
+      float* nwk[4];
+      GraphKey key{l, {st[0], st[1], st[2], st[3], wk[0], wk[1], wk[2], wk[3]}, prof.mask};
+      auto it = use_graphs ? graphs.find(key) : graphs.end();
+      if (use_graphs && it == graphs.end()) {
+        // capture this rotation's sequence once
+        GraphEntry ge{};
+        const size_t rec0 = prof.recs.size();
+        int rc = ANTS_OK;
+        if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+          rc = enqueue(st, wk, ge.nst, ge.nwk);
+          hipGraph_t graph = nullptr;
+          const hipError_t e = hipStreamEndCapture(s, &graph);
+          if (rc == ANTS_OK && e == hipSuccess && graph &&
+              hipGraphInstantiate(&ge.exec, graph, nullptr, nullptr, 0) == hipSuccess) {
+            ge.recs.assign(prof.recs.begin() + rec0, prof.recs.end());
+            it = graphs.emplace(key, std::move(ge)).first;
+          }
+          if (graph) (void)hipGraphDestroy(graph);
+        }
+        prof.recs.resize(rec0);
+        (void)hipGetLastError();
+        if (it == graphs.end()) use_graphs = false;  // capture not possible here: launch eagerly
+      }
+      if (use_graphs && it != graphs.end()) {
+        HIP_TRY(hipGraphLaunch(it->second.exec, s));
+        run_stats.graph_launches += 1;
+        prof.recs.insert(prof.recs.end(), it->second.recs.begin(), it->second.recs.end());
+        for (int k = 0; k < 4; ++k) {
+          nst[k] = it->second.nst[k];
+          nwk[k] = it->second.nwk[k];
+        }
+      } else {
+        ANTS_TRY(enqueue(st, wk, nst, nwk));
+        run_stats.eager_iterations += 1;
+      }
+      for (int k = 0; k < 4; ++k) {
+        st[k] = nst[k];
+        wk[k] = nwk[k];
+      }
+      // 7. energy, convergence, observer
       HIP_TRY(hipStreamSynchronize(s));
       prof.collect(host_stats, N, D);
       const double value = -host_stats->energy.sum / static_cast<double>(N);

@@ -109,7 +109,8 @@ def main():
         times.append(dt)
         work += st["voxel_iterations"]
         traces.append(tr)
-        log(f"step {k}: {dt * 1e3:.1f} ms, iterations {st['iterations']}, inner inversion steps {st['inv_steps']}")
+        log(f"step {k}: {dt * 1e3:.1f} ms, iterations {st['iterations']}, inner inversion steps {st['inv_steps']}, "
+            f"graph/eager iterations {st['graph_launches']}/{st['eager_iterations']}")
     total = sum(times)
     value = work / total
     st = reg.stats()

@@ -141,6 +141,8 @@ typedef struct ants_syn_stats {
   int64_t inv_steps[ANTS_MAX_LEVELS];   /* sum over iterations of the 4 inner-step counts */
   double voxel_iterations;              /* sum_l N_l * iterations_l */
   double algorithmic_bytes;             /* sum over iterations of N_l * B_vi */
+  int64_t graph_launches;               /* iterations replayed from a captured hipGraph */
+  int64_t eager_iterations;             /* iterations launched kernel by kernel */
 } ants_syn_stats;
 int ants_syn_get_stats(const ants_syn_handle* h, ants_syn_stats* out);
 
