@@ -170,6 +170,7 @@ GridDev to_dev(const HostGrid& h) {
     g.dsp[k] = h.sp[k];
     g.dorg[k] = h.org[k];
   }
+  set_divisors(g);
   return g;
 }
 
@@ -399,7 +400,7 @@ struct ants_syn_handle {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   DevBuf F, M, Fs, Ms, tmp, Fw, Mw;
-  DevBuf fld[8];
+  DevBuf fld[10];  // 4 middle fields, 4 work fields, 2 inversion residuals
   DevBuf stats, partials_cc, partials_inv, minmax;
   IterStats* host_stats = nullptr;  // pinned
   ants_syn_stats run_stats{};
@@ -502,6 +503,29 @@ int smooth_fields_on(hipStream_t s, const HostGrid& g, int np, float* const* in,
     if (prof) prof->end(ANTS_KC_SMOOTH, np * pass_bytes);
     HIP_TRY(hipGetLastError());
     return ANTS_OK;
+  }
+  // one fused pass when the kernel radius and dimension allow (taps per axis
+  // differ only through MaxWidth = size[d], checked below)
+  if (D == 3) {
+    Taps tx, ty, tz;
+    ANTS_TRY(gaussian_taps(variance, 0.001, g.n[0], &tx));
+    ANTS_TRY(gaussian_taps(variance, 0.001, g.n[1], &ty));
+    ANTS_TRY(gaussian_taps(variance, 0.001, g.n[2], &tz));
+    const bool same = tx.r == ty.r && ty.r == tz.r &&
+                      std::memcmp(tx.c, ty.c, sizeof(float) * (tx.r + 1)) == 0 &&
+                      std::memcmp(ty.c, tz.c, sizeof(float) * (ty.r + 1)) == 0;
+    const float w1f = variance < 0.5 ? static_cast<float>(1.0 - variance / 0.5) : 1.f;
+    if (same && out[0] != in[0]) {
+      const float* cin[4];
+      for (int k = 0; k < np; ++k) cin[k] = in[k];
+      if (prof) prof->begin(ANTS_KC_SMOOTH);
+      const bool ok = launch_smooth3d(gd, np, cin, cin, out, tx, w1f, 1.f - w1f, true, maxbits, s);
+      if (prof) prof->end(ANTS_KC_SMOOTH, np * pass_bytes);
+      if (ok) {
+        HIP_TRY(hipGetLastError());
+        return ANTS_OK;
+      }
+    }
   }
   // axes 0..D-2 ping-pong between work and in; the last axis lands in out
   const float* cur[4];
@@ -617,6 +641,7 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
   // field slots: state (f2m_f, f2m_i, m2m_f, m2m_i) and work (4)
   float* st[4] = {fld[0].f(), fld[1].f(), fld[2].f(), fld[3].f()};
   float* wk[4] = {fld[4].f(), fld[5].f(), fld[6].f(), fld[7].f()};
+  float* resid[2] = {fld[8].f(), fld[9].f()};
   const GridDev gFd = to_dev(gF), gMd = to_dev(gM);
   const float eps = std::numeric_limits<float>::epsilon();
   HostGrid prev{};
@@ -707,7 +732,7 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
         RedSlot* slots[2] = {dstats->inv[batch], dstats->inv[batch + 2]};
         for (int j = 0; j <= kInvMaxIter; ++j) {
           prof.begin(ANTS_KC_INV);
-          launch_inv_step(gd, 2, field, inv, slots, static_cast<Partial*>(partials_inv.p), j, s);
+          launch_inv_step(gd, 2, field, inv, resid, slots, static_cast<Partial*>(partials_inv.p), j, s);
           prof.end(ANTS_KC_INV, 0.0, batch, j);
         }
       }
@@ -1117,11 +1142,14 @@ int ants_op_invert(const ants_image_desc* gdsc, const float* d_field, const floa
     HIP_TRY(hipMemcpyAsync(d_inv, d_init, b, hipMemcpyDeviceToDevice, s));
   else
     HIP_TRY(hipMemsetAsync(d_inv, 0, b, s));
+  DevBuf rbuf;
+  ANTS_TRY(rbuf.ensure(b));
   const float* field[1] = {d_field};
   float* inv[1] = {d_inv};
+  float* res[1] = {rbuf.f()};
   RedSlot* slots[1] = {static_cast<RedSlot*>(sc.stats.p)};
   for (int j = 0; j <= kInvMaxIter; ++j)
-    launch_inv_step(gd, 1, field, inv, slots, static_cast<Partial*>(sc.partials.p), j, s);
+    launch_inv_step(gd, 1, field, inv, res, slots, static_cast<Partial*>(sc.partials.p), j, s);
   HIP_TRY(hipGetLastError());
   RedSlot r[kInvSlots];
   HIP_TRY(hipMemcpyAsync(r, sc.stats.p, sizeof(r), hipMemcpyDeviceToHost, s));

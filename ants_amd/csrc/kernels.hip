@@ -30,11 +30,19 @@ __device__ __forceinline__ int64_t lin(const GridDev& g, int x, int y, int z) {
   return static_cast<int64_t>(x) + static_cast<int64_t>(g.n[0]) * (y + static_cast<int64_t>(g.n[1]) * z);
 }
 
+__device__ __forceinline__ uint32_t fast_div(uint32_t n, uint32_t m, uint32_t s1, uint32_t s2) {
+  const uint32_t t = __umulhi(m, n);
+  return (t + ((n - t) >> s1)) >> s2;
+}
+
+// Voxel index -> (x, y, z); grids hold fewer than 2^31 voxels.
 __device__ __forceinline__ void unlin(const GridDev& g, int64_t i, int& x, int& y, int& z) {
-  x = static_cast<int>(i % g.n[0]);
-  int64_t r = i / g.n[0];
-  y = static_cast<int>(r % g.n[1]);
-  z = static_cast<int>(r / g.n[1]);
+  const uint32_t n = static_cast<uint32_t>(i);
+  const uint32_t r = fast_div(n, g.div_m[0], g.div_s1[0], g.div_s2[0]);
+  x = static_cast<int>(n - r * static_cast<uint32_t>(g.n[0]));
+  const uint32_t q = fast_div(r, g.div_m[1], g.div_s1[1], g.div_s2[1]);
+  y = static_cast<int>(r - q * static_cast<uint32_t>(g.n[1]));
+  z = static_cast<int>(q);
 }
 
 __device__ __forceinline__ bool on_boundary(const GridDev& g, int x, int y, int z) {
@@ -326,6 +334,123 @@ __global__ void __launch_bounds__(kBlock) k_gauss_field_final(GridDev g, PtrPack
   double dummy = 0.0;
   block_sum_max<kBlock>(dummy, nrm);
   if (threadIdx.x == 0) atomicMax(&maxbits[k], __float_as_uint(nrm));
+}
+
+// Fused 3-D separable Gaussian of 3-vector fields (SyN field smoothing).
+// Block = TX x TY output columns over a z-chunk of one field (blockIdx.z =
+// field * nzc + chunk).  Each plane (edges replicated, ZeroFluxNeumann) is
+// staged once in LDS, convolved along x and y from LDS, and pushed into a
+// (2R+1)-deep register ring that forms the z pass.  The epilogue zeroes the
+// boundary, blends with the input when w2 != 0, and takes the max norm.
+// The next plane is prefetched into registers while the current one is used.
+template <int R, int TX, int TY>
+__global__ void __launch_bounds__(TX* TY) k_smooth3d(GridDev g, PtrPack4 in, PtrPack4 orig, MutPack4 out, Taps t,
+                                                     int zchunk, int nzc, float w1, float w2, int boundary,
+                                                     unsigned int* maxbits) {
+  constexpr int W = 2 * R + 1;
+  constexpr int LX = TX + 2 * R, LY = TY + 2 * R;
+  constexpr int NT = TX * TY;
+  constexpr int NE = 3 * LX * LY;
+  constexpr int KP = (NE + NT - 1) / NT;
+  __shared__ float tile[3][LY][LX];
+  __shared__ float xs[3][LY][TX];
+  const int f = blockIdx.z / nzc, zb = blockIdx.z - f * nzc;
+  const float* __restrict__ src = in.p[f];
+  const float* __restrict__ org = orig.p[f];
+  float* __restrict__ dst = out.p[f];
+  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+  const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+  const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, z0 = zb * zchunk;
+  const int z1 = min(z0 + zchunk, nz);
+  const int x = x0 + tx, y = y0 + ty;
+  const bool active = x < nx && y < ny;
+  const int64_t plane = static_cast<int64_t>(nx) * ny;
+  float c[R + 1];
+#pragma unroll
+  for (int b = 0; b <= R; ++b) c[b] = b <= t.r ? t.c[b] : 0.f;
+  float ring[3][W];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < W; ++b) ring[a][b] = 0.f;
+  float pv[KP];
+  auto fetch = [&](int zq) {
+    const int zs = zq < 0 ? 0 : (zq >= nz ? nz - 1 : zq);
+#pragma unroll
+    for (int u = 0; u < KP; ++u) {
+      const int e = threadIdx.x + u * NT;
+      float v = 0.f;
+      if (e < NE) {
+        const int ly = e / (3 * LX), r = e - ly * (3 * LX);
+        const int lx = r / 3, cc = r - lx * 3;
+        int gx = x0 - R + lx, gy = y0 - R + ly;
+        gx = gx < 0 ? 0 : (gx >= nx ? nx - 1 : gx);
+        gy = gy < 0 ? 0 : (gy >= ny ? ny - 1 : gy);
+        v = src[(static_cast<int64_t>(zs) * plane + static_cast<int64_t>(gy) * nx + gx) * 3 + cc];
+      }
+      pv[u] = v;
+    }
+  };
+  fetch(z0 - R);
+  float nrm = 0.f;
+  for (int zz = z0 - R; zz < z1 + R; ++zz) {
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < KP; ++u) {
+      const int e = threadIdx.x + u * NT;
+      if (e < NE) {
+        const int ly = e / (3 * LX), r = e - ly * (3 * LX);
+        const int lx = r / 3, cc = r - lx * 3;
+        tile[cc][ly][lx] = pv[u];
+      }
+    }
+    __syncthreads();
+    if (zz + 1 < z1 + R) fetch(zz + 1);
+    for (int ly = ty; ly < LY; ly += TY) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        float acc = c[0] * tile[a][ly][tx + R];
+#pragma unroll
+        for (int b = 1; b <= R; ++b) acc += c[b] * (tile[a][ly][tx + R - b] + tile[a][ly][tx + R + b]);
+        xs[a][ly][tx] = acc;
+      }
+    }
+    __syncthreads();
+    float p[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      float acc = c[0] * xs[a][ty + R][tx];
+#pragma unroll
+      for (int b = 1; b <= R; ++b) acc += c[b] * (xs[a][ty + R - b][tx] + xs[a][ty + R + b][tx]);
+      p[a] = acc;
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+#pragma unroll
+      for (int b = 0; b < W - 1; ++b) ring[a][b] = ring[a][b + 1];
+      ring[a][W - 1] = p[a];
+    }
+    const int zc = zz - R;
+    if (active && zc >= z0 && zc < z1) {
+      const int64_t i = static_cast<int64_t>(zc) * plane + static_cast<int64_t>(y) * nx + x;
+      const bool bnd = boundary && (x == 0 || x == nx - 1 || y == 0 || y == ny - 1 || zc == 0 || zc == nz - 1);
+      float s2 = 0.f;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        float acc = c[0] * ring[a][R];
+#pragma unroll
+        for (int b = 1; b <= R; ++b) acc += c[b] * (ring[a][R - b] + ring[a][R + b]);
+        const float v = bnd ? 0.f : (w2 != 0.f ? acc * w1 + org[i * 3 + a] * w2 : acc * w1);
+        dst[i * 3 + a] = v;
+        const float q = v / g.sp[a];
+        s2 += q * q;
+      }
+      nrm = fmaxf(nrm, sqrtf(s2));
+    }
+  }
+  double dummy = 0.0;
+  block_sum_max<NT>(dummy, nrm);
+  if (threadIdx.x == 0) atomicMax(&maxbits[f], __float_as_uint(nrm));
 }
 
 // ---------------------------------------------------------------------------
@@ -690,9 +815,31 @@ __device__ __forceinline__ float inv_residual(const GridDev& g, const float* __r
   return sqrtf(s2);
 }
 
+// The residual e_j of each norm pass is stored (res), so step j+1 applies its
+// update from the stored e_j instead of gathering the field again: one
+// gather per voxel and step.  Norms are recomputed from the stored residual
+// with the same arithmetic, so decisions and results are unchanged.
 template <int D>
-__global__ void __launch_bounds__(kBlock) k_inv_step(GridDev g, PtrPack4 field, MutPack4 inv, SlotPack4 slots,
-                                                     Partial* partials, int64_t part_stride, int j) {
+__device__ __forceinline__ float resid_norm(const GridDev& g, const float e[3]) {
+  float s2 = 0.f;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const float q = e[d] / g.sp[d];
+    s2 += q * q;
+  }
+  return sqrtf(s2);
+}
+
+// Inversion steps run 1024-thread blocks over a small grid (kInvBlocks per
+// problem): steps that have converged cost one cheap dispatch, active steps
+// still fill the device, and the closing reduction reads one partial per thread.
+constexpr int kInvThreads = 1024;
+constexpr int kInvBlocks = 512;
+
+template <int D>
+__global__ void __launch_bounds__(kInvThreads) k_inv_step(GridDev g, PtrPack4 field, MutPack4 inv, MutPack4 res,
+                                                          SlotPack4 slots, Partial* partials, int64_t part_stride,
+                                                          int j) {
   const int k = blockIdx.y;
   RedSlot* st = slots.p[k];
   bool do_update = false, do_norm = false;
@@ -712,6 +859,7 @@ __global__ void __launch_bounds__(kBlock) k_inv_step(GridDev g, PtrPack4 field, 
   if (!do_update && !do_norm) return;
   const float* __restrict__ fld = field.p[k];
   float* __restrict__ iv = inv.p[k];
+  float* __restrict__ rs = res.p[k];
   const int64_t N = g.count();
   double nsum = 0.0;
   float nmax = 0.f;
@@ -719,11 +867,13 @@ __global__ void __launch_bounds__(kBlock) k_inv_step(GridDev g, PtrPack4 field, 
        i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
     int x, y, z;
     unlin(g, i, x, y, z);
-    float cur[3] = {0.f, 0.f, 0.f}, e[3];
+    float cur[3] = {0.f, 0.f, 0.f}, e[3] = {0.f, 0.f, 0.f};
 #pragma unroll
     for (int d = 0; d < D; ++d) cur[d] = iv[i * D + d];
     if (do_update) {
-      const float nrm = inv_residual<D>(g, fld, x, y, z, cur, e);
+#pragma unroll
+      for (int d = 0; d < D; ++d) e[d] = rs[i * D + d];
+      const float nrm = resid_norm<D>(g, e);
       const float lim = eps * maxprev;
       float sc = 1.f;
       if (nrm > lim) sc = lim / nrm;
@@ -736,12 +886,14 @@ __global__ void __launch_bounds__(kBlock) k_inv_step(GridDev g, PtrPack4 field, 
     }
     if (do_norm) {
       const float nrm = inv_residual<D>(g, fld, x, y, z, cur, e);
+#pragma unroll
+      for (int d = 0; d < D; ++d) rs[i * D + d] = e[d];
       nsum += nrm;
       nmax = fmaxf(nmax, nrm);
     }
   }
   if (do_norm)
-    grid_reduce<kBlock>(nsum, nmax, partials + k * part_stride, &st[j], gridDim.x, blockIdx.x);
+    grid_reduce<kInvThreads>(nsum, nmax, partials + k * part_stride, &st[j], gridDim.x, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -857,6 +1009,20 @@ void launch_gauss_field_final(const GridDev& g, int np, const float* const* in, 
                        mpack(out, np), t, axis, w1, w2, static_cast<int>(boundary), maxbits);
 }
 
+bool launch_smooth3d(const GridDev& g, int np, const float* const* in, const float* const* orig, float* const* out,
+                     const Taps& t, float w1, float w2, bool boundary, unsigned int* maxbits, hipStream_t s) {
+  constexpr int R = 6, TX = 64, TY = 8;
+  if (g.D != 3 || t.r != R || np > 4) return false;
+  const int64_t tiles = static_cast<int64_t>((g.n[0] + TX - 1) / TX) * ((g.n[1] + TY - 1) / TY) * np;
+  int64_t zc = (static_cast<int64_t>(g.n[2]) * tiles) / 1024;
+  zc = zc < 16 ? 16 : (zc > 64 ? 64 : zc);
+  const int nzc = static_cast<int>((g.n[2] + zc - 1) / zc);
+  const dim3 grid((g.n[0] + TX - 1) / TX, (g.n[1] + TY - 1) / TY, nzc * np);
+  hipLaunchKernelGGL((k_smooth3d<R, TX, TY>), grid, dim3(TX * TY), 0, s, g, pack(in, np), pack(orig, np),
+                     mpack(out, np), t, static_cast<int>(zc), nzc, w1, w2, static_cast<int>(boundary), maxbits);
+  return true;
+}
+
 void launch_warp_pair(const GridDev& gl, const GridDev& gF, const float* Fs, const float* uF, float* Fw,
                       const GridDev& gM, const float* Ms, const float* uM, float* Mw, hipStream_t s) {
   const dim3 grid(blocks_for(gl.count()));
@@ -922,20 +1088,25 @@ void launch_compose(const GridDev& g, int np, const float* const* disp, const fl
                        maxbits, rate);
 }
 
-int64_t inv_partials_needed(const GridDev& g) { return red_blocks(g.count()); }
+static unsigned int inv_blocks(int64_t n) {
+  const int64_t b = (n + kInvThreads - 1) / kInvThreads;
+  return static_cast<unsigned int>(b < 1 ? 1 : (b > kInvBlocks ? kInvBlocks : b));
+}
 
-void launch_inv_step(const GridDev& g, int np, const float* const* field, float* const* inv, RedSlot* const* slots,
-                     Partial* partials, int j, hipStream_t s) {
-  const dim3 grid(red_blocks(g.count()), np);
+int64_t inv_partials_needed(const GridDev& g) { return inv_blocks(g.count()); }
+
+void launch_inv_step(const GridDev& g, int np, const float* const* field, float* const* inv, float* const* res,
+                     RedSlot* const* slots, Partial* partials, int j, hipStream_t s) {
+  const dim3 grid(inv_blocks(g.count()), np);
   SlotPack4 sp{};
   for (int k = 0; k < np; ++k) sp.p[k] = slots[k];
   const int64_t stride = inv_partials_needed(g);
   if (g.D == 2)
-    hipLaunchKernelGGL(k_inv_step<2>, grid, dim3(kBlock), 0, s, g, pack(field, np), mpack(inv, np), sp, partials,
-                       stride, j);
+    hipLaunchKernelGGL(k_inv_step<2>, grid, dim3(kInvThreads), 0, s, g, pack(field, np), mpack(inv, np), mpack(res, np),
+                       sp, partials, stride, j);
   else
-    hipLaunchKernelGGL(k_inv_step<3>, grid, dim3(kBlock), 0, s, g, pack(field, np), mpack(inv, np), sp, partials,
-                       stride, j);
+    hipLaunchKernelGGL(k_inv_step<3>, grid, dim3(kInvThreads), 0, s, g, pack(field, np), mpack(inv, np), mpack(res, np),
+                       sp, partials, stride, j);
 }
 
 void launch_resample_fields(const GridDev& gi, const GridDev& go, int np, const float* const* in, float* const* out,

@@ -15,8 +15,26 @@ struct GridDev {
   float org[3];
   double dsp[3];   // exact geometry for double-precision coordinate paths
   double dorg[3];
+  // unsigned 32-bit division by n[0] and n[1] as multiply-high + shifts
+  // (Granlund & Montgomery, "Division by invariant integers using
+  // multiplication", 1994, fig. 4.1); set by set_divisors().
+  uint32_t div_m[2];
+  uint32_t div_s1[2];
+  uint32_t div_s2[2];
   __host__ __device__ int64_t count() const { return static_cast<int64_t>(n[0]) * n[1] * n[2]; }
 };
+
+// Fills GridDev::div_* for n[0] and n[1].
+inline void set_divisors(GridDev& g) {
+  for (int k = 0; k < 2; ++k) {
+    const uint64_t d = static_cast<uint64_t>(g.n[k] > 0 ? g.n[k] : 1);
+    uint32_t l = 0;
+    while ((1ull << l) < d) ++l;  // l = ceil(log2 d)
+    g.div_m[k] = static_cast<uint32_t>((((1ull << 32) * ((1ull << l) - d)) / d) + 1);
+    g.div_s1[k] = l < 1 ? l : 1;
+    g.div_s2[k] = l > 1 ? l - 1 : 0;
+  }
+}
 
 constexpr int kMaxTaps = 64;  // half-kernel length limit (radius <= 63)
 struct Taps {
@@ -73,6 +91,12 @@ void launch_gauss_field_final(const GridDev& g, int np, const float* const* in, 
                               float* const* out, const Taps& t, int axis, float w1, float w2, bool boundary,
                               unsigned int* maxbits, hipStream_t s);
 
+// The whole SyN field smoothing of np 3-D fields in one pass (all three axes,
+// boundary zero, blend, max norm).  Returns false when this grid or kernel
+// radius is not covered (then use launch_gauss_axis + launch_gauss_field_final).
+bool launch_smooth3d(const GridDev& g, int np, const float* const* in, const float* const* orig, float* const* out,
+                     const Taps& t, float w1, float w2, bool boundary, unsigned int* maxbits, hipStream_t s);
+
 // Fw = Fs(x + uF(x)), Mw = Ms(x + uM(x)) on level grid gl (Fs on gF, Ms on gM).
 void launch_warp_pair(const GridDev& gl, const GridDev& gF, const float* Fs, const float* uF, float* Fw,
                       const GridDev& gM, const float* Ms, const float* uM, float* Mw, hipStream_t s);
@@ -91,10 +115,11 @@ void launch_cc_direct(const GridDev& g, const float* F, const float* M, int radi
 void launch_compose(const GridDev& g, int np, const float* const* disp, const float* const* warp,
                     float* const* out, const unsigned int* maxbits, float rate, hipStream_t s);
 
-// One fused fixed-point inversion step j (0..20) for np problems (see kernels.hip).
+// One fused fixed-point inversion step j (0..20) for np problems (see
+// kernels.hip); res[k] is a field-sized buffer holding the last residual.
 int64_t inv_partials_needed(const GridDev& g);
-void launch_inv_step(const GridDev& g, int np, const float* const* field, float* const* inv, RedSlot* const* slots,
-                     Partial* partials, int j, hipStream_t s);
+void launch_inv_step(const GridDev& g, int np, const float* const* field, float* const* inv, float* const* res,
+                     RedSlot* const* slots, Partial* partials, int j, hipStream_t s);
 
 // Resample np fields from grid gi onto grid go (linear, 0 outside).
 void launch_resample_fields(const GridDev& gi, const GridDev& go, int np, const float* const* in,
