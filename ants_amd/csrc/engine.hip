@@ -233,8 +233,6 @@ int validate_params(const ants_syn_params* p) {
   if (p->convergence_window < 1) return fail(ANTS_ERR_INVALID, "convergence window must be >= 1");
   if (p->precision != ANTS_PRECISION_FP32 && p->precision != ANTS_PRECISION_BF16_PYRAMID)
     return fail(ANTS_ERR_INVALID, "unknown precision mode");
-  if (p->precision == ANTS_PRECISION_BF16_PYRAMID)
-    return fail(ANTS_ERR_UNSUPPORTED, "BF16 pyramid precision is not available in this build");
   return ANTS_OK;
 }
 
@@ -399,7 +397,7 @@ struct ants_syn_handle {
   ants_syn_params P{};
   hipStream_t stream = nullptr;
   bool own_stream = false;
-  DevBuf F, M, Fs, Ms, tmp, Fw, Mw;
+  DevBuf F, M, Fs, Ms, tmp, Fw, Mw, Fb, Mb;  // Fb, Mb: BF16 pyramid images
   DevBuf fld[10];  // 4 middle fields, 4 work fields, 2 inversion residuals
   DevBuf stats, partials_cc, partials_inv, minmax;
   IterStats* host_stats = nullptr;  // pinned
@@ -420,7 +418,7 @@ struct ants_syn_handle {
 
   ~ants_syn_handle() {
     clear_graphs();
-    DevBuf* all[] = {&F, &M, &Fs, &Ms, &tmp, &Fw, &Mw, &stats, &partials_cc, &partials_inv, &minmax};
+    DevBuf* all[] = {&F, &M, &Fs, &Ms, &tmp, &Fw, &Mw, &Fb, &Mb, &stats, &partials_cc, &partials_inv, &minmax};
     for (DevBuf* b : all) b->release();
     for (auto& b : fld) b.release();
     if (host_stats) (void)hipHostFree(host_stats);
@@ -582,6 +580,11 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
   ANTS_TRY(Fs.ensure(NF * sizeof(float)));
   ANTS_TRY(Ms.ensure(NM * sizeof(float)));
   ANTS_TRY(tmp.ensure(std::max(NF, NM) * sizeof(float)));
+  const bool bf16 = P.precision == ANTS_PRECISION_BF16_PYRAMID;
+  if (bf16) {
+    ANTS_TRY(Fb.ensure(NF * sizeof(uint16_t)));
+    ANTS_TRY(Mb.ensure(NM * sizeof(uint16_t)));
+  }
   ANTS_TRY(Fw.ensure(maxN * sizeof(float)));
   ANTS_TRY(Mw.ensure(maxN * sizeof(float)));
   for (auto& b : fld) ANTS_TRY(b.ensure(fbytes));
@@ -599,7 +602,7 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
   IterStats* dstats = static_cast<IterStats*>(stats.p);
   {
     // cached iteration graphs bake in buffer addresses and level geometry
-    std::vector<const void*> sig = {F.p, M.p, Fs.p, Ms.p, Fw.p, Mw.p, stats.p, partials_cc.p, partials_inv.p,
+    std::vector<const void*> sig = {F.p, M.p, Fs.p, Ms.p, Fw.p, Mw.p, Fb.p, Mb.p, stats.p, partials_cc.p, partials_inv.p,
                                     host_stats, static_cast<const void*>(s)};
     for (auto& b : fld) sig.push_back(b.p);
     for (int k = 0; k < 3; ++k) {
@@ -663,6 +666,11 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
     if (P.iterations[l] <= 0) continue;
     ANTS_TRY(smooth_image_on(s, gF, F.f(), Fs.f(), tmp.f(), P.sigma[l], P.sigma_in_mm != 0));
     ANTS_TRY(smooth_image_on(s, gM, M.f(), Ms.f(), tmp.f(), P.sigma[l], P.sigma_in_mm != 0));
+    if (bf16) {
+      launch_to_bf16(Fs.f(), static_cast<uint16_t*>(Fb.p), NF, s);
+      launch_to_bf16(Ms.f(), static_cast<uint16_t*>(Mb.p), NM, s);
+      HIP_TRY(hipGetLastError());
+    }
 
     ConvergenceMonitor mon(P.convergence_window);
     bool converged = false;
@@ -672,7 +680,7 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
     for (int d = 0; d < D; ++d) s_max = std::max(s_max, static_cast<int>(std::lround(g.sp[d] / gF.sp[d])));
     double gsz = 1.0;
     for (int d = 0; d < D; ++d) gsz *= std::min(2, s_max);
-    const double e_i = 4.0, e_f = 4.0;
+    const double e_i = bf16 ? 2.0 : 4.0, e_f = 4.0;
     // One SyN iteration as a fixed launch sequence over buffer roles derived
     // from the rotation (cst, cwk); it reports the rotation after the
     // iteration in (nst, nwk).  Host work here is only pointer bookkeeping,
@@ -684,7 +692,11 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
       // 1. warp both smoothed images through the inverse middle fields
       const double Nd = static_cast<double>(N);
       prof.begin(ANTS_KC_WARP);
-      launch_warp_pair(gd, gFd, Fs.f(), st_[1], Fw.f(), gMd, Ms.f(), st_[3], Mw.f(), s);
+      if (bf16)
+        launch_warp_pair_bf16(gd, gFd, static_cast<const uint16_t*>(Fb.p), st_[1], Fw.f(), gMd,
+                              static_cast<const uint16_t*>(Mb.p), st_[3], Mw.f(), s);
+      else
+        launch_warp_pair(gd, gFd, Fs.f(), st_[1], Fw.f(), gMd, Ms.f(), st_[3], Mw.f(), s);
       prof.end(ANTS_KC_WARP, Nd * (2.0 * D * e_f + 2.0 * gsz * e_i + 2.0 * e_i));
       // 2. CC value and both derivatives (g_f2m -> wk[0], g_m2m -> wk[1])
       prof.begin(ANTS_KC_CC);

@@ -77,6 +77,59 @@ __device__ __forceinline__ bool inside(const GridDev& g, const float c[3]) {
   return true;
 }
 
+// BF16 storage of smoothed pyramid images: round to nearest even on store,
+// exact widening on load.
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  const uint32_t u = __float_as_uint(f);
+  const uint32_t r = ((u >> 16) & 1u) + 0x7fffu;
+  return static_cast<uint16_t>((u + r) >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(static_cast<uint32_t>(b) << 16); }
+__device__ __forceinline__ float ld(const float* p, int64_t i) { return p[i]; }
+__device__ __forceinline__ float ld(const uint16_t* p, int64_t i) { return bf2f(p[i]); }
+
+// Trilinear sample of a scalar image of element type T (float or bf16) at an
+// inside point, neighbours clamped to the buffer.
+template <int D, class T>
+__device__ __forceinline__ float interp_img(const GridDev& g, const T* __restrict__ img, const float c[3]) {
+  int lo[3], hi[3];
+  float w[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (d < D) {
+      float fl = floorf(c[d]);
+      int b = static_cast<int>(fl);
+      float t = c[d] - fl;
+      if (b < 0) {
+        b = 0;
+        t = 0.f;
+      }
+      lo[d] = b;
+      hi[d] = b + 1 < g.n[d] ? b + 1 : b;
+      w[d] = t;
+    } else {
+      lo[d] = hi[d] = 0;
+      w[d] = 0.f;
+    }
+  }
+  const int64_t sy = g.n[0], sz = static_cast<int64_t>(g.n[0]) * g.n[1];
+  if (D == 2) {
+    const float v00 = ld(img, lo[0] + sy * lo[1]), v10 = ld(img, hi[0] + sy * lo[1]);
+    const float v01 = ld(img, lo[0] + sy * hi[1]), v11 = ld(img, hi[0] + sy * hi[1]);
+    const float a = v00 + (v10 - v00) * w[0];
+    const float b = v01 + (v11 - v01) * w[0];
+    return a + (b - a) * w[1];
+  }
+  const int64_t z0 = sz * lo[2], z1 = sz * hi[2], y0 = sy * lo[1], y1 = sy * hi[1];
+  const float a00 = ld(img, lo[0] + y0 + z0) + (ld(img, hi[0] + y0 + z0) - ld(img, lo[0] + y0 + z0)) * w[0];
+  const float a10 = ld(img, lo[0] + y1 + z0) + (ld(img, hi[0] + y1 + z0) - ld(img, lo[0] + y1 + z0)) * w[0];
+  const float a01 = ld(img, lo[0] + y0 + z1) + (ld(img, hi[0] + y0 + z1) - ld(img, lo[0] + y0 + z1)) * w[0];
+  const float a11 = ld(img, lo[0] + y1 + z1) + (ld(img, hi[0] + y1 + z1) - ld(img, lo[0] + y1 + z1)) * w[0];
+  const float b0 = a00 + (a10 - a00) * w[1];
+  const float b1 = a01 + (a11 - a01) * w[1];
+  return b0 + (b1 - b0) * w[2];
+}
+
 // Linear interpolation of NC interleaved components at an inside point:
 // neighbours clamped to the buffer (ITK Linear/VectorLinear interpolators).
 template <int D, int NC>
@@ -454,10 +507,10 @@ __global__ void __launch_bounds__(TX* TY) k_smooth3d(GridDev g, PtrPack4 in, Ptr
 }
 
 // ---------------------------------------------------------------------------
-template <int D>
-__global__ void __launch_bounds__(kBlock) k_warp_pair(GridDev gl, GridDev gF, const float* __restrict__ Fs,
+template <int D, class T>
+__global__ void __launch_bounds__(kBlock) k_warp_pair(GridDev gl, GridDev gF, const T* __restrict__ Fs,
                                                       const float* __restrict__ uF, float* __restrict__ Fw,
-                                                      GridDev gM, const float* __restrict__ Ms,
+                                                      GridDev gM, const T* __restrict__ Ms,
                                                       const float* __restrict__ uM, float* __restrict__ Mw) {
   const int64_t N = gl.count();
   const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
@@ -469,14 +522,21 @@ __global__ void __launch_bounds__(kBlock) k_warp_pair(GridDev gl, GridDev gF, co
   for (int d = 0; d < D; ++d) u[d] = uF[i * D + d];
   cidx_of<D>(gl, x, y, z, u, gF, c);
   v = 0.f;
-  if (inside<D>(gF, c)) interp<D, 1>(gF, Fs, c, &v);
+  if (inside<D>(gF, c)) v = interp_img<D, T>(gF, Fs, c);
   Fw[i] = v;
 #pragma unroll
   for (int d = 0; d < D; ++d) u[d] = uM[i * D + d];
   cidx_of<D>(gl, x, y, z, u, gM, c);
   v = 0.f;
-  if (inside<D>(gM, c)) interp<D, 1>(gM, Ms, c, &v);
+  if (inside<D>(gM, c)) v = interp_img<D, T>(gM, Ms, c);
   Mw[i] = v;
+}
+
+__global__ void __launch_bounds__(kBlock) k_to_bf16(const float* __restrict__ in, uint16_t* __restrict__ out,
+                                                    int64_t n) {
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    out[i] = f2bf(in[i]);
 }
 
 template <int D>
@@ -518,21 +578,26 @@ struct CCOut {
 };
 __device__ __forceinline__ CCOut cc_voxel(double cnt, double sf, double sm, double sff, double smm, double sfm,
                                           float f, float m, float eps) {
-  const double mf = sf / cnt, mm = sm / cnt;
-  const double sFF = sff - mf * sf - mf * sf + cnt * mf * mf;
-  const double sMM = smm - mm * sm - mm * sm + cnt * mm * mm;
-  const double sFM = sfm - mm * sf - mf * sm + cnt * mm * mf;
+  // In double the algebraically equal short forms are as accurate as ITK's
+  // sff - mf*sf - mf*sf + cnt*mf*mf, and one reciprocal replaces the divisions
+  // by sFF*sMM, sMM and sFF (1/sMM = sFF/p, 1/sFF = sMM/p).
+  const double rc = 1.0 / cnt;
+  const double mf = sf * rc, mm = sm * rc;
+  const double sFF = sff - sf * mf;
+  const double sMM = smm - sm * mm;
+  const double sFM = sfm - sf * mm;
   const double fA = f - mf, mA = m - mm;
   CCOut o;
   o.cc = 1.0;
   o.a1 = 0.f;
   o.a2 = 0.f;
   if (sFF > eps && sMM > eps) {
-    const double p = sFF * sMM;
-    o.cc = sFM * sFM / p;
-    const double a = 2.0 * sFM / p;
-    o.a1 = static_cast<float>(a * (fA - sFM / sMM * mA));
-    o.a2 = static_cast<float>(a * (mA - sFM / sFF * fA));
+    const double rp = 1.0 / (sFF * sMM);
+    const double q = sFM * rp;
+    o.cc = sFM * q;
+    const double a = 2.0 * q;
+    o.a1 = static_cast<float>(a * (fA - q * sFF * mA));
+    o.a2 = static_cast<float>(a * (mA - q * sMM * fA));
   }
   return o;
 }
@@ -1027,9 +1092,22 @@ void launch_warp_pair(const GridDev& gl, const GridDev& gF, const float* Fs, con
                       const GridDev& gM, const float* Ms, const float* uM, float* Mw, hipStream_t s) {
   const dim3 grid(blocks_for(gl.count()));
   if (gl.D == 2)
-    hipLaunchKernelGGL(k_warp_pair<2>, grid, dim3(kBlock), 0, s, gl, gF, Fs, uF, Fw, gM, Ms, uM, Mw);
+    hipLaunchKernelGGL((k_warp_pair<2, float>), grid, dim3(kBlock), 0, s, gl, gF, Fs, uF, Fw, gM, Ms, uM, Mw);
   else
-    hipLaunchKernelGGL(k_warp_pair<3>, grid, dim3(kBlock), 0, s, gl, gF, Fs, uF, Fw, gM, Ms, uM, Mw);
+    hipLaunchKernelGGL((k_warp_pair<3, float>), grid, dim3(kBlock), 0, s, gl, gF, Fs, uF, Fw, gM, Ms, uM, Mw);
+}
+
+void launch_warp_pair_bf16(const GridDev& gl, const GridDev& gF, const uint16_t* Fs, const float* uF, float* Fw,
+                           const GridDev& gM, const uint16_t* Ms, const float* uM, float* Mw, hipStream_t s) {
+  const dim3 grid(blocks_for(gl.count()));
+  if (gl.D == 2)
+    hipLaunchKernelGGL((k_warp_pair<2, uint16_t>), grid, dim3(kBlock), 0, s, gl, gF, Fs, uF, Fw, gM, Ms, uM, Mw);
+  else
+    hipLaunchKernelGGL((k_warp_pair<3, uint16_t>), grid, dim3(kBlock), 0, s, gl, gF, Fs, uF, Fw, gM, Ms, uM, Mw);
+}
+
+void launch_to_bf16(const float* in, uint16_t* out, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_to_bf16, dim3(grid_stride_blocks(n)), dim3(kBlock), 0, s, in, out, n);
 }
 
 void launch_warp_image(const GridDev& gi, const float* img, const GridDev& go, const float* u, float* out,

@@ -100,6 +100,11 @@ bool launch_smooth3d(const GridDev& g, int np, const float* const* in, const flo
 // Fw = Fs(x + uF(x)), Mw = Ms(x + uM(x)) on level grid gl (Fs on gF, Ms on gM).
 void launch_warp_pair(const GridDev& gl, const GridDev& gF, const float* Fs, const float* uF, float* Fw,
                       const GridDev& gM, const float* Ms, const float* uM, float* Mw, hipStream_t s);
+// Same with the smoothed images stored as BF16.
+void launch_warp_pair_bf16(const GridDev& gl, const GridDev& gF, const uint16_t* Fs, const float* uF, float* Fw,
+                           const GridDev& gM, const uint16_t* Ms, const float* uM, float* Mw, hipStream_t s);
+// float -> BF16 (round to nearest even).
+void launch_to_bf16(const float* in, uint16_t* out, int64_t n, hipStream_t s);
 // Single image warp (u may be null).
 void launch_warp_image(const GridDev& gi, const float* img, const GridDev& go, const float* u, float* out,
                        hipStream_t s);
