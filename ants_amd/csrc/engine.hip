@@ -12,6 +12,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -377,15 +383,26 @@ struct Profiler {
 // A captured iteration: the graph plus the buffer rotation it leaves behind
 // and the profiler records (events) it contains.
 struct GraphKey {
+  uint64_t setup;  // hash of the image geometry and stage parameters
   int level;
   const void* p[8];
   unsigned int prof_mask;
   bool operator<(const GraphKey& o) const {
+    if (setup != o.setup) return setup < o.setup;
     if (level != o.level) return level < o.level;
     if (prof_mask != o.prof_mask) return prof_mask < o.prof_mask;
     return std::lexicographical_compare(p, p + 8, o.p, o.p + 8);
   }
 };
+
+uint64_t fnv1a(const void* data, size_t n, uint64_t h = 1469598103934665603ull) {
+  const unsigned char* b = static_cast<const unsigned char*>(data);
+  for (size_t i = 0; i < n; ++i) {
+    h ^= b[i];
+    h *= 1099511628211ull;
+  }
+  return h;
+}
 struct GraphEntry {
   hipGraphExec_t exec = nullptr;
   float* nst[4];
@@ -600,27 +617,33 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
 
   hipStream_t s = stream;
   IterStats* dstats = static_cast<IterStats*>(stats.p);
+  // cached iteration graphs bake in buffer addresses (cleared when any
+  // buffer moves) and are keyed by the geometry and stage parameters
+  uint64_t setup_hash = 0;
   {
-    // cached iteration graphs bake in buffer addresses and level geometry
-    std::vector<const void*> sig = {F.p, M.p, Fs.p, Ms.p, Fw.p, Mw.p, Fb.p, Mb.p, stats.p, partials_cc.p, partials_inv.p,
-                                    host_stats, static_cast<const void*>(s)};
+    std::vector<const void*> sig = {F.p, M.p, Fs.p, Ms.p, Fw.p, Mw.p, Fb.p, Mb.p, stats.p, partials_cc.p,
+                                    partials_inv.p, host_stats, static_cast<const void*>(s)};
     for (auto& b : fld) sig.push_back(b.p);
-    for (int k = 0; k < 3; ++k) {
-      sig.push_back(reinterpret_cast<const void*>(static_cast<intptr_t>(gF.n[k])));
-      sig.push_back(reinterpret_cast<const void*>(static_cast<intptr_t>(gM.n[k])));
-    }
-    for (int k = 0; k < 3; ++k) {
-      double v[4] = {gF.sp[k], gF.org[k], gM.sp[k], gM.org[k]};
-      for (double x : v) {
-        intptr_t bits = 0;
-        std::memcpy(&bits, &x, sizeof(x) < sizeof(bits) ? sizeof(x) : sizeof(bits));
-        sig.push_back(reinterpret_cast<const void*>(bits));
-      }
-    }
-    if (sig != graph_signature) {
+    if (sig != graph_signature || graphs.size() > 512) {
       clear_graphs();
       graph_signature = sig;
     }
+    // field by field: struct padding is not hashed
+    auto hg = [&](const HostGrid& g) {
+      setup_hash = fnv1a(&g.D, sizeof(g.D), setup_hash);
+      setup_hash = fnv1a(g.n, sizeof(g.n), setup_hash);
+      setup_hash = fnv1a(g.sp, sizeof(g.sp), setup_hash);
+      setup_hash = fnv1a(g.org, sizeof(g.org), setup_hash);
+    };
+    setup_hash = 1469598103934665603ull;
+    hg(gF);
+    hg(gM);
+    const int ints[] = {P.dim, P.n_levels, P.sigma_in_mm, P.cc_radius, P.precision};
+    setup_hash = fnv1a(ints, sizeof(ints), setup_hash);
+    setup_hash = fnv1a(P.shrink, sizeof(P.shrink), setup_hash);
+    setup_hash = fnv1a(P.sigma, sizeof(P.sigma), setup_hash);
+    const double dbl[] = {P.learning_rate, P.update_variance, P.total_variance};
+    setup_hash = fnv1a(dbl, sizeof(dbl), setup_hash);
     const char* env = std::getenv("ANTS_SYN_GRAPHS");
     use_graphs = !(env && env[0] == '0');
   }
@@ -762,7 +785,7 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
     while (iter++ < P.iterations[l] && !converged) {
       float* nst[4];
       float* nwk[4];
-      GraphKey key{l, {st[0], st[1], st[2], st[3], wk[0], wk[1], wk[2], wk[3]}, prof.mask};
+      GraphKey key{setup_hash, l, {st[0], st[1], st[2], st[3], wk[0], wk[1], wk[2], wk[3]}, prof.mask};
       auto it = use_graphs ? graphs.find(key) : graphs.end();
       if (use_graphs && it == graphs.end()) {
         // capture this rotation's sequence once
@@ -1216,6 +1239,258 @@ int ants_apply_transform_labels_nn(const ants_image_desc* go, const float* d_fie
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(s));
   return ANTS_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched / served registration.
+namespace {
+
+double now_s() {
+  using namespace std::chrono;
+  return duration_cast<duration<double>>(steady_clock::now().time_since_epoch()).count();
+}
+
+int64_t footprint_bytes(const ants_syn_params& P, const HostGrid& gF, const HostGrid& gM) {
+  int64_t maxN = 0;
+  for (int l = 0; l < P.n_levels; ++l) maxN = std::max(maxN, level_grid(gF, P.shrink[l]).count());
+  const int64_t NF = gF.count(), NM = gM.count(), D = P.dim;
+  int64_t b = 0;
+  b += (NF + NM) * 4 * 2;                // preprocessed + smoothed images
+  b += std::max(NF, NM) * 4;             // smoothing scratch
+  if (P.precision == ANTS_PRECISION_BF16_PYRAMID) b += (NF + NM) * 2;
+  b += 2 * maxN * 4;                     // warped images
+  b += 10 * maxN * D * 4;                // middle, work and residual fields
+  b += (NF + NM) * 4 + NM * 4;           // request inputs (images, labels)
+  b += 2 * maxN * D * 4 + maxN * 4;      // outputs (fields, labels)
+  return b;
+}
+
+struct BatchItem {
+  int64_t id;
+  ants_syn_request req;
+  int64_t footprint;
+  double t_submit;
+};
+
+}  // namespace
+
+struct ants_batch {
+  int slots = 1;
+  int64_t budget = 0;
+  int64_t used = 0;
+  int64_t next_id = 1;
+  int64_t pending = 0;  // submitted, not yet returned to the caller
+  bool stop = false;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<BatchItem> queue;
+  std::deque<ants_request_result> done;
+  std::vector<std::thread> workers;
+  struct Template {
+    bool set = false;
+    ants_image_desc desc{};
+    DevBuf img;
+  } templ[8];
+
+  void worker(int slot);
+};
+
+void ants_batch::worker(int slot) {
+  ants_syn_handle h;
+  if (hipStreamCreateWithFlags(&h.stream, hipStreamNonBlocking) != hipSuccess) return;
+  h.own_stream = true;
+  DevBuf d_fixed, d_moving, d_lab, d_fwd, d_inv, d_lab_out;
+  for (;;) {
+    BatchItem it;
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      // admit the queue head when the budget allows (or when nothing else is held)
+      cv_work.wait(lk, [&] {
+        return stop || (!queue.empty() && (used + queue.front().footprint <= budget || used == 0));
+      });
+      if (stop && queue.empty()) return;
+      if (queue.empty()) continue;
+      it = queue.front();
+      queue.pop_front();
+      used += it.footprint;
+    }
+    ants_request_result res{};
+    res.id = it.id;
+    res.slot = slot;
+    res.user = it.req.user;
+    res.t_submit = it.t_submit;
+    res.t_start = now_s();
+    res.footprint_bytes = it.footprint;
+    int rc = ANTS_OK;
+    const ants_syn_request& r = it.req;
+    const ants_image_desc* fd = r.use_template ? &templ[r.template_index].desc : &r.fixed;
+    HostGrid gF, gM;
+    rc = validate_params(&r.params);
+    if (rc == ANTS_OK) rc = to_host_grid(fd, &gF);
+    if (rc == ANTS_OK) rc = to_host_grid(&r.moving, &gM);
+    HostGrid gl{};
+    if (rc == ANTS_OK) {
+      h.P = r.params;
+      gl = level_grid(gF, r.params.shrink[r.params.n_levels - 1]);
+      const size_t fb = static_cast<size_t>(gl.count()) * gF.D * sizeof(float);
+      if (rc == ANTS_OK) rc = d_moving.ensure(gM.count() * sizeof(float));
+      if (rc == ANTS_OK && !r.use_template) rc = d_fixed.ensure(gF.count() * sizeof(float));
+      if (rc == ANTS_OK) rc = d_fwd.ensure(fb);
+      if (rc == ANTS_OK) rc = d_inv.ensure(fb);
+      if (rc == ANTS_OK && r.host_moving_labels) rc = d_lab.ensure(gM.count() * sizeof(int32_t));
+      if (rc == ANTS_OK && r.host_out_labels) rc = d_lab_out.ensure(gl.count() * sizeof(int32_t));
+      hipStream_t s = h.stream;
+      auto hip_ok = [&](hipError_t e) {
+        if (e != hipSuccess && rc == ANTS_OK) rc = fail(ANTS_ERR_HIP, "%s", hipGetErrorString(e));
+      };
+      if (rc == ANTS_OK) {
+        hip_ok(hipMemcpyAsync(d_moving.p, r.host_moving, gM.count() * sizeof(float), hipMemcpyHostToDevice, s));
+        if (!r.use_template)
+          hip_ok(hipMemcpyAsync(d_fixed.p, r.host_fixed, gF.count() * sizeof(float), hipMemcpyHostToDevice, s));
+        if (r.host_moving_labels)
+          hip_ok(hipMemcpyAsync(d_lab.p, r.host_moving_labels, gM.count() * sizeof(int32_t),
+                                hipMemcpyHostToDevice, s));
+      }
+      const float* dfix = r.use_template ? templ[r.template_index].img.f() : d_fixed.f();
+      if (rc == ANTS_OK) rc = h.run(fd, dfix, &r.moving, d_moving.f(), d_fwd.f(), d_inv.f(), nullptr, nullptr);
+      if (rc == ANTS_OK && r.host_moving_labels && r.host_out_labels) {
+        launch_labels_nn(to_dev(gl), d_fwd.f(), to_dev(gM), static_cast<const int32_t*>(d_lab.p),
+                         static_cast<int32_t*>(d_lab_out.p), s);
+        hip_ok(hipGetLastError());
+        hip_ok(hipMemcpyAsync(r.host_out_labels, d_lab_out.p, gl.count() * sizeof(int32_t), hipMemcpyDeviceToHost,
+                              s));
+      }
+      if (rc == ANTS_OK && r.host_out_fwd) hip_ok(hipMemcpyAsync(r.host_out_fwd, d_fwd.p, fb, hipMemcpyDeviceToHost, s));
+      if (rc == ANTS_OK && r.host_out_inv) hip_ok(hipMemcpyAsync(r.host_out_inv, d_inv.p, fb, hipMemcpyDeviceToHost, s));
+      hip_ok(hipStreamSynchronize(s));
+    }
+    res.status = rc;
+    if (rc != ANTS_OK) std::snprintf(res.error, sizeof(res.error), "%s", g_last_error.c_str());
+    res.voxel_iterations = h.run_stats.voxel_iterations;
+    res.algorithmic_bytes = h.run_stats.algorithmic_bytes;
+    res.t_end = now_s();
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      used -= it.footprint;
+      done.push_back(res);
+    }
+    cv_done.notify_all();
+    cv_work.notify_all();
+  }
+}
+
+extern "C" {
+
+double ants_batch_clock(void) { return now_s(); }
+
+int64_t ants_syn_footprint(const ants_syn_params* p, const ants_image_desc* fixed, const ants_image_desc* moving) {
+  if (validate_params(p) != ANTS_OK) return -1;
+  HostGrid gF, gM;
+  if (to_host_grid(fixed, &gF) != ANTS_OK || to_host_grid(moving, &gM) != ANTS_OK) return -1;
+  return footprint_bytes(*p, gF, gM);
+}
+
+int ants_batch_create(const ants_batch_config* cfg, ants_batch** out) {
+  if (!cfg || !out) return fail(ANTS_ERR_INVALID, "null argument");
+  if (cfg->slots < 1 || cfg->slots > 64) return fail(ANTS_ERR_INVALID, "slots must be in 1..64");
+  auto* b = new (std::nothrow) ants_batch();
+  if (!b) return fail(ANTS_ERR_OOM, "out of host memory");
+  b->slots = cfg->slots;
+  b->budget = cfg->memory_budget;
+  if (b->budget <= 0) {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+      delete b;
+      return fail(ANTS_ERR_HIP, "hipMemGetInfo failed");
+    }
+    b->budget = static_cast<int64_t>(0.9 * static_cast<double>(fr));
+  }
+  for (int i = 0; i < b->slots; ++i) b->workers.emplace_back(&ants_batch::worker, b, i);
+  *out = b;
+  return ANTS_OK;
+}
+
+void ants_batch_destroy(ants_batch* b) {
+  if (!b) return;
+  {
+    std::lock_guard<std::mutex> lk(b->mu);
+    b->stop = true;
+  }
+  b->cv_work.notify_all();
+  for (auto& t : b->workers) t.join();
+  delete b;
+}
+
+int ants_batch_set_template(ants_batch* b, int index, const ants_image_desc* g, const float* host_image) {
+  if (!b || !g || !host_image) return fail(ANTS_ERR_INVALID, "null argument");
+  if (index < 0 || index >= 8) return fail(ANTS_ERR_INVALID, "template index must be in 0..7");
+  HostGrid hg;
+  ANTS_TRY(to_host_grid(g, &hg));
+  std::lock_guard<std::mutex> lk(b->mu);
+  auto& t = b->templ[index];
+  ANTS_TRY(t.img.ensure(hg.count() * sizeof(float)));
+  HIP_TRY(hipMemcpy(t.img.p, host_image, hg.count() * sizeof(float), hipMemcpyHostToDevice));
+  t.desc = *g;
+  t.set = true;
+  return ANTS_OK;
+}
+
+int ants_batch_submit(ants_batch* b, const ants_syn_request* r, int64_t* id) {
+  if (!b || !r) return fail(ANTS_ERR_INVALID, "null argument");
+  ANTS_TRY(validate_params(&r->params));
+  if (r->use_template && (r->template_index < 0 || r->template_index >= 8 || !b->templ[r->template_index].set))
+    return fail(ANTS_ERR_INVALID, "template %d is not set", r->template_index);
+  const ants_image_desc* fd = r->use_template ? &b->templ[r->template_index].desc : &r->fixed;
+  HostGrid gF, gM;
+  ANTS_TRY(to_host_grid(fd, &gF));
+  ANTS_TRY(to_host_grid(&r->moving, &gM));
+  if (gF.D != r->params.dim || gM.D != r->params.dim) return fail(ANTS_ERR_INVALID, "dimension mismatch");
+  if (!r->host_moving || (!r->use_template && !r->host_fixed)) return fail(ANTS_ERR_INVALID, "null image");
+  const int64_t fp = footprint_bytes(r->params, gF, gM);
+  BatchItem it{0, *r, fp, now_s()};
+  {
+    std::lock_guard<std::mutex> lk(b->mu);
+    if (fp > b->budget) return fail(ANTS_ERR_OOM, "request needs %lld bytes, budget is %lld", (long long)fp,
+                                    (long long)b->budget);
+    it.id = b->next_id++;
+    b->queue.push_back(it);
+    b->pending += 1;
+  }
+  if (id) *id = it.id;
+  b->cv_work.notify_one();
+  return ANTS_OK;
+}
+
+static int take_results(ants_batch* b, ants_request_result* out, int max_results, int* n_out) {
+  int n = 0;
+  while (n < max_results && !b->done.empty()) {
+    out[n++] = b->done.front();
+    b->done.pop_front();
+    b->pending -= 1;
+  }
+  if (n_out) *n_out = n;
+  return ANTS_OK;
+}
+
+int ants_batch_poll(ants_batch* b, ants_request_result* out, int max_results, int* n_out) {
+  if (!b || !out) return fail(ANTS_ERR_INVALID, "null argument");
+  std::lock_guard<std::mutex> lk(b->mu);
+  return take_results(b, out, max_results, n_out);
+}
+
+int ants_batch_wait(ants_batch* b, ants_request_result* out, int max_results, int* n_out) {
+  if (!b || !out) return fail(ANTS_ERR_INVALID, "null argument");
+  std::unique_lock<std::mutex> lk(b->mu);
+  b->cv_done.wait(lk, [&] { return !b->done.empty() || b->pending == 0; });
+  return take_results(b, out, max_results, n_out);
+}
+
+int64_t ants_batch_pending(ants_batch* b) {
+  if (!b) return -1;
+  std::lock_guard<std::mutex> lk(b->mu);
+  return b->pending;
 }
 
 }  // extern "C"

@@ -53,6 +53,111 @@ def cpu_baseline(pair, seconds_hint: float = 20.0):
             "seconds": dt}
 
 
+def _gen_pair(args):
+    """Process-pool worker: one phantom pair (fields dropped to keep pickling small)."""
+    seed, size = args
+    from ants_amd import phantoms as ph
+    p = ph.phantom_pair_3d(seed, size=size)
+    return p.fixed, p.moving, p.labels_moving
+
+
+def _pool_pairs(specs):
+    import concurrent.futures as cf
+    workers = min(16, os.cpu_count() or 4, len(specs))
+    with cf.ProcessPoolExecutor(max_workers=workers) as ex:
+        return list(ex.map(_gen_pair, specs))
+
+
+def bench_batch(args):
+    """64 independent 3-D pairs, dims U{96..192} per axis; 16 quick pairs
+    (CC r=2, 40x20x0) and 48 with the 192x224x192 schedule.  Batched slots
+    versus the same pairs one at a time on the same GPU."""
+    from ants_amd.registration import SyNStage
+    from ants_amd.serving import Batch, Request, latency_stats, run_all
+    rng = np.random.default_rng(args.seed)
+    n = args.pairs
+    specs = [(1000 + k, tuple(int(v) for v in rng.integers(96, 193, 3))) for k in range(n)]
+    t0 = time.perf_counter()
+    data = _pool_pairs(specs)
+    log(f"generated {n} pairs in {time.perf_counter() - t0:.1f} s")
+    quick = SyNStage.from_args("SyN[0.1,3,0]", "CC[f,m,1,2]", "[40x20x0,1e-6,10]", "4x2x1", "2x1x0vox")
+    full = SyNStage.from_args(**SCHEDULE)
+    reqs = [Request(quick if k < n // 4 else full, mov, fixed=fix) for k, (fix, mov, _) in enumerate(data)]
+    out = {}
+    for name, slots in (("batched", args.slots), ("one_at_a_time", 1)):
+        b = Batch(slots=slots)
+        run_all(b, reqs[: min(4, n)])  # warm-up: graphs and buffers
+        t = time.perf_counter()
+        res = run_all(b, reqs)
+        wall = time.perf_counter() - t
+        b.close()
+        bad = [r.error for r in res if r.status != 0]
+        if bad:
+            raise RuntimeError(bad[0])
+        vi = sum(r.voxel_iterations for r in res)
+        by = sum(r.algorithmic_bytes for r in res)
+        out[name] = {"slots": slots, "wall_s": wall, "pairs_per_s": n / wall, "voxel_iters_per_s": vi / wall,
+                     "algorithmic_gbs": by / wall / 1e9, "hbm_frac": by / wall / 1e9 / HBM_PEAK_GBS}
+        log(f"{name}: {n} pairs in {wall:.2f} s ({n / wall:.2f} pairs/s)")
+    line = {"metric": "batched SyN pairs/s (64 mixed-size 3-D pairs)", "value": out["batched"]["pairs_per_s"],
+            "unit": "pairs/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": f"{n} phantom pairs, dims U{{96..192}}^3, {n // 4} quick (r=2, 40x20x0), "
+                                   f"{n - n // 4} at r=4 100x70x20", "slots": args.slots},
+            "batched": out["batched"], "one_at_a_time": out["one_at_a_time"],
+            "speedup_vs_one_at_a_time": out["batched"]["pairs_per_s"] / out["one_at_a_time"]["pairs_per_s"]}
+    print(json.dumps(line), flush=True)
+
+
+def bench_serve(args):
+    """Poisson request stream at a fixed QPS: 70% quick SyN at 128^3 (r=2,
+    40x20x0), 30% SyN at 192x224x192 (r=4, 100x70x20), both against resident
+    templates, each warping a 32-label Voronoi map nearest-neighbour.  The
+    continuous-batching scheduler versus one request at a time."""
+    from ants_amd import phantoms as ph
+    from ants_amd.registration import SyNStage
+    from ants_amd.serving import Batch, Request, latency_stats, serve
+    rng = np.random.default_rng(args.seed)
+    n = args.requests
+    kinds = rng.random(n) < 0.7
+    t0 = time.perf_counter()
+    big_t = ph.phantom_pair_3d(5000, size=(192, 224, 192))
+    small_t = ph.phantom_pair_3d(5001, size=(128, 128, 128))
+    specs = [(6000 + k, (128, 128, 128) if kinds[k] else (192, 224, 192)) for k in range(n)]
+    data = _pool_pairs(specs)
+    labels = []
+    for k, (fix, mov, lab) in enumerate(data):
+        mask = lab > 0
+        labels.append(ph.voronoi_labels(7000 + k, mask, 32))
+    log(f"generated templates and {n} requests in {time.perf_counter() - t0:.1f} s")
+    quick = SyNStage.from_args("SyN[0.1,3,0]", "CC[f,m,1,2]", "[40x20x0,1e-6,10]", "4x2x1", "2x1x0vox")
+    full = SyNStage.from_args(**SCHEDULE)
+    reqs = [Request(quick if kinds[k] else full, data[k][1], fixed=None, template_index=1 if kinds[k] else 0,
+                    moving_labels=labels[k]) for k in range(n)]
+    arrivals = np.cumsum(rng.exponential(1.0 / args.qps, n))
+    arrivals -= arrivals[0]
+    out = {}
+    for name, slots in (("continuous_batching", args.slots), ("one_at_a_time", 1)):
+        b = Batch(slots=slots)
+        b.set_template(0, big_t.fixed)
+        b.set_template(1, small_t.fixed)
+        serve(b, reqs[:2], [0.0, 0.0])  # warm-up
+        res = serve(b, reqs, arrivals)
+        b.close()
+        bad = [r.error for r in res if r.status != 0]
+        if bad:
+            raise RuntimeError(bad[0])
+        st = latency_stats(res)
+        st["slots"] = slots
+        out[name] = st
+        log(f"{name}: p50 {st['p50_s'] * 1e3:.1f} ms p99 {st['p99_s'] * 1e3:.1f} ms {st['pairs_per_s']:.2f} pairs/s")
+    line = {"metric": "served SyN: pairs/s and p50/p99 latency at fixed QPS", "value": out["continuous_batching"]["pairs_per_s"],
+            "unit": "pairs/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": f"{n} Poisson requests at {args.qps} QPS: 70% 128^3 r=2 40x20x0, 30% 192x224x192 "
+                                   "r=4 100x70x20, resident templates, 32-label NN warp", "slots": args.slots},
+            "continuous_batching": out["continuous_batching"], "one_at_a_time": out["one_at_a_time"]}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -60,11 +165,21 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--mode", choices=["pair", "batch", "serve"], default="pair",
+                    help="pair: the 192x224x192 headline; batch: 64 mixed pairs; serve: Poisson request stream")
+    ap.add_argument("--pairs", type=int, default=64)
+    ap.add_argument("--slots", type=int, default=8)
+    ap.add_argument("--requests", type=int, default=40)
+    ap.add_argument("--qps", type=float, default=8.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for {args.gpus}); "
               "pairs would shard across GPUs with no communication", file=sys.stderr)
         sys.exit(2)
+    if args.mode == "batch":
+        return bench_batch(args)
+    if args.mode == "serve":
+        return bench_serve(args)
 
     from ants_amd import _lib as L
     from ants_amd import phantoms as ph

@@ -215,6 +215,61 @@ int ants_op_preprocess(int64_t n, const float* d_in, float* d_out, void* stream)
 int ants_apply_transform_labels_nn(const ants_image_desc* g_out, const float* d_field, const ants_image_desc* g_lab,
                                    const int32_t* d_labels, int32_t* d_out, void* stream);
 
+/* ---- batched and served registration ----
+ * No counterpart inside ANTs: the reference runs independent registrations
+ * as separate processes or cluster jobs
+ * (Scripts/antsMultivariateTemplateConstruction2.sh:1163-1280).  Here a batch
+ * keeps `slots` registrations in flight on one GPU, each on its own stream
+ * with a reusable engine, admits queued requests in order as slots and the
+ * device-memory budget free up, and keeps fixed templates resident. */
+typedef struct ants_batch_config {
+  int slots;              /* concurrent registrations (streams); 1..64 */
+  int64_t memory_budget;  /* device bytes admitted requests may hold; 0 = 90% of free device memory */
+} ants_batch_config;
+
+typedef struct ants_syn_request {
+  ants_syn_params params;
+  ants_image_desc fixed;               /* ignored when use_template */
+  const float* host_fixed;
+  ants_image_desc moving;
+  const float* host_moving;
+  const int32_t* host_moving_labels;   /* optional: label map on the moving grid, warped NN into the fixed grid */
+  int use_template;                    /* 1: the fixed image is resident template `template_index` */
+  int template_index;
+  float* host_out_fwd;                 /* optional outputs on the last level grid */
+  float* host_out_inv;
+  int32_t* host_out_labels;            /* optional, last level grid */
+  void* user;
+} ants_syn_request;
+
+typedef struct ants_request_result {
+  int64_t id;
+  int status;                          /* ANTS_OK or an error code */
+  int slot;
+  double t_submit, t_start, t_end;     /* seconds on ants_batch_clock() */
+  double voxel_iterations;
+  double algorithmic_bytes;
+  int64_t footprint_bytes;
+  void* user;
+  char error[256];
+} ants_request_result;
+
+typedef struct ants_batch ants_batch;
+int ants_batch_create(const ants_batch_config* cfg, ants_batch** out);
+void ants_batch_destroy(ants_batch* b);
+/* Upload a fixed template once (index 0..7); requests may then use it. */
+int ants_batch_set_template(ants_batch* b, int index, const ants_image_desc* g, const float* host_image);
+/* Queue a request (host buffers must stay valid until its result is returned). */
+int ants_batch_submit(ants_batch* b, const ants_syn_request* r, int64_t* id);
+/* Completed results: poll returns at once; wait blocks until at least one is
+ * available or nothing is pending. */
+int ants_batch_poll(ants_batch* b, ants_request_result* out, int max_results, int* n_out);
+int ants_batch_wait(ants_batch* b, ants_request_result* out, int max_results, int* n_out);
+int64_t ants_batch_pending(ants_batch* b);
+double ants_batch_clock(void);
+/* Device bytes one registration of this geometry and schedule holds. */
+int64_t ants_syn_footprint(const ants_syn_params* p, const ants_image_desc* fixed, const ants_image_desc* moving);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,112 @@
+"""Batched / served registration: every result equals the same request run
+alone (bitwise: the kernels and reductions are deterministic), templates
+stay resident, and memory admission holds requests back."""
+import numpy as np
+import pytest
+
+from ants_amd import phantoms as ph
+from ants_amd.registration import SyNRegistration, SyNStage
+from ants_amd.serving import Batch, Request, latency_stats, run_all, serve
+
+pytestmark = pytest.mark.gpu
+
+QUICK = SyNStage.from_args("SyN[0.1,3,0]", "CC[f,m,1,2]", "[8x4x0,1e-6,10]", "4x2x1", "2x1x0vox")
+FULL = SyNStage.from_args("SyN[0.1,3,0]", "CC[f,m,1,4]", "[6x4x2,1e-6,10]", "4x2x1", "2x1x0vox")
+
+
+def make_requests(n=5, seed=3):
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for k in range(n):
+        size = tuple(int(v) for v in rng.integers(32, 56, 3))
+        p = ph.phantom_pair_3d(100 + k, size=size, max_disp_mm=3.0, svf_sigma_mm=5.0)
+        reqs.append(Request(QUICK if k % 2 else FULL, p.moving, fixed=p.fixed, moving_labels=p.labels_moving,
+                            want_fields=True, tag=str(k)))
+    return reqs
+
+
+def alone(req):
+    reg = SyNRegistration(req.stage, 3)
+    r = reg.register(req.fixed, req.moving)
+    reg.close()
+    return r
+
+
+def test_batch_results_equal_single_runs():
+    reqs = make_requests()
+    b = Batch(slots=3)
+    res = run_all(b, reqs)
+    b.close()
+    assert all(r.status == 0 for r in res), [r.error for r in res]
+    # requests overlapped on the device
+    spans = sorted((r.t_start, r.t_end) for r in res)
+    assert any(spans[i + 1][0] < spans[i][1] for i in range(len(spans) - 1))
+    for r in res:
+        ref = alone(r.request)
+        np.testing.assert_array_equal(r.fwd, ref.fwd)
+        np.testing.assert_array_equal(r.inv, ref.inv)
+        assert r.voxel_iterations > 0
+
+
+def test_same_result_alone_and_in_the_mix():
+    reqs = make_requests(6, seed=9)
+    target = reqs[2]
+    b1 = Batch(slots=1)
+    solo = run_all(b1, [target])[0]
+    b1.close()
+    b4 = Batch(slots=4)
+    mix = run_all(b4, reqs)
+    b4.close()
+    m = [r for r in mix if r.request is target][0]
+    np.testing.assert_array_equal(m.fwd, solo.fwd)
+    np.testing.assert_array_equal(m.inv, solo.inv)
+    np.testing.assert_array_equal(m.labels, solo.labels)
+
+
+def test_resident_template_equals_explicit_fixed():
+    p = ph.phantom_pair_3d(21, size=(40, 44, 36), max_disp_mm=3.0, svf_sigma_mm=5.0)
+    b = Batch(slots=2)
+    b.set_template(0, p.fixed)
+    r_t = Request(FULL, p.moving, fixed=None, template_index=0, moving_labels=p.labels_moving, want_fields=True)
+    r_f = Request(FULL, p.moving, fixed=p.fixed, moving_labels=p.labels_moving, want_fields=True)
+    a, c = run_all(b, [r_t, r_f])
+    b.close()
+    np.testing.assert_array_equal(a.fwd, c.fwd)
+    np.testing.assert_array_equal(a.labels, c.labels)
+    # the label output is the nearest-neighbour warp of the moving labels by the returned field
+    import oracle as o
+    g = o.Grid(list(p.size))
+    np.testing.assert_array_equal(a.labels, o.warp_labels_nn(g, a.fwd, g, p.labels_moving))
+
+
+def test_memory_admission_serialises_and_rejects():
+    reqs = make_requests(3, seed=5)
+    from ants_amd import serving as S
+    import ctypes as C
+    lib = S._bind()
+    fp = []
+    for r in reqs:
+        prm = S.stage_params(r.stage, 3)
+        d = S.L.desc(tuple(reversed(r.fixed.shape)))
+        fp.append(lib.ants_syn_footprint(C.byref(prm), C.byref(d), C.byref(d)))
+    b = Batch(slots=3, memory_budget=max(fp) + 1)
+    res = run_all(b, reqs)
+    b.close()
+    spans = sorted((r.t_start, r.t_end) for r in res)
+    # a second request is only admitted once the first released its memory
+    for i in range(len(spans) - 1):
+        assert spans[i + 1][0] >= spans[i][1] - 1e-6
+    small = Batch(slots=1, memory_budget=1024)
+    with pytest.raises(Exception):
+        small.submit(reqs[0])
+    small.close()
+
+
+def test_serving_loop_reports_latency():
+    reqs = make_requests(6, seed=13)
+    b = Batch(slots=3)
+    res = serve(b, reqs, arrival_times=[0.0, 0.01, 0.02, 0.03, 0.2, 0.21])
+    b.close()
+    st = latency_stats(res)
+    assert st["n"] == 6 and st["p99_s"] >= st["p50_s"] > 0
+    assert all(r.t_start >= r.t_submit for r in res)
