@@ -63,6 +63,7 @@ class SynStatsC(C.Structure):
         ("algorithmic_bytes", C.c_double),
         ("graph_launches", C.c_int64),
         ("eager_iterations", C.c_int64),
+        ("inv_tails", C.c_int64),
     ]
 
 
@@ -99,6 +100,7 @@ def lib():
     L.ants_memcpy_h2d.argtypes = [vp, vp, sz, vp]
     L.ants_memcpy_d2h.argtypes = [vp, vp, sz, vp]
     L.ants_memset_device.argtypes = [vp, i, sz, vp]
+    L.ants_copy_calibration.argtypes = [vp, vp, sz, vp]
     L.ants_stream_create.argtypes = [C.POINTER(vp)]
     L.ants_stream_destroy.argtypes = [vp]
     L.ants_stream_synchronize.argtypes = [vp]

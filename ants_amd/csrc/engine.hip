@@ -303,8 +303,10 @@ struct DevBuf {
 
 // HIP-event timing of kernel classes on one stream.  Events are recorded
 // around each launch of an enabled class and read after the iteration's
-// synchronisation; inversion launches are attributed afterwards from the
-// per-step `computed` flags (inactive launches carry no bytes and no time).
+// synchronisation; the bytes of inversion launches are attributed afterwards
+// from the per-step `computed` flags.  Launches of steps that found their
+// inversion already converged count with their time and zero bytes, so the
+// class's mean launch time is the mean over every dispatch of the kernel.
 struct Profiler {
   unsigned int mask = 0;
   hipStream_t s = nullptr;
@@ -315,7 +317,8 @@ struct Profiler {
     int cls;
     hipEvent_t a, b;
     double bytes;
-    int batch, step;  // inversion launches
+    int launches;
+    int batches, j0, j1;  // inversion spans: batch bit mask, steps [j0, j1)
   };
   std::vector<Rec> recs;
   hipEvent_t a_ = nullptr;
@@ -335,12 +338,12 @@ struct Profiler {
     a_ = next();
     if (a_) (void)hipEventRecord(a_, s);
   }
-  void end(int cls, double bytes, int batch = -1, int step = -1) {
+  void end(int cls, double bytes, int launches = 1, int batches = 0, int j0 = 0, int j1 = 0) {
     if (!on(cls) || !a_) return;
     hipEvent_t b = next();
     if (!b) return;
     (void)hipEventRecord(b, s);
-    recs.push_back(Rec{cls, a_, b, bytes, batch, step});
+    recs.push_back(Rec{cls, a_, b, bytes, launches, batches, j0, j1});
     a_ = nullptr;
   }
   // stats: the iteration's statistics block (host copy); N voxels, D dims
@@ -350,23 +353,34 @@ struct Profiler {
       if (r.cls == ANTS_KC_INV) {
         bytes = 0.0;
         const int qs[2][2] = {{0, 2}, {1, 3}};
-        const int j0 = r.step < 0 ? 0 : r.step, j1 = r.step < 0 ? kInvMaxIter : r.step;
-        for (int j = j0; j <= j1; ++j)
-          for (int k = 0; k < 2; ++k) {
-            const RedSlot* S = st->inv[qs[r.batch][k]];
+        for (int bt = 0; bt < 2; ++bt) {
+          if (!((r.batches >> bt) & 1)) continue;
+          for (int j = r.j0; j < r.j1; ++j)
+            for (int k = 0; k < 2; ++k) {
+            const RedSlot* S = st->inv[qs[bt][k]];
             const bool upd = j >= 1 && S[j - 1].computed;
             const bool nrm = S[j].computed != 0;
+            // algorithmic bytes of an inversion with K norm passes:
+            // (3K + 1) field arrays (read inv, gather field, write inv per
+            // step, plus the first norm pass); the stored residual this
+            // kernel also moves is overhead, not algorithmic
             if (upd)
               bytes += 3.0 * D * 4.0 * N;
             else if (nrm)
-              bytes += 2.0 * D * 4.0 * N;
+              bytes += 1.0 * D * 4.0 * N;
           }
-        if (bytes == 0.0) continue;
+        }
       }
       float ms = 0.f;
-      if (hipEventElapsedTime(&ms, r.a, r.b) != hipSuccess) continue;
+      const hipError_t e = hipEventElapsedTime(&ms, r.a, r.b);
+      if (e != hipSuccess) {
+        (void)hipGetLastError();  // not sticky: a later launch check must not see it
+        if (std::getenv("ANTS_SYN_VERBOSE"))
+          std::fprintf(stderr, "ants_syn: no time for class %d span (%s)\n", r.cls, hipGetErrorString(e));
+        continue;
+      }
       acc.ms[r.cls] += ms;
-      acc.launches[r.cls] += 1;
+      acc.launches[r.cls] += r.launches;
       acc.bytes[r.cls] += bytes;
     }
     recs.clear();
@@ -407,7 +421,6 @@ struct GraphEntry {
   hipGraphExec_t exec = nullptr;
   float* nst[4];
   float* nwk[4];
-  std::vector<Profiler::Rec> recs;
 };
 
 struct ants_syn_handle {
@@ -425,6 +438,7 @@ struct ants_syn_handle {
   Profiler prof;
   std::map<GraphKey, GraphEntry> graphs;
   bool use_graphs = true;
+  int inv_launches = kInvLaunchesDefault;
   std::vector<const void*> graph_signature;  // buffers + geometry the cached graphs were built for
 
   void clear_graphs() {
@@ -644,8 +658,16 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
     setup_hash = fnv1a(P.sigma, sizeof(P.sigma), setup_hash);
     const double dbl[] = {P.learning_rate, P.update_variance, P.total_variance};
     setup_hash = fnv1a(dbl, sizeof(dbl), setup_hash);
+    // Profiled runs launch eagerly: events recorded by the event nodes of a
+    // replayed graph do not yield elapsed times on this runtime.
     const char* env = std::getenv("ANTS_SYN_GRAPHS");
-    use_graphs = !(env && env[0] == '0');
+    use_graphs = !(env && env[0] == '0') && prof.mask == 0;
+    // inversion steps launched per batch inside an iteration's sequence; a
+    // batch that still has work after them is finished from the host
+    const char* ks = std::getenv("ANTS_SYN_INV_LAUNCHES");
+    inv_launches = ks ? std::atoi(ks) : kInvLaunchesDefault;
+    inv_launches = std::min(std::max(inv_launches, 1), kInvMaxIter + 1);
+    setup_hash = fnv1a(&inv_launches, sizeof(inv_launches), setup_hash);
   }
   std::memset(&run_stats, 0, sizeof(run_stats));
   std::memset(&prof.acc, 0, sizeof(prof.acc));
@@ -708,6 +730,58 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
     // from the rotation (cst, cwk); it reports the rotation after the
     // iteration in (nst, nwk).  Host work here is only pointer bookkeeping,
     // so the sequence can be captured once into a hipGraph and replayed.
+    // inversion steps j0..j1-1 of one batch (two problems, one launch per step)
+    auto inv_steps_on = [&](int batch, const float* const* field, float* const* inv, int j0, int j1,
+                            const InvGate& gate) -> int {
+      RedSlot* slots[2] = {dstats->inv[batch], dstats->inv[batch + 2]};
+      for (int j = j0; j < j1; ++j)
+        launch_inv_step(gd, 2, field, inv, resid, slots, static_cast<Partial*>(partials_inv.p), j, s, gate);
+      HIP_TRY(hipGetLastError());
+      return ANTS_OK;
+    };
+    // host-completed steps: one timed span per call
+    auto inv_tail_on = [&](int batch, const float* const* field, float* const* inv, int j0) -> int {
+      prof.begin(ANTS_KC_INV);
+      ANTS_TRY(inv_steps_on(batch, field, inv, j0, kInvMaxIter + 1, kNoGate));
+      prof.end(ANTS_KC_INV, 0.0, kInvMaxIter + 1 - j0, 1 << batch, j0, kInvMaxIter + 1);
+      return ANTS_OK;
+    };
+    // After the iteration's sequence: a batch whose problems computed a norm
+    // at the last launched step has further steps to run.  If the first
+    // batch is unfinished the gated second batch did nothing, and both are
+    // run to the end here; otherwise only the second batch's tail.  In terms
+    // of the rotated buffers: batch 0 inverts st[0]/st[2] into st[1]/st[3],
+    // batch 1 inverts st[1]/st[3] into st[0]/st[2].
+    auto inv_finish = [&](bool& extra) -> int {
+      extra = false;
+      const int K = inv_launches;
+      if (K > kInvMaxIter) return ANTS_OK;
+      auto pending = [&](int batch) {
+        return host_stats->inv[batch][K - 1].computed || host_stats->inv[batch + 2][K - 1].computed;
+      };
+      const float* f0[2] = {st[0], st[2]};
+      float* i0[2] = {st[1], st[3]};
+      const float* f1[2] = {st[1], st[3]};
+      float* i1[2] = {st[0], st[2]};
+      if (pending(0)) {
+        // the gated launches of batch 1 did nothing: they carry no bytes
+        for (auto& r : prof.recs)
+          if (r.cls == ANTS_KC_INV) r.batches &= ~2;
+        ANTS_TRY(inv_tail_on(0, f0, i0, K));
+        ANTS_TRY(inv_tail_on(1, f1, i1, 0));
+        extra = true;
+      } else if (pending(1)) {
+        ANTS_TRY(inv_tail_on(1, f1, i1, K));
+        extra = true;
+      }
+      if (extra) {
+        HIP_TRY(hipMemcpyAsync(host_stats, dstats, sizeof(IterStats), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        run_stats.inv_tails += 1;
+      }
+      return ANTS_OK;
+    };
+
     auto enqueue = [&](float* const* cst, float* const* cwk, float** nst, float** nwk) -> int {
       float* st_[4] = {cst[0], cst[1], cst[2], cst[3]};
       float* wk_[4] = {cwk[0], cwk[1], cwk[2], cwk[3]};
@@ -761,17 +835,21 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
       }
       // 6. inversions: inverse middle fields from the previous inverses (in place),
       //    then forward fields as inverses of the new inverses, started from t (in place)
+      //    Only the first inv_launches steps of each batch are enqueued here:
+      //    almost every inversion converges within them.  The second batch's
+      //    launches are gated on the first having finished (a no-op
+      //    otherwise); the host completes whatever is left after reading the
+      //    step flags (inv_finish below).
+      prof.begin(ANTS_KC_INV);
       for (int batch = 0; batch < 2; ++batch) {
         const float* field[2] = {batch == 0 ? wk_[2] : st_[1], batch == 0 ? wk_[3] : st_[3]};
         float* inv[2] = {batch == 0 ? st_[1] : wk_[2], batch == 0 ? st_[3] : wk_[3]};
-        RedSlot* slots[2] = {dstats->inv[batch], dstats->inv[batch + 2]};
-        for (int j = 0; j <= kInvMaxIter; ++j) {
-          prof.begin(ANTS_KC_INV);
-          launch_inv_step(gd, 2, field, inv, resid, slots, static_cast<Partial*>(partials_inv.p), j, s);
-          prof.end(ANTS_KC_INV, 0.0, batch, j);
-        }
+        InvGate gate = kNoGate;
+        if (batch == 1 && inv_launches <= kInvMaxIter)
+          gate = InvGate{{dstats->inv[0], dstats->inv[2]}, 2, inv_launches - 1};
+        ANTS_TRY(inv_steps_on(batch, field, inv, 0, inv_launches, gate));
       }
-      HIP_TRY(hipGetLastError());
+      prof.end(ANTS_KC_INV, 0.0, 2 * inv_launches, 3, 0, inv_launches);
       std::swap(st_[0], wk_[2]);
       std::swap(st_[2], wk_[3]);
       HIP_TRY(hipMemcpyAsync(host_stats, dstats, sizeof(IterStats), hipMemcpyDeviceToHost, s));
@@ -790,27 +868,26 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
       if (use_graphs && it == graphs.end()) {
         // capture this rotation's sequence once
         GraphEntry ge{};
-        const size_t rec0 = prof.recs.size();
         int rc = ANTS_OK;
         if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
           rc = enqueue(st, wk, ge.nst, ge.nwk);
           hipGraph_t graph = nullptr;
           const hipError_t e = hipStreamEndCapture(s, &graph);
-          if (rc == ANTS_OK && e == hipSuccess && graph &&
-              hipGraphInstantiate(&ge.exec, graph, nullptr, nullptr, 0) == hipSuccess) {
-            ge.recs.assign(prof.recs.begin() + rec0, prof.recs.end());
-            it = graphs.emplace(key, std::move(ge)).first;
-          }
+          hipError_t ei = hipErrorUnknown;
+          if (rc == ANTS_OK && e == hipSuccess && graph)
+            ei = hipGraphInstantiate(&ge.exec, graph, nullptr, nullptr, 0);
+          if (ei != hipSuccess && std::getenv("ANTS_SYN_VERBOSE"))
+            std::fprintf(stderr, "ants_syn: iteration capture at level %d failed (enqueue %d: %s; end %s; instantiate %s)\n",
+                         l, rc, rc == ANTS_OK ? "" : g_last_error.c_str(), hipGetErrorString(e), hipGetErrorString(ei));
+          if (ei == hipSuccess) it = graphs.emplace(key, std::move(ge)).first;
           if (graph) (void)hipGraphDestroy(graph);
         }
-        prof.recs.resize(rec0);
         (void)hipGetLastError();
         if (it == graphs.end()) use_graphs = false;  // capture not possible here: launch eagerly
       }
       if (use_graphs && it != graphs.end()) {
         HIP_TRY(hipGraphLaunch(it->second.exec, s));
         run_stats.graph_launches += 1;
-        prof.recs.insert(prof.recs.end(), it->second.recs.begin(), it->second.recs.end());
         for (int k = 0; k < 4; ++k) {
           nst[k] = it->second.nst[k];
           nwk[k] = it->second.nwk[k];
@@ -825,6 +902,8 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
       }
       // 7. energy, convergence, observer
       HIP_TRY(hipStreamSynchronize(s));
+      bool extra = false;
+      ANTS_TRY(inv_finish(extra));
       prof.collect(host_stats, N, D);
       const double value = -host_stats->energy.sum / static_cast<double>(N);
       const double E = 0.5 * (value + value);
@@ -892,6 +971,14 @@ int ants_memcpy_d2h(void* dst, const void* d_src, size_t bytes, void* stream) {
 }
 int ants_memset_device(void* d_dst, int value, size_t bytes, void* stream) {
   HIP_TRY(hipMemsetAsync(d_dst, value, bytes, static_cast<hipStream_t>(stream)));
+  return ANTS_OK;
+}
+int ants_copy_calibration(void* d_dst, const void* d_src, size_t bytes, void* stream) {
+  if (!d_dst || !d_src || bytes % 16 != 0)
+    return fail(ANTS_ERR_INVALID, "copy_calibration: null pointer or size not a multiple of 16");
+  if (bytes == 0) return ANTS_OK;
+  launch_copy_calibration(d_src, d_dst, static_cast<int64_t>(bytes / 16), static_cast<hipStream_t>(stream));
+  HIP_TRY(hipGetLastError());
   return ANTS_OK;
 }
 int ants_stream_create(void** stream) {

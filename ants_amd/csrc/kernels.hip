@@ -539,6 +539,14 @@ __global__ void __launch_bounds__(kBlock) k_to_bf16(const float* __restrict__ in
     out[i] = f2bf(in[i]);
 }
 
+// Plain 16-byte-vector copy: reads n4*16 bytes and writes as many, once each.
+__global__ void __launch_bounds__(kBlock) k_copy_calibration(const float4* __restrict__ in, float4* __restrict__ out,
+                                                             int64_t n4) {
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n4;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    out[i] = in[i];
+}
+
 template <int D>
 __global__ void __launch_bounds__(kBlock) k_warp_image(GridDev gi, const float* __restrict__ img, GridDev go,
                                                        const float* __restrict__ u, float* __restrict__ out) {
@@ -904,7 +912,13 @@ constexpr int kInvBlocks = 512;
 template <int D>
 __global__ void __launch_bounds__(kInvThreads) k_inv_step(GridDev g, PtrPack4 field, MutPack4 inv, MutPack4 res,
                                                           SlotPack4 slots, Partial* partials, int64_t part_stride,
-                                                          int j) {
+                                                          int j, InvGate gate) {
+  // gated launch: a no-op while the inversions it depends on still had work
+  // pending after their last launched step (the host then reruns it)
+  if (gate.step >= 0) {
+    for (int q = 0; q < gate.n; ++q)
+      if (gate.slots[q][gate.step].computed) return;
+  }
   const int k = blockIdx.y;
   RedSlot* st = slots.p[k];
   bool do_update = false, do_norm = false;
@@ -1110,6 +1124,11 @@ void launch_to_bf16(const float* in, uint16_t* out, int64_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_to_bf16, dim3(grid_stride_blocks(n)), dim3(kBlock), 0, s, in, out, n);
 }
 
+void launch_copy_calibration(const void* in, void* out, int64_t n4, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy_calibration, dim3(grid_stride_blocks(n4)), dim3(kBlock), 0, s,
+                     static_cast<const float4*>(in), static_cast<float4*>(out), n4);
+}
+
 void launch_warp_image(const GridDev& gi, const float* img, const GridDev& go, const float* u, float* out,
                        hipStream_t s) {
   const dim3 grid(blocks_for(go.count()));
@@ -1174,17 +1193,17 @@ static unsigned int inv_blocks(int64_t n) {
 int64_t inv_partials_needed(const GridDev& g) { return inv_blocks(g.count()); }
 
 void launch_inv_step(const GridDev& g, int np, const float* const* field, float* const* inv, float* const* res,
-                     RedSlot* const* slots, Partial* partials, int j, hipStream_t s) {
+                     RedSlot* const* slots, Partial* partials, int j, hipStream_t s, const InvGate& gate) {
   const dim3 grid(inv_blocks(g.count()), np);
   SlotPack4 sp{};
   for (int k = 0; k < np; ++k) sp.p[k] = slots[k];
   const int64_t stride = inv_partials_needed(g);
   if (g.D == 2)
     hipLaunchKernelGGL(k_inv_step<2>, grid, dim3(kInvThreads), 0, s, g, pack(field, np), mpack(inv, np), mpack(res, np),
-                       sp, partials, stride, j);
+                       sp, partials, stride, j, gate);
   else
     hipLaunchKernelGGL(k_inv_step<3>, grid, dim3(kInvThreads), 0, s, g, pack(field, np), mpack(inv, np), mpack(res, np),
-                       sp, partials, stride, j);
+                       sp, partials, stride, j, gate);
 }
 
 void launch_resample_fields(const GridDev& gi, const GridDev& go, int np, const float* const* in, float* const* out,

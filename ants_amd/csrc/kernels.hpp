@@ -66,6 +66,7 @@ constexpr int kRedBlocks = 1024;
 
 constexpr int kInvMaxIter = 20;            // InvertDisplacementFieldImageFilter MaximumNumberOfIterations (SyN)
 constexpr int kInvSlots = kInvMaxIter + 1;
+constexpr int kInvLaunchesDefault = 4;     // inversion steps per batch in an iteration's launch sequence
 
 // Per-iteration device statistics block (zeroed by k_fill at iteration start).
 struct IterStats {
@@ -105,6 +106,7 @@ void launch_warp_pair_bf16(const GridDev& gl, const GridDev& gF, const uint16_t*
                            const GridDev& gM, const uint16_t* Ms, const float* uM, float* Mw, hipStream_t s);
 // float -> BF16 (round to nearest even).
 void launch_to_bf16(const float* in, uint16_t* out, int64_t n, hipStream_t s);
+void launch_copy_calibration(const void* in, void* out, int64_t n4, hipStream_t s);
 // Single image warp (u may be null).
 void launch_warp_image(const GridDev& gi, const float* img, const GridDev& go, const float* u, float* out,
                        hipStream_t s);
@@ -120,11 +122,21 @@ void launch_cc_direct(const GridDev& g, const float* F, const float* M, int radi
 void launch_compose(const GridDev& g, int np, const float* const* disp, const float* const* warp,
                     float* const* out, const unsigned int* maxbits, float rate, hipStream_t s);
 
+// Gate of an inversion launch: with step >= 0 the launch does nothing when
+// any of slots[0..n) computed a norm at that step, i.e. when the inversions
+// those slots track would still have run a further step.
+struct InvGate {
+  const RedSlot* slots[2];
+  int n;
+  int step;
+};
+constexpr InvGate kNoGate{{nullptr, nullptr}, 0, -1};
+
 // One fused fixed-point inversion step j (0..20) for np problems (see
 // kernels.hip); res[k] is a field-sized buffer holding the last residual.
 int64_t inv_partials_needed(const GridDev& g);
 void launch_inv_step(const GridDev& g, int np, const float* const* field, float* const* inv, float* const* res,
-                     RedSlot* const* slots, Partial* partials, int j, hipStream_t s);
+                     RedSlot* const* slots, Partial* partials, int j, hipStream_t s, const InvGate& gate = kNoGate);
 
 // Resample np fields from grid gi onto grid go (linear, 0 outside).
 void launch_resample_fields(const GridDev& gi, const GridDev& go, int np, const float* const* in,

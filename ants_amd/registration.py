@@ -227,7 +227,7 @@ class SyNRegistration:
         return {"levels_run": n, "iterations": list(s.iterations[:n]), "level_voxels": list(s.level_voxels[:n]),
                 "inv_steps": list(s.inv_steps[:n]), "voxel_iterations": s.voxel_iterations,
                 "algorithmic_bytes": s.algorithmic_bytes, "graph_launches": int(s.graph_launches),
-                "eager_iterations": int(s.eager_iterations)}
+                "eager_iterations": int(s.eager_iterations), "inv_tails": int(s.inv_tails)}
 
     def _callback(self, trace: list):
         def cb(user, level, it, e, conv, kin):

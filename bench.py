@@ -23,6 +23,9 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# kernel that carries each profiler class's work (the name rocprofv3 reports)
+CLASS_KERNEL = {"warp": "k_warp_pair", "cc": "k_cc3d", "smooth": "k_smooth3d", "compose": "k_compose",
+                "inv": "k_inv_step"}
 METRIC = "SyN CC voxel-iters/s @192x224x192 vs CPU; pairs/s + p50/p99 latency @QPS"
 SCHEDULE = dict(transform="SyN[0.1,3,0]", metric="CC[fixed,moving,1,4]", convergence="[100x70x20,1e-6,10]",
                 shrink="4x2x1", smoothing="2x1x0vox")
@@ -53,12 +56,31 @@ def cpu_baseline(pair, seconds_hint: float = 20.0):
             "seconds": dt}
 
 
+def measured_traffic(kernel: str):
+    """HBM bytes per dispatch of `kernel` from the committed counter summary
+    (two rocprofv3 passes, FETCH_SIZE and WRITE_SIZE, scaled by a copy kernel
+    of known size), or None when no summary exists for it."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", f"{kernel}_*pmc_traffic*.txt"))):
+        vals = {}
+        for ln in open(path):
+            if ":" in ln:
+                k, v = ln.split(":", 1)
+                vals[k.strip()] = v.strip()
+        if vals.get("kernel") == kernel and "traffic_bytes_per_launch" in vals:
+            return float(vals["traffic_bytes_per_launch"]), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def _gen_pair(args):
-    """Process-pool worker: one phantom pair (fields dropped to keep pickling small)."""
-    seed, size = args
+    """Process-pool worker: one phantom pair (fields dropped to keep pickling
+    small), optionally with a Voronoi label map of the moving phantom."""
+    seed, size = args[:2]
+    n_labels = args[2] if len(args) > 2 else 0
     from ants_amd import phantoms as ph
     p = ph.phantom_pair_3d(seed, size=size)
-    return p.fixed, p.moving, p.labels_moving
+    lab = ph.voronoi_labels(seed + 1, p.labels_moving > 0, n_labels) if n_labels else p.labels_moving
+    return p.fixed, p.moving, lab
 
 
 def _pool_pairs(specs):
@@ -82,13 +104,15 @@ def bench_batch(args):
     log(f"generated {n} pairs in {time.perf_counter() - t0:.1f} s")
     quick = SyNStage.from_args("SyN[0.1,3,0]", "CC[f,m,1,2]", "[40x20x0,1e-6,10]", "4x2x1", "2x1x0vox")
     full = SyNStage.from_args(**SCHEDULE)
-    reqs = [Request(quick if k < n // 4 else full, mov, fixed=fix) for k, (fix, mov, _) in enumerate(data)]
+    reqs = [Request(quick if k < n // 4 else full, mov, fixed=fix, precision=args.precision)
+            for k, (fix, mov, _) in enumerate(data)]
+    plain = [Request(r.stage, r.moving, fixed=r.fixed, precision="fp32") for r in reqs]
     out = {}
-    for name, slots in (("batched", args.slots), ("one_at_a_time", 1)):
+    for name, slots, rq in (("batched", args.slots, reqs), ("one_at_a_time", 1, plain)):
         b = Batch(slots=slots)
-        run_all(b, reqs[: min(4, n)])  # warm-up: graphs and buffers
+        run_all(b, rq[: min(4, n)])  # warm-up: graphs and buffers
         t = time.perf_counter()
-        res = run_all(b, reqs)
+        res = run_all(b, rq)
         wall = time.perf_counter() - t
         b.close()
         bad = [r.error for r in res if r.status != 0]
@@ -96,7 +120,7 @@ def bench_batch(args):
             raise RuntimeError(bad[0])
         vi = sum(r.voxel_iterations for r in res)
         by = sum(r.algorithmic_bytes for r in res)
-        out[name] = {"slots": slots, "wall_s": wall, "pairs_per_s": n / wall, "voxel_iters_per_s": vi / wall,
+        out[name] = {"slots": slots, "precision": rq[0].precision, "wall_s": wall, "pairs_per_s": n / wall, "voxel_iters_per_s": vi / wall,
                      "algorithmic_gbs": by / wall / 1e9, "hbm_frac": by / wall / 1e9 / HBM_PEAK_GBS}
         log(f"{name}: {n} pairs in {wall:.2f} s ({n / wall:.2f} pairs/s)")
     line = {"metric": "batched SyN pairs/s (64 mixed-size 3-D pairs)", "value": out["batched"]["pairs_per_s"],
@@ -122,32 +146,32 @@ def bench_serve(args):
     t0 = time.perf_counter()
     big_t = ph.phantom_pair_3d(5000, size=(192, 224, 192))
     small_t = ph.phantom_pair_3d(5001, size=(128, 128, 128))
-    specs = [(6000 + k, (128, 128, 128) if kinds[k] else (192, 224, 192)) for k in range(n)]
+    specs = [(6000 + k, (128, 128, 128) if kinds[k] else (192, 224, 192), 32) for k in range(n)]
     data = _pool_pairs(specs)
-    labels = []
-    for k, (fix, mov, lab) in enumerate(data):
-        mask = lab > 0
-        labels.append(ph.voronoi_labels(7000 + k, mask, 32))
+    labels = [lab for _, _, lab in data]
     log(f"generated templates and {n} requests in {time.perf_counter() - t0:.1f} s")
     quick = SyNStage.from_args("SyN[0.1,3,0]", "CC[f,m,1,2]", "[40x20x0,1e-6,10]", "4x2x1", "2x1x0vox")
     full = SyNStage.from_args(**SCHEDULE)
     reqs = [Request(quick if kinds[k] else full, data[k][1], fixed=None, template_index=1 if kinds[k] else 0,
-                    moving_labels=labels[k]) for k in range(n)]
+                    moving_labels=labels[k], precision=args.precision) for k in range(n)]
     arrivals = np.cumsum(rng.exponential(1.0 / args.qps, n))
     arrivals -= arrivals[0]
+    plain = [Request(r.stage, r.moving, fixed=None, template_index=r.template_index, moving_labels=r.moving_labels,
+                     precision="fp32") for r in reqs]
     out = {}
-    for name, slots in (("continuous_batching", args.slots), ("one_at_a_time", 1)):
+    for name, slots, rq in (("continuous_batching", args.slots, reqs), ("one_at_a_time", 1, plain)):
         b = Batch(slots=slots)
         b.set_template(0, big_t.fixed)
         b.set_template(1, small_t.fixed)
-        serve(b, reqs[:2], [0.0, 0.0])  # warm-up
-        res = serve(b, reqs, arrivals)
+        serve(b, rq[:2], [0.0, 0.0])  # warm-up
+        res = serve(b, rq, arrivals)
         b.close()
         bad = [r.error for r in res if r.status != 0]
         if bad:
             raise RuntimeError(bad[0])
         st = latency_stats(res)
         st["slots"] = slots
+        st["precision"] = rq[0].precision
         out[name] = st
         log(f"{name}: p50 {st['p50_s'] * 1e3:.1f} ms p99 {st['p99_s'] * 1e3:.1f} ms {st['pairs_per_s']:.2f} pairs/s")
     line = {"metric": "served SyN: pairs/s and p50/p99 latency at fixed QPS", "value": out["continuous_batching"]["pairs_per_s"],
@@ -170,8 +194,18 @@ def main():
     ap.add_argument("--pairs", type=int, default=64)
     ap.add_argument("--slots", type=int, default=8)
     ap.add_argument("--requests", type=int, default=40)
-    ap.add_argument("--qps", type=float, default=8.0)
+    ap.add_argument("--qps", type=float, default=30.0)
+    ap.add_argument("--precision", choices=["fp32", "bf16"], default="fp32",
+                    help="batch/serve modes: image pyramid precision of the batched form")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="pair mode: first copy MIB MiB three times with the calibration copy kernel "
+                         "(the known-size reference for HBM byte counters)")
+    ap.add_argument("--hw-queues", type=int, default=0,
+                    help="HIP hardware queues (0: the runtime's default of 4, which measured fastest for 8 slots)")
     args = ap.parse_args()
+    if args.hw_queues > 0 and "GPU_MAX_HW_QUEUES" not in os.environ:
+        # read by the HIP runtime at initialisation, before the first device call
+        os.environ["GPU_MAX_HW_QUEUES"] = str(min(args.hw_queues, 32))
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for {args.gpus}); "
               "pairs would shard across GPUs with no communication", file=sys.stderr)
@@ -196,6 +230,15 @@ def main():
     out_shape = pair.fixed.shape + (3,)
     d_fwd = L.DeviceArray(out_shape)
     d_inv = L.DeviceArray(out_shape)
+    if args.copy_calibration > 0:
+        nb = args.copy_calibration << 20
+        src, dst = L.DeviceArray((nb // 4,)).zero(), L.DeviceArray((nb // 4,))
+        for _ in range(3):
+            L.check(L.lib().ants_copy_calibration(dst.ptr, src.ptr, nb, None))
+        L.check(L.lib().ants_device_synchronize())
+        src.free()
+        dst.free()
+        log(f"calibration copy: 3 x {nb} bytes")
 
     # profiled run (all kernel classes) to name the dominant class
     reg.set_profiling(0x1F)
@@ -233,6 +276,7 @@ def main():
     bytes_per_launch = kacc["bytes"] / max(kacc["launches"], 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     whole_gbs = st["algorithmic_bytes"] / (total / args.steps) / 1e9
+    traffic, traffic_src = measured_traffic(CLASS_KERNEL[dominant])
 
     cpu = None
     if not args.no_cpu:
@@ -260,9 +304,10 @@ def main():
                    "parallelism": "single GPU"},
         "pairs_per_s": args.steps / total,
         "whole_job_algorithmic_gbs": whole_gbs,
-        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": bytes_per_launch},
+        "roofline": {"bound": "hbm", "kernel": CLASS_KERNEL[dominant], "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": traffic_src, "avg_launch_ms": avg_ms, "launches_per_step": kacc["launches"] / args.steps,
+                     "algorithmic_bytes_per_launch": bytes_per_launch},
         "cpu_baseline": cpu,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
         "kernel_breakdown_ms": {k: round(v["ms"], 3) for k, v in ks.items()},

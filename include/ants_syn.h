@@ -86,6 +86,9 @@ int ants_device_free(void* d_ptr);
 int ants_memcpy_h2d(void* d_dst, const void* host_src, size_t bytes, void* stream);
 int ants_memcpy_d2h(void* host_dst, const void* d_src, size_t bytes, void* stream);
 int ants_memset_device(void* d_dst, int value, size_t bytes, void* stream);
+/* A plain kernel copy of `bytes` (a multiple of 16) between device buffers:
+ * the known-size transfer that HBM byte counters are calibrated against. */
+int ants_copy_calibration(void* d_dst, const void* d_src, size_t bytes, void* stream);
 int ants_stream_create(void** stream);
 int ants_stream_destroy(void* stream);
 int ants_stream_synchronize(void* stream);
@@ -143,11 +146,14 @@ typedef struct ants_syn_stats {
   double algorithmic_bytes;             /* sum over iterations of N_l * B_vi */
   int64_t graph_launches;               /* iterations replayed from a captured hipGraph */
   int64_t eager_iterations;             /* iterations launched kernel by kernel */
+  int64_t inv_tails;                    /* iterations whose inversions ran past the steps launched with them */
 } ants_syn_stats;
 int ants_syn_get_stats(const ants_syn_handle* h, ants_syn_stats* out);
 
 /* Kernel classes timed with HIP events on the handle's stream when enabled
- * by ants_syn_set_profiling (bit k enables class k). */
+ * by ants_syn_set_profiling (bit k enables class k).  A profiled run launches
+ * its iterations kernel by kernel (no graph replay).  The inversion class is
+ * timed as one span per iteration over all its launches. */
 #define ANTS_KC_WARP 0     /* image warps through the inverse middle fields */
 #define ANTS_KC_CC 1       /* CC value + both derivatives */
 #define ANTS_KC_SMOOTH 2   /* update/total field Gaussian passes */

@@ -122,3 +122,31 @@ def test_full_size_properties():
     resid = np.sqrt((core ** 2).sum(-1))
     assert np.percentile(resid, 99) < 0.5 and resid.mean() < 0.1
     assert np.abs(res.fwd).max() < 20.0
+
+
+def test_inversion_launch_split_is_exact(monkeypatch):
+    """Launching only the first K inversion steps per batch with the
+    iteration's sequence, and finishing unconverged inversions from the host,
+    gives bit-identical results for every K (K=1 finishes every inversion
+    from the host, K=21 launches all steps), with and without graphs."""
+    pair = ph.phantom_pair_3d(5, size=(64, 72, 56))
+    stage = SyNStage.from_args("SyN[0.2,3,0]", "CC[f,m,1,3]", "[15x10x4,1e-6,10]", "4x2x1", "2x1x0vox")
+    outs = {}
+    for k, graphs in ((21, "1"), (1, "1"), (2, "1"), (4, "1"), (3, "0")):
+        monkeypatch.setenv("ANTS_SYN_INV_LAUNCHES", str(k))
+        monkeypatch.setenv("ANTS_SYN_GRAPHS", graphs)
+        reg = SyNRegistration(stage, 3)
+        res = reg.register(pair.fixed, pair.moving, pair.spacing, pair.origin, pair.spacing, pair.origin)
+        st = reg.stats()
+        reg.close()
+        outs[k] = (res, st)
+        print(f"K={k} graphs={graphs}: inversion tails {st['inv_tails']}, iterations {st['iterations']}", flush=True)
+    ref, st_ref = outs[21]
+    assert st_ref["inv_tails"] == 0
+    assert outs[1][1]["inv_tails"] == sum(st_ref["iterations"])
+    for k in (1, 2, 4, 3):
+        res, st = outs[k]
+        np.testing.assert_array_equal(res.energy, ref.energy)
+        np.testing.assert_array_equal(res.fwd, ref.fwd)
+        np.testing.assert_array_equal(res.inv, ref.inv)
+        assert st["inv_steps"] == st_ref["inv_steps"]

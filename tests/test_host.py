@@ -12,3 +12,37 @@ def test_jacobian_of_affine_field():
     u = pts @ A.T
     J = jacobian_determinant(u, (1.0, 1.0, 1.0))
     np.testing.assert_allclose(J, np.linalg.det(np.eye(3) + A), rtol=1e-12)
+
+
+def _counter_csv(path, counter, rows):
+    cols = ["Correlation_Id", "Dispatch_Id", "Agent_Id", "Grid_Size", "Kernel_Name", "Counter_Name", "Counter_Value"]
+    with open(path, "w") as fh:
+        fh.write(",".join(f'"{c}"' for c in cols) + "\n")
+        for did, name, grid, val in rows:
+            fh.write(f'{did},{did},1,{grid},"{name}","{counter}",{val}\n')
+
+
+def test_counter_traffic_scaling(tmp_path):
+    """Two counter passes combine into bytes per dispatch scaled by the copy
+    kernel: a copy of B bytes reading 2x its counter gives scale 1/2 in each
+    pass, so the kernel's KiB readings are halved."""
+    from ants_amd.profiling import traffic
+    B = 64 * 2**20
+    fetch, write = tmp_path / "f", tmp_path / "w"
+    fetch.mkdir()
+    write.mkdir()
+    copy_kib = B / 1024
+    _counter_csv(fetch / "run_counter_collection.csv", "FETCH_SIZE",
+                 [(1, "k_copy_calibration(float4 const*, float4*, long)", 1000, 2 * copy_kib),
+                  (2, "void (anonymous namespace)::k_inv_step<3>(GridDev)", 512, 400.0),
+                  (3, "void (anonymous namespace)::k_inv_step<3>(GridDev)", 512, 200.0),
+                  (4, "k_cc3d<4, 64, 4>", 64, 9999.0)])
+    _counter_csv(write / "run_counter_collection.csv", "WRITE_SIZE",
+                 [(1, "k_copy_calibration(float4 const*, float4*, long)", 1000, copy_kib),
+                  (2, "void (anonymous namespace)::k_inv_step<3>(GridDev)", 512, 100.0)])
+    out = dict(ln.split(": ", 1) for ln in traffic(str(fetch), str(write), "k_inv_step", B).splitlines()
+               if ln.startswith(("fetch_", "write_", "traffic_", "kernel")))
+    assert out["kernel"] == "k_inv_step"
+    assert float(out["fetch_bytes_per_launch"]) == 0.5 * 300.0 * 1024
+    assert float(out["write_bytes_per_launch"]) == 100.0 * 1024
+    assert float(out["traffic_bytes_per_launch"]) == 0.5 * 300.0 * 1024 + 100.0 * 1024
