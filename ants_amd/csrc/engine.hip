@@ -626,7 +626,7 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
     invp = std::max(invp, inv_partials_needed(to_dev(lg[l])));
   }
   ANTS_TRY(partials_cc.ensure(ccp * sizeof(Partial)));
-  ANTS_TRY(partials_inv.ensure(2 * invp * sizeof(Partial)));
+  ANTS_TRY(partials_inv.ensure(4 * invp * sizeof(Partial)));  // 2 step parities x 2 problems
   if (!host_stats) HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&host_stats), sizeof(IterStats), hipHostMallocDefault));
 
   hipStream_t s = stream;
@@ -1257,7 +1257,7 @@ int ants_op_invert(const ants_image_desc* gdsc, const float* d_field, const floa
   const size_t b = static_cast<size_t>(g.count()) * g.D * sizeof(float);
   OpScratch sc;
   ANTS_TRY(sc.stats.ensure(sizeof(RedSlot) * kInvSlots));
-  ANTS_TRY(sc.partials.ensure(inv_partials_needed(gd) * sizeof(Partial)));
+  ANTS_TRY(sc.partials.ensure(2 * inv_partials_needed(gd) * sizeof(Partial)));  // 2 step parities
   hipStream_t s = static_cast<hipStream_t>(stream);
   HIP_TRY(hipMemsetAsync(sc.stats.p, 0, sizeof(RedSlot) * kInvSlots, s));
   if (d_init)

@@ -132,7 +132,10 @@ __device__ __forceinline__ float interp_img(const GridDev& g, const T* __restric
 
 // Linear interpolation of NC interleaved components at an inside point:
 // neighbours clamped to the buffer (ITK Linear/VectorLinear interpolators).
-template <int D, int NC>
+// kClampAll also clamps neighbours above the buffer, so that the loads are
+// in bounds for any point and may be issued before the inside test; for an
+// inside point the result is the same.
+template <int D, int NC, bool kClampAll = false>
 __device__ __forceinline__ void interp(const GridDev& g, const float* __restrict__ img, const float c[3], float* v) {
   int lo[3], hi[3];
   float w[3];
@@ -144,6 +147,10 @@ __device__ __forceinline__ void interp(const GridDev& g, const float* __restrict
       float t = c[d] - f;
       if (b < 0) {
         b = 0;
+        t = 0.f;
+      }
+      if (kClampAll && b > g.n[d] - 1) {
+        b = g.n[d] - 1;
         t = 0.f;
       }
       lo[d] = b;
@@ -224,36 +231,49 @@ __device__ __forceinline__ void block_sum_max(double& s, float& m) {
   __syncthreads();
 }
 
-// Every block of the (sub)grid calls this once with its per-thread values.
-// Each block writes one (sum, max) partial; the last block to take a ticket
-// reduces all partials in block order, so the result does not depend on
-// scheduling.  Only the ticket is atomic (one per block).
+// Every block calls this once with its per-thread values and writes one
+// (sum, max) partial with a plain store; the kernel boundary publishes it.
 template <int NT>
-__device__ void grid_reduce(double s, float m, Partial* partials, RedSlot* slot, int nblocks, int bid) {
-  __shared__ int is_last;
+__device__ __forceinline__ void block_partial(double s, float m, Partial* out) {
   block_sum_max<NT>(s, m);
   if (threadIdx.x == 0) {
-    partials[bid].s = s;
-    partials[bid].m = m;
-    __threadfence();
-    const unsigned int t = atomicAdd(&slot->ticket, 1u);
-    is_last = (t == static_cast<unsigned int>(nblocks - 1));
+    out->s = s;
+    out->m = m;
+  }
+}
+
+// One block sums n partials: thread t takes t, t+NT, ... in order, then the
+// block tree of block_sum_max; every thread receives the result.  The order
+// is fixed, so every block that reduces the same partials gets the same bits.
+template <int NT>
+__device__ __forceinline__ void reduce_partials(const Partial* p, int n, double& s, float& m) {
+  __shared__ double r_s;
+  __shared__ float r_m;
+  double a = 0.0;
+  float b = 0.f;
+  for (int i = threadIdx.x; i < n; i += NT) {
+    a += p[i].s;
+    b = fmaxf(b, p[i].m);
+  }
+  block_sum_max<NT>(a, b);
+  if (threadIdx.x == 0) {
+    r_s = a;
+    r_m = b;
   }
   __syncthreads();
-  if (!is_last) return;
-  __threadfence();
-  double acc = 0.0;
-  float mx = 0.f;
-  for (int i = threadIdx.x; i < nblocks; i += NT) {
-    acc += __hip_atomic_load(&partials[i].s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    mx = fmaxf(mx, __hip_atomic_load(&partials[i].m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-  }
-  block_sum_max<NT>(acc, mx);
+  s = r_s;
+  m = r_m;
+}
+
+template <int NT>
+__global__ void __launch_bounds__(NT) k_reduce_slot(const Partial* p, int n, RedSlot* slot) {
+  double s;
+  float m;
+  reduce_partials<NT>(p, n, s, m);
   if (threadIdx.x == 0) {
-    slot->sum = acc;
-    slot->maxbits = __float_as_uint(mx);
+    slot->sum = s;
+    slot->maxbits = __float_as_uint(m);
     slot->computed = 1;
-    slot->ticket = 0u;
   }
 }
 
@@ -615,7 +635,7 @@ template <int D>
 __global__ void __launch_bounds__(kBlock) k_cc_direct(GridDev g, const float* __restrict__ F,
                                                       const float* __restrict__ M, int R, float eps,
                                                       float* __restrict__ g1, float* __restrict__ g2,
-                                                      Partial* partials, RedSlot* slot) {
+                                                      Partial* partials) {
   const int64_t N = g.count();
   double cc_sum = 0.0;
   for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < N;
@@ -657,7 +677,7 @@ __global__ void __launch_bounds__(kBlock) k_cc_direct(GridDev g, const float* __
       g2[i * D + d] = o.a2 * gf;
     }
   }
-  grid_reduce<kBlock>(cc_sum, 0.f, partials, slot, gridDim.x, blockIdx.x);
+  block_partial<kBlock>(cc_sum, 0.f, partials + blockIdx.x);
 }
 
 // 3-D CC with LDS-staged planes and a register ring along z.
@@ -670,7 +690,7 @@ __global__ void __launch_bounds__(kBlock) k_cc_direct(GridDev g, const float* __
 template <int R, int TX, int TY>
 __global__ void __launch_bounds__(TX* TY) k_cc3d(GridDev g, const float* __restrict__ F, const float* __restrict__ M,
                                                  float eps, int zchunk, float* __restrict__ g1,
-                                                 float* __restrict__ g2, Partial* partials, RedSlot* slot) {
+                                                 float* __restrict__ g2, Partial* partials) {
   constexpr int W = 2 * R + 1;
   constexpr int LX = TX + 2 * R, LY = TY + 2 * R;
   constexpr int NT = TX * TY;
@@ -825,9 +845,8 @@ __global__ void __launch_bounds__(TX* TY) k_cc3d(GridDev g, const float* __restr
       g2[i * 3 + 2] = o.a2 * gfz;
     }
   }
-  const int nb = gridDim.x * gridDim.y * gridDim.z;
   const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-  grid_reduce<NT>(cc_sum, 0.f, partials, slot, nb, bid);
+  block_partial<NT>(cc_sum, 0.f, partials + bid);
 }
 
 // ---------------------------------------------------------------------------
@@ -877,11 +896,12 @@ __device__ __forceinline__ float inv_residual(const GridDev& g, const float* __r
                                               const float iv[3], float e[3]) {
   float c[3], v[3] = {0.f, 0.f, 0.f};
   cidx_of<D>(g, x, y, z, iv, g, c);
-  if (inside<D>(g, c)) interp<D, D>(g, field, c, v);
+  const bool in = inside<D>(g, c);
+  interp<D, D, true>(g, field, c, v);
   float s2 = 0.f;
 #pragma unroll
   for (int d = 0; d < D; ++d) {
-    e[d] = iv[d] + v[d];
+    e[d] = iv[d] + (in ? v[d] : 0.f);
     const float q = e[d] / g.sp[d];
     s2 += q * q;
   }
@@ -903,11 +923,23 @@ __device__ __forceinline__ float resid_norm(const GridDev& g, const float e[3]) 
   return sqrtf(s2);
 }
 
-// Inversion steps run 1024-thread blocks over a small grid (kInvBlocks per
+// Inversion steps run large blocks over a bounded grid (kInvBlocks per
 // problem): steps that have converged cost one cheap dispatch, active steps
-// still fill the device, and the closing reduction reads one partial per thread.
-constexpr int kInvThreads = 1024;
-constexpr int kInvBlocks = 512;
+// still fill the device, and the closing reduction reads one partial per
+// block.  Each thread takes kInvUnroll voxels per round, one grid stride
+// apart, and issues all their loads and gathers before using them: the
+// gather depends on the loaded inverse, so a single voxel per round leaves
+// a wave with one dependent chain of memory round trips in flight.  The
+// per-thread sums run over the voxels in the same order for any unroll.
+#ifndef ANTS_INV_THREADS
+#define ANTS_INV_THREADS 512
+#endif
+#ifndef ANTS_INV_UNROLL
+#define ANTS_INV_UNROLL 2
+#endif
+constexpr int kInvThreads = ANTS_INV_THREADS;
+constexpr int kInvBlocks = 512 * 1024 / kInvThreads;
+constexpr int kInvUnroll = ANTS_INV_UNROLL;
 
 template <int D>
 __global__ void __launch_bounds__(kInvThreads) k_inv_step(GridDev g, PtrPack4 field, MutPack4 inv, MutPack4 res,
@@ -921,58 +953,83 @@ __global__ void __launch_bounds__(kInvThreads) k_inv_step(GridDev g, PtrPack4 fi
   }
   const int k = blockIdx.y;
   RedSlot* st = slots.p[k];
+  // partials of step jj: two parities x np problems x part_stride blocks
+  auto part = [&](int jj) { return partials + ((jj & 1) * static_cast<int64_t>(gridDim.y) + k) * part_stride; };
   bool do_update = false, do_norm = false;
   float maxprev = 0.f, eps = 0.f;
   if (j == 0) {
     do_norm = true;
-  } else {
-    const RedSlot& prev = st[j - 1];
-    if (prev.computed) {
-      do_update = true;
-      maxprev = __uint_as_float(prev.maxbits);
-      const double mean = prev.sum / static_cast<double>(g.count());
-      eps = (j - 1 == 0) ? 0.75f : 0.5f;
-      do_norm = j < kInvMaxIter && maxprev > kInvMaxTol && mean > 0.001;
+  } else if (st[j - 1].computed) {
+    // every block reduces the previous step's partials (same order, same bits)
+    double psum;
+    reduce_partials<kInvThreads>(part(j - 1), gridDim.x, psum, maxprev);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st[j - 1].sum = psum;
+      st[j - 1].maxbits = __float_as_uint(maxprev);
     }
+    do_update = true;
+    const double mean = psum / static_cast<double>(g.count());
+    eps = (j - 1 == 0) ? 0.75f : 0.5f;
+    do_norm = j < kInvMaxIter && maxprev > kInvMaxTol && mean > 0.001;
   }
   if (!do_update && !do_norm) return;
   const float* __restrict__ fld = field.p[k];
   float* __restrict__ iv = inv.p[k];
   float* __restrict__ rs = res.p[k];
   const int64_t N = g.count();
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  const float lim = eps * maxprev;
   double nsum = 0.0;
   float nmax = 0.f;
-  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < N;
-       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-    int x, y, z;
-    unlin(g, i, x, y, z);
-    float cur[3] = {0.f, 0.f, 0.f}, e[3] = {0.f, 0.f, 0.f};
+  for (int64_t i0 = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i0 < N; i0 += kInvUnroll * stride) {
+    float cur[kInvUnroll][3], e[kInvUnroll][3];
+    int xs[kInvUnroll], ys[kInvUnroll], zs[kInvUnroll];
+    bool ok[kInvUnroll];
 #pragma unroll
-    for (int d = 0; d < D; ++d) cur[d] = iv[i * D + d];
+    for (int u = 0; u < kInvUnroll; ++u) {
+      const int64_t i = i0 + u * stride;
+      ok[u] = i < N;
+      const int64_t ic = ok[u] ? i : N - 1;  // loads stay in bounds; stores are skipped
+      unlin(g, ic, xs[u], ys[u], zs[u]);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        cur[u][d] = d < D ? iv[ic * D + d] : 0.f;
+        e[u][d] = (do_update && d < D) ? rs[ic * D + d] : 0.f;
+      }
+    }
     if (do_update) {
 #pragma unroll
-      for (int d = 0; d < D; ++d) e[d] = rs[i * D + d];
-      const float nrm = resid_norm<D>(g, e);
-      const float lim = eps * maxprev;
-      float sc = 1.f;
-      if (nrm > lim) sc = lim / nrm;
-      const bool bnd = on_boundary(g, x, y, z);
+      for (int u = 0; u < kInvUnroll; ++u) {
+        const float nrm = resid_norm<D>(g, e[u]);
+        float sc = 1.f;
+        if (nrm > lim) sc = lim / nrm;
+        const bool bnd = on_boundary(g, xs[u], ys[u], zs[u]);
+        const int64_t i = i0 + u * stride;
 #pragma unroll
-      for (int d = 0; d < D; ++d) {
-        cur[d] = bnd ? 0.f : cur[d] + (-e[d] * sc) * eps;
-        iv[i * D + d] = cur[d];
+        for (int d = 0; d < D; ++d) {
+          cur[u][d] = bnd ? 0.f : cur[u][d] + (-e[u][d] * sc) * eps;
+          if (ok[u]) iv[i * D + d] = cur[u][d];
+        }
       }
     }
     if (do_norm) {
-      const float nrm = inv_residual<D>(g, fld, x, y, z, cur, e);
 #pragma unroll
-      for (int d = 0; d < D; ++d) rs[i * D + d] = e[d];
-      nsum += nrm;
-      nmax = fmaxf(nmax, nrm);
+      for (int u = 0; u < kInvUnroll; ++u) {
+        const float nrm = inv_residual<D>(g, fld, xs[u], ys[u], zs[u], cur[u], e[u]);
+        if (ok[u]) {
+          const int64_t i = i0 + u * stride;
+#pragma unroll
+          for (int d = 0; d < D; ++d) rs[i * D + d] = e[u][d];
+          nsum += nrm;
+          nmax = fmaxf(nmax, nrm);
+        }
+      }
     }
   }
-  if (do_norm)
-    grid_reduce<kInvThreads>(nsum, nmax, partials + k * part_stride, &st[j], gridDim.x, blockIdx.x);
+  if (do_norm) {
+    block_partial<kInvThreads>(nsum, nmax, part(j) + blockIdx.x);
+    if (blockIdx.x == 0 && threadIdx.x == 0) st[j].computed = 1;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1159,19 +1216,22 @@ void launch_cc(const GridDev& g, const float* F, const float* M, int radius, flo
   const dim3 grid((g.n[0] + kCCTX - 1) / kCCTX, (g.n[1] + kCCTY - 1) / kCCTY, (g.n[2] + zc - 1) / zc);
   if (radius == 4)
     hipLaunchKernelGGL((k_cc3d<4, kCCTX, kCCTY>), grid, dim3(kCCTX * kCCTY), 0, s, g, F, M, eps, zc, gF2M, gM2M,
-                       partials, slot);
+                       partials);
   else
     hipLaunchKernelGGL((k_cc3d<2, kCCTX, kCCTY>), grid, dim3(kCCTX * kCCTY), 0, s, g, F, M, eps, zc, gF2M, gM2M,
-                       partials, slot);
+                       partials);
+  hipLaunchKernelGGL(k_reduce_slot<kCCTX * kCCTY>, dim3(1), dim3(kCCTX * kCCTY), 0, s, partials,
+                     static_cast<int>(grid.x * grid.y * grid.z), slot);
 }
 
 void launch_cc_direct(const GridDev& g, const float* F, const float* M, int radius, float eps, float* gF2M,
                       float* gM2M, Partial* partials, RedSlot* slot, hipStream_t s) {
   const dim3 grid(red_blocks(g.count()));
   if (g.D == 2)
-    hipLaunchKernelGGL(k_cc_direct<2>, grid, dim3(kBlock), 0, s, g, F, M, radius, eps, gF2M, gM2M, partials, slot);
+    hipLaunchKernelGGL(k_cc_direct<2>, grid, dim3(kBlock), 0, s, g, F, M, radius, eps, gF2M, gM2M, partials);
   else
-    hipLaunchKernelGGL(k_cc_direct<3>, grid, dim3(kBlock), 0, s, g, F, M, radius, eps, gF2M, gM2M, partials, slot);
+    hipLaunchKernelGGL(k_cc_direct<3>, grid, dim3(kBlock), 0, s, g, F, M, radius, eps, gF2M, gM2M, partials);
+  hipLaunchKernelGGL(k_reduce_slot<kBlock>, dim3(1), dim3(kBlock), 0, s, partials, static_cast<int>(grid.x), slot);
 }
 
 void launch_compose(const GridDev& g, int np, const float* const* disp, const float* const* warp, float* const* out,

@@ -42,13 +42,14 @@ struct Taps {
   int r;
 };
 
-// Deterministic device reduction slot: per-block partials are summed in block
-// order by the last block to finish (ticket), max is an atomic on the bits of
-// a non-negative float.
+// Deterministic device reduction slot.  Blocks write plain per-block
+// partials; a later launch (the next inversion step, or a one-block reduce
+// kernel) sums them in block order, so the result does not depend on
+// scheduling and no block waits on a device-wide fence or atomic.
 struct RedSlot {
   double sum;
   unsigned int maxbits;
-  unsigned int ticket;
+  unsigned int pad0;
   int computed;
   int pad;
 };
@@ -133,7 +134,9 @@ struct InvGate {
 constexpr InvGate kNoGate{{nullptr, nullptr}, 0, -1};
 
 // One fused fixed-point inversion step j (0..20) for np problems (see
-// kernels.hip); res[k] is a field-sized buffer holding the last residual.
+// kernels.hip); res[k] is a field-sized buffer holding the last residual;
+// partials holds 2 x np x inv_partials_needed(g) entries (the partials of
+// step j are read by step j + 1).
 int64_t inv_partials_needed(const GridDev& g);
 void launch_inv_step(const GridDev& g, int np, const float* const* field, float* const* inv, float* const* res,
                      RedSlot* const* slots, Partial* partials, int j, hipStream_t s, const InvGate& gate = kNoGate);
