@@ -412,12 +412,31 @@ __global__ void __launch_bounds__(kBlock) k_gauss_field_final(GridDev g, PtrPack
 // Fused 3-D separable Gaussian of 3-vector fields (SyN field smoothing).
 // Block = TX x TY output columns over a z-chunk of one field (blockIdx.z =
 // field * nzc + chunk).  Each plane (edges replicated, ZeroFluxNeumann) is
-// staged once in LDS, convolved along x and y from LDS, and pushed into a
-// (2R+1)-deep register ring that forms the z pass.  The epilogue zeroes the
-// boundary, blends with the input when w2 != 0, and takes the max norm.
-// The next plane is prefetched into registers while the current one is used.
+// staged once in LDS, convolved along x (four outputs per task from aligned
+// 16-byte LDS reads, so each input is read once per four outputs) and along
+// y from LDS, and pushed into a (2R+1)-deep register ring that forms the z
+// pass.  The epilogue zeroes the boundary, blends with the input when
+// w2 != 0, and takes the max norm.  The next planes are prefetched into
+// registers while the current one is used.
+#ifndef ANTS_SMOOTH_PREFETCH
+#define ANTS_SMOOTH_PREFETCH 1
+#endif
+constexpr int kSmoothPrefetch = ANTS_SMOOTH_PREFETCH;
+// Tile of the fused smoothing: 16 x 32 outputs per plane measured fastest
+// over the three pyramid levels of a 192x224x192 registration.
+#ifndef ANTS_SMOOTH_TX
+#define ANTS_SMOOTH_TX 16
+#endif
+#ifndef ANTS_SMOOTH_TY
+#define ANTS_SMOOTH_TY 32
+#endif
+constexpr int kSmoothTX = ANTS_SMOOTH_TX, kSmoothTY = ANTS_SMOOTH_TY;
+#ifndef ANTS_SMOOTH_WAVES
+#define ANTS_SMOOTH_WAVES 1
+#endif
+constexpr int kSmoothWavesPerEU = ANTS_SMOOTH_WAVES;  // occupancy floor: caps registers
 template <int R, int TX, int TY>
-__global__ void __launch_bounds__(TX* TY) k_smooth3d(GridDev g, PtrPack4 in, PtrPack4 orig, MutPack4 out, Taps t,
+__global__ void __launch_bounds__(TX* TY, kSmoothWavesPerEU) k_smooth3d(GridDev g, PtrPack4 in, PtrPack4 orig, MutPack4 out, Taps t,
                                                      int zchunk, int nzc, float w1, float w2, int boundary,
                                                      unsigned int* maxbits) {
   constexpr int W = 2 * R + 1;
@@ -425,8 +444,8 @@ __global__ void __launch_bounds__(TX* TY) k_smooth3d(GridDev g, PtrPack4 in, Ptr
   constexpr int NT = TX * TY;
   constexpr int NE = 3 * LX * LY;
   constexpr int KP = (NE + NT - 1) / NT;
-  __shared__ float tile[3][LY][LX];
-  __shared__ float xs[3][LY][TX];
+  __shared__ __align__(16) float tile[3][LY][LX];
+  __shared__ __align__(16) float xs[3][LY][TX];
   const int f = blockIdx.z / nzc, zb = blockIdx.z - f * nzc;
   const float* __restrict__ src = in.p[f];
   const float* __restrict__ org = orig.p[f];
@@ -446,8 +465,12 @@ __global__ void __launch_bounds__(TX* TY) k_smooth3d(GridDev g, PtrPack4 in, Ptr
   for (int a = 0; a < 3; ++a)
 #pragma unroll
     for (int b = 0; b < W; ++b) ring[a][b] = 0.f;
-  float pv[KP];
-  auto fetch = [&](int zq) {
+  // planes are fetched kSmoothPrefetch ahead into registers: slot
+  // (zz - (z0 - R)) % P holds plane zz until it is stored to LDS, and is
+  // then refilled with plane zz + P
+  constexpr int P = kSmoothPrefetch;
+  float pv[P][KP];
+  auto fetch = [&](float* dstv, int zq) {
     const int zs = zq < 0 ? 0 : (zq >= nz ? nz - 1 : zq);
 #pragma unroll
     for (int u = 0; u < KP; ++u) {
@@ -461,12 +484,19 @@ __global__ void __launch_bounds__(TX* TY) k_smooth3d(GridDev g, PtrPack4 in, Ptr
         gy = gy < 0 ? 0 : (gy >= ny ? ny - 1 : gy);
         v = src[(static_cast<int64_t>(zs) * plane + static_cast<int64_t>(gy) * nx + gx) * 3 + cc];
       }
-      pv[u] = v;
+      dstv[u] = v;
     }
   };
-  fetch(z0 - R);
+  const int zend = z1 + R;
+#pragma unroll
+  for (int qs = 0; qs < P; ++qs)
+    if (z0 - R + qs < zend) fetch(pv[qs], z0 - R + qs);
   float nrm = 0.f;
-  for (int zz = z0 - R; zz < z1 + R; ++zz) {
+  for (int zb0 = z0 - R; zb0 < zend; zb0 += P) {
+#pragma unroll
+  for (int qs = 0; qs < P; ++qs) {
+    const int zz = zb0 + qs;
+    if (zz >= zend) break;
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < KP; ++u) {
@@ -474,18 +504,37 @@ __global__ void __launch_bounds__(TX* TY) k_smooth3d(GridDev g, PtrPack4 in, Ptr
       if (e < NE) {
         const int ly = e / (3 * LX), r = e - ly * (3 * LX);
         const int lx = r / 3, cc = r - lx * 3;
-        tile[cc][ly][lx] = pv[u];
+        tile[cc][ly][lx] = pv[qs][u];
       }
     }
     __syncthreads();
-    if (zz + 1 < z1 + R) fetch(zz + 1);
-    for (int ly = ty; ly < LY; ly += TY) {
+    if (zz + P < zend) fetch(pv[qs], zz + P);
+    // x pass, four outputs per task from aligned 16-byte LDS reads: task
+    // (ly, q) reads tile[a][ly][4q .. 4q+15] and writes xs[a][ly][4q .. 4q+3]
+    static_assert(TX % 4 == 0 && LX % 4 == 0 && TX + 12 <= LX && R <= 6, "x tiling: reads reach 4q+15 < LX");
+    for (int task = threadIdx.x; task < LY * (TX / 4); task += NT) {
+      const int ly = task / (TX / 4), q = task - ly * (TX / 4);
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
-        float acc = c[0] * tile[a][ly][tx + R];
+        float w[16];
 #pragma unroll
-        for (int b = 1; b <= R; ++b) acc += c[b] * (tile[a][ly][tx + R - b] + tile[a][ly][tx + R + b]);
-        xs[a][ly][tx] = acc;
+        for (int h = 0; h < 4; ++h) {
+          const float4 v4 = (4 * q + 4 * h < LX) ? *reinterpret_cast<const float4*>(&tile[a][ly][4 * q + 4 * h])
+                                                  : make_float4(0.f, 0.f, 0.f, 0.f);
+          w[4 * h + 0] = v4.x;
+          w[4 * h + 1] = v4.y;
+          w[4 * h + 2] = v4.z;
+          w[4 * h + 3] = v4.w;
+        }
+        float o[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          float acc = c[0] * w[e + R];
+#pragma unroll
+          for (int b = 1; b <= R; ++b) acc += c[b] * (w[e + R - b] + w[e + R + b]);
+          o[e] = acc;
+        }
+        *reinterpret_cast<float4*>(&xs[a][ly][4 * q]) = make_float4(o[0], o[1], o[2], o[3]);
       }
     }
     __syncthreads();
@@ -520,6 +569,7 @@ __global__ void __launch_bounds__(TX* TY) k_smooth3d(GridDev g, PtrPack4 in, Ptr
       }
       nrm = fmaxf(nrm, sqrtf(s2));
     }
+  }
   }
   double dummy = 0.0;
   block_sum_max<NT>(dummy, nrm);
@@ -1099,10 +1149,27 @@ MutPack4 mpack(float* const* p, int np) {
 // CC tile shapes (3-D): TX x TY columns; the z-chunk depth is chosen per grid
 // so that a launch has about 1024 blocks (between 8 and 32 planes).
 constexpr int kCCTX = 64, kCCTY = 4;
+// z-chunk of the tiled CC and smoothing kernels: aim for about
+// k*_TARGET_BLOCKS blocks, within [ZMIN, ZMAX] planes per block (each block
+// also reads 2R halo planes, so short chunks trade redundant reads for
+// parallelism on the small pyramid levels).
+#ifndef ANTS_CC_TARGET_BLOCKS
+#define ANTS_CC_TARGET_BLOCKS 1024
+#endif
+#ifndef ANTS_CC_ZMIN
+#define ANTS_CC_ZMIN 8
+#endif
+#ifndef ANTS_SMOOTH_TARGET_BLOCKS
+#define ANTS_SMOOTH_TARGET_BLOCKS 1024
+#endif
+#ifndef ANTS_SMOOTH_ZMIN
+#define ANTS_SMOOTH_ZMIN 16
+#endif
+
 int cc_zchunk(const GridDev& g) {
   const int64_t tiles = static_cast<int64_t>((g.n[0] + kCCTX - 1) / kCCTX) * ((g.n[1] + kCCTY - 1) / kCCTY);
-  int64_t zc = (static_cast<int64_t>(g.n[2]) * tiles) / 1024;
-  zc = zc < 8 ? 8 : (zc > 32 ? 32 : zc);
+  int64_t zc = (static_cast<int64_t>(g.n[2]) * tiles) / ANTS_CC_TARGET_BLOCKS;
+  zc = zc < ANTS_CC_ZMIN ? ANTS_CC_ZMIN : (zc > 32 ? 32 : zc);
   return static_cast<int>(zc);
 }
 
@@ -1147,11 +1214,11 @@ void launch_gauss_field_final(const GridDev& g, int np, const float* const* in, 
 
 bool launch_smooth3d(const GridDev& g, int np, const float* const* in, const float* const* orig, float* const* out,
                      const Taps& t, float w1, float w2, bool boundary, unsigned int* maxbits, hipStream_t s) {
-  constexpr int R = 6, TX = 64, TY = 8;
+  constexpr int R = 6, TX = kSmoothTX, TY = kSmoothTY;
   if (g.D != 3 || t.r != R || np > 4) return false;
   const int64_t tiles = static_cast<int64_t>((g.n[0] + TX - 1) / TX) * ((g.n[1] + TY - 1) / TY) * np;
-  int64_t zc = (static_cast<int64_t>(g.n[2]) * tiles) / 1024;
-  zc = zc < 16 ? 16 : (zc > 64 ? 64 : zc);
+  int64_t zc = (static_cast<int64_t>(g.n[2]) * tiles) / ANTS_SMOOTH_TARGET_BLOCKS;
+  zc = zc < ANTS_SMOOTH_ZMIN ? ANTS_SMOOTH_ZMIN : (zc > 64 ? 64 : zc);
   const int nzc = static_cast<int>((g.n[2] + zc - 1) / zc);
   const dim3 grid((g.n[0] + TX - 1) / TX, (g.n[1] + TY - 1) / TY, nzc * np);
   hipLaunchKernelGGL((k_smooth3d<R, TX, TY>), grid, dim3(TX * TY), 0, s, g, pack(in, np), pack(orig, np),
