@@ -990,6 +990,10 @@ __device__ __forceinline__ float resid_norm(const GridDev& g, const float e[3]) 
 constexpr int kInvThreads = ANTS_INV_THREADS;
 constexpr int kInvBlocks = 512 * 1024 / kInvThreads;
 constexpr int kInvUnroll = ANTS_INV_UNROLL;
+#ifndef ANTS_INV_STORE_RES
+#define ANTS_INV_STORE_RES 1
+#endif
+constexpr bool kInvStoreResidual = ANTS_INV_STORE_RES != 0;
 
 template <int D>
 __global__ void __launch_bounds__(kInvThreads) k_inv_step(GridDev g, PtrPack4 field, MutPack4 inv, MutPack4 res,
@@ -1044,13 +1048,16 @@ __global__ void __launch_bounds__(kInvThreads) k_inv_step(GridDev g, PtrPack4 fi
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
         cur[u][d] = d < D ? iv[ic * D + d] : 0.f;
-        e[u][d] = (do_update && d < D) ? rs[ic * D + d] : 0.f;
+        e[u][d] = (kInvStoreResidual && do_update && d < D) ? rs[ic * D + d] : 0.f;
       }
     }
     if (do_update) {
 #pragma unroll
       for (int u = 0; u < kInvUnroll; ++u) {
-        const float nrm = resid_norm<D>(g, e[u]);
+        // e_{j-1}: stored by the previous step, or recomputed from the same
+        // inverse and field with the same arithmetic (bitwise equal)
+        const float nrm = kInvStoreResidual ? resid_norm<D>(g, e[u])
+                                            : inv_residual<D>(g, fld, xs[u], ys[u], zs[u], cur[u], e[u]);
         float sc = 1.f;
         if (nrm > lim) sc = lim / nrm;
         const bool bnd = on_boundary(g, xs[u], ys[u], zs[u]);
@@ -1067,9 +1074,11 @@ __global__ void __launch_bounds__(kInvThreads) k_inv_step(GridDev g, PtrPack4 fi
       for (int u = 0; u < kInvUnroll; ++u) {
         const float nrm = inv_residual<D>(g, fld, xs[u], ys[u], zs[u], cur[u], e[u]);
         if (ok[u]) {
-          const int64_t i = i0 + u * stride;
+          if (kInvStoreResidual) {
+            const int64_t i = i0 + u * stride;
 #pragma unroll
-          for (int d = 0; d < D; ++d) rs[i * D + d] = e[u][d];
+            for (int d = 0; d < D; ++d) rs[i * D + d] = e[u][d];
+          }
           nsum += nrm;
           nmax = fmaxf(nmax, nrm);
         }
@@ -1148,7 +1157,16 @@ MutPack4 mpack(float* const* p, int np) {
 
 // CC tile shapes (3-D): TX x TY columns; the z-chunk depth is chosen per grid
 // so that a launch has about 1024 blocks (between 8 and 32 planes).
-constexpr int kCCTX = 64, kCCTY = 4;
+// Tile of the 3-D CC kernel: 16 x 16 columns per block measured fastest over
+// the pyramid levels of a 192x224x192 registration (the x box sums are
+// computed for TY + 2R rows per TY output rows, so square tiles waste less).
+#ifndef ANTS_CC_TX
+#define ANTS_CC_TX 16
+#endif
+#ifndef ANTS_CC_TY
+#define ANTS_CC_TY 16
+#endif
+constexpr int kCCTX = ANTS_CC_TX, kCCTY = ANTS_CC_TY;
 // z-chunk of the tiled CC and smoothing kernels: aim for about
 // k*_TARGET_BLOCKS blocks, within [ZMIN, ZMAX] planes per block (each block
 // also reads 2R halo planes, so short chunks trade redundant reads for
