@@ -431,10 +431,12 @@ constexpr int kSmoothPrefetch = ANTS_SMOOTH_PREFETCH;
 #define ANTS_SMOOTH_TY 32
 #endif
 constexpr int kSmoothTX = ANTS_SMOOTH_TX, kSmoothTY = ANTS_SMOOTH_TY;
+// Occupancy floor of the fused smoothing (waves per SIMD): 4 keeps two
+// 512-thread blocks per CU, which caps the kernel at 128 VGPRs.
 #ifndef ANTS_SMOOTH_WAVES
-#define ANTS_SMOOTH_WAVES 1
+#define ANTS_SMOOTH_WAVES 4
 #endif
-constexpr int kSmoothWavesPerEU = ANTS_SMOOTH_WAVES;  // occupancy floor: caps registers
+constexpr int kSmoothWavesPerEU = ANTS_SMOOTH_WAVES;
 template <int R, int TX, int TY>
 __global__ void __launch_bounds__(TX* TY, kSmoothWavesPerEU) k_smooth3d(GridDev g, PtrPack4 in, PtrPack4 orig, MutPack4 out, Taps t,
                                                      int zchunk, int nzc, float w1, float w2, int boundary,
@@ -470,22 +472,31 @@ __global__ void __launch_bounds__(TX* TY, kSmoothWavesPerEU) k_smooth3d(GridDev 
   // then refilled with plane zz + P
   constexpr int P = kSmoothPrefetch;
   float pv[P][KP];
+  // Loop-invariant part of each staged element's addresses: the in-plane
+  // source offset (edges replicated) and the LDS slot; -1 marks the unused
+  // tail of the last round.
+  int goff[KP], loff[KP];
+#pragma unroll
+  for (int u = 0; u < KP; ++u) {
+    const int e = threadIdx.x + u * NT;
+    goff[u] = -1;
+    loff[u] = 0;
+    if (e < NE) {
+      const int ly = e / (3 * LX), r = e - ly * (3 * LX);
+      const int lx = r / 3, cc = r - lx * 3;
+      int gx = x0 - R + lx, gy = y0 - R + ly;
+      gx = gx < 0 ? 0 : (gx >= nx ? nx - 1 : gx);
+      gy = gy < 0 ? 0 : (gy >= ny ? ny - 1 : gy);
+      goff[u] = (gy * nx + gx) * 3 + cc;
+      loff[u] = (cc * LY + ly) * LX + lx;
+    }
+  }
+  float* tile_flat = &tile[0][0][0];
   auto fetch = [&](float* dstv, int zq) {
     const int zs = zq < 0 ? 0 : (zq >= nz ? nz - 1 : zq);
+    const float* __restrict__ srcz = src + static_cast<int64_t>(zs) * plane * 3;
 #pragma unroll
-    for (int u = 0; u < KP; ++u) {
-      const int e = threadIdx.x + u * NT;
-      float v = 0.f;
-      if (e < NE) {
-        const int ly = e / (3 * LX), r = e - ly * (3 * LX);
-        const int lx = r / 3, cc = r - lx * 3;
-        int gx = x0 - R + lx, gy = y0 - R + ly;
-        gx = gx < 0 ? 0 : (gx >= nx ? nx - 1 : gx);
-        gy = gy < 0 ? 0 : (gy >= ny ? ny - 1 : gy);
-        v = src[(static_cast<int64_t>(zs) * plane + static_cast<int64_t>(gy) * nx + gx) * 3 + cc];
-      }
-      dstv[u] = v;
-    }
+    for (int u = 0; u < KP; ++u) dstv[u] = goff[u] >= 0 ? srcz[goff[u]] : 0.f;
   };
   const int zend = z1 + R;
 #pragma unroll
@@ -499,14 +510,8 @@ __global__ void __launch_bounds__(TX* TY, kSmoothWavesPerEU) k_smooth3d(GridDev 
     if (zz >= zend) break;
     __syncthreads();
 #pragma unroll
-    for (int u = 0; u < KP; ++u) {
-      const int e = threadIdx.x + u * NT;
-      if (e < NE) {
-        const int ly = e / (3 * LX), r = e - ly * (3 * LX);
-        const int lx = r / 3, cc = r - lx * 3;
-        tile[cc][ly][lx] = pv[qs][u];
-      }
-    }
+    for (int u = 0; u < KP; ++u)
+      if (goff[u] >= 0) tile_flat[loff[u]] = pv[qs][u];
     __syncthreads();
     if (zz + P < zend) fetch(pv[qs], zz + P);
     // x pass, four outputs per task from aligned 16-byte LDS reads: task
@@ -737,8 +742,12 @@ __global__ void __launch_bounds__(kBlock) k_cc_direct(GridDev g, const float* __
 // double, and pushes them into a (2R+1)-deep register ring; the ring total
 // is the window sum for the voxel R planes behind.  Outside-volume samples
 // are 0 and the count is analytic, which equals ITK's in-bounds-only sums.
+#ifndef ANTS_CC_WAVES
+#define ANTS_CC_WAVES 1
+#endif
+constexpr int kCCWavesPerEU = ANTS_CC_WAVES;  // occupancy floor (waves per SIMD)
 template <int R, int TX, int TY>
-__global__ void __launch_bounds__(TX* TY) k_cc3d(GridDev g, const float* __restrict__ F, const float* __restrict__ M,
+__global__ void __launch_bounds__(TX* TY, kCCWavesPerEU) k_cc3d(GridDev g, const float* __restrict__ F, const float* __restrict__ M,
                                                  float eps, int zchunk, float* __restrict__ g1,
                                                  float* __restrict__ g2, Partial* partials) {
   constexpr int W = 2 * R + 1;
@@ -773,18 +782,26 @@ __global__ void __launch_bounds__(TX* TY) k_cc3d(GridDev g, const float* __restr
   // global-load latency overlaps the box sums of the current plane.
   constexpr int KP = (LX * LY + NT - 1) / NT;
   float pf[KP], pm[KP];
+  // loop-invariant in-plane source offsets (-1: outside the image, reads 0)
+  // and LDS slots (-1: unused tail of the last round)
+  int goff[KP], loff[KP];
+#pragma unroll
+  for (int u = 0; u < KP; ++u) {
+    const int k = threadIdx.x + u * NT;
+    const int ly = k / LX, lx = k - ly * LX;
+    const int gx = x0 - R + lx, gy = y0 - R + ly;
+    loff[u] = k < LX * LY ? k : -1;
+    goff[u] = (k < LX * LY && gx >= 0 && gx < nx && gy >= 0 && gy < ny) ? gy * nx + gx : -1;
+  }
   auto fetch = [&](int zq) {
     const bool zin_q = zq >= 0 && zq < nz;
+    const int64_t zo = static_cast<int64_t>(zq) * plane;
 #pragma unroll
     for (int u = 0; u < KP; ++u) {
-      const int k = threadIdx.x + u * NT;
-      const int ly = k / LX, lx = k - ly * LX;
-      const int gx = x0 - R + lx, gy = y0 - R + ly;
       float f = 0.f, m = 0.f;
-      if (k < LX * LY && zin_q && gx >= 0 && gx < nx && gy >= 0 && gy < ny) {
-        const int64_t j = static_cast<int64_t>(zq) * plane + static_cast<int64_t>(gy) * nx + gx;
-        f = F[j];
-        m = M[j];
+      if (zin_q && goff[u] >= 0) {
+        f = F[zo + goff[u]];
+        m = M[zo + goff[u]];
       }
       pf[u] = f;
       pm[u] = m;
@@ -797,11 +814,9 @@ __global__ void __launch_bounds__(TX* TY) k_cc3d(GridDev g, const float* __restr
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < KP; ++u) {
-      const int k = threadIdx.x + u * NT;
-      if (k < LX * LY) {
-        const int ly = k / LX, lx = k - ly * LX;
-        q[0][ly][lx] = pf[u];
-        q[1][ly][lx] = pm[u];
+      if (loff[u] >= 0) {
+        (&q[0][0][0])[loff[u]] = pf[u];
+        (&q[1][0][0])[loff[u]] = pm[u];
       }
     }
     __syncthreads();
@@ -1234,6 +1249,7 @@ bool launch_smooth3d(const GridDev& g, int np, const float* const* in, const flo
                      const Taps& t, float w1, float w2, bool boundary, unsigned int* maxbits, hipStream_t s) {
   constexpr int R = 6, TX = kSmoothTX, TY = kSmoothTY;
   if (g.D != 3 || t.r != R || np > 4) return false;
+  if (static_cast<int64_t>(g.n[0]) * g.n[1] * 3 >= (int64_t{1} << 31)) return false;  // 32-bit in-plane offsets
   const int64_t tiles = static_cast<int64_t>((g.n[0] + TX - 1) / TX) * ((g.n[1] + TY - 1) / TY) * np;
   int64_t zc = (static_cast<int64_t>(g.n[2]) * tiles) / ANTS_SMOOTH_TARGET_BLOCKS;
   zc = zc < ANTS_SMOOTH_ZMIN ? ANTS_SMOOTH_ZMIN : (zc > 64 ? 64 : zc);
@@ -1280,7 +1296,10 @@ void launch_warp_image(const GridDev& gi, const float* img, const GridDev& go, c
     hipLaunchKernelGGL(k_warp_image<3>, grid, dim3(kBlock), 0, s, gi, img, go, u, out);
 }
 
-static bool cc_tiled(const GridDev& g, int radius) { return g.D == 3 && (radius == 2 || radius == 4); }
+static bool cc_tiled(const GridDev& g, int radius) {
+  // the tiled kernel keeps 32-bit in-plane offsets
+  return g.D == 3 && (radius == 2 || radius == 4) && static_cast<int64_t>(g.n[0]) * g.n[1] < (int64_t{1} << 31);
+}
 
 int64_t cc_partials_needed(const GridDev& g, int radius) {
   if (cc_tiled(g, radius)) {
