@@ -172,6 +172,7 @@ GridDev to_dev(const HostGrid& h) {
   for (int k = 0; k < 3; ++k) {
     g.n[k] = static_cast<int>(h.n[k]);
     g.sp[k] = static_cast<float>(h.sp[k]);
+    g.isp[k] = h.sp[k] > 0.0 ? static_cast<float>(1.0 / h.sp[k]) : 0.f;
     g.org[k] = static_cast<float>(h.org[k]);
     g.dsp[k] = h.sp[k];
     g.dorg[k] = h.org[k];

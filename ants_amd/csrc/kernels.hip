@@ -61,7 +61,7 @@ __device__ __forceinline__ void cidx_of(const GridDev& go, int x, int y, int z, 
     if (d < D) {
       float p = go.org[d] + static_cast<float>(ix[d]) * go.sp[d];
       if (u) p += u[d];
-      c[d] = (p - gi.org[d]) / gi.sp[d];
+      c[d] = (p - gi.org[d]) * gi.isp[d];
     } else {
       c[d] = 0.f;
     }
@@ -135,7 +135,9 @@ __device__ __forceinline__ float interp_img(const GridDev& g, const T* __restric
 // kClampAll also clamps neighbours above the buffer, so that the loads are
 // in bounds for any point and may be issued before the inside test; for an
 // inside point the result is the same.
-template <int D, int NC, bool kClampAll = false>
+// IT is the element-index type: int when the buffer has fewer than 2^31
+// elements (cheaper address arithmetic), int64_t otherwise.
+template <int D, int NC, bool kClampAll = false, class IT = int64_t>
 __device__ __forceinline__ void interp(const GridDev& g, const float* __restrict__ img, const float c[3], float* v) {
   int lo[3], hi[3];
   float w[3];
@@ -163,10 +165,10 @@ __device__ __forceinline__ void interp(const GridDev& g, const float* __restrict
   }
 #pragma unroll
   for (int k = 0; k < NC; ++k) v[k] = 0.f;
-  const int64_t sy = g.n[0], sz = static_cast<int64_t>(g.n[0]) * g.n[1];
+  const IT sy = g.n[0], sz = static_cast<IT>(g.n[0]) * g.n[1];
   if (D == 2) {
-    const int64_t i00 = lo[0] + sy * lo[1], i10 = hi[0] + sy * lo[1];
-    const int64_t i01 = lo[0] + sy * hi[1], i11 = hi[0] + sy * hi[1];
+    const IT i00 = lo[0] + sy * lo[1], i10 = hi[0] + sy * lo[1];
+    const IT i01 = lo[0] + sy * hi[1], i11 = hi[0] + sy * hi[1];
     const float wx = w[0], wy = w[1];
 #pragma unroll
     for (int k = 0; k < NC; ++k) {
@@ -175,9 +177,9 @@ __device__ __forceinline__ void interp(const GridDev& g, const float* __restrict
       v[k] = a + (b - a) * wy;
     }
   } else {
-    const int64_t z0 = sz * lo[2], z1 = sz * hi[2], y0 = sy * lo[1], y1 = sy * hi[1];
-    const int64_t i000 = lo[0] + y0 + z0, i100 = hi[0] + y0 + z0, i010 = lo[0] + y1 + z0, i110 = hi[0] + y1 + z0;
-    const int64_t i001 = lo[0] + y0 + z1, i101 = hi[0] + y0 + z1, i011 = lo[0] + y1 + z1, i111 = hi[0] + y1 + z1;
+    const IT z0 = sz * lo[2], z1 = sz * hi[2], y0 = sy * lo[1], y1 = sy * hi[1];
+    const IT i000 = lo[0] + y0 + z0, i100 = hi[0] + y0 + z0, i010 = lo[0] + y1 + z0, i110 = hi[0] + y1 + z0;
+    const IT i001 = lo[0] + y0 + z1, i101 = hi[0] + y0 + z1, i011 = lo[0] + y1 + z1, i111 = hi[0] + y1 + z1;
     const float wx = w[0], wy = w[1], wz = w[2];
 #pragma unroll
     for (int k = 0; k < NC; ++k) {
@@ -956,18 +958,18 @@ __global__ void __launch_bounds__(kBlock) k_compose(GridDev g, PtrPack4 disp, Pt
 // sits exactly on the 0.1 tolerance; the guard resolves that tie the same way
 // in float and double.
 constexpr float kInvMaxTol = 0.1f * (1.0f + 1e-5f);
-template <int D>
+template <int D, class IT = int64_t>
 __device__ __forceinline__ float inv_residual(const GridDev& g, const float* __restrict__ field, int x, int y, int z,
                                               const float iv[3], float e[3]) {
   float c[3], v[3] = {0.f, 0.f, 0.f};
   cidx_of<D>(g, x, y, z, iv, g, c);
   const bool in = inside<D>(g, c);
-  interp<D, D, true>(g, field, c, v);
+  interp<D, D, true, IT>(g, field, c, v);
   float s2 = 0.f;
 #pragma unroll
   for (int d = 0; d < D; ++d) {
     e[d] = iv[d] + (in ? v[d] : 0.f);
-    const float q = e[d] / g.sp[d];
+    const float q = e[d] * g.isp[d];
     s2 += q * q;
   }
   return sqrtf(s2);
@@ -982,7 +984,7 @@ __device__ __forceinline__ float resid_norm(const GridDev& g, const float e[3]) 
   float s2 = 0.f;
 #pragma unroll
   for (int d = 0; d < D; ++d) {
-    const float q = e[d] / g.sp[d];
+    const float q = e[d] * g.isp[d];
     s2 += q * q;
   }
   return sqrtf(s2);
@@ -1010,7 +1012,7 @@ constexpr int kInvUnroll = ANTS_INV_UNROLL;
 #endif
 constexpr bool kInvStoreResidual = ANTS_INV_STORE_RES != 0;
 
-template <int D>
+template <int D, class IT>
 __global__ void __launch_bounds__(kInvThreads) k_inv_step(GridDev g, PtrPack4 field, MutPack4 inv, MutPack4 res,
                                                           SlotPack4 slots, Partial* partials, int64_t part_stride,
                                                           int j, InvGate gate) {
@@ -1058,7 +1060,7 @@ __global__ void __launch_bounds__(kInvThreads) k_inv_step(GridDev g, PtrPack4 fi
     for (int u = 0; u < kInvUnroll; ++u) {
       const int64_t i = i0 + u * stride;
       ok[u] = i < N;
-      const int64_t ic = ok[u] ? i : N - 1;  // loads stay in bounds; stores are skipped
+      const IT ic = static_cast<IT>(ok[u] ? i : N - 1);  // loads stay in bounds; stores are skipped
       unlin(g, ic, xs[u], ys[u], zs[u]);
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
@@ -1072,11 +1074,11 @@ __global__ void __launch_bounds__(kInvThreads) k_inv_step(GridDev g, PtrPack4 fi
         // e_{j-1}: stored by the previous step, or recomputed from the same
         // inverse and field with the same arithmetic (bitwise equal)
         const float nrm = kInvStoreResidual ? resid_norm<D>(g, e[u])
-                                            : inv_residual<D>(g, fld, xs[u], ys[u], zs[u], cur[u], e[u]);
+                                            : inv_residual<D, IT>(g, fld, xs[u], ys[u], zs[u], cur[u], e[u]);
         float sc = 1.f;
         if (nrm > lim) sc = lim / nrm;
         const bool bnd = on_boundary(g, xs[u], ys[u], zs[u]);
-        const int64_t i = i0 + u * stride;
+        const IT i = static_cast<IT>(i0 + u * stride);
 #pragma unroll
         for (int d = 0; d < D; ++d) {
           cur[u][d] = bnd ? 0.f : cur[u][d] + (-e[u][d] * sc) * eps;
@@ -1087,10 +1089,10 @@ __global__ void __launch_bounds__(kInvThreads) k_inv_step(GridDev g, PtrPack4 fi
     if (do_norm) {
 #pragma unroll
       for (int u = 0; u < kInvUnroll; ++u) {
-        const float nrm = inv_residual<D>(g, fld, xs[u], ys[u], zs[u], cur[u], e[u]);
+        const float nrm = inv_residual<D, IT>(g, fld, xs[u], ys[u], zs[u], cur[u], e[u]);
         if (ok[u]) {
           if (kInvStoreResidual) {
-            const int64_t i = i0 + u * stride;
+            const IT i = static_cast<IT>(i0 + u * stride);
 #pragma unroll
             for (int d = 0; d < D; ++d) rs[i * D + d] = e[u][d];
           }
@@ -1362,12 +1364,19 @@ void launch_inv_step(const GridDev& g, int np, const float* const* field, float*
   SlotPack4 sp{};
   for (int k = 0; k < np; ++k) sp.p[k] = slots[k];
   const int64_t stride = inv_partials_needed(g);
-  if (g.D == 2)
-    hipLaunchKernelGGL(k_inv_step<2>, grid, dim3(kInvThreads), 0, s, g, pack(field, np), mpack(inv, np), mpack(res, np),
-                       sp, partials, stride, j, gate);
+  const bool idx32 = g.count() * g.D < (int64_t{1} << 31);
+  if (g.D == 2 && idx32)
+    hipLaunchKernelGGL((k_inv_step<2, int>), grid, dim3(kInvThreads), 0, s, g, pack(field, np), mpack(inv, np),
+                       mpack(res, np), sp, partials, stride, j, gate);
+  else if (g.D == 2)
+    hipLaunchKernelGGL((k_inv_step<2, int64_t>), grid, dim3(kInvThreads), 0, s, g, pack(field, np), mpack(inv, np),
+                       mpack(res, np), sp, partials, stride, j, gate);
+  else if (idx32)
+    hipLaunchKernelGGL((k_inv_step<3, int>), grid, dim3(kInvThreads), 0, s, g, pack(field, np), mpack(inv, np),
+                       mpack(res, np), sp, partials, stride, j, gate);
   else
-    hipLaunchKernelGGL(k_inv_step<3>, grid, dim3(kInvThreads), 0, s, g, pack(field, np), mpack(inv, np), mpack(res, np),
-                       sp, partials, stride, j, gate);
+    hipLaunchKernelGGL((k_inv_step<3, int64_t>), grid, dim3(kInvThreads), 0, s, g, pack(field, np), mpack(inv, np),
+                       mpack(res, np), sp, partials, stride, j, gate);
 }
 
 void launch_resample_fields(const GridDev& gi, const GridDev& go, int np, const float* const* in, float* const* out,

@@ -12,6 +12,8 @@ struct GridDev {
   int D;
   int n[3];
   float sp[3];
+  float isp[3];    // 1/spacing (rounded from double): index = (p - origin) * isp, as ITK's
+                   // physical-point-to-index matrix does
   float org[3];
   double dsp[3];   // exact geometry for double-precision coordinate paths
   double dorg[3];
