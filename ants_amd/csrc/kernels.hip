@@ -767,6 +767,7 @@ __global__ void __launch_bounds__(TX* TY, kCCWavesPerEU) k_cc3d(GridDev g, const
                                          (min(y + R, ny - 1) - max(y - R, 0) + 1));
 
   double ring[5][W];
+  double zsum[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
   for (int a = 0; a < 5; ++a)
 #pragma unroll
@@ -823,22 +824,33 @@ __global__ void __launch_bounds__(TX* TY, kCCWavesPerEU) k_cc3d(GridDev g, const
     }
     __syncthreads();
     if (zz + 1 < z1 + R) fetch(zz + 1);
-    for (int ly = ty; ly < LY; ly += TY) {
+    // x box sums, two adjacent outputs per task: the window of output
+    // x2 + 1 is that of x2 with the first sample dropped and one added
+    static_assert(TX % 2 == 0, "x sums run in pairs");
+    for (int task = threadIdx.x; task < LY * (TX / 2); task += NT) {
+      const int ly = task / (TX / 2), x2 = 2 * (task - ly * (TX / 2));
       double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;
 #pragma unroll
       for (int b = 0; b < W; ++b) {
-        const double f = q[0][ly][tx + b], m = q[1][ly][tx + b];
+        const double f = q[0][ly][x2 + b], m = q[1][ly][x2 + b];
         s0 += f;
         s1 += m;
         s2 += f * f;
         s3 += m * m;
         s4 += f * m;
       }
-      sx[0][ly][tx] = s0;
-      sx[1][ly][tx] = s1;
-      sx[2][ly][tx] = s2;
-      sx[3][ly][tx] = s3;
-      sx[4][ly][tx] = s4;
+      sx[0][ly][x2] = s0;
+      sx[1][ly][x2] = s1;
+      sx[2][ly][x2] = s2;
+      sx[3][ly][x2] = s3;
+      sx[4][ly][x2] = s4;
+      const double fo = q[0][ly][x2], mo = q[1][ly][x2];
+      const double fn = q[0][ly][x2 + W], mn = q[1][ly][x2 + W];
+      sx[0][ly][x2 + 1] = s0 + (fn - fo);
+      sx[1][ly][x2 + 1] = s1 + (mn - mo);
+      sx[2][ly][x2 + 1] = s2 + (fn * fn - fo * fo);
+      sx[3][ly][x2 + 1] = s3 + (mn * mn - mo * mo);
+      sx[4][ly][x2 + 1] = s4 + (fn * mn - fo * mo);
     }
     // centre values and in-plane gradients of plane zz at (x, y)
     const float fc = q[0][ty + R][tx + R], mc = q[1][ty + R][tx + R];
@@ -858,8 +870,10 @@ __global__ void __launch_bounds__(TX* TY, kCCWavesPerEU) k_cc3d(GridDev g, const
       for (int b = 0; b < W; ++b) t += sx[a][ty + b][tx];
       p[a] = t;
     }
+    // running z-window sums: add the new plane, drop the one leaving the ring
 #pragma unroll
     for (int a = 0; a < 5; ++a) {
+      zsum[a] += p[a] - ring[a][0];
 #pragma unroll
       for (int b = 0; b < W - 1; ++b) ring[a][b] = ring[a][b + 1];
       ring[a][W - 1] = p[a];
@@ -885,14 +899,7 @@ __global__ void __launch_bounds__(TX* TY, kCCWavesPerEU) k_cc3d(GridDev g, const
 
     const int zc = zz - R;
     if (active && zc >= z0 && zc < z1) {
-      double sm5[5];
-#pragma unroll
-      for (int a = 0; a < 5; ++a) {
-        double t = 0.0;
-#pragma unroll
-        for (int b = 0; b < W; ++b) t += ring[a][b];
-        sm5[a] = t;
-      }
+      const double* sm5 = zsum;
       const double cz = static_cast<double>(min(zc + R, nz - 1) - max(zc - R, 0) + 1);
       // raw centre column: rf[R+1] = zz = zc+R ... rf[1] = zc, rf[0] = zc-1, rf[2] = zc+1
       const float f = rf[1], m = rm[1];
