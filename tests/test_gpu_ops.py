@@ -120,13 +120,14 @@ def test_scale_update():
     np.testing.assert_allclose(out, ref, rtol=1e-6, atol=1e-7)
 
 
-@pytest.mark.parametrize("shape,amp", [((40, 36), 2.0), ((20, 24, 28), 2.5), ((12, 10, 14), 0.05)])
-def test_invert(shape, amp):
+@pytest.mark.parametrize("shape,amp,sp", [((40, 36), 2.0, None), ((20, 24, 28), 2.5, None), ((12, 10, 14), 0.05, None),
+                                         ((22, 18, 26), 2.0, [1.2, 0.9, 1.5])])
+def test_invert(shape, amp, sp):
     D = len(shape)
     f = smooth_random_field(shape, D, 3.0, amp)
     init = smooth_random_field(shape, D, 3.0, 0.3 * amp)
-    out, k, mx, mn = G.invert(f, init)
-    ref, kr, mxr, mnr = o.invert(grid_np(shape), f, init)
+    out, k, mx, mn = G.invert(f, init, sp)
+    ref, kr, mxr, mnr = o.invert(grid_np(shape, sp), f, init)
     assert k == kr
     np.testing.assert_allclose(out, ref, atol=2e-4 * max(amp, 1.0))
     assert mx == pytest.approx(mxr, rel=1e-4, abs=1e-6)

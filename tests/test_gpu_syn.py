@@ -72,6 +72,27 @@ def test_syn_3d_phantom_small():
     print("3d parity (energy rel, fwd vox, inv vox):", compare(res, ref, pair.spacing))
 
 
+def test_syn_3d_anisotropic_spacing_sigmas_in_mm():
+    """Non-power-of-two anisotropic spacing (continuous indices by reciprocal
+    spacing, norms in voxel units) and smoothing sigmas given in mm."""
+    pair = ph.phantom_pair_3d(3, size=(40, 44, 36), spacing=(1.2, 0.9, 1.5), max_disp_mm=4.0, svf_sigma_mm=6.0)
+    stage = SyNStage.from_args("SyN[0.2,3,0]", "CC[f,m,1,2]", "[10x6x3,1e-6,10]", "4x2x1", "2x1x0mm")
+    res = run_pair(stage, pair, force=True)
+    ref = run_oracle(stage, pair, force=True)
+    print("3d anisotropic parity:", compare(res, ref, pair.spacing))
+
+
+def test_syn_2d_direct_cc_total_smoothing_spacing():
+    """2-D with CC radius 3 (the untiled CC kernel), a total-field variance
+    (the second smoothing and its blend) and anisotropic spacing."""
+    pair = ph.blob_pair_2d(4, n=96)
+    pair.spacing = (1.3, 0.7)
+    stage = SyNStage.from_args("SyN[0.25,3,1]", "CC[f,m,1,3]", "[20x10x5,1e-6,10]", "4x2x1", "2x1x0vox")
+    res = run_pair(stage, pair, force=True)
+    ref = run_oracle(stage, pair, force=True)
+    print("2d r3 total-variance parity:", compare(res, ref, pair.spacing))
+
+
 def test_syn_3d_radius2_quick_schedule():
     pair = ph.phantom_pair_3d(5, size=(40, 40, 40), max_disp_mm=3.0, svf_sigma_mm=6.0)
     stage = SyNStage.from_args("SyN[0.1,3,0]", "CC[f,m,1,2]", "[40x20x0,1e-6,10]", "4x2x1", "2x1x0vox")
