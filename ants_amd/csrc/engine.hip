@@ -402,9 +402,11 @@ struct GraphKey {
   int level;
   const void* p[8];
   unsigned int prof_mask;
+  int inv_launches;
   bool operator<(const GraphKey& o) const {
     if (setup != o.setup) return setup < o.setup;
     if (level != o.level) return level < o.level;
+    if (inv_launches != o.inv_launches) return inv_launches < o.inv_launches;
     if (prof_mask != o.prof_mask) return prof_mask < o.prof_mask;
     return std::lexicographical_compare(p, p + 8, o.p, o.p + 8);
   }
@@ -439,7 +441,8 @@ struct ants_syn_handle {
   Profiler prof;
   std::map<GraphKey, GraphEntry> graphs;
   bool use_graphs = true;
-  int inv_launches = kInvLaunchesDefault;
+  int inv_launches = kInvLaunchesDefault;  // starting value per level
+  bool inv_launches_fixed = false;          // set by ANTS_SYN_INV_LAUNCHES: no adaptation
   std::vector<const void*> graph_signature;  // buffers + geometry the cached graphs were built for
 
   void clear_graphs() {
@@ -668,7 +671,7 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
     const char* ks = std::getenv("ANTS_SYN_INV_LAUNCHES");
     inv_launches = ks ? std::atoi(ks) : kInvLaunchesDefault;
     inv_launches = std::min(std::max(inv_launches, 1), kInvMaxIter + 1);
-    setup_hash = fnv1a(&inv_launches, sizeof(inv_launches), setup_hash);
+    inv_launches_fixed = ks != nullptr;
   }
   std::memset(&run_stats, 0, sizeof(run_stats));
   std::memset(&prof.acc, 0, sizeof(prof.acc));
@@ -719,6 +722,9 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
     }
 
     ConvergenceMonitor mon(P.convergence_window);
+    // inversion steps launched per batch with each iteration at this level;
+    // raised by one after an iteration that needed a host tail
+    int kl = inv_launches;
     bool converged = false;
     int iter = 0;
     // algorithmic bytes per voxel-iteration: image gathers + field passes
@@ -755,7 +761,7 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
     // batch 1 inverts st[1]/st[3] into st[0]/st[2].
     auto inv_finish = [&](bool& extra) -> int {
       extra = false;
-      const int K = inv_launches;
+      const int K = kl;
       if (K > kInvMaxIter) return ANTS_OK;
       auto pending = [&](int batch) {
         return host_stats->inv[batch][K - 1].computed || host_stats->inv[batch + 2][K - 1].computed;
@@ -836,7 +842,7 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
       }
       // 6. inversions: inverse middle fields from the previous inverses (in place),
       //    then forward fields as inverses of the new inverses, started from t (in place)
-      //    Only the first inv_launches steps of each batch are enqueued here:
+      //    Only the first kl steps of each batch are enqueued here:
       //    almost every inversion converges within them.  The second batch's
       //    launches are gated on the first having finished (a no-op
       //    otherwise); the host completes whatever is left after reading the
@@ -846,11 +852,11 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
         const float* field[2] = {batch == 0 ? wk_[2] : st_[1], batch == 0 ? wk_[3] : st_[3]};
         float* inv[2] = {batch == 0 ? st_[1] : wk_[2], batch == 0 ? st_[3] : wk_[3]};
         InvGate gate = kNoGate;
-        if (batch == 1 && inv_launches <= kInvMaxIter)
-          gate = InvGate{{dstats->inv[0], dstats->inv[2]}, 2, inv_launches - 1};
-        ANTS_TRY(inv_steps_on(batch, field, inv, 0, inv_launches, gate));
+        if (batch == 1 && kl <= kInvMaxIter)
+          gate = InvGate{{dstats->inv[0], dstats->inv[2]}, 2, kl - 1};
+        ANTS_TRY(inv_steps_on(batch, field, inv, 0, kl, gate));
       }
-      prof.end(ANTS_KC_INV, 0.0, 2 * inv_launches, 3, 0, inv_launches);
+      prof.end(ANTS_KC_INV, 0.0, 2 * kl, 3, 0, kl);
       std::swap(st_[0], wk_[2]);
       std::swap(st_[2], wk_[3]);
       HIP_TRY(hipMemcpyAsync(host_stats, dstats, sizeof(IterStats), hipMemcpyDeviceToHost, s));
@@ -864,7 +870,7 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
     while (iter++ < P.iterations[l] && !converged) {
       float* nst[4];
       float* nwk[4];
-      GraphKey key{setup_hash, l, {st[0], st[1], st[2], st[3], wk[0], wk[1], wk[2], wk[3]}, prof.mask};
+      GraphKey key{setup_hash, l, {st[0], st[1], st[2], st[3], wk[0], wk[1], wk[2], wk[3]}, prof.mask, kl};
       auto it = use_graphs ? graphs.find(key) : graphs.end();
       if (use_graphs && it == graphs.end()) {
         // capture this rotation's sequence once
@@ -905,6 +911,7 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
       HIP_TRY(hipStreamSynchronize(s));
       bool extra = false;
       ANTS_TRY(inv_finish(extra));
+      if (extra && !inv_launches_fixed) kl = std::min(kl + 1, kInvMaxIter + 1);
       prof.collect(host_stats, N, D);
       const double value = -host_stats->energy.sum / static_cast<double>(N);
       const double E = 0.5 * (value + value);

@@ -69,7 +69,7 @@ constexpr int kRedBlocks = 1024;
 
 constexpr int kInvMaxIter = 20;            // InvertDisplacementFieldImageFilter MaximumNumberOfIterations (SyN)
 constexpr int kInvSlots = kInvMaxIter + 1;
-constexpr int kInvLaunchesDefault = 4;     // inversion steps per batch in an iteration's launch sequence
+constexpr int kInvLaunchesDefault = 3;     // inversion steps per batch in an iteration's launch sequence
 
 // Per-iteration device statistics block (zeroed by k_fill at iteration start).
 struct IterStats {
