@@ -65,3 +65,63 @@ def test_cli_grammar():
         R.SyNStage.from_args("SyN[0.1]", "CC[a,b,1,4]", "100x70", "4x2x1", "2x1x0")
     with pytest.raises(ValueError):
         R.SyNTransform.parse("Affine[0.1]")
+
+
+def _c_layout(tmp_path, exprs):
+    """sizeof/offsetof values of the public structs as the C compiler lays
+    them out from include/ants_syn.h."""
+    import os
+    import subprocess
+    src = tmp_path / "layout.c"
+    body = "\n".join(f'  printf("%zu\\n", (size_t)({e}));' for e in exprs)
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "ants_syn.h"\nint main(void) {\n'
+                   + body + "\n  return 0;\n}\n")
+    exe = tmp_path / "layout"
+    inc = os.path.join(os.path.dirname(L.HERE), "include")
+    subprocess.run(["gcc", "-std=c99", "-I", inc, str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
+    return [int(v) for v in out]
+
+
+def test_ctypes_structs_match_c_layout(tmp_path):
+    from ants_amd import serving as S
+    structs = {"ants_image_desc": L.ImageDesc, "ants_syn_params": L.SynParamsC, "ants_syn_stats": L.SynStatsC,
+               "ants_syn_kernel_stats": L.KernelStatsC, "ants_batch_config": S.BatchConfigC,
+               "ants_syn_request": S.RequestC, "ants_request_result": S.ResultC}
+    exprs, want = [], []
+    for cname, py in structs.items():
+        exprs.append(f"sizeof({cname})")
+        want.append(C.sizeof(py))
+        for fname, _ in py._fields_:
+            exprs.append(f"offsetof({cname}, {fname})")
+            want.append(getattr(py, fname).offset)
+    assert _c_layout(tmp_path, exprs) == want
+
+
+def test_footprint_closed_form_and_errors():
+    from ants_amd import serving as S
+    stage = R.SyNStage.from_args("SyN[0.1,3,0]", "CC[f,m,1,4]", "[100x70x20,1e-6,10]", "4x2x1", "2x1x0vox")
+    p = S.stage_params(stage, 3)
+    d = L.desc((192, 224, 192))
+    fp = L.lib().ants_syn_footprint
+    fp.restype = C.c_int64
+    N = 192 * 224 * 192
+    # images (2 copies x 2 images + scratch), warped images, 10 fields,
+    # request inputs (2 images + labels) and outputs (2 fields + labels)
+    want = 2 * N * 4 * 2 + N * 4 + 2 * N * 4 + 10 * N * 3 * 4 + 3 * N * 4 + 2 * N * 3 * 4 + N * 4
+    assert fp(C.byref(p), C.byref(d), C.byref(d)) == want
+    pb = S.stage_params(stage, 3, precision="bf16")
+    assert fp(C.byref(pb), C.byref(d), C.byref(d)) == want + 2 * N * 2
+    p.dim = 5
+    assert fp(C.byref(p), C.byref(d), C.byref(d)) == -1
+
+
+def test_batch_rejects_bad_slot_counts():
+    from ants_amd import serving as S
+    S._bind()
+    cfg = S.BatchConfigC()
+    b = C.c_void_p()
+    for slots in (0, 65):
+        cfg.slots = slots
+        assert L.lib().ants_batch_create(C.byref(cfg), C.byref(b)) == 1
+        assert b"slots" in L.lib().ants_last_error()
