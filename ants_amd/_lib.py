@@ -71,6 +71,11 @@ class KernelStatsC(C.Structure):
     _fields_ = [("ms", C.c_double * 5), ("launches", C.c_int64 * 5), ("bytes", C.c_double * 5)]
 
 
+class TransformC(C.Structure):
+    _fields_ = [("kind", C.c_int), ("matrix", C.c_double * 9), ("translation", C.c_double * 3),
+                ("center", C.c_double * 3), ("field", ImageDesc), ("d_field", C.c_void_p)]
+
+
 ITER_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, C.POINTER(C.c_int))
 
 _lib = None
@@ -129,6 +134,7 @@ def lib():
     L.ants_op_resample_field.argtypes = [pdesc, vp, pdesc, vp, vp]
     L.ants_op_preprocess.argtypes = [C.c_int64, vp, vp, vp]
     L.ants_apply_transform_labels_nn.argtypes = [pdesc, vp, pdesc, vp, vp, vp]
+    L.ants_apply_transforms.argtypes = [pdesc, pdesc, vp, C.POINTER(TransformC), i, i, d, vp, vp]
     _lib = L
     return L
 
@@ -138,7 +144,8 @@ def check(rc: int):
         raise AntsError(rc, lib().ants_last_error().decode())
 
 
-def desc(size, spacing=None, origin=None) -> ImageDesc:
+def desc(size, spacing=None, origin=None, direction=None) -> ImageDesc:
+    """size in (x, y[, z]) order; direction row-major dim x dim (identity by default)."""
     D = len(size)
     g = ImageDesc()
     g.dim = D
@@ -146,8 +153,9 @@ def desc(size, spacing=None, origin=None) -> ImageDesc:
         g.size[k] = int(size[k]) if k < D else 1
         g.spacing[k] = float(spacing[k]) if spacing is not None and k < D else 1.0
         g.origin[k] = float(origin[k]) if origin is not None and k < D else 0.0
-    for r in range(D):
-        g.direction[r * D + r] = 1.0
+    dm = np.eye(D) if direction is None else np.asarray(direction, dtype=np.float64).reshape(D, D)
+    for k, v in enumerate(dm.ravel()):
+        g.direction[k] = float(v)
     return g
 
 

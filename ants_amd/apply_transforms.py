@@ -1,0 +1,137 @@
+"""antsApplyTransforms subset on the GPU: resample an image through a
+composite of affine and displacement-field transforms
+(``Examples/antsApplyTransforms.cxx:546-640``).
+
+``apply_transforms(reference, image, image_geometry, transforms, ...)``
+mirrors ``antsApplyTransforms -r reference -i image -t T1 -t T2 ... -n I
+--default-value v``: the transforms are listed as on the command line, and
+each reference point is mapped through T1 first, then T2, ...  (the
+command-line parser stores repeated options newest-first and ITK's
+CompositeTransform applies the last added transform first,
+``Examples/itkantsRegistrationHelper.h:1262-1455``).  Interpolators: Linear,
+NearestNeighbor, GenericLabel[Linear].
+"""
+from __future__ import annotations
+
+import ctypes as C
+import re
+from dataclasses import dataclass, field
+from typing import Optional, Sequence, Tuple, Union
+
+import numpy as np
+
+from . import _lib as L
+
+INTERPOLATORS = {"linear": 0, "nearestneighbor": 1, "genericlabel": 2}
+
+
+@dataclass
+class Geometry:
+    """Image grid: size (x, y[, z]), spacing, origin, direction (row-major)."""
+    size: Tuple[int, ...]
+    spacing: Optional[Sequence[float]] = None
+    origin: Optional[Sequence[float]] = None
+    direction: Optional[np.ndarray] = None
+
+    @property
+    def dim(self) -> int:
+        return len(self.size)
+
+    @property
+    def shape(self) -> Tuple[int, ...]:
+        """numpy (z, y, x) order"""
+        return tuple(int(v) for v in reversed(self.size))
+
+    def desc(self) -> L.ImageDesc:
+        return L.desc(self.size, self.spacing, self.origin, self.direction)
+
+
+@dataclass
+class AffineTransform:
+    """ITK MatrixOffsetTransformBase: y = A (x - c) + t + c.  ``invert``
+    selects the inverse transform (``-t [affine.mat,1]``)."""
+    matrix: np.ndarray
+    translation: Sequence[float]
+    center: Optional[Sequence[float]] = None
+    invert: bool = False
+
+    def parameters(self):
+        A = np.asarray(self.matrix, dtype=np.float64)
+        D = A.shape[0]
+        t = np.asarray(self.translation, dtype=np.float64)
+        c = np.zeros(D) if self.center is None else np.asarray(self.center, dtype=np.float64)
+        if not self.invert:
+            return A, t, c
+        # same centre: A' = A^-1, offset' = -A^-1 offset, t' = offset' - c + A' c
+        Ai = np.linalg.inv(A)
+        off = t + c - A @ c
+        offi = -Ai @ off
+        return Ai, offi - c + Ai @ c, c
+
+
+@dataclass
+class DisplacementFieldTransform:
+    """ITK DisplacementFieldTransform: y = x + u(x), u linear in the field's
+    grid, zero outside it.  ``field`` is (z, y, x, D) float32 in physical
+    units, on the host or an ``ants_amd._lib.DeviceArray``."""
+    field: Union[np.ndarray, L.DeviceArray]
+    geometry: Geometry
+
+
+Transform = Union[AffineTransform, DisplacementFieldTransform, None]
+
+
+def parse_interpolation(name: str) -> int:
+    """-n Linear | NearestNeighbor | GenericLabel[Linear]"""
+    m = re.fullmatch(r"\s*([A-Za-z]+)\s*(?:\[\s*([A-Za-z]*)\s*\])?\s*", name)
+    if not m:
+        raise ValueError(f"bad interpolation {name!r}")
+    base = m.group(1).lower()
+    if base == "genericlabel" and m.group(2) and m.group(2).lower() != "linear":
+        raise ValueError("GenericLabel is supported with its default Linear interpolator only")
+    if base not in INTERPOLATORS:
+        raise ValueError(f"interpolation {name!r} is not supported (Linear, NearestNeighbor, GenericLabel)")
+    return INTERPOLATORS[base]
+
+
+def apply_transforms(reference: Geometry, image, image_geometry: Geometry, transforms: Sequence[Transform] = (),
+                     interpolation: str = "Linear", default_value: float = 0.0, out=None):
+    """Resample ``image`` onto ``reference``.  ``image`` and ``out`` may be
+    numpy arrays (host) or DeviceArrays; returns a numpy array when ``out``
+    is None, else ``out``."""
+    if reference.dim != image_geometry.dim:
+        raise ValueError("reference and image dimensions differ")
+    D = reference.dim
+    if len(transforms) > 8:
+        raise ValueError("at most 8 transforms")
+    d_img = image if isinstance(image, L.DeviceArray) else L.DeviceArray.from_host(np.asarray(image, np.float32))
+    keep = [d_img]
+    xs = (L.TransformC * max(len(transforms), 1))()
+    for i, t in enumerate(transforms):
+        if t is None:
+            xs[i].kind = 0
+        elif isinstance(t, AffineTransform):
+            A, tr, c = t.parameters()
+            if A.shape != (D, D):
+                raise ValueError("affine matrix must be dim x dim")
+            xs[i].kind = 1
+            for k, v in enumerate(A.ravel()):
+                xs[i].matrix[k] = float(v)
+            for k in range(D):
+                xs[i].translation[k] = float(tr[k])
+                xs[i].center[k] = float(c[k])
+        elif isinstance(t, DisplacementFieldTransform):
+            f = t.field if isinstance(t.field, L.DeviceArray) else L.DeviceArray.from_host(np.asarray(t.field, np.float32))
+            if tuple(f.shape) != t.geometry.shape + (D,):
+                raise ValueError("displacement field shape does not match its geometry")
+            keep.append(f)
+            xs[i].kind = 2
+            xs[i].field = t.geometry.desc()
+            xs[i].d_field = f.ptr
+        else:
+            raise TypeError(f"unsupported transform {type(t).__name__}")
+    d_out = out if out is not None else L.DeviceArray(reference.shape)
+    L.check(L.lib().ants_apply_transforms(C.byref(reference.desc()), C.byref(image_geometry.desc()), d_img.ptr, xs,
+                                          len(transforms), parse_interpolation(interpolation), float(default_value),
+                                          d_out.ptr, None))
+    return d_out if out is not None else d_out.to_host()

@@ -1339,6 +1339,93 @@ int ants_apply_transform_labels_nn(const ants_image_desc* go, const float* d_fie
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
+// antsApplyTransforms subset.
+namespace {
+
+// Geometry with direction cosines: m = direction * diag(spacing) and its
+// inverse diag(1/spacing) * direction^T (orthonormal directions only).
+int geom_from_desc(const ants_image_desc* d, GeomDev* g) {
+  if (!d) return fail(ANTS_ERR_INVALID, "null image descriptor");
+  if (d->dim != 2 && d->dim != 3) return fail(ANTS_ERR_INVALID, "dim must be 2 or 3 (got %d)", d->dim);
+  const int D = d->dim;
+  std::memset(g, 0, sizeof(*g));
+  g->D = D;
+  for (int k = 0; k < 3; ++k) {
+    const int64_t n = k < D ? d->size[k] : 1;
+    if (n < 1 || n > (int64_t{1} << 30)) return fail(ANTS_ERR_INVALID, "image size out of range");
+    g->n[k] = static_cast<int>(n);
+    if (k < D && !(d->spacing[k] > 0.0)) return fail(ANTS_ERR_INVALID, "image spacing must be positive");
+    g->org[k] = k < D ? d->origin[k] : 0.0;
+  }
+  for (int r = 0; r < D; ++r)
+    for (int c = 0; c < D; ++c) {
+      double dot = 0.0;
+      for (int k = 0; k < D; ++k) dot += d->direction[k * D + r] * d->direction[k * D + c];
+      if (std::fabs(dot - (r == c ? 1.0 : 0.0)) > 1e-6)
+        return fail(ANTS_ERR_UNSUPPORTED, "image direction must be orthonormal");
+    }
+  for (int r = 0; r < D; ++r)
+    for (int c = 0; c < D; ++c) {
+      g->m[r * 3 + c] = d->direction[r * D + c] * d->spacing[c];
+      g->minv[r * 3 + c] = d->direction[c * D + r] / d->spacing[r];
+    }
+  return ANTS_OK;
+}
+
+}  // namespace
+
+extern "C" int ants_apply_transforms(const ants_image_desc* ref, const ants_image_desc* in, const float* d_in,
+                                     const ants_transform* transforms, int n_transforms, int interpolation,
+                                     double default_value, float* d_out, void* stream) {
+  ApplyDev P;
+  std::memset(&P, 0, sizeof(P));
+  ANTS_TRY(geom_from_desc(ref, &P.ref));
+  ANTS_TRY(geom_from_desc(in, &P.in));
+  const int D = P.ref.D;
+  if (P.in.D != D) return fail(ANTS_ERR_INVALID, "reference and input dimensions differ");
+  if (!d_in || !d_out) return fail(ANTS_ERR_INVALID, "null image buffer");
+  if (n_transforms < 0 || n_transforms > kMaxTransforms || (n_transforms > 0 && !transforms))
+    return fail(ANTS_ERR_INVALID, "transform count must be 0..%d", kMaxTransforms);
+  if (interpolation < ANTS_INTERP_LINEAR || interpolation > ANTS_INTERP_GENERIC_LABEL)
+    return fail(ANTS_ERR_INVALID, "unknown interpolation %d", interpolation);
+  P.nt = n_transforms;
+  P.defv = static_cast<float>(default_value);
+  for (int t = 0; t < n_transforms; ++t) {
+    const ants_transform& T = transforms[t];
+    XformDev& X = P.t[t];
+    X.kind = T.kind;
+    if (T.kind == ANTS_TRANSFORM_IDENTITY) continue;
+    if (T.kind == ANTS_TRANSFORM_AFFINE) {
+      // MatrixOffsetTransformBase: offset = t + c - A c
+      for (int r = 0; r < D; ++r) {
+        double ac = 0.0;
+        for (int k = 0; k < D; ++k) {
+          X.a[r * 3 + k] = T.matrix[r * D + k];
+          ac = ac + T.matrix[r * D + k] * T.center[k];
+        }
+        X.off[r] = T.translation[r] + T.center[r] - ac;
+      }
+    } else if (T.kind == ANTS_TRANSFORM_DISPLACEMENT) {
+      ANTS_TRY(geom_from_desc(&T.field, &X.fg));
+      if (X.fg.D != D) return fail(ANTS_ERR_INVALID, "displacement field dimension differs");
+      if (!T.d_field) return fail(ANTS_ERR_INVALID, "null displacement field");
+      X.f = T.d_field;
+    } else {
+      return fail(ANTS_ERR_INVALID, "unknown transform kind %d", T.kind);
+    }
+  }
+  GridDev go{};
+  go.D = D;
+  for (int k = 0; k < 3; ++k) go.n[k] = P.ref.n[k];
+  set_divisors(go);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  launch_apply_transforms(go, P, interpolation, d_in, d_out, s);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(s));
+  return ANTS_OK;
+}
+
+// ---------------------------------------------------------------------------
 // Batched / served registration.
 namespace {
 

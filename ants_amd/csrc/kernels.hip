@@ -1157,6 +1157,175 @@ __global__ void __launch_bounds__(kBlock) k_labels_nn(GridDev go, const float* _
   out[i] = v;
 }
 
+// ---------------------------------------------------------------------------
+// antsApplyTransforms subset: ResampleImageFilter through a composite of
+// affine and displacement-field transforms.  Coordinates are formed in
+// double with contraction off and a fixed operation order, so that a double
+// CPU path makes the same nearest-neighbour and label decisions.
+#pragma clang fp contract(off)
+template <int D>
+__device__ __forceinline__ void geom_point(const GeomDev& g, const int ix[3], double p[3]) {
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    double s = g.org[r];
+#pragma unroll
+    for (int c = 0; c < D; ++c) s = s + g.m[r * 3 + c] * static_cast<double>(ix[c]);
+    p[r] = r < D ? s : 0.0;
+  }
+}
+template <int D>
+__device__ __forceinline__ void geom_cidx(const GeomDev& g, const double p[3], double c[3]) {
+  double q[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) q[k] = k < D ? p[k] - g.org[k] : 0.0;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) s = s + g.minv[r * 3 + k] * q[k];
+    c[r] = r < D ? s : 0.0;
+  }
+}
+template <int D>
+__device__ __forceinline__ bool geom_inside(const GeomDev& g, const double c[3]) {
+#pragma unroll
+  for (int d = 0; d < D; ++d)
+    if (!(c[d] >= -0.5 && c[d] < static_cast<double>(g.n[d]) - 0.5)) return false;
+  return true;
+}
+// Neighbours and weights of linear interpolation at an inside point, the
+// upper neighbour clamped to the buffer (ITK LinearInterpolateImageFunction).
+template <int D>
+__device__ __forceinline__ void lin_setup(const GeomDev& g, const double c[3], int64_t lo[3], int64_t hi[3],
+                                          double w[3]) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (d < D) {
+      const double f = floor(c[d]);
+      int64_t b = static_cast<int64_t>(f);
+      double t = c[d] - f;
+      if (b < 0) {
+        b = 0;
+        t = 0.0;
+      }
+      lo[d] = b;
+      hi[d] = b + 1 < g.n[d] ? b + 1 : b;
+      w[d] = t;
+    } else {
+      lo[d] = hi[d] = 0;
+      w[d] = 0.0;
+    }
+  }
+}
+// corner k (bit 0: x, bit 1: y, bit 2: z) of the interpolation cell
+template <int D>
+__device__ __forceinline__ int64_t corner(const GeomDev& g, const int64_t lo[3], const int64_t hi[3], int k) {
+  const int64_t x = (k & 1) ? hi[0] : lo[0], y = (k & 2) ? hi[1] : lo[1], z = (k & 4) ? hi[2] : lo[2];
+  return x + static_cast<int64_t>(g.n[0]) * (y + (D == 3 ? static_cast<int64_t>(g.n[1]) * z : 0));
+}
+// lerp of corner values v[8] (x, then y, then z)
+template <int D>
+__device__ __forceinline__ double lerp_cell(const double v[8], const double w[3]) {
+  const double a00 = v[0] + (v[1] - v[0]) * w[0];
+  const double a10 = v[2] + (v[3] - v[2]) * w[0];
+  const double b0 = a00 + (a10 - a00) * w[1];
+  if (D == 2) return b0;
+  const double a01 = v[4] + (v[5] - v[4]) * w[0];
+  const double a11 = v[6] + (v[7] - v[6]) * w[0];
+  const double b1 = a01 + (a11 - a01) * w[1];
+  return b0 + (b1 - b0) * w[2];
+}
+
+template <int D, int INTERP>
+__global__ void __launch_bounds__(kBlock) k_apply_transforms(GridDev go, ApplyDev P, const float* __restrict__ img,
+                                                             float* __restrict__ out) {
+  const int64_t N = go.count();
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i >= N) return;
+  int x, y, z;
+  unlin(go, i, x, y, z);
+  const int ix[3] = {x, y, z};
+  double p[3];
+  geom_point<D>(P.ref, ix, p);
+  for (int t = 0; t < P.nt; ++t) {
+    const XformDev& T = P.t[t];
+    if (T.kind == 1) {
+      double q[3];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) s = s + T.a[r * 3 + k] * p[k];
+        q[r] = r < D ? s + T.off[r] : 0.0;
+      }
+#pragma unroll
+      for (int r = 0; r < 3; ++r) p[r] = q[r];
+    } else if (T.kind == 2) {
+      double c[3];
+      geom_cidx<D>(T.fg, p, c);
+      if (geom_inside<D>(T.fg, c)) {
+        int64_t lo[3], hi[3];
+        double w[3];
+        lin_setup<D>(T.fg, c, lo, hi, w);
+        int64_t idx[8];
+#pragma unroll
+        for (int k = 0; k < (1 << D); ++k) idx[k] = corner<D>(T.fg, lo, hi, k);
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+          for (int k = 0; k < (1 << D); ++k) v[k] = static_cast<double>(T.f[idx[k] * D + d]);
+          p[d] = p[d] + lerp_cell<D>(v, w);
+        }
+      }
+    }
+  }
+  double c[3];
+  geom_cidx<D>(P.in, p, c);
+  if (!geom_inside<D>(P.in, c)) {
+    out[i] = P.defv;
+    return;
+  }
+  if (INTERP == 1) {
+    int64_t id[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < D; ++d) id[d] = static_cast<int64_t>(floor(c[d] + 0.5));
+    out[i] = img[id[0] + static_cast<int64_t>(P.in.n[0]) * (id[1] + static_cast<int64_t>(P.in.n[1]) * id[2])];
+    return;
+  }
+  int64_t lo[3], hi[3];
+  double w[3];
+  lin_setup<D>(P.in, c, lo, hi, w);
+  float lab[8];
+#pragma unroll
+  for (int k = 0; k < (1 << D); ++k) lab[k] = img[corner<D>(P.in, lo, hi, k)];
+  if (INTERP == 0) {
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < (1 << D); ++k) v[k] = static_cast<double>(lab[k]);
+    out[i] = static_cast<float>(lerp_cell<D>(v, w));
+    return;
+  }
+  // generic label: the label whose indicator image interpolates highest,
+  // the smaller label on ties (labels absent from the cell interpolate to 0)
+  float best = lab[0];
+  double bestv = -1.0;
+#pragma unroll
+  for (int k = 0; k < (1 << D); ++k) {
+    const float l = lab[k];
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < (1 << D); ++q) v[q] = lab[q] == l ? 1.0 : 0.0;
+    const double val = lerp_cell<D>(v, w);
+    if (val > bestv || (val == bestv && l < best)) {
+      bestv = val;
+      best = l;
+    }
+  }
+  out[i] = best;
+}
+#pragma clang fp contract(on)
+
 inline unsigned int blocks_for(int64_t n, int bs = kBlock) { return static_cast<unsigned int>((n + bs - 1) / bs); }
 inline unsigned int red_blocks(int64_t n) {
   const unsigned int b = blocks_for(n);
@@ -1393,6 +1562,23 @@ void launch_resample_fields(const GridDev& gi, const GridDev& go, int np, const 
     hipLaunchKernelGGL(k_resample_fields<2>, grid, dim3(kBlock), 0, s, gi, go, pack(in, np), mpack(out, np));
   else
     hipLaunchKernelGGL(k_resample_fields<3>, grid, dim3(kBlock), 0, s, gi, go, pack(in, np), mpack(out, np));
+}
+
+void launch_apply_transforms(const GridDev& go, const ApplyDev& P, int interp, const float* in, float* out,
+                             hipStream_t s) {
+  const dim3 grid(blocks_for(go.count()));
+#define ANTS_APPLY_LAUNCH(DD, II) \
+  hipLaunchKernelGGL((k_apply_transforms<DD, II>), grid, dim3(kBlock), 0, s, go, P, in, out)
+  if (go.D == 2) {
+    if (interp == 0) ANTS_APPLY_LAUNCH(2, 0);
+    else if (interp == 1) ANTS_APPLY_LAUNCH(2, 1);
+    else ANTS_APPLY_LAUNCH(2, 2);
+  } else {
+    if (interp == 0) ANTS_APPLY_LAUNCH(3, 0);
+    else if (interp == 1) ANTS_APPLY_LAUNCH(3, 1);
+    else ANTS_APPLY_LAUNCH(3, 2);
+  }
+#undef ANTS_APPLY_LAUNCH
 }
 
 void launch_labels_nn(const GridDev& go, const float* u, const GridDev& gl, const int32_t* labels, int32_t* out,

@@ -147,6 +147,33 @@ void launch_inv_step(const GridDev& g, int np, const float* const* field, float*
 void launch_resample_fields(const GridDev& gi, const GridDev& go, int np, const float* const* in,
                             float* const* out, hipStream_t s);
 
+// Grid geometry with direction cosines, in double: point = org + m * index,
+// index = minv * (point - org), m = direction * diag(spacing).
+struct GeomDev {
+  int D;
+  int n[3];
+  double org[3];
+  double m[9];
+  double minv[9];
+};
+constexpr int kMaxTransforms = 8;
+struct XformDev {
+  int kind;       // 0 identity, 1 affine, 2 displacement field
+  double a[9];    // affine: y = a * x + off
+  double off[3];
+  GeomDev fg;     // displacement field geometry
+  const float* f;
+};
+struct ApplyDev {
+  int nt;
+  float defv;
+  GeomDev ref, in;
+  XformDev t[kMaxTransforms];
+};
+// interp: 0 linear, 1 nearest neighbour, 2 generic label (linear indicators)
+void launch_apply_transforms(const GridDev& go, const ApplyDev& P, int interp, const float* in, float* out,
+                             hipStream_t s);
+
 void launch_labels_nn(const GridDev& go, const float* u, const GridDev& gl, const int32_t* labels, int32_t* out,
                       hipStream_t s);
 

@@ -221,6 +221,43 @@ int ants_op_preprocess(int64_t n, const float* d_in, float* d_out, void* stream)
 int ants_apply_transform_labels_nn(const ants_image_desc* g_out, const float* d_field, const ants_image_desc* g_lab,
                                    const int32_t* d_labels, int32_t* d_out, void* stream);
 
+/* ---- antsApplyTransforms subset (Examples/antsApplyTransforms.cxx:546-640) ----
+ * Transforms of a composite (itkantsRegistrationHelper.h:1262-1455):
+ *   IDENTITY
+ *   AFFINE        MatrixOffsetTransformBase: y = A (x - c) + t + c
+ *   DISPLACEMENT  DisplacementFieldTransform: y = x + u(x), u linearly
+ *                 interpolated in the field's grid, 0 outside its buffer */
+#define ANTS_TRANSFORM_IDENTITY 0
+#define ANTS_TRANSFORM_AFFINE 1
+#define ANTS_TRANSFORM_DISPLACEMENT 2
+#define ANTS_MAX_TRANSFORMS 8
+typedef struct ants_transform {
+  int kind;
+  double matrix[9];       /* A, row-major dim x dim */
+  double translation[3];  /* t */
+  double center[3];       /* c (ITK fixed parameters) */
+  ants_image_desc field;  /* DISPLACEMENT: geometry of the field */
+  const float* d_field;   /* DISPLACEMENT: device buffer, dim interleaved physical components */
+} ants_transform;
+
+/* Interpolators (-n, make_interpolator_snip.tmpl): Linear, NearestNeighbor
+ * (half-integers round up), GenericLabel[Linear] (the label whose indicator
+ * image interpolates highest; ties go to the smaller label). */
+#define ANTS_INTERP_LINEAR 0
+#define ANTS_INTERP_NEAREST 1
+#define ANTS_INTERP_GENERIC_LABEL 2
+
+/* Resample d_in (grid `in`) onto grid `ref` (ResampleImageFilter): each
+ * reference point p is mapped through transforms[0], then transforms[1], ...
+ * -- the order of the -t options on the command line (the parser stores them
+ * reversed and the CompositeTransform applies the last added first) -- and
+ * the input is interpolated there, or default_value where the point falls
+ * outside the input buffer.  Coordinates are formed in double.  Directions
+ * may be any orthonormal matrices. */
+int ants_apply_transforms(const ants_image_desc* ref, const ants_image_desc* in, const float* d_in,
+                          const ants_transform* transforms, int n_transforms, int interpolation,
+                          double default_value, float* d_out, void* stream);
+
 /* ---- batched and served registration ----
  * No counterpart inside ANTs: the reference runs independent registrations
  * as separate processes or cluster jobs

@@ -335,3 +335,72 @@ def syn_register(params: SynParams, gF: Grid, fixed, gM: Grid, moving, precision
 
 def max_threads() -> int:
     return lib().og_max_threads()
+
+
+# --- antsApplyTransforms subset -------------------------------------------
+
+class OgGeom(C.Structure):
+    _fields_ = [("dim", C.c_int), ("n", C.c_int64 * 3), ("spacing", C.c_double * 3), ("origin", C.c_double * 3),
+                ("direction", C.c_double * 9)]
+
+
+class OgXform(C.Structure):
+    _fields_ = [("kind", C.c_int), ("matrix", C.c_double * 9), ("translation", C.c_double * 3),
+                ("center", C.c_double * 3), ("field", OgGeom), ("f", C.POINTER(C.c_float))]
+
+
+INTERP = {"linear": 0, "nearestneighbor": 1, "genericlabel": 2}
+
+
+def og_geom(size, spacing=None, origin=None, direction=None) -> OgGeom:
+    """size in (x, y[, z]) order; direction row-major dim x dim (identity by default)."""
+    D = len(size)
+    g = OgGeom()
+    g.dim = D
+    for k in range(3):
+        g.n[k] = int(size[k]) if k < D else 1
+        g.spacing[k] = float(spacing[k]) if spacing is not None and k < D else 1.0
+        g.origin[k] = float(origin[k]) if origin is not None and k < D else 0.0
+    dm = np.eye(D) if direction is None else np.asarray(direction, dtype=np.float64).reshape(D, D)
+    for k, v in enumerate(dm.ravel()):
+        g.direction[k] = float(v)
+    return g
+
+
+def apply_transforms(ref: OgGeom, inp: OgGeom, img, transforms=(), interpolation="linear", default=0.0):
+    """transforms: sequence of ("affine", A, t, c) / ("field", OgGeom, field(z,y,x,D)) / ("identity",),
+    applied to each reference point in order.  Returns float32 on the reference grid (numpy order)."""
+    D = ref.dim
+    a = np.ascontiguousarray(img, dtype=np.float32)
+    xs = (OgXform * max(len(transforms), 1))()
+    keep = []
+    for i, t in enumerate(transforms):
+        if t[0] == "identity":
+            xs[i].kind = 0
+        elif t[0] == "affine":
+            xs[i].kind = 1
+            A, tr, ce = np.asarray(t[1], float).reshape(D, D), np.asarray(t[2], float), np.asarray(t[3], float)
+            for k, v in enumerate(A.ravel()):
+                xs[i].matrix[k] = v
+            for k in range(D):
+                xs[i].translation[k] = tr[k]
+                xs[i].center[k] = ce[k]
+        elif t[0] == "field":
+            xs[i].kind = 2
+            xs[i].field = t[1]
+            f = np.ascontiguousarray(t[2], dtype=np.float32)
+            keep.append(f)
+            xs[i].f = f.ctypes.data_as(C.POINTER(C.c_float))
+        else:
+            raise ValueError(t[0])
+    shape = tuple(int(ref.n[k]) for k in range(D))[::-1]
+    out = np.empty(shape, np.float32)
+    fn = lib().og_apply_transforms_f64
+    fn.restype = C.c_int
+    fn.argtypes = [C.POINTER(OgGeom), C.POINTER(OgGeom), C.POINTER(C.c_float), C.POINTER(OgXform), C.c_int, C.c_int,
+                   C.c_double, C.POINTER(C.c_float)]
+    rc = fn(C.byref(ref), C.byref(inp), a.ctypes.data_as(C.POINTER(C.c_float)), xs, len(transforms),
+            INTERP[interpolation.lower()], float(default), out.ctypes.data_as(C.POINTER(C.c_float)))
+    if rc != 0:
+        raise ValueError("dimension mismatch")
+    return out

@@ -801,8 +801,203 @@ int syn_register(const og_params* P, const Grid& gF, const R* fixed_in, const Gr
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// antsApplyTransforms subset (Examples/antsApplyTransforms.cxx:546-640):
+// itk::ResampleImageFilter over the reference grid, with a composite of
+// MatrixOffsetTransformBase (y = A (x - c) + t + c) and
+// DisplacementFieldTransform (y = x + u(x), u by VectorLinearInterpolate-
+// ImageFunction, zero outside its buffer) applied in command-line order
+// (itkantsRegistrationHelper.h:1262-1455 adds them from the last -t option
+// to the first; itk::CompositeTransform applies the last added first).
+// Interpolation: LinearInterpolateImageFunction, NearestNeighbor (half-up
+// rounding) or GenericLabel[Linear] (Schaerer et al., "A generic interpolator
+// for multi-label images", Insight Journal 2014: the label whose binary
+// indicator image interpolates highest; labels iterate in ascending order
+// with a strict comparison, so ties go to the smaller label).  Points whose
+// continuous index lies outside [-0.5, n-0.5) get the default value.
+struct og_geom {
+  int dim;
+  int64_t n[3];
+  double spacing[3];
+  double origin[3];
+  double direction[9];  // row-major dim x dim
+};
+struct og_xform {
+  int kind;  // 0 identity, 1 affine, 2 displacement field
+  double matrix[9];
+  double translation[3];
+  double center[3];
+  og_geom field;
+  const float* f;
+};
+
+namespace {
+
+// point = origin + sum_c (direction[r][c] * spacing[c]) * index[c]
+void geom_index_to_point(const og_geom& g, const double idx[3], double p[3]) {
+  const int D = g.dim;
+  for (int r = 0; r < 3; ++r) {
+    if (r >= D) {
+      p[r] = 0.0;
+      continue;
+    }
+    double s = g.origin[r];
+    for (int c = 0; c < D; ++c) {
+      const double m = g.direction[r * D + c] * g.spacing[c];
+      s = s + m * idx[c];
+    }
+    p[r] = s;
+  }
+}
+
+// index = diag(1/spacing) direction^T (point - origin)
+void geom_point_to_index(const og_geom& g, const double p[3], double c[3]) {
+  const int D = g.dim;
+  double q[3] = {0.0, 0.0, 0.0};
+  for (int k = 0; k < D; ++k) q[k] = p[k] - g.origin[k];
+  for (int r = 0; r < 3; ++r) {
+    if (r >= D) {
+      c[r] = 0.0;
+      continue;
+    }
+    double s = 0.0;
+    for (int k = 0; k < D; ++k) {
+      const double mi = g.direction[k * D + r] / g.spacing[r];
+      s = s + mi * q[k];
+    }
+    c[r] = s;
+  }
+}
+
+bool geom_inside(const og_geom& g, const double c[3]) {
+  for (int d = 0; d < g.dim; ++d)
+    if (!(c[d] >= -0.5 && c[d] < static_cast<double>(g.n[d]) - 0.5)) return false;
+  return true;
+}
+
+// trilinear (bilinear) cell: corner flat indices and weights, upper neighbour
+// clamped to the buffer; corner k has bit 0 = x, bit 1 = y, bit 2 = z
+void geom_cell(const og_geom& g, const double c[3], int64_t idx[8], double w[3]) {
+  const int D = g.dim;
+  int64_t lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  for (int d = 0; d < 3; ++d) {
+    w[d] = 0.0;
+    if (d >= D) continue;
+    const double fl = std::floor(c[d]);
+    int64_t b = static_cast<int64_t>(fl);
+    double t = c[d] - fl;
+    if (b < 0) {
+      b = 0;
+      t = 0.0;
+    }
+    lo[d] = b;
+    hi[d] = b + 1 < g.n[d] ? b + 1 : b;
+    w[d] = t;
+  }
+  for (int k = 0; k < (1 << D); ++k) {
+    const int64_t x = (k & 1) ? hi[0] : lo[0], y = (k & 2) ? hi[1] : lo[1], z = (k & 4) ? hi[2] : lo[2];
+    idx[k] = x + g.n[0] * (y + (D == 3 ? g.n[1] * z : 0));
+  }
+}
+
+// interpolate corner values: along x, then y, then z
+double cell_lerp(int D, const double v[8], const double w[3]) {
+  const double a00 = v[0] + (v[1] - v[0]) * w[0];
+  const double a10 = v[2] + (v[3] - v[2]) * w[0];
+  const double b0 = a00 + (a10 - a00) * w[1];
+  if (D == 2) return b0;
+  const double a01 = v[4] + (v[5] - v[4]) * w[0];
+  const double a11 = v[6] + (v[7] - v[6]) * w[0];
+  const double b1 = a01 + (a11 - a01) * w[1];
+  return b0 + (b1 - b0) * w[2];
+}
+
+void xform_point(const og_xform& T, int D, double p[3]) {
+  if (T.kind == 1) {
+    double q[3] = {0.0, 0.0, 0.0};
+    for (int r = 0; r < D; ++r) {
+      double ac = 0.0;
+      for (int k = 0; k < D; ++k) ac = ac + T.matrix[r * D + k] * T.center[k];
+      const double offset = T.translation[r] + T.center[r] - ac;
+      double s = 0.0;
+      for (int k = 0; k < D; ++k) s = s + T.matrix[r * D + k] * p[k];
+      q[r] = s + offset;
+    }
+    for (int r = 0; r < D; ++r) p[r] = q[r];
+  } else if (T.kind == 2) {
+    double c[3];
+    geom_point_to_index(T.field, p, c);
+    if (!geom_inside(T.field, c)) return;
+    int64_t idx[8];
+    double w[3];
+    geom_cell(T.field, c, idx, w);
+    for (int d = 0; d < D; ++d) {
+      double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      for (int k = 0; k < (1 << D); ++k) v[k] = static_cast<double>(T.f[idx[k] * D + d]);
+      p[d] = p[d] + cell_lerp(D, v, w);
+    }
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
 // C entry points (ctypes).  `_f64` is the FP64 oracle, `_f32` the float model.
 extern "C" {
+
+// interp: 0 linear, 1 nearest neighbour, 2 generic label
+int og_apply_transforms_f64(const og_geom* ref, const og_geom* in, const float* img, const og_xform* xf, int nt,
+                            int interp, double defv, float* out) {
+  const int D = ref->dim;
+  if (in->dim != D) return -1;
+  const int64_t N = ref->n[0] * ref->n[1] * (D == 3 ? ref->n[2] : 1);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < N; ++i) {
+    const int64_t x = i % ref->n[0], y = (i / ref->n[0]) % ref->n[1], z = i / (ref->n[0] * ref->n[1]);
+    const double id[3] = {static_cast<double>(x), static_cast<double>(y), static_cast<double>(z)};
+    double p[3];
+    geom_index_to_point(*ref, id, p);
+    for (int t = 0; t < nt; ++t) xform_point(xf[t], D, p);
+    double c[3];
+    geom_point_to_index(*in, p, c);
+    if (!geom_inside(*in, c)) {
+      out[i] = static_cast<float>(defv);
+      continue;
+    }
+    if (interp == 1) {
+      int64_t q[3] = {0, 0, 0};
+      for (int d = 0; d < D; ++d) q[d] = static_cast<int64_t>(std::floor(c[d] + 0.5));
+      out[i] = img[q[0] + in->n[0] * (q[1] + in->n[1] * q[2])];
+      continue;
+    }
+    int64_t idx[8];
+    double w[3];
+    geom_cell(*in, c, idx, w);
+    float lab[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < (1 << D); ++k) lab[k] = img[idx[k]];
+    if (interp == 0) {
+      double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      for (int k = 0; k < (1 << D); ++k) v[k] = static_cast<double>(lab[k]);
+      out[i] = static_cast<float>(cell_lerp(D, v, w));
+      continue;
+    }
+    // generic label: labels in ascending order, strict comparison
+    std::vector<float> labels(lab, lab + (1 << D));
+    std::sort(labels.begin(), labels.end());
+    float best = labels[0];
+    double bestv = -1.0;
+    for (float l : labels) {
+      double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      for (int k = 0; k < (1 << D); ++k) v[k] = lab[k] == l ? 1.0 : 0.0;
+      const double val = cell_lerp(D, v, w);
+      if (val > bestv) {
+        bestv = val;
+        best = l;
+      }
+    }
+    out[i] = best;
+  }
+  return 0;
+}
 
 int og_gaussian_kernel(double variance, double max_error, int64_t max_width, double* out, int max_out) {
   std::vector<double> h = gaussian_half_kernel(variance, max_error, max_width);

@@ -87,7 +87,7 @@ def test_ctypes_structs_match_c_layout(tmp_path):
     from ants_amd import serving as S
     structs = {"ants_image_desc": L.ImageDesc, "ants_syn_params": L.SynParamsC, "ants_syn_stats": L.SynStatsC,
                "ants_syn_kernel_stats": L.KernelStatsC, "ants_batch_config": S.BatchConfigC,
-               "ants_syn_request": S.RequestC, "ants_request_result": S.ResultC}
+               "ants_syn_request": S.RequestC, "ants_request_result": S.ResultC, "ants_transform": L.TransformC}
     exprs, want = [], []
     for cname, py in structs.items():
         exprs.append(f"sizeof({cname})")

@@ -1,0 +1,153 @@
+"""Pins of the oracle's antsApplyTransforms restatement (CPU only):
+identity, integer shifts, scipy linear interpolation under general affines
+with directions, field/affine equivalence, composition order, half-up
+nearest neighbour, and a brute-force GenericLabel (per-label indicator
+images interpolated by scipy, argmax with ties to the smaller label)."""
+import numpy as np
+import pytest
+from scipy import ndimage
+
+import oracle as o
+
+rng = np.random.default_rng(7)
+
+
+def rot3(a, b, c):
+    ca, sa, cb, sb, cc, sc = np.cos(a), np.sin(a), np.cos(b), np.sin(b), np.cos(c), np.sin(c)
+    rx = np.array([[1, 0, 0], [0, ca, -sa], [0, sa, ca]])
+    ry = np.array([[cb, 0, sb], [0, 1, 0], [-sb, 0, cb]])
+    rz = np.array([[cc, -sc, 0], [sc, cc, 0], [0, 0, 1]])
+    return rz @ ry @ rx
+
+
+def points(size, sp, org, dirm):
+    """physical points of every voxel, numpy (z,y,x) order, last axis = (x,y,z)"""
+    D = len(size)
+    idx = np.stack(np.meshgrid(*[np.arange(n, dtype=np.float64) for n in reversed(size)], indexing="ij"), -1)[..., ::-1]
+    return np.asarray(org, float) + idx @ (np.asarray(dirm, float) * np.asarray(sp, float)).T
+
+
+def cidx(p, size, sp, org, dirm):
+    return ((p - np.asarray(org, float)) @ np.asarray(dirm, float)) / np.asarray(sp, float)
+
+
+def inside(c, size):
+    return np.all((c >= -0.5) & (c < np.asarray(size, float) - 0.5), axis=-1)
+
+
+def scipy_linear(img, c):
+    coords = [c[..., d] for d in reversed(range(c.shape[-1]))]
+    return ndimage.map_coordinates(img.astype(np.float64), coords, order=1, mode="nearest")
+
+
+@pytest.mark.parametrize("interp", ["linear", "nearestneighbor", "genericlabel"])
+def test_identity_copies(interp):
+    """Rounding modes copy exactly; linear copies up to the index -> point ->
+    index round trip of non-dyadic spacings (1e-16 relative in the index)."""
+    img = rng.integers(0, 5, (6, 7, 8)).astype(np.float32) + 0.5 * rng.standard_normal((6, 7, 8)).astype(np.float32)
+    if interp != "linear":
+        img = np.round(img)
+    g = o.og_geom((8, 7, 6), (1.1, 0.9, 2.0), (3, -2, 1))
+    for tl in ([], [("identity",)]):
+        out = o.apply_transforms(g, g, img, tl, interp)
+        if interp == "linear":
+            np.testing.assert_allclose(out, img, rtol=0, atol=1e-6)
+        else:
+            np.testing.assert_array_equal(out, img)
+
+
+def test_integer_translation_shifts_with_default():
+    img = rng.standard_normal((5, 6, 7)).astype(np.float32)
+    g = o.og_geom((7, 6, 5), (2.0, 1.0, 0.5))
+    out = o.apply_transforms(g, g, img, [("affine", np.eye(3), [4.0, -1.0, 0.5], [0, 0, 0])], "linear", -9.0)
+    ref = np.full_like(img, -9.0)
+    ref[:4, 1:, :5] = img[1:, :5, 2:]   # x + 2 voxels, y - 1, z + 1
+    np.testing.assert_array_equal(out, ref)
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_general_affine_with_directions_matches_scipy(D):
+    size_in = (23, 19, 17)[:D]
+    size_ref = (20, 21, 15)[:D]
+    img = ndimage.gaussian_filter(rng.standard_normal(tuple(reversed(size_in))), 1.5).astype(np.float32)
+    R_in = rot3(0.2, -0.1, 0.3)[:D, :D] if D == 3 else rot3(0, 0, 0.25)[:2, :2]
+    R_ref = rot3(-0.05, 0.15, -0.2)[:D, :D] if D == 3 else rot3(0, 0, -0.1)[:2, :2]
+    sp_in, org_in = (1.1, 0.9, 1.3)[:D], (-3.0, 2.0, 1.0)[:D]
+    sp_ref, org_ref = (1.2, 1.0, 1.4)[:D], (-1.0, 1.0, 2.0)[:D]
+    A = np.eye(D) * 1.05 + 0.03 * rng.standard_normal((D, D))
+    t, c = 0.7 * rng.standard_normal(D), np.array([5.0, 6.0, 4.0])[:D]
+    gi, gr = o.og_geom(size_in, sp_in, org_in, R_in), o.og_geom(size_ref, sp_ref, org_ref, R_ref)
+    out = o.apply_transforms(gr, gi, img, [("affine", A, t, c)], "linear", -5.0)
+    p = points(size_ref, sp_ref, org_ref, R_ref)
+    q = (p - c) @ A.T + t + c
+    ci = cidx(q, size_in, sp_in, org_in, R_in)
+    m = inside(ci, size_in)
+    assert m.mean() > 0.2
+    ref = np.where(m, scipy_linear(img, ci), -5.0)
+    np.testing.assert_allclose(out, ref, atol=2e-6)
+
+
+def test_constant_field_equals_translation_and_order_matters():
+    img = ndimage.gaussian_filter(rng.standard_normal((12, 14, 16)), 1.0).astype(np.float32)
+    g = o.og_geom((16, 14, 12), (1.0, 1.5, 0.8), (2.0, -1.0, 0.5))
+    u = np.array([1.3, -0.4, 0.7])
+    fld = np.broadcast_to(u, (12, 14, 16, 3)).astype(np.float32)
+    by_field = o.apply_transforms(g, g, img, [("field", g, fld)], "linear")
+    by_affine = o.apply_transforms(g, g, img, [("affine", np.eye(3), u.astype(np.float32), [0, 0, 0])], "linear")
+    np.testing.assert_array_equal(by_field, by_affine)
+    # composition: [field, affine] maps p -> A(p + u); [affine, field] maps p -> A p + u
+    A = np.diag([1.1, 0.95, 1.05])
+    p = points((16, 14, 12), (1.0, 1.5, 0.8), (2.0, -1.0, 0.5), np.eye(3))
+    for order, q in ((["field", "affine"], (p + u) @ A.T), (["affine", "field"], p @ A.T + u)):
+        tl = [("field", g, fld) if k == "field" else ("affine", A, [0, 0, 0], [0, 0, 0]) for k in order]
+        out = o.apply_transforms(g, g, img, tl, "linear", 0.0)
+        ci = cidx(q, (16, 14, 12), (1.0, 1.5, 0.8), (2.0, -1.0, 0.5), np.eye(3))
+        m = inside(ci, (16, 14, 12))
+        # the field is constant inside its buffer only: keep points whose field lookup is inside
+        pf = p if order[0] == "field" else p @ A.T
+        mf = inside(cidx(pf, (16, 14, 12), (1.0, 1.5, 0.8), (2.0, -1.0, 0.5), np.eye(3)), (16, 14, 12))
+        sel = m & mf
+        np.testing.assert_allclose(out[sel], scipy_linear(img, ci)[sel], atol=2e-6)
+
+
+def test_nearest_neighbour_rounds_half_up():
+    img = np.arange(4 * 5 * 6, dtype=np.float32).reshape(4, 5, 6)
+    g = o.og_geom((6, 5, 4))
+    out = o.apply_transforms(g, g, img, [("affine", np.eye(3), [0.5, 0.0, 0.0], [0, 0, 0])], "nearestneighbor", -1)
+    ref = np.full_like(img, -1.0)
+    ref[..., :5] = img[..., 1:]        # x + 0.5 rounds up to x + 1
+    np.testing.assert_array_equal(out, ref)
+
+
+def test_direction_permutation_transposes():
+    img = rng.standard_normal((3, 7, 7)).astype(np.float32)
+    swap = np.array([[0, 1, 0], [1, 0, 0], [0, 0, 1]], float)
+    gi, gr = o.og_geom((7, 7, 3)), o.og_geom((7, 7, 3), direction=swap)
+    out = o.apply_transforms(gr, gi, img, [], "nearestneighbor")
+    np.testing.assert_array_equal(out, np.transpose(img, (0, 2, 1)))
+
+
+def test_generic_label_matches_brute_force():
+    labels = np.array([0, 3, 7, 11], np.float32)
+    coarse = rng.integers(0, 4, (5, 6, 7))
+    img = labels[ndimage.zoom(coarse, 3, order=0)][:14, :16, :18].astype(np.float32)
+    size = (18, 16, 14)
+    g = o.og_geom(size, (1.0, 1.1, 0.9))
+    A = rot3(0.1, 0.05, -0.12) * 1.02
+    t, c = np.array([0.6, -0.3, 0.4]), np.array([9.0, 8.0, 6.0])
+    out = o.apply_transforms(g, g, img, [("affine", A, t, c)], "genericlabel", -1)
+    p = points(size, (1.0, 1.1, 0.9), (0, 0, 0), np.eye(3))
+    ci = cidx((p - c) @ A.T + t + c, size, (1.0, 1.1, 0.9), (0, 0, 0), np.eye(3))
+    m = inside(ci, size)
+    vals = np.stack([scipy_linear((img == l).astype(np.float64), ci) for l in labels], -1)
+    ref = np.where(m, labels[np.argmax(vals, -1)], -1.0)   # argmax: first (smallest) label on ties
+    srt = np.sort(vals, -1)
+    clear = (srt[..., -1] - srt[..., -2] > 1e-9) | ~m
+    assert clear.mean() > 0.95
+    np.testing.assert_array_equal(out[clear], ref[clear])
+    # exact ties (a point midway between two labels) go to the smaller label
+    two = np.zeros((1, 1, 2), np.float32)
+    two[..., 1] = 5.0
+    g2 = o.og_geom((2, 1, 1))
+    half = o.apply_transforms(g2, g2, two, [("affine", np.eye(3), [0.5, 0, 0], [0, 0, 0])], "genericlabel", -1)
+    assert half[0, 0, 0] == 0.0
