@@ -1357,17 +1357,23 @@ int geom_from_desc(const ants_image_desc* d, GeomDev* g) {
     if (k < D && !(d->spacing[k] > 0.0)) return fail(ANTS_ERR_INVALID, "image spacing must be positive");
     g->org[k] = k < D ? d->origin[k] : 0.0;
   }
+  // an all-zero direction means identity (header convention)
+  double dir[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  bool all_zero = true;
+  for (int i = 0; i < D * D; ++i)
+    if (d->direction[i] != 0.0) all_zero = false;
+  for (int i = 0; i < D * D; ++i) dir[i] = all_zero ? (i % (D + 1) == 0 ? 1.0 : 0.0) : d->direction[i];
   for (int r = 0; r < D; ++r)
     for (int c = 0; c < D; ++c) {
       double dot = 0.0;
-      for (int k = 0; k < D; ++k) dot += d->direction[k * D + r] * d->direction[k * D + c];
+      for (int k = 0; k < D; ++k) dot += dir[k * D + r] * dir[k * D + c];
       if (std::fabs(dot - (r == c ? 1.0 : 0.0)) > 1e-6)
         return fail(ANTS_ERR_UNSUPPORTED, "image direction must be orthonormal");
     }
   for (int r = 0; r < D; ++r)
     for (int c = 0; c < D; ++c) {
-      g->m[r * 3 + c] = d->direction[r * D + c] * d->spacing[c];
-      g->minv[r * 3 + c] = d->direction[c * D + r] / d->spacing[r];
+      g->m[r * 3 + c] = dir[r * D + c] * d->spacing[c];
+      g->minv[r * 3 + c] = dir[c * D + r] / d->spacing[r];
     }
   return ANTS_OK;
 }
@@ -1421,7 +1427,6 @@ extern "C" int ants_apply_transforms(const ants_image_desc* ref, const ants_imag
   hipStream_t s = static_cast<hipStream_t>(stream);
   launch_apply_transforms(go, P, interpolation, d_in, d_out, s);
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipStreamSynchronize(s));
   return ANTS_OK;
 }
 

@@ -253,7 +253,8 @@ typedef struct ants_transform {
  * reversed and the CompositeTransform applies the last added first) -- and
  * the input is interpolated there, or default_value where the point falls
  * outside the input buffer.  Coordinates are formed in double.  Directions
- * may be any orthonormal matrices. */
+ * may be any orthonormal matrices.  Asynchronous on `stream`; the transform
+ * array is read before the call returns. */
 int ants_apply_transforms(const ants_image_desc* ref, const ants_image_desc* in, const float* d_in,
                           const ants_transform* transforms, int n_transforms, int interpolation,
                           double default_value, float* d_out, void* stream);

@@ -86,3 +86,12 @@ def test_rejects_bad_arguments():
     bad = AT.Geometry(gi.size, gi.spacing, gi.origin, np.array([[1.0, 0.3], [0.0, 1.0]]))
     with pytest.raises(Exception):
         AT.apply_transforms(gr, img, bad, [], "Linear")
+
+
+def test_zero_direction_means_identity():
+    gi, gr, img, xf, _ = case(3, "Linear")
+    g_id = AT.Geometry(gi.size, gi.spacing, gi.origin, np.eye(3))
+    g_zero = AT.Geometry(gi.size, gi.spacing, gi.origin, np.zeros((3, 3)))
+    a = AT.apply_transforms(g_id, img, g_id, xf[1:2], "Linear")
+    b = AT.apply_transforms(g_zero, img, g_zero, xf[1:2], "Linear")
+    np.testing.assert_array_equal(a, b)
