@@ -1,0 +1,135 @@
+"""Command lines over the library, reading and writing files as ANTs does.
+
+``python -m ants_amd.cli registration -d 3 -m CC[fixed.nii.gz,moving.nii.gz,1,4]
+  -t SyN[0.1,3,0] -c [100x70x20,1e-6,10] -f 4x2x1 -s 2x1x0vox
+  -o [out_,out_Warped.nii.gz,out_InverseWarped.nii.gz]``
+runs one SyN stage with the CC metric (the antsRegistration subset this
+library replaces) and writes ``out_0Warp.nii.gz`` and
+``out_0InverseWarp.nii.gz`` (naming of Examples/
+antsRegistrationTemplateHeader.cxx:74-89) plus the optional warped images.
+
+``python -m ants_amd.cli apply -d 3 -i moving.nii.gz -r fixed.nii.gz
+  -o out.nii.gz -t out_0Warp.nii.gz -t [affine.mat,1] -n Linear
+  --default-value 0``
+is the antsApplyTransforms subset (Examples/antsApplyTransforms.cxx).
+Transforms ending in .mat/.txt/.tfm are affine files; other paths are
+displacement-field images.
+"""
+from __future__ import annotations
+
+import argparse
+import re
+import sys
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import io as aio
+from .apply_transforms import AffineTransform, DisplacementFieldTransform, Geometry, apply_transforms
+
+
+def _bracket_list(s: str) -> List[str]:
+    m = re.fullmatch(r"\s*\[(.*)\]\s*", s)
+    return [p.strip() for p in (m.group(1) if m else s).split(",")]
+
+
+def load_transform(spec: str):
+    """-t path | -t [path,invert]"""
+    parts = _bracket_list(spec)
+    path = parts[0]
+    invert = len(parts) > 1 and parts[1] not in ("0", "false", "False", "")
+    if path.lower() == "identity":
+        return None
+    if path.endswith((".mat", ".txt", ".tfm")):
+        t = aio.read_transform(path)
+        t.invert = invert
+        return t
+    if invert:
+        raise ValueError(f"{path}: a displacement field cannot be inverted here (use its InverseWarp file)")
+    fld, g = aio.read_nifti(path)
+    return DisplacementFieldTransform(np.asarray(fld, np.float32), g)
+
+
+def cmd_apply(argv: Sequence[str]) -> int:
+    ap = argparse.ArgumentParser(prog="apply")
+    ap.add_argument("-d", "--dimensionality", type=int, default=None)
+    ap.add_argument("-i", "--input", required=True)
+    ap.add_argument("-r", "--reference-image", required=True)
+    ap.add_argument("-o", "--output", required=True)
+    ap.add_argument("-t", "--transform", action="append", default=[])
+    ap.add_argument("-n", "--interpolation", default="Linear")
+    ap.add_argument("--default-value", type=float, default=0.0)
+    ap.add_argument("-v", "--verbose", type=int, default=0)
+    a = ap.parse_args(argv)
+    img, gi = aio.read_nifti(a.input)
+    ref, gr = aio.read_nifti(a.reference_image)
+    if a.dimensionality and a.dimensionality != gi.dim:
+        raise SystemExit(f"-d {a.dimensionality} but the input is {gi.dim}-D")
+    xf = [load_transform(t) for t in a.transform]
+    out = apply_transforms(gr, np.asarray(img, np.float32), gi, xf, a.interpolation, a.default_value)
+    if a.interpolation.lower().startswith(("nearest", "genericlabel")) and np.issubdtype(np.asarray(img).dtype,
+                                                                                         np.integer):
+        out = out.astype(np.asarray(img).dtype)
+    aio.write_nifti(a.output, out, gr)
+    return 0
+
+
+def cmd_registration(argv: Sequence[str]) -> int:
+    from .registration import SyNRegistration, SyNStage
+    ap = argparse.ArgumentParser(prog="registration")
+    ap.add_argument("-d", "--dimensionality", type=int, required=True)
+    ap.add_argument("-m", "--metric", required=True)
+    ap.add_argument("-t", "--transform", required=True)
+    ap.add_argument("-c", "--convergence", required=True)
+    ap.add_argument("-f", "--shrink-factors", required=True)
+    ap.add_argument("-s", "--smoothing-sigmas", required=True)
+    ap.add_argument("-o", "--output", required=True)
+    ap.add_argument("--float", default="1")
+    ap.add_argument("--precision", choices=["fp32", "bf16"], default="fp32")
+    ap.add_argument("-v", "--verbose", type=int, default=0)
+    a = ap.parse_args(argv)
+    stage = SyNStage.from_args(a.transform, a.metric, a.convergence, a.shrink_factors, a.smoothing_sigmas)
+    mparts = _bracket_list(re.sub(r"^\s*CC", "", a.metric))
+    fixed, gf = aio.read_nifti(mparts[0])
+    moving, gm = aio.read_nifti(mparts[1])
+    for g, name in ((gf, mparts[0]), (gm, mparts[1])):
+        if g.dim != a.dimensionality:
+            raise SystemExit(f"{name} is {g.dim}-D, -d {a.dimensionality}")
+        if not np.allclose(g.direction, np.eye(g.dim), atol=1e-6):
+            raise SystemExit(f"{name}: non-identity image directions are not supported by the SyN engine")
+    outs = _bracket_list(a.output)
+    prefix = outs[0]
+    reg = SyNRegistration(stage, a.dimensionality, precision=a.precision)
+    res = reg.register(np.asarray(fixed, np.float32), np.asarray(moving, np.float32), gf.spacing, gf.origin,
+                       gm.spacing, gm.origin)
+    lg = reg.level_geometry(gf.size, gf.spacing, gf.origin)  # fields live on the last level's grid
+    reg.close()
+    if a.verbose:
+        for ln in res.diagnostic_lines():
+            print(ln)
+    D = gf.dim
+    g_out = Geometry(tuple(int(lg.size[k]) for k in range(D)), tuple(lg.spacing[k] for k in range(D)),
+                     tuple(lg.origin[k] for k in range(D)), np.eye(D))
+    aio.write_nifti(prefix + "0Warp.nii.gz", res.fwd, g_out, vector=True)
+    aio.write_nifti(prefix + "0InverseWarp.nii.gz", res.inv, g_out, vector=True)
+    if len(outs) > 1 and outs[1]:
+        warped = apply_transforms(gf, np.asarray(moving, np.float32), gm,
+                                  [DisplacementFieldTransform(res.fwd, g_out)], "Linear")
+        aio.write_nifti(outs[1], warped, gf)
+    if len(outs) > 2 and outs[2]:
+        inv_warped = apply_transforms(gm, np.asarray(fixed, np.float32), gf,
+                                      [DisplacementFieldTransform(res.inv, g_out)], "Linear")
+        aio.write_nifti(outs[2], inv_warped, gm)
+    return 0
+
+
+def main(argv: Optional[Sequence[str]] = None) -> int:
+    argv = list(sys.argv[1:] if argv is None else argv)
+    if not argv or argv[0] not in ("registration", "apply"):
+        print(__doc__)
+        return 2
+    return {"registration": cmd_registration, "apply": cmd_apply}[argv[0]](argv[1:])
+
+
+if __name__ == "__main__":
+    sys.exit(main())

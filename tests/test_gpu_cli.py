@@ -1,0 +1,48 @@
+"""The command lines end to end from NIfTI files: registration writes the
+ANTs-named warp files, apply reads them back; results equal the library
+calls on the same arrays."""
+import numpy as np
+import pytest
+
+from ants_amd import cli
+from ants_amd import io as aio
+from ants_amd import phantoms as ph
+from ants_amd.apply_transforms import DisplacementFieldTransform, Geometry, apply_transforms
+from ants_amd.registration import SyNRegistration, SyNStage
+
+pytestmark = pytest.mark.gpu
+
+
+def test_registration_and_apply_from_files(tmp_path):
+    pair = ph.phantom_pair_3d(2, size=(48, 56, 40), max_disp_mm=4.0, svf_sigma_mm=6.0)
+    g = Geometry(pair.size, pair.spacing, (1.0, -2.0, 3.0), np.eye(3))
+    f, m = str(tmp_path / "fixed.nii.gz"), str(tmp_path / "moving.nii.gz")
+    aio.write_nifti(f, pair.fixed, g)
+    aio.write_nifti(m, pair.moving, g)
+    aio.write_nifti(str(tmp_path / "labels.nii.gz"), pair.labels_moving.astype(np.int16), g)
+    prefix = str(tmp_path / "out_")
+    args = ["-d", "3", "-m", f"CC[{f},{m},1,2]", "-t", "SyN[0.2,3,0]", "-c", "[10x5x2,1e-6,10]", "-f", "4x2x1",
+            "-s", "2x1x0vox", "-o", f"[{prefix},{prefix}Warped.nii.gz]"]
+    assert cli.main(["registration"] + args) == 0
+    fwd, gw = aio.read_nifti(prefix + "0Warp.nii.gz")
+    inv, _ = aio.read_nifti(prefix + "0InverseWarp.nii.gz")
+    assert fwd.shape == g.shape + (3,) and gw.size == g.size
+    # the library on the same arrays gives the same fields
+    st = SyNStage.from_args("SyN[0.2,3,0]", "CC[f,m,1,2]", "[10x5x2,1e-6,10]", "4x2x1", "2x1x0vox")
+    reg = SyNRegistration(st, 3)
+    res = reg.register(pair.fixed, pair.moving, g.spacing, g.origin, g.spacing, g.origin)
+    reg.close()
+    np.testing.assert_array_equal(fwd, res.fwd)
+    np.testing.assert_array_equal(inv, res.inv)
+    warped, _ = aio.read_nifti(prefix + "Warped.nii.gz")
+    np.testing.assert_array_equal(warped, apply_transforms(g, pair.moving, g, [DisplacementFieldTransform(fwd, g)]))
+    # apply: labels through the warp, GenericLabel, integer type kept
+    out = str(tmp_path / "labels_warped.nii.gz")
+    assert cli.main(["apply", "-d", "3", "-i", str(tmp_path / "labels.nii.gz"), "-r", f, "-o", out,
+                     "-t", prefix + "0Warp.nii.gz", "-n", "GenericLabel[Linear]"]) == 0
+    lw, _ = aio.read_nifti(out)
+    assert lw.dtype == np.int16
+    ref = apply_transforms(g, pair.labels_moving.astype(np.float32), g, [DisplacementFieldTransform(fwd, g)],
+                           "GenericLabel")
+    np.testing.assert_array_equal(lw, ref.astype(np.int16))
+    assert set(np.unique(lw)) <= set(np.unique(pair.labels_moving))
