@@ -95,21 +95,19 @@ def cmd_registration(argv: Sequence[str]) -> int:
     for g, name in ((gf, mparts[0]), (gm, mparts[1])):
         if g.dim != a.dimensionality:
             raise SystemExit(f"{name} is {g.dim}-D, -d {a.dimensionality}")
-        if not np.allclose(g.direction, np.eye(g.dim), atol=1e-6):
-            raise SystemExit(f"{name}: non-identity image directions are not supported by the SyN engine")
     outs = _bracket_list(a.output)
     prefix = outs[0]
     reg = SyNRegistration(stage, a.dimensionality, precision=a.precision)
     res = reg.register(np.asarray(fixed, np.float32), np.asarray(moving, np.float32), gf.spacing, gf.origin,
-                       gm.spacing, gm.origin)
-    lg = reg.level_geometry(gf.size, gf.spacing, gf.origin)  # fields live on the last level's grid
+                       gm.spacing, gm.origin, gf.direction, gm.direction)
+    lg = reg.level_geometry(gf.size, gf.spacing, gf.origin, direction=gf.direction)  # the fields' grid
     reg.close()
     if a.verbose:
         for ln in res.diagnostic_lines():
             print(ln)
     D = gf.dim
     g_out = Geometry(tuple(int(lg.size[k]) for k in range(D)), tuple(lg.spacing[k] for k in range(D)),
-                     tuple(lg.origin[k] for k in range(D)), np.eye(D))
+                     tuple(lg.origin[k] for k in range(D)), np.array([lg.direction[k] for k in range(D * D)]).reshape(D, D))
     aio.write_nifti(prefix + "0Warp.nii.gz", res.fwd, g_out, vector=True)
     aio.write_nifti(prefix + "0InverseWarp.nii.gz", res.inv, g_out, vector=True)
     if len(outs) > 1 and outs[1]:

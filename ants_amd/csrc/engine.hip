@@ -131,30 +131,41 @@ int gaussian_taps(double variance, double max_error, int64_t max_width, Taps* t)
 }
 
 // ---------------------------------------------------------------------------
+// Image geometry: point = org + dir * (index .* sp) (itk::ImageBase).
 struct HostGrid {
   int D;
   int64_t n[3];
   double sp[3], org[3];
+  double dir[9];  // 3x3 row-major, identity-padded
+  bool oblique;
   int64_t count() const { return n[0] * n[1] * n[2]; }
 };
 
-bool identity_direction(const ants_image_desc* d) {
-  bool all_zero = true;
-  for (int i = 0; i < 9; ++i)
-    if (d->direction[i] != 0.0) all_zero = false;
-  if (all_zero) return true;
-  for (int r = 0; r < d->dim; ++r)
-    for (int c = 0; c < d->dim; ++c)
-      if (d->direction[r * d->dim + c] != (r == c ? 1.0 : 0.0)) return false;
-  return true;
-}
 
 int to_host_grid(const ants_image_desc* d, HostGrid* g) {
   if (!d) return fail(ANTS_ERR_INVALID, "null image descriptor");
   if (d->dim != 2 && d->dim != 3) return fail(ANTS_ERR_INVALID, "dim must be 2 or 3 (got %d)", d->dim);
-  if (!identity_direction(d))
-    return fail(ANTS_ERR_UNSUPPORTED, "non-identity image direction is not supported in this release");
   g->D = d->dim;
+  // direction: row-major dim x dim, all zero meaning identity; orthonormal
+  const int D = d->dim;
+  bool all_zero = true;
+  for (int k = 0; k < D * D; ++k)
+    if (d->direction[k] != 0.0) all_zero = false;
+  g->oblique = false;
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) {
+      const double id = a == b ? 1.0 : 0.0;
+      const double v = (a < D && b < D && !all_zero) ? d->direction[a * D + b] : id;
+      g->dir[a * 3 + b] = v;
+      if (v != id) g->oblique = true;
+    }
+  for (int a = 0; a < D; ++a)
+    for (int b = 0; b < D; ++b) {
+      double dot = 0.0;
+      for (int k = 0; k < D; ++k) dot += g->dir[k * 3 + a] * g->dir[k * 3 + b];
+      if (std::fabs(dot - (a == b ? 1.0 : 0.0)) > 1e-6)
+        return fail(ANTS_ERR_UNSUPPORTED, "image direction must be orthonormal");
+    }
   for (int k = 0; k < 3; ++k) {
     g->n[k] = k < d->dim ? d->size[k] : 1;
     g->sp[k] = k < d->dim ? d->spacing[k] : 1.0;
@@ -177,6 +188,11 @@ GridDev to_dev(const HostGrid& h) {
     g.dsp[k] = h.sp[k];
     g.dorg[k] = h.org[k];
   }
+  g.oblique = h.oblique ? 1 : 0;
+  for (int k = 0; k < 9; ++k) {
+    g.dir[k] = static_cast<float>(h.dir[k]);
+    g.ddir[k] = h.dir[k];
+  }
   set_divisors(g);
   return g;
 }
@@ -189,7 +205,8 @@ void to_desc(const HostGrid& h, ants_image_desc* d) {
     d->spacing[k] = h.sp[k];
     d->origin[k] = h.org[k];
   }
-  for (int r = 0; r < h.D; ++r) d->direction[r * h.D + r] = 1.0;
+  for (int r = 0; r < h.D; ++r)
+    for (int c = 0; c < h.D; ++c) d->direction[r * h.D + c] = h.dir[r * 3 + c];
 }
 
 // RegistrationHelper::CalculateShrinkFactorsPerDimension + ShrinkImageFilter
@@ -215,13 +232,22 @@ HostGrid level_grid(const HostGrid& g, int factor) {
     f[d] = bf;
   }
   HostGrid o = g;
+  double delta[3] = {0.0, 0.0, 0.0};  // index-aligned offset of the output origin
   for (int d = 0; d < g.D; ++d) {
     o.sp[d] = g.sp[d] * f[d];
     o.n[d] = std::max<int64_t>(1, static_cast<int64_t>(std::floor(static_cast<double>(g.n[d]) / f[d])));
     const double in_centre = g.org[d] + (g.n[d] - 1) / 2.0 * g.sp[d];
     const double out_centre = g.org[d] + (o.n[d] - 1) / 2.0 * o.sp[d];
     o.org[d] = g.org[d] + (in_centre - out_centre);
+    delta[d] = (g.n[d] - 1) / 2.0 * g.sp[d] - (o.n[d] - 1) / 2.0 * o.sp[d];
   }
+  // with a direction the centres coincide in physical space: origin + dir * delta
+  if (g.oblique)
+    for (int r = 0; r < g.D; ++r) {
+      double sum = g.org[r];
+      for (int k = 0; k < g.D; ++k) sum += g.dir[r * 3 + k] * delta[k];
+      o.org[r] = sum;
+    }
   return o;
 }
 

@@ -56,6 +56,26 @@ template <int D>
 __device__ __forceinline__ void cidx_of(const GridDev& go, int x, int y, int z, const float* u, const GridDev& gi,
                                         float c[3]) {
   const int ix[3] = {x, y, z};
+  if (go.oblique | gi.oblique) {
+    // point = org_o + dir_o (ix .* sp_o) + u ; index = (dir_i^T (point - org_i)) .* isp_i
+    float q[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+      float p = go.org[r];
+#pragma unroll
+      for (int k = 0; k < D; ++k) p += go.dir[r * 3 + k] * (static_cast<float>(ix[k]) * go.sp[k]);
+      if (u) p += u[r];
+      q[r] = p - gi.org[r];
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      float t = 0.f;
+#pragma unroll
+      for (int k = 0; k < D; ++k) t += gi.dir[k * 3 + d] * q[k];
+      c[d] = d < D ? t * gi.isp[d] : 0.f;
+    }
+    return;
+  }
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     if (d < D) {
@@ -645,6 +665,19 @@ __global__ void __launch_bounds__(kBlock) k_warp_image(GridDev gi, const float* 
 // ---------------------------------------------------------------------------
 // CentralDifferenceImageFunction component on grid values (half-voxel linear
 // samples): 0 at the last index, a half difference at index 0.
+// CentralDifferenceImageFunction with UseImageDirection: a derivative taken
+// along the index axes becomes the physical vector dir * v.
+template <int D>
+__device__ __forceinline__ void rotate_to_physical(const GridDev& g, float v[3]) {
+  float r[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int a = 0; a < D; ++a)
+#pragma unroll
+    for (int k = 0; k < D; ++k) r[a] += g.dir[a * 3 + k] * v[k];
+#pragma unroll
+  for (int a = 0; a < D; ++a) v[a] = r[a];
+}
+
 __device__ __forceinline__ float cdiff(float vm, float v0, float vp, int pos, int len, float h) {
   if (pos >= len - 1) return 0.f;
   const float up = v0 + (vp - v0) * 0.5f;
@@ -720,6 +753,7 @@ __global__ void __launch_bounds__(kBlock) k_cc_direct(GridDev g, const float* __
     cc_sum += o.cc;
     const int pos[3] = {x, y, z};
     const int64_t st[3] = {1, g.n[0], static_cast<int64_t>(g.n[0]) * g.n[1]};
+    float vm[3] = {0.f, 0.f, 0.f}, vf[3] = {0.f, 0.f, 0.f};
 #pragma unroll
     for (int d = 0; d < D; ++d) {
       float gm = 0.f, gf = 0.f;
@@ -730,8 +764,17 @@ __global__ void __launch_bounds__(kBlock) k_cc_direct(GridDev g, const float* __
         gm = cdiff(mm_, m, mp, pos[d], g.n[d], g.sp[d]);
         gf = cdiff(fm_, f, fp, pos[d], g.n[d], g.sp[d]);
       }
-      g1[i * D + d] = o.a1 * gm;
-      g2[i * D + d] = o.a2 * gf;
+      vm[d] = o.a1 * gm;
+      vf[d] = o.a2 * gf;
+    }
+    if (g.oblique) {
+      rotate_to_physical<D>(g, vm);
+      rotate_to_physical<D>(g, vf);
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      g1[i * D + d] = vm[d];
+      g2[i * D + d] = vf[d];
     }
   }
   block_partial<kBlock>(cc_sum, 0.f, partials + blockIdx.x);
@@ -911,12 +954,18 @@ __global__ void __launch_bounds__(TX* TY, kCCWavesPerEU) k_cc3d(GridDev g, const
         gfz = cdiff(zc > 0 ? rf[0] : f, f, rf[2], zc, nz, g.sp[2]);
       }
       const int64_t i = static_cast<int64_t>(zc) * plane + static_cast<int64_t>(y) * nx + x;
-      g1[i * 3 + 0] = o.a1 * rgm[0][0];
-      g1[i * 3 + 1] = o.a1 * rgm[0][1];
-      g1[i * 3 + 2] = o.a1 * gmz;
-      g2[i * 3 + 0] = o.a2 * rgf[0][0];
-      g2[i * 3 + 1] = o.a2 * rgf[0][1];
-      g2[i * 3 + 2] = o.a2 * gfz;
+      float vm[3] = {o.a1 * rgm[0][0], o.a1 * rgm[0][1], o.a1 * gmz};
+      float vf[3] = {o.a2 * rgf[0][0], o.a2 * rgf[0][1], o.a2 * gfz};
+      if (g.oblique) {
+        rotate_to_physical<3>(g, vm);
+        rotate_to_physical<3>(g, vf);
+      }
+      g1[i * 3 + 0] = vm[0];
+      g1[i * 3 + 1] = vm[1];
+      g1[i * 3 + 2] = vm[2];
+      g2[i * 3 + 0] = vf[0];
+      g2[i * 3 + 1] = vf[1];
+      g2[i * 3 + 2] = vf[2];
     }
   }
   const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
@@ -1136,6 +1185,7 @@ __global__ void __launch_bounds__(kBlock) k_resample_fields(GridDev gi, GridDev 
 template <int D>
 __global__ void __launch_bounds__(kBlock) k_labels_nn(GridDev go, const float* __restrict__ u, GridDev gl,
                                                       const int32_t* __restrict__ lab, int32_t* __restrict__ out) {
+#pragma clang fp contract(off)
   const int64_t N = go.count();
   const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   if (i >= N) return;
@@ -1144,13 +1194,34 @@ __global__ void __launch_bounds__(kBlock) k_labels_nn(GridDev go, const float* _
   const int ix[3] = {x, y, z};
   int idx[3] = {0, 0, 0};
   bool in = true;
+  if (go.oblique | gl.oblique) {
+    double q[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-  for (int d = 0; d < D; ++d) {
-    double p = go.dorg[d] + static_cast<double>(ix[d]) * go.dsp[d];
-    if (u) p += static_cast<double>(u[i * D + d]);
-    const double c = (p - gl.dorg[d]) / gl.dsp[d];
-    if (!(c >= -0.5 && c < static_cast<double>(gl.n[d]) - 0.5)) in = false;
-    idx[d] = static_cast<int>(floor(c + 0.5));
+    for (int r = 0; r < D; ++r) {
+      double p = go.dorg[r];
+#pragma unroll
+      for (int k = 0; k < D; ++k) p += go.ddir[r * 3 + k] * (static_cast<double>(ix[k]) * go.dsp[k]);
+      if (u) p += static_cast<double>(u[i * D + r]);
+      q[r] = p - gl.dorg[r];
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      double t = 0.0;
+#pragma unroll
+      for (int k = 0; k < D; ++k) t += gl.ddir[k * 3 + d] * q[k];
+      const double c = t / gl.dsp[d];
+      if (!(c >= -0.5 && c < static_cast<double>(gl.n[d]) - 0.5)) in = false;
+      idx[d] = static_cast<int>(floor(c + 0.5));
+    }
+  } else {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      double p = go.dorg[d] + static_cast<double>(ix[d]) * go.dsp[d];
+      if (u) p += static_cast<double>(u[i * D + d]);
+      const double c = (p - gl.dorg[d]) / gl.dsp[d];
+      if (!(c >= -0.5 && c < static_cast<double>(gl.n[d]) - 0.5)) in = false;
+      idx[d] = static_cast<int>(floor(c + 0.5));
+    }
   }
   int32_t v = 0;
   if (in) v = lab[idx[0] + static_cast<int64_t>(gl.n[0]) * (idx[1] + static_cast<int64_t>(gl.n[1]) * idx[2])];

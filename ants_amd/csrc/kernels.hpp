@@ -17,6 +17,12 @@ struct GridDev {
   float org[3];
   double dsp[3];   // exact geometry for double-precision coordinate paths
   double dorg[3];
+  // direction cosines (row-major 3x3, identity-padded): point = org +
+  // dir * (index .* sp); `oblique` is 0 for the identity, which keeps the
+  // per-axis arithmetic
+  int oblique;
+  float dir[9];
+  double ddir[9];
   // unsigned 32-bit division by n[0] and n[1] as multiply-high + shifts
   // (Granlund & Montgomery, "Division by invariant integers using
   // multiplication", 1994, fig. 4.1); set by set_divisors().

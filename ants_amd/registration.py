@@ -204,11 +204,12 @@ class SyNRegistration:
         except Exception:
             pass
 
-    def level_geometry(self, size, spacing=None, origin=None, level: int = -1) -> L.ImageDesc:
+    def level_geometry(self, size, spacing=None, origin=None, level: int = -1, direction=None) -> L.ImageDesc:
         lvl = level if level >= 0 else self.params.n_levels - 1
         out = L.ImageDesc()
-        L.check(L.lib().ants_syn_level_geometry(C.byref(self.params), C.byref(L.desc(size, spacing, origin)),
-                                                lvl, C.byref(out)))
+        L.check(L.lib().ants_syn_level_geometry(C.byref(self.params),
+                                                C.byref(L.desc(size, spacing, origin, direction)), lvl,
+                                                C.byref(out)))
         return out
 
     def set_profiling(self, mask: int):
@@ -236,17 +237,19 @@ class SyNRegistration:
         return L.ITER_CB(cb)
 
     def register(self, fixed: np.ndarray, moving: np.ndarray, fixed_spacing=None, fixed_origin=None,
-                 moving_spacing=None, moving_origin=None) -> SyNResult:
-        """Host arrays in (numpy order, x fastest), host fields out."""
+                 moving_spacing=None, moving_origin=None, fixed_direction=None, moving_direction=None) -> SyNResult:
+        """Host arrays in (numpy order, x fastest), host fields out (physical
+        vectors on the fixed image's last level grid).  Directions are
+        row-major dim x dim orthonormal matrices (identity by default)."""
         fixed = np.ascontiguousarray(fixed, dtype=np.float32)
         moving = np.ascontiguousarray(moving, dtype=np.float32)
         if fixed.ndim != self.dim or moving.ndim != self.dim:
             raise ValueError("image dimension does not match the stage")
         fsize = tuple(reversed(fixed.shape))
         msize = tuple(reversed(moving.shape))
-        gf = L.desc(fsize, fixed_spacing, fixed_origin)
-        gm = L.desc(msize, moving_spacing, moving_origin)
-        out = self.level_geometry(fsize, fixed_spacing, fixed_origin)
+        gf = L.desc(fsize, fixed_spacing, fixed_origin, fixed_direction)
+        gm = L.desc(msize, moving_spacing, moving_origin, moving_direction)
+        out = self.level_geometry(fsize, fixed_spacing, fixed_origin, direction=fixed_direction)
         oshape = tuple(int(out.size[k]) for k in reversed(range(self.dim))) + (self.dim,)
         fwd = np.empty(oshape, np.float32)
         inv = np.empty(oshape, np.float32)

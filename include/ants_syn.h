@@ -6,9 +6,9 @@
  * `--transform SyN[step,updateVariance,totalVariance]`
  * `--metric CC[fixed,moving,1,radius]` with `-c/-f/-s` multi-resolution
  * schedules.  Images are dense x-fastest float buffers with ITK's geometry
- * (size, spacing, origin; identity direction in this release).  Displacement
- * fields are ITK `Image<Vector<float,D>,D>` buffers: D interleaved float
- * components per voxel (x fastest), physical units (mm).
+ * (size, spacing, origin, orthonormal direction).  Displacement fields are
+ * ITK `Image<Vector<float,D>,D>` buffers: D interleaved float components per
+ * voxel (x fastest), physical vectors (mm).
  *
  * Conventions: every function returns ANTS_OK (0) or an error code; no C++
  * exception crosses the ABI.  ants_last_error() returns the calling thread's

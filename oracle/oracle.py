@@ -19,7 +19,8 @@ LIB_PATH = os.path.join(_HERE, "libsyn_oracle.so")
 
 
 class OgGrid(C.Structure):
-    _fields_ = [("dim", C.c_int), ("n", C.c_int64 * 3), ("spacing", C.c_double * 3), ("origin", C.c_double * 3)]
+    _fields_ = [("dim", C.c_int), ("n", C.c_int64 * 3), ("spacing", C.c_double * 3), ("origin", C.c_double * 3),
+                ("direction", C.c_double * 9)]
 
 
 class OgParams(C.Structure):
@@ -100,10 +101,12 @@ def lib():
 
 @dataclass
 class Grid:
-    """Image grid: size (x fastest), spacing, origin; identity direction."""
+    """Image grid: size (x fastest), spacing, origin, direction (row-major
+    dim x dim orthonormal; identity by default)."""
     size: Sequence[int]
     spacing: Sequence[float] = None
     origin: Sequence[float] = None
+    direction: Optional[np.ndarray] = None
 
     def __post_init__(self):
         D = len(self.size)
@@ -111,6 +114,8 @@ class Grid:
             self.spacing = [1.0] * D
         if self.origin is None:
             self.origin = [0.0] * D
+        if self.direction is None:
+            self.direction = np.eye(D)
 
     @property
     def dim(self):
@@ -132,6 +137,8 @@ class Grid:
             g.n[d] = int(self.size[d]) if d < self.dim else 1
             g.spacing[d] = float(self.spacing[d]) if d < self.dim else 1.0
             g.origin[d] = float(self.origin[d]) if d < self.dim else 0.0
+        for k, v in enumerate(np.asarray(self.direction, dtype=np.float64).reshape(self.dim, self.dim).ravel()):
+            g.direction[k] = float(v)
         return g
 
 
@@ -157,7 +164,8 @@ def shrink_grid(g: Grid, factor: int) -> Grid:
     o = OgGrid()
     lib().og_shrink_grid(C.byref(g.c()), factor, C.byref(o))
     D = g.dim
-    return Grid([int(o.n[d]) for d in range(D)], [o.spacing[d] for d in range(D)], [o.origin[d] for d in range(D)])
+    return Grid([int(o.n[d]) for d in range(D)], [o.spacing[d] for d in range(D)], [o.origin[d] for d in range(D)],
+                np.array([o.direction[k] for k in range(D * D)]).reshape(D, D))
 
 
 def smooth_image(g: Grid, img, sigma: float, in_mm: bool = False):

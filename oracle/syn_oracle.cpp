@@ -36,6 +36,7 @@ typedef struct {
   int64_t n[3];        // size, x fastest; n[2] == 1 when dim == 2
   double spacing[3];
   double origin[3];
+  double direction[9]; // row-major dim x dim, orthonormal; all zero = identity
 } og_grid;
 
 typedef struct {
@@ -68,10 +69,15 @@ typedef struct {
 
 namespace {
 
+// Image geometry (itk::ImageBase): point = org + dir * (index .* sp),
+// index = (dir^T (point - org)) ./ sp.  `oblique` marks a non-identity
+// direction; identity grids keep the plain per-axis arithmetic.
 struct Grid {
   int D;
   int64_t n[3];
   double sp[3], org[3];
+  double dir[9];  // 3x3 row-major, identity-padded
+  bool oblique;
   int64_t count() const { return n[0] * n[1] * n[2]; }
 };
 
@@ -83,6 +89,17 @@ Grid to_grid(const og_grid* g) {
     r.sp[d] = d < g->dim ? g->spacing[d] : 1.0;
     r.org[d] = d < g->dim ? g->origin[d] : 0.0;
   }
+  bool all_zero = true;
+  for (int k = 0; k < g->dim * g->dim; ++k)
+    if (g->direction[k] != 0.0) all_zero = false;
+  r.oblique = false;
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) {
+      const double id = a == b ? 1.0 : 0.0;
+      const double v = (a < g->dim && b < g->dim && !all_zero) ? g->direction[a * g->dim + b] : id;
+      r.dir[a * 3 + b] = v;
+      if (v != id) r.oblique = true;
+    }
   return r;
 }
 
@@ -271,6 +288,16 @@ inline bool inside(const Grid& g, const double c[3]) {
 }
 
 inline void to_cidx(const Grid& g, const double p[3], double c[3]) {
+  if (g.oblique) {
+    double q[3] = {0.0, 0.0, 0.0};
+    for (int k = 0; k < g.D; ++k) q[k] = p[k] - g.org[k];
+    for (int d = 0; d < 3; ++d) {
+      double s = 0.0;
+      for (int k = 0; k < g.D; ++k) s += g.dir[k * 3 + d] * q[k];
+      c[d] = d < g.D ? s / g.sp[d] : 0.0;
+    }
+    return;
+  }
   for (int d = 0; d < 3; ++d) c[d] = d < g.D ? (p[d] - g.org[d]) / g.sp[d] : 0.0;
 }
 
@@ -315,6 +342,15 @@ void interp(const Grid& g, int ncomp, const R* img, const double c[3], double* o
 }
 
 inline void grid_point(const Grid& g, int64_t x, int64_t y, int64_t z, double p[3]) {
+  if (g.oblique) {
+    const double v[3] = {x * g.sp[0], y * g.sp[1], g.D == 3 ? z * g.sp[2] : 0.0};
+    for (int r = 0; r < 3; ++r) {
+      double s = g.org[r];
+      for (int k = 0; k < g.D; ++k) s += g.dir[r * 3 + k] * v[k];
+      p[r] = r < g.D ? s : 0.0;
+    }
+    return;
+  }
   p[0] = g.org[0] + x * g.sp[0];
   p[1] = g.org[1] + y * g.sp[1];
   p[2] = g.D == 3 ? g.org[2] + z * g.sp[2] : 0.0;
@@ -476,6 +512,20 @@ double cc_metric(const Grid& g, const R* F, const R* M, int radius, double eps, 
           for (int d = 0; d < D; ++d) {
             gm[d] *= c1;
             gf[d] *= c2;
+          }
+          // CentralDifferenceImageFunction with UseImageDirection: the
+          // index-axis derivative is turned into a physical vector
+          if (g.oblique) {
+            R rm[3] = {0, 0, 0}, rf[3] = {0, 0, 0};
+            for (int r = 0; r < D; ++r)
+              for (int k = 0; k < D; ++k) {
+                rm[r] += static_cast<R>(g.dir[r * 3 + k]) * gm[k];
+                rf[r] += static_cast<R>(g.dir[r * 3 + k]) * gf[k];
+              }
+            for (int d = 0; d < D; ++d) {
+              gm[d] = rm[d];
+              gf[d] = rf[d];
+            }
           }
         }
         for (int d = 0; d < D; ++d) {
@@ -659,12 +709,21 @@ Grid shrink_grid(const Grid& g, int factor) {
     f[d] = bf;
   }
   Grid o = g;
+  double delta[3] = {0.0, 0.0, 0.0};  // index-aligned offset of the output origin
   for (int d = 0; d < g.D; ++d) {
     o.sp[d] = g.sp[d] * f[d];
     o.n[d] = std::max<int64_t>(1, static_cast<int64_t>(std::floor(static_cast<double>(g.n[d]) / f[d])));
     const double in_c = (g.n[d] - 1) / 2.0, out_c = (o.n[d] - 1) / 2.0;
     o.org[d] = g.org[d] + (g.org[d] + in_c * g.sp[d]) - (g.org[d] + out_c * o.sp[d]);
+    delta[d] = in_c * g.sp[d] - out_c * o.sp[d];
   }
+  // with a direction the centres coincide in physical space: origin + dir * delta
+  if (g.oblique)
+    for (int r = 0; r < g.D; ++r) {
+      double s = g.org[r];
+      for (int k = 0; k < g.D; ++k) s += g.dir[r * 3 + k] * delta[k];
+      o.org[r] = s;
+    }
   return o;
 }
 
@@ -1016,6 +1075,9 @@ void og_shrink_grid(const og_grid* g, int factor, og_grid* out) {
     out->spacing[d] = r.sp[d];
     out->origin[d] = r.org[d];
   }
+  for (int k = 0; k < 9; ++k) out->direction[k] = 0.0;
+  for (int a = 0; a < r.D; ++a)
+    for (int b = 0; b < r.D; ++b) out->direction[a * r.D + b] = r.dir[a * 3 + b];
 }
 
 void og_smooth_image_f64(const og_grid* g, const double* in, double* out, double sigma, int in_mm) {

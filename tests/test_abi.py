@@ -125,3 +125,19 @@ def test_batch_rejects_bad_slot_counts():
         cfg.slots = slots
         assert L.lib().ants_batch_create(C.byref(cfg), C.byref(b)) == 1
         assert b"slots" in L.lib().ants_last_error()
+
+
+def test_level_geometry_with_direction_matches_oracle():
+    p = L.SynParamsC()
+    p.dim, p.n_levels = 3, 2
+    p.shrink[0], p.shrink[1], p.iterations[0], p.iterations[1] = 4, 2, 1, 1
+    p.learning_rate, p.update_variance, p.cc_radius, p.convergence_window = 0.1, 3.0, 4, 10
+    q, _ = np.linalg.qr(np.random.default_rng(1).standard_normal((3, 3)))
+    size, sp, org = (97, 64, 33), (1.0, 0.8, 2.5), (3.0, -7.0, 11.0)
+    for lvl in range(2):
+        out = L.ImageDesc()
+        L.check(L.lib().ants_syn_level_geometry(C.byref(p), C.byref(L.desc(size, sp, org, q)), lvl, C.byref(out)))
+        ref = o.shrink_grid(o.Grid(list(size), list(sp), list(org), q), p.shrink[lvl])
+        assert [out.size[k] for k in range(3)] == ref.size
+        np.testing.assert_allclose([out.origin[k] for k in range(3)], ref.origin, atol=1e-12)
+        np.testing.assert_allclose(np.array([out.direction[k] for k in range(9)]).reshape(3, 3), q)

@@ -171,3 +171,52 @@ def test_inversion_launch_split_is_exact(monkeypatch):
         np.testing.assert_array_equal(res.fwd, ref.fwd)
         np.testing.assert_array_equal(res.inv, ref.inv)
         assert st["inv_steps"] == st_ref["inv_steps"]
+
+
+def _rot3(a, b, c):
+    rx = np.array([[1, 0, 0], [0, np.cos(a), -np.sin(a)], [0, np.sin(a), np.cos(a)]])
+    ry = np.array([[np.cos(b), 0, np.sin(b)], [0, 1, 0], [-np.sin(b), 0, np.cos(b)]])
+    rz = np.array([[np.cos(c), -np.sin(c), 0], [np.sin(c), np.cos(c), 0], [0, 0, 1]])
+    return rz @ ry @ rx
+
+
+def test_syn_3d_oblique_directions_vs_oracle():
+    """Fixed and moving with different orthonormal directions (both improper,
+    as an LPS flip makes them) and anisotropic spacing."""
+    pair = ph.phantom_pair_3d(6, size=(40, 44, 36), spacing=(1.2, 1.0, 1.1), max_disp_mm=4.0, svf_sigma_mm=6.0)
+    # an LPS-style flip and an oblique acquisition; the moving image is turned
+    # a few degrees further; both centred at the same physical point
+    Rf = _rot3(0.3, -0.2, 0.4) @ np.diag([-1.0, -1.0, 1.0])
+    Rm = Rf @ _rot3(0.03, -0.04, 0.05)
+    half = (np.asarray(pair.size) - 1) / 2.0 * np.asarray(pair.spacing)
+    centre = np.array([5.0, -3.0, 2.0])
+    of, om = centre - Rf @ half, centre - Rm @ half
+    stage = SyNStage.from_args("SyN[0.2,3,0]", "CC[f,m,1,2]", "[8x5x3,1e-6,10]", "4x2x1", "2x1x0vox")
+    reg = SyNRegistration(stage, 3, force_iterations=True)
+    res = reg.register(pair.fixed, pair.moving, pair.spacing, of, pair.spacing, om, Rf, Rm)
+    reg.close()
+    P = o.SynParams(shrink=[4, 2, 1], sigma=[2, 1, 0], iterations=[8, 5, 3], learning_rate=0.2, update_variance=3.0,
+                    total_variance=0.0, radius=2, force_iterations=True)
+    gf = o.Grid(list(pair.size), list(pair.spacing), list(of), Rf)
+    gm = o.Grid(list(pair.size), list(pair.spacing), list(om), Rm)
+    ref = o.syn_register(P, gf, pair.fixed, gm, pair.moving)
+    assert ref.energy.max() < -0.3 and np.abs(ref.fwd).max() > 0.5, "degenerate case: the images must overlap"
+    print("3d oblique parity:", compare(res, ref, pair.spacing))
+
+
+def test_direction_equivariance_on_gpu():
+    """Shared direction R, isotropic spacing: the GPU stage gives the
+    identity-direction energies and R times its fields."""
+    pair = ph.phantom_pair_3d(8, size=(40, 36, 44), max_disp_mm=4.0, svf_sigma_mm=6.0)
+    R = _rot3(-0.35, 0.2, 0.5) @ np.diag([1.0, -1.0, 1.0])
+    stage = SyNStage.from_args("SyN[0.2,3,0]", "CC[f,m,1,4]", "[8x5x3,1e-6,10]", "4x2x1", "2x1x0vox")
+    out = []
+    for d in (None, R):
+        reg = SyNRegistration(stage, 3, force_iterations=True)
+        out.append(reg.register(pair.fixed, pair.moving, pair.spacing, pair.origin, pair.spacing, pair.origin, d, d))
+        reg.close()
+    rI, rR = out
+    assert rI.energy.max() < -0.3 and np.abs(rI.fwd).max() > 0.5
+    np.testing.assert_allclose(rR.energy, rI.energy, rtol=1e-5)
+    np.testing.assert_allclose(rR.fwd, rI.fwd @ R.T, atol=2e-3)
+    np.testing.assert_allclose(rR.inv, rI.inv @ R.T, atol=2e-3)

@@ -370,3 +370,48 @@ def test_syn_recovers_shift_2d():
     r2 = o.syn_register(P, g, M, g, F)
     np.testing.assert_allclose(r2.fwd, r.inv, atol=1e-6)
     np.testing.assert_allclose(r2.energy, r.energy, rtol=1e-9)
+
+
+def _rot3(a, b, c):
+    rx = np.array([[1, 0, 0], [0, np.cos(a), -np.sin(a)], [0, np.sin(a), np.cos(a)]])
+    ry = np.array([[np.cos(b), 0, np.sin(b)], [0, 1, 0], [-np.sin(b), 0, np.cos(b)]])
+    rz = np.array([[np.cos(c), -np.sin(c), 0], [np.sin(c), np.cos(c), 0], [0, 0, 1]])
+    return rz @ ry @ rx
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_direction_equivariance_of_the_stage(D):
+    """Both images sharing a direction R with isotropic spacing: every step
+    (warps, CC derivatives turned into physical vectors, compositions,
+    inversions, shrink origins) is equivariant and every norm invariant, so
+    the stage gives the identity-direction energies and R times its fields.
+    A dropped gradient rotation or a wrong transpose in the index/point
+    maps breaks this."""
+    from scipy import ndimage
+    rng = np.random.default_rng(5)
+    size = (26, 22, 20)[:D]
+    shape = tuple(reversed(size))
+    F = ndimage.gaussian_filter(rng.standard_normal(shape), 2.0)
+    M = ndimage.shift(F, [0.8, -1.2, 0.6][:D], order=1, mode="nearest")
+    F, M = (F - F.min()) / np.ptp(F), (M - M.min()) / np.ptp(M)
+    R = _rot3(0.4, -0.3, 0.7) if D == 3 else _rot3(0, 0, 0.6)[:2, :2]
+    R = R @ np.diag([-1.0] + [1.0] * (D - 1))            # improper, like an LPS flip
+    P = o.SynParams(shrink=[2, 1], sigma=[1, 0], iterations=[5, 3], learning_rate=0.2, update_variance=3.0,
+                    total_variance=0.0, radius=2, force_iterations=True)
+    org = [3.0, -5.0, 2.0][:D]
+    gI = o.Grid(list(size), [1.5] * D, org)
+    gR = o.Grid(list(size), [1.5] * D, org, R)
+    rI = o.syn_register(P, gI, F, gI, M)
+    rR = o.syn_register(P, gR, F, gR, M)
+    np.testing.assert_allclose(rR.energy, rI.energy, rtol=1e-9)
+    scale = np.abs(rI.fwd).max()
+    assert scale > 0.1
+    np.testing.assert_allclose(rR.fwd, rI.fwd @ R.T, atol=1e-8 * scale)
+    np.testing.assert_allclose(rR.inv, rI.inv @ R.T, atol=1e-8 * scale)
+    # shrink: the level grid keeps the direction and the physical centre
+    lr, li = o.shrink_grid(gR, 2), o.shrink_grid(gI, 2)
+    np.testing.assert_allclose(lr.direction, R)
+    cR = np.asarray(lr.origin) + R @ ((np.asarray(lr.size) - 1) / 2.0 * np.asarray(lr.spacing))
+    cR0 = np.asarray(org) + R @ ((np.asarray(size) - 1) / 2.0 * 1.5)
+    np.testing.assert_allclose(cR, cR0, atol=1e-12)
+    np.testing.assert_allclose(li.size, lr.size)
