@@ -1067,6 +1067,9 @@ constexpr int kInvUnroll = ANTS_INV_UNROLL;
 #define ANTS_INV_STORE_RES 1
 #endif
 constexpr bool kInvStoreResidual = ANTS_INV_STORE_RES != 0;
+#ifndef ANTS_INV_XCD_MAP
+#define ANTS_INV_XCD_MAP 1
+#endif
 
 template <int D, class IT>
 __global__ void __launch_bounds__(kInvThreads) k_inv_step(GridDev g, PtrPack4 field, MutPack4 inv, MutPack4 res,
@@ -1104,19 +1107,35 @@ __global__ void __launch_bounds__(kInvThreads) k_inv_step(GridDev g, PtrPack4 fi
   float* __restrict__ iv = inv.p[k];
   float* __restrict__ rs = res.p[k];
   const int64_t N = g.count();
-  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
   const float lim = eps * maxprev;
   double nsum = 0.0;
   float nmax = 0.f;
-  for (int64_t i0 = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i0 < N; i0 += kInvUnroll * stride) {
+#if ANTS_INV_XCD_MAP
+  // Workgroups are dealt to the 8 XCDs round-robin by id: give XCD k the k-th
+  // contiguous slab of blocks, and each block a contiguous run of voxels, so
+  // the neighbouring rows a trilinear gather touches sit in that XCD's L2.
+  // virtual block: a bijection of [0, nb) onto itself; XCD k gets
+  // [k * full, (k + 1) * full), the nb % 8 leftover blocks keep their place
+  const int nb = gridDim.x, full = nb / 8, b = blockIdx.x;
+  const int vb = b < 8 * full ? (b % 8) * full + b / 8 : b;
+  const int64_t chunk = (N + nb - 1) / nb;
+  const int64_t vbeg = vb * chunk, vend = min(N, vbeg + chunk);
+  const int64_t stride = blockDim.x;
+  const int64_t N_loop = vend;
+  for (int64_t i0 = vbeg + threadIdx.x; i0 < N_loop; i0 += kInvUnroll * stride) {
+#else
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  const int64_t N_loop = N;
+  for (int64_t i0 = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i0 < N_loop; i0 += kInvUnroll * stride) {
+#endif
     float cur[kInvUnroll][3], e[kInvUnroll][3];
     int xs[kInvUnroll], ys[kInvUnroll], zs[kInvUnroll];
     bool ok[kInvUnroll];
 #pragma unroll
     for (int u = 0; u < kInvUnroll; ++u) {
       const int64_t i = i0 + u * stride;
-      ok[u] = i < N;
-      const IT ic = static_cast<IT>(ok[u] ? i : N - 1);  // loads stay in bounds; stores are skipped
+      ok[u] = i < N_loop;
+      const IT ic = static_cast<IT>(ok[u] ? i : N_loop - 1);  // loads stay in bounds; stores are skipped
       unlin(g, ic, xs[u], ys[u], zs[u]);
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
