@@ -8,6 +8,7 @@
 
 #include <hip/hip_runtime.h>
 #include <cfloat>
+#include <cstdint>
 #include <cmath>
 #include <cstdio>
 
@@ -267,6 +268,29 @@ __device__ __forceinline__ void block_partial(double s, float m, Partial* out) {
 // One block sums n partials: thread t takes t, t+NT, ... in order, then the
 // block tree of block_sum_max; every thread receives the result.  The order
 // is fixed, so every block that reduces the same partials gets the same bits.
+// Workgroups are dealt to the 8 XCDs round-robin by id.  slab_block renumbers
+// them so that XCD k takes the k-th contiguous slab of block indices: the
+// neighbouring rows and planes a trilinear gather touches are then fetched
+// into that XCD's L2 by its own blocks.  A bijection of [0, nb) onto itself;
+// the nb % 8 leftover blocks keep their place.
+__device__ __forceinline__ int64_t slab_block(int64_t b, int64_t nb) {
+  const int64_t full = nb / 8;
+  return b < 8 * full ? (b % 8) * full + b / 8 : b;
+}
+
+// block index of k_compose (measured: 4.74 -> 4.64 ms per headline pair;
+// the image warp measured slower with it and keeps the plain order)
+#ifndef ANTS_GATHER_XCD_MAP
+#define ANTS_GATHER_XCD_MAP 1
+#endif
+__device__ __forceinline__ int64_t gather_block() {
+#if ANTS_GATHER_XCD_MAP
+  return slab_block(blockIdx.x, gridDim.x);
+#else
+  return blockIdx.x;
+#endif
+}
+
 template <int NT>
 __device__ __forceinline__ void reduce_partials(const Partial* p, int n, double& s, float& m) {
   __shared__ double r_s;
@@ -982,7 +1006,7 @@ __global__ void __launch_bounds__(kBlock) k_compose(GridDev g, PtrPack4 disp, Pt
   const float* __restrict__ w = warp.p[k];
   float* __restrict__ o = out.p[k];
   const int64_t N = g.count();
-  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  const int64_t i = gather_block() * blockDim.x + threadIdx.x;
   if (i >= N) return;
   float scale = 1.f;
   if (maxbits) {
@@ -1111,13 +1135,9 @@ __global__ void __launch_bounds__(kInvThreads) k_inv_step(GridDev g, PtrPack4 fi
   double nsum = 0.0;
   float nmax = 0.f;
 #if ANTS_INV_XCD_MAP
-  // Workgroups are dealt to the 8 XCDs round-robin by id: give XCD k the k-th
-  // contiguous slab of blocks, and each block a contiguous run of voxels, so
-  // the neighbouring rows a trilinear gather touches sit in that XCD's L2.
-  // virtual block: a bijection of [0, nb) onto itself; XCD k gets
-  // [k * full, (k + 1) * full), the nb % 8 leftover blocks keep their place
-  const int nb = gridDim.x, full = nb / 8, b = blockIdx.x;
-  const int vb = b < 8 * full ? (b % 8) * full + b / 8 : b;
+  // each block walks one contiguous run of voxels, the runs of one XCD adjoin
+  const int64_t nb = gridDim.x;
+  const int64_t vb = slab_block(blockIdx.x, nb);
   const int64_t chunk = (N + nb - 1) / nb;
   const int64_t vbeg = vb * chunk, vend = min(N, vbeg + chunk);
   const int64_t stride = blockDim.x;
@@ -1439,7 +1459,7 @@ MutPack4 mpack(float* const* p, int np) {
 }
 
 // CC tile shapes (3-D): TX x TY columns; the z-chunk depth is chosen per grid
-// so that a launch has about 1024 blocks (between 8 and 32 planes).
+// from the number of blocks the device holds at once (zchunk_for).
 // Tile of the 3-D CC kernel: 16 x 16 columns per block measured fastest over
 // the pyramid levels of a 192x224x192 registration (the x box sums are
 // computed for TY + 2R rows per TY output rows, so square tiles waste less).
@@ -1450,10 +1470,12 @@ MutPack4 mpack(float* const* p, int np) {
 #define ANTS_CC_TY 16
 #endif
 constexpr int kCCTX = ANTS_CC_TX, kCCTY = ANTS_CC_TY;
-// z-chunk of the tiled CC and smoothing kernels: aim for about
-// k*_TARGET_BLOCKS blocks, within [ZMIN, ZMAX] planes per block (each block
-// also reads 2R halo planes, so short chunks trade redundant reads for
-// parallelism on the small pyramid levels).
+// z-chunk of the tiled CC and smoothing kernels.  ANTS_ZCHUNK_MODEL=0 selects
+// the former rule (about k*_TARGET_BLOCKS blocks within [ZMIN, ZMAX] planes);
+// the default picks the depth from the resident-block count (zchunk_for),
+// measured 52.0 -> 49.9 ms per 192x224x192 pair (smoothing 9.45 -> 7.7 ms,
+// CC 7.72 -> 7.25 ms): at full resolution 3 chunks of 64 planes fill the
+// 512 resident blocks in one round, where 7 chunks of 31 took three.
 #ifndef ANTS_CC_TARGET_BLOCKS
 #define ANTS_CC_TARGET_BLOCKS 1024
 #endif
@@ -1467,11 +1489,59 @@ constexpr int kCCTX = ANTS_CC_TX, kCCTY = ANTS_CC_TY;
 #define ANTS_SMOOTH_ZMIN 16
 #endif
 
-int cc_zchunk(const GridDev& g) {
+#ifndef ANTS_ZCHUNK_MODEL
+#define ANTS_ZCHUNK_MODEL 1
+#endif
+#ifndef ANTS_ZCHUNK_MIN
+#define ANTS_ZCHUNK_MIN 4
+#endif
+
+// Blocks of a kernel resident on the whole device at once: blocks per CU
+// (from its register and LDS use) times the CU count.
+int64_t device_slots(const void* kernel, int threads) {
+  int dev = 0, cus = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+    cus = 256;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0) != hipSuccess || per < 1) per = 1;
+  return static_cast<int64_t>(cus) * per;
+}
+
+// z-chunk depth of a kernel that streams `columns` block columns along z,
+// reading `halo` warm-up planes per chunk: the depth that minimises
+// rounds x planes per block, where rounds = ceil(blocks / slots) waves of
+// resident blocks.  Ties go to the deeper chunk (fewer halo reads).
+int zchunk_for(int nz, int64_t columns, int halo, int64_t slots) {
+  int best_zc = nz;
+  int64_t best = INT64_MAX;
+  for (int zc = nz; zc >= 1; --zc) {
+    if (zc < ANTS_ZCHUNK_MIN && zc < nz) break;
+    const int64_t nzc = (nz + zc - 1) / zc;
+    if (zc > 1 && (nz + zc - 2) / (zc - 1) == nzc) continue;  // same chunk count as a shallower depth
+    const int64_t rounds = (columns * nzc + slots - 1) / slots;
+    const int64_t cost = rounds * (zc + halo);
+    if (cost < best) {
+      best = cost;
+      best_zc = zc;
+    }
+  }
+  return best_zc;
+}
+
+int cc_zchunk(const GridDev& g, int radius) {
   const int64_t tiles = static_cast<int64_t>((g.n[0] + kCCTX - 1) / kCCTX) * ((g.n[1] + kCCTY - 1) / kCCTY);
+#if ANTS_ZCHUNK_MODEL
+  static const int64_t slots4 =
+      device_slots(reinterpret_cast<const void*>(&k_cc3d<4, kCCTX, kCCTY>), kCCTX * kCCTY);
+  static const int64_t slots2 =
+      device_slots(reinterpret_cast<const void*>(&k_cc3d<2, kCCTX, kCCTY>), kCCTX * kCCTY);
+  return zchunk_for(g.n[2], tiles, 2 * radius, radius == 4 ? slots4 : slots2);
+#else
+  (void)radius;
   int64_t zc = (static_cast<int64_t>(g.n[2]) * tiles) / ANTS_CC_TARGET_BLOCKS;
   zc = zc < ANTS_CC_ZMIN ? ANTS_CC_ZMIN : (zc > 32 ? 32 : zc);
   return static_cast<int>(zc);
+#endif
 }
 
 }  // namespace
@@ -1519,8 +1589,13 @@ bool launch_smooth3d(const GridDev& g, int np, const float* const* in, const flo
   if (g.D != 3 || t.r != R || np > 4) return false;
   if (static_cast<int64_t>(g.n[0]) * g.n[1] * 3 >= (int64_t{1} << 31)) return false;  // 32-bit in-plane offsets
   const int64_t tiles = static_cast<int64_t>((g.n[0] + TX - 1) / TX) * ((g.n[1] + TY - 1) / TY) * np;
+#if ANTS_ZCHUNK_MODEL
+  static const int64_t slots = device_slots(reinterpret_cast<const void*>(&k_smooth3d<R, TX, TY>), TX * TY);
+  const int64_t zc = zchunk_for(g.n[2], tiles, 2 * R, slots);
+#else
   int64_t zc = (static_cast<int64_t>(g.n[2]) * tiles) / ANTS_SMOOTH_TARGET_BLOCKS;
   zc = zc < ANTS_SMOOTH_ZMIN ? ANTS_SMOOTH_ZMIN : (zc > 64 ? 64 : zc);
+#endif
   const int nzc = static_cast<int>((g.n[2] + zc - 1) / zc);
   const dim3 grid((g.n[0] + TX - 1) / TX, (g.n[1] + TY - 1) / TY, nzc * np);
   hipLaunchKernelGGL((k_smooth3d<R, TX, TY>), grid, dim3(TX * TY), 0, s, g, pack(in, np), pack(orig, np),
@@ -1571,7 +1646,7 @@ static bool cc_tiled(const GridDev& g, int radius) {
 
 int64_t cc_partials_needed(const GridDev& g, int radius) {
   if (cc_tiled(g, radius)) {
-    const int zc = cc_zchunk(g);
+    const int zc = cc_zchunk(g, radius);
     return static_cast<int64_t>((g.n[0] + kCCTX - 1) / kCCTX) * ((g.n[1] + kCCTY - 1) / kCCTY) *
            ((g.n[2] + zc - 1) / zc);
   }
@@ -1584,7 +1659,7 @@ void launch_cc(const GridDev& g, const float* F, const float* M, int radius, flo
     launch_cc_direct(g, F, M, radius, eps, gF2M, gM2M, partials, slot, s);
     return;
   }
-  const int zc = cc_zchunk(g);
+  const int zc = cc_zchunk(g, radius);
   const dim3 grid((g.n[0] + kCCTX - 1) / kCCTX, (g.n[1] + kCCTY - 1) / kCCTY, (g.n[2] + zc - 1) / zc);
   if (radius == 4)
     hipLaunchKernelGGL((k_cc3d<4, kCCTX, kCCTY>), grid, dim3(kCCTX * kCCTY), 0, s, g, F, M, eps, zc, gF2M, gM2M,
