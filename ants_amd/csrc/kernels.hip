@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 #include <cfloat>
 #include <cstdint>
+#include <type_traits>
+#include <utility>
 #include <cmath>
 #include <cstdio>
 
@@ -702,11 +704,22 @@ __device__ __forceinline__ void rotate_to_physical(const GridDev& g, float v[3])
   for (int a = 0; a < D; ++a) v[a] = r[a];
 }
 
-__device__ __forceinline__ float cdiff(float vm, float v0, float vp, int pos, int len, float h) {
+// ih = 1 / spacing (GridDev::isp): a product instead of an IEEE division
+// (ten instructions), at most an ulp from (up - dn) / h.
+#ifndef ANTS_CDIFF_MUL
+#define ANTS_CDIFF_MUL 1
+#endif
+__device__ __forceinline__ float cdiff(float vm, float v0, float vp, int pos, int len, float h, float ih) {
   if (pos >= len - 1) return 0.f;
   const float up = v0 + (vp - v0) * 0.5f;
   const float dn = pos == 0 ? v0 : vm + (v0 - vm) * 0.5f;
+#if ANTS_CDIFF_MUL
+  (void)h;
+  return (up - dn) * ih;
+#else
+  (void)ih;
   return (up - dn) / h;
+#endif
 }
 
 // Per-voxel CC value and both SyN derivatives from window sums.  The sums
@@ -785,8 +798,8 @@ __global__ void __launch_bounds__(kBlock) k_cc_direct(GridDev g, const float* __
         const bool lo = pos[d] > 0, hi = pos[d] < g.n[d] - 1;
         const float mm_ = lo ? M[i - st[d]] : m, mp = hi ? M[i + st[d]] : m;
         const float fm_ = lo ? F[i - st[d]] : f, fp = hi ? F[i + st[d]] : f;
-        gm = cdiff(mm_, m, mp, pos[d], g.n[d], g.sp[d]);
-        gf = cdiff(fm_, f, fp, pos[d], g.n[d], g.sp[d]);
+        gm = cdiff(mm_, m, mp, pos[d], g.n[d], g.sp[d], g.isp[d]);
+        gf = cdiff(fm_, f, fp, pos[d], g.n[d], g.sp[d], g.isp[d]);
       }
       vm[d] = o.a1 * gm;
       vf[d] = o.a2 * gf;
@@ -811,6 +824,20 @@ __global__ void __launch_bounds__(kBlock) k_cc_direct(GridDev g, const float* __
 // double, and pushes them into a (2R+1)-deep register ring; the ring total
 // is the window sum for the voxel R planes behind.  Outside-volume samples
 // are 0 and the count is analytic, which equals ITK's in-bounds-only sums.
+#ifndef ANTS_CC_ROTATE
+#define ANTS_CC_ROTATE 1
+#endif
+constexpr bool kCCRotate = ANTS_CC_ROTATE != 0;
+// f(integral_constant<I>, zz++) for I = I0 .. W-1 while zz < zend
+template <int I, int W, class F>
+__device__ __forceinline__ void unrolled_steps(F& f, int& zz, int zend) {
+  if constexpr (I < W) {
+    if (zz >= zend) return;
+    f(std::integral_constant<int, I>{}, zz);
+    ++zz;
+    unrolled_steps<I + 1, W>(f, zz, zend);
+  }
+}
 #ifndef ANTS_CC_WAVES
 #define ANTS_CC_WAVES 1
 #endif
@@ -833,18 +860,24 @@ __global__ void __launch_bounds__(TX* TY, kCCWavesPerEU) k_cc3d(GridDev g, const
   const double cxy = static_cast<double>((min(x + R, nx - 1) - max(x - R, 0) + 1) *
                                          (min(y + R, ny - 1) - max(y - R, 0) + 1));
 
+  // z rings: double plane sums of the last W planes, and the raw centre
+  // values and in-plane gradients of the planes still needed.  With
+  // kCCRotate the z loop is unrolled W times and plane s lives in slot
+  // s % W (compile-time indices, no register moves); otherwise the rings
+  // shift by one slot per plane.
+  constexpr int NRF = kCCRotate ? W : R + 2, NRG = kCCRotate ? W : R + 1;
   double ring[5][W];
   double zsum[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
   for (int a = 0; a < 5; ++a)
 #pragma unroll
     for (int b = 0; b < W; ++b) ring[a][b] = 0.0;
-  float rf[R + 2], rm[R + 2];          // raw centre ring (newest last)
-  float rgf[R + 1][2], rgm[R + 1][2];  // in-plane gradients ring (newest last)
+  float rf[NRF], rm[NRF];          // raw centre values
+  float rgf[NRG][2], rgm[NRG][2];  // in-plane gradients
 #pragma unroll
-  for (int b = 0; b < R + 2; ++b) rf[b] = rm[b] = 0.f;
+  for (int b = 0; b < NRF; ++b) rf[b] = rm[b] = 0.f;
 #pragma unroll
-  for (int b = 0; b < R + 1; ++b) rgf[b][0] = rgf[b][1] = rgm[b][0] = rgm[b][1] = 0.f;
+  for (int b = 0; b < NRG; ++b) rgf[b][0] = rgf[b][1] = rgm[b][0] = rgm[b][1] = 0.f;
   double cc_sum = 0.0;
   const int64_t plane = static_cast<int64_t>(nx) * ny;
 
@@ -878,8 +911,11 @@ __global__ void __launch_bounds__(TX* TY, kCCWavesPerEU) k_cc3d(GridDev g, const
     }
   };
   fetch(z0 - R);
+  const int zend = z1 + R;
 
-  for (int zz = z0 - R; zz < z1 + R; ++zz) {
+  // one plane step; PH = (zz - (z0 - R)) % W when rotating
+  auto step = [&](auto PH, int zz) {
+    constexpr int ph = decltype(PH)::value;
     const bool zin = zz >= 0 && zz < nz;
     __syncthreads();
 #pragma unroll
@@ -890,7 +926,7 @@ __global__ void __launch_bounds__(TX* TY, kCCWavesPerEU) k_cc3d(GridDev g, const
       }
     }
     __syncthreads();
-    if (zz + 1 < z1 + R) fetch(zz + 1);
+    if (zz + 1 < zend) fetch(zz + 1);
     // x box sums, two adjacent outputs per task: the window of output
     // x2 + 1 is that of x2 with the first sample dropped and one added
     static_assert(TX % 2 == 0, "x sums run in pairs");
@@ -923,10 +959,10 @@ __global__ void __launch_bounds__(TX* TY, kCCWavesPerEU) k_cc3d(GridDev g, const
     const float fc = q[0][ty + R][tx + R], mc = q[1][ty + R][tx + R];
     float gfx = 0.f, gfy = 0.f, gmx = 0.f, gmy = 0.f;
     if (active && zin) {
-      gfx = cdiff(q[0][ty + R][tx + R - 1], fc, q[0][ty + R][tx + R + 1], x, nx, g.sp[0]);
-      gmx = cdiff(q[1][ty + R][tx + R - 1], mc, q[1][ty + R][tx + R + 1], x, nx, g.sp[0]);
-      gfy = cdiff(q[0][ty + R - 1][tx + R], fc, q[0][ty + R + 1][tx + R], y, ny, g.sp[1]);
-      gmy = cdiff(q[1][ty + R - 1][tx + R], mc, q[1][ty + R + 1][tx + R], y, ny, g.sp[1]);
+      gfx = cdiff(q[0][ty + R][tx + R - 1], fc, q[0][ty + R][tx + R + 1], x, nx, g.sp[0], g.isp[0]);
+      gmx = cdiff(q[1][ty + R][tx + R - 1], mc, q[1][ty + R][tx + R + 1], x, nx, g.sp[0], g.isp[0]);
+      gfy = cdiff(q[0][ty + R - 1][tx + R], fc, q[0][ty + R + 1][tx + R], y, ny, g.sp[1], g.isp[1]);
+      gmy = cdiff(q[1][ty + R - 1][tx + R], mc, q[1][ty + R + 1][tx + R], y, ny, g.sp[1], g.isp[1]);
     }
     __syncthreads();
     double p[5];
@@ -937,49 +973,72 @@ __global__ void __launch_bounds__(TX* TY, kCCWavesPerEU) k_cc3d(GridDev g, const
       for (int b = 0; b < W; ++b) t += sx[a][ty + b][tx];
       p[a] = t;
     }
-    // running z-window sums: add the new plane, drop the one leaving the ring
+    // running z-window sums: add the new plane, drop the one W planes back;
+    // slots of plane zc = zz - R and of its z neighbours
+    int c0, cm, cp, cg;
+    if constexpr (kCCRotate) {
 #pragma unroll
-    for (int a = 0; a < 5; ++a) {
-      zsum[a] += p[a] - ring[a][0];
+      for (int a = 0; a < 5; ++a) {
+        zsum[a] += p[a] - ring[a][ph];
+        ring[a][ph] = p[a];
+      }
+      rf[ph] = fc;
+      rm[ph] = mc;
+      rgf[ph][0] = gfx;
+      rgf[ph][1] = gfy;
+      rgm[ph][0] = gmx;
+      rgm[ph][1] = gmy;
+      c0 = (ph + W - R) % W;
+      cm = (ph + W - R - 1) % W;
+      cp = (ph + W - R + 1) % W;
+      cg = c0;
+    } else {
 #pragma unroll
-      for (int b = 0; b < W - 1; ++b) ring[a][b] = ring[a][b + 1];
-      ring[a][W - 1] = p[a];
+      for (int a = 0; a < 5; ++a) {
+        zsum[a] += p[a] - ring[a][0];
+#pragma unroll
+        for (int b = 0; b < W - 1; ++b) ring[a][b] = ring[a][b + 1];
+        ring[a][W - 1] = p[a];
+      }
+#pragma unroll
+      for (int b = 0; b < R + 1; ++b) {
+        rf[b] = rf[b + 1];
+        rm[b] = rm[b + 1];
+      }
+      rf[R + 1] = fc;
+      rm[R + 1] = mc;
+#pragma unroll
+      for (int b = 0; b < R; ++b) {
+        rgf[b][0] = rgf[b + 1][0];
+        rgf[b][1] = rgf[b + 1][1];
+        rgm[b][0] = rgm[b + 1][0];
+        rgm[b][1] = rgm[b + 1][1];
+      }
+      rgf[R][0] = gfx;
+      rgf[R][1] = gfy;
+      rgm[R][0] = gmx;
+      rgm[R][1] = gmy;
+      // rf[R + 1] is plane zz = zc + R ... rf[1] = zc, rf[0] = zc - 1
+      c0 = 1;
+      cm = 0;
+      cp = 2;
+      cg = 0;
     }
-#pragma unroll
-    for (int b = 0; b < R + 1; ++b) {
-      rf[b] = rf[b + 1];
-      rm[b] = rm[b + 1];
-    }
-    rf[R + 1] = fc;
-    rm[R + 1] = mc;
-#pragma unroll
-    for (int b = 0; b < R; ++b) {
-      rgf[b][0] = rgf[b + 1][0];
-      rgf[b][1] = rgf[b + 1][1];
-      rgm[b][0] = rgm[b + 1][0];
-      rgm[b][1] = rgm[b + 1][1];
-    }
-    rgf[R][0] = gfx;
-    rgf[R][1] = gfy;
-    rgm[R][0] = gmx;
-    rgm[R][1] = gmy;
 
     const int zc = zz - R;
     if (active && zc >= z0 && zc < z1) {
-      const double* sm5 = zsum;
       const double cz = static_cast<double>(min(zc + R, nz - 1) - max(zc - R, 0) + 1);
-      // raw centre column: rf[R+1] = zz = zc+R ... rf[1] = zc, rf[0] = zc-1, rf[2] = zc+1
-      const float f = rf[1], m = rm[1];
-      const CCOut o = cc_voxel(cxy * cz, sm5[0], sm5[1], sm5[2], sm5[3], sm5[4], f, m, eps);
+      const float f = rf[c0], m = rm[c0];
+      const CCOut o = cc_voxel(cxy * cz, zsum[0], zsum[1], zsum[2], zsum[3], zsum[4], f, m, eps);
       cc_sum += o.cc;
       float gmz = 0.f, gfz = 0.f;
       if (o.a1 != 0.f || o.a2 != 0.f) {
-        gmz = cdiff(zc > 0 ? rm[0] : m, m, rm[2], zc, nz, g.sp[2]);
-        gfz = cdiff(zc > 0 ? rf[0] : f, f, rf[2], zc, nz, g.sp[2]);
+        gmz = cdiff(zc > 0 ? rm[cm] : m, m, rm[cp], zc, nz, g.sp[2], g.isp[2]);
+        gfz = cdiff(zc > 0 ? rf[cm] : f, f, rf[cp], zc, nz, g.sp[2], g.isp[2]);
       }
       const int64_t i = static_cast<int64_t>(zc) * plane + static_cast<int64_t>(y) * nx + x;
-      float vm[3] = {o.a1 * rgm[0][0], o.a1 * rgm[0][1], o.a1 * gmz};
-      float vf[3] = {o.a2 * rgf[0][0], o.a2 * rgf[0][1], o.a2 * gfz};
+      float vm[3] = {o.a1 * rgm[cg][0], o.a1 * rgm[cg][1], o.a1 * gmz};
+      float vf[3] = {o.a2 * rgf[cg][0], o.a2 * rgf[cg][1], o.a2 * gfz};
       if (g.oblique) {
         rotate_to_physical<3>(g, vm);
         rotate_to_physical<3>(g, vf);
@@ -991,6 +1050,11 @@ __global__ void __launch_bounds__(TX* TY, kCCWavesPerEU) k_cc3d(GridDev g, const
       g2[i * 3 + 1] = vf[1];
       g2[i * 3 + 2] = vf[2];
     }
+  };
+  if constexpr (kCCRotate) {
+    for (int zz = z0 - R; zz < zend;) unrolled_steps<0, W>(step, zz, zend);
+  } else {
+    for (int zz = z0 - R; zz < zend; ++zz) step(std::integral_constant<int, 0>{}, zz);
   }
   const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
   block_partial<NT>(cc_sum, 0.f, partials + bid);
