@@ -466,9 +466,30 @@ __global__ void __launch_bounds__(kBlock) k_gauss_field_final(GridDev g, PtrPack
 // pass.  The epilogue zeroes the boundary, blends with the input when
 // w2 != 0, and takes the max norm.  The next planes are prefetched into
 // registers while the current one is used.
+// f(integral_constant<I>, zz++) for I = I0 .. W-1 while zz < zend
+template <int I, int W, class F>
+__device__ __forceinline__ void unrolled_steps(F& f, int& zz, int zend) {
+  if constexpr (I < W) {
+    if (zz >= zend) return;
+    f(std::integral_constant<int, I>{}, zz);
+    ++zz;
+    unrolled_steps<I + 1, W>(f, zz, zend);
+  }
+}
+
 #ifndef ANTS_SMOOTH_PREFETCH
 #define ANTS_SMOOTH_PREFETCH 1
 #endif
+#ifndef ANTS_SMOOTH_ROTATE
+#define ANTS_SMOOTH_ROTATE 1
+#endif
+constexpr bool kSmoothRotate = ANTS_SMOOTH_ROTATE != 0;
+// max norm of u / spacing as a product with 1 / spacing (at most an ulp
+// from the quotient) instead of an IEEE division per component
+#ifndef ANTS_SMOOTH_NORM_MUL
+#define ANTS_SMOOTH_NORM_MUL 1
+#endif
+
 constexpr int kSmoothPrefetch = ANTS_SMOOTH_PREFETCH;
 // Tile of the fused smoothing: 16 x 32 outputs per plane measured fastest
 // over the three pyramid levels of a 192x224x192 registration.
@@ -551,11 +572,12 @@ __global__ void __launch_bounds__(TX* TY, kSmoothWavesPerEU) k_smooth3d(GridDev 
   for (int qs = 0; qs < P; ++qs)
     if (z0 - R + qs < zend) fetch(pv[qs], z0 - R + qs);
   float nrm = 0.f;
-  for (int zb0 = z0 - R; zb0 < zend; zb0 += P) {
-#pragma unroll
-  for (int qs = 0; qs < P; ++qs) {
-    const int zz = zb0 + qs;
-    if (zz >= zend) break;
+  // one plane step: stage plane zz (prefetched in pv[qs]), convolve it along
+  // x and y, push it into the z ring and emit output plane zz - R.  With
+  // kSmoothRotate the z loop is unrolled W times and plane s lives in ring
+  // slot PH = s % W (compile-time indices); otherwise the ring shifts.
+  auto step = [&](auto PH, int qs, int zz) {
+    constexpr int ph = decltype(PH)::value;
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < KP; ++u)
@@ -599,11 +621,17 @@ __global__ void __launch_bounds__(TX* TY, kSmoothWavesPerEU) k_smooth3d(GridDev 
       for (int b = 1; b <= R; ++b) acc += c[b] * (xs[a][ty + R - b][tx] + xs[a][ty + R + b][tx]);
       p[a] = acc;
     }
+    // ring slot of plane zz - R + k, k = -R .. R (k = R is plane zz)
+    auto slot = [](int k) { return kSmoothRotate ? (ph + W + k - R) % W : R + k; };
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
+      if constexpr (kSmoothRotate) {
+        ring[a][ph] = p[a];
+      } else {
 #pragma unroll
-      for (int b = 0; b < W - 1; ++b) ring[a][b] = ring[a][b + 1];
-      ring[a][W - 1] = p[a];
+        for (int b = 0; b < W - 1; ++b) ring[a][b] = ring[a][b + 1];
+        ring[a][W - 1] = p[a];
+      }
     }
     const int zc = zz - R;
     if (active && zc >= z0 && zc < z1) {
@@ -612,17 +640,34 @@ __global__ void __launch_bounds__(TX* TY, kSmoothWavesPerEU) k_smooth3d(GridDev 
       float s2 = 0.f;
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
-        float acc = c[0] * ring[a][R];
+        float acc = c[0] * ring[a][slot(0)];
 #pragma unroll
-        for (int b = 1; b <= R; ++b) acc += c[b] * (ring[a][R - b] + ring[a][R + b]);
+        for (int b = 1; b <= R; ++b) acc += c[b] * (ring[a][slot(-b)] + ring[a][slot(b)]);
         const float v = bnd ? 0.f : (w2 != 0.f ? acc * w1 + org[i * 3 + a] * w2 : acc * w1);
         dst[i * 3 + a] = v;
+#if ANTS_SMOOTH_NORM_MUL
+        const float q = v * g.isp[a];
+#else
         const float q = v / g.sp[a];
+#endif
         s2 += q * q;
       }
       nrm = fmaxf(nrm, sqrtf(s2));
     }
-  }
+  };
+  if constexpr (kSmoothRotate) {
+    static_assert(!kSmoothRotate || P == 1, "the rotating ring prefetches one plane ahead");
+    auto step1 = [&](auto PH, int zz) { step(PH, 0, zz); };
+    for (int zz = z0 - R; zz < zend;) unrolled_steps<0, W>(step1, zz, zend);
+  } else {
+    for (int zb0 = z0 - R; zb0 < zend; zb0 += P) {
+#pragma unroll
+      for (int qs = 0; qs < P; ++qs) {
+        const int zz = zb0 + qs;
+        if (zz >= zend) break;
+        step(std::integral_constant<int, 0>{}, qs, zz);
+      }
+    }
   }
   double dummy = 0.0;
   block_sum_max<NT>(dummy, nrm);
@@ -828,16 +873,6 @@ __global__ void __launch_bounds__(kBlock) k_cc_direct(GridDev g, const float* __
 #define ANTS_CC_ROTATE 1
 #endif
 constexpr bool kCCRotate = ANTS_CC_ROTATE != 0;
-// f(integral_constant<I>, zz++) for I = I0 .. W-1 while zz < zend
-template <int I, int W, class F>
-__device__ __forceinline__ void unrolled_steps(F& f, int& zz, int zend) {
-  if constexpr (I < W) {
-    if (zz >= zend) return;
-    f(std::integral_constant<int, I>{}, zz);
-    ++zz;
-    unrolled_steps<I + 1, W>(f, zz, zend);
-  }
-}
 #ifndef ANTS_CC_WAVES
 #define ANTS_CC_WAVES 1
 #endif
