@@ -329,8 +329,8 @@ struct DevBuf {
 };
 
 // HIP-event timing of kernel classes on one stream.  Events are recorded
-// around each launch of an enabled class and read after the iteration's
-// synchronisation; the bytes of inversion launches are attributed afterwards
+// around each launch of an enabled class and read once the iteration has
+// published its statistics; the bytes of inversion launches are attributed afterwards
 // from the per-step `computed` flags.  Launches of steps that found their
 // inversion already converged count with their time and zero bytes, so the
 // class's mean launch time is the mean over every dispatch of the kernel.
@@ -399,6 +399,9 @@ struct Profiler {
         }
       }
       float ms = 0.f;
+      // the span ended before the statistics were published: complete on
+      // the device; the runtime may not have noted it yet
+      (void)hipEventSynchronize(r.b);
       const hipError_t e = hipEventElapsedTime(&ms, r.a, r.b);
       if (e != hipSuccess) {
         (void)hipGetLastError();  // not sticky: a later launch check must not see it
@@ -452,6 +455,29 @@ struct GraphEntry {
   float* nwk[4];
 };
 
+// Waits until the iteration enqueued on `s` has published its statistics
+// (k_publish_stats sets *flag in pinned host memory).  Spinning on the flag
+// returns a few microseconds sooner than a stream synchronisation; the
+// stream is polled now and then so that a failed launch ends the wait.
+static int wait_published(const uint32_t* flag, hipStream_t s) {
+  const volatile uint32_t* f = flag;
+  for (uint64_t spin = 1;; ++spin) {
+    if (*f) break;
+    if ((spin & 255) == 0) {
+      const hipError_t q = hipStreamQuery(s);
+      if (q == hipSuccess) {
+        if (*f) break;
+        return fail(ANTS_ERR_HIP, "iteration finished without publishing its statistics");
+      }
+      if (q != hipErrorNotReady) return fail(ANTS_ERR_HIP, "iteration failed: %s", hipGetErrorString(q));
+      if (spin > 4096) std::this_thread::yield();
+    }
+    __builtin_ia32_pause();
+  }
+  std::atomic_thread_fence(std::memory_order_acquire);
+  return ANTS_OK;
+}
+
 struct ants_syn_handle {
   ants_syn_params P{};
   hipStream_t stream = nullptr;
@@ -460,6 +486,7 @@ struct ants_syn_handle {
   DevBuf fld[10];  // 4 middle fields, 4 work fields, 2 inversion residuals
   DevBuf stats, partials_cc, partials_inv, minmax;
   IterStats* host_stats = nullptr;  // pinned
+  uint32_t* host_flag = nullptr;    // pinned, coherent: set when an iteration's statistics are published
   ants_syn_stats run_stats{};
   HostGrid last_grid{};
   bool have_state = false;
@@ -483,6 +510,7 @@ struct ants_syn_handle {
     for (DevBuf* b : all) b->release();
     for (auto& b : fld) b.release();
     if (host_stats) (void)hipHostFree(host_stats);
+    if (host_flag) (void)hipHostFree(host_flag);
     if (own_stream && stream) (void)hipStreamDestroy(stream);
   }
 
@@ -658,6 +686,7 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
   ANTS_TRY(partials_cc.ensure(ccp * sizeof(Partial)));
   ANTS_TRY(partials_inv.ensure(4 * invp * sizeof(Partial)));  // 2 step parities x 2 problems
   if (!host_stats) HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&host_stats), sizeof(IterStats), hipHostMallocDefault));
+  if (!host_flag) HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&host_flag), 256, hipHostMallocCoherent));
 
   hipStream_t s = stream;
   IterStats* dstats = static_cast<IterStats*>(stats.p);
@@ -666,7 +695,7 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
   uint64_t setup_hash = 0;
   {
     std::vector<const void*> sig = {F.p, M.p, Fs.p, Ms.p, Fw.p, Mw.p, Fb.p, Mb.p, stats.p, partials_cc.p,
-                                    partials_inv.p, host_stats, static_cast<const void*>(s)};
+                                    partials_inv.p, host_stats, host_flag, static_cast<const void*>(s)};
     for (auto& b : fld) sig.push_back(b.p);
     if (sig != graph_signature || graphs.size() > 512) {
       clear_graphs();
@@ -689,7 +718,9 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
     const double dbl[] = {P.learning_rate, P.update_variance, P.total_variance};
     setup_hash = fnv1a(dbl, sizeof(dbl), setup_hash);
     // Profiled runs launch eagerly: events recorded by the event nodes of a
-    // replayed graph do not yield elapsed times on this runtime.
+    // replayed graph do not yield elapsed times on this runtime, and graphs
+    // cut into segments at the timed spans (events between the segment
+    // launches) measured slower than eager launches.
     const char* env = std::getenv("ANTS_SYN_GRAPHS");
     use_graphs = !(env && env[0] == '0') && prof.mask == 0;
     // inversion steps launched per batch inside an iteration's sequence; a
@@ -818,20 +849,23 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
     auto enqueue = [&](float* const* cst, float* const* cwk, float** nst, float** nwk) -> int {
       float* st_[4] = {cst[0], cst[1], cst[2], cst[3]};
       float* wk_[4] = {cwk[0], cwk[1], cwk[2], cwk[3]};
-      launch_fill_u32(reinterpret_cast<uint32_t*>(dstats), 0u, sizeof(IterStats) / 4, s);
-      // 1. warp both smoothed images through the inverse middle fields
+      // 1. warp both smoothed images through the inverse middle fields (the
+      //    warp's first block also zeroes this iteration's statistics)
       const double Nd = static_cast<double>(N);
+      uint32_t* stat_words = reinterpret_cast<uint32_t*>(dstats);
+      constexpr int kStatWords = static_cast<int>(sizeof(IterStats) / 4);
       prof.begin(ANTS_KC_WARP);
       if (bf16)
         launch_warp_pair_bf16(gd, gFd, static_cast<const uint16_t*>(Fb.p), st_[1], Fw.f(), gMd,
-                              static_cast<const uint16_t*>(Mb.p), st_[3], Mw.f(), s);
+                              static_cast<const uint16_t*>(Mb.p), st_[3], Mw.f(), s, stat_words, kStatWords);
       else
-        launch_warp_pair(gd, gFd, Fs.f(), st_[1], Fw.f(), gMd, Ms.f(), st_[3], Mw.f(), s);
+        launch_warp_pair(gd, gFd, Fs.f(), st_[1], Fw.f(), gMd, Ms.f(), st_[3], Mw.f(), s, stat_words, kStatWords);
       prof.end(ANTS_KC_WARP, Nd * (2.0 * D * e_f + 2.0 * gsz * e_i + 2.0 * e_i));
       // 2. CC value and both derivatives (g_f2m -> wk[0], g_m2m -> wk[1])
       prof.begin(ANTS_KC_CC);
-      launch_cc(gd, Fw.f(), Mw.f(), P.cc_radius, eps, wk_[0], wk_[1], static_cast<Partial*>(partials_cc.p),
-                &dstats->energy, s);
+      // the energy slot is reduced when the iteration's statistics are published
+      const int n_cc = launch_cc(gd, Fw.f(), Mw.f(), P.cc_radius, eps, wk_[0], wk_[1],
+                                 static_cast<Partial*>(partials_cc.p), &dstats->energy, s, false);
       prof.end(ANTS_KC_CC, Nd * (2.0 * e_i + 2.0 * D * e_f));
       HIP_TRY(hipGetLastError());
       // 3. update-field smoothing + max norms (result in wk[0], wk[1]; scratch wk[2], wk[3])
@@ -885,7 +919,9 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
       prof.end(ANTS_KC_INV, 0.0, 2 * kl, 3, 0, kl);
       std::swap(st_[0], wk_[2]);
       std::swap(st_[2], wk_[3]);
-      HIP_TRY(hipMemcpyAsync(host_stats, dstats, sizeof(IterStats), hipMemcpyDeviceToHost, s));
+      launch_publish_stats(static_cast<const Partial*>(partials_cc.p), n_cc, &dstats->energy, stat_words,
+                           reinterpret_cast<uint32_t*>(host_stats), kStatWords, host_flag, s);
+      HIP_TRY(hipGetLastError());
       for (int k = 0; k < 4; ++k) {
         nst[k] = st_[k];
         nwk[k] = wk_[k];
@@ -918,6 +954,7 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
         (void)hipGetLastError();
         if (it == graphs.end()) use_graphs = false;  // capture not possible here: launch eagerly
       }
+      *reinterpret_cast<volatile uint32_t*>(host_flag) = 0u;
       if (use_graphs && it != graphs.end()) {
         HIP_TRY(hipGraphLaunch(it->second.exec, s));
         run_stats.graph_launches += 1;
@@ -934,7 +971,7 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
         wk[k] = nwk[k];
       }
       // 7. energy, convergence, observer
-      HIP_TRY(hipStreamSynchronize(s));
+      ANTS_TRY(wait_published(host_flag, s));
       bool extra = false;
       ANTS_TRY(inv_finish(extra));
       if (extra && !inv_launches_fixed) kl = std::min(kl + 1, kInvMaxIter + 1);

@@ -325,6 +325,34 @@ __global__ void __launch_bounds__(NT) k_reduce_slot(const Partial* p, int n, Red
   }
 }
 
+// Closes an iteration (one block): reduces the CC partials into `energy`
+// when given, copies the nwords statistics words at `src` to pinned host
+// memory `dst`, and then sets *flag = 1.  The system-scope release fence
+// makes the copy visible to the host before the flag.
+template <int NT>
+__global__ void __launch_bounds__(NT) k_publish_stats(const Partial* cc_parts, int cc_n, RedSlot* energy,
+                                                      const uint32_t* src, uint32_t* dst, int nwords,
+                                                      uint32_t* flag) {
+  if (cc_parts) {
+    double sum;
+    float m;
+    reduce_partials<NT>(cc_parts, cc_n, sum, m);
+    if (threadIdx.x == 0) {
+      energy->sum = sum;
+      energy->maxbits = __float_as_uint(m);
+      energy->computed = 1;
+    }
+    __syncthreads();  // the energy words are copied below by other threads
+  }
+  for (int w = threadIdx.x; w < nwords; w += NT) dst[w] = src[w];
+  __threadfence_system();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    *reinterpret_cast<volatile uint32_t*>(flag) = 1u;
+    __threadfence_system();
+  }
+}
+
 // ---------------------------------------------------------------------------
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, int64_t n) {
   for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
@@ -675,11 +703,16 @@ __global__ void __launch_bounds__(TX* TY, kSmoothWavesPerEU) k_smooth3d(GridDev 
 }
 
 // ---------------------------------------------------------------------------
+// Block 0 also zeroes `clear` (nclear words; the iteration's statistics,
+// which only later kernels of the iteration write).
 template <int D, class T>
 __global__ void __launch_bounds__(kBlock) k_warp_pair(GridDev gl, GridDev gF, const T* __restrict__ Fs,
                                                       const float* __restrict__ uF, float* __restrict__ Fw,
                                                       GridDev gM, const T* __restrict__ Ms,
-                                                      const float* __restrict__ uM, float* __restrict__ Mw) {
+                                                      const float* __restrict__ uM, float* __restrict__ Mw,
+                                                      uint32_t* __restrict__ clear, int nclear) {
+  if (blockIdx.x == 0)
+    for (int w = threadIdx.x; w < nclear; w += blockDim.x) clear[w] = 0u;
   const int64_t N = gl.count();
   const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   if (i >= N) return;
@@ -1755,21 +1788,27 @@ bool launch_smooth3d(const GridDev& g, int np, const float* const* in, const flo
 }
 
 void launch_warp_pair(const GridDev& gl, const GridDev& gF, const float* Fs, const float* uF, float* Fw,
-                      const GridDev& gM, const float* Ms, const float* uM, float* Mw, hipStream_t s) {
+                      const GridDev& gM, const float* Ms, const float* uM, float* Mw, hipStream_t s,
+                      uint32_t* clear, int nclear) {
   const dim3 grid(blocks_for(gl.count()));
   if (gl.D == 2)
-    hipLaunchKernelGGL((k_warp_pair<2, float>), grid, dim3(kBlock), 0, s, gl, gF, Fs, uF, Fw, gM, Ms, uM, Mw);
+    hipLaunchKernelGGL((k_warp_pair<2, float>), grid, dim3(kBlock), 0, s, gl, gF, Fs, uF, Fw, gM, Ms, uM, Mw,
+                       clear, nclear);
   else
-    hipLaunchKernelGGL((k_warp_pair<3, float>), grid, dim3(kBlock), 0, s, gl, gF, Fs, uF, Fw, gM, Ms, uM, Mw);
+    hipLaunchKernelGGL((k_warp_pair<3, float>), grid, dim3(kBlock), 0, s, gl, gF, Fs, uF, Fw, gM, Ms, uM, Mw,
+                       clear, nclear);
 }
 
 void launch_warp_pair_bf16(const GridDev& gl, const GridDev& gF, const uint16_t* Fs, const float* uF, float* Fw,
-                           const GridDev& gM, const uint16_t* Ms, const float* uM, float* Mw, hipStream_t s) {
+                           const GridDev& gM, const uint16_t* Ms, const float* uM, float* Mw, hipStream_t s,
+                           uint32_t* clear, int nclear) {
   const dim3 grid(blocks_for(gl.count()));
   if (gl.D == 2)
-    hipLaunchKernelGGL((k_warp_pair<2, uint16_t>), grid, dim3(kBlock), 0, s, gl, gF, Fs, uF, Fw, gM, Ms, uM, Mw);
+    hipLaunchKernelGGL((k_warp_pair<2, uint16_t>), grid, dim3(kBlock), 0, s, gl, gF, Fs, uF, Fw, gM, Ms, uM, Mw,
+                       clear, nclear);
   else
-    hipLaunchKernelGGL((k_warp_pair<3, uint16_t>), grid, dim3(kBlock), 0, s, gl, gF, Fs, uF, Fw, gM, Ms, uM, Mw);
+    hipLaunchKernelGGL((k_warp_pair<3, uint16_t>), grid, dim3(kBlock), 0, s, gl, gF, Fs, uF, Fw, gM, Ms, uM, Mw,
+                       clear, nclear);
 }
 
 void launch_to_bf16(const float* in, uint16_t* out, int64_t n, hipStream_t s) {
@@ -1804,12 +1843,9 @@ int64_t cc_partials_needed(const GridDev& g, int radius) {
   return red_blocks(g.count());
 }
 
-void launch_cc(const GridDev& g, const float* F, const float* M, int radius, float eps, float* gF2M, float* gM2M,
-               Partial* partials, RedSlot* slot, hipStream_t s) {
-  if (!cc_tiled(g, radius)) {
-    launch_cc_direct(g, F, M, radius, eps, gF2M, gM2M, partials, slot, s);
-    return;
-  }
+int launch_cc(const GridDev& g, const float* F, const float* M, int radius, float eps, float* gF2M, float* gM2M,
+              Partial* partials, RedSlot* slot, hipStream_t s, bool reduce) {
+  if (!cc_tiled(g, radius)) return launch_cc_direct(g, F, M, radius, eps, gF2M, gM2M, partials, slot, s, reduce);
   const int zc = cc_zchunk(g, radius);
   const dim3 grid((g.n[0] + kCCTX - 1) / kCCTX, (g.n[1] + kCCTY - 1) / kCCTY, (g.n[2] + zc - 1) / zc);
   if (radius == 4)
@@ -1818,18 +1854,28 @@ void launch_cc(const GridDev& g, const float* F, const float* M, int radius, flo
   else
     hipLaunchKernelGGL((k_cc3d<2, kCCTX, kCCTY>), grid, dim3(kCCTX * kCCTY), 0, s, g, F, M, eps, zc, gF2M, gM2M,
                        partials);
-  hipLaunchKernelGGL(k_reduce_slot<kCCTX * kCCTY>, dim3(1), dim3(kCCTX * kCCTY), 0, s, partials,
-                     static_cast<int>(grid.x * grid.y * grid.z), slot);
+  const int np = static_cast<int>(grid.x * grid.y * grid.z);
+  if (reduce) hipLaunchKernelGGL(k_reduce_slot<kCCTX * kCCTY>, dim3(1), dim3(kCCTX * kCCTY), 0, s, partials, np, slot);
+  return np;
 }
 
-void launch_cc_direct(const GridDev& g, const float* F, const float* M, int radius, float eps, float* gF2M,
-                      float* gM2M, Partial* partials, RedSlot* slot, hipStream_t s) {
+int launch_cc_direct(const GridDev& g, const float* F, const float* M, int radius, float eps, float* gF2M,
+                     float* gM2M, Partial* partials, RedSlot* slot, hipStream_t s, bool reduce) {
   const dim3 grid(red_blocks(g.count()));
   if (g.D == 2)
     hipLaunchKernelGGL(k_cc_direct<2>, grid, dim3(kBlock), 0, s, g, F, M, radius, eps, gF2M, gM2M, partials);
   else
     hipLaunchKernelGGL(k_cc_direct<3>, grid, dim3(kBlock), 0, s, g, F, M, radius, eps, gF2M, gM2M, partials);
-  hipLaunchKernelGGL(k_reduce_slot<kBlock>, dim3(1), dim3(kBlock), 0, s, partials, static_cast<int>(grid.x), slot);
+  const int np = static_cast<int>(grid.x);
+  if (reduce) hipLaunchKernelGGL(k_reduce_slot<kBlock>, dim3(1), dim3(kBlock), 0, s, partials, np, slot);
+  return np;
+}
+
+void launch_publish_stats(const Partial* cc_parts, int cc_n, RedSlot* energy, const uint32_t* src, uint32_t* dst,
+                          int nwords, uint32_t* flag, hipStream_t s) {
+  static_assert(kCCTX * kCCTY == kBlock, "CC partials are reduced in the order of k_reduce_slot<kBlock>");
+  hipLaunchKernelGGL(k_publish_stats<kBlock>, dim3(1), dim3(kBlock), 0, s, cc_parts, cc_n, energy, src, dst, nwords,
+                     flag);
 }
 
 void launch_compose(const GridDev& g, int np, const float* const* disp, const float* const* warp, float* const* out,

@@ -109,10 +109,12 @@ bool launch_smooth3d(const GridDev& g, int np, const float* const* in, const flo
 
 // Fw = Fs(x + uF(x)), Mw = Ms(x + uM(x)) on level grid gl (Fs on gF, Ms on gM).
 void launch_warp_pair(const GridDev& gl, const GridDev& gF, const float* Fs, const float* uF, float* Fw,
-                      const GridDev& gM, const float* Ms, const float* uM, float* Mw, hipStream_t s);
+                      const GridDev& gM, const float* Ms, const float* uM, float* Mw, hipStream_t s,
+                      uint32_t* clear = nullptr, int nclear = 0);
 // Same with the smoothed images stored as BF16.
 void launch_warp_pair_bf16(const GridDev& gl, const GridDev& gF, const uint16_t* Fs, const float* uF, float* Fw,
-                           const GridDev& gM, const uint16_t* Ms, const float* uM, float* Mw, hipStream_t s);
+                           const GridDev& gM, const uint16_t* Ms, const float* uM, float* Mw, hipStream_t s,
+                           uint32_t* clear = nullptr, int nclear = 0);
 // float -> BF16 (round to nearest even).
 void launch_to_bf16(const float* in, uint16_t* out, int64_t n, hipStream_t s);
 void launch_copy_calibration(const void* in, void* out, int64_t n4, hipStream_t s);
@@ -122,10 +124,17 @@ void launch_warp_image(const GridDev& gi, const float* img, const GridDev& go, c
 
 // CC value + both derivatives.  `partials` must hold cc_partials_needed(g) entries.
 int64_t cc_partials_needed(const GridDev& g, int radius);
-void launch_cc(const GridDev& g, const float* F, const float* M, int radius, float eps, float* gF2M, float* gM2M,
-               Partial* partials, RedSlot* slot, hipStream_t s);
-void launch_cc_direct(const GridDev& g, const float* F, const float* M, int radius, float eps, float* gF2M,
-                      float* gM2M, Partial* partials, RedSlot* slot, hipStream_t s);
+// Returns the number of partials written; with reduce = false the energy slot
+// is left to launch_publish_stats.
+int launch_cc(const GridDev& g, const float* F, const float* M, int radius, float eps, float* gF2M, float* gM2M,
+              Partial* partials, RedSlot* slot, hipStream_t s, bool reduce = true);
+int launch_cc_direct(const GridDev& g, const float* F, const float* M, int radius, float eps, float* gF2M,
+                     float* gM2M, Partial* partials, RedSlot* slot, hipStream_t s, bool reduce = true);
+// One block: reduce cc_n CC partials into *energy (skipped when cc_parts is
+// null), copy nwords words of src to pinned host memory dst, then set *flag
+// (pinned host memory) to 1 once the copy is visible to the host.
+void launch_publish_stats(const Partial* cc_parts, int cc_n, RedSlot* energy, const uint32_t* src, uint32_t* dst,
+                          int nwords, uint32_t* flag, hipStream_t s);
 
 // out_k = warp_k + scale_k * disp_k(x + warp_k(x)), scale_k = rate / max_k when maxbits given (max>0), else 1.
 void launch_compose(const GridDev& g, int np, const float* const* disp, const float* const* warp,
