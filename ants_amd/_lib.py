@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(HERE, "libants_syn.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "ants_syn.h")
 
 ANTS_OK = 0
+ANTS_ERR_CANCELLED = 5
 ERRORS = {1: "INVALID", 2: "HIP", 3: "OOM", 4: "UNSUPPORTED", 5: "CANCELLED"}
 MAX_LEVELS = 8
 KC_NAMES = ["warp", "cc", "smooth", "compose", "inv"]
@@ -76,6 +77,16 @@ class TransformC(C.Structure):
                 ("center", C.c_double * 3), ("field", ImageDesc), ("d_field", C.c_void_p)]
 
 
+class IterRecordC(C.Structure):
+    _fields_ = [
+        ("level", C.c_int),
+        ("iteration", C.c_int),
+        ("energy", C.c_double),
+        ("convergence", C.c_double),
+        ("inv_iters", C.c_int * 4),
+    ]
+
+
 ITER_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, C.POINTER(C.c_int))
 
 _lib = None
@@ -122,6 +133,7 @@ def lib():
     L.ants_syn_register_device.argtypes = [vp, pdesc, vp, pdesc, vp, vp, vp, ITER_CB, vp]
     L.ants_syn_get_state_device.argtypes = [vp, vp, vp, vp, vp]
     L.ants_syn_get_stats.argtypes = [vp, C.POINTER(SynStatsC)]
+    L.ants_syn_get_trace.argtypes = [vp, C.POINTER(IterRecordC), C.c_int64, C.POINTER(C.c_int64)]
     L.ants_syn_set_profiling.argtypes = [vp, C.c_uint]
     L.ants_syn_get_kernel_stats.argtypes = [vp, C.POINTER(KernelStatsC)]
     L.ants_op_smooth_image.argtypes = [pdesc, vp, vp, d, i, vp]

@@ -488,6 +488,7 @@ struct ants_syn_handle {
   IterStats* host_stats = nullptr;  // pinned
   uint32_t* host_flag = nullptr;    // pinned, coherent: set when an iteration's statistics are published
   ants_syn_stats run_stats{};
+  std::vector<ants_syn_iteration_record> trace;  // the last run's iteration records
   HostGrid last_grid{};
   bool have_state = false;
   float* state[4] = {nullptr, nullptr, nullptr, nullptr};  // f2m fwd, f2m inv, m2m fwd, m2m inv
@@ -731,6 +732,7 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
     inv_launches_fixed = ks != nullptr;
   }
   std::memset(&run_stats, 0, sizeof(run_stats));
+  trace.clear();
   std::memset(&prof.acc, 0, sizeof(prof.acc));
   prof.s = s;
   prof.recs.clear();
@@ -993,6 +995,7 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
       run_stats.voxel_iterations += static_cast<double>(N);
       run_stats.algorithmic_bytes +=
           static_cast<double>(N) * ((2.0 * gsz * e_i + 14.0 * D * e_f) + (3.0 * ksum + 4.0) * D * e_f);
+      trace.push_back(ants_syn_iteration_record{l, iter, E, conv, {kin[0], kin[1], kin[2], kin[3]}});
       if (cb && cb(user, l, iter, E, conv, kin) != 0) return fail(ANTS_ERR_CANCELLED, "cancelled by the observer");
     }
   }
@@ -1179,6 +1182,16 @@ int ants_syn_get_state_device(ants_syn_handle* h, float* d_f2m_fwd, float* d_f2m
 int ants_syn_get_stats(const ants_syn_handle* h, ants_syn_stats* out) {
   if (!h || !out) return fail(ANTS_ERR_INVALID, "null argument");
   *out = h->run_stats;
+  return ANTS_OK;
+}
+
+int ants_syn_get_trace(const ants_syn_handle* h, ants_syn_iteration_record* out, int64_t capacity, int64_t* count) {
+  if (!h || !count) return fail(ANTS_ERR_INVALID, "null argument");
+  if (capacity < 0 || (capacity > 0 && !out)) return fail(ANTS_ERR_INVALID, "bad output buffer");
+  const int64_t n = static_cast<int64_t>(h->trace.size());
+  const int64_t w = std::min(n, capacity);
+  for (int64_t i = 0; i < w; ++i) out[i] = h->trace[i];
+  *count = n;
   return ANTS_OK;
 }
 

@@ -230,17 +230,37 @@ class SyNRegistration:
                 "algorithmic_bytes": s.algorithmic_bytes, "graph_launches": int(s.graph_launches),
                 "eager_iterations": int(s.eager_iterations), "inv_tails": int(s.inv_tails)}
 
-    def _callback(self, trace: list):
+    @staticmethod
+    def _callback(observer):
+        """ctypes observer for a live per-iteration callback, or a null
+        callback.  The engine keeps the trace itself (`trace()`), so no
+        callback is passed unless the caller asks for one: a Python call
+        between iterations holds the next iteration back."""
+        if observer is None:
+            return L.ITER_CB()
+
         def cb(user, level, it, e, conv, kin):
-            trace.append(IterationRecord(level, it, e, conv, tuple(kin[i] for i in range(4))))
-            return 0
+            rec = IterationRecord(level, it, e, conv, tuple(kin[i] for i in range(4)))
+            return 1 if observer(rec) else 0
         return L.ITER_CB(cb)
 
+    def trace(self) -> List[IterationRecord]:
+        """Iteration records of the last run (ants_syn_get_trace)."""
+        n = C.c_int64(0)
+        L.check(L.lib().ants_syn_get_trace(self.handle, None, 0, C.byref(n)))
+        buf = (L.IterRecordC * max(n.value, 1))()
+        L.check(L.lib().ants_syn_get_trace(self.handle, buf, n.value, C.byref(n)))
+        return [IterationRecord(r.level, r.iteration, r.energy, r.convergence, tuple(r.inv_iters))
+                for r in buf[:n.value]]
+
     def register(self, fixed: np.ndarray, moving: np.ndarray, fixed_spacing=None, fixed_origin=None,
-                 moving_spacing=None, moving_origin=None, fixed_direction=None, moving_direction=None) -> SyNResult:
+                 moving_spacing=None, moving_origin=None, fixed_direction=None, moving_direction=None,
+                 observer=None) -> SyNResult:
         """Host arrays in (numpy order, x fastest), host fields out (physical
         vectors on the fixed image's last level grid).  Directions are
-        row-major dim x dim orthonormal matrices (identity by default)."""
+        row-major dim x dim orthonormal matrices (identity by default).
+        `observer(record) -> bool` is called after every iteration (True
+        cancels the run); without it the trace is read after the run."""
         fixed = np.ascontiguousarray(fixed, dtype=np.float32)
         moving = np.ascontiguousarray(moving, dtype=np.float32)
         if fixed.ndim != self.dim or moving.ndim != self.dim:
@@ -253,21 +273,23 @@ class SyNRegistration:
         oshape = tuple(int(out.size[k]) for k in reversed(range(self.dim))) + (self.dim,)
         fwd = np.empty(oshape, np.float32)
         inv = np.empty(oshape, np.float32)
-        trace: List[IterationRecord] = []
-        cb = self._callback(trace)
+        cb = self._callback(observer)
         L.check(L.lib().ants_syn_register(self.handle, C.byref(gf), fixed.ctypes.data_as(C.c_void_p), C.byref(gm),
                                           moving.ctypes.data_as(C.c_void_p), fwd.ctypes.data_as(C.c_void_p),
                                           inv.ctypes.data_as(C.c_void_p), cb, None))
-        return SyNResult(fwd, inv, trace, self.stats())
+        return SyNResult(fwd, inv, self.trace(), self.stats())
 
     def register_device(self, gf: L.ImageDesc, d_fixed: L.DeviceArray, gm: L.ImageDesc, d_moving: L.DeviceArray,
-                        d_fwd: Optional[L.DeviceArray], d_inv: Optional[L.DeviceArray], trace: Optional[list] = None):
-        """Device buffers in and out (inputs already resident in HBM)."""
+                        d_fwd: Optional[L.DeviceArray], d_inv: Optional[L.DeviceArray], trace: Optional[list] = None,
+                        observer=None):
+        """Device buffers in and out (inputs already resident in HBM).
+        Returns the iteration records (appended to `trace` when given)."""
         tr = trace if trace is not None else []
-        cb = self._callback(tr)
+        cb = self._callback(observer)
         L.check(L.lib().ants_syn_register_device(self.handle, C.byref(gf), d_fixed.ptr, C.byref(gm), d_moving.ptr,
                                                  d_fwd.ptr if d_fwd else None, d_inv.ptr if d_inv else None, cb,
                                                  None))
+        tr.extend(self.trace())
         return tr
 
     def state(self, out_shape) -> List[np.ndarray]:

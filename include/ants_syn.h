@@ -150,6 +150,21 @@ typedef struct ants_syn_stats {
 } ants_syn_stats;
 int ants_syn_get_stats(const ants_syn_handle* h, ants_syn_stats* out);
 
+/* The last run's iteration records: the observer's arguments
+ * (Examples/antsDisplacementAndVelocityFieldRegistrationCommandIterationUpdate.h:117-195),
+ * kept whether or not an observer was passed, so that a caller can read the
+ * trace after the run instead of being called back between iterations.
+ * Writes min(capacity, total) records to out (out may be NULL when capacity
+ * is 0) and the total to *count. */
+typedef struct ants_syn_iteration_record {
+  int level;
+  int iteration;
+  double energy;
+  double convergence;
+  int inv_iters[4];
+} ants_syn_iteration_record;
+int ants_syn_get_trace(const ants_syn_handle* h, ants_syn_iteration_record* out, int64_t capacity, int64_t* count);
+
 /* Kernel classes timed with HIP events on the handle's stream when enabled
  * by ants_syn_set_profiling (bit k enables class k).  A profiled run launches
  * its iterations kernel by kernel (no graph replay).  The inversion class is

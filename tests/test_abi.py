@@ -87,7 +87,8 @@ def test_ctypes_structs_match_c_layout(tmp_path):
     from ants_amd import serving as S
     structs = {"ants_image_desc": L.ImageDesc, "ants_syn_params": L.SynParamsC, "ants_syn_stats": L.SynStatsC,
                "ants_syn_kernel_stats": L.KernelStatsC, "ants_batch_config": S.BatchConfigC,
-               "ants_syn_request": S.RequestC, "ants_request_result": S.ResultC, "ants_transform": L.TransformC}
+               "ants_syn_request": S.RequestC, "ants_request_result": S.ResultC, "ants_transform": L.TransformC,
+               "ants_syn_iteration_record": L.IterRecordC}
     exprs, want = [], []
     for cname, py in structs.items():
         exprs.append(f"sizeof({cname})")
@@ -141,3 +142,18 @@ def test_level_geometry_with_direction_matches_oracle():
         assert [out.size[k] for k in range(3)] == ref.size
         np.testing.assert_allclose([out.origin[k] for k in range(3)], ref.origin, atol=1e-12)
         np.testing.assert_allclose(np.array([out.direction[k] for k in range(9)]).reshape(3, 3), q)
+
+
+def test_observer_argument_converts():
+    """Without an observer the wrappers pass a null callback that the
+    ctypes signature accepts; with one, a callable that returns 1 to cancel."""
+    from ants_amd.registration import SyNRegistration, IterationRecord
+    null_cb = SyNRegistration._callback(None)
+    L.ITER_CB.from_param(null_cb)
+    assert not bool(null_cb)
+    kin = (C.c_int * 4)(1, 2, 3, 4)
+    got = []
+    cb = SyNRegistration._callback(lambda r: got.append(r) or r.iteration == 2)
+    L.ITER_CB.from_param(cb)
+    assert cb(None, 0, 1, -0.5, 0.1, kin) == 0 and cb(None, 0, 2, -0.6, 0.1, kin) == 1
+    assert got[0] == IterationRecord(0, 1, -0.5, 0.1, (1, 2, 3, 4))

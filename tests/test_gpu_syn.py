@@ -220,3 +220,27 @@ def test_direction_equivariance_on_gpu():
     np.testing.assert_allclose(rR.energy, rI.energy, rtol=1e-5)
     np.testing.assert_allclose(rR.fwd, rI.fwd @ R.T, atol=2e-3)
     np.testing.assert_allclose(rR.inv, rI.inv @ R.T, atol=2e-3)
+
+
+def test_trace_and_observer():
+    """The engine keeps the iteration records itself: a run without an
+    observer returns the same records as a run with a live observer, and an
+    observer returning True cancels after that iteration with
+    ANTS_ERR_CANCELLED, leaving the records up to it."""
+    pair = ph.phantom_pair_3d(2, size=(48, 56, 40))
+    stage = SyNStage.from_args("SyN[0.2,3,0]", "CC[f,m,1,2]", "[6x4,1e-6,10]", "2x1", "1x0vox")
+    reg = SyNRegistration(stage, 3)
+    quiet = reg.register(pair.fixed, pair.moving, pair.spacing, pair.origin, pair.spacing, pair.origin)
+    seen = []
+    live = reg.register(pair.fixed, pair.moving, pair.spacing, pair.origin, pair.spacing, pair.origin,
+                        observer=lambda r: seen.append(r) and False)
+    assert len(quiet.trace) == sum(quiet.stats["iterations"]) > 0
+    assert seen == live.trace == quiet.trace
+    np.testing.assert_array_equal(live.fwd, quiet.fwd)
+    with pytest.raises(L.AntsError) as exc:
+        reg.register(pair.fixed, pair.moving, pair.spacing, pair.origin, pair.spacing, pair.origin,
+                     observer=lambda r: r.iteration == 3)
+    assert exc.value.code == L.ANTS_ERR_CANCELLED
+    cut = reg.trace()
+    assert cut == quiet.trace[:3]
+    reg.close()
