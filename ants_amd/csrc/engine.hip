@@ -677,7 +677,8 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
   }
   ANTS_TRY(Fw.ensure(maxN * sizeof(float)));
   ANTS_TRY(Mw.ensure(maxN * sizeof(float)));
-  for (auto& b : fld) ANTS_TRY(b.ensure(fbytes));
+  const int n_fld = inv_stores_residual() ? 10 : 8;  // the last two hold inversion residuals
+  for (int k = 0; k < n_fld; ++k) ANTS_TRY(fld[k].ensure(fbytes));
   ANTS_TRY(stats.ensure(sizeof(IterStats)));
   int64_t ccp = 0, invp = 0;
   for (int l = 0; l < P.n_levels; ++l) {
@@ -752,7 +753,7 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
   // field slots: state (f2m_f, f2m_i, m2m_f, m2m_i) and work (4)
   float* st[4] = {fld[0].f(), fld[1].f(), fld[2].f(), fld[3].f()};
   float* wk[4] = {fld[4].f(), fld[5].f(), fld[6].f(), fld[7].f()};
-  float* resid[2] = {fld[8].f(), fld[9].f()};
+  float* resid[2] = {fld[8].f(), fld[9].f()};  // null unless the inversion stores residuals
   const GridDev gFd = to_dev(gF), gMd = to_dev(gM);
   const float eps = std::numeric_limits<float>::epsilon();
   HostGrid prev{};
@@ -1349,7 +1350,7 @@ int ants_op_invert(const ants_image_desc* gdsc, const float* d_field, const floa
   else
     HIP_TRY(hipMemsetAsync(d_inv, 0, b, s));
   DevBuf rbuf;
-  ANTS_TRY(rbuf.ensure(b));
+  if (inv_stores_residual()) ANTS_TRY(rbuf.ensure(b));
   const float* field[1] = {d_field};
   float* inv[1] = {d_inv};
   float* res[1] = {rbuf.f()};
@@ -1524,7 +1525,7 @@ int64_t footprint_bytes(const ants_syn_params& P, const HostGrid& gF, const Host
   b += std::max(NF, NM) * 4;             // smoothing scratch
   if (P.precision == ANTS_PRECISION_BF16_PYRAMID) b += (NF + NM) * 2;
   b += 2 * maxN * 4;                     // warped images
-  b += 10 * maxN * D * 4;                // middle, work and residual fields
+  b += (inv_stores_residual() ? 10 : 8) * maxN * D * 4;  // middle, work (and residual) fields
   b += (NF + NM) * 4 + NM * 4;           // request inputs (images, labels)
   b += 2 * maxN * D * 4 + maxN * 4;      // outputs (fields, labels)
   return b;

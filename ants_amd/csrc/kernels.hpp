@@ -155,6 +155,9 @@ constexpr InvGate kNoGate{{nullptr, nullptr}, 0, -1};
 // partials holds 2 x np x inv_partials_needed(g) entries (the partials of
 // step j are read by step j + 1).
 int64_t inv_partials_needed(const GridDev& g);
+// Whether the inversion kernel keeps each norm pass's residual in `res`
+// (one field array per problem); when false, `res` may be null.
+bool inv_stores_residual();
 void launch_inv_step(const GridDev& g, int np, const float* const* field, float* const* inv, float* const* res,
                      RedSlot* const* slots, Partial* partials, int j, hipStream_t s, const InvGate& gate = kNoGate);
 
