@@ -1187,10 +1187,10 @@ __device__ __forceinline__ float inv_residual(const GridDev& g, const float* __r
   return sqrtf(s2);
 }
 
-// The residual e_j of each norm pass is stored (res), so step j+1 applies its
-// update from the stored e_j instead of gathering the field again: one
-// gather per voxel and step.  Norms are recomputed from the stored residual
-// with the same arithmetic, so decisions and results are unchanged.
+// With kInvStoreResidual the residual e_j of each norm pass is stored (res)
+// and step j+1 applies its update from it instead of gathering the field
+// again; norms are then recomputed from the stored residual with the same
+// arithmetic, so decisions and results are the same either way.
 template <int D>
 __device__ __forceinline__ float resid_norm(const GridDev& g, const float e[3]) {
   float s2 = 0.f;
