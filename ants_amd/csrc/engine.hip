@@ -483,8 +483,8 @@ struct ants_syn_handle {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   DevBuf F, M, Fs, Ms, tmp, Fw, Mw, Fb, Mb;  // Fb, Mb: BF16 pyramid images
-  DevBuf fld[10];  // 4 middle fields, 4 work fields, 2 inversion residuals
-  DevBuf stats, partials_cc, partials_inv, minmax;
+  DevBuf fld[8];  // 4 middle fields, 4 work fields
+  DevBuf stats, partials_cc, partials_inv, masks_inv, minmax;
   IterStats* host_stats = nullptr;  // pinned
   uint32_t* host_flag = nullptr;    // pinned, coherent: set when an iteration's statistics are published
   ants_syn_stats run_stats{};
@@ -507,7 +507,7 @@ struct ants_syn_handle {
 
   ~ants_syn_handle() {
     clear_graphs();
-    DevBuf* all[] = {&F, &M, &Fs, &Ms, &tmp, &Fw, &Mw, &Fb, &Mb, &stats, &partials_cc, &partials_inv, &minmax};
+    DevBuf* all[] = {&F, &M, &Fs, &Ms, &tmp, &Fw, &Mw, &Fb, &Mb, &stats, &partials_cc, &partials_inv, &masks_inv, &minmax};
     for (DevBuf* b : all) b->release();
     for (auto& b : fld) b.release();
     if (host_stats) (void)hipHostFree(host_stats);
@@ -677,16 +677,17 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
   }
   ANTS_TRY(Fw.ensure(maxN * sizeof(float)));
   ANTS_TRY(Mw.ensure(maxN * sizeof(float)));
-  const int n_fld = inv_stores_residual() ? 10 : 8;  // the last two hold inversion residuals
-  for (int k = 0; k < n_fld; ++k) ANTS_TRY(fld[k].ensure(fbytes));
+  for (auto& b : fld) ANTS_TRY(b.ensure(fbytes));
   ANTS_TRY(stats.ensure(sizeof(IterStats)));
-  int64_t ccp = 0, invp = 0;
+  int64_t ccp = 0, invp = 0, invw = 0;
   for (int l = 0; l < P.n_levels; ++l) {
     ccp = std::max(ccp, cc_partials_needed(to_dev(lg[l]), P.cc_radius));
     invp = std::max(invp, inv_partials_needed(to_dev(lg[l])));
+    invw = std::max(invw, inv_mask_words(to_dev(lg[l])));
   }
   ANTS_TRY(partials_cc.ensure(ccp * sizeof(Partial)));
   ANTS_TRY(partials_inv.ensure(4 * invp * sizeof(Partial)));  // 2 step parities x 2 problems
+  ANTS_TRY(masks_inv.ensure(4 * invw * sizeof(uint64_t)));    // 2 step parities x 2 problems
   if (!host_stats) HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&host_stats), sizeof(IterStats), hipHostMallocDefault));
   if (!host_flag) HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&host_flag), 256, hipHostMallocCoherent));
 
@@ -753,7 +754,6 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
   // field slots: state (f2m_f, f2m_i, m2m_f, m2m_i) and work (4)
   float* st[4] = {fld[0].f(), fld[1].f(), fld[2].f(), fld[3].f()};
   float* wk[4] = {fld[4].f(), fld[5].f(), fld[6].f(), fld[7].f()};
-  float* resid[2] = {fld[8].f(), fld[9].f()};  // null unless the inversion stores residuals
   const GridDev gFd = to_dev(gF), gMd = to_dev(gM);
   const float eps = std::numeric_limits<float>::epsilon();
   HostGrid prev{};
@@ -801,8 +801,11 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
     auto inv_steps_on = [&](int batch, const float* const* field, float* const* inv, int j0, int j1,
                             const InvGate& gate) -> int {
       RedSlot* slots[2] = {dstats->inv[batch], dstats->inv[batch + 2]};
-      for (int j = j0; j < j1; ++j)
-        launch_inv_step(gd, 2, field, inv, resid, slots, static_cast<Partial*>(partials_inv.p), j, s, gate);
+      Partial* parts = static_cast<Partial*>(partials_inv.p);
+      uint64_t* masks = static_cast<uint64_t*>(masks_inv.p);
+      // step 0 is preceded by the sample launch that bounds its max norm
+      if (j0 == 0) launch_inv_step(gd, 2, field, inv, slots, parts, masks, -1, s, gate);
+      for (int j = j0; j < j1; ++j) launch_inv_step(gd, 2, field, inv, slots, parts, masks, j, s, gate);
       HIP_TRY(hipGetLastError());
       return ANTS_OK;
     };
@@ -810,7 +813,8 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
     auto inv_tail_on = [&](int batch, const float* const* field, float* const* inv, int j0) -> int {
       prof.begin(ANTS_KC_INV);
       ANTS_TRY(inv_steps_on(batch, field, inv, j0, kInvMaxIter + 1, kNoGate));
-      prof.end(ANTS_KC_INV, 0.0, kInvMaxIter + 1 - j0, 1 << batch, j0, kInvMaxIter + 1);
+      prof.end(ANTS_KC_INV, 0.0, kInvMaxIter + 1 - j0 + (j0 == 0 && inv_speculates(gd) ? 1 : 0), 1 << batch, j0,
+               kInvMaxIter + 1);
       return ANTS_OK;
     };
     // After the iteration's sequence: a batch whose problems computed a norm
@@ -919,7 +923,7 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
           gate = InvGate{{dstats->inv[0], dstats->inv[2]}, 2, kl - 1};
         ANTS_TRY(inv_steps_on(batch, field, inv, 0, kl, gate));
       }
-      prof.end(ANTS_KC_INV, 0.0, 2 * kl, 3, 0, kl);
+      prof.end(ANTS_KC_INV, 0.0, 2 * (kl + (inv_speculates(gd) ? 1 : 0)), 3, 0, kl);  // per batch: [sample] + kl steps
       std::swap(st_[0], wk_[2]);
       std::swap(st_[2], wk_[3]);
       launch_publish_stats(static_cast<const Partial*>(partials_cc.p), n_cc, &dstats->energy, stat_words,
@@ -1343,20 +1347,21 @@ int ants_op_invert(const ants_image_desc* gdsc, const float* d_field, const floa
   OpScratch sc;
   ANTS_TRY(sc.stats.ensure(sizeof(RedSlot) * kInvSlots));
   ANTS_TRY(sc.partials.ensure(2 * inv_partials_needed(gd) * sizeof(Partial)));  // 2 step parities
+  DevBuf masks;
+  ANTS_TRY(masks.ensure(2 * inv_mask_words(gd) * sizeof(uint64_t)));
   hipStream_t s = static_cast<hipStream_t>(stream);
   HIP_TRY(hipMemsetAsync(sc.stats.p, 0, sizeof(RedSlot) * kInvSlots, s));
   if (d_init)
     HIP_TRY(hipMemcpyAsync(d_inv, d_init, b, hipMemcpyDeviceToDevice, s));
   else
     HIP_TRY(hipMemsetAsync(d_inv, 0, b, s));
-  DevBuf rbuf;
-  if (inv_stores_residual()) ANTS_TRY(rbuf.ensure(b));
   const float* field[1] = {d_field};
   float* inv[1] = {d_inv};
-  float* res[1] = {rbuf.f()};
   RedSlot* slots[1] = {static_cast<RedSlot*>(sc.stats.p)};
-  for (int j = 0; j <= kInvMaxIter; ++j)
-    launch_inv_step(gd, 1, field, inv, res, slots, static_cast<Partial*>(sc.partials.p), j, s);
+  Partial* parts = static_cast<Partial*>(sc.partials.p);
+  uint64_t* mk = static_cast<uint64_t*>(masks.p);
+  launch_inv_step(gd, 1, field, inv, slots, parts, mk, -1, s);
+  for (int j = 0; j <= kInvMaxIter; ++j) launch_inv_step(gd, 1, field, inv, slots, parts, mk, j, s);
   HIP_TRY(hipGetLastError());
   RedSlot r[kInvSlots];
   HIP_TRY(hipMemcpyAsync(r, sc.stats.p, sizeof(r), hipMemcpyDeviceToHost, s));
@@ -1525,7 +1530,8 @@ int64_t footprint_bytes(const ants_syn_params& P, const HostGrid& gF, const Host
   b += std::max(NF, NM) * 4;             // smoothing scratch
   if (P.precision == ANTS_PRECISION_BF16_PYRAMID) b += (NF + NM) * 2;
   b += 2 * maxN * 4;                     // warped images
-  b += (inv_stores_residual() ? 10 : 8) * maxN * D * 4;  // middle, work (and residual) fields
+  b += 8 * maxN * D * 4;                 // middle and work fields
+  b += 4 * ((maxN + 63) / 64) * 8;       // inversion candidate masks
   b += (NF + NM) * 4 + NM * 4;           // request inputs (images, labels)
   b += 2 * maxN * D * 4 + maxN * 4;      // outputs (fields, labels)
   return b;

@@ -13,6 +13,7 @@
 #include <utility>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 
 namespace ants {
 namespace {
@@ -1187,37 +1188,33 @@ __device__ __forceinline__ float inv_residual(const GridDev& g, const float* __r
   return sqrtf(s2);
 }
 
-// With kInvStoreResidual the residual e_j of each norm pass is stored (res)
-// and step j+1 applies its update from it instead of gathering the field
-// again; norms are then recomputed from the stored residual with the same
-// arithmetic, so decisions and results are the same either way.
-template <int D>
-__device__ __forceinline__ float resid_norm(const GridDev& g, const float e[3]) {
-  float s2 = 0.f;
-#pragma unroll
-  for (int d = 0; d < D; ++d) {
-    const float q = e[d] * g.isp[d];
-    s2 += q * q;
-  }
-  return sqrtf(s2);
-}
-
 // Inversion steps run large blocks over a bounded grid (kInvBlocks per
 // problem): steps that have converged cost one cheap dispatch, active steps
 // still fill the device, and the closing reduction reads one partial per
-// block.  The gather depends on the loaded inverse, so each round would be
-// a chain of two memory round trips; with ANTS_INV_PREFETCH the next
-// round's inverse and residual are loaded while this round gathers (one
-// voxel per round then measured faster than two: 23.2 -> 22.7 ms per
-// 192x224x192 pair; without prefetch, two).  The per-thread sums run over
-// the voxels in the same order for any unroll.
+// block.  Block b (renumbered by slab_block) walks the contiguous voxel
+// range [b*chunk, (b+1)*chunk), chunk a multiple of 64, one voxel per thread
+// and round, so the 64 lanes of a wave cover one 64-voxel word of the
+// candidate masks below, in every launch.  The next round's inverse (and
+// mask word) are loaded while this round gathers the field.
+//
+// Speculative updates.  The update after norm pass j clamps e_j at
+// eps_j * max_j, a global maximum known only after the pass.  Launch j
+// therefore writes inv_{j+1} = inv_j - eps_j * e_j in place for every voxel
+// whose |e_j| is at most L_j = eps_j * q_j, where q_j <= max_j is the
+// largest of a sample of exact |e_j| (one word in 16; computed by the
+// previous launch from its own speculative values, or by the sample launch
+// j = -1 before the first step).  Such voxels are unclamped for certain.
+// The others ("candidates") keep inv_j and set their bit in the mask of
+// step j; launch j+1 recomputes e_j for them (the field is unchanged),
+// applies the clamped update with max_j, and goes on from the result.  The
+// voxel-wise arithmetic is that of the plain update, so the results are the
+// same bits; with no sample (L = 0) every voxel is a candidate and the
+// scheme falls back to recomputing the update in the next launch.
 #ifndef ANTS_INV_THREADS
 #define ANTS_INV_THREADS 512
 #endif
-#ifndef ANTS_INV_UNROLL
-#define ANTS_INV_UNROLL 1
-#endif
 constexpr int kInvThreads = ANTS_INV_THREADS;
+static_assert(kInvThreads % 64 == 0, "waves cover whole mask words");
 // Grid per problem: one block per kInvThreads * ANTS_INV_VPT voxels, at most
 // ANTS_INV_MAX_BLOCKS.  Every block of a step first reduces the previous
 // step's per-block partials, so fewer, fuller blocks shorten that prologue:
@@ -1227,28 +1224,74 @@ constexpr int kInvThreads = ANTS_INV_THREADS;
 #define ANTS_INV_MAX_BLOCKS 768
 #endif
 constexpr int kInvBlocks = ANTS_INV_MAX_BLOCKS;
-constexpr int kInvUnroll = ANTS_INV_UNROLL;
-// Storing each norm pass's residual saves the update pass a gather, but the
-// two extra field arrays per step push other data out of the Infinity
-// Cache: recomputing it measured 46.4 -> 46.1 ms per 192x224x192 pair.
-#ifndef ANTS_INV_STORE_RES
-#define ANTS_INV_STORE_RES 0
-#endif
-constexpr bool kInvStoreResidual = ANTS_INV_STORE_RES != 0;
-#ifndef ANTS_INV_XCD_MAP
-#define ANTS_INV_XCD_MAP 1
-#endif
-#ifndef ANTS_INV_PREFETCH
-#define ANTS_INV_PREFETCH 1
-#endif
 #ifndef ANTS_INV_WAVES
 #define ANTS_INV_WAVES 1
 #endif
+// Sampled words of the lower bound q_j: one in 16.  Wave v of a block walks
+// words base + v + 8r (base a multiple of 8), so the rule rotates the
+// sampled rounds through the waves instead of giving them all to one.
+__device__ __forceinline__ bool inv_sampled_word(int64_t w) { return (((w >> 3) + ((w & 7) << 1)) & 15) == 0; }
+
+template <int NT>
+__device__ __forceinline__ void reduce_partials_q(const Partial* p, int n, double& s, float& m, float& q) {
+  __shared__ double r_s;
+  __shared__ float r_m, r_q;
+  double a = 0.0;
+  float b = 0.f;
+  float c = 0.f;
+  for (int i = threadIdx.x; i < n; i += NT) {
+    a += p[i].s;
+    b = fmaxf(b, p[i].m);
+    c = fmaxf(c, p[i].q);
+  }
+  block_sum_max<NT>(a, b);
+  c = wave_max(c);
+  __shared__ float wq[NT / 64];
+  if ((threadIdx.x & 63) == 0) wq[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float t = 0.f;
+    for (int w = 0; w < NT / 64; ++w) t = fmaxf(t, wq[w]);
+    r_s = a;
+    r_m = b;
+    r_q = t;
+  }
+  __syncthreads();
+  s = r_s;
+  m = r_m;
+  q = r_q;
+}
+
+template <int NT>
+__device__ __forceinline__ void block_partial_q(double s, float m, float q, Partial* out) {
+  __shared__ float wq[NT / 64];
+  q = wave_max(q);
+  if ((threadIdx.x & 63) == 0) wq[threadIdx.x >> 6] = q;
+  block_sum_max<NT>(s, m);  // its barrier also publishes wq
+  if (threadIdx.x == 0) {
+    float t = 0.f;
+    for (int w = 0; w < NT / 64; ++w) t = fmaxf(t, wq[w]);
+    out->s = s;
+    out->m = m;
+    out->q = t;
+  }
+}
+
+// Voxel range of this block (see above); chunk is a multiple of 64.
+__device__ __forceinline__ void inv_range(int64_t N, int64_t& vbeg, int64_t& vend) {
+  const int64_t nb = gridDim.x;
+  const int64_t vb = slab_block(blockIdx.x, nb);
+  const int64_t chunk = ((N + nb - 1) / nb + 63) / 64 * 64;
+  vbeg = vb * chunk;
+  vend = min(N, vbeg + chunk);
+}
 
 template <int D, class IT>
-__global__ void __launch_bounds__(kInvThreads, ANTS_INV_WAVES) k_inv_step(GridDev g, PtrPack4 field, MutPack4 inv, MutPack4 res,
-                                                          SlotPack4 slots, Partial* partials, int64_t part_stride,
-                                                          int j, InvGate gate) {
+__global__ void __launch_bounds__(kInvThreads, ANTS_INV_WAVES) k_inv_step(GridDev g, PtrPack4 field, MutPack4 inv,
+                                                                          SlotPack4 slots, Partial* partials,
+                                                                          int64_t part_stride, uint64_t* masks,
+                                                                          int64_t mask_stride, int j, int speculate,
+                                                                          InvGate gate) {
   // gated launch: a no-op while the inversions it depends on still had work
   // pending after their last launched step (the host then reruns it)
   if (gate.step >= 0) {
@@ -1257,137 +1300,168 @@ __global__ void __launch_bounds__(kInvThreads, ANTS_INV_WAVES) k_inv_step(GridDe
   }
   const int k = blockIdx.y;
   RedSlot* st = slots.p[k];
-  // partials of step jj: two parities x np problems x part_stride blocks
+  // partials and masks of step jj: two parities x np problems
   auto part = [&](int jj) { return partials + ((jj & 1) * static_cast<int64_t>(gridDim.y) + k) * part_stride; };
-  bool do_update = false, do_norm = false;
-  float maxprev = 0.f, eps = 0.f;
+  auto mask = [&](int jj) { return masks + ((jj & 1) * static_cast<int64_t>(gridDim.y) + k) * mask_stride; };
+  const float* __restrict__ fld = field.p[k];
+  float* __restrict__ iv = inv.p[k];
+  const int64_t N = g.count();
+  int64_t vbeg, vend;
+  inv_range(N, vbeg, vend);
+  const int lane = threadIdx.x & 63;
+  constexpr int64_t stride = kInvThreads;
+
+  if (j < 0) {
+    // sample launch: exact |e_0| on the sampled words
+    float qmax = 0.f;
+    for (int64_t i = vbeg + threadIdx.x; i < vend; i += stride) {
+      if (!inv_sampled_word((i - lane) >> 6)) continue;  // uniform per wave
+      int x, y, z;
+      unlin(g, static_cast<IT>(i), x, y, z);
+      float cur[3], e[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) cur[d] = d < D ? iv[static_cast<IT>(i) * D + d] : 0.f;
+      qmax = fmaxf(qmax, inv_residual<D, IT>(g, fld, x, y, z, cur, e));
+    }
+    block_partial_q<kInvThreads>(0.0, 0.f, qmax, part(j) + blockIdx.x);
+    return;
+  }
+
+  bool do_fix = false, do_norm = false;
+  float maxprev = 0.f, eps_prev = 0.f, qb = 0.f;
   if (j == 0) {
     do_norm = true;
+    if (speculate) {
+      double ds;
+      float dm;
+      reduce_partials_q<kInvThreads>(part(-1), gridDim.x, ds, dm, qb);
+    }
   } else if (st[j - 1].computed) {
     // every block reduces the previous step's partials (same order, same bits)
     double psum;
-    reduce_partials<kInvThreads>(part(j - 1), gridDim.x, psum, maxprev);
+    reduce_partials_q<kInvThreads>(part(j - 1), gridDim.x, psum, maxprev, qb);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       st[j - 1].sum = psum;
       st[j - 1].maxbits = __float_as_uint(maxprev);
     }
-    do_update = true;
-    const double mean = psum / static_cast<double>(g.count());
-    eps = (j - 1 == 0) ? 0.75f : 0.5f;
+    do_fix = true;
+    const double mean = psum / static_cast<double>(N);
+    eps_prev = (j - 1 == 0) ? 0.75f : 0.5f;
     do_norm = j < kInvMaxIter && maxprev > kInvMaxTol && mean > 0.001;
   }
-  if (!do_update && !do_norm) return;
-  const float* __restrict__ fld = field.p[k];
-  float* __restrict__ iv = inv.p[k];
-  float* __restrict__ rs = res.p[k];
-  const int64_t N = g.count();
-  const float lim = eps * maxprev;
-  double nsum = 0.0;
-  float nmax = 0.f;
-#if ANTS_INV_XCD_MAP
-  // each block walks one contiguous run of voxels, the runs of one XCD adjoin
-  const int64_t nb = gridDim.x;
-  const int64_t vb = slab_block(blockIdx.x, nb);
-  const int64_t chunk = (N + nb - 1) / nb;
-  const int64_t vbeg = vb * chunk, vend = min(N, vbeg + chunk);
-  const int64_t stride = blockDim.x;
-  const int64_t N_loop = vend;
-  const int64_t i_first = vbeg + threadIdx.x;
-#else
-  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  const int64_t N_loop = N;
-  const int64_t i_first = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-#endif
-#if ANTS_INV_PREFETCH
-  // registers holding the inverse and stored residual of the next round;
-  // indices past the range are clamped into it (loads in bounds, unused)
-  float pcur[kInvUnroll][3], pres[kInvUnroll][3];
-  auto prefetch = [&](int64_t i0n) {
+  if (!do_fix && !do_norm) return;
+  const float lim = eps_prev * maxprev;      // clamp of the pending update j-1
+  const float eps = (j == 0) ? 0.75f : 0.5f;  // update j
+  const float L = eps * qb;                   // <= eps * max_j: unclamped for sure
+  // without speculation every voxel is pending: launch j recomputes and
+  // applies update j-1 everywhere, then measures e_j (no masks, no samples)
+  const uint64_t* __restrict__ mprev = do_fix && speculate ? mask(j - 1) : nullptr;
+  uint64_t* __restrict__ mcur = mask(j);
+  // the clamped update j-1 of a candidate voxel (recomputes e_{j-1})
+  auto fix = [&](int x, int y, int z, bool bnd, float cur[3]) {
+    float ep[3];
+    const float nrm = inv_residual<D, IT>(g, fld, x, y, z, cur, ep);
+    float sc = 1.f;
+    if (nrm > lim) sc = lim / nrm;
 #pragma unroll
-    for (int u = 0; u < kInvUnroll; ++u) {
-      const int64_t i = i0n + u * stride;
-      const IT ic = static_cast<IT>(i < N_loop ? i : N_loop - 1);
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        pcur[u][d] = d < D ? iv[ic * D + d] : 0.f;
-        pres[u][d] = (kInvStoreResidual && do_update && d < D) ? rs[ic * D + d] : 0.f;
-      }
-    }
+    for (int d = 0; d < D; ++d) cur[d] = bnd ? 0.f : cur[d] + (-ep[d] * sc) * eps_prev;
   };
-  if (i_first < N_loop) prefetch(i_first);
-#endif
-  for (int64_t i0 = i_first; i0 < N_loop; i0 += kInvUnroll * stride) {
-    float cur[kInvUnroll][3], e[kInvUnroll][3];
-    int xs[kInvUnroll], ys[kInvUnroll], zs[kInvUnroll];
-    bool ok[kInvUnroll];
-#if ANTS_INV_PREFETCH
-    // this round's inverse and residual were loaded during the previous
-    // round; issue the next round's loads now so they overlap the gathers
+
+  if (!do_norm && !speculate) {
+    // final launch of a plain inversion: update j-1 everywhere
+    for (int64_t i = vbeg + threadIdx.x; i < vend; i += stride) {
+      int x, y, z;
+      unlin(g, static_cast<IT>(i), x, y, z);
+      float cur[3];
 #pragma unroll
-    for (int u = 0; u < kInvUnroll; ++u) {
-      const int64_t i = i0 + u * stride;
-      ok[u] = i < N_loop;
-      const IT ic = static_cast<IT>(ok[u] ? i : N_loop - 1);
-      unlin(g, ic, xs[u], ys[u], zs[u]);
+      for (int d = 0; d < 3; ++d) cur[d] = d < D ? iv[static_cast<IT>(i) * D + d] : 0.f;
+      fix(x, y, z, on_boundary(g, x, y, z), cur);
 #pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        cur[u][d] = pcur[u][d];
-        e[u][d] = pres[u][d];
-      }
+      for (int d = 0; d < D; ++d) iv[static_cast<IT>(i) * D + d] = cur[d];
     }
-    if (i0 + kInvUnroll * stride < N_loop) prefetch(i0 + kInvUnroll * stride);
-#else
+    return;
+  }
+  if (!do_norm) {
+    // final launch of an inversion: only the pending candidates.  The
+    // wave's mask words are loaded 64 rounds at a time (lane r holds round
+    // r's word) so that the empty ones cost no load latency each.
+    const int64_t wave_first = vbeg + (threadIdx.x - lane);  // first voxel of the wave's round 0
+    for (int64_t r0 = 0; wave_first + r0 * stride < vend; r0 += 64) {
+      const int64_t my_first = wave_first + (r0 + lane) * stride;
+      const uint64_t myw = my_first < vend ? mprev[my_first >> 6] : 0ull;
+      const unsigned lo = static_cast<unsigned>(myw), hi = static_cast<unsigned>(myw >> 32);
+      for (int r = 0; r < 64 && wave_first + (r0 + r) * stride < vend; ++r) {
+        const uint64_t w = (static_cast<uint64_t>(static_cast<unsigned>(__shfl(static_cast<int>(hi), r))) << 32) |
+                           static_cast<unsigned>(__shfl(static_cast<int>(lo), r));
+        if (w == 0) continue;  // uniform per wave
+        const int64_t i = wave_first + (r0 + r) * stride + lane;
+        if (i < vend && ((w >> lane) & 1)) {
+          int x, y, z;
+          unlin(g, static_cast<IT>(i), x, y, z);
+          float cur[3];
 #pragma unroll
-    for (int u = 0; u < kInvUnroll; ++u) {
-      const int64_t i = i0 + u * stride;
-      ok[u] = i < N_loop;
-      const IT ic = static_cast<IT>(ok[u] ? i : N_loop - 1);  // loads stay in bounds; stores are skipped
-      unlin(g, ic, xs[u], ys[u], zs[u]);
+          for (int d = 0; d < 3; ++d) cur[d] = d < D ? iv[static_cast<IT>(i) * D + d] : 0.f;
+          fix(x, y, z, on_boundary(g, x, y, z), cur);
 #pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        cur[u][d] = d < D ? iv[ic * D + d] : 0.f;
-        e[u][d] = (kInvStoreResidual && do_update && d < D) ? rs[ic * D + d] : 0.f;
-      }
-    }
-#endif
-    if (do_update) {
-#pragma unroll
-      for (int u = 0; u < kInvUnroll; ++u) {
-        // e_{j-1}: stored by the previous step, or recomputed from the same
-        // inverse and field with the same arithmetic (bitwise equal)
-        const float nrm = kInvStoreResidual ? resid_norm<D>(g, e[u])
-                                            : inv_residual<D, IT>(g, fld, xs[u], ys[u], zs[u], cur[u], e[u]);
-        float sc = 1.f;
-        if (nrm > lim) sc = lim / nrm;
-        const bool bnd = on_boundary(g, xs[u], ys[u], zs[u]);
-        const IT i = static_cast<IT>(i0 + u * stride);
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-          cur[u][d] = bnd ? 0.f : cur[u][d] + (-e[u][d] * sc) * eps;
-          if (ok[u]) iv[i * D + d] = cur[u][d];
+          for (int d = 0; d < D; ++d) iv[static_cast<IT>(i) * D + d] = cur[d];
         }
       }
     }
-    if (do_norm) {
+    return;
+  }
+
+  double nsum = 0.0;
+  float nmax = 0.f, qmax = 0.f;
+  // next round's inverse and previous-step mask word (indices clamped into
+  // the range: loads stay in bounds, values past it are unused)
+  float pcur[3];
+  uint64_t pword = 0;
+  auto prefetch = [&](int64_t i) {
+    const IT ic = static_cast<IT>(i < vend ? i : vend - 1);
 #pragma unroll
-      for (int u = 0; u < kInvUnroll; ++u) {
-        const float nrm = inv_residual<D, IT>(g, fld, xs[u], ys[u], zs[u], cur[u], e[u]);
-        if (ok[u]) {
-          if (kInvStoreResidual) {
-            const IT i = static_cast<IT>(i0 + u * stride);
+    for (int d = 0; d < 3; ++d) pcur[d] = d < D ? iv[ic * D + d] : 0.f;
+    if (mprev) pword = mprev[(i - lane) >> 6];
+  };
+  const int64_t i_first = vbeg + threadIdx.x;
+  // a wave whose first voxel is in range runs; its tail lanes idle
+  if (i_first - lane < vend) prefetch(i_first);
+  for (int64_t i = i_first; i - lane < vend; i += stride) {
+    const bool ok = i < vend;
+    const IT ic = static_cast<IT>(ok ? i : vend - 1);
+    int x, y, z;
+    unlin(g, ic, x, y, z);
+    const bool bnd = on_boundary(g, x, y, z);
+    float cur[3] = {pcur[0], pcur[1], pcur[2]};
+    const uint64_t wprev = pword;
+    if (i + stride - lane < vend) prefetch(i + stride);
+    const bool fixme = do_fix && ok && (!speculate || ((wprev >> lane) & 1));
+    if (fixme) fix(x, y, z, bnd, cur);
+    float e[3];
+    const float nrm = inv_residual<D, IT>(g, fld, x, y, z, cur, e);
+    const bool spec = speculate && ok && nrm <= L;
+    float nxt[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-            for (int d = 0; d < D; ++d) rs[i * D + d] = e[u][d];
-          }
-          nsum += nrm;
-          nmax = fmaxf(nmax, nrm);
-        }
+    for (int d = 0; d < D; ++d) nxt[d] = bnd ? 0.f : cur[d] + (-e[d] * 1.f) * eps;
+    if (ok) {
+      nsum += nrm;
+      nmax = fmaxf(nmax, nrm);
+      if (spec || fixme) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) iv[ic * D + d] = spec ? nxt[d] : cur[d];
       }
     }
+    if (speculate) {
+      const uint64_t bits = __ballot(ok && !spec);
+      if (lane == 0) mcur[(i - lane) >> 6] = bits;
+    }
+    // exact |e_{j+1}| of sampled speculative voxels: the next step's bound
+    if (inv_sampled_word((i - lane) >> 6) && spec) {
+      float e2[3];
+      qmax = fmaxf(qmax, inv_residual<D, IT>(g, fld, x, y, z, nxt, e2));
+    }
   }
-  if (do_norm) {
-    block_partial<kInvThreads>(nsum, nmax, part(j) + blockIdx.x);
-    if (blockIdx.x == 0 && threadIdx.x == 0) st[j].computed = 1;
-  }
+  block_partial_q<kInvThreads>(nsum, nmax, qmax, part(j) + blockIdx.x);
+  if (blockIdx.x == 0 && threadIdx.x == 0) st[j].computed = 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -1906,27 +1980,46 @@ static unsigned int inv_blocks(int64_t n) {
 
 int64_t inv_partials_needed(const GridDev& g) { return inv_blocks(g.count()); }
 
-bool inv_stores_residual() { return kInvStoreResidual; }
+int64_t inv_mask_words(const GridDev& g) { return (g.count() + 63) / 64; }
 
-void launch_inv_step(const GridDev& g, int np, const float* const* field, float* const* inv, float* const* res,
-                     RedSlot* const* slots, Partial* partials, int j, hipStream_t s, const InvGate& gate) {
+// Speculative updates pay for the sample launch and the sampled gathers
+// only on large grids (measured per 192x224x192 pair: the 192x224x192
+// level's inversion 727 -> 555 us per iteration, the 96x112x96 level's
+// 105 -> 113 us).
+#ifndef ANTS_INV_SPEC_MIN_VOXELS
+#define ANTS_INV_SPEC_MIN_VOXELS (int64_t{1} << 21)
+#endif
+// ANTS_SYN_INV_SPEC_MIN_VOXELS overrides the threshold (read per launch, so
+// that a test can compare both schemes on one grid; results are the same
+// bits either way).
+bool inv_speculates(const GridDev& g) {
+  const char* env = std::getenv("ANTS_SYN_INV_SPEC_MIN_VOXELS");
+  const int64_t t = env ? std::atoll(env) : ANTS_INV_SPEC_MIN_VOXELS;
+  return g.count() >= t;
+}
+
+void launch_inv_step(const GridDev& g, int np, const float* const* field, float* const* inv, RedSlot* const* slots,
+                     Partial* partials, uint64_t* masks, int j, hipStream_t s, const InvGate& gate) {
   const dim3 grid(inv_blocks(g.count()), np);
   SlotPack4 sp{};
   for (int k = 0; k < np; ++k) sp.p[k] = slots[k];
   const int64_t stride = inv_partials_needed(g);
+  const int64_t mstride = inv_mask_words(g);
+  const int smp = inv_speculates(g) ? 1 : 0;
+  if (j < 0 && !smp) return;  // no sample launch without speculation
   const bool idx32 = g.count() * g.D < (int64_t{1} << 31);
   if (g.D == 2 && idx32)
-    hipLaunchKernelGGL((k_inv_step<2, int>), grid, dim3(kInvThreads), 0, s, g, pack(field, np), mpack(inv, np),
-                       mpack(res, np), sp, partials, stride, j, gate);
+    hipLaunchKernelGGL((k_inv_step<2, int>), grid, dim3(kInvThreads), 0, s, g, pack(field, np), mpack(inv, np), sp,
+                       partials, stride, masks, mstride, j, smp, gate);
   else if (g.D == 2)
     hipLaunchKernelGGL((k_inv_step<2, int64_t>), grid, dim3(kInvThreads), 0, s, g, pack(field, np), mpack(inv, np),
-                       mpack(res, np), sp, partials, stride, j, gate);
+                       sp, partials, stride, masks, mstride, j, smp, gate);
   else if (idx32)
-    hipLaunchKernelGGL((k_inv_step<3, int>), grid, dim3(kInvThreads), 0, s, g, pack(field, np), mpack(inv, np),
-                       mpack(res, np), sp, partials, stride, j, gate);
+    hipLaunchKernelGGL((k_inv_step<3, int>), grid, dim3(kInvThreads), 0, s, g, pack(field, np), mpack(inv, np), sp,
+                       partials, stride, masks, mstride, j, smp, gate);
   else
     hipLaunchKernelGGL((k_inv_step<3, int64_t>), grid, dim3(kInvThreads), 0, s, g, pack(field, np), mpack(inv, np),
-                       mpack(res, np), sp, partials, stride, j, gate);
+                       sp, partials, stride, masks, mstride, j, smp, gate);
 }
 
 void launch_resample_fields(const GridDev& gi, const GridDev& go, int np, const float* const* in, float* const* out,

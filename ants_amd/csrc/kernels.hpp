@@ -66,7 +66,7 @@ struct RedSlot {
 struct Partial {
   double s;
   float m;
-  unsigned int pad;
+  float q;  // inversion: sampled lower bound of the next step's max norm
 };
 
 // Reducing elementwise kernels run grid-stride loops over at most this many
@@ -151,15 +151,18 @@ struct InvGate {
 constexpr InvGate kNoGate{{nullptr, nullptr}, 0, -1};
 
 // One fused fixed-point inversion step j (0..20) for np problems (see
-// kernels.hip); res[k] is a field-sized buffer holding the last residual;
-// partials holds 2 x np x inv_partials_needed(g) entries (the partials of
-// step j are read by step j + 1).
+// kernels.hip): the pending updates of step j-1, the norms of step j and,
+// on grids where inv_speculates(g), its speculative update.  j = -1 is the
+// sample launch that bounds step 0's max norm from below (nothing is
+// launched without speculation); launch it before step 0.  partials holds
+// 2 x np x inv_partials_needed(g) entries and masks 2 x np x
+// inv_mask_words(g) words (step j's are read by step j + 1).
 int64_t inv_partials_needed(const GridDev& g);
-// Whether the inversion kernel keeps each norm pass's residual in `res`
-// (one field array per problem); when false, `res` may be null.
-bool inv_stores_residual();
-void launch_inv_step(const GridDev& g, int np, const float* const* field, float* const* inv, float* const* res,
-                     RedSlot* const* slots, Partial* partials, int j, hipStream_t s, const InvGate& gate = kNoGate);
+// Words of one candidate mask of the inversion (one bit per voxel).
+int64_t inv_mask_words(const GridDev& g);
+bool inv_speculates(const GridDev& g);
+void launch_inv_step(const GridDev& g, int np, const float* const* field, float* const* inv, RedSlot* const* slots,
+                     Partial* partials, uint64_t* masks, int j, hipStream_t s, const InvGate& gate = kNoGate);
 
 // Resample np fields from grid gi onto grid go (linear, 0 outside).
 void launch_resample_fields(const GridDev& gi, const GridDev& go, int np, const float* const* in,

@@ -108,9 +108,11 @@ def test_footprint_closed_form_and_errors():
     fp.restype = C.c_int64
     N = 192 * 224 * 192
     # images (2 copies x 2 images + scratch), warped images, 8 fields (4
-    # middle, 4 work; the inversion recomputes its residual), request inputs
-    # (2 images + labels) and outputs (2 fields + labels)
-    want = 2 * N * 4 * 2 + N * 4 + 2 * N * 4 + 8 * N * 3 * 4 + 3 * N * 4 + 2 * N * 3 * 4 + N * 4
+    # middle, 4 work), the inversion's candidate masks (2 parities x 2
+    # problems x one bit per voxel), request inputs (2 images + labels) and
+    # outputs (2 fields + labels)
+    want = (2 * N * 4 * 2 + N * 4 + 2 * N * 4 + 8 * N * 3 * 4 + 4 * ((N + 63) // 64) * 8 + 3 * N * 4
+            + 2 * N * 3 * 4 + N * 4)
     assert fp(C.byref(p), C.byref(d), C.byref(d)) == want
     pb = S.stage_params(stage, 3, precision="bf16")
     assert fp(C.byref(pb), C.byref(d), C.byref(d)) == want + 2 * N * 2

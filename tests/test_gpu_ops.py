@@ -134,6 +134,23 @@ def test_invert(shape, amp, sp):
     assert mn == pytest.approx(mnr, rel=1e-4, abs=1e-7)
 
 
+@pytest.mark.parametrize("shape,amp", [((20, 24, 28), 2.5), ((33, 17, 29), 4.0), ((40, 36), 2.0)])
+def test_invert_speculative_matches_plain(monkeypatch, shape, amp):
+    """Speculative updates (forced on this small grid) give the plain
+    scheme's bits, including large displacements where many voxels clamp,
+    and ragged sizes whose last mask word is partial."""
+    D = len(shape)
+    f = smooth_random_field(shape, D, 3.0, amp)
+    init = smooth_random_field(shape, D, 3.0, 0.3 * amp)
+    monkeypatch.setenv("ANTS_SYN_INV_SPEC_MIN_VOXELS", "1000000000")
+    plain = G.invert(f, init)
+    monkeypatch.setenv("ANTS_SYN_INV_SPEC_MIN_VOXELS", "0")
+    spec = G.invert(f, init)
+    assert plain[1] == spec[1] and plain[1] > 1
+    np.testing.assert_array_equal(spec[0], plain[0])
+    assert spec[2] == plain[2] and spec[3] == plain[3]
+
+
 def test_invert_zero_init_constant_shift():
     shape = (16, 18, 20)
     f = np.zeros(shape + (3,), np.float32)

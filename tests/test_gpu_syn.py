@@ -244,3 +244,32 @@ def test_trace_and_observer():
     cut = reg.trace()
     assert cut == quiet.trace[:3]
     reg.close()
+
+
+def test_speculative_inversion_is_exact(monkeypatch):
+    """The inversion's speculative updates (large grids) and the plain
+    recompute-in-the-next-launch scheme give the same bits: forced on and
+    off for the same small registration, with graphs, and with the launch
+    split that finishes every inversion from the host."""
+    pair = ph.phantom_pair_3d(6, size=(72, 80, 64))
+    stage = SyNStage.from_args("SyN[0.25,3,0]", "CC[f,m,1,3]", "[12x8,1e-6,10]", "2x1", "1x0vox")
+    outs = {}
+    for name, thresh, k in (("plain", "1000000000", None), ("spec", "0", None), ("spec_k1", "0", "1")):
+        monkeypatch.setenv("ANTS_SYN_INV_SPEC_MIN_VOXELS", thresh)
+        if k is None:
+            monkeypatch.delenv("ANTS_SYN_INV_LAUNCHES", raising=False)
+        else:
+            monkeypatch.setenv("ANTS_SYN_INV_LAUNCHES", k)
+        reg = SyNRegistration(stage, 3)
+        res = reg.register(pair.fixed, pair.moving, pair.spacing, pair.origin, pair.spacing, pair.origin)
+        outs[name] = (res, reg.stats())
+        reg.close()
+        print(f"{name}: inversion steps {outs[name][1]['inv_steps']}, tails {outs[name][1]['inv_tails']}", flush=True)
+    ref, st_ref = outs["plain"]
+    assert sum(st_ref["inv_steps"]) > 0
+    for name in ("spec", "spec_k1"):
+        res, st = outs[name]
+        np.testing.assert_array_equal(res.energy, ref.energy)
+        np.testing.assert_array_equal(res.fwd, ref.fwd)
+        np.testing.assert_array_equal(res.inv, ref.inv)
+        assert st["inv_steps"] == st_ref["inv_steps"]
