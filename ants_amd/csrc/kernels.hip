@@ -1158,14 +1158,12 @@ __global__ void __launch_bounds__(kBlock) k_compose(GridDev g, PtrPack4 disp, Pt
 }
 
 // ---------------------------------------------------------------------------
-// InvertDisplacementFieldImageFilter, one fused step.
-// Step j = 0 computes the norms n_0 of e_0 = inv_0 + field(x + inv_0).
-// Step j >= 1 runs only if n_{j-1} was computed: it recomputes e_{j-1}
-// (bitwise the same as in step j-1), applies update j-1
-//   inv_j = inv_{j-1} - eps * e_{j-1} * min(1, eps*max_{j-1}/|e_{j-1}|)
-// (eps 0.75 for the first update, 0.5 after; boundary voxels zeroed), and,
-// if j < 20 and max_{j-1} > 0.1 and mean_{j-1} > 0.001, the norms n_j of the
-// new inverse.  In-place on inv: each voxel reads and writes only itself.
+// InvertDisplacementFieldImageFilter.  Norm pass j measures
+// e_j = inv_j + field(x + inv_j) and its norms n_j; update j is
+//   inv_{j+1} = inv_j - eps * e_j * min(1, eps*max_j/|e_j|)
+// (eps 0.75 for the first update, 0.5 after; boundary voxels zeroed); the
+// loop goes on while j < 20, max_j > 0.1 and mean_j > 0.001.  In place on
+// inv: each voxel reads and writes only itself (the field is gathered).
 // The max test carries a 1e-5 relative guard: with a zero initial inverse the
 // first residual's max norm equals the learning rate, which for SyN[0.1,..]
 // sits exactly on the 0.1 tolerance; the guard resolves that tie the same way
