@@ -112,6 +112,11 @@ __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(stati
 __device__ __forceinline__ float ld(const float* p, int64_t i) { return p[i]; }
 __device__ __forceinline__ float ld(const uint16_t* p, int64_t i) { return bf2f(p[i]); }
 
+#ifndef ANTS_IMG_PAIR_LOADS
+#define ANTS_IMG_PAIR_LOADS 1
+#endif
+// two floats at 4-byte alignment: one dwordx2 global load
+typedef float float2u __attribute__((ext_vector_type(2), aligned(4)));
 // Trilinear sample of a scalar image of element type T (float or bf16) at an
 // inside point, neighbours clamped to the buffer.
 template <int D, class T>
@@ -145,6 +150,32 @@ __device__ __forceinline__ float interp_img(const GridDev& g, const T* __restric
     return a + (b - a) * w[1];
   }
   const int64_t z0 = sz * lo[2], z1 = sz * hi[2], y0 = sy * lo[1], y1 = sy * hi[1];
+#if ANTS_IMG_PAIR_LOADS
+  if (std::is_same<T, float>::value && g.n[0] >= 2) {
+    // the x neighbours are adjacent: one 8-byte load per row (dword
+    // alignment suffices for global loads).  At the last column hi == lo:
+    // the pair ending at lo is loaded instead (n[0] >= 2 keeps it in the row)
+    const bool pair = hi[0] == lo[0] + 1;
+    const int64_t x0 = pair ? lo[0] : lo[0] - 1;
+    auto row = [&](int64_t r, float& v0, float& v1) {
+      const float2u q = *reinterpret_cast<const float2u*>(reinterpret_cast<const float*>(img) + x0 + r);
+      v0 = pair ? q.x : q.y;
+      v1 = q.y;
+    };
+    float p00, q00, p10, q10, p01, q01, p11, q11;
+    row(y0 + z0, p00, q00);
+    row(y1 + z0, p10, q10);
+    row(y0 + z1, p01, q01);
+    row(y1 + z1, p11, q11);
+    const float a00 = p00 + (q00 - p00) * w[0];
+    const float a10 = p10 + (q10 - p10) * w[0];
+    const float a01 = p01 + (q01 - p01) * w[0];
+    const float a11 = p11 + (q11 - p11) * w[0];
+    const float b0 = a00 + (a10 - a00) * w[1];
+    const float b1 = a01 + (a11 - a01) * w[1];
+    return b0 + (b1 - b0) * w[2];
+  }
+#endif
   const float a00 = ld(img, lo[0] + y0 + z0) + (ld(img, hi[0] + y0 + z0) - ld(img, lo[0] + y0 + z0)) * w[0];
   const float a10 = ld(img, lo[0] + y1 + z0) + (ld(img, hi[0] + y1 + z0) - ld(img, lo[0] + y1 + z0)) * w[0];
   const float a01 = ld(img, lo[0] + y0 + z1) + (ld(img, hi[0] + y0 + z1) - ld(img, lo[0] + y0 + z1)) * w[0];
