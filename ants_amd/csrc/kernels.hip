@@ -1259,7 +1259,15 @@ constexpr int kInvBlocks = ANTS_INV_MAX_BLOCKS;
 // Sampled words of the lower bound q_j: one in 16.  Wave v of a block walks
 // words base + v + 8r (base a multiple of 8), so the rule rotates the
 // sampled rounds through the waves instead of giving them all to one.
-__device__ __forceinline__ bool inv_sampled_word(int64_t w) { return (((w >> 3) + ((w & 7) << 1)) & 15) == 0; }
+__device__ __forceinline__ bool inv_sampled_word(int64_t w, int64_t every = 16) {
+  return (((w >> 3) + ((w & 7) << 1)) & (every - 1)) == 0;
+}
+// Density of the sample launch before step 0 (a power of two): one word in
+// 64 measured cheaper than 16 (a shorter launch) and than 256 (a looser
+// bound, more candidates): inversion 20.1 / 19.9 / 20.2 ms per pair.
+#ifndef ANTS_INV_SAMPLE0_EVERY
+#define ANTS_INV_SAMPLE0_EVERY 64
+#endif
 
 template <int NT>
 __device__ __forceinline__ void reduce_partials_q(const Partial* p, int n, double& s, float& m, float& q) {
@@ -1344,7 +1352,7 @@ __global__ void __launch_bounds__(kInvThreads, ANTS_INV_WAVES) k_inv_step(GridDe
     // sample launch: exact |e_0| on the sampled words
     float qmax = 0.f;
     for (int64_t i = vbeg + threadIdx.x; i < vend; i += stride) {
-      if (!inv_sampled_word((i - lane) >> 6)) continue;  // uniform per wave
+      if (!inv_sampled_word((i - lane) >> 6, ANTS_INV_SAMPLE0_EVERY)) continue;  // uniform per wave
       int x, y, z;
       unlin(g, static_cast<IT>(i), x, y, z);
       float cur[3], e[3];
