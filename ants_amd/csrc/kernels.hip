@@ -841,12 +841,12 @@ struct CCOut {
   double cc;
   float a1, a2;  // g_f2m = a1 * grad m ; g_m2m = a2 * grad f
 };
-__device__ __forceinline__ CCOut cc_voxel(double cnt, double sf, double sm, double sff, double smm, double sfm,
-                                          float f, float m, float eps) {
+// rc = 1 / (window count)
+__device__ __forceinline__ CCOut cc_voxel_rc(double rc, double sf, double sm, double sff, double smm, double sfm,
+                                             float f, float m, float eps) {
   // In double the algebraically equal short forms are as accurate as ITK's
   // sff - mf*sf - mf*sf + cnt*mf*mf, and one reciprocal replaces the divisions
   // by sFF*sMM, sMM and sFF (1/sMM = sFF/p, 1/sFF = sMM/p).
-  const double rc = 1.0 / cnt;
   const double mf = sf * rc, mm = sm * rc;
   const double sFF = sff - sf * mf;
   const double sMM = smm - sm * mm;
@@ -865,6 +865,11 @@ __device__ __forceinline__ CCOut cc_voxel(double cnt, double sf, double sm, doub
     o.a2 = static_cast<float>(a * (mA - q * sMM * fA));
   }
   return o;
+}
+
+__device__ __forceinline__ CCOut cc_voxel(double cnt, double sf, double sm, double sff, double smm, double sfm,
+                                          float f, float m, float eps) {
+  return cc_voxel_rc(1.0 / cnt, sf, sm, sff, smm, sfm, f, m, eps);
 }
 
 // Direct CC (any D): each thread sums its (2r+1)^D window from global memory.
@@ -959,6 +964,7 @@ __global__ void __launch_bounds__(TX* TY, kCCWavesPerEU) k_cc3d(GridDev g, const
   const bool active = x < nx && y < ny;
   const double cxy = static_cast<double>((min(x + R, nx - 1) - max(x - R, 0) + 1) *
                                          (min(y + R, ny - 1) - max(y - R, 0) + 1));
+  const double rc_in = 1.0 / (cxy * static_cast<double>(W));
 
   // z rings: double plane sums of the last W planes, and the raw centre
   // values and in-plane gradients of the planes still needed.  With
@@ -1129,7 +1135,14 @@ __global__ void __launch_bounds__(TX* TY, kCCWavesPerEU) k_cc3d(GridDev g, const
     if (active && zc >= z0 && zc < z1) {
       const double cz = static_cast<double>(min(zc + R, nz - 1) - max(zc - R, 0) + 1);
       const float f = rf[c0], m = rm[c0];
-      const CCOut o = cc_voxel(cxy * cz, zsum[0], zsum[1], zsum[2], zsum[3], zsum[4], f, m, eps);
+      // 1 / count: the thread's interior value away from the z faces (the
+      // branch is uniform over the block's plane)
+      double rc;
+      if (cz == static_cast<double>(W))
+        rc = rc_in;
+      else
+        rc = 1.0 / (cxy * cz);
+      const CCOut o = cc_voxel_rc(rc, zsum[0], zsum[1], zsum[2], zsum[3], zsum[4], f, m, eps);
       cc_sum += o.cc;
       float gmz = 0.f, gfz = 0.f;
       if (o.a1 != 0.f || o.a2 != 0.f) {
