@@ -559,6 +559,18 @@ int smooth_image_on(hipStream_t s, const HostGrid& g, const float* in, float* ou
     if (out != in) HIP_TRY(hipMemcpyAsync(out, in, n * sizeof(float), hipMemcpyDeviceToDevice, s));
     return ANTS_OK;
   }
+  // one fused 3-D pass when every axis has the same taps (voxel sigmas or
+  // isotropic spacing)
+  if (g.D == 3 && (!in_mm || (g.sp[0] == g.sp[1] && g.sp[1] == g.sp[2]))) {
+    double var = sigma * sigma;
+    if (in_mm) var /= g.sp[0] * g.sp[0];
+    Taps t;
+    ANTS_TRY(gaussian_taps(var, 0.01, 32, &t));
+    if (launch_smooth3d_image(gd, in, out, t, s)) {
+      HIP_TRY(hipGetLastError());
+      return ANTS_OK;
+    }
+  }
   // ping-pong so that the last pass lands in `out`
   const float* src = in;
   for (int d = 0; d < g.D; ++d) {

@@ -566,17 +566,20 @@ constexpr int kSmoothTX = ANTS_SMOOTH_TX, kSmoothTY = ANTS_SMOOTH_TY;
 #define ANTS_SMOOTH_WAVES 4
 #endif
 constexpr int kSmoothWavesPerEU = ANTS_SMOOTH_WAVES;
-template <int R, int TX, int TY>
+// NC interleaved components (3: a displacement field, 1: an image); with
+// maxbits null no norm is taken.
+template <int R, int TX, int TY, int NC = 3>
 __global__ void __launch_bounds__(TX* TY, kSmoothWavesPerEU) k_smooth3d(GridDev g, PtrPack4 in, PtrPack4 orig, MutPack4 out, Taps t,
                                                      int zchunk, int nzc, float w1, float w2, int boundary,
                                                      unsigned int* maxbits) {
   constexpr int W = 2 * R + 1;
   constexpr int LX = TX + 2 * R, LY = TY + 2 * R;
+  constexpr int LXP = (LX + 3) / 4 * 4;  // row pitch: whole float4 chunks
   constexpr int NT = TX * TY;
-  constexpr int NE = 3 * LX * LY;
+  constexpr int NE = NC * LX * LY;
   constexpr int KP = (NE + NT - 1) / NT;
-  __shared__ __align__(16) float tile[3][LY][LX];
-  __shared__ __align__(16) float xs[3][LY][TX];
+  __shared__ __align__(16) float tile[NC][LY][LXP];
+  __shared__ __align__(16) float xs[NC][LY][TX];
   const int f = blockIdx.z / nzc, zb = blockIdx.z - f * nzc;
   const float* __restrict__ src = in.p[f];
   const float* __restrict__ org = orig.p[f];
@@ -591,9 +594,9 @@ __global__ void __launch_bounds__(TX* TY, kSmoothWavesPerEU) k_smooth3d(GridDev 
   float c[R + 1];
 #pragma unroll
   for (int b = 0; b <= R; ++b) c[b] = b <= t.r ? t.c[b] : 0.f;
-  float ring[3][W];
+  float ring[NC][W];
 #pragma unroll
-  for (int a = 0; a < 3; ++a)
+  for (int a = 0; a < NC; ++a)
 #pragma unroll
     for (int b = 0; b < W; ++b) ring[a][b] = 0.f;
   // planes are fetched kSmoothPrefetch ahead into registers: slot
@@ -611,19 +614,19 @@ __global__ void __launch_bounds__(TX* TY, kSmoothWavesPerEU) k_smooth3d(GridDev 
     goff[u] = -1;
     loff[u] = 0;
     if (e < NE) {
-      const int ly = e / (3 * LX), r = e - ly * (3 * LX);
-      const int lx = r / 3, cc = r - lx * 3;
+      const int ly = e / (NC * LX), r = e - ly * (NC * LX);
+      const int lx = r / NC, cc = r - lx * NC;
       int gx = x0 - R + lx, gy = y0 - R + ly;
       gx = gx < 0 ? 0 : (gx >= nx ? nx - 1 : gx);
       gy = gy < 0 ? 0 : (gy >= ny ? ny - 1 : gy);
-      goff[u] = (gy * nx + gx) * 3 + cc;
-      loff[u] = (cc * LY + ly) * LX + lx;
+      goff[u] = (gy * nx + gx) * NC + cc;
+      loff[u] = (cc * LY + ly) * LXP + lx;
     }
   }
   float* tile_flat = &tile[0][0][0];
   auto fetch = [&](float* dstv, int zq) {
     const int zs = zq < 0 ? 0 : (zq >= nz ? nz - 1 : zq);
-    const float* __restrict__ srcz = src + static_cast<int64_t>(zs) * plane * 3;
+    const float* __restrict__ srcz = src + static_cast<int64_t>(zs) * plane * NC;
 #pragma unroll
     for (int u = 0; u < KP; ++u) dstv[u] = goff[u] >= 0 ? srcz[goff[u]] : 0.f;
   };
@@ -645,17 +648,19 @@ __global__ void __launch_bounds__(TX* TY, kSmoothWavesPerEU) k_smooth3d(GridDev 
     __syncthreads();
     if (zz + P < zend) fetch(pv[qs], zz + P);
     // x pass, four outputs per task from aligned 16-byte LDS reads: task
-    // (ly, q) reads tile[a][ly][4q .. 4q+15] and writes xs[a][ly][4q .. 4q+3]
-    static_assert(TX % 4 == 0 && LX % 4 == 0 && TX + 12 <= LX && R <= 6, "x tiling: reads reach 4q+15 < LX");
+    // (ly, q) reads tile[a][ly][4q .. 4q+15] and writes xs[a][ly][4q .. 4q+3];
+    // the taps of output e use w[e .. e + 2R], all inside the staged row
+    // (e + 2R <= TX - 1 + 2R < LX), so unstaged pitch padding is never used
+    static_assert(TX % 4 == 0 && R >= 1 && R <= 6, "x tiling: one output quad reads 16 values");
     for (int task = threadIdx.x; task < LY * (TX / 4); task += NT) {
       const int ly = task / (TX / 4), q = task - ly * (TX / 4);
 #pragma unroll
-      for (int a = 0; a < 3; ++a) {
+      for (int a = 0; a < NC; ++a) {
         float w[16];
 #pragma unroll
         for (int h = 0; h < 4; ++h) {
-          const float4 v4 = (4 * q + 4 * h < LX) ? *reinterpret_cast<const float4*>(&tile[a][ly][4 * q + 4 * h])
-                                                  : make_float4(0.f, 0.f, 0.f, 0.f);
+          const float4 v4 = (4 * q + 4 * h < LXP) ? *reinterpret_cast<const float4*>(&tile[a][ly][4 * q + 4 * h])
+                                                   : make_float4(0.f, 0.f, 0.f, 0.f);
           w[4 * h + 0] = v4.x;
           w[4 * h + 1] = v4.y;
           w[4 * h + 2] = v4.z;
@@ -673,9 +678,9 @@ __global__ void __launch_bounds__(TX* TY, kSmoothWavesPerEU) k_smooth3d(GridDev 
       }
     }
     __syncthreads();
-    float p[3];
+    float p[NC];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
+    for (int a = 0; a < NC; ++a) {
       float acc = c[0] * xs[a][ty + R][tx];
 #pragma unroll
       for (int b = 1; b <= R; ++b) acc += c[b] * (xs[a][ty + R - b][tx] + xs[a][ty + R + b][tx]);
@@ -684,7 +689,7 @@ __global__ void __launch_bounds__(TX* TY, kSmoothWavesPerEU) k_smooth3d(GridDev 
     // ring slot of plane zz - R + k, k = -R .. R (k = R is plane zz)
     auto slot = [](int k) { return kSmoothRotate ? (ph + W + k - R) % W : R + k; };
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
+    for (int a = 0; a < NC; ++a) {
       if constexpr (kSmoothRotate) {
         ring[a][ph] = p[a];
       } else {
@@ -699,12 +704,12 @@ __global__ void __launch_bounds__(TX* TY, kSmoothWavesPerEU) k_smooth3d(GridDev 
       const bool bnd = boundary && (x == 0 || x == nx - 1 || y == 0 || y == ny - 1 || zc == 0 || zc == nz - 1);
       float s2 = 0.f;
 #pragma unroll
-      for (int a = 0; a < 3; ++a) {
+      for (int a = 0; a < NC; ++a) {
         float acc = c[0] * ring[a][slot(0)];
 #pragma unroll
         for (int b = 1; b <= R; ++b) acc += c[b] * (ring[a][slot(-b)] + ring[a][slot(b)]);
-        const float v = bnd ? 0.f : (w2 != 0.f ? acc * w1 + org[i * 3 + a] * w2 : acc * w1);
-        dst[i * 3 + a] = v;
+        const float v = bnd ? 0.f : (w2 != 0.f ? acc * w1 + org[i * NC + a] * w2 : acc * w1);
+        dst[i * NC + a] = v;
 #if ANTS_SMOOTH_NORM_MUL
         const float q = v * g.isp[a];
 #else
@@ -729,9 +734,11 @@ __global__ void __launch_bounds__(TX* TY, kSmoothWavesPerEU) k_smooth3d(GridDev 
       }
     }
   }
-  double dummy = 0.0;
-  block_sum_max<NT>(dummy, nrm);
-  if (threadIdx.x == 0) atomicMax(&maxbits[f], __float_as_uint(nrm));
+  if (maxbits) {
+    double dummy = 0.0;
+    block_sum_max<NT>(dummy, nrm);
+    if (threadIdx.x == 0) atomicMax(&maxbits[f], __float_as_uint(nrm));
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1912,6 +1919,34 @@ bool launch_smooth3d(const GridDev& g, int np, const float* const* in, const flo
   hipLaunchKernelGGL((k_smooth3d<R, TX, TY>), grid, dim3(TX * TY), 0, s, g, pack(in, np), pack(orig, np),
                      mpack(out, np), t, static_cast<int>(zc), nzc, w1, w2, static_cast<int>(boundary), maxbits);
   return true;
+}
+
+template <int R>
+static void smooth3d_image_r(const GridDev& g, const float* in, float* out, const Taps& t, hipStream_t s) {
+  constexpr int TX = kSmoothTX, TY = kSmoothTY;
+  const int64_t tiles = static_cast<int64_t>((g.n[0] + TX - 1) / TX) * ((g.n[1] + TY - 1) / TY);
+  static const int64_t slots = device_slots(reinterpret_cast<const void*>(&k_smooth3d<R, TX, TY, 1>), TX * TY);
+  const int64_t zc = zchunk_for(g.n[2], tiles, 2 * R, slots);
+  const int nzc = static_cast<int>((g.n[2] + zc - 1) / zc);
+  const dim3 grid((g.n[0] + TX - 1) / TX, (g.n[1] + TY - 1) / TY, nzc);
+  const float* ins[1] = {in};
+  float* outs[1] = {out};
+  hipLaunchKernelGGL((k_smooth3d<R, TX, TY, 1>), grid, dim3(TX * TY), 0, s, g, pack(ins, 1), pack(ins, 1),
+                     mpack(outs, 1), t, static_cast<int>(zc), nzc, 1.f, 0.f, 0, nullptr);
+}
+
+bool launch_smooth3d_image(const GridDev& g, const float* in, float* out, const Taps& t, hipStream_t s) {
+  if (g.D != 3 || in == out) return false;
+  if (static_cast<int64_t>(g.n[0]) * g.n[1] >= (int64_t{1} << 31)) return false;  // 32-bit in-plane offsets
+  switch (t.r) {
+    case 1: smooth3d_image_r<1>(g, in, out, t, s); return true;
+    case 2: smooth3d_image_r<2>(g, in, out, t, s); return true;
+    case 3: smooth3d_image_r<3>(g, in, out, t, s); return true;
+    case 4: smooth3d_image_r<4>(g, in, out, t, s); return true;
+    case 5: smooth3d_image_r<5>(g, in, out, t, s); return true;
+    case 6: smooth3d_image_r<6>(g, in, out, t, s); return true;
+    default: return false;
+  }
 }
 
 void launch_warp_pair(const GridDev& gl, const GridDev& gF, const float* Fs, const float* uF, float* Fw,

@@ -106,6 +106,9 @@ void launch_gauss_field_final(const GridDev& g, int np, const float* const* in, 
 // radius is not covered (then use launch_gauss_axis + launch_gauss_field_final).
 bool launch_smooth3d(const GridDev& g, int np, const float* const* in, const float* const* orig, float* const* out,
                      const Taps& t, float w1, float w2, bool boundary, unsigned int* maxbits, hipStream_t s);
+// The same fused 3-D pass for one scalar image with the same taps on every
+// axis (edges replicated), radius 1..6, in != out; false when not applicable.
+bool launch_smooth3d_image(const GridDev& g, const float* in, float* out, const Taps& t, hipStream_t s);
 
 // Fw = Fs(x + uF(x)), Mw = Ms(x + uM(x)) on level grid gl (Fs on gF, Ms on gM).
 void launch_warp_pair(const GridDev& gl, const GridDev& gF, const float* Fs, const float* uF, float* Fw,

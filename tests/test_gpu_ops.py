@@ -28,7 +28,13 @@ def smooth_random_field(shape, D, sigma=2.0, amp=2.0):
 
 @pytest.mark.parametrize("shape,sigma,in_mm,sp", [((33, 40), 2.0, False, None), ((17, 23, 29), 1.0, False, None),
                                                  ((16, 20, 12), 2.0, True, [1.0, 0.7, 1.3]),
-                                                 ((5, 6, 7), 0.0, False, None)])
+                                                 ((5, 6, 7), 0.0, False, None),
+                                                 # one fused 3-D pass (same taps on every axis), radii 1..6,
+                                                 # ragged tiles; radius > 6 falls back to separable passes
+                                                 ((40, 37, 35), 2.0, False, None), ((9, 70, 11), 1.5, False, None),
+                                                 ((20, 22, 24), 1.0, True, [1.2, 1.2, 1.2]),
+                                                 ((35, 33, 50), 0.6, False, None), ((3, 4, 40), 2.5, False, None),
+                                                 ((30, 30, 30), 3.0, False, None)])
 def test_smooth_image(shape, sigma, in_mm, sp):
     img = f32(rng.standard_normal(shape))
     out = G.smooth_image(img, sigma, in_mm, sp)
