@@ -1250,7 +1250,7 @@ __device__ __forceinline__ float inv_residual(const GridDev& g, const float* __r
 // eps_j * max_j, a global maximum known only after the pass.  Launch j
 // therefore writes inv_{j+1} = inv_j - eps_j * e_j in place for every voxel
 // whose |e_j| is at most L_j = eps_j * q_j, where q_j <= max_j is the
-// largest of a sample of exact |e_j| (one word in 16; computed by the
+// largest of a sample of exact |e_j| (one word in 64; computed by the
 // previous launch from its own speculative values, or by the sample launch
 // j = -1 before the first step).  Such voxels are unclamped for certain.
 // The others ("candidates") keep inv_j and set their bit in the mask of
@@ -1276,7 +1276,7 @@ constexpr int kInvBlocks = ANTS_INV_MAX_BLOCKS;
 #ifndef ANTS_INV_WAVES
 #define ANTS_INV_WAVES 1
 #endif
-// Sampled words of the lower bound q_j: one in 16.  Wave v of a block walks
+// Sampled words of the lower bound q_j (one in `every`).  Wave v of a block walks
 // words base + v + 8r (base a multiple of 8), so the rule rotates the
 // sampled rounds through the waves instead of giving them all to one.
 __device__ __forceinline__ bool inv_sampled_word(int64_t w, int64_t every = 16) {
@@ -1285,6 +1285,13 @@ __device__ __forceinline__ bool inv_sampled_word(int64_t w, int64_t every = 16) 
 // Density of the sample launch before step 0 (a power of two): one word in
 // 64 measured cheaper than 16 (a shorter launch) and than 256 (a looser
 // bound, more candidates): inversion 20.1 / 19.9 / 20.2 ms per pair.
+// In-launch sample density for the next step's bound: 1/8, 1/16, 1/32, 1/64
+// and 1/128 of the words measured 19.9, 19.6, 19.5, 19.5 and 19.45 ms of
+// inversion per 192x224x192 pair (sparser: cheaper sampling; the bound
+// stays close); 1/64 keeps some margin for broader residual spreads.
+#ifndef ANTS_INV_SAMPLE_EVERY
+#define ANTS_INV_SAMPLE_EVERY 64
+#endif
 #ifndef ANTS_INV_SAMPLE0_EVERY
 #define ANTS_INV_SAMPLE0_EVERY 64
 #endif
@@ -1512,7 +1519,7 @@ __global__ void __launch_bounds__(kInvThreads, ANTS_INV_WAVES) k_inv_step(GridDe
       if (lane == 0) mcur[(i - lane) >> 6] = bits;
     }
     // exact |e_{j+1}| of sampled speculative voxels: the next step's bound
-    if (inv_sampled_word((i - lane) >> 6) && spec) {
+    if (inv_sampled_word((i - lane) >> 6, ANTS_INV_SAMPLE_EVERY) && spec) {
       float e2[3];
       qmax = fmaxf(qmax, inv_residual<D, IT>(g, fld, x, y, z, nxt, e2));
     }
