@@ -1264,11 +1264,10 @@ __device__ __forceinline__ float inv_residual(const GridDev& g, const float* __r
 #endif
 constexpr int kInvThreads = ANTS_INV_THREADS;
 static_assert(kInvThreads % 64 == 0, "waves cover whole mask words");
-// Grid per problem: one block per kInvThreads * ANTS_INV_VPT voxels, at most
-// ANTS_INV_MAX_BLOCKS.  Every block of a step first reduces the previous
-// step's per-block partials, so fewer, fuller blocks shorten that prologue:
-// measured per 192x224x192 pair, inversion 24.5 ms (1 voxel per thread, 1024
-// blocks) -> 23.9 (4 voxels) -> 23.7 ms (4 voxels, at most 768 blocks).
+// Grid per problem: one block per kInvThreads * ANTS_INV_VPT voxels, at
+// least 256 (while there are voxels) and at most ANTS_INV_MAX_BLOCKS (see
+// inv_blocks).  Every block of a step first reduces the previous step's
+// per-block partials, so fewer, fuller blocks shorten that prologue.
 #ifndef ANTS_INV_MAX_BLOCKS
 #define ANTS_INV_MAX_BLOCKS 768
 #endif
@@ -1279,7 +1278,7 @@ constexpr int kInvBlocks = ANTS_INV_MAX_BLOCKS;
 // Sampled words of the lower bound q_j (one in `every`).  Wave v of a block walks
 // words base + v + 8r (base a multiple of 8), so the rule rotates the
 // sampled rounds through the waves instead of giving them all to one.
-__device__ __forceinline__ bool inv_sampled_word(int64_t w, int64_t every = 16) {
+__device__ __forceinline__ bool inv_sampled_word(int64_t w, int64_t every) {
   return (((w >> 3) + ((w & 7) << 1)) & (every - 1)) == 0;
 }
 // Density of the sample launch before step 0 (a power of two): one word in
@@ -2058,9 +2057,11 @@ void launch_compose(const GridDev& g, int np, const float* const* disp, const fl
                        maxbits, rate);
 }
 
-// 8 voxels per thread: at 96x112x96 the 2 x 252 blocks fit the 768
-// resident blocks (79 VGPRs: 3 blocks of 512 threads per CU) in one round
-// (inversion 19.9 -> 19.7 ms per pair against 4 voxels per thread)
+// Measured per 192x224x192 pair: inversion 24.5 ms with one voxel per thread
+// and 1024 blocks, 23.9 with 4 voxels, 23.7 with at most 768 blocks (the
+// plain scheme); with speculation, 8 voxels per thread: at 96x112x96 the
+// 2 x 252 blocks fit the 768 resident blocks (79 VGPRs: 3 blocks of 512
+// threads per CU) in one round (19.9 -> 19.7 ms against 4 voxels).
 #ifndef ANTS_INV_VPT
 #define ANTS_INV_VPT 8
 #endif
