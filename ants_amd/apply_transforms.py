@@ -105,7 +105,40 @@ def apply_transforms(reference: Geometry, image, image_geometry: Geometry, trans
     if len(transforms) > 8:
         raise ValueError("at most 8 transforms")
     d_img = image if isinstance(image, L.DeviceArray) else L.DeviceArray.from_host(np.asarray(image, np.float32))
-    keep = [d_img]
+    xs, keep = transform_array(transforms, D)
+    keep.append(d_img)
+    d_out = out if out is not None else L.DeviceArray(reference.shape)
+    L.check(L.lib().ants_apply_transforms(C.byref(reference.desc()), C.byref(image_geometry.desc()), d_img.ptr, xs,
+                                          len(transforms), parse_interpolation(interpolation), float(default_value),
+                                          d_out.ptr, None))
+    return d_out if out is not None else d_out.to_host()
+
+
+def compose_affines(transforms: Sequence[Optional[AffineTransform]]) -> AffineTransform:
+    """One AffineTransform (centre 0) equal to the list applied in order: a
+    point passes through transforms[0] first.  None entries are identities."""
+    A = None
+    off = None
+    for t in transforms:
+        if t is None:
+            continue
+        B, tr, c = t.parameters()
+        ob = tr + c - B @ c
+        if A is None:
+            A, off = B.copy(), ob.copy()
+        else:
+            A, off = B @ A, B @ off + ob
+    if A is None:
+        raise ValueError("no affine transform to compose")
+    return AffineTransform(A, off, np.zeros(A.shape[0]))
+
+
+def transform_array(transforms: Sequence[Transform], D: int):
+    """ants_transform array for the C-ABI, plus the device buffers it refers
+    to (keep them alive while the array is in use)."""
+    if len(transforms) > 8:
+        raise ValueError("at most 8 transforms")
+    keep = []
     xs = (L.TransformC * max(len(transforms), 1))()
     for i, t in enumerate(transforms):
         if t is None:
@@ -130,8 +163,4 @@ def apply_transforms(reference: Geometry, image, image_geometry: Geometry, trans
             xs[i].d_field = f.ptr
         else:
             raise TypeError(f"unsupported transform {type(t).__name__}")
-    d_out = out if out is not None else L.DeviceArray(reference.shape)
-    L.check(L.lib().ants_apply_transforms(C.byref(reference.desc()), C.byref(image_geometry.desc()), d_img.ptr, xs,
-                                          len(transforms), parse_interpolation(interpolation), float(default_value),
-                                          d_out.ptr, None))
-    return d_out if out is not None else d_out.to_host()
+    return xs, keep

@@ -7,6 +7,10 @@ runs one SyN stage with the CC metric (the antsRegistration subset this
 library replaces) and writes ``out_0Warp.nii.gz`` and
 ``out_0InverseWarp.nii.gz`` (naming of Examples/
 antsRegistrationTemplateHeader.cxx:74-89) plus the optional warped images.
+``-r affine.mat`` (repeatable, ``[file,1]`` inverts) gives the stage its
+moving initial transform, as the preceding linear stages do; the outputs are
+then ``out_0GenericAffine.mat`` (the initial transforms composed),
+``out_1Warp.nii.gz`` and ``out_1InverseWarp.nii.gz``.
 
 ``python -m ants_amd.cli apply -d 3 -i moving.nii.gz -r fixed.nii.gz
   -o out.nii.gz -t out_0Warp.nii.gz -t [affine.mat,1] -n Linear
@@ -25,7 +29,7 @@ from typing import List, Optional, Sequence
 import numpy as np
 
 from . import io as aio
-from .apply_transforms import AffineTransform, DisplacementFieldTransform, Geometry, apply_transforms
+from .apply_transforms import AffineTransform, DisplacementFieldTransform, Geometry, apply_transforms, compose_affines
 
 
 def _bracket_list(s: str) -> List[str]:
@@ -84,6 +88,8 @@ def cmd_registration(argv: Sequence[str]) -> int:
     ap.add_argument("-f", "--shrink-factors", required=True)
     ap.add_argument("-s", "--smoothing-sigmas", required=True)
     ap.add_argument("-o", "--output", required=True)
+    ap.add_argument("-r", "--initial-moving-transform", action="append", default=[],
+                    help="affine transform file or [file,useInverse]; repeated options compose in -t order")
     ap.add_argument("--float", default="1")
     ap.add_argument("--precision", choices=["fp32", "bf16"], default="fp32")
     ap.add_argument("-v", "--verbose", type=int, default=0)
@@ -97,7 +103,15 @@ def cmd_registration(argv: Sequence[str]) -> int:
             raise SystemExit(f"{name} is {g.dim}-D, -d {a.dimensionality}")
     outs = _bracket_list(a.output)
     prefix = outs[0]
+    init = []
+    for spec in a.initial_moving_transform:
+        t = load_transform(spec)
+        if isinstance(t, DisplacementFieldTransform):
+            raise SystemExit(f"{spec}: only linear initial transforms are supported")
+        if t is not None:
+            init.append(t)
     reg = SyNRegistration(stage, a.dimensionality, precision=a.precision)
+    reg.set_initial_transforms(init)
     res = reg.register(np.asarray(fixed, np.float32), np.asarray(moving, np.float32), gf.spacing, gf.origin,
                        gm.spacing, gm.origin, gf.direction, gm.direction)
     lg = reg.level_geometry(gf.size, gf.spacing, gf.origin, direction=gf.direction)  # the fields' grid
@@ -108,15 +122,24 @@ def cmd_registration(argv: Sequence[str]) -> int:
     D = gf.dim
     g_out = Geometry(tuple(int(lg.size[k]) for k in range(D)), tuple(lg.spacing[k] for k in range(D)),
                      tuple(lg.origin[k] for k in range(D)), np.array([lg.direction[k] for k in range(D * D)]).reshape(D, D))
-    aio.write_nifti(prefix + "0Warp.nii.gz", res.fwd, g_out, vector=True)
-    aio.write_nifti(prefix + "0InverseWarp.nii.gz", res.inv, g_out, vector=True)
+    # with initial transforms the output composite is [initial affine, SyN field]:
+    # <prefix>0GenericAffine.mat, <prefix>1Warp.nii.gz, <prefix>1InverseWarp.nii.gz
+    k = 0
+    fwd_chain = [DisplacementFieldTransform(res.fwd, g_out)]
+    inv_chain = [DisplacementFieldTransform(res.inv, g_out)]
+    if init:
+        aff = compose_affines(init)
+        aio.write_transform(prefix + "0GenericAffine.mat", aff)
+        k = 1
+        fwd_chain = fwd_chain + [aff]
+        inv_chain = [AffineTransform(aff.matrix, aff.translation, aff.center, invert=True)] + inv_chain
+    aio.write_nifti(prefix + f"{k}Warp.nii.gz", res.fwd, g_out, vector=True)
+    aio.write_nifti(prefix + f"{k}InverseWarp.nii.gz", res.inv, g_out, vector=True)
     if len(outs) > 1 and outs[1]:
-        warped = apply_transforms(gf, np.asarray(moving, np.float32), gm,
-                                  [DisplacementFieldTransform(res.fwd, g_out)], "Linear")
+        warped = apply_transforms(gf, np.asarray(moving, np.float32), gm, fwd_chain, "Linear")
         aio.write_nifti(outs[1], warped, gf)
     if len(outs) > 2 and outs[2]:
-        inv_warped = apply_transforms(gm, np.asarray(fixed, np.float32), gf,
-                                      [DisplacementFieldTransform(res.inv, g_out)], "Linear")
+        inv_warped = apply_transforms(gm, np.asarray(fixed, np.float32), gf, inv_chain, "Linear")
         aio.write_nifti(outs[2], inv_warped, gm)
     return 0
 

@@ -498,6 +498,11 @@ struct ants_syn_handle {
   int inv_launches = kInvLaunchesDefault;  // starting value per level
   bool inv_launches_fixed = false;          // set by ANTS_SYN_INV_LAUNCHES: no adaptation
   std::vector<const void*> graph_signature;  // buffers + geometry the cached graphs were built for
+  // initial transforms (fixed side, moving side) composed into y = A p + o;
+  // on[k] false: identity
+  bool init_on[2] = {false, false};
+  double init_A[2][9] = {};
+  double init_o[2][3] = {};
 
   void clear_graphs() {
     for (auto& kv : graphs)
@@ -524,6 +529,40 @@ struct ants_syn_handle {
 };
 
 namespace {
+
+// WarpMap of (level grid gl) -> [x + u] -> [y = A p + o] -> continuous index
+// in image grid gi, formed in double: with K = diag(1/sp_i) dir_i^T,
+// G = K A dir_l diag(sp_l), Q = K A, h = K (A org_l + o - org_i).
+WarpMap make_warp_map(const HostGrid& gl, const double* A, const double* o, const HostGrid& gi) {
+  double K[9], KA[9];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) K[r * 3 + c] = gi.dir[c * 3 + r] / gi.sp[r];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      double t = 0.0;
+      for (int k = 0; k < 3; ++k) t += K[r * 3 + k] * A[k * 3 + c];
+      KA[r * 3 + c] = t;
+    }
+  WarpMap m{};
+  double p0[3];
+  for (int r = 0; r < 3; ++r) {
+    double t = o[r] - gi.org[r];
+    for (int k = 0; k < 3; ++k) t += A[r * 3 + k] * gl.org[k];
+    p0[r] = t;
+  }
+  for (int r = 0; r < 3; ++r) {
+    double h = 0.0;
+    for (int k = 0; k < 3; ++k) h += K[r * 3 + k] * p0[k];
+    m.h[r] = static_cast<float>(h);
+    for (int c = 0; c < 3; ++c) {
+      m.Q[r * 3 + c] = static_cast<float>(KA[r * 3 + c]);
+      double g = 0.0;
+      for (int k = 0; k < 3; ++k) g += KA[r * 3 + k] * gl.dir[k * 3 + c];
+      m.G[r * 3 + c] = static_cast<float>(g * gl.sp[c]);
+    }
+  }
+  return m;
+}
 
 // Device-side PreprocessImage: min/max reduction, host computes the window
 // exactly as the CPU path does, then one elementwise pass.
@@ -732,6 +771,14 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
     setup_hash = fnv1a(P.sigma, sizeof(P.sigma), setup_hash);
     const double dbl[] = {P.learning_rate, P.update_variance, P.total_variance};
     setup_hash = fnv1a(dbl, sizeof(dbl), setup_hash);
+    for (int k = 0; k < 2; ++k) {
+      const int on = init_on[k] ? 1 : 0;
+      setup_hash = fnv1a(&on, sizeof(on), setup_hash);
+      if (on) {
+        setup_hash = fnv1a(init_A[k], sizeof(init_A[k]), setup_hash);
+        setup_hash = fnv1a(init_o[k], sizeof(init_o[k]), setup_hash);
+      }
+    }
     // Profiled runs launch eagerly: events recorded by the event nodes of a
     // replayed graph do not yield elapsed times on this runtime, and graphs
     // cut into segments at the timed spans (events between the segment
@@ -769,10 +816,17 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
   const GridDev gFd = to_dev(gF), gMd = to_dev(gM);
   const float eps = std::numeric_limits<float>::epsilon();
   HostGrid prev{};
+  const bool use_maps = init_on[0] || init_on[1];
   for (int l = 0; l < P.n_levels; ++l) {
     const HostGrid& g = lg[l];
     const GridDev gd = to_dev(g);
     const int64_t N = g.count();
+    WarpMap maps[2];
+    if (use_maps) {
+      static const double I[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, Z[3] = {0, 0, 0};
+      maps[0] = make_warp_map(g, init_on[0] ? init_A[0] : I, init_on[0] ? init_o[0] : Z, gF);
+      maps[1] = make_warp_map(g, init_on[1] ? init_A[1] : I, init_on[1] ? init_o[1] : Z, gM);
+    }
     if (l == 0) {
       for (int k = 0; k < 4; ++k) HIP_TRY(hipMemsetAsync(st[k], 0, N * D * sizeof(float), s));
     } else {
@@ -876,9 +930,11 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
       prof.begin(ANTS_KC_WARP);
       if (bf16)
         launch_warp_pair_bf16(gd, gFd, static_cast<const uint16_t*>(Fb.p), st_[1], Fw.f(), gMd,
-                              static_cast<const uint16_t*>(Mb.p), st_[3], Mw.f(), s, stat_words, kStatWords);
+                              static_cast<const uint16_t*>(Mb.p), st_[3], Mw.f(), s, stat_words, kStatWords,
+                              use_maps ? maps : nullptr);
       else
-        launch_warp_pair(gd, gFd, Fs.f(), st_[1], Fw.f(), gMd, Ms.f(), st_[3], Mw.f(), s, stat_words, kStatWords);
+        launch_warp_pair(gd, gFd, Fs.f(), st_[1], Fw.f(), gMd, Ms.f(), st_[3], Mw.f(), s, stat_words, kStatWords,
+                         use_maps ? maps : nullptr);
       prof.end(ANTS_KC_WARP, Nd * (2.0 * D * e_f + 2.0 * gsz * e_i + 2.0 * e_i));
       // 2. CC value and both derivatives (g_f2m -> wk[0], g_m2m -> wk[1])
       prof.begin(ANTS_KC_CC);
@@ -1144,6 +1200,63 @@ int ants_syn_set_stream(ants_syn_handle* h, void* stream) {
   if (h->own_stream && h->stream) HIP_TRY(hipStreamDestroy(h->stream));
   h->stream = static_cast<hipStream_t>(stream);
   h->own_stream = false;
+  return ANTS_OK;
+}
+
+int ants_syn_set_initial_transforms(ants_syn_handle* h, const ants_transform* moving, int n_moving,
+                                    const ants_transform* fixed, int n_fixed) {
+  if (!h) return fail(ANTS_ERR_INVALID, "null handle");
+  const int D = h->P.dim;
+  const ants_transform* lists[2] = {fixed, moving};
+  const int counts[2] = {n_fixed, n_moving};
+  bool on[2];
+  double A[2][9], o[2][3];
+  for (int k = 0; k < 2; ++k) {
+    const int n = counts[k];
+    if (n < 0 || n > ANTS_MAX_TRANSFORMS || (n > 0 && !lists[k]))
+      return fail(ANTS_ERR_INVALID, "initial transform count must be 0..%d", ANTS_MAX_TRANSFORMS);
+    // running composite y = Ac p + oc; the point passes through list[0] first
+    double Ac[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, oc[3] = {0, 0, 0};
+    on[k] = false;
+    for (int t = 0; t < n; ++t) {
+      const ants_transform& T = lists[k][t];
+      if (T.kind == ANTS_TRANSFORM_IDENTITY) continue;
+      if (T.kind == ANTS_TRANSFORM_DISPLACEMENT)
+        return fail(ANTS_ERR_UNSUPPORTED, "displacement-field initial transforms are not supported by the SyN stage");
+      if (T.kind != ANTS_TRANSFORM_AFFINE) return fail(ANTS_ERR_INVALID, "unknown transform kind %d", T.kind);
+      // MatrixOffsetTransformBase: y = B x + (t + c - B c), B padded to 3x3
+      double B[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, ob[3] = {0, 0, 0};
+      for (int r = 0; r < D; ++r) {
+        double bc = 0.0;
+        for (int c = 0; c < D; ++c) {
+          B[r * 3 + c] = T.matrix[r * D + c];
+          bc += T.matrix[r * D + c] * T.center[c];
+        }
+        ob[r] = T.translation[r] + T.center[r] - bc;
+      }
+      double An[9], on3[3];
+      for (int r = 0; r < 3; ++r) {
+        double t3 = ob[r];
+        for (int c = 0; c < 3; ++c) {
+          double a = 0.0;
+          for (int m = 0; m < 3; ++m) a += B[r * 3 + m] * Ac[m * 3 + c];
+          An[r * 3 + c] = a;
+          t3 += B[r * 3 + c] * oc[c];
+        }
+        on3[r] = t3;
+      }
+      std::memcpy(Ac, An, sizeof(Ac));
+      std::memcpy(oc, on3, sizeof(oc));
+      on[k] = true;
+    }
+    std::memcpy(A[k], Ac, sizeof(Ac));
+    std::memcpy(o[k], oc, sizeof(oc));
+  }
+  for (int k = 0; k < 2; ++k) {
+    h->init_on[k] = on[k];
+    std::memcpy(h->init_A[k], A[k], sizeof(A[k]));
+    std::memcpy(h->init_o[k], o[k], sizeof(o[k]));
+  }
   return ANTS_OK;
 }
 

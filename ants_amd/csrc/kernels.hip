@@ -92,6 +92,26 @@ __device__ __forceinline__ void cidx_of(const GridDev& go, int x, int y, int z, 
   }
 }
 
+// c = G ix + Q u + h (WarpMap): the level-grid point displaced by u, mapped
+// through an affine initial transform, as a continuous index.
+template <int D>
+__device__ __forceinline__ void map_cidx(const WarpMap& m, int x, int y, int z, const float* u, float c[3]) {
+  const float ix[3] = {static_cast<float>(x), static_cast<float>(y), static_cast<float>(z)};
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    if (r >= D) {
+      c[r] = 0.f;
+      continue;
+    }
+    float t = m.h[r];
+#pragma unroll
+    for (int k = 0; k < D; ++k) t += m.G[r * 3 + k] * ix[k];
+#pragma unroll
+    for (int k = 0; k < D; ++k) t += m.Q[r * 3 + k] * u[k];
+    c[r] = t;
+  }
+}
+
 // ITK ImageFunction::IsInsideBuffer on a continuous index: -0.5 <= c < n - 0.5.
 template <int D>
 __device__ __forceinline__ bool inside(const GridDev& g, const float c[3]) {
@@ -744,12 +764,13 @@ __global__ void __launch_bounds__(TX* TY, kSmoothWavesPerEU) k_smooth3d(GridDev 
 // ---------------------------------------------------------------------------
 // Block 0 also zeroes `clear` (nclear words; the iteration's statistics,
 // which only later kernels of the iteration write).
-template <int D, class T>
+template <int D, class T, bool MAP>
 __global__ void __launch_bounds__(kBlock) k_warp_pair(GridDev gl, GridDev gF, const T* __restrict__ Fs,
                                                       const float* __restrict__ uF, float* __restrict__ Fw,
                                                       GridDev gM, const T* __restrict__ Ms,
                                                       const float* __restrict__ uM, float* __restrict__ Mw,
-                                                      uint32_t* __restrict__ clear, int nclear) {
+                                                      uint32_t* __restrict__ clear, int nclear, WarpMap mF,
+                                                      WarpMap mM) {
   if (blockIdx.x == 0)
     for (int w = threadIdx.x; w < nclear; w += blockDim.x) clear[w] = 0u;
   const int64_t N = gl.count();
@@ -760,13 +781,19 @@ __global__ void __launch_bounds__(kBlock) k_warp_pair(GridDev gl, GridDev gF, co
   float u[3], c[3], v;
 #pragma unroll
   for (int d = 0; d < D; ++d) u[d] = uF[i * D + d];
-  cidx_of<D>(gl, x, y, z, u, gF, c);
+  if constexpr (MAP)
+    map_cidx<D>(mF, x, y, z, u, c);
+  else
+    cidx_of<D>(gl, x, y, z, u, gF, c);
   v = 0.f;
   if (inside<D>(gF, c)) v = interp_img<D, T>(gF, Fs, c);
   Fw[i] = v;
 #pragma unroll
   for (int d = 0; d < D; ++d) u[d] = uM[i * D + d];
-  cidx_of<D>(gl, x, y, z, u, gM, c);
+  if constexpr (MAP)
+    map_cidx<D>(mM, x, y, z, u, c);
+  else
+    cidx_of<D>(gl, x, y, z, u, gM, c);
   v = 0.f;
   if (inside<D>(gM, c)) v = interp_img<D, T>(gM, Ms, c);
   Mw[i] = v;
@@ -1955,28 +1982,41 @@ bool launch_smooth3d_image(const GridDev& g, const float* in, float* out, const 
   }
 }
 
+template <class T>
+static void warp_pair_t(const GridDev& gl, const GridDev& gF, const T* Fs, const float* uF, float* Fw,
+                        const GridDev& gM, const T* Ms, const float* uM, float* Mw, hipStream_t s, uint32_t* clear,
+                        int nclear, const WarpMap* maps) {
+  const dim3 grid(blocks_for(gl.count()));
+  const WarpMap none{};
+  const WarpMap& mF = maps ? maps[0] : none;
+  const WarpMap& mM = maps ? maps[1] : none;
+  if (gl.D == 2) {
+    if (maps)
+      hipLaunchKernelGGL((k_warp_pair<2, T, true>), grid, dim3(kBlock), 0, s, gl, gF, Fs, uF, Fw, gM, Ms, uM, Mw,
+                         clear, nclear, mF, mM);
+    else
+      hipLaunchKernelGGL((k_warp_pair<2, T, false>), grid, dim3(kBlock), 0, s, gl, gF, Fs, uF, Fw, gM, Ms, uM, Mw,
+                         clear, nclear, mF, mM);
+  } else {
+    if (maps)
+      hipLaunchKernelGGL((k_warp_pair<3, T, true>), grid, dim3(kBlock), 0, s, gl, gF, Fs, uF, Fw, gM, Ms, uM, Mw,
+                         clear, nclear, mF, mM);
+    else
+      hipLaunchKernelGGL((k_warp_pair<3, T, false>), grid, dim3(kBlock), 0, s, gl, gF, Fs, uF, Fw, gM, Ms, uM, Mw,
+                         clear, nclear, mF, mM);
+  }
+}
+
 void launch_warp_pair(const GridDev& gl, const GridDev& gF, const float* Fs, const float* uF, float* Fw,
                       const GridDev& gM, const float* Ms, const float* uM, float* Mw, hipStream_t s,
-                      uint32_t* clear, int nclear) {
-  const dim3 grid(blocks_for(gl.count()));
-  if (gl.D == 2)
-    hipLaunchKernelGGL((k_warp_pair<2, float>), grid, dim3(kBlock), 0, s, gl, gF, Fs, uF, Fw, gM, Ms, uM, Mw,
-                       clear, nclear);
-  else
-    hipLaunchKernelGGL((k_warp_pair<3, float>), grid, dim3(kBlock), 0, s, gl, gF, Fs, uF, Fw, gM, Ms, uM, Mw,
-                       clear, nclear);
+                      uint32_t* clear, int nclear, const WarpMap* maps) {
+  warp_pair_t<float>(gl, gF, Fs, uF, Fw, gM, Ms, uM, Mw, s, clear, nclear, maps);
 }
 
 void launch_warp_pair_bf16(const GridDev& gl, const GridDev& gF, const uint16_t* Fs, const float* uF, float* Fw,
                            const GridDev& gM, const uint16_t* Ms, const float* uM, float* Mw, hipStream_t s,
-                           uint32_t* clear, int nclear) {
-  const dim3 grid(blocks_for(gl.count()));
-  if (gl.D == 2)
-    hipLaunchKernelGGL((k_warp_pair<2, uint16_t>), grid, dim3(kBlock), 0, s, gl, gF, Fs, uF, Fw, gM, Ms, uM, Mw,
-                       clear, nclear);
-  else
-    hipLaunchKernelGGL((k_warp_pair<3, uint16_t>), grid, dim3(kBlock), 0, s, gl, gF, Fs, uF, Fw, gM, Ms, uM, Mw,
-                       clear, nclear);
+                           uint32_t* clear, int nclear, const WarpMap* maps) {
+  warp_pair_t<uint16_t>(gl, gF, Fs, uF, Fw, gM, Ms, uM, Mw, s, clear, nclear, maps);
 }
 
 void launch_to_bf16(const float* in, uint16_t* out, int64_t n, hipStream_t s) {

@@ -110,14 +110,25 @@ bool launch_smooth3d(const GridDev& g, int np, const float* const* in, const flo
 // axis (edges replicated), radius 1..6, in != out; false when not applicable.
 bool launch_smooth3d_image(const GridDev& g, const float* in, float* out, const Taps& t, hipStream_t s);
 
+// Continuous index in an image grid of the level-grid voxel ix displaced by
+// u and then mapped through an affine initial transform y = A p + o, folded
+// into one map: c = G ix + Q u + h (built on the host in double).
+struct WarpMap {
+  float G[9];
+  float Q[9];
+  float h[3];
+};
+
 // Fw = Fs(x + uF(x)), Mw = Ms(x + uM(x)) on level grid gl (Fs on gF, Ms on gM).
+// With `maps` (fixed, moving) the continuous indices come from the maps
+// instead of the grids' own geometry.
 void launch_warp_pair(const GridDev& gl, const GridDev& gF, const float* Fs, const float* uF, float* Fw,
                       const GridDev& gM, const float* Ms, const float* uM, float* Mw, hipStream_t s,
-                      uint32_t* clear = nullptr, int nclear = 0);
+                      uint32_t* clear = nullptr, int nclear = 0, const WarpMap* maps = nullptr);
 // Same with the smoothed images stored as BF16.
 void launch_warp_pair_bf16(const GridDev& gl, const GridDev& gF, const uint16_t* Fs, const float* uF, float* Fw,
                            const GridDev& gM, const uint16_t* Ms, const float* uM, float* Mw, hipStream_t s,
-                           uint32_t* clear = nullptr, int nclear = 0);
+                           uint32_t* clear = nullptr, int nclear = 0, const WarpMap* maps = nullptr);
 // float -> BF16 (round to nearest even).
 void launch_to_bf16(const float* in, uint16_t* out, int64_t n, hipStream_t s);
 void launch_copy_calibration(const void* in, void* out, int64_t n4, hipStream_t s);

@@ -274,6 +274,20 @@ int ants_apply_transforms(const ants_image_desc* ref, const ants_image_desc* in,
                           const ants_transform* transforms, int n_transforms, int interpolation,
                           double default_value, float* d_out, void* stream);
 
+/* Replaces SyNImageRegistrationMethod::SetMovingInitialTransform and
+ * SetFixedInitialTransform (Examples/itkantsRegistrationHelper.hxx:2119-2137,
+ * 3583-3633): the composite of -r / -q initial transforms and the preceding
+ * linear stages.  Each array lists the transforms in the order a point passes
+ * through them (the -t order of ants_apply_transforms).  Every iteration then
+ * samples the moving image at moving(x + u_m2m_inv(x)) and the fixed image at
+ * fixed(x + u_f2m_inv(x)); the output fields stay the SyN displacement fields
+ * alone, as the composite keeps the initial transforms separately.  Identity
+ * and affine transforms only (a displacement field returns
+ * ANTS_ERR_UNSUPPORTED).  Held by the handle for later ants_syn_register*
+ * calls; n_moving = n_fixed = 0 clears them. */
+int ants_syn_set_initial_transforms(ants_syn_handle* h, const ants_transform* moving, int n_moving,
+                                    const ants_transform* fixed, int n_fixed);
+
 /* ---- batched and served registration ----
  * No counterpart inside ANTs: the reference runs independent registrations
  * as separate processes or cluster jobs

@@ -182,11 +182,19 @@ def smooth_field(g: Grid, fld, variance: float):
     return out
 
 
-def warp_image(gi: Grid, img, go: Grid, u=None):
+def warp_image(gi: Grid, img, go: Grid, u=None, xforms=()):
+    """xforms: initial transforms after the displacement (see _xform_array)."""
     a = _f64(img)
     out = np.empty(go.shape, dtype=np.float64)
     keep = _f64(u) if u is not None else None
     up = _pd(keep) if keep is not None else None
+    if xforms:
+        xs, hold = _xform_array(xforms, go.dim)
+        fn = lib().og_warp_image_xf_f64
+        fn.argtypes = [C.POINTER(OgGrid), C.POINTER(C.c_double), C.POINTER(OgGrid), C.POINTER(C.c_double),
+                       C.POINTER(OgXform), C.c_int, C.POINTER(C.c_double)]
+        fn(C.byref(gi.c()), _pd(a), C.byref(go.c()), up, xs, len(xforms), _pd(out))
+        return out
     lib().og_warp_image_f64(C.byref(gi.c()), _pd(a), C.byref(go.c()), up, _pd(out))
     return out
 
@@ -305,7 +313,10 @@ class SynResult:
 
 
 def syn_register(params: SynParams, gF: Grid, fixed, gM: Grid, moving, precision: str = "f64",
-                 eps: float = FLT_EPSILON, want_state: bool = False, max_records: int = 4096) -> SynResult:
+                 eps: float = FLT_EPSILON, want_state: bool = False, max_records: int = 4096,
+                 fixed_xforms=(), moving_xforms=()) -> SynResult:
+    """fixed_xforms / moving_xforms: initial transforms (see _xform_array),
+    in the order a point passes through them."""
     L = lib()
     dt = np.float64 if precision == "f64" else np.float32
     ct = C.c_double if precision == "f64" else C.c_float
@@ -329,11 +340,22 @@ def syn_register(params: SynParams, gF: Grid, fixed, gM: Grid, moving, precision
     tr.convergence = cv.ctypes.data_as(C.POINTER(C.c_double))
     tr.inv_iters = ki.ctypes.data_as(C.POINTER(C.c_int))
     p = params.c()
-    fn = L.og_syn_register_f64 if precision == "f64" else L.og_syn_register_f32
     P = C.POINTER(ct)
-    rc = fn(C.byref(p), C.byref(gF.c()), f.ctypes.data_as(P), C.byref(gM.c()), m.ctypes.data_as(P),
-            fwd.ctypes.data_as(P), inv.ctypes.data_as(P), C.byref(tr),
-            state.ctypes.data_as(P) if state is not None else None, eps)
+    if fixed_xforms or moving_xforms:
+        fx, hold_f = _xform_array(fixed_xforms, gF.dim)
+        mx, hold_m = _xform_array(moving_xforms, gF.dim)
+        fn = L.og_syn_register_init_f64 if precision == "f64" else L.og_syn_register_init_f32
+        fn.argtypes = [C.POINTER(OgParams), C.POINTER(OgGrid), P, C.POINTER(OgGrid), P, C.POINTER(OgXform), C.c_int,
+                       C.POINTER(OgXform), C.c_int, P, P, C.POINTER(OgTrace), P, C.c_double]
+        fn.restype = C.c_int
+        rc = fn(C.byref(p), C.byref(gF.c()), f.ctypes.data_as(P), C.byref(gM.c()), m.ctypes.data_as(P), fx,
+                len(fixed_xforms), mx, len(moving_xforms), fwd.ctypes.data_as(P), inv.ctypes.data_as(P), C.byref(tr),
+                state.ctypes.data_as(P) if state is not None else None, eps)
+    else:
+        fn = L.og_syn_register_f64 if precision == "f64" else L.og_syn_register_f32
+        rc = fn(C.byref(p), C.byref(gF.c()), f.ctypes.data_as(P), C.byref(gM.c()), m.ctypes.data_as(P),
+                fwd.ctypes.data_as(P), inv.ctypes.data_as(P), C.byref(tr),
+                state.ctypes.data_as(P) if state is not None else None, eps)
     if rc != 0:
         raise RuntimeError(f"oracle syn_register failed: {rc}")
     n = tr.n_records
@@ -375,11 +397,9 @@ def og_geom(size, spacing=None, origin=None, direction=None) -> OgGeom:
     return g
 
 
-def apply_transforms(ref: OgGeom, inp: OgGeom, img, transforms=(), interpolation="linear", default=0.0):
-    """transforms: sequence of ("affine", A, t, c) / ("field", OgGeom, field(z,y,x,D)) / ("identity",),
-    applied to each reference point in order.  Returns float32 on the reference grid (numpy order)."""
-    D = ref.dim
-    a = np.ascontiguousarray(img, dtype=np.float32)
+def _xform_array(transforms, D):
+    """OgXform array for ("affine", A, t, c) / ("field", OgGeom, field(z,y,x,D)) /
+    ("identity",) tuples; returns (array, buffers to keep alive)."""
     xs = (OgXform * max(len(transforms), 1))()
     keep = []
     for i, t in enumerate(transforms):
@@ -401,6 +421,15 @@ def apply_transforms(ref: OgGeom, inp: OgGeom, img, transforms=(), interpolation
             xs[i].f = f.ctypes.data_as(C.POINTER(C.c_float))
         else:
             raise ValueError(t[0])
+    return xs, keep
+
+
+def apply_transforms(ref: OgGeom, inp: OgGeom, img, transforms=(), interpolation="linear", default=0.0):
+    """transforms: sequence of ("affine", A, t, c) / ("field", OgGeom, field(z,y,x,D)) / ("identity",),
+    applied to each reference point in order.  Returns float32 on the reference grid (numpy order)."""
+    D = ref.dim
+    a = np.ascontiguousarray(img, dtype=np.float32)
+    xs, keep = _xform_array(transforms, D)
     shape = tuple(int(ref.n[k]) for k in range(D))[::-1]
     out = np.empty(shape, np.float32)
     fn = lib().og_apply_transforms_f64

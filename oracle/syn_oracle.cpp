@@ -56,6 +56,24 @@ typedef struct {
   int threads;           // 0 = OpenMP default
 } og_params;
 
+// Geometry and transforms of a composite: antsApplyTransforms and the SyN
+// stage's initial transforms.
+struct og_geom {
+  int dim;
+  int64_t n[3];
+  double spacing[3];
+  double origin[3];
+  double direction[9];  // row-major dim x dim
+};
+struct og_xform {
+  int kind;  // 0 identity, 1 affine, 2 displacement field
+  double matrix[9];
+  double translation[3];
+  double center[3];
+  og_geom field;
+  const float* f;
+};
+
 typedef struct {
   int max_records;
   int n_records;
@@ -362,8 +380,16 @@ inline void grid_point(const Grid& g, int64_t x, int64_t y, int64_t z, double p[
 // with DownsampleImagesForMetricDerivatives on.  u == nullptr: identity.
 // Helper call sites: Examples/itkantsRegistrationHelper.hxx:719-763; in-tree
 // mirror of the SyN resampling: Examples/antsDisplacementAndVelocityFieldRegistrationCommandIterationUpdate.h:342-381.
+void xform_point(const og_xform& T, int D, double p[3]);
+
+// With initial transforms xf[0..nt) (SyNImageRegistrationMethod's
+// fixed/moving composite, Examples/itkantsRegistrationHelper.hxx:2119-2137):
+// the displaced point then passes through xf[0], xf[1], ... before the image
+// is sampled, as the composite [initial, middle^-1] applies the middle
+// field's inverse first.
 template <class R>
-void warp_image(const Grid& gi, const R* img, const Grid& go, const R* u, R* out) {
+void warp_image(const Grid& gi, const R* img, const Grid& go, const R* u, R* out, const og_xform* xf = nullptr,
+                int nt = 0) {
   const int D = go.D;
 #pragma omp parallel for schedule(static)
   for (int64_t z = 0; z < go.n[2]; ++z)
@@ -374,6 +400,7 @@ void warp_image(const Grid& gi, const R* img, const Grid& go, const R* u, R* out
         grid_point(go, x, y, z, p);
         if (u)
           for (int d = 0; d < D; ++d) p[d] += static_cast<double>(u[i * D + d]);
+        for (int t = 0; t < nt; ++t) xform_point(xf[t], D, p);
         to_cidx(gi, p, c);
         double v = 0.0;
         if (inside(gi, c)) interp<R>(gi, 1, img, c, &v);
@@ -768,7 +795,8 @@ void preprocess(int64_t N, const R* in, R* out) {
 // (same as Examples/itkantsRegistrationHelper.hxx:3698-3711).
 template <class R>
 int syn_register(const og_params* P, const Grid& gF, const R* fixed_in, const Grid& gM, const R* moving_in, R* out_fwd,
-                 R* out_inv, og_trace* trace, R* state4, double eps) {
+                 R* out_inv, og_trace* trace, R* state4, double eps, const og_xform* fx = nullptr, int nfx = 0,
+                 const og_xform* mx = nullptr, int nmx = 0) {
   set_threads(P->threads);
   const int D = gF.D;
   const int64_t NF = gF.count(), NM = gM.count();
@@ -809,8 +837,8 @@ int syn_register(const og_params* P, const Grid& gF, const R* fixed_in, const Gr
     bool converged = false;
     int iter = 0;
     while (iter++ < P->iterations[l] && !converged) {
-      warp_image<R>(gF, Fs.data(), gl, f2m_i.data(), Fw.data());
-      warp_image<R>(gM, Ms.data(), gl, m2m_i.data(), Mw.data());
+      warp_image<R>(gF, Fs.data(), gl, f2m_i.data(), Fw.data(), fx, nfx);
+      warp_image<R>(gM, Ms.data(), gl, m2m_i.data(), Mw.data(), mx, nmx);
       const double value = cc_metric<R>(gl, Fw.data(), Mw.data(), P->radius, eps, g1.data(), g2.data());
       smooth_field<R>(gl, g1.data(), u1.data(), P->update_variance);
       smooth_field<R>(gl, g2.data(), u2.data(), P->update_variance);
@@ -873,21 +901,6 @@ int syn_register(const og_params* P, const Grid& gF, const R* fixed_in, const Gr
 // indicator image interpolates highest; labels iterate in ascending order
 // with a strict comparison, so ties go to the smaller label).  Points whose
 // continuous index lies outside [-0.5, n-0.5) get the default value.
-struct og_geom {
-  int dim;
-  int64_t n[3];
-  double spacing[3];
-  double origin[3];
-  double direction[9];  // row-major dim x dim
-};
-struct og_xform {
-  int kind;  // 0 identity, 1 affine, 2 displacement field
-  double matrix[9];
-  double translation[3];
-  double center[3];
-  og_geom field;
-  const float* f;
-};
 
 namespace {
 
@@ -1145,6 +1158,27 @@ int og_syn_register_f64(const og_params* P, const og_grid* gf, const double* fix
 int og_syn_register_f32(const og_params* P, const og_grid* gf, const float* fixed, const og_grid* gm, const float* moving,
                         float* out_fwd, float* out_inv, og_trace* trace, float* state4, double eps) {
   return syn_register<float>(P, to_grid(gf), fixed, to_grid(gm), moving, out_fwd, out_inv, trace, state4, eps);
+}
+
+// The SyN stage with fixed / moving initial transforms (identity, affine or
+// displacement field), each list in the order a point passes through it.
+int og_syn_register_init_f64(const og_params* P, const og_grid* gf, const double* fixed, const og_grid* gm,
+                             const double* moving, const og_xform* fx, int nfx, const og_xform* mx, int nmx,
+                             double* out_fwd, double* out_inv, og_trace* trace, double* state4, double eps) {
+  return syn_register<double>(P, to_grid(gf), fixed, to_grid(gm), moving, out_fwd, out_inv, trace, state4, eps, fx, nfx,
+                              mx, nmx);
+}
+
+int og_syn_register_init_f32(const og_params* P, const og_grid* gf, const float* fixed, const og_grid* gm,
+                             const float* moving, const og_xform* fx, int nfx, const og_xform* mx, int nmx,
+                             float* out_fwd, float* out_inv, og_trace* trace, float* state4, double eps) {
+  return syn_register<float>(P, to_grid(gf), fixed, to_grid(gm), moving, out_fwd, out_inv, trace, state4, eps, fx, nfx,
+                             mx, nmx);
+}
+
+void og_warp_image_xf_f64(const og_grid* gi, const double* img, const og_grid* go, const double* u,
+                          const og_xform* xf, int nt, double* out) {
+  warp_image<double>(to_grid(gi), img, to_grid(go), u, out, xf, nt);
 }
 
 int og_max_threads(void) {

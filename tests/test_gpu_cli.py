@@ -46,3 +46,40 @@ def test_registration_and_apply_from_files(tmp_path):
                            "GenericLabel")
     np.testing.assert_array_equal(lw, ref.astype(np.int16))
     assert set(np.unique(lw)) <= set(np.unique(pair.labels_moving))
+
+
+def test_registration_with_initial_moving_transform(tmp_path):
+    """-r affine.mat: the stage runs with the moving initial transform, the
+    outputs are 0GenericAffine.mat + 1Warp / 1InverseWarp, and the warped
+    image goes through [warp, affine]."""
+    from ants_amd.apply_transforms import AffineTransform
+    pair = ph.phantom_pair_3d(4, size=(40, 44, 36), max_disp_mm=4.0, svf_sigma_mm=6.0)
+    g = Geometry(pair.size, pair.spacing, (0.0, 0.0, 0.0), np.eye(3))
+    f, m = str(tmp_path / "fixed.nii.gz"), str(tmp_path / "moving.nii.gz")
+    aio.write_nifti(f, pair.fixed, g)
+    aio.write_nifti(m, pair.moving, g)
+    c = np.cos(0.05), np.sin(0.05)
+    A = np.array([[c[0], -c[1], 0.0], [c[1], c[0], 0.0], [0.0, 0.0, 1.02]])
+    aff = AffineTransform(A, [0.6, -0.3, 0.4], [20.0, 22.0, 18.0])
+    aio.write_transform(str(tmp_path / "init.mat"), aff)
+    prefix = str(tmp_path / "out_")
+    args = ["-d", "3", "-m", f"CC[{f},{m},1,2]", "-t", "SyN[0.2,3,0]", "-c", "[6x4x2,1e-6,10]", "-f", "4x2x1",
+            "-s", "2x1x0vox", "-r", str(tmp_path / "init.mat"),
+            "-o", f"[{prefix},{prefix}Warped.nii.gz,{prefix}InverseWarped.nii.gz]"]
+    assert cli.main(["registration"] + args) == 0
+    fwd, _ = aio.read_nifti(prefix + "1Warp.nii.gz")
+    st = SyNStage.from_args("SyN[0.2,3,0]", "CC[f,m,1,2]", "[6x4x2,1e-6,10]", "4x2x1", "2x1x0vox")
+    reg = SyNRegistration(st, 3)
+    reg.set_initial_transforms([aff])
+    res = reg.register(pair.fixed, pair.moving, g.spacing, g.origin, g.spacing, g.origin)
+    reg.close()
+    np.testing.assert_array_equal(fwd, res.fwd)
+    back = aio.read_transform(prefix + "0GenericAffine.mat")
+    pts = np.random.default_rng(0).uniform(0, 40, (5, 3))
+    A0, t0, c0 = aff.parameters()
+    A1, t1, c1 = back.parameters()
+    np.testing.assert_allclose([A1 @ (p - c1) + t1 + c1 for p in pts], [A0 @ (p - c0) + t0 + c0 for p in pts],
+                               atol=1e-9)
+    warped, _ = aio.read_nifti(prefix + "Warped.nii.gz")
+    np.testing.assert_array_equal(warped, apply_transforms(g, pair.moving, g,
+                                                           [DisplacementFieldTransform(fwd, g), aff]))

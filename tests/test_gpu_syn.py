@@ -273,3 +273,112 @@ def test_speculative_inversion_is_exact(monkeypatch):
         np.testing.assert_array_equal(res.fwd, ref.fwd)
         np.testing.assert_array_equal(res.inv, ref.inv)
         assert st["inv_steps"] == st_ref["inv_steps"]
+
+
+# ------------------------------------------------- initial transforms of the stage
+def _affine_case(D, seed):
+    """A general affine (rotation, anisotropic scale, shear) about a centre."""
+    rng = np.random.default_rng(seed)
+    R = _rot3(*rng.uniform(-0.15, 0.15, 3)) if D == 3 else _rot3(0, 0, rng.uniform(-0.15, 0.15))[:2, :2]
+    Sh = np.eye(D)
+    Sh[0, 1] = 0.03
+    A = R @ np.diag(rng.uniform(0.96, 1.04, D)) @ Sh
+    return A, rng.uniform(-1.5, 1.5, D), rng.uniform(10, 20, D)
+
+
+def _run_with_initial(stage, pair, moving=(), fixed=(), precision="fp32", mgrid=None):
+    from ants_amd.apply_transforms import AffineTransform
+    D = len(pair.size)
+    reg = SyNRegistration(stage, D, precision=precision, force_iterations=True)
+    reg.set_initial_transforms([AffineTransform(*t) for t in moving], [AffineTransform(*t) for t in fixed])
+    sp, og, dr = (pair.spacing, pair.origin, None) if mgrid is None else mgrid
+    res = reg.register(pair.fixed, pair.moving, pair.spacing, pair.origin, sp, og, None, dr)
+    reg.close()
+    return res
+
+
+def _oracle_with_initial(stage, pair, moving=(), fixed=()):
+    P = o.SynParams(shrink=stage.shrink, sigma=stage.sigmas, iterations=stage.convergence.iterations,
+                    learning_rate=stage.transform.gradient_step, update_variance=stage.transform.update_variance,
+                    total_variance=stage.transform.total_variance, radius=stage.metric.radius,
+                    conv_threshold=stage.convergence.threshold, conv_window=stage.convergence.window,
+                    sigma_in_mm=stage.sigmas_in_mm, force_iterations=True)
+    g = o.Grid(list(pair.size), list(pair.spacing), list(pair.origin))
+    return o.syn_register(P, g, pair.fixed, g, pair.moving, fixed_xforms=[("affine",) + tuple(t) for t in fixed],
+                          moving_xforms=[("affine",) + tuple(t) for t in moving])
+
+
+def test_initial_transforms_3d_vs_oracle():
+    """Moving initial transform = a composite of two affines (the -r
+    transform and a preceding affine stage), fixed initial = a small rigid
+    one (-q), against the oracle."""
+    pair = ph.phantom_pair_3d(9, size=(40, 44, 36), max_disp_mm=4.0, svf_sigma_mm=6.0)
+    stage = SyNStage.from_args("SyN[0.2,3,0]", "CC[f,m,1,4]", "[8x5x3,1e-6,10]", "4x2x1", "2x1x0vox")
+    mv = [_affine_case(3, 1), _affine_case(3, 2)]
+    R, t, c = _affine_case(3, 3)
+    fx = [(_rot3(0.02, -0.03, 0.01), t * 0.3, c)]
+    res = _run_with_initial(stage, pair, mv, fx)
+    ref = _oracle_with_initial(stage, pair, mv, fx)
+    plain = _oracle_with_initial(stage, pair)
+    assert np.abs(plain.energy - ref.energy).max() > 1e-3, "the transforms must change the problem"
+    print("3d initial-transform parity:", compare(res, ref, pair.spacing))
+
+
+def test_initial_transform_2d_vs_oracle():
+    pair = ph.blob_pair_2d(1)
+    stage = SyNStage.from_args("SyN[0.25,3,0]", "CC[f,m,1,4]", "[20x10x5,1e-6,10]", "4x2x1", "2x1x0vox")
+    mv = [_affine_case(2, 4)]
+    res = _run_with_initial(stage, pair, mv)
+    ref = _oracle_with_initial(stage, pair, mv)
+    print("2d initial-transform parity:", compare(res, ref, pair.spacing))
+
+
+def test_initial_rigid_transform_equals_header_change_on_gpu():
+    """A rigid moving initial transform equals moving the moving header by
+    its inverse; the GPU's two coordinate paths (folded affine map, oblique
+    grid) agree."""
+    pair = ph.phantom_pair_3d(10, size=(40, 36, 44), max_disp_mm=4.0, svf_sigma_mm=6.0)
+    stage = SyNStage.from_args("SyN[0.2,3,0]", "CC[f,m,1,4]", "[8x5x3,1e-6,10]", "4x2x1", "2x1x0vox")
+    R = _rot3(0.05, -0.08, 0.06)
+    t, c = np.array([0.7, -0.4, 0.9]), np.array([20.0, 18.0, 22.0])
+    res = _run_with_initial(stage, pair, [(R, t, c)])
+    org = R.T @ (np.asarray(pair.origin) - t - c) + c
+    hdr = _run_with_initial(stage, pair, mgrid=(pair.spacing, org, R.T))
+    np.testing.assert_allclose(res.energy, hdr.energy, rtol=1e-5)
+    np.testing.assert_allclose(res.fwd, hdr.fwd, atol=5e-3)
+    np.testing.assert_allclose(res.inv, hdr.inv, atol=5e-3)
+
+
+def test_initial_transform_bf16_and_clearing():
+    pair = ph.phantom_pair_3d(11, size=(32, 36, 28), max_disp_mm=3.0, svf_sigma_mm=6.0)
+    stage = SyNStage.from_args("SyN[0.2,3,0]", "CC[f,m,1,2]", "[6x4x2,1e-6,10]", "4x2x1", "2x1x0vox")
+    mv = [_affine_case(3, 7)]
+    ref = _oracle_with_initial(stage, pair, mv)
+    b = _run_with_initial(stage, pair, mv, precision="bf16")
+    np.testing.assert_allclose(b.energy, ref.energy, rtol=5e-3)
+    # transforms stay on the handle until cleared; a cleared handle runs the plain stage
+    from ants_amd.apply_transforms import AffineTransform
+    reg = SyNRegistration(stage, 3, force_iterations=True)
+    reg.set_initial_transforms([AffineTransform(*mv[0])])
+    r1 = reg.register(pair.fixed, pair.moving, pair.spacing, pair.origin, pair.spacing, pair.origin)
+    r2 = reg.register(pair.fixed, pair.moving, pair.spacing, pair.origin, pair.spacing, pair.origin)
+    reg.set_initial_transforms()
+    r3 = reg.register(pair.fixed, pair.moving, pair.spacing, pair.origin, pair.spacing, pair.origin)
+    reg.close()
+    plain = run_pair(stage, pair, force=True)
+    np.testing.assert_array_equal(r1.fwd, r2.fwd)
+    np.testing.assert_array_equal(r3.fwd, plain.fwd)
+    assert np.abs(r1.fwd - r3.fwd).max() > 1e-3
+
+
+def test_initial_transform_errors():
+    from ants_amd.apply_transforms import AffineTransform, DisplacementFieldTransform, Geometry
+    stage = SyNStage.from_args("SyN[0.2,3,0]", "CC[f,m,1,2]", "[2x1,1e-6,10]", "2x1", "1x0vox")
+    reg = SyNRegistration(stage, 3)
+    geo = Geometry(size=(4, 4, 4), spacing=(1, 1, 1), origin=(0, 0, 0), direction=np.eye(3))
+    with pytest.raises(L.AntsError) as e:
+        reg.set_initial_transforms([DisplacementFieldTransform(np.zeros((4, 4, 4, 3), np.float32), geo)])
+    assert e.value.code == 4
+    with pytest.raises(ValueError):
+        reg.set_initial_transforms([AffineTransform(np.eye(3), [0, 0, 0])] * 9)
+    reg.close()

@@ -1,5 +1,6 @@
 """Host-side helpers (CPU only)."""
 import numpy as np
+import pytest
 
 from ants_amd.analysis import jacobian_determinant
 
@@ -46,3 +47,21 @@ def test_counter_traffic_scaling(tmp_path):
     assert float(out["fetch_bytes_per_launch"]) == 0.5 * 300.0 * 1024
     assert float(out["write_bytes_per_launch"]) == 100.0 * 1024
     assert float(out["traffic_bytes_per_launch"]) == 0.5 * 300.0 * 1024 + 100.0 * 1024
+
+
+def test_compose_affines_applies_the_first_transform_first():
+    from ants_amd.apply_transforms import AffineTransform, compose_affines
+    rng = np.random.default_rng(3)
+    ts = [AffineTransform(rng.normal(size=(3, 3)) + 2 * np.eye(3), rng.normal(size=3), rng.normal(size=3))
+          for _ in range(3)]
+    ts[1].invert = True
+
+    def apply(t, p):
+        A, tr, c = t.parameters()
+        return A @ (p - c) + tr + c
+
+    comp = compose_affines([ts[0], None, ts[1], ts[2]])
+    for p in rng.normal(size=(6, 3)):
+        np.testing.assert_allclose(apply(comp, p), apply(ts[2], apply(ts[1], apply(ts[0], p))), atol=1e-9)
+    with pytest.raises(ValueError):
+        compose_affines([None])

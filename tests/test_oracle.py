@@ -415,3 +415,112 @@ def test_direction_equivariance_of_the_stage(D):
     cR0 = np.asarray(org) + R @ ((np.asarray(size) - 1) / 2.0 * 1.5)
     np.testing.assert_allclose(cR, cR0, atol=1e-12)
     np.testing.assert_allclose(li.size, lr.size)
+
+
+# ------------------------------------------------- initial transforms of the stage
+def _rigid(D, seed):
+    rng = np.random.default_rng(seed)
+    R = _rot3(*rng.uniform(-0.3, 0.3, 3)) if D == 3 else _rot3(0, 0, rng.uniform(-0.3, 0.3))[:2, :2]
+    return R, rng.uniform(-2, 2, D), rng.uniform(5, 15, D)
+
+
+def _header_through_inverse(g: "o.Grid", inverses):
+    """Grid whose voxel idx sits at inv_k(...inv_1(point_g(idx))): sampling it
+    at p equals sampling g at T_1(...T_k(p)).  Each inverse is (A, x -> A x + b)
+    with A orthogonal-times-diagonal so the result is again a grid."""
+    D = g.dim
+    S = np.asarray(g.direction, float).reshape(D, D) @ np.diag(np.asarray(g.spacing, float))
+    org = np.asarray(g.origin, float)
+    for A, b in inverses:
+        S = A @ S
+        org = A @ org + b
+    sp = np.linalg.norm(S, axis=0)
+    return o.Grid(list(g.size), list(sp), list(org), S / sp)
+
+
+def _stage_params():
+    return o.SynParams(shrink=[2, 1], sigma=[1, 0], iterations=[5, 3], learning_rate=0.2, update_variance=3.0,
+                       total_variance=0.0, radius=2, force_iterations=True)
+
+
+def _pair(D, seed):
+    rng = np.random.default_rng(seed)
+    size = (26, 22, 20)[:D]
+    shape = tuple(reversed(size))
+    F = ndimage.gaussian_filter(rng.standard_normal(shape), 2.0)
+    M = ndimage.shift(F, [0.8, -1.2, 0.6][:D], order=1, mode="nearest")
+    return size, (F - F.min()) / np.ptp(F), (M - M.min()) / np.ptp(M)
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_initial_rigid_transform_is_a_moving_header_change(D):
+    """A rigid moving initial transform y = R (x - c) + t + c is the same as
+    moving the moving image's header by its inverse x = R^T (y - t - c) + c:
+    the stage must give the same energies and fields.  Applying the
+    transform before the displacement, transposing R or dropping the centre
+    breaks this."""
+    size, F, M = _pair(D, 11)
+    R, t, c = _rigid(D, 3)
+    gF = o.Grid(list(size), [1.2] * D, [1.0, -2.0, 0.5][:D])
+    gM = o.Grid(list(size), [1.2] * D, [0.0, -1.0, 1.5][:D])
+    gMh = _header_through_inverse(gM, [(R.T, -R.T @ (t + c) + c)])
+    P = _stage_params()
+    r1 = o.syn_register(P, gF, F, gM, M, moving_xforms=[("affine", R, t, c)])
+    r2 = o.syn_register(P, gF, F, gMh, M)
+    assert r1.energy.max() < -0.15 and np.abs(r1.fwd).max() > 0.3
+    np.testing.assert_allclose(r1.energy, r2.energy, rtol=1e-9)
+    np.testing.assert_allclose(r1.fwd, r2.fwd, atol=1e-8)
+    np.testing.assert_allclose(r1.inv, r2.inv, atol=1e-8)
+    # the transform does change the result (the test is not vacuous)
+    r0 = o.syn_register(P, gF, F, gM, M)
+    assert np.abs(r0.energy - r1.energy).max() > 1e-3
+
+
+def test_initial_scaling_and_composite_order():
+    """Anisotropic scaling y = S x + b equals a header with spacing / S; a
+    list [T0, T1] maps a point through T0 first, so it equals the header
+    moved by T1^-1 and then T0^-1 (rotations about different centres do not
+    commute, so the reversed order fails)."""
+    D = 3
+    size, F, M = _pair(D, 12)
+    gF = o.Grid(list(size), [1.0] * D, [0.0] * D)
+    gM = o.Grid(list(size), [1.0] * D, [0.5, -0.5, 0.0])
+    P = _stage_params()
+    s = np.array([1.04, 0.97, 1.02])
+    b = np.array([-0.4, 0.3, 0.2])
+    r1 = o.syn_register(P, gF, F, gM, M, moving_xforms=[("affine", np.diag(s), b, np.zeros(3))])
+    r2 = o.syn_register(P, gF, F, _header_through_inverse(gM, [(np.diag(1 / s), -b / s)]), M)
+    np.testing.assert_allclose(r1.energy, r2.energy, rtol=1e-9)
+    np.testing.assert_allclose(r1.fwd, r2.fwd, atol=1e-8)
+    R0, t0, c0 = _rigid(D, 4)
+    R1, t1, c1 = _rigid(D, 5)
+    inv0 = (R0.T, -R0.T @ (t0 + c0) + c0)
+    inv1 = (R1.T, -R1.T @ (t1 + c1) + c1)
+    xs = [("affine", R0, t0, c0), ("affine", R1, t1, c1)]
+    ra = o.syn_register(P, gF, F, gM, M, moving_xforms=xs)
+    rb = o.syn_register(P, gF, F, _header_through_inverse(gM, [inv1, inv0]), M)
+    rc = o.syn_register(P, gF, F, _header_through_inverse(gM, [inv0, inv1]), M)
+    np.testing.assert_allclose(ra.energy, rb.energy, rtol=1e-9)
+    np.testing.assert_allclose(ra.fwd, rb.fwd, atol=1e-8)
+    assert np.abs(ra.energy - rc.energy).max() > 1e-6
+
+
+def test_fixed_initial_transform_mirrors_the_moving_one():
+    """SyN is symmetric: swapping the images (same grid) swaps the forward
+    and inverse fields.  With the transform moved along, a fixed initial
+    transform must act on the fixed side exactly as a moving one does on
+    the moving side."""
+    D = 2
+    size, F, M = _pair(D, 13)
+    g = o.Grid(list(size), [1.0] * D, [0.0] * D)
+    R, t, c = _rigid(D, 6)
+    P = _stage_params()
+    x = [("affine", R, t, c)]
+    r1 = o.syn_register(P, g, F, g, M, fixed_xforms=x)
+    r2 = o.syn_register(P, g, M, g, F, moving_xforms=x)
+    np.testing.assert_allclose(r1.energy, r2.energy, rtol=1e-9)
+    np.testing.assert_allclose(r1.fwd, r2.inv, atol=1e-8)
+    np.testing.assert_allclose(r1.inv, r2.fwd, atol=1e-8)
+    r3 = o.syn_register(P, g, F, g, M, fixed_xforms=[("identity",)], moving_xforms=[("identity",)])
+    r4 = o.syn_register(P, g, F, g, M)
+    np.testing.assert_array_equal(r3.fwd, r4.fwd)
