@@ -478,6 +478,64 @@ static int wait_published(const uint32_t* flag, hipStream_t s) {
   return ANTS_OK;
 }
 
+// Initial transforms (fixed side, moving side) composed into y = A p + o;
+// on[k] false: identity.
+struct InitialXforms {
+  bool on[2] = {false, false};
+  double A[2][9] = {};
+  double o[2][3] = {};
+};
+
+// Composes each list (the point passes through list[0] first) with
+// MatrixOffsetTransformBase's y = B (x - c) + t + c, in double.
+int compose_initial(int D, const ants_transform* fixed, int n_fixed, const ants_transform* moving, int n_moving,
+                    InitialXforms* out) {
+  const ants_transform* lists[2] = {fixed, moving};
+  const int counts[2] = {n_fixed, n_moving};
+  InitialXforms x;
+  for (int k = 0; k < 2; ++k) {
+    const int n = counts[k];
+    if (n < 0 || n > ANTS_MAX_TRANSFORMS || (n > 0 && !lists[k]))
+      return fail(ANTS_ERR_INVALID, "initial transform count must be 0..%d", ANTS_MAX_TRANSFORMS);
+    double Ac[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, oc[3] = {0, 0, 0};
+    for (int t = 0; t < n; ++t) {
+      const ants_transform& T = lists[k][t];
+      if (T.kind == ANTS_TRANSFORM_IDENTITY) continue;
+      if (T.kind == ANTS_TRANSFORM_DISPLACEMENT)
+        return fail(ANTS_ERR_UNSUPPORTED, "displacement-field initial transforms are not supported by the SyN stage");
+      if (T.kind != ANTS_TRANSFORM_AFFINE) return fail(ANTS_ERR_INVALID, "unknown transform kind %d", T.kind);
+      double B[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, ob[3] = {0, 0, 0};
+      for (int r = 0; r < D; ++r) {
+        double bc = 0.0;
+        for (int c = 0; c < D; ++c) {
+          B[r * 3 + c] = T.matrix[r * D + c];
+          bc += T.matrix[r * D + c] * T.center[c];
+        }
+        ob[r] = T.translation[r] + T.center[r] - bc;
+      }
+      // running composite: y = B (Ac p + oc) + ob
+      double An[9], on[3];
+      for (int r = 0; r < 3; ++r) {
+        double v = ob[r];
+        for (int c = 0; c < 3; ++c) {
+          double a = 0.0;
+          for (int m = 0; m < 3; ++m) a += B[r * 3 + m] * Ac[m * 3 + c];
+          An[r * 3 + c] = a;
+          v += B[r * 3 + c] * oc[c];
+        }
+        on[r] = v;
+      }
+      std::memcpy(Ac, An, sizeof(Ac));
+      std::memcpy(oc, on, sizeof(oc));
+      x.on[k] = true;
+    }
+    std::memcpy(x.A[k], Ac, sizeof(Ac));
+    std::memcpy(x.o[k], oc, sizeof(oc));
+  }
+  *out = x;
+  return ANTS_OK;
+}
+
 struct ants_syn_handle {
   ants_syn_params P{};
   hipStream_t stream = nullptr;
@@ -503,6 +561,14 @@ struct ants_syn_handle {
   bool init_on[2] = {false, false};
   double init_A[2][9] = {};
   double init_o[2][3] = {};
+
+  void set_initial(const InitialXforms& x) {
+    for (int k = 0; k < 2; ++k) {
+      init_on[k] = x.on[k];
+      std::memcpy(init_A[k], x.A[k], sizeof(init_A[k]));
+      std::memcpy(init_o[k], x.o[k], sizeof(init_o[k]));
+    }
+  }
 
   void clear_graphs() {
     for (auto& kv : graphs)
@@ -1206,57 +1272,9 @@ int ants_syn_set_stream(ants_syn_handle* h, void* stream) {
 int ants_syn_set_initial_transforms(ants_syn_handle* h, const ants_transform* moving, int n_moving,
                                     const ants_transform* fixed, int n_fixed) {
   if (!h) return fail(ANTS_ERR_INVALID, "null handle");
-  const int D = h->P.dim;
-  const ants_transform* lists[2] = {fixed, moving};
-  const int counts[2] = {n_fixed, n_moving};
-  bool on[2];
-  double A[2][9], o[2][3];
-  for (int k = 0; k < 2; ++k) {
-    const int n = counts[k];
-    if (n < 0 || n > ANTS_MAX_TRANSFORMS || (n > 0 && !lists[k]))
-      return fail(ANTS_ERR_INVALID, "initial transform count must be 0..%d", ANTS_MAX_TRANSFORMS);
-    // running composite y = Ac p + oc; the point passes through list[0] first
-    double Ac[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, oc[3] = {0, 0, 0};
-    on[k] = false;
-    for (int t = 0; t < n; ++t) {
-      const ants_transform& T = lists[k][t];
-      if (T.kind == ANTS_TRANSFORM_IDENTITY) continue;
-      if (T.kind == ANTS_TRANSFORM_DISPLACEMENT)
-        return fail(ANTS_ERR_UNSUPPORTED, "displacement-field initial transforms are not supported by the SyN stage");
-      if (T.kind != ANTS_TRANSFORM_AFFINE) return fail(ANTS_ERR_INVALID, "unknown transform kind %d", T.kind);
-      // MatrixOffsetTransformBase: y = B x + (t + c - B c), B padded to 3x3
-      double B[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, ob[3] = {0, 0, 0};
-      for (int r = 0; r < D; ++r) {
-        double bc = 0.0;
-        for (int c = 0; c < D; ++c) {
-          B[r * 3 + c] = T.matrix[r * D + c];
-          bc += T.matrix[r * D + c] * T.center[c];
-        }
-        ob[r] = T.translation[r] + T.center[r] - bc;
-      }
-      double An[9], on3[3];
-      for (int r = 0; r < 3; ++r) {
-        double t3 = ob[r];
-        for (int c = 0; c < 3; ++c) {
-          double a = 0.0;
-          for (int m = 0; m < 3; ++m) a += B[r * 3 + m] * Ac[m * 3 + c];
-          An[r * 3 + c] = a;
-          t3 += B[r * 3 + c] * oc[c];
-        }
-        on3[r] = t3;
-      }
-      std::memcpy(Ac, An, sizeof(Ac));
-      std::memcpy(oc, on3, sizeof(oc));
-      on[k] = true;
-    }
-    std::memcpy(A[k], Ac, sizeof(Ac));
-    std::memcpy(o[k], oc, sizeof(oc));
-  }
-  for (int k = 0; k < 2; ++k) {
-    h->init_on[k] = on[k];
-    std::memcpy(h->init_A[k], A[k], sizeof(A[k]));
-    std::memcpy(h->init_o[k], o[k], sizeof(o[k]));
-  }
+  InitialXforms x;
+  ANTS_TRY(compose_initial(h->P.dim, fixed, n_fixed, moving, n_moving, &x));
+  h->set_initial(x);
   return ANTS_OK;
 }
 
@@ -1667,6 +1685,7 @@ struct BatchItem {
   ants_syn_request req;
   int64_t footprint;
   double t_submit;
+  InitialXforms init;  // composed at submit: the request's transform arrays need not outlive it
 };
 
 }  // namespace
@@ -1728,6 +1747,7 @@ void ants_batch::worker(int slot) {
     HostGrid gl{};
     if (rc == ANTS_OK) {
       h.P = r.params;
+      h.set_initial(it.init);
       gl = level_grid(gF, r.params.shrink[r.params.n_levels - 1]);
       const size_t fb = static_cast<size_t>(gl.count()) * gF.D * sizeof(float);
       if (rc == ANTS_OK) rc = d_moving.ensure(gM.count() * sizeof(float));
@@ -1844,7 +1864,10 @@ int ants_batch_submit(ants_batch* b, const ants_syn_request* r, int64_t* id) {
   if (gF.D != r->params.dim || gM.D != r->params.dim) return fail(ANTS_ERR_INVALID, "dimension mismatch");
   if (!r->host_moving || (!r->use_template && !r->host_fixed)) return fail(ANTS_ERR_INVALID, "null image");
   const int64_t fp = footprint_bytes(r->params, gF, gM);
-  BatchItem it{0, *r, fp, now_s()};
+  InitialXforms init;
+  ANTS_TRY(compose_initial(r->params.dim, r->fixed_initial, r->n_fixed_initial, r->moving_initial,
+                           r->n_moving_initial, &init));
+  BatchItem it{0, *r, fp, now_s(), init};
   {
     std::lock_guard<std::mutex> lk(b->mu);
     if (fp > b->budget) return fail(ANTS_ERR_OOM, "request needs %lld bytes, budget is %lld", (long long)fp,

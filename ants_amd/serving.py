@@ -36,6 +36,10 @@ class RequestC(C.Structure):
         ("host_out_inv", C.c_void_p),
         ("host_out_labels", C.c_void_p),
         ("user", C.c_void_p),
+        ("moving_initial", C.POINTER(L.TransformC)),
+        ("n_moving_initial", C.c_int),
+        ("fixed_initial", C.POINTER(L.TransformC)),
+        ("n_fixed_initial", C.c_int),
     ]
 
 
@@ -108,6 +112,8 @@ class Request:
     precision: str = "fp32"
     want_fields: bool = False
     tag: str = ""
+    moving_initial: Sequence = ()          # apply_transforms.AffineTransform list (-r / preceding linear stages)
+    fixed_initial: Sequence = ()
 
 
 @dataclass
@@ -197,6 +203,13 @@ class Batch:
             lab = np.empty(oshape, np.int32)
             r.host_moving_labels = ml.ctypes.data
             r.host_out_labels = lab.ctypes.data
+        if req.moving_initial or req.fixed_initial:
+            from .apply_transforms import transform_array
+            mx, _ = transform_array(list(req.moving_initial), dim)
+            fx, _ = transform_array(list(req.fixed_initial), dim)
+            keep += [mx, fx]
+            r.moving_initial, r.n_moving_initial = mx, len(req.moving_initial)
+            r.fixed_initial, r.n_fixed_initial = fx, len(req.fixed_initial)
         rid = C.c_int64()
         L.check(self.lib.ants_batch_submit(self.h, C.byref(r), C.byref(rid)))
         self.live[rid.value] = (req, keep, r, fwd, inv, lab)

@@ -313,6 +313,12 @@ typedef struct ants_syn_request {
   float* host_out_inv;
   int32_t* host_out_labels;            /* optional, last level grid */
   void* user;
+  /* optional initial transforms, as ants_syn_set_initial_transforms takes
+   * them; read by ants_batch_submit */
+  const ants_transform* moving_initial;
+  int n_moving_initial;
+  const ants_transform* fixed_initial;
+  int n_fixed_initial;
 } ants_syn_request;
 
 typedef struct ants_request_result {

@@ -110,3 +110,35 @@ def test_serving_loop_reports_latency():
     st = latency_stats(res)
     assert st["n"] == 6 and st["p99_s"] >= st["p50_s"] > 0
     assert all(r.t_start >= r.t_submit for r in res)
+
+
+def test_batch_requests_with_initial_transforms():
+    """Requests carrying initial transforms (mixed with plain ones on the same
+    slots) equal the same registrations run alone with
+    set_initial_transforms, bit for bit; a displacement-field initial
+    transform is refused at submit."""
+    from ants_amd import _lib as L
+    from ants_amd.apply_transforms import AffineTransform, DisplacementFieldTransform, Geometry
+    reqs = make_requests(4, seed=21)
+    c, s = np.cos(0.06), np.sin(0.06)
+    aff = AffineTransform(np.array([[c, -s, 0.0], [s, c, 0.0], [0.0, 0.0, 1.03]]), [0.5, -0.4, 0.3], [20, 22, 20])
+    reqs[0].moving_initial = [aff]
+    reqs[2].fixed_initial = [AffineTransform(np.eye(3), [0.3, 0.0, -0.2])]
+    b = Batch(slots=2)
+    res = run_all(b, reqs)
+    assert all(r.status == 0 for r in res), [r.error for r in res]
+    for r in res:
+        req = r.request
+        reg = SyNRegistration(req.stage, 3)
+        reg.set_initial_transforms(list(req.moving_initial), list(req.fixed_initial))
+        ref = reg.register(req.fixed, req.moving)
+        reg.close()
+        np.testing.assert_array_equal(r.fwd, ref.fwd)
+        np.testing.assert_array_equal(r.inv, ref.inv)
+    geo = Geometry(size=(4, 4, 4), spacing=(1, 1, 1), origin=(0, 0, 0), direction=np.eye(3))
+    bad = make_requests(1, seed=22)[0]
+    bad.moving_initial = [DisplacementFieldTransform(np.zeros((4, 4, 4, 3), np.float32), geo)]
+    with pytest.raises(L.AntsError) as e:
+        b.submit(bad)
+    assert e.value.code == 4
+    b.close()
