@@ -148,6 +148,7 @@ def lib():
     L.ants_apply_transform_labels_nn.argtypes = [pdesc, vp, pdesc, vp, vp, vp]
     L.ants_apply_transforms.argtypes = [pdesc, pdesc, vp, C.POINTER(TransformC), i, i, d, vp, vp]
     L.ants_syn_set_initial_transforms.argtypes = [vp, C.POINTER(TransformC), i, C.POINTER(TransformC), i]
+    L.ants_syn_set_initial_state_device.argtypes = [vp, pdesc, vp, vp, vp, vp]
     _lib = L
     return L
 

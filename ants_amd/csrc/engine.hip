@@ -562,6 +562,12 @@ struct ants_syn_handle {
   double init_A[2][9] = {};
   double init_o[2][3] = {};
 
+  // restored state (F2M fwd, F2M inv, M2M fwd, M2M inv on restore_grid),
+  // resampled onto the level-0 grid at the start of a run
+  DevBuf restore_buf;
+  HostGrid restore_grid{};
+  bool restore_on = false;
+
   void set_initial(const InitialXforms& x) {
     for (int k = 0; k < 2; ++k) {
       init_on[k] = x.on[k];
@@ -578,7 +584,8 @@ struct ants_syn_handle {
 
   ~ants_syn_handle() {
     clear_graphs();
-    DevBuf* all[] = {&F, &M, &Fs, &Ms, &tmp, &Fw, &Mw, &Fb, &Mb, &stats, &partials_cc, &partials_inv, &masks_inv, &minmax};
+    DevBuf* all[] = {&F,     &M,           &Fs,          &Ms,        &tmp,    &Fw,         &Mw,  &Fb,
+                     &Mb,    &stats,       &partials_cc, &partials_inv, &masks_inv, &minmax, &restore_buf};
     for (DevBuf* b : all) b->release();
     for (auto& b : fld) b.release();
     if (host_stats) (void)hipHostFree(host_stats);
@@ -893,7 +900,16 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
       maps[0] = make_warp_map(g, init_on[0] ? init_A[0] : I, init_on[0] ? init_o[0] : Z, gF);
       maps[1] = make_warp_map(g, init_on[1] ? init_A[1] : I, init_on[1] ? init_o[1] : Z, gM);
     }
-    if (l == 0) {
+    if (l == 0 && restore_on) {
+      // SyN restored from a saved state: the level-0 parameter adaptor
+      // resamples the four fields onto the level grid
+      if (restore_grid.D != D) return fail(ANTS_ERR_INVALID, "restored state dimension does not match");
+      const float* rb = restore_buf.f();
+      const int64_t nr = restore_grid.count() * D;
+      const float* in[4] = {rb, rb + nr, rb + 2 * nr, rb + 3 * nr};
+      launch_resample_fields(to_dev(restore_grid), gd, 4, in, st, s);
+      HIP_TRY(hipGetLastError());
+    } else if (l == 0) {
       for (int k = 0; k < 4; ++k) HIP_TRY(hipMemsetAsync(st[k], 0, N * D * sizeof(float), s));
     } else {
       const float* in[4] = {st[0], st[1], st[2], st[3]};
@@ -919,7 +935,7 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
     int kl = inv_launches;
     bool converged = false;
     int iter = 0;
-    // algorithmic bytes per voxel-iteration (see DESIGN.md): image gathers + field passes
+    // algorithmic bytes per voxel-iteration: image gathers + field passes
     int s_max = 1;
     for (int d = 0; d < D; ++d) s_max = std::max(s_max, static_cast<int>(std::lround(g.sp[d] / gF.sp[d])));
     double gsz = 1.0;
@@ -1324,6 +1340,31 @@ int ants_syn_get_state_device(ants_syn_handle* h, float* d_f2m_fwd, float* d_f2m
   for (int k = 0; k < 4; ++k)
     if (dst[k]) HIP_TRY(hipMemcpyAsync(dst[k], h->state[k], b, hipMemcpyDeviceToDevice, h->stream));
   HIP_TRY(hipStreamSynchronize(h->stream));
+  return ANTS_OK;
+}
+
+int ants_syn_set_initial_state_device(ants_syn_handle* h, const ants_image_desc* grid, const float* d_f2m_fwd,
+                                      const float* d_f2m_inv, const float* d_m2m_fwd, const float* d_m2m_inv) {
+  if (!h) return fail(ANTS_ERR_INVALID, "null handle");
+  const float* src[4] = {d_f2m_fwd, d_f2m_inv, d_m2m_fwd, d_m2m_inv};
+  int given = 0;
+  for (const float* p : src) given += p ? 1 : 0;
+  if (given == 0) {
+    h->restore_on = false;
+    return ANTS_OK;
+  }
+  // the reference requires both fields of both transforms ("Invalid state restoration" otherwise)
+  if (given != 4) return fail(ANTS_ERR_INVALID, "a restored state needs all four fields");
+  HostGrid g;
+  ANTS_TRY(to_host_grid(grid, &g));
+  if (g.D != h->P.dim) return fail(ANTS_ERR_INVALID, "state dimension does not match the stage");
+  const size_t b = static_cast<size_t>(g.count()) * g.D * sizeof(float);
+  ANTS_TRY(h->restore_buf.ensure(4 * b));
+  char* dst = static_cast<char*>(h->restore_buf.p);
+  for (int k = 0; k < 4; ++k) HIP_TRY(hipMemcpyAsync(dst + k * b, src[k], b, hipMemcpyDeviceToDevice, h->stream));
+  HIP_TRY(hipStreamSynchronize(h->stream));
+  h->restore_grid = g;
+  h->restore_on = true;
   return ANTS_OK;
 }
 

@@ -223,6 +223,22 @@ class SyNRegistration:
         fx, keep_f = transform_array(list(fixed), self.dim)
         L.check(L.lib().ants_syn_set_initial_transforms(self.handle, mx, len(moving), fx, len(fixed)))
 
+    def set_initial_state(self, state=None, size=None, spacing=None, origin=None, direction=None):
+        """Start later runs from a saved state (SetFixedToMiddleTransform /
+        SetMovingToMiddleTransform, restored by --restore-state): `state` is
+        the four fields F2M fwd, F2M inv, M2M fwd, M2M inv as `state()`
+        returns them, on the grid (size (x, y[, z]), spacing, origin,
+        direction); None clears it."""
+        if state is None:
+            L.check(L.lib().ants_syn_set_initial_state_device(self.handle, None, None, None, None, None))
+            return
+        if len(state) != 4:
+            raise ValueError("a state is four fields")
+        bufs = [f if isinstance(f, L.DeviceArray) else L.DeviceArray.from_host(np.asarray(f, np.float32))
+                for f in state]
+        g = L.desc(size, spacing, origin, direction)
+        L.check(L.lib().ants_syn_set_initial_state_device(self.handle, C.byref(g), *[b.ptr for b in bufs]))
+
     def set_profiling(self, mask: int):
         L.check(L.lib().ants_syn_set_profiling(self.handle, mask))
 

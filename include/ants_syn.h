@@ -134,6 +134,18 @@ int ants_syn_register_device(ants_syn_handle* h, const ants_image_desc* fixed, c
 int ants_syn_get_state_device(ants_syn_handle* h, float* d_f2m_fwd, float* d_f2m_inv, float* d_m2m_fwd,
                               float* d_m2m_inv);
 
+/* Replaces SyNImageRegistrationMethod::SetFixedToMiddleTransform +
+ * SetMovingToMiddleTransform when antsRegistration restores a saved state
+ * (--restore-state with --initialize-transforms-per-stage,
+ * Examples/itkantsRegistrationHelper.hxx:2091-2117): later runs start from
+ * these four fields on `grid`, resampled onto their level-0 grid (as fields
+ * are resampled between levels), instead of zero fields.  Device buffers of
+ * D x (grid voxels) floats each, copied before the call returns; all four
+ * NULL clears the state.  A run split into two stages this way equals the
+ * single stage. */
+int ants_syn_set_initial_state_device(ants_syn_handle* h, const ants_image_desc* grid, const float* d_f2m_fwd,
+                                      const float* d_f2m_inv, const float* d_m2m_fwd, const float* d_m2m_inv);
+
 /* Work done by the last run: iterations per level, inner inversion steps,
  * voxel-iterations and the algorithmic HBM byte count (per voxel-iteration
  * B_vi = 2*g*e_i + 14*D*e_f + sum_j (3*K_j + 1)*D*e_f, summed). */

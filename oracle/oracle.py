@@ -314,9 +314,11 @@ class SynResult:
 
 def syn_register(params: SynParams, gF: Grid, fixed, gM: Grid, moving, precision: str = "f64",
                  eps: float = FLT_EPSILON, want_state: bool = False, max_records: int = 4096,
-                 fixed_xforms=(), moving_xforms=()) -> SynResult:
+                 fixed_xforms=(), moving_xforms=(), restore=None) -> SynResult:
     """fixed_xforms / moving_xforms: initial transforms (see _xform_array),
-    in the order a point passes through them."""
+    in the order a point passes through them.  restore: (Grid, state) with
+    state (4, z, y, x, D) = F2M fwd, F2M inv, M2M fwd, M2M inv (FP64 only;
+    moving_xforms may be combined with it)."""
     L = lib()
     dt = np.float64 if precision == "f64" else np.float32
     ct = C.c_double if precision == "f64" else C.c_float
@@ -341,7 +343,22 @@ def syn_register(params: SynParams, gF: Grid, fixed, gM: Grid, moving, precision
     tr.inv_iters = ki.ctypes.data_as(C.POINTER(C.c_int))
     p = params.c()
     P = C.POINTER(ct)
-    if fixed_xforms or moving_xforms:
+    if restore is not None:
+        if precision != "f64" or fixed_xforms:
+            raise ValueError("restore: FP64 and moving initial transforms only")
+        gS, st = restore
+        st = np.ascontiguousarray(st, dtype=np.float64)
+        if st.shape != (4,) + gS.shape + (gS.dim,):
+            raise ValueError("state shape does not match its grid")
+        mx, hold_m = _xform_array(moving_xforms, gF.dim)
+        fn = L.og_syn_register_restore_f64
+        fn.argtypes = [C.POINTER(OgParams), C.POINTER(OgGrid), P, C.POINTER(OgGrid), P, C.POINTER(OgGrid), P,
+                       C.POINTER(OgXform), C.c_int, P, P, C.POINTER(OgTrace), P, C.c_double]
+        fn.restype = C.c_int
+        rc = fn(C.byref(p), C.byref(gF.c()), f.ctypes.data_as(P), C.byref(gM.c()), m.ctypes.data_as(P),
+                C.byref(gS.c()), st.ctypes.data_as(P), mx, len(moving_xforms), fwd.ctypes.data_as(P),
+                inv.ctypes.data_as(P), C.byref(tr), state.ctypes.data_as(P) if state is not None else None, eps)
+    elif fixed_xforms or moving_xforms:
         fx, hold_f = _xform_array(fixed_xforms, gF.dim)
         mx, hold_m = _xform_array(moving_xforms, gF.dim)
         fn = L.og_syn_register_init_f64 if precision == "f64" else L.og_syn_register_init_f32

@@ -188,7 +188,7 @@ double bessel_in(int n, double x) {
 // negligible or the half-width passes maxWidth; normalised to sum 1 and
 // mirrored.  Returns the half kernel c_0..c_R.  Variance t is in voxels
 // (SyN passes it unscaled; DiscreteGaussian divides by spacing^2 for mm).
-// Pinned: radius/values against scipy.special.ive (SURVEY table t=3,4,1,0.5).
+// Pinned: radius/values against scipy.special.ive (variances 3, 4, 1, 0.5).
 std::vector<double> gaussian_half_kernel(double variance, double max_error, int64_t max_width) {
   std::vector<double> c;
   const double et = std::exp(-variance);
@@ -796,7 +796,7 @@ void preprocess(int64_t N, const R* in, R* out) {
 template <class R>
 int syn_register(const og_params* P, const Grid& gF, const R* fixed_in, const Grid& gM, const R* moving_in, R* out_fwd,
                  R* out_inv, og_trace* trace, R* state4, double eps, const og_xform* fx = nullptr, int nfx = 0,
-                 const og_xform* mx = nullptr, int nmx = 0) {
+                 const og_xform* mx = nullptr, int nmx = 0, const Grid* gS = nullptr, const R* state_in = nullptr) {
   set_threads(P->threads);
   const int D = gF.D;
   const int64_t NF = gF.count(), NM = gM.count();
@@ -812,7 +812,17 @@ int syn_register(const og_params* P, const Grid& gF, const R* fixed_in, const Gr
   for (int l = 0; l < P->n_levels; ++l) {
     const Grid gl = shrink_grid(gF, P->shrink[l]);
     const int64_t N = gl.count();
-    if (l == 0) {
+    if (l == 0 && state_in) {
+      // restored state (FixedToMiddle fwd/inv, MovingToMiddle fwd/inv on grid
+      // gS): the level-0 parameter adaptor resamples all four fields onto the
+      // level grid, as it does between levels
+      const int64_t NS = gS->count();
+      std::vector<R>* dst[4] = {&f2m_f, &f2m_i, &m2m_f, &m2m_i};
+      for (int k = 0; k < 4; ++k) {
+        dst[k]->assign(N * D, R(0));
+        resample_field<R>(*gS, state_in + k * NS * D, gl, dst[k]->data());
+      }
+    } else if (l == 0) {
       f2m_f.assign(N * D, R(0));
       f2m_i.assign(N * D, R(0));
       m2m_f.assign(N * D, R(0));
@@ -1174,6 +1184,17 @@ int og_syn_register_init_f32(const og_params* P, const og_grid* gf, const float*
                              float* out_fwd, float* out_inv, og_trace* trace, float* state4, double eps) {
   return syn_register<float>(P, to_grid(gf), fixed, to_grid(gm), moving, out_fwd, out_inv, trace, state4, eps, fx, nfx,
                              mx, nmx);
+}
+
+// The SyN stage restored from a saved state: state4 holds FixedToMiddle
+// (fwd, inv) and MovingToMiddle (fwd, inv) on grid gs, one after another.
+int og_syn_register_restore_f64(const og_params* P, const og_grid* gf, const double* fixed, const og_grid* gm,
+                                const double* moving, const og_grid* gs, const double* state_in, const og_xform* mx,
+                                int nmx, double* out_fwd, double* out_inv, og_trace* trace, double* state4,
+                                double eps) {
+  const Grid g = to_grid(gs);
+  return syn_register<double>(P, to_grid(gf), fixed, to_grid(gm), moving, out_fwd, out_inv, trace, state4, eps, nullptr,
+                              0, mx, nmx, &g, state_in);
 }
 
 void og_warp_image_xf_f64(const og_grid* gi, const double* img, const og_grid* go, const double* u,

@@ -1,10 +1,12 @@
 """Whole SyN stage on the GPU against the FP64 oracle.
 
-Bars (stated in BASELINE.json's north star): per-iteration CC energy within
+Bars: per-iteration CC energy within
 1e-4 relative, forward and inverse fields within 0.05 voxel max-abs, and
 nearest-neighbour label warps bit-exact given the same field.  The full
 192x224x192 workload is checked through size-independent properties
 (inverse consistency, energy descent, determinism)."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -382,3 +384,61 @@ def test_initial_transform_errors():
     with pytest.raises(ValueError):
         reg.set_initial_transforms([AffineTransform(np.eye(3), [0, 0, 0])] * 9)
     reg.close()
+
+
+# ------------------------------------------------------------ restored state
+def _level_grid(reg, pair):
+    g = reg.level_geometry(pair.size, pair.spacing, pair.origin)
+    size = tuple(int(g.size[k]) for k in range(3))
+    return size, tuple(g.spacing[k] for k in range(3)), tuple(g.origin[k] for k in range(3))
+
+
+def test_restored_state_split_equals_single_stage():
+    """4x2 saving the state, then 1 restored from it, equals the 4x2x1 stage
+    bit for bit on the GPU; the restored part matches the oracle fed the
+    same state."""
+    pair = ph.phantom_pair_3d(12, size=(40, 44, 36), spacing=(1.1, 1.0, 1.2), max_disp_mm=4.0, svf_sigma_mm=6.0)
+    full_st = SyNStage.from_args("SyN[0.2,3,0]", "CC[f,m,1,4]", "[6x4x3,1e-6,10]", "4x2x1", "2x1x0vox")
+    a_st = SyNStage.from_args("SyN[0.2,3,0]", "CC[f,m,1,4]", "[6x4,1e-6,10]", "4x2", "2x1vox")
+    b_st = SyNStage.from_args("SyN[0.2,3,0]", "CC[f,m,1,4]", "[3,1e-6,10]", "1", "0vox")
+    full = run_pair(full_st, pair, force=True)
+    reg = SyNRegistration(a_st, 3, force_iterations=True)
+    reg.register(pair.fixed, pair.moving, pair.spacing, pair.origin, pair.spacing, pair.origin)
+    size, sp, og = _level_grid(reg, pair)
+    state = reg.state(tuple(reversed(size)) + (3,))
+    reg.close()
+    reg = SyNRegistration(b_st, 3, force_iterations=True)
+    reg.set_initial_state(state, size, sp, og)
+    b = reg.register(pair.fixed, pair.moving, pair.spacing, pair.origin, pair.spacing, pair.origin)
+    reg.close()
+    np.testing.assert_array_equal(b.energy, full.energy[-3:])
+    np.testing.assert_array_equal(b.fwd, full.fwd)
+    np.testing.assert_array_equal(b.inv, full.inv)
+    # the oracle restored from the same state
+    P = o.SynParams(shrink=[1], sigma=[0], iterations=[3], learning_rate=0.2, update_variance=3.0,
+                    total_variance=0.0, radius=4, force_iterations=True)
+    g = o.Grid(list(pair.size), list(pair.spacing), list(pair.origin))
+    ref = o.syn_register(P, g, pair.fixed, g, pair.moving, restore=(o.Grid(list(size), list(sp), list(og)),
+                                                                     np.stack(state)))
+    print("restored-state parity:", compare(b, ref, pair.spacing))
+
+
+def test_restored_state_errors_and_clearing():
+    pair = ph.phantom_pair_3d(13, size=(32, 36, 28), max_disp_mm=3.0, svf_sigma_mm=6.0)
+    st = SyNStage.from_args("SyN[0.2,3,0]", "CC[f,m,1,2]", "[4x2,1e-6,10]", "2x1", "1x0vox")
+    reg = SyNRegistration(st, 3, force_iterations=True)
+    plain = reg.register(pair.fixed, pair.moving, pair.spacing, pair.origin, pair.spacing, pair.origin)
+    shape = tuple(reversed(pair.size)) + (3,)
+    fields = [L.DeviceArray.from_host(np.zeros(shape, np.float32)) for _ in range(4)]
+    g = L.desc(pair.size, pair.spacing, pair.origin)
+    with pytest.raises(L.AntsError):
+        L.check(L.lib().ants_syn_set_initial_state_device(reg.handle, C.byref(g), fields[0].ptr, None,
+                                                          fields[2].ptr, fields[3].ptr))
+    # a zero state is the cold start; clearing returns to it as well
+    reg.set_initial_state([f.to_host() for f in fields], pair.size, pair.spacing, pair.origin)
+    z = reg.register(pair.fixed, pair.moving, pair.spacing, pair.origin, pair.spacing, pair.origin)
+    reg.set_initial_state(None)
+    c = reg.register(pair.fixed, pair.moving, pair.spacing, pair.origin, pair.spacing, pair.origin)
+    reg.close()
+    np.testing.assert_array_equal(z.fwd, plain.fwd)
+    np.testing.assert_array_equal(c.fwd, plain.fwd)

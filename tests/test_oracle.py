@@ -32,7 +32,7 @@ def test_gaussian_kernel_radius_and_values(t, eps, radius):
     assert k[0] + 2 * k[1:].sum() == pytest.approx(1.0, abs=1e-12)
 
 
-def test_gaussian_kernel_survey_values():
+def test_gaussian_kernel_tabulated_values():
     # e^-3 I_n(3) normalised, ITK GaussianOperator(variance 3, MaxError 0.001)
     k = o.gaussian_kernel(3.0, 0.001, 1000)
     np.testing.assert_allclose(k[:4], [0.243132, 0.196933, 0.111843, 0.047809], atol=2e-6)
@@ -524,3 +524,33 @@ def test_fixed_initial_transform_mirrors_the_moving_one():
     r3 = o.syn_register(P, g, F, g, M, fixed_xforms=[("identity",)], moving_xforms=[("identity",)])
     r4 = o.syn_register(P, g, F, g, M)
     np.testing.assert_array_equal(r3.fwd, r4.fwd)
+
+
+# ------------------------------------------------------------ restored state
+def test_restored_state_continues_the_stage_exactly():
+    """A stage split in two -- levels 4x2 saving the state, then level 1
+    restored from it -- is the same computation as the single 4x2x1 stage:
+    the restored fields are resampled onto the new level-0 grid as the
+    single run resamples them between levels.  Energies and fields equal
+    bit for bit; without the restore the second part differs."""
+    D = 3
+    size, F, M = _pair(D, 14)
+    g = o.Grid(list(size), [1.1, 1.0, 1.2], [2.0, -1.0, 0.5])
+    kw = dict(learning_rate=0.2, update_variance=3.0, total_variance=0.0, radius=2, force_iterations=True)
+    full = o.syn_register(o.SynParams(shrink=[4, 2, 1], sigma=[2, 1, 0], iterations=[4, 3, 2], **kw), g, F, g, M)
+    a = o.syn_register(o.SynParams(shrink=[4, 2], sigma=[2, 1], iterations=[4, 3], **kw), g, F, g, M,
+                       want_state=True)
+    gS = o.shrink_grid(g, 2)
+    P1 = o.SynParams(shrink=[1], sigma=[0], iterations=[2], **kw)
+    b = o.syn_register(P1, g, F, g, M, restore=(gS, a.state))
+    np.testing.assert_array_equal(b.energy, full.energy[-2:])
+    np.testing.assert_array_equal(b.fwd, full.fwd)
+    np.testing.assert_array_equal(b.inv, full.inv)
+    cold = o.syn_register(P1, g, F, g, M)
+    assert np.abs(cold.fwd - b.fwd).max() > 0.05
+    # a state already on the level grid is taken as it is (resampling at grid
+    # points, up to the rounding of the point-to-index map)
+    st = o.syn_register(P1, g, F, g, M, restore=(gS, a.state), want_state=True).state
+    c = o.syn_register(o.SynParams(shrink=[1], sigma=[0], iterations=[0], **kw), g, F, g, M, restore=(g, st),
+                       want_state=True)
+    np.testing.assert_allclose(c.state, st, rtol=0, atol=1e-12)
