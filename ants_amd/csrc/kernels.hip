@@ -1849,14 +1849,22 @@ constexpr int kCCTX = ANTS_CC_TX, kCCTY = ANTS_CC_TY;
 
 // Blocks of a kernel resident on the whole device at once: blocks per CU
 // (from its register and LDS use) times the CU count.
-int64_t device_slots(const void* kernel, int threads) {
+// `per_cu` > 0 overrides the occupancy query.
+int64_t device_slots(const void* kernel, int threads, int per_cu = 0) {
   int dev = 0, cus = 0, per = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
     cus = 256;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0) != hipSuccess || per < 1) per = 1;
+  if (per_cu > 0) per = per_cu;
   return static_cast<int64_t>(cus) * per;
 }
+#ifndef ANTS_CC_SLOTS_PER_CU
+#define ANTS_CC_SLOTS_PER_CU 0
+#endif
+#ifndef ANTS_SMOOTH_SLOTS_PER_CU
+#define ANTS_SMOOTH_SLOTS_PER_CU 0
+#endif
 
 // z-chunk depth of a kernel that streams `columns` block columns along z,
 // reading `halo` warm-up planes per chunk: the depth that minimises
@@ -1882,10 +1890,10 @@ int zchunk_for(int nz, int64_t columns, int halo, int64_t slots) {
 int cc_zchunk(const GridDev& g, int radius) {
   const int64_t tiles = static_cast<int64_t>((g.n[0] + kCCTX - 1) / kCCTX) * ((g.n[1] + kCCTY - 1) / kCCTY);
 #if ANTS_ZCHUNK_MODEL
-  static const int64_t slots4 =
-      device_slots(reinterpret_cast<const void*>(&k_cc3d<4, kCCTX, kCCTY>), kCCTX * kCCTY);
-  static const int64_t slots2 =
-      device_slots(reinterpret_cast<const void*>(&k_cc3d<2, kCCTX, kCCTY>), kCCTX * kCCTY);
+  static const int64_t slots4 = device_slots(reinterpret_cast<const void*>(&k_cc3d<4, kCCTX, kCCTY>),
+                                             kCCTX * kCCTY, ANTS_CC_SLOTS_PER_CU);
+  static const int64_t slots2 = device_slots(reinterpret_cast<const void*>(&k_cc3d<2, kCCTX, kCCTY>),
+                                             kCCTX * kCCTY, ANTS_CC_SLOTS_PER_CU);
   return zchunk_for(g.n[2], tiles, 2 * radius, radius == 4 ? slots4 : slots2);
 #else
   (void)radius;
@@ -1941,7 +1949,8 @@ bool launch_smooth3d(const GridDev& g, int np, const float* const* in, const flo
   if (static_cast<int64_t>(g.n[0]) * g.n[1] * 3 >= (int64_t{1} << 31)) return false;  // 32-bit in-plane offsets
   const int64_t tiles = static_cast<int64_t>((g.n[0] + TX - 1) / TX) * ((g.n[1] + TY - 1) / TY) * np;
 #if ANTS_ZCHUNK_MODEL
-  static const int64_t slots = device_slots(reinterpret_cast<const void*>(&k_smooth3d<R, TX, TY>), TX * TY);
+  static const int64_t slots =
+      device_slots(reinterpret_cast<const void*>(&k_smooth3d<R, TX, TY>), TX * TY, ANTS_SMOOTH_SLOTS_PER_CU);
   const int64_t zc = zchunk_for(g.n[2], tiles, 2 * R, slots);
 #else
   int64_t zc = (static_cast<int64_t>(g.n[2]) * tiles) / ANTS_SMOOTH_TARGET_BLOCKS;
