@@ -586,6 +586,14 @@ constexpr int kSmoothTX = ANTS_SMOOTH_TX, kSmoothTY = ANTS_SMOOTH_TY;
 #define ANTS_SMOOTH_WAVES 4
 #endif
 constexpr int kSmoothWavesPerEU = ANTS_SMOOTH_WAVES;
+// y pass: 0 = each thread convolves its own column (2R+1 LDS reads per
+// output and component); S > 0 = a task convolves S rows of one column and
+// component from S + 2R reads into a staging buffer (same arithmetic, same
+// bits).
+#ifndef ANTS_SMOOTH_YSEG
+#define ANTS_SMOOTH_YSEG 0
+#endif
+constexpr int kSmoothYSeg = ANTS_SMOOTH_YSEG;
 // NC interleaved components (3: a displacement field, 1: an image); with
 // maxbits null no norm is taken.
 template <int R, int TX, int TY, int NC = 3>
@@ -600,6 +608,7 @@ __global__ void __launch_bounds__(TX* TY, kSmoothWavesPerEU) k_smooth3d(GridDev 
   constexpr int KP = (NE + NT - 1) / NT;
   __shared__ __align__(16) float tile[NC][LY][LXP];
   __shared__ __align__(16) float xs[NC][LY][TX];
+  __shared__ float ysg[kSmoothYSeg > 0 ? NC : 1][kSmoothYSeg > 0 ? TY : 1][TX];
   const int f = blockIdx.z / nzc, zb = blockIdx.z - f * nzc;
   const float* __restrict__ src = in.p[f];
   const float* __restrict__ org = orig.p[f];
@@ -699,12 +708,37 @@ __global__ void __launch_bounds__(TX* TY, kSmoothWavesPerEU) k_smooth3d(GridDev 
     }
     __syncthreads();
     float p[NC];
+    if constexpr (kSmoothYSeg > 0) {
+      constexpr int S = kSmoothYSeg;
+      static_assert(TY % S == 0, "y segments tile the block's rows");
+      constexpr int NTASK = NC * TX * (TY / S);
+      for (int task = threadIdx.x; task < NTASK; task += NT) {
+        const int a = task / (TX * (TY / S));
+        const int r = task - a * (TX * (TY / S));
+        const int k = r / TX, cx = r - k * TX;
+        const int yr = k * S;
+        float w[S + 2 * R];
 #pragma unroll
-    for (int a = 0; a < NC; ++a) {
-      float acc = c[0] * xs[a][ty + R][tx];
+        for (int h = 0; h < S + 2 * R; ++h) w[h] = xs[a][yr + h][cx];
 #pragma unroll
-      for (int b = 1; b <= R; ++b) acc += c[b] * (xs[a][ty + R - b][tx] + xs[a][ty + R + b][tx]);
-      p[a] = acc;
+        for (int e = 0; e < S; ++e) {
+          float acc = c[0] * w[e + R];
+#pragma unroll
+          for (int b = 1; b <= R; ++b) acc += c[b] * (w[e + R - b] + w[e + R + b]);
+          ysg[a][yr + e][cx] = acc;
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int a = 0; a < NC; ++a) p[a] = ysg[a][ty][tx];
+    } else {
+#pragma unroll
+      for (int a = 0; a < NC; ++a) {
+        float acc = c[0] * xs[a][ty + R][tx];
+#pragma unroll
+        for (int b = 1; b <= R; ++b) acc += c[b] * (xs[a][ty + R - b][tx] + xs[a][ty + R + b][tx]);
+        p[a] = acc;
+      }
     }
     // ring slot of plane zz - R + k, k = -R .. R (k = R is plane zz)
     auto slot = [](int k) { return kSmoothRotate ? (ph + W + k - R) % W : R + k; };
@@ -981,6 +1015,18 @@ constexpr bool kCCRotate = ANTS_CC_ROTATE != 0;
 #define ANTS_CC_WAVES 1
 #endif
 constexpr int kCCWavesPerEU = ANTS_CC_WAVES;  // occupancy floor (waves per SIMD)
+// y box sums: 0 = each thread sums its W rows (5 W LDS reads per voxel);
+// S > 0 = a task walks S output rows of one column and quantity with a
+// running sum (W + 2 (S - 1) reads per S outputs) into a staging buffer.
+#ifndef ANTS_CC_YSEG
+#define ANTS_CC_YSEG 8
+#endif
+constexpr int kCCYSeg = ANTS_CC_YSEG;
+// x box sums: outputs per task (running sum along the row).
+#ifndef ANTS_CC_XSEG
+#define ANTS_CC_XSEG 2
+#endif
+constexpr int kCCXSeg = ANTS_CC_XSEG;
 template <int R, int TX, int TY>
 __global__ void __launch_bounds__(TX* TY, kCCWavesPerEU) k_cc3d(GridDev g, const float* __restrict__ F, const float* __restrict__ M,
                                                  float eps, int zchunk, float* __restrict__ g1,
@@ -990,6 +1036,8 @@ __global__ void __launch_bounds__(TX* TY, kCCWavesPerEU) k_cc3d(GridDev g, const
   constexpr int NT = TX * TY;
   __shared__ float q[2][LY][LX];
   __shared__ double sx[5][LY][TX];
+  constexpr int SYR = kCCYSeg > 0 ? TY : 1;
+  __shared__ double sy[kCCYSeg > 0 ? 5 : 1][SYR][TX];
   const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
   const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
   const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, z0 = blockIdx.z * zchunk;
@@ -1067,33 +1115,42 @@ __global__ void __launch_bounds__(TX* TY, kCCWavesPerEU) k_cc3d(GridDev g, const
     }
     __syncthreads();
     if (zz + 1 < zend) fetch(zz + 1);
-    // x box sums, two adjacent outputs per task: the window of output
-    // x2 + 1 is that of x2 with the first sample dropped and one added
-    static_assert(TX % 2 == 0, "x sums run in pairs");
-    for (int task = threadIdx.x; task < LY * (TX / 2); task += NT) {
-      const int ly = task / (TX / 2), x2 = 2 * (task - ly * (TX / 2));
+    // x box sums, XS adjacent outputs per task: the window of output
+    // xs + e is that of xs + e - 1 with its first sample dropped and one added
+    constexpr int XS = kCCXSeg;
+    static_assert(TX % XS == 0 && XS >= 2, "x sums run in segments");
+    for (int task = threadIdx.x; task < LY * (TX / XS); task += NT) {
+      const int ly = task / (TX / XS), xs = XS * (task - ly * (TX / XS));
       double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;
 #pragma unroll
       for (int b = 0; b < W; ++b) {
-        const double f = q[0][ly][x2 + b], m = q[1][ly][x2 + b];
+        const double f = q[0][ly][xs + b], m = q[1][ly][xs + b];
         s0 += f;
         s1 += m;
         s2 += f * f;
         s3 += m * m;
         s4 += f * m;
       }
-      sx[0][ly][x2] = s0;
-      sx[1][ly][x2] = s1;
-      sx[2][ly][x2] = s2;
-      sx[3][ly][x2] = s3;
-      sx[4][ly][x2] = s4;
-      const double fo = q[0][ly][x2], mo = q[1][ly][x2];
-      const double fn = q[0][ly][x2 + W], mn = q[1][ly][x2 + W];
-      sx[0][ly][x2 + 1] = s0 + (fn - fo);
-      sx[1][ly][x2 + 1] = s1 + (mn - mo);
-      sx[2][ly][x2 + 1] = s2 + (fn * fn - fo * fo);
-      sx[3][ly][x2 + 1] = s3 + (mn * mn - mo * mo);
-      sx[4][ly][x2 + 1] = s4 + (fn * mn - fo * mo);
+      sx[0][ly][xs] = s0;
+      sx[1][ly][xs] = s1;
+      sx[2][ly][xs] = s2;
+      sx[3][ly][xs] = s3;
+      sx[4][ly][xs] = s4;
+#pragma unroll
+      for (int e = 1; e < XS; ++e) {
+        const double fo = q[0][ly][xs + e - 1], mo = q[1][ly][xs + e - 1];
+        const double fn = q[0][ly][xs + e - 1 + W], mn = q[1][ly][xs + e - 1 + W];
+        s0 += fn - fo;
+        s1 += mn - mo;
+        s2 += fn * fn - fo * fo;
+        s3 += mn * mn - mo * mo;
+        s4 += fn * mn - fo * mo;
+        sx[0][ly][xs + e] = s0;
+        sx[1][ly][xs + e] = s1;
+        sx[2][ly][xs + e] = s2;
+        sx[3][ly][xs + e] = s3;
+        sx[4][ly][xs + e] = s4;
+      }
     }
     // centre values and in-plane gradients of plane zz at (x, y)
     const float fc = q[0][ty + R][tx + R], mc = q[1][ty + R][tx + R];
@@ -1106,12 +1163,36 @@ __global__ void __launch_bounds__(TX* TY, kCCWavesPerEU) k_cc3d(GridDev g, const
     }
     __syncthreads();
     double p[5];
+    if constexpr (kCCYSeg > 0) {
+      constexpr int S = kCCYSeg;
+      static_assert(TY % S == 0, "y segments tile the block's rows");
+      constexpr int NTASK = 5 * TX * (TY / S);
+      for (int task = threadIdx.x; task < NTASK; task += NT) {
+        const int a = task / (TX * (TY / S));
+        const int r = task - a * (TX * (TY / S));
+        const int k = r / TX, cx = r - k * TX;
+        const int yr = k * S;
+        double acc = 0.0;
 #pragma unroll
-    for (int a = 0; a < 5; ++a) {
-      double t = 0.0;
+        for (int b = 0; b < W; ++b) acc += sx[a][yr + b][cx];
+        sy[a][yr][cx] = acc;
 #pragma unroll
-      for (int b = 0; b < W; ++b) t += sx[a][ty + b][tx];
-      p[a] = t;
+        for (int e = 1; e < S; ++e) {
+          acc += sx[a][yr + e + W - 1][cx] - sx[a][yr + e - 1][cx];
+          sy[a][yr + e][cx] = acc;
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int a = 0; a < 5; ++a) p[a] = sy[a][ty][tx];
+    } else {
+#pragma unroll
+      for (int a = 0; a < 5; ++a) {
+        double t = 0.0;
+#pragma unroll
+        for (int b = 0; b < W; ++b) t += sx[a][ty + b][tx];
+        p[a] = t;
+      }
     }
     // running z-window sums: add the new plane, drop the one W planes back;
     // slots of plane zc = zz - R and of its z neighbours
