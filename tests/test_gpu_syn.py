@@ -442,3 +442,44 @@ def test_restored_state_errors_and_clearing():
     reg.close()
     np.testing.assert_array_equal(z.fwd, plain.fwd)
     np.testing.assert_array_equal(c.fwd, plain.fwd)
+
+
+# ------------------------------------------------------------ edge cases
+def _edge_case(case):
+    import dataclasses
+    if case == "ragged3d":
+        # sizes that fill no tile evenly; the coarsest grid (9x7x5) is
+        # smaller than the CC window and the smoothing kernels
+        pair = ph.phantom_pair_3d(14, size=(37, 29, 23), spacing=(1.3, 1.0, 1.1), max_disp_mm=3.0,
+                                  svf_sigma_mm=6.0)
+        st = SyNStage.from_args("SyN[0.2,3,0]", "CC[f,m,1,4]", "[6x4x2,1e-6,10]", "4x2x1", "2x1x0vox")
+        return pair, st, True
+    if case == "zero_iteration_level":
+        pair = ph.phantom_pair_3d(15, size=(40, 36, 32), max_disp_mm=3.0, svf_sigma_mm=6.0)
+        st = SyNStage.from_args("SyN[0.2,3,0]", "CC[f,m,1,2]", "[6x0x3,1e-6,10]", "4x2x1", "2x1x0vox")
+        return pair, st, True
+    if case == "constant_moving":
+        # every window is degenerate: CC 1, zero derivatives, zero fields,
+        # and the energy window converges after 10 iterations per level
+        pair = ph.phantom_pair_3d(16, size=(32, 28, 24), max_disp_mm=3.0, svf_sigma_mm=6.0)
+        pair = dataclasses.replace(pair, moving=np.full_like(pair.moving, 0.3))
+        st = SyNStage.from_args("SyN[0.2,3,0]", "CC[f,m,1,4]", "[20x20x20,1e-6,10]", "4x2x1", "2x1x0vox")
+        return pair, st, False
+    if case == "small2d":
+        pair = ph.blob_pair_2d(2, n=40, n_blobs=6, max_disp=2.0)
+        st = SyNStage.from_args("SyN[0.25,3,0]", "CC[f,m,1,4]", "[10x6x4,1e-6,10]", "4x2x1", "2x1x0vox")
+        return pair, st, True
+    raise ValueError(case)
+
+
+@pytest.mark.parametrize("case", ["ragged3d", "zero_iteration_level", "constant_moving", "small2d"])
+def test_syn_edge_cases_vs_oracle(case):
+    pair, stage, force = _edge_case(case)
+    res = run_pair(stage, pair, force=force)
+    ref = run_oracle(stage, pair, force=force)
+    print(f"{case} parity:", compare(res, ref, pair.spacing))
+    if case == "constant_moving":
+        assert np.all(res.energy == -1.0) and not res.fwd.any() and not res.inv.any()
+        assert list(np.bincount(res.levels, minlength=3)) == [10, 10, 10]
+    if case == "zero_iteration_level":
+        assert 1 not in set(res.levels.tolist())
