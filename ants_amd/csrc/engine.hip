@@ -700,6 +700,12 @@ int smooth_image_on(hipStream_t s, const HostGrid& g, const float* in, float* ou
 
 // SyN GaussianSmoothDisplacementField for np fields; maxbits (optional)
 // receives each result's max |u/spacing|.  work/out may alias in.
+// Grids below this size take the separable passes instead of the fused one.
+#ifndef ANTS_SMOOTH_FUSED_MIN_VOXELS
+#define ANTS_SMOOTH_FUSED_MIN_VOXELS 0
+#endif
+constexpr int64_t kSmoothFusedMinVoxels = ANTS_SMOOTH_FUSED_MIN_VOXELS;
+
 int smooth_fields_on(hipStream_t s, const HostGrid& g, int np, float* const* in, float* const* work,
                      float* const* out, double variance, unsigned int* maxbits, Profiler* prof = nullptr) {
   const GridDev gd = to_dev(g);
@@ -718,7 +724,7 @@ int smooth_fields_on(hipStream_t s, const HostGrid& g, int np, float* const* in,
   }
   // one fused pass when the kernel radius and dimension allow (taps per axis
   // differ only through MaxWidth = size[d], checked below)
-  if (D == 3) {
+  if (D == 3 && g.count() >= kSmoothFusedMinVoxels) {
     Taps tx, ty, tz;
     ANTS_TRY(gaussian_taps(variance, 0.001, g.n[0], &tx));
     ANTS_TRY(gaussian_taps(variance, 0.001, g.n[1], &ty));
