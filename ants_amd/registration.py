@@ -239,8 +239,11 @@ class SyNRegistration:
         g = L.desc(size, spacing, origin, direction)
         L.check(L.lib().ants_syn_set_initial_state_device(self.handle, C.byref(g), *[b.ptr for b in bufs]))
 
-    def set_profiling(self, mask: int):
+    def set_profiling(self, mask: int, every: int = 1):
+        """Time the kernel classes in `mask` on every `every`-th iteration of
+        each level (ants_syn_set_profiling / _stride)."""
         L.check(L.lib().ants_syn_set_profiling(self.handle, mask))
+        L.check(L.lib().ants_syn_set_profiling_stride(self.handle, every))
 
     def kernel_stats(self) -> dict:
         k = L.KernelStatsC()

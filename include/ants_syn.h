@@ -193,6 +193,11 @@ typedef struct ants_syn_kernel_stats {
   double bytes[ANTS_KC_COUNT];     /* algorithmic HBM bytes of those launches */
 } ants_syn_kernel_stats;
 int ants_syn_set_profiling(ants_syn_handle* h, unsigned int class_mask);
+/* Time only iterations 1, 1 + every, 1 + 2·every, ... of each level (default
+ * 1: all).  Each timing span records two events, and an event between two
+ * launches keeps the second from starting until the first has finished:
+ * sampling keeps most iterations free of that stall. */
+int ants_syn_set_profiling_stride(ants_syn_handle* h, int every);
 int ants_syn_get_kernel_stats(const ants_syn_handle* h, ants_syn_kernel_stats* out);
 
 /* ---- building blocks (device buffers) ----
