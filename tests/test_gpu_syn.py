@@ -248,6 +248,33 @@ def test_trace_and_observer():
     reg.close()
 
 
+def test_profiling_stride_and_launch_modes(monkeypatch):
+    """Timing spans on every iteration or every 4th, and graph replay or
+    kernel-by-kernel launches, leave the results bit for bit; the stride
+    times the first iteration of each level and every 4th after it."""
+    pair = ph.phantom_pair_3d(3, size=(48, 56, 40))
+    stage = SyNStage.from_args("SyN[0.2,3,0]", "CC[f,m,1,2]", "[9x6,1e-6,10]", "2x1", "1x0vox")
+    inv = 1 << L.KC_NAMES.index("inv")
+    out = {}
+    for name, mask, every, graphs in (("all", inv, 1, "0"), ("stride", inv, 4, "0"), ("none", 0, 1, "0"),
+                                      ("graphs", 0, 1, "1")):
+        monkeypatch.setenv("ANTS_SYN_GRAPHS", graphs)
+        reg = SyNRegistration(stage, 3, force_iterations=True)
+        reg.set_profiling(mask, every=every)
+        res = reg.register(pair.fixed, pair.moving, pair.spacing, pair.origin, pair.spacing, pair.origin)
+        out[name] = (res, reg.kernel_stats()["inv"], reg.stats())
+        reg.close()
+    ref = out["none"][0]
+    for name in ("all", "stride", "graphs"):
+        np.testing.assert_array_equal(out[name][0].fwd, ref.fwd)
+        np.testing.assert_array_equal(out[name][0].energy, ref.energy)
+    assert out["graphs"][2]["graph_launches"] == 15 and out["none"][2]["eager_iterations"] == 15
+    # spans: one per timed iteration (plus host tails); iterations 1, 5, 9 of 9 and 1, 5 of 6 with stride 4
+    n_all, n_stride = out["all"][1]["launches"], out["stride"][1]["launches"]
+    assert 0 < n_stride < n_all
+    assert out["none"][1]["launches"] == 0
+
+
 def test_speculative_inversion_is_exact(monkeypatch):
     """The inversion's speculative updates (large grids) and the plain
     recompute-in-the-next-launch scheme give the same bits: forced on and
