@@ -38,6 +38,21 @@ def test_gaussian_kernel_tabulated_values():
     np.testing.assert_allclose(k[:4], [0.243132, 0.196933, 0.111843, 0.047809], atol=2e-6)
 
 
+def test_gaussian_kernels_match_golden_file():
+    """tests/golden/gaussian_kernels.json (written by make_gaussian_kernels.py
+    from scipy's e^-t I_n(t)): the oracle's kernels have the same radius and
+    values for every variance / maximum error the stage uses."""
+    import json
+    import os
+    path = os.path.join(os.path.dirname(__file__), "golden", "gaussian_kernels.json")
+    cases = json.load(open(path))["kernels"]
+    assert len(cases) >= 6
+    for c in cases:
+        k = o.gaussian_kernel(c["variance"], c["max_error"], 1000)
+        assert len(k) - 1 == c["radius"], c
+        np.testing.assert_allclose(k, c["half_kernel"], rtol=1e-6, atol=1e-9)
+
+
 def test_gaussian_kernel_width_cap_and_zero():
     assert len(o.gaussian_kernel(0.0, 0.01, 32)) == 2  # identity: [1, 0]
     assert o.gaussian_kernel(0.0, 0.01, 32)[0] == 1.0
