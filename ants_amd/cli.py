@@ -90,6 +90,8 @@ def cmd_registration(argv: Sequence[str]) -> int:
     ap.add_argument("-o", "--output", required=True)
     ap.add_argument("-r", "--initial-moving-transform", action="append", default=[],
                     help="affine transform file or [file,useInverse]; repeated options compose in -t order")
+    ap.add_argument("-q", "--initial-fixed-transform", action="append", default=[],
+                    help="affine transform file or [file,useInverse] applied to the fixed image's points")
     ap.add_argument("--float", default="1")
     ap.add_argument("--precision", choices=["fp32", "bf16"], default="fp32")
     ap.add_argument("-v", "--verbose", type=int, default=0)
@@ -103,15 +105,22 @@ def cmd_registration(argv: Sequence[str]) -> int:
             raise SystemExit(f"{name} is {g.dim}-D, -d {a.dimensionality}")
     outs = _bracket_list(a.output)
     prefix = outs[0]
-    init = []
-    for spec in a.initial_moving_transform:
-        t = load_transform(spec)
-        if isinstance(t, DisplacementFieldTransform):
-            raise SystemExit(f"{spec}: only linear initial transforms are supported")
-        if t is not None:
-            init.append(t)
+    def linear(specs):
+        out = []
+        for spec in specs:
+            t = load_transform(spec)
+            if isinstance(t, DisplacementFieldTransform):
+                raise SystemExit(f"{spec}: only linear initial transforms are supported")
+            if t is not None:
+                out.append(t)
+        return out
+
+    init = linear(a.initial_moving_transform)
+    finit = linear(a.initial_fixed_transform)
+    if finit and (len(outs) > 1 and any(outs[1:])):
+        raise SystemExit("-q with warped-image outputs is not supported (write the warp files only)")
     reg = SyNRegistration(stage, a.dimensionality, precision=a.precision)
-    reg.set_initial_transforms(init)
+    reg.set_initial_transforms(init, finit)
     res = reg.register(np.asarray(fixed, np.float32), np.asarray(moving, np.float32), gf.spacing, gf.origin,
                        gm.spacing, gm.origin, gf.direction, gm.direction)
     lg = reg.level_geometry(gf.size, gf.spacing, gf.origin, direction=gf.direction)  # the fields' grid

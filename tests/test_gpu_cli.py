@@ -83,3 +83,27 @@ def test_registration_with_initial_moving_transform(tmp_path):
     warped, _ = aio.read_nifti(prefix + "Warped.nii.gz")
     np.testing.assert_array_equal(warped, apply_transforms(g, pair.moving, g,
                                                            [DisplacementFieldTransform(fwd, g), aff]))
+
+
+def test_registration_with_initial_fixed_transform(tmp_path):
+    """-q affine.mat: the fixed image's points go through the transform
+    (SetFixedInitialTransform); the warp files equal the library's."""
+    from ants_amd.apply_transforms import AffineTransform
+    pair = ph.phantom_pair_3d(5, size=(36, 40, 32), max_disp_mm=3.0, svf_sigma_mm=6.0)
+    g = Geometry(pair.size, pair.spacing, (0.0, 0.0, 0.0), np.eye(3))
+    f, m = str(tmp_path / "fixed.nii.gz"), str(tmp_path / "moving.nii.gz")
+    aio.write_nifti(f, pair.fixed, g)
+    aio.write_nifti(m, pair.moving, g)
+    aff = AffineTransform(np.diag([1.02, 0.99, 1.0]), [0.3, -0.2, 0.1], [18.0, 20.0, 16.0])
+    aio.write_transform(str(tmp_path / "q.mat"), aff)
+    prefix = str(tmp_path / "out_")
+    args = ["-d", "3", "-m", f"CC[{f},{m},1,2]", "-t", "SyN[0.2,3,0]", "-c", "[4x2,1e-6,10]", "-f", "2x1",
+            "-s", "1x0vox", "-q", str(tmp_path / "q.mat"), "-o", f"[{prefix}]"]
+    assert cli.main(["registration"] + args) == 0
+    fwd, _ = aio.read_nifti(prefix + "0Warp.nii.gz")
+    st = SyNStage.from_args("SyN[0.2,3,0]", "CC[f,m,1,2]", "[4x2,1e-6,10]", "2x1", "1x0vox")
+    reg = SyNRegistration(st, 3)
+    reg.set_initial_transforms([], [aff])
+    res = reg.register(pair.fixed, pair.moving, g.spacing, g.origin, g.spacing, g.origin)
+    reg.close()
+    np.testing.assert_array_equal(fwd, res.fwd)
