@@ -12,8 +12,8 @@ import os
 import numpy as np
 from scipy import special
 
-# (variance t, maximum error); the first three radii (6, 5, 3) are also the
-# values stated for ITK's operator in the survey of the reference
+# (variance t, maximum error): the image pyramid's and the field smoothing's
+# error bounds at the variances the stage's schedules use
 CASES = [(3.0, 0.001), (4.0, 0.01), (1.0, 0.01), (0.5, 0.001), (1.0, 0.001), (0.25, 0.01)]
 
 
