@@ -144,6 +144,7 @@ def lib():
     L.ants_op_compose.argtypes = [pdesc, vp, vp, vp, vp]
     L.ants_op_scale_update.argtypes = [pdesc, vp, d, C.POINTER(d), vp]
     L.ants_op_invert.argtypes = [pdesc, vp, vp, vp, C.POINTER(i), C.POINTER(d), C.POINTER(d), vp]
+    L.ants_op_invert_noalloc.argtypes = [pdesc, vp, vp, vp, C.POINTER(i), C.POINTER(d), C.POINTER(d), vp]
     L.ants_op_resample_field.argtypes = [pdesc, vp, pdesc, vp, vp]
     L.ants_op_preprocess.argtypes = [C.c_int64, vp, vp, vp]
     L.ants_apply_transform_labels_nn.argtypes = [pdesc, vp, pdesc, vp, vp, vp]

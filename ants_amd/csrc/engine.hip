@@ -1539,33 +1539,32 @@ int ants_op_scale_update(const ants_image_desc* gdsc, float* d_u, double rate, d
   return ANTS_OK;
 }
 
-int ants_op_invert(const ants_image_desc* gdsc, const float* d_field, const float* d_init, float* d_inv,
-                   int* host_iters, double* host_max, double* host_mean, void* stream) {
-  HostGrid g;
-  ANTS_TRY(to_host_grid(gdsc, &g));
+}  // extern "C"
+
+namespace {
+
+// The whole InvertDisplacementFieldImageFilter run on caller-provided
+// scratch: step slots (kInvSlots RedSlots, zeroed here), per-block partials
+// (2 step parities) and candidate masks (2 step parities).
+int invert_on(hipStream_t s, const HostGrid& g, const float* d_field, const float* d_init, float* d_inv,
+              RedSlot* d_slots, Partial* d_parts, uint64_t* d_masks, int* host_iters, double* host_max,
+              double* host_mean) {
+  if (!d_field || !d_inv) return fail(ANTS_ERR_INVALID, "null field buffer");
   const GridDev gd = to_dev(g);
   const size_t b = static_cast<size_t>(g.count()) * g.D * sizeof(float);
-  OpScratch sc;
-  ANTS_TRY(sc.stats.ensure(sizeof(RedSlot) * kInvSlots));
-  ANTS_TRY(sc.partials.ensure(2 * inv_partials_needed(gd) * sizeof(Partial)));  // 2 step parities
-  DevBuf masks;
-  ANTS_TRY(masks.ensure(2 * inv_mask_words(gd) * sizeof(uint64_t)));
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  HIP_TRY(hipMemsetAsync(sc.stats.p, 0, sizeof(RedSlot) * kInvSlots, s));
+  HIP_TRY(hipMemsetAsync(d_slots, 0, sizeof(RedSlot) * kInvSlots, s));
   if (d_init)
     HIP_TRY(hipMemcpyAsync(d_inv, d_init, b, hipMemcpyDeviceToDevice, s));
   else
     HIP_TRY(hipMemsetAsync(d_inv, 0, b, s));
   const float* field[1] = {d_field};
   float* inv[1] = {d_inv};
-  RedSlot* slots[1] = {static_cast<RedSlot*>(sc.stats.p)};
-  Partial* parts = static_cast<Partial*>(sc.partials.p);
-  uint64_t* mk = static_cast<uint64_t*>(masks.p);
-  launch_inv_step(gd, 1, field, inv, slots, parts, mk, -1, s);
-  for (int j = 0; j <= kInvMaxIter; ++j) launch_inv_step(gd, 1, field, inv, slots, parts, mk, j, s);
+  RedSlot* slots[1] = {d_slots};
+  launch_inv_step(gd, 1, field, inv, slots, d_parts, d_masks, -1, s);
+  for (int j = 0; j <= kInvMaxIter; ++j) launch_inv_step(gd, 1, field, inv, slots, d_parts, d_masks, j, s);
   HIP_TRY(hipGetLastError());
   RedSlot r[kInvSlots];
-  HIP_TRY(hipMemcpyAsync(r, sc.stats.p, sizeof(r), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(r, d_slots, sizeof(r), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   int k = 0;
   for (int j = 0; j < kInvSlots; ++j)
@@ -1578,6 +1577,76 @@ int ants_op_invert(const ants_image_desc* gdsc, const float* d_field, const floa
     if (host_mean) *host_mean = r[k - 1].sum / static_cast<double>(g.count());
   }
   return ANTS_OK;
+}
+
+// Scratch of ants_op_invert_noalloc: a file-scope device array, one copy per
+// GPU, held until the process ends.  Host code never passes the array's name
+// to a kernel or copy (on the host it is only a stand-in for the symbol);
+// it uses the per-device address that hipGetSymbolAddress returns.
+constexpr size_t kNoAllocScratchBytes = size_t{8} << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Address of the current device's scratch, looked up once per device.
+int scratch_of_current_device(void** out) {
+  int dev = -1;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(ANTS_ERR_UNSUPPORTED, "device index %d out of range", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(ANTS_ERR_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return ANTS_OK;
+}
+
+size_t align256(size_t v) { return (v + 255) / 256 * 256; }
+
+}  // namespace
+
+extern "C" {
+
+int ants_op_invert(const ants_image_desc* gdsc, const float* d_field, const float* d_init, float* d_inv,
+                   int* host_iters, double* host_max, double* host_mean, void* stream) {
+  HostGrid g;
+  ANTS_TRY(to_host_grid(gdsc, &g));
+  const GridDev gd = to_dev(g);
+  OpScratch sc;
+  ANTS_TRY(sc.stats.ensure(sizeof(RedSlot) * kInvSlots));
+  ANTS_TRY(sc.partials.ensure(2 * inv_partials_needed(gd) * sizeof(Partial)));  // 2 step parities
+  DevBuf masks;
+  ANTS_TRY(masks.ensure(2 * inv_mask_words(gd) * sizeof(uint64_t)));
+  return invert_on(static_cast<hipStream_t>(stream), g, d_field, d_init, d_inv, static_cast<RedSlot*>(sc.stats.p),
+                   static_cast<Partial*>(sc.partials.p), static_cast<uint64_t*>(masks.p), host_iters, host_max,
+                   host_mean);
+}
+
+int ants_op_invert_noalloc(const ants_image_desc* gdsc, const float* d_field, const float* d_init, float* d_inv,
+                           int* host_iters, double* host_max, double* host_mean, void* stream) {
+  HostGrid g;
+  ANTS_TRY(to_host_grid(gdsc, &g));
+  const GridDev gd = to_dev(g);
+  // layout: step slots | partials (2 parities) | masks (2 parities)
+  const size_t off_parts = align256(sizeof(RedSlot) * kInvSlots);
+  const size_t off_masks = align256(off_parts + 2 * inv_partials_needed(gd) * sizeof(Partial));
+  const size_t need = off_masks + 2 * inv_mask_words(gd) * sizeof(uint64_t);
+  if (need > kNoAllocScratchBytes)
+    return fail(ANTS_ERR_INVALID,
+                "ants_op_invert_noalloc: a grid of %lld voxels needs %zu scratch bytes, the fixed scratch holds %zu "
+                "(about 33 million voxels at most)",
+                static_cast<long long>(g.count()), need, kNoAllocScratchBytes);
+  void* base = nullptr;
+  ANTS_TRY(scratch_of_current_device(&base));
+  char* p = static_cast<char*>(base);
+  return invert_on(static_cast<hipStream_t>(stream), g, d_field, d_init, d_inv, reinterpret_cast<RedSlot*>(p),
+                   reinterpret_cast<Partial*>(p + off_parts), reinterpret_cast<uint64_t*>(p + off_masks), host_iters,
+                   host_max, host_mean);
 }
 
 int ants_op_resample_field(const ants_image_desc* gi, const float* d_in, const ants_image_desc* go, float* d_out,

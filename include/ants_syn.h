@@ -238,6 +238,18 @@ int ants_op_scale_update(const ants_image_desc* g, float* d_u, double rate, doub
 int ants_op_invert(const ants_image_desc* g, const float* d_field, const float* d_init, float* d_inv,
                    int* host_iters, double* host_max, double* host_mean, void* stream);
 
+/* ants_op_invert without any device allocation.  No counterpart in ANTs:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The step
+ * slots, per-block partials and candidate masks live in a fixed 8 MiB device
+ * array that the library holds per GPU until the process ends, so grids of
+ * up to about 33 million voxels are accepted (ANTS_ERR_INVALID beyond).
+ * Same arguments, results and bits as ants_op_invert.  Every call on one GPU
+ * shares that array: calls that overlap on one GPU (several streams or host
+ * threads) race and must be serialised by the caller. */
+int ants_op_invert_noalloc(const ants_image_desc* g, const float* d_field, const float* d_init, float* d_inv,
+                           int* host_iters, double* host_max, double* host_mean, void* stream);
+
 /* DisplacementFieldTransformParametersAdaptor: linear resample of a field onto
  * a new grid (itkantsRegistrationHelper.hxx:2033-2055). */
 int ants_op_resample_field(const ants_image_desc* g_in, const float* d_in, const ants_image_desc* g_out,
