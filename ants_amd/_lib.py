@@ -117,6 +117,7 @@ def lib():
     L.ants_memcpy_d2h.argtypes = [vp, vp, sz, vp]
     L.ants_memset_device.argtypes = [vp, i, sz, vp]
     L.ants_copy_calibration.argtypes = [vp, vp, sz, vp]
+    L.ants_device_memory_info.argtypes = [C.POINTER(sz), C.POINTER(sz)]
     L.ants_stream_create.argtypes = [C.POINTER(vp)]
     L.ants_stream_destroy.argtypes = [vp]
     L.ants_stream_synchronize.argtypes = [vp]
@@ -133,6 +134,7 @@ def lib():
     L.ants_syn_register_device.argtypes = [vp, pdesc, vp, pdesc, vp, vp, vp, ITER_CB, vp]
     L.ants_syn_get_state_device.argtypes = [vp, vp, vp, vp, vp]
     L.ants_syn_get_stats.argtypes = [vp, C.POINTER(SynStatsC)]
+    L.ants_syn_convergence_value.argtypes = [C.POINTER(d), i, i, C.POINTER(d)]
     L.ants_syn_get_trace.argtypes = [vp, C.POINTER(IterRecordC), C.c_int64, C.POINTER(C.c_int64)]
     L.ants_syn_set_profiling.argtypes = [vp, C.c_uint]
     L.ants_syn_set_profiling_stride.argtypes = [vp, i]
@@ -141,6 +143,7 @@ def lib():
     L.ants_op_smooth_field.argtypes = [pdesc, vp, vp, d, vp]
     L.ants_op_warp_image.argtypes = [pdesc, vp, pdesc, vp, vp, vp]
     L.ants_op_cc.argtypes = [pdesc, vp, vp, i, vp, vp, C.POINTER(d), vp]
+    L.ants_op_warp_pair.argtypes = [pdesc, pdesc, vp, vp, vp, pdesc, vp, vp, vp, i, vp]
     L.ants_op_compose.argtypes = [pdesc, vp, vp, vp, vp]
     L.ants_op_scale_update.argtypes = [pdesc, vp, d, C.POINTER(d), vp]
     L.ants_op_invert.argtypes = [pdesc, vp, vp, vp, C.POINTER(i), C.POINTER(d), C.POINTER(d), vp]
@@ -234,3 +237,10 @@ class Event:
             lib().ants_event_destroy(self.ptr)
         except Exception:
             pass
+
+
+def device_free_bytes() -> int:
+    """Free device memory (hipMemGetInfo) of the current device."""
+    f = C.c_size_t()
+    check(lib().ants_device_memory_info(C.byref(f), None))
+    return int(f.value)

@@ -25,6 +25,7 @@
 #include <cstdlib>
 #include <limits>
 #include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -584,12 +585,29 @@ struct ants_syn_handle {
     graphs.clear();
   }
 
-  ~ants_syn_handle() {
+  // device bytes the handle's grow-only workspace holds
+  int64_t device_bytes() const {
+    const DevBuf* all[] = {&F,     &M,           &Fs,          &Ms,        &tmp,    &Fw,         &Mw,  &Fb,
+                           &Mb,    &stats,       &partials_cc, &partials_inv, &masks_inv, &minmax, &restore_buf};
+    int64_t b = 0;
+    for (const DevBuf* x : all) b += static_cast<int64_t>(x->bytes);
+    for (const auto& x : fld) b += static_cast<int64_t>(x.bytes);
+    return b;
+  }
+  // frees the workspace (and the graphs that refer to it); the restored
+  // state, set by the caller, is kept
+  void release_workspace() {
     clear_graphs();
-    DevBuf* all[] = {&F,     &M,           &Fs,          &Ms,        &tmp,    &Fw,         &Mw,  &Fb,
-                     &Mb,    &stats,       &partials_cc, &partials_inv, &masks_inv, &minmax, &restore_buf};
+    graph_signature.clear();
+    DevBuf* all[] = {&F, &M, &Fs, &Ms, &tmp, &Fw, &Mw, &Fb, &Mb, &stats, &partials_cc, &partials_inv, &masks_inv, &minmax};
     for (DevBuf* b : all) b->release();
     for (auto& b : fld) b.release();
+    have_state = false;
+  }
+
+  ~ants_syn_handle() {
+    release_workspace();
+    restore_buf.release();
     if (host_stats) (void)hipHostFree(host_stats);
     if (host_flag) (void)hipHostFree(host_flag);
     if (own_stream && stream) (void)hipStreamDestroy(stream);
@@ -1234,6 +1252,13 @@ int ants_stream_synchronize(void* stream) {
   HIP_TRY(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
   return ANTS_OK;
 }
+int ants_device_memory_info(size_t* free_bytes, size_t* total_bytes) {
+  size_t f = 0, t = 0;
+  HIP_TRY(hipMemGetInfo(&f, &t));
+  if (free_bytes) *free_bytes = f;
+  if (total_bytes) *total_bytes = t;
+  return ANTS_OK;
+}
 int ants_device_synchronize(void) {
   HIP_TRY(hipDeviceSynchronize());
   return ANTS_OK;
@@ -1378,6 +1403,15 @@ int ants_syn_set_initial_state_device(ants_syn_handle* h, const ants_image_desc*
   return ANTS_OK;
 }
 
+int ants_syn_convergence_value(const double* energies, int n, int window, double* out) {
+  if (!out || n < 0 || (n > 0 && !energies)) return fail(ANTS_ERR_INVALID, "bad energy sequence");
+  if (window < 1) return fail(ANTS_ERR_INVALID, "convergence window must be >= 1");
+  ConvergenceMonitor mon(window);
+  for (int i = 0; i < n; ++i) mon.add(energies[i]);
+  *out = mon.value();
+  return ANTS_OK;
+}
+
 int ants_syn_get_stats(const ants_syn_handle* h, ants_syn_stats* out) {
   if (!h || !out) return fail(ANTS_ERR_INVALID, "null argument");
   *out = h->run_stats;
@@ -1466,6 +1500,37 @@ int ants_op_warp_image(const ants_image_desc* gi, const float* d_img, const ants
   if (a.D != b.D) return fail(ANTS_ERR_INVALID, "dimension mismatch");
   hipStream_t s = static_cast<hipStream_t>(stream);
   launch_warp_image(to_dev(a), d_img, to_dev(b), d_field, d_out, s);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(s));
+  return ANTS_OK;
+}
+
+int ants_op_warp_pair(const ants_image_desc* g_level, const ants_image_desc* g_fixed, const float* d_fixed,
+                      const float* d_u_fixed, float* d_fixed_out, const ants_image_desc* g_moving,
+                      const float* d_moving, const float* d_u_moving, float* d_moving_out, int bf16_images,
+                      void* stream) {
+  HostGrid gl, gF, gM;
+  ANTS_TRY(to_host_grid(g_level, &gl));
+  ANTS_TRY(to_host_grid(g_fixed, &gF));
+  ANTS_TRY(to_host_grid(g_moving, &gM));
+  if (gl.D != gF.D || gl.D != gM.D) return fail(ANTS_ERR_INVALID, "dimension mismatch");
+  if (!d_fixed || !d_moving || !d_u_fixed || !d_u_moving || !d_fixed_out || !d_moving_out)
+    return fail(ANTS_ERR_INVALID, "null buffer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (bf16_images) {
+    DevBuf bF, bM;
+    ANTS_TRY(bF.ensure(gF.count() * sizeof(uint16_t)));
+    ANTS_TRY(bM.ensure(gM.count() * sizeof(uint16_t)));
+    launch_to_bf16(d_fixed, static_cast<uint16_t*>(bF.p), gF.count(), s);
+    launch_to_bf16(d_moving, static_cast<uint16_t*>(bM.p), gM.count(), s);
+    launch_warp_pair_bf16(to_dev(gl), to_dev(gF), static_cast<const uint16_t*>(bF.p), d_u_fixed, d_fixed_out,
+                          to_dev(gM), static_cast<const uint16_t*>(bM.p), d_u_moving, d_moving_out, s);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s));
+    return ANTS_OK;
+  }
+  launch_warp_pair(to_dev(gl), to_dev(gF), d_fixed, d_u_fixed, d_fixed_out, to_dev(gM), d_moving, d_u_moving,
+                   d_moving_out, s);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(s));
   return ANTS_OK;
@@ -1807,20 +1872,61 @@ int64_t footprint_bytes(const ants_syn_params& P, const HostGrid& gF, const Host
   return b;
 }
 
+// An immutable version of a resident fixed template.  ants_batch_set_template
+// publishes a new version; queued and running requests keep the version that
+// was current when they were submitted (shared ownership), so replacing a
+// template never changes or frees data a request is using.
+struct TemplateVer {
+  ants_image_desc desc{};
+  DevBuf img;
+  std::atomic<int64_t>* held = nullptr;  // the batch's template byte count
+  ~TemplateVer() {
+    if (held) held->fetch_sub(static_cast<int64_t>(img.bytes));
+  }
+};
+
 struct BatchItem {
   int64_t id;
   ants_syn_request req;
   int64_t footprint;
   double t_submit;
   InitialXforms init;  // composed at submit: the request's transform arrays need not outlive it
+  std::shared_ptr<TemplateVer> templ;
 };
+
+// Per-slot staging buffers of a worker (request inputs and outputs).
+struct SlotBufs {
+  DevBuf fixed, moving, lab, fwd, inv, lab_out;
+  int64_t bytes() const {
+    return static_cast<int64_t>(fixed.bytes + moving.bytes + lab.bytes + fwd.bytes + inv.bytes + lab_out.bytes);
+  }
+  void release() {
+    fixed.release();
+    moving.release();
+    lab.release();
+    fwd.release();
+    inv.release();
+    lab_out.release();
+  }
+};
+
+// Observer of a batched registration: a non-zero cancel flag stops it at
+// the end of the current iteration.
+int batch_cancel_cb(void* user, int, int, double, double, const int*) {
+  return static_cast<std::atomic<int>*>(user)->load(std::memory_order_relaxed);
+}
 
 }  // namespace
 
+// Admission accounting.  Every slot keeps its engine workspace and staging
+// buffers between requests (they only grow), so the device memory the batch
+// holds is: templates + for each slot, what it holds now, plus for a running
+// slot the footprint of its request (the most its buffers can grow by).  A
+// request is admitted when that total stays within the budget; an idle slot
+// whose held buffers keep the queue head out frees them.
 struct ants_batch {
   int slots = 1;
   int64_t budget = 0;
-  int64_t used = 0;
   int64_t next_id = 1;
   int64_t pending = 0;  // submitted, not yet returned to the caller
   bool stop = false;
@@ -1829,33 +1935,55 @@ struct ants_batch {
   std::deque<BatchItem> queue;
   std::deque<ants_request_result> done;
   std::vector<std::thread> workers;
-  struct Template {
-    bool set = false;
-    ants_image_desc desc{};
-    DevBuf img;
-  } templ[8];
+  std::vector<hipStream_t> streams;
+  std::vector<int64_t> held;     // per slot: device bytes of its engine + staging buffers
+  std::vector<int64_t> running;  // per slot: footprint of the running request (0 when idle)
+  std::vector<int64_t> run_id;   // per slot: id of the running request (0 when idle)
+  std::unique_ptr<std::atomic<int>[]> cancel;  // per slot: cancel flag of the running request
+  std::atomic<int64_t> template_bytes{0};
+  std::shared_ptr<TemplateVer> templ[8];
 
+  int64_t committed() const {  // under mu
+    int64_t b = template_bytes.load();
+    for (int k = 0; k < slots; ++k) b += held[k] + running[k];
+    return b;
+  }
   void worker(int slot);
 };
 
 void ants_batch::worker(int slot) {
   ants_syn_handle h;
-  if (hipStreamCreateWithFlags(&h.stream, hipStreamNonBlocking) != hipSuccess) return;
-  h.own_stream = true;
-  DevBuf d_fixed, d_moving, d_lab, d_fwd, d_inv, d_lab_out;
+  h.stream = streams[slot];
+  h.own_stream = false;
+  SlotBufs sb;
+  auto slot_bytes = [&] { return h.device_bytes() + sb.bytes(); };
   for (;;) {
     BatchItem it;
     {
       std::unique_lock<std::mutex> lk(mu);
-      // admit the queue head when the budget allows (or when nothing else is held)
-      cv_work.wait(lk, [&] {
-        return stop || (!queue.empty() && (used + queue.front().footprint <= budget || used == 0));
-      });
-      if (stop && queue.empty()) return;
-      if (queue.empty()) continue;
-      it = queue.front();
+      for (;;) {
+        if (stop && queue.empty()) return;
+        if (!queue.empty()) {
+          const int64_t fp = queue.front().footprint;
+          const int64_t others = committed() - held[slot];
+          if (others + held[slot] + fp <= budget) break;  // fits with this slot's buffers kept
+          if (held[slot] > 0) {
+            // this slot's idle buffers keep the head out: free them (the
+            // head then fits now, or once other slots finish or free theirs)
+            h.release_workspace();
+            sb.release();
+            held[slot] = 0;
+            cv_work.notify_all();
+            continue;
+          }
+        }
+        cv_work.wait(lk);
+      }
+      it = std::move(queue.front());
       queue.pop_front();
-      used += it.footprint;
+      running[slot] = it.footprint;
+      run_id[slot] = it.id;
+      cancel[slot].store(0);
     }
     ants_request_result res{};
     res.id = it.id;
@@ -1866,7 +1994,7 @@ void ants_batch::worker(int slot) {
     res.footprint_bytes = it.footprint;
     int rc = ANTS_OK;
     const ants_syn_request& r = it.req;
-    const ants_image_desc* fd = r.use_template ? &templ[r.template_index].desc : &r.fixed;
+    const ants_image_desc* fd = it.templ ? &it.templ->desc : &r.fixed;
     HostGrid gF, gM;
     rc = validate_params(&r.params);
     if (rc == ANTS_OK) rc = to_host_grid(fd, &gF);
@@ -1877,35 +2005,36 @@ void ants_batch::worker(int slot) {
       h.set_initial(it.init);
       gl = level_grid(gF, r.params.shrink[r.params.n_levels - 1]);
       const size_t fb = static_cast<size_t>(gl.count()) * gF.D * sizeof(float);
-      if (rc == ANTS_OK) rc = d_moving.ensure(gM.count() * sizeof(float));
-      if (rc == ANTS_OK && !r.use_template) rc = d_fixed.ensure(gF.count() * sizeof(float));
-      if (rc == ANTS_OK) rc = d_fwd.ensure(fb);
-      if (rc == ANTS_OK) rc = d_inv.ensure(fb);
-      if (rc == ANTS_OK && r.host_moving_labels) rc = d_lab.ensure(gM.count() * sizeof(int32_t));
-      if (rc == ANTS_OK && r.host_out_labels) rc = d_lab_out.ensure(gl.count() * sizeof(int32_t));
+      if (rc == ANTS_OK) rc = sb.moving.ensure(gM.count() * sizeof(float));
+      if (rc == ANTS_OK && !it.templ) rc = sb.fixed.ensure(gF.count() * sizeof(float));
+      if (rc == ANTS_OK) rc = sb.fwd.ensure(fb);
+      if (rc == ANTS_OK) rc = sb.inv.ensure(fb);
+      if (rc == ANTS_OK && r.host_moving_labels) rc = sb.lab.ensure(gM.count() * sizeof(int32_t));
+      if (rc == ANTS_OK && r.host_out_labels) rc = sb.lab_out.ensure(gl.count() * sizeof(int32_t));
       hipStream_t s = h.stream;
       auto hip_ok = [&](hipError_t e) {
         if (e != hipSuccess && rc == ANTS_OK) rc = fail(ANTS_ERR_HIP, "%s", hipGetErrorString(e));
       };
       if (rc == ANTS_OK) {
-        hip_ok(hipMemcpyAsync(d_moving.p, r.host_moving, gM.count() * sizeof(float), hipMemcpyHostToDevice, s));
-        if (!r.use_template)
-          hip_ok(hipMemcpyAsync(d_fixed.p, r.host_fixed, gF.count() * sizeof(float), hipMemcpyHostToDevice, s));
+        hip_ok(hipMemcpyAsync(sb.moving.p, r.host_moving, gM.count() * sizeof(float), hipMemcpyHostToDevice, s));
+        if (!it.templ)
+          hip_ok(hipMemcpyAsync(sb.fixed.p, r.host_fixed, gF.count() * sizeof(float), hipMemcpyHostToDevice, s));
         if (r.host_moving_labels)
-          hip_ok(hipMemcpyAsync(d_lab.p, r.host_moving_labels, gM.count() * sizeof(int32_t),
+          hip_ok(hipMemcpyAsync(sb.lab.p, r.host_moving_labels, gM.count() * sizeof(int32_t),
                                 hipMemcpyHostToDevice, s));
       }
-      const float* dfix = r.use_template ? templ[r.template_index].img.f() : d_fixed.f();
-      if (rc == ANTS_OK) rc = h.run(fd, dfix, &r.moving, d_moving.f(), d_fwd.f(), d_inv.f(), nullptr, nullptr);
+      const float* dfix = it.templ ? it.templ->img.f() : sb.fixed.f();
+      if (rc == ANTS_OK)
+        rc = h.run(fd, dfix, &r.moving, sb.moving.f(), sb.fwd.f(), sb.inv.f(), batch_cancel_cb, &cancel[slot]);
       if (rc == ANTS_OK && r.host_moving_labels && r.host_out_labels) {
-        launch_labels_nn(to_dev(gl), d_fwd.f(), to_dev(gM), static_cast<const int32_t*>(d_lab.p),
-                         static_cast<int32_t*>(d_lab_out.p), s);
+        launch_labels_nn(to_dev(gl), sb.fwd.f(), to_dev(gM), static_cast<const int32_t*>(sb.lab.p),
+                         static_cast<int32_t*>(sb.lab_out.p), s);
         hip_ok(hipGetLastError());
-        hip_ok(hipMemcpyAsync(r.host_out_labels, d_lab_out.p, gl.count() * sizeof(int32_t), hipMemcpyDeviceToHost,
+        hip_ok(hipMemcpyAsync(r.host_out_labels, sb.lab_out.p, gl.count() * sizeof(int32_t), hipMemcpyDeviceToHost,
                               s));
       }
-      if (rc == ANTS_OK && r.host_out_fwd) hip_ok(hipMemcpyAsync(r.host_out_fwd, d_fwd.p, fb, hipMemcpyDeviceToHost, s));
-      if (rc == ANTS_OK && r.host_out_inv) hip_ok(hipMemcpyAsync(r.host_out_inv, d_inv.p, fb, hipMemcpyDeviceToHost, s));
+      if (rc == ANTS_OK && r.host_out_fwd) hip_ok(hipMemcpyAsync(r.host_out_fwd, sb.fwd.p, fb, hipMemcpyDeviceToHost, s));
+      if (rc == ANTS_OK && r.host_out_inv) hip_ok(hipMemcpyAsync(r.host_out_inv, sb.inv.p, fb, hipMemcpyDeviceToHost, s));
       hip_ok(hipStreamSynchronize(s));
     }
     res.status = rc;
@@ -1913,9 +2042,12 @@ void ants_batch::worker(int slot) {
     res.voxel_iterations = h.run_stats.voxel_iterations;
     res.algorithmic_bytes = h.run_stats.algorithmic_bytes;
     res.t_end = now_s();
+    it.templ.reset();  // a replaced template version is freed once no request holds it
     {
       std::lock_guard<std::mutex> lk(mu);
-      used -= it.footprint;
+      held[slot] = slot_bytes();
+      running[slot] = 0;
+      run_id[slot] = 0;
       done.push_back(res);
     }
     cv_done.notify_all();
@@ -1937,6 +2069,7 @@ int64_t ants_syn_footprint(const ants_syn_params* p, const ants_image_desc* fixe
 int ants_batch_create(const ants_batch_config* cfg, ants_batch** out) {
   if (!cfg || !out) return fail(ANTS_ERR_INVALID, "null argument");
   if (cfg->slots < 1 || cfg->slots > 64) return fail(ANTS_ERR_INVALID, "slots must be in 1..64");
+  *out = nullptr;
   auto* b = new (std::nothrow) ants_batch();
   if (!b) return fail(ANTS_ERR_OOM, "out of host memory");
   b->slots = cfg->slots;
@@ -1949,6 +2082,23 @@ int ants_batch_create(const ants_batch_config* cfg, ants_batch** out) {
     }
     b->budget = static_cast<int64_t>(0.9 * static_cast<double>(fr));
   }
+  // every slot's stream exists before any worker starts: a failure is the
+  // caller's error, not a worker that silently never serves
+  for (int i = 0; i < b->slots; ++i) {
+    hipStream_t s = nullptr;
+    const hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      for (hipStream_t t : b->streams) (void)hipStreamDestroy(t);
+      delete b;
+      return fail(ANTS_ERR_HIP, "hipStreamCreateWithFlags for slot %d: %s", i, hipGetErrorString(e));
+    }
+    b->streams.push_back(s);
+  }
+  b->held.assign(b->slots, 0);
+  b->running.assign(b->slots, 0);
+  b->run_id.assign(b->slots, 0);
+  b->cancel.reset(new std::atomic<int>[b->slots]);
+  for (int i = 0; i < b->slots; ++i) b->cancel[i].store(0);
   for (int i = 0; i < b->slots; ++i) b->workers.emplace_back(&ants_batch::worker, b, i);
   *out = b;
   return ANTS_OK;
@@ -1962,6 +2112,7 @@ void ants_batch_destroy(ants_batch* b) {
   }
   b->cv_work.notify_all();
   for (auto& t : b->workers) t.join();
+  for (hipStream_t s : b->streams) (void)hipStreamDestroy(s);
   delete b;
 }
 
@@ -1970,42 +2121,100 @@ int ants_batch_set_template(ants_batch* b, int index, const ants_image_desc* g, 
   if (index < 0 || index >= 8) return fail(ANTS_ERR_INVALID, "template index must be in 0..7");
   HostGrid hg;
   ANTS_TRY(to_host_grid(g, &hg));
-  std::lock_guard<std::mutex> lk(b->mu);
-  auto& t = b->templ[index];
-  ANTS_TRY(t.img.ensure(hg.count() * sizeof(float)));
-  HIP_TRY(hipMemcpy(t.img.p, host_image, hg.count() * sizeof(float), hipMemcpyHostToDevice));
-  t.desc = *g;
-  t.set = true;
+  const int64_t bytes = hg.count() * static_cast<int64_t>(sizeof(float));
+  {
+    std::lock_guard<std::mutex> lk(b->mu);
+    const int64_t replaced = b->templ[index] ? static_cast<int64_t>(b->templ[index]->img.bytes) : 0;
+    if (b->committed() + bytes > b->budget + replaced)
+      return fail(ANTS_ERR_OOM, "template of %lld bytes does not fit the budget of %lld", (long long)bytes,
+                  (long long)b->budget);
+  }
+  // a new immutable version, complete on the device before it is published
+  auto v = std::make_shared<TemplateVer>();
+  ANTS_TRY(v->img.ensure(static_cast<size_t>(bytes)));
+  HIP_TRY(hipMemcpy(v->img.p, host_image, static_cast<size_t>(bytes), hipMemcpyHostToDevice));
+  v->desc = *g;
+  v->held = &b->template_bytes;
+  b->template_bytes.fetch_add(static_cast<int64_t>(v->img.bytes));
+  std::shared_ptr<TemplateVer> old;
+  {
+    std::lock_guard<std::mutex> lk(b->mu);
+    old = std::move(b->templ[index]);
+    b->templ[index] = std::move(v);
+  }
+  old.reset();  // freed here unless queued or running requests still hold it
+  b->cv_work.notify_all();
   return ANTS_OK;
 }
 
 int ants_batch_submit(ants_batch* b, const ants_syn_request* r, int64_t* id) {
   if (!b || !r) return fail(ANTS_ERR_INVALID, "null argument");
   ANTS_TRY(validate_params(&r->params));
-  if (r->use_template && (r->template_index < 0 || r->template_index >= 8 || !b->templ[r->template_index].set))
-    return fail(ANTS_ERR_INVALID, "template %d is not set", r->template_index);
-  const ants_image_desc* fd = r->use_template ? &b->templ[r->template_index].desc : &r->fixed;
+  std::shared_ptr<TemplateVer> tv;
+  if (r->use_template) {
+    if (r->template_index < 0 || r->template_index >= 8) return fail(ANTS_ERR_INVALID, "template index out of range");
+    std::lock_guard<std::mutex> lk(b->mu);
+    tv = b->templ[r->template_index];
+    if (!tv) return fail(ANTS_ERR_INVALID, "template %d is not set", r->template_index);
+  }
+  const ants_image_desc* fd = tv ? &tv->desc : &r->fixed;
   HostGrid gF, gM;
   ANTS_TRY(to_host_grid(fd, &gF));
   ANTS_TRY(to_host_grid(&r->moving, &gM));
   if (gF.D != r->params.dim || gM.D != r->params.dim) return fail(ANTS_ERR_INVALID, "dimension mismatch");
-  if (!r->host_moving || (!r->use_template && !r->host_fixed)) return fail(ANTS_ERR_INVALID, "null image");
+  if (!r->host_moving || (!tv && !r->host_fixed)) return fail(ANTS_ERR_INVALID, "null image");
   const int64_t fp = footprint_bytes(r->params, gF, gM);
   InitialXforms init;
   ANTS_TRY(compose_initial(r->params.dim, r->fixed_initial, r->n_fixed_initial, r->moving_initial,
                            r->n_moving_initial, &init));
-  BatchItem it{0, *r, fp, now_s(), init};
+  BatchItem it{0, *r, fp, now_s(), init, tv};
   {
     std::lock_guard<std::mutex> lk(b->mu);
-    if (fp > b->budget) return fail(ANTS_ERR_OOM, "request needs %lld bytes, budget is %lld", (long long)fp,
-                                    (long long)b->budget);
+    const int64_t tb = b->template_bytes.load();
+    if (fp + tb > b->budget)
+      return fail(ANTS_ERR_OOM, "request needs %lld bytes, the budget less resident templates is %lld",
+                  (long long)fp, (long long)(b->budget - tb));
     it.id = b->next_id++;
-    b->queue.push_back(it);
+    if (id) *id = it.id;
+    b->queue.push_back(std::move(it));
     b->pending += 1;
   }
-  if (id) *id = it.id;
-  b->cv_work.notify_one();
+  b->cv_work.notify_all();
   return ANTS_OK;
+}
+
+int ants_batch_cancel(ants_batch* b, int64_t id) {
+  if (!b) return fail(ANTS_ERR_INVALID, "null argument");
+  std::lock_guard<std::mutex> lk(b->mu);
+  for (auto q = b->queue.begin(); q != b->queue.end(); ++q) {
+    if (q->id != id) continue;
+    ants_request_result res{};
+    res.id = id;
+    res.status = ANTS_ERR_CANCELLED;
+    res.slot = -1;
+    res.user = q->req.user;
+    res.t_submit = q->t_submit;
+    res.t_start = res.t_end = now_s();
+    res.footprint_bytes = q->footprint;
+    std::snprintf(res.error, sizeof(res.error), "cancelled before it started");
+    b->queue.erase(q);
+    b->done.push_back(res);
+    b->cv_done.notify_all();
+    b->cv_work.notify_all();
+    return ANTS_OK;
+  }
+  for (int k = 0; k < b->slots; ++k)
+    if (b->run_id[k] == id) {
+      b->cancel[k].store(1);
+      return ANTS_OK;
+    }
+  return fail(ANTS_ERR_INVALID, "request %lld is neither queued nor running", static_cast<long long>(id));
+}
+
+int64_t ants_batch_device_bytes(ants_batch* b) {
+  if (!b) return -1;
+  std::lock_guard<std::mutex> lk(b->mu);
+  return b->committed();
 }
 
 static int take_results(ants_batch* b, ants_request_result* out, int max_results, int* n_out) {

@@ -1358,8 +1358,10 @@ __global__ void __launch_bounds__(kBlock) k_compose(GridDev g, PtrPack4 disp, Pt
 // The max test carries a 1e-5 relative guard: with a zero initial inverse the
 // first residual's max norm equals the learning rate, which for SyN[0.1,..]
 // sits exactly on the 0.1 tolerance; the guard resolves that tie the same way
-// in float and double.
+// in float and double.  ANTS_SYN_INV_STRICT_TIE=1 selects ITK's test as
+// written instead (float max norm > 0.1f), which leaves the tie to rounding.
 constexpr float kInvMaxTol = 0.1f * (1.0f + 1e-5f);
+constexpr float kInvMaxTolStrict = 0.1f;
 template <int D, class IT = int64_t>
 __device__ __forceinline__ float inv_residual(const GridDev& g, const float* __restrict__ field, int x, int y, int z,
                                               const float iv[3], float e[3]) {
@@ -1504,7 +1506,7 @@ __global__ void __launch_bounds__(kInvThreads, ANTS_INV_WAVES) k_inv_step(GridDe
                                                                           SlotPack4 slots, Partial* partials,
                                                                           int64_t part_stride, uint64_t* masks,
                                                                           int64_t mask_stride, int j, int speculate,
-                                                                          InvGate gate) {
+                                                                          InvGate gate, float max_tol) {
   // gated launch: a no-op while the inversions it depends on still had work
   // pending after their last launched step (the host then reruns it)
   if (gate.step >= 0) {
@@ -1560,7 +1562,7 @@ __global__ void __launch_bounds__(kInvThreads, ANTS_INV_WAVES) k_inv_step(GridDe
     do_fix = true;
     const double mean = psum / static_cast<double>(N);
     eps_prev = (j - 1 == 0) ? 0.75f : 0.5f;
-    do_norm = j < kInvMaxIter && maxprev > kInvMaxTol && mean > 0.001;
+    do_norm = j < kInvMaxIter && maxprev > max_tol && mean > 0.001;
   }
   if (!do_fix && !do_norm) return;
   const float lim = eps_prev * maxprev;      // clamp of the pending update j-1
@@ -2277,18 +2279,20 @@ void launch_inv_step(const GridDev& g, int np, const float* const* field, float*
   if (j < 0 && !smp) return;  // no sample launch without speculation
   const dim3 grid_j = j < 0 ? dim3(static_cast<unsigned int>(inv_sample_blocks(grid.x)), np) : grid;
   const bool idx32 = g.count() * g.D < (int64_t{1} << 31);
+  const char* strict = std::getenv("ANTS_SYN_INV_STRICT_TIE");
+  const float tol = (strict && strict[0] == '1') ? kInvMaxTolStrict : kInvMaxTol;
   if (g.D == 2 && idx32)
     hipLaunchKernelGGL((k_inv_step<2, int>), grid_j, dim3(kInvThreads), 0, s, g, pack(field, np), mpack(inv, np), sp,
-                       partials, stride, masks, mstride, j, smp, gate);
+                       partials, stride, masks, mstride, j, smp, gate, tol);
   else if (g.D == 2)
     hipLaunchKernelGGL((k_inv_step<2, int64_t>), grid_j, dim3(kInvThreads), 0, s, g, pack(field, np), mpack(inv, np),
-                       sp, partials, stride, masks, mstride, j, smp, gate);
+                       sp, partials, stride, masks, mstride, j, smp, gate, tol);
   else if (idx32)
     hipLaunchKernelGGL((k_inv_step<3, int>), grid_j, dim3(kInvThreads), 0, s, g, pack(field, np), mpack(inv, np), sp,
-                       partials, stride, masks, mstride, j, smp, gate);
+                       partials, stride, masks, mstride, j, smp, gate, tol);
   else
     hipLaunchKernelGGL((k_inv_step<3, int64_t>), grid_j, dim3(kInvThreads), 0, s, g, pack(field, np), mpack(inv, np),
-                       sp, partials, stride, masks, mstride, j, smp, gate);
+                       sp, partials, stride, masks, mstride, j, smp, gate, tol);
 }
 
 void launch_resample_fields(const GridDev& gi, const GridDev& go, int np, const float* const* in, float* const* out,

@@ -73,6 +73,9 @@ def _bind():
     lib.ants_batch_wait.argtypes = [vp, C.POINTER(ResultC), C.c_int, C.POINTER(C.c_int)]
     lib.ants_batch_pending.argtypes = [vp]
     lib.ants_batch_pending.restype = C.c_int64
+    lib.ants_batch_cancel.argtypes = [vp, C.c_int64]
+    lib.ants_batch_device_bytes.argtypes = [vp]
+    lib.ants_batch_device_bytes.restype = C.c_int64
     lib.ants_batch_clock.restype = C.c_double
     lib.ants_syn_footprint.argtypes = [C.POINTER(L.SynParamsC), C.POINTER(L.ImageDesc), C.POINTER(L.ImageDesc)]
     lib.ants_syn_footprint.restype = C.c_int64
@@ -236,6 +239,15 @@ class Batch:
 
     def pending(self) -> int:
         return int(self.lib.ants_batch_pending(self.h))
+
+    def cancel(self, rid: int):
+        """Cancel a queued or running request (its result comes back with
+        status ANTS_ERR_CANCELLED)."""
+        L.check(self.lib.ants_batch_cancel(self.h, int(rid)))
+
+    def device_bytes(self) -> int:
+        """Device bytes held or reserved: templates, slot buffers, running requests."""
+        return int(self.lib.ants_batch_device_bytes(self.h))
 
     def drain(self) -> List[Result]:
         out = []

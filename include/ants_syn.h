@@ -93,6 +93,8 @@ int ants_stream_create(void** stream);
 int ants_stream_destroy(void* stream);
 int ants_stream_synchronize(void* stream);
 int ants_device_synchronize(void);
+/* hipMemGetInfo of the current device (free and total bytes; either may be NULL). */
+int ants_device_memory_info(size_t* free_bytes, size_t* total_bytes);
 int ants_event_create(void** event);
 int ants_event_destroy(void* event);
 int ants_event_record(void* event, void* stream);
@@ -145,6 +147,14 @@ int ants_syn_get_state_device(ants_syn_handle* h, float* d_f2m_fwd, float* d_f2m
  * single stage. */
 int ants_syn_set_initial_state_device(ants_syn_handle* h, const ants_image_desc* grid, const float* d_f2m_fwd,
                                       const float* d_f2m_inv, const float* d_m2m_fwd, const float* d_m2m_inv);
+
+/* WindowConvergenceMonitoringFunction as the SyN stage uses it to stop a
+ * level (described at Examples/antsRegistration.cxx:397-404; the stage's
+ * ConvergenceWindowSize and ConvergenceThreshold at
+ * Examples/itkantsRegistrationHelper.hxx:2157-2158): the convergence value
+ * after energies[0..n) were added in order, DBL_MAX while fewer than
+ * `window` exist.  The engine stops a level when it is below the threshold. */
+int ants_syn_convergence_value(const double* energies, int n, int window, double* out);
 
 /* Work done by the last run: iterations per level, inner inversion steps,
  * voxel-iterations and the algorithmic HBM byte count (per voxel-iteration
@@ -218,6 +228,18 @@ int ants_op_smooth_field(const ants_image_desc* g, const float* d_in, float* d_o
  * (itkantsRegistrationHelper.hxx:719-763). */
 int ants_op_warp_image(const ants_image_desc* g_img, const float* d_img, const ants_image_desc* g_out,
                        const float* d_field, float* d_out, void* stream);
+
+/* The two resamplings of one SyN iteration (SyN ComputeMetricGradientField
+ * with DownsampleImagesForMetricDerivatives, mirrored in
+ * Examples/antsDisplacementAndVelocityFieldRegistrationCommandIterationUpdate.h:342-381):
+ * on level grid g_level, fixed_out(x) = fixed(x + u_fixed(x)) and
+ * moving_out(x) = moving(x + u_moving(x)), trilinear, 0 outside each image's
+ * buffer; u_* are D x (level voxels) fields.  bf16_images != 0 rounds both
+ * images to BF16 first, as the BF16 pyramid form stores them. */
+int ants_op_warp_pair(const ants_image_desc* g_level, const ants_image_desc* g_fixed, const float* d_fixed,
+                      const float* d_u_fixed, float* d_fixed_out, const ants_image_desc* g_moving,
+                      const float* d_moving, const float* d_u_moving, float* d_moving_out, int bf16_images,
+                      void* stream);
 
 /* ANTSNeighborhoodCorrelationImageToImageMetricv4::GetValueAndDerivative on
  * two images of grid g, both SyN roles at once (metric set up at
@@ -365,15 +387,33 @@ typedef struct ants_request_result {
 typedef struct ants_batch ants_batch;
 int ants_batch_create(const ants_batch_config* cfg, ants_batch** out);
 void ants_batch_destroy(ants_batch* b);
-/* Upload a fixed template once (index 0..7); requests may then use it. */
+/* Upload a fixed template once (index 0..7); requests may then use it.
+ * Setting an index again publishes a new copy: requests submitted before
+ * keep using the copy that was current at their submission (it is freed when
+ * the last of them finishes), later ones use the new one.  Templates count
+ * against the memory budget (ANTS_ERR_OOM when one does not fit). */
 int ants_batch_set_template(ants_batch* b, int index, const ants_image_desc* g, const float* host_image);
-/* Queue a request (host buffers must stay valid until its result is returned). */
+/* Queue a request (host buffers must stay valid until its result is
+ * returned).  ANTS_ERR_OOM when its footprint plus the resident templates
+ * exceeds the memory budget. */
 int ants_batch_submit(ants_batch* b, const ants_syn_request* r, int64_t* id);
 /* Completed results: poll returns at once; wait blocks until at least one is
  * available or nothing is pending. */
 int ants_batch_poll(ants_batch* b, ants_request_result* out, int max_results, int* n_out);
 int ants_batch_wait(ants_batch* b, ants_request_result* out, int max_results, int* n_out);
 int64_t ants_batch_pending(ants_batch* b);
+/* Cancel request `id`: a queued request is dropped and returned with status
+ * ANTS_ERR_CANCELLED; a running one stops at the end of its current
+ * iteration and is returned with that status (the other requests are not
+ * affected).  ANTS_ERR_INVALID when the request is neither queued nor running
+ * (unknown, finished or already returned).  The clean failure return of a
+ * registration: Examples/itkantsRegistrationHelper.hxx:2200-2204. */
+int ants_batch_cancel(ants_batch* b, int64_t id);
+/* Device bytes the batch holds or has reserved: resident templates, every
+ * slot's retained workspace and staging buffers, and the footprint of each
+ * running request.  Admission keeps this within the memory budget; an idle
+ * slot frees its retained buffers when they keep the queue head out. */
+int64_t ants_batch_device_bytes(ants_batch* b);
 double ants_batch_clock(void);
 /* Device bytes one registration of this geometry and schedule holds. */
 int64_t ants_syn_footprint(const ants_syn_params* p, const ants_image_desc* fixed, const ants_image_desc* moving);

@@ -39,6 +39,7 @@ class OgParams(C.Structure):
         ("preprocess", C.c_int),
         ("force_iterations", C.c_int),
         ("threads", C.c_int),
+        ("inv_tie_rule", C.c_int),
     ]
 
 
@@ -279,6 +280,8 @@ class SynParams:
     preprocess: bool = True
     force_iterations: bool = False
     threads: int = 0
+    # inversion max test: 0 guarded (max > 0.1 (1 + 1e-5)), 1 strict (ITK's max > 0.1)
+    inv_tie_rule: int = 0
 
     def c(self) -> OgParams:
         p = OgParams()
@@ -297,6 +300,7 @@ class SynParams:
         p.preprocess = int(self.preprocess)
         p.force_iterations = int(self.force_iterations)
         p.threads = self.threads
+        p.inv_tie_rule = int(self.inv_tie_rule)
         return p
 
 
@@ -310,15 +314,20 @@ class SynResult:
     convergence: np.ndarray
     inv_iters: np.ndarray
     state: Optional[np.ndarray] = None
+    # (n_levels, ...) results of the schedule stopped after each level
+    level_fwd: Optional[np.ndarray] = None
+    level_inv: Optional[np.ndarray] = None
 
 
 def syn_register(params: SynParams, gF: Grid, fixed, gM: Grid, moving, precision: str = "f64",
                  eps: float = FLT_EPSILON, want_state: bool = False, max_records: int = 4096,
-                 fixed_xforms=(), moving_xforms=(), restore=None) -> SynResult:
+                 fixed_xforms=(), moving_xforms=(), restore=None, want_levels: bool = False) -> SynResult:
     """fixed_xforms / moving_xforms: initial transforms (see _xform_array),
     in the order a point passes through them.  restore: (Grid, state) with
     state (4, z, y, x, D) = F2M fwd, F2M inv, M2M fwd, M2M inv (FP64 only;
-    moving_xforms may be combined with it)."""
+    moving_xforms may be combined with it).  want_levels (FP64, no initial
+    transforms or state): also return each level's result, i.e. the output of
+    the same schedule with no iterations after that level."""
     L = lib()
     dt = np.float64 if precision == "f64" else np.float32
     ct = C.c_double if precision == "f64" else C.c_float
@@ -343,7 +352,20 @@ def syn_register(params: SynParams, gF: Grid, fixed, gM: Grid, moving, precision
     tr.inv_iters = ki.ctypes.data_as(C.POINTER(C.c_int))
     p = params.c()
     P = C.POINTER(ct)
-    if restore is not None:
+    lvf = lvi = None
+    if want_levels:
+        if precision != "f64" or restore is not None or fixed_xforms or moving_xforms:
+            raise ValueError("want_levels: FP64 without initial transforms or state")
+        lvf = np.empty((len(params.shrink),) + fwd.shape, dtype=dt)
+        lvi = np.empty_like(lvf)
+        fn = L.og_syn_register_levels_f64
+        fn.argtypes = [C.POINTER(OgParams), C.POINTER(OgGrid), P, C.POINTER(OgGrid), P, P, P, C.POINTER(OgTrace), P, P,
+                       C.c_double]
+        fn.restype = C.c_int
+        rc = fn(C.byref(p), C.byref(gF.c()), f.ctypes.data_as(P), C.byref(gM.c()), m.ctypes.data_as(P),
+                fwd.ctypes.data_as(P), inv.ctypes.data_as(P), C.byref(tr), lvf.ctypes.data_as(P),
+                lvi.ctypes.data_as(P), eps)
+    elif restore is not None:
         if precision != "f64" or fixed_xforms:
             raise ValueError("restore: FP64 and moving initial transforms only")
         gS, st = restore
@@ -377,7 +399,7 @@ def syn_register(params: SynParams, gF: Grid, fixed, gM: Grid, moving, precision
         raise RuntimeError(f"oracle syn_register failed: {rc}")
     n = tr.n_records
     return SynResult(fwd, inv, lv[:n].copy(), it[:n].copy(), en[:n].copy(), cv[:n].copy(),
-                     ki[: 4 * n].reshape(n, 4).copy(), state)
+                     ki[: 4 * n].reshape(n, 4).copy(), state, lvf, lvi)
 
 
 def max_threads() -> int:

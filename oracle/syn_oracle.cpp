@@ -54,6 +54,7 @@ typedef struct {
   int preprocess;        // window both images to [0,1] as PreprocessImage does
   int force_iterations;  // run every iteration (no convergence stop)
   int threads;           // 0 = OpenMP default
+  int inv_tie_rule;      // inversion max test: 0 guarded (max > 0.1 (1 + 1e-5)), 1 strict (max > 0.1 in R)
 } og_params;
 
 // Geometry and transforms of a composite: antsApplyTransforms and the SyN
@@ -602,14 +603,21 @@ double scale_update(const Grid& g, R* u, double lr) {
 // Pinned: a constant shift inverts exactly in the interior; the residual at
 // exit respects the tolerances; constants cross-checked with the restatement.
 //
-// Tie rule: the max test is `max > 0.1 * (1 + 1e-5)`.  When the initial
-// inverse is zero the first residual is the scaled update itself, whose max
-// norm equals the learning rate, so SyN[0.1,...] lands exactly on the 0.1
-// tolerance and the reference's decision is set by float rounding; the guard
-// makes that tie resolve the same way in every precision.
+// Tie rule (`strict` false, the default): the max test is
+// `max > 0.1 * (1 + 1e-5)`.  When the initial inverse is zero the first
+// residual is the scaled update itself, whose max norm equals the learning
+// rate, so SyN[0.1,...] lands exactly on the 0.1 tolerance and the
+// reference's decision is set by rounding; the guard makes that tie resolve
+// the same way in every precision.  `strict` follows ITK's test as written:
+// the max norm rounded to R, compared with 0.1 in R (`> 0.1f` for --float 1).
 constexpr double kInvMaxTol = 0.1 * (1.0 + 1e-5);
 template <class R>
-int invert_field(const Grid& g, const R* field, const R* init, R* inv, double* out_max, double* out_mean) {
+inline bool inv_max_above(double maxn, bool strict) {
+  return strict ? static_cast<R>(maxn) > static_cast<R>(0.1) : maxn > kInvMaxTol;
+}
+template <class R>
+int invert_field(const Grid& g, const R* field, const R* init, R* inv, double* out_max, double* out_mean,
+                 bool strict = false) {
   const int D = g.D;
   const int64_t N = g.count();
   const int64_t nx = g.n[0], ny = g.n[1], nz = g.n[2];
@@ -620,7 +628,7 @@ int invert_field(const Grid& g, const R* field, const R* init, R* inv, double* o
   std::vector<R> e(N * D), norm(N);
   double maxn = std::numeric_limits<double>::max(), meann = std::numeric_limits<double>::max();
   int it = 0;
-  while (it++ < 20 && maxn > kInvMaxTol && meann > 0.001) {
+  while (it++ < 20 && inv_max_above<R>(maxn, strict) && meann > 0.001) {
     compose<R>(g, field, inv, e.data());
     std::vector<double> psum(nz, 0.0), pmax(nz, 0.0);
 #pragma omp parallel for schedule(static)
@@ -796,8 +804,14 @@ void preprocess(int64_t N, const R* in, R* out) {
 template <class R>
 int syn_register(const og_params* P, const Grid& gF, const R* fixed_in, const Grid& gM, const R* moving_in, R* out_fwd,
                  R* out_inv, og_trace* trace, R* state4, double eps, const og_xform* fx = nullptr, int nfx = 0,
-                 const og_xform* mx = nullptr, int nmx = 0, const Grid* gS = nullptr, const R* state_in = nullptr) {
+                 const og_xform* mx = nullptr, int nmx = 0, const Grid* gS = nullptr, const R* state_in = nullptr,
+                 R* lvl_fwd = nullptr, R* lvl_inv = nullptr) {
   set_threads(P->threads);
+  const bool strict = P->inv_tie_rule != 0;
+  // per-level snapshots: the four middle fields at the end of each level
+  // before the last, with their grid
+  std::vector<std::vector<R>> snap(lvl_fwd || lvl_inv ? 4 * P->n_levels : 0);
+  std::vector<Grid> snap_grid(P->n_levels);
   const int D = gF.D;
   const int64_t NF = gF.count(), NM = gM.count();
   std::vector<R> F(fixed_in, fixed_in + NF), M(moving_in, moving_in + NM);
@@ -839,7 +853,16 @@ int syn_register(const og_params* P, const Grid& gF, const R* fixed_in, const Gr
       m2m_i = tmp;
     }
     gPrev = gl;
-    if (P->iterations[l] <= 0) continue;
+    snap_grid[l] = gl;
+    if (P->iterations[l] <= 0) {
+      if (!snap.empty()) {
+        snap[4 * l + 0] = f2m_f;
+        snap[4 * l + 1] = f2m_i;
+        snap[4 * l + 2] = m2m_f;
+        snap[4 * l + 3] = m2m_i;
+      }
+      continue;
+    }
     smooth_image<R>(gF, F.data(), Fs.data(), P->sigma[l], P->sigma_in_mm != 0);
     smooth_image<R>(gM, M.data(), Ms.data(), P->sigma[l], P->sigma_in_mm != 0);
     std::vector<R> Fw(N), Mw(N), g1(N * D), g2(N * D), u1(N * D), u2(N * D), t1(N * D), t2(N * D), s1(N * D), s2(N * D);
@@ -860,12 +883,12 @@ int syn_register(const og_params* P, const Grid& gF, const R* fixed_in, const Gr
       smooth_field<R>(gl, t2.data(), s2.data(), P->total_variance);
       int k[4];
       std::vector<R> ninv(N * D), nfwd(N * D);
-      k[0] = invert_field<R>(gl, s1.data(), f2m_i.data(), ninv.data(), nullptr, nullptr);
-      k[1] = invert_field<R>(gl, ninv.data(), s1.data(), nfwd.data(), nullptr, nullptr);
+      k[0] = invert_field<R>(gl, s1.data(), f2m_i.data(), ninv.data(), nullptr, nullptr, strict);
+      k[1] = invert_field<R>(gl, ninv.data(), s1.data(), nfwd.data(), nullptr, nullptr, strict);
       f2m_i.swap(ninv);
       f2m_f.swap(nfwd);
-      k[2] = invert_field<R>(gl, s2.data(), m2m_i.data(), ninv.data(), nullptr, nullptr);
-      k[3] = invert_field<R>(gl, ninv.data(), s2.data(), nfwd.data(), nullptr, nullptr);
+      k[2] = invert_field<R>(gl, s2.data(), m2m_i.data(), ninv.data(), nullptr, nullptr, strict);
+      k[3] = invert_field<R>(gl, ninv.data(), s2.data(), nfwd.data(), nullptr, nullptr, strict);
       m2m_i.swap(ninv);
       m2m_f.swap(nfwd);
       const double E = 0.5 * (value + value);
@@ -881,9 +904,32 @@ int syn_register(const og_params* P, const Grid& gF, const R* fixed_in, const Gr
         for (int j = 0; j < 4; ++j) trace->inv_iters[r * 4 + j] = k[j];
       }
     }
+    if (!snap.empty()) {
+      snap[4 * l + 0] = f2m_f;
+      snap[4 * l + 1] = f2m_i;
+      snap[4 * l + 2] = m2m_f;
+      snap[4 * l + 3] = m2m_i;
+    }
   }
   // final composition on the last level grid (the full-resolution grid when the last shrink is 1)
   const int64_t N = gPrev.count();
+  // Per-level results: the state at the end of level l carried through the
+  // later levels' grids by the parameter adaptor (as a schedule that runs no
+  // iterations after level l does) and composed on the last grid.
+  for (int l = 0; l < static_cast<int>(snap.size() / 4); ++l) {
+    std::vector<R> f[4];
+    for (int k = 0; k < 4; ++k) f[k] = snap[4 * l + k];
+    for (int m = l + 1; m < P->n_levels; ++m) {
+      const int64_t Nm = snap_grid[m].count();
+      for (int k = 0; k < 4; ++k) {
+        std::vector<R> tmp(Nm * D);
+        resample_field<R>(snap_grid[m - 1], f[k].data(), snap_grid[m], tmp.data());
+        f[k].swap(tmp);
+      }
+    }
+    if (lvl_fwd) compose<R>(gPrev, f[3].data(), f[0].data(), lvl_fwd + l * N * D);
+    if (lvl_inv) compose<R>(gPrev, f[1].data(), f[2].data(), lvl_inv + l * N * D);
+  }
   if (out_fwd) compose<R>(gPrev, m2m_i.data(), f2m_f.data(), out_fwd);
   if (out_inv) compose<R>(gPrev, f2m_i.data(), m2m_f.data(), out_inv);
   if (state4) {
@@ -1163,6 +1209,16 @@ void og_warp_labels_nn(const og_grid* gout, const float* u, const og_grid* glab,
 int og_syn_register_f64(const og_params* P, const og_grid* gf, const double* fixed, const og_grid* gm, const double* moving,
                         double* out_fwd, double* out_inv, og_trace* trace, double* state4, double eps) {
   return syn_register<double>(P, to_grid(gf), fixed, to_grid(gm), moving, out_fwd, out_inv, trace, state4, eps);
+}
+
+// Same, plus the result of every level: lvl_fwd / lvl_inv hold n_levels
+// fields on the last level grid, level l's being the output of the same
+// schedule with no iterations after level l.
+int og_syn_register_levels_f64(const og_params* P, const og_grid* gf, const double* fixed, const og_grid* gm,
+                               const double* moving, double* out_fwd, double* out_inv, og_trace* trace,
+                               double* lvl_fwd, double* lvl_inv, double eps) {
+  return syn_register<double>(P, to_grid(gf), fixed, to_grid(gm), moving, out_fwd, out_inv, trace, nullptr, eps,
+                              nullptr, 0, nullptr, 0, nullptr, nullptr, lvl_fwd, lvl_inv);
 }
 
 int og_syn_register_f32(const og_params* P, const og_grid* gf, const float* fixed, const og_grid* gm, const float* moving,

@@ -102,3 +102,18 @@ def labels_nn(u, labels, go, gl, out_shape):
     L.check(L.lib().ants_apply_transform_labels_nn(C.byref(go), du.ptr if du else None, C.byref(gl), dl.ptr,
                                                    out.ptr, None))
     return out.to_host()
+
+
+def warp_pair(gl, gF, F, uF, gM, M, uM, out_shape, bf16=False):
+    dF, dM, duF, duM = dev(F), dev(M), dev(uF), dev(uM)
+    oF, oM = L.DeviceArray(out_shape), L.DeviceArray(out_shape)
+    L.check(L.lib().ants_op_warp_pair(C.byref(gl), C.byref(gF), dF.ptr, duF.ptr, oF.ptr, C.byref(gM), dM.ptr, duM.ptr,
+                                      oM.ptr, int(bf16), None))
+    return oF.to_host(), oM.to_host()
+
+
+def round_bf16(a):
+    """float32 -> nearest BF16 (ties to even), returned as float32."""
+    b = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    r = ((b + 0x7FFF + ((b >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32)

@@ -142,3 +142,106 @@ def test_batch_requests_with_initial_transforms():
         b.submit(bad)
     assert e.value.code == 4
     b.close()
+
+
+def test_cancel_queued_and_running_requests():
+    """A running request stops at its next iteration boundary and a queued
+    one never starts; both come back with ANTS_ERR_CANCELLED, and every other
+    request's result is the same as when run alone, bit for bit."""
+    import time
+    from ants_amd import _lib as L
+    reqs = make_requests(5, seed=31)
+    big = ph.phantom_pair_3d(131, size=(128, 128, 128), max_disp_mm=3.0, svf_sigma_mm=5.0)
+    long_stage = SyNStage.from_args("SyN[0.1,3,0]", "CC[f,m,1,4]", "[100x100x100,1e-9,10]", "4x2x1", "2x1x0vox")
+    running = Request(long_stage, big.moving, fixed=big.fixed, want_fields=True, tag="long")
+    b = Batch(slots=2)
+    rid_run = b.submit(running)
+    # running once the batch reserves its footprint (a 128^3 run takes well over 100 ms)
+    t0 = time.perf_counter()
+    while b.device_bytes() == 0 and time.perf_counter() - t0 < 5.0:
+        time.sleep(0.0005)
+    ids = [b.submit(r) for r in reqs]
+    b.cancel(rid_run)
+    b.cancel(ids[-1])          # still queued behind the others
+    with pytest.raises(L.AntsError):
+        b.cancel(123456)       # unknown id
+    res = {r.id: r for r in b.drain()}
+    b.close()
+    assert res[rid_run].status == L.ANTS_ERR_CANCELLED
+    assert res[ids[-1]].status == L.ANTS_ERR_CANCELLED and res[ids[-1]].slot == -1
+    for rid, req in zip(ids[:-1], reqs[:-1]):
+        r = res[rid]
+        assert r.status == 0, r.error
+        ref = alone(req)
+        np.testing.assert_array_equal(r.fwd, ref.fwd)
+        np.testing.assert_array_equal(r.inv, ref.inv)
+
+
+def test_template_replaced_while_requests_are_pending():
+    """Requests submitted before a template is replaced keep the version of
+    their submission, later ones get the new version (each equal to a run
+    with that fixed image given explicitly)."""
+    pa = ph.phantom_pair_3d(41, size=(40, 44, 36), max_disp_mm=3.0, svf_sigma_mm=5.0)
+    pb = ph.phantom_pair_3d(42, size=(36, 40, 44), max_disp_mm=3.0, svf_sigma_mm=5.0)
+    b = Batch(slots=2)
+    b.set_template(0, pa.fixed)
+    before = [Request(FULL, pa.moving, fixed=None, template_index=0, want_fields=True) for _ in range(3)]
+    ids_a = [b.submit(r) for r in before]
+    b.set_template(0, pb.fixed)            # while the first requests are queued or running
+    after = Request(FULL, pb.moving, fixed=None, template_index=0, want_fields=True)
+    id_b = b.submit(after)
+    res = {r.id: r for r in b.drain()}
+    b.close()
+    ref_a = alone(Request(FULL, pa.moving, fixed=pa.fixed))
+    ref_b = alone(Request(FULL, pb.moving, fixed=pb.fixed))
+    for i in ids_a:
+        assert res[i].status == 0, res[i].error
+        np.testing.assert_array_equal(res[i].fwd, ref_a.fwd)
+    np.testing.assert_array_equal(res[id_b].fwd, ref_b.fwd)
+
+
+def test_device_memory_stays_within_budget():
+    """The device memory a batch holds (hipMemGetInfo before and after, with
+    the slots keeping their grown buffers) stays within the budget plus a
+    fixed overhead of 64 MiB (streams, small per-run blocks the footprint
+    model leaves out), and the batch's own count never exceeds the budget."""
+    import ctypes as C
+    import threading
+    from ants_amd import serving as S
+    reqs = make_requests(8, seed=51)
+    lib = S._bind()
+    fps = []
+    for r in reqs:
+        prm = S.stage_params(r.stage, 3)
+        d = S.L.desc(tuple(reversed(r.fixed.shape)))
+        fps.append(lib.ants_syn_footprint(C.byref(prm), C.byref(d), C.byref(d)))
+    budget = int(2.5 * max(fps))
+    S.L.check(lib.ants_device_synchronize())
+    free0 = S.L.device_free_bytes()
+    b = Batch(slots=4, memory_budget=budget)
+    seen = []
+    stop = threading.Event()
+
+    def sample():
+        while not stop.is_set():
+            seen.append(b.device_bytes())
+            stop.wait(0.002)
+
+    t = threading.Thread(target=sample)
+    t.start()
+    res = run_all(b, reqs)
+    stop.set()
+    t.join()
+    free1 = S.L.device_free_bytes()
+    held = b.device_bytes()
+    b.close()
+    assert all(r.status == 0 for r in res), [r.error for r in res]
+    assert max(seen) <= budget and held <= budget
+    assert free0 - free1 <= budget + (64 << 20), (free0 - free1, budget)
+    # requests that overlap in time never reserve more than the budget together
+    ev = sorted([(r.t_end, -r.footprint_bytes) for r in res] + [(r.t_start, r.footprint_bytes) for r in res])
+    load, peak = 0, 0
+    for _, d in ev:
+        load += d
+        peak = max(peak, load)
+    assert peak <= budget

@@ -157,6 +157,34 @@ def test_invert_speculative_matches_plain(monkeypatch, shape, amp):
     assert spec[2] == plain[2] and spec[3] == plain[3]
 
 
+@pytest.mark.parametrize("bf16", [False, True])
+@pytest.mark.parametrize("fshape,mshape,shrink", [((33, 40, 28), (33, 40, 28), 2), ((29, 31, 37), (25, 35, 30), 1),
+                                                   ((48, 40), (44, 46), 4)])
+def test_warp_pair(bf16, fshape, mshape, shrink):
+    """The iteration's warp kernel (both images through their inverse middle
+    fields onto the level grid, 8-byte corner-pair loads for float images)
+    against the oracle's ResampleImageFilter restatement, float and BF16
+    images (the oracle then warps the BF16-rounded image)."""
+    D = len(fshape)
+    gF = grid_np(fshape, [1.1] * D, [0.5] * D)
+    gM = grid_np(mshape, [0.9] * D, [-1.0] * D)
+    gl = o.shrink_grid(gF, shrink)
+    F = f32(rng.uniform(0, 1, fshape))
+    M = f32(rng.uniform(0, 1, mshape))
+    uF = smooth_random_field(gl.shape, D, 2.0, 3.0)
+    uM = smooth_random_field(gl.shape, D, 2.0, 3.0)
+    d = lambda g: L.desc(g.size, g.spacing, g.origin)
+    oF, oM = G.warp_pair(d(gl), d(gF), F, uF, d(gM), M, uM, gl.shape, bf16)
+    Fi, Mi = (G.round_bf16(F), G.round_bf16(M)) if bf16 else (F, M)
+    rF = o.warp_image(gF, Fi, gl, uF)
+    rM = o.warp_image(gM, Mi, gl, uM)
+    # float continuous indices (non-integer spacing and origin) against double:
+    # a few 1e-6 of a voxel times unit-range gradients
+    np.testing.assert_allclose(oF, rF, atol=1e-5)
+    np.testing.assert_allclose(oM, rM, atol=1e-5)
+    assert (rF == 0).any() or (rM == 0).any()  # some points fall outside a buffer
+
+
 def test_invert_zero_init_constant_shift():
     shape = (16, 18, 20)
     f = np.zeros(shape + (3,), np.float32)

@@ -12,8 +12,9 @@ from ants_amd.registration import SyNRegistration, SyNStage
 
 pytestmark = pytest.mark.gpu
 
-BF16_ENERGY_RTOL = 5e-3
-BF16_FIELD_MAX_VOX = 0.25
+# measured on this 48x56x48 case: energy 5.5e-5 relative, fields 0.011 voxel
+BF16_ENERGY_RTOL = 2e-4
+BF16_FIELD_MAX_VOX = 0.04
 
 
 def per_level(precision, pair, iters):

@@ -510,3 +510,25 @@ def test_syn_edge_cases_vs_oracle(case):
         assert list(np.bincount(res.levels, minlength=3)) == [10, 10, 10]
     if case == "zero_iteration_level":
         assert 1 not in set(res.levels.tolist())
+
+
+def test_strict_tie_rule_against_oracle(monkeypatch):
+    """ITK's inversion max test as written (max > 0.1, ANTS_SYN_INV_STRICT_TIE=1)
+    against the oracle with the same rule, and against the default guarded
+    rule, on the SyN[0.1,...] case whose first inversions start on the 0.1
+    edge."""
+    pair = ph.phantom_pair_3d(7, size=(48, 56, 48), max_disp_mm=4.0, svf_sigma_mm=6.0)
+    stage = SyNStage.from_args("SyN[0.1,3,0]", "CC[f,m,1,4]", "[100x70x20,1e-6,10]", "4x2x1", "2x1x0vox")
+    guarded = run_pair(stage, pair)
+    monkeypatch.setenv("ANTS_SYN_INV_STRICT_TIE", "1")
+    strict = run_pair(stage, pair)
+    P = o.SynParams(shrink=[4, 2, 1], sigma=[2, 1, 0], iterations=[100, 70, 20], learning_rate=0.1, radius=4,
+                    inv_tie_rule=1)
+    g = o.Grid(list(pair.size))
+    ref = o.syn_register(P, g, pair.fixed, g, pair.moving)
+    first = [list(r.inv_iters) for r in strict.trace[:2]]
+    print("strict vs oracle strict:", compare(strict, ref, pair.spacing), "first inner steps", first,
+          "oracle", ref.inv_iters[:2].tolist(), flush=True)
+    d = max(np.abs(strict.fwd - guarded.fwd).max(), np.abs(strict.inv - guarded.inv).max())
+    print(f"strict vs guarded GPU: max field difference {d:.2e} mm, iterations "
+          f"{np.bincount(strict.levels).tolist()} / {np.bincount(guarded.levels).tolist()}", flush=True)

@@ -569,3 +569,46 @@ def test_restored_state_continues_the_stage_exactly():
     c = o.syn_register(o.SynParams(shrink=[1], sigma=[0], iterations=[0], **kw), g, F, g, M, restore=(g, st),
                        want_state=True)
     np.testing.assert_allclose(c.state, st, rtol=0, atol=1e-12)
+
+
+# ---------------------------------------------------------------- per-level results and the tie rule
+def _small_pair_3d():
+    from ants_amd import phantoms as ph
+    return ph.phantom_pair_3d(3, size=(32, 36, 30), max_disp_mm=3.0, svf_sigma_mm=5.0)
+
+
+def test_level_results_equal_truncated_schedules():
+    """Level l's result is the output of the same schedule with no
+    iterations after level l (fields carried through the later grids, then
+    composed), bit for bit; the last level's is the stage output."""
+    pair = _small_pair_3d()
+    g = o.Grid(list(pair.size))
+    iters = [6, 4, 2]
+    P = o.SynParams(shrink=[4, 2, 1], sigma=[2, 1, 0], iterations=iters, learning_rate=0.1, radius=2,
+                    force_iterations=True)
+    full = o.syn_register(P, g, pair.fixed, g, pair.moving, want_levels=True)
+    np.testing.assert_array_equal(full.level_fwd[-1], full.fwd)
+    np.testing.assert_array_equal(full.level_inv[-1], full.inv)
+    for lv in range(2):
+        it = [n if k <= lv else 0 for k, n in enumerate(iters)]
+        Pt = o.SynParams(shrink=[4, 2, 1], sigma=[2, 1, 0], iterations=it, learning_rate=0.1, radius=2,
+                         force_iterations=True)
+        t = o.syn_register(Pt, g, pair.fixed, g, pair.moving)
+        np.testing.assert_array_equal(full.level_fwd[lv], t.fwd)
+        np.testing.assert_array_equal(full.level_inv[lv], t.inv)
+        assert np.abs(full.level_fwd[lv] - full.fwd).max() > 1e-4  # later levels changed the field
+
+
+def test_strict_tie_rule_only_differs_at_the_tie():
+    """The guarded and strict inversion max tests differ only when the max
+    norm lies within 1e-5 relative above 0.1: with a learning rate of 0.2 no
+    inversion starts on that edge, and the two rules give the same bits."""
+    pair = _small_pair_3d()
+    g = o.Grid(list(pair.size))
+    runs = []
+    for rule in (0, 1):
+        P = o.SynParams(shrink=[4, 2, 1], sigma=[2, 1, 0], iterations=[5, 3, 2], learning_rate=0.2, radius=2,
+                        force_iterations=True, inv_tie_rule=rule)
+        runs.append(o.syn_register(P, g, pair.fixed, g, pair.moving))
+    np.testing.assert_array_equal(runs[0].fwd, runs[1].fwd)
+    np.testing.assert_array_equal(runs[0].inv_iters, runs[1].inv_iters)
