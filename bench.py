@@ -12,7 +12,13 @@ launch their iterations kernel by kernel so that the events can bracket the
 dominant class (on every 4th iteration of each level: an event between two
 launches stalls the second); `untimed_replay` adds the same registration
 without events, launched kernel by kernel (the library's default) and
-replayed from iteration graphs, measured after the timed region.
+replayed from iteration graphs, measured after the timed region;
+`host_buffer_drop_in` the same registration through the host-buffer entry
+(PCIe transfers included).  The other half of the metric comes from two
+bounded legs on the same GPU: `batch` (64 mixed-size pairs, batched versus
+one at a time) and `serve` (a Poisson request stream at a fixed QPS: p50/p99
+latency, pairs/s and capacity), summarised in `batch_summary` and
+`serve_summary`.
 """
 from __future__ import annotations
 
@@ -43,13 +49,16 @@ def log(*a):
 # iterations of each level whose dominant-class launches are timed with events
 SPAN_EVERY = 4
 
-def cpu_baseline(pair):
-    """The oracle's FP32 build of the same algorithm, all host threads, on a
-    truncated schedule of the same inputs (levels 4x2x1, iterations 40x20x10:
-    about 1.1e8 voxel-iterations, some 10-15 s on 16 threads)."""
+def cpu_baseline(pair, gpu_iters, scale=0.5):
+    """The oracle's FP32 build of the same algorithm (window sums and
+    coordinates in double: a restatement, not ANTs), all host threads, on
+    the same inputs with the GPU run's per-level iteration counts scaled by
+    `scale` and forced, so that the level mix matches the timed GPU work
+    (about 1.2e8 voxel-iterations, some 10-15 s on 16 threads).  Rates are
+    reported per level from the oracle's per-iteration wall times."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as o
-    iters = [40, 20, 10]
+    iters = [max(1, int(round(scale * n))) if n > 0 else 0 for n in gpu_iters]
     P = o.SynParams(shrink=[4, 2, 1], sigma=[2, 1, 0], iterations=iters, learning_rate=0.1, update_variance=3.0,
                     total_variance=0.0, radius=4, force_iterations=True)
     g = o.Grid(list(pair.size), list(pair.spacing), list(pair.origin))
@@ -59,10 +68,18 @@ def cpu_baseline(pair):
     dt = time.perf_counter() - t0
     vox = [o.shrink_grid(g, s).count for s in P.shrink]
     work = float(sum(v * it for v, it in zip(vox, iters)))
+    per_level = {}
+    for lv in range(3):
+        sec = float(r.seconds[r.level == lv].sum())
+        if sec > 0:
+            per_level[str(lv)] = {"voxels": vox[lv], "iterations": iters[lv], "seconds": sec,
+                                  "voxel_iters_per_s": vox[lv] * iters[lv] / sec}
     return {"value": work / dt, "unit": "voxel-iters/s", "cores": threads, "kind": "port",
-            "sample": f"192x224x192 pair, schedule 40x20x10 (of 100x70x20), FP32 CPU restatement, "
-                      f"{threads} OpenMP threads, {work:.3g} voxel-iterations in {dt:.1f} s",
-            "seconds": dt}
+            "sample": f"the 192x224x192 pair with the GPU run's per-level iteration counts {list(gpu_iters)} "
+                      f"scaled by {scale} and forced ({iters}); FP32 CPU restatement of the SyN stage "
+                      f"(oracle/syn_oracle.cpp, window sums and coordinates in double, not ANTs), {threads} OpenMP "
+                      f"threads, {work:.3g} voxel-iterations in {dt:.1f} s",
+            "per_level": per_level, "seconds": dt}
 
 
 def measured_traffic(kernel: str):
@@ -99,19 +116,22 @@ def _pool_pairs(specs):
         return list(ex.map(_gen_pair, specs))
 
 
-def bench_batch(args):
-    """64 independent 3-D pairs, dims U{96..192} per axis; 16 quick pairs
-    (CC r=2, 40x20x0) and 48 with the 192x224x192 schedule.  Batched slots
-    versus the same pairs one at a time on the same GPU."""
+QUICK_ARGS = ("SyN[0.1,3,0]", "CC[f,m,1,2]", "[40x20x0,1e-6,10]", "4x2x1", "2x1x0vox")
+
+
+def batch_leg(args) -> dict:
+    """C4: 64 independent 3-D pairs, dims U{96..192} per axis; 16 quick pairs
+    (CC r=2, 40x20x0) and 48 with the 192x224x192 schedule.  The batched
+    scheduler versus the same pairs one at a time on the same GPU."""
     from ants_amd.registration import SyNStage
-    from ants_amd.serving import Batch, Request, latency_stats, run_all
+    from ants_amd.serving import Batch, Request, run_all
     rng = np.random.default_rng(args.seed)
     n = args.pairs
     specs = [(1000 + k, tuple(int(v) for v in rng.integers(96, 193, 3))) for k in range(n)]
     t0 = time.perf_counter()
     data = _pool_pairs(specs)
-    log(f"generated {n} pairs in {time.perf_counter() - t0:.1f} s")
-    quick = SyNStage.from_args("SyN[0.1,3,0]", "CC[f,m,1,2]", "[40x20x0,1e-6,10]", "4x2x1", "2x1x0vox")
+    log(f"batch: generated {n} pairs in {time.perf_counter() - t0:.1f} s")
+    quick = SyNStage.from_args(*QUICK_ARGS)
     full = SyNStage.from_args(**SCHEDULE)
     reqs = [Request(quick if k < n // 4 else full, mov, fixed=fix, precision=args.precision)
             for k, (fix, mov, _) in enumerate(data)]
@@ -129,27 +149,26 @@ def bench_batch(args):
             raise RuntimeError(bad[0])
         vi = sum(r.voxel_iterations for r in res)
         by = sum(r.algorithmic_bytes for r in res)
-        out[name] = {"slots": slots, "precision": rq[0].precision, "wall_s": wall, "pairs_per_s": n / wall, "voxel_iters_per_s": vi / wall,
-                     "algorithmic_gbs": by / wall / 1e9, "hbm_frac": by / wall / 1e9 / HBM_PEAK_GBS}
-        log(f"{name}: {n} pairs in {wall:.2f} s ({n / wall:.2f} pairs/s)")
-    line = {"metric": "batched SyN pairs/s (64 mixed-size 3-D pairs)", "value": out["batched"]["pairs_per_s"],
-            "unit": "pairs/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
-            "config": {"workload": f"{n} phantom pairs, dims U{{96..192}}^3, {n // 4} quick (r=2, 40x20x0), "
-                                   f"{n - n // 4} at r=4 100x70x20", "slots": args.slots},
+        out[name] = {"slots": slots, "precision": rq[0].precision, "wall_s": wall, "pairs_per_s": n / wall,
+                     "voxel_iters_per_s": vi / wall, "algorithmic_gbs": by / wall / 1e9,
+                     "hbm_frac": by / wall / 1e9 / HBM_PEAK_GBS}
+        log(f"batch {name}: {n} pairs in {wall:.2f} s ({n / wall:.2f} pairs/s)")
+    return {"workload": f"{n} phantom pairs, dims U{{96..192}}^3, {n // 4} quick (r=2, 40x20x0), "
+                        f"{n - n // 4} at r=4 100x70x20, images uploaded from host per request",
             "batched": out["batched"], "one_at_a_time": out["one_at_a_time"],
             "speedup_vs_one_at_a_time": out["batched"]["pairs_per_s"] / out["one_at_a_time"]["pairs_per_s"]}
-    print(json.dumps(line), flush=True)
 
 
-def bench_serve(args):
-    """Poisson request stream at a fixed QPS: 70% quick SyN at 128^3 (r=2,
-    40x20x0), 30% SyN at 192x224x192 (r=4, 100x70x20), both against resident
+def serve_leg(args) -> dict:
+    """C5: Poisson request stream at a fixed QPS: 70% quick SyN at 128^3 (r=2,
+    40x20x0), 30% SyN at 192x224x192 (r=4, 100x70x20), against resident
     templates, each warping a 32-label Voronoi map nearest-neighbour.  The
-    continuous-batching scheduler versus one request at a time."""
+    continuous-batching scheduler versus one request at a time: latency at the
+    offered QPS, and capacity (every request queued at once)."""
     from ants_amd import phantoms as ph
     from ants_amd.registration import SyNStage
-    from ants_amd.serving import Batch, Request, latency_stats, serve
-    rng = np.random.default_rng(args.seed)
+    from ants_amd.serving import Batch, Request, latency_stats, run_all, serve
+    rng = np.random.default_rng(args.seed + 7)
     n = args.requests
     kinds = rng.random(n) < 0.7
     t0 = time.perf_counter()
@@ -157,37 +176,54 @@ def bench_serve(args):
     small_t = ph.phantom_pair_3d(5001, size=(128, 128, 128))
     specs = [(6000 + k, (128, 128, 128) if kinds[k] else (192, 224, 192), 32) for k in range(n)]
     data = _pool_pairs(specs)
-    labels = [lab for _, _, lab in data]
-    log(f"generated templates and {n} requests in {time.perf_counter() - t0:.1f} s")
-    quick = SyNStage.from_args("SyN[0.1,3,0]", "CC[f,m,1,2]", "[40x20x0,1e-6,10]", "4x2x1", "2x1x0vox")
+    log(f"serve: generated templates and {n} requests in {time.perf_counter() - t0:.1f} s")
+    quick = SyNStage.from_args(*QUICK_ARGS)
     full = SyNStage.from_args(**SCHEDULE)
-    reqs = [Request(quick if kinds[k] else full, data[k][1], fixed=None, template_index=1 if kinds[k] else 0,
-                    moving_labels=labels[k], precision=args.precision) for k in range(n)]
+
+    def requests(precision):
+        return [Request(quick if kinds[k] else full, data[k][1], fixed=None, template_index=1 if kinds[k] else 0,
+                        moving_labels=data[k][2], precision=precision) for k in range(n)]
+
     arrivals = np.cumsum(rng.exponential(1.0 / args.qps, n))
     arrivals -= arrivals[0]
-    plain = [Request(r.stage, r.moving, fixed=None, template_index=r.template_index, moving_labels=r.moving_labels,
-                     precision="fp32") for r in reqs]
     out = {}
-    for name, slots, rq in (("continuous_batching", args.slots, reqs), ("one_at_a_time", 1, plain)):
+    for name, slots, prec in (("continuous_batching", args.slots, args.precision), ("one_at_a_time", 1, "fp32")):
         b = Batch(slots=slots)
         b.set_template(0, big_t.fixed)
         b.set_template(1, small_t.fixed)
-        serve(b, rq[:2], [0.0, 0.0])  # warm-up
-        res = serve(b, rq, arrivals)
-        b.close()
+        run_all(b, requests(prec)[:2])  # warm-up
+        res = serve(b, requests(prec), arrivals)
         bad = [r.error for r in res if r.status != 0]
         if bad:
             raise RuntimeError(bad[0])
         st = latency_stats(res)
-        st["slots"] = slots
-        st["precision"] = rq[0].precision
+        cap = latency_stats(run_all(b, requests(prec)))
+        b.close()
+        st.update({"slots": slots, "precision": prec, "capacity_pairs_per_s": cap["pairs_per_s"]})
         out[name] = st
-        log(f"{name}: p50 {st['p50_s'] * 1e3:.1f} ms p99 {st['p99_s'] * 1e3:.1f} ms {st['pairs_per_s']:.2f} pairs/s")
-    line = {"metric": "served SyN: pairs/s and p50/p99 latency at fixed QPS", "value": out["continuous_batching"]["pairs_per_s"],
+        log(f"serve {name}: p50 {st['p50_s'] * 1e3:.1f} ms p99 {st['p99_s'] * 1e3:.1f} ms {st['pairs_per_s']:.2f} "
+            f"pairs/s at {args.qps} QPS; capacity {cap['pairs_per_s']:.2f} pairs/s")
+    return {"workload": f"{n} Poisson requests at {args.qps} QPS: 70% 128^3 r=2 40x20x0, 30% 192x224x192 r=4 "
+                        "100x70x20, resident templates, 32-label NN warp",
+            "qps": args.qps, "continuous_batching": out["continuous_batching"], "one_at_a_time": out["one_at_a_time"],
+            "capacity_speedup_vs_one_at_a_time": out["continuous_batching"]["capacity_pairs_per_s"]
+            / out["one_at_a_time"]["capacity_pairs_per_s"]}
+
+
+def bench_batch(args):
+    leg = batch_leg(args)
+    line = {"metric": "batched SyN pairs/s (64 mixed-size 3-D pairs)", "value": leg["batched"]["pairs_per_s"],
             "unit": "pairs/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
-            "config": {"workload": f"{n} Poisson requests at {args.qps} QPS: 70% 128^3 r=2 40x20x0, 30% 192x224x192 "
-                                   "r=4 100x70x20, resident templates, 32-label NN warp", "slots": args.slots},
-            "continuous_batching": out["continuous_batching"], "one_at_a_time": out["one_at_a_time"]}
+            "config": {"workload": leg["workload"], "slots": args.slots}, **leg}
+    print(json.dumps(line), flush=True)
+
+
+def bench_serve(args):
+    leg = serve_leg(args)
+    line = {"metric": "served SyN: pairs/s and p50/p99 latency at fixed QPS",
+            "value": leg["continuous_batching"]["pairs_per_s"], "unit": "pairs/s", "n_gpus": 1,
+            "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": leg["workload"], "slots": args.slots}, **leg}
     print(json.dumps(line), flush=True)
 
 
@@ -274,6 +310,8 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="pair mode: first copy MIB MiB three times with the calibration copy kernel "
                          "(the known-size reference for HBM byte counters)")
+    ap.add_argument("--no-batch-serve", action="store_true",
+                    help="pair mode: skip the batched (64 pairs) and served (Poisson stream) legs")
     ap.add_argument("--hw-queues", type=int, default=0,
                     help="HIP hardware queues (0: the runtime's default of 4, which measured fastest for 8 slots)")
     args = ap.parse_args()
@@ -377,6 +415,20 @@ def main():
 
     plain_replay = untimed_replay(False)
     graph_replay = untimed_replay(True)
+
+    # the drop-in with host buffers (ants_syn_register): uploads of both
+    # images and downloads of both fields over PCIe included; never `value`
+    reg.register(pair.fixed, pair.moving, pair.spacing, pair.origin, pair.spacing, pair.origin)
+    th = []
+    for _ in range(min(args.steps, 5)):
+        t0 = time.perf_counter()
+        reg.register(pair.fixed, pair.moving, pair.spacing, pair.origin, pair.spacing, pair.origin)
+        th.append(time.perf_counter() - t0)
+    host_ms = sum(th) / len(th) * 1e3
+    host_drop_in = {"ms_per_step": host_ms, "value": reg.stats()["voxel_iterations"] / (host_ms * 1e-3),
+                    "note": "ants_syn_register with host buffers: 2 image uploads and 2 field downloads over PCIe "
+                            "included"}
+    log(f"host-buffer drop-in: {host_ms:.1f} ms per registration")
     if saved_env is None:
         os.environ.pop("ANTS_SYN_GRAPHS", None)
     else:
@@ -390,9 +442,15 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("timing the CPU restatement on a bounded sample ...")
-        cpu = cpu_baseline(pair)
+        cpu = cpu_baseline(pair, st["iterations"][:3])
         log(f"cpu baseline: {cpu['value']:.4g} voxel-iters/s in {cpu['seconds']:.1f} s")
         cpu.pop("seconds")
+
+    legs = {}
+    if not args.no_batch_serve:
+        reg.close()
+        legs["batch"] = batch_leg(args)
+        legs["serve"] = serve_leg(args)
 
     line = {
         "metric": METRIC,
@@ -424,7 +482,15 @@ def main():
         "cpu_baseline": cpu,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
         "kernel_breakdown_ms": {k: round(v["ms"], 3) for k, v in ks.items()},
+        "host_buffer_drop_in": host_drop_in,
     }
+    if legs:
+        sv = legs["serve"]["continuous_batching"]
+        line["serve_summary"] = {"qps": args.qps, "pairs_per_s": sv["pairs_per_s"], "p50_ms": sv["p50_s"] * 1e3,
+                                 "p99_ms": sv["p99_s"] * 1e3, "capacity_pairs_per_s": sv["capacity_pairs_per_s"]}
+        line["batch_summary"] = {"pairs_per_s": legs["batch"]["batched"]["pairs_per_s"],
+                                 "speedup_vs_one_at_a_time": legs["batch"]["speedup_vs_one_at_a_time"]}
+        line.update(legs)
     print(json.dumps(line), flush=True)
 
 

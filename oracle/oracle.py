@@ -52,6 +52,7 @@ class OgTrace(C.Structure):
         ("energy", C.POINTER(C.c_double)),
         ("convergence", C.POINTER(C.c_double)),
         ("inv_iters", C.POINTER(C.c_int)),
+        ("seconds", C.POINTER(C.c_double)),
     ]
 
 
@@ -317,6 +318,7 @@ class SynResult:
     # (n_levels, ...) results of the schedule stopped after each level
     level_fwd: Optional[np.ndarray] = None
     level_inv: Optional[np.ndarray] = None
+    seconds: Optional[np.ndarray] = None     # wall time of each iteration record
 
 
 def syn_register(params: SynParams, gF: Grid, fixed, gM: Grid, moving, precision: str = "f64",
@@ -342,6 +344,7 @@ def syn_register(params: SynParams, gF: Grid, fixed, gM: Grid, moving, precision
     en = np.zeros(max_records, np.float64)
     cv = np.zeros(max_records, np.float64)
     ki = np.zeros(max_records * 4, np.int32)
+    secs = np.zeros(max_records, np.float64)
     tr = OgTrace()
     tr.max_records = max_records
     tr.n_records = 0
@@ -350,6 +353,7 @@ def syn_register(params: SynParams, gF: Grid, fixed, gM: Grid, moving, precision
     tr.energy = en.ctypes.data_as(C.POINTER(C.c_double))
     tr.convergence = cv.ctypes.data_as(C.POINTER(C.c_double))
     tr.inv_iters = ki.ctypes.data_as(C.POINTER(C.c_int))
+    tr.seconds = secs.ctypes.data_as(C.POINTER(C.c_double))
     p = params.c()
     P = C.POINTER(ct)
     lvf = lvi = None
@@ -399,7 +403,7 @@ def syn_register(params: SynParams, gF: Grid, fixed, gM: Grid, moving, precision
         raise RuntimeError(f"oracle syn_register failed: {rc}")
     n = tr.n_records
     return SynResult(fwd, inv, lv[:n].copy(), it[:n].copy(), en[:n].copy(), cv[:n].copy(),
-                     ki[: 4 * n].reshape(n, 4).copy(), state, lvf, lvi)
+                     ki[: 4 * n].reshape(n, 4).copy(), state, lvf, lvi, secs[:n].copy())
 
 
 def max_threads() -> int:

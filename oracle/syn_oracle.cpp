@@ -20,6 +20,7 @@
 // Parity status per function is stated in the comment above it.
 
 #include <algorithm>
+#include <chrono>
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
@@ -83,6 +84,7 @@ typedef struct {
   double* energy;
   double* convergence;
   int* inv_iters;        // 4 per record: F2M inv, F2M fwd, M2M inv, M2M fwd
+  double* seconds;       // optional: wall time of each iteration
 } og_trace;
 }
 
@@ -870,6 +872,7 @@ int syn_register(const og_params* P, const Grid& gF, const R* fixed_in, const Gr
     bool converged = false;
     int iter = 0;
     while (iter++ < P->iterations[l] && !converged) {
+      const auto t_iter = std::chrono::steady_clock::now();
       warp_image<R>(gF, Fs.data(), gl, f2m_i.data(), Fw.data(), fx, nfx);
       warp_image<R>(gM, Ms.data(), gl, m2m_i.data(), Mw.data(), mx, nmx);
       const double value = cc_metric<R>(gl, Fw.data(), Mw.data(), P->radius, eps, g1.data(), g2.data());
@@ -902,6 +905,8 @@ int syn_register(const og_params* P, const Grid& gF, const R* fixed_in, const Gr
         trace->energy[r] = E;
         trace->convergence[r] = conv;
         for (int j = 0; j < 4; ++j) trace->inv_iters[r * 4 + j] = k[j];
+        if (trace->seconds)
+          trace->seconds[r] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_iter).count();
       }
     }
     if (!snap.empty()) {
