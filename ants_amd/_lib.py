@@ -139,6 +139,7 @@ def lib():
     L.ants_syn_set_profiling.argtypes = [vp, C.c_uint]
     L.ants_syn_set_profiling_stride.argtypes = [vp, i]
     L.ants_syn_get_kernel_stats.argtypes = [vp, C.POINTER(KernelStatsC)]
+    L.ants_syn_get_kernel_stats_level.argtypes = [vp, i, C.POINTER(KernelStatsC)]
     L.ants_op_smooth_image.argtypes = [pdesc, vp, vp, d, i, vp]
     L.ants_op_smooth_field.argtypes = [pdesc, vp, vp, d, vp]
     L.ants_op_warp_image.argtypes = [pdesc, vp, pdesc, vp, vp, vp]

@@ -341,6 +341,8 @@ struct Profiler {
   int every = 1;            // time iterations 1, 1 + every, 1 + 2 every, ... of each level
   hipStream_t s = nullptr;
   ants_syn_kernel_stats acc{};
+  ants_syn_kernel_stats acc_level[ANTS_MAX_LEVELS]{};  // the same per pyramid level
+  int level = 0;                                       // level of the spans being recorded
   std::vector<hipEvent_t> pool;
   size_t used = 0;
   struct Rec {
@@ -349,6 +351,7 @@ struct Profiler {
     double bytes;
     int launches;
     int batches, j0, j1;  // inversion spans: batch bit mask, steps [j0, j1)
+    int level;
   };
   std::vector<Rec> recs;
   hipEvent_t a_ = nullptr;
@@ -373,7 +376,7 @@ struct Profiler {
     hipEvent_t b = next();
     if (!b) return;
     (void)hipEventRecord(b, s);
-    recs.push_back(Rec{cls, a_, b, bytes, launches, batches, j0, j1});
+    recs.push_back(Rec{cls, a_, b, bytes, launches, batches, j0, j1, level});
     a_ = nullptr;
   }
   // stats: the iteration's statistics block (host copy); N voxels, D dims
@@ -415,6 +418,10 @@ struct Profiler {
       acc.ms[r.cls] += ms;
       acc.launches[r.cls] += r.launches;
       acc.bytes[r.cls] += bytes;
+      ants_syn_kernel_stats& al = acc_level[r.level];
+      al.ms[r.cls] += ms;
+      al.launches[r.cls] += r.launches;
+      al.bytes[r.cls] += bytes;
     }
     recs.clear();
     used = 0;
@@ -831,7 +838,7 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
   ANTS_TRY(stats.ensure(sizeof(IterStats)));
   int64_t ccp = 0, invp = 0, invw = 0;
   for (int l = 0; l < P.n_levels; ++l) {
-    ccp = std::max(ccp, cc_partials_needed(to_dev(lg[l]), P.cc_radius));
+    ccp = std::max(ccp, cc_partials_needed(to_dev(lg[l]), P.cc_radius, bf16));
     invp = std::max(invp, inv_partials_needed(to_dev(lg[l])));
     invw = std::max(invw, inv_mask_words(to_dev(lg[l])));
   }
@@ -895,6 +902,7 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
   std::memset(&run_stats, 0, sizeof(run_stats));
   trace.clear();
   std::memset(&prof.acc, 0, sizeof(prof.acc));
+  std::memset(prof.acc_level, 0, sizeof(prof.acc_level));
   prof.s = s;
   prof.recs.clear();
   prof.used = 0;
@@ -945,6 +953,7 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
       for (int k = 0; k < 4; ++k) std::swap(st[k], wk[k]);
     }
     prev = g;
+    prof.level = l;
     run_stats.levels_run = l + 1;
     run_stats.level_voxels[l] = N;
     if (P.iterations[l] <= 0) continue;
@@ -1049,7 +1058,7 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
       prof.begin(ANTS_KC_CC);
       // the energy slot is reduced when the iteration's statistics are published
       const int n_cc = launch_cc(gd, Fw.f(), Mw.f(), P.cc_radius, eps, wk_[0], wk_[1],
-                                 static_cast<Partial*>(partials_cc.p), &dstats->energy, s, false);
+                                 static_cast<Partial*>(partials_cc.p), &dstats->energy, s, false, bf16);
       prof.end(ANTS_KC_CC, Nd * (2.0 * e_i + 2.0 * D * e_f));
       HIP_TRY(hipGetLastError());
       // 3. update-field smoothing + max norms (result in wk[0], wk[1]; scratch wk[2], wk[3])
@@ -1444,6 +1453,13 @@ int ants_syn_set_profiling_stride(ants_syn_handle* h, int every) {
 int ants_syn_get_kernel_stats(const ants_syn_handle* h, ants_syn_kernel_stats* out) {
   if (!h || !out) return fail(ANTS_ERR_INVALID, "null argument");
   *out = h->prof.acc;
+  return ANTS_OK;
+}
+
+int ants_syn_get_kernel_stats_level(const ants_syn_handle* h, int level, ants_syn_kernel_stats* out) {
+  if (!h || !out) return fail(ANTS_ERR_INVALID, "null argument");
+  if (level < 0 || level >= ANTS_MAX_LEVELS) return fail(ANTS_ERR_INVALID, "level out of range");
+  *out = h->prof.acc_level[level];
   return ANTS_OK;
 }
 

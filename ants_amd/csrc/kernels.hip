@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 
 namespace ants {
 namespace {
@@ -1045,6 +1046,14 @@ constexpr bool kCCRotate = ANTS_CC_ROTATE != 0;
 #define ANTS_CC_WAVES 1
 #endif
 constexpr int kCCWavesPerEU = ANTS_CC_WAVES;  // occupancy floor (waves per SIMD)
+#ifndef ANTS_CC_WAVES_FLOAT
+#define ANTS_CC_WAVES_FLOAT 1
+#endif
+constexpr int kCCWavesPerEUFloat = ANTS_CC_WAVES_FLOAT;  // the same for float box sums
+template <class ST>
+constexpr int cc_waves() {
+  return std::is_same<ST, float>::value ? kCCWavesPerEUFloat : kCCWavesPerEU;
+}
 // y box sums: 0 = each thread sums its W rows (5 W LDS reads per voxel);
 // S > 0 = a task walks S output rows of one column and quantity with a
 // running sum (W + 2 (S - 1) reads per S outputs) into a staging buffer.
@@ -1057,17 +1066,27 @@ constexpr int kCCYSeg = ANTS_CC_YSEG;
 #define ANTS_CC_XSEG 2
 #endif
 constexpr int kCCXSeg = ANTS_CC_XSEG;
-template <int R, int TX, int TY>
-__global__ void __launch_bounds__(TX* TY, kCCWavesPerEU) k_cc3d(GridDev g, const float* __restrict__ F, const float* __restrict__ M,
+//
+// ST is the type of the box sums.  double: products of float samples are
+// exact and every sum keeps its digits.  float: the samples are stored
+// shifted by a block constant (the block's centre sample of F and of M), so
+// that sFF = sum f'^2 - (sum f')^2 / n, which is invariant to the shift,
+// does not cancel in flat windows near that value; the z window sums are
+// re-added from the ring in a fixed order every plane (no running sum to
+// drift), and the per-voxel formula is evaluated in double.  Out-of-volume
+// samples stay 0, so the shifted sums are the in-bounds sums of f - c.
+template <int R, int TX, int TY, class ST = double>
+__global__ void __launch_bounds__(TX* TY, cc_waves<ST>()) k_cc3d(GridDev g, const float* __restrict__ F, const float* __restrict__ M,
                                                  float eps, int zchunk, float* __restrict__ g1,
                                                  float* __restrict__ g2, Partial* partials) {
   constexpr int W = 2 * R + 1;
   constexpr int LX = TX + 2 * R, LY = TY + 2 * R;
   constexpr int NT = TX * TY;
+  constexpr bool kShift = std::is_same<ST, float>::value;
   __shared__ float q[2][LY][LX];
-  __shared__ double sx[5][LY][TX];
+  __shared__ ST sx[5][LY][TX];
   constexpr int SYR = kCCYSeg > 0 ? TY : 1;
-  __shared__ double sy[kCCYSeg > 0 ? 5 : 1][SYR][TX];
+  __shared__ ST sy[kCCYSeg > 0 ? 5 : 1][SYR][TX];
   const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
   const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
   int btx, bty, btz;
@@ -1086,12 +1105,13 @@ __global__ void __launch_bounds__(TX* TY, kCCWavesPerEU) k_cc3d(GridDev g, const
   // s % W (compile-time indices, no register moves); otherwise the rings
   // shift by one slot per plane.
   constexpr int NRF = kCCRotate ? W : R + 2, NRG = kCCRotate ? W : R + 1;
-  double ring[5][W];
-  double zsum[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  static_assert(kCCRotate || !kShift, "float sums need the rotating ring");
+  ST ring[5][W];
+  ST zsum[5] = {ST(0), ST(0), ST(0), ST(0), ST(0)};
 #pragma unroll
   for (int a = 0; a < 5; ++a)
 #pragma unroll
-    for (int b = 0; b < W; ++b) ring[a][b] = 0.0;
+    for (int b = 0; b < W; ++b) ring[a][b] = ST(0);
   float rf[NRF], rm[NRF];          // raw centre values
   float rgf[NRG][2], rgm[NRG][2];  // in-plane gradients
 #pragma unroll
@@ -1100,6 +1120,16 @@ __global__ void __launch_bounds__(TX* TY, kCCWavesPerEU) k_cc3d(GridDev g, const
   for (int b = 0; b < NRG; ++b) rgf[b][0] = rgf[b][1] = rgm[b][0] = rgm[b][1] = 0.f;
   double cc_sum = 0.0;
   const int64_t plane = static_cast<int64_t>(nx) * ny;
+  // block shift (float sums): the centre sample of the block's columns in
+  // the middle plane of its chunk, the same address for every thread
+  float cF = 0.f, cM = 0.f;
+  if constexpr (kShift) {
+    const int xc = min(x0 + TX / 2, nx - 1), yc = min(y0 + TY / 2, ny - 1);
+    const int zc0 = min(z0 + (z1 - z0) / 2, nz - 1);
+    const int64_t ic = static_cast<int64_t>(zc0) * plane + static_cast<int64_t>(yc) * nx + xc;
+    cF = F[ic];
+    cM = M[ic];
+  }
 
   // Plane samples are prefetched into registers one plane ahead, so their
   // global-load latency overlaps the box sums of the current plane.
@@ -1123,8 +1153,8 @@ __global__ void __launch_bounds__(TX* TY, kCCWavesPerEU) k_cc3d(GridDev g, const
     for (int u = 0; u < KP; ++u) {
       float f = 0.f, m = 0.f;
       if (zin_q && goff[u] >= 0) {
-        f = F[zo + goff[u]];
-        m = M[zo + goff[u]];
+        f = F[zo + goff[u]] - cF;
+        m = M[zo + goff[u]] - cM;
       }
       pf[u] = f;
       pm[u] = m;
@@ -1153,10 +1183,10 @@ __global__ void __launch_bounds__(TX* TY, kCCWavesPerEU) k_cc3d(GridDev g, const
     static_assert(TX % XS == 0 && XS >= 2, "x sums run in segments");
     for (int task = threadIdx.x; task < LY * (TX / XS); task += NT) {
       const int ly = task / (TX / XS), xs = XS * (task - ly * (TX / XS));
-      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;
+      ST s0 = ST(0), s1 = ST(0), s2 = ST(0), s3 = ST(0), s4 = ST(0);
 #pragma unroll
       for (int b = 0; b < W; ++b) {
-        const double f = q[0][ly][xs + b], m = q[1][ly][xs + b];
+        const ST f = q[0][ly][xs + b], m = q[1][ly][xs + b];
         s0 += f;
         s1 += m;
         s2 += f * f;
@@ -1170,8 +1200,8 @@ __global__ void __launch_bounds__(TX* TY, kCCWavesPerEU) k_cc3d(GridDev g, const
       sx[4][ly][xs] = s4;
 #pragma unroll
       for (int e = 1; e < XS; ++e) {
-        const double fo = q[0][ly][xs + e - 1], mo = q[1][ly][xs + e - 1];
-        const double fn = q[0][ly][xs + e - 1 + W], mn = q[1][ly][xs + e - 1 + W];
+        const ST fo = q[0][ly][xs + e - 1], mo = q[1][ly][xs + e - 1];
+        const ST fn = q[0][ly][xs + e - 1 + W], mn = q[1][ly][xs + e - 1 + W];
         s0 += fn - fo;
         s1 += mn - mo;
         s2 += fn * fn - fo * fo;
@@ -1194,7 +1224,7 @@ __global__ void __launch_bounds__(TX* TY, kCCWavesPerEU) k_cc3d(GridDev g, const
       gmy = cdiff(q[1][ty + R - 1][tx + R], mc, q[1][ty + R + 1][tx + R], y, ny, g.sp[1], g.isp[1]);
     }
     __syncthreads();
-    double p[5];
+    ST p[5];
     if constexpr (kCCYSeg > 0) {
       constexpr int S = kCCYSeg;
       static_assert(TY % S == 0, "y segments tile the block's rows");
@@ -1204,7 +1234,7 @@ __global__ void __launch_bounds__(TX* TY, kCCWavesPerEU) k_cc3d(GridDev g, const
         const int r = task - a * (TX * (TY / S));
         const int k = r / TX, cx = r - k * TX;
         const int yr = k * S;
-        double acc = 0.0;
+        ST acc = ST(0);
 #pragma unroll
         for (int b = 0; b < W; ++b) acc += sx[a][yr + b][cx];
         sy[a][yr][cx] = acc;
@@ -1220,7 +1250,7 @@ __global__ void __launch_bounds__(TX* TY, kCCWavesPerEU) k_cc3d(GridDev g, const
     } else {
 #pragma unroll
       for (int a = 0; a < 5; ++a) {
-        double t = 0.0;
+        ST t = ST(0);
 #pragma unroll
         for (int b = 0; b < W; ++b) t += sx[a][ty + b][tx];
         p[a] = t;
@@ -1232,8 +1262,17 @@ __global__ void __launch_bounds__(TX* TY, kCCWavesPerEU) k_cc3d(GridDev g, const
     if constexpr (kCCRotate) {
 #pragma unroll
       for (int a = 0; a < 5; ++a) {
-        zsum[a] += p[a] - ring[a][ph];
-        ring[a][ph] = p[a];
+        if constexpr (kShift) {
+          // float: the window total re-added from the slots in slot order
+          ring[a][ph] = p[a];
+          ST t = ST(0);
+#pragma unroll
+          for (int b = 0; b < W; ++b) t += ring[a][b];
+          zsum[a] = t;
+        } else {
+          zsum[a] += p[a] - ring[a][ph];
+          ring[a][ph] = p[a];
+        }
       }
       rf[ph] = fc;
       rm[ph] = mc;
@@ -1289,7 +1328,9 @@ __global__ void __launch_bounds__(TX* TY, kCCWavesPerEU) k_cc3d(GridDev g, const
         rc = rc_in;
       else
         rc = 1.0 / (cxy * cz);
-      const CCOut o = cc_voxel_rc(rc, zsum[0], zsum[1], zsum[2], zsum[3], zsum[4], f, m, eps);
+      const CCOut o = cc_voxel_rc(rc, static_cast<double>(zsum[0]), static_cast<double>(zsum[1]),
+                                  static_cast<double>(zsum[2]), static_cast<double>(zsum[3]),
+                                  static_cast<double>(zsum[4]), f, m, eps);
       cc_sum += o.cc;
       float gmz = 0.f, gfz = 0.f;
       if (o.a1 != 0.f || o.a2 != 0.f) {
@@ -2012,16 +2053,35 @@ int zchunk_for(int nz, int64_t columns, int halo, int64_t slots) {
   return best_zc;
 }
 
-int cc_zchunk(const GridDev& g, int radius) {
+// Box sums of the tiled CC: double unless the caller asks for float sums;
+// ANTS_SYN_CC_SUMS=fp32|fp64 overrides (read per launch).  Measured per
+// 192x224x192 pair: float sums 6.34 ms of CC against 5.88 ms in double (160
+// against 226 VGPRs, 3 against 2 waves per SIMD, but the same LDS
+// instruction count and the fixed-order z sums on top), so double stays the
+// default.
+bool cc_sums_fp64(bool fp32_sums) {
+  const char* e = std::getenv("ANTS_SYN_CC_SUMS");
+  if (e && std::strcmp(e, "fp64") == 0) return true;
+  if (e && std::strcmp(e, "fp32") == 0) return false;
+  return !fp32_sums;
+}
+
+int cc_zchunk(const GridDev& g, int radius, bool fp32_sums) {
   const int64_t tiles = static_cast<int64_t>((g.n[0] + kCCTX - 1) / kCCTX) * ((g.n[1] + kCCTY - 1) / kCCTY);
 #if ANTS_ZCHUNK_MODEL
-  static const int64_t slots4 = device_slots(reinterpret_cast<const void*>(&k_cc3d<4, kCCTX, kCCTY>),
+  static const int64_t slots4 = device_slots(reinterpret_cast<const void*>(&k_cc3d<4, kCCTX, kCCTY, double>),
                                              kCCTX * kCCTY, ANTS_CC_SLOTS_PER_CU);
-  static const int64_t slots2 = device_slots(reinterpret_cast<const void*>(&k_cc3d<2, kCCTX, kCCTY>),
+  static const int64_t slots2 = device_slots(reinterpret_cast<const void*>(&k_cc3d<2, kCCTX, kCCTY, double>),
                                              kCCTX * kCCTY, ANTS_CC_SLOTS_PER_CU);
-  return zchunk_for(g.n[2], tiles, 2 * radius, radius == 4 ? slots4 : slots2);
+  static const int64_t slots4f = device_slots(reinterpret_cast<const void*>(&k_cc3d<4, kCCTX, kCCTY, float>),
+                                              kCCTX * kCCTY, ANTS_CC_SLOTS_PER_CU);
+  static const int64_t slots2f = device_slots(reinterpret_cast<const void*>(&k_cc3d<2, kCCTX, kCCTY, float>),
+                                              kCCTX * kCCTY, ANTS_CC_SLOTS_PER_CU);
+  const bool d = cc_sums_fp64(fp32_sums);
+  return zchunk_for(g.n[2], tiles, 2 * radius, radius == 4 ? (d ? slots4 : slots4f) : (d ? slots2 : slots2f));
 #else
   (void)radius;
+  (void)fp32_sums;
   int64_t zc = (static_cast<int64_t>(g.n[2]) * tiles) / ANTS_CC_TARGET_BLOCKS;
   zc = zc < ANTS_CC_ZMIN ? ANTS_CC_ZMIN : (zc > 32 ? 32 : zc);
   return static_cast<int>(zc);
@@ -2176,9 +2236,9 @@ static bool cc_tiled(const GridDev& g, int radius) {
   return g.D == 3 && (radius == 2 || radius == 4) && static_cast<int64_t>(g.n[0]) * g.n[1] < (int64_t{1} << 31);
 }
 
-int64_t cc_partials_needed(const GridDev& g, int radius) {
+int64_t cc_partials_needed(const GridDev& g, int radius, bool fp32_sums) {
   if (cc_tiled(g, radius)) {
-    const int zc = cc_zchunk(g, radius);
+    const int zc = cc_zchunk(g, radius, fp32_sums);
     return static_cast<int64_t>((g.n[0] + kCCTX - 1) / kCCTX) * ((g.n[1] + kCCTY - 1) / kCCTY) *
            ((g.n[2] + zc - 1) / zc);
   }
@@ -2186,16 +2246,23 @@ int64_t cc_partials_needed(const GridDev& g, int radius) {
 }
 
 int launch_cc(const GridDev& g, const float* F, const float* M, int radius, float eps, float* gF2M, float* gM2M,
-              Partial* partials, RedSlot* slot, hipStream_t s, bool reduce) {
+              Partial* partials, RedSlot* slot, hipStream_t s, bool reduce, bool fp32_sums) {
   if (!cc_tiled(g, radius)) return launch_cc_direct(g, F, M, radius, eps, gF2M, gM2M, partials, slot, s, reduce);
-  const int zc = cc_zchunk(g, radius);
+  const int zc = cc_zchunk(g, radius, fp32_sums);
   const dim3 grid((g.n[0] + kCCTX - 1) / kCCTX, (g.n[1] + kCCTY - 1) / kCCTY, (g.n[2] + zc - 1) / zc);
-  if (radius == 4)
-    hipLaunchKernelGGL((k_cc3d<4, kCCTX, kCCTY>), grid, dim3(kCCTX * kCCTY), 0, s, g, F, M, eps, zc, gF2M, gM2M,
-                       partials);
+  const bool d = cc_sums_fp64(fp32_sums);
+  if (radius == 4 && d)
+    hipLaunchKernelGGL((k_cc3d<4, kCCTX, kCCTY, double>), grid, dim3(kCCTX * kCCTY), 0, s, g, F, M, eps, zc, gF2M,
+                       gM2M, partials);
+  else if (radius == 4)
+    hipLaunchKernelGGL((k_cc3d<4, kCCTX, kCCTY, float>), grid, dim3(kCCTX * kCCTY), 0, s, g, F, M, eps, zc, gF2M,
+                       gM2M, partials);
+  else if (d)
+    hipLaunchKernelGGL((k_cc3d<2, kCCTX, kCCTY, double>), grid, dim3(kCCTX * kCCTY), 0, s, g, F, M, eps, zc, gF2M,
+                       gM2M, partials);
   else
-    hipLaunchKernelGGL((k_cc3d<2, kCCTX, kCCTY>), grid, dim3(kCCTX * kCCTY), 0, s, g, F, M, eps, zc, gF2M, gM2M,
-                       partials);
+    hipLaunchKernelGGL((k_cc3d<2, kCCTX, kCCTY, float>), grid, dim3(kCCTX * kCCTY), 0, s, g, F, M, eps, zc, gF2M,
+                       gM2M, partials);
   const int np = static_cast<int>(grid.x * grid.y * grid.z);
   if (reduce) hipLaunchKernelGGL(k_reduce_slot<kCCTX * kCCTY>, dim3(1), dim3(kCCTX * kCCTY), 0, s, partials, np, slot);
   return np;

@@ -137,11 +137,14 @@ void launch_warp_image(const GridDev& gi, const float* img, const GridDev& go, c
                        hipStream_t s);
 
 // CC value + both derivatives.  `partials` must hold cc_partials_needed(g) entries.
-int64_t cc_partials_needed(const GridDev& g, int radius);
+int64_t cc_partials_needed(const GridDev& g, int radius, bool fp32_sums = false);
 // Returns the number of partials written; with reduce = false the energy slot
 // is left to launch_publish_stats.
+// fp32_sums: float box sums over block-shifted samples (the BF16 pyramid
+// form's precision) instead of double sums; ANTS_SYN_CC_SUMS=fp32|fp64
+// overrides the choice.
 int launch_cc(const GridDev& g, const float* F, const float* M, int radius, float eps, float* gF2M, float* gM2M,
-              Partial* partials, RedSlot* slot, hipStream_t s, bool reduce = true);
+              Partial* partials, RedSlot* slot, hipStream_t s, bool reduce = true, bool fp32_sums = false);
 int launch_cc_direct(const GridDev& g, const float* F, const float* M, int radius, float eps, float* gF2M,
                      float* gM2M, Partial* partials, RedSlot* slot, hipStream_t s, bool reduce = true);
 // One block: reduce cc_n CC partials into *energy (skipped when cc_parts is

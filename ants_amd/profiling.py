@@ -110,15 +110,40 @@ def counter_csv(path: str):
     return out, path
 
 
+def dispatch_grids(path: str) -> dict:
+    """dispatch id -> "gx x gy x gz" from the kernel_trace.csv of the same
+    rocprofv3 run (empty when the run traced no kernels).  The counter file
+    only holds the total work-item count, which two pyramid levels can share."""
+    import csv
+    d = path if os.path.isdir(path) else os.path.dirname(path)
+    found = sorted(glob.glob(os.path.join(d, "**", "*kernel_trace.csv"), recursive=True))
+    if not found:
+        return {}
+    out = {}
+    with open(found[-1], newline="") as fh:
+        rd = csv.DictReader(fh)
+        cols = {c.lower(): c for c in rd.fieldnames or []}
+        try:
+            cid = cols["dispatch_id"]
+            gx, gy, gz = cols["grid_size_x"], cols["grid_size_y"], cols["grid_size_z"]
+        except KeyError:
+            return {}
+        for r in rd:
+            out[int(r[cid])] = f"{r[gx]}x{r[gy]}x{r[gz]}"
+    return out
+
+
 def traffic(fetch_dir: str, write_dir: str, kernel: str, copy_bytes: float) -> str:
     lines = []
     per = {}
     for cname, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
         rows, path = counter_csv(d)
-        lines.append(f"{cname}: {path}")
+        grids = dispatch_grids(path)
+        lines.append(f"{cname}: {path}" + (" (grids from the run's kernel trace)" if grids else ""))
         kib = collections.defaultdict(float)  # (kernel, grid) -> summed KiB over dispatches
         disp = collections.defaultdict(set)
         for did, k, grid, ctr, v in rows:
+            grid = grids.get(did, grid)
             if ctr == cname:
                 kib[(k, grid)] += v
                 disp[(k, grid)].add(did)
@@ -148,8 +173,117 @@ def traffic(fetch_dir: str, write_dir: str, kernel: str, copy_bytes: float) -> s
     return "\n".join(lines)
 
 
+def _trace_rows(path: str):
+    """(dispatch id, short kernel name with template arguments, grid "x*y*z",
+    duration ns) from the newest kernel_trace.csv under path, in dispatch order."""
+    import csv
+    if os.path.isdir(path):
+        found = sorted(glob.glob(os.path.join(path, "**", "*kernel_trace.csv"), recursive=True))
+        if not found:
+            raise FileNotFoundError(f"no kernel_trace.csv under {path}")
+        path = found[-1]
+    rows = []
+    with open(path, newline="") as fh:
+        for r in csv.DictReader(fh):
+            rows.append((int(r["Dispatch_Id"]), _short(r["Kernel_Name"]),
+                         f"{r['Grid_Size_X']}x{r['Grid_Size_Y']}x{r['Grid_Size_Z']}",
+                         int(r["End_Timestamp"]) - int(r["Start_Timestamp"])))
+    rows.sort()
+    return rows, path
+
+
+def _levels_of(rows):
+    """Pyramid level of each dispatch of a run of SyN registrations: k_minmax
+    (preprocessing) starts level 0 of a registration and k_resample_fields
+    (the field adaptor) the next level; dispatches outside (calibration
+    copies before the first registration) get None."""
+    out, lv, started, prev_minmax = {}, None, False, False
+    for did, k, grid, ns in rows:
+        base = k.split("<")[0]
+        if base == "k_minmax":
+            if not prev_minmax:
+                lv = 0
+            started = True
+        elif base == "k_resample_fields" and started:
+            lv += 1
+        prev_minmax = base == "k_minmax"
+        out[did] = lv if started else None
+    return out
+
+
+def level_table(trace_dir: str, fetch_dir: str, write_dir: str, bench_json: str, copy_bytes: float) -> str:
+    """Per kernel and pyramid level: rocprof mean duration, algorithmic bytes
+    per launch (the engine's own count, from bench.py's per_level_kernels),
+    achieved algorithmic GB/s, and HBM bytes per launch from the two counter
+    passes (each scaled by the calibration copy kernel), against 8 TB/s."""
+    import json
+    line = None
+    for ln in open(bench_json):
+        ln = ln.strip()
+        if ln.startswith("{") and '"per_level_kernels"' in ln:
+            line = json.loads(ln)
+    if line is None:
+        raise RuntimeError(f"no bench.py JSON line with per_level_kernels in {bench_json}")
+    plk = line["per_level_kernels"]
+    level_names = list(plk.keys())
+    cls_kernel = {"k_warp_pair": "k_warp_pair<3, float, false>", "k_cc3d": "k_cc3d<4, 16, 16>",
+                  "k_smooth3d": "k_smooth3d<6, 16, 32, 3>", "k_compose": "k_compose<3>",
+                  "k_inv_step": "k_inv_step<3, int>"}
+    rows, tpath = _trace_rows(trace_dir)
+    lvl = _levels_of(rows)
+    dur = collections.defaultdict(list)
+    for did, k, grid, ns in rows:
+        if lvl.get(did) is not None:
+            dur[(k, lvl[did])].append(ns)
+    ctr = {}
+    for cname, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
+        crow, cpath = counter_csv(d)
+        trows, _ = _trace_rows(d)
+        clvl = _levels_of(trows)
+        kib = collections.defaultdict(float)
+        n = collections.defaultdict(set)
+        copy = []
+        for did, k, grid, c, v in crow:
+            if c != cname:
+                continue
+            if k.startswith("k_copy_calibration"):
+                copy.append(v)
+                continue
+            if clvl.get(did) is not None:
+                kib[(k, clvl[did])] += v
+                n[(k, clvl[did])].add(did)
+        scale = copy_bytes / (sum(copy) / len(copy) * 1024.0)
+        ctr[cname] = {key: kib[key] / len(n[key]) * 1024.0 * scale for key in kib}
+    out = [f"trace: {tpath}", "level grid      kernel                          launches  mean_us  alg_MB/launch  "
+           "alg_GB/s  frac(8TB/s)  hbm_MB/launch  hbm_GB/s  hbm/alg"]
+    for li, lname in enumerate(level_names):
+        for cls, rec in plk[lname].items():
+            kname = cls_kernel[cls]
+            ds = dur.get((kname, li))
+            if not ds:
+                continue
+            mean_us = sum(ds) / len(ds) / 1e3
+            alg = rec["algorithmic_mb_per_launch"]
+            gbs = alg * 1e6 / (mean_us * 1e-6) / 1e9
+            hbm = (ctr["FETCH_SIZE"].get((kname, li), 0.0) + ctr["WRITE_SIZE"].get((kname, li), 0.0)) / 1e6
+            hgbs = hbm * 1e6 / (mean_us * 1e-6) / 1e9
+            out.append(f"{lname:14s}  {kname:30s}  {len(ds):8d}  {mean_us:7.1f}  {alg:13.2f}  {gbs:8.0f}  "
+                       f"{gbs / 8000:11.3f}  {hbm:13.2f}  {hgbs:8.0f}  {hbm / alg if alg else 0:7.2f}")
+    return "\n".join(out)
+
+
 def main(argv=None):
     argv = list(sys.argv[1:] if argv is None else argv)
+    if argv and argv[0] == "levels":
+        ap = argparse.ArgumentParser(prog="profiling levels")
+        ap.add_argument("--trace", required=True)
+        ap.add_argument("--fetch", required=True)
+        ap.add_argument("--write", required=True)
+        ap.add_argument("--bench-json", required=True, help="stdout of the traced bench.py run")
+        ap.add_argument("--copy-bytes", type=float, required=True)
+        a = ap.parse_args(argv[1:])
+        print(level_table(a.trace, a.fetch, a.write, a.bench_json, a.copy_bytes))
+        return 0
     if argv and argv[0] == "traffic":
         ap = argparse.ArgumentParser(prog="profiling traffic")
         ap.add_argument("--fetch", required=True)

@@ -245,9 +245,14 @@ class SyNRegistration:
         L.check(L.lib().ants_syn_set_profiling(self.handle, mask))
         L.check(L.lib().ants_syn_set_profiling_stride(self.handle, every))
 
-    def kernel_stats(self) -> dict:
+    def kernel_stats(self, level: int = None) -> dict:
+        """Per kernel class: event-timed ms, launches and algorithmic bytes of
+        the last run (of one pyramid level when `level` is given)."""
         k = L.KernelStatsC()
-        L.check(L.lib().ants_syn_get_kernel_stats(self.handle, C.byref(k)))
+        if level is None:
+            L.check(L.lib().ants_syn_get_kernel_stats(self.handle, C.byref(k)))
+        else:
+            L.check(L.lib().ants_syn_get_kernel_stats_level(self.handle, level, C.byref(k)))
         return {n: {"ms": k.ms[i], "launches": int(k.launches[i]), "bytes": k.bytes[i]}
                 for i, n in enumerate(L.KC_NAMES)}
 

@@ -359,6 +359,25 @@ def main():
     reg.register_device(gf, d_fixed, gf, d_moving, d_fwd, d_inv)
     ks = reg.kernel_stats()
     dominant = max(ks, key=lambda k: ks[k]["ms"])
+    # per kernel class and pyramid level: mean launch time and algorithmic
+    # bytes per launch -> achieved GB/s against the 8 TB/s HBM peak
+    st0 = reg.stats()
+    per_level = {}
+    for lv in range(st0["levels_run"]):
+        kl = reg.kernel_stats(lv)
+        lg = reg.level_geometry(pair.size, pair.spacing, pair.origin, lv)
+        grid = "x".join(str(int(lg.size[d])) for d in range(3))
+        row = {}
+        for k, v in kl.items():
+            if v["launches"] == 0 or v["ms"] <= 0:
+                continue
+            gbs = v["bytes"] / (v["ms"] * 1e-3) / 1e9
+            row[CLASS_KERNEL[k]] = {"launches": v["launches"], "mean_us": v["ms"] * 1e3 / v["launches"],
+                                    "algorithmic_mb_per_launch": v["bytes"] / v["launches"] / 1e6,
+                                    "achieved_gbs": gbs, "frac": gbs / HBM_PEAK_GBS}
+        per_level[grid] = row
+    log("per level (kernel: mean us, GB/s):", json.dumps({g: {k: (round(v["mean_us"], 1), round(v["achieved_gbs"]))
+                                                             for k, v in r.items()} for g, r in per_level.items()}))
     log("kernel classes (ms, launches, GB/s):",
         {k: (round(v["ms"], 2), v["launches"], round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1)) for k, v in ks.items()})
     # the timed steps time the dominant class on every 4th iteration of each
@@ -482,6 +501,7 @@ def main():
         "cpu_baseline": cpu,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
         "kernel_breakdown_ms": {k: round(v["ms"], 3) for k, v in ks.items()},
+        "per_level_kernels": per_level,
         "host_buffer_drop_in": host_drop_in,
     }
     if legs:

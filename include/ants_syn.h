@@ -209,6 +209,8 @@ int ants_syn_set_profiling(ants_syn_handle* h, unsigned int class_mask);
  * sampling keeps most iterations free of that stall. */
 int ants_syn_set_profiling_stride(ants_syn_handle* h, int every);
 int ants_syn_get_kernel_stats(const ants_syn_handle* h, ants_syn_kernel_stats* out);
+/* The same restricted to the spans of pyramid level `level` of the last run. */
+int ants_syn_get_kernel_stats_level(const ants_syn_handle* h, int level, ants_syn_kernel_stats* out);
 
 /* ---- building blocks (device buffers) ----
  * Each replaces the ITK filter that SyN calls at the cited ANTs site. */

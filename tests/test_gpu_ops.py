@@ -91,6 +91,25 @@ def test_cc(shape, r):
     np.testing.assert_allclose(g2, r2, atol=2e-4 * scale)
 
 
+@pytest.mark.parametrize("shape,r", [((19, 23, 21), 4), ((24, 20, 40), 2), ((70, 5, 67), 4), ((40, 36, 48), 4)])
+def test_cc_float_shifted_sums(monkeypatch, shape, r):
+    """The tiled CC with float box sums over block-shifted samples (the BF16
+    pyramid form's precision) against the FP64 oracle, on flat noisy regions
+    (intensity 0.6, noise 0.02: sum f^2 is about 900 times the window
+    variance) next to an edge; bars written here."""
+    monkeypatch.setenv("ANTS_SYN_CC_SUMS", "fp32")
+    F = np.full(shape, 0.6)
+    F[..., : shape[-1] // 3] = 0.2
+    F = f32(F + 0.02 * rng.standard_normal(shape))
+    M = f32(0.9 * F + 0.02 * rng.standard_normal(shape) + 0.05)
+    v, g1, g2 = G.cc(F, M, r)
+    vr, r1, r2 = o.cc(grid_np(shape), F, M, r)
+    assert v == pytest.approx(vr, rel=1e-5)
+    scale = max(np.abs(r1).max(), np.abs(r2).max())
+    np.testing.assert_allclose(g1, r1, atol=1e-3 * scale)
+    np.testing.assert_allclose(g2, r2, atol=1e-3 * scale)
+
+
 def test_cc_degenerate_windows_and_spacing():
     shape = (20, 24, 28)
     F = f32(rng.uniform(0, 1, shape))
