@@ -272,8 +272,28 @@ def level_table(trace_dir: str, fetch_dir: str, write_dir: str, bench_json: str,
     return "\n".join(out)
 
 
+def trace_table(trace_dir: str) -> str:
+    """Per kernel and launch shape of a kernel_trace.csv: calls, total ms,
+    mean us and share of the kernel time."""
+    rows, path = _trace_rows(trace_dir)
+    agg = collections.OrderedDict()
+    for did, k, grid, ns in rows:
+        a = agg.setdefault((k, grid), [0, 0])
+        a[0] += 1
+        a[1] += ns
+    total = sum(v[1] for v in agg.values())
+    out = [f"trace: {path}", f"{'kernel':32s} {'grid(x,y,z)':18s} {'calls':>7s} {'total ms':>9s} {'mean us':>8s} {'%':>6s}"]
+    for (k, grid), (n, ns) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
+        out.append(f"{k:32s} {grid:18s} {n:7d} {ns / 1e6:9.3f} {ns / n / 1e3:8.2f} {100.0 * ns / total:6.1f}")
+    out.append(f"total kernel time {total / 1e6:.3f} ms")
+    return "\n".join(out)
+
+
 def main(argv=None):
     argv = list(sys.argv[1:] if argv is None else argv)
+    if argv and argv[0] == "trace":
+        print(trace_table(argv[1]))
+        return 0
     if argv and argv[0] == "levels":
         ap = argparse.ArgumentParser(prog="profiling levels")
         ap.add_argument("--trace", required=True)
