@@ -155,6 +155,9 @@ def lib():
     L.ants_apply_transforms.argtypes = [pdesc, pdesc, vp, C.POINTER(TransformC), i, i, d, vp, vp]
     L.ants_syn_set_initial_transforms.argtypes = [vp, C.POINTER(TransformC), i, C.POINTER(TransformC), i]
     L.ants_syn_set_initial_state_device.argtypes = [vp, pdesc, vp, vp, vp, vp]
+    L.ants_tuning_set.argtypes = [i, i, i]
+    L.ants_tuning_get.argtypes = [i, i, C.POINTER(i)]
+    L.ants_tuning_load.argtypes = [C.c_char_p]
     _lib = L
     return L
 

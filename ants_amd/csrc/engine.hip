@@ -11,6 +11,8 @@
 // step counts) to drive the convergence monitor and the observer.
 #include <hip/hip_runtime.h>
 
+#include <dlfcn.h>
+
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -873,6 +875,8 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
     hg(gM);
     const int ints[] = {P.dim, P.n_levels, P.sigma_in_mm, P.cc_radius, P.precision};
     setup_hash = fnv1a(ints, sizeof(ints), setup_hash);
+    const uint64_t tg = tune_generation();  // launch shapes baked into captured graphs
+    setup_hash = fnv1a(&tg, sizeof(tg), setup_hash);
     setup_hash = fnv1a(P.shrink, sizeof(P.shrink), setup_hash);
     setup_hash = fnv1a(P.sigma, sizeof(P.sigma), setup_hash);
     const double dbl[] = {P.learning_rate, P.update_variance, P.total_variance};
@@ -1767,6 +1771,98 @@ int ants_apply_transform_labels_nn(const ants_image_desc* go, const float* d_fie
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(s));
   return ANTS_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Launch-shape tuning table (tune_* in the kernel library).
+namespace {
+
+const char* const kKnobNames[kTuneKnobCount] = {"inv_blocks", "cc_zchunk", "smooth_zchunk", "cc_tile", "smooth_tile"};
+
+int knob_of(const char* name) {
+  for (int k = 0; k < kTuneKnobCount; ++k)
+    if (std::strcmp(name, kKnobNames[k]) == 0) return k;
+  return -1;
+}
+
+// Reads "knob bucket value" lines ('#' starts a comment).
+int load_tuning_file(const char* path) {
+  FILE* f = std::fopen(path, "r");
+  if (!f) return fail(ANTS_ERR_INVALID, "cannot open tuning file %s", path);
+  char line[256];
+  int n = 0, lineno = 0;
+  while (std::fgets(line, sizeof(line), f)) {
+    ++lineno;
+    char* hash = std::strchr(line, '#');
+    if (hash) *hash = 0;
+    char name[64];
+    int bucket = 0, value = 0;
+    const int got = std::sscanf(line, "%63s %d %d", name, &bucket, &value);
+    if (got <= 0) continue;
+    const int k = got == 3 ? knob_of(name) : -1;
+    if (k < 0 || tune_set(k, bucket, value) != 0) {
+      std::fclose(f);
+      return fail(ANTS_ERR_INVALID, "%s:%d: bad tuning entry", path, lineno);
+    }
+    ++n;
+  }
+  std::fclose(f);
+  return ANTS_OK;
+}
+
+// At library load: the table committed beside the library (tuning.txt in
+// the directory of libants_syn.so), or ANTS_SYN_TUNING=<file>; "none"
+// leaves the table empty (the built-in launch models decide).
+struct DefaultTuning {
+  DefaultTuning() {
+    const char* env = std::getenv("ANTS_SYN_TUNING");
+    if (env && std::strcmp(env, "none") == 0) return;
+    std::string path;
+    if (env && env[0]) {
+      path = env;
+    } else {
+      Dl_info info;
+      if (!dladdr(reinterpret_cast<void*>(&ants_version), &info) || !info.dli_fname) return;
+      path = info.dli_fname;
+      const size_t slash = path.find_last_of('/');
+      path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + "/tuning.txt";
+      FILE* f = std::fopen(path.c_str(), "r");
+      if (!f) return;
+      std::fclose(f);
+    }
+    if (load_tuning_file(path.c_str()) != ANTS_OK)
+      std::fprintf(stderr, "ants_syn: tuning table not loaded: %s\n", g_last_error.c_str());
+  }
+};
+DefaultTuning g_default_tuning;
+
+}  // namespace
+
+extern "C" {
+
+int ants_tuning_set(int knob, int log2_voxels, int value) {
+  if (tune_set(knob, log2_voxels, value) != 0)
+    return fail(ANTS_ERR_INVALID, "bad tuning entry (knob %d, bucket %d, value %d)", knob, log2_voxels, value);
+  return ANTS_OK;
+}
+
+int ants_tuning_get(int knob, int log2_voxels, int* value) {
+  if (!value || knob < 0 || knob >= kTuneKnobCount || log2_voxels < 0 || log2_voxels >= kTuneBuckets)
+    return fail(ANTS_ERR_INVALID, "bad tuning query");
+  *value = tune_get(knob, int64_t{1} << log2_voxels);
+  return ANTS_OK;
+}
+
+int ants_tuning_clear(void) {
+  tune_clear();
+  return ANTS_OK;
+}
+
+int ants_tuning_load(const char* path) {
+  if (!path) return fail(ANTS_ERR_INVALID, "null path");
+  return load_tuning_file(path);
 }
 
 }  // extern "C"

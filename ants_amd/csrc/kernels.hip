@@ -7,6 +7,7 @@
 #include "kernels.hpp"
 
 #include <hip/hip_runtime.h>
+#include <atomic>
 #include <cfloat>
 #include <cstdint>
 #include <type_traits>
@@ -2013,6 +2014,36 @@ constexpr int kCCTX = ANTS_CC_TX, kCCTY = ANTS_CC_TY;
 #define ANTS_ZCHUNK_MIN 4
 #endif
 
+}  // namespace
+
+std::atomic<int> g_tune[kTuneKnobCount][kTuneBuckets];
+std::atomic<uint64_t> g_tune_gen{1};
+
+static int tune_bucket(int64_t voxels) {
+  int b = 0;
+  while (b + 1 < kTuneBuckets && (int64_t{1} << (b + 1)) <= voxels) ++b;
+  return b;
+}
+int tune_get(int knob, int64_t voxels) {
+  if (knob < 0 || knob >= kTuneKnobCount || voxels < 1) return 0;
+  return g_tune[knob][tune_bucket(voxels)].load(std::memory_order_relaxed);
+}
+int tune_set(int knob, int bucket, int value) {
+  if (knob < 0 || knob >= kTuneKnobCount || bucket < 0 || bucket >= kTuneBuckets || value < 0) return -1;
+  if ((knob == kTuneCCTile || knob == kTuneSmoothTile) && value > 2) return -1;
+  g_tune[knob][bucket].store(value, std::memory_order_relaxed);
+  g_tune_gen.fetch_add(1);
+  return 0;
+}
+void tune_clear() {
+  for (auto& k : g_tune)
+    for (auto& v : k) v.store(0, std::memory_order_relaxed);
+  g_tune_gen.fetch_add(1);
+}
+uint64_t tune_generation() { return g_tune_gen.load(); }
+
+namespace {
+
 // Blocks of a kernel resident on the whole device at once: blocks per CU
 // (from its register and LDS use) times the CU count.
 // `per_cu` > 0 overrides the occupancy query.
@@ -2066,26 +2097,45 @@ bool cc_sums_fp64(bool fp32_sums) {
   return !fp32_sums;
 }
 
-int cc_zchunk(const GridDev& g, int radius, bool fp32_sums) {
-  const int64_t tiles = static_cast<int64_t>((g.n[0] + kCCTX - 1) / kCCTX) * ((g.n[1] + kCCTY - 1) / kCCTY);
+// Launch shape of the tiled CC: tile (tuned, else 16 x 16) and z-chunk
+// depth (tuned, else chosen by zchunk_for from the resident-block count).
+// Float sums (the BF16 form) keep the 16 x 16 tile.
+struct CCShape {
+  int tx, ty, zc;
+};
+CCShape cc_shape(const GridDev& g, int radius, bool fp32_sums) {
+  const bool d = cc_sums_fp64(fp32_sums);
+  const int64_t N = g.count();
+  const bool narrow = d && tune_get(kTuneCCTile, N) == 2;
+  CCShape sh{kCCTX, narrow ? kCCTY / 2 : kCCTY, 0};
+  const int64_t tiles = static_cast<int64_t>((g.n[0] + sh.tx - 1) / sh.tx) * ((g.n[1] + sh.ty - 1) / sh.ty);
+  const int tz = tune_get(kTuneCCZChunk, N);
+  if (tz > 0) {
+    sh.zc = tz < g.n[2] ? tz : g.n[2];
+    return sh;
+  }
 #if ANTS_ZCHUNK_MODEL
   static const int64_t slots4 = device_slots(reinterpret_cast<const void*>(&k_cc3d<4, kCCTX, kCCTY, double>),
                                              kCCTX * kCCTY, ANTS_CC_SLOTS_PER_CU);
   static const int64_t slots2 = device_slots(reinterpret_cast<const void*>(&k_cc3d<2, kCCTX, kCCTY, double>),
                                              kCCTX * kCCTY, ANTS_CC_SLOTS_PER_CU);
+  static const int64_t slots4n = device_slots(reinterpret_cast<const void*>(&k_cc3d<4, kCCTX, kCCTY / 2, double>),
+                                              kCCTX * kCCTY / 2, ANTS_CC_SLOTS_PER_CU);
+  static const int64_t slots2n = device_slots(reinterpret_cast<const void*>(&k_cc3d<2, kCCTX, kCCTY / 2, double>),
+                                              kCCTX * kCCTY / 2, ANTS_CC_SLOTS_PER_CU);
   static const int64_t slots4f = device_slots(reinterpret_cast<const void*>(&k_cc3d<4, kCCTX, kCCTY, float>),
                                               kCCTX * kCCTY, ANTS_CC_SLOTS_PER_CU);
   static const int64_t slots2f = device_slots(reinterpret_cast<const void*>(&k_cc3d<2, kCCTX, kCCTY, float>),
                                               kCCTX * kCCTY, ANTS_CC_SLOTS_PER_CU);
-  const bool d = cc_sums_fp64(fp32_sums);
-  return zchunk_for(g.n[2], tiles, 2 * radius, radius == 4 ? (d ? slots4 : slots4f) : (d ? slots2 : slots2f));
+  const int64_t slots = radius == 4 ? (!d ? slots4f : narrow ? slots4n : slots4)
+                                    : (!d ? slots2f : narrow ? slots2n : slots2);
+  sh.zc = zchunk_for(g.n[2], tiles, 2 * radius, slots);
 #else
-  (void)radius;
-  (void)fp32_sums;
   int64_t zc = (static_cast<int64_t>(g.n[2]) * tiles) / ANTS_CC_TARGET_BLOCKS;
   zc = zc < ANTS_CC_ZMIN ? ANTS_CC_ZMIN : (zc > 32 ? 32 : zc);
-  return static_cast<int>(zc);
+  sh.zc = static_cast<int>(zc);
 #endif
+  return sh;
 }
 
 }  // namespace
@@ -2127,24 +2177,40 @@ void launch_gauss_field_final(const GridDev& g, int np, const float* const* in, 
                        mpack(out, np), t, axis, w1, w2, static_cast<int>(boundary), maxbits);
 }
 
-bool launch_smooth3d(const GridDev& g, int np, const float* const* in, const float* const* orig, float* const* out,
-                     const Taps& t, float w1, float w2, bool boundary, unsigned int* maxbits, hipStream_t s) {
-  constexpr int R = 6, TX = kSmoothTX, TY = kSmoothTY;
-  if (g.D != 3 || t.r != R || np > 4) return false;
-  if (static_cast<int64_t>(g.n[0]) * g.n[1] * 3 >= (int64_t{1} << 31)) return false;  // 32-bit in-plane offsets
+// The fused field smoothing with tile TX x TY; z-chunk tuned or modelled.
+template <int TX, int TY>
+static void smooth3d_fields(const GridDev& g, int np, const float* const* in, const float* const* orig,
+                            float* const* out, const Taps& t, float w1, float w2, bool boundary,
+                            unsigned int* maxbits, hipStream_t s) {
+  constexpr int R = 6;
   const int64_t tiles = static_cast<int64_t>((g.n[0] + TX - 1) / TX) * ((g.n[1] + TY - 1) / TY) * np;
+  int64_t zc = tune_get(kTuneSmoothZChunk, g.count());
+  if (zc > 0) {
+    zc = zc < g.n[2] ? zc : g.n[2];
+  } else {
 #if ANTS_ZCHUNK_MODEL
-  static const int64_t slots =
-      device_slots(reinterpret_cast<const void*>(&k_smooth3d<R, TX, TY>), TX * TY, ANTS_SMOOTH_SLOTS_PER_CU);
-  const int64_t zc = zchunk_for(g.n[2], tiles, 2 * R, slots);
+    static const int64_t slots =
+        device_slots(reinterpret_cast<const void*>(&k_smooth3d<R, TX, TY>), TX * TY, ANTS_SMOOTH_SLOTS_PER_CU);
+    zc = zchunk_for(g.n[2], tiles, 2 * R, slots);
 #else
-  int64_t zc = (static_cast<int64_t>(g.n[2]) * tiles) / ANTS_SMOOTH_TARGET_BLOCKS;
-  zc = zc < ANTS_SMOOTH_ZMIN ? ANTS_SMOOTH_ZMIN : (zc > 64 ? 64 : zc);
+    zc = (static_cast<int64_t>(g.n[2]) * tiles) / ANTS_SMOOTH_TARGET_BLOCKS;
+    zc = zc < ANTS_SMOOTH_ZMIN ? ANTS_SMOOTH_ZMIN : (zc > 64 ? 64 : zc);
 #endif
+  }
   const int nzc = static_cast<int>((g.n[2] + zc - 1) / zc);
   const dim3 grid((g.n[0] + TX - 1) / TX, (g.n[1] + TY - 1) / TY, nzc * np);
   hipLaunchKernelGGL((k_smooth3d<R, TX, TY>), grid, dim3(TX * TY), 0, s, g, pack(in, np), pack(orig, np),
                      mpack(out, np), t, static_cast<int>(zc), nzc, w1, w2, static_cast<int>(boundary), maxbits);
+}
+
+bool launch_smooth3d(const GridDev& g, int np, const float* const* in, const float* const* orig, float* const* out,
+                     const Taps& t, float w1, float w2, bool boundary, unsigned int* maxbits, hipStream_t s) {
+  if (g.D != 3 || t.r != 6 || np > 4) return false;
+  if (static_cast<int64_t>(g.n[0]) * g.n[1] * 3 >= (int64_t{1} << 31)) return false;  // 32-bit in-plane offsets
+  if (tune_get(kTuneSmoothTile, g.count()) == 2)
+    smooth3d_fields<kSmoothTX, kSmoothTY / 2>(g, np, in, orig, out, t, w1, w2, boundary, maxbits, s);
+  else
+    smooth3d_fields<kSmoothTX, kSmoothTY>(g, np, in, orig, out, t, w1, w2, boundary, maxbits, s);
   return true;
 }
 
@@ -2238,9 +2304,9 @@ static bool cc_tiled(const GridDev& g, int radius) {
 
 int64_t cc_partials_needed(const GridDev& g, int radius, bool fp32_sums) {
   if (cc_tiled(g, radius)) {
-    const int zc = cc_zchunk(g, radius, fp32_sums);
-    return static_cast<int64_t>((g.n[0] + kCCTX - 1) / kCCTX) * ((g.n[1] + kCCTY - 1) / kCCTY) *
-           ((g.n[2] + zc - 1) / zc);
+    const CCShape sh = cc_shape(g, radius, fp32_sums);
+    return static_cast<int64_t>((g.n[0] + sh.tx - 1) / sh.tx) * ((g.n[1] + sh.ty - 1) / sh.ty) *
+           ((g.n[2] + sh.zc - 1) / sh.zc);
   }
   return red_blocks(g.count());
 }
@@ -2248,15 +2314,24 @@ int64_t cc_partials_needed(const GridDev& g, int radius, bool fp32_sums) {
 int launch_cc(const GridDev& g, const float* F, const float* M, int radius, float eps, float* gF2M, float* gM2M,
               Partial* partials, RedSlot* slot, hipStream_t s, bool reduce, bool fp32_sums) {
   if (!cc_tiled(g, radius)) return launch_cc_direct(g, F, M, radius, eps, gF2M, gM2M, partials, slot, s, reduce);
-  const int zc = cc_zchunk(g, radius, fp32_sums);
-  const dim3 grid((g.n[0] + kCCTX - 1) / kCCTX, (g.n[1] + kCCTY - 1) / kCCTY, (g.n[2] + zc - 1) / zc);
+  const CCShape sh = cc_shape(g, radius, fp32_sums);
+  const int zc = sh.zc;
+  const dim3 grid((g.n[0] + sh.tx - 1) / sh.tx, (g.n[1] + sh.ty - 1) / sh.ty, (g.n[2] + zc - 1) / zc);
   const bool d = cc_sums_fp64(fp32_sums);
-  if (radius == 4 && d)
+  const bool narrow = sh.ty != kCCTY;
+  constexpr int TY2 = kCCTY / 2;
+  if (radius == 4 && d && narrow)
+    hipLaunchKernelGGL((k_cc3d<4, kCCTX, TY2, double>), grid, dim3(kCCTX * TY2), 0, s, g, F, M, eps, zc, gF2M, gM2M,
+                       partials);
+  else if (radius == 4 && d)
     hipLaunchKernelGGL((k_cc3d<4, kCCTX, kCCTY, double>), grid, dim3(kCCTX * kCCTY), 0, s, g, F, M, eps, zc, gF2M,
                        gM2M, partials);
   else if (radius == 4)
     hipLaunchKernelGGL((k_cc3d<4, kCCTX, kCCTY, float>), grid, dim3(kCCTX * kCCTY), 0, s, g, F, M, eps, zc, gF2M,
                        gM2M, partials);
+  else if (d && narrow)
+    hipLaunchKernelGGL((k_cc3d<2, kCCTX, TY2, double>), grid, dim3(kCCTX * TY2), 0, s, g, F, M, eps, zc, gF2M, gM2M,
+                       partials);
   else if (d)
     hipLaunchKernelGGL((k_cc3d<2, kCCTX, kCCTY, double>), grid, dim3(kCCTX * kCCTY), 0, s, g, F, M, eps, zc, gF2M,
                        gM2M, partials);
@@ -2264,7 +2339,7 @@ int launch_cc(const GridDev& g, const float* F, const float* M, int radius, floa
     hipLaunchKernelGGL((k_cc3d<2, kCCTX, kCCTY, float>), grid, dim3(kCCTX * kCCTY), 0, s, g, F, M, eps, zc, gF2M,
                        gM2M, partials);
   const int np = static_cast<int>(grid.x * grid.y * grid.z);
-  if (reduce) hipLaunchKernelGGL(k_reduce_slot<kCCTX * kCCTY>, dim3(1), dim3(kCCTX * kCCTY), 0, s, partials, np, slot);
+  if (reduce) hipLaunchKernelGGL(k_reduce_slot<kBlock>, dim3(1), dim3(kBlock), 0, s, partials, np, slot);
   return np;
 }
 
@@ -2307,6 +2382,14 @@ void launch_compose(const GridDev& g, int np, const float* const* disp, const fl
 #define ANTS_INV_VPT 8
 #endif
 static unsigned int inv_blocks(int64_t n) {
+  // a tuned block count for this grid size, when the table holds one
+  const int tuned = tune_get(kTuneInvBlocks, n);
+  if (tuned > 0) {
+    const int64_t most = (n + 63) / 64;  // every block keeps at least one 64-voxel word
+    int64_t b = tuned < 2048 ? tuned : 2048;
+    b = b < most ? b : most;
+    return static_cast<unsigned int>(b < 1 ? 1 : b);
+  }
   int64_t b = (n + int64_t{kInvThreads} * ANTS_INV_VPT - 1) / (int64_t{kInvThreads} * ANTS_INV_VPT);
   // small grids keep up to 256 blocks (one per CU) at one voxel per thread
   const int64_t b1 = (n + kInvThreads - 1) / kInvThreads;

@@ -84,6 +84,26 @@ struct IterStats {
   RedSlot inv[4][kInvSlots];     // inversion norms per problem per inner step
 };
 
+// Launch-shape tuning table (filled by the project's autotuner, see
+// ants_amd/autotune.py): one value per knob and grid-size bucket
+// floor(log2(voxels)); 0 means no entry, and the launcher's own model
+// decides.  Knob values: inversion blocks per problem; CC and smoothing
+// z-chunk depth in planes; CC tile (1: 16x16, 2: 16x8) and smoothing tile
+// (1: 16x32, 2: 16x16).
+enum TuneKnob : int {
+  kTuneInvBlocks = 0,
+  kTuneCCZChunk = 1,
+  kTuneSmoothZChunk = 2,
+  kTuneCCTile = 3,
+  kTuneSmoothTile = 4,
+  kTuneKnobCount = 5
+};
+constexpr int kTuneBuckets = 40;
+int tune_get(int knob, int64_t voxels);
+int tune_set(int knob, int bucket, int value);  // 0 on success, -1 on a bad knob, bucket or value
+void tune_clear();
+uint64_t tune_generation();  // changes whenever the table does
+
 void launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
 void launch_minmax(const float* in, int64_t n, uint32_t* out2, hipStream_t s);  // out2: ordered bits (min, max)
 void launch_window(const float* in, float* out, int64_t n, float lo, float hi, double scale, double shift,

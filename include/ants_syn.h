@@ -212,6 +212,26 @@ int ants_syn_get_kernel_stats(const ants_syn_handle* h, ants_syn_kernel_stats* o
 /* The same restricted to the spans of pyramid level `level` of the last run. */
 int ants_syn_get_kernel_stats_level(const ants_syn_handle* h, int level, ants_syn_kernel_stats* out);
 
+/* ---- launch-shape tuning (no counterpart in ANTs: device launch plumbing) ----
+ * A table of launch shapes per kernel and grid-size bucket
+ * (log2_voxels = floor(log2(level voxels))), written by the project's
+ * autotuner (ants_amd/autotune.py) and loaded when the library is loaded
+ * from tuning.txt beside it (ANTS_SYN_TUNING=<file> or "none" overrides).
+ * Knobs: 0 inversion blocks per problem, 1 CC z-chunk planes, 2 smoothing
+ * z-chunk planes, 3 CC tile (1: 16x16, 2: 16x8), 4 smoothing tile
+ * (1: 16x32, 2: 16x16); value 0 = no entry (the built-in launch model
+ * decides).  Not thread-safe against running registrations: change the
+ * table between runs. */
+#define ANTS_TUNE_INV_BLOCKS 0
+#define ANTS_TUNE_CC_ZCHUNK 1
+#define ANTS_TUNE_SMOOTH_ZCHUNK 2
+#define ANTS_TUNE_CC_TILE 3
+#define ANTS_TUNE_SMOOTH_TILE 4
+int ants_tuning_set(int knob, int log2_voxels, int value);
+int ants_tuning_get(int knob, int log2_voxels, int* value);
+int ants_tuning_clear(void);
+int ants_tuning_load(const char* path);
+
 /* ---- building blocks (device buffers) ----
  * Each replaces the ITK filter that SyN calls at the cited ANTs site. */
 

@@ -65,3 +65,42 @@ def test_compose_affines_applies_the_first_transform_first():
         np.testing.assert_allclose(apply(comp, p), apply(ts[2], apply(ts[1], apply(ts[0], p))), atol=1e-9)
     with pytest.raises(ValueError):
         compose_affines([None])
+
+
+def test_tuning_table_set_get_load(tmp_path):
+    """The launch-shape tuning table: set / get per knob and log2 bucket,
+    a file in the tuner's format, and refusal of bad entries."""
+    import ctypes as C
+    from ants_amd import _lib as L
+    lib = L.lib()
+    saved = {}
+    v = C.c_int()
+    for k in range(5):
+        for b in range(40):
+            L.check(lib.ants_tuning_get(k, b, C.byref(v)))
+            saved[(k, b)] = v.value
+    try:
+        L.check(lib.ants_tuning_clear())
+        L.check(lib.ants_tuning_set(0, 20, 512))
+        L.check(lib.ants_tuning_get(0, 20, C.byref(v)))
+        assert v.value == 512
+        L.check(lib.ants_tuning_get(0, 19, C.byref(v)))
+        assert v.value == 0
+        assert lib.ants_tuning_set(3, 20, 3) == 1      # tiles are 1 or 2
+        assert lib.ants_tuning_set(7, 20, 1) == 1      # no such knob
+        assert lib.ants_tuning_set(1, 40, 8) == 1      # no such bucket
+        f = tmp_path / "tuning.txt"
+        f.write_text("# knob bucket value\ncc_zchunk 17 12\nsmooth_tile 20 2  # narrow tiles\n\n")
+        L.check(lib.ants_tuning_load(str(f).encode()))
+        L.check(lib.ants_tuning_get(1, 17, C.byref(v)))
+        assert v.value == 12
+        L.check(lib.ants_tuning_get(4, 20, C.byref(v)))
+        assert v.value == 2
+        bad = tmp_path / "bad.txt"
+        bad.write_text("cc_zchunk 17\n")
+        assert lib.ants_tuning_load(str(bad).encode()) == 1
+    finally:
+        L.check(lib.ants_tuning_clear())
+        for (k, b), val in saved.items():
+            if val:
+                L.check(lib.ants_tuning_set(k, b, val))
