@@ -565,6 +565,9 @@ struct ants_syn_handle {
   Profiler prof;
   std::map<GraphKey, GraphEntry> graphs;
   bool use_graphs = true;
+  // second stream and events of the pipelined iteration loop (created on first use)
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_main = nullptr, ev_front = nullptr, ev_inv0 = nullptr;
   int inv_launches = kInvLaunchesDefault;  // starting value per level
   bool inv_launches_fixed = false;          // set by ANTS_SYN_INV_LAUNCHES: no adaptation
   std::vector<const void*> graph_signature;  // buffers + geometry the cached graphs were built for
@@ -620,6 +623,9 @@ struct ants_syn_handle {
     if (host_stats) (void)hipHostFree(host_stats);
     if (host_flag) (void)hipHostFree(host_flag);
     if (own_stream && stream) (void)hipStreamDestroy(stream);
+    for (hipEvent_t e : {ev_main, ev_front, ev_inv0})
+      if (e) (void)hipEventDestroy(e);
+    if (stream2) (void)hipStreamDestroy(stream2);
   }
 
   int preprocess(const float* in, float* out, int64_t n);
@@ -837,17 +843,18 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
   ANTS_TRY(Fw.ensure(maxN * sizeof(float)));
   ANTS_TRY(Mw.ensure(maxN * sizeof(float)));
   for (auto& b : fld) ANTS_TRY(b.ensure(fbytes));
-  ANTS_TRY(stats.ensure(sizeof(IterStats)));
+  ANTS_TRY(stats.ensure(2 * sizeof(IterStats)));  // two iteration parities
   int64_t ccp = 0, invp = 0, invw = 0;
   for (int l = 0; l < P.n_levels; ++l) {
     ccp = std::max(ccp, cc_partials_needed(to_dev(lg[l]), P.cc_radius, bf16));
     invp = std::max(invp, inv_partials_needed(to_dev(lg[l])));
     invw = std::max(invw, inv_mask_words(to_dev(lg[l])));
   }
-  ANTS_TRY(partials_cc.ensure(ccp * sizeof(Partial)));
+  ANTS_TRY(partials_cc.ensure(2 * ccp * sizeof(Partial)));  // two iteration parities
   ANTS_TRY(partials_inv.ensure(4 * invp * sizeof(Partial)));  // 2 step parities x 2 problems
   ANTS_TRY(masks_inv.ensure(4 * invw * sizeof(uint64_t)));    // 2 step parities x 2 problems
-  if (!host_stats) HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&host_stats), sizeof(IterStats), hipHostMallocDefault));
+  if (!host_stats)
+    HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&host_stats), 2 * sizeof(IterStats), hipHostMallocDefault));
   if (!host_flag) HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&host_flag), 256, hipHostMallocCoherent));
 
   hipStream_t s = stream;
@@ -903,6 +910,31 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
     inv_launches = std::min(std::max(inv_launches, 1), kInvMaxIter + 1);
     inv_launches_fixed = ks != nullptr;
   }
+  // ANTS_SYN_PIPELINE=1 pipelines iterations over two streams (the next
+  // iteration's front end beside this one's inversions; not with graph
+  // replay, nor while the profiler times front-end kernel classes, whose
+  // spans live on the main stream).  Same bits, but measured no faster:
+  // 41.9 against 41.0 ms per 192x224x192 pair.  The overlapped kernels slow
+  // each other down (at 96x112x96 CC 31 -> 63 us, smoothing 45 -> 70 us,
+  // inversion steps 18 -> 35-49 us), so the small levels are bound by
+  // per-CU issue, not by launch latency.  Off by default.
+  bool pipelined = false;
+  {
+    const char* pe = std::getenv("ANTS_SYN_PIPELINE");
+    const unsigned int front_classes = (1u << ANTS_KC_WARP) | (1u << ANTS_KC_CC) | (1u << ANTS_KC_SMOOTH);
+    pipelined = !use_graphs && (prof.mask & front_classes) == 0 && (pe && pe[0] == '1');
+    if (pipelined && !stream2) HIP_TRY(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
+    for (hipEvent_t* e : {&ev_main, &ev_front, &ev_inv0})
+      if (pipelined && !*e) HIP_TRY(hipEventCreateWithFlags(e, hipEventDisableTiming));
+  }
+  hipStream_t s2 = stream2;
+  // whatever path leaves run(), nothing of it is still running on the second stream
+  struct SecondStreamDrain {
+    hipStream_t s2;
+    ~SecondStreamDrain() {
+      if (s2) (void)hipStreamSynchronize(s2);
+    }
+  } drain{pipelined ? s2 : nullptr};
   std::memset(&run_stats, 0, sizeof(run_stats));
   trace.clear();
   std::memset(&prof.acc, 0, sizeof(prof.acc));
@@ -986,9 +1018,18 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
     // iteration in (nst, nwk).  Host work here is only pointer bookkeeping,
     // so the sequence can be captured once into a hipGraph and replayed.
     // inversion steps j0..j1-1 of one batch (two problems, one launch per step)
+    // Statistics are double-buffered by iteration parity `par`: while
+    // iteration i's inversions run, iteration i+1's front end (warp, CC,
+    // smoothing) may already fill the other block.
+    constexpr int kStatWords = static_cast<int>(sizeof(IterStats) / 4);
+    auto stats_of = [&](int par) { return dstats + par; };
+    auto hstats_of = [&](int par) { return host_stats + par; };
+    auto flag_of = [&](int par) { return host_flag + 32 * par; };
+    auto parts_cc_of = [&](int par) { return static_cast<Partial*>(partials_cc.p) + par * ccp; };
+    // inversion steps j0..j1-1 of one batch (two problems, one launch per step)
     auto inv_steps_on = [&](int batch, const float* const* field, float* const* inv, int j0, int j1,
-                            const InvGate& gate) -> int {
-      RedSlot* slots[2] = {dstats->inv[batch], dstats->inv[batch + 2]};
+                            const InvGate& gate, int par) -> int {
+      RedSlot* slots[2] = {stats_of(par)->inv[batch], stats_of(par)->inv[batch + 2]};
       Partial* parts = static_cast<Partial*>(partials_inv.p);
       uint64_t* masks = static_cast<uint64_t*>(masks_inv.p);
       // step 0 is preceded by the sample launch that bounds its max norm
@@ -998,9 +1039,9 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
       return ANTS_OK;
     };
     // host-completed steps: one timed span per call
-    auto inv_tail_on = [&](int batch, const float* const* field, float* const* inv, int j0) -> int {
+    auto inv_tail_on = [&](int batch, const float* const* field, float* const* inv, int j0, int par) -> int {
       prof.begin(ANTS_KC_INV);
-      ANTS_TRY(inv_steps_on(batch, field, inv, j0, kInvMaxIter + 1, kNoGate));
+      ANTS_TRY(inv_steps_on(batch, field, inv, j0, kInvMaxIter + 1, kNoGate, par));
       prof.end(ANTS_KC_INV, 0.0, kInvMaxIter + 1 - j0 + (j0 == 0 && inv_speculates(gd) ? 1 : 0), 1 << batch, j0,
                kInvMaxIter + 1);
       return ANTS_OK;
@@ -1008,15 +1049,19 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
     // After the iteration's sequence: a batch whose problems computed a norm
     // at the last launched step has further steps to run.  If the first
     // batch is unfinished the gated second batch did nothing, and both are
-    // run to the end here; otherwise only the second batch's tail.  In terms
-    // of the rotated buffers: batch 0 inverts st[0]/st[2] into st[1]/st[3],
-    // batch 1 inverts st[1]/st[3] into st[0]/st[2].
-    auto inv_finish = [&](bool& extra) -> int {
+    // run to the end here (first0 = true: the inverse middle fields changed
+    // after the launches that were enqueued with the iteration); otherwise
+    // only the second batch's tail.  In terms of the rotated buffers: batch
+    // 0 inverts st[0]/st[2] into st[1]/st[3], batch 1 inverts st[1]/st[3]
+    // into st[0]/st[2].
+    auto inv_finish = [&](bool& extra, bool& first0, int par) -> int {
       extra = false;
+      first0 = false;
       const int K = kl;
       if (K > kInvMaxIter) return ANTS_OK;
+      const IterStats* hs = hstats_of(par);
       auto pending = [&](int batch) {
-        return host_stats->inv[batch][K - 1].computed || host_stats->inv[batch + 2][K - 1].computed;
+        return hs->inv[batch][K - 1].computed || hs->inv[batch + 2][K - 1].computed;
       };
       const float* f0[2] = {st[0], st[2]};
       float* i0[2] = {st[1], st[3]};
@@ -1026,99 +1071,117 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
         // the gated launches of batch 1 did nothing: they carry no bytes
         for (auto& r : prof.recs)
           if (r.cls == ANTS_KC_INV) r.batches &= ~2;
-        ANTS_TRY(inv_tail_on(0, f0, i0, K));
-        ANTS_TRY(inv_tail_on(1, f1, i1, 0));
+        ANTS_TRY(inv_tail_on(0, f0, i0, K, par));
+        ANTS_TRY(inv_tail_on(1, f1, i1, 0, par));
         extra = true;
+        first0 = true;
       } else if (pending(1)) {
-        ANTS_TRY(inv_tail_on(1, f1, i1, K));
+        ANTS_TRY(inv_tail_on(1, f1, i1, K, par));
         extra = true;
       }
       if (extra) {
-        HIP_TRY(hipMemcpyAsync(host_stats, dstats, sizeof(IterStats), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(hstats_of(par), stats_of(par), sizeof(IterStats), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
         run_stats.inv_tails += 1;
       }
       return ANTS_OK;
     };
 
-    auto enqueue = [&](float* const* cst, float* const* cwk, float** nst, float** nwk) -> int {
-      float* st_[4] = {cst[0], cst[1], cst[2], cst[3]};
-      float* wk_[4] = {cwk[0], cwk[1], cwk[2], cwk[3]};
-      // 1. warp both smoothed images through the inverse middle fields (the
-      //    warp's first block also zeroes this iteration's statistics)
-      const double Nd = static_cast<double>(N);
-      uint32_t* stat_words = reinterpret_cast<uint32_t*>(dstats);
-      constexpr int kStatWords = static_cast<int>(sizeof(IterStats) / 4);
+    const double Nd = static_cast<double>(N);
+    // Front end of an iteration on stream fs: warp both smoothed images
+    // through the inverse middle fields cst[1], cst[3] (the warp's first
+    // block also zeroes the iteration's statistics), CC value and both
+    // derivatives into wk[0], wk[1], update-field smoothing + max norms
+    // (result in wk[0], wk[1]; scratch wk[2], wk[3]).  It writes only Fw, Mw,
+    // the four work buffers and its own statistics block; wk is updated to
+    // the rotation it leaves.
+    auto front = [&](int par, float* const* cst, float* wk_[4], hipStream_t fs, int& n_cc) -> int {
+      uint32_t* stat_words = reinterpret_cast<uint32_t*>(stats_of(par));
       prof.begin(ANTS_KC_WARP);
       if (bf16)
-        launch_warp_pair_bf16(gd, gFd, static_cast<const uint16_t*>(Fb.p), st_[1], Fw.f(), gMd,
-                              static_cast<const uint16_t*>(Mb.p), st_[3], Mw.f(), s, stat_words, kStatWords,
+        launch_warp_pair_bf16(gd, gFd, static_cast<const uint16_t*>(Fb.p), cst[1], Fw.f(), gMd,
+                              static_cast<const uint16_t*>(Mb.p), cst[3], Mw.f(), fs, stat_words, kStatWords,
                               use_maps ? maps : nullptr);
       else
-        launch_warp_pair(gd, gFd, Fs.f(), st_[1], Fw.f(), gMd, Ms.f(), st_[3], Mw.f(), s, stat_words, kStatWords,
+        launch_warp_pair(gd, gFd, Fs.f(), cst[1], Fw.f(), gMd, Ms.f(), cst[3], Mw.f(), fs, stat_words, kStatWords,
                          use_maps ? maps : nullptr);
       prof.end(ANTS_KC_WARP, Nd * (2.0 * D * e_f + 2.0 * gsz * e_i + 2.0 * e_i));
-      // 2. CC value and both derivatives (g_f2m -> wk[0], g_m2m -> wk[1])
       prof.begin(ANTS_KC_CC);
       // the energy slot is reduced when the iteration's statistics are published
-      const int n_cc = launch_cc(gd, Fw.f(), Mw.f(), P.cc_radius, eps, wk_[0], wk_[1],
-                                 static_cast<Partial*>(partials_cc.p), &dstats->energy, s, false, bf16);
+      n_cc = launch_cc(gd, Fw.f(), Mw.f(), P.cc_radius, eps, wk_[0], wk_[1], parts_cc_of(par), &stats_of(par)->energy,
+                       fs, false, bf16);
       prof.end(ANTS_KC_CC, Nd * (2.0 * e_i + 2.0 * D * e_f));
       HIP_TRY(hipGetLastError());
-      // 3. update-field smoothing + max norms (result in wk[0], wk[1]; scratch wk[2], wk[3])
-      {
-        float* in[2] = {wk_[0], wk_[1]};
-        float* work[2] = {wk_[2], wk_[3]};
-        float* out[2] = {D % 2 == 1 ? wk_[2] : wk_[0], D % 2 == 1 ? wk_[3] : wk_[1]};
-        ANTS_TRY(smooth_fields_on(s, g, 2, in, work, out, P.update_variance, dstats->maxnorm, &prof));
-        if (out[0] != wk_[0]) {
-          std::swap(wk_[0], wk_[2]);
-          std::swap(wk_[1], wk_[3]);
-        }
+      float* in[2] = {wk_[0], wk_[1]};
+      float* work[2] = {wk_[2], wk_[3]};
+      float* out[2] = {D % 2 == 1 ? wk_[2] : wk_[0], D % 2 == 1 ? wk_[3] : wk_[1]};
+      ANTS_TRY(smooth_fields_on(fs, g, 2, in, work, out, P.update_variance, stats_of(par)->maxnorm, &prof));
+      if (out[0] != wk_[0]) {
+        std::swap(wk_[0], wk_[2]);
+        std::swap(wk_[1], wk_[3]);
       }
-      // 4. scaled composition onto the forward middle fields: t -> wk[2], wk[3]
+      return ANTS_OK;
+    };
+    // Back end on the main stream: scaled composition onto the forward
+    // middle fields (t -> wk[2], wk[3]), total-field smoothing, the first kl
+    // steps of both inversion batches (inverse middle fields from the
+    // previous inverses in place, then forward fields as inverses of the new
+    // inverses, started from t, in place; the second batch gated on the
+    // first having finished, the host completing whatever is left), and the
+    // publication of the statistics.  `ev_inv` (optional) is recorded once
+    // the first batch's launches are enqueued: from there on the inverse
+    // middle fields are final unless that batch needs a host tail.  st_/wk_
+    // are rotated in place.
+    auto back = [&](int par, float* st_[4], float* wk_[4], int n_cc, hipEvent_t ev_inv) -> int {
       {
         const float* disp[2] = {wk_[0], wk_[1]};
         const float* warp[2] = {st_[0], st_[2]};
         float* out[2] = {wk_[2], wk_[3]};
         prof.begin(ANTS_KC_COMPOSE);
-        launch_compose(gd, 2, disp, warp, out, dstats->maxnorm, static_cast<float>(P.learning_rate), s);
+        launch_compose(gd, 2, disp, warp, out, stats_of(par)->maxnorm, static_cast<float>(P.learning_rate), s);
         prof.end(ANTS_KC_COMPOSE, 2.0 * Nd * 3.0 * D * e_f);
         HIP_TRY(hipGetLastError());
       }
-      // 5. total-field smoothing (a copy when the variance is 0)
       if (P.total_variance > 0.0) {
         float* in[2] = {wk_[2], wk_[3]};
         float* work[2] = {wk_[0], wk_[1]};
         float* out[2] = {D % 2 == 1 ? wk_[0] : wk_[2], D % 2 == 1 ? wk_[1] : wk_[3]};
-        ANTS_TRY(smooth_fields_on(s, g, 2, in, work, out, P.total_variance, dstats->maxnorm + 2, &prof));
+        ANTS_TRY(smooth_fields_on(s, g, 2, in, work, out, P.total_variance, stats_of(par)->maxnorm + 2, &prof));
         if (out[0] != wk_[2]) {
           std::swap(wk_[0], wk_[2]);
           std::swap(wk_[1], wk_[3]);
         }
       }
-      // 6. inversions: inverse middle fields from the previous inverses (in place),
-      //    then forward fields as inverses of the new inverses, started from t (in place)
-      //    Only the first kl steps of each batch are enqueued here:
-      //    almost every inversion converges within them.  The second batch's
-      //    launches are gated on the first having finished (a no-op
-      //    otherwise); the host completes whatever is left after reading the
-      //    step flags (inv_finish below).
       prof.begin(ANTS_KC_INV);
       for (int batch = 0; batch < 2; ++batch) {
         const float* field[2] = {batch == 0 ? wk_[2] : st_[1], batch == 0 ? wk_[3] : st_[3]};
         float* inv[2] = {batch == 0 ? st_[1] : wk_[2], batch == 0 ? st_[3] : wk_[3]};
         InvGate gate = kNoGate;
         if (batch == 1 && kl <= kInvMaxIter)
-          gate = InvGate{{dstats->inv[0], dstats->inv[2]}, 2, kl - 1};
-        ANTS_TRY(inv_steps_on(batch, field, inv, 0, kl, gate));
+          gate = InvGate{{stats_of(par)->inv[0], stats_of(par)->inv[2]}, 2, kl - 1};
+        ANTS_TRY(inv_steps_on(batch, field, inv, 0, kl, gate, par));
+        if (batch == 0 && ev_inv) HIP_TRY(hipEventRecord(ev_inv, s));
       }
       prof.end(ANTS_KC_INV, 0.0, 2 * (kl + (inv_speculates(gd) ? 1 : 0)), 3, 0, kl);  // per batch: [sample] + kl steps
       std::swap(st_[0], wk_[2]);
       std::swap(st_[2], wk_[3]);
-      launch_publish_stats(static_cast<const Partial*>(partials_cc.p), n_cc, &dstats->energy, stat_words,
-                           reinterpret_cast<uint32_t*>(host_stats), kStatWords, host_flag, s);
+      launch_publish_stats(parts_cc_of(par), n_cc, &stats_of(par)->energy,
+                           reinterpret_cast<uint32_t*>(stats_of(par)), reinterpret_cast<uint32_t*>(hstats_of(par)),
+                           kStatWords, flag_of(par), s);
       HIP_TRY(hipGetLastError());
+      return ANTS_OK;
+    };
+    // One SyN iteration as a fixed launch sequence on the main stream over
+    // buffer roles derived from the rotation (cst, cwk); it reports the
+    // rotation after the iteration in (nst, nwk).  Host work here is only
+    // pointer bookkeeping, so the sequence can be captured once into a
+    // hipGraph and replayed.
+    auto enqueue = [&](float* const* cst, float* const* cwk, float** nst, float** nwk) -> int {
+      float* st_[4] = {cst[0], cst[1], cst[2], cst[3]};
+      float* wk_[4] = {cwk[0], cwk[1], cwk[2], cwk[3]};
+      int n_cc = 0;
+      ANTS_TRY(front(0, st_, wk_, s, n_cc));
+      ANTS_TRY(back(0, st_, wk_, n_cc, nullptr));
       for (int k = 0; k < 4; ++k) {
         nst[k] = st_[k];
         nwk[k] = wk_[k];
@@ -1126,65 +1189,21 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
       return ANTS_OK;
     };
 
-    while (iter++ < P.iterations[l] && !converged) {
-      float* nst[4];
-      float* nwk[4];
-      prof.active = (iter - 1) % prof.every == 0 ? prof.mask : 0u;
-      GraphKey key{setup_hash, l, {st[0], st[1], st[2], st[3], wk[0], wk[1], wk[2], wk[3]}, prof.mask, kl};
-      auto it = use_graphs ? graphs.find(key) : graphs.end();
-      if (use_graphs && it == graphs.end()) {
-        // capture this rotation's sequence once
-        GraphEntry ge{};
-        int rc = ANTS_OK;
-        if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
-          rc = enqueue(st, wk, ge.nst, ge.nwk);
-          hipGraph_t graph = nullptr;
-          const hipError_t e = hipStreamEndCapture(s, &graph);
-          hipError_t ei = hipErrorUnknown;
-          if (rc == ANTS_OK && e == hipSuccess && graph)
-            ei = hipGraphInstantiate(&ge.exec, graph, nullptr, nullptr, 0);
-          if (ei != hipSuccess && std::getenv("ANTS_SYN_VERBOSE"))
-            std::fprintf(stderr, "ants_syn: iteration capture at level %d failed (enqueue %d: %s; end %s; instantiate %s)\n",
-                         l, rc, rc == ANTS_OK ? "" : g_last_error.c_str(), hipGetErrorString(e), hipGetErrorString(ei));
-          if (ei == hipSuccess) it = graphs.emplace(key, std::move(ge)).first;
-          if (graph) (void)hipGraphDestroy(graph);
-        }
-        (void)hipGetLastError();
-        if (it == graphs.end()) use_graphs = false;  // capture not possible here: launch eagerly
-      }
-      *reinterpret_cast<volatile uint32_t*>(host_flag) = 0u;
-      if (use_graphs && it != graphs.end()) {
-        HIP_TRY(hipGraphLaunch(it->second.exec, s));
-        run_stats.graph_launches += 1;
-        for (int k = 0; k < 4; ++k) {
-          nst[k] = it->second.nst[k];
-          nwk[k] = it->second.nwk[k];
-        }
-      } else {
-        ANTS_TRY(enqueue(st, wk, nst, nwk));
-        run_stats.eager_iterations += 1;
-      }
-      for (int k = 0; k < 4; ++k) {
-        st[k] = nst[k];
-        wk[k] = nwk[k];
-      }
-      // 7. energy, convergence, observer
-      ANTS_TRY(wait_published(host_flag, s));
-      bool extra = false;
-      ANTS_TRY(inv_finish(extra));
-      if (extra && !inv_launches_fixed) kl = std::min(kl + 1, kInvMaxIter + 1);
-      prof.collect(host_stats, N, D);
-      const double value = -host_stats->energy.sum / static_cast<double>(N);
+    // 7. energy, convergence, observer, work counters of a finished iteration
+    auto account = [&](int par, bool& stop) -> int {
+      const IterStats* hs = hstats_of(par);
+      prof.collect(hs, N, D);
+      const double value = -hs->energy.sum / static_cast<double>(N);
       const double E = 0.5 * (value + value);
       int kin[4];
       for (int q = 0; q < 4; ++q) {
         int c = 0;
-        for (int j = 0; j < kInvSlots; ++j) c += host_stats->inv[q][j].computed ? 1 : 0;
+        for (int j = 0; j < kInvSlots; ++j) c += hs->inv[q][j].computed ? 1 : 0;
         kin[q] = c;
       }
       mon.add(E);
       const double conv = mon.value();
-      if (!P.force_iterations && conv < P.convergence_threshold) converged = true;
+      if (!P.force_iterations && conv < P.convergence_threshold) stop = true;
       run_stats.iterations[l] = iter;
       const int ksum = kin[0] + kin[1] + kin[2] + kin[3];
       run_stats.inv_steps[l] += ksum;
@@ -1193,6 +1212,116 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
           static_cast<double>(N) * ((2.0 * gsz * e_i + 14.0 * D * e_f) + (3.0 * ksum + 4.0) * D * e_f);
       trace.push_back(ants_syn_iteration_record{l, iter, E, conv, {kin[0], kin[1], kin[2], kin[3]}});
       if (cb && cb(user, l, iter, E, conv, kin) != 0) return fail(ANTS_ERR_CANCELLED, "cancelled by the observer");
+      return ANTS_OK;
+    };
+
+    if (pipelined) {
+      // Iteration i+1's front end runs on the second stream while iteration
+      // i's inversions finish: it needs only the new inverse middle fields
+      // (final once the first inversion batch is done) and writes only
+      // buffers that the second batch does not touch.  It is enqueued before
+      // iteration i's convergence is known; past the level's last iteration
+      // its results are simply not used.  When the first batch needs a host
+      // tail, the front end is run again after it.
+      HIP_TRY(hipEventRecord(ev_main, s));
+      HIP_TRY(hipStreamWaitEvent(s2, ev_main, 0));
+      int par = 0, n_cc = 0;
+      if (iter < P.iterations[l]) {
+        ANTS_TRY(front(par, st, wk, s2, n_cc));
+        HIP_TRY(hipEventRecord(ev_front, s2));
+      }
+      while (iter++ < P.iterations[l] && !converged) {
+        prof.active = (iter - 1) % prof.every == 0 ? prof.mask : 0u;
+        HIP_TRY(hipStreamWaitEvent(s, ev_front, 0));
+        *reinterpret_cast<volatile uint32_t*>(flag_of(par)) = 0u;
+        ANTS_TRY(back(par, st, wk, n_cc, ev_inv0));
+        run_stats.eager_iterations += 1;
+        const bool next = iter < P.iterations[l];
+        float* wk_next[4] = {wk[0], wk[1], wk[2], wk[3]};
+        int n_cc_next = 0;
+        if (next) {
+          HIP_TRY(hipStreamWaitEvent(s2, ev_inv0, 0));
+          ANTS_TRY(front(par ^ 1, st, wk_next, s2, n_cc_next));
+          HIP_TRY(hipEventRecord(ev_front, s2));
+        }
+        ANTS_TRY(wait_published(flag_of(par), s));
+        bool extra = false, first0 = false;
+        ANTS_TRY(inv_finish(extra, first0, par));
+        if (extra && !inv_launches_fixed) kl = std::min(kl + 1, kInvMaxIter + 1);
+        if (first0 && next) {
+          // the inverse middle fields changed after the front end started
+          for (int k = 0; k < 4; ++k) wk_next[k] = wk[k];
+          HIP_TRY(hipEventRecord(ev_main, s));
+          HIP_TRY(hipStreamWaitEvent(s2, ev_main, 0));
+          ANTS_TRY(front(par ^ 1, st, wk_next, s2, n_cc_next));
+          HIP_TRY(hipEventRecord(ev_front, s2));
+        }
+        for (int k = 0; k < 4; ++k) wk[k] = wk_next[k];
+        n_cc = n_cc_next;
+        bool stop = false;
+        const int rc = account(par, stop);
+        if (rc != ANTS_OK) {
+          (void)hipStreamSynchronize(s2);
+          return rc;
+        }
+        if (stop) converged = true;
+        par ^= 1;
+      }
+      // a front end enqueued past the level's last iteration finishes before
+      // the level's buffers are reused
+      HIP_TRY(hipEventRecord(ev_main, s2));
+      HIP_TRY(hipStreamWaitEvent(s, ev_main, 0));
+    } else {
+      while (iter++ < P.iterations[l] && !converged) {
+        float* nst[4];
+        float* nwk[4];
+        prof.active = (iter - 1) % prof.every == 0 ? prof.mask : 0u;
+        GraphKey key{setup_hash, l, {st[0], st[1], st[2], st[3], wk[0], wk[1], wk[2], wk[3]}, prof.mask, kl};
+        auto it = use_graphs ? graphs.find(key) : graphs.end();
+        if (use_graphs && it == graphs.end()) {
+          // capture this rotation's sequence once
+          GraphEntry ge{};
+          int rc = ANTS_OK;
+          if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+            rc = enqueue(st, wk, ge.nst, ge.nwk);
+            hipGraph_t graph = nullptr;
+            const hipError_t e = hipStreamEndCapture(s, &graph);
+            hipError_t ei = hipErrorUnknown;
+            if (rc == ANTS_OK && e == hipSuccess && graph)
+              ei = hipGraphInstantiate(&ge.exec, graph, nullptr, nullptr, 0);
+            if (ei != hipSuccess && std::getenv("ANTS_SYN_VERBOSE"))
+              std::fprintf(stderr, "ants_syn: iteration capture at level %d failed (enqueue %d: %s; end %s; instantiate %s)\n",
+                           l, rc, rc == ANTS_OK ? "" : g_last_error.c_str(), hipGetErrorString(e), hipGetErrorString(ei));
+            if (ei == hipSuccess) it = graphs.emplace(key, std::move(ge)).first;
+            if (graph) (void)hipGraphDestroy(graph);
+          }
+          (void)hipGetLastError();
+          if (it == graphs.end()) use_graphs = false;  // capture not possible here: launch eagerly
+        }
+        *reinterpret_cast<volatile uint32_t*>(flag_of(0)) = 0u;
+        if (use_graphs && it != graphs.end()) {
+          HIP_TRY(hipGraphLaunch(it->second.exec, s));
+          run_stats.graph_launches += 1;
+          for (int k = 0; k < 4; ++k) {
+            nst[k] = it->second.nst[k];
+            nwk[k] = it->second.nwk[k];
+          }
+        } else {
+          ANTS_TRY(enqueue(st, wk, nst, nwk));
+          run_stats.eager_iterations += 1;
+        }
+        for (int k = 0; k < 4; ++k) {
+          st[k] = nst[k];
+          wk[k] = nwk[k];
+        }
+        ANTS_TRY(wait_published(flag_of(0), s));
+        bool extra = false, first0 = false;
+        ANTS_TRY(inv_finish(extra, first0, 0));
+        if (extra && !inv_launches_fixed) kl = std::min(kl + 1, kInvMaxIter + 1);
+        bool stop = false;
+        ANTS_TRY(account(0, stop));
+        if (stop) converged = true;
+      }
     }
   }
   // final composition: fwd = M2M.inv o F2M.fwd ; inv = F2M.inv o M2M.fwd

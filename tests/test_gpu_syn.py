@@ -541,3 +541,41 @@ def test_strict_tie_rule_against_oracle(monkeypatch):
           f"{ref_strict.inv_iters[:1].tolist()}; vs guarded oracle "
           f"{diff(ref_strict.fwd, ref_strict.inv, ref_guarded.fwd, ref_guarded.inv):.2e} mm; vs strict GPU "
           f"{diff(ref_strict.fwd, ref_strict.inv, strict.fwd, strict.inv):.2e} mm", flush=True)
+
+
+@pytest.mark.parametrize("case", ["3d_converging", "3d_host_tails", "2d_total_variance", "3d_bf16"])
+def test_pipelined_iterations_are_exact(monkeypatch, case):
+    """The next iteration's front end (warp, CC, smoothing) on a second
+    stream beside this iteration's inversions (ANTS_SYN_PIPELINE=1) gives the same bits as the
+    strictly sequential launches: with the convergence monitor stopping
+    levels early (a front end enqueued past the last iteration is dropped),
+    with host tails on every iteration (one inversion launch per batch, so
+    the front end is run again after the first batch's tail), in 2-D with a
+    total-field variance (the separable smoothing's buffer swaps), and in
+    the BF16 form."""
+    if case == "2d_total_variance":
+        pair = ph.blob_pair_2d(6, n=96)
+        stage = SyNStage.from_args("SyN[0.25,3,1]", "CC[f,m,1,4]", "[30x20x10,1e-6,10]", "4x2x1", "2x1x0vox")
+    else:
+        pair = ph.phantom_pair_3d(8, size=(56, 64, 48), max_disp_mm=4.0, svf_sigma_mm=6.0)
+        stage = SyNStage.from_args("SyN[0.1,3,0]", "CC[f,m,1,4]", "[60x30x10,1e-6,10]", "4x2x1", "2x1x0vox")
+    if case == "3d_host_tails":
+        monkeypatch.setenv("ANTS_SYN_INV_LAUNCHES", "1")
+    prec = "bf16" if case == "3d_bf16" else "fp32"
+    outs = {}
+    for pipe in ("0", "1"):
+        monkeypatch.setenv("ANTS_SYN_PIPELINE", pipe)
+        reg = SyNRegistration(stage, len(pair.size), precision=prec)
+        res = reg.register(pair.fixed, pair.moving, pair.spacing, pair.origin, pair.spacing, pair.origin)
+        outs[pipe] = (res, reg.stats())
+        reg.close()
+    (a, sa), (b, sb) = outs["0"], outs["1"]
+    print(f"{case}: iterations {sa['iterations']}, host tails {sb['inv_tails']}", flush=True)
+    np.testing.assert_array_equal(a.energy, b.energy)
+    np.testing.assert_array_equal(a.fwd, b.fwd)
+    np.testing.assert_array_equal(a.inv, b.inv)
+    assert sa["iterations"] == sb["iterations"] and sa["inv_steps"] == sb["inv_steps"]
+    if case == "3d_host_tails":
+        assert sb["inv_tails"] == sum(sb["iterations"])
+    if case == "3d_converging":
+        assert sa["iterations"][0] < 60  # the monitor stopped a level early
