@@ -565,6 +565,11 @@ struct ants_syn_handle {
   Profiler prof;
   std::map<GraphKey, GraphEntry> graphs;
   bool use_graphs = true;
+  // host-buffer entry: device copies of the inputs and outputs (grow-only)
+  // and two pinned staging chunks for the transfers
+  DevBuf io_f, io_m, io_fwd, io_inv;
+  void* pin[2] = {nullptr, nullptr};
+  hipEvent_t pin_ev[2] = {nullptr, nullptr};
   // second stream and events of the pipelined iteration loop (created on first use)
   hipStream_t stream2 = nullptr;
   hipEvent_t ev_main = nullptr, ev_front = nullptr, ev_inv0 = nullptr;
@@ -599,8 +604,9 @@ struct ants_syn_handle {
 
   // device bytes the handle's grow-only workspace holds
   int64_t device_bytes() const {
-    const DevBuf* all[] = {&F,     &M,           &Fs,          &Ms,        &tmp,    &Fw,         &Mw,  &Fb,
-                           &Mb,    &stats,       &partials_cc, &partials_inv, &masks_inv, &minmax, &restore_buf};
+    const DevBuf* all[] = {&F,      &M,           &Fs,          &Ms,        &tmp,    &Fw,          &Mw,
+                           &Fb,     &Mb,          &stats,       &partials_cc, &partials_inv, &masks_inv,
+                           &minmax, &restore_buf, &io_f,        &io_m,      &io_fwd, &io_inv};
     int64_t b = 0;
     for (const DevBuf* x : all) b += static_cast<int64_t>(x->bytes);
     for (const auto& x : fld) b += static_cast<int64_t>(x.bytes);
@@ -611,7 +617,8 @@ struct ants_syn_handle {
   void release_workspace() {
     clear_graphs();
     graph_signature.clear();
-    DevBuf* all[] = {&F, &M, &Fs, &Ms, &tmp, &Fw, &Mw, &Fb, &Mb, &stats, &partials_cc, &partials_inv, &masks_inv, &minmax};
+    DevBuf* all[] = {&F,     &M,           &Fs,          &Ms,        &tmp,    &Fw,   &Mw,   &Fb,     &Mb,
+                     &stats, &partials_cc, &partials_inv, &masks_inv, &minmax, &io_f, &io_m, &io_fwd, &io_inv};
     for (DevBuf* b : all) b->release();
     for (auto& b : fld) b.release();
     have_state = false;
@@ -623,8 +630,10 @@ struct ants_syn_handle {
     if (host_stats) (void)hipHostFree(host_stats);
     if (host_flag) (void)hipHostFree(host_flag);
     if (own_stream && stream) (void)hipStreamDestroy(stream);
-    for (hipEvent_t e : {ev_main, ev_front, ev_inv0})
+    for (hipEvent_t e : {ev_main, ev_front, ev_inv0, pin_ev[0], pin_ev[1]})
       if (e) (void)hipEventDestroy(e);
+    for (void* q : pin)
+      if (q) (void)hipHostFree(q);
     if (stream2) (void)hipStreamDestroy(stream2);
   }
 
@@ -634,6 +643,8 @@ struct ants_syn_handle {
                     double variance, unsigned int* maxbits);
   int run(const ants_image_desc* fd, const float* d_fixed, const ants_image_desc* md, const float* d_moving,
           float* d_out_fwd, float* d_out_inv, ants_syn_iteration_cb cb, void* user);
+  int staged_upload(void* d_dst, const void* host_src, size_t bytes);
+  int staged_download(void* host_dst, const void* d_src, size_t bytes);
 };
 
 namespace {
@@ -1342,6 +1353,83 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
 }
 
 // ---------------------------------------------------------------------------
+// Host-buffer transfers.  Copies from and to the caller's pageable memory go
+// through two pinned chunks: threads copy one chunk between the caller's
+// buffer and pinned memory while the DMA engine moves the other (a pageable
+// hipMemcpy stages through one host thread).
+namespace {
+
+constexpr size_t kPinChunk = size_t{16} << 20;
+
+void parallel_copy(void* dst, const void* src, size_t bytes) {
+  const unsigned hw = std::thread::hardware_concurrency();
+  const size_t nt = std::min<size_t>(std::max<size_t>(hw / 2, 1), std::min<size_t>(8, bytes / (size_t{1} << 20) + 1));
+  if (nt <= 1) {
+    std::memcpy(dst, src, bytes);
+    return;
+  }
+  const size_t part = (bytes / nt + 63) / 64 * 64;
+  std::vector<std::thread> th;
+  for (size_t k = 1; k < nt; ++k) {
+    const size_t o = k * part;
+    if (o >= bytes) break;
+    th.emplace_back([=] { std::memcpy(static_cast<char*>(dst) + o, static_cast<const char*>(src) + o,
+                                      std::min(part, bytes - o)); });
+  }
+  std::memcpy(dst, src, std::min(part, bytes));
+  for (auto& t : th) t.join();
+}
+
+}  // namespace
+
+static int ensure_pinned(ants_syn_handle* h) {
+  for (int k = 0; k < 2; ++k) {
+    if (!h->pin[k]) HIP_TRY(hipHostMalloc(&h->pin[k], kPinChunk, hipHostMallocDefault));
+    if (!h->pin_ev[k]) HIP_TRY(hipEventCreateWithFlags(&h->pin_ev[k], hipEventDisableTiming));
+  }
+  return ANTS_OK;
+}
+
+int ants_syn_handle::staged_upload(void* d_dst, const void* host_src, size_t bytes) {
+  ANTS_TRY(ensure_pinned(this));
+  const char* src = static_cast<const char*>(host_src);
+  char* dst = static_cast<char*>(d_dst);
+  for (size_t o = 0, k = 0; o < bytes; o += kPinChunk, ++k) {
+    const int b = static_cast<int>(k & 1);
+    // the last DMA out of this chunk (this call's or an earlier one's) has finished
+    HIP_TRY(hipEventSynchronize(pin_ev[b]));
+    const size_t n = std::min(kPinChunk, bytes - o);
+    parallel_copy(pin[b], src + o, n);
+    HIP_TRY(hipMemcpyAsync(dst + o, pin[b], n, hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipEventRecord(pin_ev[b], stream));
+  }
+  return ANTS_OK;
+}
+
+int ants_syn_handle::staged_download(void* host_dst, const void* d_src, size_t bytes) {
+  ANTS_TRY(ensure_pinned(this));
+  const char* src = static_cast<const char*>(d_src);
+  char* dst = static_cast<char*>(host_dst);
+  const size_t nchunks = (bytes + kPinChunk - 1) / kPinChunk;
+  auto issue = [&](size_t k) -> int {
+    const int b = static_cast<int>(k & 1);
+    const size_t o = k * kPinChunk, n = std::min(kPinChunk, bytes - o);
+    HIP_TRY(hipMemcpyAsync(pin[b], src + o, n, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipEventRecord(pin_ev[b], stream));
+    return ANTS_OK;
+  };
+  if (nchunks > 0) ANTS_TRY(issue(0));
+  for (size_t k = 0; k < nchunks; ++k) {
+    const int b = static_cast<int>(k & 1);
+    HIP_TRY(hipEventSynchronize(pin_ev[b]));
+    if (k + 1 < nchunks) ANTS_TRY(issue(k + 1));  // the other chunk: its previous copy-out is done
+    const size_t o = k * kPinChunk;
+    parallel_copy(dst + o, pin[b], std::min(kPinChunk, bytes - o));
+  }
+  return ANTS_OK;
+}
+
+// ---------------------------------------------------------------------------
 // C-ABI
 extern "C" {
 
@@ -1488,23 +1576,19 @@ int ants_syn_register(ants_syn_handle* h, const ants_image_desc* fixed, const fl
   if (!host_fixed || !host_moving) return fail(ANTS_ERR_INVALID, "null image buffer");
   const HostGrid gl = level_grid(gF, h->P.shrink[h->P.n_levels - 1]);
   const size_t fb = static_cast<size_t>(gl.count()) * gF.D * sizeof(float);
-  DevBuf df, dm, dfwd, dinv;
-  ANTS_TRY(df.ensure(gF.count() * sizeof(float)));
-  ANTS_TRY(dm.ensure(gM.count() * sizeof(float)));
-  ANTS_TRY(dfwd.ensure(fb));
-  ANTS_TRY(dinv.ensure(fb));
-  HIP_TRY(hipMemcpyAsync(df.p, host_fixed, gF.count() * sizeof(float), hipMemcpyHostToDevice, h->stream));
-  HIP_TRY(hipMemcpyAsync(dm.p, host_moving, gM.count() * sizeof(float), hipMemcpyHostToDevice, h->stream));
-  int rc = h->run(fixed, df.f(), moving, dm.f(), dfwd.f(), dinv.f(), cb, user);
+  // the handle keeps the device copies between calls (grow-only)
+  ANTS_TRY(h->io_f.ensure(gF.count() * sizeof(float)));
+  ANTS_TRY(h->io_m.ensure(gM.count() * sizeof(float)));
+  ANTS_TRY(h->io_fwd.ensure(fb));
+  ANTS_TRY(h->io_inv.ensure(fb));
+  ANTS_TRY(h->staged_upload(h->io_f.p, host_fixed, gF.count() * sizeof(float)));
+  ANTS_TRY(h->staged_upload(h->io_m.p, host_moving, gM.count() * sizeof(float)));
+  int rc = h->run(fixed, h->io_f.f(), moving, h->io_m.f(), h->io_fwd.f(), h->io_inv.f(), cb, user);
   if (rc == ANTS_OK) {
-    if (host_out_fwd) HIP_TRY(hipMemcpyAsync(host_out_fwd, dfwd.p, fb, hipMemcpyDeviceToHost, h->stream));
-    if (host_out_inv) HIP_TRY(hipMemcpyAsync(host_out_inv, dinv.p, fb, hipMemcpyDeviceToHost, h->stream));
+    if (host_out_fwd) ANTS_TRY(h->staged_download(host_out_fwd, h->io_fwd.p, fb));
+    if (host_out_inv) ANTS_TRY(h->staged_download(host_out_inv, h->io_inv.p, fb));
     HIP_TRY(hipStreamSynchronize(h->stream));
   }
-  df.release();
-  dm.release();
-  dfwd.release();
-  dinv.release();
   return rc;
 }
 
@@ -2256,14 +2340,10 @@ void ants_batch::worker(int slot) {
       auto hip_ok = [&](hipError_t e) {
         if (e != hipSuccess && rc == ANTS_OK) rc = fail(ANTS_ERR_HIP, "%s", hipGetErrorString(e));
       };
-      if (rc == ANTS_OK) {
-        hip_ok(hipMemcpyAsync(sb.moving.p, r.host_moving, gM.count() * sizeof(float), hipMemcpyHostToDevice, s));
-        if (!it.templ)
-          hip_ok(hipMemcpyAsync(sb.fixed.p, r.host_fixed, gF.count() * sizeof(float), hipMemcpyHostToDevice, s));
-        if (r.host_moving_labels)
-          hip_ok(hipMemcpyAsync(sb.lab.p, r.host_moving_labels, gM.count() * sizeof(int32_t),
-                                hipMemcpyHostToDevice, s));
-      }
+      if (rc == ANTS_OK) rc = h.staged_upload(sb.moving.p, r.host_moving, gM.count() * sizeof(float));
+      if (rc == ANTS_OK && !it.templ) rc = h.staged_upload(sb.fixed.p, r.host_fixed, gF.count() * sizeof(float));
+      if (rc == ANTS_OK && r.host_moving_labels)
+        rc = h.staged_upload(sb.lab.p, r.host_moving_labels, gM.count() * sizeof(int32_t));
       const float* dfix = it.templ ? it.templ->img.f() : sb.fixed.f();
       if (rc == ANTS_OK)
         rc = h.run(fd, dfix, &r.moving, sb.moving.f(), sb.fwd.f(), sb.inv.f(), batch_cancel_cb, &cancel[slot]);
@@ -2271,11 +2351,10 @@ void ants_batch::worker(int slot) {
         launch_labels_nn(to_dev(gl), sb.fwd.f(), to_dev(gM), static_cast<const int32_t*>(sb.lab.p),
                          static_cast<int32_t*>(sb.lab_out.p), s);
         hip_ok(hipGetLastError());
-        hip_ok(hipMemcpyAsync(r.host_out_labels, sb.lab_out.p, gl.count() * sizeof(int32_t), hipMemcpyDeviceToHost,
-                              s));
+        if (rc == ANTS_OK) rc = h.staged_download(r.host_out_labels, sb.lab_out.p, gl.count() * sizeof(int32_t));
       }
-      if (rc == ANTS_OK && r.host_out_fwd) hip_ok(hipMemcpyAsync(r.host_out_fwd, sb.fwd.p, fb, hipMemcpyDeviceToHost, s));
-      if (rc == ANTS_OK && r.host_out_inv) hip_ok(hipMemcpyAsync(r.host_out_inv, sb.inv.p, fb, hipMemcpyDeviceToHost, s));
+      if (rc == ANTS_OK && r.host_out_fwd) rc = h.staged_download(r.host_out_fwd, sb.fwd.p, fb);
+      if (rc == ANTS_OK && r.host_out_inv) rc = h.staged_download(r.host_out_inv, sb.inv.p, fb);
       hip_ok(hipStreamSynchronize(s));
     }
     res.status = rc;
