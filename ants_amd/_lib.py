@@ -146,6 +146,7 @@ def lib():
     L.ants_op_cc.argtypes = [pdesc, vp, vp, i, vp, vp, C.POINTER(d), vp]
     L.ants_op_warp_pair.argtypes = [pdesc, pdesc, vp, vp, vp, pdesc, vp, vp, vp, i, vp]
     L.ants_op_compose.argtypes = [pdesc, vp, vp, vp, vp]
+    L.ants_op_inverse_consistency.argtypes = [pdesc, vp, vp, i, C.POINTER(d), C.POINTER(d), C.POINTER(C.c_int64), vp]
     L.ants_op_scale_update.argtypes = [pdesc, vp, d, C.POINTER(d), vp]
     L.ants_op_invert.argtypes = [pdesc, vp, vp, vp, C.POINTER(i), C.POINTER(d), C.POINTER(d), vp]
     L.ants_op_invert_noalloc.argtypes = [pdesc, vp, vp, vp, C.POINTER(i), C.POINTER(d), C.POINTER(d), vp]

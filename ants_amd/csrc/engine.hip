@@ -554,6 +554,7 @@ struct ants_syn_handle {
   bool own_stream = false;
   DevBuf F, M, Fs, Ms, tmp, Fw, Mw, Fb, Mb;  // Fb, Mb: BF16 pyramid images
   DevBuf fld[8];  // 4 middle fields, 4 work fields
+  DevBuf blend;   // 2 fields: the separable smoothing's middle pass when a 3-D variance is below 0.5
   DevBuf stats, partials_cc, partials_inv, masks_inv, minmax;
   IterStats* host_stats = nullptr;  // pinned
   uint32_t* host_flag = nullptr;    // pinned, coherent: set when an iteration's statistics are published
@@ -606,7 +607,7 @@ struct ants_syn_handle {
   int64_t device_bytes() const {
     const DevBuf* all[] = {&F,      &M,           &Fs,          &Ms,        &tmp,    &Fw,          &Mw,
                            &Fb,     &Mb,          &stats,       &partials_cc, &partials_inv, &masks_inv,
-                           &minmax, &restore_buf, &io_f,        &io_m,      &io_fwd, &io_inv};
+                           &minmax, &restore_buf, &io_f,        &io_m,      &io_fwd, &io_inv, &blend};
     int64_t b = 0;
     for (const DevBuf* x : all) b += static_cast<int64_t>(x->bytes);
     for (const auto& x : fld) b += static_cast<int64_t>(x.bytes);
@@ -618,7 +619,8 @@ struct ants_syn_handle {
     clear_graphs();
     graph_signature.clear();
     DevBuf* all[] = {&F,     &M,           &Fs,          &Ms,        &tmp,    &Fw,   &Mw,   &Fb,     &Mb,
-                     &stats, &partials_cc, &partials_inv, &masks_inv, &minmax, &io_f, &io_m, &io_fwd, &io_inv};
+                     &stats, &partials_cc, &partials_inv, &masks_inv, &minmax, &io_f, &io_m, &io_fwd, &io_inv,
+                     &blend};
     for (DevBuf* b : all) b->release();
     for (auto& b : fld) b.release();
     have_state = false;
@@ -752,8 +754,11 @@ int smooth_image_on(hipStream_t s, const HostGrid& g, const float* in, float* ou
 #endif
 constexpr int64_t kSmoothFusedMinVoxels = ANTS_SMOOTH_FUSED_MIN_VOXELS;
 
+// `extra` (np fields; needed only for 3-D fields with 0 < variance < 0.5 off
+// the fused path) keeps the middle pass out of `in`, which the blend reads.
 int smooth_fields_on(hipStream_t s, const HostGrid& g, int np, float* const* in, float* const* work,
-                     float* const* out, double variance, unsigned int* maxbits, Profiler* prof = nullptr) {
+                     float* const* out, double variance, unsigned int* maxbits, Profiler* prof = nullptr,
+                     float* const* extra = nullptr) {
   const GridDev gd = to_dev(g);
   const int D = g.D;
   const double pass_bytes = 2.0 * D * 4.0 * static_cast<double>(g.count());
@@ -791,14 +796,20 @@ int smooth_fields_on(hipStream_t s, const HostGrid& g, int np, float* const* in,
       }
     }
   }
-  // axes 0..D-2 ping-pong between work and in; the last axis lands in out
+  // axes 0..D-2 ping-pong between work and in (or, when the blend needs the
+  // input of a 3-D field, between work and extra); the last axis lands in out
+  const float w1 = variance < 0.5 ? static_cast<float>(1.0 - variance / 0.5) : 1.f;
+  const float w2 = 1.f - w1;
+  const bool keep_in = w2 != 0.f && D % 2 == 1;
+  if (keep_in && !extra)
+    return fail(ANTS_ERR_UNSUPPORTED, "update/total variance below 0.5 needs a blend buffer here");
   const float* cur[4];
   for (int k = 0; k < np; ++k) cur[k] = in[k];
   for (int d = 0; d < D - 1; ++d) {
     Taps t;
     ANTS_TRY(gaussian_taps(variance, 0.001, g.n[d], &t));
     for (int k = 0; k < np; ++k) {
-      float* dst = (d % 2 == 0) ? work[k] : in[k];
+      float* dst = (d % 2 == 0) ? work[k] : (keep_in ? extra[k] : in[k]);
       if (prof) prof->begin(ANTS_KC_SMOOTH);
       launch_gauss_axis(gd, D, cur[k], dst, t, d, s);
       if (prof) prof->end(ANTS_KC_SMOOTH, pass_bytes);
@@ -807,14 +818,8 @@ int smooth_fields_on(hipStream_t s, const HostGrid& g, int np, float* const* in,
   }
   Taps t;
   ANTS_TRY(gaussian_taps(variance, 0.001, g.n[D - 1], &t));
-  const float w1 = variance < 0.5 ? static_cast<float>(1.0 - variance / 0.5) : 1.f;
-  const float w2 = 1.f - w1;
   const float* orig[4];
   for (int k = 0; k < np; ++k) orig[k] = in[k];
-  if (w2 != 0.f && D % 2 == 1) {
-    // the ping-pong overwrote `in`; blending needs the original (variance < 0.5, rare): not supported in-place
-    return fail(ANTS_ERR_UNSUPPORTED, "update/total variance below 0.5 is not supported in this build");
-  }
   if (prof) prof->begin(ANTS_KC_SMOOTH);
   launch_gauss_field_final(gd, np, cur, orig, out, t, D - 1, w1, w2, true, maxbits, s);
   if (prof) prof->end(ANTS_KC_SMOOTH, np * pass_bytes);
@@ -854,6 +859,11 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
   ANTS_TRY(Fw.ensure(maxN * sizeof(float)));
   ANTS_TRY(Mw.ensure(maxN * sizeof(float)));
   for (auto& b : fld) ANTS_TRY(b.ensure(fbytes));
+  auto small_var = [](double v) { return v > 0.0 && v < 0.5; };
+  const bool need_blend = D == 3 && (small_var(P.update_variance) || small_var(P.total_variance));
+  if (need_blend) ANTS_TRY(blend.ensure(2 * fbytes));
+  float* const blend_f[2] = {need_blend ? blend.f() : nullptr,
+                             need_blend ? blend.f() + static_cast<size_t>(maxN) * D : nullptr};
   ANTS_TRY(stats.ensure(2 * sizeof(IterStats)));  // two iteration parities
   int64_t ccp = 0, invp = 0, invw = 0;
   for (int l = 0; l < P.n_levels; ++l) {
@@ -1126,7 +1136,8 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
       float* in[2] = {wk_[0], wk_[1]};
       float* work[2] = {wk_[2], wk_[3]};
       float* out[2] = {D % 2 == 1 ? wk_[2] : wk_[0], D % 2 == 1 ? wk_[3] : wk_[1]};
-      ANTS_TRY(smooth_fields_on(fs, g, 2, in, work, out, P.update_variance, stats_of(par)->maxnorm, &prof));
+      ANTS_TRY(smooth_fields_on(fs, g, 2, in, work, out, P.update_variance, stats_of(par)->maxnorm, &prof,
+                                need_blend ? blend_f : nullptr));
       if (out[0] != wk_[0]) {
         std::swap(wk_[0], wk_[2]);
         std::swap(wk_[1], wk_[3]);
@@ -1157,7 +1168,8 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
         float* in[2] = {wk_[2], wk_[3]};
         float* work[2] = {wk_[0], wk_[1]};
         float* out[2] = {D % 2 == 1 ? wk_[0] : wk_[2], D % 2 == 1 ? wk_[1] : wk_[3]};
-        ANTS_TRY(smooth_fields_on(s, g, 2, in, work, out, P.total_variance, stats_of(par)->maxnorm + 2, &prof));
+        ANTS_TRY(smooth_fields_on(s, g, 2, in, work, out, P.total_variance, stats_of(par)->maxnorm + 2, &prof,
+                                  need_blend ? blend_f : nullptr));
         if (out[0] != wk_[2]) {
           std::swap(wk_[0], wk_[2]);
           std::swap(wk_[1], wk_[3]);
@@ -1719,8 +1731,12 @@ int ants_op_smooth_field(const ants_image_desc* gdsc, const float* d_in, float* 
   float* in[1] = {a.f()};
   float* work[1] = {w.f()};
   float* out[1] = {d_out};
+  DevBuf x;
+  if (g.D == 3 && variance > 0.0 && variance < 0.5) ANTS_TRY(x.ensure(b));
+  float* extra[1] = {x.f()};
   HIP_TRY(hipMemsetAsync(m.p, 0, 16, s));
-  int rc = smooth_fields_on(s, g, 1, in, work, out, variance, static_cast<unsigned int*>(m.p));
+  int rc = smooth_fields_on(s, g, 1, in, work, out, variance, static_cast<unsigned int*>(m.p), nullptr,
+                            x.p ? extra : nullptr);
   if (rc == ANTS_OK) HIP_TRY(hipStreamSynchronize(s));
   return rc;
 }
@@ -1787,6 +1803,33 @@ int ants_op_cc(const ants_image_desc* gdsc, const float* d_fixed, const float* d
   HIP_TRY(hipMemcpyAsync(&r, sc.stats.p, sizeof(r), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   if (host_value) *host_value = -r.sum / static_cast<double>(g.count());
+  return ANTS_OK;
+}
+
+int ants_op_inverse_consistency(const ants_image_desc* gdsc, const float* d_fwd, const float* d_inv, int margin,
+                                double* host_max, double* host_mean, int64_t* host_count, void* stream) {
+  HostGrid g;
+  ANTS_TRY(to_host_grid(gdsc, &g));
+  if (!d_fwd || !d_inv) return fail(ANTS_ERR_INVALID, "null field buffer");
+  if (margin < 0) return fail(ANTS_ERR_INVALID, "margin must be >= 0");
+  int64_t count = 1;
+  for (int d = 0; d < g.D; ++d) count *= std::max<int64_t>(0, g.n[d] - 2 * static_cast<int64_t>(margin));
+  const GridDev gd = to_dev(g);
+  OpScratch sc;
+  ANTS_TRY(sc.stats.ensure(sizeof(RedSlot)));
+  ANTS_TRY(sc.partials.ensure(ic_partials_needed(gd) * sizeof(Partial)));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  launch_ic_residual(gd, d_fwd, d_inv, margin, static_cast<Partial*>(sc.partials.p), static_cast<RedSlot*>(sc.stats.p),
+                     s);
+  HIP_TRY(hipGetLastError());
+  RedSlot r;
+  HIP_TRY(hipMemcpyAsync(&r, sc.stats.p, sizeof(r), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  float mx;
+  std::memcpy(&mx, &r.maxbits, 4);
+  if (host_max) *host_max = mx;
+  if (host_mean) *host_mean = count > 0 ? r.sum / static_cast<double>(count) : 0.0;
+  if (host_count) *host_count = count;
   return ANTS_OK;
 }
 
@@ -2191,6 +2234,9 @@ int64_t footprint_bytes(const ants_syn_params& P, const HostGrid& gF, const Host
   if (P.precision == ANTS_PRECISION_BF16_PYRAMID) b += (NF + NM) * 2;
   b += 2 * maxN * 4;                     // warped images
   b += 8 * maxN * D * 4;                 // middle and work fields
+  auto small_var = [](double v) { return v > 0.0 && v < 0.5; };
+  if (D == 3 && (small_var(P.update_variance) || small_var(P.total_variance)))
+    b += 2 * maxN * D * 4;               // the blend buffers of a small 3-D variance
   b += 4 * ((maxN + 63) / 64) * 8;       // inversion candidate masks
   b += (NF + NM) * 4 + NM * 4;           // request inputs (images, labels)
   b += 2 * maxN * D * 4 + maxN * 4;      // outputs (fields, labels)

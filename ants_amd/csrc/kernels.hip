@@ -1390,6 +1390,43 @@ __global__ void __launch_bounds__(kBlock) k_compose(GridDev g, PtrPack4 disp, Pt
   for (int dd = 0; dd < D; ++dd) o[i * D + dd] = wv[dd] + scale * v[dd];
 }
 
+// Inverse-consistency residual r(x) = fwd(x) + inv(x + fwd(x)) (inv linear,
+// 0 outside its buffer) over the voxels at least `margin` voxels from every
+// face: per-block partials of sum |r / spacing| in double and of its max.
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_ic_residual(GridDev g, const float* __restrict__ fwd,
+                                                        const float* __restrict__ inv, int margin,
+                                                        Partial* partials) {
+  const int64_t N = g.count();
+  double sum = 0.0;
+  float mx = 0.f;
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < N;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    int x, y, z;
+    unlin(g, i, x, y, z);
+    const int pos[3] = {x, y, z};
+    bool keep = true;
+#pragma unroll
+    for (int d = 0; d < D; ++d) keep = keep && pos[d] >= margin && pos[d] < g.n[d] - margin;
+    if (!keep) continue;
+    float wv[3] = {0.f, 0.f, 0.f}, c[3], v[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int d = 0; d < D; ++d) wv[d] = fwd[i * D + d];
+    cidx_of<D>(g, x, y, z, wv, g, c);
+    if (inside<D>(g, c)) interp<D, D>(g, inv, c, v);
+    double q = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const double r = (static_cast<double>(wv[d]) + static_cast<double>(v[d])) / g.dsp[d];
+      q += r * r;
+    }
+    q = sqrt(q);
+    sum += q;
+    mx = fmaxf(mx, static_cast<float>(q));
+  }
+  block_partial<kBlock>(sum, mx, partials + blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // InvertDisplacementFieldImageFilter.  Norm pass j measures
 // e_j = inv_j + field(x + inv_j) and its norms n_j; update j is
@@ -2371,6 +2408,18 @@ void launch_compose(const GridDev& g, int np, const float* const* disp, const fl
   else
     hipLaunchKernelGGL(k_compose<3>, grid, dim3(kBlock), 0, s, g, pack(disp, np), pack(warp, np), mpack(out, np),
                        maxbits, rate);
+}
+
+int64_t ic_partials_needed(const GridDev& g) { return red_blocks(g.count()); }
+
+void launch_ic_residual(const GridDev& g, const float* fwd, const float* inv, int margin, Partial* partials,
+                        RedSlot* slot, hipStream_t s) {
+  const dim3 grid(red_blocks(g.count()));
+  if (g.D == 2)
+    hipLaunchKernelGGL(k_ic_residual<2>, grid, dim3(kBlock), 0, s, g, fwd, inv, margin, partials);
+  else
+    hipLaunchKernelGGL(k_ic_residual<3>, grid, dim3(kBlock), 0, s, g, fwd, inv, margin, partials);
+  hipLaunchKernelGGL(k_reduce_slot<kBlock>, dim3(1), dim3(kBlock), 0, s, partials, static_cast<int>(grid.x), slot);
 }
 
 // Measured per 192x224x192 pair: inversion 24.5 ms with one voxel per thread

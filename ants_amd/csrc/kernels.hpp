@@ -177,6 +177,13 @@ void launch_publish_stats(const Partial* cc_parts, int cc_n, RedSlot* energy, co
 void launch_compose(const GridDev& g, int np, const float* const* disp, const float* const* warp,
                     float* const* out, const unsigned int* maxbits, float rate, hipStream_t s);
 
+// Inverse-consistency residual |fwd(x) + inv(x + fwd(x))| / spacing over the
+// voxels at least `margin` from every face: slot->sum (double) and
+// slot->maxbits (float max).  `partials` holds ic_partials_needed(g) entries.
+int64_t ic_partials_needed(const GridDev& g);
+void launch_ic_residual(const GridDev& g, const float* fwd, const float* inv, int margin, Partial* partials,
+                        RedSlot* slot, hipStream_t s);
+
 // Gate of an inversion launch: with step >= 0 the launch does nothing when
 // any of slots[0..n) computed a norm at that step, i.e. when the inversions
 // those slots track would still have run a further step.

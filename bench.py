@@ -23,6 +23,7 @@ latency, pairs/s and capacity), summarised in `batch_summary` and
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -432,6 +433,14 @@ def main():
             f"graph/eager iterations {rs['graph_launches']}/{rs['eager_iterations']}")
         return {"ms_per_step": ms, "value": rs["voxel_iterations"] / (ms * 1e-3)}
 
+    # inverse consistency of the returned pair, FP64 sums on the device, away
+    # from the faces (16 voxels), in voxel units
+    mx, mn, cnt = C.c_double(), C.c_double(), C.c_int64()
+    L.check(L.lib().ants_op_inverse_consistency(C.byref(gf), d_fwd.ptr, d_inv.ptr, 16, C.byref(mx), C.byref(mn),
+                                                C.byref(cnt), None))
+    ic = {"margin_voxels": 16, "max_voxels": mx.value, "mean_voxels": mn.value, "voxels": cnt.value}
+    log(f"inverse consistency |fwd + inv(x + fwd)|: max {mx.value:.3f}, mean {mn.value:.4f} voxel")
+
     plain_replay = untimed_replay(False)
     graph_replay = untimed_replay(True)
 
@@ -503,6 +512,7 @@ def main():
         "kernel_breakdown_ms": {k: round(v["ms"], 3) for k, v in ks.items()},
         "per_level_kernels": per_level,
         "host_buffer_drop_in": host_drop_in,
+        "inverse_consistency": ic,
     }
     if legs:
         sv = legs["serve"]["continuous_batching"]

@@ -274,6 +274,16 @@ int ants_op_cc(const ants_image_desc* g, const float* d_fixed, const float* d_mo
  * (itkantsRegistrationHelper.hxx:3698-3711). */
 int ants_op_compose(const ants_image_desc* g, const float* d_disp, const float* d_warp, float* d_out, void* stream);
 
+/* Inverse-consistency residual of a forward / inverse field pair (the
+ * DisplacementFieldTransform's displacement and inverse displacement
+ * fields, itkantsRegistrationHelper.hxx:2206-2215): r(x) = fwd(x) +
+ * inv(x + fwd(x)) with inv linearly interpolated and 0 outside its buffer,
+ * |r| in voxel units over the voxels at least `margin` voxels from every
+ * face; the sum is accumulated in FP64 on the device.  host_max, host_mean
+ * (sum / count) and host_count may be NULL. */
+int ants_op_inverse_consistency(const ants_image_desc* g, const float* d_fwd, const float* d_inv, int margin,
+                                double* host_max, double* host_mean, int64_t* host_count, void* stream);
+
 /* SyN ScaleUpdateField: u *= rate / max|u/spacing|; host_max receives the max. */
 int ants_op_scale_update(const ants_image_desc* g, float* d_u, double rate, double* host_max, void* stream);
 

@@ -117,3 +117,12 @@ def round_bf16(a):
     b = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
     r = ((b + 0x7FFF + ((b >> 16) & 1)) >> 16) << 16
     return r.astype(np.uint32).view(np.float32)
+
+
+def inverse_consistency(fwd, inv, margin, spacing=None):
+    g = desc_of(fwd.shape[:-1], spacing)
+    a, b = dev(fwd), dev(inv)
+    mx, mn, n = C.c_double(), C.c_double(), C.c_int64()
+    L.check(L.lib().ants_op_inverse_consistency(C.byref(g), a.ptr, b.ptr, margin, C.byref(mx), C.byref(mn),
+                                                C.byref(n), None))
+    return mx.value, mn.value, n.value

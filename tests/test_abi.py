@@ -116,6 +116,10 @@ def test_footprint_closed_form_and_errors():
     assert fp(C.byref(p), C.byref(d), C.byref(d)) == want
     pb = S.stage_params(stage, 3, precision="bf16")
     assert fp(C.byref(pb), C.byref(d), C.byref(d)) == want + 2 * N * 2
+    # a 3-D update variance below 0.5 adds the two blend fields
+    small = R.SyNStage.from_args("SyN[0.1,0.25,0]", "CC[f,m,1,4]", "[100x70x20,1e-6,10]", "4x2x1", "2x1x0vox")
+    ps = S.stage_params(small, 3)
+    assert fp(C.byref(ps), C.byref(d), C.byref(d)) == want + 2 * N * 3 * 4
     p.dim = 5
     assert fp(C.byref(p), C.byref(d), C.byref(d)) == -1
 

@@ -43,12 +43,13 @@ def test_smooth_image(shape, sigma, in_mm, sp):
 
 
 @pytest.mark.parametrize("shape,var", [((30, 41), 3.0), ((13, 17, 19), 3.0), ((9, 8, 70), 3.0), ((12, 14), 0.25),
-                                       ((10, 11, 12), 0.0)])
+                                       ((10, 11, 12), 0.0), ((13, 17, 19), 0.25), ((20, 9, 14), 0.4),
+                                       ((11, 12, 10), 1.0)])
 def test_smooth_field(shape, var):
+    """Including 3-D fields with 0 < variance < 0.5 (the separable passes
+    keep the input for the blend w1 * smooth + (1 - w1) * input)."""
     D = len(shape)
     f = f32(rng.standard_normal(shape + (D,)))
-    if var < 0.5 and var > 0 and D == 3:
-        pytest.skip("variance below 0.5 is rejected for 3-D fields in this build")
     out = G.smooth_field(f, var)
     ref = o.smooth_field(grid_np(shape), f, var)
     np.testing.assert_allclose(out, ref, atol=2e-6 * np.abs(f).max())
@@ -202,6 +203,28 @@ def test_warp_pair(bf16, fshape, mshape, shrink):
     np.testing.assert_allclose(oF, rF, atol=1e-5)
     np.testing.assert_allclose(oM, rM, atol=1e-5)
     assert (rF == 0).any() or (rM == 0).any()  # some points fall outside a buffer
+
+
+@pytest.mark.parametrize("shape,margin,sp", [((20, 24, 28), 3, [1.0, 1.0, 1.0]), ((30, 26), 0, [0.8, 1.3]),
+                                             ((17, 19, 23), 5, [1.2, 0.9, 1.1])])
+def test_inverse_consistency_residual(shape, margin, sp):
+    """|fwd(x) + inv(x + fwd(x))| in voxels (FP64 sums on the device) against
+    the oracle's composition and numpy norms over the voxels `margin` from
+    the faces; an exact inverse pair (constant shift) gives 0 inside."""
+    D = len(shape)
+    fwd = smooth_random_field(shape, D, 3.0, 2.0)
+    inv = f32(-fwd + smooth_random_field(shape, D, 3.0, 0.2))
+    mx, mn, n = G.inverse_consistency(fwd, inv, margin, sp)
+    r = o.compose(grid_np(shape, sp), inv, fwd)
+    q = np.sqrt(((r / np.asarray(sp)) ** 2).sum(-1))
+    core = q[tuple(slice(margin, s - margin) for s in shape)]
+    assert n == core.size
+    assert mx == pytest.approx(core.max(), rel=1e-5)
+    assert mn == pytest.approx(core.mean(), rel=1e-5)
+    shift = np.zeros(shape + (D,), np.float32)
+    shift[...] = 0.5
+    mx0, mn0, _ = G.inverse_consistency(shift, -shift, 2, None)
+    assert mx0 == 0.0 and mn0 == 0.0
 
 
 def test_invert_zero_init_constant_shift():

@@ -579,3 +579,13 @@ def test_pipelined_iterations_are_exact(monkeypatch, case):
         assert sb["inv_tails"] == sum(sb["iterations"])
     if case == "3d_converging":
         assert sa["iterations"][0] < 60  # the monitor stopped a level early
+
+
+def test_syn_3d_small_variances_vs_oracle():
+    """3-D stage with update and total variances below 0.5 (the smoothing
+    blends the smoothed field with its input; off the fused 13-tap path)."""
+    pair = ph.phantom_pair_3d(12, size=(40, 44, 36), max_disp_mm=3.0, svf_sigma_mm=6.0)
+    stage = SyNStage.from_args("SyN[0.2,0.3,0.4]", "CC[f,m,1,2]", "[10x6x3,1e-6,10]", "4x2x1", "2x1x0vox")
+    res = run_pair(stage, pair, force=True)
+    ref = run_oracle(stage, pair, force=True)
+    print("3d small-variance parity:", compare(res, ref, pair.spacing))
