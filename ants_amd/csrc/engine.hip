@@ -496,7 +496,17 @@ struct InitialXforms {
   bool on[2] = {false, false};
   double A[2][9] = {};
   double o[2][3] = {};
+  // a list holding a displacement field is kept as it is (the warp then
+  // applies it transform by transform); n[k] entries in list[k]
+  bool general[2] = {false, false};
+  int n[2] = {0, 0};
+  ants_transform list[2][ANTS_MAX_TRANSFORMS - 1] = {};
 };
+
+namespace {
+int geom_from_desc(const ants_image_desc* d, GeomDev* g);
+int xform_to_dev(const ants_transform& T, int D, XformDev* X);
+}  // namespace
 
 // Composes each list (the point passes through list[0] first) with
 // MatrixOffsetTransformBase's y = B (x - c) + t + c, in double.
@@ -513,8 +523,13 @@ int compose_initial(int D, const ants_transform* fixed, int n_fixed, const ants_
     for (int t = 0; t < n; ++t) {
       const ants_transform& T = lists[k][t];
       if (T.kind == ANTS_TRANSFORM_IDENTITY) continue;
-      if (T.kind == ANTS_TRANSFORM_DISPLACEMENT)
-        return fail(ANTS_ERR_UNSUPPORTED, "displacement-field initial transforms are not supported by the SyN stage");
+      if (T.kind == ANTS_TRANSFORM_DISPLACEMENT) {
+        // checked here (geometry, buffer); the list is kept whole below
+        XformDev X;
+        ANTS_TRY(xform_to_dev(T, D, &X));
+        x.general[k] = true;
+        continue;
+      }
       if (T.kind != ANTS_TRANSFORM_AFFINE) return fail(ANTS_ERR_INVALID, "unknown transform kind %d", T.kind);
       double B[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, ob[3] = {0, 0, 0};
       for (int r = 0; r < D; ++r) {
@@ -543,6 +558,15 @@ int compose_initial(int D, const ants_transform* fixed, int n_fixed, const ants_
     }
     std::memcpy(x.A[k], Ac, sizeof(Ac));
     std::memcpy(x.o[k], oc, sizeof(oc));
+    if (x.general[k]) {
+      // the displaced point goes through the list after the SyN field: one
+      // slot of the warp's transform list is taken by that field
+      if (n > ANTS_MAX_TRANSFORMS - 1)
+        return fail(ANTS_ERR_UNSUPPORTED, "at most %d initial transforms with a displacement field",
+                    ANTS_MAX_TRANSFORMS - 1);
+      x.n[k] = n;
+      for (int t = 0; t < n; ++t) x.list[k][t] = lists[k][t];
+    }
   }
   *out = x;
   return ANTS_OK;
@@ -582,6 +606,9 @@ struct ants_syn_handle {
   bool init_on[2] = {false, false};
   double init_A[2][9] = {};
   double init_o[2][3] = {};
+  bool init_general[2] = {false, false};
+  int init_n[2] = {0, 0};
+  ants_transform init_list[2][ANTS_MAX_TRANSFORMS - 1] = {};
 
   // restored state (F2M fwd, F2M inv, M2M fwd, M2M inv on restore_grid),
   // resampled onto the level-0 grid at the start of a run
@@ -594,6 +621,9 @@ struct ants_syn_handle {
       init_on[k] = x.on[k];
       std::memcpy(init_A[k], x.A[k], sizeof(init_A[k]));
       std::memcpy(init_o[k], x.o[k], sizeof(init_o[k]));
+      init_general[k] = x.general[k];
+      init_n[k] = x.n[k];
+      for (int t = 0; t < x.n[k]; ++t) init_list[k][t] = x.list[k][t];
     }
   }
 
@@ -916,6 +946,20 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
         setup_hash = fnv1a(init_A[k], sizeof(init_A[k]), setup_hash);
         setup_hash = fnv1a(init_o[k], sizeof(init_o[k]), setup_hash);
       }
+      const int gen = init_general[k] ? init_n[k] : -1;
+      setup_hash = fnv1a(&gen, sizeof(gen), setup_hash);
+      for (int t = 0; t < (init_general[k] ? init_n[k] : 0); ++t) {
+        const ants_transform& T = init_list[k][t];
+        setup_hash = fnv1a(&T.kind, sizeof(T.kind), setup_hash);
+        setup_hash = fnv1a(T.matrix, sizeof(T.matrix), setup_hash);
+        setup_hash = fnv1a(T.translation, sizeof(T.translation), setup_hash);
+        setup_hash = fnv1a(T.center, sizeof(T.center), setup_hash);
+        setup_hash = fnv1a(&T.d_field, sizeof(T.d_field), setup_hash);
+        setup_hash = fnv1a(T.field.size, sizeof(T.field.size), setup_hash);
+        setup_hash = fnv1a(T.field.spacing, sizeof(T.field.spacing), setup_hash);
+        setup_hash = fnv1a(T.field.origin, sizeof(T.field.origin), setup_hash);
+        setup_hash = fnv1a(T.field.direction, sizeof(T.field.direction), setup_hash);
+      }
     }
     // Iterations launch kernel by kernel unless ANTS_SYN_GRAPHS=1: replaying
     // captured iteration graphs measured slower on this runtime (41.7 against
@@ -981,12 +1025,48 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
   const GridDev gFd = to_dev(gF), gMd = to_dev(gM);
   const float eps = std::numeric_limits<float>::epsilon();
   HostGrid prev{};
-  const bool use_maps = init_on[0] || init_on[1];
+  const bool general = init_general[0] || init_general[1];
+  const bool use_maps = !general && (init_on[0] || init_on[1]);
   for (int l = 0; l < P.n_levels; ++l) {
     const HostGrid& g = lg[l];
     const GridDev gd = to_dev(g);
     const int64_t N = g.count();
     WarpMap maps[2];
+    // with a displacement field among the initial transforms each image is
+    // warped by the antsApplyTransforms kernel through [the SyN inverse
+    // middle field on the level grid, then the side's initial transforms]
+    ApplyDev gw[2];
+    GridDev gw_out{};
+    if (general) {
+      ants_image_desc ld, id[2];
+      to_desc(g, &ld);
+      to_desc(gF, &id[0]);
+      to_desc(gM, &id[1]);
+      for (int k = 0; k < 2; ++k) {
+        ApplyDev& A = gw[k];
+        std::memset(&A, 0, sizeof(A));
+        ANTS_TRY(geom_from_desc(&ld, &A.ref));
+        ANTS_TRY(geom_from_desc(&id[k], &A.in));
+        A.defv = 0.f;
+        A.t[0].kind = ANTS_TRANSFORM_DISPLACEMENT;
+        A.t[0].fg = A.ref;  // the field lives on the level grid; its buffer is set per iteration
+        int nt = 1;
+        if (init_general[k]) {
+          for (int t = 0; t < init_n[k]; ++t) ANTS_TRY(xform_to_dev(init_list[k][t], D, &A.t[nt++]));
+        } else if (init_on[k]) {
+          XformDev& X = A.t[nt++];
+          X.kind = ANTS_TRANSFORM_AFFINE;
+          for (int r = 0; r < 3; ++r) {
+            X.off[r] = init_o[k][r];
+            for (int c = 0; c < 3; ++c) X.a[r * 3 + c] = init_A[k][r * 3 + c];
+          }
+        }
+        A.nt = nt;
+      }
+      gw_out.D = D;
+      for (int k = 0; k < 3; ++k) gw_out.n[k] = static_cast<int>(g.n[k]);
+      set_divisors(gw_out);
+    }
     if (use_maps) {
       static const double I[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, Z[3] = {0, 0, 0};
       maps[0] = make_warp_map(g, init_on[0] ? init_A[0] : I, init_on[0] ? init_o[0] : Z, gF);
@@ -1119,7 +1199,13 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
     auto front = [&](int par, float* const* cst, float* wk_[4], hipStream_t fs, int& n_cc) -> int {
       uint32_t* stat_words = reinterpret_cast<uint32_t*>(stats_of(par));
       prof.begin(ANTS_KC_WARP);
-      if (bf16)
+      if (general) {
+        launch_fill_u32(stat_words, 0u, kStatWords, fs);
+        gw[0].t[0].f = cst[1];
+        gw[1].t[0].f = cst[3];
+        launch_apply_transforms(gw_out, gw[0], ANTS_INTERP_LINEAR, Fs.f(), Fw.f(), fs);
+        launch_apply_transforms(gw_out, gw[1], ANTS_INTERP_LINEAR, Ms.f(), Mw.f(), fs);
+      } else if (bf16)
         launch_warp_pair_bf16(gd, gFd, static_cast<const uint16_t*>(Fb.p), cst[1], Fw.f(), gMd,
                               static_cast<const uint16_t*>(Mb.p), cst[3], Mw.f(), fs, stat_words, kStatWords,
                               use_maps ? maps : nullptr);
@@ -2129,6 +2215,34 @@ namespace {
 
 // Geometry with direction cosines: m = direction * diag(spacing) and its
 // inverse diag(1/spacing) * direction^T (orthonormal directions only).
+// One transform of a composite as the apply kernel takes it.
+int xform_to_dev(const ants_transform& T, int D, XformDev* Xp) {
+  XformDev& X = *Xp;
+  std::memset(&X, 0, sizeof(X));
+  X.kind = T.kind;
+  if (T.kind == ANTS_TRANSFORM_IDENTITY) return ANTS_OK;
+  if (T.kind == ANTS_TRANSFORM_AFFINE) {
+    // MatrixOffsetTransformBase: offset = t + c - A c
+    for (int r = 0; r < D; ++r) {
+      double ac = 0.0;
+      for (int k = 0; k < D; ++k) {
+        X.a[r * 3 + k] = T.matrix[r * D + k];
+        ac = ac + T.matrix[r * D + k] * T.center[k];
+      }
+      X.off[r] = T.translation[r] + T.center[r] - ac;
+    }
+    return ANTS_OK;
+  }
+  if (T.kind == ANTS_TRANSFORM_DISPLACEMENT) {
+    ANTS_TRY(geom_from_desc(&T.field, &X.fg));
+    if (X.fg.D != D) return fail(ANTS_ERR_INVALID, "displacement field dimension differs");
+    if (!T.d_field) return fail(ANTS_ERR_INVALID, "null displacement field");
+    X.f = T.d_field;
+    return ANTS_OK;
+  }
+  return fail(ANTS_ERR_INVALID, "unknown transform kind %d", T.kind);
+}
+
 int geom_from_desc(const ants_image_desc* d, GeomDev* g) {
   if (!d) return fail(ANTS_ERR_INVALID, "null image descriptor");
   if (d->dim != 2 && d->dim != 3) return fail(ANTS_ERR_INVALID, "dim must be 2 or 3 (got %d)", d->dim);
@@ -2181,30 +2295,7 @@ extern "C" int ants_apply_transforms(const ants_image_desc* ref, const ants_imag
     return fail(ANTS_ERR_INVALID, "unknown interpolation %d", interpolation);
   P.nt = n_transforms;
   P.defv = static_cast<float>(default_value);
-  for (int t = 0; t < n_transforms; ++t) {
-    const ants_transform& T = transforms[t];
-    XformDev& X = P.t[t];
-    X.kind = T.kind;
-    if (T.kind == ANTS_TRANSFORM_IDENTITY) continue;
-    if (T.kind == ANTS_TRANSFORM_AFFINE) {
-      // MatrixOffsetTransformBase: offset = t + c - A c
-      for (int r = 0; r < D; ++r) {
-        double ac = 0.0;
-        for (int k = 0; k < D; ++k) {
-          X.a[r * 3 + k] = T.matrix[r * D + k];
-          ac = ac + T.matrix[r * D + k] * T.center[k];
-        }
-        X.off[r] = T.translation[r] + T.center[r] - ac;
-      }
-    } else if (T.kind == ANTS_TRANSFORM_DISPLACEMENT) {
-      ANTS_TRY(geom_from_desc(&T.field, &X.fg));
-      if (X.fg.D != D) return fail(ANTS_ERR_INVALID, "displacement field dimension differs");
-      if (!T.d_field) return fail(ANTS_ERR_INVALID, "null displacement field");
-      X.f = T.d_field;
-    } else {
-      return fail(ANTS_ERR_INVALID, "unknown transform kind %d", T.kind);
-    }
-  }
+  for (int t = 0; t < n_transforms; ++t) ANTS_TRY(xform_to_dev(transforms[t], D, &P.t[t]));
   GridDev go{};
   go.D = D;
   for (int k = 0; k < 3; ++k) go.n[k] = P.ref.n[k];

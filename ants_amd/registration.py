@@ -215,13 +215,16 @@ class SyNRegistration:
     def set_initial_transforms(self, moving=(), fixed=()):
         """SetMovingInitialTransform / SetFixedInitialTransform
         (Examples/itkantsRegistrationHelper.hxx:2119-2137): lists of
-        ``apply_transforms.AffineTransform`` (or None for identity) in the
+        ``apply_transforms.AffineTransform`` / ``DisplacementFieldTransform``
+        (or None for identity) in the
         order a point passes through them, as ``-t`` options are listed.
         Kept for every later run of this object; empty lists clear them."""
         from .apply_transforms import transform_array
         mx, keep_m = transform_array(list(moving), self.dim)
         fx, keep_f = transform_array(list(fixed), self.dim)
         L.check(L.lib().ants_syn_set_initial_transforms(self.handle, mx, len(moving), fx, len(fixed)))
+        # the handle refers to the displacement fields' device buffers for its later runs
+        self._initial_keep = keep_m + keep_f
 
     def set_initial_state(self, state=None, size=None, spacing=None, origin=None, direction=None):
         """Start later runs from a saved state (SetFixedToMiddleTransform /

@@ -208,9 +208,9 @@ class Batch:
             r.host_out_labels = lab.ctypes.data
         if req.moving_initial or req.fixed_initial:
             from .apply_transforms import transform_array
-            mx, _ = transform_array(list(req.moving_initial), dim)
-            fx, _ = transform_array(list(req.fixed_initial), dim)
-            keep += [mx, fx]
+            mx, keep_m = transform_array(list(req.moving_initial), dim)
+            fx, keep_f = transform_array(list(req.fixed_initial), dim)
+            keep += [mx, fx] + keep_m + keep_f  # field buffers stay alive until the result returns
             r.moving_initial, r.n_moving_initial = mx, len(req.moving_initial)
             r.fixed_initial, r.n_fixed_initial = fx, len(req.fixed_initial)
         rid = C.c_int64()

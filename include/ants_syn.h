@@ -362,12 +362,16 @@ int ants_apply_transforms(const ants_image_desc* ref, const ants_image_desc* in,
  * 3583-3633): the composite of -r / -q initial transforms and the preceding
  * linear stages.  Each array lists the transforms in the order a point passes
  * through them (the -t order of ants_apply_transforms).  Every iteration then
- * samples the moving image at moving(x + u_m2m_inv(x)) and the fixed image at
- * fixed(x + u_f2m_inv(x)); the output fields stay the SyN displacement fields
- * alone, as the composite keeps the initial transforms separately.  Identity
- * and affine transforms only (a displacement field returns
- * ANTS_ERR_UNSUPPORTED).  Held by the handle for later ants_syn_register*
- * calls; n_moving = n_fixed = 0 clears them. */
+ * samples the moving image at moving(T_m(x + u_m2m_inv(x))) and the fixed image
+ * at fixed(T_f(x + u_f2m_inv(x))), T_m and T_f the two lists applied in order;
+ * the output fields stay the SyN displacement fields
+ * alone, as the composite keeps the initial transforms separately.  Identity,
+ * affine and displacement-field transforms (an earlier deformable stage or a
+ * restored composite, hxx:2119-2137); a list with a field holds at most 7
+ * transforms, and its field buffers must stay valid while the handle uses
+ * them.  Affine-only lists are folded into one map per side; a list with a
+ * field is applied transform by transform.  Held by the handle for later
+ * ants_syn_register* calls; n_moving = n_fixed = 0 clears them. */
 int ants_syn_set_initial_transforms(ants_syn_handle* h, const ants_transform* moving, int n_moving,
                                     const ants_transform* fixed, int n_fixed);
 
@@ -397,7 +401,8 @@ typedef struct ants_syn_request {
   int32_t* host_out_labels;            /* optional, last level grid */
   void* user;
   /* optional initial transforms, as ants_syn_set_initial_transforms takes
-   * them; read by ants_batch_submit */
+   * them; the arrays are read by ants_batch_submit, displacement-field
+   * buffers must stay valid until the request's result is returned */
   const ants_transform* moving_initial;
   int n_moving_initial;
   const ants_transform* fixed_initial;

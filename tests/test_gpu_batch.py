@@ -114,9 +114,8 @@ def test_serving_loop_reports_latency():
 
 def test_batch_requests_with_initial_transforms():
     """Requests carrying initial transforms (mixed with plain ones on the same
-    slots) equal the same registrations run alone with
-    set_initial_transforms, bit for bit; a displacement-field initial
-    transform is refused at submit."""
+    slots), affine ones and a displacement field, equal the same
+    registrations run alone with set_initial_transforms, bit for bit."""
     from ants_amd import _lib as L
     from ants_amd.apply_transforms import AffineTransform, DisplacementFieldTransform, Geometry
     reqs = make_requests(4, seed=21)
@@ -135,12 +134,21 @@ def test_batch_requests_with_initial_transforms():
         reg.close()
         np.testing.assert_array_equal(r.fwd, ref.fwd)
         np.testing.assert_array_equal(r.inv, ref.inv)
-    geo = Geometry(size=(4, 4, 4), spacing=(1, 1, 1), origin=(0, 0, 0), direction=np.eye(3))
-    bad = make_requests(1, seed=22)[0]
-    bad.moving_initial = [DisplacementFieldTransform(np.zeros((4, 4, 4, 3), np.float32), geo)]
-    with pytest.raises(L.AntsError) as e:
-        b.submit(bad)
-    assert e.value.code == 4
+    # a displacement-field initial transform: the request holds the field
+    # until its result returns, and equals the registration run alone
+    req = make_requests(1, seed=22)[0]
+    fs = tuple(int(v) // 2 + 4 for v in reversed(req.fixed.shape))
+    geo = Geometry(size=fs, spacing=(2.0, 2.0, 2.0), origin=(-2.0, -2.0, -2.0), direction=np.eye(3))
+    rng = np.random.default_rng(3)
+    fld = np.ascontiguousarray(rng.uniform(-0.5, 0.5, tuple(reversed(fs)) + (3,)), np.float32)
+    req.moving_initial = [DisplacementFieldTransform(fld, geo)]
+    r = run_all(b, [req])[0]
+    assert r.status == 0, r.error
+    reg = SyNRegistration(req.stage, 3)
+    reg.set_initial_transforms(list(req.moving_initial))
+    ref = reg.register(req.fixed, req.moving)
+    reg.close()
+    np.testing.assert_array_equal(r.fwd, ref.fwd)
     b.close()
 
 

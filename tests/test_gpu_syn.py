@@ -405,12 +405,59 @@ def test_initial_transform_errors():
     stage = SyNStage.from_args("SyN[0.2,3,0]", "CC[f,m,1,2]", "[2x1,1e-6,10]", "2x1", "1x0vox")
     reg = SyNRegistration(stage, 3)
     geo = Geometry(size=(4, 4, 4), spacing=(1, 1, 1), origin=(0, 0, 0), direction=np.eye(3))
+    fld = DisplacementFieldTransform(np.zeros((4, 4, 4, 3), np.float32), geo)
+    # a list with a displacement field keeps one slot for the SyN field: at most 7
     with pytest.raises(L.AntsError) as e:
-        reg.set_initial_transforms([DisplacementFieldTransform(np.zeros((4, 4, 4, 3), np.float32), geo)])
+        reg.set_initial_transforms([fld] + [AffineTransform(np.eye(3), [0, 0, 0])] * 7)
     assert e.value.code == 4
+    bad = DisplacementFieldTransform(np.zeros((4, 4, 3), np.float32), Geometry(size=(4, 4), spacing=(1, 1)))
+    with pytest.raises((L.AntsError, ValueError)):
+        reg.set_initial_transforms([bad])
     with pytest.raises(ValueError):
         reg.set_initial_transforms([AffineTransform(np.eye(3), [0, 0, 0])] * 9)
     reg.close()
+
+
+def _smooth_field(shape, D, amp, seed):
+    from scipy import ndimage
+    rng = np.random.default_rng(seed)
+    f = np.stack([ndimage.gaussian_filter(rng.standard_normal(shape), 3.0) for _ in range(D)], -1)
+    return np.ascontiguousarray(amp * f / np.abs(f).max(), np.float32)
+
+
+@pytest.mark.parametrize("D", [3, 2])
+def test_displacement_field_initial_transforms_vs_oracle(D):
+    """Initial transforms holding displacement fields (an earlier deformable
+    stage, or a restored composite): moving = [affine, field on its own
+    coarser oblique-free grid], fixed = [field], against the oracle applying
+    the same lists point by point."""
+    from ants_amd.apply_transforms import AffineTransform, DisplacementFieldTransform, Geometry
+    if D == 3:
+        pair = ph.phantom_pair_3d(13, size=(40, 44, 36), max_disp_mm=3.0, svf_sigma_mm=6.0)
+        stage = SyNStage.from_args("SyN[0.2,3,0]", "CC[f,m,1,4]", "[8x5x3,1e-6,10]", "4x2x1", "2x1x0vox")
+    else:
+        pair = ph.blob_pair_2d(5, n=96)
+        stage = SyNStage.from_args("SyN[0.25,3,0]", "CC[f,m,1,4]", "[10x6x3,1e-6,10]", "4x2x1", "2x1x0vox")
+    A, t, c = _affine_case(D, 8)
+    fsize = tuple(int(s) // 2 + 4 for s in pair.size)
+    fsp, forg = [2.0] * D, [-3.0] * D
+    fm = _smooth_field(tuple(reversed(fsize)), D, 2.0, 1)
+    ff = _smooth_field(tuple(reversed(fsize)), D, 1.0, 2)
+    geo = Geometry(size=fsize, spacing=fsp, origin=forg)
+    reg = SyNRegistration(stage, D, force_iterations=True)
+    reg.set_initial_transforms([AffineTransform(A, t, c), DisplacementFieldTransform(fm, geo)],
+                               [DisplacementFieldTransform(ff, geo)])
+    res = reg.register(pair.fixed, pair.moving, pair.spacing, pair.origin, pair.spacing, pair.origin)
+    reg.close()
+    P = o.SynParams(shrink=stage.shrink, sigma=stage.sigmas, iterations=stage.convergence.iterations,
+                    learning_rate=stage.transform.gradient_step, radius=stage.metric.radius, force_iterations=True)
+    g = o.Grid(list(pair.size), list(pair.spacing), list(pair.origin))
+    og = o.og_geom(fsize, fsp, forg)
+    ref = o.syn_register(P, g, pair.fixed, g, pair.moving, fixed_xforms=[("field", og, ff)],
+                         moving_xforms=[("affine", A, t, c), ("field", og, fm)])
+    plain = o.syn_register(P, g, pair.fixed, g, pair.moving)
+    assert np.abs(plain.energy - ref.energy).max() > 1e-3, "the transforms must change the problem"
+    print(f"{D}-d displacement-field initial transforms parity:", compare(res, ref, pair.spacing))
 
 
 # ------------------------------------------------------------ restored state

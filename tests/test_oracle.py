@@ -612,3 +612,30 @@ def test_strict_tie_rule_only_differs_at_the_tie():
         runs.append(o.syn_register(P, g, pair.fixed, g, pair.moving))
     np.testing.assert_array_equal(runs[0].fwd, runs[1].fwd)
     np.testing.assert_array_equal(runs[0].inv_iters, runs[1].inv_iters)
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_initial_constant_field_is_a_translation(D):
+    """A displacement-field initial transform whose field is a constant t on a
+    grid covering every sampled point is the affine translation by t; a field
+    then an affine equals the translation composed before the affine (the
+    list order of the composite)."""
+    size, F, M = _pair(D, 21)
+    gF = o.Grid(list(size), [1.2] * D, [1.0, -2.0, 0.5][:D])
+    gM = o.Grid(list(size), [1.2] * D, [0.0, -1.0, 1.5][:D])
+    t = np.array([0.75, -0.375, 0.3125][:D])  # exact in float32, as the field stores it
+    fsize = [int(s * 1.2 / 2.0) + 12 for s in size]
+    fg = o.og_geom(fsize, [2.0] * D, [-12.0] * D)
+    fld = np.broadcast_to(t.astype(np.float32), tuple(reversed(fsize)) + (D,)).copy()
+    P = _stage_params()
+    r_field = o.syn_register(P, gF, F, gM, M, moving_xforms=[("field", fg, fld)])
+    r_aff = o.syn_register(P, gF, F, gM, M, moving_xforms=[("affine", np.eye(D), t, np.zeros(D))])
+    np.testing.assert_allclose(r_field.energy, r_aff.energy, rtol=1e-9)
+    np.testing.assert_allclose(r_field.fwd, r_aff.fwd, atol=1e-7)
+    R, tr, c = _rigid(D, 5)
+    r_fa = o.syn_register(P, gF, F, gM, M, moving_xforms=[("field", fg, fld), ("affine", R, tr, c)])
+    # the translation first: y = R (x + t - c) + tr + c = R (x - c') + tr + c' with c' = c - t ... as one affine
+    off = R @ (t - c) + tr + c
+    r_one = o.syn_register(P, gF, F, gM, M, moving_xforms=[("affine", R, off, np.zeros(D))])
+    np.testing.assert_allclose(r_fa.energy, r_one.energy, rtol=1e-9)
+    np.testing.assert_allclose(r_fa.fwd, r_one.fwd, atol=1e-7)
