@@ -20,6 +20,8 @@ ANTS_OK = 0
 ANTS_ERR_CANCELLED = 5
 ERRORS = {1: "INVALID", 2: "HIP", 3: "OOM", 4: "UNSUPPORTED", 5: "CANCELLED"}
 MAX_LEVELS = 8
+MODEL_SYN = 0
+MODEL_BSPLINE_SYN = 1
 KC_NAMES = ["warp", "cc", "smooth", "compose", "inv"]
 
 
@@ -51,6 +53,10 @@ class SynParamsC(C.Structure):
         ("precision", C.c_int),
         ("preprocess", C.c_int),
         ("force_iterations", C.c_int),
+        ("model", C.c_int),
+        ("update_mesh", C.c_int * 3),
+        ("total_mesh", C.c_int * 3),
+        ("spline_order", C.c_int),
     ]
 
 
@@ -142,6 +148,7 @@ def lib():
     L.ants_syn_get_kernel_stats_level.argtypes = [vp, i, C.POINTER(KernelStatsC)]
     L.ants_op_smooth_image.argtypes = [pdesc, vp, vp, d, i, vp]
     L.ants_op_smooth_field.argtypes = [pdesc, vp, vp, d, vp]
+    L.ants_op_bspline_field.argtypes = [pdesc, vp, vp, C.POINTER(i), i, C.POINTER(d), vp]
     L.ants_op_warp_image.argtypes = [pdesc, vp, pdesc, vp, vp, vp]
     L.ants_op_cc.argtypes = [pdesc, vp, vp, i, vp, vp, C.POINTER(d), vp]
     L.ants_op_warp_pair.argtypes = [pdesc, pdesc, vp, vp, vp, pdesc, vp, vp, vp, i, vp]

@@ -96,10 +96,13 @@ def cmd_registration(argv: Sequence[str]) -> int:
     ap.add_argument("--precision", choices=["fp32", "bf16"], default="fp32")
     ap.add_argument("-v", "--verbose", type=int, default=0)
     a = ap.parse_args(argv)
-    stage = SyNStage.from_args(a.transform, a.metric, a.convergence, a.shrink_factors, a.smoothing_sigmas)
     mparts = _bracket_list(re.sub(r"^\s*CC", "", a.metric))
     fixed, gf = aio.read_nifti(mparts[0])
     moving, gm = aio.read_nifti(mparts[1])
+    # a BSplineSyN knot spacing becomes a mesh with the fixed image's geometry
+    # (antsRegistrationTemplateHeader.h:709-731)
+    stage = SyNStage.from_args(a.transform, a.metric, a.convergence, a.shrink_factors, a.smoothing_sigmas,
+                               gf.size, gf.spacing)
     for g, name in ((gf, mparts[0]), (gm, mparts[1])):
         if g.dim != a.dimensionality:
             raise SystemExit(f"{name} is {g.dim}-D, -d {a.dimensionality}")

@@ -269,6 +269,24 @@ int validate_params(const ants_syn_params* p) {
   if (p->convergence_window < 1) return fail(ANTS_ERR_INVALID, "convergence window must be >= 1");
   if (p->precision != ANTS_PRECISION_FP32 && p->precision != ANTS_PRECISION_BF16_PYRAMID)
     return fail(ANTS_ERR_INVALID, "unknown precision mode");
+  if (p->model != ANTS_MODEL_SYN && p->model != ANTS_MODEL_BSPLINE_SYN)
+    return fail(ANTS_ERR_INVALID, "unknown field model %d", p->model);
+  if (p->model == ANTS_MODEL_BSPLINE_SYN) {
+    if (p->spline_order < 1 || p->spline_order > 3) return fail(ANTS_ERR_UNSUPPORTED, "B-spline order must be 1..3");
+    int zeros = 0;
+    for (int d = 0; d < p->dim; ++d) {
+      if (p->update_mesh[d] < 1) return fail(ANTS_ERR_INVALID, "BSplineSyN update mesh sizes must be >= 1");
+      if (p->total_mesh[d] < 0) return fail(ANTS_ERR_INVALID, "BSplineSyN total mesh sizes must be >= 0");
+      zeros += p->total_mesh[d] == 0;
+      // the helper doubles the mesh at every level
+      const int64_t lim = int64_t{1} << 20;
+      if ((static_cast<int64_t>(p->update_mesh[d]) << (p->n_levels - 1)) > lim ||
+          (static_cast<int64_t>(p->total_mesh[d]) << (p->n_levels - 1)) > lim)
+        return fail(ANTS_ERR_INVALID, "BSplineSyN mesh sizes too large");
+    }
+    if (zeros != 0 && zeros != p->dim)
+      return fail(ANTS_ERR_INVALID, "BSplineSyN total mesh must be zero on every axis or on none");
+  }
   return ANTS_OK;
 }
 
@@ -579,6 +597,11 @@ struct ants_syn_handle {
   DevBuf F, M, Fs, Ms, tmp, Fw, Mw, Fb, Mb;  // Fb, Mb: BF16 pyramid images
   DevBuf fld[8];  // 4 middle fields, 4 work fields
   DevBuf blend;   // 2 fields: the separable smoothing's middle pass when a 3-D variance is below 0.5
+  // BSplineSyN: fit tables of every level (update and total fields; host
+  // copy kept for the asynchronous upload) and the lattices of the update
+  // fits (front end) and total fits (back end), two fields each
+  DevBuf bsp_tab, bsp_upd, bsp_tot;
+  std::vector<unsigned char> bsp_host;
   DevBuf stats, partials_cc, partials_inv, masks_inv, minmax;
   IterStats* host_stats = nullptr;  // pinned
   uint32_t* host_flag = nullptr;    // pinned, coherent: set when an iteration's statistics are published
@@ -637,7 +660,8 @@ struct ants_syn_handle {
   int64_t device_bytes() const {
     const DevBuf* all[] = {&F,      &M,           &Fs,          &Ms,        &tmp,    &Fw,          &Mw,
                            &Fb,     &Mb,          &stats,       &partials_cc, &partials_inv, &masks_inv,
-                           &minmax, &restore_buf, &io_f,        &io_m,      &io_fwd, &io_inv, &blend};
+                           &minmax, &restore_buf, &io_f,        &io_m,      &io_fwd, &io_inv, &blend,
+                           &bsp_tab, &bsp_upd,    &bsp_tot};
     int64_t b = 0;
     for (const DevBuf* x : all) b += static_cast<int64_t>(x->bytes);
     for (const auto& x : fld) b += static_cast<int64_t>(x.bytes);
@@ -650,7 +674,7 @@ struct ants_syn_handle {
     graph_signature.clear();
     DevBuf* all[] = {&F,     &M,           &Fs,          &Ms,        &tmp,    &Fw,   &Mw,   &Fb,     &Mb,
                      &stats, &partials_cc, &partials_inv, &masks_inv, &minmax, &io_f, &io_m, &io_fwd, &io_inv,
-                     &blend};
+                     &blend, &bsp_tab, &bsp_upd, &bsp_tot};
     for (DevBuf* b : all) b->release();
     for (auto& b : fld) b.release();
     have_state = false;
@@ -857,6 +881,185 @@ int smooth_fields_on(hipStream_t s, const HostGrid& g, int np, float* const* in,
   return ANTS_OK;
 }
 
+// ---- BSplineSyN: tables and passes of the B-spline field fit ----
+// BSplineSmoothDisplacementField of BSplineSyN (ITK's DisplacementField-
+// ToBSplineImageFilter with one level of BSplineScatteredDataPointSetTo-
+// ImageFilter, stationary boundary, no confidence image; configured at
+// Examples/itkantsRegistrationHelper.hxx:2222-2340).  Voxel i of an axis of n
+// voxels and S spans sits at p = i S / (n - 1) (p = S moved to S - eps); its
+// span is floor(p) and its basis values are ITK's centred BSplineKernel-
+// Function of order q at t - j + (q - 1) / 2, t = p - floor(p), j = 0..q.
+
+double bsp_basis(int q, double u) {
+  u = std::fabs(u);
+  if (q == 1) return u < 1.0 ? 1.0 - u : 0.0;
+  if (q == 2) return u < 0.5 ? 0.75 - u * u : (u < 1.5 ? 0.5 * (1.5 - u) * (1.5 - u) : 0.0);
+  if (u < 1.0) return (4.0 + u * u * (3.0 * u - 6.0)) / 6.0;
+  return u < 2.0 ? (2.0 - u) * (2.0 - u) * (2.0 - u) / 6.0 : 0.0;
+}
+
+// The filter's epsilon for points at the domain end: 100 FLT_EPSILON, times
+// ten while float(S) - eps still rounds to float(S) (S: largest span count).
+double bsp_end_eps(int max_spans) {
+  volatile float eps = 100.0f * std::numeric_limits<float>::epsilon();
+  const volatile float S = static_cast<float>(max_spans);
+  while (static_cast<float>(S - eps) == static_cast<float>(S)) eps = eps * 10.0f;
+  return static_cast<double>(eps);
+}
+
+// Layout of one fit's tables inside a table blob (byte offsets per axis:
+// span, bw, fw, lo, hi, A, Ai) and the sizes of its passes.
+struct BspLayout {
+  int D = 0, q = 3;
+  int n[3] = {1, 1, 1}, m[3] = {1, 1, 1};
+  size_t off[3][7] = {};
+  // floats per field of the intermediate lattices (x-fitted, xy-fitted, full)
+  int64_t field_floats() const {
+    return (static_cast<int64_t>(m[0]) * n[1] * n[2] + static_cast<int64_t>(m[0]) * m[1] * n[2] +
+            static_cast<int64_t>(m[0]) * m[1] * m[2]) *
+           D;
+  }
+};
+
+template <class T>
+size_t blob_put(std::vector<unsigned char>& blob, const std::vector<T>& v) {
+  size_t o = (blob.size() + 15) & ~size_t(15);
+  blob.resize(o + v.size() * sizeof(T));
+  if (!v.empty()) std::memcpy(blob.data() + o, v.data(), v.size() * sizeof(T));
+  return o;
+}
+
+// Appends the tables of a fit of grid g (mesh spans per axis, order q) to blob.
+int bsp_tables(const HostGrid& g, const int mesh[3], int q, std::vector<unsigned char>& blob, BspLayout* out) {
+  BspLayout L;
+  L.D = g.D;
+  L.q = q;
+  int max_spans = 0;
+  for (int a = 0; a < g.D; ++a) max_spans = std::max(max_spans, mesh[a]);
+  const double eps = bsp_end_eps(max_spans);
+  for (int a = 0; a < g.D; ++a) {
+    const int64_t n = g.n[a];
+    const int S = mesh[a];
+    if (n < 2) return fail(ANTS_ERR_UNSUPPORTED, "B-spline field fits need at least 2 voxels per axis");
+    if (S < 1) return fail(ANTS_ERR_INVALID, "B-spline mesh sizes must be >= 1");
+    const int m = S + q;
+    L.n[a] = static_cast<int>(n);
+    L.m[a] = m;
+    std::vector<int> span(n), lo(m), hi(m);
+    std::vector<float> bw(4 * n, 0.f), fw(4 * n, 0.f);
+    std::vector<double> bd(4 * n, 0.0), A(m, 0.0), Ai(m, 0.0);
+    for (int64_t i = 0; i < n; ++i) {
+      double p = static_cast<double>(i) * S / static_cast<double>(n - 1);
+      if (std::fabs(p - S) <= eps) p = S - eps;
+      const int si = static_cast<int>(std::floor(p));
+      const double t = p - si;
+      span[i] = si;
+      double sd = 0.0;
+      for (int j = 0; j <= q; ++j) {
+        bd[4 * i + j] = bsp_basis(q, t - j + 0.5 * (q - 1));
+        sd += bd[4 * i + j] * bd[4 * i + j];
+      }
+      const bool interior = i > 0 && i < n - 1;
+      for (int j = 0; j <= q; ++j) {
+        const double b = bd[4 * i + j];
+        bw[4 * i + j] = static_cast<float>(b);
+        fw[4 * i + j] = interior ? static_cast<float>(b * b * b / sd) : 0.f;
+      }
+    }
+    for (int k = 0; k < m; ++k) {
+      int l0 = 0;
+      while (l0 < n && span[l0] < k - q) ++l0;
+      int h0 = static_cast<int>(n) - 1;
+      while (h0 >= 0 && span[h0] > k) --h0;
+      lo[k] = l0;
+      hi[k] = h0;
+      for (int i = l0; i <= h0; ++i) {
+        const double b = bd[4 * i + (k - span[i])];
+        A[k] += b * b;
+        if (i > 0 && i < n - 1) Ai[k] += b * b;
+      }
+    }
+    L.off[a][0] = blob_put(blob, span);
+    L.off[a][1] = blob_put(blob, bw);
+    L.off[a][2] = blob_put(blob, fw);
+    L.off[a][3] = blob_put(blob, lo);
+    L.off[a][4] = blob_put(blob, hi);
+    L.off[a][5] = blob_put(blob, A);
+    L.off[a][6] = blob_put(blob, Ai);
+  }
+  *out = L;
+  return ANTS_OK;
+}
+
+BspDev bsp_dev(const BspLayout& L, const void* dbase) {
+  const unsigned char* b = static_cast<const unsigned char*>(dbase);
+  BspDev T{};
+  T.D = L.D;
+  T.q = L.q;
+  for (int a = 0; a < L.D; ++a) {
+    BspAxisDev& X = T.ax[a];
+    X.span = reinterpret_cast<const int*>(b + L.off[a][0]);
+    X.bw = reinterpret_cast<const float*>(b + L.off[a][1]);
+    X.fw = reinterpret_cast<const float*>(b + L.off[a][2]);
+    X.lo = reinterpret_cast<const int*>(b + L.off[a][3]);
+    X.hi = reinterpret_cast<const int*>(b + L.off[a][4]);
+    X.A = reinterpret_cast<const double*>(b + L.off[a][5]);
+    X.Ai = reinterpret_cast<const double*>(b + L.off[a][6]);
+    X.n = L.n[a];
+    X.m = L.m[a];
+  }
+  return T;
+}
+
+// The B-spline fit of np fields: fit along x, y (, z; the last fit pass
+// divides by the weights), then evaluate along z (,) y, x; the x evaluation
+// writes `out` and each field's max |u / spacing| into maxbits[k].  buf[k]
+// holds field_floats() floats of lattices for field k.  in and out may not alias.
+int bsp_fields_on(hipStream_t s, const HostGrid& g, const BspDev& T, const BspLayout& L, int np,
+                  const float* const* in, float* const* out, float* const* buf, unsigned int* maxbits,
+                  Profiler* prof = nullptr) {
+  const int D = g.D;
+  const int n0 = L.n[0], n1 = L.n[1], n2 = D == 3 ? L.n[2] : 1;
+  const int m0 = L.m[0], m1 = L.m[1], m2 = D == 3 ? L.m[2] : 1;
+  const float* cin[4];
+  float *t1[4], *t2[4], *lat[4];
+  for (int k = 0; k < np; ++k) {
+    cin[k] = in[k];
+    t1[k] = buf[k];
+    t2[k] = t1[k] + static_cast<int64_t>(m0) * n1 * n2 * D;
+    lat[k] = t2[k] + static_cast<int64_t>(m0) * m1 * n2 * D;
+  }
+  const float sp[3] = {static_cast<float>(g.sp[0]), static_cast<float>(g.sp[1]), static_cast<float>(g.sp[2])};
+  if (prof) prof->begin(ANTS_KC_SMOOTH);
+  const int d_in[3] = {n0, n1, n2};
+  launch_bsp_fit_axis(T, np, cin, t1, d_in, 0, false, s);
+  const float* c1[4];
+  for (int k = 0; k < np; ++k) c1[k] = t1[k];
+  const int d_x[3] = {m0, n1, n2};
+  if (D == 3) {
+    launch_bsp_fit_axis(T, np, c1, t2, d_x, 1, false, s);
+    const float* c2[4];
+    for (int k = 0; k < np; ++k) c2[k] = t2[k];
+    const int d_xy[3] = {m0, m1, n2};
+    launch_bsp_fit_axis(T, np, c2, lat, d_xy, 2, true, s);
+    const float* cl[4];
+    for (int k = 0; k < np; ++k) cl[k] = lat[k];
+    const int d_l[3] = {m0, m1, m2};
+    launch_bsp_eval_axis(T, np, cl, t2, d_l, 2, sp, nullptr, s);  // (m0, m1, n2)
+    launch_bsp_eval_axis(T, np, c2, t1, d_xy, 1, sp, nullptr, s);  // (m0, n1, n2)
+  } else {
+    launch_bsp_fit_axis(T, np, c1, lat, d_x, 1, true, s);
+    const float* cl[4];
+    for (int k = 0; k < np; ++k) cl[k] = lat[k];
+    const int d_l[3] = {m0, m1, 1};
+    launch_bsp_eval_axis(T, np, cl, t1, d_l, 1, sp, nullptr, s);  // (m0, n1)
+  }
+  launch_bsp_eval_axis(T, np, c1, out, d_x, 0, sp, maxbits, s);
+  if (prof) prof->end(ANTS_KC_SMOOTH, np * 2.0 * D * 4.0 * static_cast<double>(g.count()), D == 3 ? 6 : 4);
+  HIP_TRY(hipGetLastError());
+  return ANTS_OK;
+}
+
 }  // namespace
 
 int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const ants_image_desc* md,
@@ -894,6 +1097,33 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
   if (need_blend) ANTS_TRY(blend.ensure(2 * fbytes));
   float* const blend_f[2] = {need_blend ? blend.f() : nullptr,
                              need_blend ? blend.f() + static_cast<size_t>(maxN) * D : nullptr};
+  // BSplineSyN: the fit tables of every level (mesh << level, as the
+  // helper's adaptors set it), uploaded once per run, and the lattices
+  const bool bsp = P.model == ANTS_MODEL_BSPLINE_SYN;
+  const bool bsp_total = bsp && P.total_mesh[0] > 0;
+  std::vector<BspLayout> bsp_lu(P.n_levels), bsp_lt(P.n_levels);
+  std::vector<size_t> bsp_bu(P.n_levels, 0), bsp_bt(P.n_levels, 0);
+  int64_t bsp_fu = 0, bsp_ft = 0;  // floats per field of the update / total lattices
+  if (bsp) {
+    bsp_host.clear();
+    for (int l = 0; l < P.n_levels; ++l) {
+      int um[3] = {1, 1, 1}, tm[3] = {1, 1, 1};
+      for (int d = 0; d < D; ++d) {
+        um[d] = P.update_mesh[d] << l;
+        tm[d] = P.total_mesh[d] << l;
+      }
+      ANTS_TRY(bsp_tables(lg[l], um, P.spline_order, bsp_host, &bsp_lu[l]));
+      bsp_fu = std::max(bsp_fu, bsp_lu[l].field_floats());
+      if (bsp_total) {
+        ANTS_TRY(bsp_tables(lg[l], tm, P.spline_order, bsp_host, &bsp_lt[l]));
+        bsp_ft = std::max(bsp_ft, bsp_lt[l].field_floats());
+      }
+    }
+    ANTS_TRY(bsp_tab.ensure(bsp_host.size()));
+    HIP_TRY(hipMemcpyAsync(bsp_tab.p, bsp_host.data(), bsp_host.size(), hipMemcpyHostToDevice, stream));
+    ANTS_TRY(bsp_upd.ensure(2 * bsp_fu * sizeof(float)));
+    if (bsp_total) ANTS_TRY(bsp_tot.ensure(2 * bsp_ft * sizeof(float)));
+  }
   ANTS_TRY(stats.ensure(2 * sizeof(IterStats)));  // two iteration parities
   int64_t ccp = 0, invp = 0, invw = 0;
   for (int l = 0; l < P.n_levels; ++l) {
@@ -917,6 +1147,7 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
     std::vector<const void*> sig = {F.p, M.p, Fs.p, Ms.p, Fw.p, Mw.p, Fb.p, Mb.p, stats.p, partials_cc.p,
                                     partials_inv.p, host_stats, host_flag, static_cast<const void*>(s)};
     for (auto& b : fld) sig.push_back(b.p);
+    for (const DevBuf* b : {&bsp_tab, &bsp_upd, &bsp_tot}) sig.push_back(b->p);
     if (sig != graph_signature || graphs.size() > 512) {
       clear_graphs();
       graph_signature = sig;
@@ -939,6 +1170,9 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
     setup_hash = fnv1a(P.sigma, sizeof(P.sigma), setup_hash);
     const double dbl[] = {P.learning_rate, P.update_variance, P.total_variance};
     setup_hash = fnv1a(dbl, sizeof(dbl), setup_hash);
+    const int bints[] = {P.model,         P.update_mesh[0], P.update_mesh[1], P.update_mesh[2],
+                         P.total_mesh[0], P.total_mesh[1],  P.total_mesh[2],  P.spline_order};
+    setup_hash = fnv1a(bints, sizeof(bints), setup_hash);
     for (int k = 0; k < 2; ++k) {
       const int on = init_on[k] ? 1 : 0;
       setup_hash = fnv1a(&on, sizeof(on), setup_hash);
@@ -1031,6 +1265,8 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
     const HostGrid& g = lg[l];
     const GridDev gd = to_dev(g);
     const int64_t N = g.count();
+    const BspDev bsp_u_dev = bsp ? bsp_dev(bsp_lu[l], bsp_tab.p) : BspDev{};
+    const BspDev bsp_t_dev = bsp_total ? bsp_dev(bsp_lt[l], bsp_tab.p) : BspDev{};
     WarpMap maps[2];
     // with a displacement field among the initial transforms each image is
     // warped by the antsApplyTransforms kernel through [the SyN inverse
@@ -1221,6 +1457,14 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
       HIP_TRY(hipGetLastError());
       float* in[2] = {wk_[0], wk_[1]};
       float* work[2] = {wk_[2], wk_[3]};
+      if (bsp) {
+        // BSplineSyN: the update fields are B-spline fits (into wk[2], wk[3])
+        float* lat[2] = {bsp_upd.f(), bsp_upd.f() + bsp_fu};
+        ANTS_TRY(bsp_fields_on(fs, g, bsp_u_dev, bsp_lu[l], 2, in, work, lat, stats_of(par)->maxnorm, &prof));
+        std::swap(wk_[0], wk_[2]);
+        std::swap(wk_[1], wk_[3]);
+        return ANTS_OK;
+      }
       float* out[2] = {D % 2 == 1 ? wk_[2] : wk_[0], D % 2 == 1 ? wk_[3] : wk_[1]};
       ANTS_TRY(smooth_fields_on(fs, g, 2, in, work, out, P.update_variance, stats_of(par)->maxnorm, &prof,
                                 need_blend ? blend_f : nullptr));
@@ -1250,7 +1494,15 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
         prof.end(ANTS_KC_COMPOSE, 2.0 * Nd * 3.0 * D * e_f);
         HIP_TRY(hipGetLastError());
       }
-      if (P.total_variance > 0.0) {
+      if (bsp_total) {
+        // BSplineSyN with a total-field mesh: fit the composed fields
+        const float* in[2] = {wk_[2], wk_[3]};
+        float* out[2] = {wk_[0], wk_[1]};
+        float* lat[2] = {bsp_tot.f(), bsp_tot.f() + bsp_ft};
+        ANTS_TRY(bsp_fields_on(s, g, bsp_t_dev, bsp_lt[l], 2, in, out, lat, stats_of(par)->maxnorm + 2, &prof));
+        std::swap(wk_[0], wk_[2]);
+        std::swap(wk_[1], wk_[3]);
+      } else if (!bsp && P.total_variance > 0.0) {
         float* in[2] = {wk_[2], wk_[3]};
         float* work[2] = {wk_[0], wk_[1]};
         float* out[2] = {D % 2 == 1 ? wk_[0] : wk_[2], D % 2 == 1 ? wk_[1] : wk_[3]};
@@ -1827,6 +2079,42 @@ int ants_op_smooth_field(const ants_image_desc* gdsc, const float* d_in, float* 
   return rc;
 }
 
+int ants_op_bspline_field(const ants_image_desc* gdsc, const float* d_in, float* d_out, const int* mesh, int order,
+                          double* host_max, void* stream) {
+  HostGrid g;
+  ANTS_TRY(to_host_grid(gdsc, &g));
+  if (!d_in || !d_out || !mesh) return fail(ANTS_ERR_INVALID, "null buffer or mesh");
+  if (d_in == d_out) return fail(ANTS_ERR_INVALID, "the B-spline fit cannot run in place");
+  if (order < 1 || order > 3) return fail(ANTS_ERR_UNSUPPORTED, "B-spline order must be 1..3");
+  int m3[3] = {1, 1, 1};
+  for (int d = 0; d < g.D; ++d) {
+    if (mesh[d] < 1 || mesh[d] > (1 << 20)) return fail(ANTS_ERR_INVALID, "B-spline mesh sizes must be in 1..2^20");
+    m3[d] = mesh[d];
+  }
+  std::vector<unsigned char> blob;
+  BspLayout L;
+  ANTS_TRY(bsp_tables(g, m3, order, blob, &L));
+  DevBuf tab, lat, m;
+  ANTS_TRY(tab.ensure(blob.size()));
+  ANTS_TRY(lat.ensure(static_cast<size_t>(L.field_floats()) * sizeof(float)));
+  ANTS_TRY(m.ensure(16));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  HIP_TRY(hipMemcpyAsync(tab.p, blob.data(), blob.size(), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemsetAsync(m.p, 0, 16, s));
+  const BspDev T = bsp_dev(L, tab.p);
+  const float* in[1] = {d_in};
+  float* out[1] = {d_out};
+  float* buf[1] = {lat.f()};
+  ANTS_TRY(bsp_fields_on(s, g, T, L, 1, in, out, buf, static_cast<unsigned int*>(m.p)));
+  unsigned int bits = 0;
+  HIP_TRY(hipMemcpyAsync(&bits, m.p, 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  float mx;
+  std::memcpy(&mx, &bits, 4);
+  if (host_max) *host_max = mx;
+  return ANTS_OK;
+}
+
 int ants_op_warp_image(const ants_image_desc* gi, const float* d_img, const ants_image_desc* go, const float* d_field,
                        float* d_out, void* stream) {
   HostGrid a, b;
@@ -2328,6 +2616,25 @@ int64_t footprint_bytes(const ants_syn_params& P, const HostGrid& gF, const Host
   auto small_var = [](double v) { return v > 0.0 && v < 0.5; };
   if (D == 3 && (small_var(P.update_variance) || small_var(P.total_variance)))
     b += 2 * maxN * D * 4;               // the blend buffers of a small 3-D variance
+  if (P.model == ANTS_MODEL_BSPLINE_SYN) {
+    // the B-spline lattices of the update and total fits (2 fields each;
+    // the largest level of each) and the tables
+    int64_t fu = 0, ft = 0;
+    for (int l = 0; l < P.n_levels; ++l) {
+      const HostGrid g = level_grid(gF, P.shrink[l]);
+      int64_t nn[3] = {1, 1, 1}, mu[3] = {1, 1, 1}, mt[3] = {1, 1, 1};
+      for (int d = 0; d < D; ++d) {
+        nn[d] = g.n[d];
+        mu[d] = (static_cast<int64_t>(P.update_mesh[d]) << l) + P.spline_order;
+        mt[d] = (static_cast<int64_t>(P.total_mesh[d]) << l) + P.spline_order;
+      }
+      fu = std::max(fu, (mu[0] * nn[1] * nn[2] + mu[0] * mu[1] * nn[2] + mu[0] * mu[1] * mu[2]) * D);
+      if (P.total_mesh[0] > 0)
+        ft = std::max(ft, (mt[0] * nn[1] * nn[2] + mt[0] * mt[1] * nn[2] + mt[0] * mt[1] * mt[2]) * D);
+      b += (nn[0] + nn[1] + nn[2]) * 2 * 40 + (mu[0] + mu[1] + mu[2] + mt[0] + mt[1] + mt[2]) * 24;
+    }
+    b += 2 * (fu + ft) * 4;
+  }
   b += 4 * ((maxN + 63) / 64) * 8;       // inversion candidate masks
   b += (NF + NM) * 4 + NM * 4;           // request inputs (images, labels)
   b += 2 * maxN * D * 4 + maxN * 4;      // outputs (fields, labels)

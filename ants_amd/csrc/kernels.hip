@@ -1991,6 +1991,103 @@ __global__ void __launch_bounds__(kBlock) k_apply_transforms(GridDev go, ApplyDe
 }
 #pragma clang fp contract(on)
 
+// ---- B-spline fit of displacement fields (BSplineSyN) ----
+// One thread per output element (all D components).  Fit along `axis`:
+// control point k gathers the voxels of its range [lo_k, hi_k] with weights
+// fw[i][k - span_i]; consecutive threads take consecutive x, so the y and z
+// passes read rows that are contiguous across the wave and the x pass reads
+// overlapping short runs through the cache.
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_bsp_fit_axis(BspAxisDev A, BspDev T, PtrPack4 in, MutPack4 out, int n0,
+                                                         int n1, int n2, int axis, int last) {
+  const float* __restrict__ src = in.p[blockIdx.y];
+  float* __restrict__ dst = out.p[blockIdx.y];
+  const int o0 = axis == 0 ? A.m : n0, o1 = axis == 1 ? A.m : n1, o2 = axis == 2 ? A.m : n2;
+  const int64_t NO = static_cast<int64_t>(o0) * o1 * o2;
+  const int64_t stride = axis == 0 ? 1 : (axis == 1 ? n0 : static_cast<int64_t>(n0) * n1);
+  for (int64_t idx = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; idx < NO;
+       idx += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int x = static_cast<int>(idx % o0);
+    const int64_t r = idx / o0;
+    const int y = static_cast<int>(r % o1);
+    const int z = static_cast<int>(r / o1);
+    const int k = axis == 0 ? x : (axis == 1 ? y : z);
+    const int64_t base = (static_cast<int64_t>(axis == 2 ? 0 : z) * n1 + (axis == 1 ? 0 : y)) * n0 + (axis == 0 ? 0 : x);
+    float acc[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) acc[c] = 0.f;
+    const int lo = A.lo[k], hi = A.hi[k];
+    for (int i = lo; i <= hi; ++i) {
+      const float w = A.fw[i * 4 + (k - A.span[i])];
+      const float* v = src + (base + i * stride) * D;
+#pragma unroll
+      for (int c = 0; c < D; ++c) acc[c] = fmaf(w, v[c], acc[c]);
+    }
+    if (last) {
+      // every axis is a control-point axis now: W_k from the 1-D sums, in double
+      const int kk[3] = {x, y, z};
+      double pa = 1.0, pi = 1.0;
+      for (int d = 0; d < T.D; ++d) {
+        pa *= T.ax[d].A[kk[d]];
+        pi *= T.ax[d].Ai[kk[d]];
+      }
+      const double W = pa + (1.0e10 - 1.0) * (pa - pi);
+#pragma unroll
+      for (int c = 0; c < D; ++c) acc[c] = W != 0.0 ? static_cast<float>(static_cast<double>(acc[c]) / W) : 0.f;
+    }
+#pragma unroll
+    for (int c = 0; c < D; ++c) dst[idx * D + c] = acc[c];
+  }
+}
+
+// Evaluation along `axis`: voxel i sums control points span_i .. span_i + q
+// with the basis values bw[i][j].  The final pass (maxbits set) also takes
+// the max |u / spacing| of its field.
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_bsp_eval_axis(BspAxisDev A, PtrPack4 in, MutPack4 out, int m0, int m1,
+                                                          int m2, int axis, int q, float sp0, float sp1, float sp2,
+                                                          unsigned int* maxbits) {
+  const float* __restrict__ src = in.p[blockIdx.y];
+  float* __restrict__ dst = out.p[blockIdx.y];
+  const int o0 = axis == 0 ? A.n : m0, o1 = axis == 1 ? A.n : m1, o2 = axis == 2 ? A.n : m2;
+  const int64_t NO = static_cast<int64_t>(o0) * o1 * o2;
+  const int64_t stride = axis == 0 ? 1 : (axis == 1 ? m0 : static_cast<int64_t>(m0) * m1);
+  const float isp[3] = {1.f / sp0, 1.f / sp1, 1.f / sp2};
+  float nrm = 0.f;
+  for (int64_t idx = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; idx < NO;
+       idx += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int x = static_cast<int>(idx % o0);
+    const int64_t r = idx / o0;
+    const int y = static_cast<int>(r % o1);
+    const int z = static_cast<int>(r / o1);
+    const int i = axis == 0 ? x : (axis == 1 ? y : z);
+    const int64_t base = (static_cast<int64_t>(axis == 2 ? 0 : z) * m1 + (axis == 1 ? 0 : y)) * m0 + (axis == 0 ? 0 : x);
+    const int s0 = A.span[i];
+    float acc[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) acc[c] = 0.f;
+    for (int j = 0; j <= q; ++j) {
+      const float w = A.bw[i * 4 + j];
+      const float* v = src + (base + static_cast<int64_t>(s0 + j) * stride) * D;
+#pragma unroll
+      for (int c = 0; c < D; ++c) acc[c] = fmaf(w, v[c], acc[c]);
+    }
+    float s2 = 0.f;
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      dst[idx * D + c] = acc[c];
+      const float t = acc[c] * isp[c];
+      s2 += t * t;
+    }
+    nrm = fmaxf(nrm, sqrtf(s2));
+  }
+  if (maxbits) {
+    double dummy = 0.0;
+    block_sum_max<kBlock>(dummy, nrm);
+    if (threadIdx.x == 0) atomicMax(&maxbits[blockIdx.y], __float_as_uint(nrm));
+  }
+}
+
 inline unsigned int blocks_for(int64_t n, int bs = kBlock) { return static_cast<unsigned int>((n + bs - 1) / bs); }
 inline unsigned int red_blocks(int64_t n) {
   const unsigned int b = blocks_for(n);
@@ -2397,6 +2494,36 @@ void launch_publish_stats(const Partial* cc_parts, int cc_n, RedSlot* energy, co
   static_assert(kCCTX * kCCTY == kBlock, "CC partials are reduced in the order of k_reduce_slot<kBlock>");
   hipLaunchKernelGGL(k_publish_stats<kBlock>, dim3(1), dim3(kBlock), 0, s, cc_parts, cc_n, energy, src, dst, nwords,
                      flag);
+}
+
+void launch_bsp_fit_axis(const BspDev& T, int np, const float* const* in, float* const* out, const int dims[3],
+                         int axis, bool last, hipStream_t s) {
+  const BspAxisDev& A = T.ax[axis];
+  int64_t no = A.m;
+  for (int d = 0; d < 3; ++d)
+    if (d != axis) no *= dims[d];
+  const dim3 grid(grid_stride_blocks(no) * 4, np);
+  if (T.D == 2)
+    hipLaunchKernelGGL(k_bsp_fit_axis<2>, grid, dim3(kBlock), 0, s, A, T, pack(in, np), mpack(out, np), dims[0],
+                       dims[1], dims[2], axis, static_cast<int>(last));
+  else
+    hipLaunchKernelGGL(k_bsp_fit_axis<3>, grid, dim3(kBlock), 0, s, A, T, pack(in, np), mpack(out, np), dims[0],
+                       dims[1], dims[2], axis, static_cast<int>(last));
+}
+
+void launch_bsp_eval_axis(const BspDev& T, int np, const float* const* in, float* const* out, const int dims[3],
+                          int axis, const float sp[3], unsigned int* maxbits, hipStream_t s) {
+  const BspAxisDev& A = T.ax[axis];
+  int64_t no = A.n;
+  for (int d = 0; d < 3; ++d)
+    if (d != axis) no *= dims[d];
+  const dim3 grid(maxbits ? red_blocks(no) : grid_stride_blocks(no) * 4, np);
+  if (T.D == 2)
+    hipLaunchKernelGGL(k_bsp_eval_axis<2>, grid, dim3(kBlock), 0, s, A, pack(in, np), mpack(out, np), dims[0],
+                       dims[1], dims[2], axis, T.q, sp[0], sp[1], sp[2], maxbits);
+  else
+    hipLaunchKernelGGL(k_bsp_eval_axis<3>, grid, dim3(kBlock), 0, s, A, pack(in, np), mpack(out, np), dims[0],
+                       dims[1], dims[2], axis, T.q, sp[0], sp[1], sp[2], maxbits);
 }
 
 void launch_compose(const GridDev& g, int np, const float* const* disp, const float* const* warp, float* const* out,

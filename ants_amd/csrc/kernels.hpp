@@ -173,6 +173,39 @@ int launch_cc_direct(const GridDev& g, const float* F, const float* M, int radiu
 void launch_publish_stats(const Partial* cc_parts, int cc_n, RedSlot* energy, const uint32_t* src, uint32_t* dst,
                           int nwords, uint32_t* flag, hipStream_t s);
 
+// B-spline fit of displacement fields (BSplineSyN's update and total field
+// smoothing): one level of multilevel B-spline approximation over every
+// voxel, boundary voxels as zero vectors of weight 1e10, evaluated back on
+// the grid.  Separable in index space: the numerator lattice is three 1-D
+// fit passes (axis tables fold in 1/sum_j b_j^2 and the interior mask), the
+// denominator W_k = prod A + (1e10 - 1)(prod A - prod Ai) comes from 1-D sums,
+// and the evaluation is three 1-D passes back.  Tables per axis (n voxels,
+// m = spans + order control points):
+struct BspAxisDev {
+  const int* span;   // [n] span of each voxel
+  const float* bw;   // [n][4] basis values b_j(t_i), j = 0..order
+  const float* fw;   // [n][4] fit weights b_j^3 / sum_j b_j^2, zero at voxels 0 and n-1
+  const int* lo;     // [m] first voxel whose span lies in [k - order, k]
+  const int* hi;     // [m] last such voxel (lo > hi: none)
+  const double* A;   // [m] sum of b_{k-s}(t)^2 over all voxels of the range
+  const double* Ai;  // [m] the same over the voxels 1..n-2
+  int n, m;
+};
+struct BspDev {
+  BspAxisDev ax[3];
+  int D, q;
+};
+// Fit pass along `axis` of np D-component fields with input dims `dims`
+// (the output has m control points on that axis).  `last`: the pass that
+// completes the lattice, which it divides by W_k.
+void launch_bsp_fit_axis(const BspDev& T, int np, const float* const* in, float* const* out, const int dims[3],
+                         int axis, bool last, hipStream_t s);
+// Evaluation pass along `axis` from input dims `dims` (m on that axis) to n
+// voxels.  maxbits != nullptr: the final pass, which also takes each field's
+// max |u / spacing| into maxbits[k] (atomic max of the float bits).
+void launch_bsp_eval_axis(const BspDev& T, int np, const float* const* in, float* const* out, const int dims[3],
+                          int axis, const float sp[3], unsigned int* maxbits, hipStream_t s);
+
 // out_k = warp_k + scale_k * disp_k(x + warp_k(x)), scale_k = rate / max_k when maxbits given (max>0), else 1.
 void launch_compose(const GridDev& g, int np, const float* const* disp, const float* const* warp,
                     float* const* out, const unsigned int* maxbits, float rate, hipStream_t s);

@@ -36,19 +36,54 @@ def parse_vector(text: str, typ=int) -> List:
     return [typ(v) for v in text.strip().split("x") if v != ""]
 
 
+def mesh_for_knot_spacing(size, spacing, knot_spacing: float) -> List[int]:
+    """RegistrationHelper::CalculateMeshSizeForSpecifiedKnotSpacing
+    (Examples/itkantsRegistrationHelper.hxx:3737-3767): per axis
+    ceil((size - 1) * spacing / knot_spacing); a knot spacing of 0 gives 0."""
+    if abs(float(knot_spacing)) <= 4 * np.finfo(np.float32).eps:
+        return [0] * len(size)
+    return [int(np.ceil((int(n) - 1) * float(sp) / float(knot_spacing))) for n, sp in zip(size, spacing)]
+
+
 @dataclass
 class SyNTransform:
     """`-t SyN[gradientStep, updateFieldVariance=3, totalFieldVariance=0]`
-    (Examples/antsRegistrationTemplateHeader.h:801-815)."""
+    (Examples/antsRegistrationTemplateHeader.h:801-815) or
+    `-t BSplineSyN[gradientStep, updateMesh, totalMesh=0, splineOrder=3]`
+    (Examples/antsRegistrationTemplateHeader.h:701-743): a mesh is `AxBxC`
+    (spans per axis at the first level) or one value, a knot spacing in mm,
+    converted with the fixed image's geometry."""
     gradient_step: float
     update_variance: float = 3.0
     total_variance: float = 0.0
+    model: str = "SyN"
+    update_mesh: Optional[List[int]] = None
+    total_mesh: Optional[List[int]] = None
+    spline_order: int = 3
 
     @classmethod
-    def parse(cls, text: str) -> "SyNTransform":
+    def parse(cls, text: str, fixed_size=None, fixed_spacing=None) -> "SyNTransform":
         name, p = parse_option_function(text)
+        if name.lower() == "bsplinesyn":
+            if len(p) < 2:
+                raise ValueError("BSplineSyN needs a gradient step and an update mesh: BSplineSyN[step,mesh,...]")
+            order = int(p[3]) if len(p) > 3 else 3
+
+            def mesh(spec, dim=None):
+                v = parse_vector(spec, float)
+                if len(v) != 1:
+                    return [int(x) for x in v]
+                if int(v[0]) == 0 and dim is not None:
+                    return [0] * dim     # a zero knot spacing is a zero mesh whatever the geometry
+                if fixed_size is None:
+                    raise ValueError("a knot spacing needs the fixed image geometry (or give the mesh as AxBxC)")
+                sp = fixed_spacing if fixed_spacing is not None else [1.0] * len(fixed_size)
+                return mesh_for_knot_spacing(fixed_size, sp, int(v[0]))
+            um = mesh(p[1])
+            tm = mesh(p[2], len(um)) if len(p) > 2 else [0] * len(um)
+            return cls(float(p[0]), model="BSplineSyN", update_mesh=um, total_mesh=tm, spline_order=order)
         if name.lower() != "syn":
-            raise ValueError(f"only the SyN transform is provided here, got {name!r}")
+            raise ValueError(f"only the SyN and BSplineSyN transforms are provided here, got {name!r}")
         if not p:
             raise ValueError("SyN needs a gradient step: SyN[step,...]")
         return cls(float(p[0]), float(p[1]) if len(p) > 1 else 3.0, float(p[2]) if len(p) > 2 else 0.0)
@@ -115,8 +150,11 @@ class SyNStage:
     sigmas_in_mm: bool = False
 
     @classmethod
-    def from_args(cls, transform: str, metric: str, convergence: str, shrink: str, smoothing: str) -> "SyNStage":
-        t = SyNTransform.parse(transform)
+    def from_args(cls, transform: str, metric: str, convergence: str, shrink: str, smoothing: str,
+                  fixed_size=None, fixed_spacing=None) -> "SyNStage":
+        """fixed_size / fixed_spacing (x, y[, z]): the fixed image's geometry,
+        needed only for a BSplineSyN mesh given as a knot spacing."""
+        t = SyNTransform.parse(transform, fixed_size, fixed_spacing)
         m = CCMetric.parse(metric)
         c = Convergence.parse(convergence)
         f = parse_vector(shrink)
@@ -185,6 +223,16 @@ class SyNRegistration:
         p.update_variance = stage.transform.update_variance
         p.total_variance = stage.transform.total_variance
         p.cc_radius = stage.metric.radius
+        if stage.transform.model == "BSplineSyN":
+            t = stage.transform
+            if len(t.update_mesh) != dim or len(t.total_mesh) != dim:
+                # itkantsRegistrationHelper.hxx:2239-2243
+                raise ValueError("the mesh size(s) don't match the image dimension")
+            p.model = L.MODEL_BSPLINE_SYN
+            for d in range(dim):
+                p.update_mesh[d] = int(stage.transform.update_mesh[d])
+                p.total_mesh[d] = int(stage.transform.total_mesh[d])
+            p.spline_order = int(stage.transform.spline_order)
         p.precision = {"fp32": 0, "bf16": 1}[precision]
         p.preprocess = int(preprocess)
         p.force_iterations = int(force_iterations)

@@ -47,6 +47,10 @@ typedef struct ants_image_desc {
 #define ANTS_PRECISION_FP32 0       /* FP32 images, fields and sums */
 #define ANTS_PRECISION_BF16_PYRAMID 1 /* BF16 smoothed images, FP32 sums/fields */
 
+/* Field models of the stage (-t SyN[...] / -t BSplineSyN[...]). */
+#define ANTS_MODEL_SYN 0          /* Gaussian update/total field smoothing */
+#define ANTS_MODEL_BSPLINE_SYN 1  /* B-spline fitted update/total fields */
+
 /* The SyN stage parameters antsRegistration parses
  * (Examples/antsRegistrationTemplateHeader.h:542-815, 985-1086) and
  * RegistrationHelper hands to SyNImageRegistrationMethod
@@ -67,6 +71,17 @@ typedef struct ants_syn_params {
   int precision;                        /* ANTS_PRECISION_* */
   int preprocess;                       /* window both images to [0,1] as PreprocessImage does (1 = ANTs default) */
   int force_iterations;                 /* 1: never stop early (parity harness), 0: convergence monitor */
+  /* BSplineSyN[step, updateMesh, totalMesh=0, order=3]
+   * (Examples/antsRegistrationTemplateHeader.h:701-743, itkantsRegistrationHelper.hxx:2222-2340):
+   * mesh sizes (spans per axis) at the first level, doubled at every later
+   * level as the helper's transform adaptors do; a knot spacing is converted
+   * to a mesh by the caller (RegistrationHelper::CalculateMeshSizeForSpecified-
+   * KnotSpacing, hxx:3737-3767).  total_mesh all zero: the total field is not
+   * fitted.  update_variance / total_variance are unused in this model. */
+  int model;                            /* ANTS_MODEL_* (0 = SyN) */
+  int update_mesh[3];
+  int total_mesh[3];
+  int spline_order;                     /* 1..3 (BSplineSyN default 3) */
 } ants_syn_params;
 
 /* Per-iteration observer (antsDisplacementAndVelocityFieldRegistrationCommandIterationUpdate
@@ -244,6 +259,18 @@ int ants_op_smooth_image(const ants_image_desc* g, const float* d_in, float* d_o
  * GaussianOperator(MaxError 0.001), boundary zeroed, blend below 0.5
  * (variance set at itkantsRegistrationHelper.hxx:2161-2162). */
 int ants_op_smooth_field(const ants_image_desc* g, const float* d_in, float* d_out, double variance, void* stream);
+
+/* BSplineSyN's BSplineSmoothDisplacementField(field, mesh + order control
+ * points): DisplacementFieldToBSplineImageFilter with one fitting level,
+ * stationary boundary (boundary voxels fitted as zero vectors of weight
+ * 1e10), no confidence image, evaluated back on the field's grid -- as the
+ * helper configures BSplineSyN (itkantsRegistrationHelper.hxx:2222-2340) and
+ * as SmoothDisplacementField's B-spline mode calls the filter with one level
+ * (Examples/SmoothDisplacementField.cxx:134-165).  mesh: spans per axis
+ * (>= 1; every axis needs >= 2 voxels), order 1..3.  host_max (optional)
+ * receives the output's max |u / spacing|. */
+int ants_op_bspline_field(const ants_image_desc* g, const float* d_in, float* d_out, const int* mesh, int order,
+                          double* host_max, void* stream);
 
 /* ResampleImageFilter + LinearInterpolateImageFunction through x -> x + u(x)
  * (u on the output grid, NULL = identity), default value 0

@@ -40,6 +40,10 @@ class OgParams(C.Structure):
         ("force_iterations", C.c_int),
         ("threads", C.c_int),
         ("inv_tie_rule", C.c_int),
+        ("model", C.c_int),
+        ("update_mesh", C.c_int * 3),
+        ("total_mesh", C.c_int * 3),
+        ("spline_order", C.c_int),
     ]
 
 
@@ -85,6 +89,8 @@ def lib():
         L.og_cc_f64.restype = d
         L.og_scale_update_f64.argtypes = [pg, pd, d]
         L.og_scale_update_f64.restype = d
+        L.og_bspline_fit_field_f64.argtypes = [pg, pd, pd, P(C.c_int), C.c_int]
+        L.og_bspline_point.argtypes = [C.c_int64, C.c_int64, C.c_int, C.c_int, C.c_int, P(C.c_int), pd]
         L.og_invert_f64.argtypes = [pg, pd, pd, pd, pd, pd]
         L.og_invert_f64.restype = C.c_int
         L.og_preprocess_f64.argtypes = [C.c_int64, pd, pd]
@@ -232,6 +238,23 @@ def scale_update(g: Grid, u, lr: float):
     return a, mx
 
 
+def bspline_fit_field(g: Grid, fld, mesh, order: int = 3):
+    """BSplineSyN's B-spline fit of a displacement field (mesh: spans per axis)."""
+    a = _f64(fld)
+    out = np.empty_like(a)
+    m = (C.c_int * 3)(*[int(v) for v in list(mesh) + [1] * (3 - len(mesh))])
+    lib().og_bspline_fit_field_f64(C.byref(g.c()), _pd(a), _pd(out), m, int(order))
+    return out
+
+
+def bspline_point(i: int, n: int, spans: int, order: int = 3, max_spans: int = None):
+    """(span, basis values b_0..b_3) of voxel i of an axis of n voxels."""
+    s = C.c_int()
+    b = np.zeros(4)
+    lib().og_bspline_point(int(i), int(n), int(spans), int(order), int(max_spans or spans), C.byref(s), _pd(b))
+    return s.value, b
+
+
 def invert(g: Grid, fld, init=None):
     """Returns (inverse, inner_iterations, last_max, last_mean)."""
     f = _f64(fld)
@@ -283,6 +306,11 @@ class SynParams:
     threads: int = 0
     # inversion max test: 0 guarded (max > 0.1 (1 + 1e-5)), 1 strict (ITK's max > 0.1)
     inv_tie_rule: int = 0
+    # 0 SyN; 1 BSplineSyN[learning_rate, update_mesh, total_mesh, spline_order]
+    model: int = 0
+    update_mesh: Sequence[int] = (0, 0, 0)
+    total_mesh: Sequence[int] = (0, 0, 0)
+    spline_order: int = 3
 
     def c(self) -> OgParams:
         p = OgParams()
@@ -302,6 +330,11 @@ class SynParams:
         p.force_iterations = int(self.force_iterations)
         p.threads = self.threads
         p.inv_tie_rule = int(self.inv_tie_rule)
+        p.model = int(self.model)
+        for d in range(3):
+            p.update_mesh[d] = int(self.update_mesh[d]) if d < len(self.update_mesh) else 0
+            p.total_mesh[d] = int(self.total_mesh[d]) if d < len(self.total_mesh) else 0
+        p.spline_order = int(self.spline_order)
         return p
 
 

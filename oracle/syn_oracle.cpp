@@ -56,6 +56,10 @@ typedef struct {
   int force_iterations;  // run every iteration (no convergence stop)
   int threads;           // 0 = OpenMP default
   int inv_tie_rule;      // inversion max test: 0 guarded (max > 0.1 (1 + 1e-5)), 1 strict (max > 0.1 in R)
+  int model;             // 0 SyN (Gaussian field smoothing), 1 BSplineSyN (B-spline fits)
+  int update_mesh[3];    // BSplineSyN: spans of the update field at the first level (doubled per level)
+  int total_mesh[3];     // BSplineSyN: spans of the total field at the first level (0: no fit)
+  int spline_order;      // BSplineSyN: 1..3
 } og_params;
 
 // Geometry and transforms of a composite: antsApplyTransforms and the SyN
@@ -293,6 +297,138 @@ void smooth_field(const Grid& g, const R* in, R* out, double variance) {
         if (D == 3) bnd = bnd || z == 0 || z == g.n[2] - 1;
         for (int c = 0; c < D; ++c)
           out[i * D + c] = bnd ? R(0) : static_cast<R>(a[i * D + c] * w1 + in[i * D + c] * w2);
+      }
+}
+
+// ---------------------------------------------------------------------------
+// B-spline fit of a displacement field: BSplineSyN's BSplineSmoothDisplacement-
+// Field, i.e. ITK's DisplacementFieldToBSplineImageFilter over
+// BSplineScatteredDataPointSetToImageFilter (ITK v5.4.6, absent from the
+// reference tree) with one fitting level, stationary boundary and no
+// confidence image, as the helper configures the transform
+// (Examples/itkantsRegistrationHelper.hxx:2222-2340: mesh size << level,
+// spline order) and as the reference's own callers of the filter set it up
+// (Utilities/itkDiReCTImageFilter.hxx:975-1000,
+// Examples/SmoothDisplacementField.cxx:134-165).
+//  * every voxel is a data point at its parametric coordinate
+//    p_d = i_d * spans_d / (n_d - 1) (the B-spline domain is the field's own);
+//    p_d = spans_d is moved to spans_d - eps, eps = 100 FLT_EPSILON, times 10
+//    while float(S) == float(S) - eps for S the largest span count;
+//  * voxels on the image boundary carry a zero vector with weight 1e10, the
+//    others their vector with weight 1;
+//  * one level of multilevel B-spline approximation with confidences (Lee,
+//    Wolberg & Shin, IEEE TVCG 3(3) 1997; Tustison & Gee, Insight Journal
+//    2005): for point c with basis products B_kc over its (q+1)^D control
+//    points, delta_k += w_c B_kc^2 (z_c B_kc / sum_l B_lc^2) and
+//    omega_k += w_c B_kc^2; phi_k = delta_k / omega_k (0 where omega_k = 0);
+//  * the output is the B-spline with control points phi evaluated at every
+//    voxel; the basis is ITK's centred BSplineKernelFunction of order q at
+//    t - j + (q - 1) / 2, j = 0..q.
+// A mesh of 0 spans means no fit (the total field of BSplineSyN[., ., 0]).
+// Pinned: an independent MBA built from scipy's B-spline basis on dense
+// matrices, tensor-product separability, mirror symmetry, linearity.
+double bsp_kernel(int q, double u) {
+  u = std::fabs(u);
+  switch (q) {
+    case 1:
+      return u < 1.0 ? 1.0 - u : 0.0;
+    case 2:
+      return u < 0.5 ? 0.75 - u * u : (u < 1.5 ? 0.5 * (1.5 - u) * (1.5 - u) : 0.0);
+    default:
+      return u < 1.0 ? (4.0 - 6.0 * u * u + 3.0 * u * u * u) / 6.0 : (u < 2.0 ? (2.0 - u) * (2.0 - u) * (2.0 - u) / 6.0 : 0.0);
+  }
+}
+
+double bsp_epsilon(int max_spans) {
+  volatile float eps = 100.0f * FLT_EPSILON;
+  const volatile float S = static_cast<float>(max_spans);
+  while (static_cast<float>(S) == static_cast<float>(S - eps)) eps = eps * 10.0f;
+  return static_cast<double>(eps);
+}
+
+// span index and basis values of voxel i of an axis of n voxels and `spans` spans
+void bsp_point(int64_t i, int64_t n, int spans, int q, double eps, int* s, double b[4]) {
+  double p = n > 1 ? static_cast<double>(i) * spans / static_cast<double>(n - 1) : 0.0;
+  if (std::fabs(p - spans) <= eps) p = spans - eps;
+  const int si = static_cast<int>(std::floor(p));
+  const double t = p - si;
+  for (int j = 0; j < 4; ++j) b[j] = j <= q ? bsp_kernel(q, t - j + 0.5 * (q - 1)) : 0.0;
+  *s = si;
+}
+
+template <class R>
+void bspline_fit_field(const Grid& g, const R* in, R* out, const int mesh[3], int q) {
+  const int D = g.D;
+  const int64_t N = g.count();
+  bool fit = true;
+  for (int d = 0; d < D; ++d) fit = fit && mesh[d] > 0;
+  if (!fit) {
+    std::memcpy(out, in, sizeof(R) * N * D);
+    return;
+  }
+  int maxs = 0;
+  for (int d = 0; d < D; ++d) maxs = std::max(maxs, mesh[d]);
+  const double eps = bsp_epsilon(maxs);
+  int64_t m[3] = {1, 1, 1};
+  for (int d = 0; d < D; ++d) m[d] = mesh[d] + q;
+  const int64_t NC = m[0] * m[1] * m[2];
+  std::vector<double> delta(NC * D, 0.0), omega(NC, 0.0);
+  const int taps[3] = {q + 1, D >= 2 ? q + 1 : 1, D == 3 ? q + 1 : 1};
+  const double kBoundaryWeight = 1.0e10;
+  for (int64_t z = 0; z < g.n[2]; ++z)
+    for (int64_t y = 0; y < g.n[1]; ++y)
+      for (int64_t x = 0; x < g.n[0]; ++x) {
+        const int64_t c[3] = {x, y, z};
+        int s[3] = {0, 0, 0};
+        double b[3][4] = {{1, 0, 0, 0}, {1, 0, 0, 0}, {1, 0, 0, 0}};
+        bool bnd = false;
+        for (int d = 0; d < D; ++d) {
+          bsp_point(c[d], g.n[d], mesh[d], q, eps, &s[d], b[d]);
+          bnd = bnd || c[d] == 0 || c[d] == g.n[d] - 1;
+        }
+        const int64_t i = x + g.n[0] * (y + g.n[1] * z);
+        const double w = bnd ? kBoundaryWeight : 1.0;
+        double w2 = 0.0;
+        for (int kz = 0; kz < taps[2]; ++kz)
+          for (int ky = 0; ky < taps[1]; ++ky)
+            for (int kx = 0; kx < taps[0]; ++kx) {
+              const double B = b[0][kx] * b[1][ky] * b[2][kz];
+              w2 += B * B;
+            }
+        for (int kz = 0; kz < taps[2]; ++kz)
+          for (int ky = 0; ky < taps[1]; ++ky)
+            for (int kx = 0; kx < taps[0]; ++kx) {
+              const double B = b[0][kx] * b[1][ky] * b[2][kz];
+              const int64_t k = (s[0] + kx) + m[0] * ((s[1] + ky) + m[1] * (s[2] + kz));
+              omega[k] += w * B * B;
+              for (int cc = 0; cc < D; ++cc) {
+                const double zc = bnd ? 0.0 : static_cast<double>(in[i * D + cc]);
+                delta[k * D + cc] += w * B * B * (zc * B / w2);
+              }
+            }
+      }
+  std::vector<double> phi(NC * D, 0.0);
+  for (int64_t k = 0; k < NC; ++k)
+    if (omega[k] != 0.0)
+      for (int cc = 0; cc < D; ++cc) phi[k * D + cc] = delta[k * D + cc] / omega[k];
+#pragma omp parallel for schedule(static)
+  for (int64_t z = 0; z < g.n[2]; ++z)
+    for (int64_t y = 0; y < g.n[1]; ++y)
+      for (int64_t x = 0; x < g.n[0]; ++x) {
+        const int64_t c[3] = {x, y, z};
+        int s[3] = {0, 0, 0};
+        double b[3][4] = {{1, 0, 0, 0}, {1, 0, 0, 0}, {1, 0, 0, 0}};
+        for (int d = 0; d < D; ++d) bsp_point(c[d], g.n[d], mesh[d], q, eps, &s[d], b[d]);
+        double v[3] = {0.0, 0.0, 0.0};
+        for (int kz = 0; kz < taps[2]; ++kz)
+          for (int ky = 0; ky < taps[1]; ++ky)
+            for (int kx = 0; kx < taps[0]; ++kx) {
+              const double B = b[0][kx] * b[1][ky] * b[2][kz];
+              const int64_t k = (s[0] + kx) + m[0] * ((s[1] + ky) + m[1] * (s[2] + kz));
+              for (int cc = 0; cc < D; ++cc) v[cc] += B * phi[k * D + cc];
+            }
+        const int64_t i = x + g.n[0] * (y + g.n[1] * z);
+        for (int cc = 0; cc < D; ++cc) out[i * D + cc] = static_cast<R>(v[cc]);
       }
 }
 
@@ -871,19 +1007,36 @@ int syn_register(const og_params* P, const Grid& gF, const R* fixed_in, const Gr
     ConvMonitor mon(P->conv_window);
     bool converged = false;
     int iter = 0;
+    // BSplineSyN: mesh sizes of this level (the helper's << level)
+    const bool bsp = P->model == 1;
+    int um[3] = {0, 0, 0}, tm[3] = {0, 0, 0};
+    for (int d = 0; d < D; ++d) {
+      um[d] = P->update_mesh[d] << l;
+      tm[d] = P->total_mesh[d] << l;
+    }
     while (iter++ < P->iterations[l] && !converged) {
       const auto t_iter = std::chrono::steady_clock::now();
       warp_image<R>(gF, Fs.data(), gl, f2m_i.data(), Fw.data(), fx, nfx);
       warp_image<R>(gM, Ms.data(), gl, m2m_i.data(), Mw.data(), mx, nmx);
       const double value = cc_metric<R>(gl, Fw.data(), Mw.data(), P->radius, eps, g1.data(), g2.data());
-      smooth_field<R>(gl, g1.data(), u1.data(), P->update_variance);
-      smooth_field<R>(gl, g2.data(), u2.data(), P->update_variance);
+      if (bsp) {
+        bspline_fit_field<R>(gl, g1.data(), u1.data(), um, P->spline_order);
+        bspline_fit_field<R>(gl, g2.data(), u2.data(), um, P->spline_order);
+      } else {
+        smooth_field<R>(gl, g1.data(), u1.data(), P->update_variance);
+        smooth_field<R>(gl, g2.data(), u2.data(), P->update_variance);
+      }
       scale_update<R>(gl, u1.data(), P->learning_rate);
       scale_update<R>(gl, u2.data(), P->learning_rate);
       compose<R>(gl, u1.data(), f2m_f.data(), t1.data());
       compose<R>(gl, u2.data(), m2m_f.data(), t2.data());
-      smooth_field<R>(gl, t1.data(), s1.data(), P->total_variance);
-      smooth_field<R>(gl, t2.data(), s2.data(), P->total_variance);
+      if (bsp) {
+        bspline_fit_field<R>(gl, t1.data(), s1.data(), tm, P->spline_order);
+        bspline_fit_field<R>(gl, t2.data(), s2.data(), tm, P->spline_order);
+      } else {
+        smooth_field<R>(gl, t1.data(), s1.data(), P->total_variance);
+        smooth_field<R>(gl, t2.data(), s2.data(), P->total_variance);
+      }
       int k[4];
       std::vector<R> ninv(N * D), nfwd(N * D);
       k[0] = invert_field<R>(gl, s1.data(), f2m_i.data(), ninv.data(), nullptr, nullptr, strict);
@@ -1173,6 +1326,13 @@ double og_cc_f64(const og_grid* g, const double* F, const double* M, int radius,
   return cc_metric<double>(to_grid(g), F, M, radius, eps, g_f2m, g_m2m);
 }
 double og_scale_update_f64(const og_grid* g, double* u, double lr) { return scale_update<double>(to_grid(g), u, lr); }
+void og_bspline_fit_field_f64(const og_grid* g, const double* in, double* out, const int* mesh, int order) {
+  const int m3[3] = {mesh[0], mesh[1], g->dim == 3 ? mesh[2] : 1};
+  bspline_fit_field<double>(to_grid(g), in, out, m3, order);
+}
+void og_bspline_point(int64_t i, int64_t n, int spans, int order, int max_spans, int* span, double* b) {
+  bsp_point(i, n, spans, order, bsp_epsilon(max_spans), span, b);
+}
 int og_invert_f64(const og_grid* g, const double* field, const double* init, double* inv, double* out_max, double* out_mean) {
   return invert_field<double>(to_grid(g), field, init, inv, out_max, out_mean);
 }

@@ -30,6 +30,17 @@ def smooth_field(fld, variance, spacing=None):
     return d_out.to_host()
 
 
+def bspline_field(fld, mesh, order=3, spacing=None):
+    """BSplineSyN's B-spline fit of a field; returns (fit, max |u/spacing|)."""
+    d_in = dev(fld)
+    d_out = L.DeviceArray(fld.shape)
+    g = desc_of(fld.shape[:-1], spacing)
+    m = (C.c_int * 3)(*[int(v) for v in list(mesh) + [1] * (3 - len(mesh))])
+    mx = C.c_double()
+    L.check(L.lib().ants_op_bspline_field(C.byref(g), d_in.ptr, d_out.ptr, m, int(order), C.byref(mx), None))
+    return d_out.to_host(), mx.value
+
+
 def warp_image(img, out_shape, u=None, gi=None, go=None):
     gi = gi or desc_of(img.shape)
     go = go or desc_of(out_shape)
