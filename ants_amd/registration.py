@@ -196,6 +196,42 @@ class SyNResult:
         return [f"1DIAGNOSTIC, {r.iteration:5d}, {r.energy:.12e}, {r.convergence:.12e}" for r in self.trace]
 
 
+def stage_params(stage: "SyNStage", dim: int, precision: str = "fp32", preprocess: bool = True,
+                 force_iterations: bool = False) -> L.SynParamsC:
+    """The C parameter block of a stage (ants_syn_params)."""
+    p = L.SynParamsC()
+    p.dim = dim
+    n = len(stage.shrink)
+    if n > L.MAX_LEVELS:
+        raise ValueError(f"at most {L.MAX_LEVELS} levels")
+    p.n_levels = n
+    for i in range(n):
+        p.shrink[i] = int(stage.shrink[i])
+        p.sigma[i] = float(stage.sigmas[i])
+        p.iterations[i] = int(stage.convergence.iterations[i])
+    p.sigma_in_mm = int(stage.sigmas_in_mm)
+    p.convergence_threshold = stage.convergence.threshold
+    p.convergence_window = stage.convergence.window
+    p.learning_rate = stage.transform.gradient_step
+    p.update_variance = stage.transform.update_variance
+    p.total_variance = stage.transform.total_variance
+    p.cc_radius = stage.metric.radius
+    if stage.transform.model == "BSplineSyN":
+        t = stage.transform
+        if len(t.update_mesh) != dim or len(t.total_mesh) != dim:
+            # itkantsRegistrationHelper.hxx:2239-2243
+            raise ValueError("the mesh size(s) don't match the image dimension")
+        p.model = L.MODEL_BSPLINE_SYN
+        for d in range(dim):
+            p.update_mesh[d] = int(stage.transform.update_mesh[d])
+            p.total_mesh[d] = int(stage.transform.total_mesh[d])
+        p.spline_order = int(stage.transform.spline_order)
+    p.precision = {"fp32": 0, "bf16": 1}[precision]
+    p.preprocess = int(preprocess)
+    p.force_iterations = int(force_iterations)
+    return p
+
+
 class SyNRegistration:
     """One SyN + CC stage (RegistrationHelper::AddSyNTransform + AddMetric +
     schedule setters), run on the GPU through the C-ABI."""
@@ -206,36 +242,7 @@ class SyNRegistration:
             raise ValueError("dim must be 2 or 3")
         self.stage = stage
         self.dim = dim
-        p = L.SynParamsC()
-        p.dim = dim
-        n = len(stage.shrink)
-        if n > L.MAX_LEVELS:
-            raise ValueError(f"at most {L.MAX_LEVELS} levels")
-        p.n_levels = n
-        for i in range(n):
-            p.shrink[i] = int(stage.shrink[i])
-            p.sigma[i] = float(stage.sigmas[i])
-            p.iterations[i] = int(stage.convergence.iterations[i])
-        p.sigma_in_mm = int(stage.sigmas_in_mm)
-        p.convergence_threshold = stage.convergence.threshold
-        p.convergence_window = stage.convergence.window
-        p.learning_rate = stage.transform.gradient_step
-        p.update_variance = stage.transform.update_variance
-        p.total_variance = stage.transform.total_variance
-        p.cc_radius = stage.metric.radius
-        if stage.transform.model == "BSplineSyN":
-            t = stage.transform
-            if len(t.update_mesh) != dim or len(t.total_mesh) != dim:
-                # itkantsRegistrationHelper.hxx:2239-2243
-                raise ValueError("the mesh size(s) don't match the image dimension")
-            p.model = L.MODEL_BSPLINE_SYN
-            for d in range(dim):
-                p.update_mesh[d] = int(stage.transform.update_mesh[d])
-                p.total_mesh[d] = int(stage.transform.total_mesh[d])
-            p.spline_order = int(stage.transform.spline_order)
-        p.precision = {"fp32": 0, "bf16": 1}[precision]
-        p.preprocess = int(preprocess)
-        p.force_iterations = int(force_iterations)
+        p = stage_params(stage, dim, precision, preprocess, force_iterations)
         self.params = p
         h = C.c_void_p()
         L.check(L.lib().ants_syn_create(C.byref(p), C.byref(h)))

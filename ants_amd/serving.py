@@ -15,7 +15,7 @@ from typing import Dict, List, Optional, Sequence
 import numpy as np
 
 from . import _lib as L
-from .registration import SyNStage
+from .registration import SyNStage, stage_params
 
 
 class BatchConfigC(C.Structure):
@@ -83,25 +83,6 @@ def _bind():
     return lib
 
 
-def stage_params(stage: SyNStage, dim: int, precision: str = "fp32", force_iterations: bool = False) -> L.SynParamsC:
-    p = L.SynParamsC()
-    p.dim = dim
-    p.n_levels = len(stage.shrink)
-    for i in range(p.n_levels):
-        p.shrink[i] = int(stage.shrink[i])
-        p.sigma[i] = float(stage.sigmas[i])
-        p.iterations[i] = int(stage.convergence.iterations[i])
-    p.sigma_in_mm = int(stage.sigmas_in_mm)
-    p.convergence_threshold = stage.convergence.threshold
-    p.convergence_window = stage.convergence.window
-    p.learning_rate = stage.transform.gradient_step
-    p.update_variance = stage.transform.update_variance
-    p.total_variance = stage.transform.total_variance
-    p.cc_radius = stage.metric.radius
-    p.precision = {"fp32": 0, "bf16": 1}[precision]
-    p.preprocess = 1
-    p.force_iterations = int(force_iterations)
-    return p
 
 
 @dataclass

@@ -14,7 +14,8 @@ launches stalls the second); `untimed_replay` adds the same registration
 without events, launched kernel by kernel (the library's default) and
 replayed from iteration graphs, measured after the timed region;
 `host_buffer_drop_in` the same registration through the host-buffer entry
-(PCIe transfers included).  The other half of the metric comes from two
+(PCIe transfers included); `bspline_syn` the same pair and schedule with the
+BSplineSyN field model (BSplineSyN[0.1,26,0,3]).  The other half of the metric comes from two
 bounded legs on the same GPU: `batch` (64 mixed-size pairs, batched versus
 one at a time) and `serve` (a Poisson request stream at a fixed QPS: p50/p99
 latency, pairs/s and capacity), summarised in `batch_summary` and
@@ -292,6 +293,43 @@ def bench_apply(args):
     print(json.dumps(line), flush=True)
 
 
+def bspline_leg(pair, gf, d_fixed, d_moving, d_fwd, d_inv, steps) -> dict:
+    """The same pair and schedule with the BSplineSyN field model as
+    antsRegistrationSyN.sh -t b sets it (BSplineSyN[0.1,26,0,3]: knot spacing
+    26 mm, no total-field fit): timed registrations, then one profiled run
+    for the B-spline fit passes (the smoothing class of this model)."""
+    from ants_amd import _lib as L
+    from ants_amd.registration import SyNRegistration, SyNStage
+    sched = dict(SCHEDULE, transform="BSplineSyN[0.1,26,0,3]")
+    stage = SyNStage.from_args(**sched, fixed_size=pair.size, fixed_spacing=pair.spacing)
+    reg = SyNRegistration(stage, 3)
+    reg.register_device(gf, d_fixed, gf, d_moving, d_fwd, d_inv)
+    ts = []
+    for _ in range(steps):
+        L.check(L.lib().ants_device_synchronize())
+        t0 = time.perf_counter()
+        reg.register_device(gf, d_fixed, gf, d_moving, d_fwd, d_inv)
+        L.check(L.lib().ants_device_synchronize())
+        ts.append(time.perf_counter() - t0)
+    st = reg.stats()
+    ms = sum(ts) / len(ts) * 1e3
+    reg.set_profiling(0x1F)
+    reg.register_device(gf, d_fixed, gf, d_moving, d_fwd, d_inv)
+    ks = reg.kernel_stats()
+    fit = ks["smooth"]
+    reg.close()
+    out = {"transform": sched["transform"], "update_mesh_first_level": stage.transform.update_mesh,
+           "ms_per_step": ms, "value": st["voxel_iterations"] / (ms * 1e-3), "unit": "voxel-iters/s",
+           "iterations_run": st["iterations"],
+           "bspline_fit": {"ms": fit["ms"], "launches": fit["launches"],
+                           "achieved_gbs": fit["bytes"] / max(fit["ms"], 1e-9) / 1e6,
+                           "note": "algorithmic bytes: each fitted field read and written once"},
+           "kernel_breakdown_ms": {k: round(v["ms"], 3) for k, v in ks.items()}}
+    log(f"BSplineSyN leg: {ms:.1f} ms per registration, iterations {st['iterations']}, "
+        f"B-spline fits {fit['ms']:.2f} ms in {fit['launches']} launches")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -461,6 +499,7 @@ def main():
         os.environ.pop("ANTS_SYN_GRAPHS", None)
     else:
         os.environ["ANTS_SYN_GRAPHS"] = saved_env
+    bsp = bspline_leg(pair, gf, d_fixed, d_moving, d_fwd, d_inv, args.steps)
     avg_ms = kacc["ms"] / max(kacc["launches"], 1)
     bytes_per_launch = kacc["bytes"] / max(kacc["launches"], 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
@@ -513,6 +552,7 @@ def main():
         "per_level_kernels": per_level,
         "host_buffer_drop_in": host_drop_in,
         "inverse_consistency": ic,
+        "bspline_syn": bsp,
     }
     if legs:
         sv = legs["serve"]["continuous_batching"]

@@ -253,3 +253,21 @@ def test_device_memory_stays_within_budget():
         load += d
         peak = max(peak, load)
     assert peak <= budget
+
+
+def test_bspline_syn_requests_in_the_mix():
+    """BSplineSyN requests (update and total meshes) beside SyN ones: each
+    result equals the same registration run alone."""
+    bsp = SyNStage.from_args("BSplineSyN[0.1,6x6x6,3x3x3,3]", "CC[f,m,1,2]", "[6x4x2,1e-6,10]", "4x2x1",
+                             "2x1x0vox")
+    reqs = make_requests(4, seed=21)
+    for k in (0, 3):
+        reqs[k].stage = bsp
+    b = Batch(slots=3)
+    res = run_all(b, reqs)
+    b.close()
+    assert all(r.status == 0 for r in res), [r.error for r in res]
+    for r in res:
+        ref = alone(r.request)
+        np.testing.assert_array_equal(r.fwd, ref.fwd)
+        np.testing.assert_array_equal(r.inv, ref.inv)

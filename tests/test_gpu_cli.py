@@ -107,3 +107,29 @@ def test_registration_with_initial_fixed_transform(tmp_path):
     res = reg.register(pair.fixed, pair.moving, g.spacing, g.origin, g.spacing, g.origin)
     reg.close()
     np.testing.assert_array_equal(fwd, res.fwd)
+
+
+def test_registration_bspline_syn_knot_spacing(tmp_path):
+    """`-t BSplineSyN[0.1,26,0,3]` (antsRegistrationSyN.sh -t b): the knot
+    spacing becomes a mesh with the fixed image's geometry; the warp files
+    equal the library run with that mesh."""
+    pair = ph.phantom_pair_3d(4, size=(80, 88, 72), spacing=(1.0, 1.2, 1.0), max_disp_mm=4.0, svf_sigma_mm=8.0)
+    g = Geometry(pair.size, pair.spacing, (0.0, 0.0, 0.0), np.eye(3))
+    f, m = str(tmp_path / "fixed.nii.gz"), str(tmp_path / "moving.nii.gz")
+    aio.write_nifti(f, pair.fixed, g)
+    aio.write_nifti(m, pair.moving, g)
+    prefix = str(tmp_path / "b_")
+    args = ["-d", "3", "-m", f"CC[{f},{m},1,2]", "-t", "BSplineSyN[0.1,26,0,3]", "-c", "[8x4x2,1e-6,10]",
+            "-f", "4x2x1", "-s", "2x1x0vox", "-o", prefix]
+    assert cli.main(["registration"] + args) == 0
+    fwd, _ = aio.read_nifti(prefix + "0Warp.nii.gz")
+    _, gr = aio.read_nifti(f)           # the geometry as stored (float32 header fields)
+    mesh = [int(np.ceil((n - 1) * s / 26)) for n, s in zip(gr.size, gr.spacing)]
+    st = SyNStage.from_args(f"BSplineSyN[0.1,{'x'.join(map(str, mesh))},0,3]", "CC[f,m,1,2]", "[8x4x2,1e-6,10]",
+                            "4x2x1", "2x1x0vox")
+    reg = SyNRegistration(st, 3)
+    res = reg.register(pair.fixed, pair.moving, gr.spacing, gr.origin, gr.spacing, gr.origin, gr.direction,
+                       gr.direction)
+    reg.close()
+    np.testing.assert_array_equal(fwd, res.fwd)
+    assert np.abs(fwd).max() > 0.05
