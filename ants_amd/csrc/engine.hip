@@ -279,7 +279,7 @@ int validate_params(const ants_syn_params* p) {
       if (p->total_mesh[d] < 0) return fail(ANTS_ERR_INVALID, "BSplineSyN total mesh sizes must be >= 0");
       zeros += p->total_mesh[d] == 0;
       // the helper doubles the mesh at every level
-      const int64_t lim = int64_t{1} << 20;
+      const int64_t lim = int64_t{1} << 16;
       if ((static_cast<int64_t>(p->update_mesh[d]) << (p->n_levels - 1)) > lim ||
           (static_cast<int64_t>(p->total_mesh[d]) << (p->n_levels - 1)) > lim)
         return fail(ANTS_ERR_INVALID, "BSplineSyN mesh sizes too large");
@@ -600,7 +600,7 @@ struct ants_syn_handle {
   // BSplineSyN: fit tables of every level (update and total fields; host
   // copy kept for the asynchronous upload) and the lattices of the update
   // fits (front end) and total fits (back end), two fields each
-  DevBuf bsp_tab, bsp_upd, bsp_tot;
+  DevBuf bsp_tab, bsp_upd, bsp_tot, bsp_part;  // bsp_part: max-norm partials (front end, back end)
   std::vector<unsigned char> bsp_host;
   DevBuf stats, partials_cc, partials_inv, masks_inv, minmax;
   IterStats* host_stats = nullptr;  // pinned
@@ -661,7 +661,7 @@ struct ants_syn_handle {
     const DevBuf* all[] = {&F,      &M,           &Fs,          &Ms,        &tmp,    &Fw,          &Mw,
                            &Fb,     &Mb,          &stats,       &partials_cc, &partials_inv, &masks_inv,
                            &minmax, &restore_buf, &io_f,        &io_m,      &io_fwd, &io_inv, &blend,
-                           &bsp_tab, &bsp_upd,    &bsp_tot};
+                           &bsp_tab, &bsp_upd,    &bsp_tot,     &bsp_part};
     int64_t b = 0;
     for (const DevBuf* x : all) b += static_cast<int64_t>(x->bytes);
     for (const auto& x : fld) b += static_cast<int64_t>(x.bytes);
@@ -674,7 +674,7 @@ struct ants_syn_handle {
     graph_signature.clear();
     DevBuf* all[] = {&F,     &M,           &Fs,          &Ms,        &tmp,    &Fw,   &Mw,   &Fb,     &Mb,
                      &stats, &partials_cc, &partials_inv, &masks_inv, &minmax, &io_f, &io_m, &io_fwd, &io_inv,
-                     &blend, &bsp_tab, &bsp_upd, &bsp_tot};
+                     &blend, &bsp_tab, &bsp_upd, &bsp_tot, &bsp_part};
     for (DevBuf* b : all) b->release();
     for (auto& b : fld) b.release();
     have_state = false;
@@ -908,10 +908,11 @@ double bsp_end_eps(int max_spans) {
 }
 
 // Layout of one fit's tables inside a table blob (byte offsets per axis:
-// span, bw, fw, lo, hi, A, Ai) and the sizes of its passes.
+// span, bw, lo, goff, gw, A, Ai) and the sizes of its passes.
 struct BspLayout {
   int D = 0, q = 3;
   int n[3] = {1, 1, 1}, m[3] = {1, 1, 1};
+  int gtot[3] = {0, 0, 0};
   size_t off[3][7] = {};
   // floats per field of the intermediate lattices (x-fitted, xy-fitted, full)
   int64_t field_floats() const {
@@ -945,9 +946,9 @@ int bsp_tables(const HostGrid& g, const int mesh[3], int q, std::vector<unsigned
     const int m = S + q;
     L.n[a] = static_cast<int>(n);
     L.m[a] = m;
-    std::vector<int> span(n), lo(m), hi(m);
-    std::vector<float> bw(4 * n, 0.f), fw(4 * n, 0.f);
-    std::vector<double> bd(4 * n, 0.0), A(m, 0.0), Ai(m, 0.0);
+    std::vector<int> span(n), lo(m), goff(m + 1, 0);
+    std::vector<float> bw(4 * n, 0.f), gw;
+    std::vector<double> bd(4 * n, 0.0), fd(4 * n, 0.0), A(m, 0.0), Ai(m, 0.0);
     for (int64_t i = 0; i < n; ++i) {
       double p = static_cast<double>(i) * S / static_cast<double>(n - 1);
       if (std::fabs(p - S) <= eps) p = S - eps;
@@ -963,30 +964,39 @@ int bsp_tables(const HostGrid& g, const int mesh[3], int q, std::vector<unsigned
       for (int j = 0; j <= q; ++j) {
         const double b = bd[4 * i + j];
         bw[4 * i + j] = static_cast<float>(b);
-        fw[4 * i + j] = interior ? static_cast<float>(b * b * b / sd) : 0.f;
+        fd[4 * i + j] = interior ? b * b * b / sd : 0.0;
       }
     }
+    // control point k: voxels whose span lies in [k - q, k] (contiguous:
+    // spans do not decrease along the axis)
     for (int k = 0; k < m; ++k) {
       int l0 = 0;
       while (l0 < n && span[l0] < k - q) ++l0;
       int h0 = static_cast<int>(n) - 1;
       while (h0 >= 0 && span[h0] > k) --h0;
       lo[k] = l0;
-      hi[k] = h0;
+      goff[k] = static_cast<int>(gw.size());
       for (int i = l0; i <= h0; ++i) {
         const double b = bd[4 * i + (k - span[i])];
+        gw.push_back(static_cast<float>(fd[4 * i + (k - span[i])]));
         A[k] += b * b;
         if (i > 0 && i < n - 1) Ai[k] += b * b;
       }
     }
+    goff[m] = static_cast<int>(gw.size());
+    L.gtot[a] = goff[m];
     L.off[a][0] = blob_put(blob, span);
     L.off[a][1] = blob_put(blob, bw);
-    L.off[a][2] = blob_put(blob, fw);
-    L.off[a][3] = blob_put(blob, lo);
-    L.off[a][4] = blob_put(blob, hi);
+    L.off[a][2] = blob_put(blob, lo);
+    L.off[a][3] = blob_put(blob, goff);
+    L.off[a][4] = blob_put(blob, gw);
     L.off[a][5] = blob_put(blob, A);
     L.off[a][6] = blob_put(blob, Ai);
   }
+  // the passes index lattice planes and voxels with 32-bit integers
+  if (static_cast<int64_t>(L.m[0]) * L.m[1] * L.D >= (int64_t{1} << 31) ||
+      static_cast<int64_t>(L.n[0]) * L.n[1] * L.n[2] >= (int64_t{1} << 31))
+    return fail(ANTS_ERR_UNSUPPORTED, "B-spline lattice too large");
   *out = L;
   return ANTS_OK;
 }
@@ -1000,13 +1010,14 @@ BspDev bsp_dev(const BspLayout& L, const void* dbase) {
     BspAxisDev& X = T.ax[a];
     X.span = reinterpret_cast<const int*>(b + L.off[a][0]);
     X.bw = reinterpret_cast<const float*>(b + L.off[a][1]);
-    X.fw = reinterpret_cast<const float*>(b + L.off[a][2]);
-    X.lo = reinterpret_cast<const int*>(b + L.off[a][3]);
-    X.hi = reinterpret_cast<const int*>(b + L.off[a][4]);
+    X.lo = reinterpret_cast<const int*>(b + L.off[a][2]);
+    X.goff = reinterpret_cast<const int*>(b + L.off[a][3]);
+    X.gw = reinterpret_cast<const float*>(b + L.off[a][4]);
     X.A = reinterpret_cast<const double*>(b + L.off[a][5]);
     X.Ai = reinterpret_cast<const double*>(b + L.off[a][6]);
     X.n = L.n[a];
     X.m = L.m[a];
+    X.gtot = L.gtot[a];
   }
   return T;
 }
@@ -1016,7 +1027,7 @@ BspDev bsp_dev(const BspLayout& L, const void* dbase) {
 // writes `out` and each field's max |u / spacing| into maxbits[k].  buf[k]
 // holds field_floats() floats of lattices for field k.  in and out may not alias.
 int bsp_fields_on(hipStream_t s, const HostGrid& g, const BspDev& T, const BspLayout& L, int np,
-                  const float* const* in, float* const* out, float* const* buf, unsigned int* maxbits,
+                  const float* const* in, float* const* out, float* const* buf, unsigned int* maxbits, float* part,
                   Profiler* prof = nullptr) {
   const int D = g.D;
   const int n0 = L.n[0], n1 = L.n[1], n2 = D == 3 ? L.n[2] : 1;
@@ -1045,17 +1056,17 @@ int bsp_fields_on(hipStream_t s, const HostGrid& g, const BspDev& T, const BspLa
     const float* cl[4];
     for (int k = 0; k < np; ++k) cl[k] = lat[k];
     const int d_l[3] = {m0, m1, m2};
-    launch_bsp_eval_axis(T, np, cl, t2, d_l, 2, sp, nullptr, s);  // (m0, m1, n2)
-    launch_bsp_eval_axis(T, np, c2, t1, d_xy, 1, sp, nullptr, s);  // (m0, n1, n2)
+    launch_bsp_eval_axis(T, np, cl, t2, d_l, 2, sp, nullptr, nullptr, s);  // (m0, m1, n2)
+    launch_bsp_eval_axis(T, np, c2, t1, d_xy, 1, sp, nullptr, nullptr, s);  // (m0, n1, n2)
   } else {
     launch_bsp_fit_axis(T, np, c1, lat, d_x, 1, true, s);
     const float* cl[4];
     for (int k = 0; k < np; ++k) cl[k] = lat[k];
     const int d_l[3] = {m0, m1, 1};
-    launch_bsp_eval_axis(T, np, cl, t1, d_l, 1, sp, nullptr, s);  // (m0, n1)
+    launch_bsp_eval_axis(T, np, cl, t1, d_l, 1, sp, nullptr, nullptr, s);  // (m0, n1)
   }
-  launch_bsp_eval_axis(T, np, c1, out, d_x, 0, sp, maxbits, s);
-  if (prof) prof->end(ANTS_KC_SMOOTH, np * 2.0 * D * 4.0 * static_cast<double>(g.count()), D == 3 ? 6 : 4);
+  launch_bsp_eval_axis(T, np, c1, out, d_x, 0, sp, maxbits, part, s);
+  if (prof) prof->end(ANTS_KC_SMOOTH, np * 2.0 * D * 4.0 * static_cast<double>(g.count()), D == 3 ? 7 : 5);
   HIP_TRY(hipGetLastError());
   return ANTS_OK;
 }
@@ -1123,6 +1134,7 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
     HIP_TRY(hipMemcpyAsync(bsp_tab.p, bsp_host.data(), bsp_host.size(), hipMemcpyHostToDevice, stream));
     ANTS_TRY(bsp_upd.ensure(2 * bsp_fu * sizeof(float)));
     if (bsp_total) ANTS_TRY(bsp_tot.ensure(2 * bsp_ft * sizeof(float)));
+    ANTS_TRY(bsp_part.ensure(4 * bsp_partials_needed() * sizeof(float)));
   }
   ANTS_TRY(stats.ensure(2 * sizeof(IterStats)));  // two iteration parities
   int64_t ccp = 0, invp = 0, invw = 0;
@@ -1147,7 +1159,7 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
     std::vector<const void*> sig = {F.p, M.p, Fs.p, Ms.p, Fw.p, Mw.p, Fb.p, Mb.p, stats.p, partials_cc.p,
                                     partials_inv.p, host_stats, host_flag, static_cast<const void*>(s)};
     for (auto& b : fld) sig.push_back(b.p);
-    for (const DevBuf* b : {&bsp_tab, &bsp_upd, &bsp_tot}) sig.push_back(b->p);
+    for (const DevBuf* b : {&bsp_tab, &bsp_upd, &bsp_tot, &bsp_part}) sig.push_back(b->p);
     if (sig != graph_signature || graphs.size() > 512) {
       clear_graphs();
       graph_signature = sig;
@@ -1460,7 +1472,8 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
       if (bsp) {
         // BSplineSyN: the update fields are B-spline fits (into wk[2], wk[3])
         float* lat[2] = {bsp_upd.f(), bsp_upd.f() + bsp_fu};
-        ANTS_TRY(bsp_fields_on(fs, g, bsp_u_dev, bsp_lu[l], 2, in, work, lat, stats_of(par)->maxnorm, &prof));
+        ANTS_TRY(bsp_fields_on(fs, g, bsp_u_dev, bsp_lu[l], 2, in, work, lat, stats_of(par)->maxnorm, bsp_part.f(),
+                               &prof));
         std::swap(wk_[0], wk_[2]);
         std::swap(wk_[1], wk_[3]);
         return ANTS_OK;
@@ -1499,7 +1512,8 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
         const float* in[2] = {wk_[2], wk_[3]};
         float* out[2] = {wk_[0], wk_[1]};
         float* lat[2] = {bsp_tot.f(), bsp_tot.f() + bsp_ft};
-        ANTS_TRY(bsp_fields_on(s, g, bsp_t_dev, bsp_lt[l], 2, in, out, lat, stats_of(par)->maxnorm + 2, &prof));
+        ANTS_TRY(bsp_fields_on(s, g, bsp_t_dev, bsp_lt[l], 2, in, out, lat, stats_of(par)->maxnorm + 2,
+                               bsp_part.f() + 2 * bsp_partials_needed(), &prof));
         std::swap(wk_[0], wk_[2]);
         std::swap(wk_[1], wk_[3]);
       } else if (!bsp && P.total_variance > 0.0) {
@@ -2088,16 +2102,17 @@ int ants_op_bspline_field(const ants_image_desc* gdsc, const float* d_in, float*
   if (order < 1 || order > 3) return fail(ANTS_ERR_UNSUPPORTED, "B-spline order must be 1..3");
   int m3[3] = {1, 1, 1};
   for (int d = 0; d < g.D; ++d) {
-    if (mesh[d] < 1 || mesh[d] > (1 << 20)) return fail(ANTS_ERR_INVALID, "B-spline mesh sizes must be in 1..2^20");
+    if (mesh[d] < 1 || mesh[d] > (1 << 16)) return fail(ANTS_ERR_INVALID, "B-spline mesh sizes must be in 1..65536");
     m3[d] = mesh[d];
   }
   std::vector<unsigned char> blob;
   BspLayout L;
   ANTS_TRY(bsp_tables(g, m3, order, blob, &L));
-  DevBuf tab, lat, m;
+  DevBuf tab, lat, m, part;
   ANTS_TRY(tab.ensure(blob.size()));
   ANTS_TRY(lat.ensure(static_cast<size_t>(L.field_floats()) * sizeof(float)));
   ANTS_TRY(m.ensure(16));
+  ANTS_TRY(part.ensure(bsp_partials_needed() * sizeof(float)));
   hipStream_t s = static_cast<hipStream_t>(stream);
   HIP_TRY(hipMemcpyAsync(tab.p, blob.data(), blob.size(), hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemsetAsync(m.p, 0, 16, s));
@@ -2105,7 +2120,7 @@ int ants_op_bspline_field(const ants_image_desc* gdsc, const float* d_in, float*
   const float* in[1] = {d_in};
   float* out[1] = {d_out};
   float* buf[1] = {lat.f()};
-  ANTS_TRY(bsp_fields_on(s, g, T, L, 1, in, out, buf, static_cast<unsigned int*>(m.p)));
+  ANTS_TRY(bsp_fields_on(s, g, T, L, 1, in, out, buf, static_cast<unsigned int*>(m.p), part.f()));
   unsigned int bits = 0;
   HIP_TRY(hipMemcpyAsync(&bits, m.p, 4, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
@@ -2422,7 +2437,7 @@ int load_tuning_file(const char* path) {
   FILE* f = std::fopen(path, "r");
   if (!f) return fail(ANTS_ERR_INVALID, "cannot open tuning file %s", path);
   char line[256];
-  int n = 0, lineno = 0;
+  int lineno = 0;
   while (std::fgets(line, sizeof(line), f)) {
     ++lineno;
     char* hash = std::strchr(line, '#');
@@ -2436,7 +2451,6 @@ int load_tuning_file(const char* path) {
       std::fclose(f);
       return fail(ANTS_ERR_INVALID, "%s:%d: bad tuning entry", path, lineno);
     }
-    ++n;
   }
   std::fclose(f);
   return ANTS_OK;
@@ -2633,7 +2647,7 @@ int64_t footprint_bytes(const ants_syn_params& P, const HostGrid& gF, const Host
         ft = std::max(ft, (mt[0] * nn[1] * nn[2] + mt[0] * mt[1] * nn[2] + mt[0] * mt[1] * mt[2]) * D);
       b += (nn[0] + nn[1] + nn[2]) * 2 * 40 + (mu[0] + mu[1] + mu[2] + mt[0] + mt[1] + mt[2]) * 24;
     }
-    b += 2 * (fu + ft) * 4;
+    b += 2 * (fu + ft) * 4 + 4 * bsp_partials_needed() * 4;  // lattices, max-norm partials
   }
   b += 4 * ((maxN + 63) / 64) * 8;       // inversion candidate masks
   b += (NF + NM) * 4 + NM * 4;           // request inputs (images, labels)

@@ -12,6 +12,7 @@
 #include <cstdint>
 #include <type_traits>
 #include <utility>
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -1992,100 +1993,169 @@ __global__ void __launch_bounds__(kBlock) k_apply_transforms(GridDev go, ApplyDe
 #pragma clang fp contract(on)
 
 // ---- B-spline fit of displacement fields (BSplineSyN) ----
-// One thread per output element (all D components).  Fit along `axis`:
-// control point k gathers the voxels of its range [lo_k, hi_k] with weights
-// fw[i][k - span_i]; consecutive threads take consecutive x, so the y and z
-// passes read rows that are contiguous across the wave and the x pass reads
-// overlapping short runs through the cache.
-template <int D>
-__global__ void __launch_bounds__(kBlock) k_bsp_fit_axis(BspAxisDev A, BspDev T, PtrPack4 in, MutPack4 out, int n0,
-                                                         int n1, int n2, int axis, int last) {
-  const float* __restrict__ src = in.p[blockIdx.y];
-  float* __restrict__ dst = out.p[blockIdx.y];
-  const int o0 = axis == 0 ? A.m : n0, o1 = axis == 1 ? A.m : n1, o2 = axis == 2 ? A.m : n2;
-  const int64_t NO = static_cast<int64_t>(o0) * o1 * o2;
-  const int64_t stride = axis == 0 ? 1 : (axis == 1 ? n0 : static_cast<int64_t>(n0) * n1);
-  for (int64_t idx = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; idx < NO;
-       idx += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-    const int x = static_cast<int>(idx % o0);
-    const int64_t r = idx / o0;
-    const int y = static_cast<int>(r % o1);
-    const int z = static_cast<int>(r / o1);
-    const int k = axis == 0 ? x : (axis == 1 ? y : z);
-    const int64_t base = (static_cast<int64_t>(axis == 2 ? 0 : z) * n1 + (axis == 1 ? 0 : y)) * n0 + (axis == 0 ? 0 : x);
-    float acc[D];
-#pragma unroll
-    for (int c = 0; c < D; ++c) acc[c] = 0.f;
-    const int lo = A.lo[k], hi = A.hi[k];
-    for (int i = lo; i <= hi; ++i) {
-      const float w = A.fw[i * 4 + (k - A.span[i])];
-      const float* v = src + (base + i * stride) * D;
-#pragma unroll
-      for (int c = 0; c < D; ++c) acc[c] = fmaf(w, v[c], acc[c]);
+// A pass along one axis views its input as [outer O][axis Lin][inner I] and
+// its output as [O][Lout][I] (I counts floats: the components and the axes
+// before this one).  Fit: control point k gathers the axis positions of its
+// range [lo_k, hi_k] with weights fw[p][k - span_p]; evaluation: position v
+// sums the control points span_v .. span_v + q with weights bw[v][j].
+
+constexpr int kBspKPT = 4;  // output positions per thread of a strided pass
+#ifndef ANTS_BSP_EVAL_BLOCKS
+#define ANTS_BSP_EVAL_BLOCKS 8192
+#endif
+constexpr int kBspEvalBlocks = ANTS_BSP_EVAL_BLOCKS;  // blocks per field of the x evaluation
+
+// Passes along y or z (and x when a row does not fit in LDS): 64 lanes over
+// the contiguous inner index, 4 x kBspKPT output positions per block, blockIdx.z =
+// field * O + outer.  `last` (fit): the pass that completes the lattice
+// divides by W_k; i = (ky m0 + kx) D + c (3-D) or kx D + c (2-D).
+template <int D, bool FIT>
+__global__ void __launch_bounds__(256) k_bsp_strided(BspAxisDev A, BspDev T, PtrPack4 in, MutPack4 out, int I, int Lin,
+                                                     int Lout, int Oc, int o_base, int m0, int last) {
+  const int f = blockIdx.z / Oc;
+  const int o = o_base + static_cast<int>(blockIdx.z - f * Oc);
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  if (i >= I) return;
+  const float* __restrict__ src = in.p[f] + static_cast<int64_t>(o) * Lin * I + i;
+  float* __restrict__ dst = out.p[f] + static_cast<int64_t>(o) * Lout * I + i;
+  const int kb = (blockIdx.y * 4 + threadIdx.y) * kBspKPT;
+  for (int k = kb; k < kb + kBspKPT && k < Lout; ++k) {
+  float acc = 0.f;
+  if (FIT) {
+    const int g0 = A.goff[k], len = A.goff[k + 1] - g0;
+    const float* sp = src + static_cast<int64_t>(A.lo[k]) * I;
+    const float* w = A.gw + g0;
+    int j = 0;
+    for (; j + 4 <= len; j += 4) {  // four independent loads in flight
+      const float v0 = sp[static_cast<int64_t>(j) * I], v1 = sp[static_cast<int64_t>(j + 1) * I];
+      const float v2 = sp[static_cast<int64_t>(j + 2) * I], v3 = sp[static_cast<int64_t>(j + 3) * I];
+      acc = fmaf(w[j], v0, acc);
+      acc = fmaf(w[j + 1], v1, acc);
+      acc = fmaf(w[j + 2], v2, acc);
+      acc = fmaf(w[j + 3], v3, acc);
     }
+    for (; j < len; ++j) acc = fmaf(w[j], sp[static_cast<int64_t>(j) * I], acc);
     if (last) {
-      // every axis is a control-point axis now: W_k from the 1-D sums, in double
-      const int kk[3] = {x, y, z};
+      const int kxy = i / D;
+      const int kk[3] = {D == 3 ? kxy % m0 : kxy, D == 3 ? kxy / m0 : k, k};
       double pa = 1.0, pi = 1.0;
-      for (int d = 0; d < T.D; ++d) {
+      for (int d = 0; d < D; ++d) {
         pa *= T.ax[d].A[kk[d]];
         pi *= T.ax[d].Ai[kk[d]];
       }
       const double W = pa + (1.0e10 - 1.0) * (pa - pi);
-#pragma unroll
-      for (int c = 0; c < D; ++c) acc[c] = W != 0.0 ? static_cast<float>(static_cast<double>(acc[c]) / W) : 0.f;
+      acc = W != 0.0 ? static_cast<float>(static_cast<double>(acc) / W) : 0.f;
     }
-#pragma unroll
-    for (int c = 0; c < D; ++c) dst[idx * D + c] = acc[c];
+  } else {
+    const float* s0 = src + static_cast<int64_t>(A.span[k]) * I;
+    for (int j = 0; j <= T.q; ++j) acc = fmaf(A.bw[k * 4 + j], s0[static_cast<int64_t>(j) * I], acc);
+  }
+  dst[static_cast<int64_t>(k) * I] = acc;
   }
 }
 
-// Evaluation along `axis`: voxel i sums control points span_i .. span_i + q
-// with the basis values bw[i][j].  The final pass (maxbits set) also takes
-// the max |u / spacing| of its field.
+// Fit along x: a block stages the axis's fit weights and R consecutive rows
+// (n0 voxels x D components each, contiguous in memory) in LDS, the rows with
+// eight independent loads in flight per thread, then computes the m0 x D
+// control-point values of each row from LDS.  LDS: gtot weights, 2 m0 + 1
+// ints, R n0 D floats.
 template <int D>
-__global__ void __launch_bounds__(kBlock) k_bsp_eval_axis(BspAxisDev A, PtrPack4 in, MutPack4 out, int m0, int m1,
-                                                          int m2, int axis, int q, float sp0, float sp1, float sp2,
-                                                          unsigned int* maxbits) {
+__global__ void __launch_bounds__(256) k_bsp_fit_x(BspAxisDev A, PtrPack4 in, MutPack4 out, int rows, int R) {
+  extern __shared__ float lds[];
+  const int n0 = A.n, m0 = A.m;
+  float* wl = lds;
+  int* lol = reinterpret_cast<int*>(lds + A.gtot);
+  int* gol = lol + m0;
+  float* rowbuf = reinterpret_cast<float*>(gol + m0 + 1);
+  const int row0 = blockIdx.x * R;
+  const int nr = min(R, rows - row0);
+  const float* __restrict__ src = in.p[blockIdx.y] + static_cast<int64_t>(row0) * n0 * D;
+  float* __restrict__ dst = out.p[blockIdx.y] + static_cast<int64_t>(row0) * m0 * D;
+  const int ne = nr * n0 * D;
+  for (int e0 = threadIdx.x; e0 < ne; e0 += 256 * 8) {
+    float v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = e0 + u * 256;
+      v[u] = e < ne ? src[e] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = e0 + u * 256;
+      if (e < ne) rowbuf[e] = v[u];
+    }
+  }
+  for (int e = threadIdx.x; e < A.gtot; e += 256) wl[e] = A.gw[e];
+  for (int e = threadIdx.x; e < m0; e += 256) lol[e] = A.lo[e];
+  for (int e = threadIdx.x; e <= m0; e += 256) gol[e] = A.goff[e];
+  __syncthreads();
+  const int per_row = m0 * D;
+  for (int t = threadIdx.x; t < nr * per_row; t += 256) {
+    const int r = t / per_row;
+    const int kc = t - r * per_row;
+    const int k = kc / D;
+    const int c = kc - k * D;
+    const float* lr = rowbuf + (r * n0 + lol[k]) * D + c;
+    const int g0 = gol[k], len = gol[k + 1] - g0;
+    float acc = 0.f;
+    for (int j = 0; j < len; ++j) acc = fmaf(wl[g0 + j], lr[j * D], acc);
+    dst[t] = acc;
+  }
+}
+
+// Evaluation along x (the last pass): one voxel per thread (consecutive
+// lanes on consecutive voxels of a row), grid-stride over the voxels with
+// enough blocks to keep the dependent table -> control-point loads in
+// flight.  Each block stores its max |u / spacing| in part[field][block]
+// (when part is set); k_max_partials folds them.  Same-address device
+// atomics from every block serialise at the memory side (about 11 ns each
+// measured here), so the blocks write plain partials instead.
+template <int D>
+__global__ void __launch_bounds__(256) k_bsp_eval_x(BspAxisDev A, PtrPack4 in, MutPack4 out, int nvox, int q,
+                                                    float sp0, float sp1, float sp2, float* part) {
+  const int n0 = A.n, m0 = A.m;
+  const float isp[3] = {1.f / sp0, 1.f / sp1, 1.f / sp2};
   const float* __restrict__ src = in.p[blockIdx.y];
   float* __restrict__ dst = out.p[blockIdx.y];
-  const int o0 = axis == 0 ? A.n : m0, o1 = axis == 1 ? A.n : m1, o2 = axis == 2 ? A.n : m2;
-  const int64_t NO = static_cast<int64_t>(o0) * o1 * o2;
-  const int64_t stride = axis == 0 ? 1 : (axis == 1 ? m0 : static_cast<int64_t>(m0) * m1);
-  const float isp[3] = {1.f / sp0, 1.f / sp1, 1.f / sp2};
   float nrm = 0.f;
-  for (int64_t idx = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; idx < NO;
-       idx += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-    const int x = static_cast<int>(idx % o0);
-    const int64_t r = idx / o0;
-    const int y = static_cast<int>(r % o1);
-    const int z = static_cast<int>(r / o1);
-    const int i = axis == 0 ? x : (axis == 1 ? y : z);
-    const int64_t base = (static_cast<int64_t>(axis == 2 ? 0 : z) * m1 + (axis == 1 ? 0 : y)) * m0 + (axis == 0 ? 0 : x);
-    const int s0 = A.span[i];
+  for (int v = blockIdx.x * 256 + threadIdx.x; v < nvox; v += gridDim.x * 256) {
+    const int row = v / n0;
+    const int x = v - row * n0;
+    const float4 w = *reinterpret_cast<const float4*>(A.bw + 4 * x);
+    const float* s0 = src + (static_cast<int64_t>(row) * m0 + A.span[x]) * D;
+    const float wj[4] = {w.x, w.y, w.z, w.w};
     float acc[D];
 #pragma unroll
-    for (int c = 0; c < D; ++c) acc[c] = 0.f;
-    for (int j = 0; j <= q; ++j) {
-      const float w = A.bw[i * 4 + j];
-      const float* v = src + (base + static_cast<int64_t>(s0 + j) * stride) * D;
+    for (int c = 0; c < D; ++c) acc[c] = wj[0] * s0[c];
 #pragma unroll
-      for (int c = 0; c < D; ++c) acc[c] = fmaf(w, v[c], acc[c]);
-    }
+    for (int j = 1; j < 4; ++j)
+      if (j <= q) {
+#pragma unroll
+        for (int c = 0; c < D; ++c) acc[c] = fmaf(wj[j], s0[j * D + c], acc[c]);
+      }
     float s2 = 0.f;
 #pragma unroll
     for (int c = 0; c < D; ++c) {
-      dst[idx * D + c] = acc[c];
+      dst[static_cast<int64_t>(v) * D + c] = acc[c];
       const float t = acc[c] * isp[c];
       s2 += t * t;
     }
     nrm = fmaxf(nrm, sqrtf(s2));
   }
-  if (maxbits) {
+  if (part) {
     double dummy = 0.0;
-    block_sum_max<kBlock>(dummy, nrm);
-    if (threadIdx.x == 0) atomicMax(&maxbits[blockIdx.y], __float_as_uint(nrm));
+    block_sum_max<256>(dummy, nrm);
+    if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = nrm;
   }
+}
+
+// One block per field: the max of nb partials into maxbits[field].
+__global__ void __launch_bounds__(1024) k_max_partials(const float* __restrict__ part, int nb, unsigned int* maxbits) {
+  float m = 0.f;
+  for (int i = threadIdx.x; i < nb; i += 1024) m = fmaxf(m, part[blockIdx.x * nb + i]);
+  double dummy = 0.0;
+  block_sum_max<1024>(dummy, m);
+  if (threadIdx.x == 0) atomicMax(&maxbits[blockIdx.x], __float_as_uint(m));
 }
 
 inline unsigned int blocks_for(int64_t n, int bs = kBlock) { return static_cast<unsigned int>((n + bs - 1) / bs); }
@@ -2496,35 +2566,84 @@ void launch_publish_stats(const Partial* cc_parts, int cc_n, RedSlot* energy, co
                      flag);
 }
 
+// Strided pass along `axis` (any axis): see k_bsp_strided.
+static void bsp_strided(const BspDev& T, int np, const float* const* in, float* const* out, const int dims[3],
+                        int axis, bool fit, bool last, hipStream_t s) {
+  const BspAxisDev& A = T.ax[axis];
+  int64_t I = T.D, O = 1;
+  for (int d = 0; d < axis; ++d) I *= dims[d];
+  for (int d = axis + 1; d < T.D; ++d) O *= dims[d];
+  const int Lin = dims[axis], Lout = fit ? A.m : A.n;
+  const dim3 block(64, 4);
+  const int m0 = T.ax[0].m;
+  // blockIdx.z covers (field, outer index) pairs: outer chunks within the grid limit
+  const int64_t chunk = std::max<int64_t>(1, 65535 / np);
+  for (int64_t ob = 0; ob < O; ob += chunk) {
+    const int oc = static_cast<int>(std::min<int64_t>(chunk, O - ob));
+    const dim3 grid(static_cast<unsigned int>((I + 63) / 64),
+                    static_cast<unsigned int>((Lout + 4 * kBspKPT - 1) / (4 * kBspKPT)),
+                    static_cast<unsigned int>(oc * np));
+    const int lst = static_cast<int>(last);
+    if (T.D == 2) {
+      if (fit)
+        hipLaunchKernelGGL((k_bsp_strided<2, true>), grid, block, 0, s, A, T, pack(in, np), mpack(out, np),
+                           static_cast<int>(I), Lin, Lout, oc, static_cast<int>(ob), m0, lst);
+      else
+        hipLaunchKernelGGL((k_bsp_strided<2, false>), grid, block, 0, s, A, T, pack(in, np), mpack(out, np),
+                           static_cast<int>(I), Lin, Lout, oc, static_cast<int>(ob), m0, 0);
+    } else {
+      if (fit)
+        hipLaunchKernelGGL((k_bsp_strided<3, true>), grid, block, 0, s, A, T, pack(in, np), mpack(out, np),
+                           static_cast<int>(I), Lin, Lout, oc, static_cast<int>(ob), m0, lst);
+      else
+        hipLaunchKernelGGL((k_bsp_strided<3, false>), grid, block, 0, s, A, T, pack(in, np), mpack(out, np),
+                           static_cast<int>(I), Lin, Lout, oc, static_cast<int>(ob), m0, 0);
+    }
+  }
+}
+
 void launch_bsp_fit_axis(const BspDev& T, int np, const float* const* in, float* const* out, const int dims[3],
                          int axis, bool last, hipStream_t s) {
   const BspAxisDev& A = T.ax[axis];
-  int64_t no = A.m;
-  for (int d = 0; d < 3; ++d)
-    if (d != axis) no *= dims[d];
-  const dim3 grid(grid_stride_blocks(no) * 4, np);
+  const int64_t row_bytes = static_cast<int64_t>(A.n) * T.D * 4;
+  const int64_t table_bytes = static_cast<int64_t>(A.gtot) * 4 + (2 * static_cast<int64_t>(A.m) + 1) * 4;
+  if (axis != 0 || last || table_bytes + row_bytes > 65536) {
+    bsp_strided(T, np, in, out, dims, axis, true, last, s);
+    return;
+  }
+  const int rows = dims[1] * dims[2];
+  const int R = static_cast<int>(std::max<int64_t>(
+      1, std::min<int64_t>(8, (std::max<int64_t>(16384 - table_bytes, row_bytes)) / row_bytes)));
+  const dim3 grid(static_cast<unsigned int>((rows + R - 1) / R), np);
+  const size_t lds = static_cast<size_t>(table_bytes + R * row_bytes);
   if (T.D == 2)
-    hipLaunchKernelGGL(k_bsp_fit_axis<2>, grid, dim3(kBlock), 0, s, A, T, pack(in, np), mpack(out, np), dims[0],
-                       dims[1], dims[2], axis, static_cast<int>(last));
+    hipLaunchKernelGGL(k_bsp_fit_x<2>, grid, dim3(256), lds, s, A, pack(in, np), mpack(out, np), rows, R);
   else
-    hipLaunchKernelGGL(k_bsp_fit_axis<3>, grid, dim3(kBlock), 0, s, A, T, pack(in, np), mpack(out, np), dims[0],
-                       dims[1], dims[2], axis, static_cast<int>(last));
+    hipLaunchKernelGGL(k_bsp_fit_x<3>, grid, dim3(256), lds, s, A, pack(in, np), mpack(out, np), rows, R);
 }
 
 void launch_bsp_eval_axis(const BspDev& T, int np, const float* const* in, float* const* out, const int dims[3],
-                          int axis, const float sp[3], unsigned int* maxbits, hipStream_t s) {
-  const BspAxisDev& A = T.ax[axis];
-  int64_t no = A.n;
-  for (int d = 0; d < 3; ++d)
-    if (d != axis) no *= dims[d];
-  const dim3 grid(maxbits ? red_blocks(no) : grid_stride_blocks(no) * 4, np);
+                          int axis, const float sp[3], unsigned int* maxbits, float* part, hipStream_t s) {
+  if (axis != 0) {
+    bsp_strided(T, np, in, out, dims, axis, false, false, s);
+    return;
+  }
+  const BspAxisDev& A = T.ax[0];
+  const int64_t nvox = static_cast<int64_t>(A.n) * dims[1] * dims[2];  // < 2^31 (checked with the tables)
+  const unsigned int blocks = static_cast<unsigned int>(std::min<int64_t>((nvox + 255) / 256, kBspEvalBlocks));
+  const dim3 grid(blocks, np);
+  float* pp = maxbits ? part : nullptr;
   if (T.D == 2)
-    hipLaunchKernelGGL(k_bsp_eval_axis<2>, grid, dim3(kBlock), 0, s, A, pack(in, np), mpack(out, np), dims[0],
-                       dims[1], dims[2], axis, T.q, sp[0], sp[1], sp[2], maxbits);
+    hipLaunchKernelGGL(k_bsp_eval_x<2>, grid, dim3(256), 0, s, A, pack(in, np), mpack(out, np), static_cast<int>(nvox),
+                       T.q, sp[0], sp[1], sp[2], pp);
   else
-    hipLaunchKernelGGL(k_bsp_eval_axis<3>, grid, dim3(kBlock), 0, s, A, pack(in, np), mpack(out, np), dims[0],
-                       dims[1], dims[2], axis, T.q, sp[0], sp[1], sp[2], maxbits);
+    hipLaunchKernelGGL(k_bsp_eval_x<3>, grid, dim3(256), 0, s, A, pack(in, np), mpack(out, np), static_cast<int>(nvox),
+                       T.q, sp[0], sp[1], sp[2], pp);
+  if (maxbits)
+    hipLaunchKernelGGL(k_max_partials, dim3(np), dim3(1024), 0, s, part, static_cast<int>(blocks), maxbits);
 }
+
+int64_t bsp_partials_needed() { return kBspEvalBlocks; }
 
 void launch_compose(const GridDev& g, int np, const float* const* disp, const float* const* warp, float* const* out,
                     const unsigned int* maxbits, float rate, hipStream_t s) {

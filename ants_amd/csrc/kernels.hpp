@@ -184,12 +184,13 @@ void launch_publish_stats(const Partial* cc_parts, int cc_n, RedSlot* energy, co
 struct BspAxisDev {
   const int* span;   // [n] span of each voxel
   const float* bw;   // [n][4] basis values b_j(t_i), j = 0..order
-  const float* fw;   // [n][4] fit weights b_j^3 / sum_j b_j^2, zero at voxels 0 and n-1
   const int* lo;     // [m] first voxel whose span lies in [k - order, k]
-  const int* hi;     // [m] last such voxel (lo > hi: none)
+  const int* goff;   // [m + 1] control point k's fit weights are gw[goff[k] .. goff[k+1])
+  const float* gw;   // fit weights b^3 / sum_j b_j^2 of voxels lo[k], lo[k] + 1, ... (zero at voxels 0, n-1)
   const double* A;   // [m] sum of b_{k-s}(t)^2 over all voxels of the range
   const double* Ai;  // [m] the same over the voxels 1..n-2
   int n, m;
+  int gtot;          // goff[m] = (order + 1) n
 };
 struct BspDev {
   BspAxisDev ax[3];
@@ -201,10 +202,12 @@ struct BspDev {
 void launch_bsp_fit_axis(const BspDev& T, int np, const float* const* in, float* const* out, const int dims[3],
                          int axis, bool last, hipStream_t s);
 // Evaluation pass along `axis` from input dims `dims` (m on that axis) to n
-// voxels.  maxbits != nullptr: the final pass, which also takes each field's
-// max |u / spacing| into maxbits[k] (atomic max of the float bits).
+// voxels.  maxbits != nullptr (axis 0, the final pass): also each field's
+// max |u / spacing| into maxbits[k] (atomic max of the float bits), through
+// `part`, scratch of np * bsp_partials_needed() floats.
 void launch_bsp_eval_axis(const BspDev& T, int np, const float* const* in, float* const* out, const int dims[3],
-                          int axis, const float sp[3], unsigned int* maxbits, hipStream_t s);
+                          int axis, const float sp[3], unsigned int* maxbits, float* part, hipStream_t s);
+int64_t bsp_partials_needed();
 
 // out_k = warp_k + scale_k * disp_k(x + warp_k(x)), scale_k = rate / max_k when maxbits given (max>0), else 1.
 void launch_compose(const GridDev& g, int np, const float* const* disp, const float* const* warp,

@@ -24,8 +24,10 @@ FIELD_ATOL_VOX = 0.05
 @pytest.mark.parametrize("size,mesh,q", [((24, 28, 20), (8, 9, 6), 3), ((48, 56, 48), (16, 18, 16), 3),
                                          ((19, 13, 17), (5, 24, 4), 3), ((30, 22, 26), (7, 5, 6), 2),
                                          ((21, 25, 18), (6, 8, 5), 1), ((64, 48), (20, 11), 3),
-                                         ((33, 41), (9, 30), 2)])
+                                         ((33, 41), (9, 30), 2), ((3000, 6), (40, 6), 3)])
 def test_bspline_field_vs_oracle(size, mesh, q):
+    """(3000, 6): rows too long for the LDS-staged x fit (the strided pass
+    takes over)."""
     D = len(size)
     shape = tuple(reversed(size)) + (D,)
     sp = [1.0, 1.25, 0.8][:D]
