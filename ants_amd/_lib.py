@@ -90,6 +90,7 @@ class IterRecordC(C.Structure):
         ("energy", C.c_double),
         ("convergence", C.c_double),
         ("inv_iters", C.c_int * 4),
+        ("full_scale_cc", C.c_double),
     ]
 
 
@@ -144,6 +145,8 @@ def lib():
     L.ants_syn_get_trace.argtypes = [vp, C.POINTER(IterRecordC), C.c_int64, C.POINTER(C.c_int64)]
     L.ants_syn_set_profiling.argtypes = [vp, C.c_uint]
     L.ants_syn_set_profiling_stride.argtypes = [vp, i]
+    L.ants_syn_set_full_scale_cc_interval.argtypes = [vp, i]
+    L.ants_syn_last_full_scale_cc.argtypes = [vp, C.POINTER(d)]
     L.ants_syn_get_kernel_stats.argtypes = [vp, C.POINTER(KernelStatsC)]
     L.ants_syn_get_kernel_stats_level.argtypes = [vp, i, C.POINTER(KernelStatsC)]
     L.ants_op_smooth_image.argtypes = [pdesc, vp, vp, d, i, vp]

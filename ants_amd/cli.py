@@ -94,6 +94,8 @@ def cmd_registration(argv: Sequence[str]) -> int:
                     help="affine transform file or [file,useInverse] applied to the fixed image's points")
     ap.add_argument("--float", default="1")
     ap.add_argument("--precision", choices=["fp32", "bf16"], default="fp32")
+    ap.add_argument("--print-similarity-measure-interval", type=int, default=0,
+                    help="full-scale CC r=4 of the original images every N iterations (0: off)")
     ap.add_argument("-v", "--verbose", type=int, default=0)
     a = ap.parse_args(argv)
     mparts = _bracket_list(re.sub(r"^\s*CC", "", a.metric))
@@ -124,6 +126,7 @@ def cmd_registration(argv: Sequence[str]) -> int:
         raise SystemExit("-q with warped-image outputs is not supported (write the warp files only)")
     reg = SyNRegistration(stage, a.dimensionality, precision=a.precision)
     reg.set_initial_transforms(init, finit)
+    reg.set_full_scale_cc_interval(a.print_similarity_measure_interval)
     res = reg.register(np.asarray(fixed, np.float32), np.asarray(moving, np.float32), gf.spacing, gf.origin,
                        gm.spacing, gm.origin, gf.direction, gm.direction)
     lg = reg.level_geometry(gf.size, gf.spacing, gf.origin, direction=gf.direction)  # the fields' grid

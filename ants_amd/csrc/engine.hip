@@ -601,6 +601,10 @@ struct ants_syn_handle {
   // copy kept for the asynchronous upload) and the lattices of the update
   // fits (front end) and total fits (back end), two fields each
   DevBuf bsp_tab, bsp_upd, bsp_tot, bsp_part;  // bsp_part: max-norm partials (front end, back end)
+  // the observer's full-scale CC (--print-similarity-measure-interval)
+  int fscc_interval = 0;
+  double fscc_last = 0.0;  // of the latest iteration; 0 when not computed
+  DevBuf fscc_buf;
   std::vector<unsigned char> bsp_host;
   DevBuf stats, partials_cc, partials_inv, masks_inv, minmax;
   IterStats* host_stats = nullptr;  // pinned
@@ -661,7 +665,7 @@ struct ants_syn_handle {
     const DevBuf* all[] = {&F,      &M,           &Fs,          &Ms,        &tmp,    &Fw,          &Mw,
                            &Fb,     &Mb,          &stats,       &partials_cc, &partials_inv, &masks_inv,
                            &minmax, &restore_buf, &io_f,        &io_m,      &io_fwd, &io_inv, &blend,
-                           &bsp_tab, &bsp_upd,    &bsp_tot,     &bsp_part};
+                           &bsp_tab, &bsp_upd,    &bsp_tot,     &bsp_part,  &fscc_buf};
     int64_t b = 0;
     for (const DevBuf* x : all) b += static_cast<int64_t>(x->bytes);
     for (const auto& x : fld) b += static_cast<int64_t>(x.bytes);
@@ -674,7 +678,7 @@ struct ants_syn_handle {
     graph_signature.clear();
     DevBuf* all[] = {&F,     &M,           &Fs,          &Ms,        &tmp,    &Fw,   &Mw,   &Fb,     &Mb,
                      &stats, &partials_cc, &partials_inv, &masks_inv, &minmax, &io_f, &io_m, &io_fwd, &io_inv,
-                     &blend, &bsp_tab, &bsp_upd, &bsp_tot, &bsp_part};
+                     &blend, &bsp_tab, &bsp_upd, &bsp_tot, &bsp_part, &fscc_buf};
     for (DevBuf* b : all) b->release();
     for (auto& b : fld) b.release();
     have_state = false;
@@ -1136,6 +1140,11 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
     if (bsp_total) ANTS_TRY(bsp_tot.ensure(2 * bsp_ft * sizeof(float)));
     ANTS_TRY(bsp_part.ensure(4 * bsp_partials_needed() * sizeof(float)));
   }
+  if (fscc_interval > 0) {
+    // two resampled images, two derivative fields (CC writes them), a slot and the CC partials
+    const size_t pb = static_cast<size_t>(cc_partials_needed(to_dev(gF), 4, false)) * sizeof(Partial);
+    ANTS_TRY(fscc_buf.ensure((2 * NF + 2 * NF * D) * sizeof(float) + sizeof(RedSlot) + pb));
+  }
   ANTS_TRY(stats.ensure(2 * sizeof(IterStats)));  // two iteration parities
   int64_t ccp = 0, invp = 0, invw = 0;
   for (int l = 0; l < P.n_levels; ++l) {
@@ -1285,36 +1294,51 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
     // middle field on the level grid, then the side's initial transforms]
     ApplyDev gw[2];
     GridDev gw_out{};
-    if (general) {
-      ants_image_desc ld, id[2];
+    // side k's image (0 fixed, 1 moving) sampled on grid `out` through [the
+    // SyN inverse middle field on this level's grid, then the side's initial
+    // transforms], as ResampleImageFilter with that composite does; the
+    // field's buffer is set where the apply kernel is launched
+    auto side_apply = [&](int k, const HostGrid& out, ApplyDev* A) -> int {
+      ants_image_desc od, ld, id;
+      to_desc(out, &od);
       to_desc(g, &ld);
-      to_desc(gF, &id[0]);
-      to_desc(gM, &id[1]);
-      for (int k = 0; k < 2; ++k) {
-        ApplyDev& A = gw[k];
-        std::memset(&A, 0, sizeof(A));
-        ANTS_TRY(geom_from_desc(&ld, &A.ref));
-        ANTS_TRY(geom_from_desc(&id[k], &A.in));
-        A.defv = 0.f;
-        A.t[0].kind = ANTS_TRANSFORM_DISPLACEMENT;
-        A.t[0].fg = A.ref;  // the field lives on the level grid; its buffer is set per iteration
-        int nt = 1;
-        if (init_general[k]) {
-          for (int t = 0; t < init_n[k]; ++t) ANTS_TRY(xform_to_dev(init_list[k][t], D, &A.t[nt++]));
-        } else if (init_on[k]) {
-          XformDev& X = A.t[nt++];
-          X.kind = ANTS_TRANSFORM_AFFINE;
-          for (int r = 0; r < 3; ++r) {
-            X.off[r] = init_o[k][r];
-            for (int c = 0; c < 3; ++c) X.a[r * 3 + c] = init_A[k][r * 3 + c];
-          }
+      to_desc(k == 0 ? gF : gM, &id);
+      std::memset(A, 0, sizeof(*A));
+      ANTS_TRY(geom_from_desc(&od, &A->ref));
+      ANTS_TRY(geom_from_desc(&id, &A->in));
+      A->defv = 0.f;
+      A->t[0].kind = ANTS_TRANSFORM_DISPLACEMENT;
+      ANTS_TRY(geom_from_desc(&ld, &A->t[0].fg));
+      int nt = 1;
+      if (init_general[k]) {
+        for (int t = 0; t < init_n[k]; ++t) ANTS_TRY(xform_to_dev(init_list[k][t], D, &A->t[nt++]));
+      } else if (init_on[k]) {
+        XformDev& X = A->t[nt++];
+        X.kind = ANTS_TRANSFORM_AFFINE;
+        for (int r = 0; r < 3; ++r) {
+          X.off[r] = init_o[k][r];
+          for (int c = 0; c < 3; ++c) X.a[r * 3 + c] = init_A[k][r * 3 + c];
         }
-        A.nt = nt;
       }
-      gw_out.D = D;
-      for (int k = 0; k < 3; ++k) gw_out.n[k] = static_cast<int>(g.n[k]);
-      set_divisors(gw_out);
+      A->nt = nt;
+      return ANTS_OK;
+    };
+    auto grid_of = [&](const HostGrid& h) {
+      GridDev o{};
+      o.D = D;
+      for (int k = 0; k < 3; ++k) o.n[k] = static_cast<int>(h.n[k]);
+      set_divisors(o);
+      return o;
+    };
+    if (general) {
+      for (int k = 0; k < 2; ++k) ANTS_TRY(side_apply(k, g, &gw[k]));
+      gw_out = grid_of(g);
     }
+    // the observer's full-scale CC: both original images on the fixed grid
+    ApplyDev fs_apply[2];
+    const GridDev fs_out = grid_of(gF);
+    if (fscc_interval > 0)
+      for (int k = 0; k < 2; ++k) ANTS_TRY(side_apply(k, gF, &fs_apply[k]));
     if (use_maps) {
       static const double I[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, Z[3] = {0, 0, 0};
       maps[0] = make_warp_map(g, init_on[0] ? init_A[0] : I, init_on[0] ? init_o[0] : Z, gF);
@@ -1585,7 +1609,30 @@ int ants_syn_handle::run(const ants_image_desc* fd, const float* d_fixed, const 
       run_stats.voxel_iterations += static_cast<double>(N);
       run_stats.algorithmic_bytes +=
           static_cast<double>(N) * ((2.0 * gsz * e_i + 14.0 * D * e_f) + (3.0 * ksum + 4.0) * D * e_f);
-      trace.push_back(ants_syn_iteration_record{l, iter, E, conv, {kin[0], kin[1], kin[2], kin[3]}});
+      // antsDisplacementAndVelocityFieldRegistrationCommandIterationUpdate
+      // (Examples/...CommandIterationUpdate.h:151-159, 252-397): at iteration
+      // 1, every interval-th and the level's last, CC r=4 between the original
+      // images, each resampled onto the fixed grid through its side's
+      // composite [initial transforms, middle transform inverse]
+      fscc_last = 0.0;
+      if (fscc_interval > 0 && (iter == 1 || iter % fscc_interval == 0 || iter == P.iterations[l])) {
+        const int64_t NFF = gF.count();
+        float* img = fscc_buf.f();
+        float* grad = img + 2 * NFF;
+        fs_apply[0].t[0].f = st[1];
+        fs_apply[1].t[0].f = st[3];
+        launch_apply_transforms(fs_out, fs_apply[0], ANTS_INTERP_LINEAR, d_fixed, img, s);
+        launch_apply_transforms(fs_out, fs_apply[1], ANTS_INTERP_LINEAR, d_moving, img + NFF, s);
+        RedSlot* slot = reinterpret_cast<RedSlot*>(fscc_buf.f() + 2 * NFF + 2 * NFF * D);
+        Partial* parts = reinterpret_cast<Partial*>(slot + 1);
+        launch_cc(gFd, img, img + NFF, 4, eps, grad, grad + NFF * D, parts, slot, s, true, false);
+        HIP_TRY(hipGetLastError());
+        RedSlot hslot{};
+        HIP_TRY(hipMemcpyAsync(&hslot, slot, sizeof(RedSlot), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        fscc_last = -hslot.sum / static_cast<double>(NFF);
+      }
+      trace.push_back(ants_syn_iteration_record{l, iter, E, conv, {kin[0], kin[1], kin[2], kin[3]}, fscc_last});
       if (cb && cb(user, l, iter, E, conv, kin) != 0) return fail(ANTS_ERR_CANCELLED, "cancelled by the observer");
       return ANTS_OK;
     };
@@ -2021,6 +2068,19 @@ int ants_syn_get_trace(const ants_syn_handle* h, ants_syn_iteration_record* out,
 int ants_syn_set_profiling(ants_syn_handle* h, unsigned int class_mask) {
   if (!h) return fail(ANTS_ERR_INVALID, "null handle");
   h->prof.mask = class_mask;
+  return ANTS_OK;
+}
+
+int ants_syn_set_full_scale_cc_interval(ants_syn_handle* h, int interval) {
+  if (!h) return fail(ANTS_ERR_INVALID, "null handle");
+  if (interval < 0) return fail(ANTS_ERR_INVALID, "similarity measure interval must be >= 0");
+  h->fscc_interval = interval;
+  return ANTS_OK;
+}
+
+int ants_syn_last_full_scale_cc(const ants_syn_handle* h, double* value) {
+  if (!h || !value) return fail(ANTS_ERR_INVALID, "null argument");
+  *value = h->fscc_last;
   return ANTS_OK;
 }
 

@@ -174,6 +174,7 @@ class IterationRecord:
     energy: float
     convergence: float
     inv_iters: Tuple[int, int, int, int]
+    full_scale_cc: float = 0.0     # --print-similarity-measure-interval value (0: not computed)
 
 
 @dataclass
@@ -192,8 +193,16 @@ class SyNResult:
         return np.array([r.level for r in self.trace])
 
     def diagnostic_lines(self) -> List[str]:
-        """The observer's `1DIAGNOSTIC, iter, E, conv, ...` lines."""
-        return [f"1DIAGNOSTIC, {r.iteration:5d}, {r.energy:.12e}, {r.convergence:.12e}" for r in self.trace]
+        """The observer's `1DIAGNOSTIC, iter, E, conv, ...` lines (with the
+        full-scale CC appended where it was computed and |value| > 1e-7, as
+        the observer prints it)."""
+        out = []
+        for r in self.trace:
+            ln = f"1DIAGNOSTIC, {r.iteration:5d}, {r.energy:.12e}, {r.convergence:.12e}"
+            if abs(r.full_scale_cc) > 1e-7:
+                ln += f", {r.full_scale_cc:.12e}"
+            out.append(ln)
+        return out
 
 
 def stage_params(stage: "SyNStage", dim: int, precision: str = "fp32", preprocess: bool = True,
@@ -297,6 +306,12 @@ class SyNRegistration:
         g = L.desc(size, spacing, origin, direction)
         L.check(L.lib().ants_syn_set_initial_state_device(self.handle, C.byref(g), *[b.ptr for b in bufs]))
 
+    def set_full_scale_cc_interval(self, interval: int):
+        """--print-similarity-measure-interval: the observer's CC r=4 between
+        the original images at iteration 1, every `interval`-th and each
+        level's last (ants_syn_set_full_scale_cc_interval); 0 turns it off."""
+        L.check(L.lib().ants_syn_set_full_scale_cc_interval(self.handle, int(interval)))
+
     def set_profiling(self, mask: int, every: int = 1):
         """Time the kernel classes in `mask` on every `every`-th iteration of
         each level (ants_syn_set_profiling / _stride)."""
@@ -324,16 +339,20 @@ class SyNRegistration:
                 "eager_iterations": int(s.eager_iterations), "inv_tails": int(s.inv_tails)}
 
     @staticmethod
-    def _callback(observer):
+    def _callback(observer, handle=None):
         """ctypes observer for a live per-iteration callback, or a null
         callback.  The engine keeps the trace itself (`trace()`), so no
         callback is passed unless the caller asks for one: a Python call
-        between iterations holds the next iteration back."""
+        between iterations holds the next iteration back.  With `handle` the
+        record carries the iteration's full-scale CC."""
         if observer is None:
             return L.ITER_CB()
 
         def cb(user, level, it, e, conv, kin):
-            rec = IterationRecord(level, it, e, conv, tuple(kin[i] for i in range(4)))
+            v = C.c_double(0.0)
+            if handle:
+                L.check(L.lib().ants_syn_last_full_scale_cc(handle, C.byref(v)))
+            rec = IterationRecord(level, it, e, conv, tuple(kin[i] for i in range(4)), v.value)
             return 1 if observer(rec) else 0
         return L.ITER_CB(cb)
 
@@ -343,7 +362,7 @@ class SyNRegistration:
         L.check(L.lib().ants_syn_get_trace(self.handle, None, 0, C.byref(n)))
         buf = (L.IterRecordC * max(n.value, 1))()
         L.check(L.lib().ants_syn_get_trace(self.handle, buf, n.value, C.byref(n)))
-        return [IterationRecord(r.level, r.iteration, r.energy, r.convergence, tuple(r.inv_iters))
+        return [IterationRecord(r.level, r.iteration, r.energy, r.convergence, tuple(r.inv_iters), r.full_scale_cc)
                 for r in buf[:n.value]]
 
     def register(self, fixed: np.ndarray, moving: np.ndarray, fixed_spacing=None, fixed_origin=None,
@@ -366,7 +385,7 @@ class SyNRegistration:
         oshape = tuple(int(out.size[k]) for k in reversed(range(self.dim))) + (self.dim,)
         fwd = np.empty(oshape, np.float32)
         inv = np.empty(oshape, np.float32)
-        cb = self._callback(observer)
+        cb = self._callback(observer, self.handle)
         L.check(L.lib().ants_syn_register(self.handle, C.byref(gf), fixed.ctypes.data_as(C.c_void_p), C.byref(gm),
                                           moving.ctypes.data_as(C.c_void_p), fwd.ctypes.data_as(C.c_void_p),
                                           inv.ctypes.data_as(C.c_void_p), cb, None))
@@ -378,7 +397,7 @@ class SyNRegistration:
         """Device buffers in and out (inputs already resident in HBM).
         Returns the iteration records (appended to `trace` when given)."""
         tr = trace if trace is not None else []
-        cb = self._callback(observer)
+        cb = self._callback(observer, self.handle)
         L.check(L.lib().ants_syn_register_device(self.handle, C.byref(gf), d_fixed.ptr, C.byref(gm), d_moving.ptr,
                                                  d_fwd.ptr if d_fwd else None, d_inv.ptr if d_inv else None, cb,
                                                  None))

@@ -199,8 +199,25 @@ typedef struct ants_syn_iteration_record {
   double energy;
   double convergence;
   int inv_iters[4];
+  double full_scale_cc;  /* the observer's full-scale CC r=4 (0 when not computed at this iteration) */
 } ants_syn_iteration_record;
 int ants_syn_get_trace(const ants_syn_handle* h, ants_syn_iteration_record* out, int64_t capacity, int64_t* count);
+
+/* antsRegistration --print-similarity-measure-interval N
+ * (Examples/antsRegistrationTemplateHeader.h:175-195, set on the SyN observer
+ * at itkantsRegistrationHelper.hxx:2176-2181): at iteration 1, every N-th
+ * iteration and the last iteration of each level, the observer's CC metric of
+ * radius 4 between the ORIGINAL fixed and moving images, each resampled onto
+ * the fixed image's grid through its composite [initial transforms, middle
+ * transform inverse] (linear interpolation, 0 outside)
+ * (Examples/antsDisplacementAndVelocityFieldRegistrationCommandIterationUpdate.h:
+ * 151-159, 252-397).  The value lands in the iteration record's
+ * full_scale_cc and, inside the observer callback, in
+ * ants_syn_last_full_scale_cc (0 at iterations where it is not computed).
+ * interval 0 (default) turns it off.  Costs two full-grid resamples and a
+ * full-grid CC per computed iteration. */
+int ants_syn_set_full_scale_cc_interval(ants_syn_handle* h, int interval);
+int ants_syn_last_full_scale_cc(const ants_syn_handle* h, double* value);
 
 /* Kernel classes timed with HIP events on the handle's stream when enabled
  * by ants_syn_set_profiling (bit k enables class k).  A profiled run launches

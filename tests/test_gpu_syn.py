@@ -636,3 +636,36 @@ def test_syn_3d_small_variances_vs_oracle():
     res = run_pair(stage, pair, force=True)
     ref = run_oracle(stage, pair, force=True)
     print("3d small-variance parity:", compare(res, ref, pair.spacing))
+
+
+@pytest.mark.parametrize("shrink,iters", [("1", "7"), ("2", "7")])
+def test_full_scale_cc_matches_oracle(shrink, iters):
+    """--print-similarity-measure-interval 3: computed at iterations 1, 3, 6
+    and the last (7), zero elsewhere; the last value equals the oracle's CC
+    r=4 between the ORIGINAL images resampled onto the fixed grid through the
+    run's final inverse middle fields (on the half-resolution level grid when
+    the schedule is -f 2: interpolated, zero outside the field's buffer)."""
+    pair = ph.phantom_pair_3d(5, size=(40, 48, 36), max_disp_mm=3.0, svf_sigma_mm=5.0)
+    stage = SyNStage.from_args("SyN[0.25,3,0]", "CC[f,m,1,2]", f"[{iters},1e-6,10]", shrink, "0vox")
+    reg = SyNRegistration(stage, 3, force_iterations=True)
+    reg.set_full_scale_cc_interval(3)
+    seen = []
+    res = reg.register(pair.fixed, pair.moving, pair.spacing, pair.origin, pair.spacing, pair.origin,
+                       observer=lambda r: seen.append(r.full_scale_cc) and False)
+    lg = reg.level_geometry(pair.size, pair.spacing, pair.origin)
+    lshape = tuple(int(lg.size[k]) for k in reversed(range(3)))
+    f2m_f, f2m_i, m2m_f, m2m_i = reg.state(lshape + (3,))
+    reg.close()
+    fs = [r.full_scale_cc for r in res.trace]
+    assert fs == seen
+    assert [i + 1 for i, v in enumerate(fs) if v != 0.0] == [1, 3, 6, 7]
+    assert any("e-" in ln and ln.count(",") == 4 for ln in res.diagnostic_lines())
+    gref = o.og_geom(pair.size, pair.spacing, pair.origin)
+    gfld = o.og_geom([int(lg.size[k]) for k in range(3)], [lg.spacing[k] for k in range(3)],
+                     [lg.origin[k] for k in range(3)])
+    Fw = o.apply_transforms(gref, gref, pair.fixed, [("field", gfld, f2m_i)])
+    Mw = o.apply_transforms(gref, gref, pair.moving, [("field", gfld, m2m_i)])
+    g = o.Grid(list(pair.size), list(pair.spacing), list(pair.origin))
+    ref, _, _ = o.cc(g, Fw.astype(np.float64), Mw.astype(np.float64), 4)
+    print(f"full-scale CC -f {shrink}: gpu {fs[-1]:.9f} oracle {ref:.9f}")
+    assert fs[-1] == pytest.approx(ref, rel=1e-5)
