@@ -16,8 +16,16 @@ then ``out_0GenericAffine.mat`` (the initial transforms composed),
   -o out.nii.gz -t out_0Warp.nii.gz -t [affine.mat,1] -n Linear
   --default-value 0``
 is the antsApplyTransforms subset (Examples/antsApplyTransforms.cxx).
-Transforms ending in .mat/.txt/.tfm are affine files; other paths are
-displacement-field images.
+Transforms ending in .mat/.txt/.tfm are affine files, .h5 files ITK
+composite transforms (their components in the order they apply); other paths
+are displacement-field images.
+
+``registration ... -a 1`` writes ``out_Composite.h5`` and
+``out_InverseComposite.h5`` (--write-composite-transform) instead of the
+separate files; ``--save-state state.h5`` writes the registration state
+(the linear initial transforms, then FixedToMiddle and MovingToMiddle with
+their inverses) and ``--restore-state state.h5`` starts the stage from one
+(Examples/antsRegistrationTemplateHeader.h:293-327, 1308-1348).
 """
 from __future__ import annotations
 
@@ -37,6 +45,21 @@ def _bracket_list(s: str) -> List[str]:
     return [p.strip() for p in (m.group(1) if m else s).split(",")]
 
 
+def load_transforms(spec: str) -> list:
+    """-t path | -t [path,invert]: the transforms of one option in the order
+    they apply (an .h5 composite lists its components newest-first, the
+    order ITK's CompositeTransform applies them)."""
+    parts = _bracket_list(spec)
+    path = parts[0]
+    if path.endswith(".h5"):
+        from . import h5transform
+        if len(parts) > 1 and parts[1] not in ("0", "false", "False", ""):
+            raise ValueError(f"{path}: inverting a composite file is not supported")
+        return list(reversed(h5transform.read_composite(path)))
+    t = load_transform(spec)
+    return [] if t is None else [t]
+
+
 def load_transform(spec: str):
     """-t path | -t [path,invert]"""
     parts = _bracket_list(spec)
@@ -44,6 +67,11 @@ def load_transform(spec: str):
     invert = len(parts) > 1 and parts[1] not in ("0", "false", "False", "")
     if path.lower() == "identity":
         return None
+    if path.endswith(".h5"):
+        ts = load_transforms(spec)
+        if len(ts) != 1:
+            raise ValueError(f"{path}: a composite of {len(ts)} transforms where one is expected")
+        return ts[0]
     if path.endswith((".mat", ".txt", ".tfm")):
         t = aio.read_transform(path)
         t.invert = invert
@@ -69,7 +97,7 @@ def cmd_apply(argv: Sequence[str]) -> int:
     ref, gr = aio.read_nifti(a.reference_image)
     if a.dimensionality and a.dimensionality != gi.dim:
         raise SystemExit(f"-d {a.dimensionality} but the input is {gi.dim}-D")
-    xf = [load_transform(t) for t in a.transform]
+    xf = [x for t in a.transform for x in load_transforms(t)]
     out = apply_transforms(gr, np.asarray(img, np.float32), gi, xf, a.interpolation, a.default_value)
     if a.interpolation.lower().startswith(("nearest", "genericlabel")) and np.issubdtype(np.asarray(img).dtype,
                                                                                          np.integer):
@@ -94,6 +122,10 @@ def cmd_registration(argv: Sequence[str]) -> int:
                     help="affine transform file or [file,useInverse] applied to the fixed image's points")
     ap.add_argument("--float", default="1")
     ap.add_argument("--precision", choices=["fp32", "bf16"], default="fp32")
+    ap.add_argument("-a", "--write-composite-transform", type=int, default=0,
+                    help="1: write <prefix>Composite.h5 and <prefix>InverseComposite.h5")
+    ap.add_argument("--save-state", default=None, help="write the registration state (.h5)")
+    ap.add_argument("--restore-state", default=None, help="start from a saved registration state (.h5)")
     ap.add_argument("--print-similarity-measure-interval", type=int, default=0,
                     help="full-scale CC r=4 of the original images every N iterations (0: off)")
     ap.add_argument("-v", "--verbose", type=int, default=0)
@@ -110,26 +142,36 @@ def cmd_registration(argv: Sequence[str]) -> int:
             raise SystemExit(f"{name} is {g.dim}-D, -d {a.dimensionality}")
     outs = _bracket_list(a.output)
     prefix = outs[0]
-    def linear(specs):
-        out = []
-        for spec in specs:
-            t = load_transform(spec)
-            if isinstance(t, DisplacementFieldTransform):
-                raise SystemExit(f"{spec}: only linear initial transforms are supported")
-            if t is not None:
-                out.append(t)
-        return out
+    def initial(specs):
+        return [t for spec in specs for t in load_transforms(spec)]
 
-    init = linear(a.initial_moving_transform)
-    finit = linear(a.initial_fixed_transform)
+    init = initial(a.initial_moving_transform)
+    finit = initial(a.initial_fixed_transform)
+    restored = None
+    if a.restore_state:
+        if init or finit:
+            # antsRegistrationTemplateHeader.h:295-305
+            raise SystemExit("restore-state option is mutually exclusive with initial-moving-transform & "
+                             "initial-fixed-transform options")
+        from . import h5transform
+        lin, state, gst = h5transform.read_syn_state(a.restore_state)
+        init = list(reversed(lin))
+        restored = (state, gst)
     if finit and (len(outs) > 1 and any(outs[1:])):
         raise SystemExit("-q with warped-image outputs is not supported (write the warp files only)")
+    if any(isinstance(t, DisplacementFieldTransform) for t in init) and not a.write_composite_transform:
+        raise SystemExit("a displacement field among the initial transforms needs -a 1 (composite outputs)")
     reg = SyNRegistration(stage, a.dimensionality, precision=a.precision)
     reg.set_initial_transforms(init, finit)
+    if restored is not None:
+        state, gst = restored
+        reg.set_initial_state(state, gst.size, gst.spacing, gst.origin, gst.direction)
     reg.set_full_scale_cc_interval(a.print_similarity_measure_interval)
     res = reg.register(np.asarray(fixed, np.float32), np.asarray(moving, np.float32), gf.spacing, gf.origin,
                        gm.spacing, gm.origin, gf.direction, gm.direction)
     lg = reg.level_geometry(gf.size, gf.spacing, gf.origin, direction=gf.direction)  # the fields' grid
+    lshape = tuple(int(lg.size[k]) for k in reversed(range(gf.dim))) + (gf.dim,)
+    state = reg.state(lshape) if a.save_state else None
     reg.close()
     if a.verbose:
         for ln in res.diagnostic_lines():
@@ -137,19 +179,38 @@ def cmd_registration(argv: Sequence[str]) -> int:
     D = gf.dim
     g_out = Geometry(tuple(int(lg.size[k]) for k in range(D)), tuple(lg.spacing[k] for k in range(D)),
                      tuple(lg.origin[k] for k in range(D)), np.array([lg.direction[k] for k in range(D * D)]).reshape(D, D))
+    if state is not None:
+        from . import h5transform
+        # the state composite lists the linear transforms first (applied last)
+        h5transform.write_syn_state(a.save_state, state, g_out, [t for t in reversed(init)
+                                                                 if not isinstance(t, DisplacementFieldTransform)])
     # with initial transforms the output composite is [initial affine, SyN field]:
     # <prefix>0GenericAffine.mat, <prefix>1Warp.nii.gz, <prefix>1InverseWarp.nii.gz
     k = 0
     fwd_chain = [DisplacementFieldTransform(res.fwd, g_out)]
     inv_chain = [DisplacementFieldTransform(res.inv, g_out)]
-    if init:
+    if a.write_composite_transform:
+        # --write-composite-transform: the stage's composite and its inverse,
+        # components newest-first
+        from . import h5transform
+        fwd_chain = fwd_chain + list(init)
+        inv_init = []
+        for t in reversed(init):
+            if isinstance(t, DisplacementFieldTransform):
+                raise SystemExit("the inverse composite of a displacement-field initial transform is not available")
+            inv_init.append(AffineTransform(t.matrix, t.translation, t.center, invert=not t.invert))
+        inv_chain = inv_init + inv_chain
+        h5transform.write_composite(prefix + "Composite.h5", list(reversed(fwd_chain)))
+        h5transform.write_composite(prefix + "InverseComposite.h5", list(reversed(inv_chain)))
+    elif init:
         aff = compose_affines(init)
         aio.write_transform(prefix + "0GenericAffine.mat", aff)
         k = 1
         fwd_chain = fwd_chain + [aff]
         inv_chain = [AffineTransform(aff.matrix, aff.translation, aff.center, invert=True)] + inv_chain
-    aio.write_nifti(prefix + f"{k}Warp.nii.gz", res.fwd, g_out, vector=True)
-    aio.write_nifti(prefix + f"{k}InverseWarp.nii.gz", res.inv, g_out, vector=True)
+    if not a.write_composite_transform:
+        aio.write_nifti(prefix + f"{k}Warp.nii.gz", res.fwd, g_out, vector=True)
+        aio.write_nifti(prefix + f"{k}InverseWarp.nii.gz", res.inv, g_out, vector=True)
     if len(outs) > 1 and outs[1]:
         warped = apply_transforms(gf, np.asarray(moving, np.float32), gm, fwd_chain, "Linear")
         aio.write_nifti(outs[1], warped, gf)

@@ -133,3 +133,48 @@ def test_registration_bspline_syn_knot_spacing(tmp_path):
     reg.close()
     np.testing.assert_array_equal(fwd, res.fwd)
     assert np.abs(fwd).max() > 0.05
+
+
+def test_composite_outputs_and_state_files(tmp_path):
+    """-a 1 writes Composite.h5 / InverseComposite.h5 that apply reads back
+    (equal to the warp fields); --save-state after -f 4x2 and --restore-state
+    for -f 1 reproduce the single 4x2x1 stage bit for bit."""
+    from ants_amd import h5transform
+    pair = ph.phantom_pair_3d(6, size=(48, 40, 32), max_disp_mm=3.0, svf_sigma_mm=6.0)
+    g = Geometry(pair.size, pair.spacing, (0.0, 0.0, 0.0), np.eye(3))
+    f, m = str(tmp_path / "fixed.nii.gz"), str(tmp_path / "moving.nii.gz")
+    aio.write_nifti(f, pair.fixed, g)
+    aio.write_nifti(m, pair.moving, g)
+    base = ["-d", "3", "-m", f"CC[{f},{m},1,2]", "-t", "SyN[0.2,3,0]"]
+    # single stage, composite outputs
+    one = str(tmp_path / "one_")
+    assert cli.main(["registration"] + base + ["-c", "[6x4x3,1e-6,10]", "-f", "4x2x1", "-s", "2x1x0vox",
+                                                "-o", one, "-a", "1"]) == 0
+    comp = h5transform.read_composite(one + "Composite.h5")
+    icomp = h5transform.read_composite(one + "InverseComposite.h5")
+    assert len(comp) == 1 and len(icomp) == 1
+    ref = SyNStage.from_args("SyN[0.2,3,0]", "CC[f,m,1,2]", "[6x4x3,1e-6,10]", "4x2x1", "2x1x0vox")
+    reg = SyNRegistration(ref, 3)
+    res = reg.register(pair.fixed, pair.moving, g.spacing, g.origin, g.spacing, g.origin)
+    reg.close()
+    np.testing.assert_array_equal(comp[0].field, res.fwd)
+    np.testing.assert_array_equal(icomp[0].field, res.inv)
+    out = str(tmp_path / "w.nii.gz")
+    assert cli.main(["apply", "-d", "3", "-i", m, "-r", f, "-o", out, "-t", one + "Composite.h5"]) == 0
+    w, _ = aio.read_nifti(out)
+    np.testing.assert_array_equal(w, apply_transforms(g, pair.moving, g, [DisplacementFieldTransform(res.fwd, g)]))
+    # the same stage split in two through a state file
+    a_ = str(tmp_path / "a_")
+    st = str(tmp_path / "state.h5")
+    assert cli.main(["registration"] + base + ["-c", "[6x4,1e-6,10]", "-f", "4x2", "-s", "2x1vox", "-o", a_,
+                                                "--save-state", st]) == 0
+    lin, fields, gst = h5transform.read_syn_state(st)
+    assert lin == [] and len(fields) == 4 and tuple(gst.size) == (24, 20, 16)
+    b_ = str(tmp_path / "b_")
+    assert cli.main(["registration"] + base + ["-c", "[3,1e-6,10]", "-f", "1", "-s", "0vox", "-o", b_,
+                                                "--restore-state", st]) == 0
+    fwd_b, _ = aio.read_nifti(b_ + "0Warp.nii.gz")
+    np.testing.assert_array_equal(fwd_b, res.fwd)
+    with pytest.raises(SystemExit):
+        cli.main(["registration"] + base + ["-c", "[3,1e-6,10]", "-f", "1", "-s", "0vox", "-o", b_,
+                                            "--restore-state", st, "-r", one + "Composite.h5"])
