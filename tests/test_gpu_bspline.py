@@ -115,3 +115,31 @@ def test_bspline_syn_params_refused():
         with pytest.raises(L.AntsError):
             SyNRegistration(stage, 3)
         stage = SyNStage.from_args("BSplineSyN[0.1,4x4x4,0,3]", "CC[f,m,1,4]", "[2x2,1e-6,10]", "2x1", "1x0vox")
+
+
+@pytest.mark.parametrize("mode", ["pipeline", "graphs"])
+def test_bspline_syn_same_bits_pipelined_and_replayed(monkeypatch, mode):
+    """BSplineSyN with update and total fits (two lattice buffers, one per
+    stream) and the observer's full-scale CC: the two-stream pipelined loop
+    and replayed iteration graphs give the bits of the default launches."""
+    pair = ph.phantom_pair_3d(9, size=(48, 56, 40), max_disp_mm=4.0, svf_sigma_mm=6.0)
+    stage = SyNStage.from_args("BSplineSyN[0.15,6x7x5,3x3x3,3]", "CC[f,m,1,2]", "[20x10x4,1e-6,10]", "4x2x1",
+                               "2x1x0vox")
+    outs = {}
+    for on in ("0", "1"):
+        monkeypatch.setenv("ANTS_SYN_PIPELINE" if mode == "pipeline" else "ANTS_SYN_GRAPHS", on)
+        reg = SyNRegistration(stage, 3)
+        reg.set_full_scale_cc_interval(4)
+        res = reg.register(pair.fixed, pair.moving, pair.spacing, pair.origin, pair.spacing, pair.origin)
+        res2 = reg.register(pair.fixed, pair.moving, pair.spacing, pair.origin, pair.spacing, pair.origin)
+        outs[on] = (res, res2, reg.stats())
+        reg.close()
+    (a, a2, sa), (b, b2, sb) = outs["0"], outs["1"]
+    if mode == "graphs":
+        assert sb["graph_launches"] > 0
+    for x, y in ((a, b), (a, a2), (b, b2)):
+        np.testing.assert_array_equal(x.energy, y.energy)
+        np.testing.assert_array_equal(x.fwd, y.fwd)
+        np.testing.assert_array_equal(x.inv, y.inv)
+        assert [r.full_scale_cc for r in x.trace] == [r.full_scale_cc for r in y.trace]
+    assert sum(1 for r in a.trace if r.full_scale_cc != 0.0) >= 6
