@@ -1027,50 +1027,65 @@ BspDev bsp_dev(const BspLayout& L, const void* dbase) {
 }
 
 // The B-spline fit of np fields: fit along x, y (, z; the last fit pass
-// divides by the weights), then evaluate along z (,) y, x; the x evaluation
-// writes `out` and each field's max |u / spacing| into maxbits[k].  buf[k]
-// holds field_floats() floats of lattices for field k.  in and out may not alias.
+// divides by the weights), then evaluate along z (3-D) and along y and x in
+// one fused pass (the separate y and x passes where a row's tables exceed
+// LDS); the x evaluation writes `out` and each field's max |u / spacing|
+// into maxbits[k] through the per-block partials in `part`.  buf[k] holds
+// field_floats() floats of lattices for field k.  in and out may not alias.
 int bsp_fields_on(hipStream_t s, const HostGrid& g, const BspDev& T, const BspLayout& L, int np,
                   const float* const* in, float* const* out, float* const* buf, unsigned int* maxbits, float* part,
                   Profiler* prof = nullptr) {
   const int D = g.D;
   const int n0 = L.n[0], n1 = L.n[1], n2 = D == 3 ? L.n[2] : 1;
   const int m0 = L.m[0], m1 = L.m[1], m2 = D == 3 ? L.m[2] : 1;
-  const float* cin[4];
   float *t1[4], *t2[4], *lat[4];
   for (int k = 0; k < np; ++k) {
-    cin[k] = in[k];
     t1[k] = buf[k];
     t2[k] = t1[k] + static_cast<int64_t>(m0) * n1 * n2 * D;
     lat[k] = t2[k] + static_cast<int64_t>(m0) * m1 * n2 * D;
   }
   const float sp[3] = {static_cast<float>(g.sp[0]), static_cast<float>(g.sp[1]), static_cast<float>(g.sp[2])};
   if (prof) prof->begin(ANTS_KC_SMOOTH);
-  const int d_in[3] = {n0, n1, n2};
-  launch_bsp_fit_axis(T, np, cin, t1, d_in, 0, false, s);
+  int launches = 0;
   const float* c1[4];
-  for (int k = 0; k < np; ++k) c1[k] = t1[k];
+  const float* c2[4];
+  for (int k = 0; k < np; ++k) {
+    c1[k] = t1[k];
+    c2[k] = t2[k];
+  }
+  const int d_in[3] = {n0, n1, n2};
   const int d_x[3] = {m0, n1, n2};
+  launch_bsp_fit_axis(T, np, in, t1, d_in, 0, false, s);
   if (D == 3) {
     launch_bsp_fit_axis(T, np, c1, t2, d_x, 1, false, s);
-    const float* c2[4];
-    for (int k = 0; k < np; ++k) c2[k] = t2[k];
     const int d_xy[3] = {m0, m1, n2};
     launch_bsp_fit_axis(T, np, c2, lat, d_xy, 2, true, s);
-    const float* cl[4];
-    for (int k = 0; k < np; ++k) cl[k] = lat[k];
-    const int d_l[3] = {m0, m1, m2};
-    launch_bsp_eval_axis(T, np, cl, t2, d_l, 2, sp, nullptr, nullptr, s);  // (m0, m1, n2)
-    launch_bsp_eval_axis(T, np, c2, t1, d_xy, 1, sp, nullptr, nullptr, s);  // (m0, n1, n2)
+    launches += 3;
   } else {
     launch_bsp_fit_axis(T, np, c1, lat, d_x, 1, true, s);
-    const float* cl[4];
-    for (int k = 0; k < np; ++k) cl[k] = lat[k];
-    const int d_l[3] = {m0, m1, 1};
-    launch_bsp_eval_axis(T, np, cl, t1, d_l, 1, sp, nullptr, nullptr, s);  // (m0, n1)
+    launches += 2;
   }
-  launch_bsp_eval_axis(T, np, c1, out, d_x, 0, sp, maxbits, part, s);
-  if (prof) prof->end(ANTS_KC_SMOOTH, np * 2.0 * D * 4.0 * static_cast<double>(g.count()), D == 3 ? 7 : 5);
+  const float* cl[4];
+  for (int k = 0; k < np; ++k) cl[k] = lat[k];
+  // evaluation: z (3-D) into t2, then y and x
+  const float* ylat[4];
+  if (D == 3) {
+    const int d_l[3] = {m0, m1, m2};
+    launch_bsp_eval_axis(T, np, cl, t2, d_l, 2, sp, nullptr, nullptr, s);  // (m0, m1, n2)
+    launches += 1;
+    for (int k = 0; k < np; ++k) ylat[k] = t2[k];
+  } else {
+    for (int k = 0; k < np; ++k) ylat[k] = lat[k];
+  }
+  if (launch_bsp_eval_yx(T, np, ylat, out, n2, sp, maxbits, part, s)) {
+    launches += maxbits ? 2 : 1;
+  } else {
+    const int d_y[3] = {m0, m1, n2};
+    launch_bsp_eval_axis(T, np, ylat, t1, d_y, 1, sp, nullptr, nullptr, s);  // (m0, n1, n2)
+    launch_bsp_eval_axis(T, np, c1, out, d_x, 0, sp, maxbits, part, s);
+    launches += maxbits ? 3 : 2;
+  }
+  if (prof) prof->end(ANTS_KC_SMOOTH, np * 2.0 * D * 4.0 * static_cast<double>(g.count()), launches);
   HIP_TRY(hipGetLastError());
   return ANTS_OK;
 }

@@ -2149,6 +2149,76 @@ __global__ void __launch_bounds__(256) k_bsp_eval_x(BspAxisDev A, PtrPack4 in, M
   }
 }
 
+// Fused evaluation along y and x (the last passes): a block owns voxel rows
+// [y0, y0 + R) of one z plane; it stages the control-point rows they need
+// (the y-evaluated input (m0, m1, n2): rows span(y0) .. span(y0+R-1) + q)
+// and the x tables in LDS, evaluates the R x-rows of control points along y
+// into LDS, then the R field rows along x, and stores the block's max
+// |u / spacing| in part (same arithmetic and order as the separate passes).
+template <int D>
+__global__ void __launch_bounds__(256) k_bsp_eval_yx(BspAxisDev AX, BspAxisDev AY, PtrPack4 in, MutPack4 out,
+                                                     int R, int q, float sp0, float sp1, float sp2, float* part) {
+  extern __shared__ float lds[];
+  const int n0 = AX.n, m0 = AX.m, n1 = AY.n, m1 = AY.m;
+  const int per_row = m0 * D;
+  float* bwl = lds;
+  int* spl = reinterpret_cast<int*>(bwl + 4 * n0);
+  float* t1 = reinterpret_cast<float*>(spl + n0);
+  float* cprow = t1 + R * per_row;
+  const float isp[3] = {1.f / sp0, 1.f / sp1, 1.f / sp2};
+  const int y0 = blockIdx.x * R;
+  const int nr = min(R, n1 - y0);
+  const int z = blockIdx.y;
+  const int ky_lo = AY.span[y0];
+  const int nk = AY.span[y0 + nr - 1] + q - ky_lo + 1;
+  const float* __restrict__ src = in.p[blockIdx.z] + (static_cast<int64_t>(z) * m1 + ky_lo) * per_row;
+  float* __restrict__ dst = out.p[blockIdx.z] + (static_cast<int64_t>(z) * n1 + y0) * n0 * D;
+  for (int e = threadIdx.x; e < 4 * n0; e += 256) bwl[e] = AX.bw[e];
+  for (int e = threadIdx.x; e < n0; e += 256) spl[e] = AX.span[e];
+  for (int e = threadIdx.x; e < nk * per_row; e += 256) cprow[e] = src[e];
+  __syncthreads();
+  for (int t = threadIdx.x; t < nr * per_row; t += 256) {
+    const int r = t / per_row;
+    const int kc = t - r * per_row;
+    const int y = y0 + r;
+    const float* s0 = cprow + (AY.span[y] - ky_lo) * per_row + kc;
+    float a = 0.f;
+    for (int j = 0; j <= q; ++j) a = fmaf(AY.bw[y * 4 + j], s0[j * per_row], a);
+    t1[t] = a;
+  }
+  __syncthreads();
+  float nrm = 0.f;
+  for (int t = threadIdx.x; t < nr * n0; t += 256) {
+    const int r = t / n0;
+    const int x = t - r * n0;
+    const float* s0 = t1 + (r * m0 + spl[x]) * D;
+    const float4 w = *reinterpret_cast<const float4*>(bwl + 4 * x);
+    const float wj[4] = {w.x, w.y, w.z, w.w};
+    float acc[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) acc[c] = wj[0] * s0[c];
+#pragma unroll
+    for (int j = 1; j < 4; ++j)
+      if (j <= q) {
+#pragma unroll
+        for (int c = 0; c < D; ++c) acc[c] = fmaf(wj[j], s0[j * D + c], acc[c]);
+      }
+    float s2 = 0.f;
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      dst[static_cast<int64_t>(t) * D + c] = acc[c];
+      const float v = acc[c] * isp[c];
+      s2 += v * v;
+    }
+    nrm = fmaxf(nrm, sqrtf(s2));
+  }
+  if (part) {
+    double dummy = 0.0;
+    block_sum_max<256>(dummy, nrm);
+    if (threadIdx.x == 0) part[blockIdx.z * (gridDim.x * gridDim.y) + blockIdx.y * gridDim.x + blockIdx.x] = nrm;
+  }
+}
+
 // One block per field: the max of nb partials into maxbits[field].
 __global__ void __launch_bounds__(1024) k_max_partials(const float* __restrict__ part, int nb, unsigned int* maxbits) {
   float m = 0.f;
@@ -2644,6 +2714,31 @@ void launch_bsp_eval_axis(const BspDev& T, int np, const float* const* in, float
 }
 
 int64_t bsp_partials_needed() { return kBspEvalBlocks; }
+
+bool launch_bsp_eval_yx(const BspDev& T, int np, const float* const* in, float* const* out, int n2,
+                        const float sp[3], unsigned int* maxbits, float* part, hipStream_t s) {
+  const BspAxisDev &AX = T.ax[0], &AY = T.ax[1];
+  const int64_t per_row = static_cast<int64_t>(AX.m) * T.D;
+  const int spans = AY.m - T.q;
+  // enough rows per block that the partials fit; the control-point rows a block needs
+  int R = 8;
+  while ((static_cast<int64_t>(AY.n + R - 1) / R) * n2 > kBspEvalBlocks && R < AY.n) R *= 2;
+  if ((static_cast<int64_t>(AY.n + R - 1) / R) * n2 > kBspEvalBlocks) return false;
+  const int64_t KR = std::min<int64_t>(AY.m, (static_cast<int64_t>(R - 1) * spans) / std::max(AY.n - 1, 1) + T.q + 2);
+  const int64_t lds = (5 * static_cast<int64_t>(AX.n) + (R + KR) * per_row) * 4;
+  if (lds > 65536 || n2 > 65535) return false;
+  const dim3 grid(static_cast<unsigned int>((AY.n + R - 1) / R), static_cast<unsigned int>(n2), np);
+  float* pp = maxbits ? part : nullptr;
+  if (T.D == 2)
+    hipLaunchKernelGGL(k_bsp_eval_yx<2>, grid, dim3(256), lds, s, AX, AY, pack(in, np), mpack(out, np), R, T.q,
+                       sp[0], sp[1], sp[2], pp);
+  else
+    hipLaunchKernelGGL(k_bsp_eval_yx<3>, grid, dim3(256), lds, s, AX, AY, pack(in, np), mpack(out, np), R, T.q,
+                       sp[0], sp[1], sp[2], pp);
+  if (maxbits)
+    hipLaunchKernelGGL(k_max_partials, dim3(np), dim3(1024), 0, s, part, static_cast<int>(grid.x * grid.y), maxbits);
+  return true;
+}
 
 void launch_compose(const GridDev& g, int np, const float* const* disp, const float* const* warp, float* const* out,
                     const unsigned int* maxbits, float rate, hipStream_t s) {

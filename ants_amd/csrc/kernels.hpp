@@ -208,6 +208,11 @@ void launch_bsp_fit_axis(const BspDev& T, int np, const float* const* in, float*
 void launch_bsp_eval_axis(const BspDev& T, int np, const float* const* in, float* const* out, const int dims[3],
                           int axis, const float sp[3], unsigned int* maxbits, float* part, hipStream_t s);
 int64_t bsp_partials_needed();
+// Fused evaluation along y and x from the y-unevaluated lattice (m0, m1,
+// n2) to the field, with the max norm as above (false when the rows'
+// tables exceed LDS; then use the separate passes).
+bool launch_bsp_eval_yx(const BspDev& T, int np, const float* const* in, float* const* out, int n2,
+                        const float sp[3], unsigned int* maxbits, float* part, hipStream_t s);
 
 // out_k = warp_k + scale_k * disp_k(x + warp_k(x)), scale_k = rate / max_k when maxbits given (max>0), else 1.
 void launch_compose(const GridDev& g, int np, const float* const* disp, const float* const* warp,
