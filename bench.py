@@ -52,8 +52,10 @@ def log(*a):
 SPAN_EVERY = 4
 
 def cpu_baseline(pair, gpu_iters, scale=0.5):
-    """The oracle's FP32 build of the same algorithm (window sums and
-    coordinates in double: a restatement, not ANTs), all host threads, on
+    """The oracle's FP32 build of the same algorithm (images, fields, CC
+    window sums and the per-voxel CC formula in float as ITK's float metric
+    computes them; sampling coordinates in double: a restatement, not
+    ANTs), all host threads, on
     the same inputs with the GPU run's per-level iteration counts scaled by
     `scale` and forced, so that the level mix matches the timed GPU work
     (about 1.2e8 voxel-iterations, some 10-15 s on 16 threads).  Rates are
@@ -79,7 +81,7 @@ def cpu_baseline(pair, gpu_iters, scale=0.5):
     return {"value": work / dt, "unit": "voxel-iters/s", "cores": threads, "kind": "port",
             "sample": f"the 192x224x192 pair with the GPU run's per-level iteration counts {list(gpu_iters)} "
                       f"scaled by {scale} and forced ({iters}); FP32 CPU restatement of the SyN stage "
-                      f"(oracle/syn_oracle.cpp, window sums and coordinates in double, not ANTs), {threads} OpenMP "
+                      f"(oracle/syn_oracle.cpp: float images, fields and CC sums, double coordinates; not ANTs), {threads} OpenMP "
                       f"threads, {work:.3g} voxel-iterations in {dt:.1f} s",
             "per_level": per_level, "seconds": dt}
 

@@ -26,6 +26,7 @@
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <type_traits>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
@@ -616,11 +617,16 @@ double cc_metric(const Grid& g, const R* F, const R* M, int radius, double eps, 
   const int D = g.D;
   const int64_t N = g.count();
   const int64_t nx = g.n[0], ny = g.n[1], nz = g.n[2];
-  // 6 quantities: 1, f, m, ff, mm, fm
-  std::vector<double> q(N * 6), t(N * 6);  // window sums in double (the GPU does the same)
+  // 6 quantities: 1, f, m, ff, mm, fm.  Window sums and the per-voxel
+  // formula in A: double for the FP64 oracle (the GPU's sums are double
+  // too); float in the FP32 model, as ITK's metric computes in its float
+  // InternalComputationValueType under --float 1 (with separable box sums
+  // here, not ITK's sliding queue, so the rounding differs)
+  using A = typename std::conditional<std::is_same<R, float>::value, float, double>::type;
+  std::vector<A> q(N * 6), t(N * 6);
   for (int64_t i = 0; i < N; ++i) {
-    const double f = F[i], m = M[i];
-    double* o = &q[i * 6];
+    const A f = F[i], m = M[i];
+    A* o = &q[i * 6];
     o[0] = 1;
     o[1] = f;
     o[2] = m;
@@ -638,9 +644,9 @@ double cc_metric(const Grid& g, const R* F, const R* M, int radius, double eps, 
           const int64_t idx = x + nx * (y + ny * z);
           const int64_t pos = ax == 0 ? x : (ax == 1 ? y : z);
           const int64_t lo = std::max<int64_t>(0, pos - radius), hi = std::min<int64_t>(len - 1, pos + radius);
-          double acc[6] = {0, 0, 0, 0, 0, 0};
+          A acc[6] = {0, 0, 0, 0, 0, 0};
           for (int64_t p = lo; p <= hi; ++p) {
-            const double* s = &q[(idx + (p - pos) * stride) * 6];
+            const A* s = &q[(idx + (p - pos) * stride) * 6];
             for (int k = 0; k < 6; ++k) acc[k] += s[k];
           }
           for (int k = 0; k < 6; ++k) t[idx * 6 + k] = acc[k];
@@ -655,14 +661,14 @@ double cc_metric(const Grid& g, const R* F, const R* M, int radius, double eps, 
     for (int64_t y = 0; y < ny; ++y)
       for (int64_t x = 0; x < nx; ++x) {
         const int64_t i = x + nx * (y + ny * z);
-        const double* s = &q[i * 6];
-        const double cnt = s[0], sf = s[1], sm = s[2], sff = s[3], smm = s[4], sfm = s[5];
-        const double mf = sf / cnt, mm = sm / cnt;
-        const double sFF = sff - mf * sf - mf * sf + cnt * mf * mf;
-        const double sMM = smm - mm * sm - mm * sm + cnt * mm * mm;
-        const double sFM = sfm - mm * sf - mf * sm + cnt * mm * mf;
-        const double fA = F[i] - mf, mA = M[i] - mm;
-        double cc = 1;
+        const A* s = &q[i * 6];
+        const A cnt = s[0], sf = s[1], sm = s[2], sff = s[3], smm = s[4], sfm = s[5];
+        const A mf = sf / cnt, mm = sm / cnt;
+        const A sFF = sff - mf * sf - mf * sf + cnt * mf * mf;
+        const A sMM = smm - mm * sm - mm * sm + cnt * mm * mm;
+        const A sFM = sfm - mm * sf - mf * sm + cnt * mm * mf;
+        const A fA = F[i] - mf, mA = M[i] - mm;
+        A cc = 1;
         R gm[3] = {0, 0, 0}, gf[3] = {0, 0, 0};
         if (sFF > eps && sMM > eps) {
           cc = sFM * sFM / (sFF * sMM);
@@ -672,7 +678,7 @@ double cc_metric(const Grid& g, const R* F, const R* M, int radius, double eps, 
             gm[d] = grad_component<R>(M, i, pos[d], g.n[d], stride, g.sp[d]);
             gf[d] = grad_component<R>(F, i, pos[d], g.n[d], stride, g.sp[d]);
           }
-          const double a = 2.0 * sFM / (sFF * sMM);
+          const A a = A(2) * sFM / (sFF * sMM);
           const R c1 = static_cast<R>(a * (fA - sFM / sMM * mA));
           const R c2 = static_cast<R>(a * (mA - sFM / sFF * fA));
           for (int d = 0; d < D; ++d) {
