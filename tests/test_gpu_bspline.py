@@ -143,3 +143,36 @@ def test_bspline_syn_same_bits_pipelined_and_replayed(monkeypatch, mode):
         np.testing.assert_array_equal(x.inv, y.inv)
         assert [r.full_scale_cc for r in x.trace] == [r.full_scale_cc for r in y.trace]
     assert sum(1 for r in a.trace if r.full_scale_cc != 0.0) >= 6
+
+
+def test_bspline_syn_split_stage_with_initial_affine():
+    """BSplineSyN with a moving initial affine: 4x2 (meshes 4, 8) saving the
+    state, then -f 1 with mesh 16 restored from it, equals the 4x2x1 stage
+    (meshes 4, 8, 16) bit for bit."""
+    from ants_amd.apply_transforms import AffineTransform
+    pair = ph.phantom_pair_3d(14, size=(40, 44, 36), max_disp_mm=4.0, svf_sigma_mm=6.0)
+    c = np.array(pair.size, float) / 2
+    aff = AffineTransform(np.array([[1.02, 0.01, 0.0], [-0.01, 0.99, 0.02], [0.0, 0.01, 1.01]]), [0.4, -0.3, 0.2], c)
+    st = lambda mesh, it, f, s_: SyNStage.from_args(f"BSplineSyN[0.2,{mesh},0,3]", "CC[f,m,1,2]",  # noqa: E731
+                                                    f"[{it},1e-6,10]", f, s_)
+
+    def run(stage, state=None, grid=None):
+        reg = SyNRegistration(stage, 3, force_iterations=True)
+        reg.set_initial_transforms(moving=[aff])
+        if state is not None:
+            reg.set_initial_state(state, *grid)
+        res = reg.register(pair.fixed, pair.moving, pair.spacing, pair.origin, pair.spacing, pair.origin)
+        lg = reg.level_geometry(pair.size, pair.spacing, pair.origin)
+        size = [int(lg.size[k]) for k in range(3)]
+        out = reg.state(tuple(reversed(size)) + (3,)), (size, [lg.spacing[k] for k in range(3)],
+                                                        [lg.origin[k] for k in range(3)])
+        reg.close()
+        return res, out
+
+    full, _ = run(st("4x4x4", "5x4x3", "4x2x1", "2x1x0vox"))
+    _, (state, grid) = run(st("4x4x4", "5x4", "4x2", "2x1vox"))
+    b, _ = run(st("16x16x16", "3", "1", "0vox"), state, grid)
+    np.testing.assert_array_equal(b.energy, full.energy[-3:])
+    np.testing.assert_array_equal(b.fwd, full.fwd)
+    np.testing.assert_array_equal(b.inv, full.inv)
+    assert np.abs(full.fwd).max() > 0.1
