@@ -152,6 +152,7 @@ def lib():
     L.ants_op_smooth_image.argtypes = [pdesc, vp, vp, d, i, vp]
     L.ants_op_smooth_field.argtypes = [pdesc, vp, vp, d, vp]
     L.ants_op_bspline_field.argtypes = [pdesc, vp, vp, C.POINTER(i), i, C.POINTER(d), vp]
+    L.ants_image_center_of_mass.argtypes = [pdesc, vp, C.POINTER(d), C.POINTER(d), vp]
     L.ants_op_warp_image.argtypes = [pdesc, vp, pdesc, vp, vp, vp]
     L.ants_op_cc.argtypes = [pdesc, vp, vp, i, vp, vp, C.POINTER(d), vp]
     L.ants_op_warp_pair.argtypes = [pdesc, pdesc, vp, vp, vp, pdesc, vp, vp, vp, i, vp]

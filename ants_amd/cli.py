@@ -7,7 +7,9 @@ runs one SyN stage with the CC metric (the antsRegistration subset this
 library replaces) and writes ``out_0Warp.nii.gz`` and
 ``out_0InverseWarp.nii.gz`` (naming of Examples/
 antsRegistrationTemplateHeader.cxx:74-89) plus the optional warped images.
-``-r affine.mat`` (repeatable, ``[file,1]`` inverts) gives the stage its
+``-r affine.mat`` (repeatable, ``[file,1]`` inverts; ``[fixed,moving,f]``
+computes one from the images: f = 0 image centres, 1 centres of mass, 2
+origins) gives the stage its
 moving initial transform, as the preceding linear stages do; the outputs are
 then ``out_0GenericAffine.mat`` (the initial transforms composed),
 ``out_1Warp.nii.gz`` and ``out_1InverseWarp.nii.gz``.
@@ -51,6 +53,13 @@ def load_transforms(spec: str) -> list:
     order ITK's CompositeTransform applies them)."""
     parts = _bracket_list(spec)
     path = parts[0]
+    if len(parts) == 3 and parts[2] in ("0", "1", "2"):
+        # [fixed,moving,feature]: an initial transform computed from the images
+        from .registration import initial_transform_from_images
+        fi, gfi = aio.read_nifti(parts[0])
+        mi, gmi = aio.read_nifti(parts[1])
+        return [initial_transform_from_images(np.asarray(fi, np.float32), gfi, np.asarray(mi, np.float32), gmi,
+                                              int(parts[2]))]
     if path.endswith(".h5"):
         from . import h5transform
         if len(parts) > 1 and parts[1] not in ("0", "false", "False", ""):

@@ -2168,6 +2168,34 @@ int ants_op_smooth_field(const ants_image_desc* gdsc, const float* d_in, float* 
   return rc;
 }
 
+int ants_image_center_of_mass(const ants_image_desc* gdsc, const float* d_img, double* center, double* mass,
+                              void* stream) {
+  HostGrid g;
+  ANTS_TRY(to_host_grid(gdsc, &g));
+  if (!d_img || !center) return fail(ANTS_ERR_INVALID, "null buffer");
+  DevBuf part;
+  ANTS_TRY(part.ensure(static_cast<size_t>(kRedBlocks) * 4 * sizeof(double)));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int nb = launch_moments(to_dev(g), d_img, static_cast<double*>(part.p), s);
+  HIP_TRY(hipGetLastError());
+  std::vector<double> h(static_cast<size_t>(nb) * 4);
+  HIP_TRY(hipMemcpyAsync(h.data(), part.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  double m[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int b = 0; b < nb; ++b)
+    for (int k = 0; k < 4; ++k) m[k] += h[static_cast<size_t>(b) * 4 + k];
+  if (mass) *mass = m[0];
+  if (m[0] == 0.0) return fail(ANTS_ERR_INVALID, "the image's total intensity is zero: no centre of mass");
+  // physical point of the mean index: origin + direction diag(spacing) index
+  const double idx[3] = {m[1] / m[0], m[2] / m[0], m[3] / m[0]};
+  for (int r = 0; r < g.D; ++r) {
+    double v = g.org[r];
+    for (int c = 0; c < g.D; ++c) v += g.dir[r * 3 + c] * g.sp[c] * idx[c];
+    center[r] = v;
+  }
+  return ANTS_OK;
+}
+
 int ants_op_bspline_field(const ants_image_desc* gdsc, const float* d_in, float* d_out, const int* mesh, int order,
                           double* host_max, void* stream) {
   HostGrid g;

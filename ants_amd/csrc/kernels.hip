@@ -2219,6 +2219,39 @@ __global__ void __launch_bounds__(256) k_bsp_eval_yx(BspAxisDev AX, BspAxisDev A
   }
 }
 
+// Image moments for CenteredTransformInitializer::MomentsOn (ImageMoments-
+// Calculator's centre of gravity): per block, in double, the sums of I and
+// of I times each index coordinate; out[block * 4 + {0, 1, 2, 3}].  The
+// host sums the blocks in order and maps the index centroid to physical
+// space (a linear map, so it commutes with the mean).
+__global__ void __launch_bounds__(kBlock) k_moments(GridDev g, const float* __restrict__ img, double* out) {
+  const int64_t N = g.count();
+  double a[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < N;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    int x, y, z;
+    unlin(g, i, x, y, z);
+    const double v = img[i];
+    a[0] += v;
+    a[1] += v * x;
+    a[2] += v * y;
+    a[3] += v * z;
+  }
+  __shared__ double sh[4][kBlock / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const double w = wave_sum(a[k]);
+    if (lane == 0) sh[k][wid] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    double t = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) t += sh[threadIdx.x][w];
+    out[blockIdx.x * 4 + threadIdx.x] = t;
+  }
+}
+
 // One block per field: the max of nb partials into maxbits[field].
 __global__ void __launch_bounds__(1024) k_max_partials(const float* __restrict__ part, int nb, unsigned int* maxbits) {
   float m = 0.f;
@@ -2714,6 +2747,12 @@ void launch_bsp_eval_axis(const BspDev& T, int np, const float* const* in, float
 }
 
 int64_t bsp_partials_needed() { return kBspEvalBlocks; }
+
+int launch_moments(const GridDev& g, const float* img, double* out, hipStream_t s) {
+  const unsigned int blocks = red_blocks(g.count());
+  hipLaunchKernelGGL(k_moments, dim3(blocks), dim3(kBlock), 0, s, g, img, out);
+  return static_cast<int>(blocks);
+}
 
 bool launch_bsp_eval_yx(const BspDev& T, int np, const float* const* in, float* const* out, int n2,
                         const float sp[3], unsigned int* maxbits, float* part, hipStream_t s) {

@@ -208,6 +208,9 @@ void launch_bsp_fit_axis(const BspDev& T, int np, const float* const* in, float*
 void launch_bsp_eval_axis(const BspDev& T, int np, const float* const* in, float* const* out, const int dims[3],
                           int axis, const float sp[3], unsigned int* maxbits, float* part, hipStream_t s);
 int64_t bsp_partials_needed();
+// Per-block sums of I, I x, I y, I z over an image (index coordinates),
+// out[block * 4 + k]; returns the block count (at most kRedBlocks).
+int launch_moments(const GridDev& g, const float* img, double* out, hipStream_t s);
 // Fused evaluation along y and x from the y-unevaluated lattice (m0, m1,
 // n2) to the field, with the max norm as above (false when the rows'
 // tables exceed LDS; then use the separate passes).

@@ -412,6 +412,48 @@ class SyNRegistration:
         return [b.to_host() for b in bufs]
 
 
+def center_of_mass(image: np.ndarray, size, spacing=None, origin=None, direction=None) -> np.ndarray:
+    """ImageMomentsCalculator's centre of gravity of an image (numpy (z, y,
+    x) order), on the GPU (ants_image_center_of_mass)."""
+    d = L.DeviceArray.from_host(np.ascontiguousarray(image, np.float32))
+    g = L.desc(size, spacing, origin, direction)
+    D = len(size)
+    c = (C.c_double * 3)()
+    L.check(L.lib().ants_image_center_of_mass(C.byref(g), d.ptr, c, None, None))
+    return np.array([c[k] for k in range(D)])
+
+
+def initial_transform_from_images(fixed: np.ndarray, fixed_geom, moving: np.ndarray, moving_geom, feature: int):
+    """antsRegistration -r [fixed,moving,feature] (Examples/itkantsRegistrationHelper.h:1296-1360):
+    a rigid transform with no rotation, centred at the fixed image's feature
+    point and translated to the moving image's: 0 image centres (the
+    continuous index (size - 1) / 2, CenteredTransformInitializer::GeometryOn),
+    1 centres of mass (MomentsOn), 2 image origins.  Geometries are
+    apply_transforms.Geometry objects; returns an AffineTransform."""
+    from .apply_transforms import AffineTransform
+
+    def geom_point(geo, idx):
+        D = geo.dim
+        sp = np.ones(D) if geo.spacing is None else np.asarray(geo.spacing, float)
+        org = np.zeros(D) if geo.origin is None else np.asarray(geo.origin, float)
+        dm = np.eye(D) if geo.direction is None else np.asarray(geo.direction, float).reshape(D, D)
+        return org + dm @ (sp * np.asarray(idx, float))
+
+    D = fixed_geom.dim
+    if feature == 0:
+        cf = geom_point(fixed_geom, (np.asarray(fixed_geom.size, float) - 1) / 2)
+        cm = geom_point(moving_geom, (np.asarray(moving_geom.size, float) - 1) / 2)
+    elif feature == 1:
+        cf = center_of_mass(fixed, fixed_geom.size, fixed_geom.spacing, fixed_geom.origin, fixed_geom.direction)
+        cm = center_of_mass(moving, moving_geom.size, moving_geom.spacing, moving_geom.origin, moving_geom.direction)
+    elif feature == 2:
+        cf = geom_point(fixed_geom, np.zeros(D))
+        cm = geom_point(moving_geom, np.zeros(D))
+    else:
+        raise ValueError("initialization feature must be 0 (centres), 1 (centres of mass) or 2 (origins)")
+    return AffineTransform(np.eye(D), cm - cf, cf)
+
+
 def registration_from_cli(args: Sequence[str]) -> Tuple[SyNRegistration, SyNStage]:
     """Build a SyNRegistration from antsRegistration-style arguments
     (`-d 3 -m CC[f,m,1,4] -t SyN[0.1,3,0] -c [...] -f 4x2x1 -s 2x1x0vox`)."""

@@ -277,6 +277,16 @@ int ants_op_smooth_image(const ants_image_desc* g, const float* d_in, float* d_o
  * (variance set at itkantsRegistrationHelper.hxx:2161-2162). */
 int ants_op_smooth_field(const ants_image_desc* g, const float* d_in, float* d_out, double variance, void* stream);
 
+/* The centre of gravity of an image (ImageMomentsCalculator::GetCenterOf-
+ * Gravity: the intensity-weighted mean physical point), as
+ * CenteredTransformInitializer::MomentsOn uses it for antsRegistration
+ * -r [fixed,moving,1] (Examples/itkantsRegistrationHelper.h:1296-1348: the
+ * initial transform centred at the fixed image's centre of gravity,
+ * translated by moving minus fixed).  center[dim] receives the point, mass
+ * (optional) the intensity total; ANTS_ERR_INVALID when the total is 0. */
+int ants_image_center_of_mass(const ants_image_desc* g, const float* d_img, double* center, double* mass,
+                              void* stream);
+
 /* BSplineSyN's BSplineSmoothDisplacementField(field, mesh + order control
  * points): DisplacementFieldToBSplineImageFilter with one fitting level,
  * stationary boundary (boundary voxels fitted as zero vectors of weight

@@ -95,3 +95,48 @@ def test_zero_direction_means_identity():
     a = AT.apply_transforms(g_id, img, g_id, xf[1:2], "Linear")
     b = AT.apply_transforms(g_zero, img, g_zero, xf[1:2], "Linear")
     np.testing.assert_array_equal(a, b)
+
+
+def test_center_of_mass_and_image_initializers():
+    """ants_image_center_of_mass against numpy's intensity-weighted mean
+    physical point (oblique, anisotropic grid); -r [fixed,moving,f]
+    initializers: an image shifted by whole voxels gives that shift as the
+    translation (f = 1), centres (f = 0) and origins (f = 2) by closed form."""
+    from ants_amd import _lib as L
+    from ants_amd.registration import center_of_mass, initial_transform_from_images
+    Geometry = AT.Geometry
+    rng = np.random.default_rng(4)
+    size = (30, 26, 22)
+    sp = (1.1, 0.9, 1.3)
+    org = (5.0, -3.0, 2.0)
+    th = 0.3
+    R = np.array([[np.cos(th), -np.sin(th), 0], [np.sin(th), np.cos(th), 0], [0, 0, 1]])
+    img = rng.uniform(0, 1, (22, 26, 30)).astype(np.float32)
+    img[5:15, 6:20, 8:25] += 3.0
+    c = center_of_mass(img, size, sp, org, R)
+    z, y, x = np.meshgrid(*[np.arange(n) for n in img.shape], indexing="ij")
+    idx = np.stack([x, y, z], -1).reshape(-1, 3).astype(np.float64)
+    phys = np.asarray(org) + (idx * np.asarray(sp)) @ R.T
+    w = img.reshape(-1).astype(np.float64)
+    ref = (phys * w[:, None]).sum(0) / w.sum()
+    np.testing.assert_allclose(c, ref, rtol=0, atol=1e-9)
+    # a blob moved by (3, -2, 1) voxels inside a zero background
+    fixed = np.zeros((22, 26, 30), np.float32)
+    fixed[6:14, 8:16, 9:19] = rng.uniform(1, 2, (8, 8, 10))
+    moving = np.zeros_like(fixed)
+    moving[7:15, 6:14, 12:22] = fixed[6:14, 8:16, 9:19]
+    g = Geometry(size, sp, org, R)
+    t = initial_transform_from_images(fixed, g, moving, g, 1)
+    np.testing.assert_allclose(t.translation, R @ (np.array([3, -2, 1]) * np.asarray(sp)), atol=1e-9)
+    np.testing.assert_allclose(t.matrix, np.eye(3))
+    np.testing.assert_allclose(t.center, center_of_mass(fixed, size, sp, org, R), atol=1e-12)
+    gm = Geometry((20, 26, 40), (1.0, 1.0, 1.0), (1.0, 2.0, 3.0), np.eye(3))
+    t0 = initial_transform_from_images(fixed, g, np.zeros((40, 26, 20), np.float32), gm, 0)
+    cf = np.asarray(org) + R @ (np.asarray(sp) * (np.array(size) - 1) / 2)
+    cm = np.array([1.0, 2.0, 3.0]) + (np.array([20, 26, 40]) - 1) / 2
+    np.testing.assert_allclose(t0.center, cf, atol=1e-12)
+    np.testing.assert_allclose(t0.translation, cm - cf, atol=1e-12)
+    t2 = initial_transform_from_images(fixed, g, moving, gm, 2)
+    np.testing.assert_allclose(t2.translation, np.array([1.0, 2.0, 3.0]) - np.asarray(org), atol=1e-12)
+    with pytest.raises(L.AntsError):
+        center_of_mass(np.zeros((4, 4, 4), np.float32), (4, 4, 4))

@@ -178,3 +178,28 @@ def test_composite_outputs_and_state_files(tmp_path):
     with pytest.raises(SystemExit):
         cli.main(["registration"] + base + ["-c", "[3,1e-6,10]", "-f", "1", "-s", "0vox", "-o", b_,
                                             "--restore-state", st, "-r", one + "Composite.h5"])
+
+
+def test_registration_initial_transform_from_images(tmp_path):
+    """-r [fixed,moving,1]: the centre-of-mass initial transform (written
+    composed into 0GenericAffine.mat) equals initial_transform_from_images,
+    and the stage runs with it."""
+    from ants_amd.registration import initial_transform_from_images
+    pair = ph.phantom_pair_3d(7, size=(40, 44, 36), max_disp_mm=3.0, svf_sigma_mm=6.0)
+    g = Geometry(pair.size, pair.spacing, (0.0, 0.0, 0.0), np.eye(3))
+    gm = Geometry(pair.size, pair.spacing, (2.5, -1.5, 1.0), np.eye(3))
+    f, m = str(tmp_path / "fixed.nii.gz"), str(tmp_path / "moving.nii.gz")
+    aio.write_nifti(f, pair.fixed, g)
+    aio.write_nifti(m, pair.moving, gm)
+    prefix = str(tmp_path / "c_")
+    assert cli.main(["registration", "-d", "3", "-m", f"CC[{f},{m},1,2]", "-t", "SyN[0.2,3,0]", "-c", "[4x2,1e-6,10]",
+                     "-f", "2x1", "-s", "1x0vox", "-o", prefix, "-r", f"[{f},{m},1]"]) == 0
+    aff = aio.read_transform(prefix + "0GenericAffine.mat")
+    fi, gfi = aio.read_nifti(f)
+    mi, gmi = aio.read_nifti(m)
+    ref = initial_transform_from_images(np.asarray(fi, np.float32), gfi, np.asarray(mi, np.float32), gmi, 1)
+    A, t, c = ref.parameters()
+    p = np.array([3.0, -4.0, 5.0])
+    np.testing.assert_allclose(aff.matrix @ (p - aff.center) + aff.translation + aff.center, A @ (p - c) + t + c,
+                               atol=1e-5)
+    assert np.abs(t).max() > 1.0
