@@ -14,6 +14,15 @@ moving initial transform, as the preceding linear stages do; the outputs are
 then ``out_0GenericAffine.mat`` (the initial transforms composed),
 ``out_1Warp.nii.gz`` and ``out_1InverseWarp.nii.gz``.
 
+Linear stages come first, each with its own option group, as
+antsRegistrationSyN.sh writes them (Scripts/antsRegistrationSyN.sh:487-497):
+``-t Rigid[0.1] -m Mattes[fixed,moving,1,32,Regular,0.25] -c [1000x500x250x100,1e-6,10]
+  -f 8x4x2x1 -s 3x2x1x0vox -t Affine[0.1] -m ... -t SyN[0.1,3,0] -m CC[...] ...``
+(Translation, Rigid or Affine with MeanSquares, GC or Mattes;
+``--random-seed`` seeds their metric sampling).  Their transforms, composed
+with the -r initial transforms, become ``out_0GenericAffine.mat``; with no
+SyN stage that file is the whole output.
+
 ``python -m ants_amd.cli apply -d 3 -i moving.nii.gz -r fixed.nii.gz
   -o out.nii.gz -t out_0Warp.nii.gz -t [affine.mat,1] -n Linear
   --default-value 0``
@@ -116,14 +125,16 @@ def cmd_apply(argv: Sequence[str]) -> int:
 
 
 def cmd_registration(argv: Sequence[str]) -> int:
+    from .linear import LinearStage, register_linear
     from .registration import SyNRegistration, SyNStage
     ap = argparse.ArgumentParser(prog="registration")
     ap.add_argument("-d", "--dimensionality", type=int, required=True)
-    ap.add_argument("-m", "--metric", required=True)
-    ap.add_argument("-t", "--transform", required=True)
-    ap.add_argument("-c", "--convergence", required=True)
-    ap.add_argument("-f", "--shrink-factors", required=True)
-    ap.add_argument("-s", "--smoothing-sigmas", required=True)
+    # one -t/-m/-c/-f/-s group per stage, in the order the stages run
+    ap.add_argument("-m", "--metric", action="append", required=True)
+    ap.add_argument("-t", "--transform", action="append", required=True)
+    ap.add_argument("-c", "--convergence", action="append", required=True)
+    ap.add_argument("-f", "--shrink-factors", action="append", required=True)
+    ap.add_argument("-s", "--smoothing-sigmas", action="append", required=True)
     ap.add_argument("-o", "--output", required=True)
     ap.add_argument("-r", "--initial-moving-transform", action="append", default=[],
                     help="affine transform file or [file,useInverse]; repeated options compose in -t order")
@@ -131,6 +142,7 @@ def cmd_registration(argv: Sequence[str]) -> int:
                     help="affine transform file or [file,useInverse] applied to the fixed image's points")
     ap.add_argument("--float", default="1")
     ap.add_argument("--precision", choices=["fp32", "bf16"], default="fp32")
+    ap.add_argument("--random-seed", type=int, default=0, help="seed of the linear stages' metric sampling")
     ap.add_argument("-a", "--write-composite-transform", type=int, default=0,
                     help="1: write <prefix>Composite.h5 and <prefix>InverseComposite.h5")
     ap.add_argument("--save-state", default=None, help="write the registration state (.h5)")
@@ -139,18 +151,28 @@ def cmd_registration(argv: Sequence[str]) -> int:
                     help="full-scale CC r=4 of the original images every N iterations (0: off)")
     ap.add_argument("-v", "--verbose", type=int, default=0)
     a = ap.parse_args(argv)
-    mparts = _bracket_list(re.sub(r"^\s*CC", "", a.metric))
-    fixed, gf = aio.read_nifti(mparts[0])
-    moving, gm = aio.read_nifti(mparts[1])
-    # a BSplineSyN knot spacing becomes a mesh with the fixed image's geometry
-    # (antsRegistrationTemplateHeader.h:709-731)
-    stage = SyNStage.from_args(a.transform, a.metric, a.convergence, a.shrink_factors, a.smoothing_sigmas,
-                               gf.size, gf.spacing)
+    n = len(a.transform)
+    if not (len(a.metric) == len(a.convergence) == len(a.shrink_factors) == len(a.smoothing_sigmas) == n):
+        raise SystemExit("each stage needs one -t, -m, -c, -f and -s (in stage order)")
+    linear = [LinearStage.is_linear(t) for t in a.transform]
+    if any(not lin for lin in linear[:-1]):
+        raise SystemExit("a SyN / BSplineSyN stage must be the last stage")
+    images = {}
+
+    def image(name):
+        if name not in images:
+            images[name] = aio.read_nifti(name)
+        return images[name]
+
+    mparts = _bracket_list(re.sub(r"^\s*[A-Za-z]+", "", a.metric[-1]))
+    fixed, gf = image(mparts[0])
+    moving, gm = image(mparts[1])
     for g, name in ((gf, mparts[0]), (gm, mparts[1])):
         if g.dim != a.dimensionality:
             raise SystemExit(f"{name} is {g.dim}-D, -d {a.dimensionality}")
     outs = _bracket_list(a.output)
     prefix = outs[0]
+
     def initial(specs):
         return [t for spec in specs for t in load_transforms(spec)]
 
@@ -162,10 +184,52 @@ def cmd_registration(argv: Sequence[str]) -> int:
             # antsRegistrationTemplateHeader.h:295-305
             raise SystemExit("restore-state option is mutually exclusive with initial-moving-transform & "
                              "initial-fixed-transform options")
+        if any(linear):
+            raise SystemExit("restore-state continues a SyN stage; linear stages cannot follow it here")
         from . import h5transform
         lin, state, gst = h5transform.read_syn_state(a.restore_state)
         init = list(reversed(lin))
         restored = (state, gst)
+    # linear stages: each stage's transform is applied before the transforms
+    # gathered so far (the composite applies the newest transform first)
+    for k in range(n):
+        if not linear[k]:
+            continue
+        st = LinearStage.from_args(a.transform[k], a.metric[k], a.convergence[k], a.shrink_factors[k],
+                                   a.smoothing_sigmas[k], a.random_seed)
+        fk, gfk = image(st.fixed)
+        mk, gmk = image(st.moving)
+        if any(isinstance(t, DisplacementFieldTransform) for t in init + finit):
+            raise SystemExit("linear stages take affine initial transforms only")
+        res = register_linear(st, np.asarray(fk, np.float32), gfk, np.asarray(mk, np.float32), gmk, init, finit)
+        if a.verbose:
+            print(f"*** linear stage {k}: {a.transform[k]} {a.metric[k]} ***")
+            for ln in res.diagnostic_lines():
+                print(ln)
+        init = [res.transform] + init
+    if all(linear):
+        if finit and (len(outs) > 1 and any(outs[1:])):
+            raise SystemExit("-q with warped-image outputs is not supported (write the transform only)")
+        if a.save_state or a.restore_state or a.print_similarity_measure_interval:
+            raise SystemExit("--save-state / --restore-state / the full-scale CC need a SyN stage")
+        aff = compose_affines(init) if init else AffineTransform(np.eye(gf.dim), np.zeros(gf.dim), np.zeros(gf.dim))
+        if a.write_composite_transform:
+            from . import h5transform
+            h5transform.write_composite(prefix + "Composite.h5", [aff])
+            h5transform.write_composite(prefix + "InverseComposite.h5",
+                                        [AffineTransform(aff.matrix, aff.translation, aff.center, invert=True)])
+        else:
+            aio.write_transform(prefix + "0GenericAffine.mat", aff)
+        if len(outs) > 1 and outs[1]:
+            aio.write_nifti(outs[1], apply_transforms(gf, np.asarray(moving, np.float32), gm, [aff], "Linear"), gf)
+        if len(outs) > 2 and outs[2]:
+            inv = AffineTransform(aff.matrix, aff.translation, aff.center, invert=True)
+            aio.write_nifti(outs[2], apply_transforms(gm, np.asarray(fixed, np.float32), gf, [inv], "Linear"), gm)
+        return 0
+    # a BSplineSyN knot spacing becomes a mesh with the fixed image's geometry
+    # (antsRegistrationTemplateHeader.h:709-731)
+    stage = SyNStage.from_args(a.transform[-1], a.metric[-1], a.convergence[-1], a.shrink_factors[-1],
+                               a.smoothing_sigmas[-1], gf.size, gf.spacing)
     if finit and (len(outs) > 1 and any(outs[1:])):
         raise SystemExit("-q with warped-image outputs is not supported (write the warp files only)")
     if any(isinstance(t, DisplacementFieldTransform) for t in init) and not a.write_composite_transform:

@@ -2684,6 +2684,19 @@ int geom_from_desc(const ants_image_desc* d, GeomDev* g) {
 
 }  // namespace
 
+// Host helpers shared with the linear-stage engine (linear.hip).
+namespace ants {
+int host_error(int code, const char* msg) { return fail(code, "%s", msg); }
+int host_geom(const ants_image_desc* d, GeomDev* g) { return geom_from_desc(d, g); }
+int host_level_desc(const ants_image_desc* fixed, int factor, ants_image_desc* out) {
+  HostGrid g;
+  ANTS_TRY(to_host_grid(fixed, &g));
+  if (!out) return fail(ANTS_ERR_INVALID, "null output descriptor");
+  to_desc(level_grid(g, factor), out);
+  return ANTS_OK;
+}
+}  // namespace ants
+
 extern "C" int ants_apply_transforms(const ants_image_desc* ref, const ants_image_desc* in, const float* d_in,
                                      const ants_transform* transforms, int n_transforms, int interpolation,
                                      double default_value, float* d_out, void* stream) {

@@ -2261,6 +2261,238 @@ __global__ void __launch_bounds__(1024) k_max_partials(const float* __restrict__
   if (threadIdx.x == 0) atomicMax(&maxbits[blockIdx.x], __float_as_uint(m));
 }
 
+// ---------------------------------------------------------------------------
+// Linear stages: one pass over the virtual sample points of a level gives the
+// metric's sums at one set of stage parameters (ImageToImageMetricv4 with a
+// Translation / Euler / Affine transform under the moving initial composite).
+// Coordinates, interpolation weights and sums are double; the images float.
+__device__ __forceinline__ bool lin_sample(const GeomDev& g, const float* __restrict__ img, const double p[3],
+                                           double& v) {
+  double c[3];
+  if (g.D == 2) {
+    geom_cidx<2>(g, p, c);
+    if (!geom_inside<2>(g, c)) return false;
+    int64_t lo[3], hi[3];
+    double w[3];
+    lin_setup<2>(g, c, lo, hi, w);
+    double cv[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) cv[k] = static_cast<double>(img[corner<2>(g, lo, hi, k)]);
+    v = lerp_cell<2>(cv, w);
+    return true;
+  }
+  geom_cidx<3>(g, p, c);
+  if (!geom_inside<3>(g, c)) return false;
+  int64_t lo[3], hi[3];
+  double w[3];
+  lin_setup<3>(g, c, lo, hi, w);
+  double cv[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) cv[k] = static_cast<double>(img[corner<3>(g, lo, hi, k)]);
+  v = lerp_cell<3>(cv, w);
+  return true;
+}
+
+// Centred cubic B-spline and its derivative (Parzen window of the Mattes metric).
+__device__ __forceinline__ double bspline3(double u) {
+  const double a = fabs(u);
+  if (a < 1.0) return (4.0 - 6.0 * a * a + 3.0 * a * a * a) / 6.0;
+  if (a < 2.0) return (2.0 - a) * (2.0 - a) * (2.0 - a) / 6.0;
+  return 0.0;
+}
+__device__ __forceinline__ double bspline3_d(double u) {
+  const double a = fabs(u);
+  const double s = u > 0.0 ? 1.0 : (u < 0.0 ? -1.0 : 0.0);
+  if (a < 1.0) return s * (-2.0 * a + 1.5 * a * a);
+  if (a < 2.0) return -s * 0.5 * (2.0 - a) * (2.0 - a);
+  return 0.0;
+}
+
+template <int MODE, bool DERIV, int P>
+constexpr int lin_q() {
+  return MODE == kLinMS ? (DERIV ? 2 + P : 2)
+                        : MODE == kLinGC ? (DERIV ? 6 + 3 * P : 6) : MODE == kLinHist ? 1 : 1 + P;
+}
+
+template <int D, int MODE, bool DERIV, int P>
+__global__ void __launch_bounds__(kLinThreads) k_lin_metric(LinDev L, LinJac J, double* __restrict__ partials,
+                                                            unsigned long long* __restrict__ hist,
+                                                            unsigned int* __restrict__ fcnt) {
+  constexpr int Q = lin_q<MODE, DERIV, P>();
+  constexpr bool GRAD = DERIV && MODE != kLinHist;
+  constexpr int HB = MODE == kLinHist ? kLinLdsBins : 1;
+  __shared__ unsigned long long hsh[HB * HB];
+  __shared__ unsigned int csh[HB];
+  const bool lds_hist = MODE == kLinHist && L.bins <= kLinLdsBins;
+  if (MODE == kLinHist && lds_hist) {
+    for (int k = threadIdx.x; k < L.bins * L.bins; k += blockDim.x) hsh[k] = 0ull;
+    for (int k = threadIdx.x; k < L.bins; k += blockDim.x) csh[k] = 0u;
+    __syncthreads();
+  }
+  double acc[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) acc[q] = 0.0;
+  const int64_t n = L.pts ? L.npts : static_cast<int64_t>(L.vg.n[0]) * L.vg.n[1] * L.vg.n[2];
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    double x[3] = {0.0, 0.0, 0.0};
+    if (L.pts) {
+#pragma unroll
+      for (int d = 0; d < D; ++d) x[d] = L.pts[i * D + d];
+    } else {
+      const int64_t nx = L.vg.n[0], ny = L.vg.n[1];
+      const int ix[3] = {static_cast<int>(i % nx), static_cast<int>((i / nx) % ny), static_cast<int>(i / (nx * ny))};
+      geom_point<D>(L.vg, ix, x);
+    }
+    double pf[3] = {0.0, 0.0, 0.0}, py[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+      double s = 0.0, t = 0.0;
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        s = s + L.fa[r * 3 + k] * x[k];
+        t = t + L.ya[r * 3 + k] * x[k];
+      }
+      pf[r] = s + L.fb[r];
+      py[r] = t + L.yb[r];
+    }
+    double f, m;
+    if (!lin_sample(L.fg, L.fimg, pf, f)) continue;
+    if (!lin_sample(L.mg, L.mimg, py, m)) continue;
+    double dm[P > 0 ? P : 1];
+    if (GRAD || MODE == kLinHistDeriv) {
+      // CentralDifferenceImageFunction at py: neighbours +-spacing[d] along
+      // physical axis d, 0 when either leaves the buffer
+      double gr[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        double lo[3] = {py[0], py[1], py[2]}, hi[3] = {py[0], py[1], py[2]};
+        lo[d] = py[d] - L.msp[d];
+        hi[d] = py[d] + L.msp[d];
+        double vl, vh;
+        if (!lin_sample(L.mg, L.mimg, lo, vl)) continue;
+        if (!lin_sample(L.mg, L.mimg, hi, vh)) continue;
+        const double delta = hi[d] - lo[d];
+        gr[d] = delta > 10.0 * 2.220446049250313e-16 ? (vh - vl) / delta : 0.0;
+      }
+      // u = grad^T G (the moving initial chain's linear part), r = x - c
+      double u[3] = {0.0, 0.0, 0.0}, rr[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        double s = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) s = s + gr[d] * L.G[d * 3 + k];
+        u[k] = s;
+        rr[k] = x[k] - L.c[k];
+      }
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+          double t = J.tau[p][k];
+#pragma unroll
+          for (int j = 0; j < D; ++j) t = t + J.L[p][k * 3 + j] * rr[j];
+          s = s + u[k] * t;
+        }
+        dm[p] = s;
+      }
+    }
+    if (MODE == kLinMS) {
+      const double diff = f - m;
+      acc[0] += 1.0;
+      acc[1] += diff * diff;
+      if (DERIV) {
+#pragma unroll
+        for (int p = 0; p < P; ++p) acc[2 + p] += 2.0 * diff * dm[p];
+      }
+    } else if (MODE == kLinGC) {
+      acc[0] += 1.0;
+      acc[1] += f;
+      acc[2] += m;
+      acc[3] += f * f;
+      acc[4] += m * m;
+      acc[5] += f * m;
+      if (DERIV) {
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+          acc[6 + p] += dm[p];
+          acc[6 + P + p] += f * dm[p];
+          acc[6 + 2 * P + p] += m * dm[p];
+        }
+      }
+    } else {
+      // Mattes: fixed bin (zero-order window), moving cubic window
+      const int B = L.bins;
+      int64_t fi = static_cast<int64_t>(f / L.fbs - L.fnm);
+      fi = fi < 2 ? 2 : (fi > B - 3 ? B - 3 : fi);
+      const double mt = m / L.mbs - L.mnm;
+      int64_t mi = static_cast<int64_t>(mt);
+      mi = mi < 2 ? 2 : (mi > B - 3 ? B - 3 : mi);
+      acc[0] += 1.0;
+      if (MODE == kLinHist) {
+        if (lds_hist) atomicAdd(&csh[fi], 1u);
+        else atomicAdd(&fcnt[fi], 1u);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int64_t kk = mi - 1 + k;
+          const unsigned long long q = __double2ull_rn(bspline3(static_cast<double>(kk) - mt) * kLinHistScale);
+          if (lds_hist) atomicAdd(&hsh[fi * B + kk], q);
+          else atomicAdd(&hist[fi * B + kk], q);
+        }
+      } else {
+        double w = 0.0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int64_t kk = mi - 1 + k;
+          w += L.wtab[fi * B + kk] * (-bspline3_d(static_cast<double>(kk) - mt));
+        }
+#pragma unroll
+        for (int p = 0; p < P; ++p) acc[1 + p] += w * dm[p];
+      }
+    }
+  }
+  // block sums in a fixed order
+  constexpr int NW = kLinThreads / 64;
+  __shared__ double red[Q][NW];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const double w = wave_sum(acc[q]);
+    if (lane == 0) red[q][wid] = w;
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < Q; q += blockDim.x) {
+    double t = 0.0;
+    for (int w = 0; w < NW; ++w) t += red[q][w];
+    partials[static_cast<int64_t>(blockIdx.x) * Q + q] = t;
+  }
+  if (MODE == kLinHist && lds_hist) {
+    for (int k = threadIdx.x; k < L.bins * L.bins; k += blockDim.x)
+      if (hsh[k]) atomicAdd(&hist[k], hsh[k]);
+    for (int k = threadIdx.x; k < L.bins; k += blockDim.x)
+      if (csh[k]) atomicAdd(&fcnt[k], csh[k]);
+  }
+}
+
+// Q sums of nb block partials, each in a fixed order (one block).
+__global__ void __launch_bounds__(256) k_lin_reduce(const double* __restrict__ partials, int nb, int Q,
+                                                     double* __restrict__ out) {
+  __shared__ double sh[256];
+  for (int q = 0; q < Q; ++q) {
+    double t = 0.0;
+    for (int b = threadIdx.x; b < nb; b += 256) t += partials[static_cast<int64_t>(b) * Q + q];
+    sh[threadIdx.x] = t;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[q] = sh[0];
+    __syncthreads();
+  }
+}
+
 inline unsigned int blocks_for(int64_t n, int bs = kBlock) { return static_cast<unsigned int>((n + bs - 1) / bs); }
 inline unsigned int red_blocks(int64_t n) {
   const unsigned int b = blocks_for(n);
@@ -2907,6 +3139,55 @@ void launch_labels_nn(const GridDev& go, const float* u, const GridDev& gl, cons
     hipLaunchKernelGGL(k_labels_nn<2>, grid, dim3(kBlock), 0, s, go, u, gl, labels, out);
   else
     hipLaunchKernelGGL(k_labels_nn<3>, grid, dim3(kBlock), 0, s, go, u, gl, labels, out);
+}
+
+int lin_quantities(int mode, bool deriv, int P) {
+  if (mode == kLinMS) return deriv ? 2 + P : 2;
+  if (mode == kLinGC) return deriv ? 6 + 3 * P : 6;
+  if (mode == kLinHist) return 1;
+  return 1 + P;
+}
+
+int64_t lin_blocks(int64_t npts) {
+  const int64_t b = (npts + kLinThreads - 1) / kLinThreads;
+  return b < 1 ? 1 : (b > kLinMaxBlocks ? kLinMaxBlocks : b);
+}
+
+template <int D, int P>
+static void lin_launch_p(const LinDev& L, const LinJac& J, int mode, bool deriv, int nb, double* partials,
+                         unsigned long long* hist, unsigned int* fcnt, hipStream_t s) {
+  const dim3 grid(nb), block(kLinThreads);
+  if (!deriv) {
+    if (mode == kLinMS) hipLaunchKernelGGL((k_lin_metric<D, kLinMS, false, 0>), grid, block, 0, s, L, J, partials, hist, fcnt);
+    else if (mode == kLinGC) hipLaunchKernelGGL((k_lin_metric<D, kLinGC, false, 0>), grid, block, 0, s, L, J, partials, hist, fcnt);
+    else hipLaunchKernelGGL((k_lin_metric<D, kLinHist, false, 0>), grid, block, 0, s, L, J, partials, hist, fcnt);
+    return;
+  }
+  if (mode == kLinMS) hipLaunchKernelGGL((k_lin_metric<D, kLinMS, true, P>), grid, block, 0, s, L, J, partials, hist, fcnt);
+  else if (mode == kLinGC) hipLaunchKernelGGL((k_lin_metric<D, kLinGC, true, P>), grid, block, 0, s, L, J, partials, hist, fcnt);
+  else hipLaunchKernelGGL((k_lin_metric<D, kLinHistDeriv, true, P>), grid, block, 0, s, L, J, partials, hist, fcnt);
+}
+
+bool launch_lin_metric(const LinDev& L, const LinJac& J, int D, int P, int mode, bool deriv, double* partials,
+                       double* out, unsigned long long* hist, unsigned int* fcnt, hipStream_t s) {
+  if (mode == kLinHist) deriv = false;
+  if (mode == kLinHistDeriv) deriv = true;
+  const int64_t npts = L.pts ? L.npts : static_cast<int64_t>(L.vg.n[0]) * L.vg.n[1] * L.vg.n[2];
+  const int nb = static_cast<int>(lin_blocks(npts));
+  const int Q = lin_quantities(mode, deriv, P);
+  if (D == 2) {
+    if (!deriv || P == 2) lin_launch_p<2, 2>(L, J, mode, deriv, nb, partials, hist, fcnt, s);
+    else if (P == 3) lin_launch_p<2, 3>(L, J, mode, deriv, nb, partials, hist, fcnt, s);
+    else if (P == 6) lin_launch_p<2, 6>(L, J, mode, deriv, nb, partials, hist, fcnt, s);
+    else return false;
+  } else {
+    if (!deriv || P == 3) lin_launch_p<3, 3>(L, J, mode, deriv, nb, partials, hist, fcnt, s);
+    else if (P == 6) lin_launch_p<3, 6>(L, J, mode, deriv, nb, partials, hist, fcnt, s);
+    else if (P == 12) lin_launch_p<3, 12>(L, J, mode, deriv, nb, partials, hist, fcnt, s);
+    else return false;
+  }
+  hipLaunchKernelGGL(k_lin_reduce, dim3(1), dim3(256), 0, s, partials, nb, Q, out);
+  return true;
 }
 
 }  // namespace ants

@@ -283,6 +283,45 @@ struct ApplyDev {
 void launch_apply_transforms(const GridDev& go, const ApplyDev& P, int interp, const float* in, float* out,
                              hipStream_t s);
 
+// ---- linear stages: metric sums over a level's virtual sample points ----
+constexpr int kLinThreads = 256;
+constexpr int kLinMaxBlocks = 512;
+constexpr int kLinMaxParams = 12;
+constexpr int kLinLdsBins = 64;              // joint histograms up to 64 x 64 bins sit in LDS
+constexpr double kLinHistScale = 1099511627776.0;  // 2^40: Parzen weights as fixed-point integers
+enum LinMode { kLinMS = 0, kLinGC = 1, kLinHist = 2, kLinHistDeriv = 3 };
+// Stage Jacobian: dS/dp_p at x = L_p (x - c) + tau_p (row-major 3x3, 3).
+struct LinJac {
+  double L[kLinMaxParams][9];
+  double tau[kLinMaxParams][3];
+};
+struct LinDev {
+  GeomDev vg;               // virtual grid (points of a fully sampled level)
+  const double* pts;        // or explicit physical points, npts x D
+  int64_t npts;
+  GeomDev fg, mg;           // fixed / moving image geometry
+  const float* fimg;
+  const float* mimg;
+  double fa[9], fb[3];      // fixed point = fa x + fb
+  double ya[9], yb[3];      // moving point = ya x + yb (stage, then the moving initial chain)
+  double G[9];              // linear part of the moving initial chain
+  double c[3];              // stage centre
+  double msp[3];            // moving spacing (central differences)
+  int bins;                 // Mattes
+  double fbs, fnm, mbs, mnm;
+  const double* wtab;       // Mattes derivative pass: bins x bins weights
+};
+// Sums per mode (all double): MS {n, sum d^2 [, sum 2 d dm_p]}; GC {n, sum f,
+// sum m, sum f^2, sum m^2, sum f m [, sum dm_p, sum f dm_p, sum m dm_p]};
+// Hist {n} plus the joint histogram (bins^2 fixed-point u64) and fixed
+// counts (u32), both accumulated into zeroed buffers; HistDeriv {n, sum w dm_p}.
+int lin_quantities(int mode, bool deriv, int P);
+int64_t lin_blocks(int64_t npts);
+// partials: lin_blocks(npts) x Q doubles; out: Q doubles.  False for an
+// unsupported (D, P).
+bool launch_lin_metric(const LinDev& L, const LinJac& J, int D, int P, int mode, bool deriv, double* partials,
+                       double* out, unsigned long long* hist, unsigned int* fcnt, hipStream_t s);
+
 void launch_labels_nn(const GridDev& go, const float* u, const GridDev& gl, const int32_t* labels, int32_t* out,
                       hipStream_t s);
 

@@ -509,6 +509,105 @@ double ants_batch_clock(void);
 /* Device bytes one registration of this geometry and schedule holds. */
 int64_t ants_syn_footprint(const ants_syn_params* p, const ants_image_desc* fixed, const ants_image_desc* moving);
 
+/* ---- linear stages (-t Translation[step] / Rigid[step] / Affine[step]) ----
+ * The stages antsRegistrationSyN.sh runs before SyN
+ * (Scripts/antsRegistrationSyN.sh:487-497), as RegistrationHelper builds them
+ * (Examples/itkantsRegistrationHelper.hxx:1290-1372 optimizer, scales and
+ * sampling; 1464-1732 the transforms; itkantsRegistrationHelper.h:905-1097
+ * PrepareRegistrationMethod / AddLinearTransformToCompositeTransform):
+ * ImageRegistrationMethodv4 with a ConjugateGradientLineSearchOptimizerv4
+ * (line search over [0, 2] x the learning rate, epsilon 0.2), the learning
+ * rate re-estimated at each level's first iteration so that the step moves
+ * no corner of the virtual domain by more than `gradient_step`, parameter
+ * scales from physical shifts, and the window convergence monitor.
+ * Transforms: Translation (TranslationTransform), Rigid (Euler2DTransform /
+ * Euler3DTransform, itkantsRegistrationHelper.h:1142-1180), Affine
+ * (AffineTransform); a stage transform maps fixed points before the moving
+ * initial transforms (y = M(S(x))). */
+#define ANTS_LINEAR_TRANSLATION 0
+#define ANTS_LINEAR_RIGID 1
+#define ANTS_LINEAR_AFFINE 2
+/* Metrics (hxx:926-1010): MeanSquares; GC = CorrelationImageToImageMetricv4
+ * (minus the squared Pearson correlation); Mattes = Mattes mutual information
+ * (zero-order fixed / cubic B-spline moving Parzen windows, 2 padding bins). */
+#define ANTS_METRIC_MEAN_SQUARES 0
+#define ANTS_METRIC_GC 1
+#define ANTS_METRIC_MATTES 2
+/* Metric sampling ([...,Regular|Random|None,percentage]): REGULAR takes every
+ * ceil(1/percentage)-th virtual voxel in raster order, RANDOM
+ * floor(N x percentage) voxels drawn with replacement; both move each point by
+ * a normal variate of spacing/3 per axis (MT19937 seeded with `seed` at each
+ * level, Box-Muller normals). */
+#define ANTS_SAMPLING_NONE 0
+#define ANTS_SAMPLING_REGULAR 1
+#define ANTS_SAMPLING_RANDOM 2
+typedef struct ants_linear_params {
+  int dim;
+  int transform;                        /* ANTS_LINEAR_* */
+  double gradient_step;                 /* Rigid[step] */
+  int metric;                           /* ANTS_METRIC_* */
+  int bins;                             /* Mattes[f,m,w,bins,...] */
+  int sampling;                         /* ANTS_SAMPLING_* */
+  double sampling_percentage;           /* (0, 1] */
+  uint32_t seed;                        /* --random-seed */
+  int n_levels;
+  int shrink[ANTS_MAX_LEVELS];          /* -f (isotropic factor, per-axis rule of the helper) */
+  double sigma[ANTS_MAX_LEVELS];        /* -s */
+  int sigma_in_mm;
+  int iterations[ANTS_MAX_LEVELS];      /* -c [..,threshold,window] */
+  double convergence_threshold;
+  int convergence_window;
+  int preprocess;                       /* window both images to [0,1] (PreprocessImage) */
+} ants_linear_params;
+
+/* One optimizer iteration (the 2DIAGNOSTIC line of
+ * Examples/antsRegistrationOptimizerCommandIterationUpdate.h:158-171): the
+ * metric value at the iteration's start, the convergence value, and the
+ * learning rate the line search chose. */
+typedef struct ants_linear_record {
+  int level;
+  int iteration;
+  double value;
+  double convergence;
+  double learning_rate;
+} ants_linear_record;
+
+typedef struct ants_linear_result {
+  int n_params;
+  double params[12];                    /* the stage transform's ITK parameters */
+  ants_transform transform;             /* the same as an AFFINE (matrix, translation, centre) */
+  int iterations[ANTS_MAX_LEVELS];
+  int64_t metric_evaluations;
+} ants_linear_result;
+
+/* Replaces registrationMethod->Update() of a linear stage
+ * (itkantsRegistrationHelper.h:1085) and the transform it adds to the
+ * composite (:1094).  d_fixed / d_moving: full-resolution images in HBM.
+ * moving_initial / fixed_initial: identity or affine transforms in the
+ * order a point passes them (the composite of -r / -q and earlier stages).
+ * center: the stage transform's centre (NULL: the origin, as a new ITK
+ * transform has); initial_params: NULL for identity.  Synchronous. */
+int ants_linear_register(const ants_linear_params* p, const ants_image_desc* fixed, const float* d_fixed,
+                         const ants_image_desc* moving, const float* d_moving, const ants_transform* moving_initial,
+                         int n_moving, const ants_transform* fixed_initial, int n_fixed, const double* center,
+                         const double* initial_params, ants_linear_result* out, ants_linear_record* trace,
+                         int64_t trace_capacity, int64_t* trace_count, void* stream);
+/* One metric evaluation on a level (the ImageToImageMetricv4
+ * GetValueAndDerivative the optimizer calls): value and, with derivative,
+ * minus its derivative by the stage parameters (ITK's v4 convention). */
+int ants_linear_metric(const ants_linear_params* p, int level, const ants_image_desc* fixed, const float* d_fixed,
+                       const ants_image_desc* moving, const float* d_moving, const ants_transform* moving_initial,
+                       int n_moving, const ants_transform* fixed_initial, int n_fixed, const double* center,
+                       const double* params, int derivative, double* value, double* der, int64_t* n_valid,
+                       void* stream);
+/* Host only: a level's sample points (physical, x fastest), and the
+ * parameter scales of RegistrationParameterScalesFromPhysicalShift. */
+int ants_linear_sample_points(const ants_linear_params* p, int level, const ants_image_desc* fixed, double* out,
+                              int64_t capacity, int64_t* count);
+int ants_linear_estimate_scales(const ants_linear_params* p, int level, const ants_image_desc* fixed,
+                                const ants_transform* moving_initial, int n_moving, const double* center,
+                                const double* params, double* scales);
+
 #ifdef __cplusplus
 }
 #endif

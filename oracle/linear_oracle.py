@@ -1,0 +1,562 @@
+"""CPU restatement of antsRegistration's linear stages (test infrastructure
+only: tests/ use it as the checker of the GPU path; nothing in ants_amd
+imports it).  Everything is float64 numpy.
+
+What the reference instantiates (ITK is not in the reference tree, so each
+piece below is restated from the published method it follows and the ITK
+class it names):
+
+* Stage set-up, Examples/itkantsRegistrationHelper.hxx:1290-1372: metric
+  sampling NONE / REGULAR / RANDOM with a percentage; the
+  ConjugateGradientLineSearchOptimizerv4 with line-search limits [0, 2],
+  epsilon 0.2, learning rate and maximum physical step both the -t
+  Rigid[step] value, scales from RegistrationParameterScalesFromPhysicalShift,
+  the window convergence monitor of -c [iters,threshold,window].
+  Transforms: hxx:1464-1732 (Affine = itk::AffineTransform, Rigid =
+  Euler2DTransform / Euler3DTransform per itkantsRegistrationHelper.h:1142-1180,
+  Translation = TranslationTransform), each run by ImageRegistrationMethodv4
+  (itkantsRegistrationHelper.h:905-1097) with the moving initial composite.
+* Metrics, hxx:926-1010: MeanSquares, GC (CorrelationImageToImageMetricv4:
+  the negative squared Pearson correlation), Mattes (Mattes et al., IEEE TMI
+  22(1) 2003, with Thevenaz & Unser, IEEE TIP 9(12) 2000: zero-order Parzen
+  window on the fixed intensity, cubic B-spline window on the moving one,
+  two padding bins each side).
+* Sampling points: ImageRegistrationMethodv4's regular grid (every
+  ceil(1/p)-th virtual voxel in raster order) or random voxels, each moved by
+  a normal variate of spacing/3 per axis from a Mersenne twister (MT19937,
+  Matsumoto & Nishimura 1998; Box-Muller normals).
+* Optimizer: Polak-Ribiere conjugate gradient with the restart rule
+  gamma in [0, 5], the recursive golden-section line search (Kiefer 1953 in
+  its recursive published form) over [0, lr, 2 lr], the learning rate
+  re-estimated at each level's first iteration as max_step / (largest
+  physical shift of the virtual domain's corners under the scaled step).
+
+Conventions shared with the GPU engine: a stage transform S maps virtual
+(fixed) points into the frame of the moving initial transforms, which are
+applied after it (y = M(S(x))); the metric derivative returned is minus the
+derivative of the value (ITK's v4 convention: the optimizer adds it).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+import oracle as O
+
+TRANSLATION, RIGID, AFFINE = 0, 1, 2
+MEAN_SQUARES, GC, MATTES = 0, 1, 2
+NONE, REGULAR, RANDOM = 0, 1, 2
+DBL_EPS = np.finfo(np.float64).eps
+
+
+# ---------------------------------------------------------------------------
+# Random numbers: MT19937 (numpy's legacy RandomState seeds it with the
+# reference init_genrand), ITK-style variates.
+class Twister:
+    def __init__(self, seed: int):
+        self._bg = np.random.RandomState(int(seed) & 0xFFFFFFFF)._bit_generator
+
+    def ints(self, n: int) -> np.ndarray:
+        return np.asarray(self._bg.random_raw(n), dtype=np.uint64)
+
+    def int_below_or_equal(self, n: int) -> int:
+        """uniform integer in [0, n]: mask to the bits n uses, reject above n"""
+        used = n
+        for s in (1, 2, 4, 8, 16):
+            used |= used >> s
+        while True:
+            i = int(self.ints(1)[0]) & used
+            if i <= n:
+                return i
+
+    def normals(self, n: int) -> np.ndarray:
+        """n N(0, 1) variates, two draws each: sqrt(-2 ln(1 - u1)) cos(2 pi u2)
+        with u1 on [0, 1] (x / (2^32 - 1)) and u2 on [0, 1) (x / 2^32)"""
+        r = self.ints(2 * n).reshape(n, 2).astype(np.float64)
+        u1 = r[:, 0] * (1.0 / 4294967295.0)
+        u2 = r[:, 1] * (1.0 / 4294967296.0)
+        with np.errstate(divide="ignore"):
+            rad = np.sqrt(-2.0 * np.log(1.0 - u1) * 1.0)
+        return 0.0 + rad * np.cos(2.0 * math.pi * u2)
+
+
+# ---------------------------------------------------------------------------
+def grid_points(g: O.Grid, lin: Optional[np.ndarray] = None) -> np.ndarray:
+    """physical points of voxels (all, or the x-fastest linear indices lin)"""
+    D = g.dim
+    n = np.asarray(g.size, dtype=np.int64)
+    if lin is None:
+        lin = np.arange(int(np.prod(n)), dtype=np.int64)
+    idx = np.empty((lin.size, D), dtype=np.float64)
+    rem = lin.copy()
+    for d in range(D):
+        idx[:, d] = rem % n[d]
+        rem //= n[d]
+    m = np.asarray(g.direction, dtype=np.float64).reshape(D, D) * np.asarray(g.spacing, dtype=np.float64)[None, :]
+    return np.asarray(g.origin, dtype=np.float64)[None, :] + idx @ m.T
+
+
+def sample_points(g: O.Grid, strategy: int, percentage: float, seed: int) -> np.ndarray:
+    """virtual-domain sample points of one level"""
+    if strategy == NONE:
+        return grid_points(g)
+    D = g.dim
+    N = g.count
+    tw = Twister(seed)
+    third = np.asarray(g.spacing, dtype=np.float64) / 3.0
+    if strategy == REGULAR:
+        step = int(math.ceil(1.0 / percentage))
+        lin = np.arange(0, N, step, dtype=np.int64)
+        pts = grid_points(g, lin)
+        jit = tw.normals(lin.size * D).reshape(lin.size, D)
+        return pts + jit * third[None, :]
+    n = int(math.floor(N * percentage))
+    pts = np.empty((n, D))
+    for k in range(n):
+        i = tw.int_below_or_equal(N - 1)
+        p = grid_points(g, np.array([i], dtype=np.int64))[0]
+        pts[k] = p + tw.normals(D) * third
+    return pts
+
+
+# ---------------------------------------------------------------------------
+def n_params(kind: int, D: int) -> int:
+    return {TRANSLATION: D, RIGID: 3 if D == 2 else 6, AFFINE: D * D + D}[kind]
+
+
+def _rot3(ax, ay, az):
+    cx, sx, cy, sy, cz, sz = math.cos(ax), math.sin(ax), math.cos(ay), math.sin(ay), math.cos(az), math.sin(az)
+    rx = np.array([[1, 0, 0], [0, cx, -sx], [0, sx, cx]])
+    ry = np.array([[cy, 0, sy], [0, 1, 0], [-sy, 0, cy]])
+    rz = np.array([[cz, -sz, 0], [sz, cz, 0], [0, 0, 1]])
+    return rz, rx, ry
+
+
+def stage_matrix(kind: int, params, D: int) -> Tuple[np.ndarray, np.ndarray]:
+    """(A, t) of y = A (x - c) + c + t"""
+    p = np.asarray(params, dtype=np.float64)
+    if kind == TRANSLATION:
+        return np.eye(D), p[:D].copy()
+    if kind == AFFINE:
+        return p[: D * D].reshape(D, D).copy(), p[D * D:].copy()
+    if D == 2:
+        c, s = math.cos(p[0]), math.sin(p[0])
+        return np.array([[c, -s], [s, c]]), p[1:3].copy()
+    rz, rx, ry = _rot3(p[0], p[1], p[2])
+    return rz @ rx @ ry, p[3:6].copy()
+
+
+def jacobian_forms(kind: int, params, D: int):
+    """per parameter (L_p, tau_p): dS/dp_p at x = L_p (x - c) + tau_p"""
+    P = n_params(kind, D)
+    L = np.zeros((P, D, D))
+    tau = np.zeros((P, D))
+    p = np.asarray(params, dtype=np.float64)
+    if kind == TRANSLATION:
+        for i in range(D):
+            tau[i, i] = 1.0
+    elif kind == AFFINE:
+        for i in range(D):
+            for j in range(D):
+                L[i * D + j, i, j] = 1.0
+            tau[D * D + i, i] = 1.0
+    elif D == 2:
+        c, s = math.cos(p[0]), math.sin(p[0])
+        L[0] = np.array([[-s, -c], [c, -s]])
+        tau[1, 0] = tau[2, 1] = 1.0
+    else:
+        ax, ay, az = p[0], p[1], p[2]
+        rz, rx, ry = _rot3(ax, ay, az)
+        cx, sx, cy, sy, cz, sz = math.cos(ax), math.sin(ax), math.cos(ay), math.sin(ay), math.cos(az), math.sin(az)
+        drx = np.array([[0, 0, 0], [0, -sx, -cx], [0, cx, -sx]])
+        dry = np.array([[-sy, 0, cy], [0, 0, 0], [-cy, 0, -sy]])
+        drz = np.array([[-sz, -cz, 0], [cz, -sz, 0], [0, 0, 0]])
+        L[0] = rz @ drx @ ry
+        L[1] = rz @ rx @ dry
+        L[2] = drz @ rx @ ry
+        for i in range(3):
+            tau[3 + i, i] = 1.0
+    return L, tau
+
+
+def affine_chain(transforms, D: int) -> Tuple[np.ndarray, np.ndarray]:
+    """(A, b) of the affines applied in list order: x -> A x + b; each item
+    (matrix, translation, center) as y = M (x - c) + c + t"""
+    A = np.eye(D)
+    b = np.zeros(D)
+    for (M, t, c) in transforms:
+        M = np.asarray(M, dtype=np.float64).reshape(D, D)
+        off = np.asarray(t, dtype=np.float64) + np.asarray(c, dtype=np.float64) - M @ np.asarray(c, dtype=np.float64)
+        A, b = M @ A, M @ b + off
+    return A, b
+
+
+# ---------------------------------------------------------------------------
+def _geom(g: O.Grid):
+    D = g.dim
+    dirm = np.asarray(g.direction, dtype=np.float64).reshape(D, D)
+    sp = np.asarray(g.spacing, dtype=np.float64)
+    minv = (dirm.T) / sp[:, None]
+    return np.asarray(g.origin, dtype=np.float64), minv, np.asarray(g.size, dtype=np.int64)
+
+
+def interpolate(g: O.Grid, img: np.ndarray, pts: np.ndarray) -> Tuple[np.ndarray, np.ndarray]:
+    """linear interpolation (ITK LinearInterpolateImageFunction) at physical
+    points; inside = continuous index in [-0.5, n - 0.5) on every axis"""
+    D = g.dim
+    org, minv, n = _geom(g)
+    c = (pts - org[None, :]) @ minv.T
+    inside = np.all((c >= -0.5) & (c < n[None, :] - 0.5), axis=1)
+    cc = np.where(inside[:, None], c, 0.0)
+    f = np.floor(cc)
+    b = f.astype(np.int64)
+    t = cc - f
+    neg = b < 0
+    b = np.where(neg, 0, b)
+    t = np.where(neg, 0.0, t)
+    hi = np.where(b + 1 < n[None, :], b + 1, b)
+    flat = np.asarray(img, dtype=np.float64).reshape(-1)
+
+    def at(ix):
+        lin = ix[:, 0].copy()
+        mul = 1
+        for d in range(1, D):
+            mul *= int(n[d - 1])
+            lin = lin + ix[:, d] * mul
+        return flat[lin]
+
+    def corner(k):
+        return np.stack([hi[:, d] if (k >> d) & 1 else b[:, d] for d in range(D)], axis=1)
+
+    v = [at(corner(k)) for k in range(1 << D)]
+    a00 = v[0] + (v[1] - v[0]) * t[:, 0]
+    a10 = v[2] + (v[3] - v[2]) * t[:, 0]
+    r = a00 + (a10 - a00) * t[:, 1]
+    if D == 3:
+        a01 = v[4] + (v[5] - v[4]) * t[:, 0]
+        a11 = v[6] + (v[7] - v[6]) * t[:, 0]
+        b1 = a01 + (a11 - a01) * t[:, 1]
+        r = r + (b1 - r) * t[:, 2]
+    return np.where(inside, r, 0.0), inside
+
+
+def gradient(g: O.Grid, img: np.ndarray, pts: np.ndarray) -> np.ndarray:
+    """CentralDifferenceImageFunction at points: neighbours +-spacing[d]
+    along physical axis d, 0 where either is outside the buffer"""
+    D = g.dim
+    sp = np.asarray(g.spacing, dtype=np.float64)
+    out = np.zeros_like(pts)
+    for d in range(D):
+        lo = pts.copy()
+        hi = pts.copy()
+        lo[:, d] = pts[:, d] - sp[d]
+        hi[:, d] = pts[:, d] + sp[d]
+        vl, il = interpolate(g, img, lo)
+        vh, ih = interpolate(g, img, hi)
+        delta = hi[:, d] - lo[:, d]
+        ok = il & ih & (delta > 10.0 * DBL_EPS)
+        out[:, d] = np.where(ok, (vh - vl) / np.where(ok, delta, 1.0), 0.0)
+    return out
+
+
+def bspline3(u):
+    a = np.abs(u)
+    return np.where(a < 1.0, (4.0 - 6.0 * a * a + 3.0 * a * a * a) / 6.0,
+                    np.where(a < 2.0, (2.0 - a) ** 3 / 6.0, 0.0))
+
+
+def bspline3_d(u):
+    a = np.abs(u)
+    s = np.sign(u)
+    return np.where(a < 1.0, s * (-2.0 * a + 1.5 * a * a), np.where(a < 2.0, -s * 0.5 * (2.0 - a) ** 2, 0.0))
+
+
+@dataclass
+class MetricSetup:
+    """One level's metric problem."""
+    kind: int
+    metric: int
+    gF: O.Grid
+    fixed: np.ndarray
+    gM: O.Grid
+    moving: np.ndarray
+    points: np.ndarray               # virtual points (n, D)
+    center: np.ndarray               # stage transform centre c
+    moving_init: Tuple[np.ndarray, np.ndarray]   # (A_M, b_M) applied after the stage
+    fixed_init: Tuple[np.ndarray, np.ndarray]    # (A_F, b_F)
+    bins: int = 32
+    ranges: Optional[Tuple[float, float, float, float]] = None  # Mattes: fixed min/max, moving min/max
+
+
+def mattes_ranges(fixed: np.ndarray, moving: np.ndarray):
+    return float(np.min(fixed)), float(np.max(fixed)), float(np.min(moving)), float(np.max(moving))
+
+
+def moving_map(S: MetricSetup, params):
+    D = S.gF.dim
+    A, t = stage_matrix(S.kind, params, D)
+    c = S.center
+    AM, bM = S.moving_init
+    y = (S.points - c[None, :]) @ A.T + c[None, :] + t[None, :]
+    return y @ AM.T + bM[None, :]
+
+
+def evaluate(S: MetricSetup, params, derivative: bool = True):
+    """(value, derivative (-dvalue/dp), number of valid points)"""
+    D = S.gF.dim
+    x = S.points
+    AF, bF = S.fixed_init
+    fv, fin = interpolate(S.gF, S.fixed, x @ AF.T + bF[None, :])
+    y = moving_map(S, params)
+    mv, minside = interpolate(S.gM, S.moving, y)
+    ok = fin & minside
+    N = int(ok.sum())
+    if N == 0:
+        raise ValueError("no valid points")
+    f = fv[ok]
+    m = mv[ok]
+    P = n_params(S.kind, D)
+    dm = None
+    if derivative or S.metric == MATTES:
+        gr = gradient(S.gM, S.moving, y[ok])                    # (N, D)
+        AM, _ = S.moving_init
+        u = gr @ AM                                             # grad^T A_M
+        L, tau = jacobian_forms(S.kind, params, D)
+        r = x[ok] - S.center[None, :]
+        dm = np.einsum("nd,pde,ne->np", u, L, r) + u @ tau.T    # (N, P) d m / d p
+    if S.metric == MEAN_SQUARES:
+        diff = f - m
+        val = float(np.sum(diff * diff)) / N
+        der = (2.0 * diff[:, None] * dm).sum(0) / N if derivative else None
+        return val, der, N
+    if S.metric == GC:
+        fb = f.sum() / N
+        mb = m.sum() / N
+        fc = f - fb
+        mc = m - mb
+        fm, ff, mm = float((fc * mc).sum()), float((fc * fc).sum()), float((mc * mc).sum())
+        if not (ff > 0.0 and mm > 0.0):
+            raise ValueError("constant image over the valid points")
+        val = -fm * fm / (ff * mm)
+        der = None
+        if derivative:
+            fdm = (fc[:, None] * dm).sum(0)
+            mdm = (mc[:, None] * dm).sum(0)
+            der = 2.0 * fm / (ff * mm) * (fdm - fm / mm * mdm)
+        return val, der, N
+    # Mattes
+    B = S.bins
+    fmin, fmax, mmin, mmax = S.ranges
+    pad = 2
+    fbs = (fmax - fmin) / (B - 2 * pad)
+    mbs = (mmax - mmin) / (B - 2 * pad)
+    fnm = fmin / fbs - pad
+    mnm = mmin / mbs - pad
+    fi = np.clip((f / fbs - fnm).astype(np.int64), 2, B - 3)
+    mt = m / mbs - mnm
+    mi = np.clip(mt.astype(np.int64), 2, B - 3)
+    H = np.zeros((B, B))
+    cnt = np.bincount(fi, minlength=B).astype(np.float64)
+    for k in range(4):
+        kk = mi - 1 + k
+        np.add.at(H, (fi, kk), bspline3(kk - mt))
+    p = H / N
+    pf = cnt / N
+    pm = p.sum(0)
+    val = 0.0
+    W = np.zeros((B, B))
+    for i in range(B):
+        for j in range(B):
+            if p[i, j] > DBL_EPS and pm[j] > DBL_EPS:
+                ratio = math.log(p[i, j] / pm[j])
+                W[i, j] = ratio
+                if pf[i] > DBL_EPS:
+                    val += p[i, j] * (ratio - math.log(pf[i]))
+    val = -val
+    der = None
+    if derivative:
+        w = np.zeros(N)
+        for k in range(4):
+            kk = mi - 1 + k
+            w += W[fi, kk] * (-bspline3_d(kk - mt))
+        der = (w[:, None] * dm).sum(0) / (mbs * N)
+    return val, der, N
+
+
+# ---------------------------------------------------------------------------
+def corners(g: O.Grid) -> np.ndarray:
+    D = g.dim
+    n = np.asarray(g.size, dtype=np.int64)
+    lin = []
+    for i in range(1 << D):
+        idx = [(n[d] - 1) if (i >> d) & 1 else 0 for d in range(D)]
+        l = idx[0]
+        mul = 1
+        for d in range(1, D):
+            mul *= int(n[d - 1])
+            l += idx[d] * mul
+        lin.append(l)
+    return grid_points(g, np.asarray(lin, dtype=np.int64))
+
+
+def _max_shift(kind, params, step, center, chain, pts, D):
+    def mapped(pp):
+        A, t = stage_matrix(kind, pp, D)
+        y = (pts - center[None, :]) @ A.T + center[None, :] + t[None, :]
+        AM, bM = chain
+        return y @ AM.T + bM[None, :]
+    a = mapped(params)
+    b = mapped(np.asarray(params) + np.asarray(step))
+    return float(np.max(np.sqrt(((b - a) ** 2).sum(1))))
+
+
+def estimate_scales(kind, params, center, chain, pts, D, small=0.01):
+    """RegistrationParameterScalesFromPhysicalShift: (max corner shift of a
+    0.01 change of parameter i / 0.01)^2; zero shifts get the smallest
+    non-zero one"""
+    P = n_params(kind, D)
+    sh = np.zeros(P)
+    for i in range(P):
+        d = np.zeros(P)
+        d[i] = small
+        sh[i] = _max_shift(kind, params, d, center, chain, pts, D)
+    nz = [s for s in sh if s > DBL_EPS]
+    if not nz:
+        return np.ones(P)
+    mn = min(nz)
+    out = np.where(sh <= DBL_EPS, mn * mn, sh * sh)
+    return out / (small * small)
+
+
+@dataclass
+class LinearParams:
+    kind: int = RIGID
+    gradient_step: float = 0.1
+    metric: int = MEAN_SQUARES
+    bins: int = 32
+    sampling: int = NONE
+    percentage: float = 1.0
+    seed: int = 0
+    iterations: Sequence[int] = (20,)
+    shrink: Sequence[int] = (1,)
+    sigma: Sequence[float] = (0.0,)
+    sigma_in_mm: bool = False
+    conv_threshold: float = 1e-6
+    conv_window: int = 10
+    preprocess: bool = False
+
+
+@dataclass
+class LinearResult:
+    params: np.ndarray
+    center: np.ndarray
+    records: List[tuple] = field(default_factory=list)   # (level, iteration, value, convergence, lr)
+    iterations: List[int] = field(default_factory=list)
+    evaluations: int = 0
+
+
+class _LineSearch:
+    RESPHI = 2.0 - (1.0 + math.sqrt(5.0)) / 2.0
+
+    def __init__(self, f, epsilon=0.2, max_iterations=20):
+        self.f = f
+        self.eps = epsilon
+        self.max_it = max_iterations
+        self.it = 0
+
+    def search(self, a, b, c, fb=None):
+        if self.it > self.max_it:
+            return (c + a) / 2.0
+        self.it += 1
+        if c - b > b - a:
+            x = b + self.RESPHI * (c - b)
+        else:
+            x = b - self.RESPHI * (b - a)
+        if abs(c - a) < self.eps * (abs(b) + abs(x)):
+            return (c + a) / 2.0
+        fx = self.f(x)
+        if fb is None:
+            fb = self.f(b)
+        if fx < fb:
+            return self.search(b, x, c, fx) if c - b > b - a else self.search(a, x, b, fx)
+        return self.search(a, b, x, fb) if c - b > b - a else self.search(x, b, c, fb)
+
+
+def linear_register(P: LinearParams, gF: O.Grid, fixed, gM: O.Grid, moving, moving_init=(), fixed_init=(),
+                    center=None, params0=None) -> LinearResult:
+    """One linear stage, level by level.  moving_init / fixed_init: lists of
+    (matrix, translation, center) affines in the order a point passes them."""
+    D = gF.dim
+    F = np.asarray(fixed, dtype=np.float64)
+    M = np.asarray(moving, dtype=np.float64)
+    if P.preprocess:
+        F = O.preprocess(F)
+        M = O.preprocess(M)
+    chainM = affine_chain(moving_init, D)
+    chainF = affine_chain(fixed_init, D)
+    c = np.zeros(D) if center is None else np.asarray(center, dtype=np.float64)
+    npar = n_params(P.kind, D)
+    if params0 is None:
+        params = np.zeros(npar)
+        if P.kind in (AFFINE,):
+            params[: D * D] = np.eye(D).ravel()
+    else:
+        params = np.asarray(params0, dtype=np.float64).copy()
+    res = LinearResult(params, c)
+    for lev in range(len(P.iterations)):
+        gv = O.shrink_grid(gF, int(P.shrink[lev]))
+        Fs = O.smooth_image(gF, F, float(P.sigma[lev]), P.sigma_in_mm) if P.sigma[lev] > 0 else F
+        Ms = O.smooth_image(gM, M, float(P.sigma[lev]), P.sigma_in_mm) if P.sigma[lev] > 0 else M
+        pts = sample_points(gv, P.sampling, P.percentage, P.seed)
+        S = MetricSetup(P.kind, P.metric, gF, Fs, gM, Ms, pts, c, chainM, chainF, P.bins,
+                        mattes_ranges(Fs, Ms) if P.metric == MATTES else None)
+        cpts = corners(gv)
+        scales = estimate_scales(P.kind, params, c, chainM, cpts, D)
+        energies: List[float] = []
+        conj = np.zeros(npar)
+        last = np.zeros(npar)
+        lr = P.gradient_step
+        it = 0
+        while it < P.iterations[lev]:
+            val, der, _ = evaluate(S, params, True)
+            res.evaluations += 1
+            energies.append(val)
+            conv = O.conv_value(energies, P.conv_window)
+            if conv <= P.conv_threshold:
+                break
+            g = der / scales
+            if it == 0:
+                conj[:] = 0.0
+                last[:] = 0.0
+                ss = _max_shift(P.kind, params, g, c, chainM, cpts, D)
+                lr = 1.0 if ss <= DBL_EPS else P.gradient_step / ss
+            gamma = 0.0
+            den = float(last @ last)
+            if den > DBL_EPS:
+                gamma = float((g - last) @ g) / den
+            if gamma < 0.0 or gamma > 5.0:
+                gamma = 0.0
+            last = g.copy()
+            conj = g + gamma * conj
+            direction = conj.copy()
+
+            def f(step, direction=direction, params=params):
+                res.evaluations += 1
+                return evaluate(S, params + step * direction, False)[0]
+
+            lr = _LineSearch(f).search(lr * 0.0, lr, lr * 2.0)
+            params = params + lr * direction
+            res.records.append((lev, it + 1, val, conv, lr))
+            it += 1
+        res.iterations.append(it)
+    res.params = params
+    return res
+
+
+def stage_affine(kind: int, params, center, D: int):
+    """(matrix, translation, center) of the stage transform"""
+    A, t = stage_matrix(kind, params, D)
+    return A, t, np.asarray(center, dtype=np.float64)
