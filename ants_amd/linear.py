@@ -23,6 +23,11 @@ from .registration import Convergence, parse_option_function, parse_sigmas, pars
 
 TRANSFORMS = {"translation": 0, "rigid": 1, "affine": 2}
 METRICS = {"meansquares": 0, "gc": 1, "mattes": 2}
+# metric names antsRegistration accepts for these metrics
+# (StringToMetricType, Examples/itkantsRegistrationHelper.hxx:139-190): "mi"
+# is Mattes there; "mi2" is the joint-histogram metric, not provided here
+METRIC_ALIASES = {"meansquares": "MeanSquares", "msq": "MeanSquares", "ssd": "MeanSquares", "gc": "GC",
+                  "mattes": "Mattes", "mi": "Mattes"}
 SAMPLING = {"none": 0, "": 0, "regular": 1, "random": 2}
 
 
@@ -111,11 +116,13 @@ class LinearStage:
             raise ValueError(f"{tname} needs a gradient step: {tname}[step]")
         mname, mp = parse_option_function(metric)
         key = mname.strip().lower()
-        if key == "mi":
-            raise ValueError("the joint-histogram MI metric is not provided; Mattes[...] is the mutual-information "
-                             "metric of the linear stages here")
-        if key not in METRICS:
-            raise ValueError(f"linear-stage metrics are MeanSquares, GC and Mattes, got {mname!r}")
+        if key == "mi2":
+            raise ValueError("the joint-histogram MI metric (MI2) is not provided; MI / Mattes is the "
+                             "mutual-information metric of the linear stages here")
+        if key not in METRIC_ALIASES:
+            raise ValueError(f"linear-stage metrics are MeanSquares (MSQ, SSD), GC and Mattes (MI), got {mname!r}")
+        canon = METRIC_ALIASES[key]
+        key = canon.lower()
         if len(mp) < 2:
             raise ValueError(f"{mname} needs fixed and moving images")
         bins = int(mp[3]) if key == "mattes" and len(mp) > 3 else 32
@@ -128,7 +135,7 @@ class LinearStage:
         s, in_mm = parse_sigmas(sigmas, len(conv.iterations))
         if not (len(conv.iterations) == len(f) == len(s)):
             raise ValueError("-c, -f and -s must give the same number of levels")
-        return cls(tname.strip(), float(tp[0]), mname.strip(), bins, samp, pct, conv.iterations, conv.threshold,
+        return cls(tname.strip(), float(tp[0]), canon, bins, samp, pct, conv.iterations, conv.threshold,
                    conv.window, f, s, in_mm, int(seed), True, mp[0], mp[1])
 
     def params(self, dim: int) -> LinearParamsC:

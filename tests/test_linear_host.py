@@ -218,8 +218,16 @@ def test_parse_linear_stage():
     assert (st.metric, st.sampling, st.window, st.threshold, st.sigma_in_mm) == ("GC", "random", 5, 1e-7, True)
     with pytest.raises(ValueError):
         LN.LinearStage.from_args("Similarity[0.1]", "GC[f,m]", "[10]", "1", "0")
+    # antsRegistration's names: MI is Mattes, MSQ / SSD are MeanSquares
+    # (StringToMetricType); MI2, the joint-histogram metric, is not provided
+    st = LN.LinearStage.from_args("Rigid[0.1]", "MI[f,m,1,32,Regular,0.25]", "[10]", "1", "0")
+    assert (st.metric, st.bins, st.sampling, st.percentage) == ("Mattes", 32, "regular", 0.25)
+    assert LN.LinearStage.from_args("Rigid[0.1]", "SSD[f,m]", "[10]", "1", "0").metric == "MeanSquares"
+    assert LN.LinearStage.from_args("Rigid[0.1]", "msq[f,m]", "[10]", "1", "0").metric == "MeanSquares"
     with pytest.raises(ValueError):
-        LN.LinearStage.from_args("Rigid[0.1]", "MI[f,m,1,32]", "[10]", "1", "0")
+        LN.LinearStage.from_args("Rigid[0.1]", "MI2[f,m,1,32]", "[10]", "1", "0")
+    with pytest.raises(ValueError):
+        LN.LinearStage.from_args("Rigid[0.1]", "CC[f,m,1,4]", "[10]", "1", "0")
     with pytest.raises(ValueError):
         LN.LinearStage.from_args("Rigid[0.1]", "GC[f,m,1,1,Sparse,0.1]", "[10]", "1", "0")
     assert LN.LinearStage.is_linear("Translation[1]") and not LN.LinearStage.is_linear("SyN[0.1,3,0]")
