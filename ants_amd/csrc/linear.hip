@@ -15,11 +15,17 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/ants_syn.h"
@@ -40,6 +46,16 @@ constexpr double kDblEps = std::numeric_limits<double>::epsilon();
 constexpr double kPi = 3.14159265358979323846;
 
 int err(int code, const std::string& msg) { return host_error(code, msg.c_str()); }
+
+// Host threads for per-point set-up work: OMP_NUM_THREADS when set (the
+// share of the machine the process was given), else the hardware count, at
+// most 64.
+int64_t host_threads() {
+  int64_t t = 0;
+  if (const char* e = std::getenv("OMP_NUM_THREADS")) t = std::atoll(e);
+  if (t <= 0) t = static_cast<int64_t>(std::thread::hardware_concurrency());
+  return std::max<int64_t>(1, std::min<int64_t>(t, 64));
+}
 
 #define LIN_TRY(expr)               \
   do {                              \
@@ -84,25 +100,32 @@ class Twister {
     while (i > n);
     return i;
   }
-  // Box-Muller: u1 on [0, 1], u2 on [0, 1)
-  double normal() {
-    const double u1 = static_cast<double>(next()) * (1.0 / 4294967295.0);
-    const double u2 = static_cast<double>(next()) * (1.0 / 4294967296.0);
-    const double r = std::sqrt(-2.0 * std::log(1.0 - u1) * 1.0);
-    return 0.0 + r * std::cos(2.0 * kPi * u2);
-  }
 
  private:
+  static uint32_t mix(uint32_t a, uint32_t b, uint32_t c) {
+    const uint32_t y = (a & 0x80000000u) | (b & 0x7fffffffu);
+    return c ^ (y >> 1) ^ ((0u - (y & 1u)) & 0x9908b0dfu);
+  }
+  // the recurrence in three runs so that no index wraps: words 0..226 read
+  // the old word 397 ahead, 227..622 the new word 227 behind, 623 word 0
   void twist() {
-    for (int i = 0; i < 624; ++i) {
-      const uint32_t y = (mt_[i] & 0x80000000u) | (mt_[(i + 1) % 624] & 0x7fffffffu);
-      mt_[i] = mt_[(i + 397) % 624] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
-    }
+    int i = 0;
+    for (; i < 227; ++i) mt_[i] = mix(mt_[i], mt_[i + 1], mt_[i + 397]);
+    for (; i < 623; ++i) mt_[i] = mix(mt_[i], mt_[i + 1], mt_[i - 227]);
+    mt_[623] = mix(mt_[623], mt_[0], mt_[396]);
     idx_ = 0;
   }
   uint32_t mt_[624];
   int idx_;
 };
+
+// Box-Muller normal from two consecutive draws: u1 on [0, 1], u2 on [0, 1)
+inline double box_muller(uint32_t a, uint32_t b) {
+  const double u1 = static_cast<double>(a) * (1.0 / 4294967295.0);
+  const double u2 = static_cast<double>(b) * (1.0 / 4294967296.0);
+  const double r = std::sqrt(-2.0 * std::log(1.0 - u1) * 1.0);
+  return 0.0 + r * std::cos(2.0 * kPi * u2);
+}
 
 // ---------------------------------------------------------------------------
 // 3x3 row-major helpers (D x D used, identity-padded).
@@ -256,42 +279,87 @@ int validate(const ants_linear_params* p) {
   return ANTS_OK;
 }
 
-int level_points(const ants_linear_params* p, const ants_image_desc& lv, std::vector<double>& out) {
-  out.clear();
+// Sample points of a level: the draws come off the one MT19937 stream in
+// order (per point: for random sampling the voxel draw(s), then two draws per
+// axis for the jitter normals), which is sequential; turning them into
+// points (Box-Muller, index to physical point) is independent per point and
+// runs on host threads, so the result is the same bits on any thread count.
+// The stream is drawn in chunks into two reused buffers: while the threads
+// turn chunk c into points, this thread draws chunk c + 1.
+int64_t level_point_count(const ants_linear_params* p, const ants_image_desc& lv) {
+  const int64_t N = lv.size[0] * lv.size[1] * (lv.dim == 3 ? lv.size[2] : 1);
+  if (p->sampling == ANTS_SAMPLING_NONE) return 0;
+  if (p->sampling == ANTS_SAMPLING_REGULAR) {
+    const int64_t step = static_cast<int64_t>(std::ceil(1.0 / p->sampling_percentage));
+    return (N + step - 1) / step;
+  }
+  return static_cast<int64_t>(std::floor(static_cast<double>(N) * p->sampling_percentage));
+}
+
+// writes level_point_count() points, D doubles each, to out
+int level_points(const ants_linear_params* p, const ants_image_desc& lv, double* out) {
   const int D = lv.dim;
   const int64_t nx = lv.size[0], ny = lv.size[1], nz = D == 3 ? lv.size[2] : 1;
   const int64_t N = nx * ny * nz;
   if (p->sampling == ANTS_SAMPLING_NONE) return ANTS_OK;
   GeomDev g;
   LIN_TRY(host_geom(&lv, &g));
-  auto point = [&](int64_t lin, double x[3]) {
-    const double ix[3] = {static_cast<double>(lin % nx), static_cast<double>((lin / nx) % ny),
-                          static_cast<double>(lin / (nx * ny))};
-    for (int r = 0; r < D; ++r) {
-      double s = g.org[r];
-      for (int c = 0; c < D; ++c) s = s + g.m[r * 3 + c] * ix[c];
-      x[r] = s;
+  const int64_t n = level_point_count(p, lv);
+  const int64_t step =
+      p->sampling == ANTS_SAMPLING_REGULAR ? static_cast<int64_t>(std::ceil(1.0 / p->sampling_percentage)) : 0;
+  if (step == 0 && N - 1 > static_cast<int64_t>(0xffffffffu))
+    return err(ANTS_ERR_UNSUPPORTED, "grid too large for random sampling");
+  double third[3] = {0.0, 0.0, 0.0};
+  for (int d = 0; d < D; ++d) third[d] = lv.spacing[d] / 3.0;
+  const int per = 2 * D;
+  const int64_t kChunk = 262144;  // points per chunk
+  struct Buf {
+    std::vector<uint32_t> draws, vox;
+  } buf[2];
+  for (auto& b : buf) {
+    b.draws.resize(static_cast<size_t>(std::min(n, kChunk)) * per);
+    if (step == 0) b.vox.resize(static_cast<size_t>(std::min(n, kChunk)));
+  }
+  Twister tw(p->seed);
+  auto draw = [&](Buf& b, int64_t k0, int64_t k1) {
+    for (int64_t k = k0; k < k1; ++k) {
+      if (step == 0) b.vox[static_cast<size_t>(k - k0)] = tw.below_or_equal(static_cast<uint32_t>(N - 1));
+      uint32_t* d = &b.draws[static_cast<size_t>(k - k0) * per];
+      for (int q = 0; q < per; ++q) d[q] = tw.next();
     }
   };
-  Twister tw(p->seed);
-  double third[3];
-  for (int d = 0; d < D; ++d) third[d] = lv.spacing[d] / 3.0;
-  double x[3];
-  if (p->sampling == ANTS_SAMPLING_REGULAR) {
-    const int64_t step = static_cast<int64_t>(std::ceil(1.0 / p->sampling_percentage));
-    out.reserve(static_cast<size_t>((N + step - 1) / step) * D);
-    for (int64_t lin = 0; lin < N; lin += step) {
-      point(lin, x);
-      for (int d = 0; d < D; ++d) out.push_back(x[d] + tw.normal() * third[d]);
+  auto points = [&](const Buf& b, int64_t c0, int64_t k0, int64_t k1) {
+    for (int64_t k = k0; k < k1; ++k) {
+      const int64_t lin = step ? k * step : static_cast<int64_t>(b.vox[static_cast<size_t>(k - c0)]);
+      const double ix[3] = {static_cast<double>(lin % nx), static_cast<double>((lin / nx) % ny),
+                            static_cast<double>(lin / (nx * ny))};
+      const uint32_t* dr = &b.draws[static_cast<size_t>(k - c0) * per];
+      for (int r = 0; r < D; ++r) {
+        double x = g.org[r];
+        for (int c = 0; c < D; ++c) x = x + g.m[r * 3 + c] * ix[c];
+        out[static_cast<size_t>(k) * D + r] = x + box_muller(dr[2 * r], dr[2 * r + 1]) * third[r];
+      }
     }
-    return ANTS_OK;
-  }
-  if (N - 1 > static_cast<int64_t>(0xffffffffu)) return err(ANTS_ERR_UNSUPPORTED, "grid too large for random sampling");
-  const int64_t n = static_cast<int64_t>(std::floor(static_cast<double>(N) * p->sampling_percentage));
-  out.reserve(static_cast<size_t>(n) * D);
-  for (int64_t k = 0; k < n; ++k) {
-    point(static_cast<int64_t>(tw.below_or_equal(static_cast<uint32_t>(N - 1))), x);
-    for (int d = 0; d < D; ++d) out.push_back(x[d] + tw.normal() * third[d]);
+  };
+  const int64_t T = host_threads();
+  std::vector<std::thread> th;
+  int64_t cur0 = 0, cur1 = std::min(n, kChunk);
+  if (n > 0) draw(buf[0], cur0, cur1);
+  for (int c = 0; cur0 < n; ++c) {
+    Buf& b = buf[c & 1];
+    const int64_t span = cur1 - cur0;
+    const int64_t nt = std::max<int64_t>(1, std::min<int64_t>(T, span / 4096));
+    const int64_t part = (span + nt - 1) / nt;
+    for (int64_t t = 0; t < nt; ++t) {
+      const int64_t k0 = cur0 + t * part, k1 = std::min(cur1, k0 + part);
+      if (k0 < k1) th.emplace_back(points, std::cref(b), cur0, k0, k1);
+    }
+    const int64_t nxt0 = cur1, nxt1 = std::min(n, cur1 + kChunk);
+    if (nxt0 < n) draw(buf[(c + 1) & 1], nxt0, nxt1);
+    for (auto& x : th) x.join();
+    th.clear();
+    cur0 = nxt0;
+    cur1 = nxt1;
   }
   return ANTS_OK;
 }
@@ -383,6 +451,75 @@ struct DevMem {
   }
 };
 
+struct PinnedMem {
+  void* p = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t b) {
+    if (b <= bytes) return ANTS_OK;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+    if (hipHostMalloc(&p, b, hipHostMallocDefault) != hipSuccess) return err(ANTS_ERR_OOM, "pinned allocation failed");
+    bytes = b;
+    return ANTS_OK;
+  }
+  ~PinnedMem() {
+    if (p) (void)hipHostFree(p);
+  }
+};
+
+// Device copies of level sample points, shared between stages.  The points
+// depend only on the seed, the sampling strategy and percentage and the
+// level grid, so consecutive stages of one registration (the Rigid and
+// Affine stages of antsRegistrationSyN.sh share all of them) reuse one
+// upload.  At most ANTS_LINEAR_POINT_CACHE sets (default 8, 0 disables) are
+// kept per process, least recently used first out; a stage holds its set by
+// shared ownership, so an eviction never frees points in use.
+struct PointKey {
+  int device, dim, sampling;
+  uint32_t seed;
+  double percentage;
+  int64_t size[3];
+  double spacing[3], origin[3], direction[9];
+};
+struct PointSet {
+  DevMem mem;
+  int64_t n = 0;
+};
+struct PointCache {
+  std::mutex mu;
+  std::vector<std::pair<PointKey, std::shared_ptr<PointSet>>> sets;  // most recent first
+};
+PointCache& point_cache() {
+  static PointCache* c = new PointCache();  // never destroyed: no device frees during process exit
+  return *c;
+}
+size_t point_cache_capacity() {
+  static const size_t cap = [] {
+    const char* e = std::getenv("ANTS_LINEAR_POINT_CACHE");
+    return e ? static_cast<size_t>(std::max(0, std::atoi(e))) : static_cast<size_t>(8);
+  }();
+  return cap;
+}
+PointKey point_key(const ants_linear_params* p, const ants_image_desc& lv) {
+  PointKey k;
+  std::memset(&k, 0, sizeof(k));  // padding too: keys compare with memcmp
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  k.device = dev;
+  k.dim = lv.dim;
+  k.sampling = p->sampling;
+  k.seed = p->seed;
+  k.percentage = p->sampling_percentage;
+  for (int d = 0; d < lv.dim; ++d) {
+    k.size[d] = lv.size[d];
+    k.spacing[d] = lv.spacing[d];
+    k.origin[d] = lv.origin[d];
+  }
+  for (int i = 0; i < lv.dim * lv.dim; ++i) k.direction[i] = lv.direction[i];
+  return k;
+}
+
 // One stage's device state and its metric evaluations.
 struct LinearStage {
   ants_linear_params P{};
@@ -393,15 +530,18 @@ struct LinearStage {
   M3 MA = eye3(), FA = eye3();
   double mb[3] = {0, 0, 0}, fbv[3] = {0, 0, 0};
   double c[3] = {0, 0, 0};
-  DevMem fpre, mpre, flev, mlev, pts, partials, out, hist, fcnt, wtab;
+  DevMem fpre, mpre, flev, mlev, partials, res, wtab, minmax;
+  // res holds one evaluation's results contiguously: the Qmax sums, then for
+  // Mattes the B x B joint histogram (u64) and the B fixed-bin counts (u32);
+  // one copy brings them to the pinned mirror hres
+  PinnedMem hres, hw;
+  size_t hist_off = 0, fcnt_off = 0, res_bytes = 0;
+  std::shared_ptr<PointSet> pts;  // the level's sample points (shared, see PointCache)
   // level state
   ants_image_desc lv{};
   int64_t npts = 0;   // 0: the level grid
   double fmin = 0, fmax = 0, mmin = 0, mmax = 0;
   int64_t evaluations = 0;
-  std::vector<unsigned long long> hhost;
-  std::vector<unsigned int> chost;
-  std::vector<double> whost;
 
   int init(const ants_linear_params* p, const ants_image_desc* f, const float* d_f, const ants_image_desc* m,
            const float* d_m, const ants_transform* mi, int nmi, const ants_transform* fi, int nfi,
@@ -435,23 +575,65 @@ struct LinearStage {
     }
     const int Qmax = 6 + 3 * kLinMaxParams;
     LIN_TRY(partials.ensure(static_cast<size_t>(kLinMaxBlocks) * Qmax * sizeof(double)));
-    LIN_TRY(out.ensure(Qmax * sizeof(double)));
+    hist_off = Qmax * sizeof(double);
+    fcnt_off = hist_off;
+    res_bytes = hist_off;
     if (P.metric == ANTS_METRIC_MATTES) {
       const size_t B = static_cast<size_t>(P.bins);
-      LIN_TRY(hist.ensure(B * B * sizeof(unsigned long long)));
-      LIN_TRY(fcnt.ensure(B * sizeof(unsigned int)));
+      fcnt_off = hist_off + B * B * sizeof(unsigned long long);
+      res_bytes = fcnt_off + B * sizeof(unsigned int);
       LIN_TRY(wtab.ensure(B * B * sizeof(double)));
-      hhost.resize(B * B);
-      chost.resize(B);
-      whost.resize(B * B);
+      LIN_TRY(hw.ensure(B * B * sizeof(double)));
+    }
+    LIN_TRY(res.ensure(res_bytes));
+    LIN_TRY(hres.ensure(res_bytes));
+    return ANTS_OK;
+  }
+
+  // the level's sample points: from the cache, or made on the host and
+  // uploaded (and then offered to the cache)
+  int level_point_set() {
+    pts.reset();
+    npts = 0;
+    if (P.sampling == ANTS_SAMPLING_NONE) return ANTS_OK;
+    const PointKey key = point_key(&P, lv);
+    const size_t cap = point_cache_capacity();
+    PointCache& pc = point_cache();
+    if (cap > 0) {
+      std::lock_guard<std::mutex> lk(pc.mu);
+      for (size_t i = 0; i < pc.sets.size(); ++i)
+        if (std::memcmp(&pc.sets[i].first, &key, sizeof(key)) == 0) {
+          auto hit = pc.sets[i];
+          pc.sets.erase(pc.sets.begin() + static_cast<std::ptrdiff_t>(i));
+          pc.sets.insert(pc.sets.begin(), hit);
+          pts = hit.second;
+          npts = pts->n;
+          return ANTS_OK;
+        }
+    }
+    const int64_t n = level_point_count(&P, lv);
+    if (n == 0) return err(ANTS_ERR_INVALID, "no sample points on this level");
+    std::unique_ptr<double[]> hp(new double[static_cast<size_t>(n) * D]);
+    LIN_TRY(level_points(&P, lv, hp.get()));
+    auto set = std::make_shared<PointSet>();
+    LIN_TRY(set->mem.ensure(static_cast<size_t>(n) * D * sizeof(double)));
+    LIN_HIP(hipMemcpyAsync(set->mem.p, hp.get(), static_cast<size_t>(n) * D * sizeof(double), hipMemcpyHostToDevice,
+                           s));
+    LIN_HIP(hipStreamSynchronize(s));  // complete before any other stage may read it
+    set->n = n;
+    pts = set;
+    npts = n;
+    if (cap > 0) {
+      std::lock_guard<std::mutex> lk(pc.mu);
+      pc.sets.insert(pc.sets.begin(), std::make_pair(key, set));
+      while (pc.sets.size() > cap) pc.sets.pop_back();
     }
     return ANTS_OK;
   }
 
   int image_range(const float* d, int64_t n, double& lo, double& hi) {
-    DevMem tmp;
-    LIN_TRY(tmp.ensure(2 * sizeof(uint32_t)));
-    auto* mm = static_cast<uint32_t*>(tmp.p);
+    LIN_TRY(minmax.ensure(2 * sizeof(uint32_t)));
+    auto* mm = static_cast<uint32_t*>(minmax.p);
     launch_fill_u32(mm, 0xffffffffu, 1, s);
     launch_fill_u32(mm + 1, 0u, 1, s);
     launch_minmax(d, n, mm, s);
@@ -483,14 +665,7 @@ struct LinearStage {
       LIN_HIP(hipMemcpyAsync(flev.p, fpre.p, nf * sizeof(float), hipMemcpyDeviceToDevice, s));
       LIN_HIP(hipMemcpyAsync(mlev.p, mpre.p, nm * sizeof(float), hipMemcpyDeviceToDevice, s));
     }
-    std::vector<double> hp;
-    LIN_TRY(level_points(&P, lv, hp));
-    npts = static_cast<int64_t>(hp.size() / D);
-    if (P.sampling != ANTS_SAMPLING_NONE) {
-      if (npts == 0) return err(ANTS_ERR_INVALID, "no sample points on this level");
-      LIN_TRY(pts.ensure(hp.size() * sizeof(double)));
-      LIN_HIP(hipMemcpyAsync(pts.p, hp.data(), hp.size() * sizeof(double), hipMemcpyHostToDevice, s));
-    }
+    LIN_TRY(level_point_set());
     if (P.metric == ANTS_METRIC_MATTES) {
       LIN_TRY(image_range(static_cast<const float*>(flev.p), nf, fmin, fmax));
       LIN_TRY(image_range(static_cast<const float*>(mlev.p), nm, mmin, mmax));
@@ -506,7 +681,7 @@ struct LinearStage {
     LinDev L;
     std::memset(&L, 0, sizeof(L));
     LIN_TRY(host_geom(&lv, &L.vg));
-    L.pts = P.sampling != ANTS_SAMPLING_NONE ? static_cast<const double*>(pts.p) : nullptr;
+    L.pts = P.sampling != ANTS_SAMPLING_NONE ? static_cast<const double*>(pts->mem.p) : nullptr;
     L.npts = npts;
     L.fg = fgeom;
     L.mg = mgeom;
@@ -534,15 +709,15 @@ struct LinearStage {
     }
     LinJac J;
     jacobian_forms(P.transform, prm, D, J);
-    double* dout = static_cast<double*>(out.p);
-    double host[6 + 3 * kLinMaxParams];
+    double* dout = static_cast<double*>(res.p);
+    const double* host = static_cast<const double*>(hres.p);
     if (P.metric != ANTS_METRIC_MATTES) {
       const int mode = P.metric == ANTS_METRIC_MEAN_SQUARES ? kLinMS : kLinGC;
       if (!launch_lin_metric(L, J, D, np, mode, deriv, static_cast<double*>(partials.p), dout, nullptr, nullptr, s))
         return err(ANTS_ERR_UNSUPPORTED, "parameter count not supported");
       LIN_HIP(hipGetLastError());
       const int Q = lin_quantities(mode, deriv, np);
-      LIN_HIP(hipMemcpyAsync(host, dout, Q * sizeof(double), hipMemcpyDeviceToHost, s));
+      LIN_HIP(hipMemcpyAsync(hres.p, dout, Q * sizeof(double), hipMemcpyDeviceToHost, s));
       LIN_HIP(hipStreamSynchronize(s));
       const double N = host[0];
       *nvalid = static_cast<int64_t>(N);
@@ -573,15 +748,18 @@ struct LinearStage {
     L.mbs = (mmax - mmin) / static_cast<double>(B - 2 * pad);
     L.fnm = fmin / L.fbs - pad;
     L.mnm = mmin / L.mbs - pad;
-    LIN_HIP(hipMemsetAsync(hist.p, 0, static_cast<size_t>(B) * B * sizeof(unsigned long long), s));
-    LIN_HIP(hipMemsetAsync(fcnt.p, 0, static_cast<size_t>(B) * sizeof(unsigned int), s));
+    char* rb = static_cast<char*>(res.p);
+    LIN_HIP(hipMemsetAsync(rb + hist_off, 0, res_bytes - hist_off, s));
     launch_lin_metric(L, J, D, np, kLinHist, false, static_cast<double*>(partials.p), dout,
-                      static_cast<unsigned long long*>(hist.p), static_cast<unsigned int*>(fcnt.p), s);
+                      reinterpret_cast<unsigned long long*>(rb + hist_off), reinterpret_cast<unsigned int*>(rb + fcnt_off),
+                      s);
     LIN_HIP(hipGetLastError());
-    LIN_HIP(hipMemcpyAsync(host, dout, sizeof(double), hipMemcpyDeviceToHost, s));
-    LIN_HIP(hipMemcpyAsync(hhost.data(), hist.p, hhost.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
-    LIN_HIP(hipMemcpyAsync(chost.data(), fcnt.p, chost.size() * sizeof(unsigned int), hipMemcpyDeviceToHost, s));
+    // the count (sum 0) and everything from the histogram on, in one copy
+    LIN_HIP(hipMemcpyAsync(hres.p, res.p, res_bytes, hipMemcpyDeviceToHost, s));
     LIN_HIP(hipStreamSynchronize(s));
+    const auto* hhost = reinterpret_cast<const unsigned long long*>(static_cast<const char*>(hres.p) + hist_off);
+    const auto* chost = reinterpret_cast<const unsigned int*>(static_cast<const char*>(hres.p) + fcnt_off);
+    double* whost = static_cast<double*>(hw.p);
     const double N = host[0];
     *nvalid = static_cast<int64_t>(N);
     if (N < 1.0) return err(ANTS_ERR_INVALID, "no sample point maps inside both images");
@@ -610,13 +788,13 @@ struct LinearStage {
       }
     *value = -val;
     if (!deriv) return ANTS_OK;
-    LIN_HIP(hipMemcpyAsync(wtab.p, whost.data(), whost.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    LIN_HIP(hipMemcpyAsync(wtab.p, whost, static_cast<size_t>(B) * B * sizeof(double), hipMemcpyHostToDevice, s));
     L.wtab = static_cast<const double*>(wtab.p);
     if (!launch_lin_metric(L, J, D, np, kLinHistDeriv, true, static_cast<double*>(partials.p), dout, nullptr, nullptr,
                            s))
       return err(ANTS_ERR_UNSUPPORTED, "parameter count not supported");
     LIN_HIP(hipGetLastError());
-    LIN_HIP(hipMemcpyAsync(host, dout, (1 + np) * sizeof(double), hipMemcpyDeviceToHost, s));
+    LIN_HIP(hipMemcpyAsync(hres.p, dout, (1 + np) * sizeof(double), hipMemcpyDeviceToHost, s));
     LIN_HIP(hipStreamSynchronize(s));
     for (int q = 0; q < np; ++q) der[q] = host[1 + q] / (L.mbs * N);
     return ANTS_OK;
@@ -669,8 +847,15 @@ int run_stage(LinearStage& st, const double* p0, ants_linear_result* res, std::v
   std::memset(res, 0, sizeof(*res));
   res->n_params = np;
   double lr = P.gradient_step;
+  // ANTS_LINEAR_VERBOSE: per-level host timings on stderr (set-up: pyramid
+  // image, sample points, Mattes ranges; then the iterations)
+  const bool verbose = std::getenv("ANTS_LINEAR_VERBOSE") != nullptr;
+  using clk = std::chrono::steady_clock;
   for (int level = 0; level < P.n_levels; ++level) {
+    const auto t_set = clk::now();
+    const int64_t ev0 = st.evaluations;
     LIN_TRY(st.set_level(level));
+    const auto t_iter = clk::now();
     const std::vector<double> corners = corner_points(st.lv);
     double scales[kLinMaxParams];
     estimate_scales(P.transform, D, prm.data(), st.c, st.MA, corners, scales);
@@ -714,6 +899,15 @@ int run_stage(LinearStage& st, const double* p0, ants_linear_result* res, std::v
       ++it;
     }
     res->iterations[level] = it;
+    if (verbose) {
+      const auto t_end = clk::now();
+      const double ms_set = std::chrono::duration<double, std::milli>(t_iter - t_set).count();
+      const double ms_it = std::chrono::duration<double, std::milli>(t_end - t_iter).count();
+      const int64_t ne = st.evaluations - ev0;
+      std::fprintf(stderr, "linear level %d: %lld points, set-up %.2f ms, %d iterations, %lld evaluations in %.2f ms "
+                   "(%.3f ms each)\n", level, static_cast<long long>(st.npts), ms_set, it, static_cast<long long>(ne),
+                   ms_it, ne > 0 ? ms_it / static_cast<double>(ne) : 0.0);
+    }
   }
   for (int i = 0; i < np; ++i) res->params[i] = prm[i];
   M3 A;
@@ -775,12 +969,13 @@ int ants_linear_sample_points(const ants_linear_params* p, int level, const ants
   if (!fixed || fixed->dim != p->dim) return err(ANTS_ERR_INVALID, "fixed image dimension differs from params");
   ants_image_desc lv;
   LIN_TRY(host_level_desc(fixed, p->shrink[level], &lv));
-  std::vector<double> pts;
-  LIN_TRY(level_points(p, lv, pts));
-  const int64_t n = static_cast<int64_t>(pts.size()) / p->dim;
-  const int64_t w = std::min(n, capacity);
-  std::memcpy(out, pts.data(), static_cast<size_t>(w) * p->dim * sizeof(double));
+  const int64_t n = level_point_count(p, lv);
   *count = n;
+  if (capacity == 0 || n == 0) return ANTS_OK;
+  if (capacity >= n) return level_points(p, lv, out);
+  std::unique_ptr<double[]> pts(new double[static_cast<size_t>(n) * p->dim]);
+  LIN_TRY(level_points(p, lv, pts.get()));
+  std::memcpy(out, pts.get(), static_cast<size_t>(capacity) * p->dim * sizeof(double));
   return ANTS_OK;
 }
 
@@ -800,6 +995,13 @@ int ants_linear_estimate_scales(const ants_linear_params* p, int level, const an
   double c[3] = {0.0, 0.0, 0.0};
   for (int d = 0; d < D; ++d) c[d] = center ? center[d] : 0.0;
   estimate_scales(p->transform, D, params, c, MA, corner_points(lv), scales);
+  return ANTS_OK;
+}
+
+int ants_linear_release_points(void) {
+  PointCache& pc = point_cache();
+  std::lock_guard<std::mutex> lk(pc.mu);
+  pc.sets.clear();
   return ANTS_OK;
 }
 

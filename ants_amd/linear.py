@@ -78,6 +78,7 @@ def _bind():
     lib.ants_linear_sample_points.argtypes = [C.POINTER(LinearParamsC), i, pdesc, pd, C.c_int64,
                                               C.POINTER(C.c_int64)]
     lib.ants_linear_estimate_scales.argtypes = [C.POINTER(LinearParamsC), i, pdesc, px, i, pd, pd, pd]
+    lib.ants_linear_release_points.argtypes = []
     lib._linear_bound = True
     return lib
 
@@ -218,6 +219,11 @@ def register_linear(stage: LinearStage, fixed, fgeom: Geometry, moving, mgeom: G
             for k in range(min(n.value, cap))]
     return LinearResult(AffineTransform(A, t, c), np.array([res.params[k] for k in range(res.n_params)]),
                         [int(res.iterations[k]) for k in range(p.n_levels)], recs, int(res.metric_evaluations))
+
+
+def release_points() -> None:
+    """Free the sample-point sets kept on the device between stages."""
+    L.check(_bind().ants_linear_release_points())
 
 
 def metric_value(stage: LinearStage, level: int, fixed, fgeom: Geometry, moving, mgeom: Geometry, params,

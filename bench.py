@@ -15,7 +15,8 @@ without events, launched kernel by kernel (the library's default) and
 replayed from iteration graphs, measured after the timed region;
 `host_buffer_drop_in` the same registration through the host-buffer entry
 (PCIe transfers included); `bspline_syn` the same pair and schedule with the
-BSplineSyN field model (BSplineSyN[0.1,26,0,3]).  The other half of the metric comes from two
+BSplineSyN field model (BSplineSyN[0.1,26,0,3]); `linear_stages` the
+Rigid and Affine stages antsRegistrationSyN.sh runs before SyN.  The other half of the metric comes from two
 bounded legs on the same GPU: `batch` (64 mixed-size pairs, batched versus
 one at a time) and `serve` (a Poisson request stream at a fixed QPS: p50/p99
 latency, pairs/s and capacity), summarised in `batch_summary` and
@@ -332,6 +333,48 @@ def bspline_leg(pair, gf, d_fixed, d_moving, d_fwd, d_inv, steps) -> dict:
     return out
 
 
+def linear_leg(pair, d_fixed, d_moving) -> dict:
+    """The linear stages antsRegistrationSyN.sh runs before SyN, on the same
+    pair: -r [fixed,moving,1], then Rigid[0.1] and Affine[0.1], each with
+    MI[fixed,moving,1,32,Regular,0.25] (Mattes), -c [1000x500x250x100,1e-6,10],
+    -f 8x4x2x1, -s 3x2x1x0vox; both stages timed with the host clock."""
+    from ants_amd import _lib as L
+    from ants_amd import linear as LN
+    from ants_amd.apply_transforms import Geometry
+    from ants_amd.registration import initial_transform_from_images
+    geo = Geometry(pair.size, pair.spacing, pair.origin, np.eye(3))
+    stages = [LN.LinearStage.from_args(f"{t}[0.1]", "MI[fixed,moving,1,32,Regular,0.25]",
+                                       "[1000x500x250x100,1e-6,10]", "8x4x2x1", "3x2x1x0vox", seed=1)
+              for t in ("Rigid", "Affine")]
+
+    def run():
+        # cold start: the Rigid stage makes and uploads its sample points, the
+        # Affine stage reuses them (same seed and levels), as in one
+        # antsRegistrationSyN.sh run
+        LN.release_points()
+        chain = [initial_transform_from_images(pair.fixed, geo, pair.moving, geo, 1)]
+        out = []
+        for st in stages:
+            L.check(L.lib().ants_device_synchronize())
+            t0 = time.perf_counter()
+            r = LN.register_linear(st, d_fixed, geo, d_moving, geo, chain)
+            L.check(L.lib().ants_device_synchronize())
+            out.append((time.perf_counter() - t0, r))
+            chain = [r.transform] + chain
+        return out
+    run()  # warm-up: library set-up and first-touch of the level buffers
+    res = run()
+    legs = {}
+    for st, (sec, r) in zip(stages, res):
+        legs[st.transform] = {"ms": sec * 1e3, "iterations_run": r.iterations, "metric_evaluations": r.evaluations,
+                              "ms_per_evaluation": sec * 1e3 / max(r.evaluations, 1)}
+    log("linear stages: " + ", ".join(f"{k} {v['ms']:.1f} ms ({v['iterations_run']}, {v['metric_evaluations']} "
+                                      f"metric passes)" for k, v in legs.items()))
+    return {"stages": "-r [f,m,1]; Rigid[0.1] then Affine[0.1], MI[f,m,1,32,Regular,0.25], "
+                      "-c [1000x500x250x100,1e-6,10] -f 8x4x2x1 -s 3x2x1x0vox (antsRegistrationSyN.sh)",
+            **legs, "total_ms": sum(v["ms"] for v in legs.values())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -502,6 +545,7 @@ def main():
     else:
         os.environ["ANTS_SYN_GRAPHS"] = saved_env
     bsp = bspline_leg(pair, gf, d_fixed, d_moving, d_fwd, d_inv, args.steps)
+    lin = linear_leg(pair, d_fixed, d_moving)
     avg_ms = kacc["ms"] / max(kacc["launches"], 1)
     bytes_per_launch = kacc["bytes"] / max(kacc["launches"], 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
@@ -555,6 +599,7 @@ def main():
         "host_buffer_drop_in": host_drop_in,
         "inverse_consistency": ic,
         "bspline_syn": bsp,
+        "linear_stages": lin,
     }
     if legs:
         sv = legs["serve"]["continuous_batching"]

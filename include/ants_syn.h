@@ -607,6 +607,15 @@ int ants_linear_sample_points(const ants_linear_params* p, int level, const ants
 int ants_linear_estimate_scales(const ants_linear_params* p, int level, const ants_image_desc* fixed,
                                 const ants_transform* moving_initial, int n_moving, const double* center,
                                 const double* params, double* scales);
+/* Sample points are kept on the device between stages: they depend only on
+ * the seed, the sampling strategy and percentage and the level grid, so the
+ * Affine stage that follows a Rigid stage with the same metric sampling
+ * reuses the Rigid stage's points (ITK draws the same points for both from
+ * the same seed).  At most ANTS_LINEAR_POINT_CACHE sets (default 8; 0 turns
+ * the cache off) are held per process.  This call frees every set that no
+ * running stage holds.  No counterpart in the reference: ITK regenerates
+ * the points in each stage. */
+int ants_linear_release_points(void);
 
 #ifdef __cplusplus
 }

@@ -262,3 +262,27 @@ def test_cli_linear_stages_then_syn(tmp_path):
     r = LN.register_linear(st, F, gf, M, gm)
     got = aio.read_transform(p2 + "0GenericAffine.mat")
     np.testing.assert_allclose(got.parameters()[1], r.transform.parameters()[1], rtol=0, atol=1e-12)
+
+
+def test_point_cache_reuse_is_exact():
+    """Sample points kept on the device between stages are the points a
+    fresh stage makes: a stage run cold (cache released), again warm, and
+    one with another seed in between (its own points, not the first seed's)
+    give the first stage's parameters bit for bit, and the other seed gives
+    a different result."""
+    D = 3
+    gF, F, gM, M, mi, fi, c, _ = setup(D, "Rigid", "Mattes", "Regular", 0.25, 1, 0.0)
+    st = LN.LinearStage("Rigid", 0.25, "Mattes", bins=24, sampling="Random", percentage=0.3, iterations=(6, 4),
+                        shrink=(2, 1), sigma=(1.0, 0.0), seed=21, preprocess=True)
+    other = LN.LinearStage("Rigid", 0.25, "Mattes", bins=24, sampling="Random", percentage=0.3, iterations=(6, 4),
+                           shrink=(2, 1), sigma=(1.0, 0.0), seed=22, preprocess=True)
+    LN.release_points()
+    cold = LN.register_linear(st, F, gF, M, gM, mi, (), c)
+    oth = LN.register_linear(other, F, gF, M, gM, mi, (), c)
+    warm = LN.register_linear(st, F, gF, M, gM, mi, (), c)
+    LN.release_points()
+    again = LN.register_linear(st, F, gF, M, gM, mi, (), c)
+    np.testing.assert_array_equal(warm.params, cold.params)
+    np.testing.assert_array_equal(again.params, cold.params)
+    assert warm.records == cold.records
+    assert not np.array_equal(oth.params, cold.params)
