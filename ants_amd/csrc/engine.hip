@@ -3142,3 +3142,415 @@ int64_t ants_batch_pending(ants_batch* b) {
 }
 
 }  // extern "C"
+
+// ===========================================================================
+// Dense field-transform stages with the CC metric: -t GaussianDisplacementField
+// and -t Exponential (Examples/itkantsRegistrationHelper.hxx:1733-1847,
+// 3112-3277), each run by ImageRegistrationMethodv4.  Per level: the virtual
+// domain is the shrunk fixed grid, the images are smoothed at full
+// resolution, the transform's field (displacement u, or velocity v with
+// u = exp(v)) is resampled onto the new grid.  Per iteration: the metric --
+// fixed image at the virtual points through the fixed initial transforms,
+// moving image at y = movingInit(x + u(x)) and its central-difference
+// gradient there, CC box sums over the virtual grid, value -sum localCC / N,
+// derivative c1 * A^T grad M(y) per voxel (local support: not divided by N)
+// -- then the optimizer: GaussianDisplacementField runs the conjugate-
+// gradient line search of every non-SyN stage (hxx:1290-1372; rate
+// estimated at each level's first iteration as step / max |g|, scales 1 for
+// a field in physical units) with the update u <- u + smooth(w, updVar),
+// then smooth(u, totVar); Exponential runs plain gradient descent without
+// a scales estimator (hxx:1405-1418): v <- v + smooth(step g, updVar),
+// then smooth(v, velVar), u = exp(v) by scaling and squaring.
+namespace {
+
+// Recursive golden-section search over [a, c] with interior b (published
+// recursive form; epsilon 0.2, at most 20 steps: the helper's line-search
+// settings, hxx:1357-1362).
+template <class F>
+struct GoldenSection {
+  F f;
+  int it = 0;
+  int rc = ANTS_OK;
+  double search(double a, double b, double c, double fb, bool have_fb) {
+    const double resphi = 2.0 - (1.0 + std::sqrt(5.0)) / 2.0;
+    if (it > 20) return (c + a) / 2.0;
+    ++it;
+    const double x = (c - b > b - a) ? b + resphi * (c - b) : b - resphi * (b - a);
+    if (std::fabs(c - a) < 0.2 * (std::fabs(b) + std::fabs(x))) return (c + a) / 2.0;
+    double fx = 0.0;
+    if ((rc = f(x, &fx)) != ANTS_OK) return b;
+    if (!have_fb && (rc = f(b, &fb)) != ANTS_OK) return b;
+    if (fx < fb) return (c - b > b - a) ? search(b, x, c, fx, true) : search(a, x, b, fx, true);
+    return (c - b > b - a) ? search(a, b, x, fb, true) : search(x, b, c, fb, true);
+  }
+};
+
+int validate_field_params(const ants_field_params* p) {
+  if (!p) return fail(ANTS_ERR_INVALID, "null parameters");
+  if (p->dim != 2 && p->dim != 3) return fail(ANTS_ERR_INVALID, "dim must be 2 or 3");
+  if (p->model != ANTS_FIELD_GAUSSIAN_DISPLACEMENT && p->model != ANTS_FIELD_EXPONENTIAL)
+    return fail(ANTS_ERR_UNSUPPORTED, "field model must be GaussianDisplacementField or Exponential");
+  if (p->n_levels < 1 || p->n_levels > ANTS_MAX_LEVELS) return fail(ANTS_ERR_INVALID, "n_levels must be 1..8");
+  for (int l = 0; l < p->n_levels; ++l) {
+    if (p->shrink[l] < 1) return fail(ANTS_ERR_INVALID, "shrink factors must be >= 1");
+    if (!(p->sigma[l] >= 0.0)) return fail(ANTS_ERR_INVALID, "smoothing sigmas must be >= 0");
+    if (p->iterations[l] < 0) return fail(ANTS_ERR_INVALID, "iterations must be >= 0");
+  }
+  if (p->convergence_window < 1) return fail(ANTS_ERR_INVALID, "convergence window must be >= 1");
+  if (!(p->learning_rate > 0.0)) return fail(ANTS_ERR_INVALID, "gradient step must be > 0");
+  if (!(p->update_variance >= 0.0) || !(p->total_variance >= 0.0))
+    return fail(ANTS_ERR_INVALID, "field variances must be >= 0");
+  if (p->integration_steps < 0 || p->integration_steps > 30)
+    return fail(ANTS_ERR_INVALID, "integration steps must be 0..30");
+  if (p->cc_radius < 1 || p->cc_radius > 16) return fail(ANTS_ERR_INVALID, "CC radius must be 1..16");
+  return ANTS_OK;
+}
+
+struct FieldEngine {
+  ants_field_params P{};
+  int D = 3;
+  hipStream_t s = nullptr;
+  HostGrid gF{}, gM{}, gl{};
+  GeomDev fgeo{}, mgeo{}, lgeo{};
+  FieldMap fmap{}, mmap{};
+  DevBuf F, M, Fs, Ms, tmp, minmax;
+  DevBuf Fw, Mw, G;
+  DevBuf state, u, g, last, conj, trial, upd, sm, work, extra, phi2;
+  DevBuf ccpart, slot, dpart, dout, mbits;
+  int64_t evaluations = 0;
+  double minsp = 1.0;
+
+  int buf(DevBuf& b, int64_t floats) { return b.ensure(static_cast<size_t>(std::max<int64_t>(floats, 1)) * 4); }
+
+  int init(const ants_field_params* p, const ants_image_desc* fd, const float* d_fixed, const ants_image_desc* md,
+           const float* d_moving, const ants_transform* mi, int nmi, const ants_transform* fi, int nfi,
+           void* stream) {
+    ANTS_TRY(validate_field_params(p));
+    P = *p;
+    D = P.dim;
+    s = static_cast<hipStream_t>(stream);
+    if (!fd || !md || !d_fixed || !d_moving) return fail(ANTS_ERR_INVALID, "null image");
+    if (fd->dim != D || md->dim != D) return fail(ANTS_ERR_INVALID, "image dimension differs from params");
+    ANTS_TRY(to_host_grid(fd, &gF));
+    ANTS_TRY(to_host_grid(md, &gM));
+    ANTS_TRY(geom_from_desc(fd, &fgeo));
+    ANTS_TRY(geom_from_desc(md, &mgeo));
+    InitialXforms X;
+    ANTS_TRY(compose_initial(D, fi, nfi, mi, nmi, &X));
+    if (X.general[0] || X.general[1])
+      return fail(ANTS_ERR_UNSUPPORTED, "field-transform stages take affine initial transforms only");
+    for (int k = 0; k < 9; ++k) {
+      fmap.a[k] = X.A[0][k];
+      mmap.a[k] = X.A[1][k];
+    }
+    for (int k = 0; k < 3; ++k) {
+      fmap.b[k] = X.o[0][k];
+      mmap.b[k] = X.o[1][k];
+      fmap.sp[k] = gF.sp[k];
+      mmap.sp[k] = gM.sp[k];
+    }
+    const int64_t NF = gF.count(), NM = gM.count();
+    ANTS_TRY(buf(F, NF));
+    ANTS_TRY(buf(M, NM));
+    ANTS_TRY(buf(Fs, NF));
+    ANTS_TRY(buf(Ms, NM));
+    ANTS_TRY(buf(tmp, std::max(NF, NM)));
+    if (P.preprocess) {
+      uint32_t pair[2];
+      ANTS_TRY(preprocess_on(s, minmax, d_fixed, F.f(), NF, pair));
+      ANTS_TRY(preprocess_on(s, minmax, d_moving, M.f(), NM, pair));
+    } else {
+      HIP_TRY(hipMemcpyAsync(F.p, d_fixed, NF * sizeof(float), hipMemcpyDeviceToDevice, s));
+      HIP_TRY(hipMemcpyAsync(M.p, d_moving, NM * sizeof(float), hipMemcpyDeviceToDevice, s));
+    }
+    int64_t nmax = 1;
+    for (int l = 0; l < P.n_levels; ++l) nmax = std::max(nmax, level_grid(gF, P.shrink[l]).count());
+    const int64_t nf = nmax * D;
+    ANTS_TRY(buf(Fw, nmax));
+    ANTS_TRY(buf(Mw, nmax));
+    ANTS_TRY(buf(G, nf));
+    for (DevBuf* b : {&state, &u, &g, &last, &conj, &trial, &upd, &sm, &work, &extra, &phi2}) ANTS_TRY(buf(*b, nf));
+    int64_t ncc = 1;
+    for (int l = 0; l < P.n_levels; ++l)
+      ncc = std::max(ncc, cc_partials_needed(to_dev(level_grid(gF, P.shrink[l])), P.cc_radius));
+    ANTS_TRY(ccpart.ensure(static_cast<size_t>(ncc) * sizeof(Partial)));
+    ANTS_TRY(slot.ensure(sizeof(RedSlot)));
+    ANTS_TRY(dpart.ensure(static_cast<size_t>(field_dots_partials(nmax)) * 4 * sizeof(double)));
+    ANTS_TRY(dout.ensure(4 * sizeof(double)));
+    ANTS_TRY(mbits.ensure(16));
+    return ANTS_OK;
+  }
+
+  int64_t nvox() const { return gl.count(); }
+
+  // the level grid, its smoothed images and the fixed samples; `prev`: the
+  // previous level's grid (the state field is resampled from it), or null
+  int set_level(int l, const HostGrid* prev) {
+    gl = level_grid(gF, P.shrink[l]);
+    ants_image_desc ld;
+    to_desc(gl, &ld);
+    ANTS_TRY(geom_from_desc(&ld, &lgeo));
+    minsp = gl.sp[0];
+    for (int d = 1; d < D; ++d) minsp = std::min(minsp, gl.sp[d]);
+    ANTS_TRY(smooth_image_on(s, gF, F.f(), Fs.f(), tmp.f(), P.sigma[l], P.sigma_in_mm != 0));
+    ANTS_TRY(smooth_image_on(s, gM, M.f(), Ms.f(), tmp.f(), P.sigma[l], P.sigma_in_mm != 0));
+    const int64_t nf = nvox() * D;
+    if (!prev) {
+      HIP_TRY(hipMemsetAsync(state.p, 0, nf * sizeof(float), s));
+    } else {
+      const float* in[1] = {state.f()};
+      float* out[1] = {trial.f()};
+      launch_resample_fields(to_dev(*prev), to_dev(gl), 1, in, out, s);
+      std::swap(state.p, trial.p);
+      std::swap(state.bytes, trial.bytes);
+    }
+    launch_field_sample(lgeo, fgeo, Fs.f(), nullptr, fmap, Fw.f(), nullptr, s);
+    HIP_TRY(hipGetLastError());
+    if (P.model == ANTS_FIELD_EXPONENTIAL) {
+      int n = 0;
+      ANTS_TRY(exponentiate(state.f(), u.f(), 1, P.integration_steps, &n));
+    }
+    return ANTS_OK;
+  }
+
+  int dots(const float* a, const float* b, double out[4]) {
+    launch_field_dots(D, nvox(), a, b, static_cast<double*>(dpart.p), static_cast<double*>(dout.p), s);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, dout.p, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return ANTS_OK;
+  }
+
+  // value (and with deriv the derivative into g) at displacement ufield (null: 0)
+  int metric(const float* ufield, bool deriv, double* value) {
+    ++evaluations;
+    launch_field_sample(lgeo, mgeo, Ms.f(), ufield, mmap, Mw.f(), deriv ? G.f() : nullptr, s);
+    launch_cc(to_dev(gl), Fw.f(), Mw.f(), P.cc_radius, std::numeric_limits<float>::epsilon(),
+              deriv ? g.f() : nullptr, nullptr, static_cast<Partial*>(ccpart.p), static_cast<RedSlot*>(slot.p), s,
+              true, false, G.f());
+    HIP_TRY(hipGetLastError());
+    RedSlot r;
+    HIP_TRY(hipMemcpyAsync(&r, slot.p, sizeof(r), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    *value = -r.sum / static_cast<double>(nvox());
+    return ANTS_OK;
+  }
+
+  // out = smooth(in, var) (GaussianSmoothDisplacementField), or a copy when
+  // var <= 0; in is used as scratch by the separable path
+  int smooth_into(float* in, float* out, double var) {
+    if (!(var > 0.0)) {
+      if (out != in) HIP_TRY(hipMemcpyAsync(out, in, nvox() * D * sizeof(float), hipMemcpyDeviceToDevice, s));
+      return ANTS_OK;
+    }
+    float* i1[1] = {in};
+    float* w1[1] = {work.f()};
+    float* o1[1] = {out};
+    float* e1[1] = {extra.f()};
+    return smooth_fields_on(s, gl, 1, i1, w1, o1, var, static_cast<unsigned int*>(mbits.p), nullptr, e1);
+  }
+
+  // ExponentialDisplacementFieldImageFilter: steps given, or automatic (the
+  // smallest n with max |v| / 2^n below half the smallest spacing,
+  // floor(2 + log2(max |v| / min spacing) + 1), capped at the transform's
+  // default of 10); phi = sign v / 2^n, then n times phi <- phi + phi(x + phi)
+  int exponentiate(const float* v, float* out, int sign, int steps, int* used) {
+    const int64_t nf = nvox() * D;
+    int n = steps;
+    if (n <= 0) {
+      double q[4];
+      ANTS_TRY(dots(v, nullptr, q));
+      const double m2 = q[3] / (minsp * minsp);
+      n = 0;
+      if (m2 > 0.0) {
+        const double nfl = 2.0 + 0.5 * std::log(m2) / std::log(2.0);
+        if (nfl >= 0.0) n = std::min(static_cast<int>(nfl + 1.0), 10);
+      }
+    }
+    *used = n;
+    launch_field_axpy(nf, nullptr, static_cast<float>(sign) / static_cast<float>(1u << n), v, out, s);
+    float* cur = out;
+    float* nxt = phi2.f();
+    const GridDev gd = to_dev(gl);
+    for (int k = 0; k < n; ++k) {
+      const float* dsp[1] = {cur};
+      float* o1[1] = {nxt};
+      launch_compose(gd, 1, dsp, dsp, o1, nullptr, 1.f, s);
+      std::swap(cur, nxt);
+    }
+    if (cur != out) HIP_TRY(hipMemcpyAsync(out, cur, nf * sizeof(float), hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipGetLastError());
+    return ANTS_OK;
+  }
+
+  // GaussianDisplacementField update with step s along conj into dst:
+  // dst = state + smooth(s conj, updVar), then smooth(dst, totVar)
+  int gdf_update(double step, float* dst) {
+    const int64_t nf = nvox() * D;
+    launch_field_axpy(nf, nullptr, static_cast<float>(step), conj.f(), upd.f(), s);
+    const float* src = upd.f();
+    if (P.update_variance > 0.0) {
+      ANTS_TRY(smooth_into(upd.f(), sm.f(), P.update_variance));
+      src = sm.f();
+    }
+    launch_field_axpy(nf, state.f(), 1.f, src, dst, s);
+    if (P.total_variance > 0.0) {
+      ANTS_TRY(smooth_into(dst, sm.f(), P.total_variance));
+      HIP_TRY(hipMemcpyAsync(dst, sm.p, nf * sizeof(float), hipMemcpyDeviceToDevice, s));
+    }
+    HIP_TRY(hipGetLastError());
+    return ANTS_OK;
+  }
+
+  int run(std::vector<ants_linear_record>& trace, int* iterations_out) {
+    HostGrid prev{};
+    for (int l = 0; l < P.n_levels; ++l) {
+      ANTS_TRY(set_level(l, l ? &prev : nullptr));
+      prev = gl;
+      const int64_t nf = nvox() * D;
+      std::vector<double> energies;
+      double lr = P.learning_rate;
+      int it = 0;
+      while (it < P.iterations[l]) {
+        double val = 0.0;
+        const bool gdf = P.model == ANTS_FIELD_GAUSSIAN_DISPLACEMENT;
+        ANTS_TRY(metric(gdf ? state.f() : u.f(), true, &val));
+        energies.push_back(val);
+        double conv = 0.0;
+        ANTS_TRY(ants_syn_convergence_value(energies.data(), static_cast<int>(energies.size()),
+                                            P.convergence_window, &conv));
+        if (conv <= P.convergence_threshold && !P.force_iterations) break;
+        if (gdf) {
+          if (it == 0) {
+            HIP_TRY(hipMemsetAsync(conj.p, 0, nf * sizeof(float), s));
+            HIP_TRY(hipMemsetAsync(last.p, 0, nf * sizeof(float), s));
+            double q[4];
+            ANTS_TRY(dots(g.f(), nullptr, q));
+            const double ss = std::sqrt(q[3]);
+            lr = ss <= std::numeric_limits<double>::epsilon() ? 1.0 : P.learning_rate / ss;
+          }
+          double q[4];
+          ANTS_TRY(dots(g.f(), last.f(), q));  // g.last, last.last, g.g
+          double gamma = 0.0;
+          if (q[1] > std::numeric_limits<double>::epsilon()) gamma = (q[2] - q[0]) / q[1];
+          if (gamma < 0.0 || gamma > 5.0) gamma = 0.0;
+          HIP_TRY(hipMemcpyAsync(last.p, g.p, nf * sizeof(float), hipMemcpyDeviceToDevice, s));
+          launch_field_axpy(nf, g.f(), static_cast<float>(gamma), conj.f(), conj.f(), s);
+          auto trial_value = [this](double step, double* v) -> int {
+            ANTS_TRY(gdf_update(step, trial.f()));
+            return metric(trial.f(), false, v);
+          };
+          GoldenSection<decltype(trial_value)> ls{trial_value};
+          const double step = ls.search(lr * 0.0, lr, lr * 2.0, 0.0, false);
+          if (ls.rc != ANTS_OK) return ls.rc;
+          lr = step;
+          ANTS_TRY(gdf_update(lr, trial.f()));
+          std::swap(state.p, trial.p);
+          std::swap(state.bytes, trial.bytes);
+        } else {
+          lr = P.learning_rate;
+          launch_field_axpy(nf, nullptr, static_cast<float>(lr), g.f(), upd.f(), s);
+          const float* src = upd.f();
+          if (P.update_variance > 0.0) {
+            ANTS_TRY(smooth_into(upd.f(), sm.f(), P.update_variance));
+            src = sm.f();
+          }
+          launch_field_axpy(nf, state.f(), 1.f, src, trial.f(), s);
+          if (P.total_variance > 0.0) {
+            ANTS_TRY(smooth_into(trial.f(), state.f(), P.total_variance));
+          } else {
+            std::swap(state.p, trial.p);
+            std::swap(state.bytes, trial.bytes);
+          }
+          int n = 0;
+          ANTS_TRY(exponentiate(state.f(), u.f(), 1, P.integration_steps, &n));
+        }
+        trace.push_back(ants_linear_record{l, it + 1, val, conv, lr});
+        ++it;
+      }
+      if (iterations_out) iterations_out[l] = it;
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    return ANTS_OK;
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+int ants_field_register(const ants_field_params* p, const ants_image_desc* fixed, const float* d_fixed,
+                        const ants_image_desc* moving, const float* d_moving, const ants_transform* moving_initial,
+                        int n_moving, const ants_transform* fixed_initial, int n_fixed, float* d_out_field,
+                        float* d_out_inverse, float* d_out_velocity, ants_linear_record* trace,
+                        int64_t trace_capacity, int64_t* trace_count, int* iterations_out, void* stream) {
+  if (trace_capacity < 0 || (trace_capacity > 0 && !trace)) return fail(ANTS_ERR_INVALID, "bad trace buffer");
+  FieldEngine e;
+  ANTS_TRY(e.init(p, fixed, d_fixed, moving, d_moving, moving_initial, n_moving, fixed_initial, n_fixed, stream));
+  std::vector<ants_linear_record> tr;
+  ANTS_TRY(e.run(tr, iterations_out));
+  const int64_t w = std::min<int64_t>(trace_capacity, static_cast<int64_t>(tr.size()));
+  for (int64_t i = 0; i < w; ++i) trace[i] = tr[i];
+  if (trace_count) *trace_count = static_cast<int64_t>(tr.size());
+  const size_t bytes = static_cast<size_t>(e.nvox()) * e.D * sizeof(float);
+  hipStream_t s = e.s;
+  if (e.P.model == ANTS_FIELD_GAUSSIAN_DISPLACEMENT) {
+    if (d_out_inverse || d_out_velocity)
+      return fail(ANTS_ERR_INVALID, "GaussianDisplacementField has no inverse or velocity output");
+    if (d_out_field) HIP_TRY(hipMemcpyAsync(d_out_field, e.state.p, bytes, hipMemcpyDeviceToDevice, s));
+  } else {
+    int n = 0;
+    if (d_out_field) HIP_TRY(hipMemcpyAsync(d_out_field, e.u.p, bytes, hipMemcpyDeviceToDevice, s));
+    if (d_out_inverse) ANTS_TRY(e.exponentiate(e.state.f(), d_out_inverse, -1, e.P.integration_steps, &n));
+    if (d_out_velocity) HIP_TRY(hipMemcpyAsync(d_out_velocity, e.state.p, bytes, hipMemcpyDeviceToDevice, s));
+  }
+  HIP_TRY(hipStreamSynchronize(s));
+  return ANTS_OK;
+}
+
+int ants_field_metric(const ants_field_params* p, int level, const ants_image_desc* fixed, const float* d_fixed,
+                      const ants_image_desc* moving, const float* d_moving, const ants_transform* moving_initial,
+                      int n_moving, const ants_transform* fixed_initial, int n_fixed, const float* d_field,
+                      double* value, float* d_derivative, void* stream) {
+  if (!value) return fail(ANTS_ERR_INVALID, "null value");
+  FieldEngine e;
+  ANTS_TRY(e.init(p, fixed, d_fixed, moving, d_moving, moving_initial, n_moving, fixed_initial, n_fixed, stream));
+  if (level < 0 || level >= p->n_levels) return fail(ANTS_ERR_INVALID, "level out of range");
+  e.gl = level_grid(e.gF, p->shrink[level]);
+  ants_image_desc ld;
+  to_desc(e.gl, &ld);
+  ANTS_TRY(geom_from_desc(&ld, &e.lgeo));
+  ANTS_TRY(smooth_image_on(e.s, e.gF, e.F.f(), e.Fs.f(), e.tmp.f(), p->sigma[level], p->sigma_in_mm != 0));
+  ANTS_TRY(smooth_image_on(e.s, e.gM, e.M.f(), e.Ms.f(), e.tmp.f(), p->sigma[level], p->sigma_in_mm != 0));
+  launch_field_sample(e.lgeo, e.fgeo, e.Fs.f(), nullptr, e.fmap, e.Fw.f(), nullptr, e.s);
+  ANTS_TRY(e.metric(d_field, d_derivative != nullptr, value));
+  if (d_derivative)
+    HIP_TRY(hipMemcpyAsync(d_derivative, e.g.p, static_cast<size_t>(e.nvox()) * e.D * sizeof(float),
+                           hipMemcpyDeviceToDevice, e.s));
+  HIP_TRY(hipStreamSynchronize(e.s));
+  return ANTS_OK;
+}
+
+int ants_op_exponential(const ants_image_desc* g, const float* d_v, float* d_out, int steps, int sign,
+                        int* steps_used, void* stream) {
+  if (!g || !d_v || !d_out) return fail(ANTS_ERR_INVALID, "null argument");
+  if (sign != 1 && sign != -1) return fail(ANTS_ERR_INVALID, "sign must be 1 or -1");
+  if (steps < 0 || steps > 30) return fail(ANTS_ERR_INVALID, "steps must be 0..30");
+  FieldEngine e;
+  e.s = static_cast<hipStream_t>(stream);
+  e.D = g->dim;
+  ANTS_TRY(to_host_grid(g, &e.gl));
+  e.minsp = e.gl.sp[0];
+  for (int d = 1; d < e.D; ++d) e.minsp = std::min(e.minsp, e.gl.sp[d]);
+  const int64_t n = e.gl.count();
+  ANTS_TRY(e.buf(e.phi2, n * e.D));
+  ANTS_TRY(e.dpart.ensure(static_cast<size_t>(field_dots_partials(n)) * 4 * sizeof(double)));
+  ANTS_TRY(e.dout.ensure(4 * sizeof(double)));
+  int used = 0;
+  ANTS_TRY(e.exponentiate(d_v, d_out, sign, steps, &used));
+  HIP_TRY(hipStreamSynchronize(e.s));
+  if (steps_used) *steps_used = used;
+  return ANTS_OK;
+}
+
+}  // extern "C"

@@ -974,11 +974,13 @@ __device__ __forceinline__ CCOut cc_voxel(double cnt, double sf, double sm, doub
 }
 
 // Direct CC (any D): each thread sums its (2r+1)^D window from global memory.
-template <int D>
+// EXT: the moving gradient comes from gext (physical, at each voxel's mapped
+// point: the v4 metric of a field transform); only g1 = a1 * gext is written.
+template <int D, bool EXT = false>
 __global__ void __launch_bounds__(kBlock) k_cc_direct(GridDev g, const float* __restrict__ F,
                                                       const float* __restrict__ M, int R, float eps,
                                                       float* __restrict__ g1, float* __restrict__ g2,
-                                                      Partial* partials) {
+                                                      Partial* partials, const float* __restrict__ gext) {
   const int64_t N = g.count();
   double cc_sum = 0.0;
   for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < N;
@@ -1004,6 +1006,13 @@ __global__ void __launch_bounds__(kBlock) k_cc_direct(GridDev g, const float* __
     const float f = F[i], m = M[i];
     const CCOut o = cc_voxel(cnt, sf, sm, sff, smm, sfm, f, m, eps);
     cc_sum += o.cc;
+    if constexpr (EXT) {
+      if (g1) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) g1[i * D + d] = o.a1 * gext[i * D + d];
+      }
+      continue;
+    }
     const int pos[3] = {x, y, z};
     const int64_t st[3] = {1, g.n[0], static_cast<int64_t>(g.n[0]) * g.n[1]};
     float vm[3] = {0.f, 0.f, 0.f}, vf[3] = {0.f, 0.f, 0.f};
@@ -1077,10 +1086,13 @@ constexpr int kCCXSeg = ANTS_CC_XSEG;
 // re-added from the ring in a fixed order every plane (no running sum to
 // drift), and the per-voxel formula is evaluated in double.  Out-of-volume
 // samples stay 0, so the shifted sums are the in-bounds sums of f - c.
-template <int R, int TX, int TY, class ST = double>
+// EXT: the moving gradient is read from gext (see k_cc_direct); the
+// in-plane and z gradients are not formed and only g1 is written.
+template <int R, int TX, int TY, class ST = double, bool EXT = false>
 __global__ void __launch_bounds__(TX* TY, cc_waves<ST>()) k_cc3d(GridDev g, const float* __restrict__ F, const float* __restrict__ M,
                                                  float eps, int zchunk, float* __restrict__ g1,
-                                                 float* __restrict__ g2, Partial* partials) {
+                                                 float* __restrict__ g2, Partial* partials,
+                                                 const float* __restrict__ gext) {
   constexpr int W = 2 * R + 1;
   constexpr int LX = TX + 2 * R, LY = TY + 2 * R;
   constexpr int NT = TX * TY;
@@ -1219,7 +1231,7 @@ __global__ void __launch_bounds__(TX* TY, cc_waves<ST>()) k_cc3d(GridDev g, cons
     // centre values and in-plane gradients of plane zz at (x, y)
     const float fc = q[0][ty + R][tx + R], mc = q[1][ty + R][tx + R];
     float gfx = 0.f, gfy = 0.f, gmx = 0.f, gmy = 0.f;
-    if (active && zin) {
+    if (!EXT && active && zin) {
       gfx = cdiff(q[0][ty + R][tx + R - 1], fc, q[0][ty + R][tx + R + 1], x, nx, g.sp[0], g.isp[0]);
       gmx = cdiff(q[1][ty + R][tx + R - 1], mc, q[1][ty + R][tx + R + 1], x, nx, g.sp[0], g.isp[0]);
       gfy = cdiff(q[0][ty + R - 1][tx + R], fc, q[0][ty + R + 1][tx + R], y, ny, g.sp[1], g.isp[1]);
@@ -1334,12 +1346,20 @@ __global__ void __launch_bounds__(TX* TY, cc_waves<ST>()) k_cc3d(GridDev g, cons
                                   static_cast<double>(zsum[2]), static_cast<double>(zsum[3]),
                                   static_cast<double>(zsum[4]), f, m, eps);
       cc_sum += o.cc;
+      const int64_t i = static_cast<int64_t>(zc) * plane + static_cast<int64_t>(y) * nx + x;
+      if constexpr (EXT) {
+        if (g1) {
+          g1[i * 3 + 0] = o.a1 * gext[i * 3 + 0];
+          g1[i * 3 + 1] = o.a1 * gext[i * 3 + 1];
+          g1[i * 3 + 2] = o.a1 * gext[i * 3 + 2];
+        }
+        return;
+      }
       float gmz = 0.f, gfz = 0.f;
       if (o.a1 != 0.f || o.a2 != 0.f) {
         gmz = cdiff(zc > 0 ? rm[cm] : m, m, rm[cp], zc, nz, g.sp[2], g.isp[2]);
         gfz = cdiff(zc > 0 ? rf[cm] : f, f, rf[cp], zc, nz, g.sp[2], g.isp[2]);
       }
-      const int64_t i = static_cast<int64_t>(zc) * plane + static_cast<int64_t>(y) * nx + x;
       float vm[3] = {o.a1 * rgm[cg][0], o.a1 * rgm[cg][1], o.a1 * gmz};
       float vf[3] = {o.a2 * rgf[cg][0], o.a2 * rgf[cg][1], o.a2 * gfz};
       if (g.oblique) {
@@ -2361,14 +2381,16 @@ __global__ void __launch_bounds__(kLinThreads) k_lin_metric(LinDev L, LinJac J, 
     if (!lin_sample(L.mg, L.mimg, py, m)) continue;
     double dm[P > 0 ? P : 1];
     if (GRAD || MODE == kLinHistDeriv) {
-      // CentralDifferenceImageFunction at py: neighbours +-spacing[d] along
-      // physical axis d, 0 when either leaves the buffer
+      // CentralDifferenceImageFunction::Evaluate(point) at py: neighbours
+      // -+ half a spacing along physical axis d, 0 when either leaves the
+      // buffer
       double gr[3] = {0.0, 0.0, 0.0};
 #pragma unroll
       for (int d = 0; d < D; ++d) {
         double lo[3] = {py[0], py[1], py[2]}, hi[3] = {py[0], py[1], py[2]};
-        lo[d] = py[d] - L.msp[d];
-        hi[d] = py[d] + L.msp[d];
+        const double off = 0.5 * L.msp[d];
+        lo[d] = py[d] - off;
+        hi[d] = py[d] + off;
         double vl, vh;
         if (!lin_sample(L.mg, L.mimg, lo, vl)) continue;
         if (!lin_sample(L.mg, L.mimg, hi, vh)) continue;
@@ -2491,6 +2513,143 @@ __global__ void __launch_bounds__(256) k_lin_reduce(const double* __restrict__ p
     if (threadIdx.x == 0) out[q] = sh[0];
     __syncthreads();
   }
+}
+
+// ---------------------------------------------------------------------------
+// Dense field-transform stages (GaussianDisplacementField, Exponential).
+// The image at y = A (x + u(x)) + b for each voxel x of the virtual grid
+// (u may be null), linear, 0 outside its buffer; with GRAD also the
+// CentralDifferenceImageFunction gradient at y -- samples half a spacing
+// either side along each physical axis, (hi - lo) / (their distance), 0 for
+// an axis where either leaves the buffer -- taken back through A (A^T grad):
+// the derivative of the metric by the displacement at x.  Double
+// coordinates, contraction off (as the antsApplyTransforms kernel).
+template <int D, bool GRAD>
+__global__ void __launch_bounds__(kBlock) k_field_sample(GeomDev vg, GeomDev ig, const float* __restrict__ img,
+                                                         const float* __restrict__ u, FieldMap map,
+                                                         float* __restrict__ out, float* __restrict__ grad) {
+  const int64_t N = static_cast<int64_t>(vg.n[0]) * vg.n[1] * (D == 3 ? vg.n[2] : 1);
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int64_t nx = vg.n[0], ny = vg.n[1];
+  const int ix[3] = {static_cast<int>(i % nx), static_cast<int>((i / nx) % ny), static_cast<int>(i / (nx * ny))};
+  double x[3];
+  geom_point<D>(vg, ix, x);
+  if (u) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) x[d] = x[d] + static_cast<double>(u[i * D + d]);
+  }
+  double y[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int r = 0; r < D; ++r) {
+    double t = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) t = t + map.a[r * 3 + k] * x[k];
+    y[r] = t + map.b[r];
+  }
+  double v = 0.0;
+  if (!lin_sample(ig, img, y, v)) v = 0.0;
+  out[i] = static_cast<float>(v);
+  if constexpr (GRAD) {
+    double gr[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      double lo[3] = {y[0], y[1], y[2]}, hi[3] = {y[0], y[1], y[2]};
+      const double off = 0.5 * map.sp[d];
+      lo[d] = y[d] - off;
+      hi[d] = y[d] + off;
+      double vl, vh;
+      if (!lin_sample(ig, img, lo, vl)) continue;
+      if (!lin_sample(ig, img, hi, vh)) continue;
+      const double delta = hi[d] - lo[d];
+      gr[d] = delta > 10.0 * 2.220446049250313e-16 ? (vh - vl) / delta : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      double t = 0.0;
+#pragma unroll
+      for (int d = 0; d < D; ++d) t = t + gr[d] * map.a[d * 3 + k];
+      grad[i * D + k] = static_cast<float>(t);
+    }
+  }
+}
+
+// Per-block partials of a field pair: sum a.b, sum b.b, sum a.a, max |a|^2
+// (b may be null: zeros), over n vectors of D components; fixed order
+// within a block.
+template <int D>
+__global__ void __launch_bounds__(kLinThreads) k_field_dots(int64_t n, const float* __restrict__ a,
+                                                            const float* __restrict__ b,
+                                                            double* __restrict__ partials) {
+  double ab = 0.0, bb = 0.0, aa = 0.0, am = 0.0;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    double s2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const double av = a[i * D + d];
+      s2 += av * av;
+      if (b) {
+        const double bv = b[i * D + d];
+        ab += av * bv;
+        bb += bv * bv;
+      }
+    }
+    aa += s2;
+    am = fmax(am, s2);
+  }
+  constexpr int NW = kLinThreads / 64;
+  __shared__ double red[4][NW];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  ab = wave_sum(ab);
+  bb = wave_sum(bb);
+  aa = wave_sum(aa);
+  for (int o = 32; o > 0; o >>= 1) am = fmax(am, __shfl_xor(am, o));
+  if (lane == 0) {
+    red[0][wid] = ab;
+    red[1][wid] = bb;
+    red[2][wid] = aa;
+    red[3][wid] = am;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, m = 0.0;
+    for (int w = 0; w < NW; ++w) {
+      s0 += red[0][w];
+      s1 += red[1][w];
+      s2 += red[2][w];
+      m = fmax(m, red[3][w]);
+    }
+    partials[blockIdx.x * 4 + 0] = s0;
+    partials[blockIdx.x * 4 + 1] = s1;
+    partials[blockIdx.x * 4 + 2] = s2;
+    partials[blockIdx.x * 4 + 3] = m;
+  }
+}
+
+// One thread: the block partials in block order (three sums, a max).
+__global__ void k_field_dots_final(const double* __restrict__ partials, int nb, double* __restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, m = 0.0;
+  for (int b = 0; b < nb; ++b) {
+    s0 += partials[b * 4 + 0];
+    s1 += partials[b * 4 + 1];
+    s2 += partials[b * 4 + 2];
+    m = fmax(m, partials[b * 4 + 3]);
+  }
+  out[0] = s0;
+  out[1] = s1;
+  out[2] = s2;
+  out[3] = m;
+}
+
+// out = a + s * b elementwise (a null: s * b); out may alias a or b.
+__global__ void __launch_bounds__(kBlock) k_field_axpy(int64_t n, const float* a, float sc, const float* b,
+                                                       float* out) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float t = sc * b[i];
+  out[i] = a ? a[i] + t : t;
 }
 
 inline unsigned int blocks_for(int64_t n, int bs = kBlock) { return static_cast<unsigned int>((n + bs - 1) / bs); }
@@ -2851,44 +3010,65 @@ int64_t cc_partials_needed(const GridDev& g, int radius, bool fp32_sums) {
 }
 
 int launch_cc(const GridDev& g, const float* F, const float* M, int radius, float eps, float* gF2M, float* gM2M,
-              Partial* partials, RedSlot* slot, hipStream_t s, bool reduce, bool fp32_sums) {
-  if (!cc_tiled(g, radius)) return launch_cc_direct(g, F, M, radius, eps, gF2M, gM2M, partials, slot, s, reduce);
-  const CCShape sh = cc_shape(g, radius, fp32_sums);
+              Partial* partials, RedSlot* slot, hipStream_t s, bool reduce, bool fp32_sums, const float* gext) {
+  if (!cc_tiled(g, radius))
+    return launch_cc_direct(g, F, M, radius, eps, gF2M, gM2M, partials, slot, s, reduce, gext);
+  const CCShape sh = cc_shape(g, radius, fp32_sums && !gext);
   const int zc = sh.zc;
   const dim3 grid((g.n[0] + sh.tx - 1) / sh.tx, (g.n[1] + sh.ty - 1) / sh.ty, (g.n[2] + zc - 1) / zc);
-  const bool d = cc_sums_fp64(fp32_sums);
+  const bool d = gext || cc_sums_fp64(fp32_sums);
   const bool narrow = sh.ty != kCCTY;
   constexpr int TY2 = kCCTY / 2;
-  if (radius == 4 && d && narrow)
+  if (gext) {
+    // field transforms: double sums, moving gradient from gext
+    if (radius == 4 && narrow)
+      hipLaunchKernelGGL((k_cc3d<4, kCCTX, TY2, double, true>), grid, dim3(kCCTX * TY2), 0, s, g, F, M, eps, zc, gF2M,
+                         gM2M, partials, gext);
+    else if (radius == 4)
+      hipLaunchKernelGGL((k_cc3d<4, kCCTX, kCCTY, double, true>), grid, dim3(kCCTX * kCCTY), 0, s, g, F, M, eps, zc,
+                         gF2M, gM2M, partials, gext);
+    else if (narrow)
+      hipLaunchKernelGGL((k_cc3d<2, kCCTX, TY2, double, true>), grid, dim3(kCCTX * TY2), 0, s, g, F, M, eps, zc, gF2M,
+                         gM2M, partials, gext);
+    else
+      hipLaunchKernelGGL((k_cc3d<2, kCCTX, kCCTY, double, true>), grid, dim3(kCCTX * kCCTY), 0, s, g, F, M, eps, zc,
+                         gF2M, gM2M, partials, gext);
+  } else if (radius == 4 && d && narrow)
     hipLaunchKernelGGL((k_cc3d<4, kCCTX, TY2, double>), grid, dim3(kCCTX * TY2), 0, s, g, F, M, eps, zc, gF2M, gM2M,
-                       partials);
+                       partials, nullptr);
   else if (radius == 4 && d)
     hipLaunchKernelGGL((k_cc3d<4, kCCTX, kCCTY, double>), grid, dim3(kCCTX * kCCTY), 0, s, g, F, M, eps, zc, gF2M,
-                       gM2M, partials);
+                       gM2M, partials, nullptr);
   else if (radius == 4)
     hipLaunchKernelGGL((k_cc3d<4, kCCTX, kCCTY, float>), grid, dim3(kCCTX * kCCTY), 0, s, g, F, M, eps, zc, gF2M,
-                       gM2M, partials);
+                       gM2M, partials, nullptr);
   else if (d && narrow)
     hipLaunchKernelGGL((k_cc3d<2, kCCTX, TY2, double>), grid, dim3(kCCTX * TY2), 0, s, g, F, M, eps, zc, gF2M, gM2M,
-                       partials);
+                       partials, nullptr);
   else if (d)
     hipLaunchKernelGGL((k_cc3d<2, kCCTX, kCCTY, double>), grid, dim3(kCCTX * kCCTY), 0, s, g, F, M, eps, zc, gF2M,
-                       gM2M, partials);
+                       gM2M, partials, nullptr);
   else
     hipLaunchKernelGGL((k_cc3d<2, kCCTX, kCCTY, float>), grid, dim3(kCCTX * kCCTY), 0, s, g, F, M, eps, zc, gF2M,
-                       gM2M, partials);
+                       gM2M, partials, nullptr);
   const int np = static_cast<int>(grid.x * grid.y * grid.z);
   if (reduce) hipLaunchKernelGGL(k_reduce_slot<kBlock>, dim3(1), dim3(kBlock), 0, s, partials, np, slot);
   return np;
 }
 
 int launch_cc_direct(const GridDev& g, const float* F, const float* M, int radius, float eps, float* gF2M,
-                     float* gM2M, Partial* partials, RedSlot* slot, hipStream_t s, bool reduce) {
+                     float* gM2M, Partial* partials, RedSlot* slot, hipStream_t s, bool reduce, const float* gext) {
   const dim3 grid(red_blocks(g.count()));
-  if (g.D == 2)
-    hipLaunchKernelGGL(k_cc_direct<2>, grid, dim3(kBlock), 0, s, g, F, M, radius, eps, gF2M, gM2M, partials);
+  if (g.D == 2 && gext)
+    hipLaunchKernelGGL((k_cc_direct<2, true>), grid, dim3(kBlock), 0, s, g, F, M, radius, eps, gF2M, gM2M, partials,
+                       gext);
+  else if (gext)
+    hipLaunchKernelGGL((k_cc_direct<3, true>), grid, dim3(kBlock), 0, s, g, F, M, radius, eps, gF2M, gM2M, partials,
+                       gext);
+  else if (g.D == 2)
+    hipLaunchKernelGGL(k_cc_direct<2>, grid, dim3(kBlock), 0, s, g, F, M, radius, eps, gF2M, gM2M, partials, nullptr);
   else
-    hipLaunchKernelGGL(k_cc_direct<3>, grid, dim3(kBlock), 0, s, g, F, M, radius, eps, gF2M, gM2M, partials);
+    hipLaunchKernelGGL(k_cc_direct<3>, grid, dim3(kBlock), 0, s, g, F, M, radius, eps, gF2M, gM2M, partials, nullptr);
   const int np = static_cast<int>(grid.x);
   if (reduce) hipLaunchKernelGGL(k_reduce_slot<kBlock>, dim3(1), dim3(kBlock), 0, s, partials, np, slot);
   return np;
@@ -3190,4 +3370,34 @@ bool launch_lin_metric(const LinDev& L, const LinJac& J, int D, int P, int mode,
   return true;
 }
 
+
+void launch_field_sample(const GeomDev& vg, const GeomDev& ig, const float* img, const float* u, const FieldMap& map,
+                         float* out, float* grad, hipStream_t s) {
+  const int64_t n = static_cast<int64_t>(vg.n[0]) * vg.n[1] * (vg.D == 3 ? vg.n[2] : 1);
+  const dim3 grid(blocks_for(n));
+  if (vg.D == 2 && grad)
+    hipLaunchKernelGGL((k_field_sample<2, true>), grid, dim3(kBlock), 0, s, vg, ig, img, u, map, out, grad);
+  else if (vg.D == 2)
+    hipLaunchKernelGGL((k_field_sample<2, false>), grid, dim3(kBlock), 0, s, vg, ig, img, u, map, out, grad);
+  else if (grad)
+    hipLaunchKernelGGL((k_field_sample<3, true>), grid, dim3(kBlock), 0, s, vg, ig, img, u, map, out, grad);
+  else
+    hipLaunchKernelGGL((k_field_sample<3, false>), grid, dim3(kBlock), 0, s, vg, ig, img, u, map, out, grad);
+}
+
+int field_dots_partials(int64_t n) { return static_cast<int>(lin_blocks(n)); }
+
+void launch_field_dots(int D, int64_t n, const float* a, const float* b, double* partials, double* out,
+                       hipStream_t s) {
+  const int nb = field_dots_partials(n);
+  if (D == 2)
+    hipLaunchKernelGGL(k_field_dots<2>, dim3(nb), dim3(kLinThreads), 0, s, n, a, b, partials);
+  else
+    hipLaunchKernelGGL(k_field_dots<3>, dim3(nb), dim3(kLinThreads), 0, s, n, a, b, partials);
+  hipLaunchKernelGGL(k_field_dots_final, dim3(1), dim3(64), 0, s, partials, nb, out);
+}
+
+void launch_field_axpy(int64_t n, const float* a, float sc, const float* b, float* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_field_axpy, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, a, sc, b, out);
+}
 }  // namespace ants

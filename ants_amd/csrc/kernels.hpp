@@ -163,10 +163,15 @@ int64_t cc_partials_needed(const GridDev& g, int radius, bool fp32_sums = false)
 // fp32_sums: float box sums over block-shifted samples (the BF16 pyramid
 // form's precision) instead of double sums; ANTS_SYN_CC_SUMS=fp32|fp64
 // overrides the choice.
+// gext (field transforms): the moving image's physical gradient at each
+// voxel's mapped point; then gF2M = a1 * gext (not written when gF2M is
+// null: the value alone) and gM2M is not written.
 int launch_cc(const GridDev& g, const float* F, const float* M, int radius, float eps, float* gF2M, float* gM2M,
-              Partial* partials, RedSlot* slot, hipStream_t s, bool reduce = true, bool fp32_sums = false);
+              Partial* partials, RedSlot* slot, hipStream_t s, bool reduce = true, bool fp32_sums = false,
+              const float* gext = nullptr);
 int launch_cc_direct(const GridDev& g, const float* F, const float* M, int radius, float eps, float* gF2M,
-                     float* gM2M, Partial* partials, RedSlot* slot, hipStream_t s, bool reduce = true);
+                     float* gM2M, Partial* partials, RedSlot* slot, hipStream_t s, bool reduce = true,
+                     const float* gext = nullptr);
 // One block: reduce cc_n CC partials into *energy (skipped when cc_parts is
 // null), copy nwords words of src to pinned host memory dst, then set *flag
 // (pinned host memory) to 1 once the copy is visible to the host.
@@ -321,6 +326,25 @@ int64_t lin_blocks(int64_t npts);
 // unsupported (D, P).
 bool launch_lin_metric(const LinDev& L, const LinJac& J, int D, int P, int mode, bool deriv, double* partials,
                        double* out, unsigned long long* hist, unsigned int* fcnt, hipStream_t s);
+
+// Dense field-transform stages: y = a (x + u(x)) + b maps a virtual-grid
+// voxel into an image; sp = that image's spacing (central differences).
+struct FieldMap {
+  double a[9];
+  double b[3];
+  double sp[3];
+};
+// out = img(y) (0 outside), grad (optional) = a^T grad img(y) per voxel.
+void launch_field_sample(const GeomDev& vg, const GeomDev& ig, const float* img, const float* u, const FieldMap& map,
+                         float* out, float* grad, hipStream_t s);
+// out[0] = sum a.b, out[1] = sum b.b, out[2] = sum a.a, out[3] = max |a|^2
+// over n D-vectors (b may be null); partials: field_dots_partials(n) x 4
+// doubles.
+int field_dots_partials(int64_t n);
+void launch_field_dots(int D, int64_t n, const float* a, const float* b, double* partials, double* out,
+                       hipStream_t s);
+// out = a + sc * b over n floats (a may be null; out may alias a or b).
+void launch_field_axpy(int64_t n, const float* a, float sc, const float* b, float* out, hipStream_t s);
 
 void launch_labels_nn(const GridDev& go, const float* u, const GridDev& gl, const int32_t* labels, int32_t* out,
                       hipStream_t s);

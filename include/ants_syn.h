@@ -617,6 +617,67 @@ int ants_linear_estimate_scales(const ants_linear_params* p, int level, const an
  * the points in each stage. */
 int ants_linear_release_points(void);
 
+/* ---- dense field-transform stages with the CC metric ----
+ * -t GaussianDisplacementField[step, updateVariance, totalVariance]
+ *    (Examples/itkantsRegistrationHelper.hxx:1733-1847: ImageRegistrationMethodv4
+ *    with GaussianSmoothingOnUpdateDisplacementFieldTransform and the
+ *    conjugate-gradient line-search optimizer of hxx:1290-1372)
+ * -t Exponential[step, updateVariance, velocityVariance, integrationSteps]
+ *    (hxx:3112-3277: GaussianExponentialDiffeomorphicTransform with the
+ *    gradient-descent optimizer of hxx:1405-1418; integration steps 0 =
+ *    automatic), parsed at Examples/antsRegistrationTemplateHeader.h:692-699, 871-886.
+ * Variances are in voxel^2 (variance space, as the CLI gives them). */
+#define ANTS_FIELD_GAUSSIAN_DISPLACEMENT 0
+#define ANTS_FIELD_EXPONENTIAL 1
+typedef struct ants_field_params {
+  int dim;
+  int model;                            /* ANTS_FIELD_* */
+  int n_levels;
+  int shrink[ANTS_MAX_LEVELS];          /* -f */
+  double sigma[ANTS_MAX_LEVELS];        /* -s */
+  int sigma_in_mm;
+  int iterations[ANTS_MAX_LEVELS];      /* -c [..,threshold,window] */
+  double convergence_threshold;
+  int convergence_window;
+  double learning_rate;                 /* step */
+  double update_variance;
+  double total_variance;                /* GaussianDisplacementField: total field; Exponential: velocity field */
+  int integration_steps;                /* Exponential: squarings of exp(v); 0 = automatic */
+  int cc_radius;                        /* CC[f,m,1,radius] */
+  int preprocess;                       /* window both images to [0,1] (PreprocessImage) */
+  int force_iterations;                 /* 1: never stop early (parity harness) */
+} ants_field_params;
+
+/* Replaces registrationMethod->Update() of such a stage and the transform it
+ * adds to the composite (hxx:1838-1846, 3268-3276).  d_fixed / d_moving:
+ * full-resolution images in HBM; moving_initial / fixed_initial: identity
+ * or affine transforms in the order a point passes them.  Outputs on the
+ * last level's grid (ants_syn_level_geometry of the last level), D floats
+ * per voxel, any may be NULL: d_out_field the displacement field (for
+ * Exponential exp(v)), d_out_inverse exp(-v) (Exponential only),
+ * d_out_velocity v (Exponential only).  trace: the 2DIAGNOSTIC records
+ * (value, convergence, learning rate per iteration); iterations_out: per
+ * level.  Synchronous. */
+int ants_field_register(const ants_field_params* p, const ants_image_desc* fixed, const float* d_fixed,
+                        const ants_image_desc* moving, const float* d_moving, const ants_transform* moving_initial,
+                        int n_moving, const ants_transform* fixed_initial, int n_fixed, float* d_out_field,
+                        float* d_out_inverse, float* d_out_velocity, ants_linear_record* trace,
+                        int64_t trace_capacity, int64_t* trace_count, int* iterations_out, void* stream);
+/* One metric evaluation of such a stage on a level for a given displacement
+ * (level grid, D floats per voxel; NULL = zero): the CC value and, with
+ * d_derivative, the derivative field (c1 * A^T grad M at each mapped
+ * point).  No counterpart call in ANTs: the metric's GetValueAndDerivative
+ * as the optimizer sees it, for tests. */
+int ants_field_metric(const ants_field_params* p, int level, const ants_image_desc* fixed, const float* d_fixed,
+                      const ants_image_desc* moving, const float* d_moving, const ants_transform* moving_initial,
+                      int n_moving, const ants_transform* fixed_initial, int n_fixed, const float* d_field,
+                      double* value, float* d_derivative, void* stream);
+/* exp(v) (sign +1) or exp(-v) (sign -1) of a velocity field on grid g by
+ * ExponentialDisplacementFieldImageFilter's scaling and squaring (steps 0 =
+ * automatic); *steps_used receives the squaring count. */
+int ants_op_exponential(const ants_image_desc* g, const float* d_v, float* d_out, int steps, int sign,
+                        int* steps_used, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

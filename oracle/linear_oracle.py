@@ -244,16 +244,18 @@ def interpolate(g: O.Grid, img: np.ndarray, pts: np.ndarray) -> Tuple[np.ndarray
 
 
 def gradient(g: O.Grid, img: np.ndarray, pts: np.ndarray) -> np.ndarray:
-    """CentralDifferenceImageFunction at points: neighbours +-spacing[d]
-    along physical axis d, 0 where either is outside the buffer"""
+    """CentralDifferenceImageFunction::Evaluate(point): neighbours -+ half a
+    spacing along physical axis d, (hi - lo) / delta, 0 where either is
+    outside the buffer"""
     D = g.dim
     sp = np.asarray(g.spacing, dtype=np.float64)
     out = np.zeros_like(pts)
     for d in range(D):
         lo = pts.copy()
         hi = pts.copy()
-        lo[:, d] = pts[:, d] - sp[d]
-        hi[:, d] = pts[:, d] + sp[d]
+        off = 0.5 * sp[d]
+        lo[:, d] = pts[:, d] - off
+        hi[:, d] = pts[:, d] + off
         vl, il = interpolate(g, img, lo)
         vh, ih = interpolate(g, img, hi)
         delta = hi[:, d] - lo[:, d]

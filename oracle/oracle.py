@@ -87,6 +87,8 @@ def lib():
         L.og_resample_field_f64.argtypes = [pg, pd, pg, pd]
         L.og_cc_f64.argtypes = [pg, pd, pd, C.c_int, d, pd, pd]
         L.og_cc_f64.restype = d
+        L.og_cc_ext_f64.argtypes = [pg, pd, pd, C.c_int, d, pd, pd]
+        L.og_cc_ext_f64.restype = d
         L.og_scale_update_f64.argtypes = [pg, pd, d]
         L.og_scale_update_f64.restype = d
         L.og_bspline_fit_field_f64.argtypes = [pg, pd, pd, P(C.c_int), C.c_int]
@@ -230,6 +232,17 @@ def cc(g: Grid, F, M, radius: int, eps: float = FLT_EPSILON):
     g2 = np.empty(g.shape + (g.dim,))
     v = lib().og_cc_f64(C.byref(g.c()), _pd(f), _pd(m), radius, eps, _pd(g1), _pd(g2))
     return v, g1, g2
+
+
+def cc_ext(g: Grid, F, M, radius: int, gext, eps: float = FLT_EPSILON):
+    """CC value and c1 * gext per voxel: the v4 metric's derivative with the
+    moving gradient taken at each voxel's mapped point (field transforms)."""
+    f = _f64(F)
+    m = _f64(M)
+    ge = _f64(gext)
+    out = np.empty(g.shape + (g.dim,))
+    v = lib().og_cc_ext_f64(C.byref(g.c()), _pd(f), _pd(m), radius, eps, _pd(ge), _pd(out))
+    return v, out
 
 
 def scale_update(g: Grid, u, lr: float):

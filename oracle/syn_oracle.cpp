@@ -612,8 +612,13 @@ inline R grad_component(const R* img, int64_t i, int64_t pos, int64_t len, int64
   return static_cast<R>((up - dn) / h);
 }
 
+// gext (optional): the moving image's gradient at each voxel's mapped point,
+// already physical (ImageToImageMetricv4 with a field transform: the
+// CentralDifferenceImageFunction of the moving image at the mapped point);
+// then g_f2m = c1 * gext and g_m2m is not written.
 template <class R>
-double cc_metric(const Grid& g, const R* F, const R* M, int radius, double eps, R* g_f2m, R* g_m2m) {
+double cc_metric(const Grid& g, const R* F, const R* M, int radius, double eps, R* g_f2m, R* g_m2m,
+                 const R* gext = nullptr) {
   const int D = g.D;
   const int64_t N = g.count();
   const int64_t nx = g.n[0], ny = g.n[1], nz = g.n[2];
@@ -681,13 +686,15 @@ double cc_metric(const Grid& g, const R* F, const R* M, int radius, double eps, 
           const A a = A(2) * sFM / (sFF * sMM);
           const R c1 = static_cast<R>(a * (fA - sFM / sMM * mA));
           const R c2 = static_cast<R>(a * (mA - sFM / sFF * fA));
+          if (gext)
+            for (int d = 0; d < D; ++d) gm[d] = gext[i * D + d];
           for (int d = 0; d < D; ++d) {
             gm[d] *= c1;
             gf[d] *= c2;
           }
           // CentralDifferenceImageFunction with UseImageDirection: the
           // index-axis derivative is turned into a physical vector
-          if (g.oblique) {
+          if (g.oblique && !gext) {
             R rm[3] = {0, 0, 0}, rf[3] = {0, 0, 0};
             for (int r = 0; r < D; ++r)
               for (int k = 0; k < D; ++k) {
@@ -702,7 +709,7 @@ double cc_metric(const Grid& g, const R* F, const R* M, int radius, double eps, 
         }
         for (int d = 0; d < D; ++d) {
           if (g_f2m) g_f2m[i * D + d] = gm[d];
-          if (g_m2m) g_m2m[i * D + d] = gf[d];
+          if (g_m2m && !gext) g_m2m[i * D + d] = gf[d];
         }
         local += static_cast<double>(cc);
       }
@@ -1330,6 +1337,10 @@ void og_resample_field_f64(const og_grid* gi, const double* in, const og_grid* g
 }
 double og_cc_f64(const og_grid* g, const double* F, const double* M, int radius, double eps, double* g_f2m, double* g_m2m) {
   return cc_metric<double>(to_grid(g), F, M, radius, eps, g_f2m, g_m2m);
+}
+double og_cc_ext_f64(const og_grid* g, const double* F, const double* M, int radius, double eps, const double* gext,
+                     double* out) {
+  return cc_metric<double>(to_grid(g), F, M, radius, eps, out, nullptr, gext);
 }
 double og_scale_update_f64(const og_grid* g, double* u, double lr) { return scale_update<double>(to_grid(g), u, lr); }
 void og_bspline_fit_field_f64(const og_grid* g, const double* in, double* out, const int* mesh, int order) {

@@ -189,6 +189,23 @@ def test_oracle_metric_values_closed_forms():
     assert v == pytest.approx(-mi, rel=1e-9)
 
 
+def test_central_difference_at_points_hand_values():
+    """CentralDifferenceImageFunction::Evaluate(point): samples half a
+    spacing either side, divided by their distance.  On the impulse
+    0, 0, 1, 0, 0 (spacing 2) at x = 4.5 (index 2.25): 0.25 at index 2.75,
+    0.75 at 1.75 -> -0.5 / 2 = -0.25 (whole-spacing neighbours would give
+    -0.0625); at index 0.1 the lower neighbour leaves the buffer -> 0."""
+    g = O.Grid((5, 1, 1), (2.0, 1.0, 1.0), (0.0, 0.0, 0.0), np.eye(3))
+    img = np.zeros((1, 1, 5))
+    img[0, 0, 2] = 1.0
+    pts = np.array([[4.5, 0.0, 0.0], [0.2, 0.0, 0.0], [3.0, 0.0, 0.0]])
+    gr = LO.gradient(g, img, pts)
+    assert gr[0, 0] == pytest.approx(-0.25, abs=1e-15)
+    assert gr[1, 0] == 0.0
+    assert gr[2, 0] == pytest.approx(0.5, abs=1e-15)    # index 1.5: (1 at index 2 - 0 at 1) / 2 (not 0.125)
+    assert np.all(gr[:, 1:] == 0.0)                      # one-voxel axes: neighbours outside
+
+
 def test_golden_section_search():
     """On a unimodal function the search brackets the minimum to its
     tolerance; on a decreasing one it runs to the upper end."""
