@@ -21,7 +21,10 @@ antsRegistrationSyN.sh writes them (Scripts/antsRegistrationSyN.sh:487-497):
 (Translation, Rigid or Affine with MeanSquares, GC or Mattes;
 ``--random-seed`` seeds their metric sampling).  Their transforms, composed
 with the -r initial transforms, become ``out_0GenericAffine.mat``; with no
-SyN stage that file is the whole output.
+SyN stage that file is the whole output.  The last stage may instead be a
+dense field-transform stage, ``-t GaussianDisplacementField[0.25,3,0]`` or
+``-t Exponential[0.25,3,0.5,0]`` with ``-m CC[...]``: it writes
+``out_{k}Warp.nii.gz`` (Exponential also ``out_{k}InverseWarp.nii.gz``).
 
 ``python -m ants_amd.cli apply -d 3 -i moving.nii.gz -r fixed.nii.gz
   -o out.nii.gz -t out_0Warp.nii.gz -t [affine.mat,1] -n Linear
@@ -49,6 +52,8 @@ import numpy as np
 
 from . import io as aio
 from .apply_transforms import AffineTransform, DisplacementFieldTransform, Geometry, apply_transforms, compose_affines
+from .fieldmodels import FieldStage, register_field
+from .fieldmodels import level_geometry as field_level_geometry
 
 
 def _bracket_list(s: str) -> List[str]:
@@ -156,7 +161,7 @@ def cmd_registration(argv: Sequence[str]) -> int:
         raise SystemExit("each stage needs one -t, -m, -c, -f and -s (in stage order)")
     linear = [LinearStage.is_linear(t) for t in a.transform]
     if any(not lin for lin in linear[:-1]):
-        raise SystemExit("a SyN / BSplineSyN stage must be the last stage")
+        raise SystemExit("a SyN / BSplineSyN / GaussianDisplacementField / Exponential stage must be the last stage")
     images = {}
 
     def image(name):
@@ -226,6 +231,8 @@ def cmd_registration(argv: Sequence[str]) -> int:
             inv = AffineTransform(aff.matrix, aff.translation, aff.center, invert=True)
             aio.write_nifti(outs[2], apply_transforms(gm, np.asarray(fixed, np.float32), gf, [inv], "Linear"), gm)
         return 0
+    if FieldStage.is_field(a.transform[-1]):
+        return _field_stage_outputs(a, fixed, gf, moving, gm, init, finit, outs, prefix)
     # a BSplineSyN knot spacing becomes a mesh with the fixed image's geometry
     # (antsRegistrationTemplateHeader.h:709-731)
     stage = SyNStage.from_args(a.transform[-1], a.metric[-1], a.convergence[-1], a.shrink_factors[-1],
@@ -290,6 +297,47 @@ def cmd_registration(argv: Sequence[str]) -> int:
     if len(outs) > 2 and outs[2]:
         inv_warped = apply_transforms(gm, np.asarray(fixed, np.float32), gf, inv_chain, "Linear")
         aio.write_nifti(outs[2], inv_warped, gm)
+    return 0
+
+
+def _field_stage_outputs(a, fixed, gf, moving, gm, init, finit, outs, prefix) -> int:
+    """A GaussianDisplacementField or Exponential last stage: its field (and
+    for Exponential the inverse) written as <prefix>{k}Warp.nii.gz /
+    <prefix>{k}InverseWarp.nii.gz after 0GenericAffine.mat when initial
+    transforms precede it, as antsRegistration names a displacement-field
+    transform; GaussianDisplacementField keeps no inverse field."""
+    st = FieldStage.from_args(a.transform[-1], a.metric[-1], a.convergence[-1], a.shrink_factors[-1],
+                              a.smoothing_sigmas[-1])
+    if any(isinstance(t, DisplacementFieldTransform) for t in init + finit):
+        raise SystemExit("field-transform stages take affine initial transforms only")
+    if a.save_state or a.restore_state or a.print_similarity_measure_interval or a.write_composite_transform:
+        raise SystemExit("--save-state / --restore-state / the full-scale CC / -a 1 need a SyN stage here")
+    if finit and (len(outs) > 1 and any(outs[1:])):
+        raise SystemExit("-q with warped-image outputs is not supported (write the warp files only)")
+    res = register_field(st, np.asarray(fixed, np.float32), gf, np.asarray(moving, np.float32), gm, init, finit)
+    if a.verbose:
+        for (lev, it, v, cv, lr) in res.records:
+            print(f" 2DIAGNOSTIC, {it:5d}, {v:.12e}, {cv:.12e}, ")
+    g_out = field_level_geometry(st, gf, len(st.shrink) - 1)
+    k = 0
+    fwd_chain = [DisplacementFieldTransform(res.field, g_out)]
+    inv_chain = [DisplacementFieldTransform(res.inverse, g_out)] if res.inverse is not None else None
+    if init:
+        aff = compose_affines(init)
+        aio.write_transform(prefix + "0GenericAffine.mat", aff)
+        k = 1
+        fwd_chain = fwd_chain + [aff]
+        if inv_chain is not None:
+            inv_chain = [AffineTransform(aff.matrix, aff.translation, aff.center, invert=True)] + inv_chain
+    aio.write_nifti(prefix + f"{k}Warp.nii.gz", res.field, g_out, vector=True)
+    if res.inverse is not None:
+        aio.write_nifti(prefix + f"{k}InverseWarp.nii.gz", res.inverse, g_out, vector=True)
+    if len(outs) > 1 and outs[1]:
+        aio.write_nifti(outs[1], apply_transforms(gf, np.asarray(moving, np.float32), gm, fwd_chain, "Linear"), gf)
+    if len(outs) > 2 and outs[2]:
+        if inv_chain is None:
+            raise SystemExit("GaussianDisplacementField keeps no inverse: no inverse-warped output")
+        aio.write_nifti(outs[2], apply_transforms(gm, np.asarray(fixed, np.float32), gf, inv_chain, "Linear"), gm)
     return 0
 
 

@@ -152,6 +152,25 @@ def _device_image(img):
     return img if isinstance(img, L.DeviceArray) else L.DeviceArray.from_host(np.asarray(img, np.float32))
 
 
+def level_geometry(stage: FieldStage, fgeom: Geometry, level: int) -> Geometry:
+    """A level's grid (the shrink rule of ants_syn_level_geometry, the
+    helper's for every stage type): the grid of that level's fields."""
+    D = fgeom.dim
+    size = tuple(reversed(level_shape(stage, fgeom, level)))
+    sp = L.SynParamsC()
+    sp.dim = D
+    sp.n_levels = len(stage.shrink)
+    for k, f in enumerate(stage.shrink):
+        sp.shrink[k] = int(f)
+        sp.iterations[k] = 1
+    sp.convergence_threshold, sp.convergence_window = 1e-6, 10
+    sp.learning_rate, sp.update_variance, sp.cc_radius = 0.1, 3.0, int(stage.radius)
+    out = L.ImageDesc()
+    L.check(L.lib().ants_syn_level_geometry(C.byref(sp), C.byref(fgeom.desc()), level, C.byref(out)))
+    return Geometry(size, tuple(out.spacing[k] for k in range(D)), tuple(out.origin[k] for k in range(D)),
+                    np.array([out.direction[k] for k in range(D * D)]).reshape(D, D))
+
+
 def level_shape(stage: FieldStage, fgeom: Geometry, level: int):
     """numpy shape (z, y, x) of a level's grid (ants_syn_level_geometry: the
     helper's shrink rule, the same for every stage type)."""

@@ -16,7 +16,8 @@ replayed from iteration graphs, measured after the timed region;
 `host_buffer_drop_in` the same registration through the host-buffer entry
 (PCIe transfers included); `bspline_syn` the same pair and schedule with the
 BSplineSyN field model (BSplineSyN[0.1,26,0,3]); `linear_stages` the
-Rigid and Affine stages antsRegistrationSyN.sh runs before SyN.  The other half of the metric comes from two
+Rigid and Affine stages antsRegistrationSyN.sh runs before SyN; `field_stages`
+GaussianDisplacementField and Exponential with the CC metric.  The other half of the metric comes from two
 bounded legs on the same GPU: `batch` (64 mixed-size pairs, batched versus
 one at a time) and `serve` (a Poisson request stream at a fixed QPS: p50/p99
 latency, pairs/s and capacity), summarised in `batch_summary` and
@@ -375,6 +376,32 @@ def linear_leg(pair, d_fixed, d_moving) -> dict:
             **legs, "total_ms": sum(v["ms"] for v in legs.values())}
 
 
+def field_leg(pair, d_fixed, d_moving) -> dict:
+    """The dense field-transform stages on the same pair and schedule with the
+    CC metric r=4: GaussianDisplacementField[0.25,3,0] (conjugate-gradient
+    line search) and Exponential[40,3,0.5,0] (gradient descent on the
+    velocity, exp by scaling and squaring; its step multiplies the raw CC
+    derivative, hence the large value); one timed run each after a warm-up."""
+    from ants_amd import _lib as L
+    from ants_amd import fieldmodels as FM
+    from ants_amd.apply_transforms import Geometry
+    geo = Geometry(pair.size, pair.spacing, pair.origin, np.eye(3))
+    out = {}
+    for tr in ("GaussianDisplacementField[0.25,3,0]", "Exponential[40,3,0.5,0]"):
+        st = FM.FieldStage.from_args(tr, "CC[fixed,moving,1,4]", SCHEDULE["convergence"], SCHEDULE["shrink"],
+                                     SCHEDULE["smoothing"])
+        FM.register_field(st, d_fixed, geo, d_moving, geo)
+        L.check(L.lib().ants_device_synchronize())
+        t0 = time.perf_counter()
+        res = FM.register_field(st, d_fixed, geo, d_moving, geo)
+        sec = time.perf_counter() - t0
+        work = sum(int(np.prod(FM.level_shape(st, geo, lv))) * it for lv, it in enumerate(res.iterations))
+        out[tr] = {"ms": sec * 1e3, "iterations_run": res.iterations, "voxel_iters_per_s": work / sec,
+                   "energy_first_last": [res.records[0][2], res.records[-1][2]] if res.records else None}
+        log(f"field stage {tr}: {sec * 1e3:.1f} ms, iterations {res.iterations}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -546,6 +573,7 @@ def main():
         os.environ["ANTS_SYN_GRAPHS"] = saved_env
     bsp = bspline_leg(pair, gf, d_fixed, d_moving, d_fwd, d_inv, args.steps)
     lin = linear_leg(pair, d_fixed, d_moving)
+    fld = field_leg(pair, d_fixed, d_moving)
     avg_ms = kacc["ms"] / max(kacc["launches"], 1)
     bytes_per_launch = kacc["bytes"] / max(kacc["launches"], 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
@@ -600,6 +628,7 @@ def main():
         "inverse_consistency": ic,
         "bspline_syn": bsp,
         "linear_stages": lin,
+        "field_stages": fld,
     }
     if legs:
         sv = legs["serve"]["continuous_batching"]

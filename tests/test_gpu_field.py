@@ -153,3 +153,44 @@ def test_stage_recovers_translation(model):
     mean = res.field[core].reshape(-1, 3).mean(axis=0)
     print(f"\nmodel {model}: mean core displacement {mean}, want {shift[::-1]}")
     assert np.abs(mean - shift[::-1]).max() < 0.35
+
+
+@pytest.mark.parametrize("model", ["GaussianDisplacementField[0.4,2,0.5]", "Exponential[15,2,0.5,0]"])
+def test_cli_field_stage(tmp_path, model):
+    """cli registration with a rigid stage and then a field stage writes
+    0GenericAffine.mat and 1Warp.nii.gz (Exponential also 1InverseWarp),
+    equal to the library run on the same arrays; the warped image is the
+    moving image through [field, affine]."""
+    from ants_amd import cli
+    from ants_amd import io as aio
+    from ants_amd import linear as LN
+    from ants_amd.apply_transforms import DisplacementFieldTransform, apply_transforms
+    g, F, M = images(3, 5)
+    f, m = str(tmp_path / "fixed.nii.gz"), str(tmp_path / "moving.nii.gz")
+    aio.write_nifti(f, F, g)
+    aio.write_nifti(m, M, g)
+    F, gf = aio.read_nifti(f)
+    M, gm = aio.read_nifti(m)
+    prefix = str(tmp_path / "out_")
+    rigid = ["Rigid[0.2]", f"GC[{f},{m},1,1,Regular,0.5]", "[6,1e-6,10]", "1", "0vox"]
+    fld = [model, f"CC[{f},{m},1,2]", "[4x3,1e-6,10]", "2x1", "1x0vox"]
+    args = ["-d", "3", "--random-seed", "2"]
+    for t, mm, cc, ff, ss in (rigid, fld):
+        args += ["-t", t, "-m", mm, "-c", cc, "-f", ff, "-s", ss]
+    args += ["-o", f"[{prefix},{prefix}Warped.nii.gz]"]
+    assert cli.main(["registration"] + args) == 0
+    st = LN.LinearStage.from_args(*rigid, seed=2)
+    r = LN.register_linear(st, F, gf, M, gm)
+    fs = FM.FieldStage.from_args(*fld)
+    res = FM.register_field(fs, F, gf, M, gm, [r.transform])
+    fwd, _ = aio.read_nifti(prefix + "1Warp.nii.gz")
+    np.testing.assert_array_equal(fwd, res.field)
+    if fs.model == FM.EXPONENTIAL:
+        inv, _ = aio.read_nifti(prefix + "1InverseWarp.nii.gz")
+        np.testing.assert_array_equal(inv, res.inverse)
+    else:
+        import os
+        assert not os.path.exists(prefix + "1InverseWarp.nii.gz")
+    warped, _ = aio.read_nifti(prefix + "Warped.nii.gz")
+    want = apply_transforms(gf, M, gm, [DisplacementFieldTransform(res.field, gf), r.transform], "Linear")
+    np.testing.assert_array_equal(warped, want)

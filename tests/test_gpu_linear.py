@@ -141,11 +141,14 @@ def test_stage_matches_oracle(metric):
     """Two levels (shrink 2 with smoothing, then full resolution) of a Rigid
     stage under a moving initial affine, images windowed and smoothed (in
     float on the GPU, in double in the oracle): the same iteration counts,
-    every record's value within 1e-4 relative (the SyN stage's energy bar),
-    parameters within 1e-3.  The float/double image difference (~1e-7) can
-    flip a golden-section comparison between two nearly equal trial values,
-    which moves one step's learning rate and the records after it by ~1e-5;
-    with identical inputs the stage agrees to rounding (next test)."""
+    and every record's value within 1e-6 relative up to the first iteration
+    whose line search picks a learning rate more than 1e-3 apart -- the
+    float/double image difference (~1e-7) can flip a golden-section
+    comparison between two nearly equal trial values, after which the two
+    paths differ by the step taken -- which comes no earlier than half-way;
+    after it every value within 1e-3 relative and the parameters within
+    0.01.  With identical inputs the stage agrees to rounding throughout
+    (next test)."""
     D = 3
     gF, F, gM, M, mi, fi, c, _ = setup(D, "Rigid", metric, "Regular", 0.25, 1, 0.0)
     st = LN.LinearStage("Rigid", 0.25, metric, bins=24, sampling="Regular", percentage=0.25, iterations=(12, 8),
@@ -157,10 +160,15 @@ def test_stage_matches_oracle(metric):
     print(f"\n{metric}: gpu iterations {res.iterations} params {res.params}\n oracle {ref.iterations} {ref.params}")
     assert res.iterations == ref.iterations
     assert len(res.records) == len(ref.records)
-    for a, b in zip(res.records, ref.records):
+    split = next((k for k, (a, b) in enumerate(zip(res.records, ref.records))
+                  if abs(a[4] - b[4]) > 1e-3 * max(abs(a[4]), abs(b[4]))), len(ref.records))
+    print(f"  line searches agree on {split} of {len(ref.records)} iterations")
+    assert split >= len(ref.records) // 2
+    for k, (a, b) in enumerate(zip(res.records, ref.records)):
         assert a[:2] == b[:2]
-        assert abs(a[2] - b[2]) <= 1e-4 * max(1e-3, abs(b[2])), (a, b)
-    np.testing.assert_allclose(res.params, ref.params, rtol=0, atol=1e-3)
+        tol = 1e-6 if k <= split else 1e-3
+        assert abs(a[2] - b[2]) <= tol * max(1e-3, abs(b[2])), (k, a, b)
+    np.testing.assert_allclose(res.params, ref.params, rtol=0, atol=1e-5 if split == len(ref.records) else 1e-2)
 
 
 @pytest.mark.parametrize("D,kind,metric", [(3, "Affine", "GC"), (2, "Rigid", "Mattes"), (3, "Translation", "MeanSquares")])
