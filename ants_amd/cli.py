@@ -22,9 +22,10 @@ antsRegistrationSyN.sh writes them (Scripts/antsRegistrationSyN.sh:487-497):
 ``--random-seed`` seeds their metric sampling).  Their transforms, composed
 with the -r initial transforms, become ``out_0GenericAffine.mat``; with no
 SyN stage that file is the whole output.  The last stage may instead be a
-dense field-transform stage, ``-t GaussianDisplacementField[0.25,3,0]`` or
-``-t Exponential[0.25,3,0.5,0]`` with ``-m CC[...]``: it writes
-``out_{k}Warp.nii.gz`` (Exponential also ``out_{k}InverseWarp.nii.gz``).
+dense field-transform stage, ``-t GaussianDisplacementField[0.25,3,0]``,
+``-t Exponential[0.25,3,0.5,0]``, ``-t BSplineDisplacementField[0.25,26,0,3]``
+or ``-t BSplineExponential[0.25,26,0,0,3]`` with ``-m CC[...]``: it writes
+``out_{k}Warp.nii.gz`` (the exponential forms also ``out_{k}InverseWarp.nii.gz``).
 
 ``python -m ants_amd.cli apply -d 3 -i moving.nii.gz -r fixed.nii.gz
   -o out.nii.gz -t out_0Warp.nii.gz -t [affine.mat,1] -n Linear
@@ -301,13 +302,13 @@ def cmd_registration(argv: Sequence[str]) -> int:
 
 
 def _field_stage_outputs(a, fixed, gf, moving, gm, init, finit, outs, prefix) -> int:
-    """A GaussianDisplacementField or Exponential last stage: its field (and
-    for Exponential the inverse) written as <prefix>{k}Warp.nii.gz /
+    """A dense field-transform last stage: its field (and for the exponential
+    forms the inverse) written as <prefix>{k}Warp.nii.gz /
     <prefix>{k}InverseWarp.nii.gz after 0GenericAffine.mat when initial
     transforms precede it, as antsRegistration names a displacement-field
-    transform; GaussianDisplacementField keeps no inverse field."""
+    transform; the displacement-field forms keep no inverse field."""
     st = FieldStage.from_args(a.transform[-1], a.metric[-1], a.convergence[-1], a.shrink_factors[-1],
-                              a.smoothing_sigmas[-1])
+                              a.smoothing_sigmas[-1], gf.size, gf.spacing)
     if any(isinstance(t, DisplacementFieldTransform) for t in init + finit):
         raise SystemExit("field-transform stages take affine initial transforms only")
     if a.save_state or a.restore_state or a.print_similarity_measure_interval or a.write_composite_transform:
@@ -336,7 +337,7 @@ def _field_stage_outputs(a, fixed, gf, moving, gm, init, finit, outs, prefix) ->
         aio.write_nifti(outs[1], apply_transforms(gf, np.asarray(moving, np.float32), gm, fwd_chain, "Linear"), gf)
     if len(outs) > 2 and outs[2]:
         if inv_chain is None:
-            raise SystemExit("GaussianDisplacementField keeps no inverse: no inverse-warped output")
+            raise SystemExit("a displacement-field stage keeps no inverse: no inverse-warped output")
         aio.write_nifti(outs[2], apply_transforms(gm, np.asarray(fixed, np.float32), gf, inv_chain, "Linear"), gm)
     return 0
 

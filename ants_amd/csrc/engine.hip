@@ -3188,8 +3188,17 @@ struct GoldenSection {
 int validate_field_params(const ants_field_params* p) {
   if (!p) return fail(ANTS_ERR_INVALID, "null parameters");
   if (p->dim != 2 && p->dim != 3) return fail(ANTS_ERR_INVALID, "dim must be 2 or 3");
-  if (p->model != ANTS_FIELD_GAUSSIAN_DISPLACEMENT && p->model != ANTS_FIELD_EXPONENTIAL)
-    return fail(ANTS_ERR_UNSUPPORTED, "field model must be GaussianDisplacementField or Exponential");
+  if (p->model < ANTS_FIELD_GAUSSIAN_DISPLACEMENT || p->model > ANTS_FIELD_BSPLINE_EXPONENTIAL)
+    return fail(ANTS_ERR_UNSUPPORTED,
+                "field model must be GaussianDisplacementField, Exponential, BSplineDisplacementField or "
+                "BSplineExponential");
+  if (p->model >= ANTS_FIELD_BSPLINE_DISPLACEMENT) {
+    if (p->spline_order < 1 || p->spline_order > 3) return fail(ANTS_ERR_UNSUPPORTED, "B-spline order must be 1..3");
+    for (int d = 0; d < p->dim; ++d)
+      if (p->update_mesh[d] < 0 || p->total_mesh[d] < 0 || p->update_mesh[d] > (1 << 12) ||
+          p->total_mesh[d] > (1 << 12))
+        return fail(ANTS_ERR_INVALID, "B-spline meshes must be in 0..4096 per axis");
+  }
   if (p->n_levels < 1 || p->n_levels > ANTS_MAX_LEVELS) return fail(ANTS_ERR_INVALID, "n_levels must be 1..8");
   for (int l = 0; l < p->n_levels; ++l) {
     if (p->shrink[l] < 1) return fail(ANTS_ERR_INVALID, "shrink factors must be >= 1");
@@ -3219,6 +3228,64 @@ struct FieldEngine {
   DevBuf ccpart, slot, dpart, dout, mbits;
   int64_t evaluations = 0;
   double minsp = 1.0;
+  // B-spline forms: the level's fit tables of the update and total fields
+  // (mesh << (level + 1); a zero mesh on any axis: no fit)
+  bool fit_u = false, fit_t = false;
+  BspLayout LU{}, LT{};
+  BspDev TU{}, TT{};
+  DevBuf tabU, tabT, lat, bpart;
+
+  bool bspline() const { return P.model >= ANTS_FIELD_BSPLINE_DISPLACEMENT; }
+  bool expo() const { return P.model == ANTS_FIELD_EXPONENTIAL || P.model == ANTS_FIELD_BSPLINE_EXPONENTIAL; }
+  bool smooths_update() const { return bspline() ? fit_u : P.update_variance > 0.0; }
+  bool smooths_total() const { return bspline() ? fit_t : P.total_variance > 0.0; }
+
+  int level_tables(int l) {
+    fit_u = fit_t = false;
+    if (!bspline()) return ANTS_OK;
+    int mu[3] = {1, 1, 1}, mt[3] = {1, 1, 1};
+    bool zu = false, zt = false;
+    for (int d = 0; d < D; ++d) {
+      mu[d] = P.update_mesh[d] << (l + 1);
+      mt[d] = P.total_mesh[d] << (l + 1);
+      zu = zu || mu[d] == 0;
+      zt = zt || mt[d] == 0;
+    }
+    int64_t lat_floats = 1;
+    if (!zu) {
+      std::vector<unsigned char> blob;
+      ANTS_TRY(bsp_tables(gl, mu, P.spline_order, blob, &LU));
+      ANTS_TRY(tabU.ensure(blob.size()));
+      HIP_TRY(hipMemcpyAsync(tabU.p, blob.data(), blob.size(), hipMemcpyHostToDevice, s));
+      HIP_TRY(hipStreamSynchronize(s));  // blob is freed on return
+      TU = bsp_dev(LU, tabU.p);
+      lat_floats = std::max(lat_floats, LU.field_floats());
+      fit_u = true;
+    }
+    if (!zt) {
+      std::vector<unsigned char> blob;
+      ANTS_TRY(bsp_tables(gl, mt, P.spline_order, blob, &LT));
+      ANTS_TRY(tabT.ensure(blob.size()));
+      HIP_TRY(hipMemcpyAsync(tabT.p, blob.data(), blob.size(), hipMemcpyHostToDevice, s));
+      HIP_TRY(hipStreamSynchronize(s));
+      TT = bsp_dev(LT, tabT.p);
+      lat_floats = std::max(lat_floats, LT.field_floats());
+      fit_t = true;
+    }
+    ANTS_TRY(lat.ensure(static_cast<size_t>(lat_floats) * sizeof(float)));
+    ANTS_TRY(bpart.ensure(static_cast<size_t>(bsp_partials_needed()) * sizeof(float)));
+    return ANTS_OK;
+  }
+
+  // the update (or total / velocity) field's smoothing: Gaussian with the
+  // stage's variance, or the B-spline fit of the level's mesh; in is scratch
+  int smooth_field_of(bool total, float* in, float* out) {
+    if (!bspline()) return smooth_into(in, out, total ? P.total_variance : P.update_variance);
+    const float* i1[1] = {in};
+    float* o1[1] = {out};
+    float* b1[1] = {lat.f()};
+    return bsp_fields_on(s, gl, total ? TT : TU, total ? LT : LU, 1, i1, o1, b1, nullptr, bpart.f());
+  }
 
   int buf(DevBuf& b, int64_t floats) { return b.ensure(static_cast<size_t>(std::max<int64_t>(floats, 1)) * 4); }
 
@@ -3306,7 +3373,8 @@ struct FieldEngine {
     }
     launch_field_sample(lgeo, fgeo, Fs.f(), nullptr, fmap, Fw.f(), nullptr, s);
     HIP_TRY(hipGetLastError());
-    if (P.model == ANTS_FIELD_EXPONENTIAL) {
+    ANTS_TRY(level_tables(l));
+    if (expo()) {
       int n = 0;
       ANTS_TRY(exponentiate(state.f(), u.f(), 1, P.integration_steps, &n));
     }
@@ -3389,13 +3457,13 @@ struct FieldEngine {
     const int64_t nf = nvox() * D;
     launch_field_axpy(nf, nullptr, static_cast<float>(step), conj.f(), upd.f(), s);
     const float* src = upd.f();
-    if (P.update_variance > 0.0) {
-      ANTS_TRY(smooth_into(upd.f(), sm.f(), P.update_variance));
+    if (smooths_update()) {
+      ANTS_TRY(smooth_field_of(false, upd.f(), sm.f()));
       src = sm.f();
     }
     launch_field_axpy(nf, state.f(), 1.f, src, dst, s);
-    if (P.total_variance > 0.0) {
-      ANTS_TRY(smooth_into(dst, sm.f(), P.total_variance));
+    if (smooths_total()) {
+      ANTS_TRY(smooth_field_of(true, dst, sm.f()));
       HIP_TRY(hipMemcpyAsync(dst, sm.p, nf * sizeof(float), hipMemcpyDeviceToDevice, s));
     }
     HIP_TRY(hipGetLastError());
@@ -3413,7 +3481,7 @@ struct FieldEngine {
       int it = 0;
       while (it < P.iterations[l]) {
         double val = 0.0;
-        const bool gdf = P.model == ANTS_FIELD_GAUSSIAN_DISPLACEMENT;
+        const bool gdf = !expo();
         ANTS_TRY(metric(gdf ? state.f() : u.f(), true, &val));
         energies.push_back(val);
         double conv = 0.0;
@@ -3451,13 +3519,13 @@ struct FieldEngine {
           lr = P.learning_rate;
           launch_field_axpy(nf, nullptr, static_cast<float>(lr), g.f(), upd.f(), s);
           const float* src = upd.f();
-          if (P.update_variance > 0.0) {
-            ANTS_TRY(smooth_into(upd.f(), sm.f(), P.update_variance));
+          if (smooths_update()) {
+            ANTS_TRY(smooth_field_of(false, upd.f(), sm.f()));
             src = sm.f();
           }
           launch_field_axpy(nf, state.f(), 1.f, src, trial.f(), s);
-          if (P.total_variance > 0.0) {
-            ANTS_TRY(smooth_into(trial.f(), state.f(), P.total_variance));
+          if (smooths_total()) {
+            ANTS_TRY(smooth_field_of(true, trial.f(), state.f()));
           } else {
             std::swap(state.p, trial.p);
             std::swap(state.bytes, trial.bytes);
@@ -3494,9 +3562,9 @@ int ants_field_register(const ants_field_params* p, const ants_image_desc* fixed
   if (trace_count) *trace_count = static_cast<int64_t>(tr.size());
   const size_t bytes = static_cast<size_t>(e.nvox()) * e.D * sizeof(float);
   hipStream_t s = e.s;
-  if (e.P.model == ANTS_FIELD_GAUSSIAN_DISPLACEMENT) {
+  if (!e.expo()) {
     if (d_out_inverse || d_out_velocity)
-      return fail(ANTS_ERR_INVALID, "GaussianDisplacementField has no inverse or velocity output");
+      return fail(ANTS_ERR_INVALID, "a displacement-field stage has no inverse or velocity output");
     if (d_out_field) HIP_TRY(hipMemcpyAsync(d_out_field, e.state.p, bytes, hipMemcpyDeviceToDevice, s));
   } else {
     int n = 0;

@@ -1,9 +1,11 @@
 """Host-side mirror of antsRegistration's dense field-transform stages over the
-gfx950 library: ``-t GaussianDisplacementField[step,updateVariance,totalVariance]``
-and ``-t Exponential[step,updateVariance,velocityVariance,integrationSteps]``
+gfx950 library: ``-t GaussianDisplacementField[step,updateVariance,totalVariance]``,
+``-t Exponential[step,updateVariance,velocityVariance,integrationSteps]``,
+``-t BSplineDisplacementField[step,updateMesh,totalMesh,order]`` and
+``-t BSplineExponential[step,updateMesh,velocityMesh,integrationSteps,order]``
 with ``-m CC[fixed,moving,1,radius]``
-(Examples/antsRegistrationTemplateHeader.h:692-699, 871-886;
-Examples/itkantsRegistrationHelper.hxx:1733-1847, 3112-3277).
+(Examples/antsRegistrationTemplateHeader.h:692-743, 871-935;
+Examples/itkantsRegistrationHelper.hxx:1733-1982, 3112-3461).
 
 ``FieldStage.from_args`` reads the stage's options, ``register_field`` runs it
 through ``ants_field_register`` (include/ants_syn.h) and returns the
@@ -21,10 +23,11 @@ import numpy as np
 from . import _lib as L
 from .apply_transforms import Geometry, transform_array
 from .linear import LinearRecordC
-from .registration import CCMetric, Convergence, parse_option_function, parse_sigmas, parse_vector
+from .registration import CCMetric, Convergence, SyNTransform, parse_option_function, parse_sigmas, parse_vector
 
-MODELS = {"gaussiandisplacementfield": 0, "gdf": 0, "exponential": 1, "exp": 1}
-GAUSSIAN_DISPLACEMENT, EXPONENTIAL = 0, 1
+MODELS = {"gaussiandisplacementfield": 0, "gdf": 0, "exponential": 1, "exp": 1, "bsplinedisplacementfield": 2,
+          "bsplineexponential": 3}
+GAUSSIAN_DISPLACEMENT, EXPONENTIAL, BSPLINE_DISPLACEMENT, BSPLINE_EXPONENTIAL = 0, 1, 2, 3
 
 
 class FieldParamsC(C.Structure):
@@ -45,6 +48,9 @@ class FieldParamsC(C.Structure):
         ("cc_radius", C.c_int),
         ("preprocess", C.c_int),
         ("force_iterations", C.c_int),
+        ("update_mesh", C.c_int * 3),
+        ("total_mesh", C.c_int * 3),
+        ("spline_order", C.c_int),
     ]
 
 
@@ -84,21 +90,48 @@ class FieldStage:
     force_iterations: bool = False
     fixed: str = ""
     moving: str = ""
+    update_mesh: Sequence[int] = (0, 0, 0)   # B-spline forms: spans per axis at the base level
+    total_mesh: Sequence[int] = (0, 0, 0)    # B-spline forms: total / velocity field (0: no fit)
+    spline_order: int = 3
+
+    @property
+    def expo(self) -> bool:
+        return self.model in (EXPONENTIAL, BSPLINE_EXPONENTIAL)
 
     @staticmethod
     def is_field(transform_arg: str) -> bool:
         return parse_option_function(transform_arg)[0].strip().lower() in MODELS
 
     @classmethod
-    def from_args(cls, transform: str, metric: str, convergence: str, shrink: str, sigmas: str) -> "FieldStage":
+    def from_args(cls, transform: str, metric: str, convergence: str, shrink: str, sigmas: str,
+                  fixed_size=None, fixed_spacing=None) -> "FieldStage":
+        """fixed_size / fixed_spacing (x, y[, z]): the fixed image's geometry,
+        needed only for a B-spline mesh given as a knot spacing."""
         tname, tp = parse_option_function(transform)
         key = tname.strip().lower()
         if key not in MODELS:
-            raise ValueError(f"field stages are GaussianDisplacementField or Exponential, got {tname!r}")
+            raise ValueError("field stages are GaussianDisplacementField, Exponential, BSplineDisplacementField "
+                             f"or BSplineExponential, got {tname!r}")
         model = MODELS[key]
-        if len(tp) < 3:
-            raise ValueError(f"{tname} needs [step, updateVariance, {'totalVariance' if model == 0 else 'velocityVariance'}]")
-        steps = int(tp[3]) if model == EXPONENTIAL and len(tp) > 3 else 0
+        um, tm, order = (0, 0, 0), (0, 0, 0), 3
+        if model in (BSPLINE_DISPLACEMENT, BSPLINE_EXPONENTIAL):
+            # Examples/antsRegistrationTemplateHeader.h:701-743 (order is
+            # parameter 3) and 888-935 (integration steps 3, order 4)
+            if len(tp) < 2:
+                raise ValueError(f"{tname} needs [step, updateMesh, ...]")
+            order = int(tp[3 if model == BSPLINE_DISPLACEMENT else 4]) if len(tp) > (
+                3 if model == BSPLINE_DISPLACEMENT else 4) else 3
+            fake = "BSplineSyN[" + ",".join([tp[0], tp[1], tp[2] if len(tp) > 2 else "0", str(order)]) + "]"
+            t = SyNTransform.parse(fake, fixed_size, fixed_spacing)
+            um, tm = tuple(t.update_mesh), tuple(t.total_mesh)
+            steps = int(tp[3]) if model == BSPLINE_EXPONENTIAL and len(tp) > 3 else 0
+            uvar = tvar = 0.0
+        else:
+            if len(tp) < 3:
+                raise ValueError(f"{tname} needs [step, updateVariance, "
+                                 f"{'totalVariance' if model == 0 else 'velocityVariance'}]")
+            steps = int(tp[3]) if model == EXPONENTIAL and len(tp) > 3 else 0
+            uvar, tvar = float(tp[1]), float(tp[2])
         m = CCMetric.parse(metric)
         if m.weight != 1.0:
             raise ValueError("a single CC metric with weight 1 is supported")
@@ -107,8 +140,8 @@ class FieldStage:
         s, in_mm = parse_sigmas(sigmas, len(conv.iterations))
         if not (len(conv.iterations) == len(f) == len(s)):
             raise ValueError("-c, -f and -s must give the same number of levels")
-        return cls(model, float(tp[0]), float(tp[1]), float(tp[2]), steps, m.radius, conv.iterations,
-                   conv.threshold, conv.window, f, s, in_mm, True, False, m.fixed, m.moving)
+        return cls(model, float(tp[0]), uvar, tvar, steps, m.radius, conv.iterations, conv.threshold, conv.window, f,
+                   s, in_mm, True, False, m.fixed, m.moving, um, tm, order)
 
     def params(self, dim: int) -> FieldParamsC:
         p = FieldParamsC()
@@ -132,6 +165,10 @@ class FieldStage:
         p.cc_radius = int(self.radius)
         p.preprocess = int(self.preprocess)
         p.force_iterations = int(self.force_iterations)
+        for d in range(3):
+            p.update_mesh[d] = int(self.update_mesh[d]) if d < len(self.update_mesh) else 0
+            p.total_mesh[d] = int(self.total_mesh[d]) if d < len(self.total_mesh) else 0
+        p.spline_order = int(self.spline_order)
         return p
 
 
@@ -199,8 +236,8 @@ def register_field(stage: FieldStage, fixed, fgeom: Geometry, moving, mgeom: Geo
     xf, keep_f = transform_array(list(fixed_initial), D)
     shape = level_shape(stage, fgeom, len(stage.shrink) - 1) + (D,)
     out = L.DeviceArray(shape)
-    inv = L.DeviceArray(shape) if stage.model == EXPONENTIAL else None
-    vel = L.DeviceArray(shape) if stage.model == EXPONENTIAL else None
+    inv = L.DeviceArray(shape) if stage.expo else None
+    vel = L.DeviceArray(shape) if stage.expo else None
     cap = int(sum(int(v) for v in stage.iterations)) + 1
     tr = (LinearRecordC * cap)()
     n = C.c_int64()

@@ -17,7 +17,8 @@ replayed from iteration graphs, measured after the timed region;
 (PCIe transfers included); `bspline_syn` the same pair and schedule with the
 BSplineSyN field model (BSplineSyN[0.1,26,0,3]); `linear_stages` the
 Rigid and Affine stages antsRegistrationSyN.sh runs before SyN; `field_stages`
-GaussianDisplacementField and Exponential with the CC metric.  The other half of the metric comes from two
+GaussianDisplacementField, Exponential and their B-spline forms with the CC
+metric.  The other half of the metric comes from two
 bounded legs on the same GPU: `batch` (64 mixed-size pairs, batched versus
 one at a time) and `serve` (a Poisson request stream at a fixed QPS: p50/p99
 latency, pairs/s and capacity), summarised in `batch_summary` and
@@ -379,17 +380,19 @@ def linear_leg(pair, d_fixed, d_moving) -> dict:
 def field_leg(pair, d_fixed, d_moving) -> dict:
     """The dense field-transform stages on the same pair and schedule with the
     CC metric r=4: GaussianDisplacementField[0.25,3,0] (conjugate-gradient
-    line search) and Exponential[40,3,0.5,0] (gradient descent on the
-    velocity, exp by scaling and squaring; its step multiplies the raw CC
-    derivative, hence the large value); one timed run each after a warm-up."""
+    line search), Exponential[40,3,0.5,0] (gradient descent on the velocity,
+    exp by scaling and squaring; its step multiplies the raw CC derivative,
+    hence the large value) and their B-spline forms with a 26 mm knot
+    spacing; one timed run each after a warm-up."""
     from ants_amd import _lib as L
     from ants_amd import fieldmodels as FM
     from ants_amd.apply_transforms import Geometry
     geo = Geometry(pair.size, pair.spacing, pair.origin, np.eye(3))
     out = {}
-    for tr in ("GaussianDisplacementField[0.25,3,0]", "Exponential[40,3,0.5,0]"):
+    for tr in ("GaussianDisplacementField[0.25,3,0]", "Exponential[40,3,0.5,0]",
+               "BSplineDisplacementField[0.25,26,0,3]", "BSplineExponential[40,26,0,0,3]"):
         st = FM.FieldStage.from_args(tr, "CC[fixed,moving,1,4]", SCHEDULE["convergence"], SCHEDULE["shrink"],
-                                     SCHEDULE["smoothing"])
+                                     SCHEDULE["smoothing"], pair.size, pair.spacing)
         FM.register_field(st, d_fixed, geo, d_moving, geo)
         L.check(L.lib().ants_device_synchronize())
         t0 = time.perf_counter()

@@ -629,6 +629,15 @@ int ants_linear_release_points(void);
  * Variances are in voxel^2 (variance space, as the CLI gives them). */
 #define ANTS_FIELD_GAUSSIAN_DISPLACEMENT 0
 #define ANTS_FIELD_EXPONENTIAL 1
+/* The B-spline forms (hxx:1848-1982, 3278-3461): the update and total /
+ * velocity fields fitted by B-splines instead of Gaussian smoothing
+ * (BSplineSmoothingOnUpdateDisplacementFieldTransform, BSplineExponential-
+ * DiffeomorphicTransform), meshes doubled per level as mesh << (level + 1);
+ * a zero mesh means no fit of that field.
+ * -t BSplineDisplacementField[step, updateMesh, totalMesh=0, order=3]
+ * -t BSplineExponential[step, updateMesh, velocityMesh=0, integrationSteps, order=3] */
+#define ANTS_FIELD_BSPLINE_DISPLACEMENT 2
+#define ANTS_FIELD_BSPLINE_EXPONENTIAL 3
 typedef struct ants_field_params {
   int dim;
   int model;                            /* ANTS_FIELD_* */
@@ -646,6 +655,9 @@ typedef struct ants_field_params {
   int cc_radius;                        /* CC[f,m,1,radius] */
   int preprocess;                       /* window both images to [0,1] (PreprocessImage) */
   int force_iterations;                 /* 1: never stop early (parity harness) */
+  int update_mesh[3];                   /* B-spline forms: spans per axis at the base level */
+  int total_mesh[3];                    /* B-spline forms: total / velocity field (0: no fit) */
+  int spline_order;                     /* B-spline forms: 1..3 */
 } ants_field_params;
 
 /* Replaces registrationMethod->Update() of such a stage and the transform it

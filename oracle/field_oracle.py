@@ -8,6 +8,11 @@ primitives of oracle.py (smoothing, composition, resampling, CC sums).
   -t Exponential[step, updateVariance, velocityVariance, integrationSteps=0]
      (Examples/itkantsRegistrationHelper.hxx:3112-3277,
       Examples/antsRegistrationTemplateHeader.h:692-699, 871-886)
+  -t BSplineDisplacementField[step, updateMesh, totalMesh=0, order=3] and
+  -t BSplineExponential[step, updateMesh, velocityMesh=0, steps, order=3]
+     (hxx:1848-1982, 3278-3461; h:701-743, 888-935): the same stages with
+     the update and total / velocity fields fitted by B-splines (the
+     BSplineSyN stage's fit) on meshes doubled per level, mesh << (level + 1)
 
 Both run ImageRegistrationMethodv4: per level the virtual domain is the
 shrunk fixed grid, the images are smoothed at full resolution, and the
@@ -65,7 +70,7 @@ import numpy as np
 import oracle as O
 import linear_oracle as LO
 
-GAUSSIAN_DISPLACEMENT, EXPONENTIAL = 0, 1
+GAUSSIAN_DISPLACEMENT, EXPONENTIAL, BSPLINE_DISPLACEMENT, BSPLINE_EXPONENTIAL = 0, 1, 2, 3
 DBL_EPS = np.finfo(np.float64).eps
 EXP_DEFAULT_STEPS = 10
 
@@ -86,6 +91,9 @@ class FieldParams:
     conv_window: int = 10
     preprocess: bool = True
     force_iterations: bool = False
+    update_mesh: Sequence[int] = (0, 0, 0)   # B-spline forms: spans at the base level
+    total_mesh: Sequence[int] = (0, 0, 0)    # B-spline forms: total / velocity field (0: no fit)
+    spline_order: int = 3
 
 
 @dataclass
@@ -132,6 +140,23 @@ def exponentiate(g: O.Grid, v, steps: int = 0, inverse: bool = False):
 
 def smooth(g: O.Grid, fld, variance: float):
     return O.smooth_field(g, fld, variance) if variance > 0.0 else np.asarray(fld, dtype=np.float64).copy()
+
+
+def _smoother(P: FieldParams, gl: O.Grid, lev: int, total: bool):
+    """The update or total / velocity field's smoothing on level lev: Gaussian
+    with the stage's variance, or for the B-spline forms the B-spline fit of
+    mesh << (lev + 1) (BSplineSmoothingOnUpdateDisplacementFieldTransform /
+    BSplineExponentialDiffeomorphicTransform with the helper's per-level
+    meshes, Examples/itkantsRegistrationHelper.hxx:1934-1941, 3371-3377),
+    none for a zero mesh (control points not above the order)."""
+    if P.model in (GAUSSIAN_DISPLACEMENT, EXPONENTIAL):
+        var = P.total_variance if total else P.update_variance
+        return lambda f: smooth(gl, f, var)
+    base = P.total_mesh if total else P.update_mesh
+    mesh = [int(m) << (lev + 1) for m in base[: gl.dim]]
+    if any(m == 0 for m in mesh):
+        return lambda f: np.asarray(f, dtype=np.float64).copy()
+    return lambda f: O.bspline_fit_field(gl, f, mesh, P.spline_order)
 
 
 class _Metric:
@@ -185,7 +210,9 @@ def field_register(P: FieldParams, gF: O.Grid, fixed, gM: O.Grid, moving, moving
             state = O.resample_field(gprev, state, gl)
         gprev = gl
         metric = _Metric(gl, gF, Fs, gM, Ms, P.radius, chainM, chainF)
-        u = exponentiate(gl, state, P.integration_steps)[0] if P.model == EXPONENTIAL else state
+        expo = P.model in (EXPONENTIAL, BSPLINE_EXPONENTIAL)
+        s_upd, s_tot = _smoother(P, gl, lev, False), _smoother(P, gl, lev, True)
+        u = exponentiate(gl, state, P.integration_steps)[0] if expo else state
         energies: List[float] = []
         conj = np.zeros_like(state)
         last = np.zeros_like(state)
@@ -198,7 +225,7 @@ def field_register(P: FieldParams, gF: O.Grid, fixed, gM: O.Grid, moving, moving
             conv = O.conv_value(energies, P.conv_window)
             if conv <= P.conv_threshold and not P.force_iterations:
                 break
-            if P.model == GAUSSIAN_DISPLACEMENT:
+            if not expo:
                 if it == 0:
                     conj[:] = 0.0
                     last[:] = 0.0
@@ -214,9 +241,8 @@ def field_register(P: FieldParams, gF: O.Grid, fixed, gM: O.Grid, moving, moving
                 conj = g + gamma * conj
                 direction = conj.copy()
 
-                def update(step, base=state, direction=direction, gl=gl):
-                    w = base + smooth(gl, step * direction, P.update_variance)
-                    return smooth(gl, w, P.total_variance)
+                def update(step, base=state, direction=direction):
+                    return s_tot(base + s_upd(step * direction))
 
                 def f(step, update=update):
                     res.evaluations += 1
@@ -227,13 +253,12 @@ def field_register(P: FieldParams, gF: O.Grid, fixed, gM: O.Grid, moving, moving
                 u = state
             else:
                 lr = P.learning_rate
-                w = state + smooth(gl, lr * g, P.update_variance)
-                state = smooth(gl, w, P.total_variance)
+                state = s_tot(state + s_upd(lr * g))
                 u = exponentiate(gl, state, P.integration_steps)[0]
             res.records.append((lev, it + 1, val, conv, lr))
             it += 1
         res.iterations.append(it)
-    if P.model == EXPONENTIAL:
+    if P.model in (EXPONENTIAL, BSPLINE_EXPONENTIAL):
         res.velocity = state
         res.field = exponentiate(gprev, state, P.integration_steps)[0]
         res.inverse = exponentiate(gprev, state, P.integration_steps, inverse=True)[0]

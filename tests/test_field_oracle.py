@@ -120,19 +120,18 @@ def test_exponential_matches_integrated_flow():
     assert np.abs(res)[core].max() < 0.05
 
 
-@pytest.mark.parametrize("model", [FO.GAUSSIAN_DISPLACEMENT, FO.EXPONENTIAL])
+@pytest.mark.parametrize("model", [FO.GAUSSIAN_DISPLACEMENT, FO.EXPONENTIAL, FO.BSPLINE_DISPLACEMENT,
+                                   FO.BSPLINE_EXPONENTIAL])
 def test_stage_recovers_translation(model):
     """A blob moved by a known sub-voxel translation: the stage's field over
     the blob's core approaches that translation and the energy falls."""
     g = O.Grid((32, 30, 28))
     shift = np.array([1.2, -0.8, 0.6])        # z, y, x voxels of the moving content
     F, M = blob_pair(g.shape, tuple(shift), seed=3)
-    if model == FO.GAUSSIAN_DISPLACEMENT:
-        P = FO.FieldParams(model=model, learning_rate=0.5, update_variance=2.0, total_variance=0.5, radius=2,
-                           iterations=(30, 20), shrink=(2, 1), sigma=(1.0, 0.0), force_iterations=True)
-    else:
-        P = FO.FieldParams(model=model, learning_rate=20.0, update_variance=2.0, total_variance=0.5, radius=2,
-                           iterations=(30, 20), shrink=(2, 1), sigma=(1.0, 0.0), force_iterations=True)
+    expo = model in (FO.EXPONENTIAL, FO.BSPLINE_EXPONENTIAL)
+    P = FO.FieldParams(model=model, learning_rate=20.0 if expo else 0.5, update_variance=2.0, total_variance=0.5,
+                       radius=2, iterations=(30, 20), shrink=(2, 1), sigma=(1.0, 0.0), force_iterations=True,
+                       update_mesh=(4, 4, 4), total_mesh=(0, 0, 0), spline_order=3)
     res = FO.field_register(P, g, F, g, M)
     core = tuple(slice(s // 2 - 3, s // 2 + 3) for s in g.shape)
     mean = res.field[core].reshape(-1, 3).mean(axis=0)       # x, y, z components

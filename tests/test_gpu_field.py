@@ -96,28 +96,42 @@ def test_exponential_matches_oracle(D, steps):
         np.testing.assert_allclose(out, ref, rtol=0, atol=2e-4)
 
 
-def run_both(model, D, iterations, shrink, sigma, lr, uvar, tvar, seed=2, steps=0, minit=()):
+def run_both(model, D, iterations, shrink, sigma, lr, uvar, tvar, seed=2, steps=0, minit=(), umesh=(0, 0, 0),
+             tmesh=(0, 0, 0), order=3):
     g, F, M = images(D, seed)
-    st = FM.FieldStage(model, lr, uvar, tvar, steps, 2, iterations, 1e-6, 10, shrink, sigma, False, True, True)
+    st = FM.FieldStage(model, lr, uvar, tvar, steps, 2, iterations, 1e-6, 10, shrink, sigma, False, True, True,
+                       update_mesh=umesh, total_mesh=tmesh, spline_order=order)
     res = FM.register_field(st, F, g, M, g, minit)
     P = FO.FieldParams(model=model, learning_rate=lr, update_variance=uvar, total_variance=tvar,
                        integration_steps=steps, radius=2, iterations=iterations, shrink=shrink, sigma=sigma,
-                       force_iterations=True)
+                       force_iterations=True, update_mesh=umesh, total_mesh=tmesh, spline_order=order)
     ref = FO.field_register(P, og(g), F, og(g), M, chain(minit, D))
     return g, res, ref
 
 
-@pytest.mark.parametrize("model,D", [(FM.GAUSSIAN_DISPLACEMENT, 3), (FM.EXPONENTIAL, 3),
-                                     (FM.GAUSSIAN_DISPLACEMENT, 2), (FM.EXPONENTIAL, 2)])
+STAGES = [(FM.GAUSSIAN_DISPLACEMENT, 3), (FM.EXPONENTIAL, 3), (FM.GAUSSIAN_DISPLACEMENT, 2), (FM.EXPONENTIAL, 2),
+          (FM.BSPLINE_DISPLACEMENT, 3), (FM.BSPLINE_EXPONENTIAL, 3), (FM.BSPLINE_DISPLACEMENT, 2),
+          (FM.BSPLINE_EXPONENTIAL, 2)]
+
+
+@pytest.mark.parametrize("model,D", STAGES, ids=[f"model{m}-{d}d" for m, d in STAGES])
 def test_stage_matches_oracle(model, D):
     """Two levels of the stage with preprocessing, pyramid smoothing and
-    both field smoothings: the same records (levels, iterations), values
+    both field smoothings (Gaussian, or B-spline fits of orders 2 and 3 on
+    meshes doubled per level): the same records (levels, iterations), values
     within 1e-4 relative, learning rates within 1e-3 relative, the final
     fields within 0.05 voxel (mean within 0.005)."""
-    lr = 0.4 if model == FM.GAUSSIAN_DISPLACEMENT else 15.0
+    expo = model in (FM.EXPONENTIAL, FM.BSPLINE_EXPONENTIAL)
+    lr = 15.0 if expo else 0.4
     c = np.array([10.0, 8.0, 6.0][:D])
     minit = [AffineTransform(rot(D, 0.02), np.full(D, 0.2), c)] if D == 3 else []
-    g, res, ref = run_both(model, D, (6, 4), (2, 1), (1.0, 0.0), lr, 2.0, 0.5, minit=minit)
+    order = 2 if D == 2 else 3
+    # B-spline forms: update meshes of 4 spans per axis (8 and 16 on the two
+    # levels); the exponential form also fits its velocity field (2 spans)
+    umesh = (4, 4, 4)[:D]
+    tmesh = (2, 2, 2)[:D] if model == FM.BSPLINE_EXPONENTIAL else (0, 0, 0)[:D]
+    g, res, ref = run_both(model, D, (6, 4), (2, 1), (1.0, 0.0), lr, 2.0, 0.5, minit=minit, umesh=umesh,
+                           tmesh=tmesh, order=order)
     assert res.iterations == ref.iterations
     assert len(res.records) == len(ref.records)
     for a, b in zip(res.records, ref.records):
@@ -129,7 +143,7 @@ def test_stage_matches_oracle(model, D):
     print(f"\nmodel {model} D={D}: energies {ref.records[0][2]:.5f} -> {ref.records[-1][2]:.5f}; field max dev "
           f"{dev.max():.2e}, mean {dev.mean():.2e} voxel")
     assert dev.max() < 0.05 and dev.mean() < 0.005
-    if model == FM.EXPONENTIAL:
+    if expo:
         assert np.abs(res.inverse - ref.inverse).max() / sp.min() < 0.05
         assert np.abs(res.velocity - ref.velocity).max() / sp.min() < 0.05
 
@@ -155,7 +169,8 @@ def test_stage_recovers_translation(model):
     assert np.abs(mean - shift[::-1]).max() < 0.35
 
 
-@pytest.mark.parametrize("model", ["GaussianDisplacementField[0.4,2,0.5]", "Exponential[15,2,0.5,0]"])
+@pytest.mark.parametrize("model", ["GaussianDisplacementField[0.4,2,0.5]", "Exponential[15,2,0.5,0]",
+                                   "BSplineExponential[15,3x3x2,2x2x1,0,3]"])
 def test_cli_field_stage(tmp_path, model):
     """cli registration with a rigid stage and then a field stage writes
     0GenericAffine.mat and 1Warp.nii.gz (Exponential also 1InverseWarp),
@@ -185,7 +200,7 @@ def test_cli_field_stage(tmp_path, model):
     res = FM.register_field(fs, F, gf, M, gm, [r.transform])
     fwd, _ = aio.read_nifti(prefix + "1Warp.nii.gz")
     np.testing.assert_array_equal(fwd, res.field)
-    if fs.model == FM.EXPONENTIAL:
+    if fs.expo:
         inv, _ = aio.read_nifti(prefix + "1InverseWarp.nii.gz")
         np.testing.assert_array_equal(inv, res.inverse)
     else:
