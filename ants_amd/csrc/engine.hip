@@ -3224,7 +3224,7 @@ struct FieldEngine {
   FieldMap fmap{}, mmap{};
   DevBuf F, M, Fs, Ms, tmp, minmax;
   DevBuf Fw, Mw, G;
-  DevBuf state, u, g, last, conj, trial, upd, sm, work, extra, phi2;
+  DevBuf state, u, g, last, conj, trial, upd, sm, work, extra, phi2, sdir;
   DevBuf ccpart, slot, dpart, dout, mbits;
   int64_t evaluations = 0;
   double minsp = 1.0;
@@ -3336,7 +3336,8 @@ struct FieldEngine {
     ANTS_TRY(buf(Fw, nmax));
     ANTS_TRY(buf(Mw, nmax));
     ANTS_TRY(buf(G, nf));
-    for (DevBuf* b : {&state, &u, &g, &last, &conj, &trial, &upd, &sm, &work, &extra, &phi2}) ANTS_TRY(buf(*b, nf));
+    for (DevBuf* b : {&state, &u, &g, &last, &conj, &trial, &upd, &sm, &work, &extra, &phi2, &sdir})
+      ANTS_TRY(buf(*b, nf));
     int64_t ncc = 1;
     for (int l = 0; l < P.n_levels; ++l)
       ncc = std::max(ncc, cc_partials_needed(to_dev(level_grid(gF, P.shrink[l])), P.cc_radius));
@@ -3451,17 +3452,25 @@ struct FieldEngine {
     return ANTS_OK;
   }
 
-  // GaussianDisplacementField update with step s along conj into dst:
-  // dst = state + smooth(s conj, updVar), then smooth(dst, totVar)
+  // the smoothed search direction of this iteration, sdir = smooth(conj):
+  // the update smoothing (Gaussian or B-spline fit) is linear, so the update
+  // of a step s, smooth(s conj), is s sdir and one smoothing serves every
+  // trial of the line search
+  int smooth_direction() {
+    const int64_t nf = nvox() * D;
+    if (!smooths_update()) {
+      HIP_TRY(hipMemcpyAsync(sdir.p, conj.p, nf * sizeof(float), hipMemcpyDeviceToDevice, s));
+      return ANTS_OK;
+    }
+    HIP_TRY(hipMemcpyAsync(upd.p, conj.p, nf * sizeof(float), hipMemcpyDeviceToDevice, s));
+    return smooth_field_of(false, upd.f(), sdir.f());
+  }
+
+  // displacement-form update with step s into dst: dst = state + s sdir,
+  // then the total field's smoothing
   int gdf_update(double step, float* dst) {
     const int64_t nf = nvox() * D;
-    launch_field_axpy(nf, nullptr, static_cast<float>(step), conj.f(), upd.f(), s);
-    const float* src = upd.f();
-    if (smooths_update()) {
-      ANTS_TRY(smooth_field_of(false, upd.f(), sm.f()));
-      src = sm.f();
-    }
-    launch_field_axpy(nf, state.f(), 1.f, src, dst, s);
+    launch_field_axpy(nf, state.f(), static_cast<float>(step), sdir.f(), dst, s);
     if (smooths_total()) {
       ANTS_TRY(smooth_field_of(true, dst, sm.f()));
       HIP_TRY(hipMemcpyAsync(dst, sm.p, nf * sizeof(float), hipMemcpyDeviceToDevice, s));
@@ -3504,6 +3513,7 @@ struct FieldEngine {
           if (gamma < 0.0 || gamma > 5.0) gamma = 0.0;
           HIP_TRY(hipMemcpyAsync(last.p, g.p, nf * sizeof(float), hipMemcpyDeviceToDevice, s));
           launch_field_axpy(nf, g.f(), static_cast<float>(gamma), conj.f(), conj.f(), s);
+          ANTS_TRY(smooth_direction());
           auto trial_value = [this](double step, double* v) -> int {
             ANTS_TRY(gdf_update(step, trial.f()));
             return metric(trial.f(), false, v);

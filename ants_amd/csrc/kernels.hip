@@ -2627,20 +2627,34 @@ __global__ void __launch_bounds__(kLinThreads) k_field_dots(int64_t n, const flo
   }
 }
 
-// One thread: the block partials in block order (three sums, a max).
-__global__ void k_field_dots_final(const double* __restrict__ partials, int nb, double* __restrict__ out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// One block: the block partials reduced in a fixed order (thread t takes
+// partials t, t + 256, ... in turn, then a fixed tree over the threads):
+// three sums and a max, the same bits on every run.
+__global__ void __launch_bounds__(256) k_field_dots_final(const double* __restrict__ partials, int nb,
+                                                          double* __restrict__ out) {
+  __shared__ double sh[4][256];
   double s0 = 0.0, s1 = 0.0, s2 = 0.0, m = 0.0;
-  for (int b = 0; b < nb; ++b) {
+  for (int b = threadIdx.x; b < nb; b += 256) {
     s0 += partials[b * 4 + 0];
     s1 += partials[b * 4 + 1];
     s2 += partials[b * 4 + 2];
     m = fmax(m, partials[b * 4 + 3]);
   }
-  out[0] = s0;
-  out[1] = s1;
-  out[2] = s2;
-  out[3] = m;
+  sh[0][threadIdx.x] = s0;
+  sh[1][threadIdx.x] = s1;
+  sh[2][threadIdx.x] = s2;
+  sh[3][threadIdx.x] = m;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) {
+      sh[0][threadIdx.x] += sh[0][threadIdx.x + h];
+      sh[1][threadIdx.x] += sh[1][threadIdx.x + h];
+      sh[2][threadIdx.x] += sh[2][threadIdx.x + h];
+      sh[3][threadIdx.x] = fmax(sh[3][threadIdx.x], sh[3][threadIdx.x + h]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) out[threadIdx.x] = sh[threadIdx.x][0];
 }
 
 // out = a + s * b elementwise (a null: s * b); out may alias a or b.
@@ -2875,14 +2889,14 @@ void launch_gauss_field_final(const GridDev& g, int np, const float* const* in, 
                        mpack(out, np), t, axis, w1, w2, static_cast<int>(boundary), maxbits);
 }
 
-// The fused field smoothing with tile TX x TY; z-chunk tuned or modelled.
-template <int TX, int TY>
+// The fused field smoothing with tile TX x TY and kernel radius R; z-chunk
+// tuned (radius 6, the SyN default variance 3) or modelled.
+template <int TX, int TY, int R = 6>
 static void smooth3d_fields(const GridDev& g, int np, const float* const* in, const float* const* orig,
                             float* const* out, const Taps& t, float w1, float w2, bool boundary,
                             unsigned int* maxbits, hipStream_t s) {
-  constexpr int R = 6;
   const int64_t tiles = static_cast<int64_t>((g.n[0] + TX - 1) / TX) * ((g.n[1] + TY - 1) / TY) * np;
-  int64_t zc = tune_get(kTuneSmoothZChunk, g.count());
+  int64_t zc = R == 6 ? tune_get(kTuneSmoothZChunk, g.count()) : 0;
   if (zc > 0) {
     zc = zc < g.n[2] ? zc : g.n[2];
   } else {
@@ -2903,12 +2917,23 @@ static void smooth3d_fields(const GridDev& g, int np, const float* const* in, co
 
 bool launch_smooth3d(const GridDev& g, int np, const float* const* in, const float* const* orig, float* const* out,
                      const Taps& t, float w1, float w2, bool boundary, unsigned int* maxbits, hipStream_t s) {
-  if (g.D != 3 || t.r != 6 || np > 4) return false;
+  if (g.D != 3 || t.r < 1 || t.r > 6 || np > 4) return false;
   if (static_cast<int64_t>(g.n[0]) * g.n[1] * 3 >= (int64_t{1} << 31)) return false;  // 32-bit in-plane offsets
+  constexpr int TX = kSmoothTX, TY = kSmoothTY;
+  switch (t.r) {
+    // other variances (field models' update / total / velocity smoothing,
+    // SyN with another update variance): the 16 x 32 tile, modelled z-chunks
+    case 1: smooth3d_fields<TX, TY, 1>(g, np, in, orig, out, t, w1, w2, boundary, maxbits, s); return true;
+    case 2: smooth3d_fields<TX, TY, 2>(g, np, in, orig, out, t, w1, w2, boundary, maxbits, s); return true;
+    case 3: smooth3d_fields<TX, TY, 3>(g, np, in, orig, out, t, w1, w2, boundary, maxbits, s); return true;
+    case 4: smooth3d_fields<TX, TY, 4>(g, np, in, orig, out, t, w1, w2, boundary, maxbits, s); return true;
+    case 5: smooth3d_fields<TX, TY, 5>(g, np, in, orig, out, t, w1, w2, boundary, maxbits, s); return true;
+    default: break;
+  }
   if (tune_get(kTuneSmoothTile, g.count()) == 2)
-    smooth3d_fields<kSmoothTX, kSmoothTY / 2>(g, np, in, orig, out, t, w1, w2, boundary, maxbits, s);
+    smooth3d_fields<TX, TY / 2>(g, np, in, orig, out, t, w1, w2, boundary, maxbits, s);
   else
-    smooth3d_fields<kSmoothTX, kSmoothTY>(g, np, in, orig, out, t, w1, w2, boundary, maxbits, s);
+    smooth3d_fields<TX, TY>(g, np, in, orig, out, t, w1, w2, boundary, maxbits, s);
   return true;
 }
 
@@ -3394,7 +3419,7 @@ void launch_field_dots(int D, int64_t n, const float* a, const float* b, double*
     hipLaunchKernelGGL(k_field_dots<2>, dim3(nb), dim3(kLinThreads), 0, s, n, a, b, partials);
   else
     hipLaunchKernelGGL(k_field_dots<3>, dim3(nb), dim3(kLinThreads), 0, s, n, a, b, partials);
-  hipLaunchKernelGGL(k_field_dots_final, dim3(1), dim3(64), 0, s, partials, nb, out);
+  hipLaunchKernelGGL(k_field_dots_final, dim3(1), dim3(256), 0, s, partials, nb, out);
 }
 
 void launch_field_axpy(int64_t n, const float* a, float sc, const float* b, float* out, hipStream_t s) {

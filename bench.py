@@ -405,6 +405,19 @@ def field_leg(pair, d_fixed, d_moving) -> dict:
     return out
 
 
+def bench_stages(args):
+    """The linear and field-transform stage legs on the headline pair, one
+    JSON line (for profiling them apart from the SyN headline)."""
+    from ants_amd import _lib as L
+    from ants_amd import phantoms as ph
+    pair = ph.phantom_pair_3d(args.seed)
+    d_fixed = L.DeviceArray.from_host(pair.fixed)
+    d_moving = L.DeviceArray.from_host(pair.moving)
+    line = {"mode": "stages", "linear_stages": linear_leg(pair, d_fixed, d_moving),
+            "field_stages": field_leg(pair, d_fixed, d_moving)}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -412,9 +425,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--mode", choices=["pair", "batch", "serve", "apply"], default="pair",
+    ap.add_argument("--mode", choices=["pair", "batch", "serve", "apply", "stages"], default="pair",
                     help="pair: the 192x224x192 headline; batch: 64 mixed pairs; serve: Poisson request stream; "
-                         "apply: antsApplyTransforms resampling")
+                         "apply: antsApplyTransforms resampling; stages: the linear and field-transform stage "
+                         "legs alone")
     ap.add_argument("--pairs", type=int, default=64)
     ap.add_argument("--slots", type=int, default=8)
     ap.add_argument("--requests", type=int, default=40)
@@ -442,6 +456,8 @@ def main():
         return bench_serve(args)
     if args.mode == "apply":
         return bench_apply(args)
+    if args.mode == "stages":
+        return bench_stages(args)
 
     from ants_amd import _lib as L
     from ants_amd import phantoms as ph
