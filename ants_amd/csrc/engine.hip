@@ -3390,10 +3390,11 @@ struct FieldEngine {
     return ANTS_OK;
   }
 
-  // value (and with deriv the derivative into g) at displacement ufield (null: 0)
-  int metric(const float* ufield, bool deriv, double* value) {
+  // value (and with deriv the derivative into g) at displacement ufield
+  // (null: 0), or at ufield + s2 u2 when u2 is given
+  int metric(const float* ufield, bool deriv, double* value, const float* u2 = nullptr, float s2 = 0.f) {
     ++evaluations;
-    launch_field_sample(lgeo, mgeo, Ms.f(), ufield, mmap, Mw.f(), deriv ? G.f() : nullptr, s);
+    launch_field_sample(lgeo, mgeo, Ms.f(), ufield, mmap, Mw.f(), deriv ? G.f() : nullptr, s, u2, s2);
     launch_cc(to_dev(gl), Fw.f(), Mw.f(), P.cc_radius, std::numeric_limits<float>::epsilon(),
               deriv ? g.f() : nullptr, nullptr, static_cast<Partial*>(ccpart.p), static_cast<RedSlot*>(slot.p), s,
               true, false, G.f());
@@ -3514,7 +3515,10 @@ struct FieldEngine {
           HIP_TRY(hipMemcpyAsync(last.p, g.p, nf * sizeof(float), hipMemcpyDeviceToDevice, s));
           launch_field_axpy(nf, g.f(), static_cast<float>(gamma), conj.f(), conj.f(), s);
           ANTS_TRY(smooth_direction());
+          // a trial without total smoothing is sampled at state + s sdir
+          // directly (the same float operations as the update writes)
           auto trial_value = [this](double step, double* v) -> int {
+            if (!smooths_total()) return metric(state.f(), false, v, sdir.f(), static_cast<float>(step));
             ANTS_TRY(gdf_update(step, trial.f()));
             return metric(trial.f(), false, v);
           };

@@ -2524,10 +2524,14 @@ __global__ void __launch_bounds__(256) k_lin_reduce(const double* __restrict__ p
 // an axis where either leaves the buffer -- taken back through A (A^T grad):
 // the derivative of the metric by the displacement at x.  Double
 // coordinates, contraction off (as the antsApplyTransforms kernel).
+// With u2 the displacement is u + s2 u2, formed as the field update kernel
+// forms it (the same float operations): a line-search trial is sampled
+// without writing the trial field.
 template <int D, bool GRAD>
 __global__ void __launch_bounds__(kBlock) k_field_sample(GeomDev vg, GeomDev ig, const float* __restrict__ img,
-                                                         const float* __restrict__ u, FieldMap map,
-                                                         float* __restrict__ out, float* __restrict__ grad) {
+                                                         const float* __restrict__ u, const float* __restrict__ u2,
+                                                         float s2, FieldMap map, float* __restrict__ out,
+                                                         float* __restrict__ grad) {
   const int64_t N = static_cast<int64_t>(vg.n[0]) * vg.n[1] * (D == 3 ? vg.n[2] : 1);
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= N) return;
@@ -2537,7 +2541,14 @@ __global__ void __launch_bounds__(kBlock) k_field_sample(GeomDev vg, GeomDev ig,
   geom_point<D>(vg, ix, x);
   if (u) {
 #pragma unroll
-    for (int d = 0; d < D; ++d) x[d] = x[d] + static_cast<double>(u[i * D + d]);
+    for (int d = 0; d < D; ++d) {
+      float ud = u[i * D + d];
+      if (u2) {
+        const float t = s2 * u2[i * D + d];
+        ud = ud + t;
+      }
+      x[d] = x[d] + static_cast<double>(ud);
+    }
   }
   double y[3] = {0.0, 0.0, 0.0};
 #pragma unroll
@@ -3397,17 +3408,17 @@ bool launch_lin_metric(const LinDev& L, const LinJac& J, int D, int P, int mode,
 
 
 void launch_field_sample(const GeomDev& vg, const GeomDev& ig, const float* img, const float* u, const FieldMap& map,
-                         float* out, float* grad, hipStream_t s) {
+                         float* out, float* grad, hipStream_t s, const float* u2, float s2) {
   const int64_t n = static_cast<int64_t>(vg.n[0]) * vg.n[1] * (vg.D == 3 ? vg.n[2] : 1);
   const dim3 grid(blocks_for(n));
   if (vg.D == 2 && grad)
-    hipLaunchKernelGGL((k_field_sample<2, true>), grid, dim3(kBlock), 0, s, vg, ig, img, u, map, out, grad);
+    hipLaunchKernelGGL((k_field_sample<2, true>), grid, dim3(kBlock), 0, s, vg, ig, img, u, u2, s2, map, out, grad);
   else if (vg.D == 2)
-    hipLaunchKernelGGL((k_field_sample<2, false>), grid, dim3(kBlock), 0, s, vg, ig, img, u, map, out, grad);
+    hipLaunchKernelGGL((k_field_sample<2, false>), grid, dim3(kBlock), 0, s, vg, ig, img, u, u2, s2, map, out, grad);
   else if (grad)
-    hipLaunchKernelGGL((k_field_sample<3, true>), grid, dim3(kBlock), 0, s, vg, ig, img, u, map, out, grad);
+    hipLaunchKernelGGL((k_field_sample<3, true>), grid, dim3(kBlock), 0, s, vg, ig, img, u, u2, s2, map, out, grad);
   else
-    hipLaunchKernelGGL((k_field_sample<3, false>), grid, dim3(kBlock), 0, s, vg, ig, img, u, map, out, grad);
+    hipLaunchKernelGGL((k_field_sample<3, false>), grid, dim3(kBlock), 0, s, vg, ig, img, u, u2, s2, map, out, grad);
 }
 
 int field_dots_partials(int64_t n) { return static_cast<int>(lin_blocks(n)); }

@@ -334,9 +334,10 @@ struct FieldMap {
   double b[3];
   double sp[3];
 };
-// out = img(y) (0 outside), grad (optional) = a^T grad img(y) per voxel.
+// out = img(y) (0 outside), grad (optional) = a^T grad img(y) per voxel;
+// with u2 the displacement is u + s2 u2 (formed as launch_field_axpy does).
 void launch_field_sample(const GeomDev& vg, const GeomDev& ig, const float* img, const float* u, const FieldMap& map,
-                         float* out, float* grad, hipStream_t s);
+                         float* out, float* grad, hipStream_t s, const float* u2 = nullptr, float s2 = 0.f);
 // out[0] = sum a.b, out[1] = sum b.b, out[2] = sum a.a, out[3] = max |a|^2
 // over n D-vectors (b may be null); partials: field_dots_partials(n) x 4
 // doubles.
