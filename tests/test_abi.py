@@ -85,10 +85,14 @@ def _c_layout(tmp_path, exprs):
 
 def test_ctypes_structs_match_c_layout(tmp_path):
     from ants_amd import serving as S
+    from ants_amd import linear as LN
+    from ants_amd import fieldmodels as FM
     structs = {"ants_image_desc": L.ImageDesc, "ants_syn_params": L.SynParamsC, "ants_syn_stats": L.SynStatsC,
                "ants_syn_kernel_stats": L.KernelStatsC, "ants_batch_config": S.BatchConfigC,
                "ants_syn_request": S.RequestC, "ants_request_result": S.ResultC, "ants_transform": L.TransformC,
-               "ants_syn_iteration_record": L.IterRecordC}
+               "ants_syn_iteration_record": L.IterRecordC, "ants_linear_params": LN.LinearParamsC,
+               "ants_linear_record": LN.LinearRecordC, "ants_linear_result": LN.LinearResultC,
+               "ants_field_params": FM.FieldParamsC}
     exprs, want = [], []
     for cname, py in structs.items():
         exprs.append(f"sizeof({cname})")
@@ -164,3 +168,34 @@ def test_observer_argument_converts():
     L.ITER_CB.from_param(cb)
     assert cb(None, 0, 1, -0.5, 0.1, kin) == 0 and cb(None, 0, 2, -0.6, 0.1, kin) == 1
     assert got[0] == IterationRecord(0, 1, -0.5, 0.1, (1, 2, 3, 4))
+
+
+def test_field_stage_grammar():
+    """-t GaussianDisplacementField / Exponential / BSplineDisplacementField /
+    BSplineExponential as antsRegistrationTemplateHeader.h:692-743, 871-935
+    reads them: variances, integration steps, meshes (AxBxC or a knot
+    spacing with the fixed geometry), spline order positions."""
+    from ants_amd import fieldmodels as FM
+    st = FM.FieldStage.from_args("GaussianDisplacementField[0.25,3,0.5]", "CC[f,m,1,2]", "[20x10,1e-7,5]", "2x1",
+                                 "1x0mm")
+    assert (st.model, st.gradient_step, st.update_variance, st.total_variance, st.radius) == (0, 0.25, 3.0, 0.5, 2)
+    assert list(st.iterations) == [20, 10] and st.threshold == 1e-7 and st.window == 5 and st.sigma_in_mm
+    st = FM.FieldStage.from_args("Exponential[0.5,3,0.5,7]", "CC[f,m]", "[5]", "1", "0")
+    assert (st.model, st.integration_steps, st.radius, st.expo) == (1, 7, 4, True)
+    st = FM.FieldStage.from_args("Exponential[0.5,3,0.5]", "CC[f,m,1,4]", "[5]", "1", "0")
+    assert st.integration_steps == 0
+    st = FM.FieldStage.from_args("BSplineDisplacementField[0.3,4x5x6,2x2x2,2]", "CC[f,m,1,4]", "[5]", "1", "0")
+    assert (st.model, tuple(st.update_mesh), tuple(st.total_mesh), st.spline_order) == (2, (4, 5, 6), (2, 2, 2), 2)
+    st = FM.FieldStage.from_args("BSplineDisplacementField[0.3,4x5x6]", "CC[f,m,1,4]", "[5]", "1", "0")
+    assert tuple(st.total_mesh) == (0, 0, 0) and st.spline_order == 3 and not st.expo
+    # knot spacing 26 mm on 192x224x192 at 1 mm: ceil(191/26) = 8, ceil(223/26) = 9
+    st = FM.FieldStage.from_args("BSplineExponential[0.3,26,0,5,2]", "CC[f,m,1,4]", "[5]", "1", "0",
+                                 (192, 224, 192), (1.0, 1.0, 1.0))
+    assert (tuple(st.update_mesh), tuple(st.total_mesh), st.integration_steps, st.spline_order) == \
+        ((8, 9, 8), (0, 0, 0), 5, 2)
+    for bad in ("SyN[0.1,3,0]", "GaussianDisplacementField[0.25,3]", "BSplineExponential[0.3]"):
+        with pytest.raises(ValueError):
+            FM.FieldStage.from_args(bad, "CC[f,m,1,4]", "[5]", "1", "0")
+    with pytest.raises(ValueError):
+        FM.FieldStage.from_args("Exponential[0.5,3,0.5]", "MI[f,m,1,32]", "[5]", "1", "0")
+    assert FM.FieldStage.is_field("Exponential[1,2,3]") and not FM.FieldStage.is_field("BSplineSyN[0.1,26]")
