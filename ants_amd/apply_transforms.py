@@ -81,16 +81,30 @@ class DisplacementFieldTransform:
 Transform = Union[AffineTransform, DisplacementFieldTransform, None]
 
 
+BSPLINE_BASE = 16          # ANTS_INTERP_BSPLINE_ORDER(q) = 16 + q
+BSPLINE_DEFAULT_ORDER = 3  # BSplineInterpolateImageFunction's default
+
+
 def parse_interpolation(name: str) -> int:
-    """-n Linear | NearestNeighbor | GenericLabel[Linear]"""
-    m = re.fullmatch(r"\s*([A-Za-z]+)\s*(?:\[\s*([A-Za-z]*)\s*\])?\s*", name)
+    """-n Linear | NearestNeighbor | GenericLabel[Linear] | BSpline[order]
+    (Examples/make_interpolator_snip.tmpl:18-40; BSpline's order is 0..5,
+    3 when omitted)."""
+    m = re.fullmatch(r"\s*([A-Za-z]+)\s*(?:\[\s*([A-Za-z0-9]*)\s*\])?\s*", name)
     if not m:
         raise ValueError(f"bad interpolation {name!r}")
     base = m.group(1).lower()
+    if base == "bspline":
+        arg = m.group(2)
+        if arg and not arg.isdigit():
+            raise ValueError(f"bad B-spline order in {name!r}")
+        order = int(arg) if arg else BSPLINE_DEFAULT_ORDER
+        if not 0 <= order <= 5:
+            raise ValueError(f"B-spline order must be 0..5, got {order}")
+        return BSPLINE_BASE + order
     if base == "genericlabel" and m.group(2) and m.group(2).lower() != "linear":
         raise ValueError("GenericLabel is supported with its default Linear interpolator only")
     if base not in INTERPOLATORS:
-        raise ValueError(f"interpolation {name!r} is not supported (Linear, NearestNeighbor, GenericLabel)")
+        raise ValueError(f"interpolation {name!r} is not supported (Linear, NearestNeighbor, GenericLabel, BSpline)")
     return INTERPOLATORS[base]
 
 

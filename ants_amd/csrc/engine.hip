@@ -2709,7 +2709,9 @@ extern "C" int ants_apply_transforms(const ants_image_desc* ref, const ants_imag
   if (!d_in || !d_out) return fail(ANTS_ERR_INVALID, "null image buffer");
   if (n_transforms < 0 || n_transforms > kMaxTransforms || (n_transforms > 0 && !transforms))
     return fail(ANTS_ERR_INVALID, "transform count must be 0..%d", kMaxTransforms);
-  if (interpolation < ANTS_INTERP_LINEAR || interpolation > ANTS_INTERP_GENERIC_LABEL)
+  const bool bspline =
+      interpolation >= ANTS_INTERP_BSPLINE_ORDER(0) && interpolation <= ANTS_INTERP_BSPLINE_ORDER(5);
+  if (!bspline && (interpolation < ANTS_INTERP_LINEAR || interpolation > ANTS_INTERP_GENERIC_LABEL))
     return fail(ANTS_ERR_INVALID, "unknown interpolation %d", interpolation);
   P.nt = n_transforms;
   P.defv = static_cast<float>(default_value);
@@ -2719,8 +2721,22 @@ extern "C" int ants_apply_transforms(const ants_image_desc* ref, const ants_imag
   for (int k = 0; k < 3; ++k) go.n[k] = P.ref.n[k];
   set_divisors(go);
   hipStream_t s = static_cast<hipStream_t>(stream);
-  launch_apply_transforms(go, P, interpolation, d_in, d_out, s);
-  HIP_TRY(hipGetLastError());
+  if (!bspline) {
+    launch_apply_transforms(go, P, interpolation, d_in, d_out, s);
+    HIP_TRY(hipGetLastError());
+    return ANTS_OK;
+  }
+  const int order = interpolation - ANTS_INTERP_BSPLINE_ORDER(0);
+  const int64_t NI = static_cast<int64_t>(P.in.n[0]) * P.in.n[1] * (D == 3 ? P.in.n[2] : 1);
+  void* coef = nullptr;
+  HIP_TRY(hipMallocAsync(&coef, static_cast<size_t>(NI) * sizeof(double), s));
+  P.coef = static_cast<const double*>(coef);
+  launch_bspline_coefficients(P.in.n, D, d_in, static_cast<double*>(coef), order, s);
+  launch_apply_transforms(go, P, 16 + order, d_in, d_out, s);
+  const hipError_t le = hipGetLastError();
+  const hipError_t fe = hipFreeAsync(coef, s);
+  HIP_TRY(le);
+  HIP_TRY(fe);
   return ANTS_OK;
 }
 

@@ -1922,6 +1922,75 @@ __device__ __forceinline__ double lerp_cell(const double v[8], const double w[3]
   return b0 + (b1 - b0) * w[2];
 }
 
+// BSplineInterpolateImageFunction's support and weights for order Q at
+// continuous index x: the Q + 1 samples from start = floor(x) - Q/2 (odd Q)
+// or floor(x + 1/2) - Q/2 (even Q), weighted by the centred B-spline of
+// degree Q at x - j, written per order as the polynomial pieces that the
+// fractional offset w selects (Unser, IEEE SPM 16(6) 1999, eq. 8) so that no
+// division is needed.
+template <int Q>
+__device__ __forceinline__ int64_t bspline_weights(double x, double wt[Q + 1]) {
+  const double fl = floor((Q & 1) ? x : x + 0.5);
+  const double w = x - fl;  // odd Q: [0, 1); even Q: [-1/2, 1/2)
+  if constexpr (Q == 0) {
+    wt[0] = 1.0;
+  } else if constexpr (Q == 1) {
+    wt[0] = 1.0 - w;
+    wt[1] = w;
+  } else if constexpr (Q == 2) {
+    const double a = 0.5 - w, b = 0.5 + w;
+    wt[0] = 0.5 * a * a;
+    wt[1] = 0.75 - w * w;
+    wt[2] = 0.5 * b * b;
+  } else if constexpr (Q == 3) {
+    const double a = 1.0 - w, w2 = w * w, a2 = a * a;
+    wt[0] = (1.0 / 6.0) * a2 * a;
+    wt[1] = (2.0 / 3.0) - w2 + 0.5 * w2 * w;
+    wt[2] = (2.0 / 3.0) - a2 + 0.5 * a2 * a;
+    wt[3] = (1.0 / 6.0) * w2 * w;
+  } else if constexpr (Q == 4) {
+    // |t| < 1/2: 115/192 - 5/8 t^2 + 1/4 t^4; 1/2 <= |t| < 3/2:
+    // 55/96 + 5/24 t - 5/4 t^2 + 5/6 t^3 - 1/6 t^4; 3/2 <= |t| < 5/2: (5/2 - |t|)^4 / 24
+    auto mid = [](double t) {
+      return (55.0 / 96.0) + t * ((5.0 / 24.0) + t * ((-5.0 / 4.0) + t * ((5.0 / 6.0) + t * (-1.0 / 6.0))));
+    };
+    const double a = 0.5 - w, b = 0.5 + w, w2 = w * w, a2 = a * a, b2 = b * b;
+    wt[0] = (1.0 / 24.0) * a2 * a2;
+    wt[1] = mid(1.0 + w);
+    wt[2] = (115.0 / 192.0) + w2 * ((-5.0 / 8.0) + 0.25 * w2);
+    wt[3] = mid(1.0 - w);
+    wt[4] = (1.0 / 24.0) * b2 * b2;
+  } else {
+    static_assert(Q == 5, "B-spline order 0..5");
+    // |t| < 1: 11/20 - 1/2 t^2 + 1/4 t^4 - 1/12 |t|^5; 1 <= |t| < 2:
+    // 17/40 + 5/8 t - 7/4 t^2 + 5/4 t^3 - 3/8 t^4 + 1/24 t^5; 2 <= |t| < 3: (3 - |t|)^5 / 120
+    auto in = [](double t) {
+      const double t2 = t * t;
+      return (11.0 / 20.0) + t2 * (-0.5 + t2 * (0.25 + t * (-1.0 / 12.0)));
+    };
+    auto mid = [](double t) {
+      return (17.0 / 40.0) +
+             t * ((5.0 / 8.0) + t * ((-7.0 / 4.0) + t * ((5.0 / 4.0) + t * ((-3.0 / 8.0) + t * (1.0 / 24.0)))));
+    };
+    const double a = 1.0 - w, a2 = a * a, w2 = w * w;
+    wt[0] = (1.0 / 120.0) * a2 * a2 * a;
+    wt[1] = mid(1.0 + w);
+    wt[2] = in(w);
+    wt[3] = in(a);
+    wt[4] = mid(2.0 - w);
+    wt[5] = (1.0 / 120.0) * w2 * w2 * w;
+  }
+  return static_cast<int64_t>(fl) - Q / 2;
+}
+// Mirror (whole-sample symmetric) index into [0, len): period 2 (len - 1).
+__device__ __forceinline__ int64_t mirror_index(int64_t j, int64_t len) {
+  if (len == 1) return 0;
+  const int64_t period = 2 * (len - 1);
+  j = j < 0 ? -j : j;
+  j = j % period;
+  return j >= len ? period - j : j;
+}
+
 template <int D, int INTERP>
 __global__ void __launch_bounds__(kBlock) k_apply_transforms(GridDev go, ApplyDev P, const float* __restrict__ img,
                                                              float* __restrict__ out) {
@@ -1979,6 +2048,44 @@ __global__ void __launch_bounds__(kBlock) k_apply_transforms(GridDev go, ApplyDe
     out[i] = img[id[0] + static_cast<int64_t>(P.in.n[0]) * (id[1] + static_cast<int64_t>(P.in.n[1]) * id[2])];
     return;
   }
+  if constexpr (INTERP >= 16) {
+    // BSplineInterpolateImageFunction of order Q over the coefficients,
+    // indices mirrored into the buffer
+    constexpr int Q = INTERP - 16;
+    int64_t idx[3][Q + 1];
+    double wt[3][Q + 1];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      if (d >= D) {
+#pragma unroll
+        for (int k = 0; k <= Q; ++k) {
+          idx[d][k] = 0;
+          wt[d][k] = 0.0;
+        }
+        continue;
+      }
+      const int64_t start = bspline_weights<Q>(c[d], wt[d]);
+#pragma unroll
+      for (int k = 0; k <= Q; ++k) idx[d][k] = mirror_index(start + k, P.in.n[d]);
+    }
+    constexpr int QZ = D == 3 ? Q : 0;
+    if constexpr (D == 2) wt[2][0] = 1.0;
+    double acc = 0.0;
+#pragma unroll
+    for (int kz = 0; kz <= QZ; ++kz) {
+      const int64_t oz = idx[2][kz] * P.in.n[1];
+#pragma unroll
+      for (int ky = 0; ky <= Q; ++ky) {
+        const double* __restrict__ row = P.coef + (oz + idx[1][ky]) * P.in.n[0];
+        double r = 0.0;
+#pragma unroll
+        for (int kx = 0; kx <= Q; ++kx) r = r + wt[0][kx] * row[idx[0][kx]];
+        acc = acc + (wt[2][kz] * wt[1][ky]) * r;
+      }
+    }
+    out[i] = static_cast<float>(acc);
+    return;
+  } else {
   int64_t lo[3], hi[3];
   double w[3];
   lin_setup<D>(P.in, c, lo, hi, w);
@@ -2009,6 +2116,182 @@ __global__ void __launch_bounds__(kBlock) k_apply_transforms(GridDev go, ApplyDe
     }
   }
   out[i] = best;
+  }
+}
+
+// BSplineDecompositionImageFilter along one axis: every line is multiplied
+// by the gain prod (1 - z)(1 - 1/z) of the order's poles, then filtered per
+// pole by a causal pass (initial value: the mirrored geometric sum,
+// truncated where |z|^n falls below 1e-10) and an anticausal pass (Unser,
+// Aldroubi & Eden, IEEE TSP 41(2) 1993; the recursive filter of Unser,
+// IEEE Signal Processing Magazine 16(6) 1999).
+struct DecompPoles {
+  int np;
+  int64_t horizon[2];
+  double z[2];
+  double gain;
+};
+
+template <typename I>
+__device__ __forceinline__ void decompose_line(double* __restrict__ L, I len, I step, const DecompPoles& P) {
+  for (int p = 0; p < P.np; ++p) {
+    const double z = P.z[p];
+    double c0;
+    if (P.horizon[p] < static_cast<int64_t>(len)) {
+      double zn = z, sum = L[0];
+      for (I n = 1; n < static_cast<I>(P.horizon[p]); ++n) {
+        sum = sum + zn * L[n * step];
+        zn = zn * z;
+      }
+      c0 = sum;
+    } else {
+      double zn = z;
+      const double iz = 1.0 / z;
+      double z2n = pow(z, static_cast<double>(len - 1));
+      double sum = L[0] + z2n * L[(len - 1) * step];
+      z2n = z2n * z2n * iz;
+      for (I n = 1; n <= len - 2; ++n) {
+        sum = sum + (zn + z2n) * L[n * step];
+        zn = zn * z;
+        z2n = z2n * iz;
+      }
+      c0 = sum / (1.0 - zn * zn);
+    }
+    L[0] = c0;
+    double prev = c0;
+    I k = 1;
+    for (; k + 4 <= len; k += 4) {
+      const double a0 = L[k * step], a1 = L[(k + 1) * step], a2 = L[(k + 2) * step], a3 = L[(k + 3) * step];
+      const double b0 = a0 + z * prev;
+      const double b1 = a1 + z * b0;
+      const double b2 = a2 + z * b1;
+      const double b3 = a3 + z * b2;
+      L[k * step] = b0;
+      L[(k + 1) * step] = b1;
+      L[(k + 2) * step] = b2;
+      L[(k + 3) * step] = b3;
+      prev = b3;
+    }
+    for (; k < len; ++k) {
+      prev = L[k * step] + z * prev;
+      L[k * step] = prev;
+    }
+    prev = (z / (z * z - 1.0)) * (z * L[(len - 2) * step] + prev);
+    L[(len - 1) * step] = prev;
+    k = len - 2;
+    for (; k - 3 >= 0; k -= 4) {
+      const double a0 = L[k * step], a1 = L[(k - 1) * step], a2 = L[(k - 2) * step], a3 = L[(k - 3) * step];
+      const double b0 = z * (prev - a0);
+      const double b1 = z * (b0 - a1);
+      const double b2 = z * (b1 - a2);
+      const double b3 = z * (b2 - a3);
+      L[k * step] = b0;
+      L[(k - 1) * step] = b1;
+      L[(k - 2) * step] = b2;
+      L[(k - 3) * step] = b3;
+      prev = b3;
+    }
+    for (; k >= 0; --k) {
+      prev = z * (prev - L[k * step]);
+      L[k * step] = prev;
+    }
+  }
+}
+
+// A tile of TL lines staged in LDS: the block (one wave) loads the tile with
+// coalesced reads, multiplied by the gain (from the float image on the first
+// axis), each of the first TL lanes runs its line's recursions in LDS with
+// the next four samples loaded ahead, and the wave writes the tile back.  X:
+// lines along the contiguous axis (rows of the tile padded to len + 1
+// doubles); else TL neighbouring lines interleaved, sample k of line r at
+// k * TL + r.
+template <bool X, int TL>
+__global__ void __launch_bounds__(64) k_bspline_decompose_lds(const float* __restrict__ fin, double* __restrict__ c,
+                                                              int64_t lines, int64_t len, int64_t stride,
+                                                              int64_t outer_stride, DecompPoles P) {
+  extern __shared__ double lds_d[];
+  const int tid = threadIdx.x;
+  int64_t base;
+  int nr;
+  const int n = static_cast<int>(len);  // a tile holds at most 8192 samples
+  if constexpr (X) {
+    const int64_t r0 = static_cast<int64_t>(blockIdx.x) * TL;
+    nr = static_cast<int>(min(static_cast<int64_t>(TL), lines - r0));
+    base = r0 * len;
+  } else {
+    const int64_t tpb = (stride + TL - 1) / TL;  // tiles across the lines of one outer index
+    const int64_t b = blockIdx.x / tpb, a0 = (blockIdx.x % tpb) * TL;
+    nr = static_cast<int>(min(static_cast<int64_t>(TL), stride - a0));
+    base = b * outer_stride + a0;
+  }
+  static_assert((TL & (TL - 1)) == 0 && TL <= 64, "TL: a power of two up to 64");
+  constexpr int sh = TL == 64 ? 6 : TL == 32 ? 5 : TL == 16 ? 4 : 3;
+  // tile element e: X: row e / n, sample e % n (contiguous in memory);
+  // else sample e >> sh of line e & (TL - 1)
+  const int total = X ? nr * n : n * TL;
+  auto gaddr = [&](int e, bool& ok) -> int64_t {
+    if constexpr (X) {
+      ok = e < total;
+      return base + e;
+    } else {
+      const int k = e >> sh, r = e & (TL - 1);
+      ok = e < total && r < nr;
+      return base + k * stride + r;
+    }
+  };
+  auto laddr = [&](int e) -> int {
+    if constexpr (X) {
+      const int r = e / n;
+      return e + r;  // r * (n + 1) + (e - r * n)
+    } else {
+      return e;
+    }
+  };
+  // eight independent loads in flight per lane
+  for (int e0 = tid; e0 < total; e0 += 64 * 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      bool ok;
+      const int64_t g = gaddr(e0 + 64 * u, ok);
+      v[u] = ok ? (fin ? static_cast<double>(fin[g]) : c[g]) : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      bool ok;
+      (void)gaddr(e0 + 64 * u, ok);
+      if (ok) lds_d[laddr(e0 + 64 * u)] = v[u] * P.gain;
+    }
+  }
+  __syncthreads();
+  if (tid < nr) {
+    if constexpr (X) decompose_line<int>(lds_d + tid * (n + 1), n, 1, P);
+    else decompose_line<int>(lds_d + tid, n, TL, P);
+  }
+  __syncthreads();
+  for (int e = tid; e < total; e += 64) {
+    bool ok;
+    const int64_t g = gaddr(e, ok);
+    if (ok) c[g] = lds_d[laddr(e)];
+  }
+}
+
+// Lines too long for an LDS tile: one thread per line in global memory
+// (line l: offset l % stride within the axes before this one, l / stride
+// over the axes after it).
+__global__ void __launch_bounds__(kBlock) k_bspline_decompose(double* __restrict__ c, int64_t lines, int64_t len,
+                                                              int64_t stride, int64_t outer_stride, DecompPoles P) {
+  const int64_t l = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (l >= lines) return;
+  double* __restrict__ v = c + (l / stride) * outer_stride + l % stride;
+  for (int64_t k = 0; k < len; ++k) v[k * stride] = v[k * stride] * P.gain;
+  decompose_line<int64_t>(v, len, stride, P);
+}
+
+__global__ void __launch_bounds__(kBlock) k_to_double(const float* __restrict__ in, double* __restrict__ out,
+                                                      int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = static_cast<double>(in[i]);
 }
 #pragma clang fp contract(on)
 
@@ -3336,16 +3619,93 @@ void launch_apply_transforms(const GridDev& go, const ApplyDev& P, int interp, c
   const dim3 grid(blocks_for(go.count()));
 #define ANTS_APPLY_LAUNCH(DD, II) \
   hipLaunchKernelGGL((k_apply_transforms<DD, II>), grid, dim3(kBlock), 0, s, go, P, in, out)
-  if (go.D == 2) {
-    if (interp == 0) ANTS_APPLY_LAUNCH(2, 0);
-    else if (interp == 1) ANTS_APPLY_LAUNCH(2, 1);
-    else ANTS_APPLY_LAUNCH(2, 2);
-  } else {
-    if (interp == 0) ANTS_APPLY_LAUNCH(3, 0);
-    else if (interp == 1) ANTS_APPLY_LAUNCH(3, 1);
-    else ANTS_APPLY_LAUNCH(3, 2);
+#define ANTS_APPLY_DIM(DD)                   \
+  switch (interp) {                          \
+    case 0: ANTS_APPLY_LAUNCH(DD, 0); break;   \
+    case 1: ANTS_APPLY_LAUNCH(DD, 1); break;   \
+    case 2: ANTS_APPLY_LAUNCH(DD, 2); break;   \
+    case 16: ANTS_APPLY_LAUNCH(DD, 16); break; \
+    case 17: ANTS_APPLY_LAUNCH(DD, 17); break; \
+    case 18: ANTS_APPLY_LAUNCH(DD, 18); break; \
+    case 19: ANTS_APPLY_LAUNCH(DD, 19); break; \
+    case 20: ANTS_APPLY_LAUNCH(DD, 20); break; \
+    default: ANTS_APPLY_LAUNCH(DD, 21); break; \
   }
+  if (go.D == 2) {
+    ANTS_APPLY_DIM(2)
+  } else {
+    ANTS_APPLY_DIM(3)
+  }
+#undef ANTS_APPLY_DIM
 #undef ANTS_APPLY_LAUNCH
+}
+
+void launch_bspline_coefficients(const int n[3], int D, const float* img, double* coef, int order, hipStream_t s) {
+  const int64_t N = static_cast<int64_t>(n[0]) * n[1] * (D == 3 ? n[2] : 1);
+  // poles of the B-spline of each order (Unser 1999, table 1)
+  DecompPoles P{};
+  switch (order) {
+    case 2: P.z[0] = std::sqrt(8.0) - 3.0; P.np = 1; break;
+    case 3: P.z[0] = std::sqrt(3.0) - 2.0; P.np = 1; break;
+    case 4:
+      P.z[0] = std::sqrt(664.0 - std::sqrt(438976.0)) + std::sqrt(304.0) - 19.0;
+      P.z[1] = std::sqrt(664.0 + std::sqrt(438976.0)) - std::sqrt(304.0) - 19.0;
+      P.np = 2;
+      break;
+    case 5:
+      P.z[0] = std::sqrt(135.0 / 2.0 - std::sqrt(17745.0 / 4.0)) + std::sqrt(105.0 / 4.0) - 13.0 / 2.0;
+      P.z[1] = std::sqrt(135.0 / 2.0 + std::sqrt(17745.0 / 4.0)) - std::sqrt(105.0 / 4.0) - 13.0 / 2.0;
+      P.np = 2;
+      break;
+    default: break;  // orders 0 and 1 interpolate the samples themselves
+  }
+  P.gain = 1.0;
+  for (int p = 0; p < P.np; ++p) {
+    P.gain = P.gain * (1.0 - P.z[p]) * (1.0 - 1.0 / P.z[p]);
+    // causal initialisation: geometric sum truncated where |z|^n < 1e-10
+    P.horizon[p] = static_cast<int64_t>(std::ceil(std::log(1e-10) / std::log(std::fabs(P.z[p]))));
+  }
+  bool converted = false;
+  for (int d = 0; d < D && P.np > 0; ++d) {
+    const int64_t len = n[d];
+    if (len < 2) continue;
+    int64_t inner = 1;
+    for (int k = 0; k < d; ++k) inner *= n[k];
+    const int64_t lines = N / len;
+    // widest tile of 64..8 lines within 64 KiB of LDS
+    const int64_t row = d == 0 ? len + 1 : len;
+    int TL = 64;
+    while (TL >= 8 && TL * row * 8 > 65536) TL /= 2;
+    const float* fin = converted ? nullptr : img;
+    if (TL < 8) {
+      // lines too long for LDS: one thread per line in global memory
+      if (!converted)
+        hipLaunchKernelGGL(k_to_double, dim3(blocks_for(N)), dim3(kBlock), 0, s, img, coef, N);
+      hipLaunchKernelGGL(k_bspline_decompose, dim3(blocks_for(lines)), dim3(kBlock), 0, s, coef, lines, len, inner,
+                         inner * len, P);
+    } else {
+      const int64_t tiles = d == 0 ? (lines + TL - 1) / TL : (N / (inner * len)) * ((inner + TL - 1) / TL);
+      const dim3 grid(static_cast<unsigned int>(tiles));
+      const size_t lds = static_cast<size_t>(TL * row * 8);
+#define ANTS_DECOMP(XX, TT)                                                                                    \
+  hipLaunchKernelGGL((k_bspline_decompose_lds<XX, TT>), grid, dim3(64), lds, s, fin, coef, lines, len,        \
+                     XX ? int64_t{1} : inner, XX ? len : inner * len, P)
+      if (d == 0) {
+        if (TL == 64) ANTS_DECOMP(true, 64);
+        else if (TL == 32) ANTS_DECOMP(true, 32);
+        else if (TL == 16) ANTS_DECOMP(true, 16);
+        else ANTS_DECOMP(true, 8);
+      } else {
+        if (TL == 64) ANTS_DECOMP(false, 64);
+        else if (TL == 32) ANTS_DECOMP(false, 32);
+        else if (TL == 16) ANTS_DECOMP(false, 16);
+        else ANTS_DECOMP(false, 8);
+      }
+#undef ANTS_DECOMP
+    }
+    converted = true;
+  }
+  if (!converted) hipLaunchKernelGGL(k_to_double, dim3(blocks_for(N)), dim3(kBlock), 0, s, img, coef, N);
 }
 
 void launch_labels_nn(const GridDev& go, const float* u, const GridDev& gl, const int32_t* labels, int32_t* out,

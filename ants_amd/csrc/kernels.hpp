@@ -283,10 +283,16 @@ struct ApplyDev {
   float defv;
   GeomDev ref, in;
   XformDev t[kMaxTransforms];
+  const double* coef;  // B-spline: the input's interpolation coefficients (double)
 };
-// interp: 0 linear, 1 nearest neighbour, 2 generic label (linear indicators)
+// interp: 0 linear, 1 nearest neighbour, 2 generic label (linear indicators),
+// 16 + q B-spline of order q (0..5) over P.coef
 void launch_apply_transforms(const GridDev& go, const ApplyDev& P, int interp, const float* in, float* out,
                              hipStream_t s);
+// BSplineDecompositionImageFilter: the B-spline interpolation coefficients of
+// img (dims n, x fastest) for `order` 0..5 into coef (double), mirror
+// boundaries, causal initialisation to tolerance 1e-10.
+void launch_bspline_coefficients(const int n[3], int D, const float* img, double* coef, int order, hipStream_t s);
 
 // ---- linear stages: metric sums over a level's virtual sample points ----
 constexpr int kLinThreads = 256;

@@ -237,9 +237,10 @@ def bench_serve(args):
 def bench_apply(args):
     """antsApplyTransforms subset: a 192x224x192 image (and a 32-label map)
     resampled onto the same grid through [displacement field, affine] with
-    Linear, NearestNeighbor and GenericLabel interpolation.  Algorithmic
-    bytes per output voxel: field 3*4 (read in place) + input 4 (gathered
-    once) + output 4."""
+    Linear, NearestNeighbor, GenericLabel and BSpline[3] interpolation (the
+    last including its coefficient decomposition).  Algorithmic bytes per
+    output voxel: field 3*4 (read in place) + input 4 (gathered once) +
+    output 4."""
     from scipy import ndimage
     from ants_amd import _lib as L
     from ants_amd import apply_transforms as AT
@@ -260,7 +261,8 @@ def bench_apply(args):
     bytes_per_voxel = 3 * 4 + 4 + 4
     res = {}
     e0, e1 = L.Event(), L.Event()
-    for interp, src in (("Linear", d_img), ("NearestNeighbor", d_lab), ("GenericLabel", d_lab)):
+    for interp, src in (("Linear", d_img), ("NearestNeighbor", d_lab), ("GenericLabel", d_lab),
+                        ("BSpline[3]", d_img)):
         for _ in range(2):
             AT.apply_transforms(g, src, g, xf, interp, out=d_out)
         L.check(L.lib().ants_device_synchronize())

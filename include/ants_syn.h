@@ -394,10 +394,15 @@ typedef struct ants_transform {
 
 /* Interpolators (-n, make_interpolator_snip.tmpl): Linear, NearestNeighbor
  * (half-integers round up), GenericLabel[Linear] (the label whose indicator
- * image interpolates highest; ties go to the smaller label). */
+ * image interpolates highest; ties go to the smaller label), BSpline[order]
+ * (Examples/make_interpolator_snip.tmpl:30-40: BSplineInterpolateImageFunction
+ * of order 0..5 over the input's BSplineDecompositionImageFilter coefficients,
+ * mirror boundaries; ANTS_INTERP_BSPLINE is the default order 3). */
 #define ANTS_INTERP_LINEAR 0
 #define ANTS_INTERP_NEAREST 1
 #define ANTS_INTERP_GENERIC_LABEL 2
+#define ANTS_INTERP_BSPLINE_ORDER(q) (16 + (q))
+#define ANTS_INTERP_BSPLINE ANTS_INTERP_BSPLINE_ORDER(3)
 
 /* Resample d_in (grid `in`) onto grid `ref` (ResampleImageFilter): each
  * reference point p is mapped through transforms[0], then transforms[1], ...
@@ -406,7 +411,9 @@ typedef struct ants_transform {
  * the input is interpolated there, or default_value where the point falls
  * outside the input buffer.  Coordinates are formed in double.  Directions
  * may be any orthonormal matrices.  Asynchronous on `stream`; the transform
- * array is read before the call returns. */
+ * array is read before the call returns.  B-spline interpolation
+ * holds the input's coefficients (8 bytes a voxel) in a stream-ordered
+ * allocation for the call's duration. */
 int ants_apply_transforms(const ants_image_desc* ref, const ants_image_desc* in, const float* d_in,
                           const ants_transform* transforms, int n_transforms, int interpolation,
                           double default_value, float* d_out, void* stream);

@@ -469,6 +469,9 @@ class OgXform(C.Structure):
 
 
 INTERP = {"linear": 0, "nearestneighbor": 1, "genericlabel": 2}
+for _q in range(6):
+    INTERP[f"bspline[{_q}]"] = 16 + _q
+INTERP["bspline"] = 16 + 3
 
 
 def og_geom(size, spacing=None, origin=None, direction=None) -> OgGeom:
@@ -528,5 +531,5 @@ def apply_transforms(ref: OgGeom, inp: OgGeom, img, transforms=(), interpolation
     rc = fn(C.byref(ref), C.byref(inp), a.ctypes.data_as(C.POINTER(C.c_float)), xs, len(transforms),
             INTERP[interpolation.lower()], float(default), out.ctypes.data_as(C.POINTER(C.c_float)))
     if rc != 0:
-        raise ValueError("dimension mismatch")
+        raise ValueError("dimension mismatch" if rc == -1 else "unknown interpolation")
     return out

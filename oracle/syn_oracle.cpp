@@ -1237,18 +1237,159 @@ void xform_point(const og_xform& T, int D, double p[3]) {
   }
 }
 
+// BSplineDecompositionImageFilter (ITK, absent from the reference tree; used
+// by BSplineInterpolateImageFunction, which antsApplyTransforms -n BSpline[q]
+// builds at Examples/make_interpolator_snip.tmpl:30-40): along each axis in
+// turn, every line is multiplied by the gain prod_k (1 - z_k)(1 - 1/z_k) of
+// the order's poles, then per pole filtered causally, c+[k] = c[k] +
+// z c+[k - 1], and anticausally, c-[k] = z (c-[k + 1] - c+[k]), with the
+// mirror-boundary initial values of Unser, Aldroubi & Eden (IEEE TSP 41(2),
+// 1993) as restated in Unser (IEEE SPM 16(6), 1999): the causal start is the
+// geometric sum over the mirrored line, truncated after ceil(ln 1e-10 /
+// ln|z|) terms when that is shorter than the line, and the anticausal start
+// c-[N-1] = z / (z^2 - 1) (z c+[N-2] + c+[N-1]).  Lines of length 1 are left
+// as they are.  Parity: pinned by test_oracle_apply.py against scipy's
+// mirror-mode spline_filter / map_coordinates (same recursion, independent
+// code), and by the interpolation property (the spline passes through the
+// samples).
+void bspline_poles(int q, double z[2], int* np) {
+  *np = 0;
+  if (q == 2) {
+    z[0] = std::sqrt(8.0) - 3.0;
+    *np = 1;
+  } else if (q == 3) {
+    z[0] = std::sqrt(3.0) - 2.0;
+    *np = 1;
+  } else if (q == 4) {
+    z[0] = std::sqrt(664.0 - std::sqrt(438976.0)) + std::sqrt(304.0) - 19.0;
+    z[1] = std::sqrt(664.0 + std::sqrt(438976.0)) - std::sqrt(304.0) - 19.0;
+    *np = 2;
+  } else if (q == 5) {
+    z[0] = std::sqrt(67.5 - std::sqrt(4436.25)) + std::sqrt(26.25) - 6.5;
+    z[1] = std::sqrt(67.5 + std::sqrt(4436.25)) - std::sqrt(26.25) - 6.5;
+    *np = 2;
+  }
+}
+
+void bspline_line(std::vector<double>& c, const double* z, int np) {
+  const int64_t N = static_cast<int64_t>(c.size());
+  if (N < 2 || np == 0) return;
+  double gain = 1.0;
+  for (int k = 0; k < np; ++k) gain *= (1.0 - z[k]) * (1.0 - 1.0 / z[k]);
+  for (double& v : c) v *= gain;
+  for (int k = 0; k < np; ++k) {
+    const double zk = z[k];
+    const int64_t horizon = static_cast<int64_t>(std::ceil(std::log(1e-10) / std::log(std::fabs(zk))));
+    double c0 = 0.0;
+    if (horizon < N) {
+      for (int64_t n = 0; n < horizon; ++n) c0 += std::pow(zk, static_cast<double>(n)) * c[n];
+    } else {
+      // full mirrored sum: sum_{n=0}^{2N-3} z^n c[mirror(n)] / (1 - z^(2N-2))
+      double acc = 0.0;
+      for (int64_t n = 0; n <= 2 * N - 3; ++n) {
+        const int64_t m = n < N ? n : 2 * (N - 1) - n;
+        acc += std::pow(zk, static_cast<double>(n)) * c[m];
+      }
+      c0 = acc / (1.0 - std::pow(zk, static_cast<double>(2 * N - 2)));
+    }
+    c[0] = c0;
+    for (int64_t n = 1; n < N; ++n) c[n] += zk * c[n - 1];
+    c[N - 1] = zk / (zk * zk - 1.0) * (zk * c[N - 2] + c[N - 1]);
+    for (int64_t n = N - 2; n >= 0; --n) c[n] = zk * (c[n + 1] - c[n]);
+  }
+}
+
+void bspline_coefficients(const og_geom& g, const float* img, int q, std::vector<double>& coef) {
+  const int D = g.dim;
+  const int64_t n[3] = {g.n[0], g.n[1], D == 3 ? g.n[2] : 1};
+  const int64_t N = n[0] * n[1] * n[2];
+  coef.assign(img, img + N);
+  double z[2];
+  int np;
+  bspline_poles(q, z, &np);
+  for (int d = 0; d < D; ++d) {
+    const int64_t len = n[d];
+    const int64_t stride = d == 0 ? 1 : (d == 1 ? n[0] : n[0] * n[1]);
+    std::vector<double> line(len);
+    for (int64_t b = 0; b < N / len; ++b) {
+      // the line through the voxel whose axis-d index is 0 and whose other
+      // indices enumerate b
+      const int64_t lo = b % stride, hi = b / stride;
+      const int64_t base = hi * stride * len + lo;
+      for (int64_t k = 0; k < len; ++k) line[k] = coef[base + k * stride];
+      bspline_line(line, z, np);
+      for (int64_t k = 0; k < len; ++k) coef[base + k * stride] = line[k];
+    }
+  }
+}
+
+// Centred B-spline of degree q at t by its truncated-power closed form,
+// beta_q(t) = 1/q! sum_{k=0}^{q+1} (-1)^k C(q+1, k) (t + (q+1)/2 - k)_+^q,
+// with degree 0 the indicator of [-1/2, 1/2).
+double bspline_beta(int q, double t) {
+  if (q == 0) return (t >= -0.5 && t < 0.5) ? 1.0 : 0.0;
+  double s = 0.0, binom = 1.0, fact = 1.0;
+  for (int k = 2; k <= q; ++k) fact *= k;
+  for (int k = 0; k <= q + 1; ++k) {
+    const double u = t + 0.5 * (q + 1) - k;
+    if (u > 0.0) s += ((k & 1) ? -binom : binom) * std::pow(u, q);
+    binom = binom * (q + 1 - k) / (k + 1);
+  }
+  return s / fact;
+}
+
+// BSplineInterpolateImageFunction::Evaluate at continuous index c: the
+// q + 1 coefficients per axis from floor(c) - q/2 (odd q) or floor(c + 1/2)
+// - q/2 (even q), mirror-reflected into the buffer (period 2(n - 1)),
+// weighted by beta_q(c - j).
+double bspline_evaluate(const og_geom& g, const std::vector<double>& coef, int q, const double c[3]) {
+  const int D = g.dim;
+  int64_t idx[3][6] = {};
+  double w[3][6] = {};
+  for (int d = 0; d < 3; ++d) {
+    if (d >= D) {
+      w[d][0] = 1.0;
+      continue;
+    }
+    const int64_t len = g.n[d];
+    const int64_t start = static_cast<int64_t>(std::floor((q % 2) ? c[d] : c[d] + 0.5)) - q / 2;
+    for (int k = 0; k <= q; ++k) {
+      int64_t j = start + k;
+      w[d][k] = bspline_beta(q, c[d] - static_cast<double>(j));
+      if (len == 1) {
+        j = 0;
+      } else {
+        while (j < 0 || j >= len) j = j < 0 ? -j : 2 * (len - 1) - j;
+      }
+      idx[d][k] = j;
+    }
+  }
+  double acc = 0.0;
+  const int qz = D == 3 ? q : 0;
+  for (int kz = 0; kz <= qz; ++kz)
+    for (int ky = 0; ky <= q; ++ky)
+      for (int kx = 0; kx <= q; ++kx)
+        acc += w[2][kz] * w[1][ky] * w[0][kx] * coef[idx[0][kx] + g.n[0] * (idx[1][ky] + g.n[1] * idx[2][kz])];
+  return acc;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
 // C entry points (ctypes).  `_f64` is the FP64 oracle, `_f32` the float model.
 extern "C" {
 
-// interp: 0 linear, 1 nearest neighbour, 2 generic label
+// interp: 0 linear, 1 nearest neighbour, 2 generic label, 16 + q B-spline
+// of order q (0..5)
 int og_apply_transforms_f64(const og_geom* ref, const og_geom* in, const float* img, const og_xform* xf, int nt,
                             int interp, double defv, float* out) {
   const int D = ref->dim;
   if (in->dim != D) return -1;
   const int64_t N = ref->n[0] * ref->n[1] * (D == 3 ? ref->n[2] : 1);
+  const int q = interp - 16;
+  std::vector<double> coef;
+  if (q >= 0 && q <= 5) bspline_coefficients(*in, img, q, coef);
+  else if (interp < 0 || interp > 2) return -2;
 #pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < N; ++i) {
     const int64_t x = i % ref->n[0], y = (i / ref->n[0]) % ref->n[1], z = i / (ref->n[0] * ref->n[1]);
@@ -1262,10 +1403,14 @@ int og_apply_transforms_f64(const og_geom* ref, const og_geom* in, const float* 
       out[i] = static_cast<float>(defv);
       continue;
     }
+    if (interp >= 16) {
+      out[i] = static_cast<float>(bspline_evaluate(*in, coef, q, c));
+      continue;
+    }
     if (interp == 1) {
-      int64_t q[3] = {0, 0, 0};
-      for (int d = 0; d < D; ++d) q[d] = static_cast<int64_t>(std::floor(c[d] + 0.5));
-      out[i] = img[q[0] + in->n[0] * (q[1] + in->n[1] * q[2])];
+      int64_t r[3] = {0, 0, 0};
+      for (int d = 0; d < D; ++d) r[d] = static_cast<int64_t>(std::floor(c[d] + 0.5));
+      out[i] = img[r[0] + in->n[0] * (r[1] + in->n[1] * r[2])];
       continue;
     }
     int64_t idx[8];

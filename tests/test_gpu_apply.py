@@ -1,7 +1,8 @@
 """antsApplyTransforms subset on the GPU against the oracle's restatement:
 composite affine + displacement-field transforms with directions and
 non-dyadic spacings; Linear within 1e-6, NearestNeighbor and GenericLabel
-bit-exact (coordinates are formed in double in the same order on both)."""
+bit-exact (coordinates are formed in double in the same order on both),
+BSpline[0..5] within 2e-6 of the image range."""
 import numpy as np
 import pytest
 from scipy import ndimage
@@ -69,6 +70,56 @@ def test_apply_transforms_matches_oracle(D, interp):
         np.testing.assert_array_equal(out, ref)
 
 
+@pytest.mark.parametrize("D", [2, 3])
+@pytest.mark.parametrize("q", [0, 1, 2, 3, 4, 5])
+def test_apply_bspline_matches_oracle(D, q):
+    """-n BSpline[q]: coefficients by the recursive decomposition (double),
+    evaluation by Cox-de Boor weights on the GPU against the oracle's
+    truncated-power weights; float output within 2e-6 of the image range
+    (order 0 is a pure selection, compared exactly)."""
+    gi, gr, img, xf, ox = case(D, "Linear")
+    out = AT.apply_transforms(gr, img, gi, xf, f"BSpline[{q}]", default_value=-7.0)
+    ref = o.apply_transforms(og(gr), og(gi), img, ox, f"bspline[{q}]", -7.0)
+    inside = ref != -7.0
+    assert 0.2 < inside.mean() < 1.0
+    np.testing.assert_array_equal(out == -7.0, ~inside)
+    err = np.abs(out - ref).max()
+    print(f"\n{D}-D BSpline[{q}] max |gpu - oracle| = {err:.3e}")
+    if q == 0:
+        np.testing.assert_array_equal(out, ref)
+    else:
+        assert err <= 2e-6 * np.abs(img).max()
+
+
+def test_apply_bspline_default_order_and_thin_axis():
+    """BSpline without an order is order 3; an axis of length 1 (a single
+    slice) and one of length 2 take the decomposition's short-line paths."""
+    gi = AT.Geometry((30, 2, 1), (1.0, 1.0, 1.0), (0.0, 0.0, 0.0))
+    img = rng.standard_normal(gi.shape).astype(np.float32)
+    xf = [AT.AffineTransform(np.eye(3), [0.3, 0.2, 0.0], [0.0, 0.0, 0.0])]
+    ox = [("affine", np.eye(3), np.array([0.3, 0.2, 0.0]), np.zeros(3))]
+    a = AT.apply_transforms(gi, img, gi, xf, "BSpline")
+    b = AT.apply_transforms(gi, img, gi, xf, "BSpline[3]")
+    ref = o.apply_transforms(og(gi), og(gi), img, ox, "bspline[3]")
+    np.testing.assert_array_equal(a, b)
+    np.testing.assert_allclose(a, ref, rtol=0, atol=2e-6 * np.abs(img).max())
+
+
+@pytest.mark.parametrize("q", [3, 5])
+def test_apply_bspline_long_lines(q):
+    """Lines of 1100 samples exceed an LDS tile and take the one-thread-per-
+    line decomposition; the short axis stays tiled."""
+    gi = AT.Geometry((1100, 5), (1.0, 1.0), (0.0, 0.0))
+    img = ndimage.gaussian_filter(rng.standard_normal(gi.shape), 2.0).astype(np.float32)
+    A = np.array([[0.97, 0.02], [0.0, 1.0]])
+    xf = [AT.AffineTransform(A, [3.3, 0.3], [550.0, 2.0])]
+    ox = [("affine", A, np.array([3.3, 0.3]), np.array([550.0, 2.0]))]
+    out = AT.apply_transforms(gi, img, gi, xf, f"BSpline[{q}]", default_value=-7.0)
+    ref = o.apply_transforms(og(gi), og(gi), img, ox, f"bspline[{q}]", -7.0)
+    assert (ref != -7.0).mean() > 0.5
+    assert np.abs(out - ref).max() <= 2e-6 * np.abs(img).max()
+
+
 def test_inverse_affine_round_trip():
     """-t affine then -t [affine,1] maps every point back onto itself."""
     gi, gr, img, _, _ = case(3, "NearestNeighbor")
@@ -81,8 +132,9 @@ def test_inverse_affine_round_trip():
 
 def test_rejects_bad_arguments():
     gi, gr, img, _, _ = case(2, "Linear")
-    with pytest.raises(ValueError):
-        AT.parse_interpolation("BSpline[3]")
+    for bad in ("BSpline[6]", "BSpline[x]", "Gaussian"):
+        with pytest.raises(ValueError):
+            AT.parse_interpolation(bad)
     bad = AT.Geometry(gi.size, gi.spacing, gi.origin, np.array([[1.0, 0.3], [0.0, 1.0]]))
     with pytest.raises(Exception):
         AT.apply_transforms(gr, img, bad, [], "Linear")

@@ -104,3 +104,17 @@ def test_tuning_table_set_get_load(tmp_path):
         for (k, b), val in saved.items():
             if val:
                 L.check(lib.ants_tuning_set(k, b, val))
+
+
+def test_parse_interpolation_grammar():
+    """-n names (make_interpolator_snip.tmpl): case-insensitive, BSpline's
+    order 0..5 defaulting to 3, GenericLabel with Linear only."""
+    from ants_amd.apply_transforms import parse_interpolation
+    assert parse_interpolation("Linear") == 0
+    assert parse_interpolation("nearestNeighbor") == 1
+    assert parse_interpolation("GenericLabel[Linear]") == 2
+    assert parse_interpolation("BSpline") == 16 + 3
+    assert [parse_interpolation(f"BSpline[{q}]") for q in range(6)] == [16 + q for q in range(6)]
+    for bad in ("BSpline[6]", "BSpline[-1]", "BSpline[2.5]", "GenericLabel[NearestNeighbor]", "Gaussian", "Sinc[3]"):
+        with pytest.raises(ValueError):
+            parse_interpolation(bad)

@@ -335,12 +335,16 @@ def test_shrink_grid_closed_forms():
     assert g.spacing[2] == 3.0  # |3*1 - 2| = 1 ties |3*2 - 2|? no: 4 > 1 -> factor 1
 
 
-def test_preprocess_window():
-    x = rng.uniform(-3, 5, 1000)
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_preprocess_window(seed):
+    """Window [lo, hi] in float (lo = the minimum rounded to float, which may
+    round above it: samples below lo clamp to 0), rescaled to [0, 1)."""
+    x = np.random.default_rng(seed).uniform(-3, 5, 1000)
     out = o.preprocess(x)
     lo = np.float32(x.min())
     hi = np.float32(x.max() + (x.max() - x.min()) / 2560.0)
-    np.testing.assert_allclose(out, (x - float(lo)) / (float(hi) - float(lo)), atol=1e-12)
+    want = np.clip((x - float(lo)) / (float(hi) - float(lo)), 0.0, 1.0)
+    np.testing.assert_allclose(out, want, atol=1e-12)
     assert out.min() >= 0 and out.max() < 1
 
 

@@ -151,3 +151,66 @@ def test_generic_label_matches_brute_force():
     g2 = o.og_geom((2, 1, 1))
     half = o.apply_transforms(g2, g2, two, [("affine", np.eye(3), [0.5, 0, 0], [0, 0, 0])], "genericlabel", -1)
     assert half[0, 0, 0] == 0.0
+
+
+def _bspline_case(D, seed):
+    r = np.random.default_rng(seed)
+    size = (23, 19, 17)[:D]
+    img = ndimage.gaussian_filter(r.standard_normal(tuple(reversed(size))), 1.0).astype(np.float32)
+    g = o.og_geom(size, (1.0,) * D, (0.0,) * D)
+    return r, size, img, g
+
+
+@pytest.mark.parametrize("D", [2, 3])
+@pytest.mark.parametrize("q", [0, 1, 2, 3, 4, 5])
+def test_bspline_matches_scipy_mirror_spline(D, q):
+    """-n BSpline[q] through an affine with rotation and scaling equals
+    scipy's map_coordinates(order=q, mode='mirror') -- an independent
+    implementation of the same mirror-boundary B-spline interpolation -- at
+    every point inside the buffer, edges included; outside gets the default."""
+    r, size, img, g = _bspline_case(D, 40 + q)
+    A = np.eye(D) * 1.05 + 0.06 * r.standard_normal((D, D))
+    t = 1.5 + 0.5 * r.standard_normal(D)
+    c = np.asarray(size, float) / 2
+    out = o.apply_transforms(g, g, img, [("affine", A, t, c)], f"bspline[{q}]", -5.0)
+    pts = points(size, (1.0,) * D, (0.0,) * D, np.eye(D))
+    mapped = (pts - c) @ A.T + t + c
+    ci = cidx(mapped, size, (1.0,) * D, (0.0,) * D, np.eye(D))
+    ins = inside(ci, size)
+    coords = [ci[..., d] for d in reversed(range(D))]
+    ref = ndimage.map_coordinates(img.astype(np.float64), coords, order=q, mode="mirror")
+    assert 0.3 < ins.mean() < 1.0
+    if q == 0:
+        # nearest sample, half-integers rounding up (scipy may round them
+        # the other way, so compare away from ties)
+        tie = np.any(np.abs(ci - np.floor(ci) - 0.5) < 1e-9, axis=-1)
+        m = ins & ~tie
+        np.testing.assert_array_equal(out[m], ref[m].astype(np.float32))
+    else:
+        np.testing.assert_allclose(out[ins], ref[ins], rtol=0, atol=2e-6 * np.abs(img).max())
+    assert np.all(out[~ins] == -5.0)
+
+
+@pytest.mark.parametrize("q", [2, 3, 4, 5])
+def test_bspline_interpolates_samples(q):
+    """At voxel centres the spline passes through the samples, whatever the
+    order (a wrong gain or pole leaves a smoothed image)."""
+    _, size, img, g = _bspline_case(3, 7)
+    out = o.apply_transforms(g, g, img, [], f"bspline[{q}]")
+    np.testing.assert_allclose(out, img, rtol=0, atol=1e-6 * np.abs(img).max())
+
+
+def test_bspline_reproduces_polynomials():
+    """A cubic spline reproduces polynomials of degree <= 3 away from the
+    mirror boundary exactly, so a quadratic ramp is interpolated exactly in
+    the interior at any sub-voxel shift."""
+    size = (40, 36)
+    y, x = np.meshgrid(np.arange(36.0), np.arange(40.0), indexing="ij")
+    img = (0.01 * (x - 20) ** 2 - 0.02 * (y - 15) * (x - 20) + 0.3 * y).astype(np.float32)
+    g = o.og_geom(size, (1.0, 1.0), (0.0, 0.0))
+    shift = np.array([0.37, -0.61])
+    out = o.apply_transforms(g, g, img, [("affine", np.eye(2), shift, np.zeros(2))], "bspline[3]")
+    xs, ys = x + shift[0], y + shift[1]
+    want = 0.01 * (xs - 20) ** 2 - 0.02 * (ys - 15) * (xs - 20) + 0.3 * ys
+    core = (slice(8, -8), slice(8, -8))
+    np.testing.assert_allclose(out[core], want[core], rtol=0, atol=2e-5)
