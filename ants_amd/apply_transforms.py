@@ -147,6 +147,29 @@ def compose_affines(transforms: Sequence[Optional[AffineTransform]]) -> AffineTr
     return AffineTransform(A, off, np.zeros(A.shape[0]))
 
 
+def compose_transforms_to_field(reference: Geometry, transforms: Sequence[Transform] = (), out=None):
+    """antsApplyTransforms -o [field,1] (Examples/antsApplyTransforms.cxx:706-760):
+    the displacement T(p) - p of every reference point through the
+    transforms in -t order, shape reference.shape + (dim,), physical
+    components.  Returns a numpy array when ``out`` is None, else ``out``."""
+    D = reference.dim
+    xs, keep = transform_array(transforms, D)
+    d_out = out if out is not None else L.DeviceArray(reference.shape + (D,))
+    L.check(L.lib().ants_compose_transforms_to_field(C.byref(reference.desc()), xs, len(transforms), d_out.ptr, None))
+    return d_out if out is not None else d_out.to_host()
+
+
+def collapse_linear(transforms: Sequence[Transform], D: int) -> "AffineTransform":
+    """antsApplyTransforms -o Linear[file] (Examples/antsApplyTransforms.cxx:647-688):
+    the single affine equal to the -t list of affine (and identity)
+    transforms; a displacement field among them is refused, as there."""
+    if any(t is not None and not isinstance(t, AffineTransform) for t in transforms):
+        raise ValueError("the transform or set of transforms is not linear")
+    if all(t is None for t in transforms):
+        return AffineTransform(np.eye(D), np.zeros(D), np.zeros(D))
+    return compose_affines(transforms)
+
+
 def transform_array(transforms: Sequence[Transform], D: int):
     """ants_transform array for the C-ABI, plus the device buffers it refers
     to (keep them alive while the array is in use)."""

@@ -33,7 +33,13 @@ or ``-t BSplineExponential[0.25,26,0,0,3]`` with ``-m CC[...]``: it writes
 is the antsApplyTransforms subset (Examples/antsApplyTransforms.cxx).
 Transforms ending in .mat/.txt/.tfm are affine files, .h5 files ITK
 composite transforms (their components in the order they apply); other paths
-are displacement-field images.
+are displacement-field images.  ``-o [field.nii.gz,1]`` (or
+``DisplacementField[field.nii.gz]``; ``.h5`` writes it as a transform) writes
+the composite as one displacement field on the reference grid,
+``-o Linear[affine.mat(,1)]`` the affine list collapsed into one (inverted
+with 1) and ``-o CompositeTransform[out.h5]`` the list as a composite file;
+those need no input image.  ``-n BSpline[3]`` selects B-spline
+interpolation of order 0..5.
 
 ``registration ... -a 1`` writes ``out_Composite.h5`` and
 ``out_InverseComposite.h5`` (--write-composite-transform) instead of the
@@ -47,7 +53,7 @@ from __future__ import annotations
 import argparse
 import re
 import sys
-from typing import List, Optional, Sequence
+from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -106,27 +112,76 @@ def load_transform(spec: str):
     return DisplacementFieldTransform(np.asarray(fld, np.float32), g)
 
 
+def parse_apply_output(spec: str) -> Tuple[str, str, bool]:
+    """-o forms of antsApplyTransforms (Examples/antsApplyTransforms.cxx:643-770):
+    ("image", path, _) for `out` / `[out,0]`; ("field", path, _) for
+    `[out,1]` / `DisplacementField[out]`; ("linear", path, invert) for
+    `Linear[out(,1)]`; ("composite", path, _) for `CompositeTransform[out]`."""
+    m = re.fullmatch(r"\s*([A-Za-z]*)\s*\[(.*)\]\s*", spec)
+    if not m:
+        return "image", spec.strip(), False
+    name, parts = m.group(1).lower(), [p.strip() for p in m.group(2).split(",")]
+    flag = len(parts) > 1 and parts[1] not in ("", "0")
+    if name == "linear":
+        return "linear", parts[0], flag
+    if name == "compositetransform":
+        return "composite", parts[0], False
+    if name == "displacementfield" or (name == "" and flag):
+        return "field", parts[0], False
+    if name == "":
+        return "image", parts[0], False
+    raise ValueError(f"unknown output form {spec!r}")
+
+
 def cmd_apply(argv: Sequence[str]) -> int:
+    from .apply_transforms import collapse_linear, compose_transforms_to_field
     ap = argparse.ArgumentParser(prog="apply")
     ap.add_argument("-d", "--dimensionality", type=int, default=None)
-    ap.add_argument("-i", "--input", required=True)
-    ap.add_argument("-r", "--reference-image", required=True)
+    ap.add_argument("-i", "--input", default=None)
+    ap.add_argument("-r", "--reference-image", default=None)
     ap.add_argument("-o", "--output", required=True)
     ap.add_argument("-t", "--transform", action="append", default=[])
     ap.add_argument("-n", "--interpolation", default="Linear")
     ap.add_argument("--default-value", type=float, default=0.0)
     ap.add_argument("-v", "--verbose", type=int, default=0)
     a = ap.parse_args(argv)
-    img, gi = aio.read_nifti(a.input)
+    kind, path, invert = parse_apply_output(a.output)
+    xf = [x for t in a.transform for x in load_transforms(t)]
+    if kind == "linear":
+        D = a.dimensionality or (np.asarray(xf[0].matrix).shape[0] if xf and xf[0] is not None else 3)
+        t = collapse_linear(xf, D)
+        A, tr, c = t.parameters()
+        if invert:
+            t = AffineTransform(A, tr, c, invert=True)
+            A, tr, c = t.parameters()
+        aio.write_transform(path, AffineTransform(A, tr, c))
+        return 0
+    if kind == "composite":
+        from .h5transform import write_composite
+        # queue order: the transform applied first is added last
+        write_composite(path, [t for t in reversed(xf) if t is not None])
+        return 0
+    if not a.reference_image:
+        raise SystemExit("-r is required")
     ref, gr = aio.read_nifti(a.reference_image)
+    if kind == "field":
+        fld = compose_transforms_to_field(gr, xf)
+        if path.endswith((".h5", ".hdf5")):
+            from .h5transform import write_composite
+            write_composite(path, [DisplacementFieldTransform(fld, gr)])
+        else:
+            aio.write_nifti(path, fld, gr, vector=True)
+        return 0
+    if not a.input:
+        raise SystemExit("Output is not a transform, and there is no input image")
+    img, gi = aio.read_nifti(a.input)
     if a.dimensionality and a.dimensionality != gi.dim:
         raise SystemExit(f"-d {a.dimensionality} but the input is {gi.dim}-D")
-    xf = [x for t in a.transform for x in load_transforms(t)]
     out = apply_transforms(gr, np.asarray(img, np.float32), gi, xf, a.interpolation, a.default_value)
     if a.interpolation.lower().startswith(("nearest", "genericlabel")) and np.issubdtype(np.asarray(img).dtype,
                                                                                          np.integer):
         out = out.astype(np.asarray(img).dtype)
-    aio.write_nifti(a.output, out, gr)
+    aio.write_nifti(path, out, gr)
     return 0
 
 

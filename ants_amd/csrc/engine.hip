@@ -2740,6 +2740,26 @@ extern "C" int ants_apply_transforms(const ants_image_desc* ref, const ants_imag
   return ANTS_OK;
 }
 
+extern "C" int ants_compose_transforms_to_field(const ants_image_desc* ref, const ants_transform* transforms,
+                                                int n_transforms, float* d_out, void* stream) {
+  ApplyDev P;
+  std::memset(&P, 0, sizeof(P));
+  ANTS_TRY(geom_from_desc(ref, &P.ref));
+  const int D = P.ref.D;
+  if (!d_out) return fail(ANTS_ERR_INVALID, "null output buffer");
+  if (n_transforms < 0 || n_transforms > kMaxTransforms || (n_transforms > 0 && !transforms))
+    return fail(ANTS_ERR_INVALID, "transform count must be 0..%d", kMaxTransforms);
+  P.nt = n_transforms;
+  for (int t = 0; t < n_transforms; ++t) ANTS_TRY(xform_to_dev(transforms[t], D, &P.t[t]));
+  GridDev go{};
+  go.D = D;
+  for (int k = 0; k < 3; ++k) go.n[k] = P.ref.n[k];
+  set_divisors(go);
+  launch_compose_field(go, P, d_out, static_cast<hipStream_t>(stream));
+  HIP_TRY(hipGetLastError());
+  return ANTS_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Batched / served registration.
 namespace {

@@ -1991,17 +1991,11 @@ __device__ __forceinline__ int64_t mirror_index(int64_t j, int64_t len) {
   return j >= len ? period - j : j;
 }
 
-template <int D, int INTERP>
-__global__ void __launch_bounds__(kBlock) k_apply_transforms(GridDev go, ApplyDev P, const float* __restrict__ img,
-                                                             float* __restrict__ out) {
-  const int64_t N = go.count();
-  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-  if (i >= N) return;
-  int x, y, z;
-  unlin(go, i, x, y, z);
-  const int ix[3] = {x, y, z};
-  double p[3];
-  geom_point<D>(P.ref, ix, p);
+// A physical point through the transform list in order: affine
+// y = A p + off, or displacement field y = p + u(p) (u linearly
+// interpolated in the field's grid, zero outside it).
+template <int D>
+__device__ __forceinline__ void apply_chain(const ApplyDev& P, double p[3]) {
   for (int t = 0; t < P.nt; ++t) {
     const XformDev& T = P.t[t];
     if (T.kind == 1) {
@@ -2035,6 +2029,39 @@ __global__ void __launch_bounds__(kBlock) k_apply_transforms(GridDev go, ApplyDe
       }
     }
   }
+}
+
+// TransformToDisplacementFieldFilter: the displacement T(p) - p of each
+// reference point p, D interleaved physical components per voxel.
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_compose_field(GridDev go, ApplyDev P, float* __restrict__ out) {
+  const int64_t N = go.count();
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i >= N) return;
+  int x, y, z;
+  unlin(go, i, x, y, z);
+  const int ix[3] = {x, y, z};
+  double p0[3], p[3];
+  geom_point<D>(P.ref, ix, p0);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) p[d] = p0[d];
+  apply_chain<D>(P, p);
+#pragma unroll
+  for (int d = 0; d < D; ++d) out[i * D + d] = static_cast<float>(p[d] - p0[d]);
+}
+
+template <int D, int INTERP>
+__global__ void __launch_bounds__(kBlock) k_apply_transforms(GridDev go, ApplyDev P, const float* __restrict__ img,
+                                                             float* __restrict__ out) {
+  const int64_t N = go.count();
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i >= N) return;
+  int x, y, z;
+  unlin(go, i, x, y, z);
+  const int ix[3] = {x, y, z};
+  double p[3];
+  geom_point<D>(P.ref, ix, p);
+  apply_chain<D>(P, p);
   double c[3];
   geom_cidx<D>(P.in, p, c);
   if (!geom_inside<D>(P.in, c)) {
@@ -3638,6 +3665,14 @@ void launch_apply_transforms(const GridDev& go, const ApplyDev& P, int interp, c
   }
 #undef ANTS_APPLY_DIM
 #undef ANTS_APPLY_LAUNCH
+}
+
+void launch_compose_field(const GridDev& go, const ApplyDev& P, float* out, hipStream_t s) {
+  const dim3 grid(blocks_for(go.count()));
+  if (go.D == 2)
+    hipLaunchKernelGGL(k_compose_field<2>, grid, dim3(kBlock), 0, s, go, P, out);
+  else
+    hipLaunchKernelGGL(k_compose_field<3>, grid, dim3(kBlock), 0, s, go, P, out);
 }
 
 void launch_bspline_coefficients(const int n[3], int D, const float* img, double* coef, int order, hipStream_t s) {

@@ -289,6 +289,9 @@ struct ApplyDev {
 // 16 + q B-spline of order q (0..5) over P.coef
 void launch_apply_transforms(const GridDev& go, const ApplyDev& P, int interp, const float* in, float* out,
                              hipStream_t s);
+// TransformToDisplacementFieldFilter: T(p) - p for every point p of grid go
+// (P.ref), T = P's transform list; D interleaved components per voxel.
+void launch_compose_field(const GridDev& go, const ApplyDev& P, float* out, hipStream_t s);
 // BSplineDecompositionImageFilter: the B-spline interpolation coefficients of
 // img (dims n, x fastest) for `order` 0..5 into coef (double), mirror
 // boundaries, causal initialisation to tolerance 1e-10.

@@ -418,6 +418,15 @@ int ants_apply_transforms(const ants_image_desc* ref, const ants_image_desc* in,
                           const ants_transform* transforms, int n_transforms, int interpolation,
                           double default_value, float* d_out, void* stream);
 
+/* antsApplyTransforms -o [field,1] / -o DisplacementField[field]
+ * (Examples/antsApplyTransforms.cxx:706-760, TransformToDisplacementFieldFilter):
+ * d_out[i] = T(p_i) - p_i for every point p_i of grid `ref`, T the transform
+ * list applied in order as in ants_apply_transforms; dim interleaved
+ * physical components per voxel (the displacement-field layout).
+ * Asynchronous on `stream`. */
+int ants_compose_transforms_to_field(const ants_image_desc* ref, const ants_transform* transforms,
+                                     int n_transforms, float* d_out, void* stream);
+
 /* Replaces SyNImageRegistrationMethod::SetMovingInitialTransform and
  * SetFixedInitialTransform (Examples/itkantsRegistrationHelper.hxx:2119-2137,
  * 3583-3633): the composite of -r / -q initial transforms and the preceding

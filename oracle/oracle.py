@@ -533,3 +533,17 @@ def apply_transforms(ref: OgGeom, inp: OgGeom, img, transforms=(), interpolation
     if rc != 0:
         raise ValueError("dimension mismatch" if rc == -1 else "unknown interpolation")
     return out
+
+
+def compose_field(ref: OgGeom, transforms=()):
+    """T(p) - p at every point of `ref` (transforms as in apply_transforms);
+    float32, shape (z, y, x, D) in numpy order."""
+    D = ref.dim
+    xs, keep = _xform_array(transforms, D)
+    shape = tuple(int(ref.n[k]) for k in range(D))[::-1] + (D,)
+    out = np.empty(shape, np.float32)
+    fn = lib().og_compose_field_f64
+    fn.restype = C.c_int
+    fn.argtypes = [C.POINTER(OgGeom), C.POINTER(OgXform), C.c_int, C.POINTER(C.c_float)]
+    fn(C.byref(ref), xs, len(transforms), out.ctypes.data_as(C.POINTER(C.c_float)))
+    return out

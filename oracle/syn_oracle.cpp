@@ -1443,6 +1443,25 @@ int og_apply_transforms_f64(const og_geom* ref, const og_geom* in, const float* 
   return 0;
 }
 
+// TransformToDisplacementFieldFilter (ITK; called at
+// Examples/antsApplyTransforms.cxx:716-724): T(p) - p at every point p of
+// `ref`, T the transform list applied in order; dim interleaved components.
+int og_compose_field_f64(const og_geom* ref, const og_xform* xf, int nt, float* out) {
+  const int D = ref->dim;
+  const int64_t N = ref->n[0] * ref->n[1] * (D == 3 ? ref->n[2] : 1);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < N; ++i) {
+    const int64_t x = i % ref->n[0], y = (i / ref->n[0]) % ref->n[1], z = i / (ref->n[0] * ref->n[1]);
+    const double id[3] = {static_cast<double>(x), static_cast<double>(y), static_cast<double>(z)};
+    double p0[3], p[3];
+    geom_index_to_point(*ref, id, p0);
+    for (int d = 0; d < 3; ++d) p[d] = p0[d];
+    for (int t = 0; t < nt; ++t) xform_point(xf[t], D, p);
+    for (int d = 0; d < D; ++d) out[i * D + d] = static_cast<float>(p[d] - p0[d]);
+  }
+  return 0;
+}
+
 int og_gaussian_kernel(double variance, double max_error, int64_t max_width, double* out, int max_out) {
   std::vector<double> h = gaussian_half_kernel(variance, max_error, max_width);
   const int n = static_cast<int>(h.size());

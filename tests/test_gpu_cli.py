@@ -203,3 +203,48 @@ def test_registration_initial_transform_from_images(tmp_path):
     np.testing.assert_allclose(aff.matrix @ (p - aff.center) + aff.translation + aff.center, A @ (p - c) + t + c,
                                atol=1e-5)
     assert np.abs(t).max() > 1.0
+
+
+def test_apply_transform_outputs(tmp_path):
+    """apply -o [field,1] / DisplacementField[x.h5] / CompositeTransform[x.h5]:
+    the written field equals compose_transforms_to_field of the same list,
+    the .h5 forms read back to the same field and transforms, and resampling
+    through either equals resampling through the list."""
+    from ants_amd.apply_transforms import AffineTransform, compose_transforms_to_field
+    from ants_amd.h5transform import read_composite
+    rng = np.random.default_rng(8)
+    g = Geometry((30, 28, 24), (1.2, 1.0, 1.1), (2.0, -3.0, 1.0), np.eye(3))
+    ref = str(tmp_path / "ref.nii.gz")
+    img = rng.standard_normal(g.shape).astype(np.float32)
+    aio.write_nifti(ref, img, g)
+    fld = (0.8 * rng.standard_normal(g.shape + (3,))).astype(np.float32)
+    fpath, apath = str(tmp_path / "w.nii.gz"), str(tmp_path / "a.mat")
+    aio.write_nifti(fpath, fld, g, vector=True)
+    aff = AffineTransform(np.eye(3) * 1.02, [0.5, -0.4, 0.3], [15.0, 12.0, 10.0])
+    aio.write_transform(apath, aff)
+    # the geometries as the files hold them (NIfTI stores float32 headers)
+    _, g = aio.read_nifti(ref)
+    xf = cli.load_transforms(fpath) + cli.load_transforms(apath)
+    want = compose_transforms_to_field(g, xf)
+    out = str(tmp_path / "composite_field.nii.gz")
+    assert cli.main(["apply", "-d", "3", "-r", ref, "-o", f"[{out},1]", "-t", fpath, "-t", apath]) == 0
+    got, gg = aio.read_nifti(out)
+    np.testing.assert_array_equal(got, want)
+    assert gg.size == g.size
+    h5 = str(tmp_path / "composite_field.h5")
+    assert cli.main(["apply", "-d", "3", "-r", ref, "-o", f"DisplacementField[{h5}]", "-t", fpath, "-t", apath]) == 0
+    (t,) = read_composite(h5)
+    np.testing.assert_array_equal(np.asarray(t.field), want)
+    comp = str(tmp_path / "composite.h5")
+    assert cli.main(["apply", "-d", "3", "-o", f"CompositeTransform[{comp}]", "-t", fpath, "-t", apath]) == 0
+    warped = str(tmp_path / "warped.nii.gz")
+    assert cli.main(["apply", "-d", "3", "-i", ref, "-r", ref, "-o", warped, "-t", comp]) == 0
+    a, _ = aio.read_nifti(warped)
+    # the composite lists the same transforms in the same order (its affine
+    # parameters stored as float, as --float 1 writes them)
+    xc = cli.load_transforms(comp)
+    assert [type(t) for t in xc] == [type(t) for t in xf]
+    np.testing.assert_array_equal(np.asarray(xc[0].field), np.asarray(xf[0].field))
+    for u, v in zip(xc[1].parameters(), xf[1].parameters()):
+        np.testing.assert_allclose(u, v, rtol=1e-6, atol=1e-6)
+    np.testing.assert_array_equal(a, apply_transforms(g, img, g, xc))

@@ -118,3 +118,51 @@ def test_parse_interpolation_grammar():
     for bad in ("BSpline[6]", "BSpline[-1]", "BSpline[2.5]", "GenericLabel[NearestNeighbor]", "Gaussian", "Sinc[3]"):
         with pytest.raises(ValueError):
             parse_interpolation(bad)
+
+
+def test_apply_output_forms():
+    """antsApplyTransforms -o grammar (Examples/antsApplyTransforms.cxx:643-770)."""
+    from ants_amd.cli import parse_apply_output
+    assert parse_apply_output("out.nii.gz") == ("image", "out.nii.gz", False)
+    assert parse_apply_output("[out.nii.gz,0]") == ("image", "out.nii.gz", False)
+    assert parse_apply_output("[warp.nii.gz,1]") == ("field", "warp.nii.gz", False)
+    assert parse_apply_output("DisplacementField[warp.nii.gz]") == ("field", "warp.nii.gz", False)
+    assert parse_apply_output("Linear[a.mat]") == ("linear", "a.mat", False)
+    assert parse_apply_output("Linear[a.mat,1]") == ("linear", "a.mat", True)
+    assert parse_apply_output("CompositeTransform[c.h5]") == ("composite", "c.h5", False)
+    with pytest.raises(ValueError):
+        parse_apply_output("Gaussian[x.nii.gz]")
+
+
+def test_apply_collapse_linear(tmp_path):
+    """-o Linear[file(,1)] writes the -t affines collapsed into one (and its
+    inverse): the written transform maps points as the list does; a
+    displacement field in the list is refused."""
+    from ants_amd import io as aio
+    from ants_amd.apply_transforms import AffineTransform, DisplacementFieldTransform, Geometry, collapse_linear
+    from ants_amd.cli import cmd_apply
+    rng = np.random.default_rng(3)
+    ts = [AffineTransform(np.eye(3) + 0.1 * rng.standard_normal((3, 3)), rng.standard_normal(3), rng.standard_normal(3))
+          for _ in range(2)]
+    paths = []
+    for k, t in enumerate(ts):
+        paths.append(str(tmp_path / f"t{k}.mat"))
+        aio.write_transform(paths[-1], t)
+    pts = rng.standard_normal((20, 3)) * 10
+
+    def apply(t, p):
+        A, tr, c = t.parameters()
+        return (p - c) @ A.T + tr + c
+
+    want = apply(ts[1], apply(ts[0], pts))
+    for inv in (0, 1):
+        out = str(tmp_path / f"c{inv}.mat")
+        assert cmd_apply(["-d", "3", "-o", f"Linear[{out},{inv}]", "-t", paths[0], "-t", paths[1]]) == 0
+        got = apply(aio.read_transform(out), pts)
+        if inv:
+            np.testing.assert_allclose(apply(aio.read_transform(out), want), pts, atol=1e-9)
+        else:
+            np.testing.assert_allclose(got, want, atol=1e-9)
+    g = Geometry((4, 4, 4))
+    with pytest.raises(ValueError):
+        collapse_linear([ts[0], DisplacementFieldTransform(np.zeros((4, 4, 4, 3), np.float32), g)], 3)

@@ -214,3 +214,39 @@ def test_bspline_reproduces_polynomials():
     want = 0.01 * (xs - 20) ** 2 - 0.02 * (ys - 15) * (xs - 20) + 0.3 * ys
     core = (slice(8, -8), slice(8, -8))
     np.testing.assert_allclose(out[core], want[core], rtol=0, atol=2e-5)
+
+
+def test_compose_field_affine_closed_form():
+    """-o [field,1] of an affine: A (p - c) + t + c - p at every reference
+    point (oblique, anisotropic reference grid)."""
+    r = np.random.default_rng(5)
+    size, sp, org, R = (9, 8, 7), (1.1, 0.9, 1.3), (2.0, -1.0, 0.5), rot3(0.2, -0.1, 0.3)
+    A = np.eye(3) + 0.05 * r.standard_normal((3, 3))
+    t, c = r.standard_normal(3), np.array([3.0, 1.0, -2.0])
+    g = o.og_geom(size, sp, org, R)
+    out = o.compose_field(g, [("affine", A, t, c)])
+    p = points(size, sp, org, R)
+    want = (p - c) @ A.T + t + c - p
+    np.testing.assert_allclose(out, want, rtol=0, atol=2e-6)
+    assert out.shape == (7, 8, 9, 3)
+    np.testing.assert_array_equal(o.compose_field(g, []), 0.0)
+
+
+def test_compose_field_displacement_and_order():
+    """A field on the reference grid itself comes back unchanged (its samples
+    are read at their own points); [field, affine] is the affine applied to
+    p + u(p); points the field grid does not cover pass unchanged."""
+    r = np.random.default_rng(6)
+    size = (10, 9, 8)
+    g = o.og_geom(size, (1.0, 1.0, 1.0), (0.0, 0.0, 0.0))
+    u = (0.3 * r.standard_normal((8, 9, 10, 3))).astype(np.float32)
+    np.testing.assert_allclose(o.compose_field(g, [("field", g, u)]), u, rtol=0, atol=1e-6)
+    A = np.array([[0.0, -1.0, 0.0], [1.0, 0.0, 0.0], [0.0, 0.0, 1.0]])
+    t, c = np.array([0.5, 0.0, -0.25]), np.zeros(3)
+    out = o.compose_field(g, [("field", g, u), ("affine", A, t, c)])
+    p = points(size, (1.0, 1.0, 1.0), (0.0, 0.0, 0.0), np.eye(3))
+    q = p + u
+    np.testing.assert_allclose(out, q @ A.T + t - p, rtol=0, atol=2e-6)
+    # a field grid shifted far away covers no reference point
+    gfar = o.og_geom(size, (1.0, 1.0, 1.0), (100.0, 0.0, 0.0))
+    np.testing.assert_array_equal(o.compose_field(g, [("field", gfar, u)]), 0.0)
