@@ -22,7 +22,10 @@ import numpy as np
 
 from . import _lib as L
 
-INTERPOLATORS = {"linear": 0, "nearestneighbor": 1, "genericlabel": 2}
+INTERPOLATORS = {"linear": 0, "nearestneighbor": 1, "genericlabel": 2,
+                 # WindowedSincInterpolateImageFunction, radius 3, zero boundary
+                 "cosinewindowedsinc": 32, "hammingwindowedsinc": 33, "lanczoswindowedsinc": 34,
+                 "blackmanwindowedsinc": 35, "welchwindowedsinc": 36}
 
 
 @dataclass
@@ -86,8 +89,9 @@ BSPLINE_DEFAULT_ORDER = 3  # BSplineInterpolateImageFunction's default
 
 
 def parse_interpolation(name: str) -> int:
-    """-n Linear | NearestNeighbor | GenericLabel[Linear] | BSpline[order]
-    (Examples/make_interpolator_snip.tmpl:18-40; BSpline's order is 0..5,
+    """-n Linear | NearestNeighbor | GenericLabel[Linear] | BSpline[order] |
+    {Cosine,Hamming,Lanczos,Blackman,Welch}WindowedSinc
+    (Examples/make_interpolator_snip.tmpl:18-110; BSpline's order is 0..5,
     3 when omitted)."""
     m = re.fullmatch(r"\s*([A-Za-z]+)\s*(?:\[\s*([A-Za-z0-9]*)\s*\])?\s*", name)
     if not m:
@@ -104,7 +108,8 @@ def parse_interpolation(name: str) -> int:
     if base == "genericlabel" and m.group(2) and m.group(2).lower() != "linear":
         raise ValueError("GenericLabel is supported with its default Linear interpolator only")
     if base not in INTERPOLATORS:
-        raise ValueError(f"interpolation {name!r} is not supported (Linear, NearestNeighbor, GenericLabel, BSpline)")
+        raise ValueError(f"interpolation {name!r} is not supported (Linear, NearestNeighbor, GenericLabel, BSpline, "
+                         "*WindowedSinc)")
     return INTERPOLATORS[base]
 
 

@@ -2711,7 +2711,9 @@ extern "C" int ants_apply_transforms(const ants_image_desc* ref, const ants_imag
     return fail(ANTS_ERR_INVALID, "transform count must be 0..%d", kMaxTransforms);
   const bool bspline =
       interpolation >= ANTS_INTERP_BSPLINE_ORDER(0) && interpolation <= ANTS_INTERP_BSPLINE_ORDER(5);
-  if (!bspline && (interpolation < ANTS_INTERP_LINEAR || interpolation > ANTS_INTERP_GENERIC_LABEL))
+  const bool sinc =
+      interpolation >= ANTS_INTERP_COSINE_WINDOWED_SINC && interpolation <= ANTS_INTERP_WELCH_WINDOWED_SINC;
+  if (!bspline && !sinc && (interpolation < ANTS_INTERP_LINEAR || interpolation > ANTS_INTERP_GENERIC_LABEL))
     return fail(ANTS_ERR_INVALID, "unknown interpolation %d", interpolation);
   P.nt = n_transforms;
   P.defv = static_cast<float>(default_value);

@@ -1994,6 +1994,54 @@ __device__ __forceinline__ int64_t mirror_index(int64_t j, int64_t len) {
 // A physical point through the transform list in order: affine
 // y = A p + off, or displacement field y = p + u(p) (u linearly
 // interpolated in the field's grid, zero outside it).
+// WindowedSincInterpolateImageFunction, radius m = 3: the kernel
+// window(x) * sinc(x), sinc(x) = sin(pi x) / (pi x); windows (Meijering,
+// Niessen & Viergever, Med. Image Anal. 3(2) 1999): 0 cosine
+// cos(pi x / 2m), 1 Hamming 0.54 + 0.46 cos(pi x / m), 2 Lanczos
+// sinc(x / m), 3 Blackman 0.42 + 0.5 cos(pi x / m) + 0.08 cos(2 pi x / m),
+// 4 Welch 1 - x^2 / m^2.
+constexpr int kSincRadius = 3;
+// The six weights at offsets x_k = dist + 2 - k (k = 0..5) for 0 < dist < 1,
+// from one sincos per angle instead of one per tap: sin(pi x_k) =
+// (-1)^k sin(pi dist), and the windows' cos / sin of pi x_k / 3 (pi x_k / 6
+// for the cosine window) by angle addition from x_0 with the constants
+// cos(k pi / 3), sin(k pi / 3) (and k pi / 6, 2 k pi / 3).
+template <int W>
+__device__ __forceinline__ void sinc_weights(double dist, double wt[6]) {
+  constexpr double pi = 3.14159265358979323846;
+  constexpr double h = 0.86602540378443864676;  // sin(pi / 3)
+  constexpr double C3[6] = {1.0, 0.5, -0.5, -1.0, -0.5, 0.5};
+  constexpr double S3[6] = {0.0, h, h, 0.0, -h, -h};
+  const double x0 = dist + 2.0;
+  const double s0 = sin(pi * dist);
+  double st, ct;  // sin, cos of pi x0 / 3 (pi x0 / 6 for the cosine window)
+  sincos(x0 * (W == 0 ? pi / 6.0 : pi / 3.0), &st, &ct);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const double xk = x0 - k;
+    const double sinc = ((k & 1) ? -s0 : s0) / (pi * xk);
+    double w;
+    if constexpr (W == 0) {
+      constexpr double C6[6] = {1.0, h, 0.5, 0.0, -0.5, -h};
+      constexpr double S6[6] = {0.0, 0.5, h, 1.0, h, 0.5};
+      w = ct * C6[k] + st * S6[k];  // cos(pi x_k / 6)
+    } else if constexpr (W == 1) {
+      w = 0.54 + 0.46 * (ct * C3[k] + st * S3[k]);
+    } else if constexpr (W == 2) {
+      w = (st * C3[k] - ct * S3[k]) / (xk * (pi / 3.0));  // sinc(x_k / 3)
+    } else if constexpr (W == 3) {
+      constexpr double C23[6] = {1.0, -0.5, -0.5, 1.0, -0.5, -0.5};
+      constexpr double S23[6] = {0.0, h, -h, 0.0, h, -h};
+      const double c2 = ct * ct - st * st, s2 = 2.0 * st * ct;  // of 2 pi x0 / 3
+      w = 0.42 + 0.5 * (ct * C3[k] + st * S3[k]) + 0.08 * (c2 * C23[k] + s2 * S23[k]);
+    } else {
+      static_assert(W == 4, "window 0..4");
+      w = 1.0 - xk * xk * (1.0 / 9.0);
+    }
+    wt[k] = w * sinc;
+  }
+}
+
 template <int D>
 __device__ __forceinline__ void apply_chain(const ApplyDev& P, double p[3]) {
   for (int t = 0; t < P.nt; ++t) {
@@ -2075,7 +2123,59 @@ __global__ void __launch_bounds__(kBlock) k_apply_transforms(GridDev go, ApplyDe
     out[i] = img[id[0] + static_cast<int64_t>(P.in.n[0]) * (id[1] + static_cast<int64_t>(P.in.n[1]) * id[2])];
     return;
   }
-  if constexpr (INTERP >= 16) {
+  if constexpr (INTERP >= 32) {
+    // WindowedSincInterpolateImageFunction: the 2m samples from floor(c) -
+    // m + 1 to floor(c) + m on each axis, weighted by window(x) sinc(x) of
+    // their offset x, windows (Meijering, Niessen & Viergever, Med. Image
+    // Anal. 3(2) 1999, m = 3): cosine cos(pi x / 2m), Hamming 0.54 +
+    // 0.46 cos(pi x / m), Lanczos sinc(x / m), Blackman 0.42 +
+    // 0.5 cos(pi x / m) + 0.08 cos(2 pi x / m), Welch 1 - x^2 / m^2; a delta
+    // at floor(c) where c is whole; samples outside the buffer 0
+    constexpr int Wn = INTERP - 32, K = 2 * kSincRadius;
+    int64_t base[3] = {0, 0, 0};
+    double wt[3][K];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) wt[d][k] = 0.0;
+      if (d >= D) continue;
+      const double fl = floor(c[d]);
+      base[d] = static_cast<int64_t>(fl) - (kSincRadius - 1);
+      const double dist = c[d] - fl;
+      if (dist == 0.0) wt[d][kSincRadius - 1] = 1.0;
+      else sinc_weights<Wn>(dist, wt[d]);
+    }
+    // separable sum: rows outside the buffer are skipped, taps outside a
+    // row read nothing (their weight is zeroed; the clamped index stays in
+    // the row)
+    int xo[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int64_t x = base[0] + k;
+      const bool in = x >= 0 && x < P.in.n[0];
+      xo[k] = in ? static_cast<int>(x) : 0;
+      if (!in) wt[0][k] = 0.0;
+    }
+    constexpr int KZ = D == 3 ? K : 1;
+    double acc = 0.0;
+    for (int kz = 0; kz < KZ; ++kz) {
+      const int64_t z = base[2] + kz;
+      if (D == 3 && (z < 0 || z >= P.in.n[2])) continue;
+      double plane = 0.0;
+      for (int ky = 0; ky < K; ++ky) {
+        const int64_t y = base[1] + ky;
+        if (y < 0 || y >= P.in.n[1]) continue;
+        const float* __restrict__ row = img + (z * P.in.n[1] + y) * P.in.n[0];
+        double r = 0.0;
+#pragma unroll
+        for (int kx = 0; kx < K; ++kx) r = fma(wt[0][kx], static_cast<double>(row[xo[kx]]), r);
+        plane = fma(wt[1][ky], r, plane);
+      }
+      acc = D == 3 ? fma(wt[2][kz], plane, acc) : plane;
+    }
+    out[i] = static_cast<float>(acc);
+    return;
+  } else if constexpr (INTERP >= 16) {
     // BSplineInterpolateImageFunction of order Q over the coefficients,
     // indices mirrored into the buffer
     constexpr int Q = INTERP - 16;
@@ -3656,7 +3756,12 @@ void launch_apply_transforms(const GridDev& go, const ApplyDev& P, int interp, c
     case 18: ANTS_APPLY_LAUNCH(DD, 18); break; \
     case 19: ANTS_APPLY_LAUNCH(DD, 19); break; \
     case 20: ANTS_APPLY_LAUNCH(DD, 20); break; \
-    default: ANTS_APPLY_LAUNCH(DD, 21); break; \
+    case 21: ANTS_APPLY_LAUNCH(DD, 21); break; \
+    case 32: ANTS_APPLY_LAUNCH(DD, 32); break; \
+    case 33: ANTS_APPLY_LAUNCH(DD, 33); break; \
+    case 34: ANTS_APPLY_LAUNCH(DD, 34); break; \
+    case 35: ANTS_APPLY_LAUNCH(DD, 35); break; \
+    default: ANTS_APPLY_LAUNCH(DD, 36); break; \
   }
   if (go.D == 2) {
     ANTS_APPLY_DIM(2)

@@ -286,7 +286,8 @@ struct ApplyDev {
   const double* coef;  // B-spline: the input's interpolation coefficients (double)
 };
 // interp: 0 linear, 1 nearest neighbour, 2 generic label (linear indicators),
-// 16 + q B-spline of order q (0..5) over P.coef
+// 16 + q B-spline of order q (0..5) over P.coef, 32 + w windowed sinc of
+// radius 3 (w: 0 cosine, 1 Hamming, 2 Lanczos, 3 Blackman, 4 Welch)
 void launch_apply_transforms(const GridDev& go, const ApplyDev& P, int interp, const float* in, float* out,
                              hipStream_t s);
 // TransformToDisplacementFieldFilter: T(p) - p for every point p of grid go

@@ -237,8 +237,8 @@ def bench_serve(args):
 def bench_apply(args):
     """antsApplyTransforms subset: a 192x224x192 image (and a 32-label map)
     resampled onto the same grid through [displacement field, affine] with
-    Linear, NearestNeighbor, GenericLabel and BSpline[3] interpolation (the
-    last including its coefficient decomposition).  Algorithmic bytes per
+    Linear, NearestNeighbor, GenericLabel, BSpline[3] (including its
+    coefficient decomposition) and LanczosWindowedSinc interpolation.  Algorithmic bytes per
     output voxel: field 3*4 (read in place) + input 4 (gathered once) +
     output 4."""
     from scipy import ndimage
@@ -262,7 +262,7 @@ def bench_apply(args):
     res = {}
     e0, e1 = L.Event(), L.Event()
     for interp, src in (("Linear", d_img), ("NearestNeighbor", d_lab), ("GenericLabel", d_lab),
-                        ("BSpline[3]", d_img)):
+                        ("BSpline[3]", d_img), ("LanczosWindowedSinc", d_img)):
         for _ in range(2):
             AT.apply_transforms(g, src, g, xf, interp, out=d_out)
         L.check(L.lib().ants_device_synchronize())

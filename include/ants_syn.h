@@ -403,6 +403,15 @@ typedef struct ants_transform {
 #define ANTS_INTERP_GENERIC_LABEL 2
 #define ANTS_INTERP_BSPLINE_ORDER(q) (16 + (q))
 #define ANTS_INTERP_BSPLINE ANTS_INTERP_BSPLINE_ORDER(3)
+/* WindowedSincInterpolateImageFunction of radius 3 with a constant (zero)
+ * boundary, make_interpolator_snip.tmpl:77-110: CosineWindowedSinc,
+ * HammingWindowedSinc, LanczosWindowedSinc, BlackmanWindowedSinc,
+ * WelchWindowedSinc. */
+#define ANTS_INTERP_COSINE_WINDOWED_SINC 32
+#define ANTS_INTERP_HAMMING_WINDOWED_SINC 33
+#define ANTS_INTERP_LANCZOS_WINDOWED_SINC 34
+#define ANTS_INTERP_BLACKMAN_WINDOWED_SINC 35
+#define ANTS_INTERP_WELCH_WINDOWED_SINC 36
 
 /* Resample d_in (grid `in`) onto grid `ref` (ResampleImageFilter): each
  * reference point p is mapped through transforms[0], then transforms[1], ...

@@ -472,6 +472,8 @@ INTERP = {"linear": 0, "nearestneighbor": 1, "genericlabel": 2}
 for _q in range(6):
     INTERP[f"bspline[{_q}]"] = 16 + _q
 INTERP["bspline"] = 16 + 3
+for _w, _n in enumerate(("cosine", "hamming", "lanczos", "blackman", "welch")):
+    INTERP[_n + "windowedsinc"] = 32 + _w
 
 
 def og_geom(size, spacing=None, origin=None, direction=None) -> OgGeom:

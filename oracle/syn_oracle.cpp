@@ -1373,6 +1373,64 @@ double bspline_evaluate(const og_geom& g, const std::vector<double>& coef, int q
   return acc;
 }
 
+// WindowedSincInterpolateImageFunction<.., 3, window, ConstantBoundaryCondition>
+// (ITK, absent from the reference tree; selected at
+// Examples/make_interpolator_snip.tmpl:77-110): per axis the six samples
+// floor(c) - 2 .. floor(c) + 3 with weights window(x) sinc(x) at their
+// offsets x = c - j (a delta on floor(c) when c is whole), zero outside the
+// buffer; window 0 cosine cos(pi x / 6), 1 Hamming 0.54 + 0.46 cos(pi x / 3),
+// 2 Lanczos sinc(x / 3), 3 Blackman 0.42 + 0.5 cos(pi x / 3) +
+// 0.08 cos(2 pi x / 3), 4 Welch 1 - x^2 / 9 (Meijering et al. 1999).
+// Parity: pinned by test_oracle_apply.py against a numpy evaluation of
+// the definition at scattered points near the edges (np.sinc), exact
+// samples at whole indices and the reconstruction of a band-limited
+// signal; the window constants and the offset convention are ITK's as
+// documented, not read from ITK's source (unpinned against ITK itself).
+double sinc_window(int w, double x) {
+  const double pi = 3.14159265358979323846;
+  const double sinc = x == 0.0 ? 1.0 : std::sin(pi * x) / (pi * x);
+  double win = 1.0;
+  switch (w) {
+    case 0: win = std::cos(pi * x / 6.0); break;
+    case 1: win = 0.54 + 0.46 * std::cos(pi * x / 3.0); break;
+    case 2: win = x == 0.0 ? 1.0 : std::sin(pi * x / 3.0) / (pi * x / 3.0); break;
+    case 3: win = 0.42 + 0.5 * std::cos(pi * x / 3.0) + 0.08 * std::cos(2.0 * pi * x / 3.0); break;
+    default: win = 1.0 - x * x / 9.0; break;
+  }
+  return win * sinc;
+}
+
+double sinc_evaluate(const og_geom& g, const float* img, int w, const double c[3]) {
+  const int D = g.dim;
+  double wt[3][6];
+  int64_t base[3] = {0, 0, 0};
+  for (int d = 0; d < 3; ++d) {
+    for (int k = 0; k < 6; ++k) wt[d][k] = (d >= D && k == 0) ? 1.0 : 0.0;
+    if (d >= D) continue;
+    const double fl = std::floor(c[d]);
+    base[d] = static_cast<int64_t>(fl) - 2;
+    const double dist = c[d] - fl;
+    for (int k = 0; k < 6; ++k) {
+      const double off = dist + 2.0 - k;  // c - (floor(c) - 2 + k)
+      wt[d][k] = dist == 0.0 ? (k == 2 ? 1.0 : 0.0) : sinc_window(w, off);
+    }
+  }
+  double acc = 0.0;
+  const int KZ = D == 3 ? 6 : 1;
+  for (int kz = 0; kz < KZ; ++kz)
+    for (int ky = 0; ky < 6; ++ky)
+      for (int kx = 0; kx < 6; ++kx) {
+        const int64_t x = base[0] + kx, y = base[1] + ky, z = base[2] + kz;
+        const bool in = x >= 0 && x < g.n[0] && y >= 0 && y < g.n[1] && (D == 2 || (z >= 0 && z < g.n[2]));
+        double v = in ? static_cast<double>(img[x + g.n[0] * (y + g.n[1] * z)]) : 0.0;
+        v *= wt[0][kx];
+        v *= wt[1][ky];
+        if (D == 3) v *= wt[2][kz];
+        acc += v;
+      }
+  return acc;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1380,7 +1438,8 @@ double bspline_evaluate(const og_geom& g, const std::vector<double>& coef, int q
 extern "C" {
 
 // interp: 0 linear, 1 nearest neighbour, 2 generic label, 16 + q B-spline
-// of order q (0..5)
+// of order q (0..5), 32 + w windowed sinc (w: cosine, Hamming, Lanczos,
+// Blackman, Welch)
 int og_apply_transforms_f64(const og_geom* ref, const og_geom* in, const float* img, const og_xform* xf, int nt,
                             int interp, double defv, float* out) {
   const int D = ref->dim;
@@ -1389,7 +1448,7 @@ int og_apply_transforms_f64(const og_geom* ref, const og_geom* in, const float* 
   const int q = interp - 16;
   std::vector<double> coef;
   if (q >= 0 && q <= 5) bspline_coefficients(*in, img, q, coef);
-  else if (interp < 0 || interp > 2) return -2;
+  else if ((interp < 0 || interp > 2) && (interp < 32 || interp > 36)) return -2;
 #pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < N; ++i) {
     const int64_t x = i % ref->n[0], y = (i / ref->n[0]) % ref->n[1], z = i / (ref->n[0] * ref->n[1]);
@@ -1401,6 +1460,10 @@ int og_apply_transforms_f64(const og_geom* ref, const og_geom* in, const float* 
     geom_point_to_index(*in, p, c);
     if (!geom_inside(*in, c)) {
       out[i] = static_cast<float>(defv);
+      continue;
+    }
+    if (interp >= 32) {
+      out[i] = static_cast<float>(sinc_evaluate(*in, img, interp - 32, c));
       continue;
     }
     if (interp >= 16) {

@@ -2,7 +2,7 @@
 composite affine + displacement-field transforms with directions and
 non-dyadic spacings; Linear within 1e-6, NearestNeighbor and GenericLabel
 bit-exact (coordinates are formed in double in the same order on both),
-BSpline[0..5] within 2e-6 of the image range."""
+BSpline[0..5] and the windowed sincs within 2e-6 of the image range."""
 import numpy as np
 import pytest
 from scipy import ndimage
@@ -89,6 +89,24 @@ def test_apply_bspline_matches_oracle(D, q):
         np.testing.assert_array_equal(out, ref)
     else:
         assert err <= 2e-6 * np.abs(img).max()
+
+
+@pytest.mark.parametrize("D", [2, 3])
+@pytest.mark.parametrize("window", ["Cosine", "Hamming", "Lanczos", "Blackman", "Welch"])
+def test_apply_windowed_sinc_matches_oracle(D, window):
+    """-n <window>WindowedSinc (radius 3, zero boundary) against the oracle:
+    the same sample order and products; device and host sin/cos may differ
+    by an ulp, so within 2e-6 of the image range."""
+    gi, gr, img, xf, ox = case(D, "Linear")
+    name = window + "WindowedSinc"
+    out = AT.apply_transforms(gr, img, gi, xf, name, default_value=-7.0)
+    ref = o.apply_transforms(og(gr), og(gi), img, ox, name.lower(), -7.0)
+    inside = ref != -7.0
+    assert 0.2 < inside.mean() < 1.0
+    np.testing.assert_array_equal(out == -7.0, ~inside)
+    err = np.abs(out - ref).max()
+    print(f"\n{D}-D {name} max |gpu - oracle| = {err:.3e}")
+    assert err <= 2e-6 * np.abs(img).max()
 
 
 def test_apply_bspline_default_order_and_thin_axis():

@@ -115,7 +115,11 @@ def test_parse_interpolation_grammar():
     assert parse_interpolation("GenericLabel[Linear]") == 2
     assert parse_interpolation("BSpline") == 16 + 3
     assert [parse_interpolation(f"BSpline[{q}]") for q in range(6)] == [16 + q for q in range(6)]
-    for bad in ("BSpline[6]", "BSpline[-1]", "BSpline[2.5]", "GenericLabel[NearestNeighbor]", "Gaussian", "Sinc[3]"):
+    assert parse_interpolation("LanczosWindowedSinc") == 34
+    assert [parse_interpolation(w + "WindowedSinc") for w in ("Cosine", "hamming", "Lanczos", "BLACKMAN", "Welch")] \
+        == [32, 33, 34, 35, 36]
+    for bad in ("BSpline[6]", "BSpline[-1]", "BSpline[2.5]", "GenericLabel[NearestNeighbor]", "Gaussian", "Sinc[3]",
+                "MultiLabel"):
         with pytest.raises(ValueError):
             parse_interpolation(bad)
 

@@ -250,3 +250,76 @@ def test_compose_field_displacement_and_order():
     # a field grid shifted far away covers no reference point
     gfar = o.og_geom(size, (1.0, 1.0, 1.0), (100.0, 0.0, 0.0))
     np.testing.assert_array_equal(o.compose_field(g, [("field", gfar, u)]), 0.0)
+
+
+_WINDOWS = {
+    "cosine": lambda x: np.cos(np.pi * x / 6.0),
+    "hamming": lambda x: 0.54 + 0.46 * np.cos(np.pi * x / 3.0),
+    "lanczos": lambda x: np.sinc(x / 3.0),
+    "blackman": lambda x: 0.42 + 0.5 * np.cos(np.pi * x / 3.0) + 0.08 * np.cos(2 * np.pi * x / 3.0),
+    "welch": lambda x: 1.0 - x * x / 9.0,
+}
+
+
+def _sinc_brute(img, c, window):
+    """sum over the 6^D samples floor(c)-2 .. floor(c)+3 of
+    img * prod_d window(x_d) sinc(x_d), zero outside, delta when whole."""
+    D = img.ndim
+    out = 0.0
+    fl = np.floor(c)
+    axes = []
+    for d in range(D):
+        js = fl[d] - 2 + np.arange(6)
+        x = c[d] - js
+        w = np.where(c[d] == fl[d], (js == fl[d]).astype(float), _WINDOWS[window](x) * np.sinc(x))
+        axes.append((js.astype(int), w))
+    for idx in np.ndindex(*(6,) * D):
+        j = [axes[d][0][idx[d]] for d in range(D)]
+        w = np.prod([axes[d][1][idx[d]] for d in range(D)])
+        inside = all(0 <= j[d] < img.shape[D - 1 - d] for d in range(D))
+        if inside:
+            out += img[tuple(reversed(j))] * w
+    return out
+
+
+@pytest.mark.parametrize("window", list(_WINDOWS))
+@pytest.mark.parametrize("D", [2, 3])
+def test_windowed_sinc_matches_definition(window, D):
+    """The oracle's windowed sinc against a direct numpy evaluation of the
+    definition (np.sinc) at points over the whole buffer, edges included,
+    through a rotation and shift."""
+    r = np.random.default_rng(60 + D)
+    size = (13, 11, 9)[:D]
+    img = r.standard_normal(tuple(reversed(size))).astype(np.float32)
+    g = o.og_geom(size, (1.0,) * D, (0.0,) * D)
+    A = np.eye(D) * 0.97 + 0.05 * r.standard_normal((D, D))
+    t, c = 0.4 * r.standard_normal(D), np.asarray(size, float) / 2
+    out = o.apply_transforms(g, g, img, [("affine", A, t, c)], window + "windowedsinc", -9.0)
+    pts = points(size, (1.0,) * D, (0.0,) * D, np.eye(D))
+    ci = (pts - c) @ A.T + t + c
+    ins = inside(ci, size)
+    assert 0.5 < ins.mean() < 1.0
+    flat_c, flat_o = ci.reshape(-1, D), out.reshape(-1)
+    for k in np.flatnonzero(ins.reshape(-1))[:: max(1, ins.sum() // 60)]:
+        assert flat_o[k] == pytest.approx(_sinc_brute(img.astype(np.float64), flat_c[k], window), abs=2e-6)
+    assert np.all(out[~ins] == -9.0)
+
+
+@pytest.mark.parametrize("window", list(_WINDOWS))
+def test_windowed_sinc_samples_and_bandlimited_signal(window):
+    """Whole indices return the samples; a slowly varying cosine is
+    reconstructed at half-voxel shifts in the interior to within the
+    window's ripple (a weight set one sample off moves it by ~0.3)."""
+    size = (48, 40)
+    y, x = np.meshgrid(np.arange(40.0), np.arange(48.0), indexing="ij")
+    f = lambda xx, yy: np.cos(2 * np.pi * 0.05 * xx) * np.cos(2 * np.pi * 0.04 * yy + 0.3)
+    img = f(x, y).astype(np.float32)
+    g = o.og_geom(size, (1.0, 1.0), (0.0, 0.0))
+    name = window + "windowedsinc"
+    whole = o.apply_transforms(g, g, img, [("affine", np.eye(2), np.array([2.0, -1.0]), np.zeros(2))], name, 0.0)
+    np.testing.assert_array_equal(whole[1:, :-2], img[:-1, 2:])  # out(x, y) = img(x + 2, y - 1)
+    half = o.apply_transforms(g, g, img, [("affine", np.eye(2), np.array([0.5, 0.5]), np.zeros(2))], name, 0.0)
+    core = (slice(6, -6), slice(6, -6))
+    err = np.abs(half - f(x + 0.5, y + 0.5))[core].max()
+    print(f"\n{window}: band-limited reconstruction error {err:.4f}")
+    assert err < 0.05
