@@ -164,6 +164,25 @@ def compose_transforms_to_field(reference: Geometry, transforms: Sequence[Transf
     return d_out if out is not None else d_out.to_host()
 
 
+def apply_transforms_to_points(points, transforms: Sequence[Transform] = (), dim: int = 3) -> np.ndarray:
+    """antsApplyTransformsToPoints (Examples/antsApplyTransformsToPoints.cxx:145-190):
+    the first ``dim`` columns of ``points`` (n rows, physical coordinates)
+    through the transforms in -t order; further columns are copied.  Returns
+    a float64 array of the same shape."""
+    pts = np.asarray(points, dtype=np.float64)
+    if pts.ndim != 2 or pts.shape[1] < dim:
+        raise ValueError(f"points must be an (n, >= {dim}) array")
+    out = pts.copy()
+    n = pts.shape[0]
+    if n == 0:
+        return out
+    xs, keep = transform_array(transforms, dim)
+    d_pts = L.DeviceArray.from_host(np.ascontiguousarray(pts[:, :dim]))
+    L.check(L.lib().ants_apply_transforms_to_points(dim, d_pts.ptr, n, xs, len(transforms), d_pts.ptr, None))
+    out[:, :dim] = d_pts.to_host()
+    return out
+
+
 def collapse_linear(transforms: Sequence[Transform], D: int) -> "AffineTransform":
     """antsApplyTransforms -o Linear[file] (Examples/antsApplyTransforms.cxx:647-688):
     the single affine equal to the -t list of affine (and identity)

@@ -41,6 +41,12 @@ with 1) and ``-o CompositeTransform[out.h5]`` the list as a composite file;
 those need no input image.  ``-n BSpline[3]`` selects B-spline
 interpolation of order 0..5.
 
+``python -m ants_amd.cli apply-points -d 3 -i points.csv -o out.csv
+  -t [affine.mat,1] -t out_0InverseWarp.nii.gz`` is antsApplyTransformsToPoints
+(Examples/antsApplyTransformsToPoints.cxx): the x,y[,z] columns of the CSV
+(physical LPS coordinates, a header row) through the transforms in -t order,
+the other columns copied.
+
 ``registration ... -a 1`` writes ``out_Composite.h5`` and
 ``out_InverseComposite.h5`` (--write-composite-transform) instead of the
 separate files; ``--save-state state.h5`` writes the registration state
@@ -397,12 +403,30 @@ def _field_stage_outputs(a, fixed, gf, moving, gm, init, finit, outs, prefix) ->
     return 0
 
 
+def cmd_apply_points(argv: Sequence[str]) -> int:
+    """antsApplyTransformsToPoints -d D -i in.csv -o out.csv -t T1 -t T2 ..."""
+    from .apply_transforms import apply_transforms_to_points
+    ap = argparse.ArgumentParser(prog="apply-points")
+    ap.add_argument("-d", "--dimensionality", type=int, required=True)
+    ap.add_argument("-i", "--input", required=True)
+    ap.add_argument("-o", "--output", required=True)
+    ap.add_argument("-t", "--transform", action="append", default=[])
+    a = ap.parse_args(argv)
+    names, pts = aio.read_points_csv(a.input)
+    if pts.shape[1] < a.dimensionality:
+        raise SystemExit(f"the point file has fewer than {a.dimensionality} columns")
+    xf = [x for t in a.transform for x in load_transforms(t)]
+    aio.write_points_csv(_bracket_list(a.output)[0], names, apply_transforms_to_points(pts, xf, a.dimensionality))
+    return 0
+
+
 def main(argv: Optional[Sequence[str]] = None) -> int:
     argv = list(sys.argv[1:] if argv is None else argv)
-    if not argv or argv[0] not in ("registration", "apply"):
+    cmds = {"registration": cmd_registration, "apply": cmd_apply, "apply-points": cmd_apply_points}
+    if not argv or argv[0] not in cmds:
         print(__doc__)
         return 2
-    return {"registration": cmd_registration, "apply": cmd_apply}[argv[0]](argv[1:])
+    return cmds[argv[0]](argv[1:])
 
 
 if __name__ == "__main__":

@@ -2742,6 +2742,23 @@ extern "C" int ants_apply_transforms(const ants_image_desc* ref, const ants_imag
   return ANTS_OK;
 }
 
+extern "C" int ants_apply_transforms_to_points(int dim, const double* d_in, int64_t n_points,
+                                               const ants_transform* transforms, int n_transforms, double* d_out,
+                                               void* stream) {
+  if (dim != 2 && dim != 3) return fail(ANTS_ERR_INVALID, "dimension must be 2 or 3");
+  if (n_points < 0) return fail(ANTS_ERR_INVALID, "negative point count");
+  if (n_points > 0 && (!d_in || !d_out)) return fail(ANTS_ERR_INVALID, "null point buffer");
+  if (n_transforms < 0 || n_transforms > kMaxTransforms || (n_transforms > 0 && !transforms))
+    return fail(ANTS_ERR_INVALID, "transform count must be 0..%d", kMaxTransforms);
+  ApplyDev P;
+  std::memset(&P, 0, sizeof(P));
+  P.nt = n_transforms;
+  for (int t = 0; t < n_transforms; ++t) ANTS_TRY(xform_to_dev(transforms[t], dim, &P.t[t]));
+  launch_transform_points(dim, P, d_in, d_out, n_points, static_cast<hipStream_t>(stream));
+  HIP_TRY(hipGetLastError());
+  return ANTS_OK;
+}
+
 extern "C" int ants_compose_transforms_to_field(const ants_image_desc* ref, const ants_transform* transforms,
                                                 int n_transforms, float* d_out, void* stream) {
   ApplyDev P;

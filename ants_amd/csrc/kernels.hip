@@ -2098,6 +2098,21 @@ __global__ void __launch_bounds__(kBlock) k_compose_field(GridDev go, ApplyDev P
   for (int d = 0; d < D; ++d) out[i * D + d] = static_cast<float>(p[d] - p0[d]);
 }
 
+// CompositeTransform::TransformPoint over a point list: point i (D
+// doubles at in + i * D) through the transform list, to out (may equal in).
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_transform_points(ApplyDev P, const double* in, double* out,
+                                                             int64_t n) {
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  double p[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int d = 0; d < D; ++d) p[d] = in[i * D + d];
+  apply_chain<D>(P, p);
+#pragma unroll
+  for (int d = 0; d < D; ++d) out[i * D + d] = p[d];
+}
+
 template <int D, int INTERP>
 __global__ void __launch_bounds__(kBlock) k_apply_transforms(GridDev go, ApplyDev P, const float* __restrict__ img,
                                                              float* __restrict__ out) {
@@ -3770,6 +3785,15 @@ void launch_apply_transforms(const GridDev& go, const ApplyDev& P, int interp, c
   }
 #undef ANTS_APPLY_DIM
 #undef ANTS_APPLY_LAUNCH
+}
+
+void launch_transform_points(int D, const ApplyDev& P, const double* in, double* out, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  const dim3 grid(blocks_for(n));
+  if (D == 2)
+    hipLaunchKernelGGL(k_transform_points<2>, grid, dim3(kBlock), 0, s, P, in, out, n);
+  else
+    hipLaunchKernelGGL(k_transform_points<3>, grid, dim3(kBlock), 0, s, P, in, out, n);
 }
 
 void launch_compose_field(const GridDev& go, const ApplyDev& P, float* out, hipStream_t s) {

@@ -290,6 +290,9 @@ struct ApplyDev {
 // radius 3 (w: 0 cosine, 1 Hamming, 2 Lanczos, 3 Blackman, 4 Welch)
 void launch_apply_transforms(const GridDev& go, const ApplyDev& P, int interp, const float* in, float* out,
                              hipStream_t s);
+// CompositeTransform::TransformPoint of n points (D doubles each, in place
+// allowed) through P's transform list.
+void launch_transform_points(int D, const ApplyDev& P, const double* in, double* out, int64_t n, hipStream_t s);
 // TransformToDisplacementFieldFilter: T(p) - p for every point p of grid go
 // (P.ref), T = P's transform list; D interleaved components per voxel.
 void launch_compose_field(const GridDev& go, const ApplyDev& P, float* out, hipStream_t s);

@@ -250,3 +250,35 @@ def write_transform(path: str, t: AffineTransform) -> None:
         fh.write(f"Transform: {name}\n")
         fh.write("Parameters: " + " ".join(repr(float(v)) for v in params) + "\n")
         fh.write("FixedParameters: " + " ".join(repr(float(v)) for v in c) + "\n")
+
+
+def read_points_csv(path: str):
+    """A point file as antsApplyTransformsToPoints reads it
+    (Examples/antsApplyTransformsToPoints.cxx:57-85): a header row of
+    column names (x,y,z,t,label,...), then comma-separated rows; cells that
+    are not numbers read as NaN.  Returns (names, float64 array)."""
+    import csv
+    with open(path, newline="") as fh:
+        rows = [r for r in csv.reader(fh, delimiter=",", quotechar='"') if r]
+    if not rows:
+        raise ValueError(f"{path}: empty point file")
+    names = [c.strip() for c in rows[0]]
+
+    def num(v):
+        try:
+            return float(v)
+        except ValueError:
+            return float("nan")
+
+    data = np.array([[num(v) for v in r] + [float("nan")] * (len(names) - len(r)) for r in rows[1:]],
+                    dtype=np.float64).reshape(-1, len(names))
+    return names, data
+
+
+def write_points_csv(path: str, names, data) -> None:
+    """The header row, then one row per point (numbers in full precision)."""
+    data = np.asarray(data, dtype=np.float64)
+    with open(path, "w") as fh:
+        fh.write(",".join(names) + "\n")
+        for row in data:
+            fh.write(",".join(repr(float(v)) for v in row) + "\n")

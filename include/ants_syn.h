@@ -427,6 +427,16 @@ int ants_apply_transforms(const ants_image_desc* ref, const ants_image_desc* in,
                           const ants_transform* transforms, int n_transforms, int interpolation,
                           double default_value, float* d_out, void* stream);
 
+/* antsApplyTransformsToPoints (Examples/antsApplyTransformsToPoints.cxx:145-190):
+ * each of n_points physical points (dim doubles at d_in + i * dim) through
+ * the transform list in -t order, CompositeTransform::TransformPoint, into
+ * d_out (may equal d_in).  The list is the one ants_apply_transforms uses
+ * for a reference image, so it maps reference-space points to input-space
+ * points.  Asynchronous on `stream`. */
+int ants_apply_transforms_to_points(int dim, const double* d_in, int64_t n_points,
+                                    const ants_transform* transforms, int n_transforms, double* d_out,
+                                    void* stream);
+
 /* antsApplyTransforms -o [field,1] / -o DisplacementField[field]
  * (Examples/antsApplyTransforms.cxx:706-760, TransformToDisplacementFieldFilter):
  * d_out[i] = T(p_i) - p_i for every point p_i of grid `ref`, T the transform

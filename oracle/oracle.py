@@ -537,6 +537,19 @@ def apply_transforms(ref: OgGeom, inp: OgGeom, img, transforms=(), interpolation
     return out
 
 
+def transform_points(points, transforms=(), dim=3):
+    """The first `dim` columns of each row through the transform list."""
+    pts = np.ascontiguousarray(np.asarray(points, np.float64)[:, :dim])
+    xs, keep = _xform_array(transforms, dim)
+    out = np.empty_like(pts)
+    fn = lib().og_transform_points_f64
+    fn.restype = None
+    fn.argtypes = [C.c_int, C.POINTER(C.c_double), C.c_int64, C.POINTER(OgXform), C.c_int, C.POINTER(C.c_double)]
+    fn(dim, pts.ctypes.data_as(C.POINTER(C.c_double)), pts.shape[0], xs, len(transforms),
+       out.ctypes.data_as(C.POINTER(C.c_double)))
+    return out
+
+
 def compose_field(ref: OgGeom, transforms=()):
     """T(p) - p at every point of `ref` (transforms as in apply_transforms);
     float32, shape (z, y, x, D) in numpy order."""

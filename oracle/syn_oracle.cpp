@@ -1506,6 +1506,17 @@ int og_apply_transforms_f64(const og_geom* ref, const og_geom* in, const float* 
   return 0;
 }
 
+// CompositeTransform::TransformPoint of n points (dim doubles each), as
+// antsApplyTransformsToPoints calls it (Examples/antsApplyTransformsToPoints.cxx:166-180).
+void og_transform_points_f64(int dim, const double* in, int64_t n, const og_xform* xf, int nt, double* out) {
+  for (int64_t i = 0; i < n; ++i) {
+    double p[3] = {0.0, 0.0, 0.0};
+    for (int d = 0; d < dim; ++d) p[d] = in[i * dim + d];
+    for (int t = 0; t < nt; ++t) xform_point(xf[t], dim, p);
+    for (int d = 0; d < dim; ++d) out[i * dim + d] = p[d];
+  }
+}
+
 // TransformToDisplacementFieldFilter (ITK; called at
 // Examples/antsApplyTransforms.cxx:716-724): T(p) - p at every point p of
 // `ref`, T the transform list applied in order; dim interleaved components.

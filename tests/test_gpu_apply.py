@@ -159,6 +159,28 @@ def test_compose_transforms_to_field_matches_oracle(D):
     np.testing.assert_allclose(a[both], b[both], rtol=0, atol=1e-4 * np.abs(img).max())
 
 
+@pytest.mark.parametrize("D", [2, 3])
+def test_apply_transforms_to_points_matches_oracle(D):
+    """antsApplyTransformsToPoints: points through [field, affine, identity]
+    against the oracle (double coordinates, within 1e-9 mm); extra columns
+    pass through; at the reference grid's points the result is the point
+    plus the composite field there."""
+    gi, gr, img, xf, ox = case(D, "Linear")
+    pts = np.column_stack([rng.uniform(-10, 40, (500, D)), np.arange(500.0), np.full(500, np.nan)])
+    got = AT.apply_transforms_to_points(pts, xf, D)
+    ref = o.transform_points(pts, ox, D)
+    assert got.shape == pts.shape
+    np.testing.assert_allclose(got[:, :D], ref, rtol=0, atol=1e-9)
+    np.testing.assert_array_equal(got[:, D], pts[:, D])
+    assert np.isnan(got[:, D + 1]).all()
+    A = np.asarray(gr.direction, float) * np.asarray(gr.spacing, float)
+    idx = np.stack(np.meshgrid(*[np.arange(n, dtype=float) for n in reversed(gr.size)], indexing="ij"), -1)[..., ::-1]
+    grid_pts = (np.asarray(gr.origin, float) + idx @ A.T).reshape(-1, D)
+    fld = AT.compose_transforms_to_field(gr, xf).reshape(-1, D)
+    np.testing.assert_allclose(AT.apply_transforms_to_points(grid_pts, xf, D), grid_pts + fld, rtol=0, atol=3e-5)
+    assert AT.apply_transforms_to_points(np.zeros((0, D)), xf, D).shape == (0, D)
+
+
 def test_inverse_affine_round_trip():
     """-t affine then -t [affine,1] maps every point back onto itself."""
     gi, gr, img, _, _ = case(3, "NearestNeighbor")

@@ -248,3 +248,33 @@ def test_apply_transform_outputs(tmp_path):
     for u, v in zip(xc[1].parameters(), xf[1].parameters()):
         np.testing.assert_allclose(u, v, rtol=1e-6, atol=1e-6)
     np.testing.assert_array_equal(a, apply_transforms(g, img, g, xc))
+
+
+def test_apply_points_csv(tmp_path):
+    """apply-points: a CSV with x,y,z,t,label,comment columns through
+    [affine, inverse affine] comes back unchanged, and through a field and an
+    affine equals the library call; the header and extra columns are kept."""
+    from ants_amd.apply_transforms import AffineTransform, apply_transforms_to_points
+    rng = np.random.default_rng(9)
+    pin = str(tmp_path / "pts.csv")
+    pts = np.column_stack([rng.uniform(0, 20, (40, 3)), np.zeros(40), np.arange(40.0)])
+    with open(pin, "w") as fh:
+        fh.write("x,y,z,t,label,comment\n")
+        for r in pts:
+            fh.write(",".join(repr(float(v)) for v in r) + ",pt\n")
+    apath = str(tmp_path / "a.mat")
+    aio.write_transform(apath, AffineTransform(np.eye(3) * 1.1, [1.0, -2.0, 0.5], [10.0, 10.0, 10.0]))
+    out = str(tmp_path / "out.csv")
+    assert cli.main(["apply-points", "-d", "3", "-i", pin, "-o", out, "-t", apath, "-t", f"[{apath},1]"]) == 0
+    names, got = aio.read_points_csv(out)
+    assert names == ["x", "y", "z", "t", "label", "comment"]
+    np.testing.assert_allclose(got[:, :5], pts, rtol=0, atol=1e-12)
+    assert np.isnan(got[:, 5]).all()
+    g = Geometry((24, 24, 24), (1.0, 1.0, 1.0), (0.0, 0.0, 0.0), np.eye(3))
+    fld = (0.5 * rng.standard_normal(g.shape + (3,))).astype(np.float32)
+    fpath = str(tmp_path / "w.nii.gz")
+    aio.write_nifti(fpath, fld, g, vector=True)
+    assert cli.main(["apply-points", "-d", "3", "-i", pin, "-o", out, "-t", fpath, "-t", apath]) == 0
+    _, got = aio.read_points_csv(out)
+    want = apply_transforms_to_points(got[:, :5] * 0 + pts, cli.load_transforms(fpath) + cli.load_transforms(apath), 3)
+    np.testing.assert_array_equal(got[:, :5], want)

@@ -323,3 +323,25 @@ def test_windowed_sinc_samples_and_bandlimited_signal(window):
     err = np.abs(half - f(x + 0.5, y + 0.5))[core].max()
     print(f"\n{window}: band-limited reconstruction error {err:.4f}")
     assert err < 0.05
+
+
+def test_transform_points_closed_forms():
+    """Points through [field, affine]: a constant field inside its grid is a
+    translation, points outside it pass unchanged, then the affine
+    A (p - c) + t + c; at grid points the result equals the composite
+    field's T(p) - p plus p."""
+    g = o.og_geom((10, 9, 8), (1.5, 1.0, 2.0), (1.0, -2.0, 0.5))
+    u = np.zeros((8, 9, 10, 3), np.float32)
+    u[...] = [0.5, -0.25, 1.0]
+    A = np.array([[0.9, 0.1, 0.0], [-0.1, 1.1, 0.05], [0.0, 0.02, 1.0]])
+    t, c = np.array([1.0, 2.0, -1.0]), np.array([5.0, 3.0, 4.0])
+    xs = [("field", g, u), ("affine", A, t, c)]
+    inside_pts = np.array([[3.0, 1.0, 4.0], [10.0, 3.0, 10.0]])
+    outside_pts = np.array([[-50.0, 0.0, 0.0]])
+    got = o.transform_points(np.vstack([inside_pts, outside_pts]), xs)
+    q = np.vstack([inside_pts + [0.5, -0.25, 1.0], outside_pts])
+    np.testing.assert_allclose(got, (q - c) @ A.T + t + c, rtol=0, atol=1e-12)
+    p = points((10, 9, 8), (1.5, 1.0, 2.0), (1.0, -2.0, 0.5), np.eye(3)).reshape(-1, 3)
+    fld = o.compose_field(g, xs).reshape(-1, 3)
+    np.testing.assert_allclose(o.transform_points(p, xs), p + fld, rtol=0, atol=2e-6)
+    np.testing.assert_array_equal(o.transform_points(p[:0], xs).shape, (0, 3))
