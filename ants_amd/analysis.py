@@ -1,8 +1,10 @@
-"""Host-side field diagnostics used in reports (not on the GPU path).
+"""Field diagnostics.
 
-``jacobian_determinant``: det(I + grad u) of a displacement field by central
-differences (one-sided at the edges), u in mm on a grid of the given
-spacing; numpy (z,y,x,D) / (y,x,D) layout with components in x,y,z order.
+``jacobian_determinant``: host-side det(I + grad u) of a displacement field by
+central differences (one-sided at the edges), u in mm on a grid of the given
+spacing; numpy (z,y,x,D) / (y,x,D) layout with components in x,y,z order; used
+in reports.  ``create_jacobian_determinant_image`` is the GPU form of ANTs'
+CreateJacobianDeterminantImage.
 """
 from __future__ import annotations
 
@@ -27,3 +29,23 @@ def field_errors(a: np.ndarray, b: np.ndarray, spacing) -> dict:
     ja, jb = jacobian_determinant(a, spacing), jacobian_determinant(b, spacing)
     return {"max_vox": float(d.max()), "mean_vox": float(d.mean()),
             "jac_max_abs": float(np.abs(ja - jb).max()), "jac_mean_abs": float(np.abs(ja - jb).mean())}
+
+
+def create_jacobian_determinant_image(field, geometry, do_log: bool = False, deformation_gradient: bool = False):
+    """CreateJacobianDeterminantImage dim field out [doLogJacobian] [0]
+    [deformationGradient] (Examples/CreateJacobianDeterminantImage.cxx:60-160):
+    max(det(I + R-rotated five-point gradient), 0), or its log clamped at
+    0.001, on the GPU (ants_jacobian_determinant).  ``field``: numpy or
+    DeviceArray of shape geometry.shape + (dim,).  With
+    ``deformation_gradient`` returns the (..., dim, dim) matrices instead."""
+    import ctypes as C
+    from . import _lib as L
+    D = geometry.dim
+    f = field if isinstance(field, L.DeviceArray) else L.DeviceArray.from_host(np.asarray(field, np.float32))
+    if tuple(f.shape) != tuple(geometry.shape) + (D,):
+        raise ValueError("field shape does not match its geometry")
+    out = L.DeviceArray(geometry.shape)
+    grad = L.DeviceArray(tuple(geometry.shape) + (D, D)) if deformation_gradient else None
+    L.check(L.lib().ants_jacobian_determinant(C.byref(geometry.desc()), f.ptr, int(bool(do_log)), out.ptr,
+                                              grad.ptr if grad is not None else None, None))
+    return grad.to_host() if deformation_gradient else out.to_host()

@@ -47,6 +47,10 @@ interpolation of order 0..5.
 (physical LPS coordinates, a header row) through the transforms in -t order,
 the other columns copied.
 
+``python -m ants_amd.cli jacobian 3 out_0Warp.nii.gz jac.nii.gz [doLog]
+  [useGeometric] [deformationGradient]`` is CreateJacobianDeterminantImage
+(the geometric form is not built).
+
 ``registration ... -a 1`` writes ``out_Composite.h5`` and
 ``out_InverseComposite.h5`` (--write-composite-transform) instead of the
 separate files; ``--save-state state.h5`` writes the registration state
@@ -420,9 +424,35 @@ def cmd_apply_points(argv: Sequence[str]) -> int:
     return 0
 
 
+def cmd_jacobian(argv: Sequence[str]) -> int:
+    """CreateJacobianDeterminantImage imageDimension deformationField
+    outputImage [doLogJacobian=0] [useGeometric=0] [deformationGradient=0]
+    (Examples/CreateJacobianDeterminantImage.cxx:215-240)."""
+    from .analysis import create_jacobian_determinant_image
+    if len(argv) < 3:
+        raise SystemExit("usage: jacobian imageDimension deformationField outputImage [doLogJacobian=0] "
+                         "[useGeometric=0] [deformationGradient=0]")
+    dim = int(argv[0])
+    flags = [int(v) for v in argv[3:6]] + [0] * (3 - len(argv[3:6]))
+    if flags[1]:
+        raise SystemExit("useGeometric (GeometricJacobianDeterminantImageFilter) is not supported")
+    fld, g = aio.read_nifti(argv[1])
+    if g.dim != dim:
+        raise SystemExit(f"imageDimension {dim} but the field is {g.dim}-D")
+    out = create_jacobian_determinant_image(np.asarray(fld, np.float32), g, do_log=bool(flags[0]),
+                                            deformation_gradient=bool(flags[2]))
+    if flags[2]:
+        # row-major J[0][0], J[0][1], ... as a vector image
+        aio.write_nifti(argv[2], out.reshape(out.shape[:-2] + (dim * dim,)), g, vector=True)
+    else:
+        aio.write_nifti(argv[2], out, g)
+    return 0
+
+
 def main(argv: Optional[Sequence[str]] = None) -> int:
     argv = list(sys.argv[1:] if argv is None else argv)
-    cmds = {"registration": cmd_registration, "apply": cmd_apply, "apply-points": cmd_apply_points}
+    cmds = {"registration": cmd_registration, "apply": cmd_apply, "apply-points": cmd_apply_points,
+            "jacobian": cmd_jacobian}
     if not argv or argv[0] not in cmds:
         print(__doc__)
         return 2

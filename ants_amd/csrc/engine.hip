@@ -2742,6 +2742,34 @@ extern "C" int ants_apply_transforms(const ants_image_desc* ref, const ants_imag
   return ANTS_OK;
 }
 
+extern "C" int ants_jacobian_determinant(const ants_image_desc* grid, const float* d_field, int do_log,
+                                        float* d_out, float* d_gradient, void* stream) {
+  GeomDev g;
+  ANTS_TRY(geom_from_desc(grid, &g));
+  if (!d_field || !d_out) return fail(ANTS_ERR_INVALID, "null field or output buffer");
+  const int D = g.D;
+  JacobianDev J;
+  std::memset(&J, 0, sizeof(J));
+  J.D = D;
+  for (int k = 0; k < 3; ++k) {
+    J.n[k] = g.n[k];
+    J.isp[k] = k < D ? 1.0 / grid->spacing[k] : 0.0;
+  }
+  bool all_zero = true;
+  for (int i = 0; i < D * D; ++i)
+    if (grid->direction[i] != 0.0) all_zero = false;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c)
+      J.R[r * 3 + c] = (r < D && c < D && !all_zero) ? grid->direction[r * D + c] : (r == c ? 1.0 : 0.0);
+  GridDev go{};
+  go.D = D;
+  for (int k = 0; k < 3; ++k) go.n[k] = g.n[k];
+  set_divisors(go);
+  launch_jacobian(go, J, d_field, do_log ? 1 : 0, d_out, d_gradient, static_cast<hipStream_t>(stream));
+  HIP_TRY(hipGetLastError());
+  return ANTS_OK;
+}
+
 extern "C" int ants_apply_transforms_to_points(int dim, const double* d_in, int64_t n_points,
                                                const ants_transform* transforms, int n_transforms, double* d_out,
                                                void* stream) {

@@ -3787,6 +3787,74 @@ void launch_apply_transforms(const GridDev& go, const ApplyDev& P, int interp, c
 #undef ANTS_APPLY_LAUNCH
 }
 
+// CreateJacobianDeterminantImage over a displacement field u (D physical
+// components per voxel): F[i][j] = (-u_i(x+2e_j) + 8 u_i(x+e_j) - 8 u_i(x-e_j)
+// + u_i(x-2e_j)) / 12 / spacing_j with neighbours clamped to the buffer
+// (zero-flux Neumann), each row then rotated by the direction matrix
+// (row_i <- R row_i), plus the identity; out = max(det F, 0), or
+// log(max(det F, 0.001)) with `logj`; grad (nullable) receives F row-major.
+
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_jacobian(GridDev g, JacobianDev J, const float* __restrict__ u,
+                                                     int logj, float* __restrict__ out, float* __restrict__ grad) {
+  const int64_t N = g.count();
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i >= N) return;
+  int x, y, z;
+  unlin(g, i, x, y, z);
+  const int ix[3] = {x, y, z};
+  double F[3][3];
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    int64_t off[4];
+    const int dd[4] = {2, 1, -1, -2};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      int q[3] = {ix[0], ix[1], ix[2]};
+      q[j] = min(max(q[j] + dd[k], 0), J.n[j] - 1);
+      off[k] = (static_cast<int64_t>(q[2]) * J.n[1] + q[1]) * J.n[0] + q[0];
+    }
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      const double n2 = u[off[0] * D + c], n1 = u[off[1] * D + c], p1 = u[off[2] * D + c], p2 = u[off[3] * D + c];
+      F[c][j] = J.isp[j] * (-n2 + 8.0 * n1 - 8.0 * p1 + p2) / 12.0;
+    }
+  }
+  double G[3][3];
+#pragma unroll
+  for (int r = 0; r < D; ++r)
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < D; ++k) acc = acc + J.R[c * 3 + k] * F[r][k];
+      G[r][c] = acc + (r == c ? 1.0 : 0.0);
+    }
+  if (grad) {
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+      for (int c = 0; c < D; ++c) grad[i * D * D + r * D + c] = static_cast<float>(G[r][c]);
+  }
+  double det;
+  if constexpr (D == 2) {
+    det = G[0][0] * G[1][1] - G[0][1] * G[1][0];
+  } else {
+    det = G[0][0] * (G[1][1] * G[2][2] - G[1][2] * G[2][1]) - G[0][1] * (G[1][0] * G[2][2] - G[1][2] * G[2][0]) +
+          G[0][2] * (G[1][0] * G[2][1] - G[1][1] * G[2][0]);
+  }
+  out[i] = static_cast<float>(logj ? log(fmax(fmax(det, 0.0), 0.001)) : fmax(det, 0.0));
+}
+
+void launch_jacobian(const GridDev& g, const JacobianDev& J, const float* u, int logj, float* out, float* grad,
+                     hipStream_t s) {
+  const dim3 grid(blocks_for(g.count()));
+  if (g.D == 2)
+    hipLaunchKernelGGL(k_jacobian<2>, grid, dim3(kBlock), 0, s, g, J, u, logj, out, grad);
+  else
+    hipLaunchKernelGGL(k_jacobian<3>, grid, dim3(kBlock), 0, s, g, J, u, logj, out, grad);
+}
+
 void launch_transform_points(int D, const ApplyDev& P, const double* in, double* out, int64_t n, hipStream_t s) {
   if (n <= 0) return;
   const dim3 grid(blocks_for(n));

@@ -290,6 +290,17 @@ struct ApplyDev {
 // radius 3 (w: 0 cosine, 1 Hamming, 2 Lanczos, 3 Blackman, 4 Welch)
 void launch_apply_transforms(const GridDev& go, const ApplyDev& P, int interp, const float* in, float* out,
                              hipStream_t s);
+// CreateJacobianDeterminantImage: det of I + the direction-rotated
+// fourth-order central-difference gradient of a displacement field, clamped
+// at 0 (log(max(., 0.001)) with logj); grad (nullable): the D x D matrices.
+struct JacobianDev {
+  int D;
+  int n[3];
+  double isp[3];
+  double R[9];
+};
+void launch_jacobian(const GridDev& g, const JacobianDev& J, const float* u, int logj, float* out, float* grad,
+                     hipStream_t s);
 // CompositeTransform::TransformPoint of n points (D doubles each, in place
 // allowed) through P's transform list.
 void launch_transform_points(int D, const ApplyDev& P, const double* in, double* out, int64_t n, hipStream_t s);

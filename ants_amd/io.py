@@ -96,13 +96,14 @@ def _geometry_to_ras(geom: Geometry) -> Tuple[np.ndarray, np.ndarray]:
 
 def write_nifti(path: str, data: np.ndarray, geom: Geometry, vector: bool = False) -> None:
     """Write a scalar image (numpy (z,y,x) / (y,x)) or, with vector=True, a
-    displacement field ((z,y,x,D) interleaved) as a NIfTI-1 single file."""
+    vector image ((z,y,x,K) interleaved: a displacement field has K = D) as
+    a NIfTI-1 single file."""
     D = geom.dim
     a = np.asarray(data)
     if vector:
-        if a.shape != geom.shape + (D,):
-            raise ValueError(f"field shape {a.shape} does not match {geom.shape + (D,)}")
-        planar = np.moveaxis(a, -1, 0)  # (D, z, y, x): component slowest
+        if a.ndim != D + 1 or a.shape[:-1] != geom.shape:
+            raise ValueError(f"vector image shape {a.shape} does not match {geom.shape} + (components,)")
+        planar = np.moveaxis(a, -1, 0)  # (K, z, y, x): component slowest
     else:
         if a.shape != geom.shape:
             raise ValueError(f"image shape {a.shape} does not match {geom.shape}")
@@ -112,7 +113,8 @@ def write_nifti(path: str, data: np.ndarray, geom: Geometry, vector: bool = Fals
     code = _CODES[planar.dtype]
     size = list(geom.size) + [1] * (3 - D)
     sp = list(geom.spacing if geom.spacing is not None else [1.0] * D) + [1.0] * (3 - D)
-    dims = [5, size[0], size[1], size[2], 1, D, 1, 1] if vector else [D, size[0], size[1], size[2] if D == 3 else 1,
+    dims = [5, size[0], size[1], size[2], 1, a.shape[-1] if vector else 1, 1, 1] if vector else \
+        [D, size[0], size[1], size[2] if D == 3 else 1,
                                                                        1, 1, 1, 1]
     R, org = _geometry_to_ras(geom)
     b, c, d, qfac = _rotation_to_quatern(R)
@@ -142,7 +144,7 @@ def write_nifti(path: str, data: np.ndarray, geom: Geometry, vector: bool = Fals
 def read_nifti(path: str) -> Tuple[np.ndarray, Geometry]:
     """Read a NIfTI-1 image.  Scalars come back as numpy (z,y,x) / (y,x) in
     the file's data type (scl_slope/inter applied as float32 when set);
-    5-D vector images as (z,y,x,D) interleaved.  Geometry in LPS."""
+    5-D vector images as (z,y,x,K) interleaved.  Geometry in LPS."""
     with _open(path, "rb") as fh:
         raw = fh.read()
     if len(raw) < 348:
@@ -164,7 +166,9 @@ def read_nifti(path: str) -> Tuple[np.ndarray, Geometry]:
     if vector and intent not in (0, NIFTI_INTENT_VECTOR):
         raise ValueError(f"unsupported 5-D intent {intent}")
     n = [max(int(v), 1) for v in dims[1:4]]
-    D = 2 if (n[2] == 1 and (nd == 2 or (vector and dims[5] == 2))) else 3
+    # a single-slice vector image is 2-D when it holds 2 (a 2-D field) or 4
+    # (a 2 x 2 matrix) components
+    D = 2 if (n[2] == 1 and (nd == 2 or (vector and dims[5] in (2, 4)))) else 3
     ncomp = dims[5] if vector else 1
     dt = np.dtype(_DTYPES[code]).newbyteorder(end)
     count = n[0] * n[1] * n[2] * ncomp

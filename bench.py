@@ -276,6 +276,23 @@ def bench_apply(args):
         res[interp] = {"ms": ms, "voxels_per_s": N / (ms * 1e-3),
                        "achieved_gbs": N * bytes_per_voxel / (ms * 1e-3) / 1e9}
         log(f"{interp}: {ms:.3f} ms per 192x224x192 resample")
+    # CreateJacobianDeterminantImage of the same field: 12 bytes read per
+    # voxel (its neighbours come from the caches) + 4 written
+    import ctypes as C
+    d_jac = L.DeviceArray(g.shape)
+    gdesc = g.desc()
+    for _ in range(2):
+        L.check(L.lib().ants_jacobian_determinant(C.byref(gdesc), d_fld.ptr, 0, d_jac.ptr, None, None))
+    L.check(L.lib().ants_device_synchronize())
+    e0.record()
+    for _ in range(10):
+        L.check(L.lib().ants_jacobian_determinant(C.byref(gdesc), d_fld.ptr, 0, d_jac.ptr, None, None))
+    e1.record()
+    L.check(L.lib().ants_device_synchronize())
+    jms = e0.elapsed_ms(e1) / 10
+    res["CreateJacobianDeterminantImage"] = {"ms": jms, "voxels_per_s": N / (jms * 1e-3),
+                                             "achieved_gbs": N * 16 / (jms * 1e-3) / 1e9}
+    log(f"CreateJacobianDeterminantImage: {jms:.3f} ms per 192x224x192 field")
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))

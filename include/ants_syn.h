@@ -427,6 +427,20 @@ int ants_apply_transforms(const ants_image_desc* ref, const ants_image_desc* in,
                           const ants_transform* transforms, int n_transforms, int interpolation,
                           double default_value, float* d_out, void* stream);
 
+/* CreateJacobianDeterminantImage dim field out [doLogJacobian]
+ * (Examples/CreateJacobianDeterminantImage.cxx:60-160 with
+ * Utilities/itkDeformationFieldGradientTensorImageFilter.hxx and
+ * Utilities/itkDeterminantTensorImageFilter.hxx): per voxel of the
+ * displacement field d_field (grid `grid`, dim interleaved physical
+ * components) the gradient by the five-point central difference with the
+ * neighbours clamped to the buffer, divided by the spacing, each row rotated
+ * by the grid's direction, plus the identity; d_out = max(det, 0), or
+ * log(max(det, 0.001)) when do_log.  d_gradient (nullable; dim*dim floats
+ * a voxel, row-major) receives that matrix: the tool's deformationGradient
+ * output.  Arithmetic in double, outputs float.  Asynchronous on `stream`. */
+int ants_jacobian_determinant(const ants_image_desc* grid, const float* d_field, int do_log, float* d_out,
+                              float* d_gradient, void* stream);
+
 /* antsApplyTransformsToPoints (Examples/antsApplyTransformsToPoints.cxx:145-190):
  * each of n_points physical points (dim doubles at d_in + i * dim) through
  * the transform list in -t order, CompositeTransform::TransformPoint, into

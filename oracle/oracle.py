@@ -537,6 +537,22 @@ def apply_transforms(ref: OgGeom, inp: OgGeom, img, transforms=(), interpolation
     return out
 
 
+def jacobian(g: OgGeom, field, do_log=False, gradient=False):
+    """CreateJacobianDeterminantImage of a (z, y, x, D) field on grid g:
+    the clamped determinant (or its log), or with `gradient` the matrices."""
+    D = g.dim
+    f = np.ascontiguousarray(field, dtype=np.float32)
+    shape = f.shape[:-1]
+    out = np.empty(shape, np.float64)
+    grad = np.empty(shape + (D, D), np.float64) if gradient else None
+    fn = lib().og_jacobian_f64
+    fn.restype = None
+    fn.argtypes = [C.POINTER(OgGeom), C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    fn(C.byref(g), f.ctypes.data_as(C.POINTER(C.c_float)), int(bool(do_log)), out.ctypes.data_as(C.POINTER(C.c_double)),
+       grad.ctypes.data_as(C.POINTER(C.c_double)) if gradient else None)
+    return grad if gradient else out
+
+
 def transform_points(points, transforms=(), dim=3):
     """The first `dim` columns of each row through the transform list."""
     pts = np.ascontiguousarray(np.asarray(points, np.float64)[:, :dim])

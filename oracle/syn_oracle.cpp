@@ -1506,6 +1506,76 @@ int og_apply_transforms_f64(const og_geom* ref, const og_geom* in, const float* 
   return 0;
 }
 
+// CreateJacobianDeterminantImage (Examples/CreateJacobianDeterminantImage.cxx:60-160):
+// DeformationFieldGradientTensorImageFilter (Utilities/itkDeformationField
+// GradientTensorImageFilter.hxx:113-185: the five-point difference
+// (-u(x+2) + 8u(x+1) - 8u(x-1) + u(x-2)) / 12 per axis over the zero-flux
+// Neumann neighbourhood, weight 1/spacing, rows rotated by the direction,
+// + I), DeterminantTensorImageFilter, then max(det, 0) or
+// log(max(det, 0.001)).  `field`: dim components per voxel; grad (nullable)
+// receives the matrices row-major.  Parity: pinned by test_oracle_apply.py
+// (linear and cubic fields on oblique grids against their analytic
+// Jacobians, the clamped-stencil edge value by hand, the clamps).
+void og_jacobian_f64(const og_geom* g, const float* field, int do_log, double* out, double* grad) {
+  const int D = g->dim;
+  const int64_t n[3] = {g->n[0], g->n[1], D == 3 ? g->n[2] : 1};
+  const int64_t N = n[0] * n[1] * n[2];
+  bool all_zero = true;
+  for (int k = 0; k < D * D; ++k)
+    if (g->direction[k] != 0.0) all_zero = false;
+#pragma omp parallel for schedule(static)
+  for (int64_t v = 0; v < N; ++v) {
+    const int64_t ix[3] = {v % n[0], (v / n[0]) % n[1], v / (n[0] * n[1])};
+    double F[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int j = 0; j < D; ++j) {
+      auto at = [&](int64_t step, int c) {
+        int64_t q[3] = {ix[0], ix[1], ix[2]};
+        q[j] = std::min(std::max(q[j] + step, int64_t{0}), n[j] - 1);
+        return static_cast<double>(field[((q[2] * n[1] + q[1]) * n[0] + q[0]) * D + c]);
+      };
+      for (int c = 0; c < D; ++c)
+        F[c][j] = (1.0 / g->spacing[j]) * (-at(2, c) + 8.0 * at(1, c) - 8.0 * at(-1, c) + at(-2, c)) / 12.0;
+    }
+    double G[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    for (int r = 0; r < D; ++r)
+      for (int c = 0; c < D; ++c) {
+        double acc = 0.0;
+        for (int k = 0; k < D; ++k) {
+          const double Rck = all_zero ? (c == k ? 1.0 : 0.0) : g->direction[c * D + k];
+          acc += Rck * F[r][k];
+        }
+        G[r][c] = acc + (r == c ? 1.0 : 0.0);
+      }
+    if (grad)
+      for (int r = 0; r < D; ++r)
+        for (int c = 0; c < D; ++c) grad[v * D * D + r * D + c] = G[r][c];
+    // determinant by Gaussian elimination with partial pivoting
+    double M[3][3];
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) M[r][c] = G[r][c];
+    double det = 1.0;
+    for (int c = 0; c < D; ++c) {
+      int piv = c;
+      for (int r = c + 1; r < D; ++r)
+        if (std::fabs(M[r][c]) > std::fabs(M[piv][c])) piv = r;
+      if (M[piv][c] == 0.0) {
+        det = 0.0;
+        break;
+      }
+      if (piv != c) {
+        for (int k = 0; k < 3; ++k) std::swap(M[c][k], M[piv][k]);
+        det = -det;
+      }
+      det *= M[c][c];
+      for (int r = c + 1; r < D; ++r) {
+        const double f = M[r][c] / M[c][c];
+        for (int k = c; k < D; ++k) M[r][k] -= f * M[c][k];
+      }
+    }
+    out[v] = do_log ? std::log(std::max(std::max(det, 0.0), 0.001)) : std::max(det, 0.0);
+  }
+}
+
 // CompositeTransform::TransformPoint of n points (dim doubles each), as
 // antsApplyTransformsToPoints calls it (Examples/antsApplyTransformsToPoints.cxx:166-180).
 void og_transform_points_f64(int dim, const double* in, int64_t n, const og_xform* xf, int nt, double* out) {

@@ -278,3 +278,26 @@ def test_apply_points_csv(tmp_path):
     _, got = aio.read_points_csv(out)
     want = apply_transforms_to_points(got[:, :5] * 0 + pts, cli.load_transforms(fpath) + cli.load_transforms(apath), 3)
     np.testing.assert_array_equal(got[:, :5], want)
+
+
+def test_jacobian_cli(tmp_path):
+    """jacobian D field out [log] [0] [gradient]: files equal the library
+    call; the gradient form is written as a D*D-component vector image."""
+    from ants_amd.analysis import create_jacobian_determinant_image
+    rng = np.random.default_rng(12)
+    g = Geometry((20, 18, 16), (1.1, 1.0, 0.9), (0.0, 0.0, 0.0), np.eye(3))
+    fld = (0.5 * rng.standard_normal(g.shape + (3,))).astype(np.float32)
+    fpath = str(tmp_path / "w.nii.gz")
+    aio.write_nifti(fpath, fld, g, vector=True)
+    fld, g = aio.read_nifti(fpath)
+    out = str(tmp_path / "jac.nii.gz")
+    for log in (0, 1):
+        assert cli.main(["jacobian", "3", fpath, out, str(log)]) == 0
+        got, _ = aio.read_nifti(out)
+        np.testing.assert_array_equal(got, create_jacobian_determinant_image(fld, g, do_log=bool(log)))
+    assert cli.main(["jacobian", "3", fpath, out, "0", "0", "1"]) == 0
+    got, _ = aio.read_nifti(out)
+    want = create_jacobian_determinant_image(fld, g, deformation_gradient=True)
+    np.testing.assert_array_equal(got, want.reshape(g.shape + (9,)))
+    with pytest.raises(SystemExit):
+        cli.main(["jacobian", "3", fpath, out, "0", "1"])

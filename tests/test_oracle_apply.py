@@ -345,3 +345,58 @@ def test_transform_points_closed_forms():
     fld = o.compose_field(g, xs).reshape(-1, 3)
     np.testing.assert_allclose(o.transform_points(p, xs), p + fld, rtol=0, atol=2e-6)
     np.testing.assert_array_equal(o.transform_points(p[:0], xs).shape, (0, 3))
+
+
+def _phys(size, sp, org, R):
+    return points(size, sp, org, R)
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_jacobian_linear_and_cubic_fields(D):
+    """CreateJacobianDeterminantImage: u(p) = (A - I) p on an oblique,
+    anisotropic grid gives det A at every voxel two or more from the faces
+    (the five-point stencil is exact there; a missing direction rotation or a
+    transposed one fails on the oblique grid); a cubic field gives its
+    analytic Jacobian in the interior; the gradient output is I + du/dp."""
+    r = np.random.default_rng(20 + D)
+    size = (14, 12, 10)[:D]
+    sp = (1.3, 0.8, 1.1)[:D]
+    org = (2.0, -1.0, 3.0)[:D]
+    R = rot3(0.3, -0.2, 0.4) if D == 3 else np.array([[np.cos(0.5), -np.sin(0.5)], [np.sin(0.5), np.cos(0.5)]])
+    g = o.og_geom(size, sp, org, R)
+    p = _phys(size, sp, org, R)
+    A = np.eye(D) + 0.1 * r.standard_normal((D, D))
+    u = (p @ (A - np.eye(D)).T).astype(np.float64)
+    core = (slice(2, -2),) * D
+    J = o.jacobian(g, u.astype(np.float32))
+    np.testing.assert_allclose(J[core], np.linalg.det(A), rtol=0, atol=2e-5)
+    G = o.jacobian(g, u.astype(np.float32), gradient=True)
+    np.testing.assert_allclose(G[core], np.broadcast_to(A, G[core].shape), rtol=0, atol=2e-5)
+    # cubic: u_0 = a p_1^3, derivative 3 a p_1^2 along axis 1
+    a = 2e-4
+    u3 = np.zeros_like(u)
+    u3[..., 0] = a * p[..., 1] ** 3
+    G3 = o.jacobian(g, u3.astype(np.float32), gradient=True)
+    want = np.zeros(G3.shape)
+    want[..., 0, 1] = 3 * a * p[..., 1] ** 2
+    want += np.eye(D)
+    np.testing.assert_allclose(G3[core], want[core], rtol=0, atol=2e-4 * np.abs(want).max())
+
+
+def test_jacobian_edge_stencil_and_clamps():
+    """At x = 0 with u_x = a x (unit spacing) the clamped stencil gives
+    (-2a + 8a - 8*0 + 0) / 12 = a/2; folding fields clamp to 0, the log to
+    log(0.001)."""
+    g = o.og_geom((8, 6), (1.0, 1.0), (0.0, 0.0))
+    u = np.zeros((6, 8, 2), np.float32)
+    a = 0.25                                     # exact in float
+    u[..., 0] = a * np.arange(8.0)
+    G = o.jacobian(g, u, gradient=True)
+    assert G[3, 0, 0, 0] == pytest.approx(1 + a / 2, abs=1e-12)
+    assert G[3, 1, 0, 0] == pytest.approx(1 + (-3 * a + 8 * 2 * a - 8 * 0 + 0) / 12, abs=1e-12)
+    assert G[3, 4, 0, 0] == pytest.approx(1 + a, abs=1e-12)
+    u[..., 0] = -2.0 * np.arange(8.0)            # J = 1 - 2 < 0 in the interior
+    assert o.jacobian(g, u)[3, 4] == 0.0
+    assert o.jacobian(g, u, do_log=True)[3, 4] == pytest.approx(np.log(0.001))
+    u[..., 0] = 0.5 * np.arange(8.0)             # J = 1.5
+    assert o.jacobian(g, u, do_log=True)[3, 4] == pytest.approx(np.log(1.5), abs=1e-12)
