@@ -167,7 +167,7 @@ def lib():
     L.ants_apply_transforms.argtypes = [pdesc, pdesc, vp, C.POINTER(TransformC), i, i, d, vp, vp]
     L.ants_compose_transforms_to_field.argtypes = [pdesc, C.POINTER(TransformC), i, vp, vp]
     L.ants_apply_transforms_to_points.argtypes = [i, vp, C.c_int64, C.POINTER(TransformC), i, vp, vp]
-    L.ants_jacobian_determinant.argtypes = [pdesc, vp, i, vp, vp, vp]
+    L.ants_jacobian_determinant.argtypes = [pdesc, vp, i, i, vp, vp, vp]
     L.ants_syn_set_initial_transforms.argtypes = [vp, C.POINTER(TransformC), i, C.POINTER(TransformC), i]
     L.ants_syn_set_initial_state_device.argtypes = [vp, pdesc, vp, vp, vp, vp]
     L.ants_tuning_set.argtypes = [i, i, i]

@@ -31,13 +31,16 @@ def field_errors(a: np.ndarray, b: np.ndarray, spacing) -> dict:
             "jac_max_abs": float(np.abs(ja - jb).max()), "jac_mean_abs": float(np.abs(ja - jb).mean())}
 
 
-def create_jacobian_determinant_image(field, geometry, do_log: bool = False, deformation_gradient: bool = False):
+def create_jacobian_determinant_image(field, geometry, do_log: bool = False, deformation_gradient: bool = False,
+                                      geometric: bool = False):
     """CreateJacobianDeterminantImage dim field out [doLogJacobian] [0]
     [deformationGradient] (Examples/CreateJacobianDeterminantImage.cxx:60-160):
     max(det(I + R-rotated five-point gradient), 0), or its log clamped at
     0.001, on the GPU (ants_jacobian_determinant).  ``field``: numpy or
     DeviceArray of shape geometry.shape + (dim,).  With
-    ``deformation_gradient`` returns the (..., dim, dim) matrices instead."""
+    ``deformation_gradient`` returns the (..., dim, dim) matrices instead;
+    ``geometric`` is useGeometric (GeometricJacobianDeterminantImageFilter:
+    moved-simplex volume ratios)."""
     import ctypes as C
     from . import _lib as L
     D = geometry.dim
@@ -46,6 +49,8 @@ def create_jacobian_determinant_image(field, geometry, do_log: bool = False, def
         raise ValueError("field shape does not match its geometry")
     out = L.DeviceArray(geometry.shape)
     grad = L.DeviceArray(tuple(geometry.shape) + (D, D)) if deformation_gradient else None
-    L.check(L.lib().ants_jacobian_determinant(C.byref(geometry.desc()), f.ptr, int(bool(do_log)), out.ptr,
-                                              grad.ptr if grad is not None else None, None))
+    if geometric and deformation_gradient:
+        raise ValueError("the geometric form has no gradient output")
+    L.check(L.lib().ants_jacobian_determinant(C.byref(geometry.desc()), f.ptr, int(bool(do_log)), int(bool(geometric)),
+                                              out.ptr, grad.ptr if grad is not None else None, None))
     return grad.to_host() if deformation_gradient else out.to_host()

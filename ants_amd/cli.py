@@ -48,8 +48,7 @@ interpolation of order 0..5.
 the other columns copied.
 
 ``python -m ants_amd.cli jacobian 3 out_0Warp.nii.gz jac.nii.gz [doLog]
-  [useGeometric] [deformationGradient]`` is CreateJacobianDeterminantImage
-(the geometric form is not built).
+  [useGeometric] [deformationGradient]`` is CreateJacobianDeterminantImage.
 
 ``registration ... -a 1`` writes ``out_Composite.h5`` and
 ``out_InverseComposite.h5`` (--write-composite-transform) instead of the
@@ -434,13 +433,13 @@ def cmd_jacobian(argv: Sequence[str]) -> int:
                          "[useGeometric=0] [deformationGradient=0]")
     dim = int(argv[0])
     flags = [int(v) for v in argv[3:6]] + [0] * (3 - len(argv[3:6]))
-    if flags[1]:
-        raise SystemExit("useGeometric (GeometricJacobianDeterminantImageFilter) is not supported")
+    if flags[1] and flags[2]:
+        raise SystemExit("useGeometric has no deformation-gradient output")
     fld, g = aio.read_nifti(argv[1])
     if g.dim != dim:
         raise SystemExit(f"imageDimension {dim} but the field is {g.dim}-D")
     out = create_jacobian_determinant_image(np.asarray(fld, np.float32), g, do_log=bool(flags[0]),
-                                            deformation_gradient=bool(flags[2]))
+                                            deformation_gradient=bool(flags[2]), geometric=bool(flags[1]))
     if flags[2]:
         # row-major J[0][0], J[0][1], ... as a vector image
         aio.write_nifti(argv[2], out.reshape(out.shape[:-2] + (dim * dim,)), g, vector=True)

@@ -2743,10 +2743,11 @@ extern "C" int ants_apply_transforms(const ants_image_desc* ref, const ants_imag
 }
 
 extern "C" int ants_jacobian_determinant(const ants_image_desc* grid, const float* d_field, int do_log,
-                                        float* d_out, float* d_gradient, void* stream) {
+                                        int geometric, float* d_out, float* d_gradient, void* stream) {
   GeomDev g;
   ANTS_TRY(geom_from_desc(grid, &g));
   if (!d_field || !d_out) return fail(ANTS_ERR_INVALID, "null field or output buffer");
+  if (geometric && d_gradient) return fail(ANTS_ERR_INVALID, "the geometric form has no gradient output");
   const int D = g.D;
   JacobianDev J;
   std::memset(&J, 0, sizeof(J));
@@ -2765,7 +2766,32 @@ extern "C" int ants_jacobian_determinant(const ants_image_desc* grid, const floa
   go.D = D;
   for (int k = 0; k < 3; ++k) go.n[k] = g.n[k];
   set_divisors(go);
-  launch_jacobian(go, J, d_field, do_log ? 1 : 0, d_out, d_gradient, static_cast<hipStream_t>(stream));
+  if (geometric) {
+    // the unmoved simplex: index offsets through direction and spacing
+    const double s3 = 0.25 * std::sqrt(3.0), t2 = 0.5 / std::sqrt(2.0);
+    const double o2[3][2] = {{0.0, s3}, {-0.5, -s3}, {0.5, -s3}};
+    const double o3[4][3] = {{0.5, 0.0, -t2}, {-0.5, 0.0, -t2}, {0.0, 0.5, t2}, {0.0, -0.5, t2}};
+    double q[4][3] = {};
+    for (int v = 0; v <= D; ++v)
+      for (int r = 0; r < D; ++r)
+        for (int c = 0; c < D; ++c) q[v][r] += J.R[r * 3 + c] * grid->spacing[c] * (D == 2 ? o2[v][c] : o3[v][c]);
+    double vol;
+    if (D == 2) {
+      vol = 0.5 * std::fabs((q[0][0] - q[1][0]) * (q[0][1] - q[2][1]) - (q[0][0] - q[2][0]) * (q[0][1] - q[1][1]));
+    } else {
+      double a[3], b[3], c[3];
+      for (int r = 0; r < 3; ++r) {
+        a[r] = q[0][r] - q[3][r];
+        b[r] = q[1][r] - q[3][r];
+        c[r] = q[2][r] - q[3][r];
+      }
+      vol = std::fabs(a[0] * (b[1] * c[2] - b[2] * c[1]) + a[1] * (b[2] * c[0] - b[0] * c[2]) +
+                      a[2] * (b[0] * c[1] - b[1] * c[0])) / 6.0;
+    }
+    launch_jacobian_geometric(go, J, d_field, do_log ? 1 : 0, vol, d_out, static_cast<hipStream_t>(stream));
+  } else {
+    launch_jacobian(go, J, d_field, do_log ? 1 : 0, d_out, d_gradient, static_cast<hipStream_t>(stream));
+  }
   HIP_TRY(hipGetLastError());
   return ANTS_OK;
 }

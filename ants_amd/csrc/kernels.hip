@@ -3846,6 +3846,100 @@ __global__ void __launch_bounds__(kBlock) k_jacobian(GridDev g, JacobianDev J, c
   out[i] = static_cast<float>(logj ? log(fmax(fmax(det, 0.0), 0.001)) : fmax(det, 0.0));
 }
 
+// GeometricJacobianDeterminantImageFilter: the simplex around each voxel
+// (triangle: index offsets (0, s), (-1/2, -s), (1/2, -s), s = sqrt(3)/4;
+// tetrahedron: (1/2, 0, -t), (-1/2, 0, -t), (0, 1/2, t), (0, -1/2, t),
+// t = 1/(2 sqrt 2)), each vertex moved by the field linearly interpolated
+// there (corner indices clamped to the buffer), the moved simplex's
+// |volume| over the unmoved one; log(max(., 0.001)) with `logj`.  Vertex
+// positions are taken relative to the voxel's point (the common term
+// cancels in the volume).
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_jacobian_geometric(GridDev g, JacobianDev J, const float* __restrict__ u,
+                                                               int logj, double undisplaced,
+                                                               float* __restrict__ out) {
+  const int64_t N = g.count();
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i >= N) return;
+  int x, y, z;
+  unlin(g, i, x, y, z);
+  const int ix[3] = {x, y, z};
+  constexpr int NV = D + 1;
+  const double s3 = 0.25 * 1.7320508075688772935, t2 = 0.5 / 1.4142135623730950488;
+  const double offs2[3][2] = {{0.0, s3}, {-0.5, -s3}, {0.5, -s3}};
+  const double offs3[4][3] = {{0.5, 0.0, -t2}, {-0.5, 0.0, -t2}, {0.0, 0.5, t2}, {0.0, -0.5, t2}};
+  double q[NV][3];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    double off[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int d = 0; d < D; ++d) off[d] = D == 2 ? offs2[v][d] : offs3[v][d];
+    // interpolate u at ix + off: per axis floor, fraction, clamped corners
+    int64_t lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    double w[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const double c = static_cast<double>(ix[d]) + off[d];
+      const double fl = floor(c);
+      w[d] = c - fl;
+      const int64_t b = static_cast<int64_t>(fl);
+      lo[d] = min(max(b, int64_t{0}), static_cast<int64_t>(J.n[d] - 1));
+      hi[d] = min(max(b + 1, int64_t{0}), static_cast<int64_t>(J.n[d] - 1));
+    }
+    double disp[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < (1 << D); ++k) {
+      double ov = 1.0;
+      int64_t id[3] = {0, 0, 0};
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        const bool up = (k >> d) & 1;
+        ov = ov * (up ? w[d] : 1.0 - w[d]);
+        id[d] = up ? hi[d] : lo[d];
+      }
+      const int64_t lin = (id[2] * J.n[1] + id[1]) * J.n[0] + id[0];
+#pragma unroll
+      for (int c = 0; c < D; ++c) disp[c] = disp[c] + ov * static_cast<double>(u[lin * D + c]);
+    }
+    // physical offset M off = R diag(spacing) off, plus the displacement
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+      double m = 0.0;
+#pragma unroll
+      for (int c = 0; c < D; ++c) m = m + J.R[r * 3 + c] * (off[c] / J.isp[c]);
+      q[v][r] = m + disp[r];
+    }
+  }
+  double vol;
+  if constexpr (D == 2) {
+    const double abx = q[0][0] - q[1][0], aby = q[0][1] - q[1][1];
+    const double acx = q[0][0] - q[2][0], acy = q[0][1] - q[2][1];
+    vol = 0.5 * fabs(abx * acy - acx * aby);
+  } else {
+    double ad[3], bd[3], cd[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      ad[r] = q[0][r] - q[3][r];
+      bd[r] = q[1][r] - q[3][r];
+      cd[r] = q[2][r] - q[3][r];
+    }
+    const double cx = bd[1] * cd[2] - bd[2] * cd[1], cy = bd[2] * cd[0] - bd[0] * cd[2],
+                 cz = bd[0] * cd[1] - bd[1] * cd[0];
+    vol = fabs(ad[0] * cx + ad[1] * cy + ad[2] * cz) / 6.0;
+  }
+  const double ratio = vol / undisplaced;
+  out[i] = static_cast<float>(logj ? log(fmax(ratio, 0.001)) : ratio);
+}
+
+void launch_jacobian_geometric(const GridDev& g, const JacobianDev& J, const float* u, int logj, double undisplaced,
+                               float* out, hipStream_t s) {
+  const dim3 grid(blocks_for(g.count()));
+  if (g.D == 2)
+    hipLaunchKernelGGL(k_jacobian_geometric<2>, grid, dim3(kBlock), 0, s, g, J, u, logj, undisplaced, out);
+  else
+    hipLaunchKernelGGL(k_jacobian_geometric<3>, grid, dim3(kBlock), 0, s, g, J, u, logj, undisplaced, out);
+}
+
 void launch_jacobian(const GridDev& g, const JacobianDev& J, const float* u, int logj, float* out, float* grad,
                      hipStream_t s) {
   const dim3 grid(blocks_for(g.count()));

@@ -301,6 +301,11 @@ struct JacobianDev {
 };
 void launch_jacobian(const GridDev& g, const JacobianDev& J, const float* u, int logj, float* out, float* grad,
                      hipStream_t s);
+// GeometricJacobianDeterminantImageFilter: |volume| of the voxel's simplex
+// moved by the linearly interpolated field over `undisplaced` (its volume
+// unmoved); log(max(., 0.001)) with logj.
+void launch_jacobian_geometric(const GridDev& g, const JacobianDev& J, const float* u, int logj, double undisplaced,
+                               float* out, hipStream_t s);
 // CompositeTransform::TransformPoint of n points (D doubles each, in place
 // allowed) through P's transform list.
 void launch_transform_points(int D, const ApplyDev& P, const double* in, double* out, int64_t n, hipStream_t s);

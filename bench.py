@@ -282,11 +282,11 @@ def bench_apply(args):
     d_jac = L.DeviceArray(g.shape)
     gdesc = g.desc()
     for _ in range(2):
-        L.check(L.lib().ants_jacobian_determinant(C.byref(gdesc), d_fld.ptr, 0, d_jac.ptr, None, None))
+        L.check(L.lib().ants_jacobian_determinant(C.byref(gdesc), d_fld.ptr, 0, 0, d_jac.ptr, None, None))
     L.check(L.lib().ants_device_synchronize())
     e0.record()
     for _ in range(10):
-        L.check(L.lib().ants_jacobian_determinant(C.byref(gdesc), d_fld.ptr, 0, d_jac.ptr, None, None))
+        L.check(L.lib().ants_jacobian_determinant(C.byref(gdesc), d_fld.ptr, 0, 0, d_jac.ptr, None, None))
     e1.record()
     L.check(L.lib().ants_device_synchronize())
     jms = e0.elapsed_ms(e1) / 10

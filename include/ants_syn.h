@@ -437,9 +437,13 @@ int ants_apply_transforms(const ants_image_desc* ref, const ants_image_desc* in,
  * by the grid's direction, plus the identity; d_out = max(det, 0), or
  * log(max(det, 0.001)) when do_log.  d_gradient (nullable; dim*dim floats
  * a voxel, row-major) receives that matrix: the tool's deformationGradient
- * output.  Arithmetic in double, outputs float.  Asynchronous on `stream`. */
-int ants_jacobian_determinant(const ants_image_desc* grid, const float* d_field, int do_log, float* d_out,
-                              float* d_gradient, void* stream);
+ * output.  geometric != 0 is useGeometric
+ * (Utilities/itkGeometricJacobianDeterminantImageFilter.hxx:88-335): the
+ * |volume| of each voxel's triangle / tetrahedron moved by the linearly
+ * interpolated field over its unmoved volume (no gradient output then).
+ * Arithmetic in double, outputs float.  Asynchronous on `stream`. */
+int ants_jacobian_determinant(const ants_image_desc* grid, const float* d_field, int do_log, int geometric,
+                              float* d_out, float* d_gradient, void* stream);
 
 /* antsApplyTransformsToPoints (Examples/antsApplyTransformsToPoints.cxx:145-190):
  * each of n_points physical points (dim doubles at d_in + i * dim) through

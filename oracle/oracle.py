@@ -537,12 +537,21 @@ def apply_transforms(ref: OgGeom, inp: OgGeom, img, transforms=(), interpolation
     return out
 
 
-def jacobian(g: OgGeom, field, do_log=False, gradient=False):
+def jacobian(g: OgGeom, field, do_log=False, gradient=False, geometric=False):
     """CreateJacobianDeterminantImage of a (z, y, x, D) field on grid g:
-    the clamped determinant (or its log), or with `gradient` the matrices."""
+    the clamped determinant (or its log), or with `gradient` the matrices;
+    `geometric`: the moved-simplex volume ratio."""
     D = g.dim
     f = np.ascontiguousarray(field, dtype=np.float32)
     shape = f.shape[:-1]
+    if geometric:
+        out = np.empty(shape, np.float64)
+        fn = lib().og_jacobian_geometric_f64
+        fn.restype = None
+        fn.argtypes = [C.POINTER(OgGeom), C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_double)]
+        fn(C.byref(g), f.ctypes.data_as(C.POINTER(C.c_float)), int(bool(do_log)),
+           out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
     out = np.empty(shape, np.float64)
     grad = np.empty(shape + (D, D), np.float64) if gradient else None
     fn = lib().og_jacobian_f64

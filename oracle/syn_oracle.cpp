@@ -1576,6 +1576,84 @@ void og_jacobian_f64(const og_geom* g, const float* field, int do_log, double* o
   }
 }
 
+// GeometricJacobianDeterminantImageFilter
+// (Utilities/itkGeometricJacobianDeterminantImageFilter.hxx:88-335): at each
+// voxel's physical point p the simplex p + M o_k (M = direction * spacing;
+// o_k the triangle (0, r3), (-1/2, -r3), (1/2, -r3), r3 = sqrt(3)/4, or the
+// tetrahedron (+-1/2, 0, -1/(2 sqrt 2)), (0, +-1/2, 1/(2 sqrt 2))), each
+// vertex displaced by the field at it (VectorLinearInterpolateImageFunction:
+// corners clamped to the buffer), |moved volume| / |unmoved volume|;
+// log(max(., 0.001)) with do_log.  Parity: pinned by test_oracle_apply.py
+// (zero and uniform fields give 1, a linear field |det A| away from the
+// faces, agreement with the gradient-tensor form on a smooth field).
+void og_jacobian_geometric_f64(const og_geom* g, const float* field, int do_log, double* out) {
+  const int D = g->dim;
+  const int64_t n[3] = {g->n[0], g->n[1], D == 3 ? g->n[2] : 1};
+  const int64_t N = n[0] * n[1] * n[2];
+  const double r3 = 0.25 * std::sqrt(3.0), t2 = 0.5 / std::sqrt(2.0);
+  const double tri[3][3] = {{0.0, r3, 0.0}, {-0.5, -r3, 0.0}, {0.5, -r3, 0.0}};
+  const double tet[4][3] = {{0.5, 0.0, -t2}, {-0.5, 0.0, -t2}, {0.0, 0.5, t2}, {0.0, -0.5, t2}};
+  auto volume = [&](const double P[4][3]) {
+    if (D == 2) {
+      const double ab0 = P[0][0] - P[1][0], ab1 = P[0][1] - P[1][1];
+      const double ac0 = P[0][0] - P[2][0], ac1 = P[0][1] - P[2][1];
+      return 0.5 * std::fabs(ab0 * ac1 - ac0 * ab1);
+    }
+    double a[3], b[3], c[3];
+    for (int r = 0; r < 3; ++r) {
+      a[r] = P[0][r] - P[3][r];
+      b[r] = P[1][r] - P[3][r];
+      c[r] = P[2][r] - P[3][r];
+    }
+    const double x = b[1] * c[2] - b[2] * c[1], y = b[2] * c[0] - b[0] * c[2], z = b[0] * c[1] - b[1] * c[0];
+    return std::fabs(a[0] * x + a[1] * y + a[2] * z) / 6.0;
+  };
+  // the unmoved simplex at the origin index
+  double base[4][3] = {};
+  {
+    const double zero[3] = {0.0, 0.0, 0.0};
+    double p0[3];
+    geom_index_to_point(*g, zero, p0);
+    for (int v = 0; v <= D; ++v) {
+      double pv[3];
+      geom_index_to_point(*g, D == 2 ? tri[v] : tet[v], pv);
+      for (int r = 0; r < 3; ++r) base[v][r] = pv[r] - p0[r];
+    }
+  }
+  const double undisplaced = volume(base);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < N; ++i) {
+    const double id[3] = {static_cast<double>(i % n[0]), static_cast<double>((i / n[0]) % n[1]),
+                          static_cast<double>(i / (n[0] * n[1]))};
+    double p[3];
+    geom_index_to_point(*g, id, p);
+    double P[4][3] = {};
+    for (int v = 0; v <= D; ++v) {
+      double q[3] = {0.0, 0.0, 0.0};
+      for (int r = 0; r < D; ++r) q[r] = p[r] + base[v][r];
+      double c[3];
+      geom_point_to_index(*g, q, c);
+      double disp[3] = {0.0, 0.0, 0.0};
+      for (int k = 0; k < (1 << D); ++k) {
+        double w = 1.0;
+        int64_t j[3] = {0, 0, 0};
+        for (int d = 0; d < D; ++d) {
+          const double fl = std::floor(c[d]);
+          const double t = c[d] - fl;
+          const int64_t b = static_cast<int64_t>(fl) + ((k >> d) & 1);
+          j[d] = std::min(std::max(b, int64_t{0}), n[d] - 1);
+          w *= ((k >> d) & 1) ? t : 1.0 - t;
+        }
+        const int64_t lin = (j[2] * n[1] + j[1]) * n[0] + j[0];
+        for (int r = 0; r < D; ++r) disp[r] += w * static_cast<double>(field[lin * D + r]);
+      }
+      for (int r = 0; r < D; ++r) P[v][r] = q[r] + disp[r];
+    }
+    const double ratio = volume(P) / undisplaced;
+    out[i] = do_log ? std::log(std::max(ratio, 0.001)) : ratio;
+  }
+}
+
 // CompositeTransform::TransformPoint of n points (dim doubles each), as
 // antsApplyTransformsToPoints calls it (Examples/antsApplyTransformsToPoints.cxx:166-180).
 void og_transform_points_f64(int dim, const double* in, int64_t n, const og_xform* xf, int nt, double* out) {

@@ -200,6 +200,13 @@ def test_jacobian_determinant_matches_oracle(D):
     assert (J == 0).mean() > 0.01 and (J > 0).mean() > 0.5
     G = create_jacobian_determinant_image(fld, gr, deformation_gradient=True)
     np.testing.assert_allclose(G, o.jacobian(ogr, fld, gradient=True), rtol=2e-6, atol=2e-6)
+    # useGeometric: the moved-simplex volume ratio (vertex offsets taken
+    # relative to the voxel on the GPU, absolute in the oracle: 1e-12 mm
+    # apart in the coordinates)
+    for do_log in (False, True):
+        got = create_jacobian_determinant_image(fld, gr, do_log=do_log, geometric=True)
+        ref = o.jacobian(ogr, fld, do_log=do_log, geometric=True)
+        np.testing.assert_allclose(got, ref, rtol=1e-5, atol=1e-5)
 
 
 def test_inverse_affine_round_trip():

@@ -299,5 +299,8 @@ def test_jacobian_cli(tmp_path):
     got, _ = aio.read_nifti(out)
     want = create_jacobian_determinant_image(fld, g, deformation_gradient=True)
     np.testing.assert_array_equal(got, want.reshape(g.shape + (9,)))
+    assert cli.main(["jacobian", "3", fpath, out, "0", "1"]) == 0
+    got, _ = aio.read_nifti(out)
+    np.testing.assert_array_equal(got, create_jacobian_determinant_image(fld, g, geometric=True))
     with pytest.raises(SystemExit):
-        cli.main(["jacobian", "3", fpath, out, "0", "1"])
+        cli.main(["jacobian", "3", fpath, out, "0", "1", "1"])

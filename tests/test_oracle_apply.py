@@ -400,3 +400,32 @@ def test_jacobian_edge_stencil_and_clamps():
     assert o.jacobian(g, u, do_log=True)[3, 4] == pytest.approx(np.log(0.001))
     u[..., 0] = 0.5 * np.arange(8.0)             # J = 1.5
     assert o.jacobian(g, u, do_log=True)[3, 4] == pytest.approx(np.log(1.5), abs=1e-12)
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_jacobian_geometric_closed_forms(D):
+    """useGeometric: zero and uniform fields give 1 everywhere; a linear
+    field (A - I) p gives |det A| wherever the simplex's interpolation
+    corners stay inside (one voxel from the faces) on an oblique grid; on a
+    smooth field it agrees with the gradient-tensor form to first order."""
+    r = np.random.default_rng(30 + D)
+    size = (12, 11, 10)[:D]
+    sp = (1.2, 0.9, 1.1)[:D]
+    org = (1.0, 2.0, -1.0)[:D]
+    R = rot3(0.2, 0.3, -0.1) if D == 3 else np.array([[np.cos(0.4), -np.sin(0.4)], [np.sin(0.4), np.cos(0.4)]])
+    g = o.og_geom(size, sp, org, R)
+    shape = tuple(reversed(size)) + (D,)
+    np.testing.assert_allclose(o.jacobian(g, np.zeros(shape, np.float32), geometric=True), 1.0, atol=1e-12)
+    np.testing.assert_allclose(o.jacobian(g, np.full(shape, 0.7, np.float32), geometric=True), 1.0, atol=1e-6)
+    p = points(size, sp, org, R)
+    A = np.eye(D) + 0.08 * r.standard_normal((D, D))
+    u = (p @ (A - np.eye(D)).T).astype(np.float32)
+    core = (slice(1, -1),) * D
+    Jg = o.jacobian(g, u, geometric=True)
+    np.testing.assert_allclose(Jg[core], abs(np.linalg.det(A)), rtol=0, atol=2e-5)
+    assert o.jacobian(g, u, geometric=True, do_log=True)[core] == pytest.approx(np.log(abs(np.linalg.det(A))), abs=2e-5)
+    x = p[..., 0] / 10.0
+    us = np.zeros(shape, np.float32)
+    us[..., 0] = 0.3 * np.sin(x)
+    Jt = o.jacobian(g, us)
+    np.testing.assert_allclose(Jg[core] * 0 + o.jacobian(g, us, geometric=True)[core], Jt[core], rtol=0, atol=3e-3)
