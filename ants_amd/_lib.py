@@ -215,6 +215,15 @@ class DeviceArray:
         check(lib().ants_memcpy_h2d(d.ptr, a.ctypes.data_as(C.c_void_p), a.nbytes, None))
         return d
 
+    def copy_from_host(self, a: np.ndarray):
+        """Overwrite the buffer with a host array of the same size."""
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        if a.nbytes != self.nbytes:
+            raise ValueError(f"host array of {a.nbytes} bytes for a {self.nbytes}-byte buffer")
+        check(lib().ants_memcpy_h2d(self.ptr, a.ctypes.data_as(C.c_void_p), a.nbytes, None))
+        check(lib().ants_stream_synchronize(None))
+        return self
+
     def to_host(self) -> np.ndarray:
         out = np.empty(self.shape, self.dtype)
         check(lib().ants_memcpy_d2h(out.ctypes.data_as(C.c_void_p), self.ptr, self.nbytes, None))

@@ -152,6 +152,32 @@ def compose_affines(transforms: Sequence[Optional[AffineTransform]]) -> AffineTr
     return AffineTransform(A, off, np.zeros(A.shape[0]))
 
 
+def apply_transforms_series(reference: Geometry, series, image_geometry: Geometry,
+                            transforms: Sequence[Transform] = (), interpolation: str = "Linear",
+                            default_value: float = 0.0) -> np.ndarray:
+    """antsApplyTransforms -e 3 (Examples/antsApplyTransforms.cxx:516-536,
+    903-960): every volume of a (t, z, y, x) series resampled onto
+    ``reference`` through the same transforms; returns (t,) + reference.shape.
+    The transform list is set up once; the volumes go through the device one
+    after another in one output buffer."""
+    a = np.asarray(series)
+    if a.ndim != image_geometry.dim + 1 or a.shape[1:] != image_geometry.shape:
+        raise ValueError(f"series shape {a.shape} does not match (t,) + {image_geometry.shape}")
+    D = reference.dim
+    xs, keep = transform_array(transforms, D)
+    code = parse_interpolation(interpolation)
+    d_in = L.DeviceArray(image_geometry.shape)
+    d_out = L.DeviceArray(reference.shape)
+    out = np.empty((a.shape[0],) + reference.shape, np.float32)
+    rdesc, idesc = reference.desc(), image_geometry.desc()
+    for k in range(a.shape[0]):
+        d_in.copy_from_host(np.ascontiguousarray(a[k], np.float32))
+        L.check(L.lib().ants_apply_transforms(C.byref(rdesc), C.byref(idesc), d_in.ptr, xs, len(transforms), code,
+                                              float(default_value), d_out.ptr, None))
+        out[k] = d_out.to_host()
+    return out
+
+
 def compose_transforms_to_field(reference: Geometry, transforms: Sequence[Transform] = (), out=None):
     """antsApplyTransforms -o [field,1] (Examples/antsApplyTransforms.cxx:706-760):
     the displacement T(p) - p of every reference point through the

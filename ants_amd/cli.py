@@ -41,6 +41,9 @@ with 1) and ``-o CompositeTransform[out.h5]`` the list as a composite file;
 those need no input image.  ``-n BSpline[3]`` selects B-spline
 interpolation of order 0..5.
 
+``apply -e 3`` resamples every volume of a 4-D time series (the output keeps
+the input's time spacing and offset).
+
 ``python -m ants_amd.cli apply-points -d 3 -i points.csv -o out.csv
   -t [affine.mat,1] -t out_0InverseWarp.nii.gz`` is antsApplyTransformsToPoints
 (Examples/antsApplyTransformsToPoints.cxx): the x,y[,z] columns of the CSV
@@ -152,8 +155,12 @@ def cmd_apply(argv: Sequence[str]) -> int:
     ap.add_argument("-t", "--transform", action="append", default=[])
     ap.add_argument("-n", "--interpolation", default="Linear")
     ap.add_argument("--default-value", type=float, default=0.0)
+    ap.add_argument("-e", "--input-image-type", type=int, default=0,
+                    help="0 scalar, 3 time series (a 4-D image, each volume resampled)")
     ap.add_argument("-v", "--verbose", type=int, default=0)
     a = ap.parse_args(argv)
+    if a.input_image_type not in (0, 3):
+        raise SystemExit("-e: scalar (0) and time-series (3) inputs are supported")
     kind, path, invert = parse_apply_output(a.output)
     xf = [x for t in a.transform for x in load_transforms(t)]
     if kind == "linear":
@@ -183,6 +190,12 @@ def cmd_apply(argv: Sequence[str]) -> int:
         return 0
     if not a.input:
         raise SystemExit("Output is not a transform, and there is no input image")
+    if a.input_image_type == 3:
+        from .apply_transforms import apply_transforms_series
+        series, gs, t_sp, t_org = aio.read_nifti_series(a.input)
+        out = apply_transforms_series(gr, series, gs, xf, a.interpolation, a.default_value)
+        aio.write_nifti_series(path, out, gr, t_sp, t_org)
+        return 0
     img, gi = aio.read_nifti(a.input)
     if a.dimensionality and a.dimensionality != gi.dim:
         raise SystemExit(f"-d {a.dimensionality} but the input is {gi.dim}-D")

@@ -147,6 +147,11 @@ def read_nifti(path: str) -> Tuple[np.ndarray, Geometry]:
     5-D vector images as (z,y,x,K) interleaved.  Geometry in LPS."""
     with _open(path, "rb") as fh:
         raw = fh.read()
+    return read_nifti_from_bytes(raw)
+
+
+def read_nifti_from_bytes(raw: bytes) -> Tuple[np.ndarray, Geometry]:
+    """read_nifti of a file's bytes."""
     if len(raw) < 348:
         raise ValueError("not a NIfTI-1 file")
     end = "<" if struct.unpack_from("<i", raw, 0)[0] == 348 else ">"
@@ -202,6 +207,63 @@ def read_nifti(path: str) -> Tuple[np.ndarray, Geometry]:
     geom = Geometry(tuple(n[:D]), tuple(float(v) for v in sp[:D]), tuple(float(v) for v in org_lps[:D]),
                     R_lps[:D, :D].copy())
     return a, geom
+
+
+def write_nifti_series(path: str, data: np.ndarray, geom: Geometry, t_spacing: float = 1.0,
+                       t_origin: float = 0.0) -> None:
+    """A 3-D time series (numpy (t, z, y, x)) as a 4-D NIfTI-1 file: the
+    spatial header from `geom`, pixdim[4] = t_spacing, toffset = t_origin
+    (ITK's fourth origin component)."""
+    a = np.asarray(data)
+    if geom.dim != 3 or a.ndim != 4 or a.shape[1:] != geom.shape:
+        raise ValueError(f"series shape {a.shape} does not match (t,) + {geom.shape}")
+    write_nifti(path, a[0], geom)   # header template (one volume)
+    with _open(path, "rb") as fh:
+        raw = bytearray(fh.read())
+    vox = int(struct.unpack_from("<f", raw, 108)[0])
+    hdr = raw[:vox]
+    dims = list(struct.unpack_from("<8h", hdr, 40))
+    dims[0], dims[4] = 4, a.shape[0]
+    struct.pack_into("<8h", hdr, 40, *dims)
+    struct.pack_into("<f", hdr, 76 + 4 * 4, float(t_spacing))
+    struct.pack_into("<f", hdr, 136, float(t_origin))
+    planar = a if a.dtype in _CODES else a.astype(np.float32)
+    if _CODES[planar.dtype] != struct.unpack_from("<h", hdr, 70)[0]:
+        raise ValueError("series data type differs from its first volume's")
+    with _open(path, "wb") as fh:
+        fh.write(bytes(hdr))
+        fh.write(np.ascontiguousarray(planar).astype(planar.dtype.newbyteorder("<"), copy=False).tobytes())
+
+
+def read_nifti_series(path: str):
+    """A 4-D NIfTI-1 scalar image as (numpy (t, z, y, x), spatial Geometry,
+    t_spacing, t_origin); a 3-D image reads as a series of one."""
+    with _open(path, "rb") as fh:
+        raw = fh.read()
+    end = "<" if struct.unpack_from("<i", raw, 0)[0] == 348 else ">"
+    dims = list(struct.unpack_from(end + "8h", raw, 40))
+    nt = int(dims[4]) if dims[0] >= 4 else 1
+    t_sp = float(struct.unpack_from(end + "f", raw, 76 + 16)[0]) if dims[0] >= 4 else 1.0
+    t_org = float(struct.unpack_from(end + "f", raw, 136)[0]) if dims[0] >= 4 else 0.0
+    if dims[0] >= 5 and dims[5] > 1:
+        raise ValueError("a vector image is not a time series")
+    # read the spatial part through the 3-D reader with the time axis folded away
+    hdr = bytearray(raw)
+    d3 = list(dims)
+    d3[0], d3[4] = 3, 1
+    struct.pack_into(end + "8h", hdr, 40, *d3)
+    vol, geom = read_nifti_from_bytes(bytes(hdr))
+    if geom.dim != 3:
+        raise ValueError("time series are 3-D in space")
+    code = struct.unpack_from(end + "h", raw, 70)[0]
+    dt = np.dtype(_DTYPES[code]).newbyteorder(end)
+    vox_offset = int(struct.unpack_from(end + "f", raw, 108)[0])
+    n = geom.shape
+    a = np.frombuffer(raw, dtype=dt, count=nt * int(np.prod(n)), offset=vox_offset).astype(dt.newbyteorder("="))
+    slope, inter = struct.unpack_from(end + "2f", raw, 112)
+    if slope not in (0.0, 1.0) or inter != 0.0:
+        a = a.astype(np.float32) * np.float32(slope if slope != 0.0 else 1.0) + np.float32(inter)
+    return a.reshape((nt,) + n), geom, (t_sp if t_sp > 0 else 1.0), t_org
 
 
 # ------------------------------------------------------------ transform files

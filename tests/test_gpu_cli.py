@@ -304,3 +304,30 @@ def test_jacobian_cli(tmp_path):
     np.testing.assert_array_equal(got, create_jacobian_determinant_image(fld, g, geometric=True))
     with pytest.raises(SystemExit):
         cli.main(["jacobian", "3", fpath, out, "0", "1", "1"])
+
+
+def test_apply_time_series(tmp_path):
+    """apply -e 3: each volume of a 4-D series through [field, affine] equals
+    the scalar resample of that volume; the output has the reference's
+    spatial grid and the input's time spacing and offset."""
+    from ants_amd.apply_transforms import AffineTransform, apply_transforms_series
+    rng = np.random.default_rng(13)
+    gi = Geometry((22, 20, 18), (1.2, 1.1, 1.3), (1.0, 2.0, -3.0), np.eye(3))
+    gr = Geometry((18, 16, 15), (1.4, 1.3, 1.5), (2.0, 3.0, -2.0), np.eye(3))
+    series = rng.standard_normal((3,) + gi.shape).astype(np.float32)
+    sp, rp = str(tmp_path / "ts.nii.gz"), str(tmp_path / "ref.nii.gz")
+    aio.write_nifti_series(sp, series, gi, 2.0, 0.5)
+    aio.write_nifti(rp, np.zeros(gr.shape, np.float32), gr)
+    apath = str(tmp_path / "a.mat")
+    aio.write_transform(apath, AffineTransform(np.eye(3) * 0.98, [0.4, -0.3, 0.2], [12.0, 12.0, 10.0]))
+    out = str(tmp_path / "out.nii.gz")
+    assert cli.main(["apply", "-d", "3", "-e", "3", "-i", sp, "-r", rp, "-o", out, "-t", apath,
+                     "-n", "BSpline[3]"]) == 0
+    got, g2, tsp, tor = aio.read_nifti_series(out)
+    assert got.shape == (3,) + gr.shape and (tsp, tor) == (2.0, 0.5) and g2.size == gr.size
+    s_arr, gs, _, _ = aio.read_nifti_series(sp)
+    _, grf = aio.read_nifti(rp)
+    xf = cli.load_transforms(apath)
+    for k in range(3):
+        np.testing.assert_array_equal(got[k], apply_transforms(grf, s_arr[k], gs, xf, "BSpline[3]"))
+    np.testing.assert_array_equal(apply_transforms_series(grf, s_arr, gs, xf, "BSpline[3]"), got)

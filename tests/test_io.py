@@ -122,3 +122,28 @@ def test_transform_round_trip(tmp_path, ext):
         from scipy.io import loadmat
         m = loadmat(p)
         assert m["AffineTransform_double_3_3"].shape == (12, 1) and m["fixed"].shape == (3, 1)
+
+
+def test_nifti_series_round_trip(tmp_path):
+    """4-D NIfTI: the spatial header, pixdim[4] (time spacing) and toffset
+    survive a round trip; a 3-D image reads as a series of one."""
+    from ants_amd.io import read_nifti_series, write_nifti_series
+    rng = np.random.default_rng(2)
+    th = 0.3
+    R = np.array([[np.cos(th), -np.sin(th), 0.0], [np.sin(th), np.cos(th), 0.0], [0.0, 0.0, 1.0]])
+    g = Geometry((7, 6, 5), (1.5, 1.0, 2.0), (3.0, -4.0, 5.0), R)
+    data = rng.standard_normal((4,) + g.shape).astype(np.float32)
+    p = str(tmp_path / "ts.nii.gz")
+    write_nifti_series(p, data, g, 2.5, 1.25)
+    a, g2, tsp, tor = read_nifti_series(p)
+    np.testing.assert_array_equal(a, data)
+    assert (tsp, tor) == (2.5, 1.25)
+    assert g2.size == g.size
+    np.testing.assert_allclose(g2.spacing, g.spacing, atol=1e-6)
+    np.testing.assert_allclose(g2.origin, g.origin, atol=1e-5)
+    np.testing.assert_allclose(g2.direction, R, atol=1e-6)
+    from ants_amd.io import write_nifti
+    p3 = str(tmp_path / "v.nii.gz")
+    write_nifti(p3, data[0], g)
+    a3, _, tsp3, _ = read_nifti_series(p3)
+    assert a3.shape == (1,) + g.shape and tsp3 == 1.0
