@@ -54,3 +54,33 @@ def create_jacobian_determinant_image(field, geometry, do_log: bool = False, def
     L.check(L.lib().ants_jacobian_determinant(C.byref(geometry.desc()), f.ptr, int(bool(do_log)), int(bool(geometric)),
                                               out.ptr, grad.ptr if grad is not None else None, None))
     return grad.to_host() if deformation_gradient else out.to_host()
+
+
+def measure_cc_similarity(fixed, fixed_geometry, moving, moving_geometry, radius: int) -> float:
+    """MeasureImageSimilarity -m CC[fixed, moving, 1, radius]
+    (Examples/MeasureImageSimilarity.cxx:205-220, 312-414): the
+    ANTSNeighborhoodCorrelation metric value -mean(local CC) on the fixed
+    image's grid with identity transforms, on the GPU (ants_op_cc); a
+    moving image on another grid is first resampled onto the fixed one
+    (linear, 0 outside, as the metric's interpolator samples it)."""
+    import ctypes as C
+    from . import _lib as L
+    from .apply_transforms import apply_transforms
+    if radius < 1:
+        raise ValueError("CC radius must be >= 1")
+    f = np.asarray(fixed, np.float32)
+    m = np.asarray(moving, np.float32)
+    same = (moving_geometry.size == fixed_geometry.size and
+            np.allclose(moving_geometry.spacing or 1.0, fixed_geometry.spacing or 1.0) and
+            np.allclose(moving_geometry.origin or 0.0, fixed_geometry.origin or 0.0) and
+            (moving_geometry.direction is None) == (fixed_geometry.direction is None) and
+            (moving_geometry.direction is None or np.allclose(moving_geometry.direction, fixed_geometry.direction)))
+    if not same:
+        m = apply_transforms(fixed_geometry, m, moving_geometry, [], "Linear", 0.0)
+    D = fixed_geometry.dim
+    d_f, d_m = L.DeviceArray.from_host(f), L.DeviceArray.from_host(m)
+    g1, g2 = L.DeviceArray(f.shape + (D,)), L.DeviceArray(f.shape + (D,))
+    value = C.c_double()
+    L.check(L.lib().ants_op_cc(C.byref(fixed_geometry.desc()), d_f.ptr, d_m.ptr, int(radius), g1.ptr, g2.ptr,
+                               C.byref(value), None))
+    return float(value.value)

@@ -51,7 +51,9 @@ the input's time spacing and offset).
 the other columns copied.
 
 ``python -m ants_amd.cli jacobian 3 out_0Warp.nii.gz jac.nii.gz [doLog]
-  [useGeometric] [deformationGradient]`` is CreateJacobianDeterminantImage.
+  [useGeometric] [deformationGradient]`` is CreateJacobianDeterminantImage;
+``similarity -d 3 -m CC[fixed.nii.gz,moving.nii.gz,1,4]`` prints
+MeasureImageSimilarity's CC value.
 
 ``registration ... -a 1`` writes ``out_Composite.h5`` and
 ``out_InverseComposite.h5`` (--write-composite-transform) instead of the
@@ -461,10 +463,32 @@ def cmd_jacobian(argv: Sequence[str]) -> int:
     return 0
 
 
+def cmd_similarity(argv: Sequence[str]) -> int:
+    """MeasureImageSimilarity -d D -m CC[fixed,moving,weight,radius]: prints
+    the metric value (the CC form; other metrics are not built here)."""
+    from .analysis import measure_cc_similarity
+    ap = argparse.ArgumentParser(prog="similarity")
+    ap.add_argument("-d", "--dimensionality", type=int, required=True)
+    ap.add_argument("-m", "--metric", required=True)
+    a = ap.parse_args(argv)
+    m = re.fullmatch(r"\s*([A-Za-z]+)\s*\[(.*)\]\s*", a.metric)
+    if not m or m.group(1).lower() != "cc":
+        raise SystemExit("similarity: only CC[fixed,moving,weight,radius] is supported")
+    parts = [p.strip() for p in m.group(2).split(",")]
+    if len(parts) < 4:
+        raise SystemExit("CC needs [fixed,moving,weight,radius]")
+    fixed, gf = aio.read_nifti(parts[0])
+    moving, gm = aio.read_nifti(parts[1])
+    if gf.dim != a.dimensionality:
+        raise SystemExit(f"-d {a.dimensionality} but the fixed image is {gf.dim}-D")
+    print(repr(measure_cc_similarity(fixed, gf, moving, gm, int(parts[3]))))
+    return 0
+
+
 def main(argv: Optional[Sequence[str]] = None) -> int:
     argv = list(sys.argv[1:] if argv is None else argv)
     cmds = {"registration": cmd_registration, "apply": cmd_apply, "apply-points": cmd_apply_points,
-            "jacobian": cmd_jacobian}
+            "jacobian": cmd_jacobian, "similarity": cmd_similarity}
     if not argv or argv[0] not in cmds:
         print(__doc__)
         return 2

@@ -331,3 +331,31 @@ def test_apply_time_series(tmp_path):
     for k in range(3):
         np.testing.assert_array_equal(got[k], apply_transforms(grf, s_arr[k], gs, xf, "BSpline[3]"))
     np.testing.assert_array_equal(apply_transforms_series(grf, s_arr, gs, xf, "BSpline[3]"), got)
+
+
+def test_similarity_cc(tmp_path, capsys):
+    """similarity -m CC[f,m,1,r]: the printed value is the oracle's CC
+    energy of the pair (same grid) for r = 2 and 4; a moving image on
+    another grid is resampled onto the fixed grid first."""
+    import oracle as o
+    from ants_amd.analysis import measure_cc_similarity
+    from ants_amd.apply_transforms import apply_transforms as AT_apply
+    pair = ph.phantom_pair_3d(4, size=(40, 36, 32), max_disp_mm=3.0, svf_sigma_mm=5.0)
+    g = Geometry(pair.size, pair.spacing, (0.0, 0.0, 0.0), np.eye(3))
+    f, m = str(tmp_path / "f.nii.gz"), str(tmp_path / "m.nii.gz")
+    aio.write_nifti(f, pair.fixed, g)
+    aio.write_nifti(m, pair.moving, g)
+    F, gF = aio.read_nifti(f)
+    M, _ = aio.read_nifti(m)
+    og_ = o.Grid(tuple(gF.size), tuple(gF.spacing), tuple(gF.origin))
+    for r in (2, 4):
+        capsys.readouterr()
+        assert cli.main(["similarity", "-d", "3", "-m", f"CC[{f},{m},1,{r}]"]) == 0
+        val = float(capsys.readouterr().out.strip())
+        want, _, _ = o.cc(og_, F.astype(np.float64), M.astype(np.float64), r)
+        assert val == pytest.approx(want, rel=1e-5)
+    gm2 = Geometry((30, 28, 24), (1.3, 1.2, 1.3), (1.0, 0.5, 0.0), np.eye(3))
+    M2 = AT_apply(gm2, M, gF, [], "Linear")
+    v2 = measure_cc_similarity(F, gF, M2, gm2, 2)
+    want2, _, _ = o.cc(og_, F.astype(np.float64), AT_apply(gF, M2, gm2, [], "Linear").astype(np.float64), 2)
+    assert v2 == pytest.approx(want2, rel=1e-5)
