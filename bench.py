@@ -18,7 +18,8 @@ replayed from iteration graphs, measured after the timed region;
 BSplineSyN field model (BSplineSyN[0.1,26,0,3]); `linear_stages` the
 Rigid and Affine stages antsRegistrationSyN.sh runs before SyN; `field_stages`
 GaussianDisplacementField, Exponential and their B-spline forms with the CC
-metric.  The other half of the metric comes from two
+metric; `apply_transforms` the consumers of the output field (resampling
+with each interpolator, CreateJacobianDeterminantImage).  The other half of the metric comes from two
 bounded legs on the same GPU: `batch` (64 mixed-size pairs, batched versus
 one at a time) and `serve` (a Poisson request stream at a fixed QPS: p50/p99
 latency, pairs/s and capacity), summarised in `batch_summary` and
@@ -234,25 +235,26 @@ def bench_serve(args):
     print(json.dumps(line), flush=True)
 
 
-def bench_apply(args):
+def apply_leg(pair, seed):
     """antsApplyTransforms subset: a 192x224x192 image (and a 32-label map)
     resampled onto the same grid through [displacement field, affine] with
     Linear, NearestNeighbor, GenericLabel, BSpline[3] (including its
-    coefficient decomposition) and LanczosWindowedSinc interpolation.  Algorithmic bytes per
+    coefficient decomposition) and LanczosWindowedSinc interpolation, and
+    CreateJacobianDeterminantImage of the field.  Algorithmic bytes per
     output voxel: field 3*4 (read in place) + input 4 (gathered once) +
-    output 4."""
+    output 4.  Returns (per-operation timings, the set-up for the CPU leg)."""
+    import ctypes as C
     from scipy import ndimage
     from ants_amd import _lib as L
     from ants_amd import apply_transforms as AT
     from ants_amd import phantoms as ph
-    pair = ph.phantom_pair_3d(args.seed)
     g = AT.Geometry(pair.size, pair.spacing, pair.origin)
-    rng = np.random.default_rng(args.seed)
+    rng = np.random.default_rng(seed)
     fld = ndimage.gaussian_filter(rng.standard_normal(g.shape + (3,)), (6, 6, 6, 0)).astype(np.float32)
     fld *= 4.0 / np.abs(fld).max()
     A = np.eye(3) * 1.02 + 0.01 * rng.standard_normal((3, 3))
     ctr = [0.5 * s for s in pair.size]
-    labels = ph.voronoi_labels(args.seed + 1, pair.labels_moving > 0, 32).astype(np.float32)
+    labels = ph.voronoi_labels(seed + 1, pair.labels_moving > 0, 32).astype(np.float32)
     d_img, d_lab = L.DeviceArray.from_host(pair.moving), L.DeviceArray.from_host(labels)
     d_fld = L.DeviceArray.from_host(fld)
     d_out = L.DeviceArray(g.shape)
@@ -278,7 +280,6 @@ def bench_apply(args):
         log(f"{interp}: {ms:.3f} ms per 192x224x192 resample")
     # CreateJacobianDeterminantImage of the same field: 12 bytes read per
     # voxel (its neighbours come from the caches) + 4 written
-    import ctypes as C
     d_jac = L.DeviceArray(g.shape)
     gdesc = g.desc()
     for _ in range(2):
@@ -293,6 +294,14 @@ def bench_apply(args):
     res["CreateJacobianDeterminantImage"] = {"ms": jms, "voxels_per_s": N / (jms * 1e-3),
                                              "achieved_gbs": N * 16 / (jms * 1e-3) / 1e9}
     log(f"CreateJacobianDeterminantImage: {jms:.3f} ms per 192x224x192 field")
+    return res, (g, fld, A, ctr, N, bytes_per_voxel)
+
+
+def bench_apply(args):
+    """The apply leg alone, with its CPU baseline and roofline line."""
+    from ants_amd import phantoms as ph
+    pair = ph.phantom_pair_3d(args.seed)
+    res, (g, fld, A, ctr, N, bytes_per_voxel) = apply_leg(pair, args.seed)
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -612,6 +621,7 @@ def main():
     bsp = bspline_leg(pair, gf, d_fixed, d_moving, d_fwd, d_inv, args.steps)
     lin = linear_leg(pair, d_fixed, d_moving)
     fld = field_leg(pair, d_fixed, d_moving)
+    apl, _ = apply_leg(pair, args.seed)
     avg_ms = kacc["ms"] / max(kacc["launches"], 1)
     bytes_per_launch = kacc["bytes"] / max(kacc["launches"], 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
@@ -667,6 +677,7 @@ def main():
         "bspline_syn": bsp,
         "linear_stages": lin,
         "field_stages": fld,
+        "apply_transforms": {k: {"ms": round(v["ms"], 4), "voxels_per_s": v["voxels_per_s"]} for k, v in apl.items()},
     }
     if legs:
         sv = legs["serve"]["continuous_batching"]
