@@ -70,11 +70,17 @@ def measure_cc_similarity(fixed, fixed_geometry, moving, moving_geometry, radius
         raise ValueError("CC radius must be >= 1")
     f = np.asarray(fixed, np.float32)
     m = np.asarray(moving, np.float32)
-    same = (moving_geometry.size == fixed_geometry.size and
-            np.allclose(moving_geometry.spacing or 1.0, fixed_geometry.spacing or 1.0) and
-            np.allclose(moving_geometry.origin or 0.0, fixed_geometry.origin or 0.0) and
-            (moving_geometry.direction is None) == (fixed_geometry.direction is None) and
-            (moving_geometry.direction is None or np.allclose(moving_geometry.direction, fixed_geometry.direction)))
+    def parts(gm):
+        D = gm.dim
+        sp = np.ones(D) if gm.spacing is None else np.asarray(gm.spacing, float)
+        org = np.zeros(D) if gm.origin is None else np.asarray(gm.origin, float)
+        dr = np.eye(D) if gm.direction is None else np.asarray(gm.direction, float).reshape(D, D)
+        if not dr.any():
+            dr = np.eye(D)  # an all-zero direction means identity
+        return tuple(int(v) for v in gm.size), sp, org, dr
+
+    (sf, pf, of, rf), (sm, pm, om, rm) = parts(fixed_geometry), parts(moving_geometry)
+    same = sf == sm and np.array_equal(pf, pm) and np.array_equal(of, om) and np.array_equal(rf, rm)
     if not same:
         m = apply_transforms(fixed_geometry, m, moving_geometry, [], "Linear", 0.0)
     D = fixed_geometry.dim
