@@ -90,3 +90,32 @@ def measure_cc_similarity(fixed, fixed_geometry, moving, moving_geometry, radius
     L.check(L.lib().ants_op_cc(C.byref(fixed_geometry.desc()), d_f.ptr, d_m.ptr, int(radius), g1.ptr, g2.ptr,
                                C.byref(value), None))
     return float(value.value)
+
+
+def smooth_displacement_field(field, geometry, variance=None, mesh=None, order: int = 3) -> np.ndarray:
+    """SmoothDisplacementField (Examples/SmoothDisplacementField.cxx:46-180) on
+    the GPU: with ``variance``, per-axis GaussianOperator smoothing (maximum
+    error 0.001), the boundary zeroed and a blend with the input below
+    variance 0.5 (ants_op_smooth_field); with ``mesh`` (spans per axis), the
+    one-level DisplacementFieldToBSplineImageFilter fit of the given order
+    with a stationary boundary, evaluated on the field's grid
+    (ants_op_bspline_field)."""
+    import ctypes as C
+    from . import _lib as L
+    if (variance is None) == (mesh is None):
+        raise ValueError("give either a variance or a mesh")
+    D = geometry.dim
+    f = np.asarray(field, np.float32)
+    if f.shape != tuple(geometry.shape) + (D,):
+        raise ValueError("field shape does not match its geometry")
+    d_in, d_out = L.DeviceArray.from_host(f), L.DeviceArray(f.shape)
+    desc = geometry.desc()
+    if variance is not None:
+        L.check(L.lib().ants_op_smooth_field(C.byref(desc), d_in.ptr, d_out.ptr, float(variance), None))
+    else:
+        if len(list(mesh)) != D:
+            raise ValueError(f"mesh needs {D} entries")
+        m = (C.c_int * 3)(*[int(v) for v in mesh], *([1] * (3 - D)))
+        mx = C.c_double()
+        L.check(L.lib().ants_op_bspline_field(C.byref(desc), d_in.ptr, d_out.ptr, m, int(order), C.byref(mx), None))
+    return d_out.to_host()

@@ -53,7 +53,8 @@ the other columns copied.
 ``python -m ants_amd.cli jacobian 3 out_0Warp.nii.gz jac.nii.gz [doLog]
   [useGeometric] [deformationGradient]`` is CreateJacobianDeterminantImage;
 ``similarity -d 3 -m CC[fixed.nii.gz,moving.nii.gz,1,4]`` prints
-MeasureImageSimilarity's CC value.
+MeasureImageSimilarity's CC value; ``smooth-field 3 in.nii.gz out.nii.gz 2.0``
+(a variance) or ``... 8x8x8`` (a B-spline mesh) is SmoothDisplacementField.
 
 ``registration ... -a 1`` writes ``out_Composite.h5`` and
 ``out_InverseComposite.h5`` (--write-composite-transform) instead of the
@@ -463,6 +464,36 @@ def cmd_jacobian(argv: Sequence[str]) -> int:
     return 0
 
 
+def cmd_smooth_field(argv: Sequence[str]) -> int:
+    """SmoothDisplacementField imageDimension inputField outputField
+    variance_or_mesh_size_base_level [numberOfLevels=1] [splineOrder=3]
+    [estimateInverse=0] (Examples/SmoothDisplacementField.cxx:185-200):
+    one value is a Gaussian variance, dim values (AxBxC) a B-spline mesh."""
+    from .analysis import smooth_displacement_field
+    if len(argv) < 4:
+        raise SystemExit("usage: smooth-field imageDimension inputField outputField variance_or_mesh "
+                         "[numberOfLevels=1] [splineOrder=3] [estimateInverse=0]")
+    dim = int(argv[0])
+    fld, g = aio.read_nifti(argv[1])
+    if g.dim != dim:
+        raise SystemExit(f"imageDimension {dim} but the field is {g.dim}-D")
+    vals = [float(v) for v in argv[3].split("x")]
+    levels = int(argv[4]) if len(argv) > 4 else 1
+    order = int(argv[5]) if len(argv) > 5 else 3
+    if len(argv) > 6 and int(argv[6]) != 0:
+        raise SystemExit("estimateInverse is not supported")
+    if len(vals) == 1:
+        out = smooth_displacement_field(fld, g, variance=vals[0])
+    elif len(vals) == dim:
+        if levels != 1:
+            raise SystemExit("only one B-spline fitting level is supported")
+        out = smooth_displacement_field(fld, g, mesh=[int(v) for v in vals], order=order)
+    else:
+        raise SystemExit(f"give one variance or {dim} mesh sizes")
+    aio.write_nifti(argv[2], out, g, vector=True)
+    return 0
+
+
 def cmd_similarity(argv: Sequence[str]) -> int:
     """MeasureImageSimilarity -d D -m CC[fixed,moving,weight,radius]: prints
     the metric value (the CC form; other metrics are not built here)."""
@@ -488,7 +519,7 @@ def cmd_similarity(argv: Sequence[str]) -> int:
 def main(argv: Optional[Sequence[str]] = None) -> int:
     argv = list(sys.argv[1:] if argv is None else argv)
     cmds = {"registration": cmd_registration, "apply": cmd_apply, "apply-points": cmd_apply_points,
-            "jacobian": cmd_jacobian, "similarity": cmd_similarity}
+            "jacobian": cmd_jacobian, "similarity": cmd_similarity, "smooth-field": cmd_smooth_field}
     if not argv or argv[0] not in cmds:
         print(__doc__)
         return 2

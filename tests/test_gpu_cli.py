@@ -359,3 +359,26 @@ def test_similarity_cc(tmp_path, capsys):
     v2 = measure_cc_similarity(F, gF, M2, gm2, 2)
     want2, _, _ = o.cc(og_, F.astype(np.float64), AT_apply(gF, M2, gm2, [], "Linear").astype(np.float64), 2)
     assert v2 == pytest.approx(want2, rel=1e-5)
+
+
+def test_smooth_field_cli(tmp_path):
+    """smooth-field: a variance runs the Gaussian displacement-field
+    smoothing (boundary zero), AxBxC the one-level B-spline fit; the files
+    equal the ops' outputs on the same field."""
+    rng = np.random.default_rng(14)
+    g = Geometry((24, 22, 20), (1.0, 1.2, 0.9), (0.0, 0.0, 0.0), np.eye(3))
+    fld = rng.standard_normal(g.shape + (3,)).astype(np.float32)
+    fin, fout = str(tmp_path / "w.nii.gz"), str(tmp_path / "s.nii.gz")
+    aio.write_nifti(fin, fld, g, vector=True)
+    fld, g = aio.read_nifti(fin)
+    assert cli.main(["smooth-field", "3", fin, fout, "2.0"]) == 0
+    got, _ = aio.read_nifti(fout)
+    from ants_amd.analysis import smooth_displacement_field
+    np.testing.assert_array_equal(got, smooth_displacement_field(fld, g, variance=2.0))
+    assert not got[0].any() and not got[:, 0].any() and not got[..., -1, :].any()
+    assert 0 < np.abs(got).max() < np.abs(fld).max()
+    assert cli.main(["smooth-field", "3", fin, fout, "4x4x3", "1", "3"]) == 0
+    got, _ = aio.read_nifti(fout)
+    np.testing.assert_array_equal(got, smooth_displacement_field(fld, g, mesh=[4, 4, 3], order=3))
+    with pytest.raises(SystemExit):
+        cli.main(["smooth-field", "3", fin, fout, "4x4x3", "2"])
