@@ -2164,6 +2164,7 @@ __global__ void __launch_bounds__(kBlock) k_apply_transforms(GridDev go, ApplyDe
     // row read nothing (their weight is zeroed; the clamped index stays in
     // the row)
     int xo[K];
+    const bool xin_all = base[0] >= 0 && base[0] + K <= P.in.n[0];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int64_t x = base[0] + k;
@@ -2182,8 +2183,15 @@ __global__ void __launch_bounds__(kBlock) k_apply_transforms(GridDev go, ApplyDe
         if (y < 0 || y >= P.in.n[1]) continue;
         const float* __restrict__ row = img + (z * P.in.n[1] + y) * P.in.n[0];
         double r = 0.0;
+        if (xin_all) {
+          // interior: the taps are contiguous from base[0]
+          const float* __restrict__ r0 = row + base[0];
 #pragma unroll
-        for (int kx = 0; kx < K; ++kx) r = fma(wt[0][kx], static_cast<double>(row[xo[kx]]), r);
+          for (int kx = 0; kx < K; ++kx) r = fma(wt[0][kx], static_cast<double>(r0[kx]), r);
+        } else {
+#pragma unroll
+          for (int kx = 0; kx < K; ++kx) r = fma(wt[0][kx], static_cast<double>(row[xo[kx]]), r);
+        }
         plane = fma(wt[1][ky], r, plane);
       }
       acc = D == 3 ? fma(wt[2][kz], plane, acc) : plane;
